@@ -1,0 +1,316 @@
+// Device DOP853 (Dormand-Prince 8(5,3) with dense output), one orbit per lane.
+//
+// Follows galpy/util/leung_dop853.c:231-517 step for step: the initial step
+// heuristic, the 12 stages, the mixed 5th/3rd-order error norm, the step-size
+// controller (safe=0.9, beta=0, fac1=0.333, fac2=6) and the 7th-order dense
+// output evaluated at every requested time passed by an accepted step.  Each
+// lane owns its own step size, so the accepted/rejected sequence of every
+// orbit is galpy's; lanes of a wavefront execute the stages together and only
+// the accept branch (FSAL evaluation + dense output) is masked per lane.
+// The tableau is the published one (Hairer, Norsett & Wanner, dop853.f).
+#pragma once
+#include "integrators_dev.h"
+
+namespace ghdp {
+constexpr double c2 = 0.526001519587677318785587544488e-1, c3 = 0.789002279381515978178381316732e-1,
+  c4 = 0.118350341907227396726757197510, c5 = 0.281649658092772603273242802490,
+  c6 = 0.333333333333333333333333333333, c7 = 0.25, c8 = 0.307692307692307692307692307692,
+  c9 = 0.651282051282051282051282051282, c10 = 0.6, c11 = 0.857142857142857142857142857142,
+  c14 = 0.1, c15 = 0.2, c16 = 0.777777777777777777777777777778;
+constexpr double a21 = 5.26001519587677318785587544488e-2,
+  a31 = 1.97250569845378994544595329183e-2, a32 = 5.91751709536136983633785987549e-2,
+  a41 = 2.95875854768068491816892993775e-2, a43 = 8.87627564304205475450678981324e-2,
+  a51 = 2.41365134159266685502369798665e-1, a53 = -8.84549479328286085344864962717e-1,
+  a54 = 9.24834003261792003115737966543e-1,
+  a61 = 3.7037037037037037037037037037e-2, a64 = 1.70828608729473871279604482173e-1,
+  a65 = 1.25467687566822425016691814123e-1,
+  a71 = 3.7109375e-2, a74 = 1.70252211019544039314978060272e-1,
+  a75 = 6.02165389804559606850219397283e-2, a76 = -1.7578125e-2,
+  a81 = 3.70920001185047927108779319836e-2, a84 = 1.70383925712239993810214054705e-1,
+  a85 = 1.07262030446373284651809199168e-1, a86 = -1.53194377486244017527936158236e-2,
+  a87 = 8.27378916381402288758473766002e-3,
+  a91 = 6.24110958716075717114429577812e-1, a94 = -3.36089262944694129406857109825e0,
+  a95 = -8.68219346841726006818189891453e-1, a96 = 2.75920996994467083049415600797e1,
+  a97 = 2.01540675504778934086186788979e1, a98 = -4.34898841810699588477366255144e1,
+  a101 = 4.77662536438264365890433908527e-1, a104 = -2.48811461997166764192642586468e0,
+  a105 = -5.90290826836842996371446475743e-1, a106 = 2.12300514481811942347288949897e1,
+  a107 = 1.52792336328824235832596922938e1, a108 = -3.32882109689848629194453265587e1,
+  a109 = -2.03312017085086261358222928593e-2,
+  a111 = -9.3714243008598732571704021658e-1, a114 = 5.18637242884406370830023853209e0,
+  a115 = 1.09143734899672957818500254654e0, a116 = -8.14978701074692612513997267357e0,
+  a117 = -1.85200656599969598641566180701e1, a118 = 2.27394870993505042818970056734e1,
+  a119 = 2.49360555267965238987089396762e0, a1110 = -3.0467644718982195003823669022e0,
+  a121 = 2.27331014751653820792359768449e0, a124 = -1.05344954667372501984066689879e1,
+  a125 = -2.00087205822486249909675718444e0, a126 = -1.79589318631187989172765950534e1,
+  a127 = 2.79488845294199600508499808837e1, a128 = -2.85899827713502369474065508674e0,
+  a129 = -8.87285693353062954433549289258e0, a1210 = 1.23605671757943030647266201528e1,
+  a1211 = 6.43392746015763530355970484046e-1,
+  a141 = 5.61675022830479523392909219681e-2, a147 = 2.53500210216624811088794765333e-1,
+  a148 = -2.46239037470802489917441475441e-1, a149 = -1.24191423263816360469010140626e-1,
+  a1410 = 1.5329179827876569731206322685e-1, a1411 = 8.20105229563468988491666602057e-3,
+  a1412 = 7.56789766054569976138603589584e-3, a1413 = -8.298e-3,
+  a151 = 3.18346481635021405060768473261e-2, a156 = 2.83009096723667755288322961402e-2,
+  a157 = 5.35419883074385676223797384372e-2, a158 = -5.49237485713909884646569340306e-2,
+  a1511 = -1.08347328697249322858509316994e-4, a1512 = 3.82571090835658412954920192323e-4,
+  a1513 = -3.40465008687404560802977114492e-4, a1514 = 1.41312443674632500278074618366e-1,
+  a161 = -4.28896301583791923408573538692e-1, a166 = -4.69762141536116384314449447206e0,
+  a167 = 7.68342119606259904184240953878e0, a168 = 4.06898981839711007970213554331e0,
+  a169 = 3.56727187455281109270669543021e-1, a1613 = -1.39902416515901462129418009734e-3,
+  a1614 = 2.9475147891527723389556272149e0, a1615 = -9.15095847217987001081870187138e0;
+constexpr double b1 = 5.42937341165687622380535766363e-2, b6 = 4.45031289275240888144113950566,
+  b7 = 1.89151789931450038304281599044, b8 = -5.8012039600105847814672114227,
+  b9 = 3.1116436695781989440891606237e-1, b10 = -1.52160949662516078556178806805e-1,
+  b11 = 2.01365400804030348374776537501e-1, b12 = 4.47106157277725905176885569043e-2;
+constexpr double bhh1 = 0.244094488188976377952755905512, bhh2 = 0.733846688281611857341361741547,
+  bhh3 = 0.220588235294117647058823529412e-1;
+constexpr double er1 = 0.1312004499419488073250102996e-1, er6 = -0.1225156446376204440720569753e+1,
+  er7 = -0.4957589496572501915214079952, er8 = 0.1664377182454986536961530415e+1,
+  er9 = -0.3503288487499736816886487290, er10 = 0.3341791187130174790297318841,
+  er11 = 0.8192320648511571246570742613e-1, er12 = -0.2235530786388629525884427845e-1;
+constexpr double d41 = -0.84289382761090128651353491142e+1, d46 = 0.56671495351937776962531783590,
+  d47 = -0.30689499459498916912797304727e+1, d48 = 0.23846676565120698287728149680e+1,
+  d49 = 0.21170345824450282767155149946e+1, d410 = -0.87139158377797299206789907490,
+  d411 = 0.22404374302607882758541771650e+1, d412 = 0.63157877876946881815570249290,
+  d413 = -0.88990336451333310820698117400e-1, d414 = 0.18148505520854727256656404962e+2,
+  d415 = -0.91946323924783554000451984436e+1, d416 = -0.44360363875948939664310572000e+1;
+constexpr double d51 = 0.10427508642579134603413151009e+2, d56 = 0.24228349177525818288430175319e+3,
+  d57 = 0.16520045171727028198505394887e+3, d58 = -0.37454675472269020279518312152e+3,
+  d59 = -0.22113666853125306036270938578e+2, d510 = 0.77334326684722638389603898808e+1,
+  d511 = -0.30674084731089398182061213626e+2, d512 = -0.93321305264302278729567221706e+1,
+  d513 = 0.15697238121770843886131091075e+2, d514 = -0.31139403219565177677282850411e+2,
+  d515 = -0.93529243588444783865713862664e+1, d516 = 0.35816841486394083752465898540e+2;
+constexpr double d61 = 0.19985053242002433820987653617e+2, d66 = -0.38703730874935176555105901742e+3,
+  d67 = -0.18917813819516756882830838328e+3, d68 = 0.52780815920542364900561016686e+3,
+  d69 = -0.11573902539959630126141871134e+2, d610 = 0.68812326946963000169666922661e+1,
+  d611 = -0.10006050966910838403183860980e+1, d612 = 0.77771377980534432092869265740,
+  d613 = -0.27782057523535084065932004339e+1, d614 = -0.60196695231264120758267380846e+2,
+  d615 = 0.84320405506677161018159903784e+2, d616 = 0.11992291136182789328035130030e+2;
+constexpr double d71 = -0.25693933462703749003312586129e+2, d76 = -0.15418974869023643374053993627e+3,
+  d77 = -0.23152937917604549567536039109e+3, d78 = 0.35763911791061412378285349910e+3,
+  d79 = 0.93405324183624310003907691704e+2, d710 = -0.37458323136451633156875139351e+2,
+  d711 = 0.10409964950896230045147246184e+3, d712 = 0.29840293426660503123344363579e+2,
+  d713 = -0.43533456590011143754432175058e+2, d714 = 0.96324553959188282948394950600e+2,
+  d715 = -0.39177261675615439165231486172e+2, d716 = -0.14972683625798562581422125276e+3;
+constexpr double uround = 2.3e-16;
+}  // namespace ghdp
+
+// RHS of the first-order system: (v, a(q)) -- evalRectDeriv,
+// integrateFullOrbit.c:1326-1351
+__device__ __forceinline__ void gh_deriv(const GhPotTable* __restrict__ T, double t,
+                                         const double* y, double* k) {
+  k[0] = y[3];
+  k[1] = y[4];
+  k[2] = y[5];
+  gh_rect_force(T, t, y[0], y[1], y[2], k[3], k[4], k[5]);
+}
+
+__device__ __forceinline__ double gh_csign(double x, double y) {
+  return (y > 0.0) ? fabs(x) : -fabs(x);
+}
+
+struct GhDopStats {
+  long long accepted;
+  long long rejected;
+};
+
+// Integrates one orbit.  y0: rectangular state, advanced in place (as galpy's
+// dop853 does with its y0).  Rows are written through `out` (cylindrical).
+// rtol/atol are natural logs (integratePlanarOrbit.py:712-724).
+__device__ GhDopStats gh_dop853(const GhPotTable* __restrict__ T, double* y0, int nt,
+                                const double* __restrict__ t, double rtol_log, double atol_log,
+                                double* __restrict__ out, int* err,
+                                const volatile int* cancel) {
+  using namespace ghdp;
+  const double rtol = exp(rtol_log);
+  const double atol = exp(atol_log);
+  const double safe = 0.9;
+  // galpy also tracks facold for Lund stabilisation, but with beta = 0 the
+  // factor pow(facold, beta) is exactly 1 and facold never enters a result
+  const double expo1 = 1.0 / 8.0;
+  const double facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;
+  const double hmax = t[nt - 1] - t[0];
+  const double pos_neg = gh_csign(1.0, hmax);
+  double yy1[6], k1[6], k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6];
+  GhDopStats st = {0, 0};
+  gh_store_cyl(out, y0[0], y0[1], y0[2], y0[3], y0[4], y0[5]);
+  gh_deriv(T, t[0], y0, k1);
+  double dnf = 0.0, dny = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const double sk = atol + rtol * fabs(y0[i]);
+    double sqr = k1[i] / sk;
+    dnf += sqr * sqr;
+    sqr = y0[i] / sk;
+    dny += sqr * sqr;
+  }
+  double h = gh_csign(fmin(sqrt(dny / dnf) * 0.01, fabs(hmax)), pos_neg);
+#pragma unroll
+  for (int i = 0; i < 6; i++) k3[i] = y0[i] + h * k1[i];
+  gh_deriv(T, t[0] + h, k3, k2);
+  double der2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const double sk = atol + rtol * fabs(y0[i]);
+    const double sqr = (k2[i] - k1[i]) / sk;
+    der2 += sqr * sqr;
+  }
+  der2 = sqrt(der2) / h;
+  const double der12 = fmax(fabs(der2), sqrt(dnf));
+  const double h1 = pow(0.01 / der12, 1.0 / 8.0);
+  h = gh_csign(fmin(100.0 * fabs(h), fmin(fabs(h1), fabs(hmax))), pos_neg);
+  int reject = 0;
+  double t_current = t[0], t_old = t[0], t_old_older = t[0];
+  int done = 0;
+  while (done < nt - 1) {
+    if (cancel != nullptr && ((st.accepted + st.rejected) & 255) == 0 && *cancel) {
+      *err = -10;
+      break;
+    }
+    h = pos_neg * fmax(fabs(h), 1e3 * uround);
+#pragma unroll
+    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * a21 * k1[i];
+    gh_deriv(T, t_current + c2 * h, yy1, k2);
+#pragma unroll
+    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a31 * k1[i] + a32 * k2[i]);
+    gh_deriv(T, t_current + c3 * h, yy1, k3);
+#pragma unroll
+    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a41 * k1[i] + a43 * k3[i]);
+    gh_deriv(T, t_current + c4 * h, yy1, k4);
+#pragma unroll
+    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a51 * k1[i] + a53 * k3[i] + a54 * k4[i]);
+    gh_deriv(T, t_current + c5 * h, yy1, k5);
+#pragma unroll
+    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a61 * k1[i] + a64 * k4[i] + a65 * k5[i]);
+    gh_deriv(T, t_current + c6 * h, yy1, k6);
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+      yy1[i] = y0[i] + h * (a71 * k1[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]);
+    gh_deriv(T, t_current + c7 * h, yy1, k7);
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+      yy1[i] = y0[i] + h * (a81 * k1[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] + a87 * k7[i]);
+    gh_deriv(T, t_current + c8 * h, yy1, k8);
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+      yy1[i] = y0[i] + h * (a91 * k1[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] + a97 * k7[i] +
+                            a98 * k8[i]);
+    gh_deriv(T, t_current + c9 * h, yy1, k9);
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+      yy1[i] = y0[i] + h * (a101 * k1[i] + a104 * k4[i] + a105 * k5[i] + a106 * k6[i] +
+                            a107 * k7[i] + a108 * k8[i] + a109 * k9[i]);
+    gh_deriv(T, t_current + c10 * h, yy1, k10);
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+      yy1[i] = y0[i] + h * (a111 * k1[i] + a114 * k4[i] + a115 * k5[i] + a116 * k6[i] +
+                            a117 * k7[i] + a118 * k8[i] + a119 * k9[i] + a1110 * k10[i]);
+    gh_deriv(T, t_current + c11 * h, yy1, k2);
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+      yy1[i] = y0[i] + h * (a121 * k1[i] + a124 * k4[i] + a125 * k5[i] + a126 * k6[i] +
+                            a127 * k7[i] + a128 * k8[i] + a129 * k9[i] + a1210 * k10[i] +
+                            a1211 * k2[i]);
+    t_old_older = t_old;
+    t_old = t_current;
+    t_current = t_current + h;
+    gh_deriv(T, t_current, yy1, k3);
+    double err5 = 0.0, err3 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      k4[i] = b1 * k1[i] + b6 * k6[i] + b7 * k7[i] + b8 * k8[i] + b9 * k9[i] + b10 * k10[i] +
+              b11 * k2[i] + b12 * k3[i];
+      k5[i] = y0[i] + h * k4[i];
+      const double sk = atol + rtol * fmax(fabs(y0[i]), fabs(k5[i]));
+      double erri = k4[i] - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i];
+      double sqr = erri / sk;
+      err3 += sqr * sqr;
+      erri = er1 * k1[i] + er6 * k6[i] + er7 * k7[i] + er8 * k8[i] + er9 * k9[i] + er10 * k10[i] +
+             er11 * k2[i] + er12 * k3[i];
+      sqr = erri / sk;
+      err5 += sqr * sqr;
+    }
+    double deno = err5 + 0.01 * err3;
+    if (deno <= 0.0) deno = 1.0;
+    const double errv = fabs(h) * err5 * sqrt(1.0 / (deno * 6.0));
+    const double fac11 = pow(errv, expo1);
+    const double fac = fmax(facc2, fmin(facc1, fac11 / safe));
+    double hnew = h / fac;
+    if (errv <= 1.0) {
+      st.accepted++;
+      gh_deriv(T, t_current, k5, k4);
+      // dense output is only needed when a requested time falls inside this
+      // step; the three extra stages feed nothing else (the next step
+      // recomputes k2, k3, k10), so they are skipped otherwise
+      const bool need_dense = pos_neg * t[done + 1] < pos_neg * t_current;
+      if (need_dense) {
+        double rc1[6], rc2[6], rc3[6], rc4[6], rc5[6], rc6[6], rc7[6], rc8[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+          rc1[i] = y0[i];
+          const double ydiff = k5[i] - y0[i];
+          rc2[i] = ydiff;
+          const double bspl = h * k1[i] - ydiff;
+          rc3[i] = bspl;
+          rc4[i] = ydiff - h * k4[i] - bspl;
+          rc5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
+                   d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
+          rc6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
+                   d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
+          rc7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
+                   d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
+          rc8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
+                   d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+          yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
+                                a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] + a1413 * k4[i]);
+        gh_deriv(T, t_old + c14 * h, yy1, k10);
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+          yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
+                                a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] + a1514 * k10[i]);
+        gh_deriv(T, t_old + c15 * h, yy1, k2);
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+          yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
+                                a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] + a1615 * k2[i]);
+        gh_deriv(T, t_old + c16 * h, yy1, k3);
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+          rc5[i] = h * (rc5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
+          rc6[i] = h * (rc6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
+          rc7[i] = h * (rc7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
+          rc8[i] = h * (rc8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
+        }
+        while (done < nt - 1 && pos_neg * t[done + 1] < pos_neg * t_current) {
+          const double s = (t[done + 1] - t_old) / h;
+          const double s1 = 1.0 - s;
+          double yo[6];
+#pragma unroll
+          for (int i = 0; i < 6; i++)
+            yo[i] = rc1[i] + s * (rc2[i] + s1 * (rc3[i] + s * (rc4[i] + s1 * (rc5[i] + s * (rc6[i] +
+                                  s1 * (rc7[i] + s * rc8[i]))))));
+          done++;
+          gh_store_cyl(out + 6L * done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        k1[i] = k4[i];
+        y0[i] = k5[i];
+      }
+      hnew = (fabs(hnew) > fabs(hmax)) ? pos_neg * hmax : hnew;
+      if (reject) hnew = pos_neg * fmin(fabs(hnew), fabs(h));
+      reject = 0;
+    } else {
+      st.rejected++;
+      hnew = h / fmin(facc1, fac11 / safe);
+      reject = 1;
+      t_current = t_old;
+      t_old = t_old_older;
+    }
+    h = hnew;
+  }
+  return st;
+}

@@ -1,0 +1,525 @@
+// MI355X orbit-integration engine: kernels and C-ABI entry points.
+//
+// One orbit per lane, FP64 throughout, potential table in device memory read
+// with wavefront-uniform (scalar) loads.  The C-ABI mirrors galpy's
+// integrateFullOrbit (galpy/orbit/orbit_c_ext/integrateFullOrbit.c:1046-1150):
+// cylindrical initial conditions in, cylindrical rows out, one call for all
+// orbits.  Declarations and argument meaning: include/galpy_hip.h.
+#include <hip/hip_runtime.h>
+#include <signal.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <cstdarg>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/galpy_hip.h"
+#include "dop853_dev.h"
+
+#define GH_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+constexpr int kBlock = 64;  // one wavefront per workgroup
+
+thread_local char g_errbuf[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_errbuf, sizeof(g_errbuf), fmt, ap);
+  va_end(ap);
+}
+
+#define GH_CHECK(expr)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                __LINE__);                                                      \
+      return -1;                                                                \
+    }                                                                           \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Interrupt flag: fine-grained pinned host memory that running kernels poll.
+// ---------------------------------------------------------------------------
+volatile int* g_irq_host = nullptr;
+int* g_irq_dev = nullptr;
+
+int* irq_device_ptr() {
+  if (g_irq_dev) return g_irq_dev;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    return nullptr;
+  g_irq_host = static_cast<volatile int*>(p);
+  *g_irq_host = 0;
+  void* d = nullptr;
+  if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) return nullptr;
+  g_irq_dev = static_cast<int*>(d);
+  return g_irq_dev;
+}
+
+// ---------------------------------------------------------------------------
+// Host decode of galpy's packed potential into the device table.
+// ---------------------------------------------------------------------------
+int nargs_of(int code) {
+  switch (code) {
+    case GH_LOGHALO: return 4;
+    case GH_DEHNENBAR: return 6;
+    case GH_MIYAMOTONAGAI: return 3;
+    case GH_HERNQUIST: return 2;
+    case GH_NFW: return 2;
+    case GH_POWERSPHCUTOFF: return 5;
+    default: return -1;
+  }
+}
+
+// smallest x >= a+1 with 2 e^-x x^(a-1) < 2^-60 Gamma(a): beyond it the upper
+// incomplete gamma is below 1/128 ulp of Gamma(a)
+double incgamma_xbig(double a, double gamma_a) {
+  double x = std::max(a + 1.0, 2.0 * (a - 1.0) + 1.0);
+  const double lim = std::ldexp(gamma_a, -60);
+  while (2.0 * std::exp(-x + (a - 1.0) * std::log(x)) >= lim) x += 0.25;
+  return x;
+}
+
+int fill_series(double a, double* ser) {
+  // ser[n] = 1/(a (a+1) ... (a+n)); the last term at x = a+1 must be negligible
+  double v = 1.0 / a;
+  for (int n = 0; n < GH_IG_NSER; n++) {
+    ser[n] = v;
+    v /= (a + n + 1.0);
+  }
+  const double last = ser[GH_IG_NSER - 1] * std::pow(a + 1.0, GH_IG_NSER - 1);
+  return (last < 1e-18 * ser[0]) ? 0 : -1;
+}
+
+int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPotTable* T) {
+  std::memset(T, 0, sizeof(*T));
+  if (npot < 1 || npot > GH_MAX_POT) {
+    set_error("npot=%d outside [1, %d]", npot, GH_MAX_POT);
+    return -1;
+  }
+  T->npot = npot;
+  for (int i = 0; i < npot; i++) {
+    const int code = pot_type[i];
+    const int na = nargs_of(code);
+    if (na < 0) {
+      set_error("pot_type %d (component %d) has no HIP kernel", code, i);
+      return -1;
+    }
+    GhComponent& C = T->comp[i];
+    C.type = code;
+    for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
+    switch (code) {
+      case GH_POWERSPHCUTOFF: {
+        const double amp = C.p[0], alpha = C.p[1], rc = C.p[2];
+        const double a = 1.5 - 0.5 * alpha, a2 = 1.0 - 0.5 * alpha;
+        if (!(a > 0.0)) {
+          set_error("PowerSphericalPotentialwCutoff alpha=%g needs alpha < 3", alpha);
+          return -1;
+        }
+        C.c[0] = a;
+        C.c[1] = std::tgamma(a);
+        C.c[2] = amp * 2.0 * M_PI * std::pow(rc, 3.0 - alpha);
+        C.c[3] = 1.0 / (rc * rc);
+        C.c[4] = incgamma_xbig(a, C.c[1]);
+        if (fill_series(a, T->ser[i][0]) != 0) {
+          set_error("incomplete-gamma series too short for a=%g", a);
+          return -1;
+        }
+        if (a2 > 0.0) {
+          C.c[5] = a2;
+          C.c[6] = std::tgamma(a2);
+          C.c[7] = incgamma_xbig(a2, C.c[6]);
+          fill_series(a2, T->ser[i][1]);
+        } else {  // galpy's Eval calls gsl_sf_gamma(1 - alpha/2): undefined here
+          C.c[5] = a2;
+          C.c[6] = NAN;
+          C.c[7] = INFINITY;
+          for (int n = 0; n < GH_IG_NSER; n++) T->ser[i][1][n] = NAN;
+        }
+        C.c[8] = 1.0 / rc;
+        break;
+      }
+      case GH_MIYAMOTONAGAI: C.c[0] = C.p[2] * C.p[2]; break;
+      case GH_NFW: C.c[0] = 1.0 / C.p[1]; break;
+      case GH_HERNQUIST: C.c[0] = 0.5 * C.p[0]; break;
+      case GH_LOGHALO:
+        C.c[0] = 1.0 / (C.p[1] * C.p[1]);
+        C.nonaxi = C.p[3] < 1.0;
+        break;
+      case GH_DEHNENBAR:
+        C.c[0] = 1.0 / C.p[3];
+        C.nonaxi = 1;
+        break;
+    }
+    T->nonaxi |= C.nonaxi;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// leapfrog (bovy_symplecticode.c:132-224) and symplec4 (:250-377)
+template <int METHOD>
+__global__ void __launch_bounds__(kBlock)
+gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
+                     int indiv_t, double dt_in, double rtol, double atol,
+                     const GhPotTable* __restrict__ T, double* __restrict__ result,
+                     int* __restrict__ err, double* __restrict__ stats,
+                     const volatile int* cancel) {
+  const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (orbit >= nobj) return;
+  const double* tt = t + (indiv_t ? orbit * nt : 0);
+  double w[6];
+  {
+    double v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = yo[orbit * 6 + k];
+    gh_cyl_to_rect(v, w);
+#pragma unroll
+    for (int k = 0; k < 6; k++) yo[orbit * 6 + k] = w[k];  // galpy converts yo in place
+  }
+  GhVec3 q{w[0], w[1], w[2]}, p{w[3], w[4], w[5]};
+  double* out = result + orbit * (long)nt * 6;
+  gh_store_cyl(out, q.x, q.y, q.z, p.x, p.y, p.z);
+  int e = 0;
+  const double init_dt = tt[1] - tt[0];
+  double dt = dt_in;
+  if (dt_in == GH_DT_SENTINEL) {
+    if (METHOD == 0)
+      dt = gh_leapfrog_estimate_step(T, q, p, init_dt, tt[0], rtol, atol);
+    else
+      dt = gh_symplec4_estimate_step(T, q, p, init_dt, tt[0], rtol, atol);
+  }
+  const long ndt = (long)(init_dt / dt);
+  double to = tt[0];
+  for (int ii = 0; ii < nt - 1; ii++) {
+    if (cancel != nullptr && (ii & 63) == 0 && *cancel) {
+      e = -10;
+      break;
+    }
+    if (METHOD == 0) {
+      // leapfrog: half drift, ndt-1 fused kick+drift, kick + half drift
+      GhVec3 q12 = gh_drift(q, p, dt / 2.0);
+      for (long jj = 0; jj < ndt - 1; jj++) {
+        p = gh_drift(p, gh_accel(T, to + dt / 2.0, q12), dt);
+        q12 = gh_drift(q12, p, dt);
+        to = to + dt;
+      }
+      p = gh_drift(p, gh_accel(T, to + dt / 2.0, q12), dt);
+      q = gh_drift(q12, p, dt / 2.0);
+      to = to + dt;
+    } else {
+      const double c1 = GH_S4C1, c2 = GH_S4C2, c3 = GH_S4C3, c4 = GH_S4C4;
+      const double d1 = GH_S4D1, d2 = GH_S4D2, d3 = GH_S4D3;
+      q = gh_drift(q, p, c1 * dt);
+      to += c1 * dt;
+      for (long jj = 0; jj < ndt - 1; jj++) {
+        p = gh_drift(p, gh_accel(T, to, q), d1 * dt);
+        q = gh_drift(q, p, c2 * dt);
+        to += c2 * dt;
+        p = gh_drift(p, gh_accel(T, to, q), d2 * dt);
+        q = gh_drift(q, p, c3 * dt);
+        to += c3 * dt;
+        p = gh_drift(p, gh_accel(T, to, q), d3 * dt);
+        q = gh_drift(q, p, (c4 + c1) * dt);
+        to += (c4 + c1) * dt;
+      }
+      p = gh_drift(p, gh_accel(T, to, q), d1 * dt);
+      q = gh_drift(q, p, c2 * dt);
+      to += c2 * dt;
+      p = gh_drift(p, gh_accel(T, to, q), d2 * dt);
+      q = gh_drift(q, p, c3 * dt);
+      to += c3 * dt;
+      p = gh_drift(p, gh_accel(T, to, q), d3 * dt);
+      q = gh_drift(q, p, c4 * dt);
+      to += c4 * dt;
+    }
+    gh_store_cyl(out + 6L * (ii + 1), q.x, q.y, q.z, p.x, p.y, p.z);
+  }
+  err[orbit] = e;
+  if (stats) {
+    stats[orbit * 3 + 0] = (double)(ndt * (long)(nt - 1));
+    stats[orbit * 3 + 1] = 0.0;
+    stats[orbit * 3 + 2] = dt;
+  }
+}
+
+// dop853 (leung_dop853.c:231-517)
+__global__ void __launch_bounds__(kBlock)
+gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
+                 int indiv_t, double rtol, double atol, const GhPotTable* __restrict__ T,
+                 double* __restrict__ result, int* __restrict__ err, double* __restrict__ stats,
+                 const volatile int* cancel) {
+  const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (orbit >= nobj) return;
+  const double* tt = t + (indiv_t ? orbit * nt : 0);
+  double y[6];
+  {
+    double v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = yo[orbit * 6 + k];
+    gh_cyl_to_rect(v, y);
+  }
+  int e = 0;
+  const GhDopStats st =
+      gh_dop853(T, y, nt, tt, rtol, atol, result + orbit * (long)nt * 6, &e, cancel);
+#pragma unroll
+  for (int k = 0; k < 6; k++) yo[orbit * 6 + k] = y[k];  // galpy's y0 holds the last state
+  err[orbit] = e;
+  if (stats) {
+    stats[orbit * 3 + 0] = (double)st.accepted;
+    stats[orbit * 3 + 1] = (double)st.rejected;
+    stats[orbit * 3 + 2] = 0.0;
+  }
+}
+
+// Energy / Jacobi drift along stored cylindrical rows.
+__global__ void __launch_bounds__(kBlock)
+gh_energy_kernel(int nobj, int nt, const double* __restrict__ rows, const double* __restrict__ t,
+                 int indiv_t, const GhPotTable* __restrict__ T, double omega,
+                 double* __restrict__ e0, double* __restrict__ demax) {
+  const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (orbit >= nobj) return;
+  const double* tt = t + (indiv_t ? orbit * nt : 0);
+  const double* r = rows + orbit * (long)nt * 6;
+  double E0 = 0.0, dmax = 0.0;
+  for (int j = 0; j < nt; j++) {
+    const double R = r[6L * j + 0], vR = r[6L * j + 1], vT = r[6L * j + 2];
+    const double z = r[6L * j + 3], vz = r[6L * j + 4], phi = r[6L * j + 5];
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    const double E = gh_potential(T, R, z, cp, sp, tt[j]) + 0.5 * (vR * vR + vT * vT + vz * vz) -
+                     omega * R * vT;
+    if (j == 0)
+      E0 = E;
+    else
+      dmax = fmax(dmax, fabs((E - E0) / E0));
+  }
+  e0[orbit] = E0;
+  demax[orbit] = dmax;
+}
+
+__global__ void __launch_bounds__(kBlock)
+gh_eval_kernel(int n, const double* __restrict__ R, const double* __restrict__ z,
+               const double* __restrict__ phi, const double* __restrict__ t,
+               const GhPotTable* __restrict__ T, double* __restrict__ pot,
+               double* __restrict__ FR, double* __restrict__ Fz, double* __restrict__ Fphi) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sp, cp;
+  sincos(phi[i], &sp, &cp);
+  if (pot) pot[i] = gh_potential(T, R[i], z[i], cp, sp, t[i]);
+  const GhCyl F = gh_cyl_forces(T, R[i], R[i] * R[i], z[i], cp, sp, t[i]);
+  if (FR) FR[i] = F.FR;
+  if (Fz) Fz[i] = F.Fz;
+  if (Fphi) Fphi[i] = F.Fphi;
+}
+
+int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                     const GhPotTable* d_table, double dt, double rtol, double atol,
+                     double* d_result, int* d_err, int odeint_type, double* d_stats,
+                     hipStream_t stream) {
+  if (nobj == 0) return 0;
+  if (nobj < 0 || nt < 2) {
+    set_error("invalid sizes nobj=%d nt=%d (need nt >= 2)", nobj, nt);
+    return -1;
+  }
+  const int* cancel = irq_device_ptr();
+  const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
+  switch (odeint_type) {
+    case 0:
+      hipLaunchKernelGGL(gh_symplectic_kernel<0>, grid, block, 0, stream, nobj, d_yo, nt, d_t,
+                         indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+      break;
+    case 3:
+      hipLaunchKernelGGL(gh_symplectic_kernel<3>, grid, block, 0, stream, nobj, d_yo, nt, d_t,
+                         indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+      break;
+    case 6:
+      hipLaunchKernelGGL(gh_dop853_kernel, grid, block, 0, stream, nobj, d_yo, nt, d_t, indiv_t,
+                         rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+      break;
+    default:
+      set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, 6 dop853)",
+                odeint_type);
+      return -1;
+  }
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+// SIGINT during the blocking host call sets the polled flag, as galpy's
+// handler sets `interrupted` (bovy_symplecticode.c:38-45).
+void on_sigint(int) {
+  if (g_irq_host) *g_irq_host = 1;
+}
+
+int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npot, int* pot_type,
+                   double* pot_args, double dt, double rtol, double atol, double* result,
+                   int* err, int odeint_type, void (*cb)(void)) {
+  if (nobj <= 0) return 0;
+  if (nt < 2) {
+    set_error("nt=%d: need at least two output times", nt);
+    return -1;
+  }
+  if (odeint_type != 0 && odeint_type != 3 && odeint_type != 6) {
+    set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, 6 dop853)",
+              odeint_type);
+    return -1;
+  }
+  GhPotTable table;
+  if (decode_potential(npot, pot_type, pot_args, &table) != 0) return -1;
+  hipStream_t stream;
+  GH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  void* d_table = nullptr;
+  double *d_yo = nullptr, *d_t = nullptr, *d_res = nullptr;
+  int* d_err = nullptr;
+  int rc = 0;
+  struct sigaction old_action, action;
+  std::memset(&action, 0, sizeof(action));
+  action.sa_handler = on_sigint;
+  irq_device_ptr();
+  if (g_irq_host) *g_irq_host = 0;
+  sigaction(SIGINT, &action, &old_action);
+  do {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { rc = -1; set_error("hipMemGetInfo failed"); break; }
+    const size_t per_orbit = (size_t)nt * 6 * 8 + 48 + 4 + (indiv_t ? (size_t)nt * 8 : 0);
+    size_t budget = free_b / 2;
+    long chunk = (long)std::min<size_t>((size_t)nobj, std::max<size_t>(1, budget / per_orbit));
+    if (hipMalloc(&d_table, sizeof(GhPotTable)) != hipSuccess ||
+        hipMalloc((void**)&d_yo, (size_t)chunk * 48) != hipSuccess ||
+        hipMalloc((void**)&d_t, (size_t)nt * 8 * (indiv_t ? chunk : 1)) != hipSuccess ||
+        hipMalloc((void**)&d_res, (size_t)chunk * nt * 48) != hipSuccess ||
+        hipMalloc((void**)&d_err, (size_t)chunk * 4) != hipSuccess) {
+      rc = -1;
+      set_error("device allocation failed (chunk of %ld orbits, nt=%d)", chunk, nt);
+      break;
+    }
+    if (hipMemcpyAsync(d_table, &table, sizeof(GhPotTable), hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+    if (!indiv_t &&
+        hipMemcpyAsync(d_t, t, (size_t)nt * 8, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+    for (long o0 = 0; o0 < nobj && rc == 0; o0 += chunk) {
+      const long n = std::min<long>(chunk, nobj - o0);
+      if (hipMemcpyAsync(d_yo, yo + 6 * o0, (size_t)n * 48, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+      if (indiv_t && hipMemcpyAsync(d_t, t + (size_t)nt * o0, (size_t)n * nt * 8,
+                                    hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+      if (launch_integrate((int)n, d_yo, nt, d_t, indiv_t, (const GhPotTable*)d_table, dt, rtol,
+                           atol, d_res, d_err, odeint_type, nullptr, stream) != 0) { rc = -1; break; }
+      if (hipMemcpyAsync(result + (size_t)6 * nt * o0, d_res, (size_t)n * nt * 48,
+                         hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipMemcpyAsync(err + o0, d_err, (size_t)n * 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipMemcpyAsync(yo + 6 * o0, d_yo, (size_t)n * 48, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipStreamSynchronize(stream) != hipSuccess) {
+        rc = -1;
+        set_error("device copy/sync failed: %s", hipGetErrorString(hipGetLastError()));
+        break;
+      }
+      if (cb)
+        for (long i = 0; i < n; i++) cb();
+    }
+  } while (0);
+  sigaction(SIGINT, &old_action, nullptr);
+  if (g_irq_host) *g_irq_host = 0;
+  (void)hipStreamSynchronize(stream);
+  (void)hipFree(d_table);
+  (void)hipFree(d_yo);
+  (void)hipFree(d_t);
+  (void)hipFree(d_res);
+  (void)hipFree(d_err);
+  (void)hipStreamDestroy(stream);
+  return rc;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+GH_EXPORT void integrateFullOrbit(int nobj, double* yo, int nt, double* t, int indiv_t, int npot,
+                                  int* pot_type, double* pot_args, void* pot_tfuncs, double dt,
+                                  double rtol, double atol, double* result, int* err,
+                                  int odeint_type, void (*cb)(void)) {
+  g_errbuf[0] = 0;
+  int rc;
+  if (pot_tfuncs != nullptr) {
+    set_error("pot_tfuncs must be NULL: no supported potential uses time functions");
+    rc = -1;
+  } else {
+    for (int i = 0; i < nobj; i++) err[i] = 0;
+    rc = host_integrate(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol,
+                        result, err, odeint_type, cb);
+  }
+  if (rc != 0) {
+    for (int i = 0; i < nobj; i++) err[i] = -1;
+    fprintf(stderr, "galpy_hip: integrateFullOrbit failed: %s\n", g_errbuf);
+  }
+}
+
+GH_EXPORT int integrateFullOrbit_hip(int nobj, double* d_yo, int nt, const double* d_t,
+                                     int indiv_t, const void* d_table, double dt, double rtol,
+                                     double atol, double* d_result, int* d_err, int odeint_type,
+                                     double* d_stats, void* stream) {
+  g_errbuf[0] = 0;
+  if (d_table == nullptr) {
+    set_error("d_table is NULL");
+    return -1;
+  }
+  return launch_integrate(nobj, d_yo, nt, d_t, indiv_t, (const GhPotTable*)d_table, dt, rtol, atol,
+                          d_result, d_err, odeint_type, d_stats, (hipStream_t)stream);
+}
+
+GH_EXPORT int gh_potential_upload(int npot, const int* pot_type, const double* pot_args,
+                                  void** d_table) {
+  g_errbuf[0] = 0;
+  GhPotTable table;
+  if (decode_potential(npot, pot_type, pot_args, &table) != 0) return -1;
+  void* d = nullptr;
+  GH_CHECK(hipMalloc(&d, sizeof(GhPotTable)));
+  GH_CHECK(hipMemcpy(d, &table, sizeof(GhPotTable), hipMemcpyHostToDevice));
+  *d_table = d;
+  return 0;
+}
+
+GH_EXPORT int gh_potential_free(void* d_table) {
+  GH_CHECK(hipFree(d_table));
+  return 0;
+}
+
+GH_EXPORT int gh_energy_drift_hip(int nobj, int nt, const double* d_rows, const double* d_t,
+                                  int indiv_t, const void* d_table, double omega, double* d_e0,
+                                  double* d_demax, void* stream) {
+  g_errbuf[0] = 0;
+  if (nobj <= 0) return 0;
+  const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
+  hipLaunchKernelGGL(gh_energy_kernel, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows, d_t,
+                     indiv_t, (const GhPotTable*)d_table, omega, d_e0, d_demax);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+GH_EXPORT int gh_eval_hip(int n, const double* d_R, const double* d_z, const double* d_phi,
+                          const double* d_t, const void* d_table, double* d_pot, double* d_FR,
+                          double* d_Fz, double* d_Fphi, void* stream) {
+  g_errbuf[0] = 0;
+  if (n <= 0) return 0;
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
+  hipLaunchKernelGGL(gh_eval_kernel, grid, block, 0, (hipStream_t)stream, n, d_R, d_z, d_phi, d_t,
+                     (const GhPotTable*)d_table, d_pot, d_FR, d_Fz, d_Fphi);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+GH_EXPORT const char* galpy_hip_last_error(void) { return g_errbuf; }
+
+GH_EXPORT void galpy_hip_set_interrupt(int value) {
+  irq_device_ptr();
+  if (g_irq_host) *g_irq_host = value;
+}

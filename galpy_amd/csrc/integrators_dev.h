@@ -1,0 +1,170 @@
+// Device integrators: leapfrog, symplec4 (+ step estimators) and DOP853,
+// one orbit per lane, FP64 state in registers.
+//
+// The arithmetic sequence follows galpy's C integrators
+// (galpy/util/bovy_symplecticode.c:132-377 and :588-808 for the symplectic
+// methods, galpy/util/leung_dop853.c:231-517 for DOP853) so that the GPU
+// reproduces galpy's trajectories to rounding, including the per-orbit step
+// size chosen by the estimators and the (long) truncation of init_dt/dt.
+#pragma once
+#include "potential_dev.h"
+
+#define GH_DT_SENTINEL (-9999.99)  // integrateFullOrbit.py:666-667
+#define GH_MAX_DT_REDUCE 10000.0   // bovy_symplecticode.c:35
+
+// symplec4 coefficients (Kinoshita et al.), bovy_symplecticode.c:96-99
+#define GH_S4C1 0.6756035959798289
+#define GH_S4C2 (-0.1756035959798288)
+#define GH_S4C3 (-0.1756035959798288)
+#define GH_S4C4 0.6756035959798289
+#define GH_S4D1 1.3512071919596578
+#define GH_S4D2 (-1.7024143839193153)
+#define GH_S4D3 1.3512071919596578
+
+struct GhVec3 {
+  double x, y, z;
+};
+
+__device__ __forceinline__ GhVec3 gh_drift(GhVec3 q, GhVec3 p, double h) {
+  return GhVec3{q.x + h * p.x, q.y + h * p.y, q.z + h * p.z};
+}
+
+__device__ __forceinline__ GhVec3 gh_accel(const GhPotTable* __restrict__ T, double t, GhVec3 q) {
+  GhVec3 a;
+  gh_rect_force(T, t, q.x, q.y, q.z, a.x, a.y, a.z);
+  return a;
+}
+
+// log-space scale of the step estimators, bovy_symplecticode.c:606-623
+__device__ __forceinline__ void gh_est_scales(GhVec3 q, GhVec3 p, double rtol, double atol,
+                                              double& sq, double& sp) {
+  const double mq = fmax(fmax(log(fabs(q.x)), log(fabs(q.y))), log(fabs(q.z)));
+  const double mp = fmax(fmax(log(fabs(p.x)), log(fabs(p.y))), log(fabs(p.z)));
+  double c = fmax(atol, rtol + mq);
+  sq = log(exp(atol - c) + exp(rtol + mq - c)) + c;
+  c = fmax(atol, rtol + mp);
+  sp = log(exp(atol - c) + exp(rtol + mp - c)) + c;
+}
+
+// error norm of the estimators, bovy_symplecticode.c:641-646 / 795-800
+__device__ __forceinline__ double gh_est_err(GhVec3 q11, GhVec3 q12, GhVec3 p11, GhVec3 p12,
+                                             double sq, double sp) {
+  double e = 0.0;
+  e += exp(2.0 * log(fabs(q11.x - q12.x)) - 2.0 * sq);
+  e += exp(2.0 * log(fabs(p11.x - p12.x)) - 2.0 * sp);
+  e += exp(2.0 * log(fabs(q11.y - q12.y)) - 2.0 * sq);
+  e += exp(2.0 * log(fabs(p11.y - p12.y)) - 2.0 * sp);
+  e += exp(2.0 * log(fabs(q11.z - q12.z)) - 2.0 * sq);
+  e += exp(2.0 * log(fabs(p11.z - p12.z)) - 2.0 * sp);
+  return sqrt(e / 2.0 / 3.0);
+}
+
+// leapfrog_estimate_step, bovy_symplecticode.c:588-663
+__device__ double gh_leapfrog_estimate_step(const GhPotTable* __restrict__ T, GhVec3 qo,
+                                            GhVec3 po, double dt, double to, double rtol,
+                                            double atol) {
+  double err = 2.0;
+  const double init_dt = dt;
+  double sq, sp;
+  gh_est_scales(qo, po, rtol, atol, sq, sp);
+  dt *= 2.0;
+  while (err > 1.0 && init_dt / dt < GH_MAX_DT_REDUCE) {
+    dt /= 2.0;
+    GhVec3 q12 = gh_drift(qo, po, dt / 2.0);
+    GhVec3 a = gh_accel(T, to + dt / 2.0, q12);
+    const GhVec3 p11 = gh_drift(po, a, dt);
+    const GhVec3 q11 = gh_drift(q12, p11, dt / 2.0);
+    q12 = gh_drift(qo, po, dt / 4.0);
+    a = gh_accel(T, to + dt / 4.0, q12);
+    const GhVec3 ptmp = gh_drift(po, a, dt / 2.0);
+    const GhVec3 qtmp = gh_drift(q12, ptmp, dt / 2.0);
+    a = gh_accel(T, to + 3.0 * dt / 4.0, qtmp);
+    const GhVec3 p12 = gh_drift(ptmp, a, dt / 2.0);
+    q12 = gh_drift(qtmp, p12, dt / 4.0);
+    err = gh_est_err(q11, q12, p11, p12, sq, sp);
+  }
+  return dt;
+}
+
+// symplec4_estimate_step, bovy_symplecticode.c:665-808
+__device__ double gh_symplec4_estimate_step(const GhPotTable* __restrict__ T, GhVec3 qo,
+                                            GhVec3 po, double dt, double to, double rtol,
+                                            double atol) {
+  const double c1 = GH_S4C1, c2 = GH_S4C2, c3 = GH_S4C3, c4 = GH_S4C4;
+  const double d1 = GH_S4D1, d2 = GH_S4D2, d3 = GH_S4D3;
+  double err = 2.0;
+  const double init_dt = dt;
+  double sq, sp;
+  gh_est_scales(qo, po, rtol, atol, sq, sp);
+  dt *= 2.0;
+  while (err > 1.0 && init_dt / dt < GH_MAX_DT_REDUCE) {
+    dt /= 2.0;
+    // one step of dt
+    GhVec3 q = gh_drift(qo, po, c1 * dt);
+    to += c1 * dt;
+    GhVec3 p = gh_drift(po, gh_accel(T, to, q), d1 * dt);
+    q = gh_drift(q, p, c2 * dt);
+    to += c2 * dt;
+    p = gh_drift(p, gh_accel(T, to, q), d2 * dt);
+    q = gh_drift(q, p, c3 * dt);
+    to += c3 * dt;
+    const GhVec3 p11 = gh_drift(p, gh_accel(T, to, q), d3 * dt);
+    const GhVec3 q11 = gh_drift(q, p11, c4 * dt);
+    to += c4 * dt;
+    to -= dt;
+    // two steps of dt/2
+    q = gh_drift(qo, po, c1 * dt / 2.0);
+    to += c1 * dt / 2.0;
+    p = gh_drift(po, gh_accel(T, to, q), d1 * dt / 2.0);
+    q = gh_drift(q, p, c2 * dt / 2.0);
+    to += c2 * dt / 2.0;
+    p = gh_drift(p, gh_accel(T, to, q), d2 * dt / 2.0);
+    q = gh_drift(q, p, c3 * dt / 2.0);
+    to += c3 * dt / 2.0;
+    p = gh_drift(p, gh_accel(T, to, q), d3 * dt / 2.0);
+    q = gh_drift(q, p, (c1 + c4) * dt / 2.0);
+    to += (c1 + c4) * dt / 2.0;
+    p = gh_drift(p, gh_accel(T, to, q), d1 * dt / 2.0);
+    q = gh_drift(q, p, c2 * dt / 2.0);
+    to += c2 * dt / 2.0;
+    p = gh_drift(p, gh_accel(T, to, q), d2 * dt / 2.0);
+    q = gh_drift(q, p, c3 * dt / 2.0);
+    to += c3 * dt / 2.0;
+    p = gh_drift(p, gh_accel(T, to, q), d3 * dt / 2.0);
+    q = gh_drift(q, p, c4 * dt / 2.0);
+    to += c4 * dt / 2.0;
+    err = gh_est_err(q11, q, p11, p, sq, sp);
+    to -= dt;
+  }
+  return dt;
+}
+
+// bovy_coords.c:101-115 (cylindrical -> rectangular)
+__device__ __forceinline__ void gh_cyl_to_rect(const double* v, double* w) {
+  double sp, cp;
+  sincos(v[5], &sp, &cp);
+  const double R = v[0], vR = v[1], vT = v[2];
+  w[0] = R * cp;
+  w[1] = R * sp;
+  w[2] = v[3];
+  w[3] = vR * cp - vT * sp;
+  w[4] = vR * sp + vT * cp;
+  w[5] = v[4];
+}
+
+// bovy_coords.c:125-139 (rectangular -> cylindrical, phi = atan2(y, x)),
+// stored as one 48-byte row
+__device__ __forceinline__ void gh_store_cyl(double* __restrict__ out, double x, double y,
+                                             double z, double vx, double vy, double vz) {
+  const double R = sqrt(x * x + y * y);
+  const double phi = atan2(y, x);
+  double cp = 1.0, sp = 0.0;
+  if (R > 0.0) {
+    cp = x / R;
+    sp = y / R;
+  }
+  double2* o = reinterpret_cast<double2*>(out);
+  o[0] = make_double2(R, vx * cp + vy * sp);
+  o[1] = make_double2(-vx * sp + vy * cp, z);
+  o[2] = make_double2(vz, phi);
+}

@@ -1,0 +1,302 @@
+// Device-side potential table and force kernels (FP64, one orbit per lane).
+//
+// galpy packs a potential as (npot, pot_type[], pot_args[])
+// (galpy/orbit/integrateFullOrbit.py:43-614) and its C driver turns that into
+// a per-thread array of function-pointer records that may cache values
+// (galpy/orbit/orbit_c_ext/integrateFullOrbit.c:69-1045,
+// galpy/potential/potential_c_ext/galpy_potentials.h:16-120).
+// Here the packed arrays are decoded once on the host into a compact,
+// read-only table in device memory.  Kernels index it with wavefront-uniform
+// addresses, so every parameter is fetched with scalar loads through the
+// scalar cache and broadcast to all 64 lanes.  Constants that galpy
+// recomputes on every call (Gamma(a), 2 pi rc^(3-alpha), 1/rc^2, the series
+// coefficients of the incomplete gamma function) are precomputed.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define GH_MAX_POT 8
+#define GH_IG_NSER 32  // terms of the incomplete-gamma series (enough for a <= 1.5)
+
+enum GhPotCode {
+  GH_LOGHALO = 0,
+  GH_DEHNENBAR = 1,
+  GH_MIYAMOTONAGAI = 5,
+  GH_HERNQUIST = 8,
+  GH_NFW = 9,
+  GH_POWERSPHCUTOFF = 15,
+};
+
+struct GhComponent {
+  int type;
+  int nonaxi;    // 1 if the component exerts a phi torque
+  double p[8];   // raw galpy arguments, in galpy's order
+  double c[10];  // precomputed constants, per type (see gh_decode_potential)
+};
+
+struct GhPotTable {
+  int npot;
+  int nonaxi;   // some component needs phi
+  int pad[2];
+  GhComponent comp[GH_MAX_POT];
+  // series coefficients 1/(a (a+1) ... (a+n)) of the incomplete gamma function
+  // for the force exponent a = 1.5 - alpha/2 ([0]) and the potential-value
+  // exponent 1 - alpha/2 ([1]) of PowerSphericalPotentialwCutoff components
+  double ser[GH_MAX_POT][2][GH_IG_NSER];
+};
+
+// ---------------------------------------------------------------------------
+// Lower incomplete gamma gamma(a, x) (replaces GSL's gsl_sf_gamma_inc_P and
+// gsl_sf_gamma_inc at PowerSphericalPotentialwCutoff.c:10,23).
+//   x < a+1 : gamma(a,x) = e^-x x^a sum_n x^n / (a (a+1) ... (a+n))
+//             (A&S 6.5.29), Horner over host-computed coefficients: no
+//             divisions, fixed trip count.
+//   x >= a+1: Gamma(a,x) by Legendre's continued fraction (A&S 6.5.31),
+//             modified Lentz; gamma = Gamma(a) - Gamma(a,x).
+//   x >= xbig: e^-x x^(a-1) < 2^-60 Gamma(a), so the upper tail cannot change
+//             the rounded result and gamma = Gamma(a).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double gh_lower_incgamma(double a, double x, double gamma_a,
+                                                   double xbig,
+                                                   const double* __restrict__ ser) {
+  if (x >= xbig) return gamma_a;
+  if (x <= 0.0) return 0.0;
+  const double pref = exp(a * log(x) - x);
+  if (x < a + 1.0) {
+    double s = ser[GH_IG_NSER - 1];
+#pragma unroll
+    for (int n = GH_IG_NSER - 2; n >= 0; n--) s = fma(s, x, ser[n]);
+    return pref * s;
+  }
+  const double tiny = 1e-300;
+  const double eps = 1.1102230246251565e-16;
+  double b = x + 1.0 - a;
+  double c = 1.0 / tiny;
+  double d = 1.0 / b;
+  double h = d;
+  for (int i = 1; i < 500; i++) {
+    const double an = -i * (i - a);
+    b += 2.0;
+    d = fma(an, d, b);
+    if (fabs(d) < tiny) d = tiny;
+    c = b + an / c;
+    if (fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < eps) break;
+  }
+  return gamma_a - pref * h;
+}
+
+// DehnenBarPotential.c:5-17 (quintic turn-on of the bar)
+__device__ __forceinline__ double gh_bar_smooth(double t, double tform, double tsteady) {
+  if (t < tform) return 0.0;
+  if (t < tsteady) {
+    const double xi = 2.0 * (t - tform) / (tsteady - tform) - 1.0;
+    const double xi2 = xi * xi;
+    const double xi3 = xi2 * xi;
+    return 3.0 / 16.0 * xi3 * xi2 - 5.0 / 8.0 * xi3 + 15.0 / 16.0 * xi + 0.5;
+  }
+  return 1.0;
+}
+
+// cos/sin of 2(phi - omegab t - barphi) from cos(2 phi), sin(2 phi) by angle
+// addition (galpy forms phi with acos, integrateFullOrbit.c:1316-1320)
+__device__ __forceinline__ void gh_bar_angle(double cos2p, double sin2p, double omegab,
+                                             double barphi, double t, double& cang,
+                                             double& sang) {
+  double sb, cb;
+  sincos(2.0 * (omegab * t + barphi), &sb, &cb);
+  cang = cos2p * cb + sin2p * sb;
+  sang = sin2p * cb - cos2p * sb;
+}
+
+struct GhCyl {
+  double FR, Fz, Fphi;
+};
+
+// Sum of cylindrical forces over components in index order
+// (galpy_potentials.c:120-179).  cosphi/sinphi = x/R, y/R.
+__device__ __forceinline__ GhCyl gh_cyl_forces(const GhPotTable* __restrict__ T, double R,
+                                               double R2, double z, double cosphi,
+                                               double sinphi, double t) {
+  GhCyl F = {0.0, 0.0, 0.0};
+  const double z2 = z * z;
+  const double r2 = R2 + z2;
+  const int npot = T->npot;
+  for (int i = 0; i < npot; i++) {
+    const GhComponent* __restrict__ C = &T->comp[i];
+    switch (C->type) {
+      case GH_POWERSPHCUTOFF: {
+        // PowerSphericalPotentialwCutoff.c:9-11,25-81:
+        //   F = -amp M(r2) / r^3 (R, z), M = 2 pi rc^(3-alpha) gamma(a, r2/rc^2)
+        const double x = r2 * C->c[3];
+        const double g = gh_lower_incgamma(C->c[0], x, C->c[1], C->c[4], T->ser[i][0]);
+        const double f = -C->c[2] * g / (r2 * sqrt(r2));
+        F.FR += f * R;
+        F.Fz += f * z;
+        break;
+      }
+      case GH_MIYAMOTONAGAI: {
+        // MiyamotoNagaiPotential.c:16-53
+        const double amp = C->p[0], a = C->p[1];
+        const double sqrtbz = sqrt(z2 + C->c[0]);
+        const double asqrtbz = a + sqrtbz;
+        const double D = R2 + asqrtbz * asqrtbz;
+        const double Dm32 = 1.0 / (D * sqrt(D));
+        F.FR += -amp * R * Dm32;
+        F.Fz += (a == 0.0) ? -amp * z * Dm32 : -amp * z * asqrtbz / sqrtbz * Dm32;
+        break;
+      }
+      case GH_NFW: {
+        // NFWPotential.c:16-49
+        const double amp = C->p[0], a = C->p[1];
+        const double r = sqrt(r2);
+        const double br = 1.0 / (r2 * (a + r)) - log1p(r * C->c[0]) / (r * r2);
+        F.FR += amp * R * br;
+        F.Fz += amp * z * br;
+        break;
+      }
+      case GH_HERNQUIST: {
+        // HernquistPotential.c:15-46: F = -amp x / (2 a^2 r) (1 + r/a)^-2
+        const double a = C->p[1];
+        const double r = sqrt(r2);
+        const double apr = a + r;
+        const double f = -C->c[0] / (r * apr * apr);
+        F.FR += f * R;
+        F.Fz += f * z;
+        break;
+      }
+      case GH_LOGHALO: {
+        // LogarithmicHaloPotential.c:21-91
+        const double amp = C->p[0], core2 = C->p[2], onem1overb2 = C->p[3];
+        const double zq2 = z2 * C->c[0];  // (z/q)^2
+        if (onem1overb2 < 1.0) {
+          const double Rt2 = R2 * (1.0 - onem1overb2 * sinphi * sinphi);
+          const double inv = 1.0 / (Rt2 + zq2 + core2);
+          F.FR += -amp * Rt2 / R * inv;
+          F.Fz += -amp * z * C->c[0] * inv;
+          F.Fphi += amp * R2 * inv * (2.0 * sinphi * cosphi) * onem1overb2 * 0.5;
+        } else {
+          const double inv = 1.0 / (R2 + zq2 + core2);
+          F.FR += -amp * R * inv;
+          F.Fz += -amp * z * C->c[0] * inv;
+        }
+        break;
+      }
+      case GH_DEHNENBAR: {
+        // DehnenBarPotential.c:18-124; args amp*Af, tform, tsteady, rb, omegab, barphi
+        const double amp = C->p[0], rb = C->p[3];
+        const double smooth = gh_bar_smooth(t, C->p[1], C->p[2]);
+        const double cos2p = cosphi * cosphi - sinphi * sinphi;
+        const double sin2p = 2.0 * sinphi * cosphi;
+        double cang, sang;
+        gh_bar_angle(cos2p, sin2p, C->p[4], C->p[5], t, cang, sang);
+        const double r = sqrt(r2);
+        const double r4 = r2 * r2;
+        const double as = amp * smooth;
+        if (r <= rb) {
+          const double u = r * C->c[0];  // r/rb
+          const double u3 = u * u * u;
+          F.FR += -as * cang * (u3 * R * (3.0 * R2 + 2.0 * z2) - 4.0 * R * z2) / r4;
+          F.Fphi += 2.0 * as * sang * (u3 - 2.0) * R2 / r2;
+          F.Fz += -as * cang * (u3 + 4.0) * R2 * z / r4;
+        } else {
+          const double u = rb / r;
+          const double u3 = u * u * u;
+          F.FR += -as * cang * u3 * R / r4 * (3.0 * R2 - 2.0 * z2);
+          F.Fphi += -2.0 * as * sang * u3 * R2 / r2;
+          F.Fz += -5.0 * as * cang * u3 * R2 * z / r4;
+        }
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  return F;
+}
+
+// evalRectForce (integrateFullOrbit.c:1307-1325): rectangular acceleration.
+__device__ __forceinline__ void gh_rect_force(const GhPotTable* __restrict__ T, double t,
+                                              double x, double y, double z, double& ax,
+                                              double& ay, double& az) {
+  const double R2 = x * x + y * y;
+  const double R = sqrt(R2);
+  const double invR = 1.0 / R;
+  const double cosphi = x * invR;
+  const double sinphi = y * invR;
+  const GhCyl F = gh_cyl_forces(T, R, R2, z, cosphi, sinphi, t);
+  ax = cosphi * F.FR - invR * sinphi * F.Fphi;
+  ay = sinphi * F.FR + invR * cosphi * F.Fphi;
+  az = F.Fz;
+}
+
+// Potential value (per-component Eval functions; DehnenBar from
+// DehnenBarPotential.py:173-200, which has no C Eval).
+__device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T, double R,
+                                               double z, double cosphi, double sinphi,
+                                               double t) {
+  double pot = 0.0;
+  const double R2 = R * R;
+  const double z2 = z * z;
+  const double r2 = R2 + z2;
+  for (int i = 0; i < T->npot; i++) {
+    const GhComponent* __restrict__ C = &T->comp[i];
+    switch (C->type) {
+      case GH_POWERSPHCUTOFF: {
+        // PowerSphericalPotentialwCutoff.c:12-24
+        const double r = sqrt(r2);
+        const double x = r2 * C->c[3];
+        const double g2 = gh_lower_incgamma(C->c[5], x, C->c[6], C->c[7], T->ser[i][1]);
+        const double g1 = gh_lower_incgamma(C->c[0], x, C->c[1], C->c[4], T->ser[i][0]);
+        pot += C->c[2] / r * (r * C->c[8] * g2 - g1);
+        break;
+      }
+      case GH_MIYAMOTONAGAI: {
+        const double as = C->p[1] + sqrt(z2 + C->c[0]);
+        pot += -C->p[0] / sqrt(R2 + as * as);
+        break;
+      }
+      case GH_NFW: {
+        const double r = sqrt(r2);
+        pot += -C->p[0] * log1p(r * C->c[0]) / r;
+        break;
+      }
+      case GH_HERNQUIST: {
+        const double r = sqrt(r2);
+        pot += -C->p[0] / (1.0 + r / C->p[1]) / 2.0 / C->p[1];
+        break;
+      }
+      case GH_LOGHALO: {
+        const double amp = C->p[0], core2 = C->p[2], onem1overb2 = C->p[3];
+        const double zq2 = z2 * C->c[0];
+        const double Rt2 = (onem1overb2 < 1.0) ? R2 * (1.0 - onem1overb2 * sinphi * sinphi) : R2;
+        pot += 0.5 * amp * log(Rt2 + zq2 + core2);
+        break;
+      }
+      case GH_DEHNENBAR: {
+        const double amp = C->p[0], rb = C->p[3];
+        const double smooth = gh_bar_smooth(t, C->p[1], C->p[2]);
+        const double cos2p = cosphi * cosphi - sinphi * sinphi;
+        const double sin2p = 2.0 * sinphi * cosphi;
+        double cang, sang;
+        gh_bar_angle(cos2p, sin2p, C->p[4], C->p[5], t, cang, sang);
+        const double r = sqrt(r2);
+        double out;
+        if (r <= rb) {
+          const double u = r / rb;
+          out = (u * u * u - 2.0) * R2 / r2;
+        } else {
+          const double u = rb / r;
+          out = -u * u * u * R2 / r2;
+        }
+        pot += out * amp * smooth * cang;
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  return pot;
+}

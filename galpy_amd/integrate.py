@@ -1,0 +1,196 @@
+"""Host front end of the HIP integrators (galpy's integrateFullOrbit_c twin).
+
+``integrateFullOrbit_hip`` keeps the argument meaning and return values of
+galpy's ``integrateFullOrbit_c`` (galpy/orbit/integrateFullOrbit.py:616-749):
+packing through ``parse_pot``, tolerances as natural logs
+(integratePlanarOrbit.py:712-724), ``dt=None`` -> -9999.99, a preallocated
+``(nobj, nt, 6)`` result, a per-orbit ``err``, and ``KeyboardInterrupt`` when
+an orbit reports -10.  The call goes through ctypes to the C-ABI symbol
+``integrateFullOrbit`` of libgalpy_hip.so.
+
+``DeviceIntegrator`` is the HBM-resident path: inputs and outputs are torch
+tensors on the GPU and the kernels run asynchronously on torch's current
+stream (used by the benchmark, batching and serving layers).
+"""
+
+import ctypes
+
+import numpy
+
+from . import _lib
+from .potential import parse_pot
+
+DT_SENTINEL = -9999.99
+
+METHOD_CODES = {
+    "leapfrog_hip": 0,
+    "symplec4_hip": 3,
+    "dop853_hip": 6,
+}
+
+
+def _parse_integrator(int_method):
+    """integratePlanarOrbit.py:690-709 for the HIP method names (unknown -> 0)."""
+    return METHOD_CODES.get(int_method.lower(), 0)
+
+
+def _parse_tol(rtol, atol):
+    """integratePlanarOrbit.py:712-724: natural logs, default 1e-12."""
+    rtol = -12.0 * numpy.log(10.0) if rtol is None else numpy.log(rtol)
+    atol = -12.0 * numpy.log(10.0) if atol is None else numpy.log(atol)
+    return (rtol, atol)
+
+
+def integrateFullOrbit_hip(pot, yo, t, int_method, rtol=None, atol=None, progressbar=False, dt=None):
+    """Integrate N full 3D orbits on the GPU.
+
+    Parameters mirror galpy's ``integrateFullOrbit_c``: ``yo`` is ``[N,6]`` or
+    ``[6]`` in (R,vR,vT,z,vz,phi); ``t`` is 1D (shared) or ``[N,nt]``.
+    Returns ``(result, err)`` with ``result`` of shape ``(N,nt,6)`` (or
+    ``(nt,6)`` for a single orbit).
+    """
+    lib = _lib.load()
+    single_obj = len(numpy.shape(yo)) == 1
+    yo = numpy.require(numpy.atleast_2d(yo), dtype=numpy.float64, requirements=["C", "W"])
+    nobj = len(yo)
+    rtol, atol = _parse_tol(rtol, atol)
+    npot, pot_type, pot_args, _ = parse_pot(pot)
+    int_method_c = _parse_integrator(int_method)
+    if dt is None:
+        dt = DT_SENTINEL
+    t = numpy.require(t, dtype=numpy.float64, requirements=["C", "W"])
+    indiv_t = t.ndim > 1
+    nt = t.shape[-1]
+    result = numpy.empty((nobj, nt, 6))
+    err = numpy.zeros(nobj, dtype=numpy.int32)
+    cb = None
+    if progressbar and nobj > 1:
+        try:
+            import tqdm
+
+            pbar = tqdm.tqdm(total=nobj, leave=False)
+            cb = ctypes.CFUNCTYPE(None)(pbar.update)
+        except ImportError:  # pragma: no cover
+            cb = None
+    lib.integrateFullOrbit(
+        ctypes.c_int(nobj), yo, ctypes.c_int(nt), t, ctypes.c_int(int(indiv_t)),
+        ctypes.c_int(npot), pot_type, pot_args, None, ctypes.c_double(dt),
+        ctypes.c_double(rtol), ctypes.c_double(atol), result, err,
+        ctypes.c_int(int_method_c), ctypes.cast(cb, ctypes.c_void_p) if cb is not None else None,
+    )
+    if cb is not None:
+        pbar.close()
+    if numpy.any(err == -10):
+        raise KeyboardInterrupt("Orbit integration interrupted by CTRL-C (SIGINT)")
+    if numpy.any(err == -1):
+        raise RuntimeError(f"HIP orbit integration failed: {_lib.last_error()}")
+    if single_obj:
+        return (result[0], err[0])
+    return (result, err)
+
+
+class PotentialTable:
+    """A potential decoded into a device-resident table (gh_potential_upload)."""
+
+    def __init__(self, pot):
+        self.pot = pot
+        npot, pot_type, pot_args, _ = parse_pot(pot)
+        self.npot, self.pot_type, self.pot_args = npot, pot_type, pot_args
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(
+            lib.gh_potential_upload(
+                npot, pot_type.ctypes.data, pot_args.ctypes.data, ctypes.byref(handle)
+            ),
+            "gh_potential_upload",
+        )
+        self.handle = handle
+
+    @property
+    def ptr(self):
+        return self.handle.value
+
+    def __del__(self):
+        try:
+            if self.handle is not None and self.handle.value:
+                _lib.load().gh_potential_free(self.handle)
+                self.handle = None
+        except Exception:  # interpreter shutdown
+            pass
+
+
+def _ptr(x):
+    return 0 if x is None else x.data_ptr()
+
+
+def integrate_device(table, yo, t, int_method, rtol=None, atol=None, dt=None, result=None,
+                     err=None, stats=None, stream=None):
+    """Launch an integration on device tensors (asynchronous on ``stream``).
+
+    ``yo``: float64 ``[N,6]`` cuda tensor (overwritten with the rectangular
+    state, as galpy does), ``t``: ``[nt]`` or ``[N,nt]``; ``result`` ``[N,nt,6]``,
+    ``err`` int32 ``[N]`` and ``stats`` float64 ``[N,3]`` are allocated when
+    not given.  Returns ``(result, err, stats)``.
+    """
+    import torch
+
+    assert yo.is_cuda and yo.dtype == torch.float64 and yo.is_contiguous()
+    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+    nobj = yo.shape[0]
+    indiv_t = t.dim() > 1
+    nt = t.shape[-1]
+    if indiv_t:
+        assert t.shape == (nobj, nt)
+    if result is None:
+        result = torch.empty((nobj, nt, 6), dtype=torch.float64, device=yo.device)
+    if err is None:
+        err = torch.zeros(nobj, dtype=torch.int32, device=yo.device)
+    assert result.shape == (nobj, nt, 6) and result.is_contiguous()
+    assert err.shape == (nobj,) and err.dtype == torch.int32
+    if stats is not None:
+        assert stats.shape == (nobj, 3) and stats.dtype == torch.float64
+    rtol, atol = _parse_tol(rtol, atol)
+    if stream is None:
+        stream = torch.cuda.current_stream(yo.device)
+    code = METHOD_CODES[int_method] if isinstance(int_method, str) else int(int_method)
+    lib = _lib.load()
+    rc = lib.integrateFullOrbit_hip(
+        nobj, yo.data_ptr(), nt, t.data_ptr(), int(indiv_t), table.ptr,
+        DT_SENTINEL if dt is None else float(dt), rtol, atol, result.data_ptr(), err.data_ptr(),
+        code, _ptr(stats), stream.cuda_stream,
+    )
+    _lib.check(rc, "integrateFullOrbit_hip")
+    return result, err, stats
+
+
+def energy_drift_device(table, rows, t, omega=0.0, stream=None):
+    """Per-orbit E(0) and max |E(t)-E(0)|/|E(0)| (Jacobi with ``omega``) on device."""
+    import torch
+
+    nobj, nt, _ = rows.shape
+    e0 = torch.empty(nobj, dtype=torch.float64, device=rows.device)
+    demax = torch.empty(nobj, dtype=torch.float64, device=rows.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(rows.device)
+    rc = _lib.load().gh_energy_drift_hip(
+        nobj, nt, rows.data_ptr(), t.data_ptr(), int(t.dim() > 1), table.ptr, float(omega),
+        e0.data_ptr(), demax.data_ptr(), stream.cuda_stream,
+    )
+    _lib.check(rc, "gh_energy_drift_hip")
+    return e0, demax
+
+
+def eval_device(table, R, z, phi, t, stream=None):
+    """Potential value and cylindrical forces at points (device tensors)."""
+    import torch
+
+    n = R.numel()
+    outs = [torch.empty(n, dtype=torch.float64, device=R.device) for _ in range(4)]
+    if stream is None:
+        stream = torch.cuda.current_stream(R.device)
+    rc = _lib.load().gh_eval_hip(
+        n, R.data_ptr(), z.data_ptr(), phi.data_ptr(), t.data_ptr(), table.ptr,
+        *[o.data_ptr() for o in outs], stream.cuda_stream,
+    )
+    _lib.check(rc, "gh_eval_hip")
+    return tuple(outs)

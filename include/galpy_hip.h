@@ -1,0 +1,89 @@
+/*
+ * galpy_hip.h -- C ABI of the MI355X (gfx950) orbit-integration engine.
+ *
+ * Drop-in boundary for galpy's C orbit integration: the host packs the
+ * potential exactly as galpy's `_parse_pot` does (integer pot_type codes,
+ * float64 pot_args), and calls one function for all orbits, as galpy's
+ * Python layer calls `integrateFullOrbit` through ctypes
+ * (galpy/orbit/integrateFullOrbit.py:687-734).  Plain pointers and sizes
+ * only; no HIP or torch types appear in these signatures.
+ *
+ * Supported potentials (pot_type codes, integrateFullOrbit.c:80-345):
+ *   0 LogarithmicHaloPotential, 1 DehnenBarPotential, 5 MiyamotoNagaiPotential,
+ *   8 HernquistPotential, 9 NFWPotential, 15 PowerSphericalPotentialwCutoff.
+ * Supported odeint_type codes (integrateFullOrbit.c:1091-1132):
+ *   0 leapfrog, 3 symplec4, 6 dop853.
+ * Any other code, or nt < 2, sets err[i] = -1 for every orbit and records a
+ * message readable with galpy_hip_last_error().
+ */
+#ifndef GALPY_HIP_H
+#define GALPY_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces galpy/orbit/orbit_c_ext/integrateFullOrbit.c:1046-1061
+ * (`integrateFullOrbit`), same signature and argument meaning:
+ *   yo[nobj*6]      initial conditions (R,vR,vT,z,vz,phi); like galpy, it is
+ *                   overwritten with the rectangular state (the initial one
+ *                   for leapfrog/symplec4, the last integrator state for
+ *                   dop853) -- callers pass a copy
+ *   t[nt] or t[nobj*nt] (indiv_t=1) output times, equally spaced
+ *   dt              fixed step, or -9999.99 to estimate one per orbit
+ *   rtol, atol      natural logarithms of the tolerances
+ *   result[nobj*nt*6] output rows (R,vR,vT,z,vz,phi); row 0 is the IC
+ *   err[nobj]       0 ok, -10 interrupted by SIGINT, -1 unsupported input
+ *   pot_tfuncs      must be NULL (no supported potential uses time functions)
+ *   cb              called once per finished orbit (may be NULL)
+ * All pointers are host memory.  Work is streamed through the GPU in chunks
+ * of orbits sized to free device memory. */
+void integrateFullOrbit(int nobj, double *yo, int nt, double *t, int indiv_t, int npot,
+                        int *pot_type, double *pot_args, void *pot_tfuncs, double dt,
+                        double rtol, double atol, double *result, int *err, int odeint_type,
+                        void (*cb)(void));
+
+/* Device-resident twin of integrateFullOrbit for batching/serving callers
+ * (no reference counterpart; it is integrateFullOrbit.c:1046 with the
+ * buffers already in HBM).  d_yo, d_t, d_result, d_err and d_stats are device
+ * pointers; d_table comes from gh_potential_upload.  d_stats (nullable) gets
+ * 3 doubles per orbit: orbit-steps (accepted steps for dop853), rejected
+ * steps, step size used (symplectic).  stream is a hipStream_t (NULL = the
+ * default stream).  The call is asynchronous with respect to the host.
+ * Returns 0 on success, nonzero on a launch/argument error. */
+int integrateFullOrbit_hip(int nobj, double *d_yo, int nt, const double *d_t, int indiv_t,
+                           const void *d_table, double dt, double rtol, double atol,
+                           double *d_result, int *d_err, int odeint_type, double *d_stats,
+                           void *stream);
+
+/* Decodes galpy's packed potential (integrateFullOrbit.c:69-1045 semantics)
+ * into a device-resident table; *d_table receives a device pointer. */
+int gh_potential_upload(int npot, const int *pot_type, const double *pot_args, void **d_table);
+int gh_potential_free(void *d_table);
+
+/* Energy (omega = 0) or Jacobi integral E - omega*Lz along stored
+ * trajectories (Orbits.py:2902 E, :3298 Jacobi): d_rows[nobj*nt*6]
+ * cylindrical rows, d_t[nt] (or [nobj*nt] with indiv_t).  Writes per orbit
+ * d_e0[i] = integral at row 0 and d_demax[i] = max_j |E_j - E_0| / |E_0|.
+ * Asynchronous on stream. */
+int gh_energy_drift_hip(int nobj, int nt, const double *d_rows, const double *d_t, int indiv_t,
+                        const void *d_table, double omega, double *d_e0, double *d_demax,
+                        void *stream);
+
+/* Potential value and cylindrical forces at n points (device pointers). */
+int gh_eval_hip(int n, const double *d_R, const double *d_z, const double *d_phi,
+                const double *d_t, const void *d_table, double *d_pot, double *d_FR,
+                double *d_Fz, double *d_Fphi, void *stream);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char *galpy_hip_last_error(void);
+
+/* Sets the interrupt flag polled by running kernels (as galpy's SIGINT
+ * handler sets `interrupted`, bovy_symplecticode.c:38-45); orbits that see
+ * it stop with err = -10.  Pass 0 to clear. */
+void galpy_hip_set_interrupt(int value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GALPY_HIP_H */

@@ -1,0 +1,510 @@
+// CPU oracle: integrators -- TEST INFRASTRUCTURE ONLY (see oracle.hpp).
+//
+// Restates galpy's leapfrog / symplec4 (+ step estimators) from
+// galpy/util/bovy_symplecticode.c and DOP853 from galpy/util/leung_dop853.c,
+// keeping the reference's arithmetic sequence so results agree to rounding.
+// Templated on the scalar type R (double, or the op-counting CountedD).
+#pragma once
+#include "oracle.hpp"
+
+namespace orc {
+
+constexpr double kDtSentinel = -9999.99;      // integrateFullOrbit.py:666-667
+constexpr double kMaxDtReduce = 10000.;       // bovy_symplecticode.c:35
+
+// symplec4 coefficients, bovy_symplecticode.c:96-99 (Kinoshita et al.)
+constexpr double S4C1 = 0.6756035959798289, S4C2 = -0.1756035959798288,
+                 S4C3 = -0.1756035959798288, S4C4 = 0.6756035959798289;
+constexpr double S4D1 = 1.3512071919596578, S4D2 = -1.7024143839193153,
+                 S4D3 = 1.3512071919596578;
+
+struct IntStats {
+  long long steps;     // orbit-steps: ndt*(nt-1) (symplectic) / accepted (DOP853)
+  long long rejected;  // DOP853 rejected steps
+  long long rhs;       // force / RHS evaluations
+  double dt;           // step actually used (symplectic)
+};
+
+template <class R> struct ForceCtx {
+  int npot;
+  const Component* P;
+  long long* nrhs;
+  void force(R t, const R* q, R* a) const { (*nrhs)++; rect_force(npot, P, t, q, a); }
+  void deriv(R t, const R* q, R* a) const { (*nrhs)++; rect_deriv(npot, P, t, q, a); }
+};
+
+// bovy_symplecticode.c:63-72
+template <class R> inline void leapq(const R* q, const R* p, R dt, R* qn) {
+  for (int i = 0; i < 3; i++) qn[i] = q[i] + dt * p[i];
+}
+template <class R> inline void leapp(const R* p, R dt, const R* a, R* pn) {
+  for (int i = 0; i < 3; i++) pn[i] = p[i] + dt * a[i];
+}
+
+// Shared scale set-up of the step estimators, bovy_symplecticode.c:606-623
+template <class R> void estimate_scales(const R* qo, const R* po, double rtol, double atol,
+                                        R& sq, R& sp) {
+  R mq = log(fabs(qo[0]));
+  for (int i = 1; i < 3; i++) if (log(fabs(qo[i])) > mq) mq = log(fabs(qo[i]));
+  R mp = log(fabs(po[0]));
+  for (int i = 1; i < 3; i++) if (log(fabs(po[i])) > mp) mp = log(fabs(po[i]));
+  R c = rmax(R(atol), R(rtol) + mq);
+  sq = log(exp(R(atol) - c) + exp(R(rtol) + mq - c)) + c;
+  c = rmax(R(atol), R(rtol) + mp);
+  sp = log(exp(R(atol) - c) + exp(R(rtol) + mp - c)) + c;
+}
+
+// error norm, bovy_symplecticode.c:641-646 / 795-800
+template <class R> R estimate_err(const R* q11, const R* q12, const R* p11, const R* p12, R sq, R sp) {
+  R err = 0.;
+  for (int i = 0; i < 3; i++) {
+    err += exp(R(2.) * log(fabs(q11[i] - q12[i])) - R(2.) * sq);
+    err += exp(R(2.) * log(fabs(p11[i] - p12[i])) - R(2.) * sp);
+  }
+  return sqrt(err / R(2.) / R(3.));
+}
+
+// leapfrog_estimate_step, bovy_symplecticode.c:588-663
+template <class R>
+R leapfrog_estimate_step(const ForceCtx<R>& F, const R* qo, const R* po, R dt, R to,
+                         double rtol, double atol) {
+  R err = 2.;
+  R init_dt = dt;
+  R q11[3], q12[3], p11[3], p12[3], qtmp[3], ptmp[3], a[3];
+  R sq, sp;
+  estimate_scales(qo, po, rtol, atol, sq, sp);
+  dt *= 2.;
+  while (err > R(1.) && init_dt / dt < R(kMaxDtReduce)) {
+    dt /= 2.;
+    leapq(qo, po, dt / R(2.), q12);
+    F.force(to + dt / R(2.), q12, a);
+    leapp(po, dt, a, p11);
+    leapq(q12, p11, dt / R(2.), q11);
+    leapq(qo, po, dt / R(4.), q12);
+    F.force(to + dt / R(4.), q12, a);
+    leapp(po, dt / R(2.), a, ptmp);
+    leapq(q12, ptmp, dt / R(2.), qtmp);
+    F.force(to + R(3.) * dt / R(4.), qtmp, a);
+    leapp(ptmp, dt / R(2.), a, p12);
+    leapq(qtmp, p12, dt / R(4.), q12);
+    err = estimate_err(q11, q12, p11, p12, sq, sp);
+  }
+  return dt;
+}
+
+// symplec4_estimate_step, bovy_symplecticode.c:665-808
+template <class R>
+R symplec4_estimate_step(const ForceCtx<R>& F, const R* qo, const R* po, R dt, R to,
+                         double rtol, double atol) {
+  const R c1 = S4C1, c2 = S4C2, c3 = S4C3, c4 = S4C4, d1 = S4D1, d2 = S4D2, d3 = S4D3;
+  R err = 2.;
+  R init_dt = dt;
+  R q11[3], q12[3], p11[3], p12[3], qtmp[3], ptmp[3], a[3];
+  R sq, sp;
+  estimate_scales(qo, po, rtol, atol, sq, sp);
+  dt *= 2.;
+  while (err > R(1.) && init_dt / dt < R(kMaxDtReduce)) {
+    dt /= 2.;
+    // one step of dt
+    leapq(qo, po, c1 * dt, q12);
+    to += c1 * dt;
+    F.force(to, q12, a);
+    leapp(po, d1 * dt, a, p12);
+    leapq(q12, p12, c2 * dt, qtmp);
+    to += c2 * dt;
+    F.force(to, qtmp, a);
+    leapp(p12, d2 * dt, a, ptmp);
+    leapq(qtmp, ptmp, c3 * dt, q12);
+    to += c3 * dt;
+    F.force(to, q12, a);
+    leapp(ptmp, d3 * dt, a, p11);
+    leapq(q12, p11, c4 * dt, q11);
+    to += c4 * dt;
+    to -= dt;
+    // two steps of dt/2
+    leapq(qo, po, c1 * dt / R(2.), q12);
+    to += c1 * dt / R(2.);
+    F.force(to, q12, a);
+    leapp(po, d1 * dt / R(2.), a, p12);
+    leapq(q12, p12, c2 * dt / R(2.), qtmp);
+    to += c2 * dt / R(2.);
+    F.force(to, qtmp, a);
+    leapp(p12, d2 * dt / R(2.), a, ptmp);
+    leapq(qtmp, ptmp, c3 * dt / R(2.), q12);
+    to += c3 * dt / R(2.);
+    F.force(to, q12, a);
+    leapp(ptmp, d3 * dt / R(2.), a, p12);
+    leapq(q12, p12, (c1 + c4) * dt / R(2.), qtmp);
+    to += (c1 + c4) * dt / R(2.);
+    F.force(to, qtmp, a);
+    leapp(p12, d1 * dt / R(2.), a, ptmp);
+    leapq(qtmp, ptmp, c2 * dt / R(2.), q12);
+    to += c2 * dt / R(2.);
+    F.force(to, q12, a);
+    leapp(ptmp, d2 * dt / R(2.), a, p12);
+    leapq(q12, p12, c3 * dt / R(2.), qtmp);
+    to += c3 * dt / R(2.);
+    F.force(to, qtmp, a);
+    leapp(p12, d3 * dt / R(2.), a, ptmp);
+    leapq(qtmp, ptmp, c4 * dt / R(2.), q12);
+    to += c4 * dt / R(2.);
+    for (int i = 0; i < 3; i++) p12[i] = ptmp[i];
+    err = estimate_err(q11, q12, p11, p12, sq, sp);
+    to -= dt;
+  }
+  return dt;
+}
+
+template <class R> inline void save6(const R* q, const R* p, double* out) {
+  for (int i = 0; i < 3; i++) { out[i] = dval(q[i]); out[3 + i] = dval(p[i]); }
+}
+
+// leapfrog, bovy_symplecticode.c:132-224.  yo = [x,y,z,vx,vy,vz];
+// result = nt rows of 6; returns the step count in st.
+template <class R>
+void leapfrog(const ForceCtx<R>& F, const double* yo, int nt, double dt_in, const double* t,
+              double rtol, double atol, double* result, int* err, IntStats* st) {
+  R qo[3], po[3], q12[3], p12[3], a[3];
+  for (int i = 0; i < 3; i++) { qo[i] = yo[i]; po[i] = yo[3 + i]; }
+  save6(qo, po, result);
+  result += 6;
+  *err = 0;
+  R init_dt = R(t[1]) - R(t[0]);
+  R dt = dt_in;
+  if (dt_in == kDtSentinel) dt = leapfrog_estimate_step(F, qo, po, init_dt, R(t[0]), rtol, atol);
+  long ndt = (long)(dval(init_dt / dt));
+  R to = t[0];
+  for (int ii = 0; ii < nt - 1; ii++) {
+    leapq(qo, po, dt / R(2.), q12);
+    for (long jj = 0; jj < ndt - 1; jj++) {
+      F.force(to + dt / R(2.), q12, a);
+      leapp(po, dt, a, p12);
+      leapq(q12, p12, dt, qo);
+      to = to + dt;
+      for (int k = 0; k < 3; k++) { q12[k] = qo[k]; po[k] = p12[k]; }
+    }
+    F.force(to + dt / R(2.), q12, a);
+    leapp(po, dt, a, po);
+    leapq(q12, po, dt / R(2.), qo);
+    to = to + dt;
+    save6(qo, po, result);
+    result += 6;
+  }
+  if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
+}
+
+// symplec4, bovy_symplecticode.c:250-377
+template <class R>
+void symplec4(const ForceCtx<R>& F, const double* yo, int nt, double dt_in, const double* t,
+              double rtol, double atol, double* result, int* err, IntStats* st) {
+  const R c1 = S4C1, c2 = S4C2, c3 = S4C3, c4 = S4C4, d1 = S4D1, d2 = S4D2, d3 = S4D3;
+  R qo[3], po[3], q12[3], p12[3], a[3];
+  for (int i = 0; i < 3; i++) { qo[i] = yo[i]; po[i] = yo[3 + i]; }
+  save6(qo, po, result);
+  result += 6;
+  *err = 0;
+  R init_dt = R(t[1]) - R(t[0]);
+  R dt = dt_in;
+  if (dt_in == kDtSentinel) dt = symplec4_estimate_step(F, qo, po, init_dt, R(t[0]), rtol, atol);
+  long ndt = (long)(dval(init_dt / dt));
+  R to = t[0];
+  for (int ii = 0; ii < nt - 1; ii++) {
+    leapq(qo, po, c1 * dt, q12);
+    to += c1 * dt;
+    for (long jj = 0; jj < ndt - 1; jj++) {
+      F.force(to, q12, a);
+      leapp(po, d1 * dt, a, p12);
+      leapq(q12, p12, c2 * dt, qo);
+      to += c2 * dt;
+      F.force(to, qo, a);
+      leapp(p12, d2 * dt, a, po);
+      leapq(qo, po, c3 * dt, q12);
+      to += c3 * dt;
+      F.force(to, q12, a);
+      leapp(po, d3 * dt, a, p12);
+      leapq(q12, p12, (c4 + c1) * dt, qo);
+      to += (c4 + c1) * dt;
+      for (int k = 0; k < 3; k++) { q12[k] = qo[k]; po[k] = p12[k]; }
+    }
+    F.force(to, q12, a);
+    leapp(po, d1 * dt, a, p12);
+    leapq(q12, p12, c2 * dt, qo);
+    to += c2 * dt;
+    F.force(to, qo, a);
+    leapp(p12, d2 * dt, a, po);
+    leapq(qo, po, c3 * dt, q12);
+    to += c3 * dt;
+    F.force(to, q12, a);
+    leapp(po, d3 * dt, a, p12);
+    leapq(q12, p12, c4 * dt, qo);
+    to += c4 * dt;
+    for (int k = 0; k < 3; k++) po[k] = p12[k];
+    save6(qo, po, result);
+    result += 6;
+  }
+  if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
+}
+
+// ---------------------------------------------------------------------------
+// DOP853 tableau (Hairer, Norsett & Wanner, "Solving ODEs I", dop853.f);
+// the same published constants galpy uses at leung_dop853.c:41-205.
+// ---------------------------------------------------------------------------
+namespace dp {
+constexpr double c2 = 0.526001519587677318785587544488e-1, c3 = 0.789002279381515978178381316732e-1,
+  c4 = 0.118350341907227396726757197510, c5 = 0.281649658092772603273242802490,
+  c6 = 0.333333333333333333333333333333, c7 = 0.25, c8 = 0.307692307692307692307692307692,
+  c9 = 0.651282051282051282051282051282, c10 = 0.6, c11 = 0.857142857142857142857142857142,
+  c14 = 0.1, c15 = 0.2, c16 = 0.777777777777777777777777777778;
+constexpr double a21 = 5.26001519587677318785587544488e-2,
+  a31 = 1.97250569845378994544595329183e-2, a32 = 5.91751709536136983633785987549e-2,
+  a41 = 2.95875854768068491816892993775e-2, a43 = 8.87627564304205475450678981324e-2,
+  a51 = 2.41365134159266685502369798665e-1, a53 = -8.84549479328286085344864962717e-1,
+  a54 = 9.24834003261792003115737966543e-1,
+  a61 = 3.7037037037037037037037037037e-2, a64 = 1.70828608729473871279604482173e-1,
+  a65 = 1.25467687566822425016691814123e-1,
+  a71 = 3.7109375e-2, a74 = 1.70252211019544039314978060272e-1,
+  a75 = 6.02165389804559606850219397283e-2, a76 = -1.7578125e-2,
+  a81 = 3.70920001185047927108779319836e-2, a84 = 1.70383925712239993810214054705e-1,
+  a85 = 1.07262030446373284651809199168e-1, a86 = -1.53194377486244017527936158236e-2,
+  a87 = 8.27378916381402288758473766002e-3,
+  a91 = 6.24110958716075717114429577812e-1, a94 = -3.36089262944694129406857109825e0,
+  a95 = -8.68219346841726006818189891453e-1, a96 = 2.75920996994467083049415600797e1,
+  a97 = 2.01540675504778934086186788979e1, a98 = -4.34898841810699588477366255144e1,
+  a101 = 4.77662536438264365890433908527e-1, a104 = -2.48811461997166764192642586468e0,
+  a105 = -5.90290826836842996371446475743e-1, a106 = 2.12300514481811942347288949897e1,
+  a107 = 1.52792336328824235832596922938e1, a108 = -3.32882109689848629194453265587e1,
+  a109 = -2.03312017085086261358222928593e-2,
+  a111 = -9.3714243008598732571704021658e-1, a114 = 5.18637242884406370830023853209e0,
+  a115 = 1.09143734899672957818500254654e0, a116 = -8.14978701074692612513997267357e0,
+  a117 = -1.85200656599969598641566180701e1, a118 = 2.27394870993505042818970056734e1,
+  a119 = 2.49360555267965238987089396762e0, a1110 = -3.0467644718982195003823669022e0,
+  a121 = 2.27331014751653820792359768449e0, a124 = -1.05344954667372501984066689879e1,
+  a125 = -2.00087205822486249909675718444e0, a126 = -1.79589318631187989172765950534e1,
+  a127 = 2.79488845294199600508499808837e1, a128 = -2.85899827713502369474065508674e0,
+  a129 = -8.87285693353062954433549289258e0, a1210 = 1.23605671757943030647266201528e1,
+  a1211 = 6.43392746015763530355970484046e-1,
+  a141 = 5.61675022830479523392909219681e-2, a147 = 2.53500210216624811088794765333e-1,
+  a148 = -2.46239037470802489917441475441e-1, a149 = -1.24191423263816360469010140626e-1,
+  a1410 = 1.5329179827876569731206322685e-1, a1411 = 8.20105229563468988491666602057e-3,
+  a1412 = 7.56789766054569976138603589584e-3, a1413 = -8.298e-3,
+  a151 = 3.18346481635021405060768473261e-2, a156 = 2.83009096723667755288322961402e-2,
+  a157 = 5.35419883074385676223797384372e-2, a158 = -5.49237485713909884646569340306e-2,
+  a1511 = -1.08347328697249322858509316994e-4, a1512 = 3.82571090835658412954920192323e-4,
+  a1513 = -3.40465008687404560802977114492e-4, a1514 = 1.41312443674632500278074618366e-1,
+  a161 = -4.28896301583791923408573538692e-1, a166 = -4.69762141536116384314449447206e0,
+  a167 = 7.68342119606259904184240953878e0, a168 = 4.06898981839711007970213554331e0,
+  a169 = 3.56727187455281109270669543021e-1, a1613 = -1.39902416515901462129418009734e-3,
+  a1614 = 2.9475147891527723389556272149e0, a1615 = -9.15095847217987001081870187138e0;
+constexpr double b1 = 5.42937341165687622380535766363e-2, b6 = 4.45031289275240888144113950566,
+  b7 = 1.89151789931450038304281599044, b8 = -5.8012039600105847814672114227,
+  b9 = 3.1116436695781989440891606237e-1, b10 = -1.52160949662516078556178806805e-1,
+  b11 = 2.01365400804030348374776537501e-1, b12 = 4.47106157277725905176885569043e-2;
+constexpr double bhh1 = 0.244094488188976377952755905512, bhh2 = 0.733846688281611857341361741547,
+  bhh3 = 0.220588235294117647058823529412e-1;
+constexpr double er1 = 0.1312004499419488073250102996e-1, er6 = -0.1225156446376204440720569753e+1,
+  er7 = -0.4957589496572501915214079952, er8 = 0.1664377182454986536961530415e+1,
+  er9 = -0.3503288487499736816886487290, er10 = 0.3341791187130174790297318841,
+  er11 = 0.8192320648511571246570742613e-1, er12 = -0.2235530786388629525884427845e-1;
+constexpr double d41 = -0.84289382761090128651353491142e+1, d46 = 0.56671495351937776962531783590,
+  d47 = -0.30689499459498916912797304727e+1, d48 = 0.23846676565120698287728149680e+1,
+  d49 = 0.21170345824450282767155149946e+1, d410 = -0.87139158377797299206789907490,
+  d411 = 0.22404374302607882758541771650e+1, d412 = 0.63157877876946881815570249290,
+  d413 = -0.88990336451333310820698117400e-1, d414 = 0.18148505520854727256656404962e+2,
+  d415 = -0.91946323924783554000451984436e+1, d416 = -0.44360363875948939664310572000e+1;
+constexpr double d51 = 0.10427508642579134603413151009e+2, d56 = 0.24228349177525818288430175319e+3,
+  d57 = 0.16520045171727028198505394887e+3, d58 = -0.37454675472269020279518312152e+3,
+  d59 = -0.22113666853125306036270938578e+2, d510 = 0.77334326684722638389603898808e+1,
+  d511 = -0.30674084731089398182061213626e+2, d512 = -0.93321305264302278729567221706e+1,
+  d513 = 0.15697238121770843886131091075e+2, d514 = -0.31139403219565177677282850411e+2,
+  d515 = -0.93529243588444783865713862664e+1, d516 = 0.35816841486394083752465898540e+2;
+constexpr double d61 = 0.19985053242002433820987653617e+2, d66 = -0.38703730874935176555105901742e+3,
+  d67 = -0.18917813819516756882830838328e+3, d68 = 0.52780815920542364900561016686e+3,
+  d69 = -0.11573902539959630126141871134e+2, d610 = 0.68812326946963000169666922661e+1,
+  d611 = -0.10006050966910838403183860980e+1, d612 = 0.77771377980534432092869265740,
+  d613 = -0.27782057523535084065932004339e+1, d614 = -0.60196695231264120758267380846e+2,
+  d615 = 0.84320405506677161018159903784e+2, d616 = 0.11992291136182789328035130030e+2;
+constexpr double d71 = -0.25693933462703749003312586129e+2, d76 = -0.15418974869023643374053993627e+3,
+  d77 = -0.23152937917604549567536039109e+3, d78 = 0.35763911791061412378285349910e+3,
+  d79 = 0.93405324183624310003907691704e+2, d710 = -0.37458323136451633156875139351e+2,
+  d711 = 0.10409964950896230045147246184e+3, d712 = 0.29840293426660503123344363579e+2,
+  d713 = -0.43533456590011143754432175058e+2, d714 = 0.96324553959188282948394950600e+2,
+  d715 = -0.39177261675615439165231486172e+2, d716 = -0.14972683625798562581422125276e+3;
+constexpr double uround = 2.3e-16;
+}  // namespace dp
+
+template <class R> inline R csign(R x, R y) { return (y > R(0.)) ? fabs(x) : -fabs(x); }
+
+// dop853, leung_dop853.c:231-517.  y0 (6, rectangular) is advanced in place,
+// as in the reference; result = nt rows of 6 at the requested times.
+template <class R>
+void dop853(const ForceCtx<R>& F, double* y0d, int nt, const double* t, double rtol_log,
+            double atol_log, double* result, int* err, IntStats* st) {
+  using namespace dp;
+  const int dim = 6;
+  R rtol = exp(R(rtol_log));
+  R atol = exp(R(atol_log));
+  const R safe = 0.9, beta = 0.0;
+  R facold = 1.0e-4;
+  const R expo1 = R(1.0 / 8.0) - beta * R(0.2);
+  const R facc1 = R(1.0) / R(0.333), facc2 = R(1.0) / R(6.0);
+  R hmax = R(t[nt - 1]) - R(t[0]);
+  R pos_neg = csign(R(1.0), hmax);
+  R y0[6], yy1[6], k1[6], k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6];
+  R rc1[6], rc2[6], rc3[6], rc4[6], rc5[6], rc6[6], rc7[6], rc8[6];
+  long long nacc = 0, nrej = 0;
+  for (int i = 0; i < dim; i++) { y0[i] = y0d[i]; result[i] = y0d[i]; }
+  result += dim;
+  *err = 0;
+  F.deriv(R(t[0]), y0, k1);
+  R dnf = 0.0, dny = 0.0, sk, sqr;
+  for (int i = 0; i < dim; i++) {
+    sk = atol + rtol * fabs(y0[i]);
+    sqr = k1[i] / sk;
+    dnf += sqr * sqr;
+    sqr = y0[i] / sk;
+    dny += sqr * sqr;
+  }
+  R h = csign(rmin(sqrt(dny / dnf) * R(0.01), fabs(hmax)), pos_neg);
+  for (int i = 0; i < dim; i++) k3[i] = y0[i] + h * k1[i];
+  F.deriv(R(t[0]) + h, k3, k2);
+  R der2 = 0.0;
+  for (int i = 0; i < dim; i++) {
+    sk = atol + rtol * fabs(y0[i]);
+    sqr = (k2[i] - k1[i]) / sk;
+    der2 += sqr * sqr;
+  }
+  der2 = sqrt(der2) / h;
+  R der12 = rmax(fabs(der2), sqrt(dnf));
+  R h1 = pow(R(0.01) / der12, R(1.0 / 8.0));
+  h = csign(rmin(R(100.0) * fabs(h), rmin(fabs(h1), fabs(hmax))), pos_neg);
+  int reject = 0;
+  R t_current = t[0], t_old = t[0], t_old_older = t[0];
+  int done = 0;
+  while (done < nt - 1) {
+    h = pos_neg * rmax(fabs(h), R(1e3 * uround));
+    for (int i = 0; i < dim; i++) yy1[i] = y0[i] + h * R(a21) * k1[i];
+    F.deriv(t_current + R(c2) * h, yy1, k2);
+    for (int i = 0; i < dim; i++) yy1[i] = y0[i] + h * (R(a31) * k1[i] + R(a32) * k2[i]);
+    F.deriv(t_current + R(c3) * h, yy1, k3);
+    for (int i = 0; i < dim; i++) yy1[i] = y0[i] + h * (R(a41) * k1[i] + R(a43) * k3[i]);
+    F.deriv(t_current + R(c4) * h, yy1, k4);
+    for (int i = 0; i < dim; i++) yy1[i] = y0[i] + h * (R(a51) * k1[i] + R(a53) * k3[i] + R(a54) * k4[i]);
+    F.deriv(t_current + R(c5) * h, yy1, k5);
+    for (int i = 0; i < dim; i++) yy1[i] = y0[i] + h * (R(a61) * k1[i] + R(a64) * k4[i] + R(a65) * k5[i]);
+    F.deriv(t_current + R(c6) * h, yy1, k6);
+    for (int i = 0; i < dim; i++)
+      yy1[i] = y0[i] + h * (R(a71) * k1[i] + R(a74) * k4[i] + R(a75) * k5[i] + R(a76) * k6[i]);
+    F.deriv(t_current + R(c7) * h, yy1, k7);
+    for (int i = 0; i < dim; i++)
+      yy1[i] = y0[i] + h * (R(a81) * k1[i] + R(a84) * k4[i] + R(a85) * k5[i] + R(a86) * k6[i] + R(a87) * k7[i]);
+    F.deriv(t_current + R(c8) * h, yy1, k8);
+    for (int i = 0; i < dim; i++)
+      yy1[i] = y0[i] + h * (R(a91) * k1[i] + R(a94) * k4[i] + R(a95) * k5[i] + R(a96) * k6[i] + R(a97) * k7[i] +
+                            R(a98) * k8[i]);
+    F.deriv(t_current + R(c9) * h, yy1, k9);
+    for (int i = 0; i < dim; i++)
+      yy1[i] = y0[i] + h * (R(a101) * k1[i] + R(a104) * k4[i] + R(a105) * k5[i] + R(a106) * k6[i] +
+                            R(a107) * k7[i] + R(a108) * k8[i] + R(a109) * k9[i]);
+    F.deriv(t_current + R(c10) * h, yy1, k10);
+    for (int i = 0; i < dim; i++)
+      yy1[i] = y0[i] + h * (R(a111) * k1[i] + R(a114) * k4[i] + R(a115) * k5[i] + R(a116) * k6[i] +
+                            R(a117) * k7[i] + R(a118) * k8[i] + R(a119) * k9[i] + R(a1110) * k10[i]);
+    F.deriv(t_current + R(c11) * h, yy1, k2);
+    for (int i = 0; i < dim; i++)
+      yy1[i] = y0[i] + h * (R(a121) * k1[i] + R(a124) * k4[i] + R(a125) * k5[i] + R(a126) * k6[i] +
+                            R(a127) * k7[i] + R(a128) * k8[i] + R(a129) * k9[i] + R(a1210) * k10[i] +
+                            R(a1211) * k2[i]);
+    t_old_older = t_old;
+    t_old = t_current;
+    t_current = t_current + h;
+    F.deriv(t_current, yy1, k3);
+    for (int i = 0; i < dim; i++) {
+      k4[i] = R(b1) * k1[i] + R(b6) * k6[i] + R(b7) * k7[i] + R(b8) * k8[i] + R(b9) * k9[i] + R(b10) * k10[i] +
+              R(b11) * k2[i] + R(b12) * k3[i];
+      k5[i] = y0[i] + h * k4[i];
+    }
+    R errv = 0.0, err2 = 0.0, erri;
+    for (int i = 0; i < dim; i++) {
+      sk = atol + rtol * rmax(fabs(y0[i]), fabs(k5[i]));
+      erri = k4[i] - R(bhh1) * k1[i] - R(bhh2) * k9[i] - R(bhh3) * k3[i];
+      sqr = erri / sk;
+      err2 += sqr * sqr;
+      erri = R(er1) * k1[i] + R(er6) * k6[i] + R(er7) * k7[i] + R(er8) * k8[i] + R(er9) * k9[i] +
+             R(er10) * k10[i] + R(er11) * k2[i] + R(er12) * k3[i];
+      sqr = erri / sk;
+      errv += sqr * sqr;
+    }
+    R deno = errv + R(0.01) * err2;
+    if (deno <= R(0.0)) deno = 1.0;
+    errv = fabs(h) * errv * sqrt(R(1.0) / (deno * R((double)dim)));
+    R fac11 = pow(errv, expo1);
+    R fac = fac11 / pow(facold, beta);
+    fac = rmax(facc2, rmin(facc1, fac / safe));
+    R hnew = h / fac;
+    if (errv <= R(1.0)) {
+      nacc++;
+      facold = rmax(errv, R(1.0e-4));
+      F.deriv(t_current, k5, k4);
+      R ydiff, bspl;
+      for (int i = 0; i < dim; i++) {
+        rc1[i] = y0[i];
+        ydiff = k5[i] - y0[i];
+        rc2[i] = ydiff;
+        bspl = h * k1[i] - ydiff;
+        rc3[i] = bspl;
+        rc4[i] = ydiff - h * k4[i] - bspl;
+        rc5[i] = R(d41) * k1[i] + R(d46) * k6[i] + R(d47) * k7[i] + R(d48) * k8[i] + R(d49) * k9[i] +
+                 R(d410) * k10[i] + R(d411) * k2[i] + R(d412) * k3[i];
+        rc6[i] = R(d51) * k1[i] + R(d56) * k6[i] + R(d57) * k7[i] + R(d58) * k8[i] + R(d59) * k9[i] +
+                 R(d510) * k10[i] + R(d511) * k2[i] + R(d512) * k3[i];
+        rc7[i] = R(d61) * k1[i] + R(d66) * k6[i] + R(d67) * k7[i] + R(d68) * k8[i] + R(d69) * k9[i] +
+                 R(d610) * k10[i] + R(d611) * k2[i] + R(d612) * k3[i];
+        rc8[i] = R(d71) * k1[i] + R(d76) * k6[i] + R(d77) * k7[i] + R(d78) * k8[i] + R(d79) * k9[i] +
+                 R(d710) * k10[i] + R(d711) * k2[i] + R(d712) * k3[i];
+      }
+      for (int i = 0; i < dim; i++)
+        yy1[i] = y0[i] + h * (R(a141) * k1[i] + R(a147) * k7[i] + R(a148) * k8[i] + R(a149) * k9[i] +
+                              R(a1410) * k10[i] + R(a1411) * k2[i] + R(a1412) * k3[i] + R(a1413) * k4[i]);
+      F.deriv(t_old + R(c14) * h, yy1, k10);
+      for (int i = 0; i < dim; i++)
+        yy1[i] = y0[i] + h * (R(a151) * k1[i] + R(a156) * k6[i] + R(a157) * k7[i] + R(a158) * k8[i] +
+                              R(a1511) * k2[i] + R(a1512) * k3[i] + R(a1513) * k4[i] + R(a1514) * k10[i]);
+      F.deriv(t_old + R(c15) * h, yy1, k2);
+      for (int i = 0; i < dim; i++)
+        yy1[i] = y0[i] + h * (R(a161) * k1[i] + R(a166) * k6[i] + R(a167) * k7[i] + R(a168) * k8[i] +
+                              R(a169) * k9[i] + R(a1613) * k4[i] + R(a1614) * k10[i] + R(a1615) * k2[i]);
+      F.deriv(t_old + R(c16) * h, yy1, k3);
+      for (int i = 0; i < dim; i++) {
+        rc5[i] = h * (rc5[i] + R(d413) * k4[i] + R(d414) * k10[i] + R(d415) * k2[i] + R(d416) * k3[i]);
+        rc6[i] = h * (rc6[i] + R(d513) * k4[i] + R(d514) * k10[i] + R(d515) * k2[i] + R(d516) * k3[i]);
+        rc7[i] = h * (rc7[i] + R(d613) * k4[i] + R(d614) * k10[i] + R(d615) * k2[i] + R(d616) * k3[i]);
+        rc8[i] = h * (rc8[i] + R(d713) * k4[i] + R(d714) * k10[i] + R(d715) * k2[i] + R(d716) * k3[i]);
+        k1[i] = k4[i];
+        y0[i] = k5[i];
+      }
+      while (done < nt - 1 && pos_neg * R(t[done + 1]) < pos_neg * t_current) {
+        R s = (R(t[done + 1]) - t_old) / h;
+        R s1 = R(1.0) - s;
+        for (int i = 0; i < dim; i++)
+          result[i] = dval(rc1[i] + s * (rc2[i] + s1 * (rc3[i] + s * (rc4[i] + s1 * (rc5[i] + s * (rc6[i] +
+                                                       s1 * (rc7[i] + s * rc8[i])))))));
+        result += dim;
+        done++;
+      }
+      hnew = (fabs(hnew) > fabs(hmax)) ? pos_neg * hmax : hnew;
+      if (reject) hnew = pos_neg * rmin(fabs(hnew), fabs(h));
+      reject = 0;
+    } else {
+      nrej++;
+      hnew = h / rmin(facc1, fac11 / safe);
+      reject = 1;
+      t_current = t_old;
+      t_old = t_old_older;
+    }
+    h = hnew;
+  }
+  for (int i = 0; i < dim; i++) y0d[i] = dval(y0[i]);
+  if (st) { st->steps = nacc; st->rejected = nrej; st->dt = 0.; }
+}
+
+}  // namespace orc

@@ -1,0 +1,426 @@
+// CPU oracle for galpy's C orbit-integration path -- TEST INFRASTRUCTURE ONLY.
+//
+// This header restates, in plain C++ (templated on the scalar type), the
+// algorithm galpy runs inside `integrateFullOrbit`: the six potential force
+// kernels used by the benchmark configurations, the cylindrical aggregators,
+// the rectangular force callbacks, the coordinate transforms, leapfrog,
+// symplec4 (+ their step estimators) and DOP853 with dense output.  Only the
+// tests, the smoke check and the CPU-baseline leg of the benchmark may load
+// the library built from it; the HIP product never calls it.
+//
+// Templating on the scalar type R lets the same code run
+//   * with R = double   -> the oracle proper (and the CPU baseline), and
+//   * with R = CountedD -> an op-counting twin that tallies FP64 operations,
+//     which gives the algorithmic FLOP count per orbit-step.
+//
+// Every function cites the reference file:line it follows.  Operation order
+// inside each formula follows the reference so that oracle and reference agree
+// to rounding, except where the reference calls GSL (absent here): the
+// incomplete gamma function is this file's own (series + continued fraction,
+// written from the mathematics) and is pinned against mpmath.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cfloat>
+#include <algorithm>
+
+namespace orc {
+
+using std::sqrt; using std::log; using std::exp; using std::pow;
+using std::fabs; using std::sin; using std::cos; using std::acos;
+using std::atan2; using std::tgamma;
+
+inline double dval(double x) { return x; }
+
+// ---------------------------------------------------------------------------
+// Op-counting scalar.  Each IEEE add/sub/mul/div/sqrt is one counted op; each
+// libm transcendental call (exp, log, pow, sin, cos, acos, atan2) is counted
+// separately as a "special" call.  Comparisons, fabs, negation and selects are
+// free.  Counters are thread-local so OpenMP loops can count in parallel.
+// ---------------------------------------------------------------------------
+struct OpCount {
+  unsigned long long add, mul, div, sqrt, special;
+};
+extern thread_local OpCount g_ops;
+
+struct CountedD {
+  double v;
+  CountedD() : v(0.) {}
+  CountedD(double x) : v(x) {}
+};
+inline double dval(CountedD x) { return x.v; }
+inline CountedD operator+(CountedD a, CountedD b) { g_ops.add++; return CountedD(a.v + b.v); }
+inline CountedD operator-(CountedD a, CountedD b) { g_ops.add++; return CountedD(a.v - b.v); }
+inline CountedD operator*(CountedD a, CountedD b) { g_ops.mul++; return CountedD(a.v * b.v); }
+inline CountedD operator/(CountedD a, CountedD b) { g_ops.div++; return CountedD(a.v / b.v); }
+inline CountedD operator-(CountedD a) { return CountedD(-a.v); }
+inline CountedD& operator+=(CountedD& a, CountedD b) { a = a + b; return a; }
+inline CountedD& operator-=(CountedD& a, CountedD b) { a = a - b; return a; }
+inline CountedD& operator*=(CountedD& a, CountedD b) { a = a * b; return a; }
+inline CountedD& operator/=(CountedD& a, CountedD b) { a = a / b; return a; }
+inline bool operator<(CountedD a, CountedD b) { return a.v < b.v; }
+inline bool operator>(CountedD a, CountedD b) { return a.v > b.v; }
+inline bool operator<=(CountedD a, CountedD b) { return a.v <= b.v; }
+inline bool operator>=(CountedD a, CountedD b) { return a.v >= b.v; }
+inline bool operator==(CountedD a, CountedD b) { return a.v == b.v; }
+inline bool operator!=(CountedD a, CountedD b) { return a.v != b.v; }
+inline CountedD sqrt(CountedD a) { g_ops.sqrt++; return CountedD(std::sqrt(a.v)); }
+inline CountedD fabs(CountedD a) { return CountedD(std::fabs(a.v)); }
+inline CountedD log(CountedD a) { g_ops.special++; return CountedD(std::log(a.v)); }
+inline CountedD exp(CountedD a) { g_ops.special++; return CountedD(std::exp(a.v)); }
+inline CountedD pow(CountedD a, CountedD b) { g_ops.special++; return CountedD(std::pow(a.v, b.v)); }
+inline CountedD sin(CountedD a) { g_ops.special++; return CountedD(std::sin(a.v)); }
+inline CountedD cos(CountedD a) { g_ops.special++; return CountedD(std::cos(a.v)); }
+inline CountedD acos(CountedD a) { g_ops.special++; return CountedD(std::acos(a.v)); }
+inline CountedD atan2(CountedD a, CountedD b) { g_ops.special++; return CountedD(std::atan2(a.v, b.v)); }
+
+template <class R> inline R rmax(R a, R b) { return (a > b) ? a : b; }
+template <class R> inline R rmin(R a, R b) { return (a < b) ? a : b; }
+
+// ---------------------------------------------------------------------------
+// Incomplete gamma function (replaces GSL's gsl_sf_gamma_inc_P and
+// gsl_sf_gamma_inc at PowerSphericalPotentialwCutoff.c:10,23).
+//   lower  gamma(a,x) = int_0^x t^(a-1) e^-t dt
+//   series (x < a+1): gamma(a,x) = e^-x x^a sum_{n>=0} x^n / (a (a+1)...(a+n))
+//   continued fraction (x >= a+1), modified Lentz:
+//     Gamma(a,x) = e^-x x^a / (x+1-a - 1(1-a)/(x+3-a - 2(2-a)/(x+5-a - ...)))
+// (Abramowitz & Stegun 6.5.29 and 6.5.31; Numerical Recipes 2nd ed. 6.2.)
+// Pinned against mpmath at 50 digits (tests/golden/incgamma.json).
+// ---------------------------------------------------------------------------
+constexpr double kIGEps = DBL_EPSILON / 2.;
+constexpr double kIGTiny = 1e-300;
+
+template <class R> R incgamma_series_sum(double a, R x) {
+  R ap = a;
+  R del = R(1.) / ap;
+  R sum = del;
+  for (int n = 1; n < 2000; n++) {
+    ap += 1.;
+    del *= x / ap;
+    sum += del;
+    if (fabs(del) < fabs(sum) * kIGEps) break;
+  }
+  return sum;
+}
+
+template <class R> R incgamma_cf(double a, R x) {
+  R b = x + 1. - a;
+  R c = R(1. / kIGTiny);
+  R d = R(1.) / b;
+  R h = d;
+  for (int i = 1; i < 2000; i++) {
+    double an = -i * (i - a);
+    b += 2.;
+    d = R(an) * d + b;
+    if (fabs(d) < R(kIGTiny)) d = kIGTiny;
+    c = b + R(an) / c;
+    if (fabs(c) < R(kIGTiny)) c = kIGTiny;
+    d = R(1.) / d;
+    R del = d * c;
+    h *= del;
+    if (fabs(del - 1.) < R(kIGEps)) break;
+  }
+  return h;
+}
+
+// lower incomplete gamma gamma(a,x); gamma_a = Gamma(a)
+template <class R> R lower_incgamma(double a, R x, double gamma_a) {
+  if (x <= R(0.)) return R(0.);
+  R pref = exp(R(a) * log(x) - x);
+  if (x < R(a + 1.)) return pref * incgamma_series_sum(a, x);
+  return R(gamma_a) - pref * incgamma_cf(a, x);
+}
+// upper incomplete gamma Gamma(a,x)
+template <class R> R upper_incgamma(double a, R x, double gamma_a) {
+  if (x <= R(0.)) return R(gamma_a);
+  R pref = exp(R(a) * log(x) - x);
+  if (x < R(a + 1.)) return R(gamma_a) - pref * incgamma_series_sum(a, x);
+  return pref * incgamma_cf(a, x);
+}
+
+// ---------------------------------------------------------------------------
+// Potential components.  Parameters are decoded once from galpy's packed
+// (pot_type, pot_args) arrays (integrateFullOrbit.c:69-1045, cases 0,1,5,8,9,15)
+// and kept read-only; constants that the reference recomputes on every call
+// (Gamma(a), 2 pi rc^(3-alpha)) are precomputed with the same operations.
+// ---------------------------------------------------------------------------
+enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kHernquist = 8,
+               kNFW = 9, kPowerSphCutoff = 15 };
+
+struct Component {
+  int type;
+  double p[8];  // raw arguments in galpy's order
+  double c[6];  // precomputed constants
+};
+
+// nargs per code, as consumed from pot_args (integrateFullOrbit.c:80-345)
+inline int component_nargs(int code) {
+  switch (code) {
+    case kLogHalo: return 4;
+    case kDehnenBar: return 6;
+    case kMiyamotoNagai: return 3;
+    case kHernquist: return 2;
+    case kNFW: return 2;
+    case kPowerSphCutoff: return 5;
+    default: return -1;
+  }
+}
+
+// Decode; returns 0 on success, -1 on an unsupported code.
+inline int parse_potential(int npot, const int* pot_type, const double* pot_args,
+                           Component* out) {
+  for (int i = 0; i < npot; i++) {
+    int code = pot_type[i];
+    int na = component_nargs(code);
+    if (na < 0) return -1;
+    Component& C = out[i];
+    C.type = code;
+    std::memset(C.p, 0, sizeof(C.p));
+    std::memset(C.c, 0, sizeof(C.c));
+    for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
+    if (code == kPowerSphCutoff) {
+      double alpha = C.p[1], rc = C.p[2];
+      C.c[0] = 1.5 - 0.5 * alpha;                  // a for the force
+      C.c[1] = tgamma(1.5 - 0.5 * alpha);          // Gamma(a)
+      C.c[2] = 2. * M_PI * pow(rc, 3. - alpha);    // 2 pi rc^(3-alpha)
+      C.c[3] = 1. - 0.5 * alpha;                   // for Eval
+      C.c[4] = tgamma(1. - 0.5 * alpha);
+    }
+  }
+  return 0;
+}
+
+// PowerSphericalPotentialwCutoff.c:9-11 (mass), :25-46 (Rforce), :60-81 (zforce)
+// Returns the radial factor force/r-vector: F_R = f*R, F_z = f*z.
+template <class R> R psc_force_factor(const Component& C, R r2) {
+  double amp = C.p[0];
+  double rc = C.p[2];
+  R x = r2 / R(rc) / R(rc);
+  // mass = 2 pi rc^(3-alpha) * (Gamma(a) * P(a, x)), P = gamma(a,x)/Gamma(a)
+  R P = lower_incgamma(C.c[0], x, C.c[1]) / R(C.c[1]);
+  R mass = R(C.c[2]) * (R(C.c[1]) * P);
+  return -R(amp) * mass / pow(r2, R(1.5));
+}
+// PowerSphericalPotentialwCutoff.c:12-24
+template <class R> R psc_eval(const Component& C, R Rc, R Z) {
+  double amp = C.p[0], rc = C.p[2];
+  R r2 = Rc * Rc + Z * Z;
+  R r = sqrt(r2);
+  R x = r2 / R(rc) / R(rc);
+  R g1 = lower_incgamma(C.c[3], x, C.c[4]);   // Gamma(1-a/2) - Gamma(1-a/2, x)
+  R g2 = lower_incgamma(C.c[0], x, C.c[1]);   // Gamma(1.5-a/2) - Gamma(1.5-a/2, x)
+  return R(amp) * R(C.c[2]) / r * (r / R(rc) * g1 - g2);
+}
+
+// MiyamotoNagaiPotential.c:5-15 (Eval), :16-26 (Rforce), :38-53 (zforce)
+template <class R> void mn_forces(const Component& C, R Rc, R z, R& FR, R& Fz) {
+  double amp = C.p[0], a = C.p[1], b = C.p[2];
+  FR = -R(amp) * Rc * pow(Rc * Rc + pow(R(a) + pow(z * z + R(b * b), R(0.5)), R(2.)), R(-1.5));
+  R sqrtbz = pow(R(b * b) + z * z, R(.5));
+  R asqrtbz = R(a) + sqrtbz;
+  if (a == 0.)
+    Fz = R(amp) * (-z * pow(Rc * Rc + asqrtbz * asqrtbz, R(-1.5)));
+  else
+    Fz = R(amp) * (-z * asqrtbz / sqrtbz * pow(Rc * Rc + asqrtbz * asqrtbz, R(-1.5)));
+}
+template <class R> R mn_eval(const Component& C, R Rc, R z) {
+  double amp = C.p[0], a = C.p[1], b = C.p[2];
+  return -R(amp) * pow(Rc * Rc + pow(R(a) + pow(z * z + R(b * b), R(0.5)), R(2.)), R(-0.5));
+}
+
+// NFWPotential.c:5-15 (Eval), :16-27 (Rforce), :38-49 (zforce)
+template <class R> void nfw_forces(const Component& C, R Rc, R Z, R& FR, R& Fz) {
+  double amp = C.p[0], a = C.p[1];
+  R Rz = Rc * Rc + Z * Z;
+  R sqrtRz = pow(Rz, R(0.5));
+  // the reference evaluates the bracket twice (once per component); the
+  // value is identical, so it is formed once here
+  R br = R(1.) / Rz / (R(a) + sqrtRz) - log(R(1.) + sqrtRz / R(a)) / sqrtRz / Rz;
+  FR = R(amp) * Rc * br;
+  Fz = R(amp) * Z * br;
+}
+template <class R> R nfw_eval(const Component& C, R Rc, R Z) {
+  double amp = C.p[0], a = C.p[1];
+  R sqrtRz = pow(Rc * Rc + Z * Z, R(0.5));
+  return -R(amp) * log(R(1.) + sqrtRz / R(a)) / sqrtRz;
+}
+
+// HernquistPotential.c:5-14 (Eval), :15-25 (Rforce), :36-46 (zforce)
+template <class R> void hq_forces(const Component& C, R Rc, R Z, R& FR, R& Fz) {
+  double amp = C.p[0], a = C.p[1];
+  R sqrtRz = pow(Rc * Rc + Z * Z, R(0.5));
+  R f = pow(R(1.) + sqrtRz / R(a), R(-2.));
+  FR = -R(amp) * Rc / R(a) / sqrtRz * f / R(2.) / R(a);
+  Fz = -R(amp) * Z / R(a) / sqrtRz * f / R(2.) / R(a);
+}
+template <class R> R hq_eval(const Component& C, R Rc, R Z) {
+  double amp = C.p[0], a = C.p[1];
+  R sqrtRz = pow(Rc * Rc + Z * Z, R(0.5));
+  return -R(amp) / (R(1.) + sqrtRz / R(a)) / R(2.) / R(a);
+}
+
+// LogarithmicHaloPotential.c:5-20 (Eval), :21-38 (Rforce), :55-72 (zforce),
+// :73-91 (phitorque).  args: amp, q, core^2, 1-1/b^2 (>=1 means axisymmetric)
+template <class R> void lh_forces(const Component& C, R Rc, R Z, R phi, R& FR, R& Fz, R& Fphi) {
+  double amp = C.p[0], q = C.p[1], c = C.p[2], onem1overb2 = C.p[3];
+  R zq = Z / R(q);
+  if (onem1overb2 < 1) {
+    R s = sin(phi);
+    R Rt2 = Rc * Rc * (R(1.) - R(onem1overb2) * pow(s, R(2)));
+    R den = Rt2 + zq * zq + R(c);
+    FR = -R(amp) * Rt2 / Rc / den;
+    Fz = -R(amp) * zq / R(q) / den;
+    Fphi = R(amp) * Rc * Rc / den * sin(R(2.) * phi) * R(onem1overb2) / R(2.);
+  } else {
+    FR = -R(amp) * Rc / (Rc * Rc + zq * zq + R(c));
+    Fz = -R(amp) * zq / R(q) / (Rc * Rc + zq * zq + R(c));
+    Fphi = R(0.);
+  }
+}
+template <class R> R lh_eval(const Component& C, R Rc, R Z, R phi) {
+  double amp = C.p[0], q = C.p[1], c = C.p[2], onem1overb2 = C.p[3];
+  R zq = Z / R(q);
+  if (onem1overb2 < 1)
+    return R(0.5) * R(amp) * log(Rc * Rc * (R(1.) - R(onem1overb2) * pow(sin(phi), R(2))) + zq * zq + R(c));
+  return R(0.5) * R(amp) * log(Rc * Rc + zq * zq + R(c));
+}
+
+// DehnenBarPotential.c:5-17 (smooth), :18-40 (Rforce), :60-82 (phitorque),
+// :102-124 (zforce).  args: amp*Af, tform, tsteady, rb, omegab, barphi
+template <class R> R dehnen_smooth(R t, double tform, double tsteady) {
+  if (t < R(tform)) return R(0.);
+  if (t < R(tsteady)) {
+    R deltat = t - R(tform);
+    R xi = R(2.) * deltat / R(tsteady - tform) - R(1.);
+    return R(3. / 16.) * pow(xi, R(5.)) - R(5. / 8.) * pow(xi, R(3.)) + R(15. / 16.) * xi + R(.5);
+  }
+  return R(1.);
+}
+template <class R> void bar_forces(const Component& C, R Rc, R z, R phi, R t, R& FR, R& Fz, R& Fphi) {
+  double amp = C.p[0], tform = C.p[1], tsteady = C.p[2], rb = C.p[3], omegab = C.p[4], barphi = C.p[5];
+  R smooth = dehnen_smooth(t, tform, tsteady);
+  R r2 = Rc * Rc + z * z;
+  R r = sqrt(r2);
+  R ang = R(2.) * (phi - R(omegab) * t - R(barphi));
+  R cang = cos(ang);
+  R sang = sin(ang);
+  if (r <= R(rb)) {
+    FR = -R(amp) * smooth * cang * (pow(r / R(rb), R(3.)) * Rc * (R(3.) * Rc * Rc + R(2.) * z * z) - R(4.) * Rc * z * z) / pow(r, R(4.));
+    Fphi = R(2.) * R(amp) * smooth * sang * (pow(r / R(rb), R(3.)) - R(2.)) * Rc * Rc / r2;
+    Fz = -R(amp) * smooth * cang * (pow(r / R(rb), R(3.)) + R(4.)) * Rc * Rc * z / pow(r, R(4.));
+  } else {
+    FR = -R(amp) * smooth * cang * pow(R(rb) / r, R(3.)) * Rc / pow(r, R(4)) * (R(3.) * Rc * Rc - R(2.) * z * z);
+    Fphi = -R(2.) * R(amp) * smooth * sang * pow(R(rb) / r, R(3.)) * Rc * Rc / r2;
+    Fz = -R(5.) * R(amp) * smooth * cang * pow(R(rb) / r, R(3.)) * Rc * Rc * z / pow(r, R(4));
+  }
+}
+// DehnenBarPotential.py:173-200 (the C library has no Eval for the bar)
+template <class R> R bar_eval(const Component& C, R Rc, R z, R phi, R t) {
+  double amp = C.p[0], tform = C.p[1], tsteady = C.p[2], rb = C.p[3], omegab = C.p[4], barphi = C.p[5];
+  R smooth = dehnen_smooth(t, tform, tsteady);
+  R r2 = Rc * Rc + z * z;
+  R r = sqrt(r2);
+  R out;
+  if (r <= R(rb))
+    out = (pow(r / R(rb), R(3.)) - R(2.)) * (Rc * Rc / r2);
+  else
+    out = -pow(R(rb) / r, R(3.)) * R(1.) / (R(1.) + z * z / (Rc * Rc));
+  return out * R(amp) * smooth * cos(R(2.) * (phi - R(omegab) * t - R(barphi)));
+}
+
+// Aggregated cylindrical forces: galpy_potentials.c:120-179 sums each of
+// Rforce, phitorque, zforce over components in index order starting from 0.
+// Summing the three per component gives the identical three sums.
+template <class R>
+void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
+                R& FR, R& Fz, R& Fphi) {
+  FR = 0.; Fz = 0.; Fphi = 0.;
+  for (int i = 0; i < npot; i++) {
+    const Component& C = P[i];
+    R fr = 0., fz = 0., fp = 0.;
+    switch (C.type) {
+      case kPowerSphCutoff: {
+        R r2 = Rc * Rc + z * z;
+        R f = psc_force_factor(C, r2);
+        fr = f * Rc; fz = f * z;
+        break;
+      }
+      case kMiyamotoNagai: mn_forces(C, Rc, z, fr, fz); break;
+      case kNFW: nfw_forces(C, Rc, z, fr, fz); break;
+      case kHernquist: hq_forces(C, Rc, z, fr, fz); break;
+      case kLogHalo: lh_forces(C, Rc, z, phi, fr, fz, fp); break;
+      case kDehnenBar: bar_forces(C, Rc, z, phi, t, fr, fz, fp); break;
+    }
+    FR += fr; Fz += fz; Fphi += fp;
+  }
+}
+
+// Potential value: galpy_potentials.c evaluatePotentials analog (sum in
+// component order); the bar uses the Python expression.
+template <class R>
+R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
+  R pot = 0.;
+  for (int i = 0; i < npot; i++) {
+    const Component& C = P[i];
+    switch (C.type) {
+      case kPowerSphCutoff: pot += psc_eval(C, Rc, z); break;
+      case kMiyamotoNagai: pot += mn_eval(C, Rc, z); break;
+      case kNFW: pot += nfw_eval(C, Rc, z); break;
+      case kHernquist: pot += hq_eval(C, Rc, z); break;
+      case kLogHalo: pot += lh_eval(C, Rc, z, phi); break;
+      case kDehnenBar: pot += bar_eval(C, Rc, z, phi, t); break;
+    }
+  }
+  return pot;
+}
+
+// evalRectForce: integrateFullOrbit.c:1307-1325
+template <class R>
+void rect_force(int npot, const Component* P, R t, const R* q, R* a) {
+  R x = q[0], y = q[1], z = q[2];
+  R Rc = sqrt(x * x + y * y);
+  R phi = acos(x / Rc);
+  R sinphi = y / Rc;
+  R cosphi = x / Rc;
+  if (y < R(0.)) phi = R(2. * M_PI) - phi;
+  R FR, Fz, Fphi;
+  cyl_forces(npot, P, Rc, z, phi, t, FR, Fz, Fphi);
+  a[0] = cosphi * FR - R(1.) / Rc * sinphi * Fphi;
+  a[1] = sinphi * FR + R(1.) / Rc * cosphi * Fphi;
+  a[2] = Fz;
+}
+
+// evalRectDeriv: integrateFullOrbit.c:1326-1351 (vR, vT only feed dissipative
+// forces, none of which are in scope, so they are not formed)
+template <class R>
+void rect_deriv(int npot, const Component* P, R t, const R* q, R* a) {
+  a[0] = q[3]; a[1] = q[4]; a[2] = q[5];
+  rect_force(npot, P, t, q, a + 3);
+}
+
+// bovy_coords.c:101-115
+template <class R> void cyl_to_rect(R* v) {
+  R Rc = v[0], phi = v[5];
+  R cp = cos(phi), sp = sin(phi);
+  R vR = v[1], vT = v[2];
+  v[0] = Rc * cp;
+  v[1] = Rc * sp;
+  v[2] = v[3];
+  v[5] = v[4];
+  v[3] = vR * cp - vT * sp;
+  v[4] = vR * sp + vT * cp;
+}
+// bovy_coords.c:125-139
+template <class R> void rect_to_cyl(R* v) {
+  R x = v[0], y = v[1], vx = v[3], vy = v[4];
+  v[3] = v[2];
+  v[4] = v[5];
+  v[5] = atan2(y, x);
+  R cp = cos(v[5]), sp = sin(v[5]);
+  v[0] = sqrt(x * x + y * y);
+  v[1] = vx * cp + vy * sp;
+  v[2] = -vx * sp + vy * cp;
+}
+
+}  // namespace orc

@@ -1,0 +1,171 @@
+// CPU oracle C-ABI -- TEST INFRASTRUCTURE ONLY.
+//
+// `oracle_integrateFullOrbit` has the signature of galpy's
+// `integrateFullOrbit` (galpy/orbit/orbit_c_ext/integrateFullOrbit.c:1046-1061)
+// and the same OpenMP structure (:1133, schedule(dynamic,1) over orbits).
+// The remaining entry points expose the pieces (forces, potential values,
+// incomplete gamma, coordinate transforms) for unit checks, plus an
+// op-counting run that yields the algorithmic FLOP count per orbit-step.
+#include <omp.h>
+#include <cstdio>
+#include <vector>
+#include "integrators.hpp"
+
+namespace orc {
+thread_local OpCount g_ops = {0, 0, 0, 0, 0};
+}
+
+using namespace orc;
+
+#define ORC_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+constexpr int kMaxPot = 16;
+
+template <class R>
+void integrate_one(int npot, const Component* P, int odeint_type, double* yo, int nt,
+                   const double* t, double dt, double rtol, double atol, double* result,
+                   int* err, IntStats* st) {
+  long long nrhs = 0;
+  ForceCtx<R> F{npot, P, &nrhs};
+  IntStats local = {0, 0, 0, 0.};
+  cyl_to_rect(yo);  // integrateFullOrbit.c:1135 (in place, as the reference)
+  switch (odeint_type) {
+    case 0: leapfrog(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
+    case 3: symplec4(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
+    case 6: dop853(F, yo, nt, t, rtol, atol, result, err, &local); break;
+    default: *err = -1; return;
+  }
+  for (int j = 0; j < nt; j++) rect_to_cyl(result + 6 * j);  // :1139-1140
+  local.rhs = nrhs;
+  if (st) *st = local;
+}
+
+int run(int nobj, double* yo, int nt, const double* t, int indiv_t, int npot,
+        const int* pot_type, const double* pot_args, double dt, double rtol, double atol,
+        double* result, int* err, int odeint_type, void (*cb)(void), long long* steps,
+        long long* rejected, long long* nrhs, double* dt_used, int nthreads) {
+  if (npot > kMaxPot || npot < 0 || nt < 2) {
+    for (int i = 0; i < nobj; i++) err[i] = -1;
+    return -1;
+  }
+  Component P[kMaxPot];
+  if (parse_potential(npot, pot_type, pot_args, P) != 0) {
+    for (int i = 0; i < nobj; i++) err[i] = -1;
+    return -1;
+  }
+  int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+  if (nobj < nth) nth = nobj > 0 ? nobj : 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nth)
+  for (int ii = 0; ii < nobj; ii++) {
+    IntStats st = {0, 0, 0, 0.};
+    integrate_one<double>(npot, P, odeint_type, yo + 6 * ii, nt, t + (long)nt * ii * indiv_t, dt,
+                          rtol, atol, result + 6L * nt * ii, err + ii, &st);
+    if (steps) steps[ii] = st.steps;
+    if (rejected) rejected[ii] = st.rejected;
+    if (nrhs) nrhs[ii] = st.rhs;
+    if (dt_used) dt_used[ii] = st.dt;
+    if (cb) cb();
+  }
+  return 0;
+}
+
+}  // namespace
+
+// Same signature and argument meaning as galpy's integrateFullOrbit.
+// pot_tfuncs must be NULL (none of the supported potentials use them).
+ORC_EXPORT void oracle_integrateFullOrbit(int nobj, double* yo, int nt, double* t, int indiv_t,
+                                          int npot, int* pot_type, double* pot_args,
+                                          void* pot_tfuncs, double dt, double rtol, double atol,
+                                          double* result, int* err, int odeint_type,
+                                          void (*cb)(void)) {
+  (void)pot_tfuncs;
+  run(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result, err,
+      odeint_type, cb, nullptr, nullptr, nullptr, nullptr, 0);
+}
+
+// As above plus per-orbit statistics and an explicit thread count (0 = all).
+ORC_EXPORT int oracle_integrate_stats(int nobj, double* yo, int nt, double* t, int indiv_t,
+                                      int npot, int* pot_type, double* pot_args, double dt,
+                                      double rtol, double atol, double* result, int* err,
+                                      int odeint_type, long long* steps, long long* rejected,
+                                      long long* nrhs, double* dt_used, int nthreads) {
+  return run(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result, err,
+             odeint_type, nullptr, steps, rejected, nrhs, dt_used, nthreads);
+}
+
+// Op-counting run (single thread).  out[0..7] = add, mul, div, sqrt, special,
+// orbit-steps, rejected steps, force/RHS evaluations, summed over the orbits.
+ORC_EXPORT int oracle_count_ops(int nobj, double* yo, int nt, double* t, int indiv_t, int npot,
+                                int* pot_type, double* pot_args, double dt, double rtol,
+                                double atol, double* result, int* err, int odeint_type,
+                                double* out) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  g_ops = OpCount{0, 0, 0, 0, 0};
+  long long steps = 0, rej = 0, rhs = 0;
+  for (int ii = 0; ii < nobj; ii++) {
+    IntStats st = {0, 0, 0, 0.};
+    // the coordinate transforms act on plain doubles and are not counted:
+    // they are per output row, not per orbit-step
+    integrate_one<CountedD>(npot, P, odeint_type, yo + 6 * ii, nt, t + (long)nt * ii * indiv_t,
+                            dt, rtol, atol, result + 6L * nt * ii, err + ii, &st);
+    steps += st.steps;
+    rej += st.rejected;
+    rhs += st.rhs;
+  }
+  out[0] = (double)g_ops.add;
+  out[1] = (double)g_ops.mul;
+  out[2] = (double)g_ops.div;
+  out[3] = (double)g_ops.sqrt;
+  out[4] = (double)g_ops.special;
+  out[5] = (double)steps;
+  out[6] = (double)rej;
+  out[7] = (double)rhs;
+  return 0;
+}
+
+// Cylindrical forces (FR, Fz, Fphi) at n points.
+ORC_EXPORT int oracle_eval_forces(int npot, int* pot_type, double* pot_args, int n,
+                                  const double* R, const double* z, const double* phi,
+                                  const double* t, double* FR, double* Fz, double* Fphi) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  for (int i = 0; i < n; i++) cyl_forces<double>(npot, P, R[i], z[i], phi[i], t[i], FR[i], Fz[i], Fphi[i]);
+  return 0;
+}
+
+// Potential value at n points.
+ORC_EXPORT int oracle_eval_potential(int npot, int* pot_type, double* pot_args, int n,
+                                     const double* R, const double* z, const double* phi,
+                                     const double* t, double* out) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  for (int i = 0; i < n; i++) out[i] = potential_value<double>(npot, P, R[i], z[i], phi[i], t[i]);
+  return 0;
+}
+
+// Rectangular accelerations (evalRectForce) at n points q[n*3] -> a[n*3].
+ORC_EXPORT int oracle_rect_force(int npot, int* pot_type, double* pot_args, int n,
+                                 const double* t, const double* q, double* a) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  for (int i = 0; i < n; i++) rect_force<double>(npot, P, t[i], q + 3 * i, a + 3 * i);
+  return 0;
+}
+
+ORC_EXPORT void oracle_lower_incgamma(double a, int n, const double* x, double* out) {
+  double ga = std::tgamma(a);
+  for (int i = 0; i < n; i++) out[i] = lower_incgamma<double>(a, x[i], ga);
+}
+ORC_EXPORT void oracle_upper_incgamma(double a, int n, const double* x, double* out) {
+  double ga = std::tgamma(a);
+  for (int i = 0; i < n; i++) out[i] = upper_incgamma<double>(a, x[i], ga);
+}
+ORC_EXPORT void oracle_cyl_to_rect(int n, double* v) {
+  for (int i = 0; i < n; i++) cyl_to_rect<double>(v + 6 * i);
+}
+ORC_EXPORT void oracle_rect_to_cyl(int n, double* v) {
+  for (int i = 0; i < n; i++) rect_to_cyl<double>(v + 6 * i);
+}

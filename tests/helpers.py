@@ -1,0 +1,140 @@
+"""Shared test helpers: the CPU oracle (oracle/liboracle.so) as the checker,
+seeded initial conditions and the benchmark potentials.
+
+Only tests, the smoke check and the CPU-baseline leg of bench.py use the
+oracle; it is never part of the product path."""
+
+import ctypes
+import os
+
+import numpy
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+LN1EM12 = -12.0 * numpy.log(10.0)
+DT_SENTINEL = -9999.99
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _oracle = ctypes.CDLL(ORACLE_PATH)
+    return _oracle
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def oracle_integrate(pot_type, pot_args, yo, t, method, dt=None, rtol=LN1EM12, atol=LN1EM12,
+                     nthreads=0):
+    """Run the oracle's integrateFullOrbit (returns result, err, stats[N,4],
+    yo_after) where stats = (steps, rejected, rhs, dt)."""
+    lib = oracle()
+    yo = numpy.array(numpy.atleast_2d(yo), dtype=numpy.float64, order="C")
+    t = numpy.ascontiguousarray(t, dtype=numpy.float64)
+    n = len(yo)
+    nt = t.shape[-1]
+    indiv = int(t.ndim > 1)
+    res = numpy.empty((n, nt, 6))
+    err = numpy.zeros(n, dtype=numpy.int32)
+    steps = numpy.zeros(n, dtype=numpy.int64)
+    rej = numpy.zeros(n, dtype=numpy.int64)
+    rhs = numpy.zeros(n, dtype=numpy.int64)
+    dtu = numpy.zeros(n)
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    lib.oracle_integrate_stats(
+        ctypes.c_int(n), _p(yo), ctypes.c_int(nt), _p(t), ctypes.c_int(indiv), ctypes.c_int(len(pt)),
+        _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(rtol),
+        ctypes.c_double(atol), _p(res), _p(err), ctypes.c_int(method), _p(steps), _p(rej), _p(rhs),
+        _p(dtu), ctypes.c_int(nthreads),
+    )
+    stats = numpy.stack([steps, rej, rhs, dtu], axis=1).astype(float)
+    return res, err, stats, yo
+
+
+def oracle_forces(pot_type, pot_args, R, z, phi=None, t=None):
+    lib = oracle()
+    R = numpy.ascontiguousarray(R, dtype=float)
+    n = R.size
+    z = numpy.ascontiguousarray(numpy.broadcast_to(z, R.shape), dtype=float)
+    phi = numpy.ascontiguousarray(numpy.zeros(n) if phi is None else numpy.broadcast_to(phi, R.shape), dtype=float)
+    t = numpy.ascontiguousarray(numpy.zeros(n) if t is None else numpy.broadcast_to(t, R.shape), dtype=float)
+    out = [numpy.empty(n) for _ in range(3)]
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    rc = lib.oracle_eval_forces(len(pt), _p(pt), _p(pa), n, _p(R), _p(z), _p(phi), _p(t), *[_p(o) for o in out])
+    assert rc == 0
+    return tuple(out)
+
+
+def oracle_potential(pot_type, pot_args, R, z, phi=None, t=None):
+    lib = oracle()
+    R = numpy.ascontiguousarray(R, dtype=float)
+    n = R.size
+    z = numpy.ascontiguousarray(numpy.broadcast_to(z, R.shape), dtype=float)
+    phi = numpy.ascontiguousarray(numpy.zeros(n) if phi is None else numpy.broadcast_to(phi, R.shape), dtype=float)
+    t = numpy.ascontiguousarray(numpy.zeros(n) if t is None else numpy.broadcast_to(t, R.shape), dtype=float)
+    out = numpy.empty(n)
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    rc = lib.oracle_eval_potential(len(pt), _p(pt), _p(pa), n, _p(R), _p(z), _p(phi), _p(t), _p(out))
+    assert rc == 0
+    return out
+
+
+def oracle_count_ops(pot_type, pot_args, yo, t, method, dt=None):
+    lib = oracle()
+    yo = numpy.array(numpy.atleast_2d(yo), dtype=numpy.float64, order="C")
+    t = numpy.ascontiguousarray(t, dtype=numpy.float64)
+    n, nt = len(yo), t.shape[-1]
+    res = numpy.empty((n, nt, 6))
+    err = numpy.zeros(n, dtype=numpy.int32)
+    out = numpy.zeros(8)
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    rc = lib.oracle_count_ops(
+        ctypes.c_int(n), _p(yo), ctypes.c_int(nt), _p(t), ctypes.c_int(int(t.ndim > 1)), ctypes.c_int(len(pt)),
+        _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(LN1EM12),
+        ctypes.c_double(LN1EM12), _p(res), _p(err), ctypes.c_int(method), _p(out),
+    )
+    assert rc == 0
+    keys = ["add", "mul", "div", "sqrt", "special", "steps", "rejected", "rhs"]
+    return dict(zip(keys, out))
+
+
+def ics(n, seed=0):
+    """Seeded initial conditions of the benchmark configurations:
+    R~U(0.5,2), z~N(0,0.1), phi~U(0,2pi), vR,vz~N(0,0.1), vT~N(1,0.1);
+    rows are (R,vR,vT,z,vz,phi)."""
+    rng = numpy.random.default_rng(seed)
+    R = rng.uniform(0.5, 2.0, n)
+    z = rng.normal(0.0, 0.1, n)
+    phi = rng.uniform(0.0, 2.0 * numpy.pi, n)
+    vR = rng.normal(0.0, 0.1, n)
+    vz = rng.normal(0.0, 0.1, n)
+    vT = rng.normal(1.0, 0.1, n)
+    return numpy.stack([R, vR, vT, z, vz, phi], axis=1)
+
+
+def bar_potential():
+    """LogarithmicHaloPotential(q=0.9) + DehnenBarPotential(omegab=1.85, rb=0.8, Af=0.01)."""
+    from galpy_amd.potential import DehnenBarPotential, LogarithmicHaloPotential
+
+    return [LogarithmicHaloPotential(q=0.9), DehnenBarPotential(omegab=1.85, rb=0.8, Af=0.01)]
+
+
+def phase_diff(a, b):
+    """Max over rows of |a-b| per component with unit floor, phi modulo 2 pi.
+    a, b: [..., 6] rows (R,vR,vT,z,vz,phi)."""
+    d = numpy.abs(a - b)
+    dphi = numpy.abs((a[..., 5] - b[..., 5] + numpy.pi) % (2 * numpy.pi) - numpy.pi)
+    d[..., 5] = dphi
+    scale = numpy.maximum(numpy.abs(b), 1.0)
+    scale[..., 5] = numpy.pi
+    return (d / scale).max()
