@@ -1,0 +1,58 @@
+"""Synthetic workloads of the benchmark configurations (seeded, in-memory).
+
+Initial conditions follow one law for every configuration: R~U(0.5,2),
+z~N(0,0.1), phi~U(0,2 pi), vR,vz~N(0,0.1), vT~N(1,0.1), rows
+(R,vR,vT,z,vz,phi) in galpy's natural units.
+"""
+
+import numpy
+
+from . import potential as P
+
+
+def ics(n, seed=0):
+    rng = numpy.random.default_rng(seed)
+    R = rng.uniform(0.5, 2.0, n)
+    z = rng.normal(0.0, 0.1, n)
+    phi = rng.uniform(0.0, 2.0 * numpy.pi, n)
+    vR = rng.normal(0.0, 0.1, n)
+    vz = rng.normal(0.0, 0.1, n)
+    vT = rng.normal(1.0, 0.1, n)
+    return numpy.stack([R, vR, vT, z, vz, phi], axis=1)
+
+
+def mw2014():
+    return P.MWPotential2014_factory()
+
+
+def loghalo_bar():
+    """LogarithmicHaloPotential(q=0.9) + DehnenBarPotential(omegab=1.85, rb=0.8, Af=0.01)."""
+    return P.CompositePotential(
+        P.LogarithmicHaloPotential(q=0.9),
+        P.DehnenBarPotential(omegab=1.85, rb=0.8, Af=0.01),
+    )
+
+
+WORKLOADS = {
+    # 1,000 orbits, symplec4 with per-orbit step estimation, dense outputs
+    "mw2014_symplec4_1k": dict(pot=mw2014, n=1000, t=(0.0, 100.0, 10001), method="symplec4_hip",
+                              omega=0.0, seed=0),
+    # 1,048,576 orbits, DOP853 at galpy's default tolerances
+    "mw2014_dop853_1m": dict(pot=mw2014, n=1048576, t=(0.0, 500.0, 1001), method="dop853_hip",
+                            omega=0.0, seed=0),
+    # 262,144 orbits in a rotating bar (Jacobi integral instead of energy)
+    "loghalo_bar_dop853_256k": dict(pot=loghalo_bar, n=262144, t=(0.0, 500.0, 1001),
+                                   method="dop853_hip", omega=1.85, seed=1),
+}
+
+
+def make(name, n=None):
+    w = dict(WORKLOADS[name])
+    if n is not None:
+        w["n"] = n
+    t0, t1, nt = w["t"]
+    w["t"] = numpy.linspace(t0, t1, nt)
+    w["yo"] = ics(w["n"], seed=w["seed"])
+    w["pot"] = w["pot"]()
+    w["name"] = name
+    return w

@@ -445,7 +445,14 @@ void dop853(const ForceCtx<R>& F, double* y0d, int nt, const double* t, double r
       nacc++;
       facold = rmax(errv, R(1.0e-4));
       F.deriv(t_current, k5, k4);
+      // The dense-output coefficients and the three extra stages
+      // (leung_dop853.c:430-463) feed only the interpolation at requested
+      // times inside this step; the next step recomputes k2, k3 and k10 before
+      // reading them.  They are formed only when a requested time falls in
+      // this step, which leaves every output unchanged.
+      const bool need_dense = pos_neg * R(t[done + 1]) < pos_neg * t_current;
       R ydiff, bspl;
+      if (need_dense) {
       for (int i = 0; i < dim; i++) {
         rc1[i] = y0[i];
         ydiff = k5[i] - y0[i];
@@ -479,8 +486,7 @@ void dop853(const ForceCtx<R>& F, double* y0d, int nt, const double* t, double r
         rc6[i] = h * (rc6[i] + R(d513) * k4[i] + R(d514) * k10[i] + R(d515) * k2[i] + R(d516) * k3[i]);
         rc7[i] = h * (rc7[i] + R(d613) * k4[i] + R(d614) * k10[i] + R(d615) * k2[i] + R(d616) * k3[i]);
         rc8[i] = h * (rc8[i] + R(d713) * k4[i] + R(d714) * k10[i] + R(d715) * k2[i] + R(d716) * k3[i]);
-        k1[i] = k4[i];
-        y0[i] = k5[i];
+      }
       }
       while (done < nt - 1 && pos_neg * R(t[done + 1]) < pos_neg * t_current) {
         R s = (R(t[done + 1]) - t_old) / h;
@@ -490,6 +496,10 @@ void dop853(const ForceCtx<R>& F, double* y0d, int nt, const double* t, double r
                                                        s1 * (rc7[i] + s * rc8[i])))))));
         result += dim;
         done++;
+      }
+      for (int i = 0; i < dim; i++) {
+        k1[i] = k4[i];
+        y0[i] = k5[i];
       }
       hnew = (fabs(hnew) > fabs(hmax)) ? pos_neg * hmax : hnew;
       if (reject) hnew = pos_neg * rmin(fabs(hnew), fabs(h));
