@@ -108,209 +108,221 @@ __device__ __forceinline__ double gh_csign(double x, double y) {
   return (y > 0.0) ? fabs(x) : -fabs(x);
 }
 
-struct GhDopStats {
-  long long accepted;
-  long long rejected;
+// Per-lane DOP853 state between steps (leung_dop853.c:231-517 locals).
+struct GhDopState {
+  double y[6];   // current state (galpy's y0)
+  double k1[6];  // derivative at y (FSAL)
+  double h, t_cur, t_old, t_old_older, hmax, pos_neg;
+  int done;      // output rows written after row 0
+  int reject;
+  long long nacc, nrej;
 };
 
-// Integrates one orbit.  y0: rectangular state, advanced in place (as galpy's
-// dop853 does with its y0).  Rows are written through `out` (cylindrical).
-// rtol/atol are natural logs (integratePlanarOrbit.py:712-724).
-__device__ GhDopStats gh_dop853(const GhPotTable* __restrict__ T, double* y0, int nt,
-                                const double* __restrict__ t, double rtol_log, double atol_log,
-                                double* __restrict__ out, int* err,
-                                const volatile int* cancel) {
-  using namespace ghdp;
-  const double rtol = exp(rtol_log);
-  const double atol = exp(atol_log);
-  const double safe = 0.9;
-  // galpy also tracks facold for Lund stabilisation, but with beta = 0 the
-  // factor pow(facold, beta) is exactly 1 and facold never enters a result
-  const double expo1 = 1.0 / 8.0;
-  const double facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;
-  const double hmax = t[nt - 1] - t[0];
-  const double pos_neg = gh_csign(1.0, hmax);
-  double yy1[6], k1[6], k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6];
-  GhDopStats st = {0, 0};
-  gh_store_cyl(out, y0[0], y0[1], y0[2], y0[3], y0[4], y0[5]);
-  gh_deriv(T, t[0], y0, k1);
+// Start an orbit: row 0, first derivative and the initial step heuristic
+// (leung_dop853.c:276-325).  s.y holds the rectangular initial condition.
+__device__ __forceinline__ void gh_dop_init(const GhPotTable* __restrict__ T,
+                                            const double* __restrict__ t, int nt, double rtol,
+                                            double atol, GhDopState& s,
+                                            double* __restrict__ out) {
+  s.hmax = t[nt - 1] - t[0];
+  s.pos_neg = gh_csign(1.0, s.hmax);
+  s.done = 0;
+  s.reject = 0;
+  s.nacc = 0;
+  s.nrej = 0;
+  gh_store_cyl(out, s.y[0], s.y[1], s.y[2], s.y[3], s.y[4], s.y[5]);
+  gh_deriv(T, t[0], s.y, s.k1);
   double dnf = 0.0, dny = 0.0;
 #pragma unroll
   for (int i = 0; i < 6; i++) {
-    const double sk = atol + rtol * fabs(y0[i]);
-    double sqr = k1[i] / sk;
+    const double sk = atol + rtol * fabs(s.y[i]);
+    double sqr = s.k1[i] / sk;
     dnf += sqr * sqr;
-    sqr = y0[i] / sk;
+    sqr = s.y[i] / sk;
     dny += sqr * sqr;
   }
-  double h = gh_csign(fmin(sqrt(dny / dnf) * 0.01, fabs(hmax)), pos_neg);
+  double h = gh_csign(fmin(sqrt(dny / dnf) * 0.01, fabs(s.hmax)), s.pos_neg);
+  double k2[6], k3[6];
 #pragma unroll
-  for (int i = 0; i < 6; i++) k3[i] = y0[i] + h * k1[i];
+  for (int i = 0; i < 6; i++) k3[i] = s.y[i] + h * s.k1[i];
   gh_deriv(T, t[0] + h, k3, k2);
   double der2 = 0.0;
 #pragma unroll
   for (int i = 0; i < 6; i++) {
-    const double sk = atol + rtol * fabs(y0[i]);
-    const double sqr = (k2[i] - k1[i]) / sk;
+    const double sk = atol + rtol * fabs(s.y[i]);
+    const double sqr = (k2[i] - s.k1[i]) / sk;
     der2 += sqr * sqr;
   }
   der2 = sqrt(der2) / h;
   const double der12 = fmax(fabs(der2), sqrt(dnf));
   const double h1 = pow(0.01 / der12, 1.0 / 8.0);
-  h = gh_csign(fmin(100.0 * fabs(h), fmin(fabs(h1), fabs(hmax))), pos_neg);
-  int reject = 0;
-  double t_current = t[0], t_old = t[0], t_old_older = t[0];
-  int done = 0;
-  while (done < nt - 1) {
-    if (cancel != nullptr && ((st.accepted + st.rejected) & 255) == 0 && *cancel) {
-      *err = -10;
-      break;
-    }
-    h = pos_neg * fmax(fabs(h), 1e3 * uround);
+  s.h = gh_csign(fmin(100.0 * fabs(h), fmin(fabs(h1), fabs(s.hmax))), s.pos_neg);
+  s.t_cur = t[0];
+  s.t_old = t[0];
+  s.t_old_older = t[0];
+}
+
+// One attempted step (leung_dop853.c:335-492); returns true once every
+// requested time has been written.
+__device__ __forceinline__ bool gh_dop_step(const GhPotTable* __restrict__ T,
+                                            const double* __restrict__ t, int nt, double rtol,
+                                            double atol, GhDopState& s,
+                                            double* __restrict__ out) {
+  using namespace ghdp;
+  const double safe = 0.9;
+  const double expo1 = 1.0 / 8.0;
+  const double facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;
+  double yy1[6], k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6];
+  const double* y0 = s.y;
+  const double* k1 = s.k1;
+  const double h = s.pos_neg * fmax(fabs(s.h), 1e3 * uround);
+  const double tc = s.t_cur;
+  // galpy's dop853 ignores the result of the twelve stages when rejected, so
+  // galpy's accept/reject flow is kept exactly; only the accept branch is
+  // masked per lane.  (galpy also tracks facold for Lund stabilisation, but
+  // with beta = 0 pow(facold, beta) is exactly 1 and never enters a result.)
 #pragma unroll
-    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * a21 * k1[i];
-    gh_deriv(T, t_current + c2 * h, yy1, k2);
+  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * a21 * k1[i];
+  gh_deriv(T, tc + c2 * h, yy1, k2);
 #pragma unroll
-    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a31 * k1[i] + a32 * k2[i]);
-    gh_deriv(T, t_current + c3 * h, yy1, k3);
+  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a31 * k1[i] + a32 * k2[i]);
+  gh_deriv(T, tc + c3 * h, yy1, k3);
 #pragma unroll
-    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a41 * k1[i] + a43 * k3[i]);
-    gh_deriv(T, t_current + c4 * h, yy1, k4);
+  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a41 * k1[i] + a43 * k3[i]);
+  gh_deriv(T, tc + c4 * h, yy1, k4);
 #pragma unroll
-    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a51 * k1[i] + a53 * k3[i] + a54 * k4[i]);
-    gh_deriv(T, t_current + c5 * h, yy1, k5);
+  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a51 * k1[i] + a53 * k3[i] + a54 * k4[i]);
+  gh_deriv(T, tc + c5 * h, yy1, k5);
 #pragma unroll
-    for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a61 * k1[i] + a64 * k4[i] + a65 * k5[i]);
-    gh_deriv(T, t_current + c6 * h, yy1, k6);
+  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a61 * k1[i] + a64 * k4[i] + a65 * k5[i]);
+  gh_deriv(T, tc + c6 * h, yy1, k6);
 #pragma unroll
-    for (int i = 0; i < 6; i++)
-      yy1[i] = y0[i] + h * (a71 * k1[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]);
-    gh_deriv(T, t_current + c7 * h, yy1, k7);
+  for (int i = 0; i < 6; i++)
+    yy1[i] = y0[i] + h * (a71 * k1[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]);
+  gh_deriv(T, tc + c7 * h, yy1, k7);
 #pragma unroll
-    for (int i = 0; i < 6; i++)
-      yy1[i] = y0[i] + h * (a81 * k1[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] + a87 * k7[i]);
-    gh_deriv(T, t_current + c8 * h, yy1, k8);
+  for (int i = 0; i < 6; i++)
+    yy1[i] = y0[i] + h * (a81 * k1[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] + a87 * k7[i]);
+  gh_deriv(T, tc + c8 * h, yy1, k8);
 #pragma unroll
-    for (int i = 0; i < 6; i++)
-      yy1[i] = y0[i] + h * (a91 * k1[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] + a97 * k7[i] +
-                            a98 * k8[i]);
-    gh_deriv(T, t_current + c9 * h, yy1, k9);
+  for (int i = 0; i < 6; i++)
+    yy1[i] = y0[i] + h * (a91 * k1[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] + a97 * k7[i] +
+                          a98 * k8[i]);
+  gh_deriv(T, tc + c9 * h, yy1, k9);
 #pragma unroll
-    for (int i = 0; i < 6; i++)
-      yy1[i] = y0[i] + h * (a101 * k1[i] + a104 * k4[i] + a105 * k5[i] + a106 * k6[i] +
-                            a107 * k7[i] + a108 * k8[i] + a109 * k9[i]);
-    gh_deriv(T, t_current + c10 * h, yy1, k10);
+  for (int i = 0; i < 6; i++)
+    yy1[i] = y0[i] + h * (a101 * k1[i] + a104 * k4[i] + a105 * k5[i] + a106 * k6[i] +
+                          a107 * k7[i] + a108 * k8[i] + a109 * k9[i]);
+  gh_deriv(T, tc + c10 * h, yy1, k10);
 #pragma unroll
-    for (int i = 0; i < 6; i++)
-      yy1[i] = y0[i] + h * (a111 * k1[i] + a114 * k4[i] + a115 * k5[i] + a116 * k6[i] +
-                            a117 * k7[i] + a118 * k8[i] + a119 * k9[i] + a1110 * k10[i]);
-    gh_deriv(T, t_current + c11 * h, yy1, k2);
+  for (int i = 0; i < 6; i++)
+    yy1[i] = y0[i] + h * (a111 * k1[i] + a114 * k4[i] + a115 * k5[i] + a116 * k6[i] +
+                          a117 * k7[i] + a118 * k8[i] + a119 * k9[i] + a1110 * k10[i]);
+  gh_deriv(T, tc + c11 * h, yy1, k2);
 #pragma unroll
-    for (int i = 0; i < 6; i++)
-      yy1[i] = y0[i] + h * (a121 * k1[i] + a124 * k4[i] + a125 * k5[i] + a126 * k6[i] +
-                            a127 * k7[i] + a128 * k8[i] + a129 * k9[i] + a1210 * k10[i] +
-                            a1211 * k2[i]);
-    t_old_older = t_old;
-    t_old = t_current;
-    t_current = t_current + h;
-    gh_deriv(T, t_current, yy1, k3);
-    double err5 = 0.0, err3 = 0.0;
+  for (int i = 0; i < 6; i++)
+    yy1[i] = y0[i] + h * (a121 * k1[i] + a124 * k4[i] + a125 * k5[i] + a126 * k6[i] +
+                          a127 * k7[i] + a128 * k8[i] + a129 * k9[i] + a1210 * k10[i] +
+                          a1211 * k2[i]);
+  s.t_old_older = s.t_old;
+  s.t_old = tc;
+  s.t_cur = tc + h;
+  gh_deriv(T, s.t_cur, yy1, k3);
+  double err5 = 0.0, err3 = 0.0;
 #pragma unroll
-    for (int i = 0; i < 6; i++) {
-      k4[i] = b1 * k1[i] + b6 * k6[i] + b7 * k7[i] + b8 * k8[i] + b9 * k9[i] + b10 * k10[i] +
-              b11 * k2[i] + b12 * k3[i];
-      k5[i] = y0[i] + h * k4[i];
-      const double sk = atol + rtol * fmax(fabs(y0[i]), fabs(k5[i]));
-      double erri = k4[i] - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i];
-      double sqr = erri / sk;
-      err3 += sqr * sqr;
-      erri = er1 * k1[i] + er6 * k6[i] + er7 * k7[i] + er8 * k8[i] + er9 * k9[i] + er10 * k10[i] +
-             er11 * k2[i] + er12 * k3[i];
-      sqr = erri / sk;
-      err5 += sqr * sqr;
-    }
-    double deno = err5 + 0.01 * err3;
-    if (deno <= 0.0) deno = 1.0;
-    const double errv = fabs(h) * err5 * sqrt(1.0 / (deno * 6.0));
-    const double fac11 = pow(errv, expo1);
-    const double fac = fmax(facc2, fmin(facc1, fac11 / safe));
-    double hnew = h / fac;
-    if (errv <= 1.0) {
-      st.accepted++;
-      gh_deriv(T, t_current, k5, k4);
-      // dense output is only needed when a requested time falls inside this
-      // step; the three extra stages feed nothing else (the next step
-      // recomputes k2, k3, k10), so they are skipped otherwise
-      const bool need_dense = pos_neg * t[done + 1] < pos_neg * t_current;
-      if (need_dense) {
-        double rc1[6], rc2[6], rc3[6], rc4[6], rc5[6], rc6[6], rc7[6], rc8[6];
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-          rc1[i] = y0[i];
-          const double ydiff = k5[i] - y0[i];
-          rc2[i] = ydiff;
-          const double bspl = h * k1[i] - ydiff;
-          rc3[i] = bspl;
-          rc4[i] = ydiff - h * k4[i] - bspl;
-          rc5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
-                   d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
-          rc6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
-                   d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
-          rc7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
-                   d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
-          rc8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
-                   d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
-        }
-#pragma unroll
-        for (int i = 0; i < 6; i++)
-          yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
-                                a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] + a1413 * k4[i]);
-        gh_deriv(T, t_old + c14 * h, yy1, k10);
-#pragma unroll
-        for (int i = 0; i < 6; i++)
-          yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
-                                a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] + a1514 * k10[i]);
-        gh_deriv(T, t_old + c15 * h, yy1, k2);
-#pragma unroll
-        for (int i = 0; i < 6; i++)
-          yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
-                                a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] + a1615 * k2[i]);
-        gh_deriv(T, t_old + c16 * h, yy1, k3);
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-          rc5[i] = h * (rc5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
-          rc6[i] = h * (rc6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
-          rc7[i] = h * (rc7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
-          rc8[i] = h * (rc8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
-        }
-        while (done < nt - 1 && pos_neg * t[done + 1] < pos_neg * t_current) {
-          const double s = (t[done + 1] - t_old) / h;
-          const double s1 = 1.0 - s;
-          double yo[6];
-#pragma unroll
-          for (int i = 0; i < 6; i++)
-            yo[i] = rc1[i] + s * (rc2[i] + s1 * (rc3[i] + s * (rc4[i] + s1 * (rc5[i] + s * (rc6[i] +
-                                  s1 * (rc7[i] + s * rc8[i]))))));
-          done++;
-          gh_store_cyl(out + 6L * done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
-        }
-      }
+  for (int i = 0; i < 6; i++) {
+    k4[i] = b1 * k1[i] + b6 * k6[i] + b7 * k7[i] + b8 * k8[i] + b9 * k9[i] + b10 * k10[i] +
+            b11 * k2[i] + b12 * k3[i];
+    k5[i] = y0[i] + h * k4[i];
+    const double sk = atol + rtol * fmax(fabs(y0[i]), fabs(k5[i]));
+    double erri = k4[i] - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i];
+    double sqr = erri / sk;
+    err3 += sqr * sqr;
+    erri = er1 * k1[i] + er6 * k6[i] + er7 * k7[i] + er8 * k8[i] + er9 * k9[i] + er10 * k10[i] +
+           er11 * k2[i] + er12 * k3[i];
+    sqr = erri / sk;
+    err5 += sqr * sqr;
+  }
+  double deno = err5 + 0.01 * err3;
+  if (deno <= 0.0) deno = 1.0;
+  const double errv = fabs(h) * err5 * sqrt(1.0 / (deno * 6.0));
+  const double fac11 = pow(errv, expo1);
+  const double fac = fmax(facc2, fmin(facc1, fac11 / safe));
+  double hnew = h / fac;
+  if (errv <= 1.0) {
+    s.nacc++;
+    gh_deriv(T, s.t_cur, k5, k4);
+    // dense output only when a requested time falls inside this step; the
+    // three extra stages feed nothing else (the next step recomputes k2, k3
+    // and k10), so they are skipped otherwise -- outputs are unchanged
+    if (s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) {
+      double rc1[6], rc2[6], rc3[6], rc4[6], rc5[6], rc6[6], rc7[6], rc8[6];
 #pragma unroll
       for (int i = 0; i < 6; i++) {
-        k1[i] = k4[i];
-        y0[i] = k5[i];
+        rc1[i] = y0[i];
+        const double ydiff = k5[i] - y0[i];
+        rc2[i] = ydiff;
+        const double bspl = h * k1[i] - ydiff;
+        rc3[i] = bspl;
+        rc4[i] = ydiff - h * k4[i] - bspl;
+        rc5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
+                 d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
+        rc6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
+                 d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
+        rc7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
+                 d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
+        rc8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
+                 d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
       }
-      hnew = (fabs(hnew) > fabs(hmax)) ? pos_neg * hmax : hnew;
-      if (reject) hnew = pos_neg * fmin(fabs(hnew), fabs(h));
-      reject = 0;
-    } else {
-      st.rejected++;
-      hnew = h / fmin(facc1, fac11 / safe);
-      reject = 1;
-      t_current = t_old;
-      t_old = t_old_older;
+#pragma unroll
+      for (int i = 0; i < 6; i++)
+        yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
+                              a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] + a1413 * k4[i]);
+      gh_deriv(T, s.t_old + c14 * h, yy1, k10);
+#pragma unroll
+      for (int i = 0; i < 6; i++)
+        yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
+                              a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] + a1514 * k10[i]);
+      gh_deriv(T, s.t_old + c15 * h, yy1, k2);
+#pragma unroll
+      for (int i = 0; i < 6; i++)
+        yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
+                              a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] + a1615 * k2[i]);
+      gh_deriv(T, s.t_old + c16 * h, yy1, k3);
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        rc5[i] = h * (rc5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
+        rc6[i] = h * (rc6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
+        rc7[i] = h * (rc7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
+        rc8[i] = h * (rc8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
+      }
+      while (s.done < nt - 1 && s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) {
+        const double sd = (t[s.done + 1] - s.t_old) / h;
+        const double s1 = 1.0 - sd;
+        double yo[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+          yo[i] = rc1[i] + sd * (rc2[i] + s1 * (rc3[i] + sd * (rc4[i] + s1 * (rc5[i] + sd * (rc6[i] +
+                              s1 * (rc7[i] + sd * rc8[i]))))));
+        s.done++;
+        gh_store_cyl(out + 6L * s.done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
+      }
     }
-    h = hnew;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      s.k1[i] = k4[i];
+      s.y[i] = k5[i];
+    }
+    hnew = (fabs(hnew) > fabs(s.hmax)) ? s.pos_neg * s.hmax : hnew;
+    if (s.reject) hnew = s.pos_neg * fmin(fabs(hnew), fabs(h));
+    s.reject = 0;
+  } else {
+    s.nrej++;
+    hnew = h / fmin(facc1, fac11 / safe);
+    s.reject = 1;
+    s.t_cur = s.t_old;
+    s.t_old = s.t_old_older;
   }
-  return st;
+  s.h = hnew;
+  return s.done >= nt - 1;
 }

@@ -126,6 +126,7 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
         C.c[2] = amp * 2.0 * M_PI * std::pow(rc, 3.0 - alpha);
         C.c[3] = 1.0 / (rc * rc);
         C.c[4] = incgamma_xbig(a, C.c[1]);
+        C.c[9] = std::ldexp(C.c[1], -58);  // continued-fraction stopping tolerance
         if (fill_series(a, T->ser[i][0]) != 0) {
           set_error("incomplete-gamma series too short for a=%g", a);
           return -1;
@@ -134,6 +135,7 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
           C.c[5] = a2;
           C.c[6] = std::tgamma(a2);
           C.c[7] = incgamma_xbig(a2, C.c[6]);
+          C.c[10] = std::ldexp(C.c[6], -58);
           fill_series(a2, T->ser[i][1]);
         } else {  // galpy's Eval calls gsl_sf_gamma(1 - alpha/2): undefined here
           C.c[5] = a2;
@@ -251,34 +253,97 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
   }
 }
 
-// dop853 (leung_dop853.c:231-517)
-__global__ void __launch_bounds__(kBlock)
+// dop853 (leung_dop853.c:231-517), persistent lanes.  The grid holds only as
+// many wavefronts as the GPU keeps resident; lane g starts on orbit g and,
+// whenever its orbit finishes, claims the next unstarted orbit from a global
+// counter (one atomic per wavefront per refill, ranks from the ballot mask).
+// Orbits differ ~10x in accepted-step count at fixed tolerance, so without
+// refill a wavefront runs as long as its slowest lane.
+// Waves per SIMD the register allocator targets (build option; the default
+// lets the compiler use up to 512 registers per lane at one wave per SIMD).
+#ifdef GH_DOP_WAVES_PER_EU
+#define GH_DOP_ATTR __attribute__((amdgpu_waves_per_eu(GH_DOP_WAVES_PER_EU, GH_DOP_WAVES_PER_EU)))
+#else
+#define GH_DOP_ATTR
+#endif
+
+__global__ void __launch_bounds__(kBlock) GH_DOP_ATTR
 gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
-                 int indiv_t, double rtol, double atol, const GhPotTable* __restrict__ T,
+                 int indiv_t, double rtol_log, double atol_log, const GhPotTable* __restrict__ T,
                  double* __restrict__ result, int* __restrict__ err, double* __restrict__ stats,
-                 const volatile int* cancel) {
-  const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (orbit >= nobj) return;
-  const double* tt = t + (indiv_t ? orbit * nt : 0);
-  double y[6];
-  {
+                 const volatile int* cancel, unsigned long long* __restrict__ counter) {
+  const double rtol = exp(rtol_log);
+  const double atol = exp(atol_log);
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  GhDopState s;
+  const double* tt = t;
+  double* out = result;
+  bool active = false, exhausted = false, stop = false;
+
+  auto start = [&](long o) {
+    tt = t + (indiv_t ? o * nt : 0);
+    out = result + o * (long)nt * 6;
     double v[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) v[k] = yo[orbit * 6 + k];
-    gh_cyl_to_rect(v, y);
-  }
-  int e = 0;
-  const GhDopStats st =
-      gh_dop853(T, y, nt, tt, rtol, atol, result + orbit * (long)nt * 6, &e, cancel);
+    for (int k = 0; k < 6; k++) v[k] = yo[o * 6 + k];
+    gh_cyl_to_rect(v, s.y);
+    gh_dop_init(T, tt, nt, rtol, atol, s, out);
+  };
+  auto finish = [&](long o, int e) {
 #pragma unroll
-  for (int k = 0; k < 6; k++) yo[orbit * 6 + k] = y[k];  // galpy's y0 holds the last state
-  err[orbit] = e;
-  if (stats) {
-    stats[orbit * 3 + 0] = (double)st.accepted;
-    stats[orbit * 3 + 1] = (double)st.rejected;
-    stats[orbit * 3 + 2] = 0.0;
+    for (int k = 0; k < 6; k++) yo[o * 6 + k] = s.y[k];  // galpy's y0 holds the last state
+    err[o] = e;
+    if (stats) {
+      stats[o * 3 + 0] = (double)s.nacc;
+      stats[o * 3 + 1] = (double)s.nrej;
+      stats[o * 3 + 2] = 0.0;
+    }
+  };
+
+  if (orbit < nobj) {
+    start(orbit);
+    active = true;
+  }
+  for (unsigned iter = 0;; iter++) {
+    if (cancel != nullptr && (iter & 255u) == 0 && *cancel) stop = true;
+    if (active) {
+      if (stop) {
+        finish(orbit, -10);
+        active = false;
+      } else if (gh_dop_step(T, tt, nt, rtol, atol, s, out)) {
+        finish(orbit, 0);
+        active = false;
+      }
+    }
+    const bool want = !active && !exhausted;
+    const unsigned long long m = __ballot(want);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+      base = __shfl(base, leader);
+      if (want) {
+        const long o = (long)(base + (unsigned long long)__popcll(m & below));
+        if (o < nobj) {
+          orbit = o;
+          if (stop) {
+            err[o] = -10;  // interrupted before it started
+          } else {
+            start(o);
+            active = true;
+          }
+        } else {
+          exhausted = true;
+        }
+      }
+    }
+    if (__all(!active && exhausted)) break;
   }
 }
+
+__global__ void gh_set_counter(unsigned long long* c, unsigned long long v) { *c = v; }
 
 // Energy / Jacobi drift along stored cylindrical rows.
 __global__ void __launch_bounds__(kBlock)
@@ -322,6 +387,38 @@ gh_eval_kernel(int n, const double* __restrict__ R, const double* __restrict__ z
   if (Fphi) Fphi[i] = F.Fphi;
 }
 
+// Work counters for the persistent kernels: one small device array per
+// device, allocated on first use and kept for the life of the process; each
+// launch takes the next slot, so concurrent launches on different streams
+// never share a counter.
+constexpr int kCounterSlots = 4096;
+unsigned long long* g_counters[64] = {nullptr};
+unsigned g_counter_next = 0;
+int g_dop_resident_blocks[64] = {0};
+
+unsigned long long* next_counter(int dev) {
+  if (dev < 0 || dev >= 64) return nullptr;
+  if (!g_counters[dev]) {
+    void* p = nullptr;
+    if (hipMalloc(&p, kCounterSlots * sizeof(unsigned long long)) != hipSuccess) return nullptr;
+    g_counters[dev] = static_cast<unsigned long long*>(p);
+  }
+  const unsigned slot = __atomic_fetch_add(&g_counter_next, 1u, __ATOMIC_RELAXED) % kCounterSlots;
+  return g_counters[dev] + slot;
+}
+
+// workgroups of gh_dop853_kernel the device keeps resident at once
+int dop_resident_blocks(int dev) {
+  if (dev >= 0 && dev < 64 && g_dop_resident_blocks[dev] > 0) return g_dop_resident_blocks[dev];
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dop853_kernel, kBlock, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  const int n = std::max(1, per_cu) * std::max(1, ncu);
+  if (dev >= 0 && dev < 64) g_dop_resident_blocks[dev] = n;
+  return n;
+}
+
 int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                      const GhPotTable* d_table, double dt, double rtol, double atol,
                      double* d_result, int* d_err, int odeint_type, double* d_stats,
@@ -342,10 +439,22 @@ int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indi
       hipLaunchKernelGGL(gh_symplectic_kernel<3>, grid, block, 0, stream, nobj, d_yo, nt, d_t,
                          indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
       break;
-    case 6:
-      hipLaunchKernelGGL(gh_dop853_kernel, grid, block, 0, stream, nobj, d_yo, nt, d_t, indiv_t,
-                         rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+    case 6: {
+      int dev = 0;
+      GH_CHECK(hipGetDevice(&dev));
+      unsigned long long* counter = next_counter(dev);
+      const int resident = dop_resident_blocks(dev);
+      if (counter == nullptr || resident <= 0) {
+        set_error("could not set up the DOP853 work counter on device %d", dev);
+        return -1;
+      }
+      const unsigned nblocks = (unsigned)std::min<long>((long)grid.x, (long)resident);
+      hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter,
+                         (unsigned long long)nblocks * kBlock);
+      hipLaunchKernelGGL(gh_dop853_kernel, dim3(nblocks), block, 0, stream, nobj, d_yo, nt, d_t,
+                         indiv_t, rtol, atol, d_table, d_result, d_err, d_stats, cancel, counter);
       break;
+    }
     default:
       set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, 6 dop853)",
                 odeint_type);
