@@ -96,6 +96,78 @@ int fill_series(double a, double* ser) {
   return (last < 1e-18 * ser[0]) ? 0 : -1;
 }
 
+// Upper incomplete gamma Gamma(a,x), x >= a+1, in long double: Legendre's
+// continued fraction by the modified Lentz method (A&S 6.5.31).  Used only on
+// the host to fit the device tables.
+long double upper_gamma_ld(long double a, long double x) {
+  const long double tiny = 1e-300L;
+  long double b = x + 1.0L - a, c = 1.0L / tiny, d = 1.0L / b, h = d;
+  for (int i = 1; i < 100000; i++) {
+    const long double an = -i * (i - a);
+    b += 2.0L;
+    d = an * d + b;
+    if (fabsl(d) < tiny) d = tiny;
+    c = b + an / c;
+    if (fabsl(c) < tiny) c = tiny;
+    d = 1.0L / d;
+    const long double del = d * c;
+    h *= del;
+    if (fabsl(del - 1.0L) < 1e-19L) break;
+  }
+  return expl(a * logl(x) - x) * h;
+}
+
+// Fit the piecewise polynomial table of Gamma(a,x) on the octaves covering
+// [a+1, xbig): GH_TAB_PER_OCT segments per octave, interpolation at the
+// GH_TAB_NC Chebyshev nodes of each segment, stored as monomial coefficients
+// in the local coordinate t in [-1, 1).  Returns the first octave's frexp
+// exponent, or a large value if the range needs more than GH_TAB_MAXSEG.
+int fit_upper_gamma_table(double a, double xbig, double tab[GH_TAB_MAXSEG][GH_TAB_NC]) {
+  int e0, e1;
+  std::frexp(a + 1.0, &e0);
+  std::frexp(xbig, &e1);
+  if ((e1 - e0 + 1) * GH_TAB_PER_OCT > GH_TAB_MAXSEG) return 1 << 20;
+  const int N = GH_TAB_NC;
+  for (int e = e0; e <= e1; e++) {
+    for (int k = 0; k < GH_TAB_PER_OCT; k++) {
+      long double f[GH_TAB_NC], cheb[GH_TAB_NC];
+      for (int j = 0; j < N; j++) {
+        const long double tj = cosl(M_PI * (j + 0.5L) / N);
+        const long double m = 0.5L + (k + (tj + 1.0L) / 2.0L) / (2.0L * GH_TAB_PER_OCT);
+        f[j] = upper_gamma_ld(a, ldexpl(m, e));
+      }
+      for (int n = 0; n < N; n++) {
+        long double acc = 0.0L;
+        for (int j = 0; j < N; j++) acc += f[j] * cosl(M_PI * n * (j + 0.5L) / N);
+        cheb[n] = acc * 2.0L / N;
+      }
+      cheb[0] *= 0.5L;
+      // Chebyshev -> monomial: T_{n+1} = 2 t T_n - T_{n-1}
+      long double Tm1[GH_TAB_NC] = {0}, T0[GH_TAB_NC] = {0}, mono[GH_TAB_NC] = {0};
+      T0[0] = 1.0L;
+      for (int n = 0; n < N; n++) {
+        for (int q = 0; q < N; q++) mono[q] += cheb[n] * T0[q];
+        long double Tn1[GH_TAB_NC] = {0};
+        for (int q = 0; q < N; q++) {
+          if (q + 1 < N) Tn1[q + 1] += 2.0L * T0[q];
+          Tn1[q] -= (n == 0) ? 0.0L : Tm1[q];
+        }
+        if (n == 0) {  // T_1 = t
+          for (int q = 0; q < N; q++) Tn1[q] = 0.0L;
+          Tn1[1] = 1.0L;
+        }
+        for (int q = 0; q < N; q++) {
+          Tm1[q] = T0[q];
+          T0[q] = Tn1[q];
+        }
+      }
+      const int seg = (e - e0) * GH_TAB_PER_OCT + k;
+      for (int q = 0; q < N; q++) tab[seg][q] = (double)mono[q];
+    }
+  }
+  return e0;
+}
+
 int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPotTable* T) {
   std::memset(T, 0, sizeof(*T));
   if (npot < 1 || npot > GH_MAX_POT) {
@@ -103,6 +175,7 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
     return -1;
   }
   T->npot = npot;
+  int nslots = 0;
   for (int i = 0; i < npot; i++) {
     const int code = pot_type[i];
     const int na = nargs_of(code);
@@ -144,6 +217,17 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
           for (int n = 0; n < GH_IG_NSER; n++) T->ser[i][1][n] = NAN;
         }
         C.c[8] = 1.0 / rc;
+        if (nslots >= GH_TAB_SLOTS) {
+          set_error("at most %d PowerSphericalPotentialwCutoff components per potential", GH_TAB_SLOTS);
+          return -1;
+        }
+        C.c[11] = nslots;
+        C.c[12] = fit_upper_gamma_table(a, C.c[4], T->tab[nslots]);
+        if (C.c[12] > 1000) {
+          set_error("incomplete-gamma table range too wide for a=%g", a);
+          return -1;
+        }
+        nslots++;
         break;
       }
       case GH_MIYAMOTONAGAI: C.c[0] = C.p[2] * C.p[2]; break;
@@ -623,6 +707,52 @@ GH_EXPORT int gh_eval_hip(int n, const double* d_R, const double* d_z, const dou
   hipLaunchKernelGGL(gh_eval_kernel, grid, block, 0, (hipStream_t)stream, n, d_R, d_z, d_phi, d_t,
                      (const GhPotTable*)d_table, d_pot, d_FR, d_Fz, d_Fphi);
   GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+// Host evaluation of the force-kernel incomplete gamma gamma(a, x) of
+// component `comp` (a PowerSphericalPotentialwCutoff) exactly as the device
+// computes it from the decoded table: table on [a+1, xbig), series below,
+// Gamma(a) above.  For checks of the table fit without a GPU.
+GH_EXPORT int gh_host_bulge_lower_gamma(int npot, const int* pot_type, const double* pot_args,
+                                        int comp, int n, const double* x, double* out) {
+  g_errbuf[0] = 0;
+  static GhPotTable T;
+  if (decode_potential(npot, pot_type, pot_args, &T) != 0) return -1;
+  if (comp < 0 || comp >= npot || T.comp[comp].type != GH_POWERSPHCUTOFF) {
+    set_error("component %d is not a PowerSphericalPotentialwCutoff", comp);
+    return -1;
+  }
+  const GhComponent& C = T.comp[comp];
+  const double a = C.c[0];
+  const double* tab = &T.tab[(int)C.c[11]][0][0];
+  for (int i = 0; i < n; i++) {
+    const double xi = x[i];
+    double g;
+    if (xi >= C.c[4]) {
+      g = C.c[1];
+    } else if (xi <= 0.0) {
+      g = 0.0;
+    } else if (xi < a + 1.0) {
+      const double pref = std::exp(a * std::log(xi) - xi);
+      double sacc = T.ser[comp][0][GH_IG_NSER - 1];
+      for (int q = GH_IG_NSER - 2; q >= 0; q--) sacc = std::fma(sacc, xi, T.ser[comp][0][q]);
+      g = pref * sacc;
+    } else {
+      int e;
+      const double m = std::frexp(xi, &e);
+      const double u = (m - 0.5) * (2.0 * GH_TAB_PER_OCT);
+      int k = (int)u;
+      k = k > GH_TAB_PER_OCT - 1 ? GH_TAB_PER_OCT - 1 : k;
+      const int seg = (e - (int)C.c[12]) * GH_TAB_PER_OCT + k;
+      const double t = 2.0 * (u - k) - 1.0;
+      const double* c = tab + seg * GH_TAB_NC;
+      double v = c[GH_TAB_NC - 1];
+      for (int q = GH_TAB_NC - 2; q >= 0; q--) v = std::fma(v, t, c[q]);
+      g = C.c[1] - v;
+    }
+    out[i] = g;
+  }
   return 0;
 }
 

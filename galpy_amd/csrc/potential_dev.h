@@ -16,6 +16,12 @@
 
 #define GH_MAX_POT 8
 #define GH_IG_NSER 32  // terms of the incomplete-gamma series (enough for a <= 1.5)
+// piecewise-polynomial table of Gamma(a,x) on [a+1, xbig): GH_TAB_PER_OCT
+// segments per binary octave of x, degree GH_TAB_NC-1 in the local coordinate
+#define GH_TAB_PER_OCT 8
+#define GH_TAB_MAXSEG 64
+#define GH_TAB_NC 11
+#define GH_TAB_SLOTS 2  // PowerSphericalPotentialwCutoff components with a table
 
 enum GhPotCode {
   GH_LOGHALO = 0,
@@ -30,7 +36,7 @@ struct GhComponent {
   int type;
   int nonaxi;    // 1 if the component exerts a phi torque
   double p[8];   // raw galpy arguments, in galpy's order
-  double c[12];  // precomputed constants, per type (decoded on the host)
+  double c[14];  // precomputed constants, per type (decoded on the host)
 };
 
 struct GhPotTable {
@@ -42,6 +48,9 @@ struct GhPotTable {
   // for the force exponent a = 1.5 - alpha/2 ([0]) and the potential-value
   // exponent 1 - alpha/2 ([1]) of PowerSphericalPotentialwCutoff components
   double ser[GH_MAX_POT][2][GH_IG_NSER];
+  // Gamma(a,x) tables for the force exponent of up to GH_TAB_SLOTS
+  // PowerSphericalPotentialwCutoff components (slot in comp.c[11])
+  double tab[GH_TAB_SLOTS][GH_TAB_MAXSEG][GH_TAB_NC];
 };
 
 // ---------------------------------------------------------------------------
@@ -104,6 +113,38 @@ __device__ __forceinline__ double gh_lower_incgamma(double a, double x, double g
   return gamma_a - pref * (Am1 / Bm1);
 }
 
+// gamma(a,x) for the force kernel.  On [a+1, xbig) the upper tail
+// Gamma(a,x) comes from a host-fitted table instead of the continued
+// fraction: segment boundaries split each binary octave of x into
+// GH_TAB_PER_OCT equal parts, so every segment lies at least 8 half-widths
+// from the branch point of Gamma(a,x) at x = 0 and a degree-10 polynomial in
+// the local coordinate reproduces it to ~1e-17 Gamma(a).  The segment index
+// comes from the exponent and mantissa bits of x (frexp), so the evaluation
+// is a gather of 11 coefficients and 10 FMAs, with no exp/log and no
+// division.  Below a+1 the series branch of gh_lower_incgamma is used.
+// c[0] = a, c[1] = Gamma(a), c[4] = xbig, c[9] = continued-fraction
+// tolerance, c[11] = table slot, c[12] = frexp exponent of the first octave.
+__device__ __forceinline__ double gh_lower_incgamma_tab(const GhComponent* __restrict__ C,
+                                                       const double* __restrict__ ser,
+                                                       const double* __restrict__ tab,
+                                                       double x) {
+  const double a = C->c[0];
+  if (x >= C->c[4]) return C->c[1];
+  if (x < a + 1.0) return gh_lower_incgamma(a, x, C->c[1], C->c[4], C->c[9], ser);
+  int e;
+  const double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+  const double u = (m - 0.5) * (2.0 * GH_TAB_PER_OCT);  // in [0, GH_TAB_PER_OCT)
+  int k = (int)u;
+  k = k > GH_TAB_PER_OCT - 1 ? GH_TAB_PER_OCT - 1 : k;
+  const int seg = (e - (int)C->c[12]) * GH_TAB_PER_OCT + k;
+  const double t = 2.0 * (u - k) - 1.0;  // local coordinate in [-1, 1)
+  const double* __restrict__ c = tab + seg * GH_TAB_NC;
+  double v = c[GH_TAB_NC - 1];
+#pragma unroll
+  for (int n = GH_TAB_NC - 2; n >= 0; n--) v = fma(v, t, c[n]);
+  return C->c[1] - v;
+}
+
 // DehnenBarPotential.c:5-17 (quintic turn-on of the bar)
 __device__ __forceinline__ double gh_bar_smooth(double t, double tform, double tsteady) {
   if (t < tform) return 0.0;
@@ -139,6 +180,8 @@ __device__ __forceinline__ GhCyl gh_cyl_forces(const GhPotTable* __restrict__ T,
   GhCyl F = {0.0, 0.0, 0.0};
   const double z2 = z * z;
   const double r2 = R2 + z2;
+  const double rinv = rsqrt(r2);  // shared by the spherical components
+  const double r = r2 * rinv;
   const int npot = T->npot;
   for (int i = 0; i < npot; i++) {
     const GhComponent* __restrict__ C = &T->comp[i];
@@ -147,8 +190,9 @@ __device__ __forceinline__ GhCyl gh_cyl_forces(const GhPotTable* __restrict__ T,
         // PowerSphericalPotentialwCutoff.c:9-11,25-81:
         //   F = -amp M(r2) / r^3 (R, z), M = 2 pi rc^(3-alpha) gamma(a, r2/rc^2)
         const double x = r2 * C->c[3];
-        const double g = gh_lower_incgamma(C->c[0], x, C->c[1], C->c[4], C->c[9], T->ser[i][0]);
-        const double f = -C->c[2] * g / (r2 * sqrt(r2));
+        const double g =
+            gh_lower_incgamma_tab(C, T->ser[i][0], &T->tab[(int)C->c[11]][0][0], x);
+        const double f = -C->c[2] * g * (rinv * rinv * rinv);
         F.FR += f * R;
         F.Fz += f * z;
         break;
@@ -156,19 +200,21 @@ __device__ __forceinline__ GhCyl gh_cyl_forces(const GhPotTable* __restrict__ T,
       case GH_MIYAMOTONAGAI: {
         // MiyamotoNagaiPotential.c:16-53
         const double amp = C->p[0], a = C->p[1];
-        const double sqrtbz = sqrt(z2 + C->c[0]);
-        const double asqrtbz = a + sqrtbz;
+        const double s2 = z2 + C->c[0];
+        const double isbz = rsqrt(s2);
+        const double asqrtbz = a + s2 * isbz;
         const double D = R2 + asqrtbz * asqrtbz;
-        const double Dm32 = 1.0 / (D * sqrt(D));
+        const double iD = rsqrt(D);
+        const double Dm32 = iD * iD * iD;
         F.FR += -amp * R * Dm32;
-        F.Fz += (a == 0.0) ? -amp * z * Dm32 : -amp * z * asqrtbz / sqrtbz * Dm32;
+        F.Fz += (a == 0.0) ? -amp * z * Dm32 : -amp * z * asqrtbz * isbz * Dm32;
         break;
       }
       case GH_NFW: {
         // NFWPotential.c:16-49
         const double amp = C->p[0], a = C->p[1];
-        const double r = sqrt(r2);
-        const double br = 1.0 / (r2 * (a + r)) - log1p(r * C->c[0]) / (r * r2);
+        const double rinv2 = rinv * rinv;
+        const double br = rinv2 / (a + r) - log1p(r * C->c[0]) * (rinv2 * rinv);
         F.FR += amp * R * br;
         F.Fz += amp * z * br;
         break;
@@ -176,9 +222,8 @@ __device__ __forceinline__ GhCyl gh_cyl_forces(const GhPotTable* __restrict__ T,
       case GH_HERNQUIST: {
         // HernquistPotential.c:15-46: F = -amp x / (2 a^2 r) (1 + r/a)^-2
         const double a = C->p[1];
-        const double r = sqrt(r2);
         const double apr = a + r;
-        const double f = -C->c[0] / (r * apr * apr);
+        const double f = -C->c[0] * rinv / (apr * apr);
         F.FR += f * R;
         F.Fz += f * z;
         break;
@@ -208,7 +253,6 @@ __device__ __forceinline__ GhCyl gh_cyl_forces(const GhPotTable* __restrict__ T,
         const double sin2p = 2.0 * sinphi * cosphi;
         double cang, sang;
         gh_bar_angle(cos2p, sin2p, C->p[4], C->p[5], t, cang, sang);
-        const double r = sqrt(r2);
         const double r4 = r2 * r2;
         const double as = amp * smooth;
         if (r <= rb) {
@@ -238,8 +282,8 @@ __device__ __forceinline__ void gh_rect_force(const GhPotTable* __restrict__ T, 
                                               double x, double y, double z, double& ax,
                                               double& ay, double& az) {
   const double R2 = x * x + y * y;
-  const double R = sqrt(R2);
-  const double invR = 1.0 / R;
+  const double invR = rsqrt(R2);
+  const double R = R2 * invR;
   const double cosphi = x * invR;
   const double sinphi = y * invR;
   const GhCyl F = gh_cyl_forces(T, R, R2, z, cosphi, sinphi, t);

