@@ -96,12 +96,12 @@ constexpr double uround = 2.3e-16;
 
 // RHS of the first-order system: (v, a(q)) -- evalRectDeriv,
 // integrateFullOrbit.c:1326-1351
-__device__ __forceinline__ void gh_deriv(const GhPotTable* __restrict__ T, double t,
-                                         const double* y, double* k) {
+template <class F>
+__device__ __forceinline__ void gh_deriv(const F& T, double t, const double* y, double* k) {
   k[0] = y[3];
   k[1] = y[4];
   k[2] = y[5];
-  gh_rect_force(T, t, y[0], y[1], y[2], k[3], k[4], k[5]);
+  T.rect(t, y[0], y[1], y[2], k[3], k[4], k[5]);
 }
 
 __device__ __forceinline__ double gh_csign(double x, double y) {
@@ -120,7 +120,17 @@ struct GhDopState {
 
 // Start an orbit: row 0, first derivative and the initial step heuristic
 // (leung_dop853.c:276-325).  s.y holds the rectangular initial condition.
-__device__ __forceinline__ void gh_dop_init(const GhPotTable* __restrict__ T,
+// GH_DOP_OUTLINE moves the rarely executed parts (orbit start, dense output)
+// out of line to shrink the step loop's code; measured slower by default
+// because the calls pass stage vectors through scratch memory.
+#ifdef GH_DOP_OUTLINE
+#define GH_DOP_RARE __noinline__
+#else
+#define GH_DOP_RARE __forceinline__
+#endif
+
+template <class F>
+__device__ GH_DOP_RARE void gh_dop_init(const F& T,
                                             const double* __restrict__ t, int nt, double rtol,
                                             double atol, GhDopState& s,
                                             double* __restrict__ out) {
@@ -162,9 +172,78 @@ __device__ __forceinline__ void gh_dop_init(const GhPotTable* __restrict__ T,
   s.t_old_older = t[0];
 }
 
+// Dense output of an accepted step (leung_dop853.c:430-474): the continuous
+// extension coefficients, three extra stages and the interpolation at every
+// requested time inside the step.  Out of line: it runs on roughly one step
+// in seven, and inlining its three force evaluations would push the step
+// loop out of the instruction cache.  k2, k3, k10 are clobbered (the next
+// step recomputes them).
+template <class F>
+__device__ GH_DOP_RARE void gh_dop_dense(const F& T, const double* __restrict__ t, int nt,
+                                          GhDopState& s, double* __restrict__ out, double h,
+                                          const double* y0, const double* k1, double* k2,
+                                          double* k3, const double* k4, const double* k5,
+                                          const double* k6, const double* k7, const double* k8,
+                                          const double* k9, double* k10) {
+  using namespace ghdp;
+  double yy1[6];
+  double rc1[6], rc2[6], rc3[6], rc4[6], rc5[6], rc6[6], rc7[6], rc8[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    rc1[i] = y0[i];
+    const double ydiff = k5[i] - y0[i];
+    rc2[i] = ydiff;
+    const double bspl = h * k1[i] - ydiff;
+    rc3[i] = bspl;
+    rc4[i] = ydiff - h * k4[i] - bspl;
+    rc5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
+             d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
+    rc6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
+             d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
+    rc7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
+             d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
+    rc8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
+             d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+    yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
+                          a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] + a1413 * k4[i]);
+  gh_deriv(T, s.t_old + c14 * h, yy1, k10);
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+    yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
+                          a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] + a1514 * k10[i]);
+  gh_deriv(T, s.t_old + c15 * h, yy1, k2);
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+    yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
+                          a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] + a1615 * k2[i]);
+  gh_deriv(T, s.t_old + c16 * h, yy1, k3);
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    rc5[i] = h * (rc5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
+    rc6[i] = h * (rc6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
+    rc7[i] = h * (rc7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
+    rc8[i] = h * (rc8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
+  }
+  while (s.done < nt - 1 && s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) {
+    const double sd = (t[s.done + 1] - s.t_old) / h;
+    const double s1 = 1.0 - sd;
+    double yo[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+      yo[i] = rc1[i] + sd * (rc2[i] + s1 * (rc3[i] + sd * (rc4[i] + s1 * (rc5[i] + sd * (rc6[i] +
+                          s1 * (rc7[i] + sd * rc8[i]))))));
+    s.done++;
+    gh_store_cyl(out + 6L * s.done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
+  }
+}
+
 // One attempted step (leung_dop853.c:335-492); returns true once every
 // requested time has been written.
-__device__ __forceinline__ bool gh_dop_step(const GhPotTable* __restrict__ T,
+template <class F>
+__device__ __forceinline__ bool gh_dop_step(const F& T,
                                             const double* __restrict__ t, int nt, double rtol,
                                             double atol, GhDopState& s,
                                             double* __restrict__ out) {
@@ -255,59 +334,8 @@ __device__ __forceinline__ bool gh_dop_step(const GhPotTable* __restrict__ T,
     // dense output only when a requested time falls inside this step; the
     // three extra stages feed nothing else (the next step recomputes k2, k3
     // and k10), so they are skipped otherwise -- outputs are unchanged
-    if (s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) {
-      double rc1[6], rc2[6], rc3[6], rc4[6], rc5[6], rc6[6], rc7[6], rc8[6];
-#pragma unroll
-      for (int i = 0; i < 6; i++) {
-        rc1[i] = y0[i];
-        const double ydiff = k5[i] - y0[i];
-        rc2[i] = ydiff;
-        const double bspl = h * k1[i] - ydiff;
-        rc3[i] = bspl;
-        rc4[i] = ydiff - h * k4[i] - bspl;
-        rc5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
-                 d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
-        rc6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
-                 d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
-        rc7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
-                 d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
-        rc8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
-                 d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
-      }
-#pragma unroll
-      for (int i = 0; i < 6; i++)
-        yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
-                              a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] + a1413 * k4[i]);
-      gh_deriv(T, s.t_old + c14 * h, yy1, k10);
-#pragma unroll
-      for (int i = 0; i < 6; i++)
-        yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
-                              a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] + a1514 * k10[i]);
-      gh_deriv(T, s.t_old + c15 * h, yy1, k2);
-#pragma unroll
-      for (int i = 0; i < 6; i++)
-        yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
-                              a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] + a1615 * k2[i]);
-      gh_deriv(T, s.t_old + c16 * h, yy1, k3);
-#pragma unroll
-      for (int i = 0; i < 6; i++) {
-        rc5[i] = h * (rc5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
-        rc6[i] = h * (rc6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
-        rc7[i] = h * (rc7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
-        rc8[i] = h * (rc8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
-      }
-      while (s.done < nt - 1 && s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) {
-        const double sd = (t[s.done + 1] - s.t_old) / h;
-        const double s1 = 1.0 - sd;
-        double yo[6];
-#pragma unroll
-        for (int i = 0; i < 6; i++)
-          yo[i] = rc1[i] + sd * (rc2[i] + s1 * (rc3[i] + sd * (rc4[i] + s1 * (rc5[i] + sd * (rc6[i] +
-                              s1 * (rc7[i] + sd * rc8[i]))))));
-        s.done++;
-        gh_store_cyl(out + 6L * s.done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
-      }
-    }
+    if (s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur)
+      gh_dop_dense(T, t, nt, s, out, h, y0, k1, k2, k3, k4, k5, k6, k7, k8, k9, k10);
 #pragma unroll
     for (int i = 0; i < 6; i++) {
       s.k1[i] = k4[i];

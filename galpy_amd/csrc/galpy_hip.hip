@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <vector>
 #include <algorithm>
+#include <initializer_list>
 
 #include "../../include/galpy_hip.h"
 #include "dop853_dev.h"
@@ -252,7 +253,7 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
 // ---------------------------------------------------------------------------
 
 // leapfrog (bovy_symplecticode.c:132-224) and symplec4 (:250-377)
-template <int METHOD>
+template <int METHOD, class Spec>
 __global__ void __launch_bounds__(kBlock)
 gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
                      int indiv_t, double dt_in, double rtol, double atol,
@@ -261,6 +262,7 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
                      const volatile int* cancel) {
   const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (orbit >= nobj) return;
+  const GhForce<Spec> F{T};
   const double* tt = t + (indiv_t ? orbit * nt : 0);
   double w[6];
   {
@@ -279,9 +281,9 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
   double dt = dt_in;
   if (dt_in == GH_DT_SENTINEL) {
     if (METHOD == 0)
-      dt = gh_leapfrog_estimate_step(T, q, p, init_dt, tt[0], rtol, atol);
+      dt = gh_leapfrog_estimate_step(F, q, p, init_dt, tt[0], rtol, atol);
     else
-      dt = gh_symplec4_estimate_step(T, q, p, init_dt, tt[0], rtol, atol);
+      dt = gh_symplec4_estimate_step(F, q, p, init_dt, tt[0], rtol, atol);
   }
   const long ndt = (long)(init_dt / dt);
   double to = tt[0];
@@ -294,11 +296,11 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
       // leapfrog: half drift, ndt-1 fused kick+drift, kick + half drift
       GhVec3 q12 = gh_drift(q, p, dt / 2.0);
       for (long jj = 0; jj < ndt - 1; jj++) {
-        p = gh_drift(p, gh_accel(T, to + dt / 2.0, q12), dt);
+        p = gh_drift(p, gh_accel(F, to + dt / 2.0, q12), dt);
         q12 = gh_drift(q12, p, dt);
         to = to + dt;
       }
-      p = gh_drift(p, gh_accel(T, to + dt / 2.0, q12), dt);
+      p = gh_drift(p, gh_accel(F, to + dt / 2.0, q12), dt);
       q = gh_drift(q12, p, dt / 2.0);
       to = to + dt;
     } else {
@@ -307,23 +309,23 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
       q = gh_drift(q, p, c1 * dt);
       to += c1 * dt;
       for (long jj = 0; jj < ndt - 1; jj++) {
-        p = gh_drift(p, gh_accel(T, to, q), d1 * dt);
+        p = gh_drift(p, gh_accel(F, to, q), d1 * dt);
         q = gh_drift(q, p, c2 * dt);
         to += c2 * dt;
-        p = gh_drift(p, gh_accel(T, to, q), d2 * dt);
+        p = gh_drift(p, gh_accel(F, to, q), d2 * dt);
         q = gh_drift(q, p, c3 * dt);
         to += c3 * dt;
-        p = gh_drift(p, gh_accel(T, to, q), d3 * dt);
+        p = gh_drift(p, gh_accel(F, to, q), d3 * dt);
         q = gh_drift(q, p, (c4 + c1) * dt);
         to += (c4 + c1) * dt;
       }
-      p = gh_drift(p, gh_accel(T, to, q), d1 * dt);
+      p = gh_drift(p, gh_accel(F, to, q), d1 * dt);
       q = gh_drift(q, p, c2 * dt);
       to += c2 * dt;
-      p = gh_drift(p, gh_accel(T, to, q), d2 * dt);
+      p = gh_drift(p, gh_accel(F, to, q), d2 * dt);
       q = gh_drift(q, p, c3 * dt);
       to += c3 * dt;
-      p = gh_drift(p, gh_accel(T, to, q), d3 * dt);
+      p = gh_drift(p, gh_accel(F, to, q), d3 * dt);
       q = gh_drift(q, p, c4 * dt);
       to += c4 * dt;
     }
@@ -343,19 +345,23 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
 // counter (one atomic per wavefront per refill, ranks from the ballot mask).
 // Orbits differ ~10x in accepted-step count at fixed tolerance, so without
 // refill a wavefront runs as long as its slowest lane.
-// Waves per SIMD the register allocator targets (build option; the default
-// lets the compiler use up to 512 registers per lane at one wave per SIMD).
-#ifdef GH_DOP_WAVES_PER_EU
-#define GH_DOP_ATTR __attribute__((amdgpu_waves_per_eu(GH_DOP_WAVES_PER_EU, GH_DOP_WAVES_PER_EU)))
-#else
-#define GH_DOP_ATTR
+// Waves per SIMD the register allocator targets.  With the force specialised
+// per composition, one wave per SIMD with ~380 registers and no spills
+// measured fastest on MI355X (2.47 s vs 2.83 s at two waves with 512 B/lane
+// of spills, for 1M orbits of MWPotential2014); build with
+// -DGH_DOP_WAVES_PER_EU=2 to trade registers for occupancy.
+#ifndef GH_DOP_WAVES_PER_EU
+#define GH_DOP_WAVES_PER_EU 1
 #endif
+#define GH_DOP_ATTR __attribute__((amdgpu_waves_per_eu(GH_DOP_WAVES_PER_EU, GH_DOP_WAVES_PER_EU)))
 
+template <class Spec>
 __global__ void __launch_bounds__(kBlock) GH_DOP_ATTR
 gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
                  int indiv_t, double rtol_log, double atol_log, const GhPotTable* __restrict__ T,
                  double* __restrict__ result, int* __restrict__ err, double* __restrict__ stats,
                  const volatile int* cancel, unsigned long long* __restrict__ counter) {
+  const GhForce<Spec> F{T};
   const double rtol = exp(rtol_log);
   const double atol = exp(atol_log);
   const int lane = threadIdx.x & 63;
@@ -373,7 +379,7 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
 #pragma unroll
     for (int k = 0; k < 6; k++) v[k] = yo[o * 6 + k];
     gh_cyl_to_rect(v, s.y);
-    gh_dop_init(T, tt, nt, rtol, atol, s, out);
+    gh_dop_init(F, tt, nt, rtol, atol, s, out);
   };
   auto finish = [&](long o, int e) {
 #pragma unroll
@@ -396,7 +402,7 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
       if (stop) {
         finish(orbit, -10);
         active = false;
-      } else if (gh_dop_step(T, tt, nt, rtol, atol, s, out)) {
+      } else if (gh_dop_step(F, tt, nt, rtol, atol, s, out)) {
         finish(orbit, 0);
         active = false;
       }
@@ -491,43 +497,49 @@ unsigned long long* next_counter(int dev) {
   return g_counters[dev] + slot;
 }
 
-// workgroups of gh_dop853_kernel the device keeps resident at once
+// workgroups of gh_dop853_kernel<Spec> the device keeps resident at once
+template <class Spec>
 int dop_resident_blocks(int dev) {
-  if (dev >= 0 && dev < 64 && g_dop_resident_blocks[dev] > 0) return g_dop_resident_blocks[dev];
+  static int cached[64] = {0};
+  if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
   int per_cu = 0, ncu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dop853_kernel, kBlock, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dop853_kernel<Spec>, kBlock, 0) !=
+          hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
   const int n = std::max(1, per_cu) * std::max(1, ncu);
-  if (dev >= 0 && dev < 64) g_dop_resident_blocks[dev] = n;
+  if (dev >= 0 && dev < 64) cached[dev] = n;
   return n;
 }
 
-int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
-                     const GhPotTable* d_table, double dt, double rtol, double atol,
-                     double* d_result, int* d_err, int odeint_type, double* d_stats,
-                     hipStream_t stream) {
-  if (nobj == 0) return 0;
-  if (nobj < 0 || nt < 2) {
-    set_error("invalid sizes nobj=%d nt=%d (need nt >= 2)", nobj, nt);
-    return -1;
-  }
+// Host handle of an uploaded potential: the device table plus the component
+// codes, so launches can pick a kernel specialised for the composition.
+struct GhPotHandle {
+  GhPotTable* dev;
+  int npot;
+  int types[GH_MAX_POT];
+};
+
+template <class Spec>
+int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                const GhPotTable* d_table, double dt, double rtol, double atol, double* d_result,
+                int* d_err, int odeint_type, double* d_stats, hipStream_t stream) {
   const int* cancel = irq_device_ptr();
   const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
   switch (odeint_type) {
     case 0:
-      hipLaunchKernelGGL(gh_symplectic_kernel<0>, grid, block, 0, stream, nobj, d_yo, nt, d_t,
-                         indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+      hipLaunchKernelGGL((gh_symplectic_kernel<0, Spec>), grid, block, 0, stream, nobj, d_yo, nt,
+                         d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
       break;
     case 3:
-      hipLaunchKernelGGL(gh_symplectic_kernel<3>, grid, block, 0, stream, nobj, d_yo, nt, d_t,
-                         indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+      hipLaunchKernelGGL((gh_symplectic_kernel<3, Spec>), grid, block, 0, stream, nobj, d_yo, nt,
+                         d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
       break;
     case 6: {
       int dev = 0;
       GH_CHECK(hipGetDevice(&dev));
       unsigned long long* counter = next_counter(dev);
-      const int resident = dop_resident_blocks(dev);
+      const int resident = dop_resident_blocks<Spec>(dev);
       if (counter == nullptr || resident <= 0) {
         set_error("could not set up the DOP853 work counter on device %d", dev);
         return -1;
@@ -535,8 +547,9 @@ int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indi
       const unsigned nblocks = (unsigned)std::min<long>((long)grid.x, (long)resident);
       hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter,
                          (unsigned long long)nblocks * kBlock);
-      hipLaunchKernelGGL(gh_dop853_kernel, dim3(nblocks), block, 0, stream, nobj, d_yo, nt, d_t,
-                         indiv_t, rtol, atol, d_table, d_result, d_err, d_stats, cancel, counter);
+      hipLaunchKernelGGL(gh_dop853_kernel<Spec>, dim3(nblocks), block, 0, stream, nobj, d_yo, nt,
+                         d_t, indiv_t, rtol, atol, d_table, d_result, d_err, d_stats, cancel,
+                         counter);
       break;
     }
     default:
@@ -546,6 +559,41 @@ int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indi
   }
   GH_CHECK(hipGetLastError());
   return 0;
+}
+
+bool types_are(const GhPotHandle* h, std::initializer_list<int> codes) {
+  if ((int)codes.size() != h->npot) return false;
+  int i = 0;
+  for (int c : codes)
+    if (h->types[i++] != c) return false;
+  return true;
+}
+
+// Compositions with a specialised kernel: MWPotential2014, the rotating-bar
+// model, and single Hernquist / logarithmic halos; anything else runs the
+// generic kernel that dispatches on the component type at run time.
+int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                     const GhPotHandle* h, double dt, double rtol, double atol, double* d_result,
+                     int* d_err, int odeint_type, double* d_stats, hipStream_t stream) {
+  if (nobj == 0) return 0;
+  if (nobj < 0 || nt < 2) {
+    set_error("invalid sizes nobj=%d nt=%d (need nt >= 2)", nobj, nt);
+    return -1;
+  }
+#define GH_LAUNCH(SPEC)                                                                        \
+  return launch_spec<SPEC>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result, d_err, \
+                           odeint_type, d_stats, stream)
+  using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
+  using LogBar = GhSpec<GH_LOGHALO, GH_DEHNENBAR>;
+  using Hern = GhSpec<GH_HERNQUIST>;
+  using Log = GhSpec<GH_LOGHALO>;
+  using Any = GhSpec<>;
+  if (types_are(h, {GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW})) GH_LAUNCH(MW2014);
+  if (types_are(h, {GH_LOGHALO, GH_DEHNENBAR})) GH_LAUNCH(LogBar);
+  if (types_are(h, {GH_HERNQUIST})) GH_LAUNCH(Hern);
+  if (types_are(h, {GH_LOGHALO})) GH_LAUNCH(Log);
+  GH_LAUNCH(Any);
+#undef GH_LAUNCH
 }
 
 // SIGINT during the blocking host call sets the polled flag, as galpy's
@@ -572,6 +620,10 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
   hipStream_t stream;
   GH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   void* d_table = nullptr;
+  GhPotHandle handle;
+  handle.dev = nullptr;
+  handle.npot = table.npot;
+  for (int i = 0; i < GH_MAX_POT; i++) handle.types[i] = table.comp[i].type;
   double *d_yo = nullptr, *d_t = nullptr, *d_res = nullptr;
   int* d_err = nullptr;
   int rc = 0;
@@ -597,6 +649,7 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
       break;
     }
     if (hipMemcpyAsync(d_table, &table, sizeof(GhPotTable), hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+    handle.dev = (GhPotTable*)d_table;
     if (!indiv_t &&
         hipMemcpyAsync(d_t, t, (size_t)nt * 8, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
     for (long o0 = 0; o0 < nobj && rc == 0; o0 += chunk) {
@@ -604,7 +657,7 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
       if (hipMemcpyAsync(d_yo, yo + 6 * o0, (size_t)n * 48, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
       if (indiv_t && hipMemcpyAsync(d_t, t + (size_t)nt * o0, (size_t)n * nt * 8,
                                     hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
-      if (launch_integrate((int)n, d_yo, nt, d_t, indiv_t, (const GhPotTable*)d_table, dt, rtol,
+      if (launch_integrate((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
                            atol, d_res, d_err, odeint_type, nullptr, stream) != 0) { rc = -1; break; }
       if (hipMemcpyAsync(result + (size_t)6 * nt * o0, d_res, (size_t)n * nt * 48,
                          hipMemcpyDeviceToHost, stream) != hipSuccess ||
@@ -657,55 +710,72 @@ GH_EXPORT void integrateFullOrbit(int nobj, double* yo, int nt, double* t, int i
 }
 
 GH_EXPORT int integrateFullOrbit_hip(int nobj, double* d_yo, int nt, const double* d_t,
-                                     int indiv_t, const void* d_table, double dt, double rtol,
+                                     int indiv_t, const void* pot, double dt, double rtol,
                                      double atol, double* d_result, int* d_err, int odeint_type,
                                      double* d_stats, void* stream) {
   g_errbuf[0] = 0;
-  if (d_table == nullptr) {
-    set_error("d_table is NULL");
+  if (pot == nullptr) {
+    set_error("potential handle is NULL");
     return -1;
   }
-  return launch_integrate(nobj, d_yo, nt, d_t, indiv_t, (const GhPotTable*)d_table, dt, rtol, atol,
+  return launch_integrate(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot, dt, rtol, atol,
                           d_result, d_err, odeint_type, d_stats, (hipStream_t)stream);
 }
 
 GH_EXPORT int gh_potential_upload(int npot, const int* pot_type, const double* pot_args,
-                                  void** d_table) {
+                                  void** pot) {
   g_errbuf[0] = 0;
   GhPotTable table;
   if (decode_potential(npot, pot_type, pot_args, &table) != 0) return -1;
   void* d = nullptr;
   GH_CHECK(hipMalloc(&d, sizeof(GhPotTable)));
   GH_CHECK(hipMemcpy(d, &table, sizeof(GhPotTable), hipMemcpyHostToDevice));
-  *d_table = d;
+  GhPotHandle* h = new GhPotHandle;
+  h->dev = (GhPotTable*)d;
+  h->npot = table.npot;
+  for (int i = 0; i < GH_MAX_POT; i++) h->types[i] = table.comp[i].type;
+  *pot = h;
   return 0;
 }
 
-GH_EXPORT int gh_potential_free(void* d_table) {
-  GH_CHECK(hipFree(d_table));
+GH_EXPORT int gh_potential_free(void* pot) {
+  if (pot == nullptr) return 0;
+  GhPotHandle* h = (GhPotHandle*)pot;
+  const hipError_t e = hipFree(h->dev);
+  delete h;
+  if (e != hipSuccess) {
+    set_error("hipFree failed: %s", hipGetErrorString(e));
+    return -1;
+  }
   return 0;
+}
+
+// device address of the table behind a handle (for callers that build their
+// own launch descriptors, e.g. batched work lists)
+GH_EXPORT const void* gh_potential_device_ptr(const void* pot) {
+  return pot ? ((const GhPotHandle*)pot)->dev : nullptr;
 }
 
 GH_EXPORT int gh_energy_drift_hip(int nobj, int nt, const double* d_rows, const double* d_t,
-                                  int indiv_t, const void* d_table, double omega, double* d_e0,
+                                  int indiv_t, const void* pot, double omega, double* d_e0,
                                   double* d_demax, void* stream) {
   g_errbuf[0] = 0;
   if (nobj <= 0) return 0;
   const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
   hipLaunchKernelGGL(gh_energy_kernel, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows, d_t,
-                     indiv_t, (const GhPotTable*)d_table, omega, d_e0, d_demax);
+                     indiv_t, ((const GhPotHandle*)pot)->dev, omega, d_e0, d_demax);
   GH_CHECK(hipGetLastError());
   return 0;
 }
 
 GH_EXPORT int gh_eval_hip(int n, const double* d_R, const double* d_z, const double* d_phi,
-                          const double* d_t, const void* d_table, double* d_pot, double* d_FR,
+                          const double* d_t, const void* pot, double* d_pot, double* d_FR,
                           double* d_Fz, double* d_Fphi, void* stream) {
   g_errbuf[0] = 0;
   if (n <= 0) return 0;
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
   hipLaunchKernelGGL(gh_eval_kernel, grid, block, 0, (hipStream_t)stream, n, d_R, d_z, d_phi, d_t,
-                     (const GhPotTable*)d_table, d_pot, d_FR, d_Fz, d_Fphi);
+                     ((const GhPotHandle*)pot)->dev, d_pot, d_FR, d_Fz, d_Fphi);
   GH_CHECK(hipGetLastError());
   return 0;
 }

@@ -29,9 +29,10 @@ __device__ __forceinline__ GhVec3 gh_drift(GhVec3 q, GhVec3 p, double h) {
   return GhVec3{q.x + h * p.x, q.y + h * p.y, q.z + h * p.z};
 }
 
-__device__ __forceinline__ GhVec3 gh_accel(const GhPotTable* __restrict__ T, double t, GhVec3 q) {
+template <class F>
+__device__ __forceinline__ GhVec3 gh_accel(const F& T, double t, GhVec3 q) {
   GhVec3 a;
-  gh_rect_force(T, t, q.x, q.y, q.z, a.x, a.y, a.z);
+  T.rect(t, q.x, q.y, q.z, a.x, a.y, a.z);
   return a;
 }
 
@@ -60,7 +61,8 @@ __device__ __forceinline__ double gh_est_err(GhVec3 q11, GhVec3 q12, GhVec3 p11,
 }
 
 // leapfrog_estimate_step, bovy_symplecticode.c:588-663
-__device__ double gh_leapfrog_estimate_step(const GhPotTable* __restrict__ T, GhVec3 qo,
+template <class F>
+__device__ double gh_leapfrog_estimate_step(const F& T, GhVec3 qo,
                                             GhVec3 po, double dt, double to, double rtol,
                                             double atol) {
   double err = 2.0;
@@ -87,7 +89,8 @@ __device__ double gh_leapfrog_estimate_step(const GhPotTable* __restrict__ T, Gh
 }
 
 // symplec4_estimate_step, bovy_symplecticode.c:665-808
-__device__ double gh_symplec4_estimate_step(const GhPotTable* __restrict__ T, GhVec3 qo,
+template <class F>
+__device__ double gh_symplec4_estimate_step(const F& T, GhVec3 qo,
                                             GhVec3 po, double dt, double to, double rtol,
                                             double atol) {
   const double c1 = GH_S4C1, c2 = GH_S4C2, c3 = GH_S4C3, c4 = GH_S4C4;

@@ -113,6 +113,19 @@ __device__ __forceinline__ double gh_lower_incgamma(double a, double x, double g
   return gamma_a - pref * (Am1 / Bm1);
 }
 
+// Series branch of gh_lower_incgamma (x < a+1, i.e. r < rc sqrt(a+1)), kept
+// out of line: orbits rarely enter it, and every inlined copy of the force
+// would otherwise carry its exp/log code through the instruction cache.
+__device__ __noinline__ double gh_lower_incgamma_series(double a, double x,
+                                                       const double* __restrict__ ser) {
+  if (x <= 0.0) return 0.0;
+  const double pref = exp(a * log(x) - x);
+  double s = ser[GH_IG_NSER - 1];
+#pragma unroll
+  for (int n = GH_IG_NSER - 2; n >= 0; n--) s = fma(s, x, ser[n]);
+  return pref * s;
+}
+
 // gamma(a,x) for the force kernel.  On [a+1, xbig) the upper tail
 // Gamma(a,x) comes from a host-fitted table instead of the continued
 // fraction: segment boundaries split each binary octave of x into
@@ -130,7 +143,7 @@ __device__ __forceinline__ double gh_lower_incgamma_tab(const GhComponent* __res
                                                        double x) {
   const double a = C->c[0];
   if (x >= C->c[4]) return C->c[1];
-  if (x < a + 1.0) return gh_lower_incgamma(a, x, C->c[1], C->c[4], C->c[9], ser);
+  if (x < a + 1.0) return gh_lower_incgamma_series(a, x, ser);
   int e;
   const double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
   const double u = (m - 0.5) * (2.0 * GH_TAB_PER_OCT);  // in [0, GH_TAB_PER_OCT)
@@ -172,112 +185,183 @@ struct GhCyl {
   double FR, Fz, Fphi;
 };
 
+// Geometry shared by all components of one force evaluation.
+struct GhGeom {
+  double R, R2, z, z2, r2, rinv, r, cosphi, sinphi, t;
+};
+
+// Force of one component of type TYPE (index i in the table), added to F.
+// Component kernels: PowerSphericalPotentialwCutoff.c:9-81,
+// MiyamotoNagaiPotential.c:16-53, NFWPotential.c:16-49,
+// HernquistPotential.c:15-46, LogarithmicHaloPotential.c:21-91,
+// DehnenBarPotential.c:18-124.
+template <int TYPE>
+__device__ __forceinline__ void gh_comp_force(const GhPotTable* __restrict__ T, int i,
+                                              const GhGeom& g, GhCyl& F);
+
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_POWERSPHCUTOFF>(const GhPotTable* __restrict__ T,
+                                                                 int i, const GhGeom& g, GhCyl& F) {
+  // F = -amp M(r2) / r^3 (R, z), M = 2 pi rc^(3-alpha) gamma(a, r2/rc^2)
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double x = g.r2 * C->c[3];
+  const double gam = gh_lower_incgamma_tab(C, T->ser[i][0], &T->tab[(int)C->c[11]][0][0], x);
+  const double f = -C->c[2] * gam * (g.rinv * g.rinv * g.rinv);
+  F.FR += f * g.R;
+  F.Fz += f * g.z;
+}
+
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_MIYAMOTONAGAI>(const GhPotTable* __restrict__ T,
+                                                                int i, const GhGeom& g, GhCyl& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0], a = C->p[1];
+  const double s2 = g.z2 + C->c[0];
+  const double isbz = rsqrt(s2);
+  const double asqrtbz = a + s2 * isbz;
+  const double D = g.R2 + asqrtbz * asqrtbz;
+  const double iD = rsqrt(D);
+  const double Dm32 = iD * iD * iD;
+  F.FR += -amp * g.R * Dm32;
+  F.Fz += (a == 0.0) ? -amp * g.z * Dm32 : -amp * g.z * asqrtbz * isbz * Dm32;
+}
+
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_NFW>(const GhPotTable* __restrict__ T, int i,
+                                                      const GhGeom& g, GhCyl& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0], a = C->p[1];
+  const double rinv2 = g.rinv * g.rinv;
+  const double br = rinv2 / (a + g.r) - log1p(g.r * C->c[0]) * (rinv2 * g.rinv);
+  F.FR += amp * g.R * br;
+  F.Fz += amp * g.z * br;
+}
+
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_HERNQUIST>(const GhPotTable* __restrict__ T, int i,
+                                                            const GhGeom& g, GhCyl& F) {
+  // F = -amp x / (2 a^2 r) (1 + r/a)^-2 = -(amp/2) x / (r (a + r)^2)
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double apr = C->p[1] + g.r;
+  const double f = -C->c[0] * g.rinv / (apr * apr);
+  F.FR += f * g.R;
+  F.Fz += f * g.z;
+}
+
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_LOGHALO>(const GhPotTable* __restrict__ T, int i,
+                                                          const GhGeom& g, GhCyl& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0], core2 = C->p[2], onem1overb2 = C->p[3];
+  const double zq2 = g.z2 * C->c[0];  // (z/q)^2
+  if (onem1overb2 < 1.0) {
+    const double Rt2 = g.R2 * (1.0 - onem1overb2 * g.sinphi * g.sinphi);
+    const double inv = 1.0 / (Rt2 + zq2 + core2);
+    F.FR += -amp * Rt2 / g.R * inv;
+    F.Fz += -amp * g.z * C->c[0] * inv;
+    F.Fphi += amp * g.R2 * inv * (2.0 * g.sinphi * g.cosphi) * onem1overb2 * 0.5;
+  } else {
+    const double inv = 1.0 / (g.R2 + zq2 + core2);
+    F.FR += -amp * g.R * inv;
+    F.Fz += -amp * g.z * C->c[0] * inv;
+  }
+}
+
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_DEHNENBAR>(const GhPotTable* __restrict__ T, int i,
+                                                            const GhGeom& g, GhCyl& F) {
+  // args amp*Af, tform, tsteady, rb, omegab, barphi
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0], rb = C->p[3];
+  const double smooth = gh_bar_smooth(g.t, C->p[1], C->p[2]);
+  const double cos2p = g.cosphi * g.cosphi - g.sinphi * g.sinphi;
+  const double sin2p = 2.0 * g.sinphi * g.cosphi;
+  double cang, sang;
+  gh_bar_angle(cos2p, sin2p, C->p[4], C->p[5], g.t, cang, sang);
+  const double r4inv = g.rinv * g.rinv * g.rinv * g.rinv;
+  const double r2inv = g.rinv * g.rinv;
+  const double as = amp * smooth;
+  if (g.r <= rb) {
+    const double u = g.r * C->c[0];  // r/rb
+    const double u3 = u * u * u;
+    F.FR += -as * cang * (u3 * g.R * (3.0 * g.R2 + 2.0 * g.z2) - 4.0 * g.R * g.z2) * r4inv;
+    F.Fphi += 2.0 * as * sang * (u3 - 2.0) * g.R2 * r2inv;
+    F.Fz += -as * cang * (u3 + 4.0) * g.R2 * g.z * r4inv;
+  } else {
+    const double u = rb * g.rinv;
+    const double u3 = u * u * u;
+    F.FR += -as * cang * u3 * g.R * r4inv * (3.0 * g.R2 - 2.0 * g.z2);
+    F.Fphi += -2.0 * as * sang * u3 * g.R2 * r2inv;
+    F.Fz += -5.0 * as * cang * u3 * g.R2 * g.z * r4inv;
+  }
+}
+
+// run-time dispatch on the component type (generic kernels)
+__device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__ T, int i,
+                                                  const GhGeom& g, GhCyl& F) {
+  switch (T->comp[i].type) {
+    case GH_POWERSPHCUTOFF: gh_comp_force<GH_POWERSPHCUTOFF>(T, i, g, F); break;
+    case GH_MIYAMOTONAGAI: gh_comp_force<GH_MIYAMOTONAGAI>(T, i, g, F); break;
+    case GH_NFW: gh_comp_force<GH_NFW>(T, i, g, F); break;
+    case GH_HERNQUIST: gh_comp_force<GH_HERNQUIST>(T, i, g, F); break;
+    case GH_LOGHALO: gh_comp_force<GH_LOGHALO>(T, i, g, F); break;
+    case GH_DEHNENBAR: gh_comp_force<GH_DEHNENBAR>(T, i, g, F); break;
+    default: break;
+  }
+}
+
+// A potential composition known at compile time: GhSpec<15, 5, 9> is
+// MWPotential2014; GhSpec<> is any composition, dispatched at run time.
+// Specialised kernels inline only the components they use, which keeps the
+// many inlined force evaluations of DOP853 inside the instruction cache.
+template <int... Codes>
+struct GhSpec {};
+
+template <class Spec>
+struct GhCylSum;
+
+template <>
+struct GhCylSum<GhSpec<>> {
+  __device__ static __forceinline__ void add(const GhPotTable* __restrict__ T, const GhGeom& g,
+                                             GhCyl& F) {
+    const int npot = T->npot;
+    for (int i = 0; i < npot; i++) gh_comp_force_dyn(T, i, g, F);
+  }
+};
+
+template <int... Codes>
+struct GhCylSum<GhSpec<Codes...>> {
+  __device__ static __forceinline__ void add(const GhPotTable* __restrict__ T, const GhGeom& g,
+                                             GhCyl& F) {
+    int i = 0;
+    (gh_comp_force<Codes>(T, i++, g, F), ...);  // component order = summation order
+  }
+};
+
 // Sum of cylindrical forces over components in index order
 // (galpy_potentials.c:120-179).  cosphi/sinphi = x/R, y/R.
+template <class Spec = GhSpec<>>
 __device__ __forceinline__ GhCyl gh_cyl_forces(const GhPotTable* __restrict__ T, double R,
                                                double R2, double z, double cosphi,
                                                double sinphi, double t) {
+  GhGeom g;
+  g.R = R;
+  g.R2 = R2;
+  g.z = z;
+  g.z2 = z * z;
+  g.r2 = R2 + g.z2;
+  g.rinv = rsqrt(g.r2);  // shared by the spherical components
+  g.r = g.r2 * g.rinv;
+  g.cosphi = cosphi;
+  g.sinphi = sinphi;
+  g.t = t;
   GhCyl F = {0.0, 0.0, 0.0};
-  const double z2 = z * z;
-  const double r2 = R2 + z2;
-  const double rinv = rsqrt(r2);  // shared by the spherical components
-  const double r = r2 * rinv;
-  const int npot = T->npot;
-  for (int i = 0; i < npot; i++) {
-    const GhComponent* __restrict__ C = &T->comp[i];
-    switch (C->type) {
-      case GH_POWERSPHCUTOFF: {
-        // PowerSphericalPotentialwCutoff.c:9-11,25-81:
-        //   F = -amp M(r2) / r^3 (R, z), M = 2 pi rc^(3-alpha) gamma(a, r2/rc^2)
-        const double x = r2 * C->c[3];
-        const double g =
-            gh_lower_incgamma_tab(C, T->ser[i][0], &T->tab[(int)C->c[11]][0][0], x);
-        const double f = -C->c[2] * g * (rinv * rinv * rinv);
-        F.FR += f * R;
-        F.Fz += f * z;
-        break;
-      }
-      case GH_MIYAMOTONAGAI: {
-        // MiyamotoNagaiPotential.c:16-53
-        const double amp = C->p[0], a = C->p[1];
-        const double s2 = z2 + C->c[0];
-        const double isbz = rsqrt(s2);
-        const double asqrtbz = a + s2 * isbz;
-        const double D = R2 + asqrtbz * asqrtbz;
-        const double iD = rsqrt(D);
-        const double Dm32 = iD * iD * iD;
-        F.FR += -amp * R * Dm32;
-        F.Fz += (a == 0.0) ? -amp * z * Dm32 : -amp * z * asqrtbz * isbz * Dm32;
-        break;
-      }
-      case GH_NFW: {
-        // NFWPotential.c:16-49
-        const double amp = C->p[0], a = C->p[1];
-        const double rinv2 = rinv * rinv;
-        const double br = rinv2 / (a + r) - log1p(r * C->c[0]) * (rinv2 * rinv);
-        F.FR += amp * R * br;
-        F.Fz += amp * z * br;
-        break;
-      }
-      case GH_HERNQUIST: {
-        // HernquistPotential.c:15-46: F = -amp x / (2 a^2 r) (1 + r/a)^-2
-        const double a = C->p[1];
-        const double apr = a + r;
-        const double f = -C->c[0] * rinv / (apr * apr);
-        F.FR += f * R;
-        F.Fz += f * z;
-        break;
-      }
-      case GH_LOGHALO: {
-        // LogarithmicHaloPotential.c:21-91
-        const double amp = C->p[0], core2 = C->p[2], onem1overb2 = C->p[3];
-        const double zq2 = z2 * C->c[0];  // (z/q)^2
-        if (onem1overb2 < 1.0) {
-          const double Rt2 = R2 * (1.0 - onem1overb2 * sinphi * sinphi);
-          const double inv = 1.0 / (Rt2 + zq2 + core2);
-          F.FR += -amp * Rt2 / R * inv;
-          F.Fz += -amp * z * C->c[0] * inv;
-          F.Fphi += amp * R2 * inv * (2.0 * sinphi * cosphi) * onem1overb2 * 0.5;
-        } else {
-          const double inv = 1.0 / (R2 + zq2 + core2);
-          F.FR += -amp * R * inv;
-          F.Fz += -amp * z * C->c[0] * inv;
-        }
-        break;
-      }
-      case GH_DEHNENBAR: {
-        // DehnenBarPotential.c:18-124; args amp*Af, tform, tsteady, rb, omegab, barphi
-        const double amp = C->p[0], rb = C->p[3];
-        const double smooth = gh_bar_smooth(t, C->p[1], C->p[2]);
-        const double cos2p = cosphi * cosphi - sinphi * sinphi;
-        const double sin2p = 2.0 * sinphi * cosphi;
-        double cang, sang;
-        gh_bar_angle(cos2p, sin2p, C->p[4], C->p[5], t, cang, sang);
-        const double r4 = r2 * r2;
-        const double as = amp * smooth;
-        if (r <= rb) {
-          const double u = r * C->c[0];  // r/rb
-          const double u3 = u * u * u;
-          F.FR += -as * cang * (u3 * R * (3.0 * R2 + 2.0 * z2) - 4.0 * R * z2) / r4;
-          F.Fphi += 2.0 * as * sang * (u3 - 2.0) * R2 / r2;
-          F.Fz += -as * cang * (u3 + 4.0) * R2 * z / r4;
-        } else {
-          const double u = rb / r;
-          const double u3 = u * u * u;
-          F.FR += -as * cang * u3 * R / r4 * (3.0 * R2 - 2.0 * z2);
-          F.Fphi += -2.0 * as * sang * u3 * R2 / r2;
-          F.Fz += -5.0 * as * cang * u3 * R2 * z / r4;
-        }
-        break;
-      }
-      default:
-        break;
-    }
-  }
+  GhCylSum<Spec>::add(T, g, F);
   return F;
 }
 
 // evalRectForce (integrateFullOrbit.c:1307-1325): rectangular acceleration.
+template <class Spec = GhSpec<>>
 __device__ __forceinline__ void gh_rect_force(const GhPotTable* __restrict__ T, double t,
                                               double x, double y, double z, double& ax,
                                               double& ay, double& az) {
@@ -286,11 +370,35 @@ __device__ __forceinline__ void gh_rect_force(const GhPotTable* __restrict__ T, 
   const double R = R2 * invR;
   const double cosphi = x * invR;
   const double sinphi = y * invR;
-  const GhCyl F = gh_cyl_forces(T, R, R2, z, cosphi, sinphi, t);
+  const GhCyl F = gh_cyl_forces<Spec>(T, R, R2, z, cosphi, sinphi, t);
   ax = cosphi * F.FR - invR * sinphi * F.Fphi;
   ay = sinphi * F.FR + invR * cosphi * F.Fphi;
   az = F.Fz;
 }
+
+// Force functor handed to the integrators: the table plus the compile-time
+// composition.
+#ifdef GH_FORCE_NOINLINE
+template <class Spec>
+__device__ __noinline__ void gh_rect_force_call(const GhPotTable* __restrict__ T, double t,
+                                                double x, double y, double z, double& ax,
+                                                double& ay, double& az) {
+  gh_rect_force<Spec>(T, t, x, y, z, ax, ay, az);
+}
+#endif
+
+template <class Spec>
+struct GhForce {
+  const GhPotTable* __restrict__ T;
+  __device__ __forceinline__ void rect(double t, double x, double y, double z, double& ax,
+                                       double& ay, double& az) const {
+#ifdef GH_FORCE_NOINLINE
+    gh_rect_force_call<Spec>(T, t, x, y, z, ax, ay, az);
+#else
+    gh_rect_force<Spec>(T, t, x, y, z, ax, ay, az);
+#endif
+  }
+};
 
 // Potential value (per-component Eval functions; DehnenBar from
 // DehnenBarPotential.py:173-200, which has no C Eval).

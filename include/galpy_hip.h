@@ -46,20 +46,24 @@ void integrateFullOrbit(int nobj, double *yo, int nt, double *t, int indiv_t, in
 /* Device-resident twin of integrateFullOrbit for batching/serving callers
  * (no reference counterpart; it is integrateFullOrbit.c:1046 with the
  * buffers already in HBM).  d_yo, d_t, d_result, d_err and d_stats are device
- * pointers; d_table comes from gh_potential_upload.  d_stats (nullable) gets
+ * pointers; pot is a handle from gh_potential_upload.  d_stats (nullable) gets
  * 3 doubles per orbit: orbit-steps (accepted steps for dop853), rejected
  * steps, step size used (symplectic).  stream is a hipStream_t (NULL = the
  * default stream).  The call is asynchronous with respect to the host.
  * Returns 0 on success, nonzero on a launch/argument error. */
 int integrateFullOrbit_hip(int nobj, double *d_yo, int nt, const double *d_t, int indiv_t,
-                           const void *d_table, double dt, double rtol, double atol,
+                           const void *pot, double dt, double rtol, double atol,
                            double *d_result, int *d_err, int odeint_type, double *d_stats,
                            void *stream);
 
 /* Decodes galpy's packed potential (integrateFullOrbit.c:69-1045 semantics)
- * into a device-resident table; *d_table receives a device pointer. */
-int gh_potential_upload(int npot, const int *pot_type, const double *pot_args, void **d_table);
-int gh_potential_free(void *d_table);
+ * into a device-resident table; *pot receives an opaque host handle (device
+ * table + component codes, used to select a kernel specialised for the
+ * composition).  gh_potential_free releases both. */
+int gh_potential_upload(int npot, const int *pot_type, const double *pot_args, void **pot);
+int gh_potential_free(void *pot);
+/* Device address of the table behind a handle. */
+const void *gh_potential_device_ptr(const void *pot);
 
 /* Energy (omega = 0) or Jacobi integral E - omega*Lz along stored
  * trajectories (Orbits.py:2902 E, :3298 Jacobi): d_rows[nobj*nt*6]
@@ -67,13 +71,19 @@ int gh_potential_free(void *d_table);
  * d_e0[i] = integral at row 0 and d_demax[i] = max_j |E_j - E_0| / |E_0|.
  * Asynchronous on stream. */
 int gh_energy_drift_hip(int nobj, int nt, const double *d_rows, const double *d_t, int indiv_t,
-                        const void *d_table, double omega, double *d_e0, double *d_demax,
+                        const void *pot, double omega, double *d_e0, double *d_demax,
                         void *stream);
 
 /* Potential value and cylindrical forces at n points (device pointers). */
 int gh_eval_hip(int n, const double *d_R, const double *d_z, const double *d_phi,
-                const double *d_t, const void *d_table, double *d_pot, double *d_FR,
+                const double *d_t, const void *pot, double *d_pot, double *d_FR,
                 double *d_Fz, double *d_Fphi, void *stream);
+
+/* Host evaluation of the force-kernel incomplete gamma of component `comp`
+ * (a PowerSphericalPotentialwCutoff) from the decoded table, exactly as the
+ * device computes it; for checks of the table without a GPU. */
+int gh_host_bulge_lower_gamma(int npot, const int *pot_type, const double *pot_args, int comp,
+                              int n, const double *x, double *out);
 
 /* Message of the last error on the calling thread ("" if none). */
 const char *galpy_hip_last_error(void);
