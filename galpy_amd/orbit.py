@@ -1,0 +1,226 @@
+"""``Orbit``: the user-facing container, mirroring galpy's ``Orbit`` for the
+HIP integration methods.
+
+Mirrors galpy/orbit/Orbits.py: construction from ``[R,vR,vT,z,vz,phi]``
+rows (one or many orbits, :293), ``integrate(t, pot, method, dt, rtol, atol)``
+(:1987-2061) dispatching to the C-ABI through ``integrateFullOrbit_hip``
+(the ``integrateFullOrbit_c`` call at :1806), and the phase-space and
+integral accessors used by galpy's tests: ``R, vR, vT, z, vz, phi, x, y,
+vx, vy`` (:2868), ``getOrbit``, ``E`` (:2902), ``Jacobi`` (:3298), ``L``,
+``Lz``, ``ER``, ``Ez``, ``rperi``, ``rap``, ``zmax``, ``e``.
+
+Only the HIP methods ``leapfrog_hip``, ``symplec4_hip`` and ``dop853_hip``
+integrate; there is no silent fallback to another backend.  As in galpy,
+``numpy.copy`` of the initial conditions is passed (Orbits.py:1793) because
+the C-ABI overwrites its ``yo`` argument.
+"""
+
+import numpy
+
+from .integrate import METHOD_CODES, integrateFullOrbit_hip
+from .potential import evaluatePotentials, flatten
+
+_METHODS = tuple(METHOD_CODES)
+
+
+def _check_integrate_dt(t, dt):
+    """Orbits.py:9456-9462: dt must divide the output spacing (rounding
+    check, tolerance 1e-10); the C code then truncates init_dt/dt."""
+    if dt is None:
+        return True
+    dt = numpy.fabs(dt)
+    time_step = numpy.fabs(t[1] - t[0])
+    return numpy.fabs(round(time_step / dt) * dt - time_step) <= 10.0**-10.0
+
+
+class Orbit:
+    """One or many orbits in a potential; rows are (R,vR,vT,z,vz,phi)."""
+
+    def __init__(self, vxvv):
+        vxvv = numpy.array(vxvv, dtype=float)
+        self._single = vxvv.ndim == 1
+        vxvv = numpy.atleast_2d(vxvv)
+        if vxvv.shape[-1] != 6:
+            raise ValueError("this Orbit supports full 3D orbits: rows of (R,vR,vT,z,vz,phi)")
+        self.vxvv = vxvv
+        self.t = None
+        self.orbit = None
+        self._pot = None
+        self._err = None
+
+    # --- shape ---------------------------------------------------------------
+    @property
+    def shape(self):
+        return () if self._single else (len(self.vxvv),)
+
+    def __len__(self):
+        return 1 if self._single else len(self.vxvv)
+
+    @property
+    def size(self):
+        return len(self.vxvv)
+
+    # --- integration --------------------------------------------------------
+    def integrate(self, t, pot, method="symplec4_hip", progressbar=False, dt=None, rtol=None,
+                  atol=None):
+        """Integrate every orbit over the times ``t`` (1D, shared by all
+        orbits, or [N, nt] per orbit), galpy's Orbits.py:1624-1956 path for a
+        C method: checks, one integration call, and storage of the rows."""
+        if method.lower() not in _METHODS:
+            raise ValueError(
+                f"{method} is not a method of this engine; use one of {', '.join(_METHODS)}"
+            )
+        t = numpy.array(t, dtype=float)
+        tt = t if t.ndim == 1 else t[0]
+        if len(tt) < 2:
+            raise ValueError("integrate needs at least two output times")
+        steps = numpy.diff(tt)
+        if not numpy.allclose(steps, steps[0], rtol=1e-10, atol=1e-14):
+            raise ValueError("Input time array must be equally spaced")
+        if not _check_integrate_dt(tt, dt):
+            raise ValueError("dt input (integrator stepsize) for Orbit.integrate must be an integer divisor of the output stepsize")
+        flatten(pot)  # validates the potential
+        out, err = integrateFullOrbit_hip(pot, numpy.copy(self.vxvv), t, method.lower(),
+                                          rtol=rtol, atol=atol, progressbar=progressbar, dt=dt)
+        self.orbit = out
+        self._err = err
+        self.t = t
+        self._pot = pot
+        return None
+
+    def _check_integrated(self):
+        if self.orbit is None:
+            raise AttributeError("Integrate orbit first")
+
+    def _rows(self, t=None):
+        """Rows at times t (None: all output times); t must lie on the grid."""
+        self._check_integrated()
+        if t is None:
+            return self.orbit
+        grid = self.t if self.t.ndim == 1 else self.t[0]
+        ts = numpy.atleast_1d(numpy.asarray(t, dtype=float))
+        idx = numpy.searchsorted(grid, ts) if grid[-1] > grid[0] else numpy.searchsorted(-grid, -ts)
+        idx = numpy.clip(idx, 0, len(grid) - 1)
+        for k, tv in enumerate(ts):
+            for cand in (idx[k] - 1, idx[k], idx[k] + 1):
+                if 0 <= cand < len(grid) and abs(grid[cand] - tv) <= 1e-10 * max(1.0, abs(tv)):
+                    idx[k] = cand
+                    break
+            else:
+                raise ValueError(f"time {tv} is not one of the output times of the integration")
+        rows = self.orbit[:, idx]
+        return rows[:, 0] if numpy.ndim(t) == 0 else rows
+
+    def _out(self, a):
+        return a[0] if self._single else a
+
+    def getOrbit(self):
+        self._check_integrated()
+        return self._out(self.orbit.copy())
+
+    def time(self):
+        self._check_integrated()
+        return self.t
+
+    # --- coordinates --------------------------------------------------------
+    def _col(self, k, t):
+        if self.orbit is None and t is None:
+            return self._out(self.vxvv[:, k])
+        return self._out(self._rows(t)[..., k])
+
+    def R(self, t=None):
+        return self._col(0, t)
+
+    def vR(self, t=None):
+        return self._col(1, t)
+
+    def vT(self, t=None):
+        return self._col(2, t)
+
+    def z(self, t=None):
+        return self._col(3, t)
+
+    def vz(self, t=None):
+        return self._col(4, t)
+
+    def phi(self, t=None):
+        return self._col(5, t)
+
+    def x(self, t=None):
+        return self.R(t) * numpy.cos(self.phi(t))
+
+    def y(self, t=None):
+        return self.R(t) * numpy.sin(self.phi(t))
+
+    def vx(self, t=None):
+        phi = self.phi(t)
+        return self.vR(t) * numpy.cos(phi) - self.vT(t) * numpy.sin(phi)
+
+    def vy(self, t=None):
+        phi = self.phi(t)
+        return self.vR(t) * numpy.sin(phi) + self.vT(t) * numpy.cos(phi)
+
+    # --- integrals ------------------------------------------------------------
+    def _times_like(self, t, rows):
+        if t is not None:
+            return numpy.broadcast_to(numpy.asarray(t, dtype=float), rows.shape[:-1])
+        tt = self.t if self.t.ndim > 1 else self.t[None, :]
+        return numpy.broadcast_to(tt, rows.shape[:-1])
+
+    def E(self, t=None, pot=None):
+        """Energy (Orbits.py:2902): potential + kinetic, per orbit and time."""
+        pot = self._pot if pot is None else pot
+        rows = self._rows(t) if self.orbit is not None else self.vxvv
+        tt = self._times_like(t, rows) if self.orbit is not None else 0.0
+        R, vR, vT, z, vz, phi = numpy.moveaxis(rows, -1, 0)
+        return self._out(evaluatePotentials(pot, R, z, phi, tt) + 0.5 * (vR**2 + vT**2 + vz**2))
+
+    def Jacobi(self, t=None, OmegaP=None, pot=None):
+        """Jacobi integral E - OmegaP Lz (Orbits.py:3298); OmegaP defaults
+        to the pattern speed of a bar in the potential."""
+        pot = self._pot if pot is None else pot
+        if OmegaP is None:
+            OmegaP = next((p.OmegaP() for p in flatten(pot) if hasattr(p, "OmegaP")), None)
+            if OmegaP is None:
+                raise ValueError("OmegaP must be given for a potential without a pattern speed")
+        return self.E(t=t, pot=pot) - OmegaP * self.Lz(t)
+
+    def Lz(self, t=None):
+        return self.R(t) * self.vT(t)
+
+    def L(self, t=None):
+        """Angular momentum vector (x,y,z components) from the rectangular state."""
+        x, y, z = self.x(t), self.y(t), self.z(t)
+        vx, vy, vz = self.vx(t), self.vy(t), self.vz(t)
+        return numpy.stack([y * vz - z * vy, z * vx - x * vz, x * vy - y * vx], axis=-1)
+
+    def ER(self, t=None, pot=None):
+        pot = self._pot if pot is None else pot
+        R, vR, vT = self.R(t), self.vR(t), self.vT(t)
+        return evaluatePotentials(pot, R, 0.0) + 0.5 * (vR**2 + vT**2)
+
+    def Ez(self, t=None, pot=None):
+        pot = self._pot if pot is None else pot
+        R, z, vz = self.R(t), self.z(t), self.vz(t)
+        return evaluatePotentials(pot, R, z) - evaluatePotentials(pot, R, 0.0) + 0.5 * vz**2
+
+    def _r(self):
+        o = self._rows()
+        return numpy.sqrt(o[..., 0] ** 2 + o[..., 3] ** 2)
+
+    def rperi(self):
+        return self._out(self._r().min(axis=-1))
+
+    def rap(self):
+        return self._out(self._r().max(axis=-1))
+
+    def zmax(self):
+        return self._out(numpy.fabs(self._rows()[..., 3]).max(axis=-1))
+
+    def e(self):
+        r = self._r()
+        rap, rperi = r.max(axis=-1), r.min(axis=-1)
+        return self._out((rap - rperi) / (rap + rperi))
+
+
+Orbits = Orbit
