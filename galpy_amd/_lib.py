@@ -55,6 +55,7 @@ SIGNATURES = {
         [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
     ),
     "gh_potential_free": (ctypes.c_int, [ctypes.c_void_p]),
+    "gh_potential_device_ptr": (ctypes.c_void_p, [ctypes.c_void_p]),
     "gh_energy_drift_hip": (
         ctypes.c_int,
         [

@@ -16,7 +16,7 @@
 #include <initializer_list>
 
 #include "../../include/galpy_hip.h"
-#include "dop853_dev.h"
+#include "batch_dev.h"
 
 #define GH_EXPORT extern "C" __attribute__((visibility("default")))
 
@@ -776,6 +776,91 @@ GH_EXPORT int gh_eval_hip(int n, const double* d_R, const double* d_z, const dou
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
   hipLaunchKernelGGL(gh_eval_kernel, grid, block, 0, (hipStream_t)stream, n, d_R, d_z, d_phi, d_t,
                      ((const GhPotHandle*)pot)->dev, d_pot, d_FR, d_Fz, d_Fphi);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched integration
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void gh_batch_reset_kernel(long long nstate, GhLaneState* __restrict__ state, int ntiles,
+                                      int* __restrict__ tile_done, int* __restrict__ remaining) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long k = i; k < nstate; k += stride) state[k].status = 0;
+  for (long long k = i; k < ntiles; k += stride) tile_done[k] = 0;
+  if (i == 0) *remaining = ntiles;
+}
+
+template <int OPL, bool DOP>
+int batch_resident_blocks(int dev) {
+  static int cached[64] = {0};
+  if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_batch_kernel<OPL, DOP>, 64, 0) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  const int n = std::max(1, per_cu) * std::max(1, ncu);
+  if (dev >= 0 && dev < 64) cached[dev] = n;
+  return n;
+}
+
+}  // namespace
+
+GH_EXPORT size_t gh_batch_state_bytes(void) { return sizeof(GhLaneState); }
+
+GH_EXPORT int gh_batch_reset(int ntiles, long long nstate, void* d_state, int* d_tile_done,
+                             int* d_remaining, void* stream) {
+  g_errbuf[0] = 0;
+  irq_device_ptr();  // set up outside any graph capture of the launches
+  const long long work = std::max<long long>(nstate, ntiles);
+  const unsigned nb = (unsigned)std::max<long long>(1, std::min<long long>((work + 255) / 256, 65535));
+  hipLaunchKernelGGL(gh_batch_reset_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, nstate,
+                     (GhLaneState*)d_state, ntiles, d_tile_done, d_remaining);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+GH_EXPORT int gh_batch_launch(const gh_batch_group* d_groups, const gh_batch_tile* d_tiles,
+                              int ntiles, void* d_state, int* d_tile_done,
+                              unsigned long long* d_counter, int* d_remaining, long long budget,
+                              int opl, int with_dop, void* stream) {
+  g_errbuf[0] = 0;
+  if (ntiles <= 0) return 0;
+  if (opl != 1 && opl != 2) {
+    set_error("orbits per lane must be 1 or 2 (got %d)", opl);
+    return -1;
+  }
+  int dev = 0;
+  GH_CHECK(hipGetDevice(&dev));
+  const int key = (opl - 1) * 2 + (with_dop ? 1 : 0);
+  const int resident = key == 0   ? batch_resident_blocks<1, false>(dev)
+                       : key == 1 ? batch_resident_blocks<1, true>(dev)
+                       : key == 2 ? batch_resident_blocks<2, false>(dev)
+                                  : batch_resident_blocks<2, true>(dev);
+  if (resident <= 0) {
+    set_error("occupancy query failed for the batch kernel");
+    return -1;
+  }
+  const unsigned nblocks = (unsigned)std::min(ntiles, resident);
+  const long long b = budget > 0 ? budget : (1ll << 62);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, st, d_counter, 0ull);
+  const int* cancel = irq_device_ptr();
+  GhLaneState* state = (GhLaneState*)d_state;
+#define GH_BATCH(O, D)                                                                          \
+  hipLaunchKernelGGL((gh_batch_kernel<O, D>), dim3(nblocks), dim3(64), 0, st, d_groups, d_tiles, \
+                     ntiles, state, d_tile_done, d_counter, d_remaining, b, cancel)
+  switch (key) {
+    case 0: GH_BATCH(1, false); break;
+    case 1: GH_BATCH(1, true); break;
+    case 2: GH_BATCH(2, false); break;
+    default: GH_BATCH(2, true); break;
+  }
+#undef GH_BATCH
   GH_CHECK(hipGetLastError());
   return 0;
 }

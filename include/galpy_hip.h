@@ -85,6 +85,55 @@ int gh_eval_hip(int n, const double *d_R, const double *d_z, const double *d_phi
 int gh_host_bulge_lower_gamma(int npot, const int *pot_type, const double *pot_args, int comp,
                               int n, const double *x, double *out);
 
+/* ------------------------------------------------------------------------
+ * Batched integration (no reference counterpart: galpy integrates one group
+ * of orbits per call).  Many groups -- each with its own potential, output
+ * times, step and method -- run in one persistent launch.  All pointers in
+ * the descriptors are device pointers.
+ * ------------------------------------------------------------------------ */
+#include <stddef.h>
+
+typedef struct gh_batch_group {
+  const void *table;   /* gh_potential_device_ptr(handle) */
+  const double *t;     /* [nt] output times */
+  double *yo;          /* [n][6] initial conditions (R,vR,vT,z,vz,phi); overwritten as galpy does */
+  double *result;      /* [n][nt][6] output rows */
+  int *err;            /* [n] */
+  double *stats;       /* [n][3] orbit-steps, rejected steps, step used; may be NULL */
+  double dt;           /* fixed step, or -9999.99 to estimate per orbit (symplectic) */
+  double rtol, atol;   /* natural logarithms */
+  int n, nt;
+  int method;          /* 0 leapfrog, 3 symplec4, 6 dop853 */
+  int pad;
+} gh_batch_group;
+
+typedef struct gh_batch_tile {
+  int group;           /* index into the group array */
+  int first;           /* first orbit of the group in this tile */
+  int count;           /* orbits in the tile, <= 64 * orbits-per-lane */
+  int pad;
+  long long state;     /* first per-orbit state record of the tile */
+} gh_batch_tile;
+
+/* Bytes of resumable state per orbit. */
+size_t gh_batch_state_bytes(void);
+
+/* Prepares a batch for its first launch: zeroes nstate state records and
+ * ntiles tile flags and sets *d_remaining = ntiles (kernels only, so the
+ * sequence may be captured in a graph). */
+int gh_batch_reset(int ntiles, long long nstate, void *d_state, int *d_tile_done,
+                   int *d_remaining, void *stream);
+
+/* One persistent launch: wavefronts claim unfinished tiles from *d_counter
+ * (reset to 0 here by a kernel) and advance each orbit by at most `budget`
+ * steps (<= 0: to completion).  opl = orbits per lane (1 or 2; tiles must
+ * hold at most 64*opl orbits).  with_dop must be nonzero if any group uses
+ * dop853 (selects the kernel instance that carries it).  *d_remaining counts
+ * unfinished tiles.  Asynchronous on stream. */
+int gh_batch_launch(const gh_batch_group *d_groups, const gh_batch_tile *d_tiles, int ntiles,
+                    void *d_state, int *d_tile_done, unsigned long long *d_counter,
+                    int *d_remaining, long long budget, int opl, int with_dop, void *stream);
+
 /* Message of the last error on the calling thread ("" if none). */
 const char *galpy_hip_last_error(void);
 
