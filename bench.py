@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1024, help="orbits in the CPU-baseline sample")
     ap.add_argument("--count-sample", type=int, default=64, help="orbits in the op-count sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ngroups", type=int, default=4096, help="groups of the batching workload")
     return ap.parse_args()
 
 
@@ -76,12 +77,88 @@ def cpu_baseline_leg(w, sample, count_sample):
     }
 
 
+def bench_batching(args):
+    """4,096 groups (log-uniform sizes in [1, 1e4], MWPotential2014 /
+    Hernquist / LogarithmicHalo, lengths U(10,1000), 2-1001 outputs,
+    symplec4 at dt=0.01) in one persistent launch, against the same groups
+    launched one integrateFullOrbit_hip call per group on a sample."""
+    import torch
+
+    from galpy_amd.batching import Batch, autotune, make_groups
+    from galpy_amd.integrate import integrate_device
+
+    groups = make_groups(ngroups=args.ngroups, seed=0)
+    opl, timings = autotune(groups)
+    # plain path on a sample of groups: one launch per group, in sequence
+    rng = numpy.random.default_rng(1)
+    sample = [groups[i] for i in sorted(rng.choice(len(groups), size=min(128, len(groups)), replace=False))]
+    b_s = Batch(sample, opl=opl)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b_s.run()
+    torch.cuda.synchronize()
+    t_batch_sample = time.perf_counter() - t0
+    steps_sample = b_s.orbit_steps()
+    del b_s
+    stats_plain = 0.0
+    t0 = time.perf_counter()
+    for g in sample:
+        from galpy_amd.integrate import PotentialTable
+
+        tab = PotentialTable(g["pot"])
+        yo = torch.tensor(g["yo"], device="cuda")
+        tt = torch.tensor(g["t"], device="cuda")
+        st = torch.zeros((len(g["yo"]), 3), dtype=torch.float64, device="cuda")
+        integrate_device(tab, yo, tt, g["method"], dt=g["dt"], stats=st)
+        torch.cuda.synchronize()
+        stats_plain += st[:, 0].sum().item()
+    t_plain = time.perf_counter() - t0
+    # full batch
+    b = Batch(groups, opl=opl)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    steps = b.orbit_steps()
+    nerr = sum(int((e != 0).sum().item()) for e in b.errs)
+    out = {
+        "metric": "FP64 orbit-steps/s on 1 MI355X, batched groups",
+        "value": steps / el,
+        "unit": "orbit-steps/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": el * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batching_{len(groups)}_groups", "orbits": b.norbits,
+                   "tiles": b.ntiles, "distinct_potential_tables": len(b.pots), "method": "symplec4_hip",
+                   "dt": 0.01, "opl": opl, "autotune_orbit_steps_per_s": timings},
+        "orbit_steps_per_pass": steps,
+        "orbits_with_error": nerr,
+        "plain_one_launch_per_group": {
+            "sample_groups": len(sample),
+            "orbit_steps_per_s": stats_plain / t_plain,
+            "batched_same_sample_orbit_steps_per_s": steps_sample / t_batch_sample,
+            "batched_full_over_plain": (steps / el) / (stats_plain / t_plain),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload.startswith("batching"):
+        bench_batching(args)
+        return
 
     import torch
 

@@ -233,10 +233,11 @@ def _signature(groups):
     return f"{'+'.join(methods)}|{codes}"
 
 
-def autotune(groups, candidates=(1, 2), sample_groups=64, budget=2000, device="cuda", use_cache=True):
-    """Pick orbits-per-lane for a batch: time a bounded launch of a sample
-    of the groups for each candidate (orbit-steps/s), cache the winner per
-    device name and workload signature, and return (opl, timings)."""
+def autotune(groups, candidates=(1, 2), sample_groups=32, budget=200, device="cuda", use_cache=True):
+    """Pick orbits-per-lane for a batch: run a sample of the groups to
+    completion with each candidate (after a bounded warm-up launch), compare
+    orbit-steps/s, cache the winner per device name and workload signature,
+    and return (opl, timings)."""
     import torch
 
     dev = torch.device(device)
@@ -257,12 +258,10 @@ def autotune(groups, candidates=(1, 2), sample_groups=64, budget=2000, device="c
     timings = {}
     for opl in candidates:
         b = Batch(sample, opl=opl, device=device, potentials=pots)
-        b.reset()
-        b.launch(budget // 4)  # warm-up (first-orbit set-up, code load)
+        b.run(budget=budget)  # warm-up: code load, first-launch set-up
         torch.cuda.synchronize(dev)
-        b.reset()
         t0 = time.perf_counter()
-        b.launch(budget)
+        b.run()  # the sample to completion; per-orbit step counts land at the end
         torch.cuda.synchronize(dev)
         el = time.perf_counter() - t0
         steps = b.orbit_steps()

@@ -86,6 +86,27 @@ int gh_host_bulge_lower_gamma(int npot, const int *pot_type, const double *pot_a
                               int n, const double *x, double *out);
 
 /* ------------------------------------------------------------------------
+ * Differentiation (no reference counterpart; galpy offers the forward
+ * tangent map integrate_dxdv, bovy_symplecticode.c:1053-1268).
+ * ------------------------------------------------------------------------ */
+/* N full leapfrog (method 0) or symplec4 (3) steps of size dt from the
+ * rectangular states d_w0[n][6] (x,y,z,vx,vy,vz) to d_wT[n][6] -- galpy's
+ * steppers on t = t0 + k dt.  Device pointers; asynchronous on stream. */
+int gh_flow_hip(int n, const double *d_w0, double *d_wT, double t0, double dt, long long nsteps,
+                const void *pot, int method, void *stream);
+
+/* Reverse-time adjoint of gh_flow_hip for time-independent potentials:
+ * from the final states d_wT and d_gT = dL/dw(T) ([n][6]) it recomputes the
+ * trajectory backwards by reversibility (nothing stored) and writes
+ * d_lam0 = dL/dw(0) ([n][6]), d_w0rec = the recomputed initial states
+ * (nullable) and d_dtheta[n][8][3] = dL/d(packed argument j of component c)
+ * per orbit (nullable; slots: MiyamotoNagai amp,a,b; NFW amp,a; Hernquist
+ * amp,a; PowerSphericalPotentialwCutoff and LogarithmicHalo amp). */
+int gh_adjoint_hip(int n, const double *d_wT, const double *d_gT, double dt, long long nsteps,
+                   const void *pot, int method, double *d_lam0, double *d_w0rec, double *d_dtheta,
+                   void *stream);
+
+/* ------------------------------------------------------------------------
  * Batched integration (no reference counterpart: galpy integrates one group
  * of orbits per call).  Many groups -- each with its own potential, output
  * times, step and method -- run in one persistent launch.  All pointers in

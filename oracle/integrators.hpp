@@ -245,6 +245,87 @@ void symplec4(const ForceCtx<R>& F, const double* yo, int nt, double dt_in, cons
   if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
 }
 
+// Variational (tangent-map) steppers, bovy_symplecticode.c:1053-1268:
+// the same DKD sequence on the augmented state [q | dq_1..dq_nde],
+// [p | dp_1..dp_nde]; the kick of each deviation column is h K dq_j
+// (rect_force_dxdv).  The step is estimated from the base orbit only, so the
+// base trajectory is bit-identical to a plain run.  w = rect state
+// [x,y,z,vx,vy,vz] followed by nde deviation blocks of 6 in the same order;
+// out = the final augmented state.  t = output grid (only its spacing and
+// count matter, as for the plain steppers).  Returns false if a component
+// has no restated Hessian.
+template <class R>
+bool symplectic_dxdv(int npot, const Component* P, int method, const double* w_in, int nde, int nt,
+                     const double* t, double dt_in, double rtol, double atol, double* out) {
+  const int nb = 1 + nde;
+  R q[3 * 7], p[3 * 7], a[3 * 7];
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < 3; k++) {
+      q[3 * b + k] = w_in[6 * b + k];
+      p[3 * b + k] = w_in[6 * b + 3 + k];
+    }
+  long long nrhs = 0;
+  ForceCtx<R> F{npot, P, &nrhs};
+  R init_dt = R(t[1]) - R(t[0]);
+  R dt = dt_in;
+  if (dt_in == kDtSentinel)
+    dt = (method == 0) ? leapfrog_estimate_step(F, q, p, init_dt, R(t[0]), rtol, atol)
+                       : symplec4_estimate_step(F, q, p, init_dt, R(t[0]), rtol, atol);
+  long ndt = (long)(dval(init_dt / dt));
+  R to = t[0];
+  auto drift = [&](R h) {
+    for (int i = 0; i < 3 * nb; i++) q[i] = q[i] + h * p[i];
+  };
+  bool ok = true;
+  auto kick = [&](R tk, R h) {
+    ok &= rect_force_dxdv(npot, P, tk, q, a, nde);
+    for (int i = 0; i < 3 * nb; i++) p[i] = p[i] + h * a[i];
+  };
+  const R c1 = S4C1, c2 = S4C2, c3 = S4C3, c4 = S4C4, d1 = S4D1, d2 = S4D2, d3 = S4D3;
+  for (int ii = 0; ii < nt - 1; ii++) {
+    if (method == 0) {
+      drift(dt / R(2.));
+      for (long jj = 0; jj < ndt - 1; jj++) {
+        kick(to + dt / R(2.), dt);
+        drift(dt);
+        to = to + dt;
+      }
+      kick(to + dt / R(2.), dt);
+      drift(dt / R(2.));
+      to = to + dt;
+    } else {
+      drift(c1 * dt);
+      to += c1 * dt;
+      for (long jj = 0; jj < ndt - 1; jj++) {
+        kick(to, d1 * dt);
+        drift(c2 * dt);
+        to += c2 * dt;
+        kick(to, d2 * dt);
+        drift(c3 * dt);
+        to += c3 * dt;
+        kick(to, d3 * dt);
+        drift((c4 + c1) * dt);
+        to += (c4 + c1) * dt;
+      }
+      kick(to, d1 * dt);
+      drift(c2 * dt);
+      to += c2 * dt;
+      kick(to, d2 * dt);
+      drift(c3 * dt);
+      to += c3 * dt;
+      kick(to, d3 * dt);
+      drift(c4 * dt);
+      to += c4 * dt;
+    }
+  }
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < 3; k++) {
+      out[6 * b + k] = dval(q[3 * b + k]);
+      out[6 * b + 3 + k] = dval(p[3 * b + k]);
+    }
+  return ok;
+}
+
 // ---------------------------------------------------------------------------
 // DOP853 tableau (Hairer, Norsett & Wanner, "Solving ODEs I", dop853.f);
 // the same published constants galpy uses at leung_dop853.c:41-205.

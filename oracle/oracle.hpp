@@ -356,6 +356,121 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
   }
 }
 
+// Second derivatives d2Phi/dR2, d2Phi/dz2, d2Phi/dRdz of the axisymmetric
+// components (the phi derivatives vanish): PowerSphericalPotentialwCutoff.c
+// R2deriv/z2deriv/Rzderiv (Phi'' and Phi' of r), MiyamotoNagaiPotential.c:66-119,
+// NFWPotential.c:63-107, HernquistPotential.c (spherical Phi', Phi''),
+// LogarithmicHaloPotential.c (axisymmetric branch).  Returns false for a
+// component without a restated Hessian (the bar, triaxial haloes).
+template <class R>
+bool cyl_hessian(int npot, const Component* P, R Rc, R z, R& R2, R& z2, R& Rz) {
+  R2 = 0.; z2 = 0.; Rz = 0.;
+  for (int i = 0; i < npot; i++) {
+    const Component& C = P[i];
+    R r2 = Rc * Rc + z * z;
+    R r = sqrt(r2);
+    switch (C.type) {
+      case kPowerSphCutoff: {
+        double amp = C.p[0], alpha = C.p[1], rc = C.p[2];
+        R x = r2 / R(rc) / R(rc);
+        R mass = R(C.c[2]) * lower_incgamma(C.c[0], x, C.c[1]);
+        R Phipp = R(amp) * (R(4. * M_PI) * pow(r2, R(-0.5 * alpha)) * exp(-r2 / R(rc) / R(rc)) -
+                            R(2.) * mass / pow(r2, R(1.5)));
+        R Phip = R(amp) * mass / r2;
+        R ir2 = R(1.) / r2, ir3 = ir2 / r;
+        R2 += Phipp * Rc * Rc * ir2 + Phip * z * z * ir3;
+        z2 += Phipp * z * z * ir2 + Phip * Rc * Rc * ir3;
+        Rz += Rc * z * (Phipp * ir2 - Phip * ir3);
+        break;
+      }
+      case kMiyamotoNagai: {
+        double amp = C.p[0], a = C.p[1], b = C.p[2];
+        R b2 = R(b * b);
+        R sqrtbz = pow(b2 + z * z, R(0.5));
+        R asqrtbz = R(a) + sqrtbz;
+        R denom = Rc * Rc + asqrtbz * asqrtbz;
+        R2 += R(amp) * (pow(denom, R(-1.5)) - R(3.) * Rc * Rc * pow(denom, R(-2.5)));
+        if (a == 0.) {
+          z2 += R(amp) * (b2 + Rc * Rc - R(2.) * z * z) * pow(b2 + Rc * Rc + z * z, R(-2.5));
+          Rz += R(amp) * (R(-3.) * Rc * z * pow(denom, R(-2.5)));
+        } else {
+          R z4 = z * z * z * z;
+          z2 += R(amp) * (R(a * a * a) * b2 + R(a * a) * (R(3.) * b2 - R(2.) * z * z) * sqrtbz +
+                          (b2 + Rc * Rc - R(2.) * z * z) * pow(b2 + z * z, R(1.5)) +
+                          R(a) * (R(3.) * b2 * b2 - R(4.) * z4 + b2 * (Rc * Rc - z * z))) /
+                (pow(b2 + z * z, R(1.5)) * pow(denom, R(2.5)));
+          Rz += R(amp) * (R(-3.) * Rc * z * asqrtbz / sqrtbz * pow(denom, R(-2.5)));
+        }
+        break;
+      }
+      case kNFW: {
+        double amp = C.p[0], a = C.p[1];
+        R dphi = log(R(1.) + r / R(a)) / r / r - R(1.) / r / (R(a) + r);
+        R ar = R(a) + r;
+        R d2phi = (r * (R(2. * a) + R(3.) * r) - R(2.) * ar * ar * log(R(1.) + r / R(a))) / r / r / r / ar / ar;
+        R2 += R(amp) * (d2phi * Rc * Rc / r / r + dphi * z * z / r / r / r);
+        z2 += R(amp) * (d2phi * z * z / r / r + dphi * Rc * Rc / r / r / r);
+        Rz += R(amp) * Rc * z * (d2phi / r / r - dphi / r / r / r);
+        break;
+      }
+      case kHernquist: {
+        double amp = C.p[0], a = C.p[1];
+        R ar = R(a) + r;
+        R dphi = R(amp) / (R(2.) * ar * ar);
+        R d2phi = -R(amp) / (ar * ar * ar);
+        R2 += d2phi * Rc * Rc / r2 + dphi * z * z / (r2 * r);
+        z2 += d2phi * z * z / r2 + dphi * Rc * Rc / (r2 * r);
+        Rz += Rc * z * (d2phi / r2 - dphi / (r2 * r));
+        break;
+      }
+      case kLogHalo: {
+        double amp = C.p[0], q = C.p[1], c = C.p[2], onem1overb2 = C.p[3];
+        if (onem1overb2 < 1) return false;
+        R D = Rc * Rc + z * z / R(q * q) + R(c);
+        R2 += R(amp) * (D - R(2.) * Rc * Rc) / (D * D);
+        z2 += R(amp) * (D / R(q * q) - R(2.) * z * z / R(q * q * q * q)) / (D * D);
+        Rz += -R(amp) * R(2.) * Rc * z / R(q * q) / (D * D);
+        break;
+      }
+      default:
+        return false;
+    }
+  }
+  return true;
+}
+
+// evalRectForce_dxdv, integrateFullOrbit.c:1619-1681: base acceleration in
+// a[0..2] (as rect_force) and, per deviation column j, K dq_j with K the
+// Cartesian Hessian of -Phi assembled from the cylindrical second derivatives
+// (axisymmetric: the phitorque and phi-derivative terms vanish).
+template <class R>
+bool rect_force_dxdv(int npot, const Component* P, R t, const R* q, R* a, int nde) {
+  rect_force(npot, P, t, q, a);
+  if (nde == 0) return true;
+  R x = q[0], y = q[1], z = q[2];
+  R Rc = sqrt(x * x + y * y);
+  R sinphi = y / Rc, cosphi = x / Rc;
+  R R2d, z2d, Rzd;
+  if (!cyl_hessian(npot, P, Rc, z, R2d, z2d, Rzd)) return false;
+  R phi = acos(x / Rc);
+  if (y < R(0.)) phi = R(2. * M_PI) - phi;
+  R FR, Fz, Fphi;
+  cyl_forces(npot, P, Rc, z, phi, t, FR, Fz, Fphi);
+  R dFxdx = -cosphi * cosphi * R2d + sinphi * sinphi / Rc * FR;
+  R dFxdy = -sinphi * cosphi * R2d - cosphi * sinphi / Rc * FR;
+  R dFydy = -sinphi * sinphi * R2d + cosphi * cosphi / Rc * FR;
+  R dFxdz = -cosphi * Rzd;
+  R dFydz = -sinphi * Rzd;
+  R dFzdz = -z2d;
+  for (int j = 1; j <= nde; j++) {
+    R dx = q[3 * j], dy = q[3 * j + 1], dz = q[3 * j + 2];
+    a[3 * j] = dFxdx * dx + dFxdy * dy + dFxdz * dz;
+    a[3 * j + 1] = dFxdy * dx + dFydy * dy + dFydz * dz;
+    a[3 * j + 2] = dFxdz * dx + dFydz * dy + dFzdz * dz;
+  }
+  return true;
+}
+
 // Potential value: galpy_potentials.c evaluatePotentials analog (sum in
 // component order); the bar uses the Python expression.
 template <class R>

@@ -169,3 +169,33 @@ ORC_EXPORT void oracle_cyl_to_rect(int n, double* v) {
 ORC_EXPORT void oracle_rect_to_cyl(int n, double* v) {
   for (int i = 0; i < n; i++) rect_to_cyl<double>(v + 6 * i);
 }
+
+// Tangent-map integration of n orbits (bovy_symplecticode.c:1053-1268 with
+// integrateFullOrbit.c:1619-1681): w[n][(1+nde)*6] rectangular base state and
+// deviation columns in, out[n][(1+nde)*6] final augmented state.  method 0
+// (leapfrog) or 3 (symplec4); nde <= 6.  Returns -1 on an unsupported
+// component or method.
+ORC_EXPORT int oracle_symplectic_dxdv(int n, const double* w, int nde, int nt, const double* t,
+                                      int npot, int* pot_type, double* pot_args, double dt,
+                                      double rtol, double atol, int method, double* out) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  if ((method != 0 && method != 3) || nde < 0 || nde > 6) return -1;
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+  for (int i = 0; i < n; i++)
+    bad += !symplectic_dxdv<double>(npot, P, method, w + (long)i * 6 * (1 + nde), nde, nt, t, dt,
+                                    rtol, atol, out + (long)i * 6 * (1 + nde));
+  return bad ? -1 : 0;
+}
+
+// Cylindrical second derivatives (R2deriv, z2deriv, Rzderiv) at n points.
+ORC_EXPORT int oracle_eval_hessian(int npot, int* pot_type, double* pot_args, int n,
+                                   const double* R, const double* z, double* R2, double* z2,
+                                   double* Rz) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  for (int i = 0; i < n; i++)
+    if (!cyl_hessian<double>(npot, P, R[i], z[i], R2[i], z2[i], Rz[i])) return -1;
+  return 0;
+}

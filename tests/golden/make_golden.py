@@ -8,8 +8,8 @@ incgamma.json    lower/upper incomplete gamma gamma(a,x), Gamma(a,x) for the
                  1 - alpha/2) over x in [1e-8, 1e3].
 potentials.json  forces and potential values of the six supported potentials
                  at seeded points, from closed forms in mpmath; forces are the
-                 analytic gradients of the potential (checked here against a
-                 high-precision numerical derivative before writing).
+                 50-digit numerical derivatives of the closed-form potential;
+                 second derivatives R2/z2/Rz for the axisymmetric ones.
 """
 
 import json
@@ -118,8 +118,14 @@ def potentials():
             phi = float(rng.uniform(0.0, 2 * numpy.pi))
             t = float(rng.uniform(-30.0, 10.0)) if name == "dehnenbar" else 0.0
             FR, Fz, Fp = forces(fn, mp.mpf(R), mp.mpf(z), mp.mpf(phi), mp.mpf(t))
-            pts.append({"R": R, "z": z, "phi": phi, "t": t, "pot": float(fn(mp.mpf(R), mp.mpf(z), mp.mpf(phi), mp.mpf(t))),
-                        "FR": float(FR), "Fz": float(Fz), "Fphi": float(Fp)})
+            row = {"R": R, "z": z, "phi": phi, "t": t, "pot": float(fn(mp.mpf(R), mp.mpf(z), mp.mpf(phi), mp.mpf(t))),
+                   "FR": float(FR), "Fz": float(Fz), "Fphi": float(Fp)}
+            if name in ("psc", "mn", "nfw", "hernquist", "loghalo_q09"):
+                f2 = lambda a, b: fn(a, b, mp.mpf(phi), mp.mpf(t))
+                row["R2"] = float(mp.diff(f2, (mp.mpf(R), mp.mpf(z)), (2, 0)))
+                row["z2"] = float(mp.diff(f2, (mp.mpf(R), mp.mpf(z)), (0, 2)))
+                row["Rz"] = float(mp.diff(f2, (mp.mpf(R), mp.mpf(z)), (1, 1)))
+            pts.append(row)
         out[name] = {"pot_type": [code], "pot_args": args, "points": pts}
     return out
 

@@ -138,3 +138,55 @@ def phase_diff(a, b):
     scale = numpy.maximum(numpy.abs(b), 1.0)
     scale[..., 5] = numpy.pi
     return (d / scale).max()
+
+
+def oracle_dxdv(pot_type, pot_args, w, nde, t, method, dt=None):
+    """Oracle tangent-map integration: w [n, 6*(1+nde)] rectangular base
+    state + deviation columns -> final augmented states."""
+    lib = oracle()
+    w = numpy.ascontiguousarray(numpy.atleast_2d(w), dtype=numpy.float64)
+    t = numpy.ascontiguousarray(t, dtype=numpy.float64)
+    out = numpy.empty_like(w)
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    rc = lib.oracle_symplectic_dxdv(
+        ctypes.c_int(len(w)), _p(w), ctypes.c_int(nde), ctypes.c_int(len(t)), _p(t), ctypes.c_int(len(pt)),
+        _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(LN1EM12),
+        ctypes.c_double(LN1EM12), ctypes.c_int(method), _p(out),
+    )
+    assert rc == 0
+    return out
+
+
+def oracle_stm(pot_type, pot_args, w0, t, method, dt=None):
+    """Full 6x6 state-transition matrices M[n, 6, 6] (M[:, i, j] = dw_i(T)/dw_j(0))
+    and the final base states, from the oracle's tangent map with the six
+    unit deviation columns."""
+    w0 = numpy.atleast_2d(w0)
+    n = len(w0)
+    aug = numpy.zeros((n, 6 * 7))
+    aug[:, :6] = w0
+    for j in range(6):
+        aug[:, 6 * (j + 1) + j] = 1.0
+    out = oracle_dxdv(pot_type, pot_args, aug, 6, t, method, dt=dt)
+    M = out[:, 6:].reshape(n, 6, 6).transpose(0, 2, 1)
+    return M, out[:, :6]
+
+
+def oracle_hessian(pot_type, pot_args, R, z):
+    lib = oracle()
+    R = numpy.ascontiguousarray(R, dtype=float)
+    z = numpy.ascontiguousarray(numpy.broadcast_to(z, R.shape), dtype=float)
+    out = [numpy.empty(R.size) for _ in range(3)]
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    rc = lib.oracle_eval_hessian(len(pt), _p(pt), _p(pa), R.size, _p(R), _p(z), *[_p(o) for o in out])
+    assert rc == 0
+    return tuple(out)
+
+
+def cyl_to_rect(v):
+    v = numpy.atleast_2d(v)
+    R, vR, vT, z, vz, phi = v.T
+    c, s = numpy.cos(phi), numpy.sin(phi)
+    return numpy.stack([R * c, R * s, z, vR * c - vT * s, vR * s + vT * c, vz], axis=1)
