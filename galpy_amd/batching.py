@@ -30,14 +30,12 @@ from .potential import parse_pot
 GROUP_DTYPE = numpy.dtype(
     [
         ("table", "<u8"), ("t", "<u8"), ("yo", "<u8"), ("result", "<u8"), ("err", "<u8"),
-        ("stats", "<u8"), ("dt", "<f8"), ("rtol", "<f8"), ("atol", "<f8"), ("n", "<i4"),
-        ("nt", "<i4"), ("method", "<i4"), ("pad", "<i4"),
+        ("stats", "<u8"), ("state", "<u8"), ("dt", "<f8"), ("rtol", "<f8"), ("atol", "<f8"),
+        ("n", "<i4"), ("nt", "<i4"), ("method", "<i4"), ("pad", "<i4"),
     ]
 )
-TILE_DTYPE = numpy.dtype(
-    [("group", "<i4"), ("first", "<i4"), ("count", "<i4"), ("pad", "<i4"), ("state", "<i8")]
-)
-assert GROUP_DTYPE.itemsize == 88 and TILE_DTYPE.itemsize == 24
+TILE_DTYPE = numpy.dtype([("group", "<i4"), ("first", "<i4"), ("count", "<i4"), ("pad", "<i4")])
+assert GROUP_DTYPE.itemsize == 96 and TILE_DTYPE.itemsize == 16
 
 # relative cost of one force evaluation, for the longest-first tile order
 _POT_COST = {15: 4.0, 5: 1.0, 9: 1.5, 8: 0.6, 0: 0.5, 1: 1.5}
@@ -45,10 +43,10 @@ _METHOD_FORCES = {0: 1.0, 3: 3.0, 6: 16.0}
 
 _BATCH_SIGS = {
     "gh_batch_state_bytes": (ctypes.c_size_t, []),
-    "gh_batch_reset": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong] + [ctypes.c_void_p] * 4),
+    "gh_batch_reset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3),
     "gh_batch_launch": (
         ctypes.c_int,
-        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
         + [ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
     ),
 }
@@ -105,6 +103,7 @@ class Batch:
         self.state_bytes = int(lib.gh_batch_state_bytes())
         G = numpy.zeros(len(self.groups), dtype=GROUP_DTYPE)
         self.yo, self.t, self.results, self.errs, self.stats, self.tables = [], [], [], [], [], []
+        self.state = []
         costs = []
         self.with_dop = False
         for i, g in enumerate(self.groups):
@@ -117,13 +116,15 @@ class Batch:
             res = torch.empty((n, nt, 6), dtype=torch.float64, device=self.device)
             err = torch.zeros(n, dtype=torch.int32, device=self.device)
             st = torch.zeros((n, 3), dtype=torch.float64, device=self.device)
+            state = torch.empty(n * self.state_bytes, dtype=torch.uint8, device=self.device)
             method = METHOD_CODES[g.get("method", "symplec4_hip")]
             self.with_dop |= method == 6
             rtol, atol = _parse_tol(g.get("rtol"), g.get("atol"))
             dt = g.get("dt")
             G[i] = (lib.gh_potential_device_ptr(tab.handle), t.data_ptr(), yo.data_ptr(), res.data_ptr(),
-                    err.data_ptr(), st.data_ptr(), DT_SENTINEL if dt is None else float(dt), rtol, atol,
-                    n, nt, method, 0)
+                    err.data_ptr(), st.data_ptr(), state.data_ptr(), DT_SENTINEL if dt is None else float(dt),
+                    rtol, atol, n, nt, method, 0)
+            self.state.append(state)
             self.yo.append(yo)
             self.t.append(t)
             self.results.append(res)
@@ -142,19 +143,18 @@ class Batch:
             n = len(g["yo"])
             for first in range(0, n, per_tile):
                 cnt = min(per_tile, n - first)
-                tiles.append((i, first, cnt, 0, nstate, costs[i]))
-                nstate += cnt
+                tiles.append((i, first, cnt, 0, costs[i]))
+            nstate += n
         if order == "longest_first":
-            tiles.sort(key=lambda x: -x[5])
+            tiles.sort(key=lambda x: -x[4])
         T = numpy.zeros(len(tiles), dtype=TILE_DTYPE)
         for k, tl in enumerate(tiles):
-            T[k] = tl[:5]
+            T[k] = tl[:4]
         self.T = T
         self.ntiles = len(tiles)
         self.nstate = nstate
         self.d_groups = torch.from_numpy(G.view(numpy.uint8).copy()).to(self.device)
         self.d_tiles = torch.from_numpy(T.view(numpy.uint8).copy()).to(self.device)
-        self.d_state = torch.empty(max(1, nstate) * self.state_bytes, dtype=torch.uint8, device=self.device)
         self.d_tile_done = torch.zeros(max(1, self.ntiles), dtype=torch.int32, device=self.device)
         self.d_remaining = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.d_counter = torch.zeros(1, dtype=torch.int64, device=self.device)
@@ -166,7 +166,7 @@ class Batch:
 
     def reset(self, stream=None):
         s = self._stream(stream)
-        rc = _lib_batch().gh_batch_reset(self.ntiles, self.nstate, self.d_state.data_ptr(),
+        rc = _lib_batch().gh_batch_reset(self.d_groups.data_ptr(), self.d_tiles.data_ptr(), self.ntiles,
                                          self.d_tile_done.data_ptr(), self.d_remaining.data_ptr(),
                                          s.cuda_stream)
         _lib.check(rc, "gh_batch_reset")
@@ -175,7 +175,7 @@ class Batch:
         """One persistent launch (asynchronous); budget <= 0 runs to completion."""
         s = self._stream(stream)
         rc = _lib_batch().gh_batch_launch(
-            self.d_groups.data_ptr(), self.d_tiles.data_ptr(), self.ntiles, self.d_state.data_ptr(),
+            self.d_groups.data_ptr(), self.d_tiles.data_ptr(), self.ntiles,
             self.d_tile_done.data_ptr(), self.d_counter.data_ptr(), self.d_remaining.data_ptr(),
             int(budget), self.opl, int(self.with_dop), s.cuda_stream,
         )

@@ -138,8 +138,8 @@ __device__ __forceinline__ void gh_batch_finish(const gh_batch_group& g, long o,
 // number of orbits of the tile (OPL per lane) still unfinished in this lane.
 template <int OPL, bool DOP>
 __device__ int gh_batch_run_tile(const gh_batch_group& g, const gh_batch_tile& tile,
-                             GhLaneState* __restrict__ state, long long budget,
-                             const volatile int* cancel) {
+                             long long budget, const volatile int* cancel) {
+  GhLaneState* __restrict__ state = (GhLaneState*)g.state;
   const GhForce<GhSpec<>> Fo{(const GhPotTable*)g.table};
   const int lane = threadIdx.x & 63;
   GhLaneState s[OPL];
@@ -151,7 +151,7 @@ __device__ int gh_batch_run_tile(const gh_batch_group& g, const gh_batch_tile& t
     act[u] = k < tile.count;
     o[u] = tile.first + k;
     if (act[u]) {
-      s[u] = state[tile.state + k];
+      s[u] = state[tile.first + k];
       if (s[u].status == 0) gh_batch_start<DOP>(Fo, g, o[u], s[u]);
       act[u] = s[u].status == 1;
     }
@@ -202,7 +202,7 @@ __device__ int gh_batch_run_tile(const gh_batch_group& g, const gh_batch_tile& t
   for (int u = 0; u < OPL; u++) {
     const int k = lane + 64 * u;
     if (k < tile.count) {
-      state[tile.state + k] = s[u];
+      state[tile.first + k] = s[u];
       left += s[u].status != 2;
     }
   }
@@ -217,7 +217,7 @@ __device__ int gh_batch_run_tile(const gh_batch_group& g, const gh_batch_tile& t
 template <int OPL, bool DOP>
 __global__ void __launch_bounds__(64)
 gh_batch_kernel(const gh_batch_group* __restrict__ groups, const gh_batch_tile* __restrict__ tiles,
-                int ntiles, GhLaneState* __restrict__ state, int* __restrict__ tile_done,
+                int ntiles, int* __restrict__ tile_done,
                 unsigned long long* __restrict__ counter, int* __restrict__ remaining,
                 long long budget, const volatile int* cancel) {
   const int lane = threadIdx.x & 63;
@@ -229,10 +229,10 @@ gh_batch_kernel(const gh_batch_group* __restrict__ groups, const gh_batch_tile* 
     if (tile_done[t]) continue;
     const gh_batch_tile tile = tiles[t];
     const gh_batch_group g = groups[gh_uniform(tile.group)];
-    const int left = gh_batch_run_tile<OPL, DOP>(g, tile, state, budget, cancel);
+    const int left = gh_batch_run_tile<OPL, DOP>(g, tile, budget, cancel);
     if (!__any(left != 0) && lane == 0) {
       tile_done[t] = 1;
-      atomicSub(remaining, 1);
+      if (remaining) atomicSub(remaining, 1);
     }
   }
 }

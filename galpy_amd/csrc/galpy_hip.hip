@@ -960,13 +960,19 @@ GH_EXPORT int gh_adjoint_hip(int n, const double* d_wT, const double* d_gT, doub
 // ---------------------------------------------------------------------------
 namespace {
 
-__global__ void gh_batch_reset_kernel(long long nstate, GhLaneState* __restrict__ state, int ntiles,
+// one workgroup per tile: mark the tile's orbits unstarted, clear its flag
+__global__ void gh_batch_reset_kernel(const gh_batch_group* __restrict__ groups,
+                                      const gh_batch_tile* __restrict__ tiles, int ntiles,
                                       int* __restrict__ tile_done, int* __restrict__ remaining) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long k = i; k < nstate; k += stride) state[k].status = 0;
-  for (long long k = i; k < ntiles; k += stride) tile_done[k] = 0;
-  if (i == 0) *remaining = ntiles;
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const gh_batch_tile tile = tiles[t];
+  GhLaneState* state = (GhLaneState*)groups[tile.group].state;
+  for (int k = threadIdx.x; k < tile.count; k += blockDim.x) state[tile.first + k].status = 0;
+  if (threadIdx.x == 0) {
+    tile_done[t] = 0;
+    if (t == 0 && remaining) *remaining = ntiles;
+  }
 }
 
 template <int OPL, bool DOP>
@@ -987,22 +993,21 @@ int batch_resident_blocks(int dev) {
 
 GH_EXPORT size_t gh_batch_state_bytes(void) { return sizeof(GhLaneState); }
 
-GH_EXPORT int gh_batch_reset(int ntiles, long long nstate, void* d_state, int* d_tile_done,
-                             int* d_remaining, void* stream) {
+GH_EXPORT int gh_batch_reset(const gh_batch_group* d_groups, const gh_batch_tile* d_tiles,
+                             int ntiles, int* d_tile_done, int* d_remaining, void* stream) {
   g_errbuf[0] = 0;
   irq_device_ptr();  // set up outside any graph capture of the launches
-  const long long work = std::max<long long>(nstate, ntiles);
-  const unsigned nb = (unsigned)std::max<long long>(1, std::min<long long>((work + 255) / 256, 65535));
-  hipLaunchKernelGGL(gh_batch_reset_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, nstate,
-                     (GhLaneState*)d_state, ntiles, d_tile_done, d_remaining);
+  if (ntiles <= 0) return 0;
+  hipLaunchKernelGGL(gh_batch_reset_kernel, dim3((unsigned)ntiles), dim3(128), 0,
+                     (hipStream_t)stream, d_groups, d_tiles, ntiles, d_tile_done, d_remaining);
   GH_CHECK(hipGetLastError());
   return 0;
 }
 
 GH_EXPORT int gh_batch_launch(const gh_batch_group* d_groups, const gh_batch_tile* d_tiles,
-                              int ntiles, void* d_state, int* d_tile_done,
-                              unsigned long long* d_counter, int* d_remaining, long long budget,
-                              int opl, int with_dop, void* stream) {
+                              int ntiles, int* d_tile_done, unsigned long long* d_counter,
+                              int* d_remaining, long long budget, int opl, int with_dop,
+                              void* stream) {
   g_errbuf[0] = 0;
   if (ntiles <= 0) return 0;
   if (opl != 1 && opl != 2) {
@@ -1025,10 +1030,9 @@ GH_EXPORT int gh_batch_launch(const gh_batch_group* d_groups, const gh_batch_til
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, st, d_counter, 0ull);
   const int* cancel = irq_device_ptr();
-  GhLaneState* state = (GhLaneState*)d_state;
 #define GH_BATCH(O, D)                                                                          \
   hipLaunchKernelGGL((gh_batch_kernel<O, D>), dim3(nblocks), dim3(64), 0, st, d_groups, d_tiles, \
-                     ntiles, state, d_tile_done, d_counter, d_remaining, b, cancel)
+                     ntiles, d_tile_done, d_counter, d_remaining, b, cancel)
   switch (key) {
     case 0: GH_BATCH(1, false); break;
     case 1: GH_BATCH(1, true); break;

@@ -121,6 +121,7 @@ typedef struct gh_batch_group {
   double *result;      /* [n][nt][6] output rows */
   int *err;            /* [n] */
   double *stats;       /* [n][3] orbit-steps, rejected steps, step used; may be NULL */
+  void *state;         /* [n] resumable state records (gh_batch_state_bytes() each) */
   double dt;           /* fixed step, or -9999.99 to estimate per orbit (symplectic) */
   double rtol, atol;   /* natural logarithms */
   int n, nt;
@@ -133,17 +134,17 @@ typedef struct gh_batch_tile {
   int first;           /* first orbit of the group in this tile */
   int count;           /* orbits in the tile, <= 64 * orbits-per-lane */
   int pad;
-  long long state;     /* first per-orbit state record of the tile */
 } gh_batch_tile;
 
 /* Bytes of resumable state per orbit. */
 size_t gh_batch_state_bytes(void);
 
-/* Prepares a batch for its first launch: zeroes nstate state records and
- * ntiles tile flags and sets *d_remaining = ntiles (kernels only, so the
- * sequence may be captured in a graph). */
-int gh_batch_reset(int ntiles, long long nstate, void *d_state, int *d_tile_done,
-                   int *d_remaining, void *stream);
+/* Prepares tiles for their first launch: marks the state records of their
+ * orbits unstarted, clears their tile flags and sets *d_remaining = ntiles
+ * (d_remaining may be NULL).  Kernels only, so the sequence may be captured
+ * in a graph. */
+int gh_batch_reset(const gh_batch_group *d_groups, const gh_batch_tile *d_tiles, int ntiles,
+                   int *d_tile_done, int *d_remaining, void *stream);
 
 /* One persistent launch: wavefronts claim unfinished tiles from *d_counter
  * (reset to 0 here by a kernel) and advance each orbit by at most `budget`
@@ -152,8 +153,8 @@ int gh_batch_reset(int ntiles, long long nstate, void *d_state, int *d_tile_done
  * dop853 (selects the kernel instance that carries it).  *d_remaining counts
  * unfinished tiles.  Asynchronous on stream. */
 int gh_batch_launch(const gh_batch_group *d_groups, const gh_batch_tile *d_tiles, int ntiles,
-                    void *d_state, int *d_tile_done, unsigned long long *d_counter,
-                    int *d_remaining, long long budget, int opl, int with_dop, void *stream);
+                    int *d_tile_done, unsigned long long *d_counter, int *d_remaining,
+                    long long budget, int opl, int with_dop, void *stream);
 
 /* Message of the last error on the calling thread ("" if none). */
 const char *galpy_hip_last_error(void);
