@@ -1,6 +1,9 @@
 """Continuous-batching server: FIFO memory admission (CPU), and on the GPU
 every request's result in a mixed, overlapping stream equals the same request
-integrated alone; memory-limited admission still completes everything."""
+integrated alone through integrateFullOrbit_hip, within the parity bars
+(1e-10 symplectic, 1e-8 DOP853 -- the mixed launch runs the generic batch
+kernel, whose rounding can move a DOP853 step decision); memory-limited
+admission still completes everything."""
 
 import numpy
 import pytest
@@ -41,7 +44,8 @@ def test_each_result_same_alone_and_in_the_mix():
     assert m["iterations"] > 1
     for r in srv.finished:
         assert (r.err.cpu().numpy() == 0).all()
-        assert phase_diff(r.result.cpu().numpy(), _alone(r)) < 1e-12, r.id
+        tol = 1e-8 if r.method == "dop853_hip" else 1e-10
+        assert phase_diff(r.result.cpu().numpy(), _alone(r)) < tol, r.id
     assert m["p99_latency_s"] >= m["p50_latency_s"] > 0
 
 
