@@ -61,10 +61,10 @@ def cpu_baseline_leg(w, sample, count_sample):
     nthreads = len(os.sched_getaffinity(0))
     yo = w["yo"][:sample]
     t0 = time.perf_counter()
-    _, err, stats, _ = oracle_integrate(pt, pa, yo, w["t"], code, nthreads=nthreads)
+    _, err, stats, _ = oracle_integrate(pt, pa, yo, w["t"], code, dt=w.get("dt"), nthreads=nthreads)
     el = time.perf_counter() - t0
     steps = stats[:, 0].sum()
-    ops = oracle_count_ops(pt, pa, w["yo"][:count_sample], w["t"], code)
+    ops = oracle_count_ops(pt, pa, w["yo"][:count_sample], w["t"], code, dt=w.get("dt"))
     flops = ops["add"] + ops["mul"] + ops["div"] + ops["sqrt"] + ops["special"]
     return {
         "value": steps / el,
@@ -194,6 +194,31 @@ def bench_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_copy(args):
+    """A plain device-to-device copy of known size: the calibration run for
+    the FETCH_SIZE / WRITE_SIZE hardware counters (bytes read and written by
+    one copy launch are known exactly)."""
+    import torch
+
+    nbytes = 1 << 31
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"metric": "device copy bandwidth (counter calibration)", "value": 2 * nbytes * args.steps / el / 1e9,
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "copy", "bytes_read_per_launch": nbytes,
+                                 "bytes_written_per_launch": nbytes}}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -205,6 +230,9 @@ def main():
         return
     if args.workload.startswith("serving"):
         bench_serving(args)
+        return
+    if args.workload == "copy":
+        bench_copy(args)
         return
 
     import torch
@@ -232,7 +260,8 @@ def main():
         yo.copy_(yo0)
         if ev is not None:
             ev[0].record(stream)
-        integrate_device(table, yo, tt, w["method"], result=result, err=err, stats=stats, stream=stream)
+        integrate_device(table, yo, tt, w["method"], dt=w.get("dt"), result=result, err=err, stats=stats,
+                         stream=stream)
         if ev is not None:
             ev[1].record(stream)
 
@@ -278,6 +307,7 @@ def main():
             "t": [float(t_np[0]), float(t_np[-1]), nt],
             "rtol": 1e-12,
             "atol": 1e-12,
+            "dt": w.get("dt"),
             "output_bytes": n * nt * 48,
         },
         "orbit_steps_per_pass": steps_per_pass,
@@ -304,7 +334,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": None,
-            "kernel": "gh_dop853_kernel",
+            "kernel": "gh_dop853_kernel" if w["method"] == "dop853_hip" else "gh_symplectic_kernel",
             "note": ("FP64 vector ALU bound (no MFMA: nothing in the path is a contraction; "
                      "78.6 TFLOP/s is MI355X's FP64 peak). achieved = counted algorithmic FP64 ops "
                      "per orbit-step x orbit-steps per launch / mean launch time; +,-,*,/,sqrt and "

@@ -64,7 +64,7 @@ __device__ __forceinline__ void gh_comp_adj<GH_NFW>(const GhPotTable* __restrict
   const double r = r2 * rinv;
   const double ar = aa + r;
   const double iar = 1.0 / ar;
-  const double L = log1p(r * C->c[0]);
+  const double L = gh_log_ge1(1.0 + r * C->c[0]);
   const double ri2 = rinv * rinv, ri3 = ri2 * rinv;
   const double h = ri2 * iar - L * ri3;
   const double hp = -3.0 * ri3 * iar - ri2 * iar * iar + 3.0 * L * ri2 * ri2;

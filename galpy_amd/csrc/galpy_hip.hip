@@ -1090,6 +1090,11 @@ GH_EXPORT int gh_host_bulge_lower_gamma(int npot, const int* pot_type, const dou
   return 0;
 }
 
+GH_EXPORT int gh_host_log_ge1(int n, const double* y, double* out) {
+  for (int i = 0; i < n; i++) out[i] = gh_log_ge1(y[i]);
+  return 0;
+}
+
 GH_EXPORT const char* galpy_hip_last_error(void) { return g_errbuf; }
 
 GH_EXPORT void galpy_hip_set_interrupt(int value) {

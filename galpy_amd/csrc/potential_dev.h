@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "gh_math.h"
+
 #define GH_MAX_POT 8
 #define GH_IG_NSER 32  // terms of the incomplete-gamma series (enough for a <= 1.5)
 // piecewise-polynomial table of Gamma(a,x) on [a+1, xbig): GH_TAB_PER_OCT
@@ -232,7 +234,8 @@ __device__ __forceinline__ void gh_comp_force<GH_NFW>(const GhPotTable* __restri
   const GhComponent* __restrict__ C = &T->comp[i];
   const double amp = C->p[0], a = C->p[1];
   const double rinv2 = g.rinv * g.rinv;
-  const double br = rinv2 / (a + g.r) - log1p(g.r * C->c[0]) * (rinv2 * g.rinv);
+  // log(1 + r/a) as NFWPotential.c:26 forms it, with the short log of gh_math.h
+  const double br = rinv2 / (a + g.r) - gh_log_ge1(1.0 + g.r * C->c[0]) * (rinv2 * g.rinv);
   F.FR += amp * g.R * br;
   F.Fz += amp * g.z * br;
 }
@@ -428,7 +431,7 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
       }
       case GH_NFW: {
         const double r = sqrt(r2);
-        pot += -C->p[0] * log1p(r * C->c[0]) / r;
+        pot += -C->p[0] * gh_log_ge1(1.0 + r * C->c[0]) / r;
         break;
       }
       case GH_HERNQUIST: {

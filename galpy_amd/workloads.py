@@ -40,6 +40,9 @@ WORKLOADS = {
     # 1,048,576 orbits, DOP853 at galpy's default tolerances
     "mw2014_dop853_1m": dict(pot=mw2014, n=1048576, t=(0.0, 500.0, 1001), method="dop853_hip",
                             omega=0.0, seed=0),
+    # 1,048,576 orbits, symplec4 at a fixed step (the force kernel at scale)
+    "mw2014_symplec4_1m": dict(pot=mw2014, n=1048576, t=(0.0, 100.0, 101), method="symplec4_hip",
+                              omega=0.0, seed=0, dt=0.01),
     # 262,144 orbits in a rotating bar (Jacobi integral instead of energy)
     "loghalo_bar_dop853_256k": dict(pot=loghalo_bar, n=262144, t=(0.0, 500.0, 1001),
                                    method="dop853_hip", omega=1.85, seed=1),

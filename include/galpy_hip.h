@@ -85,6 +85,11 @@ int gh_eval_hip(int n, const double *d_R, const double *d_z, const double *d_phi
 int gh_host_bulge_lower_gamma(int npot, const int *pot_type, const double *pot_args, int comp,
                               int n, const double *x, double *out);
 
+/* Host evaluation of the force kernels' logarithm of y >= 1 (the
+ * log(1 + r/a) of NFWPotential.c:26), the same code the device runs; for
+ * checks without a GPU. */
+int gh_host_log_ge1(int n, const double *y, double *out);
+
 /* ------------------------------------------------------------------------
  * Differentiation (no reference counterpart; galpy offers the forward
  * tangent map integrate_dxdv, bovy_symplecticode.c:1053-1268).

@@ -77,6 +77,34 @@ def test_host_table_incgamma_vs_mpmath():
         assert numpy.all(numpy.abs(out / lo - 1) < 5e-14), a
 
 
+def test_host_log_ge1_vs_mpmath():
+    """The force kernels' logarithm (NFW's log(1 + r/a)), evaluated on the
+    host by the same code the device runs, within 1.5 ulp of mpmath's 40-digit
+    log over [1, 2^60], including the reduction boundaries sqrt(2) 2^k and
+    exact powers of two."""
+    import mpmath
+
+    from galpy_amd import _lib
+
+    lib = _lib.load()
+    rng = numpy.random.default_rng(11)
+    y = numpy.concatenate([
+        1.0 + rng.uniform(0.0, 3.0, 3000),
+        numpy.exp2(rng.uniform(0.0, 60.0, 3000)),
+        1.0 + numpy.array([0.0, 1e-300, 1e-16, 2.2e-16, 1e-12, 1e-8, 1e-4]),
+        numpy.sqrt(2.0) * numpy.exp2(numpy.arange(0, 20)),
+        numpy.nextafter(numpy.sqrt(2.0) * numpy.exp2(numpy.arange(0, 20)), 0.0),
+        numpy.exp2(numpy.arange(0, 60)).astype(float),
+    ])
+    out = numpy.empty_like(y)
+    assert lib.gh_host_log_ge1(len(y), y.ctypes.data, out.ctypes.data) == 0
+    mpmath.mp.dps = 40
+    ref = numpy.array([float(mpmath.log(mpmath.mpf(float(v)))) for v in y])
+    ulp = numpy.spacing(numpy.maximum(numpy.abs(ref), numpy.finfo(float).tiny))
+    assert numpy.all(numpy.abs(out - ref) <= 1.5 * ulp)
+    assert out[numpy.flatnonzero(y == 1.0)[0]] == 0.0
+
+
 # ---------------------------------------------------------------- potentials
 @pytest.mark.parametrize("name", ["psc", "mn", "nfw", "hernquist", "loghalo_q09", "loghalo_triax", "dehnenbar"])
 def test_oracle_potentials_vs_mpmath(name):
