@@ -129,7 +129,8 @@ struct GhDopState {
 #define GH_DOP_RARE __forceinline__
 #endif
 
-template <class F>
+// STORE = false (a pilot run that only counts steps) writes no output rows.
+template <class F, bool STORE = true>
 __device__ GH_DOP_RARE void gh_dop_init(const F& T,
                                             const double* __restrict__ t, int nt, double rtol,
                                             double atol, GhDopState& s,
@@ -140,7 +141,7 @@ __device__ GH_DOP_RARE void gh_dop_init(const F& T,
   s.reject = 0;
   s.nacc = 0;
   s.nrej = 0;
-  gh_store_cyl(out, s.y[0], s.y[1], s.y[2], s.y[3], s.y[4], s.y[5]);
+  if (STORE) gh_store_cyl(out, s.y[0], s.y[1], s.y[2], s.y[3], s.y[4], s.y[5]);
   gh_deriv(T, t[0], s.y, s.k1);
   double dnf = 0.0, dny = 0.0;
 #pragma unroll
@@ -178,7 +179,7 @@ __device__ GH_DOP_RARE void gh_dop_init(const F& T,
 // in seven, and inlining its three force evaluations would push the step
 // loop out of the instruction cache.  k2, k3, k10 are clobbered (the next
 // step recomputes them).
-template <class F>
+template <class F, bool STORE = true>
 __device__ GH_DOP_RARE void gh_dop_dense(const F& T, const double* __restrict__ t, int nt,
                                           GhDopState& s, double* __restrict__ out, double h,
                                           const double* y0, const double* k1, double* k2,
@@ -236,13 +237,13 @@ __device__ GH_DOP_RARE void gh_dop_dense(const F& T, const double* __restrict__ 
       yo[i] = rc1[i] + sd * (rc2[i] + s1 * (rc3[i] + sd * (rc4[i] + s1 * (rc5[i] + sd * (rc6[i] +
                           s1 * (rc7[i] + sd * rc8[i]))))));
     s.done++;
-    gh_store_cyl(out + 6L * s.done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
+    if (STORE) gh_store_cyl(out + 6L * s.done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
   }
 }
 
 // One attempted step (leung_dop853.c:335-492); returns true once every
 // requested time has been written.
-template <class F>
+template <class F, bool STORE = true>
 __device__ __forceinline__ bool gh_dop_step(const F& T,
                                             const double* __restrict__ t, int nt, double rtol,
                                             double atol, GhDopState& s,
@@ -335,7 +336,7 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
     // three extra stages feed nothing else (the next step recomputes k2, k3
     // and k10), so they are skipped otherwise -- outputs are unchanged
     if (s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur)
-      gh_dop_dense(T, t, nt, s, out, h, y0, k1, k2, k3, k4, k5, k6, k7, k8, k9, k10);
+      gh_dop_dense<F, STORE>(T, t, nt, s, out, h, y0, k1, k2, k3, k4, k5, k6, k7, k8, k9, k10);
 #pragma unroll
     for (int i = 0; i < 6; i++) {
       s.k1[i] = k4[i];

@@ -6,7 +6,9 @@
 // cylindrical initial conditions in, cylindrical rows out, one call for all
 // orbits.  Declarations and argument meaning: include/galpy_hip.h.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 #include <signal.h>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -344,8 +346,17 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
 // many wavefronts as the GPU keeps resident; lane g starts on orbit g and,
 // whenever its orbit finishes, claims the next unstarted orbit from a global
 // counter (one atomic per wavefront per refill, ranks from the ballot mask).
-// Orbits differ ~10x in accepted-step count at fixed tolerance, so without
+// Orbits differ ~25x in accepted-step count at fixed tolerance, so without
 // refill a wavefront runs as long as its slowest lane.
+//
+// `order` (nullable) permutes the claim sequence: claim k integrates orbit
+// order[k].  With the orbits sorted by expected cost, longest first, the
+// lanes finish together instead of waiting on a few long orbits started
+// last (list scheduling; the expected cost comes from a PILOT launch of the
+// same kernel over the first few percent of the time range, which writes
+// only cost[o] = attempted steps and no output rows).  Every orbit is
+// integrated by the same code whatever the order, so results do not depend
+// on it.
 // Waves per SIMD the register allocator targets.  With the force specialised
 // per composition, one wave per SIMD with ~380 registers and no spills
 // measured fastest on MI355X (2.47 s vs 2.83 s at two waves with 512 B/lane
@@ -356,18 +367,20 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
 #endif
 #define GH_DOP_ATTR __attribute__((amdgpu_waves_per_eu(GH_DOP_WAVES_PER_EU, GH_DOP_WAVES_PER_EU)))
 
-template <class Spec>
+template <class Spec, bool PILOT>
 __global__ void __launch_bounds__(kBlock) GH_DOP_ATTR
 gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
                  int indiv_t, double rtol_log, double atol_log, const GhPotTable* __restrict__ T,
                  double* __restrict__ result, int* __restrict__ err, double* __restrict__ stats,
-                 const volatile int* cancel, unsigned long long* __restrict__ counter) {
+                 const volatile int* cancel, unsigned long long* __restrict__ counter,
+                 const int* __restrict__ order, unsigned* __restrict__ cost) {
   const GhForce<Spec> F{T};
   const double rtol = exp(rtol_log);
   const double atol = exp(atol_log);
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ull << lane) - 1ull;
-  long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long claim = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long orbit = 0;
   GhDopState s;
   const double* tt = t;
   double* out = result;
@@ -375,14 +388,18 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
 
   auto start = [&](long o) {
     tt = t + (indiv_t ? o * nt : 0);
-    out = result + o * (long)nt * 6;
+    out = PILOT ? nullptr : result + o * (long)nt * 6;
     double v[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) v[k] = yo[o * 6 + k];
     gh_cyl_to_rect(v, s.y);
-    gh_dop_init(F, tt, nt, rtol, atol, s, out);
+    gh_dop_init<GhForce<Spec>, !PILOT>(F, tt, nt, rtol, atol, s, out);
   };
   auto finish = [&](long o, int e) {
+    if (PILOT) {
+      cost[o] = (unsigned)(s.nacc + s.nrej);
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < 6; k++) yo[o * 6 + k] = s.y[k];  // galpy's y0 holds the last state
     err[o] = e;
@@ -393,7 +410,8 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
     }
   };
 
-  if (orbit < nobj) {
+  if (claim < nobj) {
+    orbit = order ? (long)order[claim] : claim;
     start(orbit);
     active = true;
   }
@@ -403,7 +421,7 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
       if (stop) {
         finish(orbit, -10);
         active = false;
-      } else if (gh_dop_step(F, tt, nt, rtol, atol, s, out)) {
+      } else if (gh_dop_step<GhForce<Spec>, !PILOT>(F, tt, nt, rtol, atol, s, out)) {
         finish(orbit, 0);
         active = false;
       }
@@ -416,13 +434,13 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
       if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
       base = __shfl(base, leader);
       if (want) {
-        const long o = (long)(base + (unsigned long long)__popcll(m & below));
-        if (o < nobj) {
-          orbit = o;
+        claim = (long)(base + (unsigned long long)__popcll(m & below));
+        if (claim < nobj) {
+          orbit = order ? (long)order[claim] : claim;
           if (stop) {
-            err[o] = -10;  // interrupted before it started
+            if (!PILOT) err[orbit] = -10;  // interrupted before it started
           } else {
-            start(o);
+            start(orbit);
             active = true;
           }
         } else {
@@ -432,6 +450,18 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
     }
     if (__all(!active && exhausted)) break;
   }
+}
+
+// Pilot time grid {t0, t0 + frac (t_end - t0)} for the cost estimate.
+__global__ void gh_pilot_grid(const double* __restrict__ t, int nt, double frac,
+                              double* __restrict__ pt) {
+  pt[0] = t[0];
+  pt[1] = t[0] + frac * (t[nt - 1] - t[0]);
+}
+
+__global__ void gh_iota(int n, int* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = i;
 }
 
 __global__ void gh_set_counter(unsigned long long* c, unsigned long long v) { *c = v; }
@@ -485,7 +515,6 @@ gh_eval_kernel(int n, const double* __restrict__ R, const double* __restrict__ z
 constexpr int kCounterSlots = 4096;
 unsigned long long* g_counters[64] = {nullptr};
 unsigned g_counter_next = 0;
-int g_dop_resident_blocks[64] = {0};
 
 unsigned long long* next_counter(int dev) {
   if (dev < 0 || dev >= 64) return nullptr;
@@ -498,13 +527,28 @@ unsigned long long* next_counter(int dev) {
   return g_counters[dev] + slot;
 }
 
+// Pilot length for the DOP853 cost estimate, as a fraction of the time
+// range: the pilot's step counts correlate with the full run's at ~0.99 for
+// MWPotential2014 orbits, for ~2% of the work.
+constexpr double kDopPilotFrac = 0.02;
+
+// GALPY_HIP_DOP_ORDER=0 turns the longest-first ordering off.
+bool dop_order_enabled() {
+  static const int v = [] {
+    const char* e = std::getenv("GALPY_HIP_DOP_ORDER");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  return v != 0;
+}
+
 // workgroups of gh_dop853_kernel<Spec> the device keeps resident at once
 template <class Spec>
 int dop_resident_blocks(int dev) {
   static int cached[64] = {0};
   if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
   int per_cu = 0, ncu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dop853_kernel<Spec>, kBlock, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dop853_kernel<Spec, false>, kBlock,
+                                                   0) !=
           hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
@@ -539,18 +583,61 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
     case 6: {
       int dev = 0;
       GH_CHECK(hipGetDevice(&dev));
-      unsigned long long* counter = next_counter(dev);
       const int resident = dop_resident_blocks<Spec>(dev);
-      if (counter == nullptr || resident <= 0) {
-        set_error("could not set up the DOP853 work counter on device %d", dev);
+      if (resident <= 0) {
+        set_error("could not query DOP853 occupancy on device %d", dev);
         return -1;
       }
       const unsigned nblocks = (unsigned)std::min<long>((long)grid.x, (long)resident);
+      // Longest-first order from a pilot run, when some lane gets more than
+      // one orbit (otherwise all orbits start at once and order is moot).
+      const int* d_order = nullptr;
+      void* work = nullptr;
+      if (!indiv_t && (long)nobj > (long)nblocks * kBlock && dop_order_enabled()) {
+        size_t sort_bytes = 0;
+        GH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(
+            nullptr, sort_bytes, (const unsigned*)nullptr, (unsigned*)nullptr, (const int*)nullptr,
+            (int*)nullptr, nobj, 0, 32, stream));
+        const size_t n4 = ((size_t)nobj * 4 + 255) / 256 * 256;
+        GH_CHECK(hipMallocAsync(&work, 4 * n4 + 256 + sort_bytes, stream));
+        char* w = static_cast<char*>(work);
+        unsigned* d_cost = reinterpret_cast<unsigned*>(w);
+        unsigned* d_cost_sorted = reinterpret_cast<unsigned*>(w + n4);
+        int* d_iota = reinterpret_cast<int*>(w + 2 * n4);
+        int* d_perm = reinterpret_cast<int*>(w + 3 * n4);
+        double* d_pt = reinterpret_cast<double*>(w + 4 * n4);
+        void* d_sort_tmp = w + 4 * n4 + 256;
+        unsigned long long* pcounter = next_counter(dev);
+        if (pcounter == nullptr) {
+          (void)hipFreeAsync(work, stream);
+          set_error("could not set up the DOP853 work counter on device %d", dev);
+          return -1;
+        }
+        hipLaunchKernelGGL(gh_pilot_grid, dim3(1), dim3(1), 0, stream, d_t, nt, kDopPilotFrac, d_pt);
+        hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, pcounter,
+                           (unsigned long long)nblocks * kBlock);
+        hipLaunchKernelGGL((gh_dop853_kernel<Spec, true>), dim3(nblocks), block, 0, stream, nobj,
+                           d_yo, 2, d_pt, 0, rtol, atol, d_table, nullptr, nullptr, nullptr, cancel,
+                           pcounter, nullptr, d_cost);
+        hipLaunchKernelGGL(gh_iota, dim3((unsigned)((nobj + 255) / 256)), dim3(256), 0, stream, nobj,
+                           d_iota);
+        GH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(d_sort_tmp, sort_bytes, d_cost,
+                                                              d_cost_sorted, d_iota, d_perm, nobj,
+                                                              0, 32, stream));
+        d_order = d_perm;
+      }
+      unsigned long long* counter = next_counter(dev);
+      if (counter == nullptr) {
+        if (work) (void)hipFreeAsync(work, stream);
+        set_error("could not set up the DOP853 work counter on device %d", dev);
+        return -1;
+      }
       hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter,
                          (unsigned long long)nblocks * kBlock);
-      hipLaunchKernelGGL(gh_dop853_kernel<Spec>, dim3(nblocks), block, 0, stream, nobj, d_yo, nt,
-                         d_t, indiv_t, rtol, atol, d_table, d_result, d_err, d_stats, cancel,
-                         counter);
+      hipLaunchKernelGGL((gh_dop853_kernel<Spec, false>), dim3(nblocks), block, 0, stream, nobj,
+                         d_yo, nt, d_t, indiv_t, rtol, atol, d_table, d_result, d_err, d_stats,
+                         cancel, counter, d_order, nullptr);
+      if (work) GH_CHECK(hipFreeAsync(work, stream));
       break;
     }
     default:

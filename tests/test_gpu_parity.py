@@ -77,6 +77,27 @@ def test_dop853_mw2014():
     assert phase_diff(res, ref) < 1e-8
 
 
+def test_dop853_longest_first_order_leaves_results_unchanged():
+    """More orbits than resident lanes turns on the pilot run and the
+    longest-first claim order; each orbit must still come out bit-identical
+    to the same orbit integrated in a call too small to be reordered."""
+    import torch
+
+    from galpy_amd.integrate import PotentialTable, integrate_device
+    from galpy_amd.potential import HernquistPotential, NFWPotential
+
+    n, split = 70000, 60000
+    t = torch.linspace(0.0, 3.0, 4, dtype=torch.float64, device="cuda")
+    for pot in (_mw(), [HernquistPotential(a=1.3), NFWPotential(a=3.0, amp=2.0)]):
+        table = PotentialTable(pot)
+        yo = torch.tensor(ics(n, seed=21), device="cuda")
+        whole, err, _ = integrate_device(table, yo.clone(), t, "dop853_hip")
+        parts = [integrate_device(table, yo[a:b].clone(), t, "dop853_hip")[0] for a, b in ((0, split), (split, n))]
+        torch.cuda.synchronize()
+        assert int((err != 0).sum()) == 0
+        assert torch.equal(whole, torch.cat(parts))
+
+
 def test_dop853_bar_jacobi():
     pot = bar_potential()
     yo = ics(128, seed=4)
