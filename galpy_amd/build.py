@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgalpy_hip.so")
 
 HIP_SOURCES = ["galpy_hip.hip"]
-HIP_HEADERS = ["potential_dev.h", "integrators_dev.h", "dop853_dev.h", "batch_dev.h", "adjoint_dev.h"]
+HIP_HEADERS = ["potential_dev.h", "integrators_dev.h", "dop853_dev.h", "dop853_rkn.h", "batch_dev.h",
+               "adjoint_dev.h", "gh_math.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -94,6 +94,8 @@ constexpr double d71 = -0.25693933462703749003312586129e+2, d76 = -0.15418974869
 constexpr double uround = 2.3e-16;
 }  // namespace ghdp
 
+#include "dop853_rkn.h"  // Nystrom-form sums over this tableau (generated)
+
 // RHS of the first-order system: (v, a(q)) -- evalRectDeriv,
 // integrateFullOrbit.c:1326-1351
 template <class F>
@@ -173,69 +175,34 @@ __device__ GH_DOP_RARE void gh_dop_init(const F& T,
   s.t_old_older = t[0];
 }
 
-// Dense output of an accepted step (leung_dop853.c:430-474): the continuous
-// extension coefficients, three extra stages and the interpolation at every
-// requested time inside the step.  Out of line: it runs on roughly one step
-// in seven, and inlining its three force evaluations would push the step
-// loop out of the instruction cache.  k2, k3, k10 are clobbered (the next
-// step recomputes them).
+// The step and the dense output are written in Nystrom form (dop853_rkn.h,
+// generated from the tableau above): the twelve stages keep only their
+// accelerations A[j] and form only their positions, since for y' = (p, a(q))
+// the stage velocities are linear in p0 and the A's.  This is the same method
+// as leung_dop853.c, with the position sums regrouped (same value in exact
+// arithmetic); it halves the registers the stages occupy.
+
+// Dense output of an accepted step (leung_dop853.c:430-474): three extra
+// stages, the continuous-extension coefficients and the interpolation at
+// every requested time inside the step.  It runs on roughly one accepted
+// step in seven.
 template <class F, bool STORE = true>
 __device__ GH_DOP_RARE void gh_dop_dense(const F& T, const double* __restrict__ t, int nt,
                                           GhDopState& s, double* __restrict__ out, double h,
-                                          const double* y0, const double* k1, double* k2,
-                                          double* k3, const double* k4, const double* k5,
-                                          const double* k6, const double* k7, const double* k8,
-                                          const double* k9, double* k10) {
+                                          const GhVec3& q0, const GhVec3& p0, const GhVec3& qn,
+                                          const GhVec3& pn, GhVec3* A) {
   using namespace ghdp;
-  double yy1[6];
-  double rc1[6], rc2[6], rc3[6], rc4[6], rc5[6], rc6[6], rc7[6], rc8[6];
-#pragma unroll
-  for (int i = 0; i < 6; i++) {
-    rc1[i] = y0[i];
-    const double ydiff = k5[i] - y0[i];
-    rc2[i] = ydiff;
-    const double bspl = h * k1[i] - ydiff;
-    rc3[i] = bspl;
-    rc4[i] = ydiff - h * k4[i] - bspl;
-    rc5[i] = d41 * k1[i] + d46 * k6[i] + d47 * k7[i] + d48 * k8[i] + d49 * k9[i] +
-             d410 * k10[i] + d411 * k2[i] + d412 * k3[i];
-    rc6[i] = d51 * k1[i] + d56 * k6[i] + d57 * k7[i] + d58 * k8[i] + d59 * k9[i] +
-             d510 * k10[i] + d511 * k2[i] + d512 * k3[i];
-    rc7[i] = d61 * k1[i] + d66 * k6[i] + d67 * k7[i] + d68 * k8[i] + d69 * k9[i] +
-             d610 * k10[i] + d611 * k2[i] + d612 * k3[i];
-    rc8[i] = d71 * k1[i] + d76 * k6[i] + d77 * k7[i] + d78 * k8[i] + d79 * k9[i] +
-             d710 * k10[i] + d711 * k2[i] + d712 * k3[i];
-  }
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-    yy1[i] = y0[i] + h * (a141 * k1[i] + a147 * k7[i] + a148 * k8[i] + a149 * k9[i] +
-                          a1410 * k10[i] + a1411 * k2[i] + a1412 * k3[i] + a1413 * k4[i]);
-  gh_deriv(T, s.t_old + c14 * h, yy1, k10);
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-    yy1[i] = y0[i] + h * (a151 * k1[i] + a156 * k6[i] + a157 * k7[i] + a158 * k8[i] +
-                          a1511 * k2[i] + a1512 * k3[i] + a1513 * k4[i] + a1514 * k10[i]);
-  gh_deriv(T, s.t_old + c15 * h, yy1, k2);
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-    yy1[i] = y0[i] + h * (a161 * k1[i] + a166 * k6[i] + a167 * k7[i] + a168 * k8[i] +
-                          a169 * k9[i] + a1613 * k4[i] + a1614 * k10[i] + a1615 * k2[i]);
-  gh_deriv(T, s.t_old + c16 * h, yy1, k3);
-#pragma unroll
-  for (int i = 0; i < 6; i++) {
-    rc5[i] = h * (rc5[i] + d413 * k4[i] + d414 * k10[i] + d415 * k2[i] + d416 * k3[i]);
-    rc6[i] = h * (rc6[i] + d513 * k4[i] + d514 * k10[i] + d515 * k2[i] + d516 * k3[i]);
-    rc7[i] = h * (rc7[i] + d613 * k4[i] + d614 * k10[i] + d615 * k2[i] + d616 * k3[i]);
-    rc8[i] = h * (rc8[i] + d713 * k4[i] + d714 * k10[i] + d715 * k2[i] + d716 * k3[i]);
-  }
+  A[14] = gh_accel(T, s.t_old + c14 * h, gh_rkn_q14(h, q0, p0, pn, A));
+  A[15] = gh_accel(T, s.t_old + c15 * h, gh_rkn_q15(h, q0, p0, pn, A));
+  A[16] = gh_accel(T, s.t_old + c16 * h, gh_rkn_q16(h, q0, p0, pn, A));
+  // the continuous extension is re-formed per requested time (a step holds
+  // one output time in the common case), so its 48 coefficients are never
+  // live at once
   while (s.done < nt - 1 && s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) {
     const double sd = (t[s.done + 1] - s.t_old) / h;
     const double s1 = 1.0 - sd;
     double yo[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++)
-      yo[i] = rc1[i] + sd * (rc2[i] + s1 * (rc3[i] + sd * (rc4[i] + s1 * (rc5[i] + sd * (rc6[i] +
-                          s1 * (rc7[i] + sd * rc8[i]))))));
+    gh_rkn_dense_eval(h, sd, s1, q0, p0, qn, pn, A, yo);
     s.done++;
     if (STORE) gh_store_cyl(out + 6L * s.done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
   }
@@ -252,77 +219,34 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
   const double safe = 0.9;
   const double expo1 = 1.0 / 8.0;
   const double facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;
-  double yy1[6], k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6];
-  const double* y0 = s.y;
-  const double* k1 = s.k1;
   const double h = s.pos_neg * fmax(fabs(s.h), 1e3 * uround);
   const double tc = s.t_cur;
+  const GhVec3 q0{s.y[0], s.y[1], s.y[2]}, p0{s.y[3], s.y[4], s.y[5]};
+  const GhVec3 none{0.0, 0.0, 0.0};
   // galpy's dop853 ignores the result of the twelve stages when rejected, so
   // galpy's accept/reject flow is kept exactly; only the accept branch is
   // masked per lane.  (galpy also tracks facold for Lund stabilisation, but
   // with beta = 0 pow(facold, beta) is exactly 1 and never enters a result.)
-#pragma unroll
-  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * a21 * k1[i];
-  gh_deriv(T, tc + c2 * h, yy1, k2);
-#pragma unroll
-  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a31 * k1[i] + a32 * k2[i]);
-  gh_deriv(T, tc + c3 * h, yy1, k3);
-#pragma unroll
-  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a41 * k1[i] + a43 * k3[i]);
-  gh_deriv(T, tc + c4 * h, yy1, k4);
-#pragma unroll
-  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a51 * k1[i] + a53 * k3[i] + a54 * k4[i]);
-  gh_deriv(T, tc + c5 * h, yy1, k5);
-#pragma unroll
-  for (int i = 0; i < 6; i++) yy1[i] = y0[i] + h * (a61 * k1[i] + a64 * k4[i] + a65 * k5[i]);
-  gh_deriv(T, tc + c6 * h, yy1, k6);
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-    yy1[i] = y0[i] + h * (a71 * k1[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]);
-  gh_deriv(T, tc + c7 * h, yy1, k7);
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-    yy1[i] = y0[i] + h * (a81 * k1[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] + a87 * k7[i]);
-  gh_deriv(T, tc + c8 * h, yy1, k8);
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-    yy1[i] = y0[i] + h * (a91 * k1[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] + a97 * k7[i] +
-                          a98 * k8[i]);
-  gh_deriv(T, tc + c9 * h, yy1, k9);
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-    yy1[i] = y0[i] + h * (a101 * k1[i] + a104 * k4[i] + a105 * k5[i] + a106 * k6[i] +
-                          a107 * k7[i] + a108 * k8[i] + a109 * k9[i]);
-  gh_deriv(T, tc + c10 * h, yy1, k10);
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-    yy1[i] = y0[i] + h * (a111 * k1[i] + a114 * k4[i] + a115 * k5[i] + a116 * k6[i] +
-                          a117 * k7[i] + a118 * k8[i] + a119 * k9[i] + a1110 * k10[i]);
-  gh_deriv(T, tc + c11 * h, yy1, k2);
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-    yy1[i] = y0[i] + h * (a121 * k1[i] + a124 * k4[i] + a125 * k5[i] + a126 * k6[i] +
-                          a127 * k7[i] + a128 * k8[i] + a129 * k9[i] + a1210 * k10[i] +
-                          a1211 * k2[i]);
+  GhVec3 A[17];
+  A[1] = GhVec3{s.k1[3], s.k1[4], s.k1[5]};
+  A[2] = gh_accel(T, tc + c2 * h, gh_rkn_q2(h, q0, p0, none, A));
+  A[3] = gh_accel(T, tc + c3 * h, gh_rkn_q3(h, q0, p0, none, A));
+  A[4] = gh_accel(T, tc + c4 * h, gh_rkn_q4(h, q0, p0, none, A));
+  A[5] = gh_accel(T, tc + c5 * h, gh_rkn_q5(h, q0, p0, none, A));
+  A[6] = gh_accel(T, tc + c6 * h, gh_rkn_q6(h, q0, p0, none, A));
+  A[7] = gh_accel(T, tc + c7 * h, gh_rkn_q7(h, q0, p0, none, A));
+  A[8] = gh_accel(T, tc + c8 * h, gh_rkn_q8(h, q0, p0, none, A));
+  A[9] = gh_accel(T, tc + c9 * h, gh_rkn_q9(h, q0, p0, none, A));
+  A[10] = gh_accel(T, tc + c10 * h, gh_rkn_q10(h, q0, p0, none, A));
+  A[11] = gh_accel(T, tc + c11 * h, gh_rkn_q11(h, q0, p0, none, A));
+  const GhVec3 q12 = gh_rkn_q12(h, q0, p0, none, A);
   s.t_old_older = s.t_old;
   s.t_old = tc;
   s.t_cur = tc + h;
-  gh_deriv(T, s.t_cur, yy1, k3);
-  double err5 = 0.0, err3 = 0.0;
-#pragma unroll
-  for (int i = 0; i < 6; i++) {
-    k4[i] = b1 * k1[i] + b6 * k6[i] + b7 * k7[i] + b8 * k8[i] + b9 * k9[i] + b10 * k10[i] +
-            b11 * k2[i] + b12 * k3[i];
-    k5[i] = y0[i] + h * k4[i];
-    const double sk = atol + rtol * fmax(fabs(y0[i]), fabs(k5[i]));
-    double erri = k4[i] - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i];
-    double sqr = erri / sk;
-    err3 += sqr * sqr;
-    erri = er1 * k1[i] + er6 * k6[i] + er7 * k7[i] + er8 * k8[i] + er9 * k9[i] + er10 * k10[i] +
-           er11 * k2[i] + er12 * k3[i];
-    sqr = erri / sk;
-    err5 += sqr * sqr;
-  }
+  A[12] = gh_accel(T, s.t_cur, q12);
+  GhVec3 qn, pn;
+  double err5, err3;
+  gh_rkn_solution(h, q0, p0, A, atol, rtol, qn, pn, err5, err3);
   double deno = err5 + 0.01 * err3;
   if (deno <= 0.0) deno = 1.0;
   const double errv = fabs(h) * err5 * sqrt(1.0 / (deno * 6.0));
@@ -331,17 +255,16 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
   double hnew = h / fac;
   if (errv <= 1.0) {
     s.nacc++;
-    gh_deriv(T, s.t_cur, k5, k4);
+    A[13] = gh_accel(T, s.t_cur, qn);  // FSAL: k13 = f(y_new)
     // dense output only when a requested time falls inside this step; the
-    // three extra stages feed nothing else (the next step recomputes k2, k3
-    // and k10), so they are skipped otherwise -- outputs are unchanged
+    // three extra stages feed nothing else, so they are skipped otherwise --
+    // outputs are unchanged
     if (s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur)
-      gh_dop_dense<F, STORE>(T, t, nt, s, out, h, y0, k1, k2, k3, k4, k5, k6, k7, k8, k9, k10);
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-      s.k1[i] = k4[i];
-      s.y[i] = k5[i];
-    }
+      gh_dop_dense<F, STORE>(T, t, nt, s, out, h, q0, p0, qn, pn, A);
+    s.y[0] = qn.x; s.y[1] = qn.y; s.y[2] = qn.z;
+    s.y[3] = pn.x; s.y[4] = pn.y; s.y[5] = pn.z;
+    s.k1[0] = pn.x; s.k1[1] = pn.y; s.k1[2] = pn.z;
+    s.k1[3] = A[13].x; s.k1[4] = A[13].y; s.k1[5] = A[13].z;
     hnew = (fabs(hnew) > fabs(s.hmax)) ? s.pos_neg * s.hmax : hnew;
     if (s.reject) hnew = s.pos_neg * fmin(fabs(hnew), fabs(h));
     s.reject = 0;

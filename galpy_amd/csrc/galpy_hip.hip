@@ -357,18 +357,15 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
 // only cost[o] = attempted steps and no output rows).  Every orbit is
 // integrated by the same code whatever the order, so results do not depend
 // on it.
-// Waves per SIMD the register allocator targets.  With the force specialised
-// per composition, one wave per SIMD with ~380 registers and no spills
-// measured fastest on MI355X (2.47 s vs 2.83 s at two waves with 512 B/lane
-// of spills, for 1M orbits of MWPotential2014); build with
-// -DGH_DOP_WAVES_PER_EU=2 to trade registers for occupancy.
-#ifndef GH_DOP_WAVES_PER_EU
-#define GH_DOP_WAVES_PER_EU 1
-#endif
-#define GH_DOP_ATTR __attribute__((amdgpu_waves_per_eu(GH_DOP_WAVES_PER_EU, GH_DOP_WAVES_PER_EU)))
-
-template <class Spec, bool PILOT>
-__global__ void __launch_bounds__(kBlock) GH_DOP_ATTR
+// Two instances per composition, built for 1 and 2 waves per SIMD (WPE):
+// at 2 waves the register allocator has 256 registers per lane and spills
+// only values used outside the stage loop (pointers, counters, the refill
+// and dense-output paths), and the second wave hides the first one's FP64
+// and memory latency -- 11% faster for 1M orbits of MWPotential2014 on
+// MI355X.  With few orbits per lane the longer tail of the doubled lane
+// count costs more than that, so the launcher picks WPE from the orbit count.
+template <class Spec, bool PILOT, int WPE>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
                  int indiv_t, double rtol_log, double atol_log, const GhPotTable* __restrict__ T,
                  double* __restrict__ result, int* __restrict__ err, double* __restrict__ stats,
@@ -541,14 +538,23 @@ bool dop_order_enabled() {
   return v != 0;
 }
 
+// GALPY_HIP_DOP_WAVES=1 or 2 forces the DOP853 occupancy variant.
+int dop_waves_forced() {
+  static const int v = [] {
+    const char* e = std::getenv("GALPY_HIP_DOP_WAVES");
+    return (e && (e[0] == '1' || e[0] == '2') && e[1] == 0) ? e[0] - '0' : 0;
+  }();
+  return v;
+}
+
 // workgroups of gh_dop853_kernel<Spec> the device keeps resident at once
-template <class Spec>
+template <class Spec, int WPE>
 int dop_resident_blocks(int dev) {
   static int cached[64] = {0};
   if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
   int per_cu = 0, ncu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dop853_kernel<Spec, false>, kBlock,
-                                                   0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dop853_kernel<Spec, false, WPE>,
+                                                   kBlock, 0) !=
           hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
@@ -564,6 +570,77 @@ struct GhPotHandle {
   int npot;
   int types[GH_MAX_POT];
 };
+
+// DOP853 launch at WPE waves per SIMD: optional pilot + longest-first
+// order, then the persistent kernel.  The pilot's cost keys and the order
+// live in one stream-ordered temporary allocation, freed after the launch.
+template <class Spec, int WPE>
+int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                  const GhPotTable* d_table, double rtol, double atol, double* d_result,
+                  int* d_err, double* d_stats, const int* cancel, int dev, hipStream_t stream) {
+  const int resident = dop_resident_blocks<Spec, WPE>(dev);
+  if (resident <= 0) {
+    set_error("could not query DOP853 occupancy on device %d", dev);
+    return -1;
+  }
+  const unsigned nblocks =
+      (unsigned)std::min<long>((long)((nobj + kBlock - 1) / kBlock), (long)resident);
+  const unsigned long long lanes = (unsigned long long)nblocks * kBlock;
+  unsigned long long* counter = next_counter(dev);
+  if (counter == nullptr) {
+    set_error("could not set up the DOP853 work counter on device %d", dev);
+    return -1;
+  }
+  // Longest-first order from a pilot run, when some lane gets more than one
+  // orbit (otherwise all orbits start at once and the order is moot).
+  const int* d_order = nullptr;
+  void* work = nullptr;
+  hipError_t e = hipSuccess;
+  if (!indiv_t && (unsigned long long)nobj > lanes && dop_order_enabled()) {
+    size_t sort_bytes = 0;
+    e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, (const unsigned*)nullptr,
+                                                     (unsigned*)nullptr, (const int*)nullptr,
+                                                     (int*)nullptr, nobj, 0, 32, stream);
+    const size_t n4 = ((size_t)nobj * 4 + 255) / 256 * 256;
+    if (e == hipSuccess) e = hipMallocAsync(&work, 4 * n4 + 256 + sort_bytes, stream);
+    if (e == hipSuccess) {
+      char* w = static_cast<char*>(work);
+      unsigned* d_cost = reinterpret_cast<unsigned*>(w);
+      unsigned* d_cost_sorted = reinterpret_cast<unsigned*>(w + n4);
+      int* d_iota = reinterpret_cast<int*>(w + 2 * n4);
+      int* d_perm = reinterpret_cast<int*>(w + 3 * n4);
+      double* d_pt = reinterpret_cast<double*>(w + 4 * n4);
+      void* d_sort_tmp = w + 4 * n4 + 256;
+      hipLaunchKernelGGL(gh_pilot_grid, dim3(1), dim3(1), 0, stream, d_t, nt, kDopPilotFrac, d_pt);
+      hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, lanes);
+      hipLaunchKernelGGL((gh_dop853_kernel<Spec, true, WPE>), dim3(nblocks), dim3(kBlock), 0,
+                         stream, nobj, d_yo, 2, d_pt, 0, rtol, atol, d_table, nullptr, nullptr,
+                         nullptr, cancel, counter, nullptr, d_cost);
+      hipLaunchKernelGGL(gh_iota, dim3((unsigned)((nobj + 255) / 256)), dim3(256), 0, stream,
+                         nobj, d_iota);
+      e = hipcub::DeviceRadixSort::SortPairsDescending(d_sort_tmp, sort_bytes, d_cost,
+                                                       d_cost_sorted, d_iota, d_perm, nobj, 0, 32,
+                                                       stream);
+      d_order = d_perm;
+    }
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, lanes);
+    hipLaunchKernelGGL((gh_dop853_kernel<Spec, false, WPE>), dim3(nblocks), dim3(kBlock), 0,
+                       stream, nobj, d_yo, nt, d_t, indiv_t, rtol, atol, d_table, d_result, d_err,
+                       d_stats, cancel, counter, d_order, nullptr);
+    e = hipGetLastError();
+  }
+  if (work) {
+    const hipError_t ef = hipFreeAsync(work, stream);
+    if (e == hipSuccess) e = ef;
+  }
+  if (e != hipSuccess) {
+    set_error("DOP853 launch failed: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
 
 template <class Spec>
 int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
@@ -583,62 +660,13 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
     case 6: {
       int dev = 0;
       GH_CHECK(hipGetDevice(&dev));
-      const int resident = dop_resident_blocks<Spec>(dev);
-      if (resident <= 0) {
-        set_error("could not query DOP853 occupancy on device %d", dev);
-        return -1;
-      }
-      const unsigned nblocks = (unsigned)std::min<long>((long)grid.x, (long)resident);
-      // Longest-first order from a pilot run, when some lane gets more than
-      // one orbit (otherwise all orbits start at once and order is moot).
-      const int* d_order = nullptr;
-      void* work = nullptr;
-      if (!indiv_t && (long)nobj > (long)nblocks * kBlock && dop_order_enabled()) {
-        size_t sort_bytes = 0;
-        GH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(
-            nullptr, sort_bytes, (const unsigned*)nullptr, (unsigned*)nullptr, (const int*)nullptr,
-            (int*)nullptr, nobj, 0, 32, stream));
-        const size_t n4 = ((size_t)nobj * 4 + 255) / 256 * 256;
-        GH_CHECK(hipMallocAsync(&work, 4 * n4 + 256 + sort_bytes, stream));
-        char* w = static_cast<char*>(work);
-        unsigned* d_cost = reinterpret_cast<unsigned*>(w);
-        unsigned* d_cost_sorted = reinterpret_cast<unsigned*>(w + n4);
-        int* d_iota = reinterpret_cast<int*>(w + 2 * n4);
-        int* d_perm = reinterpret_cast<int*>(w + 3 * n4);
-        double* d_pt = reinterpret_cast<double*>(w + 4 * n4);
-        void* d_sort_tmp = w + 4 * n4 + 256;
-        unsigned long long* pcounter = next_counter(dev);
-        if (pcounter == nullptr) {
-          (void)hipFreeAsync(work, stream);
-          set_error("could not set up the DOP853 work counter on device %d", dev);
-          return -1;
-        }
-        hipLaunchKernelGGL(gh_pilot_grid, dim3(1), dim3(1), 0, stream, d_t, nt, kDopPilotFrac, d_pt);
-        hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, pcounter,
-                           (unsigned long long)nblocks * kBlock);
-        hipLaunchKernelGGL((gh_dop853_kernel<Spec, true>), dim3(nblocks), block, 0, stream, nobj,
-                           d_yo, 2, d_pt, 0, rtol, atol, d_table, nullptr, nullptr, nullptr, cancel,
-                           pcounter, nullptr, d_cost);
-        hipLaunchKernelGGL(gh_iota, dim3((unsigned)((nobj + 255) / 256)), dim3(256), 0, stream, nobj,
-                           d_iota);
-        GH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(d_sort_tmp, sort_bytes, d_cost,
-                                                              d_cost_sorted, d_iota, d_perm, nobj,
-                                                              0, 32, stream));
-        d_order = d_perm;
-      }
-      unsigned long long* counter = next_counter(dev);
-      if (counter == nullptr) {
-        if (work) (void)hipFreeAsync(work, stream);
-        set_error("could not set up the DOP853 work counter on device %d", dev);
-        return -1;
-      }
-      hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter,
-                         (unsigned long long)nblocks * kBlock);
-      hipLaunchKernelGGL((gh_dop853_kernel<Spec, false>), dim3(nblocks), block, 0, stream, nobj,
-                         d_yo, nt, d_t, indiv_t, rtol, atol, d_table, d_result, d_err, d_stats,
-                         cancel, counter, d_order, nullptr);
-      if (work) GH_CHECK(hipFreeAsync(work, stream));
-      break;
+      const int r2 = dop_resident_blocks<Spec, 2>(dev);
+      const int forced = dop_waves_forced();
+      const bool two = forced ? forced == 2 : (long)nobj >= 6L * r2 * kBlock;
+      return two ? launch_dop853<Spec, 2>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
+                                          d_result, d_err, d_stats, cancel, dev, stream)
+                 : launch_dop853<Spec, 1>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
+                                          d_result, d_err, d_stats, cancel, dev, stream);
     }
     default:
       set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, 6 dop853)",

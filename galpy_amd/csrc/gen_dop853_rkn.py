@@ -1,0 +1,334 @@
+"""Generate dop853_rkn.h: DOP853 for second-order systems, in Nystrom form.
+
+For y = (q, p) with y' = (p, a(t, q)) every stage derivative is
+k_j = (V_j, A_j), where V_j is the velocity of the stage argument and
+A_j = a(t_j, Q_j).  With the tableau a_sj of DOP853 (Hairer, Norsett &
+Wanner's dop853.f, as used by galpy's leung_dop853.c:41-205):
+
+    V_s = p0 + h sum_j a_sj A_j
+    Q_s = q0 + h sum_j a_sj V_j
+        = q0 + h (QP_s p0 + h sum_l QA_sl A_l),
+      QP_s = sum_j a_sj,  QA_sl = sum_j a_sj a_jl.
+
+The same expansion applies to the solution weights b, the error weights
+(er and b - bhh) and the dense-output weights d.  Only the accelerations A_j
+(3 numbers per stage) then have to be kept, instead of the full k_j (6), which
+halves the register footprint of the kernel, and a stage argument needs only
+its positions Q_s (the force does not depend on V_s, and V_s enters every
+later sum through the expanded coefficients).  The velocity components of the
+solution, the error estimate and the dense output keep the reference's
+arithmetic term for term; the position components are the same sums in exact
+arithmetic, evaluated in a different order (rounding differs at the last
+bit).
+
+The stage that evaluates the new state (FSAL, stage 13: k13 = f(y_new)) has
+V_13 = p_new, which the kernels hold anyway, so it enters the position sums of
+the dense stages 14-16 and of the dense coefficients through p_new directly.
+
+The derived coefficients are exact rational functions of the tableau's
+double values (fractions.Fraction), rounded once to double.
+
+    python galpy_amd/csrc/gen_dop853_rkn.py          # rewrite dop853_rkn.h
+    python galpy_amd/csrc/gen_dop853_rkn.py --check  # exit 1 if out of date
+"""
+
+import os
+import re
+import sys
+from fractions import Fraction
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLEAU_SRC = os.path.join(HERE, "dop853_dev.h")
+OUT = os.path.join(HERE, "dop853_rkn.h")
+
+MAIN_STAGES = list(range(2, 13))  # stages 2..12 of a step
+DENSE_STAGES = [14, 15, 16]
+FSAL = 13
+
+
+def parse_tableau(path=TABLEAU_SRC):
+    """Constants of namespace ghdp as {name: float}."""
+    src = open(path).read()
+    ns = src[src.index("namespace ghdp {"):src.index("}  // namespace ghdp")]
+    vals = {}
+    for name, num in re.findall(r"\b([a-z]+[0-9]+)\s*=\s*([-+0-9.eE]+)", ns):
+        vals[name] = float(num)
+    return vals
+
+
+def _split(prefix, name, rows):
+    """a1410 -> (14, 10): the unique split into a valid (row, col)."""
+    digits = name[len(prefix):]
+    out = []
+    for k in range(1, len(digits)):
+        r, c = digits[:k], digits[k:]
+        if r.startswith("0") or c.startswith("0"):
+            continue
+        r, c = int(r), int(c)
+        if r in rows and 1 <= c < max(r, 17):
+            out.append((r, c))
+    if len(out) != 1:
+        raise ValueError(f"ambiguous coefficient name {name}: {out}")
+    return out[0]
+
+
+def tableau(vals):
+    """a[s][j], c[s], b[j], bhh{1,9,12}, er[j], d[i][j] (i = 4..7) as doubles."""
+    a, c, b, er, d = {}, {}, {}, {}, {}
+    for name, v in vals.items():
+        if re.fullmatch(r"a\d+", name):
+            s, j = _split("a", name, set(MAIN_STAGES + DENSE_STAGES))
+            a.setdefault(s, {})[j] = v
+        elif re.fullmatch(r"c\d+", name):
+            c[int(name[1:])] = v
+        elif re.fullmatch(r"b\d+", name):
+            b[int(name[1:])] = v
+        elif re.fullmatch(r"er\d+", name):
+            er[int(name[2:])] = v
+        elif re.fullmatch(r"d\d+", name):
+            i, j = _split("d", name, {4, 5, 6, 7})
+            d.setdefault(i, {})[j] = v
+    bhh = {1: vals["bhh1"], 9: vals["bhh2"], 12: vals["bhh3"]}
+    return dict(a=a, c=c, b=b, bhh=bhh, er=er, d=d, names=_names(vals))
+
+
+def _names(vals):
+    """(kind, row, col) -> C++ name in namespace ghdp."""
+    m = {}
+    for name in vals:
+        if re.fullmatch(r"a\d+", name):
+            m[("a",) + _split("a", name, set(MAIN_STAGES + DENSE_STAGES))] = name
+        elif re.fullmatch(r"b\d+", name):
+            m[("b", int(name[1:]))] = name
+        elif re.fullmatch(r"er\d+", name):
+            m[("er", int(name[2:]))] = name
+        elif re.fullmatch(r"d\d+", name):
+            m[("d",) + _split("d", name, {4, 5, 6, 7})] = name
+    m[("bhh", 1)], m[("bhh", 9)], m[("bhh", 12)] = "bhh1", "bhh2", "bhh3"
+    return m
+
+
+def nystrom(T):
+    """Position coefficients, exact (Fractions) before rounding.
+
+    Returns dict with, per stage s: QP[s] (of h p0), QN[s] (of h p_new; dense
+    stages only), QA[s][l] (of h^2 A_l); for the solution BP, BA; for the
+    errors E3P, E3A, E5P, E5A; for the dense rows DP[i], DN[i], DA[i][l]."""
+    F = Fraction
+    a = {s: {j: F(v) for j, v in row.items()} for s, row in T["a"].items()}
+    # velocity of stage j in terms of h A_l: V_j = p0 + h sum_l VA[j][l] A_l
+    VA = {1: {}}
+    for s in MAIN_STAGES + DENSE_STAGES:
+        VA[s] = dict(a[s])
+    # V_13 (= p_new) is a basis element of its own; it never needs expanding
+
+    def expand(row):
+        """sum_j w_j V_j  ->  (P, N, A): P p0 + N p_new + h sum_l A[l] A_l."""
+        P, N, A = F(0), F(0), {}
+        for j, w in row.items():
+            if j == FSAL:
+                N += w
+                continue
+            P += w
+            for l, v in VA[j].items():
+                A[l] = A.get(l, F(0)) + w * v
+        return P, N, {l: v for l, v in A.items() if v != 0}
+
+    out = {"QP": {}, "QN": {}, "QA": {}}
+    for s in MAIN_STAGES + DENSE_STAGES:
+        P, N, A = expand(a[s])
+        out["QP"][s], out["QN"][s], out["QA"][s] = P, N, A
+    b = {j: F(v) for j, v in T["b"].items()}
+    out["BP"], _, out["BA"] = expand(b)
+    w3 = dict(b)
+    for j, v in T["bhh"].items():
+        w3[j] = w3.get(j, F(0)) - F(v)
+    out["E3P"], _, out["E3A"] = expand(w3)
+    out["E5P"], _, out["E5A"] = expand({j: F(v) for j, v in T["er"].items()})
+    out["DP"], out["DN"], out["DA"] = {}, {}, {}
+    for i, row in T["d"].items():
+        out["DP"][i], out["DN"][i], out["DA"][i] = expand({j: F(v) for j, v in row.items()})
+    return out
+
+
+def _hex(x):
+    return float(x).hex()
+
+
+def _sum(terms):
+    """C++ sum of (coef, expr) in the given order."""
+    parts = [f"{c} * {e}" for c, e in terms]
+    return " + ".join(parts) if parts else "0.0"
+
+
+def emit(vals):
+    T = tableau(vals)
+    N = nystrom(T)
+    nm = T["names"]
+    L = []
+    w = L.append
+    w("// Generated by gen_dop853_rkn.py from the tableau in dop853_dev.h -- do not edit.")
+    w("// DOP853 in Nystrom form for y = (q, p), y' = (p, a): see the generator's")
+    w("// docstring for the derivation.  A[j] is the acceleration of stage j")
+    w("// (A[13] = a(q_new), the FSAL stage); q0, p0 the state at the start of the")
+    w("// step, pn the new velocity.")
+    w("#pragma once")
+    w('#include "integrators_dev.h"')
+    w("")
+    w("namespace ghrk {")
+    consts = []
+
+    def const(name, val):
+        consts.append(f"constexpr double {name} = {_hex(val)};")
+        return name
+
+    # position coefficients
+    names = {}
+    for s in MAIN_STAGES + DENSE_STAGES:
+        names[("QP", s)] = const(f"qp{s}", N["QP"][s])
+        if N["QN"][s] != 0:
+            names[("QN", s)] = const(f"qn{s}", N["QN"][s])
+        for l in sorted(N["QA"][s]):
+            names[("QA", s, l)] = const(f"qa{s}_{l}", N["QA"][s][l])
+    names["BP"] = const("bp", N["BP"])
+    for l in sorted(N["BA"]):
+        names[("BA", l)] = const(f"ba_{l}", N["BA"][l])
+    names["E3P"] = const("e3p", N["E3P"])
+    for l in sorted(N["E3A"]):
+        names[("E3A", l)] = const(f"e3a_{l}", N["E3A"][l])
+    names["E5P"] = const("e5p", N["E5P"])
+    for l in sorted(N["E5A"]):
+        names[("E5A", l)] = const(f"e5a_{l}", N["E5A"][l])
+    for i in sorted(N["DP"]):
+        names[("DP", i)] = const(f"dp{i}", N["DP"][i])
+        names[("DN", i)] = const(f"dn{i}", N["DN"][i])
+        for l in sorted(N["DA"][i]):
+            names[("DA", i, l)] = const(f"da{i}_{l}", N["DA"][i][l])
+    L.extend(consts)
+    w("}  // namespace ghrk")
+    w("")
+    C = ("x", "y", "z")
+
+    def ghdp(key):
+        return "ghdp::" + nm[key]
+
+    # stage arguments: positions only -- the stage velocities V_s enter every
+    # later sum through the expanded coefficients, so they are never formed
+    for s in MAIN_STAGES + DENSE_STAGES:
+        w(f"// position argument of stage {s}")
+        w(f"__device__ __forceinline__ GhVec3 gh_rkn_q{s}(double h, const GhVec3& q0, const GhVec3& p0,")
+        w("                                             const GhVec3& pn, const GhVec3* A) {")
+        w("  (void)pn;")
+        w("  GhVec3 Q;")
+        for cc in C:
+            qa = [(f"ghrk::{names[('QA', s, l)]}", f"A[{l}].{cc}") for l in sorted(N["QA"][s])]
+            lin = f"ghrk::{names[('QP', s)]} * p0.{cc}"
+            if ("QN", s) in names:
+                lin += f" + ghrk::{names[('QN', s)]} * pn.{cc}"
+            if qa:
+                w(f"  Q.{cc} = q0.{cc} + h * ({lin} + h * ({_sum(qa)}));")
+            else:
+                w(f"  Q.{cc} = q0.{cc} + h * ({lin});")
+        w("  return Q;")
+        w("}")
+        w("")
+
+    # solution and error norm
+    w("// New state (q_new, p_new) and the two error sums of leung_dop853.c:376-404:")
+    w("// per component, sk = atol + rtol max(|y0|, |y_new|), err3 += (e3/sk)^2,")
+    w("// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k.")
+    w("__device__ __forceinline__ void gh_rkn_solution(double h, const GhVec3& q0, const GhVec3& p0,")
+    w("                                               const GhVec3* A, double atol, double rtol,")
+    w("                                               GhVec3& qn, GhVec3& pn, double& err5,")
+    w("                                               double& err3) {")
+    w("  err5 = 0.0;")
+    w("  err3 = 0.0;")
+    bj = sorted(T["b"])
+    hj = sorted(T["bhh"])
+    ej = sorted(T["er"])
+    for cc in C:  # positions (components 0..2)
+        ba = [(f"ghrk::{names[('BA', l)]}", f"A[{l}].{cc}") for l in sorted(N["BA"])]
+        w("  {")
+        w(f"    const double kq = ghrk::bp * p0.{cc} + h * ({_sum(ba)});")
+        w(f"    qn.{cc} = q0.{cc} + h * kq;")
+        w(f"    const double sk = atol + rtol * fmax(fabs(q0.{cc}), fabs(qn.{cc}));")
+        e3 = [(f"ghrk::{names[('E3A', l)]}", f"A[{l}].{cc}") for l in sorted(N["E3A"])]
+        e5 = [(f"ghrk::{names[('E5A', l)]}", f"A[{l}].{cc}") for l in sorted(N["E5A"])]
+        w(f"    double sqr = (ghrk::e3p * p0.{cc} + h * ({_sum(e3)})) / sk;")
+        w("    err3 += sqr * sqr;")
+        w(f"    sqr = (ghrk::e5p * p0.{cc} + h * ({_sum(e5)})) / sk;")
+        w("    err5 += sqr * sqr;")
+        w("  }")
+    for cc in C:  # velocities (components 3..5): the reference's arithmetic
+        kb = [(ghdp(("b", j)), f"A[{j}].{cc}") for j in bj]
+        w("  {")
+        w(f"    const double kp = {_sum(kb)};")
+        w(f"    pn.{cc} = p0.{cc} + h * kp;")
+        w(f"    const double sk = atol + rtol * fmax(fabs(p0.{cc}), fabs(pn.{cc}));")
+        hh = " - ".join([f"{ghdp(('bhh', j))} * A[{j}].{cc}" for j in hj])
+        w(f"    double sqr = (kp - {hh}) / sk;")
+        w("    err3 += sqr * sqr;")
+        ee = [(ghdp(("er", j)), f"A[{j}].{cc}") for j in ej]
+        w(f"    sqr = ({_sum(ee)}) / sk;")
+        w("    err5 += sqr * sqr;")
+        w("  }")
+    w("}")
+    w("")
+
+    # dense output: the continuous extension evaluated at one time, one
+    # component at a time, so only 8 coefficients are live at once
+    w("// Dense output (leung_dop853.c:430-474) of the accepted step at the time")
+    w("// t_old + sd h (s1 = 1 - sd), after the dense stages 14-16: per component")
+    w("// the coefficients rc1..rc8 of the continuous extension, then")
+    w("// rc1 + sd (rc2 + s1 (rc3 + sd (rc4 + s1 (rc5 + sd (rc6 + s1 (rc7 + sd rc8)))))).")
+    w("__device__ __forceinline__ void gh_rkn_dense_eval(double h, double sd, double s1, const GhVec3& q0,")
+    w("                                                 const GhVec3& p0, const GhVec3& qn,")
+    w("                                                 const GhVec3& pn, const GhVec3* A, double* yo) {")
+
+    def poly(k):
+        return (f"    yo[{k}] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * "
+                f"(r6 + sd * r7))))));")
+
+    for k, cc in enumerate(C):
+        w("  {")
+        w(f"    const double ydiff = qn.{cc} - q0.{cc};")
+        w(f"    const double bspl = h * p0.{cc} - ydiff;")
+        w(f"    const double r0 = q0.{cc}, r1 = ydiff, r2 = bspl;")
+        w(f"    const double r3 = ydiff - h * pn.{cc} - bspl;")
+        for r, i in enumerate(sorted(N["DP"])):
+            da = [(f"ghrk::{names[('DA', i, l)]}", f"A[{l}].{cc}") for l in sorted(N["DA"][i])]
+            w(f"    const double r{4 + r} = h * (ghrk::{names[('DP', i)]} * p0.{cc} + ghrk::{names[('DN', i)]} * pn.{cc} + h * ({_sum(da)}));")
+        w(poly(k))
+        w("  }")
+    for k, cc in enumerate(C):
+        kk = k + 3
+        w("  {")
+        w(f"    const double ydiff = pn.{cc} - p0.{cc};")
+        w(f"    const double bspl = h * A[1].{cc} - ydiff;")
+        w(f"    const double r0 = p0.{cc}, r1 = ydiff, r2 = bspl;")
+        w(f"    const double r3 = ydiff - h * A[13].{cc} - bspl;")
+        for r, i in enumerate(sorted(T["d"])):
+            dd = [(ghdp(("d", i, j)), f"A[{j}].{cc}") for j in sorted(T["d"][i])]
+            w(f"    const double r{4 + r} = h * ({_sum(dd)});")
+        w(poly(kk))
+        w("  }")
+    w("}")
+    return "\n".join(L) + "\n"
+
+
+def main(argv):
+    text = emit(parse_tableau())
+    if "--check" in argv:
+        cur = open(OUT).read() if os.path.exists(OUT) else ""
+        if cur != text:
+            print("dop853_rkn.h is out of date; run gen_dop853_rkn.py")
+            return 1
+        return 0
+    with open(OUT, "w") as f:
+        f.write(text)
+    print("wrote", OUT)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
