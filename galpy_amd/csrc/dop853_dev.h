@@ -217,7 +217,6 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
                                             double* __restrict__ out) {
   using namespace ghdp;
   const double safe = 0.9;
-  const double expo1 = 1.0 / 8.0;
   const double facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;
   const double h = s.pos_neg * fmax(fabs(s.h), 1e3 * uround);
   const double tc = s.t_cur;
@@ -250,7 +249,8 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
   double deno = err5 + 0.01 * err3;
   if (deno <= 0.0) deno = 1.0;
   const double errv = fabs(h) * err5 * sqrt(1.0 / (deno * 6.0));
-  const double fac11 = pow(errv, expo1);
+  // errv^(1/8) (leung_dop853.c uses pow) as three square roots
+  const double fac11 = sqrt(sqrt(sqrt(errv)));
   const double fac = fmax(facc2, fmin(facc1, fac11 / safe));
   double hnew = h / fac;
   if (errv <= 1.0) {

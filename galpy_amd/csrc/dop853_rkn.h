@@ -352,7 +352,8 @@ __device__ __forceinline__ GhVec3 gh_rkn_q16(double h, const GhVec3& q0, const G
 
 // New state (q_new, p_new) and the two error sums of leung_dop853.c:376-404:
 // per component, sk = atol + rtol max(|y0|, |y_new|), err3 += (e3/sk)^2,
-// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k.
+// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k (one
+// reciprocal of sk per component; the norm only steers the step size).
 __device__ __forceinline__ void gh_rkn_solution(double h, const GhVec3& q0, const GhVec3& p0,
                                                const GhVec3* A, double atol, double rtol,
                                                GhVec3& qn, GhVec3& pn, double& err5,
@@ -362,55 +363,55 @@ __device__ __forceinline__ void gh_rkn_solution(double h, const GhVec3& q0, cons
   {
     const double kq = ghrk::bp * p0.x + h * (ghrk::ba_1 * A[1].x + ghrk::ba_4 * A[4].x + ghrk::ba_5 * A[5].x + ghrk::ba_6 * A[6].x + ghrk::ba_7 * A[7].x + ghrk::ba_8 * A[8].x + ghrk::ba_9 * A[9].x + ghrk::ba_10 * A[10].x + ghrk::ba_11 * A[11].x);
     qn.x = q0.x + h * kq;
-    const double sk = atol + rtol * fmax(fabs(q0.x), fabs(qn.x));
-    double sqr = (ghrk::e3p * p0.x + h * (ghrk::e3a_1 * A[1].x + ghrk::e3a_4 * A[4].x + ghrk::e3a_5 * A[5].x + ghrk::e3a_6 * A[6].x + ghrk::e3a_7 * A[7].x + ghrk::e3a_8 * A[8].x + ghrk::e3a_9 * A[9].x + ghrk::e3a_10 * A[10].x + ghrk::e3a_11 * A[11].x)) / sk;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.x), fabs(qn.x)));
+    double sqr = (ghrk::e3p * p0.x + h * (ghrk::e3a_1 * A[1].x + ghrk::e3a_4 * A[4].x + ghrk::e3a_5 * A[5].x + ghrk::e3a_6 * A[6].x + ghrk::e3a_7 * A[7].x + ghrk::e3a_8 * A[8].x + ghrk::e3a_9 * A[9].x + ghrk::e3a_10 * A[10].x + ghrk::e3a_11 * A[11].x)) * isk;
     err3 += sqr * sqr;
-    sqr = (ghrk::e5p * p0.x + h * (ghrk::e5a_1 * A[1].x + ghrk::e5a_4 * A[4].x + ghrk::e5a_5 * A[5].x + ghrk::e5a_6 * A[6].x + ghrk::e5a_7 * A[7].x + ghrk::e5a_8 * A[8].x + ghrk::e5a_9 * A[9].x + ghrk::e5a_10 * A[10].x + ghrk::e5a_11 * A[11].x)) / sk;
+    sqr = (ghrk::e5p * p0.x + h * (ghrk::e5a_1 * A[1].x + ghrk::e5a_4 * A[4].x + ghrk::e5a_5 * A[5].x + ghrk::e5a_6 * A[6].x + ghrk::e5a_7 * A[7].x + ghrk::e5a_8 * A[8].x + ghrk::e5a_9 * A[9].x + ghrk::e5a_10 * A[10].x + ghrk::e5a_11 * A[11].x)) * isk;
     err5 += sqr * sqr;
   }
   {
     const double kq = ghrk::bp * p0.y + h * (ghrk::ba_1 * A[1].y + ghrk::ba_4 * A[4].y + ghrk::ba_5 * A[5].y + ghrk::ba_6 * A[6].y + ghrk::ba_7 * A[7].y + ghrk::ba_8 * A[8].y + ghrk::ba_9 * A[9].y + ghrk::ba_10 * A[10].y + ghrk::ba_11 * A[11].y);
     qn.y = q0.y + h * kq;
-    const double sk = atol + rtol * fmax(fabs(q0.y), fabs(qn.y));
-    double sqr = (ghrk::e3p * p0.y + h * (ghrk::e3a_1 * A[1].y + ghrk::e3a_4 * A[4].y + ghrk::e3a_5 * A[5].y + ghrk::e3a_6 * A[6].y + ghrk::e3a_7 * A[7].y + ghrk::e3a_8 * A[8].y + ghrk::e3a_9 * A[9].y + ghrk::e3a_10 * A[10].y + ghrk::e3a_11 * A[11].y)) / sk;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.y), fabs(qn.y)));
+    double sqr = (ghrk::e3p * p0.y + h * (ghrk::e3a_1 * A[1].y + ghrk::e3a_4 * A[4].y + ghrk::e3a_5 * A[5].y + ghrk::e3a_6 * A[6].y + ghrk::e3a_7 * A[7].y + ghrk::e3a_8 * A[8].y + ghrk::e3a_9 * A[9].y + ghrk::e3a_10 * A[10].y + ghrk::e3a_11 * A[11].y)) * isk;
     err3 += sqr * sqr;
-    sqr = (ghrk::e5p * p0.y + h * (ghrk::e5a_1 * A[1].y + ghrk::e5a_4 * A[4].y + ghrk::e5a_5 * A[5].y + ghrk::e5a_6 * A[6].y + ghrk::e5a_7 * A[7].y + ghrk::e5a_8 * A[8].y + ghrk::e5a_9 * A[9].y + ghrk::e5a_10 * A[10].y + ghrk::e5a_11 * A[11].y)) / sk;
+    sqr = (ghrk::e5p * p0.y + h * (ghrk::e5a_1 * A[1].y + ghrk::e5a_4 * A[4].y + ghrk::e5a_5 * A[5].y + ghrk::e5a_6 * A[6].y + ghrk::e5a_7 * A[7].y + ghrk::e5a_8 * A[8].y + ghrk::e5a_9 * A[9].y + ghrk::e5a_10 * A[10].y + ghrk::e5a_11 * A[11].y)) * isk;
     err5 += sqr * sqr;
   }
   {
     const double kq = ghrk::bp * p0.z + h * (ghrk::ba_1 * A[1].z + ghrk::ba_4 * A[4].z + ghrk::ba_5 * A[5].z + ghrk::ba_6 * A[6].z + ghrk::ba_7 * A[7].z + ghrk::ba_8 * A[8].z + ghrk::ba_9 * A[9].z + ghrk::ba_10 * A[10].z + ghrk::ba_11 * A[11].z);
     qn.z = q0.z + h * kq;
-    const double sk = atol + rtol * fmax(fabs(q0.z), fabs(qn.z));
-    double sqr = (ghrk::e3p * p0.z + h * (ghrk::e3a_1 * A[1].z + ghrk::e3a_4 * A[4].z + ghrk::e3a_5 * A[5].z + ghrk::e3a_6 * A[6].z + ghrk::e3a_7 * A[7].z + ghrk::e3a_8 * A[8].z + ghrk::e3a_9 * A[9].z + ghrk::e3a_10 * A[10].z + ghrk::e3a_11 * A[11].z)) / sk;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.z), fabs(qn.z)));
+    double sqr = (ghrk::e3p * p0.z + h * (ghrk::e3a_1 * A[1].z + ghrk::e3a_4 * A[4].z + ghrk::e3a_5 * A[5].z + ghrk::e3a_6 * A[6].z + ghrk::e3a_7 * A[7].z + ghrk::e3a_8 * A[8].z + ghrk::e3a_9 * A[9].z + ghrk::e3a_10 * A[10].z + ghrk::e3a_11 * A[11].z)) * isk;
     err3 += sqr * sqr;
-    sqr = (ghrk::e5p * p0.z + h * (ghrk::e5a_1 * A[1].z + ghrk::e5a_4 * A[4].z + ghrk::e5a_5 * A[5].z + ghrk::e5a_6 * A[6].z + ghrk::e5a_7 * A[7].z + ghrk::e5a_8 * A[8].z + ghrk::e5a_9 * A[9].z + ghrk::e5a_10 * A[10].z + ghrk::e5a_11 * A[11].z)) / sk;
+    sqr = (ghrk::e5p * p0.z + h * (ghrk::e5a_1 * A[1].z + ghrk::e5a_4 * A[4].z + ghrk::e5a_5 * A[5].z + ghrk::e5a_6 * A[6].z + ghrk::e5a_7 * A[7].z + ghrk::e5a_8 * A[8].z + ghrk::e5a_9 * A[9].z + ghrk::e5a_10 * A[10].z + ghrk::e5a_11 * A[11].z)) * isk;
     err5 += sqr * sqr;
   }
   {
     const double kp = ghdp::b1 * A[1].x + ghdp::b6 * A[6].x + ghdp::b7 * A[7].x + ghdp::b8 * A[8].x + ghdp::b9 * A[9].x + ghdp::b10 * A[10].x + ghdp::b11 * A[11].x + ghdp::b12 * A[12].x;
     pn.x = p0.x + h * kp;
-    const double sk = atol + rtol * fmax(fabs(p0.x), fabs(pn.x));
-    double sqr = (kp - ghdp::bhh1 * A[1].x - ghdp::bhh2 * A[9].x - ghdp::bhh3 * A[12].x) / sk;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.x), fabs(pn.x)));
+    double sqr = (kp - ghdp::bhh1 * A[1].x - ghdp::bhh2 * A[9].x - ghdp::bhh3 * A[12].x) * isk;
     err3 += sqr * sqr;
-    sqr = (ghdp::er1 * A[1].x + ghdp::er6 * A[6].x + ghdp::er7 * A[7].x + ghdp::er8 * A[8].x + ghdp::er9 * A[9].x + ghdp::er10 * A[10].x + ghdp::er11 * A[11].x + ghdp::er12 * A[12].x) / sk;
+    sqr = (ghdp::er1 * A[1].x + ghdp::er6 * A[6].x + ghdp::er7 * A[7].x + ghdp::er8 * A[8].x + ghdp::er9 * A[9].x + ghdp::er10 * A[10].x + ghdp::er11 * A[11].x + ghdp::er12 * A[12].x) * isk;
     err5 += sqr * sqr;
   }
   {
     const double kp = ghdp::b1 * A[1].y + ghdp::b6 * A[6].y + ghdp::b7 * A[7].y + ghdp::b8 * A[8].y + ghdp::b9 * A[9].y + ghdp::b10 * A[10].y + ghdp::b11 * A[11].y + ghdp::b12 * A[12].y;
     pn.y = p0.y + h * kp;
-    const double sk = atol + rtol * fmax(fabs(p0.y), fabs(pn.y));
-    double sqr = (kp - ghdp::bhh1 * A[1].y - ghdp::bhh2 * A[9].y - ghdp::bhh3 * A[12].y) / sk;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.y), fabs(pn.y)));
+    double sqr = (kp - ghdp::bhh1 * A[1].y - ghdp::bhh2 * A[9].y - ghdp::bhh3 * A[12].y) * isk;
     err3 += sqr * sqr;
-    sqr = (ghdp::er1 * A[1].y + ghdp::er6 * A[6].y + ghdp::er7 * A[7].y + ghdp::er8 * A[8].y + ghdp::er9 * A[9].y + ghdp::er10 * A[10].y + ghdp::er11 * A[11].y + ghdp::er12 * A[12].y) / sk;
+    sqr = (ghdp::er1 * A[1].y + ghdp::er6 * A[6].y + ghdp::er7 * A[7].y + ghdp::er8 * A[8].y + ghdp::er9 * A[9].y + ghdp::er10 * A[10].y + ghdp::er11 * A[11].y + ghdp::er12 * A[12].y) * isk;
     err5 += sqr * sqr;
   }
   {
     const double kp = ghdp::b1 * A[1].z + ghdp::b6 * A[6].z + ghdp::b7 * A[7].z + ghdp::b8 * A[8].z + ghdp::b9 * A[9].z + ghdp::b10 * A[10].z + ghdp::b11 * A[11].z + ghdp::b12 * A[12].z;
     pn.z = p0.z + h * kp;
-    const double sk = atol + rtol * fmax(fabs(p0.z), fabs(pn.z));
-    double sqr = (kp - ghdp::bhh1 * A[1].z - ghdp::bhh2 * A[9].z - ghdp::bhh3 * A[12].z) / sk;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.z), fabs(pn.z)));
+    double sqr = (kp - ghdp::bhh1 * A[1].z - ghdp::bhh2 * A[9].z - ghdp::bhh3 * A[12].z) * isk;
     err3 += sqr * sqr;
-    sqr = (ghdp::er1 * A[1].z + ghdp::er6 * A[6].z + ghdp::er7 * A[7].z + ghdp::er8 * A[8].z + ghdp::er9 * A[9].z + ghdp::er10 * A[10].z + ghdp::er11 * A[11].z + ghdp::er12 * A[12].z) / sk;
+    sqr = (ghdp::er1 * A[1].z + ghdp::er6 * A[6].z + ghdp::er7 * A[7].z + ghdp::er8 * A[8].z + ghdp::er9 * A[9].z + ghdp::er10 * A[10].z + ghdp::er11 * A[11].z + ghdp::er12 * A[12].z) * isk;
     err5 += sqr * sqr;
   }
 }

@@ -236,7 +236,8 @@ def emit(vals):
     # solution and error norm
     w("// New state (q_new, p_new) and the two error sums of leung_dop853.c:376-404:")
     w("// per component, sk = atol + rtol max(|y0|, |y_new|), err3 += (e3/sk)^2,")
-    w("// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k.")
+    w("// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k (one")
+    w("// reciprocal of sk per component; the norm only steers the step size).")
     w("__device__ __forceinline__ void gh_rkn_solution(double h, const GhVec3& q0, const GhVec3& p0,")
     w("                                               const GhVec3* A, double atol, double rtol,")
     w("                                               GhVec3& qn, GhVec3& pn, double& err5,")
@@ -251,12 +252,12 @@ def emit(vals):
         w("  {")
         w(f"    const double kq = ghrk::bp * p0.{cc} + h * ({_sum(ba)});")
         w(f"    qn.{cc} = q0.{cc} + h * kq;")
-        w(f"    const double sk = atol + rtol * fmax(fabs(q0.{cc}), fabs(qn.{cc}));")
+        w(f"    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.{cc}), fabs(qn.{cc})));")
         e3 = [(f"ghrk::{names[('E3A', l)]}", f"A[{l}].{cc}") for l in sorted(N["E3A"])]
         e5 = [(f"ghrk::{names[('E5A', l)]}", f"A[{l}].{cc}") for l in sorted(N["E5A"])]
-        w(f"    double sqr = (ghrk::e3p * p0.{cc} + h * ({_sum(e3)})) / sk;")
+        w(f"    double sqr = (ghrk::e3p * p0.{cc} + h * ({_sum(e3)})) * isk;")
         w("    err3 += sqr * sqr;")
-        w(f"    sqr = (ghrk::e5p * p0.{cc} + h * ({_sum(e5)})) / sk;")
+        w(f"    sqr = (ghrk::e5p * p0.{cc} + h * ({_sum(e5)})) * isk;")
         w("    err5 += sqr * sqr;")
         w("  }")
     for cc in C:  # velocities (components 3..5): the reference's arithmetic
@@ -264,12 +265,12 @@ def emit(vals):
         w("  {")
         w(f"    const double kp = {_sum(kb)};")
         w(f"    pn.{cc} = p0.{cc} + h * kp;")
-        w(f"    const double sk = atol + rtol * fmax(fabs(p0.{cc}), fabs(pn.{cc}));")
+        w(f"    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.{cc}), fabs(pn.{cc})));")
         hh = " - ".join([f"{ghdp(('bhh', j))} * A[{j}].{cc}" for j in hj])
-        w(f"    double sqr = (kp - {hh}) / sk;")
+        w(f"    double sqr = (kp - {hh}) * isk;")
         w("    err3 += sqr * sqr;")
         ee = [(ghdp(("er", j)), f"A[{j}].{cc}") for j in ej]
-        w(f"    sqr = ({_sum(ee)}) / sk;")
+        w(f"    sqr = ({_sum(ee)}) * isk;")
         w("    err5 += sqr * sqr;")
         w("  }")
     w("}")

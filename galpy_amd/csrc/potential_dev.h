@@ -183,13 +183,40 @@ __device__ __forceinline__ void gh_bar_angle(double cos2p, double sin2p, double 
   sang = sin2p * cb - cos2p * sb;
 }
 
-struct GhCyl {
-  double FR, Fz, Fphi;
+// 1/sqrt(x) for positive normal x: the hardware estimate refined by one
+// Newton-Raphson step with its second-order term, y (1 + e/2 + 3e^2/8),
+// e = 1 - x y^2 -- the library rsqrt's arithmetic without its selects for
+// zero, infinite and denormal arguments (a force argument is never one).
+__device__ __forceinline__ double gh_rsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// 1/x for positive normal x: hardware estimate plus two Newton-Raphson steps
+// (within an ulp), instead of the correctly rounded division sequence.
+__device__ __forceinline__ double gh_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+// Forces are accumulated in the form the rectangular acceleration needs:
+// F_R / R, F_z and F_phi / R^2.  For every component here F_R carries a
+// factor R and F_phi a factor R^2 (or x y), so
+//   a_x = x (F_R/R) - y (F_phi/R^2),  a_y = y (F_R/R) + x (F_phi/R^2),
+// which is evalRectForce (integrateFullOrbit.c:1307-1325) without forming R,
+// phi, cos(phi) or sin(phi): no square root of x^2 + y^2, and 1/R^2 only for
+// the non-axisymmetric components.
+struct GhAcc {
+  double FRoR, Fz, FphioR2;
 };
 
 // Geometry shared by all components of one force evaluation.
 struct GhGeom {
-  double R, R2, z, z2, r2, rinv, r, cosphi, sinphi, t;
+  double x, y, z, R2, z2, r2, rinv, r, t;
 };
 
 // Force of one component of type TYPE (index i in the table), added to F.
@@ -199,109 +226,113 @@ struct GhGeom {
 // DehnenBarPotential.c:18-124.
 template <int TYPE>
 __device__ __forceinline__ void gh_comp_force(const GhPotTable* __restrict__ T, int i,
-                                              const GhGeom& g, GhCyl& F);
+                                              const GhGeom& g, GhAcc& F);
 
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_POWERSPHCUTOFF>(const GhPotTable* __restrict__ T,
-                                                                 int i, const GhGeom& g, GhCyl& F) {
+                                                                 int i, const GhGeom& g, GhAcc& F) {
   // F = -amp M(r2) / r^3 (R, z), M = 2 pi rc^(3-alpha) gamma(a, r2/rc^2)
   const GhComponent* __restrict__ C = &T->comp[i];
   const double x = g.r2 * C->c[3];
   const double gam = gh_lower_incgamma_tab(C, T->ser[i][0], &T->tab[(int)C->c[11]][0][0], x);
   const double f = -C->c[2] * gam * (g.rinv * g.rinv * g.rinv);
-  F.FR += f * g.R;
+  F.FRoR += f;
   F.Fz += f * g.z;
 }
 
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_MIYAMOTONAGAI>(const GhPotTable* __restrict__ T,
-                                                                int i, const GhGeom& g, GhCyl& F) {
+                                                                int i, const GhGeom& g, GhAcc& F) {
   const GhComponent* __restrict__ C = &T->comp[i];
   const double amp = C->p[0], a = C->p[1];
   const double s2 = g.z2 + C->c[0];
-  const double isbz = rsqrt(s2);
+  const double isbz = gh_rsqrt(s2);
   const double asqrtbz = a + s2 * isbz;
   const double D = g.R2 + asqrtbz * asqrtbz;
-  const double iD = rsqrt(D);
+  const double iD = gh_rsqrt(D);
   const double Dm32 = iD * iD * iD;
-  F.FR += -amp * g.R * Dm32;
+  F.FRoR += -amp * Dm32;
   F.Fz += (a == 0.0) ? -amp * g.z * Dm32 : -amp * g.z * asqrtbz * isbz * Dm32;
 }
 
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_NFW>(const GhPotTable* __restrict__ T, int i,
-                                                      const GhGeom& g, GhCyl& F) {
+                                                      const GhGeom& g, GhAcc& F) {
   const GhComponent* __restrict__ C = &T->comp[i];
   const double amp = C->p[0], a = C->p[1];
   const double rinv2 = g.rinv * g.rinv;
   // log(1 + r/a) as NFWPotential.c:26 forms it, with the short log of gh_math.h
-  const double br = rinv2 / (a + g.r) - gh_log_ge1(1.0 + g.r * C->c[0]) * (rinv2 * g.rinv);
-  F.FR += amp * g.R * br;
+  const double br = rinv2 * gh_rcp(a + g.r) - gh_log_ge1(1.0 + g.r * C->c[0]) * (rinv2 * g.rinv);
+  F.FRoR += amp * br;
   F.Fz += amp * g.z * br;
 }
 
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_HERNQUIST>(const GhPotTable* __restrict__ T, int i,
-                                                            const GhGeom& g, GhCyl& F) {
+                                                            const GhGeom& g, GhAcc& F) {
   // F = -amp x / (2 a^2 r) (1 + r/a)^-2 = -(amp/2) x / (r (a + r)^2)
   const GhComponent* __restrict__ C = &T->comp[i];
   const double apr = C->p[1] + g.r;
-  const double f = -C->c[0] * g.rinv / (apr * apr);
-  F.FR += f * g.R;
+  const double f = -C->c[0] * g.rinv * gh_rcp(apr * apr);
+  F.FRoR += f;
   F.Fz += f * g.z;
 }
 
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_LOGHALO>(const GhPotTable* __restrict__ T, int i,
-                                                          const GhGeom& g, GhCyl& F) {
+                                                          const GhGeom& g, GhAcc& F) {
   const GhComponent* __restrict__ C = &T->comp[i];
   const double amp = C->p[0], core2 = C->p[2], onem1overb2 = C->p[3];
   const double zq2 = g.z2 * C->c[0];  // (z/q)^2
   if (onem1overb2 < 1.0) {
-    const double Rt2 = g.R2 * (1.0 - onem1overb2 * g.sinphi * g.sinphi);
-    const double inv = 1.0 / (Rt2 + zq2 + core2);
-    F.FR += -amp * Rt2 / g.R * inv;
+    // R_t^2 = R^2 (1 - (1 - 1/b^2) sin^2 phi) = R^2 - (1 - 1/b^2) y^2
+    const double Rt2 = g.R2 - onem1overb2 * g.y * g.y;
+    const double inv = gh_rcp(Rt2 + zq2 + core2);
+    const double iR2 = gh_rcp(g.R2);
+    F.FRoR += -amp * Rt2 * inv * iR2;
     F.Fz += -amp * g.z * C->c[0] * inv;
-    F.Fphi += amp * g.R2 * inv * (2.0 * g.sinphi * g.cosphi) * onem1overb2 * 0.5;
+    // F_phi = amp R^2 inv sin(phi) cos(phi) (1 - 1/b^2) = amp inv x y (1 - 1/b^2)
+    F.FphioR2 += amp * inv * g.x * g.y * onem1overb2 * iR2;
   } else {
-    const double inv = 1.0 / (g.R2 + zq2 + core2);
-    F.FR += -amp * g.R * inv;
+    const double inv = gh_rcp(g.R2 + zq2 + core2);
+    F.FRoR += -amp * inv;
     F.Fz += -amp * g.z * C->c[0] * inv;
   }
 }
 
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_DEHNENBAR>(const GhPotTable* __restrict__ T, int i,
-                                                            const GhGeom& g, GhCyl& F) {
+                                                            const GhGeom& g, GhAcc& F) {
   // args amp*Af, tform, tsteady, rb, omegab, barphi
   const GhComponent* __restrict__ C = &T->comp[i];
   const double amp = C->p[0], rb = C->p[3];
   const double smooth = gh_bar_smooth(g.t, C->p[1], C->p[2]);
-  const double cos2p = g.cosphi * g.cosphi - g.sinphi * g.sinphi;
-  const double sin2p = 2.0 * g.sinphi * g.cosphi;
+  const double iR2 = gh_rcp(g.R2);
+  const double cos2p = (g.x * g.x - g.y * g.y) * iR2;  // cos(2 phi)
+  const double sin2p = 2.0 * g.x * g.y * iR2;          // sin(2 phi)
   double cang, sang;
   gh_bar_angle(cos2p, sin2p, C->p[4], C->p[5], g.t, cang, sang);
-  const double r4inv = g.rinv * g.rinv * g.rinv * g.rinv;
   const double r2inv = g.rinv * g.rinv;
+  const double r4inv = r2inv * r2inv;
   const double as = amp * smooth;
   if (g.r <= rb) {
     const double u = g.r * C->c[0];  // r/rb
     const double u3 = u * u * u;
-    F.FR += -as * cang * (u3 * g.R * (3.0 * g.R2 + 2.0 * g.z2) - 4.0 * g.R * g.z2) * r4inv;
-    F.Fphi += 2.0 * as * sang * (u3 - 2.0) * g.R2 * r2inv;
+    F.FRoR += -as * cang * (u3 * (3.0 * g.R2 + 2.0 * g.z2) - 4.0 * g.z2) * r4inv;
+    F.FphioR2 += 2.0 * as * sang * (u3 - 2.0) * r2inv;
     F.Fz += -as * cang * (u3 + 4.0) * g.R2 * g.z * r4inv;
   } else {
     const double u = rb * g.rinv;
     const double u3 = u * u * u;
-    F.FR += -as * cang * u3 * g.R * r4inv * (3.0 * g.R2 - 2.0 * g.z2);
-    F.Fphi += -2.0 * as * sang * u3 * g.R2 * r2inv;
+    F.FRoR += -as * cang * u3 * r4inv * (3.0 * g.R2 - 2.0 * g.z2);
+    F.FphioR2 += -2.0 * as * sang * u3 * r2inv;
     F.Fz += -5.0 * as * cang * u3 * g.R2 * g.z * r4inv;
   }
 }
 
 // run-time dispatch on the component type (generic kernels)
 __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__ T, int i,
-                                                  const GhGeom& g, GhCyl& F) {
+                                                  const GhGeom& g, GhAcc& F) {
   switch (T->comp[i].type) {
     case GH_POWERSPHCUTOFF: gh_comp_force<GH_POWERSPHCUTOFF>(T, i, g, F); break;
     case GH_MIYAMOTONAGAI: gh_comp_force<GH_MIYAMOTONAGAI>(T, i, g, F); break;
@@ -321,46 +352,57 @@ template <int... Codes>
 struct GhSpec {};
 
 template <class Spec>
-struct GhCylSum;
+struct GhAccSum;
 
 template <>
-struct GhCylSum<GhSpec<>> {
+struct GhAccSum<GhSpec<>> {
   __device__ static __forceinline__ void add(const GhPotTable* __restrict__ T, const GhGeom& g,
-                                             GhCyl& F) {
+                                             GhAcc& F) {
     const int npot = T->npot;
     for (int i = 0; i < npot; i++) gh_comp_force_dyn(T, i, g, F);
   }
 };
 
 template <int... Codes>
-struct GhCylSum<GhSpec<Codes...>> {
+struct GhAccSum<GhSpec<Codes...>> {
   __device__ static __forceinline__ void add(const GhPotTable* __restrict__ T, const GhGeom& g,
-                                             GhCyl& F) {
+                                             GhAcc& F) {
     int i = 0;
     (gh_comp_force<Codes>(T, i++, g, F), ...);  // component order = summation order
   }
 };
 
-// Sum of cylindrical forces over components in index order
-// (galpy_potentials.c:120-179).  cosphi/sinphi = x/R, y/R.
+// Sum over components in index order (galpy_potentials.c:120-179), from the
+// rectangular position.
+template <class Spec = GhSpec<>>
+__device__ __forceinline__ GhAcc gh_acc_forces(const GhPotTable* __restrict__ T, double x,
+                                               double y, double z, double t) {
+  GhGeom g;
+  g.x = x;
+  g.y = y;
+  g.z = z;
+  g.R2 = x * x + y * y;
+  g.z2 = z * z;
+  g.r2 = g.R2 + g.z2;
+  g.rinv = gh_rsqrt(g.r2);  // shared by the spherical components
+  g.r = g.r2 * g.rinv;
+  g.t = t;
+  GhAcc F = {0.0, 0.0, 0.0};
+  GhAccSum<Spec>::add(T, g, F);
+  return F;
+}
+
+// Cylindrical forces (F_R, F_z, F_phi) at (R, z, phi); cosphi/sinphi = x/R, y/R.
+struct GhCyl {
+  double FR, Fz, Fphi;
+};
+
 template <class Spec = GhSpec<>>
 __device__ __forceinline__ GhCyl gh_cyl_forces(const GhPotTable* __restrict__ T, double R,
                                                double R2, double z, double cosphi,
                                                double sinphi, double t) {
-  GhGeom g;
-  g.R = R;
-  g.R2 = R2;
-  g.z = z;
-  g.z2 = z * z;
-  g.r2 = R2 + g.z2;
-  g.rinv = rsqrt(g.r2);  // shared by the spherical components
-  g.r = g.r2 * g.rinv;
-  g.cosphi = cosphi;
-  g.sinphi = sinphi;
-  g.t = t;
-  GhCyl F = {0.0, 0.0, 0.0};
-  GhCylSum<Spec>::add(T, g, F);
-  return F;
+  const GhAcc F = gh_acc_forces<Spec>(T, R * cosphi, R * sinphi, z, t);
+  return GhCyl{R * F.FRoR, F.Fz, R2 * F.FphioR2};
 }
 
 // evalRectForce (integrateFullOrbit.c:1307-1325): rectangular acceleration.
@@ -368,14 +410,9 @@ template <class Spec = GhSpec<>>
 __device__ __forceinline__ void gh_rect_force(const GhPotTable* __restrict__ T, double t,
                                               double x, double y, double z, double& ax,
                                               double& ay, double& az) {
-  const double R2 = x * x + y * y;
-  const double invR = rsqrt(R2);
-  const double R = R2 * invR;
-  const double cosphi = x * invR;
-  const double sinphi = y * invR;
-  const GhCyl F = gh_cyl_forces<Spec>(T, R, R2, z, cosphi, sinphi, t);
-  ax = cosphi * F.FR - invR * sinphi * F.Fphi;
-  ay = sinphi * F.FR + invR * cosphi * F.Fphi;
+  const GhAcc F = gh_acc_forces<Spec>(T, x, y, z, t);
+  ax = x * F.FRoR - y * F.FphioR2;
+  ay = y * F.FRoR + x * F.FphioR2;
   az = F.Fz;
 }
 
