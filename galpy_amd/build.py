@@ -31,6 +31,12 @@ HIPCC_FLAGS = [
     # no packed-FP32 instructions anywhere in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-fvisibility=hidden",
+    # Machine-level LICM hoists every FP64 literal of the integrators' loops
+    # (tableau and polynomial coefficients, 2 SGPRs or VGPRs each) out of
+    # the loop, where at two waves per SIMD they no longer fit the register
+    # budget and spill to scratch; materialised at each use they cost only
+    # scalar moves (DOP853 kernel: 381 spilled VGPRs -> none).
+    "-mllvm", "-disable-machine-licm",
 ]
 
 

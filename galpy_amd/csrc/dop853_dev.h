@@ -182,39 +182,73 @@ __device__ GH_DOP_RARE void gh_dop_init(const F& T,
 // as leung_dop853.c, with the position sums regrouped (same value in exact
 // arithmetic); it halves the registers the stages occupy.
 
-// Dense output of an accepted step (leung_dop853.c:430-474): three extra
-// stages, the continuous-extension coefficients and the interpolation at
-// every requested time inside the step.  It runs on roughly one accepted
-// step in seven.
+// What the dense output of one accepted step needs (leung_dop853.c:430-474):
+// the step's start and end states, the accelerations of the stages the
+// continuous extension uses (A1, A4..A13; A[j] of stage j, A13 = FSAL), the
+// step's start time and size, and the number of output rows written before
+// it.  A lane can hand this record to another lane (deferred dense output).
+struct GhDenseJob {
+  GhVec3 q0, p0, qn, pn;
+  GhVec3 A[11];  // stages 1, 4, 5, ..., 13
+  double t_old, h;
+  int done0;
+};
+
+// Dense output of one accepted step: the three extra stages 14-16 and the
+// continuous extension at every requested time inside the step, written as
+// rows done0+1, done0+2, ...  Returns the number of rows written.
 template <class F, bool STORE = true>
-__device__ GH_DOP_RARE void gh_dop_dense(const F& T, const double* __restrict__ t, int nt,
-                                          GhDopState& s, double* __restrict__ out, double h,
-                                          const GhVec3& q0, const GhVec3& p0, const GhVec3& qn,
-                                          const GhVec3& pn, GhVec3* A) {
+__device__ GH_DOP_RARE int gh_dop_dense(const F& T, const double* __restrict__ t, int nt,
+                                         const GhDenseJob& J, double* __restrict__ out) {
   using namespace ghdp;
-  A[14] = gh_accel(T, s.t_old + c14 * h, gh_rkn_q14(h, q0, p0, pn, A));
-  A[15] = gh_accel(T, s.t_old + c15 * h, gh_rkn_q15(h, q0, p0, pn, A));
-  A[16] = gh_accel(T, s.t_old + c16 * h, gh_rkn_q16(h, q0, p0, pn, A));
+  const double h = J.h;
+  GhVec3 A[17];
+  A[1] = J.A[0];
+#pragma unroll
+  for (int j = 4; j <= 13; j++) A[j] = J.A[j - 3];
+  A[2] = A[3] = GhVec3{0.0, 0.0, 0.0};  // unused by stages 14-16 and the extension
+  A[14] = gh_accel(T, J.t_old + c14 * h, gh_rkn_q14(h, J.q0, J.p0, J.pn, A));
+  A[15] = gh_accel(T, J.t_old + c15 * h, gh_rkn_q15(h, J.q0, J.p0, J.pn, A));
+  A[16] = gh_accel(T, J.t_old + c16 * h, gh_rkn_q16(h, J.q0, J.p0, J.pn, A));
   // the continuous extension is re-formed per requested time (a step holds
   // one output time in the common case), so its 48 coefficients are never
   // live at once
-  while (s.done < nt - 1 && s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) {
-    const double sd = (t[s.done + 1] - s.t_old) / h;
+  const double pos_neg = (h > 0.0) ? 1.0 : -1.0;
+  const double t_cur = J.t_old + h;
+  int done = J.done0;
+  while (done < nt - 1 && pos_neg * t[done + 1] < pos_neg * t_cur) {
+    const double sd = (t[done + 1] - J.t_old) / h;
     const double s1 = 1.0 - sd;
     double yo[6];
-    gh_rkn_dense_eval(h, sd, s1, q0, p0, qn, pn, A, yo);
-    s.done++;
-    if (STORE) gh_store_cyl(out + 6L * s.done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
+    gh_rkn_dense_eval(h, sd, s1, J.q0, J.p0, J.qn, J.pn, A, yo);
+    done++;
+    if (STORE) gh_store_cyl(out + 6L * done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
   }
+  return done - J.done0;
 }
 
+// Dense-output handling of gh_dop_step.
+enum GhDenseMode {
+  GH_DENSE_SKIP = 0,    // count the passed output times only (pilot runs)
+  GH_DENSE_INLINE = 1,  // compute them in this lane now
+  GH_DENSE_DEFER = 2    // hand the step's GhDenseJob to sink(J) and count them
+};
+
+struct GhNoSink {
+  __device__ void operator()(const GhDenseJob&) const {}
+};
+
 // One attempted step (leung_dop853.c:335-492); returns true once every
-// requested time has been written.
-template <class F, bool STORE = true>
+// requested time has been handled.  With GH_DENSE_DEFER, sink(J) is called
+// (from inside the accept branch, by exactly the lanes whose step passed
+// output times) as soon as the job is formed, and s.done already counts its
+// rows.
+template <class F, int DENSE = GH_DENSE_INLINE, class Sink = GhNoSink>
 __device__ __forceinline__ bool gh_dop_step(const F& T,
                                             const double* __restrict__ t, int nt, double rtol,
                                             double atol, GhDopState& s,
-                                            double* __restrict__ out) {
+                                            double* __restrict__ out,
+                                            const Sink& sink = Sink()) {
   using namespace ghdp;
   const double safe = 0.9;
   const double facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;
@@ -259,8 +293,29 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
     // dense output only when a requested time falls inside this step; the
     // three extra stages feed nothing else, so they are skipped otherwise --
     // outputs are unchanged
-    if (s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur)
-      gh_dop_dense<F, STORE>(T, t, nt, s, out, h, q0, p0, qn, pn, A);
+    if (s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) {
+      if (DENSE == GH_DENSE_SKIP) {
+        while (s.done < nt - 1 && s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) s.done++;
+      } else {
+        GhDenseJob J;
+        J.q0 = q0;
+        J.p0 = p0;
+        J.qn = qn;
+        J.pn = pn;
+        J.A[0] = A[1];
+#pragma unroll
+        for (int j = 4; j <= 13; j++) J.A[j - 3] = A[j];
+        J.t_old = tc;
+        J.h = h;
+        J.done0 = s.done;
+        if (DENSE == GH_DENSE_INLINE) {
+          s.done += gh_dop_dense<F, true>(T, t, nt, J, out);
+        } else {
+          sink(J);
+          while (s.done < nt - 1 && s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) s.done++;
+        }
+      }
+    }
     s.y[0] = qn.x; s.y[1] = qn.y; s.y[2] = qn.z;
     s.y[3] = pn.x; s.y[4] = pn.y; s.y[5] = pn.z;
     s.k1[0] = pn.x; s.k1[1] = pn.y; s.k1[2] = pn.z;

@@ -364,13 +364,194 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
 // and memory latency -- 11% faster for 1M orbits of MWPotential2014 on
 // MI355X.  With few orbits per lane the longer tail of the doubled lane
 // count costs more than that, so the launcher picks WPE from the orbit count.
+// Deferred dense output.  A lane whose accepted step passes output times
+// stores the step's GhDenseJob in its wavefront's LDS queue and steps on;
+// once the queue holds kDenseFlush jobs the whole wavefront computes them,
+// one job per lane.  Done in line, the three extra force evaluations of the
+// dense output run for nearly every wavefront iteration (some lane almost
+// always needs them) with ~1 lane in 8 active.  The queue is a structure of
+// arrays (field k of slot j at k * kDenseCap + j, so a wavefront's stores hit
+// consecutive banks); 47 doubles + 2 ints per job, 52 jobs = 19,968 bytes per
+// wavefront, so 8 single-wavefront workgroups still fit a CU's 160 KB LDS.
+constexpr int kDenseCap = 52;
+constexpr int kDenseFlush = 44;
+constexpr int kJobDoubles = 47;
+
+template <class D, class I>
+__device__ __forceinline__ void gh_job_store(D* __restrict__ qd, I* __restrict__ qi, int cap,
+                                             int slot, const GhDenseJob& J, int orbit) {
+  double v[kJobDoubles];
+  const GhVec3 g[4] = {J.q0, J.p0, J.qn, J.pn};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    v[3 * k] = g[k].x;
+    v[3 * k + 1] = g[k].y;
+    v[3 * k + 2] = g[k].z;
+  }
+#pragma unroll
+  for (int k = 0; k < 11; k++) {
+    v[12 + 3 * k] = J.A[k].x;
+    v[13 + 3 * k] = J.A[k].y;
+    v[14 + 3 * k] = J.A[k].z;
+  }
+  v[45] = J.t_old;
+  v[46] = J.h;
+#pragma unroll
+  for (int k = 0; k < kJobDoubles; k++) qd[k * cap + slot] = v[k];
+  qi[slot] = J.done0;
+  qi[cap + slot] = orbit;
+}
+
+__device__ __forceinline__ void gh_job_load(const double* __restrict__ qd,
+                                            const int* __restrict__ qi, int slot, GhDenseJob& J,
+                                            int& orbit) {
+  double v[kJobDoubles];
+#pragma unroll
+  for (int k = 0; k < kJobDoubles; k++) v[k] = qd[k * kDenseCap + slot];
+  J.q0 = GhVec3{v[0], v[1], v[2]};
+  J.p0 = GhVec3{v[3], v[4], v[5]};
+  J.qn = GhVec3{v[6], v[7], v[8]};
+  J.pn = GhVec3{v[9], v[10], v[11]};
+#pragma unroll
+  for (int k = 0; k < 11; k++) J.A[k] = GhVec3{v[12 + 3 * k], v[13 + 3 * k], v[14 + 3 * k]};
+  J.t_old = v[45];
+  J.h = v[46];
+  J.done0 = qi[slot];
+  orbit = qi[kDenseCap + slot];
+}
+
+// Jobs that did not fit the LDS queue wait in a per-wavefront area of device
+// memory (kOverflowCap slots) and move into the queue after a flush.  Each
+// overflow job is contiguous (47 doubles, then done0 and the orbit as two
+// ints), so one address plus immediate offsets reaches all its fields;
+// volatile so the copy reads see the wavefront's own earlier stores.
+constexpr int kOverflowCap = 64;
+constexpr int kJobWords = kJobDoubles + 1;  // doubles per overflow slot
+constexpr size_t kOverflowBytes = (size_t)kOverflowCap * kJobWords * 8;
+
+__device__ __forceinline__ void gh_job_store_ovf(volatile double* od, int slot,
+                                                 const GhDenseJob& J, int orbit) {
+  volatile double* p = od + (long)slot * kJobWords;
+  p[0] = J.q0.x; p[1] = J.q0.y; p[2] = J.q0.z;
+  p[3] = J.p0.x; p[4] = J.p0.y; p[5] = J.p0.z;
+  p[6] = J.qn.x; p[7] = J.qn.y; p[8] = J.qn.z;
+  p[9] = J.pn.x; p[10] = J.pn.y; p[11] = J.pn.z;
+#pragma unroll
+  for (int k = 0; k < 11; k++) {
+    p[12 + 3 * k] = J.A[k].x;
+    p[13 + 3 * k] = J.A[k].y;
+    p[14 + 3 * k] = J.A[k].z;
+  }
+  p[45] = J.t_old;
+  p[46] = J.h;
+  volatile int* pi = reinterpret_cast<volatile int*>(p + kJobDoubles);
+  pi[0] = J.done0;
+  pi[1] = orbit;
+}
+
+__device__ __forceinline__ void gh_job_copy_in(const volatile double* od, int from,
+                                               double* __restrict__ qd, int* __restrict__ qi,
+                                               int slot) {
+  const volatile double* p = od + (long)from * kJobWords;
+#pragma unroll 8
+  for (int k = 0; k < kJobDoubles; k++) qd[k * kDenseCap + slot] = p[k];
+  const volatile int* pi = reinterpret_cast<const volatile int*>(p + kJobDoubles);
+  qi[slot] = pi[0];
+  qi[kDenseCap + slot] = pi[1];
+}
+
+// Loads the accelerations A1, A4..A13 of job `slot` into A[1], A[4..13].
+__device__ __forceinline__ void gh_job_load_acc(const double* __restrict__ qd, int slot,
+                                                GhVec3* A) {
+  A[1] = GhVec3{qd[12 * kDenseCap + slot], qd[13 * kDenseCap + slot], qd[14 * kDenseCap + slot]};
+#pragma unroll
+  for (int j = 4; j <= 13; j++) {
+    const int k = 12 + 3 * (j - 3);
+    A[j] = GhVec3{qd[k * kDenseCap + slot], qd[(k + 1) * kDenseCap + slot],
+                  qd[(k + 2) * kDenseCap + slot]};
+  }
+  A[2] = A[3] = GhVec3{0.0, 0.0, 0.0};  // unused by stages 14-16 and the extension
+}
+
+__device__ __forceinline__ GhVec3 gh_job_vec(const double* __restrict__ qd, int slot, int k) {
+  return GhVec3{qd[k * kDenseCap + slot], qd[(k + 1) * kDenseCap + slot],
+                qd[(k + 2) * kDenseCap + slot]};
+}
+
+// Dense output of queued job `slot` (gh_dop_dense's arithmetic), in phases
+// that each re-read the job from LDS: stage 14, stage 15, stage 16, then the
+// extension at every requested time.  The signal fences keep the phases'
+// loads apart, so only the three new accelerations are live from one phase
+// to the next -- a whole job plus a force evaluation would not fit the
+// register budget of two wavefronts per SIMD next to the lane's own state.
+template <class F>
+__device__ __forceinline__ void gh_job_run(const F& T, const double* __restrict__ qd,
+                                           const int* __restrict__ qi, int slot,
+                                           const double* __restrict__ t0, int nt, int indiv_t,
+                                           double* __restrict__ result) {
+  using namespace ghdp;
+  const double t_old = qd[45 * kDenseCap + slot], h = qd[46 * kDenseCap + slot];
+  GhVec3 A14, A15, A16;
+  {
+    GhVec3 A[17];
+    gh_job_load_acc(qd, slot, A);
+    A14 = gh_accel(T, t_old + c14 * h,
+                   gh_rkn_q14(h, gh_job_vec(qd, slot, 0), gh_job_vec(qd, slot, 3),
+                              gh_job_vec(qd, slot, 9), A));
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  {
+    GhVec3 A[17];
+    gh_job_load_acc(qd, slot, A);
+    A[14] = A14;
+    A15 = gh_accel(T, t_old + c15 * h,
+                   gh_rkn_q15(h, gh_job_vec(qd, slot, 0), gh_job_vec(qd, slot, 3),
+                              gh_job_vec(qd, slot, 9), A));
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  {
+    GhVec3 A[17];
+    gh_job_load_acc(qd, slot, A);
+    A[14] = A14;
+    A[15] = A15;
+    A16 = gh_accel(T, t_old + c16 * h,
+                   gh_rkn_q16(h, gh_job_vec(qd, slot, 0), gh_job_vec(qd, slot, 3),
+                              gh_job_vec(qd, slot, 9), A));
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  const int o = qi[kDenseCap + slot];
+  const double* __restrict__ t = t0 + (indiv_t ? (long)o * nt : 0);
+  double* __restrict__ out = result + (long)o * nt * 6;
+  const double pos_neg = (h > 0.0) ? 1.0 : -1.0;
+  const double t_cur = t_old + h;
+  int done = qi[slot];
+  while (done < nt - 1 && pos_neg * t[done + 1] < pos_neg * t_cur) {
+    const double sd = (t[done + 1] - t_old) / h;
+    const double s1 = 1.0 - sd;
+    double yo[6];
+    {
+      GhVec3 A[17];
+      gh_job_load_acc(qd, slot, A);
+      A[14] = A14;
+      A[15] = A15;
+      A[16] = A16;
+      gh_rkn_dense_eval(h, sd, s1, gh_job_vec(qd, slot, 0), gh_job_vec(qd, slot, 3),
+                        gh_job_vec(qd, slot, 6), gh_job_vec(qd, slot, 9), A, yo);
+    }
+    done++;
+    gh_store_cyl(out + 6L * done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  }
+}
+
 template <class Spec, bool PILOT, int WPE>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
                  int indiv_t, double rtol_log, double atol_log, const GhPotTable* __restrict__ T,
                  double* __restrict__ result, int* __restrict__ err, double* __restrict__ stats,
                  const volatile int* cancel, unsigned long long* __restrict__ counter,
-                 const int* __restrict__ order, unsigned* __restrict__ cost) {
+                 const int* __restrict__ order, unsigned* __restrict__ cost,
+                 char* __restrict__ overflow) {
   const GhForce<Spec> F{T};
   const double rtol = exp(rtol_log);
   const double atol = exp(atol_log);
@@ -379,6 +560,16 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
   long claim = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long orbit = 0;
   GhDopState s;
+  __shared__ double qd[PILOT ? 1 : kJobDoubles * kDenseCap];
+  __shared__ int qi[PILOT ? 1 : 2 * kDenseCap];
+  int queued = 0;  // jobs in this wavefront's queue (the same in every lane)
+  volatile double* od = reinterpret_cast<volatile double*>(overflow + blockIdx.x * kOverflowBytes);
+  auto flush = [&]() {
+    __syncthreads();  // the queue's stores are visible to every lane
+    if (lane < queued) gh_job_run(F, qd, qi, lane, t, nt, indiv_t, result);
+    __syncthreads();  // every job is read before slots are reused
+    queued = 0;
+  };
   const double* tt = t;
   double* out = result;
   bool active = false, exhausted = false, stop = false;
@@ -412,40 +603,89 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
     start(orbit);
     active = true;
   }
-  for (unsigned iter = 0;; iter++) {
-    if (cancel != nullptr && (iter & 255u) == 0 && *cancel) stop = true;
-    if (active) {
-      if (stop) {
-        finish(orbit, -10);
-        active = false;
-      } else if (gh_dop_step<GhForce<Spec>, !PILOT>(F, tt, nt, rtol, atol, s, out)) {
-        finish(orbit, 0);
-        active = false;
-      }
-    }
-    const bool want = !active && !exhausted;
-    const unsigned long long m = __ballot(want);
-    if (m) {
-      const int leader = __ffsll((long long)m) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-      base = __shfl(base, leader);
-      if (want) {
-        claim = (long)(base + (unsigned long long)__popcll(m & below));
-        if (claim < nobj) {
-          orbit = order ? (long)order[claim] : claim;
-          if (stop) {
-            if (!PILOT) err[orbit] = -10;  // interrupted before it started
-          } else {
-            start(orbit);
-            active = true;
-          }
-        } else {
-          exhausted = true;
+  // The stepping loop never flushes the queue itself: it runs until the
+  // queue is due (or jobs overflowed, or the wavefront is done), and the
+  // flush runs between stepping phases, so the stage loop's register
+  // allocation never has to make room for a job's dense output.
+  unsigned iter = 0;
+  int over = 0;  // jobs waiting in the overflow area
+  bool all_done = false;
+  for (;;) {
+    for (;; iter++) {
+      if (cancel != nullptr && (iter & 255u) == 0 && *cancel) stop = true;
+      // the dense-output job of an accepted step that passes output times is
+      // stored from inside the step, as soon as it is formed (never held in
+      // registers): the lanes calling the sink are exactly those with a job,
+      // so a ballot there ranks them.  A wavefront can produce up to 64 jobs
+      // in one step, more than the queue has room for: those wait in the
+      // overflow area.
+      bool need = false;
+      const int room = kDenseCap - queued;
+      auto sink = [&](const GhDenseJob& J) {
+        const int rank = __popcll(__ballot(1) & below);
+        if (rank < room)
+          gh_job_store(qd, qi, kDenseCap, queued + rank, J, (int)orbit);
+        else
+          gh_job_store_ovf(od, rank - room, J, (int)orbit);
+        need = true;
+      };
+      if (active) {
+        if (stop) {
+          finish(orbit, -10);
+          active = false;
+        } else if (gh_dop_step<GhForce<Spec>, PILOT ? GH_DENSE_SKIP : GH_DENSE_DEFER>(
+                       F, tt, nt, rtol, atol, s, out, sink)) {
+          finish(orbit, 0);
+          active = false;
         }
       }
+      if (!PILOT) {
+        const int nnew = __popcll(__ballot(need));
+        queued += min(nnew, room);
+        over = max(nnew - room, 0);
+      }
+      const bool want = !active && !exhausted;
+      const unsigned long long m = __ballot(want);
+      if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader);
+        if (want) {
+          claim = (long)(base + (unsigned long long)__popcll(m & below));
+          if (claim < nobj) {
+            orbit = order ? (long)order[claim] : claim;
+            if (stop) {
+              if (!PILOT) err[orbit] = -10;  // interrupted before it started
+            } else {
+              start(orbit);
+              active = true;
+            }
+          } else {
+            exhausted = true;
+          }
+        }
+      }
+      all_done = __all(!active && exhausted);
+      if (all_done || over > 0 || queued >= kDenseFlush) {
+        iter++;
+        break;
+      }
     }
-    if (__all(!active && exhausted)) break;
+    if (!PILOT) {
+      // Every dense output runs through this one flush path, so results do
+      // not depend on timing; overflowed jobs move in by rounds.
+      for (int from = 0; over > 0;) {
+        flush();
+        const int take = min(over, kDenseCap);
+        if (lane < take) gh_job_copy_in(od, from + lane, qd, qi, lane);
+        queued = take;
+        from += take;
+        over -= take;
+      }
+      if (queued >= kDenseFlush || (all_done && queued > 0)) flush();
+    }
+    if (all_done) break;
   }
 }
 
@@ -591,44 +831,48 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
     set_error("could not set up the DOP853 work counter on device %d", dev);
     return -1;
   }
-  // Longest-first order from a pilot run, when some lane gets more than one
-  // orbit (otherwise all orbits start at once and the order is moot).
-  const int* d_order = nullptr;
-  void* work = nullptr;
+  // One stream-ordered temporary allocation: the dense-output overflow area
+  // (kOverflowBytes per workgroup) and, when some lane gets more than one
+  // orbit, the pilot's cost keys and the longest-first order (otherwise all
+  // orbits start at once and the order is moot).
+  const bool ordered = !indiv_t && (unsigned long long)nobj > lanes && dop_order_enabled();
+  size_t sort_bytes = 0;
   hipError_t e = hipSuccess;
-  if (!indiv_t && (unsigned long long)nobj > lanes && dop_order_enabled()) {
-    size_t sort_bytes = 0;
+  if (ordered)
     e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, (const unsigned*)nullptr,
                                                      (unsigned*)nullptr, (const int*)nullptr,
                                                      (int*)nullptr, nobj, 0, 32, stream);
-    const size_t n4 = ((size_t)nobj * 4 + 255) / 256 * 256;
-    if (e == hipSuccess) e = hipMallocAsync(&work, 4 * n4 + 256 + sort_bytes, stream);
-    if (e == hipSuccess) {
-      char* w = static_cast<char*>(work);
-      unsigned* d_cost = reinterpret_cast<unsigned*>(w);
-      unsigned* d_cost_sorted = reinterpret_cast<unsigned*>(w + n4);
-      int* d_iota = reinterpret_cast<int*>(w + 2 * n4);
-      int* d_perm = reinterpret_cast<int*>(w + 3 * n4);
-      double* d_pt = reinterpret_cast<double*>(w + 4 * n4);
-      void* d_sort_tmp = w + 4 * n4 + 256;
-      hipLaunchKernelGGL(gh_pilot_grid, dim3(1), dim3(1), 0, stream, d_t, nt, kDopPilotFrac, d_pt);
-      hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, lanes);
-      hipLaunchKernelGGL((gh_dop853_kernel<Spec, true, WPE>), dim3(nblocks), dim3(kBlock), 0,
-                         stream, nobj, d_yo, 2, d_pt, 0, rtol, atol, d_table, nullptr, nullptr,
-                         nullptr, cancel, counter, nullptr, d_cost);
-      hipLaunchKernelGGL(gh_iota, dim3((unsigned)((nobj + 255) / 256)), dim3(256), 0, stream,
-                         nobj, d_iota);
-      e = hipcub::DeviceRadixSort::SortPairsDescending(d_sort_tmp, sort_bytes, d_cost,
-                                                       d_cost_sorted, d_iota, d_perm, nobj, 0, 32,
-                                                       stream);
-      d_order = d_perm;
-    }
+  const size_t ovf = ((size_t)nblocks * kOverflowBytes + 255) / 256 * 256;
+  const size_t n4 = ordered ? ((size_t)nobj * 4 + 255) / 256 * 256 : 0;
+  void* work = nullptr;
+  if (e == hipSuccess)
+    e = hipMallocAsync(&work, ovf + (ordered ? 4 * n4 + 256 + sort_bytes : 0), stream);
+  char* d_overflow = static_cast<char*>(work);
+  const int* d_order = nullptr;
+  if (e == hipSuccess && ordered) {
+    char* w = static_cast<char*>(work) + ovf;
+    unsigned* d_cost = reinterpret_cast<unsigned*>(w);
+    unsigned* d_cost_sorted = reinterpret_cast<unsigned*>(w + n4);
+    int* d_iota = reinterpret_cast<int*>(w + 2 * n4);
+    int* d_perm = reinterpret_cast<int*>(w + 3 * n4);
+    double* d_pt = reinterpret_cast<double*>(w + 4 * n4);
+    void* d_sort_tmp = w + 4 * n4 + 256;
+    hipLaunchKernelGGL(gh_pilot_grid, dim3(1), dim3(1), 0, stream, d_t, nt, kDopPilotFrac, d_pt);
+    hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, lanes);
+    hipLaunchKernelGGL((gh_dop853_kernel<Spec, true, WPE>), dim3(nblocks), dim3(kBlock), 0,
+                       stream, nobj, d_yo, 2, d_pt, 0, rtol, atol, d_table, nullptr, nullptr,
+                       nullptr, cancel, counter, nullptr, d_cost, nullptr);
+    hipLaunchKernelGGL(gh_iota, dim3((unsigned)((nobj + 255) / 256)), dim3(256), 0, stream, nobj,
+                       d_iota);
+    e = hipcub::DeviceRadixSort::SortPairsDescending(d_sort_tmp, sort_bytes, d_cost, d_cost_sorted,
+                                                     d_iota, d_perm, nobj, 0, 32, stream);
+    d_order = d_perm;
   }
   if (e == hipSuccess) {
     hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, lanes);
     hipLaunchKernelGGL((gh_dop853_kernel<Spec, false, WPE>), dim3(nblocks), dim3(kBlock), 0,
                        stream, nobj, d_yo, nt, d_t, indiv_t, rtol, atol, d_table, d_result, d_err,
-                       d_stats, cancel, counter, d_order, nullptr);
+                       d_stats, cancel, counter, d_order, nullptr, d_overflow);
     e = hipGetLastError();
   }
   if (work) {

@@ -98,6 +98,59 @@ def test_dop853_longest_first_order_leaves_results_unchanged():
         assert torch.equal(whole, torch.cat(parts))
 
 
+def test_dop853_lockstep_lanes_all_need_dense_output():
+    """130 copies of one orbit step in lockstep, so every lane of a wavefront
+    passes an output time in the same step (more dense-output jobs at once
+    than the per-wavefront queue holds): every copy must come out identical
+    and match the oracle."""
+    for pot in (_mw(), bar_potential()):
+        yo = numpy.repeat(ics(1, seed=12), 130, axis=0)
+        t = numpy.linspace(0.0, 20.0, 201)
+        res, err = _hip(pot, yo, t, "dop853_hip")
+        assert numpy.all(err == 0)
+        assert numpy.array_equal(res, numpy.broadcast_to(res[:1], res.shape))
+        pt, pa = _packed(pot)
+        ref, rerr, _, _ = oracle_integrate(pt, pa, yo[:1], t, 6)
+        assert phase_diff(res[:1], ref) < 1e-8
+
+
+def test_dop853_energy_outliers_are_the_oracles():
+    """The largest |dE/E| of the DOP853 benchmark orbits (first 65,536 of the
+    1M-orbit MWPotential2014 workload, t = linspace(0, 500, 1001), rtol = atol
+    = 1e-12) belong to the method, not to this implementation: the oracle
+    integrates the same orbits to within 1e-8 of the GPU trajectories and its
+    own energy drift equals the GPU's to within 1% (and 1e-12)."""
+    import torch
+
+    from galpy_amd import workloads
+    from galpy_amd.integrate import PotentialTable, energy_drift_device, integrate_device
+
+    from helpers import oracle_potential
+
+    w = workloads.make("mw2014_dop853_1m", n=65536)
+    table = PotentialTable(w["pot"])
+    t = torch.tensor(w["t"], device="cuda")
+    res, err, _ = integrate_device(table, torch.tensor(w["yo"], device="cuda"), t, "dop853_hip")
+    _, demax = energy_drift_device(table, res, t)
+    worst = torch.argsort(demax, descending=True)[:6].cpu().numpy()
+    rows = res[torch.as_tensor(worst, device="cuda")].cpu().numpy()
+    assert int((err != 0).sum()) == 0
+    pt, pa = _packed(w["pot"])
+    ref, rerr, _, _ = oracle_integrate(pt, pa, w["yo"][worst], w["t"], 6)
+    assert numpy.all(rerr == 0)
+    assert phase_diff(rows, ref) < 1e-8
+
+    def drift(r):
+        R, vR, vT, z, vz, phi = (r[..., k].ravel() for k in range(6))
+        E = 0.5 * (vR**2 + vT**2 + vz**2) + oracle_potential(pt, pa, R, z, phi)
+        E = E.reshape(r.shape[:2])
+        return numpy.abs(E - E[:, :1]).max(axis=1) / numpy.abs(E[:, 0])
+
+    dg, dr = drift(rows), drift(ref)
+    assert dg.max() > 1e-8  # the outliers are outliers
+    assert numpy.all(numpy.abs(dg - dr) <= 0.01 * dr + 1e-12), (dg, dr)
+
+
 def test_dop853_bar_jacobi():
     pot = bar_potential()
     yo = ics(128, seed=4)
