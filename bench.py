@@ -29,6 +29,17 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the integration kernel, measured on MI355X with two
+# rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of
+#   python bench.py --steps 1 --warmup 0 --no-cpu-baseline
+# each scaled by what the counter reads for a plain 2 GiB device copy
+# (python bench.py --workload copy): FETCH_SIZE reads 1 GiB (x2), WRITE_SIZE
+# 2 GiB (x1).  Valid for the workload at its default size and this build.
+MEASURED_TRAFFIC = {
+    # 304,802.6 KiB fetched x 2 + 77,530,416.5 KiB written
+    "mw2014_dop853_1m": (304802.625 * 2.0 + 77530416.5) * 1024.0,
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -333,7 +344,9 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get(w["name"]) if args.norbits is None else None,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "algorithmic_bytes": float(n * nt * 48 + 2 * n * 48),
             "kernel": "gh_dop853_kernel" if w["method"] == "dop853_hip" else "gh_symplectic_kernel",
             "note": ("FP64 vector ALU bound (no MFMA: nothing in the path is a contraction; "
                      "78.6 TFLOP/s is MI355X's FP64 peak). achieved = counted algorithmic FP64 ops "

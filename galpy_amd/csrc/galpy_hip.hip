@@ -357,13 +357,11 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
 // only cost[o] = attempted steps and no output rows).  Every orbit is
 // integrated by the same code whatever the order, so results do not depend
 // on it.
-// Two instances per composition, built for 1 and 2 waves per SIMD (WPE):
-// at 2 waves the register allocator has 256 registers per lane and spills
-// only values used outside the stage loop (pointers, counters, the refill
-// and dense-output paths), and the second wave hides the first one's FP64
-// and memory latency -- 11% faster for 1M orbits of MWPotential2014 on
-// MI355X.  With few orbits per lane the longer tail of the doubled lane
-// count costs more than that, so the launcher picks WPE from the orbit count.
+// Built for 1 and 2 waves per SIMD (WPE).  At 2 waves the kernel fits 256
+// registers per lane without spilling (239 VGPRs for MWPotential2014) and the
+// second wave hides the first one's FP64 and LDS latency: on MI355X it is
+// faster at every size measured (131,072 to 1,048,576 orbits: 1.3-1.6x), so
+// it is the default; GALPY_HIP_DOP_WAVES=1 selects the other instance.
 // Deferred dense output.  A lane whose accepted step passes output times
 // stores the step's GhDenseJob in its wavefront's LDS queue and steps on;
 // once the queue holds kDenseFlush jobs the whole wavefront computes them,
@@ -374,12 +372,16 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
 // consecutive banks); 47 doubles + 2 ints per job, 52 jobs = 19,968 bytes per
 // wavefront, so 8 single-wavefront workgroups still fit a CU's 160 KB LDS.
 constexpr int kDenseCap = 52;
-constexpr int kDenseFlush = 44;
+// Flush at 32 queued jobs: a step adds ~7.5 jobs per wavefront on average
+// (MWPotential2014, 1001 outputs), so the 20 free slots almost never
+// overflow, and on MI355X 32 measured fastest (817 ms for 1M orbits, against
+// 822 at 36, 864 at 44 -- where overflow is frequent -- and 830 at 24).
+constexpr int kDenseFlush = 32;
 constexpr int kJobDoubles = 47;
 
-template <class D, class I>
-__device__ __forceinline__ void gh_job_store(D* __restrict__ qd, I* __restrict__ qi, int cap,
+__device__ __forceinline__ void gh_job_store(double* __restrict__ qd, int* __restrict__ qi,
                                              int slot, const GhDenseJob& J, int orbit) {
+  constexpr int cap = kDenseCap;
   double v[kJobDoubles];
   const GhVec3 g[4] = {J.q0, J.p0, J.qn, J.pn};
 #pragma unroll
@@ -400,24 +402,6 @@ __device__ __forceinline__ void gh_job_store(D* __restrict__ qd, I* __restrict__
   for (int k = 0; k < kJobDoubles; k++) qd[k * cap + slot] = v[k];
   qi[slot] = J.done0;
   qi[cap + slot] = orbit;
-}
-
-__device__ __forceinline__ void gh_job_load(const double* __restrict__ qd,
-                                            const int* __restrict__ qi, int slot, GhDenseJob& J,
-                                            int& orbit) {
-  double v[kJobDoubles];
-#pragma unroll
-  for (int k = 0; k < kJobDoubles; k++) v[k] = qd[k * kDenseCap + slot];
-  J.q0 = GhVec3{v[0], v[1], v[2]};
-  J.p0 = GhVec3{v[3], v[4], v[5]};
-  J.qn = GhVec3{v[6], v[7], v[8]};
-  J.pn = GhVec3{v[9], v[10], v[11]};
-#pragma unroll
-  for (int k = 0; k < 11; k++) J.A[k] = GhVec3{v[12 + 3 * k], v[13 + 3 * k], v[14 + 3 * k]};
-  J.t_old = v[45];
-  J.h = v[46];
-  J.done0 = qi[slot];
-  orbit = qi[kDenseCap + slot];
 }
 
 // Jobs that did not fit the LDS queue wait in a per-wavefront area of device
@@ -624,7 +608,7 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
       auto sink = [&](const GhDenseJob& J) {
         const int rank = __popcll(__ballot(1) & below);
         if (rank < room)
-          gh_job_store(qd, qi, kDenseCap, queued + rank, J, (int)orbit);
+          gh_job_store(qd, qi, queued + rank, J, (int)orbit);
         else
           gh_job_store_ovf(od, rank - room, J, (int)orbit);
         need = true;
@@ -778,7 +762,7 @@ bool dop_order_enabled() {
   return v != 0;
 }
 
-// GALPY_HIP_DOP_WAVES=1 or 2 forces the DOP853 occupancy variant.
+// GALPY_HIP_DOP_WAVES=1 or 2 selects the DOP853 occupancy variant (default 2).
 int dop_waves_forced() {
   static const int v = [] {
     const char* e = std::getenv("GALPY_HIP_DOP_WAVES");
@@ -904,9 +888,7 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
     case 6: {
       int dev = 0;
       GH_CHECK(hipGetDevice(&dev));
-      const int r2 = dop_resident_blocks<Spec, 2>(dev);
-      const int forced = dop_waves_forced();
-      const bool two = forced ? forced == 2 : (long)nobj >= 6L * r2 * kBlock;
+      const bool two = dop_waves_forced() != 1;
       return two ? launch_dop853<Spec, 2>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
                                           d_result, d_err, d_stats, cancel, dev, stream)
                  : launch_dop853<Spec, 1>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
