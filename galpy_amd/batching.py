@@ -61,6 +61,24 @@ def _lib_batch():
     return lib
 
 
+# gh_batch_launch flags (include/galpy_hip.h)
+GH_BATCH_DOP = 1
+GH_BATCH_SPECIALISED = 2
+# compositions (packed pot_type codes) with a force specialised in the batch
+# kernel: MWPotential2014, LogarithmicHalo + DehnenBar, Hernquist,
+# LogarithmicHalo
+SPECIALISED_CODES = {(15, 5, 9), (0, 1), (8,), (0,)}
+
+
+def launch_flags(methods, pot_types):
+    """gh_batch_launch flags for groups with these method codes and packed
+    pot_type arrays."""
+    flags = GH_BATCH_DOP if any(m == 6 for m in methods) else 0
+    if all(tuple(int(c) for c in pt) in SPECIALISED_CODES for pt in pot_types):
+        flags |= GH_BATCH_SPECIALISED
+    return flags
+
+
 class PotentialCache:
     """Device tables deduplicated by packed content (many groups share a
     potential, e.g. MWPotential2014)."""
@@ -105,9 +123,10 @@ class Batch:
         self.yo, self.t, self.results, self.errs, self.stats, self.tables = [], [], [], [], [], []
         self.state = []
         costs = []
-        self.with_dop = False
+        methods, pot_types = [], []
         for i, g in enumerate(self.groups):
             tab, pt, pa = self.pots.get(g["pot"])
+            pot_types.append(pt)
             yo = torch.as_tensor(numpy.ascontiguousarray(g["yo"], dtype=numpy.float64), device=self.device)
             t = torch.as_tensor(numpy.ascontiguousarray(g["t"], dtype=numpy.float64), device=self.device)
             n, nt = yo.shape[0], t.shape[0]
@@ -118,7 +137,7 @@ class Batch:
             st = torch.zeros((n, 3), dtype=torch.float64, device=self.device)
             state = torch.empty(n * self.state_bytes, dtype=torch.uint8, device=self.device)
             method = METHOD_CODES[g.get("method", "symplec4_hip")]
-            self.with_dop |= method == 6
+            methods.append(method)
             rtol, atol = _parse_tol(g.get("rtol"), g.get("atol"))
             dt = g.get("dt")
             G[i] = (lib.gh_potential_device_ptr(tab.handle), t.data_ptr(), yo.data_ptr(), res.data_ptr(),
@@ -136,6 +155,7 @@ class Batch:
             per_step = _METHOD_FORCES[method] * sum(_POT_COST.get(int(c), 1.0) for c in pt)
             costs.append(span / step * per_step if method != 6 else span * per_step * 20.0)
         self.G = G
+        self.flags = launch_flags(methods, pot_types)
         tiles = []
         per_tile = 64 * self.opl
         nstate = 0
@@ -177,7 +197,7 @@ class Batch:
         rc = _lib_batch().gh_batch_launch(
             self.d_groups.data_ptr(), self.d_tiles.data_ptr(), self.ntiles,
             self.d_tile_done.data_ptr(), self.d_counter.data_ptr(), self.d_remaining.data_ptr(),
-            int(budget), self.opl, int(self.with_dop), s.cuda_stream,
+            int(budget), self.opl, self.flags, s.cuda_stream,
         )
         _lib.check(rc, "gh_batch_launch")
 

@@ -134,13 +134,14 @@ __device__ __forceinline__ void gh_batch_finish(const gh_batch_group& g, long o,
   s.status = 2;
 }
 
-// Advance the orbits of one tile by at most `budget` steps each; returns the
-// number of orbits of the tile (OPL per lane) still unfinished in this lane.
-template <int OPL, bool DOP>
+// Advance the orbits of one tile by at most `budget` steps each, with the
+// force of composition Spec; returns the number of orbits of the tile (OPL
+// per lane) still unfinished in this lane.
+template <int OPL, bool DOP, class Spec>
 __device__ int gh_batch_run_tile(const gh_batch_group& g, const gh_batch_tile& tile,
                              long long budget, const volatile int* cancel) {
   GhLaneState* __restrict__ state = (GhLaneState*)g.state;
-  const GhForce<GhSpec<>> Fo{(const GhPotTable*)g.table};
+  const GhForce<Spec> Fo{(const GhPotTable*)g.table};
   const int lane = threadIdx.x & 63;
   GhLaneState s[OPL];
   long o[OPL];
@@ -209,13 +210,48 @@ __device__ int gh_batch_run_tile(const gh_batch_group& g, const gh_batch_tile& t
   return left;
 }
 
+// Compositions with a specialised force: the same ones the single-group
+// kernels specialise (MWPotential2014, LogarithmicHalo + DehnenBar,
+// Hernquist, LogarithmicHalo); 0 = any other.
+enum GhSpecCode { GH_SPEC_ANY = 0, GH_SPEC_MW2014, GH_SPEC_LOGBAR, GH_SPEC_HERN, GH_SPEC_LOG };
+
+__device__ __forceinline__ int gh_spec_code(const GhPotTable* __restrict__ T) {
+  const int n = T->npot;
+  const int c0 = T->comp[0].type;
+  if (n == 3 && c0 == GH_POWERSPHCUTOFF && T->comp[1].type == GH_MIYAMOTONAGAI &&
+      T->comp[2].type == GH_NFW)
+    return GH_SPEC_MW2014;
+  if (n == 2 && c0 == GH_LOGHALO && T->comp[1].type == GH_DEHNENBAR) return GH_SPEC_LOGBAR;
+  if (n == 1 && c0 == GH_HERNQUIST) return GH_SPEC_HERN;
+  if (n == 1 && c0 == GH_LOGHALO) return GH_SPEC_LOG;
+  return GH_SPEC_ANY;
+}
+
+// A tile whose composition has no specialised force in a kernel built
+// without the generic one: its orbits finish at once with err = -1.
+__device__ __forceinline__ int gh_batch_reject_tile(const gh_batch_group& g,
+                                                    const gh_batch_tile& tile) {
+  GhLaneState* __restrict__ state = (GhLaneState*)g.state;
+  for (int k = threadIdx.x & 63; k < tile.count; k += 64) {
+    g.err[tile.first + k] = -1;
+    state[tile.first + k].status = 2;
+  }
+  return 0;
+}
+
 // Persistent batch kernel: wavefronts claim tiles until the queue is empty.
 // tile_done[i] != 0 marks a finished tile (skipped by later launches);
 // *remaining counts unfinished tiles.  DOP = the batch has DOP853 groups
 // (its register demand is far above the symplectic steppers', so
-// symplectic-only batches use the instance without it).
-template <int OPL, bool DOP>
+// symplectic-only batches use the instance without it).  Each tile runs the
+// force specialised for its group's composition (a wavefront-uniform
+// branch); GEN = the instance also carries the run-time dispatch force for
+// other compositions, whose register demand would otherwise set the whole
+// kernel's occupancy.  Occupancy targets: 3 waves per SIMD with one orbit
+// per lane, 2 with two, 1 for the DOP853 instances.
+template <int OPL, bool DOP, bool GEN>
 __global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(DOP ? 1 : 4 - OPL, DOP ? 1 : 4 - OPL)))
 gh_batch_kernel(const gh_batch_group* __restrict__ groups, const gh_batch_tile* __restrict__ tiles,
                 int ntiles, int* __restrict__ tile_done,
                 unsigned long long* __restrict__ counter, int* __restrict__ remaining,
@@ -229,7 +265,28 @@ gh_batch_kernel(const gh_batch_group* __restrict__ groups, const gh_batch_tile* 
     if (tile_done[t]) continue;
     const gh_batch_tile tile = tiles[t];
     const gh_batch_group g = groups[gh_uniform(tile.group)];
-    const int left = gh_batch_run_tile<OPL, DOP>(g, tile, budget, cancel);
+    int left;
+    switch (gh_uniform(gh_spec_code((const GhPotTable*)g.table))) {
+      case GH_SPEC_MW2014:
+        left = gh_batch_run_tile<OPL, DOP, GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>>(
+            g, tile, budget, cancel);
+        break;
+      case GH_SPEC_LOGBAR:
+        left = gh_batch_run_tile<OPL, DOP, GhSpec<GH_LOGHALO, GH_DEHNENBAR>>(g, tile, budget, cancel);
+        break;
+      case GH_SPEC_HERN:
+        left = gh_batch_run_tile<OPL, DOP, GhSpec<GH_HERNQUIST>>(g, tile, budget, cancel);
+        break;
+      case GH_SPEC_LOG:
+        left = gh_batch_run_tile<OPL, DOP, GhSpec<GH_LOGHALO>>(g, tile, budget, cancel);
+        break;
+      default:
+        if constexpr (GEN)
+          left = gh_batch_run_tile<OPL, DOP, GhSpec<>>(g, tile, budget, cancel);
+        else
+          left = gh_batch_reject_tile(g, tile);
+        break;
+    }
     if (!__any(left != 0) && lane == 0) {
       tile_done[t] = 1;
       if (remaining) atomicSub(remaining, 1);

@@ -1316,12 +1316,13 @@ __global__ void gh_batch_reset_kernel(const gh_batch_group* __restrict__ groups,
   }
 }
 
-template <int OPL, bool DOP>
+template <int OPL, bool DOP, bool GEN>
 int batch_resident_blocks(int dev) {
   static int cached[64] = {0};
   if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
   int per_cu = 0, ncu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_batch_kernel<OPL, DOP>, 64, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_batch_kernel<OPL, DOP, GEN>, 64,
+                                                   0) !=
           hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
@@ -1347,7 +1348,7 @@ GH_EXPORT int gh_batch_reset(const gh_batch_group* d_groups, const gh_batch_tile
 
 GH_EXPORT int gh_batch_launch(const gh_batch_group* d_groups, const gh_batch_tile* d_tiles,
                               int ntiles, int* d_tile_done, unsigned long long* d_counter,
-                              int* d_remaining, long long budget, int opl, int with_dop,
+                              int* d_remaining, long long budget, int opl, int flags,
                               void* stream) {
   g_errbuf[0] = 0;
   if (ntiles <= 0) return 0;
@@ -1357,11 +1358,20 @@ GH_EXPORT int gh_batch_launch(const gh_batch_group* d_groups, const gh_batch_til
   }
   int dev = 0;
   GH_CHECK(hipGetDevice(&dev));
-  const int key = (opl - 1) * 2 + (with_dop ? 1 : 0);
-  const int resident = key == 0   ? batch_resident_blocks<1, false>(dev)
-                       : key == 1 ? batch_resident_blocks<1, true>(dev)
-                       : key == 2 ? batch_resident_blocks<2, false>(dev)
-                                  : batch_resident_blocks<2, true>(dev);
+  const bool dop = (flags & GH_BATCH_DOP) != 0;
+  const bool gen = (flags & GH_BATCH_SPECIALISED) == 0;
+  const int key = (opl - 1) * 4 + (dop ? 2 : 0) + (gen ? 1 : 0);
+  int resident = 0;
+  switch (key) {
+    case 0: resident = batch_resident_blocks<1, false, false>(dev); break;
+    case 1: resident = batch_resident_blocks<1, false, true>(dev); break;
+    case 2: resident = batch_resident_blocks<1, true, false>(dev); break;
+    case 3: resident = batch_resident_blocks<1, true, true>(dev); break;
+    case 4: resident = batch_resident_blocks<2, false, false>(dev); break;
+    case 5: resident = batch_resident_blocks<2, false, true>(dev); break;
+    case 6: resident = batch_resident_blocks<2, true, false>(dev); break;
+    default: resident = batch_resident_blocks<2, true, true>(dev); break;
+  }
   if (resident <= 0) {
     set_error("occupancy query failed for the batch kernel");
     return -1;
@@ -1371,14 +1381,18 @@ GH_EXPORT int gh_batch_launch(const gh_batch_group* d_groups, const gh_batch_til
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, st, d_counter, 0ull);
   const int* cancel = irq_device_ptr();
-#define GH_BATCH(O, D)                                                                          \
-  hipLaunchKernelGGL((gh_batch_kernel<O, D>), dim3(nblocks), dim3(64), 0, st, d_groups, d_tiles, \
-                     ntiles, d_tile_done, d_counter, d_remaining, b, cancel)
+#define GH_BATCH(O, D, G)                                                                    \
+  hipLaunchKernelGGL((gh_batch_kernel<O, D, G>), dim3(nblocks), dim3(64), 0, st, d_groups,  \
+                     d_tiles, ntiles, d_tile_done, d_counter, d_remaining, b, cancel)
   switch (key) {
-    case 0: GH_BATCH(1, false); break;
-    case 1: GH_BATCH(1, true); break;
-    case 2: GH_BATCH(2, false); break;
-    default: GH_BATCH(2, true); break;
+    case 0: GH_BATCH(1, false, false); break;
+    case 1: GH_BATCH(1, false, true); break;
+    case 2: GH_BATCH(1, true, false); break;
+    case 3: GH_BATCH(1, true, true); break;
+    case 4: GH_BATCH(2, false, false); break;
+    case 5: GH_BATCH(2, false, true); break;
+    case 6: GH_BATCH(2, true, false); break;
+    default: GH_BATCH(2, true, true); break;
   }
 #undef GH_BATCH
   GH_CHECK(hipGetLastError());

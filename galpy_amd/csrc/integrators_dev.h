@@ -61,8 +61,11 @@ __device__ __forceinline__ double gh_est_err(GhVec3 q11, GhVec3 q12, GhVec3 p11,
 }
 
 // leapfrog_estimate_step, bovy_symplecticode.c:588-663
+// The step estimators run once per orbit; they are inlined so that a kernel's
+// register count is not set by a separately compiled copy (a call's callee is
+// budgeted at its own, larger, register use).
 template <class F>
-__device__ double gh_leapfrog_estimate_step(const F& T, GhVec3 qo,
+__device__ __forceinline__ double gh_leapfrog_estimate_step(const F& T, GhVec3 qo,
                                             GhVec3 po, double dt, double to, double rtol,
                                             double atol) {
   double err = 2.0;
@@ -90,7 +93,7 @@ __device__ double gh_leapfrog_estimate_step(const F& T, GhVec3 qo,
 
 // symplec4_estimate_step, bovy_symplecticode.c:665-808
 template <class F>
-__device__ double gh_symplec4_estimate_step(const F& T, GhVec3 qo,
+__device__ __forceinline__ double gh_symplec4_estimate_step(const F& T, GhVec3 qo,
                                             GhVec3 po, double dt, double to, double rtol,
                                             double atol) {
   const double c1 = GH_S4C1, c2 = GH_S4C2, c3 = GH_S4C3, c4 = GH_S4C4;

@@ -18,7 +18,7 @@ import time
 import numpy
 
 from . import _lib
-from .batching import GROUP_DTYPE, TILE_DTYPE, PotentialCache, _lib_batch
+from .batching import GROUP_DTYPE, TILE_DTYPE, PotentialCache, _lib_batch, launch_flags
 from .integrate import DT_SENTINEL, METHOD_CODES, _parse_tol
 
 
@@ -126,6 +126,7 @@ class Server:
                       METHOD_CODES[r.method], 0)
             r.record = rec
             r.table = tab
+            r.pot_type = pt
             firsts = numpy.arange(0, r.n, self.per_tile)
             r.tiles = numpy.stack([firsts, numpy.minimum(self.per_tile, r.n - firsts)], axis=1)
             r.tile_open = numpy.ones(len(firsts), dtype=bool)
@@ -160,10 +161,10 @@ class Server:
         d_groups = torch.from_numpy(recs.view(numpy.uint8).copy()).to(self.device)
         d_tiles = torch.from_numpy(T.view(numpy.uint8).copy()).to(self.device)
         d_done = torch.zeros(len(T), dtype=torch.int32, device=self.device)
-        with_dop = any(r.method == "dop853_hip" for r in self.active)
+        flags = launch_flags([METHOD_CODES[r.method] for r in self.active], [r.pot_type for r in self.active])
         s = torch.cuda.current_stream(self.device)
         rc = self.lib.gh_batch_launch(d_groups.data_ptr(), d_tiles.data_ptr(), len(T), d_done.data_ptr(),
-                                      self.d_counter.data_ptr(), None, self.budget, self.opl, int(with_dop),
+                                      self.d_counter.data_ptr(), None, self.budget, self.opl, flags,
                                       s.cuda_stream)
         _lib.check(rc, "gh_batch_launch")
         done = d_done.cpu().numpy()  # waits for the launch

@@ -151,15 +151,24 @@ size_t gh_batch_state_bytes(void);
 int gh_batch_reset(const gh_batch_group *d_groups, const gh_batch_tile *d_tiles, int ntiles,
                    int *d_tile_done, int *d_remaining, void *stream);
 
+/* Flags of gh_batch_launch. */
+#define GH_BATCH_DOP 1         /* some group uses dop853 */
+#define GH_BATCH_SPECIALISED 2 /* every group's potential is MWPotential2014 ([15,5,9]),
+                                  LogarithmicHalo+DehnenBar ([0,1]), Hernquist ([8]) or
+                                  LogarithmicHalo ([0]): run only the forces specialised
+                                  for those (higher occupancy); a group with another
+                                  composition then finishes at once with err = -1 */
+
 /* One persistent launch: wavefronts claim unfinished tiles from *d_counter
  * (reset to 0 here by a kernel) and advance each orbit by at most `budget`
  * steps (<= 0: to completion).  opl = orbits per lane (1 or 2; tiles must
- * hold at most 64*opl orbits).  with_dop must be nonzero if any group uses
- * dop853 (selects the kernel instance that carries it).  *d_remaining counts
- * unfinished tiles.  Asynchronous on stream. */
+ * hold at most 64*opl orbits).  flags: GH_BATCH_DOP must be set if any group
+ * uses dop853 (selects the kernel instance that carries it); see
+ * GH_BATCH_SPECIALISED.  *d_remaining counts unfinished tiles.  Asynchronous
+ * on stream. */
 int gh_batch_launch(const gh_batch_group *d_groups, const gh_batch_tile *d_tiles, int ntiles,
                     int *d_tile_done, unsigned long long *d_counter, int *d_remaining,
-                    long long budget, int opl, int with_dop, void *stream);
+                    long long budget, int opl, int flags, void *stream);
 
 /* Message of the last error on the calling thread ("" if none). */
 const char *galpy_hip_last_error(void);
