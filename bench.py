@@ -9,6 +9,9 @@ over the K timed steps.  |dE/E| is measured on device afterwards.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
+NAME: mw2014_dop853_1m (default), loghalo_bar_dop853_256k, mw2014_symplec4_1m,
+mw2014_symplec4_1k, batching, serving, adjoint, copy (counter calibration).
+
 Prints one JSON line (see the keys below).  The CPU baseline leg times the
 CPU oracle (a C++/OpenMP restatement of galpy's C integrators) on a bounded
 sample of the same workload on this host's cores; the same leg counts the
@@ -205,6 +208,69 @@ def bench_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args):
+    """Differentiation workload: 65,536 MWPotential2014 orbits, symplec4 and
+    leapfrog, 2,000 steps of dt = 0.01; loss = sum of squared distances of the
+    final rectangular states to targets drawn as perturbed forward solutions;
+    gradients w.r.t. every initial condition and NFW amp/a, MiyamotoNagai a/b
+    through the reverse-time adjoint (orbit_flow's backward).  value = orbit-
+    steps of the differentiated problem per second (one gradient = forward
+    flow + reverse pass over n x 2,000 orbit-steps), symplec4; the line also
+    carries each method's central finite-difference check (h = 1e-6) of the
+    four parameter gradients."""
+    import torch
+
+    from galpy_amd import workloads
+    from galpy_amd.adjoint import ParamSpec, _RawTable, cyl_to_rect_torch, flow_raw, orbit_flow
+    from galpy_amd.potential import MWPotential2014
+
+    n, nsteps, dt, h = args.norbits or 65536, 2000, 0.01, 1e-6
+    w0 = cyl_to_rect_torch(torch.tensor(workloads.ics(n, seed=0), device="cuda"))
+    spec = ParamSpec(MWPotential2014, [(2, 0), (2, 1), (1, 1), (1, 2)])
+    theta0 = spec.theta0()
+    tab = _RawTable(spec.pot_type, spec.pot_args)
+    pert = torch.tensor(numpy.random.default_rng(7).normal(0.0, 0.01, (n, 6)), device="cuda")
+    per = {}
+    for method in ("symplec4_hip", "leapfrog_hip"):
+        target = flow_raw(tab, w0 + pert, dt, nsteps, method)
+
+        def gradient():
+            theta = torch.tensor(theta0, device="cuda", requires_grad=True)
+            w = w0.clone().requires_grad_(True)
+            loss = ((orbit_flow(w, theta, spec, dt, nsteps, method) - target) ** 2).sum()
+            loss.backward()
+            return loss, theta.grad, w.grad
+
+        for _ in range(args.warmup):
+            gradient()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            loss, gth, gw = gradient()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / args.steps
+
+        def L(th):
+            return ((flow_raw(_RawTable(*spec.packed(th)), w0, dt, nsteps, method) - target) ** 2).sum().item()
+
+        rel = []
+        for k in range(len(theta0)):
+            e = numpy.zeros_like(theta0)
+            e[k] = h
+            fd = (L(theta0 + e) - L(theta0 - e)) / (2 * h)
+            rel.append(abs(gth[k].item() - fd) / abs(fd))
+        per[method] = {"orbit_steps_per_s": n * nsteps / el, "ms_per_gradient": 1e3 * el, "loss": loss.item(),
+                       "fd_max_rel_err_params": max(rel), "grad_params": gth.tolist()}
+    v = per["symplec4_hip"]
+    print(json.dumps({"metric": "FP64 orbit-steps/s of loss + gradient (forward flow + reverse-time adjoint)",
+                      "value": v["orbit_steps_per_s"], "unit": "orbit-steps/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": v["ms_per_gradient"], "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "adjoint_mw2014_65536x2000", "orbits": n, "steps_per_orbit": nsteps,
+                                 "dt": dt, "params": "NFW amp, a; MiyamotoNagai a, b", "fd_h": h},
+                      "methods": per}), flush=True)
+
+
 def bench_copy(args):
     """A plain device-to-device copy of known size: the calibration run for
     the FETCH_SIZE / WRITE_SIZE hardware counters (bytes read and written by
@@ -244,6 +310,9 @@ def main():
         return
     if args.workload == "copy":
         bench_copy(args)
+        return
+    if args.workload.startswith("adjoint"):
+        bench_adjoint(args)
         return
 
     import torch
