@@ -248,10 +248,10 @@ __device__ __forceinline__ int gh_batch_reject_tile(const gh_batch_group& g,
 // branch); GEN = the instance also carries the run-time dispatch force for
 // other compositions, whose register demand would otherwise set the whole
 // kernel's occupancy.  Occupancy targets: 3 waves per SIMD with one orbit
-// per lane, 2 with two, 1 for the DOP853 instances.
+// per lane, 2 with two or with DOP853 (which then spills a few registers).
 template <int OPL, bool DOP, bool GEN>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(DOP ? 1 : 4 - OPL, DOP ? 1 : 4 - OPL)))
+__attribute__((amdgpu_waves_per_eu(DOP ? 2 : 4 - OPL, DOP ? 2 : 4 - OPL)))
 gh_batch_kernel(const gh_batch_group* __restrict__ groups, const gh_batch_tile* __restrict__ tiles,
                 int ntiles, int* __restrict__ tile_done,
                 unsigned long long* __restrict__ counter, int* __restrict__ remaining,
