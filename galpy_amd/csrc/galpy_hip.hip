@@ -547,7 +547,8 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
   __shared__ double qd[PILOT ? 1 : kJobDoubles * kDenseCap];
   __shared__ int qi[PILOT ? 1 : 2 * kDenseCap];
   int queued = 0;  // jobs in this wavefront's queue (the same in every lane)
-  volatile double* od = reinterpret_cast<volatile double*>(overflow + blockIdx.x * kOverflowBytes);
+  volatile double* od =
+      PILOT ? nullptr : reinterpret_cast<volatile double*>(overflow + blockIdx.x * kOverflowBytes);
   auto flush = [&]() {
     __syncthreads();  // the queue's stores are visible to every lane
     if (lane < queued) gh_job_run(F, qd, qi, lane, t, nt, indiv_t, result);
