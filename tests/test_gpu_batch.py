@@ -104,3 +104,34 @@ def test_autotune_returns_a_candidate(tmp_path, monkeypatch):
     assert opl in (1, 2) and set(timings) == {1, 2}
     opl2, _ = autotune(groups, sample_groups=8, budget=200)  # cached
     assert opl2 == opl
+
+
+def test_generic_composition_and_specialised_only_flag():
+    """A composition without a specialised force (Hernquist + NFW) runs in
+    the batch through the run-time dispatch force and matches the same group
+    alone; forcing the specialised-only kernel instead finishes exactly that
+    group with err = -1 and leaves the others unchanged."""
+    import torch
+
+    from galpy_amd import workloads
+    from galpy_amd.batching import GH_BATCH_SPECIALISED, Batch
+    from galpy_amd.potential import HernquistPotential, NFWPotential
+
+    t = numpy.linspace(0.0, 3.0, 7)
+    odd = dict(pot=[HernquistPotential(a=1.2), NFWPotential(a=4.0, amp=2.0)], yo=workloads.ics(70, seed=51), t=t,
+               method="symplec4_hip", dt=0.01)
+    mw = dict(pot=workloads.mw2014(), yo=workloads.ics(90, seed=52), t=t, method="leapfrog_hip", dt=0.01)
+    b = Batch([odd, mw])
+    assert not b.flags & GH_BATCH_SPECIALISED
+    b.run()
+    torch.cuda.synchronize()
+    for i, g in enumerate((odd, mw)):
+        assert (b.errs[i].cpu().numpy() == 0).all()
+        assert phase_diff(b.results[i].cpu().numpy(), _alone(g)) < 1e-12, i
+    b2 = Batch([odd, mw])
+    b2.flags |= GH_BATCH_SPECIALISED
+    b2.run()
+    torch.cuda.synchronize()
+    assert (b2.errs[0].cpu().numpy() == -1).all()
+    assert (b2.errs[1].cpu().numpy() == 0).all()
+    assert phase_diff(b2.results[1].cpu().numpy(), _alone(mw)) < 1e-12
