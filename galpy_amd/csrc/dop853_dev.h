@@ -122,18 +122,9 @@ struct GhDopState {
 
 // Start an orbit: row 0, first derivative and the initial step heuristic
 // (leung_dop853.c:276-325).  s.y holds the rectangular initial condition.
-// GH_DOP_OUTLINE moves the rarely executed parts (orbit start, dense output)
-// out of line to shrink the step loop's code; measured slower by default
-// because the calls pass stage vectors through scratch memory.
-#ifdef GH_DOP_OUTLINE
-#define GH_DOP_RARE __noinline__
-#else
-#define GH_DOP_RARE __forceinline__
-#endif
-
 // STORE = false (a pilot run that only counts steps) writes no output rows.
 template <class F, bool STORE = true>
-__device__ GH_DOP_RARE void gh_dop_init(const F& T,
+__device__ __forceinline__ void gh_dop_init(const F& T,
                                             const double* __restrict__ t, int nt, double rtol,
                                             double atol, GhDopState& s,
                                             double* __restrict__ out) {
@@ -198,7 +189,7 @@ struct GhDenseJob {
 // continuous extension at every requested time inside the step, written as
 // rows done0+1, done0+2, ...  Returns the number of rows written.
 template <class F, bool STORE = true>
-__device__ GH_DOP_RARE int gh_dop_dense(const F& T, const double* __restrict__ t, int nt,
+__device__ __forceinline__ int gh_dop_dense(const F& T, const double* __restrict__ t, int nt,
                                          const GhDenseJob& J, double* __restrict__ out) {
   using namespace ghdp;
   const double h = J.h;

@@ -418,25 +418,12 @@ __device__ __forceinline__ void gh_rect_force(const GhPotTable* __restrict__ T, 
 
 // Force functor handed to the integrators: the table plus the compile-time
 // composition.
-#ifdef GH_FORCE_NOINLINE
-template <class Spec>
-__device__ __noinline__ void gh_rect_force_call(const GhPotTable* __restrict__ T, double t,
-                                                double x, double y, double z, double& ax,
-                                                double& ay, double& az) {
-  gh_rect_force<Spec>(T, t, x, y, z, ax, ay, az);
-}
-#endif
-
 template <class Spec>
 struct GhForce {
   const GhPotTable* __restrict__ T;
   __device__ __forceinline__ void rect(double t, double x, double y, double z, double& ax,
                                        double& ay, double& az) const {
-#ifdef GH_FORCE_NOINLINE
-    gh_rect_force_call<Spec>(T, t, x, y, z, ax, ay, az);
-#else
     gh_rect_force<Spec>(T, t, x, y, z, ax, ay, az);
-#endif
   }
 };
 
