@@ -54,6 +54,10 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
     ),
+    "gh_planar_potential_upload": (
+        ctypes.c_int,
+        [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
+    ),
     "gh_potential_free": (ctypes.c_int, [ctypes.c_void_p]),
     "gh_potential_device_ptr": (ctypes.c_void_p, [ctypes.c_void_p]),
     "gh_energy_drift_hip": (
@@ -71,6 +75,11 @@ SIGNATURES = {
     "galpy_hip_last_error": (ctypes.c_char_p, []),
     "galpy_hip_set_interrupt": (None, [ctypes.c_int]),
 }
+
+
+# the planar entry points take the same arguments as the full ones
+SIGNATURES["integratePlanarOrbit"] = SIGNATURES["integrateFullOrbit"]
+SIGNATURES["integratePlanarOrbit_hip"] = SIGNATURES["integrateFullOrbit_hip"]
 
 
 def load(path=None):

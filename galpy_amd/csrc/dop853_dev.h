@@ -97,48 +97,50 @@ constexpr double uround = 2.3e-16;
 #include "dop853_rkn.h"  // Nystrom-form sums over this tableau (generated)
 
 // RHS of the first-order system: (v, a(q)) -- evalRectDeriv,
-// integrateFullOrbit.c:1326-1351
-template <class F>
+// integrateFullOrbit.c:1326-1351 (evalPlanarRectDeriv,
+// integratePlanarOrbit.c:1032-1055, for planar orbits)
+template <class F, class V>
 __device__ __forceinline__ void gh_deriv(const F& T, double t, const double* y, double* k) {
-  k[0] = y[3];
-  k[1] = y[4];
-  k[2] = y[5];
-  T.rect(t, y[0], y[1], y[2], k[3], k[4], k[5]);
+  GhPhase<V>::set(k, GhPhase<V>::p(y), gh_accel(T, t, GhPhase<V>::q(y)));
 }
 
 __device__ __forceinline__ double gh_csign(double x, double y) {
   return (y > 0.0) ? fabs(x) : -fabs(x);
 }
 
-// Per-lane DOP853 state between steps (leung_dop853.c:231-517 locals).
-struct GhDopState {
-  double y[6];   // current state (galpy's y0)
-  double k1[6];  // derivative at y (FSAL)
+// Per-lane DOP853 state between steps (leung_dop853.c:231-517 locals), for
+// a full (V = GhVec3, dim 6) or planar (V = GhVec2, dim 4) orbit.
+template <class V>
+struct GhDopStateT {
+  double y[GhPhase<V>::N];   // current state (galpy's y0)
+  double k1[GhPhase<V>::N];  // derivative at y (FSAL)
   double h, t_cur, t_old, t_old_older, hmax, pos_neg;
   int done;      // output rows written after row 0
   int reject;
   long long nacc, nrej;
 };
+using GhDopState = GhDopStateT<GhVec3>;
 
 // Start an orbit: row 0, first derivative and the initial step heuristic
 // (leung_dop853.c:276-325).  s.y holds the rectangular initial condition.
 // STORE = false (a pilot run that only counts steps) writes no output rows.
-template <class F, bool STORE = true>
+template <class F, bool STORE = true, class V>
 __device__ __forceinline__ void gh_dop_init(const F& T,
                                             const double* __restrict__ t, int nt, double rtol,
-                                            double atol, GhDopState& s,
+                                            double atol, GhDopStateT<V>& s,
                                             double* __restrict__ out) {
+  constexpr int N = GhPhase<V>::N;
   s.hmax = t[nt - 1] - t[0];
   s.pos_neg = gh_csign(1.0, s.hmax);
   s.done = 0;
   s.reject = 0;
   s.nacc = 0;
   s.nrej = 0;
-  if (STORE) gh_store_cyl(out, s.y[0], s.y[1], s.y[2], s.y[3], s.y[4], s.y[5]);
-  gh_deriv(T, t[0], s.y, s.k1);
+  if (STORE) GhPhase<V>::store(out, GhPhase<V>::q(s.y), GhPhase<V>::p(s.y));
+  gh_deriv<F, V>(T, t[0], s.y, s.k1);
   double dnf = 0.0, dny = 0.0;
 #pragma unroll
-  for (int i = 0; i < 6; i++) {
+  for (int i = 0; i < N; i++) {
     const double sk = atol + rtol * fabs(s.y[i]);
     double sqr = s.k1[i] / sk;
     dnf += sqr * sqr;
@@ -146,13 +148,13 @@ __device__ __forceinline__ void gh_dop_init(const F& T,
     dny += sqr * sqr;
   }
   double h = gh_csign(fmin(sqrt(dny / dnf) * 0.01, fabs(s.hmax)), s.pos_neg);
-  double k2[6], k3[6];
+  double k2[N], k3[N];
 #pragma unroll
-  for (int i = 0; i < 6; i++) k3[i] = s.y[i] + h * s.k1[i];
-  gh_deriv(T, t[0] + h, k3, k2);
+  for (int i = 0; i < N; i++) k3[i] = s.y[i] + h * s.k1[i];
+  gh_deriv<F, V>(T, t[0] + h, k3, k2);
   double der2 = 0.0;
 #pragma unroll
-  for (int i = 0; i < 6; i++) {
+  for (int i = 0; i < N; i++) {
     const double sk = atol + rtol * fabs(s.y[i]);
     const double sqr = (k2[i] - s.k1[i]) / sk;
     der2 += sqr * sqr;
@@ -178,9 +180,10 @@ __device__ __forceinline__ void gh_dop_init(const F& T,
 // continuous extension uses (A1, A4..A13; A[j] of stage j, A13 = FSAL), the
 // step's start time and size, and the number of output rows written before
 // it.  A lane can hand this record to another lane (deferred dense output).
-struct GhDenseJob {
-  GhVec3 q0, p0, qn, pn;
-  GhVec3 A[11];  // stages 1, 4, 5, ..., 13
+template <class V>
+struct GhDenseJobT {
+  V q0, p0, qn, pn;
+  V A[11];  // stages 1, 4, 5, ..., 13
   double t_old, h;
   int done0;
 };
@@ -188,16 +191,17 @@ struct GhDenseJob {
 // Dense output of one accepted step: the three extra stages 14-16 and the
 // continuous extension at every requested time inside the step, written as
 // rows done0+1, done0+2, ...  Returns the number of rows written.
-template <class F, bool STORE = true>
+template <class F, bool STORE = true, class V>
 __device__ __forceinline__ int gh_dop_dense(const F& T, const double* __restrict__ t, int nt,
-                                         const GhDenseJob& J, double* __restrict__ out) {
+                                         const GhDenseJobT<V>& J, double* __restrict__ out) {
   using namespace ghdp;
+  constexpr int N = GhPhase<V>::N;
   const double h = J.h;
-  GhVec3 A[17];
+  V A[17];
   A[1] = J.A[0];
 #pragma unroll
   for (int j = 4; j <= 13; j++) A[j] = J.A[j - 3];
-  A[2] = A[3] = GhVec3{0.0, 0.0, 0.0};  // unused by stages 14-16 and the extension
+  A[2] = A[3] = V{};  // unused by stages 14-16 and the extension
   A[14] = gh_accel(T, J.t_old + c14 * h, gh_rkn_q14(h, J.q0, J.p0, J.pn, A));
   A[15] = gh_accel(T, J.t_old + c15 * h, gh_rkn_q15(h, J.q0, J.p0, J.pn, A));
   A[16] = gh_accel(T, J.t_old + c16 * h, gh_rkn_q16(h, J.q0, J.p0, J.pn, A));
@@ -210,10 +214,10 @@ __device__ __forceinline__ int gh_dop_dense(const F& T, const double* __restrict
   while (done < nt - 1 && pos_neg * t[done + 1] < pos_neg * t_cur) {
     const double sd = (t[done + 1] - J.t_old) / h;
     const double s1 = 1.0 - sd;
-    double yo[6];
+    double yo[N];
     gh_rkn_dense_eval(h, sd, s1, J.q0, J.p0, J.qn, J.pn, A, yo);
     done++;
-    if (STORE) gh_store_cyl(out + 6L * done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
+    if (STORE) GhPhase<V>::store(out + (long)N * done, GhPhase<V>::q(yo), GhPhase<V>::p(yo));
   }
   return done - J.done0;
 }
@@ -226,7 +230,8 @@ enum GhDenseMode {
 };
 
 struct GhNoSink {
-  __device__ void operator()(const GhDenseJob&) const {}
+  template <class J>
+  __device__ void operator()(const J&) const {}
 };
 
 // One attempted step (leung_dop853.c:335-492); returns true once every
@@ -234,25 +239,26 @@ struct GhNoSink {
 // (from inside the accept branch, by exactly the lanes whose step passed
 // output times) as soon as the job is formed, and s.done already counts its
 // rows.
-template <class F, int DENSE = GH_DENSE_INLINE, class Sink = GhNoSink>
+template <class F, int DENSE = GH_DENSE_INLINE, class Sink = GhNoSink, class V>
 __device__ __forceinline__ bool gh_dop_step(const F& T,
                                             const double* __restrict__ t, int nt, double rtol,
-                                            double atol, GhDopState& s,
+                                            double atol, GhDopStateT<V>& s,
                                             double* __restrict__ out,
                                             const Sink& sink = Sink()) {
   using namespace ghdp;
+  constexpr int N = GhPhase<V>::N;
   const double safe = 0.9;
   const double facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;
   const double h = s.pos_neg * fmax(fabs(s.h), 1e3 * uround);
   const double tc = s.t_cur;
-  const GhVec3 q0{s.y[0], s.y[1], s.y[2]}, p0{s.y[3], s.y[4], s.y[5]};
-  const GhVec3 none{0.0, 0.0, 0.0};
+  const V q0 = GhPhase<V>::q(s.y), p0 = GhPhase<V>::p(s.y);
+  const V none{};
   // galpy's dop853 ignores the result of the twelve stages when rejected, so
   // galpy's accept/reject flow is kept exactly; only the accept branch is
   // masked per lane.  (galpy also tracks facold for Lund stabilisation, but
   // with beta = 0 pow(facold, beta) is exactly 1 and never enters a result.)
-  GhVec3 A[17];
-  A[1] = GhVec3{s.k1[3], s.k1[4], s.k1[5]};
+  V A[17];
+  A[1] = GhPhase<V>::p(s.k1);
   A[2] = gh_accel(T, tc + c2 * h, gh_rkn_q2(h, q0, p0, none, A));
   A[3] = gh_accel(T, tc + c3 * h, gh_rkn_q3(h, q0, p0, none, A));
   A[4] = gh_accel(T, tc + c4 * h, gh_rkn_q4(h, q0, p0, none, A));
@@ -263,17 +269,17 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
   A[9] = gh_accel(T, tc + c9 * h, gh_rkn_q9(h, q0, p0, none, A));
   A[10] = gh_accel(T, tc + c10 * h, gh_rkn_q10(h, q0, p0, none, A));
   A[11] = gh_accel(T, tc + c11 * h, gh_rkn_q11(h, q0, p0, none, A));
-  const GhVec3 q12 = gh_rkn_q12(h, q0, p0, none, A);
+  const V q12 = gh_rkn_q12(h, q0, p0, none, A);
   s.t_old_older = s.t_old;
   s.t_old = tc;
   s.t_cur = tc + h;
   A[12] = gh_accel(T, s.t_cur, q12);
-  GhVec3 qn, pn;
+  V qn, pn;
   double err5, err3;
   gh_rkn_solution(h, q0, p0, A, atol, rtol, qn, pn, err5, err3);
   double deno = err5 + 0.01 * err3;
   if (deno <= 0.0) deno = 1.0;
-  const double errv = fabs(h) * err5 * sqrt(1.0 / (deno * 6.0));
+  const double errv = fabs(h) * err5 * sqrt(1.0 / (deno * (double)N));
   // errv^(1/8) (leung_dop853.c uses pow) as three square roots
   const double fac11 = sqrt(sqrt(sqrt(errv)));
   const double fac = fmax(facc2, fmin(facc1, fac11 / safe));
@@ -288,7 +294,7 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
       if (DENSE == GH_DENSE_SKIP) {
         while (s.done < nt - 1 && s.pos_neg * t[s.done + 1] < s.pos_neg * s.t_cur) s.done++;
       } else {
-        GhDenseJob J;
+        GhDenseJobT<V> J;
         J.q0 = q0;
         J.p0 = p0;
         J.qn = qn;
@@ -307,10 +313,8 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
         }
       }
     }
-    s.y[0] = qn.x; s.y[1] = qn.y; s.y[2] = qn.z;
-    s.y[3] = pn.x; s.y[4] = pn.y; s.y[5] = pn.z;
-    s.k1[0] = pn.x; s.k1[1] = pn.y; s.k1[2] = pn.z;
-    s.k1[3] = A[13].x; s.k1[4] = A[13].y; s.k1[5] = A[13].z;
+    GhPhase<V>::set(s.y, qn, pn);
+    GhPhase<V>::set(s.k1, pn, A[13]);
     hnew = (fabs(hnew) > fabs(s.hmax)) ? s.pos_neg * s.hmax : hnew;
     if (s.reject) hnew = s.pos_neg * fmin(fabs(hnew), fabs(h));
     s.reject = 0;

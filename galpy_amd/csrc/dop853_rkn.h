@@ -490,3 +490,243 @@ __device__ __forceinline__ void gh_rkn_dense_eval(double h, double sd, double s1
     yo[5] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
   }
 }
+// position argument of stage 2
+__device__ __forceinline__ GhVec2 gh_rkn_q2(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp2 * p0.x);
+  Q.y = q0.y + h * (ghrk::qp2 * p0.y);
+  return Q;
+}
+
+// position argument of stage 3
+__device__ __forceinline__ GhVec2 gh_rkn_q3(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp3 * p0.x + h * (ghrk::qa3_1 * A[1].x));
+  Q.y = q0.y + h * (ghrk::qp3 * p0.y + h * (ghrk::qa3_1 * A[1].y));
+  return Q;
+}
+
+// position argument of stage 4
+__device__ __forceinline__ GhVec2 gh_rkn_q4(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp4 * p0.x + h * (ghrk::qa4_1 * A[1].x + ghrk::qa4_2 * A[2].x));
+  Q.y = q0.y + h * (ghrk::qp4 * p0.y + h * (ghrk::qa4_1 * A[1].y + ghrk::qa4_2 * A[2].y));
+  return Q;
+}
+
+// position argument of stage 5
+__device__ __forceinline__ GhVec2 gh_rkn_q5(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp5 * p0.x + h * (ghrk::qa5_1 * A[1].x + ghrk::qa5_2 * A[2].x + ghrk::qa5_3 * A[3].x));
+  Q.y = q0.y + h * (ghrk::qp5 * p0.y + h * (ghrk::qa5_1 * A[1].y + ghrk::qa5_2 * A[2].y + ghrk::qa5_3 * A[3].y));
+  return Q;
+}
+
+// position argument of stage 6
+__device__ __forceinline__ GhVec2 gh_rkn_q6(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp6 * p0.x + h * (ghrk::qa6_1 * A[1].x + ghrk::qa6_3 * A[3].x + ghrk::qa6_4 * A[4].x));
+  Q.y = q0.y + h * (ghrk::qp6 * p0.y + h * (ghrk::qa6_1 * A[1].y + ghrk::qa6_3 * A[3].y + ghrk::qa6_4 * A[4].y));
+  return Q;
+}
+
+// position argument of stage 7
+__device__ __forceinline__ GhVec2 gh_rkn_q7(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp7 * p0.x + h * (ghrk::qa7_1 * A[1].x + ghrk::qa7_3 * A[3].x + ghrk::qa7_4 * A[4].x + ghrk::qa7_5 * A[5].x));
+  Q.y = q0.y + h * (ghrk::qp7 * p0.y + h * (ghrk::qa7_1 * A[1].y + ghrk::qa7_3 * A[3].y + ghrk::qa7_4 * A[4].y + ghrk::qa7_5 * A[5].y));
+  return Q;
+}
+
+// position argument of stage 8
+__device__ __forceinline__ GhVec2 gh_rkn_q8(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp8 * p0.x + h * (ghrk::qa8_1 * A[1].x + ghrk::qa8_3 * A[3].x + ghrk::qa8_4 * A[4].x + ghrk::qa8_5 * A[5].x + ghrk::qa8_6 * A[6].x));
+  Q.y = q0.y + h * (ghrk::qp8 * p0.y + h * (ghrk::qa8_1 * A[1].y + ghrk::qa8_3 * A[3].y + ghrk::qa8_4 * A[4].y + ghrk::qa8_5 * A[5].y + ghrk::qa8_6 * A[6].y));
+  return Q;
+}
+
+// position argument of stage 9
+__device__ __forceinline__ GhVec2 gh_rkn_q9(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp9 * p0.x + h * (ghrk::qa9_1 * A[1].x + ghrk::qa9_3 * A[3].x + ghrk::qa9_4 * A[4].x + ghrk::qa9_5 * A[5].x + ghrk::qa9_6 * A[6].x + ghrk::qa9_7 * A[7].x));
+  Q.y = q0.y + h * (ghrk::qp9 * p0.y + h * (ghrk::qa9_1 * A[1].y + ghrk::qa9_3 * A[3].y + ghrk::qa9_4 * A[4].y + ghrk::qa9_5 * A[5].y + ghrk::qa9_6 * A[6].y + ghrk::qa9_7 * A[7].y));
+  return Q;
+}
+
+// position argument of stage 10
+__device__ __forceinline__ GhVec2 gh_rkn_q10(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp10 * p0.x + h * (ghrk::qa10_1 * A[1].x + ghrk::qa10_3 * A[3].x + ghrk::qa10_4 * A[4].x + ghrk::qa10_5 * A[5].x + ghrk::qa10_6 * A[6].x + ghrk::qa10_7 * A[7].x + ghrk::qa10_8 * A[8].x));
+  Q.y = q0.y + h * (ghrk::qp10 * p0.y + h * (ghrk::qa10_1 * A[1].y + ghrk::qa10_3 * A[3].y + ghrk::qa10_4 * A[4].y + ghrk::qa10_5 * A[5].y + ghrk::qa10_6 * A[6].y + ghrk::qa10_7 * A[7].y + ghrk::qa10_8 * A[8].y));
+  return Q;
+}
+
+// position argument of stage 11
+__device__ __forceinline__ GhVec2 gh_rkn_q11(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp11 * p0.x + h * (ghrk::qa11_1 * A[1].x + ghrk::qa11_3 * A[3].x + ghrk::qa11_4 * A[4].x + ghrk::qa11_5 * A[5].x + ghrk::qa11_6 * A[6].x + ghrk::qa11_7 * A[7].x + ghrk::qa11_8 * A[8].x + ghrk::qa11_9 * A[9].x));
+  Q.y = q0.y + h * (ghrk::qp11 * p0.y + h * (ghrk::qa11_1 * A[1].y + ghrk::qa11_3 * A[3].y + ghrk::qa11_4 * A[4].y + ghrk::qa11_5 * A[5].y + ghrk::qa11_6 * A[6].y + ghrk::qa11_7 * A[7].y + ghrk::qa11_8 * A[8].y + ghrk::qa11_9 * A[9].y));
+  return Q;
+}
+
+// position argument of stage 12
+__device__ __forceinline__ GhVec2 gh_rkn_q12(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp12 * p0.x + h * (ghrk::qa12_1 * A[1].x + ghrk::qa12_3 * A[3].x + ghrk::qa12_4 * A[4].x + ghrk::qa12_5 * A[5].x + ghrk::qa12_6 * A[6].x + ghrk::qa12_7 * A[7].x + ghrk::qa12_8 * A[8].x + ghrk::qa12_9 * A[9].x + ghrk::qa12_10 * A[10].x));
+  Q.y = q0.y + h * (ghrk::qp12 * p0.y + h * (ghrk::qa12_1 * A[1].y + ghrk::qa12_3 * A[3].y + ghrk::qa12_4 * A[4].y + ghrk::qa12_5 * A[5].y + ghrk::qa12_6 * A[6].y + ghrk::qa12_7 * A[7].y + ghrk::qa12_8 * A[8].y + ghrk::qa12_9 * A[9].y + ghrk::qa12_10 * A[10].y));
+  return Q;
+}
+
+// position argument of stage 14
+__device__ __forceinline__ GhVec2 gh_rkn_q14(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp14 * p0.x + ghrk::qn14 * pn.x + h * (ghrk::qa14_1 * A[1].x + ghrk::qa14_4 * A[4].x + ghrk::qa14_5 * A[5].x + ghrk::qa14_6 * A[6].x + ghrk::qa14_7 * A[7].x + ghrk::qa14_8 * A[8].x + ghrk::qa14_9 * A[9].x + ghrk::qa14_10 * A[10].x + ghrk::qa14_11 * A[11].x));
+  Q.y = q0.y + h * (ghrk::qp14 * p0.y + ghrk::qn14 * pn.y + h * (ghrk::qa14_1 * A[1].y + ghrk::qa14_4 * A[4].y + ghrk::qa14_5 * A[5].y + ghrk::qa14_6 * A[6].y + ghrk::qa14_7 * A[7].y + ghrk::qa14_8 * A[8].y + ghrk::qa14_9 * A[9].y + ghrk::qa14_10 * A[10].y + ghrk::qa14_11 * A[11].y));
+  return Q;
+}
+
+// position argument of stage 15
+__device__ __forceinline__ GhVec2 gh_rkn_q15(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp15 * p0.x + ghrk::qn15 * pn.x + h * (ghrk::qa15_1 * A[1].x + ghrk::qa15_4 * A[4].x + ghrk::qa15_5 * A[5].x + ghrk::qa15_6 * A[6].x + ghrk::qa15_7 * A[7].x + ghrk::qa15_8 * A[8].x + ghrk::qa15_9 * A[9].x + ghrk::qa15_10 * A[10].x + ghrk::qa15_11 * A[11].x + ghrk::qa15_12 * A[12].x + ghrk::qa15_13 * A[13].x));
+  Q.y = q0.y + h * (ghrk::qp15 * p0.y + ghrk::qn15 * pn.y + h * (ghrk::qa15_1 * A[1].y + ghrk::qa15_4 * A[4].y + ghrk::qa15_5 * A[5].y + ghrk::qa15_6 * A[6].y + ghrk::qa15_7 * A[7].y + ghrk::qa15_8 * A[8].y + ghrk::qa15_9 * A[9].y + ghrk::qa15_10 * A[10].y + ghrk::qa15_11 * A[11].y + ghrk::qa15_12 * A[12].y + ghrk::qa15_13 * A[13].y));
+  return Q;
+}
+
+// position argument of stage 16
+__device__ __forceinline__ GhVec2 gh_rkn_q16(double h, const GhVec2& q0, const GhVec2& p0,
+                                             const GhVec2& pn, const GhVec2* A) {
+  (void)pn;
+  GhVec2 Q;
+  Q.x = q0.x + h * (ghrk::qp16 * p0.x + ghrk::qn16 * pn.x + h * (ghrk::qa16_1 * A[1].x + ghrk::qa16_4 * A[4].x + ghrk::qa16_5 * A[5].x + ghrk::qa16_6 * A[6].x + ghrk::qa16_7 * A[7].x + ghrk::qa16_8 * A[8].x + ghrk::qa16_9 * A[9].x + ghrk::qa16_10 * A[10].x + ghrk::qa16_11 * A[11].x + ghrk::qa16_12 * A[12].x + ghrk::qa16_13 * A[13].x + ghrk::qa16_14 * A[14].x));
+  Q.y = q0.y + h * (ghrk::qp16 * p0.y + ghrk::qn16 * pn.y + h * (ghrk::qa16_1 * A[1].y + ghrk::qa16_4 * A[4].y + ghrk::qa16_5 * A[5].y + ghrk::qa16_6 * A[6].y + ghrk::qa16_7 * A[7].y + ghrk::qa16_8 * A[8].y + ghrk::qa16_9 * A[9].y + ghrk::qa16_10 * A[10].y + ghrk::qa16_11 * A[11].y + ghrk::qa16_12 * A[12].y + ghrk::qa16_13 * A[13].y + ghrk::qa16_14 * A[14].y));
+  return Q;
+}
+
+// New state (q_new, p_new) and the two error sums of leung_dop853.c:376-404:
+// per component, sk = atol + rtol max(|y0|, |y_new|), err3 += (e3/sk)^2,
+// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k (one
+// reciprocal of sk per component; the norm only steers the step size).
+__device__ __forceinline__ void gh_rkn_solution(double h, const GhVec2& q0, const GhVec2& p0,
+                                               const GhVec2* A, double atol, double rtol,
+                                               GhVec2& qn, GhVec2& pn, double& err5,
+                                               double& err3) {
+  err5 = 0.0;
+  err3 = 0.0;
+  {
+    const double kq = ghrk::bp * p0.x + h * (ghrk::ba_1 * A[1].x + ghrk::ba_4 * A[4].x + ghrk::ba_5 * A[5].x + ghrk::ba_6 * A[6].x + ghrk::ba_7 * A[7].x + ghrk::ba_8 * A[8].x + ghrk::ba_9 * A[9].x + ghrk::ba_10 * A[10].x + ghrk::ba_11 * A[11].x);
+    qn.x = q0.x + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.x), fabs(qn.x)));
+    double sqr = (ghrk::e3p * p0.x + h * (ghrk::e3a_1 * A[1].x + ghrk::e3a_4 * A[4].x + ghrk::e3a_5 * A[5].x + ghrk::e3a_6 * A[6].x + ghrk::e3a_7 * A[7].x + ghrk::e3a_8 * A[8].x + ghrk::e3a_9 * A[9].x + ghrk::e3a_10 * A[10].x + ghrk::e3a_11 * A[11].x)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.x + h * (ghrk::e5a_1 * A[1].x + ghrk::e5a_4 * A[4].x + ghrk::e5a_5 * A[5].x + ghrk::e5a_6 * A[6].x + ghrk::e5a_7 * A[7].x + ghrk::e5a_8 * A[8].x + ghrk::e5a_9 * A[9].x + ghrk::e5a_10 * A[10].x + ghrk::e5a_11 * A[11].x)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kq = ghrk::bp * p0.y + h * (ghrk::ba_1 * A[1].y + ghrk::ba_4 * A[4].y + ghrk::ba_5 * A[5].y + ghrk::ba_6 * A[6].y + ghrk::ba_7 * A[7].y + ghrk::ba_8 * A[8].y + ghrk::ba_9 * A[9].y + ghrk::ba_10 * A[10].y + ghrk::ba_11 * A[11].y);
+    qn.y = q0.y + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.y), fabs(qn.y)));
+    double sqr = (ghrk::e3p * p0.y + h * (ghrk::e3a_1 * A[1].y + ghrk::e3a_4 * A[4].y + ghrk::e3a_5 * A[5].y + ghrk::e3a_6 * A[6].y + ghrk::e3a_7 * A[7].y + ghrk::e3a_8 * A[8].y + ghrk::e3a_9 * A[9].y + ghrk::e3a_10 * A[10].y + ghrk::e3a_11 * A[11].y)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.y + h * (ghrk::e5a_1 * A[1].y + ghrk::e5a_4 * A[4].y + ghrk::e5a_5 * A[5].y + ghrk::e5a_6 * A[6].y + ghrk::e5a_7 * A[7].y + ghrk::e5a_8 * A[8].y + ghrk::e5a_9 * A[9].y + ghrk::e5a_10 * A[10].y + ghrk::e5a_11 * A[11].y)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].x + ghdp::b6 * A[6].x + ghdp::b7 * A[7].x + ghdp::b8 * A[8].x + ghdp::b9 * A[9].x + ghdp::b10 * A[10].x + ghdp::b11 * A[11].x + ghdp::b12 * A[12].x;
+    pn.x = p0.x + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.x), fabs(pn.x)));
+    double sqr = (kp - ghdp::bhh1 * A[1].x - ghdp::bhh2 * A[9].x - ghdp::bhh3 * A[12].x) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].x + ghdp::er6 * A[6].x + ghdp::er7 * A[7].x + ghdp::er8 * A[8].x + ghdp::er9 * A[9].x + ghdp::er10 * A[10].x + ghdp::er11 * A[11].x + ghdp::er12 * A[12].x) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].y + ghdp::b6 * A[6].y + ghdp::b7 * A[7].y + ghdp::b8 * A[8].y + ghdp::b9 * A[9].y + ghdp::b10 * A[10].y + ghdp::b11 * A[11].y + ghdp::b12 * A[12].y;
+    pn.y = p0.y + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.y), fabs(pn.y)));
+    double sqr = (kp - ghdp::bhh1 * A[1].y - ghdp::bhh2 * A[9].y - ghdp::bhh3 * A[12].y) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].y + ghdp::er6 * A[6].y + ghdp::er7 * A[7].y + ghdp::er8 * A[8].y + ghdp::er9 * A[9].y + ghdp::er10 * A[10].y + ghdp::er11 * A[11].y + ghdp::er12 * A[12].y) * isk;
+    err5 += sqr * sqr;
+  }
+}
+
+// Dense output (leung_dop853.c:430-474) of the accepted step at the time
+// t_old + sd h (s1 = 1 - sd), after the dense stages 14-16: per component
+// the coefficients rc1..rc8 of the continuous extension, then
+// rc1 + sd (rc2 + s1 (rc3 + sd (rc4 + s1 (rc5 + sd (rc6 + s1 (rc7 + sd rc8)))))).
+__device__ __forceinline__ void gh_rkn_dense_eval(double h, double sd, double s1, const GhVec2& q0,
+                                                 const GhVec2& p0, const GhVec2& qn,
+                                                 const GhVec2& pn, const GhVec2* A, double* yo) {
+  {
+    const double ydiff = qn.x - q0.x;
+    const double bspl = h * p0.x - ydiff;
+    const double r0 = q0.x, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.x - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.x + ghrk::dn4 * pn.x + h * (ghrk::da4_1 * A[1].x + ghrk::da4_4 * A[4].x + ghrk::da4_5 * A[5].x + ghrk::da4_6 * A[6].x + ghrk::da4_7 * A[7].x + ghrk::da4_8 * A[8].x + ghrk::da4_9 * A[9].x + ghrk::da4_10 * A[10].x + ghrk::da4_11 * A[11].x + ghrk::da4_12 * A[12].x + ghrk::da4_13 * A[13].x + ghrk::da4_14 * A[14].x + ghrk::da4_15 * A[15].x));
+    const double r5 = h * (ghrk::dp5 * p0.x + ghrk::dn5 * pn.x + h * (ghrk::da5_1 * A[1].x + ghrk::da5_4 * A[4].x + ghrk::da5_5 * A[5].x + ghrk::da5_6 * A[6].x + ghrk::da5_7 * A[7].x + ghrk::da5_8 * A[8].x + ghrk::da5_9 * A[9].x + ghrk::da5_10 * A[10].x + ghrk::da5_11 * A[11].x + ghrk::da5_12 * A[12].x + ghrk::da5_13 * A[13].x + ghrk::da5_14 * A[14].x + ghrk::da5_15 * A[15].x));
+    const double r6 = h * (ghrk::dp6 * p0.x + ghrk::dn6 * pn.x + h * (ghrk::da6_1 * A[1].x + ghrk::da6_4 * A[4].x + ghrk::da6_5 * A[5].x + ghrk::da6_6 * A[6].x + ghrk::da6_7 * A[7].x + ghrk::da6_8 * A[8].x + ghrk::da6_9 * A[9].x + ghrk::da6_10 * A[10].x + ghrk::da6_11 * A[11].x + ghrk::da6_12 * A[12].x + ghrk::da6_13 * A[13].x + ghrk::da6_14 * A[14].x + ghrk::da6_15 * A[15].x));
+    const double r7 = h * (ghrk::dp7 * p0.x + ghrk::dn7 * pn.x + h * (ghrk::da7_1 * A[1].x + ghrk::da7_4 * A[4].x + ghrk::da7_5 * A[5].x + ghrk::da7_6 * A[6].x + ghrk::da7_7 * A[7].x + ghrk::da7_8 * A[8].x + ghrk::da7_9 * A[9].x + ghrk::da7_10 * A[10].x + ghrk::da7_11 * A[11].x + ghrk::da7_12 * A[12].x + ghrk::da7_13 * A[13].x + ghrk::da7_14 * A[14].x + ghrk::da7_15 * A[15].x));
+    yo[0] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = qn.y - q0.y;
+    const double bspl = h * p0.y - ydiff;
+    const double r0 = q0.y, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.y - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.y + ghrk::dn4 * pn.y + h * (ghrk::da4_1 * A[1].y + ghrk::da4_4 * A[4].y + ghrk::da4_5 * A[5].y + ghrk::da4_6 * A[6].y + ghrk::da4_7 * A[7].y + ghrk::da4_8 * A[8].y + ghrk::da4_9 * A[9].y + ghrk::da4_10 * A[10].y + ghrk::da4_11 * A[11].y + ghrk::da4_12 * A[12].y + ghrk::da4_13 * A[13].y + ghrk::da4_14 * A[14].y + ghrk::da4_15 * A[15].y));
+    const double r5 = h * (ghrk::dp5 * p0.y + ghrk::dn5 * pn.y + h * (ghrk::da5_1 * A[1].y + ghrk::da5_4 * A[4].y + ghrk::da5_5 * A[5].y + ghrk::da5_6 * A[6].y + ghrk::da5_7 * A[7].y + ghrk::da5_8 * A[8].y + ghrk::da5_9 * A[9].y + ghrk::da5_10 * A[10].y + ghrk::da5_11 * A[11].y + ghrk::da5_12 * A[12].y + ghrk::da5_13 * A[13].y + ghrk::da5_14 * A[14].y + ghrk::da5_15 * A[15].y));
+    const double r6 = h * (ghrk::dp6 * p0.y + ghrk::dn6 * pn.y + h * (ghrk::da6_1 * A[1].y + ghrk::da6_4 * A[4].y + ghrk::da6_5 * A[5].y + ghrk::da6_6 * A[6].y + ghrk::da6_7 * A[7].y + ghrk::da6_8 * A[8].y + ghrk::da6_9 * A[9].y + ghrk::da6_10 * A[10].y + ghrk::da6_11 * A[11].y + ghrk::da6_12 * A[12].y + ghrk::da6_13 * A[13].y + ghrk::da6_14 * A[14].y + ghrk::da6_15 * A[15].y));
+    const double r7 = h * (ghrk::dp7 * p0.y + ghrk::dn7 * pn.y + h * (ghrk::da7_1 * A[1].y + ghrk::da7_4 * A[4].y + ghrk::da7_5 * A[5].y + ghrk::da7_6 * A[6].y + ghrk::da7_7 * A[7].y + ghrk::da7_8 * A[8].y + ghrk::da7_9 * A[9].y + ghrk::da7_10 * A[10].y + ghrk::da7_11 * A[11].y + ghrk::da7_12 * A[12].y + ghrk::da7_13 * A[13].y + ghrk::da7_14 * A[14].y + ghrk::da7_15 * A[15].y));
+    yo[1] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.x - p0.x;
+    const double bspl = h * A[1].x - ydiff;
+    const double r0 = p0.x, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].x - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].x + ghdp::d46 * A[6].x + ghdp::d47 * A[7].x + ghdp::d48 * A[8].x + ghdp::d49 * A[9].x + ghdp::d410 * A[10].x + ghdp::d411 * A[11].x + ghdp::d412 * A[12].x + ghdp::d413 * A[13].x + ghdp::d414 * A[14].x + ghdp::d415 * A[15].x + ghdp::d416 * A[16].x);
+    const double r5 = h * (ghdp::d51 * A[1].x + ghdp::d56 * A[6].x + ghdp::d57 * A[7].x + ghdp::d58 * A[8].x + ghdp::d59 * A[9].x + ghdp::d510 * A[10].x + ghdp::d511 * A[11].x + ghdp::d512 * A[12].x + ghdp::d513 * A[13].x + ghdp::d514 * A[14].x + ghdp::d515 * A[15].x + ghdp::d516 * A[16].x);
+    const double r6 = h * (ghdp::d61 * A[1].x + ghdp::d66 * A[6].x + ghdp::d67 * A[7].x + ghdp::d68 * A[8].x + ghdp::d69 * A[9].x + ghdp::d610 * A[10].x + ghdp::d611 * A[11].x + ghdp::d612 * A[12].x + ghdp::d613 * A[13].x + ghdp::d614 * A[14].x + ghdp::d615 * A[15].x + ghdp::d616 * A[16].x);
+    const double r7 = h * (ghdp::d71 * A[1].x + ghdp::d76 * A[6].x + ghdp::d77 * A[7].x + ghdp::d78 * A[8].x + ghdp::d79 * A[9].x + ghdp::d710 * A[10].x + ghdp::d711 * A[11].x + ghdp::d712 * A[12].x + ghdp::d713 * A[13].x + ghdp::d714 * A[14].x + ghdp::d715 * A[15].x + ghdp::d716 * A[16].x);
+    yo[2] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.y - p0.y;
+    const double bspl = h * A[1].y - ydiff;
+    const double r0 = p0.y, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].y - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].y + ghdp::d46 * A[6].y + ghdp::d47 * A[7].y + ghdp::d48 * A[8].y + ghdp::d49 * A[9].y + ghdp::d410 * A[10].y + ghdp::d411 * A[11].y + ghdp::d412 * A[12].y + ghdp::d413 * A[13].y + ghdp::d414 * A[14].y + ghdp::d415 * A[15].y + ghdp::d416 * A[16].y);
+    const double r5 = h * (ghdp::d51 * A[1].y + ghdp::d56 * A[6].y + ghdp::d57 * A[7].y + ghdp::d58 * A[8].y + ghdp::d59 * A[9].y + ghdp::d510 * A[10].y + ghdp::d511 * A[11].y + ghdp::d512 * A[12].y + ghdp::d513 * A[13].y + ghdp::d514 * A[14].y + ghdp::d515 * A[15].y + ghdp::d516 * A[16].y);
+    const double r6 = h * (ghdp::d61 * A[1].y + ghdp::d66 * A[6].y + ghdp::d67 * A[7].y + ghdp::d68 * A[8].y + ghdp::d69 * A[9].y + ghdp::d610 * A[10].y + ghdp::d611 * A[11].y + ghdp::d612 * A[12].y + ghdp::d613 * A[13].y + ghdp::d614 * A[14].y + ghdp::d615 * A[15].y + ghdp::d616 * A[16].y);
+    const double r7 = h * (ghdp::d71 * A[1].y + ghdp::d76 * A[6].y + ghdp::d77 * A[7].y + ghdp::d78 * A[8].y + ghdp::d79 * A[9].y + ghdp::d710 * A[10].y + ghdp::d711 * A[11].y + ghdp::d712 * A[12].y + ghdp::d713 * A[13].y + ghdp::d714 * A[14].y + ghdp::d715 * A[15].y + ghdp::d716 * A[16].y);
+    yo[3] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+}

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 #include <initializer_list>
 
 #include "../../include/galpy_hip.h"
@@ -68,14 +69,17 @@ int* irq_device_ptr() {
 // ---------------------------------------------------------------------------
 // Host decode of galpy's packed potential into the device table.
 // ---------------------------------------------------------------------------
-int nargs_of(int code) {
+// Arguments per component: integrateFullOrbit.c:80-345 for full orbits,
+// integratePlanarOrbit.c:64-223 for planar ones (whose
+// PowerSphericalPotentialwCutoff has no cache slots: 3 arguments, not 5).
+int nargs_of(int code, bool planar) {
   switch (code) {
     case GH_LOGHALO: return 4;
     case GH_DEHNENBAR: return 6;
     case GH_MIYAMOTONAGAI: return 3;
     case GH_HERNQUIST: return 2;
     case GH_NFW: return 2;
-    case GH_POWERSPHCUTOFF: return 5;
+    case GH_POWERSPHCUTOFF: return planar ? 3 : 5;
     default: return -1;
   }
 }
@@ -172,7 +176,8 @@ int fit_upper_gamma_table(double a, double xbig, double tab[GH_TAB_MAXSEG][GH_TA
   return e0;
 }
 
-int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPotTable* T) {
+int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPotTable* T,
+                     bool planar = false) {
   std::memset(T, 0, sizeof(*T));
   if (npot < 1 || npot > GH_MAX_POT) {
     set_error("npot=%d outside [1, %d]", npot, GH_MAX_POT);
@@ -182,7 +187,7 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
   int nslots = 0;
   for (int i = 0; i < npot; i++) {
     const int code = pot_type[i];
-    const int na = nargs_of(code);
+    const int na = nargs_of(code, planar);
     if (na < 0) {
       set_error("pot_type %d (component %d) has no HIP kernel", code, i);
       return -1;
@@ -255,8 +260,9 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
 // Kernels
 // ---------------------------------------------------------------------------
 
-// leapfrog (bovy_symplecticode.c:132-224) and symplec4 (:250-377)
-template <int METHOD, class Spec>
+// leapfrog (bovy_symplecticode.c:132-224) and symplec4 (:250-377), for full
+// (V = GhVec3) or planar (V = GhVec2) orbits
+template <int METHOD, class Spec, class V>
 __global__ void __launch_bounds__(kBlock)
 gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
                      int indiv_t, double dt_in, double rtol, double atol,
@@ -265,20 +271,21 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
                      const volatile int* cancel) {
   const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (orbit >= nobj) return;
+  constexpr int N = GhPhase<V>::N;
   const GhForce<Spec> F{T};
   const double* tt = t + (indiv_t ? orbit * nt : 0);
-  double w[6];
+  double w[N];
   {
-    double v[6];
+    double v[N];
 #pragma unroll
-    for (int k = 0; k < 6; k++) v[k] = yo[orbit * 6 + k];
-    gh_cyl_to_rect(v, w);
+    for (int k = 0; k < N; k++) v[k] = yo[orbit * N + k];
+    GhPhase<V>::to_rect(v, w);
 #pragma unroll
-    for (int k = 0; k < 6; k++) yo[orbit * 6 + k] = w[k];  // galpy converts yo in place
+    for (int k = 0; k < N; k++) yo[orbit * N + k] = w[k];  // galpy converts yo in place
   }
-  GhVec3 q{w[0], w[1], w[2]}, p{w[3], w[4], w[5]};
-  double* out = result + orbit * (long)nt * 6;
-  gh_store_cyl(out, q.x, q.y, q.z, p.x, p.y, p.z);
+  V q = GhPhase<V>::q(w), p = GhPhase<V>::p(w);
+  double* out = result + orbit * (long)nt * N;
+  GhPhase<V>::store(out, q, p);
   int e = 0;
   const double init_dt = tt[1] - tt[0];
   double dt = dt_in;
@@ -297,7 +304,7 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
     }
     if (METHOD == 0) {
       // leapfrog: half drift, ndt-1 fused kick+drift, kick + half drift
-      GhVec3 q12 = gh_drift(q, p, dt / 2.0);
+      V q12 = gh_drift(q, p, dt / 2.0);
       for (long jj = 0; jj < ndt - 1; jj++) {
         p = gh_drift(p, gh_accel(F, to + dt / 2.0, q12), dt);
         q12 = gh_drift(q12, p, dt);
@@ -332,7 +339,7 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
       q = gh_drift(q, p, c4 * dt);
       to += c4 * dt;
     }
-    gh_store_cyl(out + 6L * (ii + 1), q.x, q.y, q.z, p.x, p.y, p.z);
+    GhPhase<V>::store(out + (long)N * (ii + 1), q, p);
   }
   err[orbit] = e;
   if (stats) {
@@ -363,103 +370,123 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
 // faster at every size measured (131,072 to 1,048,576 orbits: 1.3-1.6x), so
 // it is the default; GALPY_HIP_DOP_WAVES=1 selects the other instance.
 // Deferred dense output.  A lane whose accepted step passes output times
-// stores the step's GhDenseJob in its wavefront's LDS queue and steps on;
+// stores the step's GhDenseJobT in its wavefront's LDS queue and steps on;
 // once the queue holds kDenseFlush jobs the whole wavefront computes them,
 // one job per lane.  Done in line, the three extra force evaluations of the
 // dense output run for nearly every wavefront iteration (some lane almost
 // always needs them) with ~1 lane in 8 active.  The queue is a structure of
 // arrays (field k of slot j at k * kDenseCap + j, so a wavefront's stores hit
 // consecutive banks); 47 doubles + 2 ints per job, 52 jobs = 19,968 bytes per
-// wavefront, so 8 single-wavefront workgroups still fit a CU's 160 KB LDS.
+// wavefront (13,728 for planar orbits), so 8 single-wavefront workgroups
+// still fit a CU's 160 KB LDS.
 constexpr int kDenseCap = 52;
 // Flush at 32 queued jobs: a step adds ~7.5 jobs per wavefront on average
 // (MWPotential2014, 1001 outputs), so the 20 free slots almost never
 // overflow, and on MI355X 32 measured fastest (817 ms for 1M orbits, against
 // 822 at 36, 864 at 44 -- where overflow is frequent -- and 830 at 24).
 constexpr int kDenseFlush = 32;
-constexpr int kJobDoubles = 47;
+// Doubles per queued job: q0, p0, qn, pn and 11 accelerations (15 vectors of
+// D components), then t_old and h -- 47 for full orbits, 32 for planar ones.
+template <class V>
+constexpr int kJobDoubles = 15 * (GhPhase<V>::N / 2) + 2;
 
+__device__ __forceinline__ void gh_put(double* v, GhVec3 a) {
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+}
+__device__ __forceinline__ void gh_put(double* v, GhVec2 a) {
+  v[0] = a.x;
+  v[1] = a.y;
+}
+
+// the job as kJobDoubles<V> doubles, in queue field order
+template <class V>
+__device__ __forceinline__ void gh_job_pack(const GhDenseJobT<V>& J, double* v) {
+  constexpr int D = GhPhase<V>::N / 2;
+  gh_put(v, J.q0);
+  gh_put(v + D, J.p0);
+  gh_put(v + 2 * D, J.qn);
+  gh_put(v + 3 * D, J.pn);
+#pragma unroll
+  for (int k = 0; k < 11; k++) gh_put(v + 4 * D + D * k, J.A[k]);
+  v[15 * D] = J.t_old;
+  v[15 * D + 1] = J.h;
+}
+
+template <class V>
 __device__ __forceinline__ void gh_job_store(double* __restrict__ qd, int* __restrict__ qi,
-                                             int slot, const GhDenseJob& J, int orbit) {
+                                             int slot, const GhDenseJobT<V>& J, int orbit) {
   constexpr int cap = kDenseCap;
-  double v[kJobDoubles];
-  const GhVec3 g[4] = {J.q0, J.p0, J.qn, J.pn};
+  double v[kJobDoubles<V>];
+  gh_job_pack(J, v);
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    v[3 * k] = g[k].x;
-    v[3 * k + 1] = g[k].y;
-    v[3 * k + 2] = g[k].z;
-  }
-#pragma unroll
-  for (int k = 0; k < 11; k++) {
-    v[12 + 3 * k] = J.A[k].x;
-    v[13 + 3 * k] = J.A[k].y;
-    v[14 + 3 * k] = J.A[k].z;
-  }
-  v[45] = J.t_old;
-  v[46] = J.h;
-#pragma unroll
-  for (int k = 0; k < kJobDoubles; k++) qd[k * cap + slot] = v[k];
+  for (int k = 0; k < kJobDoubles<V>; k++) qd[k * cap + slot] = v[k];
   qi[slot] = J.done0;
   qi[cap + slot] = orbit;
 }
 
 // Jobs that did not fit the LDS queue wait in a per-wavefront area of device
 // memory (kOverflowCap slots) and move into the queue after a flush.  Each
-// overflow job is contiguous (47 doubles, then done0 and the orbit as two
+// overflow job is contiguous (its doubles, then done0 and the orbit as two
 // ints), so one address plus immediate offsets reaches all its fields;
 // volatile so the copy reads see the wavefront's own earlier stores.
 constexpr int kOverflowCap = 64;
-constexpr int kJobWords = kJobDoubles + 1;  // doubles per overflow slot
-constexpr size_t kOverflowBytes = (size_t)kOverflowCap * kJobWords * 8;
+template <class V>
+constexpr int kJobWords = kJobDoubles<V> + 1;  // doubles per overflow slot
+template <class V>
+constexpr size_t kOverflowBytes = (size_t)kOverflowCap * kJobWords<V> * 8;
 
+template <class V>
 __device__ __forceinline__ void gh_job_store_ovf(volatile double* od, int slot,
-                                                 const GhDenseJob& J, int orbit) {
-  volatile double* p = od + (long)slot * kJobWords;
-  p[0] = J.q0.x; p[1] = J.q0.y; p[2] = J.q0.z;
-  p[3] = J.p0.x; p[4] = J.p0.y; p[5] = J.p0.z;
-  p[6] = J.qn.x; p[7] = J.qn.y; p[8] = J.qn.z;
-  p[9] = J.pn.x; p[10] = J.pn.y; p[11] = J.pn.z;
+                                                 const GhDenseJobT<V>& J, int orbit) {
+  volatile double* p = od + (long)slot * kJobWords<V>;
+  double v[kJobDoubles<V>];
+  gh_job_pack(J, v);
 #pragma unroll
-  for (int k = 0; k < 11; k++) {
-    p[12 + 3 * k] = J.A[k].x;
-    p[13 + 3 * k] = J.A[k].y;
-    p[14 + 3 * k] = J.A[k].z;
-  }
-  p[45] = J.t_old;
-  p[46] = J.h;
-  volatile int* pi = reinterpret_cast<volatile int*>(p + kJobDoubles);
+  for (int k = 0; k < kJobDoubles<V>; k++) p[k] = v[k];
+  volatile int* pi = reinterpret_cast<volatile int*>(p + kJobDoubles<V>);
   pi[0] = J.done0;
   pi[1] = orbit;
 }
 
+template <class V>
 __device__ __forceinline__ void gh_job_copy_in(const volatile double* od, int from,
                                                double* __restrict__ qd, int* __restrict__ qi,
                                                int slot) {
-  const volatile double* p = od + (long)from * kJobWords;
+  const volatile double* p = od + (long)from * kJobWords<V>;
 #pragma unroll 8
-  for (int k = 0; k < kJobDoubles; k++) qd[k * kDenseCap + slot] = p[k];
-  const volatile int* pi = reinterpret_cast<const volatile int*>(p + kJobDoubles);
+  for (int k = 0; k < kJobDoubles<V>; k++) qd[k * kDenseCap + slot] = p[k];
+  const volatile int* pi = reinterpret_cast<const volatile int*>(p + kJobDoubles<V>);
   qi[slot] = pi[0];
   qi[kDenseCap + slot] = pi[1];
 }
 
-// Loads the accelerations A1, A4..A13 of job `slot` into A[1], A[4..13].
-__device__ __forceinline__ void gh_job_load_acc(const double* __restrict__ qd, int slot,
-                                                GhVec3* A) {
-  A[1] = GhVec3{qd[12 * kDenseCap + slot], qd[13 * kDenseCap + slot], qd[14 * kDenseCap + slot]};
-#pragma unroll
-  for (int j = 4; j <= 13; j++) {
-    const int k = 12 + 3 * (j - 3);
-    A[j] = GhVec3{qd[k * kDenseCap + slot], qd[(k + 1) * kDenseCap + slot],
-                  qd[(k + 2) * kDenseCap + slot]};
-  }
-  A[2] = A[3] = GhVec3{0.0, 0.0, 0.0};  // unused by stages 14-16 and the extension
+// vector at queue field k of job `slot`
+__device__ __forceinline__ void gh_job_vec(const double* __restrict__ qd, int slot, int k,
+                                           GhVec3& v) {
+  v = GhVec3{qd[k * kDenseCap + slot], qd[(k + 1) * kDenseCap + slot],
+             qd[(k + 2) * kDenseCap + slot]};
+}
+__device__ __forceinline__ void gh_job_vec(const double* __restrict__ qd, int slot, int k,
+                                           GhVec2& v) {
+  v = GhVec2{qd[k * kDenseCap + slot], qd[(k + 1) * kDenseCap + slot]};
+}
+template <class V>
+__device__ __forceinline__ V gh_job_vec(const double* __restrict__ qd, int slot, int k) {
+  V v;
+  gh_job_vec(qd, slot, k, v);
+  return v;
 }
 
-__device__ __forceinline__ GhVec3 gh_job_vec(const double* __restrict__ qd, int slot, int k) {
-  return GhVec3{qd[k * kDenseCap + slot], qd[(k + 1) * kDenseCap + slot],
-                qd[(k + 2) * kDenseCap + slot]};
+// Loads the accelerations A1, A4..A13 of job `slot` into A[1], A[4..13].
+template <class V>
+__device__ __forceinline__ void gh_job_load_acc(const double* __restrict__ qd, int slot, V* A) {
+  constexpr int D = GhPhase<V>::N / 2;
+  A[1] = gh_job_vec<V>(qd, slot, 4 * D);
+#pragma unroll
+  for (int j = 4; j <= 13; j++) A[j] = gh_job_vec<V>(qd, slot, 4 * D + D * (j - 3));
+  A[2] = A[3] = V{};  // unused by stages 14-16 and the extension
 }
 
 // Dense output of queued job `slot` (gh_dop_dense's arithmetic), in phases
@@ -468,67 +495,68 @@ __device__ __forceinline__ GhVec3 gh_job_vec(const double* __restrict__ qd, int 
 // loads apart, so only the three new accelerations are live from one phase
 // to the next -- a whole job plus a force evaluation would not fit the
 // register budget of two wavefronts per SIMD next to the lane's own state.
-template <class F>
+template <class F, class V>
 __device__ __forceinline__ void gh_job_run(const F& T, const double* __restrict__ qd,
                                            const int* __restrict__ qi, int slot,
                                            const double* __restrict__ t0, int nt, int indiv_t,
                                            double* __restrict__ result) {
   using namespace ghdp;
-  const double t_old = qd[45 * kDenseCap + slot], h = qd[46 * kDenseCap + slot];
-  GhVec3 A14, A15, A16;
+  constexpr int D = GhPhase<V>::N / 2, N = GhPhase<V>::N;
+  const double t_old = qd[15 * D * kDenseCap + slot], h = qd[(15 * D + 1) * kDenseCap + slot];
+  V A14, A15, A16;
   {
-    GhVec3 A[17];
+    V A[17];
     gh_job_load_acc(qd, slot, A);
     A14 = gh_accel(T, t_old + c14 * h,
-                   gh_rkn_q14(h, gh_job_vec(qd, slot, 0), gh_job_vec(qd, slot, 3),
-                              gh_job_vec(qd, slot, 9), A));
+                   gh_rkn_q14(h, gh_job_vec<V>(qd, slot, 0), gh_job_vec<V>(qd, slot, D),
+                              gh_job_vec<V>(qd, slot, 3 * D), A));
   }
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   {
-    GhVec3 A[17];
+    V A[17];
     gh_job_load_acc(qd, slot, A);
     A[14] = A14;
     A15 = gh_accel(T, t_old + c15 * h,
-                   gh_rkn_q15(h, gh_job_vec(qd, slot, 0), gh_job_vec(qd, slot, 3),
-                              gh_job_vec(qd, slot, 9), A));
+                   gh_rkn_q15(h, gh_job_vec<V>(qd, slot, 0), gh_job_vec<V>(qd, slot, D),
+                              gh_job_vec<V>(qd, slot, 3 * D), A));
   }
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   {
-    GhVec3 A[17];
+    V A[17];
     gh_job_load_acc(qd, slot, A);
     A[14] = A14;
     A[15] = A15;
     A16 = gh_accel(T, t_old + c16 * h,
-                   gh_rkn_q16(h, gh_job_vec(qd, slot, 0), gh_job_vec(qd, slot, 3),
-                              gh_job_vec(qd, slot, 9), A));
+                   gh_rkn_q16(h, gh_job_vec<V>(qd, slot, 0), gh_job_vec<V>(qd, slot, D),
+                              gh_job_vec<V>(qd, slot, 3 * D), A));
   }
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   const int o = qi[kDenseCap + slot];
   const double* __restrict__ t = t0 + (indiv_t ? (long)o * nt : 0);
-  double* __restrict__ out = result + (long)o * nt * 6;
+  double* __restrict__ out = result + (long)o * nt * N;
   const double pos_neg = (h > 0.0) ? 1.0 : -1.0;
   const double t_cur = t_old + h;
   int done = qi[slot];
   while (done < nt - 1 && pos_neg * t[done + 1] < pos_neg * t_cur) {
     const double sd = (t[done + 1] - t_old) / h;
     const double s1 = 1.0 - sd;
-    double yo[6];
+    double yo[N];
     {
-      GhVec3 A[17];
+      V A[17];
       gh_job_load_acc(qd, slot, A);
       A[14] = A14;
       A[15] = A15;
       A[16] = A16;
-      gh_rkn_dense_eval(h, sd, s1, gh_job_vec(qd, slot, 0), gh_job_vec(qd, slot, 3),
-                        gh_job_vec(qd, slot, 6), gh_job_vec(qd, slot, 9), A, yo);
+      gh_rkn_dense_eval(h, sd, s1, gh_job_vec<V>(qd, slot, 0), gh_job_vec<V>(qd, slot, D),
+                        gh_job_vec<V>(qd, slot, 2 * D), gh_job_vec<V>(qd, slot, 3 * D), A, yo);
     }
     done++;
-    gh_store_cyl(out + 6L * done, yo[0], yo[1], yo[2], yo[3], yo[4], yo[5]);
+    GhPhase<V>::store(out + (long)N * done, GhPhase<V>::q(yo), GhPhase<V>::p(yo));
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
   }
 }
 
-template <class Spec, bool PILOT, int WPE>
+template <class Spec, bool PILOT, int WPE, class V>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
                  int indiv_t, double rtol_log, double atol_log, const GhPotTable* __restrict__ T,
@@ -543,15 +571,17 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
   const unsigned long long below = (1ull << lane) - 1ull;
   long claim = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long orbit = 0;
-  GhDopState s;
-  __shared__ double qd[PILOT ? 1 : kJobDoubles * kDenseCap];
+  constexpr int N = GhPhase<V>::N;
+  GhDopStateT<V> s;
+  __shared__ double qd[PILOT ? 1 : kJobDoubles<V> * kDenseCap];
   __shared__ int qi[PILOT ? 1 : 2 * kDenseCap];
   int queued = 0;  // jobs in this wavefront's queue (the same in every lane)
   volatile double* od =
-      PILOT ? nullptr : reinterpret_cast<volatile double*>(overflow + blockIdx.x * kOverflowBytes);
+      PILOT ? nullptr
+            : reinterpret_cast<volatile double*>(overflow + blockIdx.x * kOverflowBytes<V>);
   auto flush = [&]() {
     __syncthreads();  // the queue's stores are visible to every lane
-    if (lane < queued) gh_job_run(F, qd, qi, lane, t, nt, indiv_t, result);
+    if (lane < queued) gh_job_run<GhForce<Spec>, V>(F, qd, qi, lane, t, nt, indiv_t, result);
     __syncthreads();  // every job is read before slots are reused
     queued = 0;
   };
@@ -561,11 +591,11 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
 
   auto start = [&](long o) {
     tt = t + (indiv_t ? o * nt : 0);
-    out = PILOT ? nullptr : result + o * (long)nt * 6;
-    double v[6];
+    out = PILOT ? nullptr : result + o * (long)nt * N;
+    double v[N];
 #pragma unroll
-    for (int k = 0; k < 6; k++) v[k] = yo[o * 6 + k];
-    gh_cyl_to_rect(v, s.y);
+    for (int k = 0; k < N; k++) v[k] = yo[o * N + k];
+    GhPhase<V>::to_rect(v, s.y);
     gh_dop_init<GhForce<Spec>, !PILOT>(F, tt, nt, rtol, atol, s, out);
   };
   auto finish = [&](long o, int e) {
@@ -574,7 +604,7 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
       return;
     }
 #pragma unroll
-    for (int k = 0; k < 6; k++) yo[o * 6 + k] = s.y[k];  // galpy's y0 holds the last state
+    for (int k = 0; k < N; k++) yo[o * N + k] = s.y[k];  // galpy's y0 holds the last state
     err[o] = e;
     if (stats) {
       stats[o * 3 + 0] = (double)s.nacc;
@@ -606,7 +636,7 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
       // overflow area.
       bool need = false;
       const int room = kDenseCap - queued;
-      auto sink = [&](const GhDenseJob& J) {
+      auto sink = [&](const GhDenseJobT<V>& J) {
         const int rank = __popcll(__ballot(1) & below);
         if (rank < room)
           gh_job_store(qd, qi, queued + rank, J, (int)orbit);
@@ -663,7 +693,7 @@ gh_dop853_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
       for (int from = 0; over > 0;) {
         flush();
         const int take = min(over, kDenseCap);
-        if (lane < take) gh_job_copy_in(od, from + lane, qd, qi, lane);
+        if (lane < take) gh_job_copy_in<V>(od, from + lane, qd, qi, lane);
         queued = take;
         from += take;
         over -= take;
@@ -773,12 +803,12 @@ int dop_waves_forced() {
 }
 
 // workgroups of gh_dop853_kernel<Spec> the device keeps resident at once
-template <class Spec, int WPE>
+template <class Spec, int WPE, class V>
 int dop_resident_blocks(int dev) {
   static int cached[64] = {0};
   if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
   int per_cu = 0, ncu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dop853_kernel<Spec, false, WPE>,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dop853_kernel<Spec, false, WPE, V>,
                                                    kBlock, 0) !=
           hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -799,11 +829,11 @@ struct GhPotHandle {
 // DOP853 launch at WPE waves per SIMD: optional pilot + longest-first
 // order, then the persistent kernel.  The pilot's cost keys and the order
 // live in one stream-ordered temporary allocation, freed after the launch.
-template <class Spec, int WPE>
+template <class Spec, int WPE, class V>
 int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                   const GhPotTable* d_table, double rtol, double atol, double* d_result,
                   int* d_err, double* d_stats, const int* cancel, int dev, hipStream_t stream) {
-  const int resident = dop_resident_blocks<Spec, WPE>(dev);
+  const int resident = dop_resident_blocks<Spec, WPE, V>(dev);
   if (resident <= 0) {
     set_error("could not query DOP853 occupancy on device %d", dev);
     return -1;
@@ -827,7 +857,7 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
     e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, (const unsigned*)nullptr,
                                                      (unsigned*)nullptr, (const int*)nullptr,
                                                      (int*)nullptr, nobj, 0, 32, stream);
-  const size_t ovf = ((size_t)nblocks * kOverflowBytes + 255) / 256 * 256;
+  const size_t ovf = ((size_t)nblocks * kOverflowBytes<V> + 255) / 256 * 256;
   const size_t n4 = ordered ? ((size_t)nobj * 4 + 255) / 256 * 256 : 0;
   void* work = nullptr;
   if (e == hipSuccess)
@@ -844,7 +874,7 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
     void* d_sort_tmp = w + 4 * n4 + 256;
     hipLaunchKernelGGL(gh_pilot_grid, dim3(1), dim3(1), 0, stream, d_t, nt, kDopPilotFrac, d_pt);
     hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, lanes);
-    hipLaunchKernelGGL((gh_dop853_kernel<Spec, true, WPE>), dim3(nblocks), dim3(kBlock), 0,
+    hipLaunchKernelGGL((gh_dop853_kernel<Spec, true, WPE, V>), dim3(nblocks), dim3(kBlock), 0,
                        stream, nobj, d_yo, 2, d_pt, 0, rtol, atol, d_table, nullptr, nullptr,
                        nullptr, cancel, counter, nullptr, d_cost, nullptr);
     hipLaunchKernelGGL(gh_iota, dim3((unsigned)((nobj + 255) / 256)), dim3(256), 0, stream, nobj,
@@ -855,7 +885,7 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
   }
   if (e == hipSuccess) {
     hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, lanes);
-    hipLaunchKernelGGL((gh_dop853_kernel<Spec, false, WPE>), dim3(nblocks), dim3(kBlock), 0,
+    hipLaunchKernelGGL((gh_dop853_kernel<Spec, false, WPE, V>), dim3(nblocks), dim3(kBlock), 0,
                        stream, nobj, d_yo, nt, d_t, indiv_t, rtol, atol, d_table, d_result, d_err,
                        d_stats, cancel, counter, d_order, nullptr, d_overflow);
     e = hipGetLastError();
@@ -871,7 +901,9 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
   return 0;
 }
 
-template <class Spec>
+// Planar DOP853 runs at 2 waves per SIMD only (its state is two thirds of a
+// full orbit's, so it never needs the single-wave instance).
+template <class Spec, class V>
 int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                 const GhPotTable* d_table, double dt, double rtol, double atol, double* d_result,
                 int* d_err, int odeint_type, double* d_stats, hipStream_t stream) {
@@ -879,21 +911,27 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
   const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
   switch (odeint_type) {
     case 0:
-      hipLaunchKernelGGL((gh_symplectic_kernel<0, Spec>), grid, block, 0, stream, nobj, d_yo, nt,
-                         d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+      hipLaunchKernelGGL((gh_symplectic_kernel<0, Spec, V>), grid, block, 0, stream, nobj, d_yo,
+                         nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
+                         cancel);
       break;
     case 3:
-      hipLaunchKernelGGL((gh_symplectic_kernel<3, Spec>), grid, block, 0, stream, nobj, d_yo, nt,
-                         d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+      hipLaunchKernelGGL((gh_symplectic_kernel<3, Spec, V>), grid, block, 0, stream, nobj, d_yo,
+                         nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
+                         cancel);
       break;
     case 6: {
       int dev = 0;
       GH_CHECK(hipGetDevice(&dev));
+      if (std::is_same<V, GhVec2>::value)
+        return launch_dop853<Spec, 2, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
+                                         d_result, d_err, d_stats, cancel, dev, stream);
       const bool two = dop_waves_forced() != 1;
-      return two ? launch_dop853<Spec, 2>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
-                                          d_result, d_err, d_stats, cancel, dev, stream)
-                 : launch_dop853<Spec, 1>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
-                                          d_result, d_err, d_stats, cancel, dev, stream);
+      return two ? launch_dop853<Spec, 2, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
+                                             d_result, d_err, d_stats, cancel, dev, stream)
+                 : launch_dop853<Spec, 1, GhVec3>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol,
+                                                  atol, d_result, d_err, d_stats, cancel, dev,
+                                                  stream);
     }
     default:
       set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, 6 dop853)",
@@ -915,6 +953,8 @@ bool types_are(const GhPotHandle* h, std::initializer_list<int> codes) {
 // Compositions with a specialised kernel: MWPotential2014, the rotating-bar
 // model, and single Hernquist / logarithmic halos; anything else runs the
 // generic kernel that dispatches on the component type at run time.
+// V = GhVec3 integrates full orbits, GhVec2 planar ones.
+template <class V>
 int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                      const GhPotHandle* h, double dt, double rtol, double atol, double* d_result,
                      int* d_err, int odeint_type, double* d_stats, hipStream_t stream) {
@@ -924,8 +964,8 @@ int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indi
     return -1;
   }
 #define GH_LAUNCH(SPEC)                                                                        \
-  return launch_spec<SPEC>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result, d_err, \
-                           odeint_type, d_stats, stream)
+  return launch_spec<SPEC, V>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result, \
+                              d_err, odeint_type, d_stats, stream)
   using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
   using LogBar = GhSpec<GH_LOGHALO, GH_DEHNENBAR>;
   using Hern = GhSpec<GH_HERNQUIST>;
@@ -945,9 +985,12 @@ void on_sigint(int) {
   if (g_irq_host) *g_irq_host = 1;
 }
 
+// Blocking host-buffer integration of full (planar = false, rows of 6) or
+// planar (rows of 4) orbits, streamed through the GPU in chunks.
 int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npot, int* pot_type,
                    double* pot_args, double dt, double rtol, double atol, double* result,
-                   int* err, int odeint_type, void (*cb)(void)) {
+                   int* err, int odeint_type, void (*cb)(void), bool planar) {
+  const size_t N = planar ? 4 : 6;  // doubles per phase-space row
   if (nobj <= 0) return 0;
   if (nt < 2) {
     set_error("nt=%d: need at least two output times", nt);
@@ -959,7 +1002,7 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
     return -1;
   }
   GhPotTable table;
-  if (decode_potential(npot, pot_type, pot_args, &table) != 0) return -1;
+  if (decode_potential(npot, pot_type, pot_args, &table, planar) != 0) return -1;
   hipStream_t stream;
   GH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   void* d_table = nullptr;
@@ -979,13 +1022,13 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
   do {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { rc = -1; set_error("hipMemGetInfo failed"); break; }
-    const size_t per_orbit = (size_t)nt * 6 * 8 + 48 + 4 + (indiv_t ? (size_t)nt * 8 : 0);
+    const size_t per_orbit = (size_t)nt * N * 8 + N * 8 + 4 + (indiv_t ? (size_t)nt * 8 : 0);
     size_t budget = free_b / 2;
     long chunk = (long)std::min<size_t>((size_t)nobj, std::max<size_t>(1, budget / per_orbit));
     if (hipMalloc(&d_table, sizeof(GhPotTable)) != hipSuccess ||
-        hipMalloc((void**)&d_yo, (size_t)chunk * 48) != hipSuccess ||
+        hipMalloc((void**)&d_yo, (size_t)chunk * N * 8) != hipSuccess ||
         hipMalloc((void**)&d_t, (size_t)nt * 8 * (indiv_t ? chunk : 1)) != hipSuccess ||
-        hipMalloc((void**)&d_res, (size_t)chunk * nt * 48) != hipSuccess ||
+        hipMalloc((void**)&d_res, (size_t)chunk * nt * N * 8) != hipSuccess ||
         hipMalloc((void**)&d_err, (size_t)chunk * 4) != hipSuccess) {
       rc = -1;
       set_error("device allocation failed (chunk of %ld orbits, nt=%d)", chunk, nt);
@@ -997,15 +1040,19 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
         hipMemcpyAsync(d_t, t, (size_t)nt * 8, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
     for (long o0 = 0; o0 < nobj && rc == 0; o0 += chunk) {
       const long n = std::min<long>(chunk, nobj - o0);
-      if (hipMemcpyAsync(d_yo, yo + 6 * o0, (size_t)n * 48, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+      if (hipMemcpyAsync(d_yo, yo + N * o0, (size_t)n * N * 8, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
       if (indiv_t && hipMemcpyAsync(d_t, t + (size_t)nt * o0, (size_t)n * nt * 8,
                                     hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
-      if (launch_integrate((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
-                           atol, d_res, d_err, odeint_type, nullptr, stream) != 0) { rc = -1; break; }
-      if (hipMemcpyAsync(result + (size_t)6 * nt * o0, d_res, (size_t)n * nt * 48,
+      const int lrc =
+          planar ? launch_integrate<GhVec2>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
+                                            atol, d_res, d_err, odeint_type, nullptr, stream)
+                 : launch_integrate<GhVec3>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
+                                            atol, d_res, d_err, odeint_type, nullptr, stream);
+      if (lrc != 0) { rc = -1; break; }
+      if (hipMemcpyAsync(result + N * nt * o0, d_res, (size_t)n * nt * N * 8,
                          hipMemcpyDeviceToHost, stream) != hipSuccess ||
           hipMemcpyAsync(err + o0, d_err, (size_t)n * 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
-          hipMemcpyAsync(yo + 6 * o0, d_yo, (size_t)n * 48, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipMemcpyAsync(yo + N * o0, d_yo, (size_t)n * N * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
           hipStreamSynchronize(stream) != hipSuccess) {
         rc = -1;
         set_error("device copy/sync failed: %s", hipGetErrorString(hipGetLastError()));
@@ -1032,10 +1079,11 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-GH_EXPORT void integrateFullOrbit(int nobj, double* yo, int nt, double* t, int indiv_t, int npot,
-                                  int* pot_type, double* pot_args, void* pot_tfuncs, double dt,
-                                  double rtol, double atol, double* result, int* err,
-                                  int odeint_type, void (*cb)(void)) {
+static void integrate_host_abi(const char* name, int nobj, double* yo, int nt, double* t,
+                               int indiv_t, int npot, int* pot_type, double* pot_args,
+                               void* pot_tfuncs, double dt, double rtol, double atol,
+                               double* result, int* err, int odeint_type, void (*cb)(void),
+                               bool planar) {
   g_errbuf[0] = 0;
   int rc;
   if (pot_tfuncs != nullptr) {
@@ -1044,12 +1092,28 @@ GH_EXPORT void integrateFullOrbit(int nobj, double* yo, int nt, double* t, int i
   } else {
     for (int i = 0; i < nobj; i++) err[i] = 0;
     rc = host_integrate(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol,
-                        result, err, odeint_type, cb);
+                        result, err, odeint_type, cb, planar);
   }
   if (rc != 0) {
     for (int i = 0; i < nobj; i++) err[i] = -1;
-    fprintf(stderr, "galpy_hip: integrateFullOrbit failed: %s\n", g_errbuf);
+    fprintf(stderr, "galpy_hip: %s failed: %s\n", name, g_errbuf);
   }
+}
+
+GH_EXPORT void integrateFullOrbit(int nobj, double* yo, int nt, double* t, int indiv_t, int npot,
+                                  int* pot_type, double* pot_args, void* pot_tfuncs, double dt,
+                                  double rtol, double atol, double* result, int* err,
+                                  int odeint_type, void (*cb)(void)) {
+  integrate_host_abi("integrateFullOrbit", nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args,
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, cb, false);
+}
+
+GH_EXPORT void integratePlanarOrbit(int nobj, double* yo, int nt, double* t, int indiv_t,
+                                    int npot, int* pot_type, double* pot_args, void* pot_tfuncs,
+                                    double dt, double rtol, double atol, double* result, int* err,
+                                    int odeint_type, void (*cb)(void)) {
+  integrate_host_abi("integratePlanarOrbit", nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args,
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, cb, true);
 }
 
 GH_EXPORT int integrateFullOrbit_hip(int nobj, double* d_yo, int nt, const double* d_t,
@@ -1061,15 +1125,28 @@ GH_EXPORT int integrateFullOrbit_hip(int nobj, double* d_yo, int nt, const doubl
     set_error("potential handle is NULL");
     return -1;
   }
-  return launch_integrate(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot, dt, rtol, atol,
-                          d_result, d_err, odeint_type, d_stats, (hipStream_t)stream);
+  return launch_integrate<GhVec3>(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot, dt, rtol,
+                                  atol, d_result, d_err, odeint_type, d_stats, (hipStream_t)stream);
 }
 
-GH_EXPORT int gh_potential_upload(int npot, const int* pot_type, const double* pot_args,
-                                  void** pot) {
+GH_EXPORT int integratePlanarOrbit_hip(int nobj, double* d_yo, int nt, const double* d_t,
+                                       int indiv_t, const void* pot, double dt, double rtol,
+                                       double atol, double* d_result, int* d_err, int odeint_type,
+                                       double* d_stats, void* stream) {
+  g_errbuf[0] = 0;
+  if (pot == nullptr) {
+    set_error("potential handle is NULL");
+    return -1;
+  }
+  return launch_integrate<GhVec2>(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot, dt, rtol,
+                                  atol, d_result, d_err, odeint_type, d_stats, (hipStream_t)stream);
+}
+
+static int upload_potential(int npot, const int* pot_type, const double* pot_args, void** pot,
+                            bool planar) {
   g_errbuf[0] = 0;
   GhPotTable table;
-  if (decode_potential(npot, pot_type, pot_args, &table) != 0) return -1;
+  if (decode_potential(npot, pot_type, pot_args, &table, planar) != 0) return -1;
   void* d = nullptr;
   GH_CHECK(hipMalloc(&d, sizeof(GhPotTable)));
   GH_CHECK(hipMemcpy(d, &table, sizeof(GhPotTable), hipMemcpyHostToDevice));
@@ -1079,6 +1156,16 @@ GH_EXPORT int gh_potential_upload(int npot, const int* pot_type, const double* p
   for (int i = 0; i < GH_MAX_POT; i++) h->types[i] = table.comp[i].type;
   *pot = h;
   return 0;
+}
+
+GH_EXPORT int gh_potential_upload(int npot, const int* pot_type, const double* pot_args,
+                                  void** pot) {
+  return upload_potential(npot, pot_type, pot_args, pot, false);
+}
+
+GH_EXPORT int gh_planar_potential_upload(int npot, const int* pot_type, const double* pot_args,
+                                         void** pot) {
+  return upload_potential(npot, pot_type, pot_args, pot, true);
 }
 
 GH_EXPORT int gh_potential_free(void* pot) {

@@ -207,113 +207,115 @@ def emit(vals):
     L.extend(consts)
     w("}  // namespace ghrk")
     w("")
-    C = ("x", "y", "z")
-
+    # the same functions for full (x, y, z) and planar (x, y) states: the
+    # position/velocity split is per component, so the planar sums are the
+    # spatial ones with the z component left out
     def ghdp(key):
         return "ghdp::" + nm[key]
 
-    # stage arguments: positions only -- the stage velocities V_s enter every
-    # later sum through the expanded coefficients, so they are never formed
-    for s in MAIN_STAGES + DENSE_STAGES:
-        w(f"// position argument of stage {s}")
-        w(f"__device__ __forceinline__ GhVec3 gh_rkn_q{s}(double h, const GhVec3& q0, const GhVec3& p0,")
-        w("                                             const GhVec3& pn, const GhVec3* A) {")
-        w("  (void)pn;")
-        w("  GhVec3 Q;")
-        for cc in C:
-            qa = [(f"ghrk::{names[('QA', s, l)]}", f"A[{l}].{cc}") for l in sorted(N["QA"][s])]
-            lin = f"ghrk::{names[('QP', s)]} * p0.{cc}"
-            if ("QN", s) in names:
-                lin += f" + ghrk::{names[('QN', s)]} * pn.{cc}"
-            if qa:
-                w(f"  Q.{cc} = q0.{cc} + h * ({lin} + h * ({_sum(qa)}));")
-            else:
-                w(f"  Q.{cc} = q0.{cc} + h * ({lin});")
-        w("  return Q;")
+    for V, C in (("GhVec3", ("x", "y", "z")), ("GhVec2", ("x", "y"))):
+        # stage arguments: positions only -- the stage velocities V_s enter every
+        # later sum through the expanded coefficients, so they are never formed
+        for s in MAIN_STAGES + DENSE_STAGES:
+            w(f"// position argument of stage {s}")
+            w(f"__device__ __forceinline__ {V} gh_rkn_q{s}(double h, const {V}& q0, const {V}& p0,")
+            w(f"                                             const {V}& pn, const {V}* A) {{")
+            w("  (void)pn;")
+            w(f"  {V} Q;")
+            for cc in C:
+                qa = [(f"ghrk::{names[('QA', s, l)]}", f"A[{l}].{cc}") for l in sorted(N["QA"][s])]
+                lin = f"ghrk::{names[('QP', s)]} * p0.{cc}"
+                if ("QN", s) in names:
+                    lin += f" + ghrk::{names[('QN', s)]} * pn.{cc}"
+                if qa:
+                    w(f"  Q.{cc} = q0.{cc} + h * ({lin} + h * ({_sum(qa)}));")
+                else:
+                    w(f"  Q.{cc} = q0.{cc} + h * ({lin});")
+            w("  return Q;")
+            w("}")
+            w("")
+
+        # solution and error norm
+        w("// New state (q_new, p_new) and the two error sums of leung_dop853.c:376-404:")
+        w("// per component, sk = atol + rtol max(|y0|, |y_new|), err3 += (e3/sk)^2,")
+        w("// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k (one")
+        w("// reciprocal of sk per component; the norm only steers the step size).")
+        w(f"__device__ __forceinline__ void gh_rkn_solution(double h, const {V}& q0, const {V}& p0,")
+        w(f"                                               const {V}* A, double atol, double rtol,")
+        w(f"                                               {V}& qn, {V}& pn, double& err5,")
+        w("                                               double& err3) {")
+        w("  err5 = 0.0;")
+        w("  err3 = 0.0;")
+        bj = sorted(T["b"])
+        hj = sorted(T["bhh"])
+        ej = sorted(T["er"])
+        for cc in C:  # positions
+            ba = [(f"ghrk::{names[('BA', l)]}", f"A[{l}].{cc}") for l in sorted(N["BA"])]
+            w("  {")
+            w(f"    const double kq = ghrk::bp * p0.{cc} + h * ({_sum(ba)});")
+            w(f"    qn.{cc} = q0.{cc} + h * kq;")
+            w(f"    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.{cc}), fabs(qn.{cc})));")
+            e3 = [(f"ghrk::{names[('E3A', l)]}", f"A[{l}].{cc}") for l in sorted(N["E3A"])]
+            e5 = [(f"ghrk::{names[('E5A', l)]}", f"A[{l}].{cc}") for l in sorted(N["E5A"])]
+            w(f"    double sqr = (ghrk::e3p * p0.{cc} + h * ({_sum(e3)})) * isk;")
+            w("    err3 += sqr * sqr;")
+            w(f"    sqr = (ghrk::e5p * p0.{cc} + h * ({_sum(e5)})) * isk;")
+            w("    err5 += sqr * sqr;")
+            w("  }")
+        for cc in C:  # velocities: the reference's arithmetic
+            kb = [(ghdp(("b", j)), f"A[{j}].{cc}") for j in bj]
+            w("  {")
+            w(f"    const double kp = {_sum(kb)};")
+            w(f"    pn.{cc} = p0.{cc} + h * kp;")
+            w(f"    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.{cc}), fabs(pn.{cc})));")
+            hh = " - ".join([f"{ghdp(('bhh', j))} * A[{j}].{cc}" for j in hj])
+            w(f"    double sqr = (kp - {hh}) * isk;")
+            w("    err3 += sqr * sqr;")
+            ee = [(ghdp(("er", j)), f"A[{j}].{cc}") for j in ej]
+            w(f"    sqr = ({_sum(ee)}) * isk;")
+            w("    err5 += sqr * sqr;")
+            w("  }")
         w("}")
         w("")
 
-    # solution and error norm
-    w("// New state (q_new, p_new) and the two error sums of leung_dop853.c:376-404:")
-    w("// per component, sk = atol + rtol max(|y0|, |y_new|), err3 += (e3/sk)^2,")
-    w("// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k (one")
-    w("// reciprocal of sk per component; the norm only steers the step size).")
-    w("__device__ __forceinline__ void gh_rkn_solution(double h, const GhVec3& q0, const GhVec3& p0,")
-    w("                                               const GhVec3* A, double atol, double rtol,")
-    w("                                               GhVec3& qn, GhVec3& pn, double& err5,")
-    w("                                               double& err3) {")
-    w("  err5 = 0.0;")
-    w("  err3 = 0.0;")
-    bj = sorted(T["b"])
-    hj = sorted(T["bhh"])
-    ej = sorted(T["er"])
-    for cc in C:  # positions (components 0..2)
-        ba = [(f"ghrk::{names[('BA', l)]}", f"A[{l}].{cc}") for l in sorted(N["BA"])]
-        w("  {")
-        w(f"    const double kq = ghrk::bp * p0.{cc} + h * ({_sum(ba)});")
-        w(f"    qn.{cc} = q0.{cc} + h * kq;")
-        w(f"    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.{cc}), fabs(qn.{cc})));")
-        e3 = [(f"ghrk::{names[('E3A', l)]}", f"A[{l}].{cc}") for l in sorted(N["E3A"])]
-        e5 = [(f"ghrk::{names[('E5A', l)]}", f"A[{l}].{cc}") for l in sorted(N["E5A"])]
-        w(f"    double sqr = (ghrk::e3p * p0.{cc} + h * ({_sum(e3)})) * isk;")
-        w("    err3 += sqr * sqr;")
-        w(f"    sqr = (ghrk::e5p * p0.{cc} + h * ({_sum(e5)})) * isk;")
-        w("    err5 += sqr * sqr;")
-        w("  }")
-    for cc in C:  # velocities (components 3..5): the reference's arithmetic
-        kb = [(ghdp(("b", j)), f"A[{j}].{cc}") for j in bj]
-        w("  {")
-        w(f"    const double kp = {_sum(kb)};")
-        w(f"    pn.{cc} = p0.{cc} + h * kp;")
-        w(f"    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.{cc}), fabs(pn.{cc})));")
-        hh = " - ".join([f"{ghdp(('bhh', j))} * A[{j}].{cc}" for j in hj])
-        w(f"    double sqr = (kp - {hh}) * isk;")
-        w("    err3 += sqr * sqr;")
-        ee = [(ghdp(("er", j)), f"A[{j}].{cc}") for j in ej]
-        w(f"    sqr = ({_sum(ee)}) * isk;")
-        w("    err5 += sqr * sqr;")
-        w("  }")
-    w("}")
-    w("")
+        # dense output: the continuous extension evaluated at one time, one
+        # component at a time, so only 8 coefficients are live at once
+        w("// Dense output (leung_dop853.c:430-474) of the accepted step at the time")
+        w("// t_old + sd h (s1 = 1 - sd), after the dense stages 14-16: per component")
+        w("// the coefficients rc1..rc8 of the continuous extension, then")
+        w("// rc1 + sd (rc2 + s1 (rc3 + sd (rc4 + s1 (rc5 + sd (rc6 + s1 (rc7 + sd rc8)))))).")
+        w(f"__device__ __forceinline__ void gh_rkn_dense_eval(double h, double sd, double s1, const {V}& q0,")
+        w(f"                                                 const {V}& p0, const {V}& qn,")
+        w(f"                                                 const {V}& pn, const {V}* A, double* yo) {{")
 
-    # dense output: the continuous extension evaluated at one time, one
-    # component at a time, so only 8 coefficients are live at once
-    w("// Dense output (leung_dop853.c:430-474) of the accepted step at the time")
-    w("// t_old + sd h (s1 = 1 - sd), after the dense stages 14-16: per component")
-    w("// the coefficients rc1..rc8 of the continuous extension, then")
-    w("// rc1 + sd (rc2 + s1 (rc3 + sd (rc4 + s1 (rc5 + sd (rc6 + s1 (rc7 + sd rc8)))))).")
-    w("__device__ __forceinline__ void gh_rkn_dense_eval(double h, double sd, double s1, const GhVec3& q0,")
-    w("                                                 const GhVec3& p0, const GhVec3& qn,")
-    w("                                                 const GhVec3& pn, const GhVec3* A, double* yo) {")
+        def poly(k):
+            return (f"    yo[{k}] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * "
+                    f"(r6 + sd * r7))))));")
 
-    def poly(k):
-        return (f"    yo[{k}] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * "
-                f"(r6 + sd * r7))))));")
-
-    for k, cc in enumerate(C):
-        w("  {")
-        w(f"    const double ydiff = qn.{cc} - q0.{cc};")
-        w(f"    const double bspl = h * p0.{cc} - ydiff;")
-        w(f"    const double r0 = q0.{cc}, r1 = ydiff, r2 = bspl;")
-        w(f"    const double r3 = ydiff - h * pn.{cc} - bspl;")
-        for r, i in enumerate(sorted(N["DP"])):
-            da = [(f"ghrk::{names[('DA', i, l)]}", f"A[{l}].{cc}") for l in sorted(N["DA"][i])]
-            w(f"    const double r{4 + r} = h * (ghrk::{names[('DP', i)]} * p0.{cc} + ghrk::{names[('DN', i)]} * pn.{cc} + h * ({_sum(da)}));")
-        w(poly(k))
-        w("  }")
-    for k, cc in enumerate(C):
-        kk = k + 3
-        w("  {")
-        w(f"    const double ydiff = pn.{cc} - p0.{cc};")
-        w(f"    const double bspl = h * A[1].{cc} - ydiff;")
-        w(f"    const double r0 = p0.{cc}, r1 = ydiff, r2 = bspl;")
-        w(f"    const double r3 = ydiff - h * A[13].{cc} - bspl;")
-        for r, i in enumerate(sorted(T["d"])):
-            dd = [(ghdp(("d", i, j)), f"A[{j}].{cc}") for j in sorted(T["d"][i])]
-            w(f"    const double r{4 + r} = h * ({_sum(dd)});")
-        w(poly(kk))
-        w("  }")
-    w("}")
+        for k, cc in enumerate(C):
+            w("  {")
+            w(f"    const double ydiff = qn.{cc} - q0.{cc};")
+            w(f"    const double bspl = h * p0.{cc} - ydiff;")
+            w(f"    const double r0 = q0.{cc}, r1 = ydiff, r2 = bspl;")
+            w(f"    const double r3 = ydiff - h * pn.{cc} - bspl;")
+            for r, i in enumerate(sorted(N["DP"])):
+                da = [(f"ghrk::{names[('DA', i, l)]}", f"A[{l}].{cc}") for l in sorted(N["DA"][i])]
+                w(f"    const double r{4 + r} = h * (ghrk::{names[('DP', i)]} * p0.{cc} + ghrk::{names[('DN', i)]} * pn.{cc} + h * ({_sum(da)}));")
+            w(poly(k))
+            w("  }")
+        for k, cc in enumerate(C):
+            kk = k + len(C)
+            w("  {")
+            w(f"    const double ydiff = pn.{cc} - p0.{cc};")
+            w(f"    const double bspl = h * A[1].{cc} - ydiff;")
+            w(f"    const double r0 = p0.{cc}, r1 = ydiff, r2 = bspl;")
+            w(f"    const double r3 = ydiff - h * A[13].{cc} - bspl;")
+            for r, i in enumerate(sorted(T["d"])):
+                dd = [(ghdp(("d", i, j)), f"A[{j}].{cc}") for j in sorted(T["d"][i])]
+                w(f"    const double r{4 + r} = h * ({_sum(dd)});")
+            w(poly(kk))
+            w("  }")
+        w("}")
     return "\n".join(L) + "\n"
 
 

@@ -21,18 +21,38 @@
 #define GH_S4D2 (-1.7024143839193153)
 #define GH_S4D3 1.3512071919596578
 
+// Position, velocity or acceleration of a full (GhVec3) or planar (GhVec2)
+// orbit.  The integrators are templates over the two; galpy runs the same
+// C integrators with dim = 3 (integrateFullOrbit.c:1091-1132) or dim = 2
+// (integratePlanarOrbit.c:788-825).
 struct GhVec3 {
   double x, y, z;
+};
+
+struct GhVec2 {
+  double x, y;
 };
 
 __device__ __forceinline__ GhVec3 gh_drift(GhVec3 q, GhVec3 p, double h) {
   return GhVec3{q.x + h * p.x, q.y + h * p.y, q.z + h * p.z};
 }
 
+__device__ __forceinline__ GhVec2 gh_drift(GhVec2 q, GhVec2 p, double h) {
+  return GhVec2{q.x + h * p.x, q.y + h * p.y};
+}
+
 template <class F>
 __device__ __forceinline__ GhVec3 gh_accel(const F& T, double t, GhVec3 q) {
   GhVec3 a;
   T.rect(t, q.x, q.y, q.z, a.x, a.y, a.z);
+  return a;
+}
+
+// planar acceleration (evalPlanarRectForce, integratePlanarOrbit.c:1015-1031)
+template <class F>
+__device__ __forceinline__ GhVec2 gh_accel(const F& T, double t, GhVec2 q) {
+  GhVec2 a;
+  T.rect(t, q.x, q.y, a.x, a.y);
   return a;
 }
 
@@ -60,14 +80,33 @@ __device__ __forceinline__ double gh_est_err(GhVec3 q11, GhVec3 q12, GhVec3 p11,
   return sqrt(e / 2.0 / 3.0);
 }
 
+__device__ __forceinline__ void gh_est_scales(GhVec2 q, GhVec2 p, double rtol, double atol,
+                                              double& sq, double& sp) {
+  const double mq = fmax(log(fabs(q.x)), log(fabs(q.y)));
+  const double mp = fmax(log(fabs(p.x)), log(fabs(p.y)));
+  double c = fmax(atol, rtol + mq);
+  sq = log(exp(atol - c) + exp(rtol + mq - c)) + c;
+  c = fmax(atol, rtol + mp);
+  sp = log(exp(atol - c) + exp(rtol + mp - c)) + c;
+}
+
+__device__ __forceinline__ double gh_est_err(GhVec2 q11, GhVec2 q12, GhVec2 p11, GhVec2 p12,
+                                             double sq, double sp) {
+  double e = 0.0;
+  e += exp(2.0 * log(fabs(q11.x - q12.x)) - 2.0 * sq);
+  e += exp(2.0 * log(fabs(p11.x - p12.x)) - 2.0 * sp);
+  e += exp(2.0 * log(fabs(q11.y - q12.y)) - 2.0 * sq);
+  e += exp(2.0 * log(fabs(p11.y - p12.y)) - 2.0 * sp);
+  return sqrt(e / 2.0 / 2.0);
+}
+
 // leapfrog_estimate_step, bovy_symplecticode.c:588-663
 // The step estimators run once per orbit; they are inlined so that a kernel's
 // register count is not set by a separately compiled copy (a call's callee is
 // budgeted at its own, larger, register use).
-template <class F>
-__device__ __forceinline__ double gh_leapfrog_estimate_step(const F& T, GhVec3 qo,
-                                            GhVec3 po, double dt, double to, double rtol,
-                                            double atol) {
+template <class F, class V>
+__device__ __forceinline__ double gh_leapfrog_estimate_step(const F& T, V qo, V po, double dt,
+                                                            double to, double rtol, double atol) {
   double err = 2.0;
   const double init_dt = dt;
   double sq, sp;
@@ -75,16 +114,16 @@ __device__ __forceinline__ double gh_leapfrog_estimate_step(const F& T, GhVec3 q
   dt *= 2.0;
   while (err > 1.0 && init_dt / dt < GH_MAX_DT_REDUCE) {
     dt /= 2.0;
-    GhVec3 q12 = gh_drift(qo, po, dt / 2.0);
-    GhVec3 a = gh_accel(T, to + dt / 2.0, q12);
-    const GhVec3 p11 = gh_drift(po, a, dt);
-    const GhVec3 q11 = gh_drift(q12, p11, dt / 2.0);
+    V q12 = gh_drift(qo, po, dt / 2.0);
+    V a = gh_accel(T, to + dt / 2.0, q12);
+    const V p11 = gh_drift(po, a, dt);
+    const V q11 = gh_drift(q12, p11, dt / 2.0);
     q12 = gh_drift(qo, po, dt / 4.0);
     a = gh_accel(T, to + dt / 4.0, q12);
-    const GhVec3 ptmp = gh_drift(po, a, dt / 2.0);
-    const GhVec3 qtmp = gh_drift(q12, ptmp, dt / 2.0);
+    const V ptmp = gh_drift(po, a, dt / 2.0);
+    const V qtmp = gh_drift(q12, ptmp, dt / 2.0);
     a = gh_accel(T, to + 3.0 * dt / 4.0, qtmp);
-    const GhVec3 p12 = gh_drift(ptmp, a, dt / 2.0);
+    const V p12 = gh_drift(ptmp, a, dt / 2.0);
     q12 = gh_drift(qtmp, p12, dt / 4.0);
     err = gh_est_err(q11, q12, p11, p12, sq, sp);
   }
@@ -92,10 +131,9 @@ __device__ __forceinline__ double gh_leapfrog_estimate_step(const F& T, GhVec3 q
 }
 
 // symplec4_estimate_step, bovy_symplecticode.c:665-808
-template <class F>
-__device__ __forceinline__ double gh_symplec4_estimate_step(const F& T, GhVec3 qo,
-                                            GhVec3 po, double dt, double to, double rtol,
-                                            double atol) {
+template <class F, class V>
+__device__ __forceinline__ double gh_symplec4_estimate_step(const F& T, V qo, V po, double dt,
+                                                            double to, double rtol, double atol) {
   const double c1 = GH_S4C1, c2 = GH_S4C2, c3 = GH_S4C3, c4 = GH_S4C4;
   const double d1 = GH_S4D1, d2 = GH_S4D2, d3 = GH_S4D3;
   double err = 2.0;
@@ -106,16 +144,16 @@ __device__ __forceinline__ double gh_symplec4_estimate_step(const F& T, GhVec3 q
   while (err > 1.0 && init_dt / dt < GH_MAX_DT_REDUCE) {
     dt /= 2.0;
     // one step of dt
-    GhVec3 q = gh_drift(qo, po, c1 * dt);
+    V q = gh_drift(qo, po, c1 * dt);
     to += c1 * dt;
-    GhVec3 p = gh_drift(po, gh_accel(T, to, q), d1 * dt);
+    V p = gh_drift(po, gh_accel(T, to, q), d1 * dt);
     q = gh_drift(q, p, c2 * dt);
     to += c2 * dt;
     p = gh_drift(p, gh_accel(T, to, q), d2 * dt);
     q = gh_drift(q, p, c3 * dt);
     to += c3 * dt;
-    const GhVec3 p11 = gh_drift(p, gh_accel(T, to, q), d3 * dt);
-    const GhVec3 q11 = gh_drift(q, p11, c4 * dt);
+    const V p11 = gh_drift(p, gh_accel(T, to, q), d3 * dt);
+    const V q11 = gh_drift(q, p11, c4 * dt);
     to += c4 * dt;
     to -= dt;
     // two steps of dt/2
@@ -174,3 +212,70 @@ __device__ __forceinline__ void gh_store_cyl(double* __restrict__ out, double x,
   o[1] = make_double2(-vx * sp + vy * cp, z);
   o[2] = make_double2(vz, phi);
 }
+
+// bovy_coords.c:57-69 (polar -> rectangular, planar orbits)
+__device__ __forceinline__ void gh_polar_to_rect(const double* v, double* w) {
+  double sp, cp;
+  sincos(v[3], &sp, &cp);
+  const double R = v[0], vR = v[1], vT = v[2];
+  w[0] = R * cp;
+  w[1] = R * sp;
+  w[2] = vR * cp - vT * sp;
+  w[3] = vR * sp + vT * cp;
+}
+
+// bovy_coords.c:79-91 (rectangular -> polar, phi = atan2(y, x)), stored as
+// one 32-byte row
+__device__ __forceinline__ void gh_store_polar(double* __restrict__ out, double x, double y,
+                                               double vx, double vy) {
+  const double R = sqrt(x * x + y * y);
+  const double phi = atan2(y, x);
+  double cp = 1.0, sp = 0.0;
+  if (R > 0.0) {
+    cp = x / R;
+    sp = y / R;
+  }
+  double2* o = reinterpret_cast<double2*>(out);
+  o[0] = make_double2(R, vx * cp + vy * sp);
+  o[1] = make_double2(-vx * sp + vy * cp, phi);
+}
+
+// Row layout of full and planar orbits: N doubles per row, galpy's
+// (R,vR,vT,z,vz,phi) / (R,vR,vT,phi) outside and the rectangular
+// (q, p) = (x,y,z,vx,vy,vz) / (x,y,vx,vy) inside the integrators.
+template <class V>
+struct GhPhase;
+
+template <>
+struct GhPhase<GhVec3> {
+  static constexpr int N = 6;
+  __device__ static __forceinline__ void to_rect(const double* v, double* w) {
+    gh_cyl_to_rect(v, w);
+  }
+  __device__ static __forceinline__ GhVec3 q(const double* w) { return GhVec3{w[0], w[1], w[2]}; }
+  __device__ static __forceinline__ GhVec3 p(const double* w) { return GhVec3{w[3], w[4], w[5]}; }
+  __device__ static __forceinline__ void set(double* w, GhVec3 q, GhVec3 p) {
+    w[0] = q.x; w[1] = q.y; w[2] = q.z;
+    w[3] = p.x; w[4] = p.y; w[5] = p.z;
+  }
+  __device__ static __forceinline__ void store(double* __restrict__ out, GhVec3 q, GhVec3 p) {
+    gh_store_cyl(out, q.x, q.y, q.z, p.x, p.y, p.z);
+  }
+};
+
+template <>
+struct GhPhase<GhVec2> {
+  static constexpr int N = 4;
+  __device__ static __forceinline__ void to_rect(const double* v, double* w) {
+    gh_polar_to_rect(v, w);
+  }
+  __device__ static __forceinline__ GhVec2 q(const double* w) { return GhVec2{w[0], w[1]}; }
+  __device__ static __forceinline__ GhVec2 p(const double* w) { return GhVec2{w[2], w[3]}; }
+  __device__ static __forceinline__ void set(double* w, GhVec2 q, GhVec2 p) {
+    w[0] = q.x; w[1] = q.y;
+    w[2] = p.x; w[3] = p.y;
+  }
+  __device__ static __forceinline__ void store(double* __restrict__ out, GhVec2 q, GhVec2 p) {
+    gh_store_polar(out, q.x, q.y, p.x, p.y);
+  }
+};

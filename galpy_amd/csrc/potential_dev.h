@@ -425,6 +425,17 @@ struct GhForce {
                                        double& ay, double& az) const {
     gh_rect_force<Spec>(T, t, x, y, z, ax, ay, az);
   }
+  // Planar orbits (evalPlanarRectForce, integratePlanarOrbit.c:1015-1031).
+  // galpy's planar kernels (e.g. NFWPotentialPlanarRforce, NFWPotential.c:28-37;
+  // DehnenBarPotentialPlanarRforce, DehnenBarPotential.c:41-59) are the full
+  // forces at z = 0, so the full kernels run with z = 0 and their z terms
+  // fold away.
+  __device__ __forceinline__ void rect(double t, double x, double y, double& ax,
+                                       double& ay) const {
+    const GhAcc F = gh_acc_forces<Spec>(T, x, y, 0.0, t);
+    ax = x * F.FRoR - y * F.FphioR2;
+    ay = y * F.FRoR + x * F.FphioR2;
+  }
 };
 
 // Potential value (per-component Eval functions; DehnenBar from

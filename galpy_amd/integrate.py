@@ -6,9 +6,12 @@ packing through ``parse_pot``, tolerances as natural logs
 (integratePlanarOrbit.py:712-724), ``dt=None`` -> -9999.99, a preallocated
 ``(nobj, nt, 6)`` result, a per-orbit ``err``, and ``KeyboardInterrupt`` when
 an orbit reports -10.  The call goes through ctypes to the C-ABI symbol
-``integrateFullOrbit`` of libgalpy_hip.so.
+``integrateFullOrbit`` of libgalpy_hip.so.  ``integratePlanarOrbit_hip`` is
+the same for planar orbits (galpy's ``integratePlanarOrbit_c``,
+galpy/orbit/integratePlanarOrbit.py:989-1100): rows of (R,vR,vT,phi), the
+planar packing ``parse_pot_planar`` and the symbol ``integratePlanarOrbit``.
 
-``DeviceIntegrator`` is the HBM-resident path: inputs and outputs are torch
+``integrate_device`` is the HBM-resident path: inputs and outputs are torch
 tensors on the GPU and the kernels run asynchronously on torch's current
 stream (used by the benchmark, batching and serving layers).
 """
@@ -18,7 +21,7 @@ import ctypes
 import numpy
 
 from . import _lib
-from .potential import parse_pot
+from .potential import parse_pot, parse_pot_planar
 
 DT_SENTINEL = -9999.99
 
@@ -49,19 +52,36 @@ def integrateFullOrbit_hip(pot, yo, t, int_method, rtol=None, atol=None, progres
     Returns ``(result, err)`` with ``result`` of shape ``(N,nt,6)`` (or
     ``(nt,6)`` for a single orbit).
     """
+    return _integrate_host("integrateFullOrbit", parse_pot, 6, pot, yo, t, int_method, rtol, atol,
+                           progressbar, dt)
+
+
+def integratePlanarOrbit_hip(pot, yo, t, int_method, rtol=None, atol=None, progressbar=False,
+                             dt=None):
+    """Integrate N planar orbits on the GPU (galpy's ``integratePlanarOrbit_c``,
+    integratePlanarOrbit.py:989-1100): ``yo`` is ``[N,4]`` or ``[4]`` in
+    (R,vR,vT,phi); returns ``(result, err)`` with ``result`` of shape
+    ``(N,nt,4)`` (or ``(nt,4)``)."""
+    return _integrate_host("integratePlanarOrbit", parse_pot_planar, 4, pot, yo, t, int_method,
+                           rtol, atol, progressbar, dt)
+
+
+def _integrate_host(symbol, parse, ncol, pot, yo, t, int_method, rtol, atol, progressbar, dt):
     lib = _lib.load()
     single_obj = len(numpy.shape(yo)) == 1
     yo = numpy.require(numpy.atleast_2d(yo), dtype=numpy.float64, requirements=["C", "W"])
+    if yo.shape[-1] != ncol:
+        raise ValueError(f"{symbol} takes initial conditions of {ncol} phase-space coordinates")
     nobj = len(yo)
     rtol, atol = _parse_tol(rtol, atol)
-    npot, pot_type, pot_args, _ = parse_pot(pot)
+    npot, pot_type, pot_args, _ = parse(pot)
     int_method_c = _parse_integrator(int_method)
     if dt is None:
         dt = DT_SENTINEL
     t = numpy.require(t, dtype=numpy.float64, requirements=["C", "W"])
     indiv_t = t.ndim > 1
     nt = t.shape[-1]
-    result = numpy.empty((nobj, nt, 6))
+    result = numpy.empty((nobj, nt, ncol))
     err = numpy.zeros(nobj, dtype=numpy.int32)
     cb = None
     if progressbar and nobj > 1:
@@ -72,7 +92,7 @@ def integrateFullOrbit_hip(pot, yo, t, int_method, rtol=None, atol=None, progres
             cb = ctypes.CFUNCTYPE(None)(pbar.update)
         except ImportError:  # pragma: no cover
             cb = None
-    lib.integrateFullOrbit(
+    getattr(lib, symbol)(
         ctypes.c_int(nobj), yo, ctypes.c_int(nt), t, ctypes.c_int(int(indiv_t)),
         ctypes.c_int(npot), pot_type, pot_args, None, ctypes.c_double(dt),
         ctypes.c_double(rtol), ctypes.c_double(atol), result, err,
@@ -90,19 +110,22 @@ def integrateFullOrbit_hip(pot, yo, t, int_method, rtol=None, atol=None, progres
 
 
 class PotentialTable:
-    """A potential decoded into a device-resident table (gh_potential_upload)."""
+    """A potential decoded into a device-resident table (gh_potential_upload,
+    or gh_planar_potential_upload from the planar packing when ``planar``)."""
 
-    def __init__(self, pot):
+    def __init__(self, pot, planar=False):
         self.pot = pot
-        npot, pot_type, pot_args, _ = parse_pot(pot)
+        self.planar = bool(planar)
+        npot, pot_type, pot_args, _ = (parse_pot_planar if planar else parse_pot)(pot)
         self.npot, self.pot_type, self.pot_args = npot, pot_type, pot_args
         lib = _lib.load()
         handle = ctypes.c_void_p()
+        upload = "gh_planar_potential_upload" if planar else "gh_potential_upload"
         _lib.check(
-            lib.gh_potential_upload(
+            getattr(lib, upload)(
                 npot, pot_type.ctypes.data, pot_args.ctypes.data, ctypes.byref(handle)
             ),
-            "gh_potential_upload",
+            upload,
         )
         self.handle = handle
 
@@ -130,11 +153,18 @@ def integrate_device(table, yo, t, int_method, rtol=None, atol=None, dt=None, re
     ``yo``: float64 ``[N,6]`` cuda tensor (overwritten with the rectangular
     state, as galpy does), ``t``: ``[nt]`` or ``[N,nt]``; ``result`` ``[N,nt,6]``,
     ``err`` int32 ``[N]`` and ``stats`` float64 ``[N,3]`` are allocated when
-    not given.  Returns ``(result, err, stats)``.
+    not given.  Returns ``(result, err, stats)``.  A ``yo`` of ``[N,4]``
+    rows (R,vR,vT,phi) integrates planar orbits (integratePlanarOrbit_hip,
+    ``result`` ``[N,nt,4]``); ``table`` must then come from the planar packing.
     """
     import torch
 
     assert yo.is_cuda and yo.dtype == torch.float64 and yo.is_contiguous()
+    ncol = yo.shape[-1]
+    assert yo.dim() == 2 and ncol in (4, 6)
+    planar = ncol == 4
+    if planar != getattr(table, "planar", False):
+        raise ValueError("planar orbits need a PotentialTable(pot, planar=True), full orbits a full one")
     assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
     nobj = yo.shape[0]
     indiv_t = t.dim() > 1
@@ -142,10 +172,10 @@ def integrate_device(table, yo, t, int_method, rtol=None, atol=None, dt=None, re
     if indiv_t:
         assert t.shape == (nobj, nt)
     if result is None:
-        result = torch.empty((nobj, nt, 6), dtype=torch.float64, device=yo.device)
+        result = torch.empty((nobj, nt, ncol), dtype=torch.float64, device=yo.device)
     if err is None:
         err = torch.zeros(nobj, dtype=torch.int32, device=yo.device)
-    assert result.shape == (nobj, nt, 6) and result.is_contiguous()
+    assert result.shape == (nobj, nt, ncol) and result.is_contiguous()
     assert err.shape == (nobj,) and err.dtype == torch.int32
     if stats is not None:
         assert stats.shape == (nobj, 3) and stats.dtype == torch.float64
@@ -154,12 +184,13 @@ def integrate_device(table, yo, t, int_method, rtol=None, atol=None, dt=None, re
         stream = torch.cuda.current_stream(yo.device)
     code = METHOD_CODES[int_method] if isinstance(int_method, str) else int(int_method)
     lib = _lib.load()
-    rc = lib.integrateFullOrbit_hip(
+    symbol = "integratePlanarOrbit_hip" if planar else "integrateFullOrbit_hip"
+    rc = getattr(lib, symbol)(
         nobj, yo.data_ptr(), nt, t.data_ptr(), int(indiv_t), table.ptr,
         DT_SENTINEL if dt is None else float(dt), rtol, atol, result.data_ptr(), err.data_ptr(),
         code, _ptr(stats), stream.cuda_stream,
     )
-    _lib.check(rc, "integrateFullOrbit_hip")
+    _lib.check(rc, symbol)
     return result, err, stats
 
 

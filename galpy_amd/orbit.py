@@ -2,9 +2,11 @@
 HIP integration methods.
 
 Mirrors galpy/orbit/Orbits.py: construction from ``[R,vR,vT,z,vz,phi]``
-rows (one or many orbits, :293), ``integrate(t, pot, method, dt, rtol, atol)``
-(:1987-2061) dispatching to the C-ABI through ``integrateFullOrbit_hip``
-(the ``integrateFullOrbit_c`` call at :1806), and the phase-space and
+rows, or planar ``[R,vR,vT,phi]`` rows (one or many orbits, :293),
+``integrate(t, pot, method, dt, rtol, atol)`` (:1987-2061) dispatching to the
+C-ABI through ``integrateFullOrbit_hip`` / ``integratePlanarOrbit_hip`` (the
+``integrateFullOrbit_c`` / ``integratePlanarOrbit_c`` calls at :1806 and
+:1867), and the phase-space and
 integral accessors used by galpy's tests: ``R, vR, vT, z, vz, phi, x, y,
 vx, vy`` (:2868), ``getOrbit``, ``E`` (:2902), ``Jacobi`` (:3298), ``L``,
 ``Lz``, ``ER``, ``Ez``, ``rperi``, ``rap``, ``zmax``, ``e``.
@@ -17,7 +19,7 @@ the C-ABI overwrites its ``yo`` argument.
 
 import numpy
 
-from .integrate import METHOD_CODES, integrateFullOrbit_hip
+from .integrate import METHOD_CODES, integrateFullOrbit_hip, integratePlanarOrbit_hip
 from .potential import evaluatePotentials, flatten
 
 _METHODS = tuple(METHOD_CODES)
@@ -34,15 +36,20 @@ def _check_integrate_dt(t, dt):
 
 
 class Orbit:
-    """One or many orbits in a potential; rows are (R,vR,vT,z,vz,phi)."""
+    """One or many orbits in a potential; rows are (R,vR,vT,z,vz,phi), or
+    (R,vR,vT,phi) for planar orbits."""
 
     def __init__(self, vxvv):
         vxvv = numpy.array(vxvv, dtype=float)
         self._single = vxvv.ndim == 1
         vxvv = numpy.atleast_2d(vxvv)
-        if vxvv.shape[-1] != 6:
-            raise ValueError("this Orbit supports full 3D orbits: rows of (R,vR,vT,z,vz,phi)")
+        if vxvv.shape[-1] not in (4, 6):
+            raise ValueError(
+                "this Orbit supports full 3D orbits, rows of (R,vR,vT,z,vz,phi), and planar "
+                "orbits, rows of (R,vR,vT,phi)"
+            )
         self.vxvv = vxvv
+        self._planar = vxvv.shape[-1] == 4
         self.t = None
         self.orbit = None
         self._pot = None
@@ -59,6 +66,12 @@ class Orbit:
     @property
     def size(self):
         return len(self.vxvv)
+
+    def dim(self):
+        return 2 if self._planar else 3
+
+    def phasedim(self):
+        return self.vxvv.shape[-1]
 
     # --- integration --------------------------------------------------------
     def integrate(self, t, pot, method="symplec4_hip", progressbar=False, dt=None, rtol=None,
@@ -80,8 +93,9 @@ class Orbit:
         if not _check_integrate_dt(tt, dt):
             raise ValueError("dt input (integrator stepsize) for Orbit.integrate must be an integer divisor of the output stepsize")
         flatten(pot)  # validates the potential
-        out, err = integrateFullOrbit_hip(pot, numpy.copy(self.vxvv), t, method.lower(),
-                                          rtol=rtol, atol=atol, progressbar=progressbar, dt=dt)
+        integrator = integratePlanarOrbit_hip if self._planar else integrateFullOrbit_hip
+        out, err = integrator(pot, numpy.copy(self.vxvv), t, method.lower(), rtol=rtol, atol=atol,
+                              progressbar=progressbar, dt=dt)
         self.orbit = out
         self._err = err
         self.t = t
@@ -138,13 +152,17 @@ class Orbit:
         return self._col(2, t)
 
     def z(self, t=None):
+        if self._planar:
+            raise AttributeError("linear and planar orbits do not have z()")  # Orbits.py:4726
         return self._col(3, t)
 
     def vz(self, t=None):
+        if self._planar:
+            raise AttributeError("linear and planar orbits do not have vz()")
         return self._col(4, t)
 
     def phi(self, t=None):
-        return self._col(5, t)
+        return self._col(self.phasedim() - 1, t)
 
     def x(self, t=None):
         return self.R(t) * numpy.cos(self.phi(t))
@@ -172,6 +190,9 @@ class Orbit:
         pot = self._pot if pot is None else pot
         rows = self._rows(t) if self.orbit is not None else self.vxvv
         tt = self._times_like(t, rows) if self.orbit is not None else 0.0
+        if self._planar:
+            R, vR, vT, phi = numpy.moveaxis(rows, -1, 0)
+            return self._out(evaluatePotentials(pot, R, 0.0, phi, tt) + 0.5 * (vR**2 + vT**2))
         R, vR, vT, z, vz, phi = numpy.moveaxis(rows, -1, 0)
         return self._out(evaluatePotentials(pot, R, z, phi, tt) + 0.5 * (vR**2 + vT**2 + vz**2))
 
@@ -189,7 +210,10 @@ class Orbit:
         return self.R(t) * self.vT(t)
 
     def L(self, t=None):
-        """Angular momentum vector (x,y,z components) from the rectangular state."""
+        """Angular momentum vector (x,y,z components) from the rectangular
+        state; R vT for planar orbits (Orbits.py:3147-3149)."""
+        if self._planar:
+            return self.Lz(t)
         x, y, z = self.x(t), self.y(t), self.z(t)
         vx, vy, vz = self.vx(t), self.vy(t), self.vz(t)
         return numpy.stack([y * vz - z * vy, z * vx - x * vz, x * vy - y * vx], axis=-1)
@@ -206,6 +230,8 @@ class Orbit:
 
     def _r(self):
         o = self._rows()
+        if self._planar:
+            return o[..., 0]
         return numpy.sqrt(o[..., 0] ** 2 + o[..., 3] ** 2)
 
     def rperi(self):
@@ -215,6 +241,8 @@ class Orbit:
         return self._out(self._r().max(axis=-1))
 
     def zmax(self):
+        if self._planar:
+            raise AttributeError("linear and planar orbits do not have zmax()")
         return self._out(numpy.fabs(self._rows()[..., 3]).max(axis=-1))
 
     def e(self):

@@ -11,7 +11,10 @@ scipy like galpy's Python layer; the integration itself runs in HIP.
 
 ``parse_pot`` produces galpy's ``(npot, pot_type, pot_args, pot_tfuncs)``
 exactly as ``_parse_pot`` does (galpy/orbit/integrateFullOrbit.py:43-614) for
-these classes: same integer codes and argument order.
+these classes: same integer codes and argument order.  ``parse_pot_planar``
+is the planar packing (galpy/orbit/integratePlanarOrbit.py:33-663, for the
+potentials galpy's ``toPlanarPotential`` turns into
+planarPotentialFromRZPotential / planarPotentialFromFullPotential).
 """
 
 import numpy
@@ -52,6 +55,11 @@ class Potential:
 
     def c_args(self):
         raise NotImplementedError
+
+    def c_args_planar(self):
+        """Arguments in the planar packing (integratePlanarOrbit.py:67-196);
+        the same as the full packing except where noted."""
+        return self.c_args()
 
     def __add__(self, other):
         return CompositePotential(*(_as_list(self) + _as_list(other)))
@@ -134,6 +142,9 @@ class PowerSphericalPotentialwCutoff(Potential):
 
     def c_args(self):
         return [self._amp, self.alpha, self.rc, 0.0, 0.0]  # two cache slots
+
+    def c_args_planar(self):
+        return [self._amp, self.alpha, self.rc]  # integratePlanarOrbit.py:192-196: no cache
 
 
 class MiyamotoNagaiPotential(Potential):
@@ -380,6 +391,27 @@ def parse_pot(pot):
             )
         pot_type.append(p.pot_code)
         pot_args.extend(p.c_args())
+    return (
+        len(comps),
+        numpy.array(pot_type, dtype=numpy.int32, order="C"),
+        numpy.array(pot_args, dtype=numpy.float64, order="C"),
+        [],
+    )
+
+
+def parse_pot_planar(pot):
+    """galpy's planar ``_parse_pot`` (integratePlanarOrbit.py:33-663) for the
+    supported potentials, taken in the plane z = 0: returns
+    ``(npot, pot_type int32[], pot_args float64[], [])``."""
+    comps = flatten(pot)
+    pot_type, pot_args = [], []
+    for p in comps:
+        if not isinstance(p, Potential) or p.pot_code is None:
+            raise NotImplementedError(
+                f"Potential {type(p).__name__} is not supported by the HIP orbit-integration backend"
+            )
+        pot_type.append(p.pot_code)
+        pot_args.extend(p.c_args_planar())
     return (
         len(comps),
         numpy.array(pot_type, dtype=numpy.int32, order="C"),

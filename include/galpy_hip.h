@@ -43,6 +43,23 @@ void integrateFullOrbit(int nobj, double *yo, int nt, double *t, int indiv_t, in
                         double rtol, double atol, double *result, int *err, int odeint_type,
                         void (*cb)(void));
 
+/* Replaces galpy/orbit/orbit_c_ext/integratePlanarOrbit.c:743-758
+ * (`integratePlanarOrbit`), same signature and argument meaning as
+ * integrateFullOrbit above for planar orbits:
+ *   yo[nobj*4]      initial conditions (R,vR,vT,phi), overwritten with the
+ *                   rectangular state (x,y,vx,vy) as galpy does
+ *   result[nobj*nt*4] output rows (R,vR,vT,phi); row 0 is the IC
+ *   pot_type/pot_args in galpy's planar packing
+ *                   (integratePlanarOrbit.py:33-663; integratePlanarOrbit.c:55-739):
+ *                   the same codes, PowerSphericalPotentialwCutoff with 3
+ *                   arguments (amp, alpha, rc) instead of 5
+ * The planar forces are galpy's planar kernels (planarRforce,
+ * planarphitorque), which equal the full forces in the plane z = 0. */
+void integratePlanarOrbit(int nobj, double *yo, int nt, double *t, int indiv_t, int npot,
+                          int *pot_type, double *pot_args, void *pot_tfuncs, double dt,
+                          double rtol, double atol, double *result, int *err, int odeint_type,
+                          void (*cb)(void));
+
 /* Device-resident twin of integrateFullOrbit for batching/serving callers
  * (no reference counterpart; it is integrateFullOrbit.c:1046 with the
  * buffers already in HBM).  d_yo, d_t, d_result, d_err and d_stats are device
@@ -56,11 +73,22 @@ int integrateFullOrbit_hip(int nobj, double *d_yo, int nt, const double *d_t, in
                            double *d_result, int *d_err, int odeint_type, double *d_stats,
                            void *stream);
 
+/* Device-resident twin of integratePlanarOrbit (integratePlanarOrbit.c:743
+ * with the buffers in HBM): as integrateFullOrbit_hip with d_yo[nobj*4]
+ * and d_result[nobj*nt*4]; pot from gh_planar_potential_upload. */
+int integratePlanarOrbit_hip(int nobj, double *d_yo, int nt, const double *d_t, int indiv_t,
+                             const void *pot, double dt, double rtol, double atol,
+                             double *d_result, int *d_err, int odeint_type, double *d_stats,
+                             void *stream);
+
 /* Decodes galpy's packed potential (integrateFullOrbit.c:69-1045 semantics)
  * into a device-resident table; *pot receives an opaque host handle (device
  * table + component codes, used to select a kernel specialised for the
  * composition).  gh_potential_free releases both. */
 int gh_potential_upload(int npot, const int *pot_type, const double *pot_args, void **pot);
+/* The same from galpy's planar packing (integratePlanarOrbit.c:55-739). */
+int gh_planar_potential_upload(int npot, const int *pot_type, const double *pot_args,
+                               void **pot);
 int gh_potential_free(void *pot);
 /* Device address of the table behind a handle. */
 const void *gh_potential_device_ptr(const void *pot);

@@ -3,7 +3,10 @@
 // Restates galpy's leapfrog / symplec4 (+ step estimators) from
 // galpy/util/bovy_symplecticode.c and DOP853 from galpy/util/leung_dop853.c,
 // keeping the reference's arithmetic sequence so results agree to rounding.
-// Templated on the scalar type R (double, or the op-counting CountedD).
+// Templated on the scalar type R (double, or the op-counting CountedD) and on
+// the spatial dimension D: 3 for integrateFullOrbit, 2 for
+// integratePlanarOrbit (the reference passes dim = 3 / 2 to the symplectic
+// steppers and 6 / 4 to dop853, integratePlanarOrbit.c:788-825).
 #pragma once
 #include "oracle.hpp"
 
@@ -25,29 +28,39 @@ struct IntStats {
   double dt;           // step actually used (symplectic)
 };
 
-template <class R> struct ForceCtx {
+// evalRectForce / evalRectDeriv (full orbits, D = 3) or evalPlanarRectForce /
+// evalPlanarRectDeriv (planar orbits, D = 2)
+template <class R, int D = 3> struct ForceCtx {
   int npot;
   const Component* P;
   long long* nrhs;
-  void force(R t, const R* q, R* a) const { (*nrhs)++; rect_force(npot, P, t, q, a); }
-  void deriv(R t, const R* q, R* a) const { (*nrhs)++; rect_deriv(npot, P, t, q, a); }
+  void force(R t, const R* q, R* a) const {
+    (*nrhs)++;
+    if constexpr (D == 2) planar_rect_force(npot, P, t, q, a);
+    else rect_force(npot, P, t, q, a);
+  }
+  void deriv(R t, const R* q, R* a) const {
+    (*nrhs)++;
+    if constexpr (D == 2) planar_rect_deriv(npot, P, t, q, a);
+    else rect_deriv(npot, P, t, q, a);
+  }
 };
 
 // bovy_symplecticode.c:63-72
-template <class R> inline void leapq(const R* q, const R* p, R dt, R* qn) {
-  for (int i = 0; i < 3; i++) qn[i] = q[i] + dt * p[i];
+template <int D, class R> inline void leapq(const R* q, const R* p, R dt, R* qn) {
+  for (int i = 0; i < D; i++) qn[i] = q[i] + dt * p[i];
 }
-template <class R> inline void leapp(const R* p, R dt, const R* a, R* pn) {
-  for (int i = 0; i < 3; i++) pn[i] = p[i] + dt * a[i];
+template <int D, class R> inline void leapp(const R* p, R dt, const R* a, R* pn) {
+  for (int i = 0; i < D; i++) pn[i] = p[i] + dt * a[i];
 }
 
 // Shared scale set-up of the step estimators, bovy_symplecticode.c:606-623
-template <class R> void estimate_scales(const R* qo, const R* po, double rtol, double atol,
-                                        R& sq, R& sp) {
+template <int D, class R> void estimate_scales(const R* qo, const R* po, double rtol, double atol,
+                                               R& sq, R& sp) {
   R mq = log(fabs(qo[0]));
-  for (int i = 1; i < 3; i++) if (log(fabs(qo[i])) > mq) mq = log(fabs(qo[i]));
+  for (int i = 1; i < D; i++) if (log(fabs(qo[i])) > mq) mq = log(fabs(qo[i]));
   R mp = log(fabs(po[0]));
-  for (int i = 1; i < 3; i++) if (log(fabs(po[i])) > mp) mp = log(fabs(po[i]));
+  for (int i = 1; i < D; i++) if (log(fabs(po[i])) > mp) mp = log(fabs(po[i]));
   R c = rmax(R(atol), R(rtol) + mq);
   sq = log(exp(R(atol) - c) + exp(R(rtol) + mq - c)) + c;
   c = rmax(R(atol), R(rtol) + mp);
@@ -55,192 +68,193 @@ template <class R> void estimate_scales(const R* qo, const R* po, double rtol, d
 }
 
 // error norm, bovy_symplecticode.c:641-646 / 795-800
-template <class R> R estimate_err(const R* q11, const R* q12, const R* p11, const R* p12, R sq, R sp) {
+template <int D, class R> R estimate_err(const R* q11, const R* q12, const R* p11, const R* p12, R sq, R sp) {
   R err = 0.;
-  for (int i = 0; i < 3; i++) {
+  for (int i = 0; i < D; i++) {
     err += exp(R(2.) * log(fabs(q11[i] - q12[i])) - R(2.) * sq);
     err += exp(R(2.) * log(fabs(p11[i] - p12[i])) - R(2.) * sp);
   }
-  return sqrt(err / R(2.) / R(3.));
+  return sqrt(err / R(2.) / R((double)D));
 }
 
 // leapfrog_estimate_step, bovy_symplecticode.c:588-663
-template <class R>
-R leapfrog_estimate_step(const ForceCtx<R>& F, const R* qo, const R* po, R dt, R to,
+template <int D, class R>
+R leapfrog_estimate_step(const ForceCtx<R, D>& F, const R* qo, const R* po, R dt, R to,
                          double rtol, double atol) {
   R err = 2.;
   R init_dt = dt;
   R q11[3], q12[3], p11[3], p12[3], qtmp[3], ptmp[3], a[3];
   R sq, sp;
-  estimate_scales(qo, po, rtol, atol, sq, sp);
+  estimate_scales<D>(qo, po, rtol, atol, sq, sp);
   dt *= 2.;
   while (err > R(1.) && init_dt / dt < R(kMaxDtReduce)) {
     dt /= 2.;
-    leapq(qo, po, dt / R(2.), q12);
+    leapq<D>(qo, po, dt / R(2.), q12);
     F.force(to + dt / R(2.), q12, a);
-    leapp(po, dt, a, p11);
-    leapq(q12, p11, dt / R(2.), q11);
-    leapq(qo, po, dt / R(4.), q12);
+    leapp<D>(po, dt, a, p11);
+    leapq<D>(q12, p11, dt / R(2.), q11);
+    leapq<D>(qo, po, dt / R(4.), q12);
     F.force(to + dt / R(4.), q12, a);
-    leapp(po, dt / R(2.), a, ptmp);
-    leapq(q12, ptmp, dt / R(2.), qtmp);
+    leapp<D>(po, dt / R(2.), a, ptmp);
+    leapq<D>(q12, ptmp, dt / R(2.), qtmp);
     F.force(to + R(3.) * dt / R(4.), qtmp, a);
-    leapp(ptmp, dt / R(2.), a, p12);
-    leapq(qtmp, p12, dt / R(4.), q12);
-    err = estimate_err(q11, q12, p11, p12, sq, sp);
+    leapp<D>(ptmp, dt / R(2.), a, p12);
+    leapq<D>(qtmp, p12, dt / R(4.), q12);
+    err = estimate_err<D>(q11, q12, p11, p12, sq, sp);
   }
   return dt;
 }
 
 // symplec4_estimate_step, bovy_symplecticode.c:665-808
-template <class R>
-R symplec4_estimate_step(const ForceCtx<R>& F, const R* qo, const R* po, R dt, R to,
+template <int D, class R>
+R symplec4_estimate_step(const ForceCtx<R, D>& F, const R* qo, const R* po, R dt, R to,
                          double rtol, double atol) {
   const R c1 = S4C1, c2 = S4C2, c3 = S4C3, c4 = S4C4, d1 = S4D1, d2 = S4D2, d3 = S4D3;
   R err = 2.;
   R init_dt = dt;
   R q11[3], q12[3], p11[3], p12[3], qtmp[3], ptmp[3], a[3];
   R sq, sp;
-  estimate_scales(qo, po, rtol, atol, sq, sp);
+  estimate_scales<D>(qo, po, rtol, atol, sq, sp);
   dt *= 2.;
   while (err > R(1.) && init_dt / dt < R(kMaxDtReduce)) {
     dt /= 2.;
     // one step of dt
-    leapq(qo, po, c1 * dt, q12);
+    leapq<D>(qo, po, c1 * dt, q12);
     to += c1 * dt;
     F.force(to, q12, a);
-    leapp(po, d1 * dt, a, p12);
-    leapq(q12, p12, c2 * dt, qtmp);
+    leapp<D>(po, d1 * dt, a, p12);
+    leapq<D>(q12, p12, c2 * dt, qtmp);
     to += c2 * dt;
     F.force(to, qtmp, a);
-    leapp(p12, d2 * dt, a, ptmp);
-    leapq(qtmp, ptmp, c3 * dt, q12);
+    leapp<D>(p12, d2 * dt, a, ptmp);
+    leapq<D>(qtmp, ptmp, c3 * dt, q12);
     to += c3 * dt;
     F.force(to, q12, a);
-    leapp(ptmp, d3 * dt, a, p11);
-    leapq(q12, p11, c4 * dt, q11);
+    leapp<D>(ptmp, d3 * dt, a, p11);
+    leapq<D>(q12, p11, c4 * dt, q11);
     to += c4 * dt;
     to -= dt;
     // two steps of dt/2
-    leapq(qo, po, c1 * dt / R(2.), q12);
+    leapq<D>(qo, po, c1 * dt / R(2.), q12);
     to += c1 * dt / R(2.);
     F.force(to, q12, a);
-    leapp(po, d1 * dt / R(2.), a, p12);
-    leapq(q12, p12, c2 * dt / R(2.), qtmp);
+    leapp<D>(po, d1 * dt / R(2.), a, p12);
+    leapq<D>(q12, p12, c2 * dt / R(2.), qtmp);
     to += c2 * dt / R(2.);
     F.force(to, qtmp, a);
-    leapp(p12, d2 * dt / R(2.), a, ptmp);
-    leapq(qtmp, ptmp, c3 * dt / R(2.), q12);
+    leapp<D>(p12, d2 * dt / R(2.), a, ptmp);
+    leapq<D>(qtmp, ptmp, c3 * dt / R(2.), q12);
     to += c3 * dt / R(2.);
     F.force(to, q12, a);
-    leapp(ptmp, d3 * dt / R(2.), a, p12);
-    leapq(q12, p12, (c1 + c4) * dt / R(2.), qtmp);
+    leapp<D>(ptmp, d3 * dt / R(2.), a, p12);
+    leapq<D>(q12, p12, (c1 + c4) * dt / R(2.), qtmp);
     to += (c1 + c4) * dt / R(2.);
     F.force(to, qtmp, a);
-    leapp(p12, d1 * dt / R(2.), a, ptmp);
-    leapq(qtmp, ptmp, c2 * dt / R(2.), q12);
+    leapp<D>(p12, d1 * dt / R(2.), a, ptmp);
+    leapq<D>(qtmp, ptmp, c2 * dt / R(2.), q12);
     to += c2 * dt / R(2.);
     F.force(to, q12, a);
-    leapp(ptmp, d2 * dt / R(2.), a, p12);
-    leapq(q12, p12, c3 * dt / R(2.), qtmp);
+    leapp<D>(ptmp, d2 * dt / R(2.), a, p12);
+    leapq<D>(q12, p12, c3 * dt / R(2.), qtmp);
     to += c3 * dt / R(2.);
     F.force(to, qtmp, a);
-    leapp(p12, d3 * dt / R(2.), a, ptmp);
-    leapq(qtmp, ptmp, c4 * dt / R(2.), q12);
+    leapp<D>(p12, d3 * dt / R(2.), a, ptmp);
+    leapq<D>(qtmp, ptmp, c4 * dt / R(2.), q12);
     to += c4 * dt / R(2.);
-    for (int i = 0; i < 3; i++) p12[i] = ptmp[i];
-    err = estimate_err(q11, q12, p11, p12, sq, sp);
+    for (int i = 0; i < D; i++) p12[i] = ptmp[i];
+    err = estimate_err<D>(q11, q12, p11, p12, sq, sp);
     to -= dt;
   }
   return dt;
 }
 
-template <class R> inline void save6(const R* q, const R* p, double* out) {
-  for (int i = 0; i < 3; i++) { out[i] = dval(q[i]); out[3 + i] = dval(p[i]); }
+// save_qp, bovy_symplecticode.c:74-78
+template <int D, class R> inline void save_qp(const R* q, const R* p, double* out) {
+  for (int i = 0; i < D; i++) { out[i] = dval(q[i]); out[D + i] = dval(p[i]); }
 }
 
-// leapfrog, bovy_symplecticode.c:132-224.  yo = [x,y,z,vx,vy,vz];
-// result = nt rows of 6; returns the step count in st.
-template <class R>
-void leapfrog(const ForceCtx<R>& F, const double* yo, int nt, double dt_in, const double* t,
+// leapfrog, bovy_symplecticode.c:132-224.  yo = [q, p] (2D values);
+// result = nt rows of 2D; returns the step count in st.
+template <int D, class R>
+void leapfrog(const ForceCtx<R, D>& F, const double* yo, int nt, double dt_in, const double* t,
               double rtol, double atol, double* result, int* err, IntStats* st) {
   R qo[3], po[3], q12[3], p12[3], a[3];
-  for (int i = 0; i < 3; i++) { qo[i] = yo[i]; po[i] = yo[3 + i]; }
-  save6(qo, po, result);
-  result += 6;
+  for (int i = 0; i < D; i++) { qo[i] = yo[i]; po[i] = yo[D + i]; }
+  save_qp<D>(qo, po, result);
+  result += 2 * D;
   *err = 0;
   R init_dt = R(t[1]) - R(t[0]);
   R dt = dt_in;
-  if (dt_in == kDtSentinel) dt = leapfrog_estimate_step(F, qo, po, init_dt, R(t[0]), rtol, atol);
+  if (dt_in == kDtSentinel) dt = leapfrog_estimate_step<D>(F, qo, po, init_dt, R(t[0]), rtol, atol);
   long ndt = (long)(dval(init_dt / dt));
   R to = t[0];
   for (int ii = 0; ii < nt - 1; ii++) {
-    leapq(qo, po, dt / R(2.), q12);
+    leapq<D>(qo, po, dt / R(2.), q12);
     for (long jj = 0; jj < ndt - 1; jj++) {
       F.force(to + dt / R(2.), q12, a);
-      leapp(po, dt, a, p12);
-      leapq(q12, p12, dt, qo);
+      leapp<D>(po, dt, a, p12);
+      leapq<D>(q12, p12, dt, qo);
       to = to + dt;
-      for (int k = 0; k < 3; k++) { q12[k] = qo[k]; po[k] = p12[k]; }
+      for (int k = 0; k < D; k++) { q12[k] = qo[k]; po[k] = p12[k]; }
     }
     F.force(to + dt / R(2.), q12, a);
-    leapp(po, dt, a, po);
-    leapq(q12, po, dt / R(2.), qo);
+    leapp<D>(po, dt, a, po);
+    leapq<D>(q12, po, dt / R(2.), qo);
     to = to + dt;
-    save6(qo, po, result);
-    result += 6;
+    save_qp<D>(qo, po, result);
+    result += 2 * D;
   }
   if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
 }
 
 // symplec4, bovy_symplecticode.c:250-377
-template <class R>
-void symplec4(const ForceCtx<R>& F, const double* yo, int nt, double dt_in, const double* t,
+template <int D, class R>
+void symplec4(const ForceCtx<R, D>& F, const double* yo, int nt, double dt_in, const double* t,
               double rtol, double atol, double* result, int* err, IntStats* st) {
   const R c1 = S4C1, c2 = S4C2, c3 = S4C3, c4 = S4C4, d1 = S4D1, d2 = S4D2, d3 = S4D3;
   R qo[3], po[3], q12[3], p12[3], a[3];
-  for (int i = 0; i < 3; i++) { qo[i] = yo[i]; po[i] = yo[3 + i]; }
-  save6(qo, po, result);
-  result += 6;
+  for (int i = 0; i < D; i++) { qo[i] = yo[i]; po[i] = yo[D + i]; }
+  save_qp<D>(qo, po, result);
+  result += 2 * D;
   *err = 0;
   R init_dt = R(t[1]) - R(t[0]);
   R dt = dt_in;
-  if (dt_in == kDtSentinel) dt = symplec4_estimate_step(F, qo, po, init_dt, R(t[0]), rtol, atol);
+  if (dt_in == kDtSentinel) dt = symplec4_estimate_step<D>(F, qo, po, init_dt, R(t[0]), rtol, atol);
   long ndt = (long)(dval(init_dt / dt));
   R to = t[0];
   for (int ii = 0; ii < nt - 1; ii++) {
-    leapq(qo, po, c1 * dt, q12);
+    leapq<D>(qo, po, c1 * dt, q12);
     to += c1 * dt;
     for (long jj = 0; jj < ndt - 1; jj++) {
       F.force(to, q12, a);
-      leapp(po, d1 * dt, a, p12);
-      leapq(q12, p12, c2 * dt, qo);
+      leapp<D>(po, d1 * dt, a, p12);
+      leapq<D>(q12, p12, c2 * dt, qo);
       to += c2 * dt;
       F.force(to, qo, a);
-      leapp(p12, d2 * dt, a, po);
-      leapq(qo, po, c3 * dt, q12);
+      leapp<D>(p12, d2 * dt, a, po);
+      leapq<D>(qo, po, c3 * dt, q12);
       to += c3 * dt;
       F.force(to, q12, a);
-      leapp(po, d3 * dt, a, p12);
-      leapq(q12, p12, (c4 + c1) * dt, qo);
+      leapp<D>(po, d3 * dt, a, p12);
+      leapq<D>(q12, p12, (c4 + c1) * dt, qo);
       to += (c4 + c1) * dt;
-      for (int k = 0; k < 3; k++) { q12[k] = qo[k]; po[k] = p12[k]; }
+      for (int k = 0; k < D; k++) { q12[k] = qo[k]; po[k] = p12[k]; }
     }
     F.force(to, q12, a);
-    leapp(po, d1 * dt, a, p12);
-    leapq(q12, p12, c2 * dt, qo);
+    leapp<D>(po, d1 * dt, a, p12);
+    leapq<D>(q12, p12, c2 * dt, qo);
     to += c2 * dt;
     F.force(to, qo, a);
-    leapp(p12, d2 * dt, a, po);
-    leapq(qo, po, c3 * dt, q12);
+    leapp<D>(p12, d2 * dt, a, po);
+    leapq<D>(qo, po, c3 * dt, q12);
     to += c3 * dt;
     F.force(to, q12, a);
-    leapp(po, d3 * dt, a, p12);
-    leapq(q12, p12, c4 * dt, qo);
+    leapp<D>(po, d3 * dt, a, p12);
+    leapq<D>(q12, p12, c4 * dt, qo);
     to += c4 * dt;
-    for (int k = 0; k < 3; k++) po[k] = p12[k];
-    save6(qo, po, result);
-    result += 6;
+    for (int k = 0; k < D; k++) po[k] = p12[k];
+    save_qp<D>(qo, po, result);
+    result += 2 * D;
   }
   if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
 }
@@ -269,8 +283,8 @@ bool symplectic_dxdv(int npot, const Component* P, int method, const double* w_i
   R init_dt = R(t[1]) - R(t[0]);
   R dt = dt_in;
   if (dt_in == kDtSentinel)
-    dt = (method == 0) ? leapfrog_estimate_step(F, q, p, init_dt, R(t[0]), rtol, atol)
-                       : symplec4_estimate_step(F, q, p, init_dt, R(t[0]), rtol, atol);
+    dt = (method == 0) ? leapfrog_estimate_step<3>(F, q, p, init_dt, R(t[0]), rtol, atol)
+                       : symplec4_estimate_step<3>(F, q, p, init_dt, R(t[0]), rtol, atol);
   long ndt = (long)(dval(init_dt / dt));
   R to = t[0];
   auto drift = [&](R h) {
@@ -415,13 +429,14 @@ constexpr double uround = 2.3e-16;
 
 template <class R> inline R csign(R x, R y) { return (y > R(0.)) ? fabs(x) : -fabs(x); }
 
-// dop853, leung_dop853.c:231-517.  y0 (6, rectangular) is advanced in place,
-// as in the reference; result = nt rows of 6 at the requested times.
-template <class R>
-void dop853(const ForceCtx<R>& F, double* y0d, int nt, const double* t, double rtol_log,
+// dop853, leung_dop853.c:231-517.  y0 (2D values, rectangular) is advanced
+// in place, as in the reference; result = nt rows of 2D at the requested
+// times.
+template <int D, class R>
+void dop853(const ForceCtx<R, D>& F, double* y0d, int nt, const double* t, double rtol_log,
             double atol_log, double* result, int* err, IntStats* st) {
   using namespace dp;
-  const int dim = 6;
+  const int dim = 2 * D;
   R rtol = exp(R(rtol_log));
   R atol = exp(R(atol_log));
   const R safe = 0.9, beta = 0.0;

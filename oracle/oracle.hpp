@@ -4,7 +4,9 @@
 // algorithm galpy runs inside `integrateFullOrbit`: the six potential force
 // kernels used by the benchmark configurations, the cylindrical aggregators,
 // the rectangular force callbacks, the coordinate transforms, leapfrog,
-// symplec4 (+ their step estimators) and DOP853 with dense output.  Only the
+// symplec4 (+ their step estimators) and DOP853 with dense output -- and the
+// planar twins of the forces, callbacks and transforms that
+// `integratePlanarOrbit` uses.  Only the
 // tests, the smoke check and the CPU-baseline leg of the benchmark may load
 // the library built from it; the HIP product never calls it.
 //
@@ -154,25 +156,27 @@ struct Component {
   double c[6];  // precomputed constants
 };
 
-// nargs per code, as consumed from pot_args (integrateFullOrbit.c:80-345)
-inline int component_nargs(int code) {
+// nargs per code, as consumed from pot_args (integrateFullOrbit.c:80-345;
+// planar packing integratePlanarOrbit.c:64-223, where
+// PowerSphericalPotentialwCutoff has 3 arguments and no cache slots)
+inline int component_nargs(int code, bool planar = false) {
   switch (code) {
     case kLogHalo: return 4;
     case kDehnenBar: return 6;
     case kMiyamotoNagai: return 3;
     case kHernquist: return 2;
     case kNFW: return 2;
-    case kPowerSphCutoff: return 5;
+    case kPowerSphCutoff: return planar ? 3 : 5;
     default: return -1;
   }
 }
 
 // Decode; returns 0 on success, -1 on an unsupported code.
 inline int parse_potential(int npot, const int* pot_type, const double* pot_args,
-                           Component* out) {
+                           Component* out, bool planar = false) {
   for (int i = 0; i < npot; i++) {
     int code = pot_type[i];
-    int na = component_nargs(code);
+    int na = component_nargs(code, planar);
     if (na < 0) return -1;
     Component& C = out[i];
     C.type = code;
@@ -471,6 +475,104 @@ bool rect_force_dxdv(int npot, const Component* P, R t, const R* q, R* a, int nd
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// Planar forces (integratePlanarOrbit.c:55-739 wires each code to these
+// functions): galpy's separate planarRforce / planarphitorque kernels, as
+// written in the component files.
+// ---------------------------------------------------------------------------
+// PowerSphericalPotentialwCutoff.c:47-58
+template <class R> R psc_planar_Rforce(const Component& C, R Rc) {
+  double amp = C.p[0], rc = C.p[2];
+  R r2 = Rc * Rc;
+  R x = r2 / R(rc) / R(rc);
+  R P = lower_incgamma(C.c[0], x, C.c[1]) / R(C.c[1]);
+  R mass = R(C.c[2]) * (R(C.c[1]) * P);
+  return -R(amp) * mass / r2;
+}
+// MiyamotoNagaiPotential.c:27-37
+template <class R> R mn_planar_Rforce(const Component& C, R Rc) {
+  double amp = C.p[0], a = C.p[1], b = C.p[2];
+  return -R(amp) * Rc * pow(Rc * Rc + pow(R(a + b), R(2.)), R(-1.5));
+}
+// NFWPotential.c:28-37
+template <class R> R nfw_planar_Rforce(const Component& C, R Rc) {
+  double amp = C.p[0], a = C.p[1];
+  return R(amp) / Rc * (R(1.) / (R(a) + Rc) - log(R(1.) + Rc / R(a)) / Rc);
+}
+// HernquistPotential.c:26-35
+template <class R> R hq_planar_Rforce(const Component& C, R Rc) {
+  double amp = C.p[0], a = C.p[1];
+  return -R(amp) / R(a) * pow(R(1.) + Rc / R(a), R(-2.)) / R(2.) / R(a);
+}
+// LogarithmicHaloPotential.c:39-54 (planarRforce), :92-107 (planarphitorque)
+template <class R> void lh_planar_forces(const Component& C, R Rc, R phi, R& FR, R& Fphi) {
+  double amp = C.p[0], c = C.p[2], onem1overb2 = C.p[3];
+  if (onem1overb2 < 1) {
+    R Rt2 = Rc * Rc * (R(1.) - R(onem1overb2) * pow(sin(phi), R(2)));
+    FR = -R(amp) * Rt2 / Rc / (Rt2 + R(c));
+    Fphi = R(amp) * Rc * Rc / (Rt2 + R(c)) * sin(R(2) * phi) * R(onem1overb2) / R(2.);
+  } else {
+    FR = -R(amp) * Rc / (Rc * Rc + R(c));
+    Fphi = R(0.);
+  }
+}
+// DehnenBarPotential.c:41-59 (planarRforce), :83-101 (planarphitorque)
+template <class R> void bar_planar_forces(const Component& C, R Rc, R phi, R t, R& FR, R& Fphi) {
+  double amp = C.p[0], tform = C.p[1], tsteady = C.p[2], rb = C.p[3], omegab = C.p[4], barphi = C.p[5];
+  R smooth = dehnen_smooth(t, tform, tsteady);
+  R ang = R(2.) * (phi - R(omegab) * t - R(barphi));
+  if (Rc <= R(rb)) {
+    FR = -R(3.) * R(amp) * smooth * cos(ang) * pow(Rc / R(rb), R(3.)) / Rc;
+    Fphi = R(2.) * R(amp) * smooth * sin(ang) * (pow(Rc / R(rb), R(3.)) - R(2.));
+  } else {
+    FR = -R(3.) * R(amp) * smooth * cos(ang) * pow(R(rb) / Rc, R(3.)) / Rc;
+    Fphi = -R(2.) * R(amp) * smooth * sin(ang) * pow(R(rb) / Rc, R(3.));
+  }
+}
+
+// calcPlanarRforce / calcPlanarphitorque, galpy_potentials.c:180-209: sums
+// over components in index order from 0.
+template <class R>
+void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fphi) {
+  FR = 0.; Fphi = 0.;
+  for (int i = 0; i < npot; i++) {
+    const Component& C = P[i];
+    R fr = 0., fp = 0.;
+    switch (C.type) {
+      case kPowerSphCutoff: fr = psc_planar_Rforce(C, Rc); break;
+      case kMiyamotoNagai: fr = mn_planar_Rforce(C, Rc); break;
+      case kNFW: fr = nfw_planar_Rforce(C, Rc); break;
+      case kHernquist: fr = hq_planar_Rforce(C, Rc); break;
+      case kLogHalo: lh_planar_forces(C, Rc, phi, fr, fp); break;
+      case kDehnenBar: bar_planar_forces(C, Rc, phi, t, fr, fp); break;
+    }
+    FR += fr; Fphi += fp;
+  }
+}
+
+// evalPlanarRectForce: integratePlanarOrbit.c:1015-1031
+template <class R>
+void planar_rect_force(int npot, const Component* P, R t, const R* q, R* a) {
+  R x = q[0], y = q[1];
+  R Rc = sqrt(x * x + y * y);
+  R phi = acos(x / Rc);
+  R sinphi = y / Rc;
+  R cosphi = x / Rc;
+  if (y < R(0.)) phi = R(2. * M_PI) - phi;
+  R FR, Fphi;
+  planar_forces(npot, P, Rc, phi, t, FR, Fphi);
+  a[0] = cosphi * FR - R(1.) / Rc * sinphi * Fphi;
+  a[1] = sinphi * FR + R(1.) / Rc * cosphi * Fphi;
+}
+
+// evalPlanarRectDeriv: integratePlanarOrbit.c:1032-1055 (vR, vT feed only
+// dissipative forces, none in scope)
+template <class R>
+void planar_rect_deriv(int npot, const Component* P, R t, const R* q, R* a) {
+  a[0] = q[2]; a[1] = q[3];
+  planar_rect_force(npot, P, t, q, a + 2);
+}
+
 // Potential value: galpy_potentials.c evaluatePotentials analog (sum in
 // component order); the bar uses the Python expression.
 template <class R>
@@ -525,6 +627,25 @@ template <class R> void cyl_to_rect(R* v) {
   v[5] = v[4];
   v[3] = vR * cp - vT * sp;
   v[4] = vR * sp + vT * cp;
+}
+// bovy_coords.c:57-69 (planar: (R,vR,vT,phi) -> (x,y,vx,vy))
+template <class R> void polar_to_rect(R* v) {
+  R Rc = v[0], phi = v[3];
+  R cp = cos(phi), sp = sin(phi);
+  R vR = v[1], vT = v[2];
+  v[0] = Rc * cp;
+  v[1] = Rc * sp;
+  v[2] = vR * cp - vT * sp;
+  v[3] = vR * sp + vT * cp;
+}
+// bovy_coords.c:79-91 (planar: (x,y,vx,vy) -> (R,vR,vT,phi))
+template <class R> void rect_to_polar(R* v) {
+  R x = v[0], y = v[1], vx = v[2], vy = v[3];
+  v[3] = atan2(y, x);
+  R cp = cos(v[3]), sp = sin(v[3]);
+  v[0] = sqrt(x * x + y * y);
+  v[1] = vx * cp + vy * sp;
+  v[2] = -vx * sp + vy * cp;
 }
 // bovy_coords.c:125-139
 template <class R> void rect_to_cyl(R* v) {

@@ -2,7 +2,9 @@
 //
 // `oracle_integrateFullOrbit` has the signature of galpy's
 // `integrateFullOrbit` (galpy/orbit/orbit_c_ext/integrateFullOrbit.c:1046-1061)
-// and the same OpenMP structure (:1133, schedule(dynamic,1) over orbits).
+// and the same OpenMP structure (:1133, schedule(dynamic,1) over orbits);
+// `oracle_integratePlanarOrbit` is the same for galpy's `integratePlanarOrbit`
+// (galpy/orbit/orbit_c_ext/integratePlanarOrbit.c:743-845).
 // The remaining entry points expose the pieces (forces, potential values,
 // incomplete gamma, coordinate transforms) for unit checks, plus an
 // op-counting run that yields the algorithmic FLOP count per orbit-step.
@@ -23,35 +25,45 @@ namespace {
 
 constexpr int kMaxPot = 16;
 
-template <class R>
+// One orbit: integrateFullOrbit.c:1133-1141 (D = 3, rows of 6) or
+// integratePlanarOrbit.c:826-835 (D = 2, rows of 4).
+template <int D, class R>
 void integrate_one(int npot, const Component* P, int odeint_type, double* yo, int nt,
                    const double* t, double dt, double rtol, double atol, double* result,
                    int* err, IntStats* st) {
+  constexpr int N = 2 * D;
   long long nrhs = 0;
-  ForceCtx<R> F{npot, P, &nrhs};
+  ForceCtx<R, D> F{npot, P, &nrhs};
   IntStats local = {0, 0, 0, 0.};
-  cyl_to_rect(yo);  // integrateFullOrbit.c:1135 (in place, as the reference)
+  // in place, as the reference (integrateFullOrbit.c:1135, integratePlanarOrbit.c:828)
+  if (D == 3) cyl_to_rect(yo);
+  else polar_to_rect(yo);
   switch (odeint_type) {
-    case 0: leapfrog(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
-    case 3: symplec4(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
-    case 6: dop853(F, yo, nt, t, rtol, atol, result, err, &local); break;
+    case 0: leapfrog<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
+    case 3: symplec4<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
+    case 6: dop853<D>(F, yo, nt, t, rtol, atol, result, err, &local); break;
     default: *err = -1; return;
   }
-  for (int j = 0; j < nt; j++) rect_to_cyl(result + 6 * j);  // :1139-1140
+  for (int j = 0; j < nt; j++) {  // integrateFullOrbit.c:1139-1140, integratePlanarOrbit.c:832-833
+    if (D == 3) rect_to_cyl(result + N * j);
+    else rect_to_polar(result + N * j);
+  }
   local.rhs = nrhs;
   if (st) *st = local;
 }
 
+template <int D>
 int run(int nobj, double* yo, int nt, const double* t, int indiv_t, int npot,
         const int* pot_type, const double* pot_args, double dt, double rtol, double atol,
         double* result, int* err, int odeint_type, void (*cb)(void), long long* steps,
         long long* rejected, long long* nrhs, double* dt_used, int nthreads) {
+  constexpr int N = 2 * D;
   if (npot > kMaxPot || npot < 0 || nt < 2) {
     for (int i = 0; i < nobj; i++) err[i] = -1;
     return -1;
   }
   Component P[kMaxPot];
-  if (parse_potential(npot, pot_type, pot_args, P) != 0) {
+  if (parse_potential(npot, pot_type, pot_args, P, D == 2) != 0) {
     for (int i = 0; i < nobj; i++) err[i] = -1;
     return -1;
   }
@@ -60,8 +72,8 @@ int run(int nobj, double* yo, int nt, const double* t, int indiv_t, int npot,
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nth)
   for (int ii = 0; ii < nobj; ii++) {
     IntStats st = {0, 0, 0, 0.};
-    integrate_one<double>(npot, P, odeint_type, yo + 6 * ii, nt, t + (long)nt * ii * indiv_t, dt,
-                          rtol, atol, result + 6L * nt * ii, err + ii, &st);
+    integrate_one<D, double>(npot, P, odeint_type, yo + N * ii, nt, t + (long)nt * ii * indiv_t,
+                             dt, rtol, atol, result + (long)N * nt * ii, err + ii, &st);
     if (steps) steps[ii] = st.steps;
     if (rejected) rejected[ii] = st.rejected;
     if (nrhs) nrhs[ii] = st.rhs;
@@ -81,8 +93,20 @@ ORC_EXPORT void oracle_integrateFullOrbit(int nobj, double* yo, int nt, double* 
                                           double* result, int* err, int odeint_type,
                                           void (*cb)(void)) {
   (void)pot_tfuncs;
-  run(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result, err,
-      odeint_type, cb, nullptr, nullptr, nullptr, nullptr, 0);
+  run<3>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result, err,
+         odeint_type, cb, nullptr, nullptr, nullptr, nullptr, 0);
+}
+
+// Same signature and argument meaning as galpy's integratePlanarOrbit
+// (integratePlanarOrbit.c:743-758): rows of (R,vR,vT,phi), planar packing.
+ORC_EXPORT void oracle_integratePlanarOrbit(int nobj, double* yo, int nt, double* t, int indiv_t,
+                                            int npot, int* pot_type, double* pot_args,
+                                            void* pot_tfuncs, double dt, double rtol, double atol,
+                                            double* result, int* err, int odeint_type,
+                                            void (*cb)(void)) {
+  (void)pot_tfuncs;
+  run<2>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result, err,
+         odeint_type, cb, nullptr, nullptr, nullptr, nullptr, 0);
 }
 
 // As above plus per-orbit statistics and an explicit thread count (0 = all).
@@ -91,26 +115,37 @@ ORC_EXPORT int oracle_integrate_stats(int nobj, double* yo, int nt, double* t, i
                                       double rtol, double atol, double* result, int* err,
                                       int odeint_type, long long* steps, long long* rejected,
                                       long long* nrhs, double* dt_used, int nthreads) {
-  return run(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result, err,
-             odeint_type, nullptr, steps, rejected, nrhs, dt_used, nthreads);
+  return run<3>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result, err,
+                odeint_type, nullptr, steps, rejected, nrhs, dt_used, nthreads);
 }
 
-// Op-counting run (single thread).  out[0..7] = add, mul, div, sqrt, special,
-// orbit-steps, rejected steps, force/RHS evaluations, summed over the orbits.
-ORC_EXPORT int oracle_count_ops(int nobj, double* yo, int nt, double* t, int indiv_t, int npot,
-                                int* pot_type, double* pot_args, double dt, double rtol,
-                                double atol, double* result, int* err, int odeint_type,
-                                double* out) {
+// The planar twin of oracle_integrate_stats (rows of 4, planar packing).
+ORC_EXPORT int oracle_integrate_planar_stats(int nobj, double* yo, int nt, double* t, int indiv_t,
+                                             int npot, int* pot_type, double* pot_args, double dt,
+                                             double rtol, double atol, double* result, int* err,
+                                             int odeint_type, long long* steps, long long* rejected,
+                                             long long* nrhs, double* dt_used, int nthreads) {
+  return run<2>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result, err,
+                odeint_type, nullptr, steps, rejected, nrhs, dt_used, nthreads);
+}
+
+namespace {
+template <int D>
+int count_ops(int nobj, double* yo, int nt, double* t, int indiv_t, int npot, int* pot_type,
+              double* pot_args, double dt, double rtol, double atol, double* result, int* err,
+              int odeint_type, double* out) {
+  constexpr int N = 2 * D;
   Component P[kMaxPot];
-  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P, D == 2) != 0) return -1;
   g_ops = OpCount{0, 0, 0, 0, 0};
   long long steps = 0, rej = 0, rhs = 0;
   for (int ii = 0; ii < nobj; ii++) {
     IntStats st = {0, 0, 0, 0.};
     // the coordinate transforms act on plain doubles and are not counted:
     // they are per output row, not per orbit-step
-    integrate_one<CountedD>(npot, P, odeint_type, yo + 6 * ii, nt, t + (long)nt * ii * indiv_t,
-                            dt, rtol, atol, result + 6L * nt * ii, err + ii, &st);
+    integrate_one<D, CountedD>(npot, P, odeint_type, yo + N * ii, nt,
+                               t + (long)nt * ii * indiv_t, dt, rtol, atol,
+                               result + (long)N * nt * ii, err + ii, &st);
     steps += st.steps;
     rej += st.rejected;
     rhs += st.rhs;
@@ -124,6 +159,53 @@ ORC_EXPORT int oracle_count_ops(int nobj, double* yo, int nt, double* t, int ind
   out[6] = (double)rej;
   out[7] = (double)rhs;
   return 0;
+}
+}  // namespace
+
+// Op-counting run (single thread).  out[0..7] = add, mul, div, sqrt, special,
+// orbit-steps, rejected steps, force/RHS evaluations, summed over the orbits.
+ORC_EXPORT int oracle_count_ops(int nobj, double* yo, int nt, double* t, int indiv_t, int npot,
+                                int* pot_type, double* pot_args, double dt, double rtol,
+                                double atol, double* result, int* err, int odeint_type,
+                                double* out) {
+  return count_ops<3>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result,
+                      err, odeint_type, out);
+}
+
+// The same for planar orbits (rows of 4, planar packing).
+ORC_EXPORT int oracle_count_ops_planar(int nobj, double* yo, int nt, double* t, int indiv_t,
+                                       int npot, int* pot_type, double* pot_args, double dt,
+                                       double rtol, double atol, double* result, int* err,
+                                       int odeint_type, double* out) {
+  return count_ops<2>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result,
+                      err, odeint_type, out);
+}
+
+// Planar forces (calcPlanarRforce, calcPlanarphitorque) at n points, from
+// the planar packing.
+ORC_EXPORT int oracle_eval_planar_forces(int npot, int* pot_type, double* pot_args, int n,
+                                         const double* R, const double* phi, const double* t,
+                                         double* FR, double* Fphi) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P, true) != 0) return -1;
+  for (int i = 0; i < n; i++) planar_forces<double>(npot, P, R[i], phi[i], t[i], FR[i], Fphi[i]);
+  return 0;
+}
+
+// Planar rectangular accelerations (evalPlanarRectForce), q[n*2] -> a[n*2].
+ORC_EXPORT int oracle_planar_rect_force(int npot, int* pot_type, double* pot_args, int n,
+                                        const double* t, const double* q, double* a) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P, true) != 0) return -1;
+  for (int i = 0; i < n; i++) planar_rect_force<double>(npot, P, t[i], q + 2 * i, a + 2 * i);
+  return 0;
+}
+
+ORC_EXPORT void oracle_polar_to_rect(int n, double* v) {
+  for (int i = 0; i < n; i++) polar_to_rect<double>(v + 4 * i);
+}
+ORC_EXPORT void oracle_rect_to_polar(int n, double* v) {
+  for (int i = 0; i < n; i++) rect_to_polar<double>(v + 4 * i);
 }
 
 // Cylindrical forces (FR, Fz, Fphi) at n points.

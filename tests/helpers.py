@@ -31,16 +31,20 @@ def _p(a):
 
 
 def oracle_integrate(pot_type, pot_args, yo, t, method, dt=None, rtol=LN1EM12, atol=LN1EM12,
-                     nthreads=0):
+                     nthreads=0, planar=False):
     """Run the oracle's integrateFullOrbit (returns result, err, stats[N,4],
-    yo_after) where stats = (steps, rejected, rhs, dt)."""
+    yo_after) where stats = (steps, rejected, rhs, dt).  planar=True runs
+    its integratePlanarOrbit instead: rows of (R,vR,vT,phi) and the planar
+    packing."""
     lib = oracle()
     yo = numpy.array(numpy.atleast_2d(yo), dtype=numpy.float64, order="C")
     t = numpy.ascontiguousarray(t, dtype=numpy.float64)
     n = len(yo)
     nt = t.shape[-1]
     indiv = int(t.ndim > 1)
-    res = numpy.empty((n, nt, 6))
+    ncol = 4 if planar else 6
+    assert yo.shape[1] == ncol
+    res = numpy.empty((n, nt, ncol))
     err = numpy.zeros(n, dtype=numpy.int32)
     steps = numpy.zeros(n, dtype=numpy.int64)
     rej = numpy.zeros(n, dtype=numpy.int64)
@@ -48,7 +52,8 @@ def oracle_integrate(pot_type, pot_args, yo, t, method, dt=None, rtol=LN1EM12, a
     dtu = numpy.zeros(n)
     pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
-    lib.oracle_integrate_stats(
+    fn = lib.oracle_integrate_planar_stats if planar else lib.oracle_integrate_stats
+    fn(
         ctypes.c_int(n), _p(yo), ctypes.c_int(nt), _p(t), ctypes.c_int(indiv), ctypes.c_int(len(pt)),
         _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(rtol),
         ctypes.c_double(atol), _p(res), _p(err), ctypes.c_int(method), _p(steps), _p(rej), _p(rhs),
@@ -122,6 +127,34 @@ def ics(n, seed=0):
     return numpy.stack([R, vR, vT, z, vz, phi], axis=1)
 
 
+def planar_ics(n, seed=0):
+    """Planar rows (R,vR,vT,phi) of the same law as ics (the z, vz draws are
+    dropped)."""
+    return numpy.ascontiguousarray(ics(n, seed)[:, [0, 1, 2, 5]])
+
+
+def in_plane(planar_rows):
+    """Full rows (R,vR,vT,0,0,phi) of planar rows (R,vR,vT,phi)."""
+    p = numpy.asarray(planar_rows)
+    z = numpy.zeros(p.shape[:-1] + (1,))
+    return numpy.concatenate([p[..., :3], z, z, p[..., 3:]], axis=-1)
+
+
+def oracle_planar_forces(pot_type, pot_args, R, phi=None, t=None):
+    """The oracle's calcPlanarRforce / calcPlanarphitorque (planar packing)."""
+    lib = oracle()
+    R = numpy.ascontiguousarray(R, dtype=float)
+    n = R.size
+    phi = numpy.ascontiguousarray(numpy.zeros(n) if phi is None else numpy.broadcast_to(phi, R.shape), dtype=float)
+    t = numpy.ascontiguousarray(numpy.zeros(n) if t is None else numpy.broadcast_to(t, R.shape), dtype=float)
+    FR, Fphi = numpy.empty(n), numpy.empty(n)
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    rc = lib.oracle_eval_planar_forces(len(pt), _p(pt), _p(pa), n, _p(R), _p(phi), _p(t), _p(FR), _p(Fphi))
+    assert rc == 0
+    return FR, Fphi
+
+
 def bar_potential():
     """LogarithmicHaloPotential(q=0.9) + DehnenBarPotential(omegab=1.85, rb=0.8, Af=0.01)."""
     from galpy_amd.potential import DehnenBarPotential, LogarithmicHaloPotential
@@ -131,12 +164,13 @@ def bar_potential():
 
 def phase_diff(a, b):
     """Max over rows of |a-b| per component with unit floor, phi modulo 2 pi.
-    a, b: [..., 6] rows (R,vR,vT,z,vz,phi)."""
+    a, b: [..., 6] rows (R,vR,vT,z,vz,phi) or [..., 4] planar rows
+    (R,vR,vT,phi); phi is the last column of both."""
     d = numpy.abs(a - b)
-    dphi = numpy.abs((a[..., 5] - b[..., 5] + numpy.pi) % (2 * numpy.pi) - numpy.pi)
-    d[..., 5] = dphi
+    dphi = numpy.abs((a[..., -1] - b[..., -1] + numpy.pi) % (2 * numpy.pi) - numpy.pi)
+    d[..., -1] = dphi
     scale = numpy.maximum(numpy.abs(b), 1.0)
-    scale[..., 5] = numpy.pi
+    scale[..., -1] = numpy.pi
     return (d / scale).max()
 
 

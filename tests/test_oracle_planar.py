@@ -1,0 +1,185 @@
+"""Pins the oracle's planar path (integratePlanarOrbit: planar force kernels,
+evalPlanarRectForce/Deriv, polar transforms, dim-2 symplectic steppers and
+dim-4 DOP853) before it is trusted as the checker of the HIP planar path.
+CPU only.
+
+The planar path is tied to the full-orbit path, which test_oracle.py pins
+against mpmath and galpy's published numbers, by identities that hold in
+exact arithmetic:
+  * galpy's planar kernels (planarRforce, planarphitorque) are its full
+    kernels in the plane z = 0;
+  * an orbit started in the plane with vz = 0 stays there, so at a fixed step
+    the planar steppers retrace the full ones;
+  * the step estimators' error norms differ only by their dimension,
+    sqrt(err / 2 / dim) (bovy_symplecticode.c:641-646), and the log-space
+    scale s of both moves with log-tolerances one for one, so the planar
+    estimator at (rtol, atol) is the full one at (rtol, atol) - ln sqrt(3/2).
+A dropped term, a wrong sign or a wrong dimension breaks one of these."""
+
+import numpy
+import pytest
+
+from helpers import (
+    LN1EM12,
+    bar_potential,
+    in_plane,
+    oracle_forces,
+    oracle_integrate,
+    oracle_planar_forces,
+    phase_diff,
+    planar_ics,
+)
+
+
+def _pots():
+    from galpy_amd.potential import HernquistPotential, LogarithmicHaloPotential, MWPotential2014
+
+    return {
+        "MWPotential2014": MWPotential2014,
+        "LogHalo+bar": bar_potential(),
+        "Hernquist": [HernquistPotential(a=1.3)],
+        "LogHalo q=0.8": [LogarithmicHaloPotential(q=0.8)],
+        "triaxial LogHalo": [LogarithmicHaloPotential(q=0.9, b=0.8)],
+    }
+
+
+def _packed(pot, planar):
+    from galpy_amd.potential import parse_pot, parse_pot_planar
+
+    _, pt, pa, _ = (parse_pot_planar if planar else parse_pot)(pot)
+    return pt, pa
+
+
+def test_planar_packing_layout():
+    """integratePlanarOrbit.py:67-196: the codes of the full packing;
+    PowerSphericalPotentialwCutoff without its two cache slots."""
+    from galpy_amd.potential import MWPotential2014
+
+    pt, pa = _packed(MWPotential2014, True)
+    ft, fa = _packed(MWPotential2014, False)
+    assert list(pt) == [15, 5, 9] == list(ft)
+    assert len(pa) == 3 + 3 + 2 and len(fa) == 5 + 3 + 2
+    assert numpy.array_equal(pa[:3], fa[:3]) and numpy.array_equal(pa[3:], fa[5:])
+    bt, ba = _packed(bar_potential(), True)
+    assert list(bt) == [0, 1] and numpy.array_equal(ba, _packed(bar_potential(), False)[1])
+
+
+@pytest.mark.parametrize("name", list(_pots()))
+def test_planar_forces_are_full_forces_in_plane(name):
+    pot = _pots()[name]
+    rng = numpy.random.default_rng(0)
+    R = rng.uniform(0.05, 5.0, 2000)
+    phi = rng.uniform(0.0, 2.0 * numpy.pi, 2000)
+    t = rng.uniform(-5.0, 5.0, 2000)
+    FR, Fz, Fphi = oracle_forces(*_packed(pot, False), R, 0.0, phi, t)
+    PR, Pphi = oracle_planar_forces(*_packed(pot, True), R, phi, t)
+    assert numpy.all(Fz == 0.0)
+    assert numpy.abs(PR - FR).max() <= 4e-15 * numpy.abs(FR).max()
+    assert numpy.abs(Pphi - Fphi).max() <= 4e-15 * max(numpy.abs(Fphi).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["MWPotential2014", "LogHalo+bar"])
+@pytest.mark.parametrize("code", [3, 0])
+def test_planar_orbit_is_full_orbit_in_plane(name, code):
+    pot = _pots()[name]
+    yo = planar_ics(64, seed=1)
+    t = numpy.linspace(0.0, 10.0, 101)
+    rp, errp, _, yop = oracle_integrate(*_packed(pot, True), yo, t, code, dt=0.01, planar=True)
+    rf, errf, _, _ = oracle_integrate(*_packed(pot, False), in_plane(yo), t, code, dt=0.01)
+    assert numpy.all(errp == 0) and numpy.all(errf == 0)
+    assert numpy.all(rf[..., 3:5] == 0.0)  # the full orbit never leaves the plane
+    assert phase_diff(in_plane(rp), rf) < 1e-12
+    # yo is overwritten with the rectangular initial state (x, y, vx, vy)
+    R, vR, vT, phi = yo.T
+    c, s = numpy.cos(phi), numpy.sin(phi)
+    assert numpy.allclose(yop, numpy.stack([R * c, R * s, vR * c - vT * s, vR * s + vT * c], 1),
+                          rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("code", [3, 0])
+def test_planar_step_estimator_dimension(code):
+    """The planar estimator (dim 2) at galpy's default tolerances chooses the
+    step the full estimator (dim 3) chooses at tolerances lowered by
+    ln sqrt(3/2); at equal tolerances some orbits differ, so the test sees
+    the dimension."""
+    d = 0.5 * numpy.log(1.5)
+    yo = planar_ics(64, seed=1)
+    t = numpy.linspace(0.0, 10.0, 101)
+    for name in ("MWPotential2014", "LogHalo+bar"):
+        pot = _pots()[name]
+        _, _, sp, _ = oracle_integrate(*_packed(pot, True), yo, t, code, planar=True)
+        _, _, sf, _ = oracle_integrate(*_packed(pot, False), in_plane(yo), t, code,
+                                       rtol=LN1EM12 - d, atol=LN1EM12 - d)
+        _, _, s0, _ = oracle_integrate(*_packed(pot, False), in_plane(yo), t, code)
+        assert numpy.array_equal(sp[:, 3], sf[:, 3]), name
+        assert not numpy.array_equal(sp[:, 3], s0[:, 3]), name
+
+
+def test_planar_dop853_agrees_with_full_and_converges():
+    """DOP853 at dim 4 and dim 6 control the error differently, so the two
+    agree to the tolerance, not to rounding; tightening the tolerance
+    converges the planar solution."""
+    for name in ("MWPotential2014", "LogHalo+bar"):
+        pot = _pots()[name]
+        yo = planar_ics(32, seed=2)
+        t = numpy.linspace(0.0, 10.0, 101)
+        rp, errp, sp, _ = oracle_integrate(*_packed(pot, True), yo, t, 6, planar=True)
+        rf, errf, sf, _ = oracle_integrate(*_packed(pot, False), in_plane(yo), t, 6)
+        assert numpy.all(errp == 0) and numpy.all(errf == 0)
+        assert phase_diff(in_plane(rp), rf) < 1e-8
+        # the dim-4 norm divides by 4, not 6: larger error estimates, more steps
+        assert sp[:, 0].sum() > sf[:, 0].sum()
+    pt, pa = _packed(_pots()["MWPotential2014"], True)
+    yo = [1.0, 0.1, 1.05, 0.3]
+    t = numpy.linspace(0.0, 20.0, 41)
+    ref, *_ = oracle_integrate(pt, pa, yo, t, 6, rtol=numpy.log(1e-14), atol=numpy.log(1e-14),
+                               planar=True)
+    e = []
+    for tol in (1e-8, 1e-10):
+        r, *_ = oracle_integrate(pt, pa, yo, t, 6, rtol=numpy.log(tol), atol=numpy.log(tol),
+                                 planar=True)
+        e.append(numpy.abs(r - ref)[..., :3].max())
+    assert e[0] < 2e-5 and e[1] < e[0] / 20
+
+
+def test_planar_circular_orbit_closed_form():
+    """phi(t) = vc t / R for a circular orbit in a spherical potential:
+    DOP853 to its tolerance, the steppers at their order (4, 2)."""
+    pt, pa = [8], [1.0, 1.3]  # Hernquist
+    R0 = 1.1
+    FR, _ = oracle_planar_forces(pt, pa, numpy.array([R0]))
+    vc = numpy.sqrt(-R0 * FR[0])
+    t = numpy.linspace(0.0, 40.0, 81)
+
+    def run(code, dt=None):
+        res, *_ = oracle_integrate(pt, pa, [R0, 0.0, vc, 0.0], t, code, dt=dt, planar=True)
+        dphi = numpy.abs((res[0, :, 3] - vc / R0 * t + numpy.pi) % (2 * numpy.pi) - numpy.pi)
+        return res[0], dphi
+
+    res, dphi = run(6)
+    assert dphi.max() < 1e-9 and numpy.abs(res[:, 0] - R0).max() < 1e-10
+    for code, order in ((3, 4), (0, 2)):
+        errs = [run(code, dt=h)[1].max() for h in (0.1, 0.05)]
+        assert abs(numpy.log2(errs[0] / errs[1]) - order) < 0.2, (code, errs)
+
+
+def test_planar_time_reversibility_and_jacobi():
+    from galpy_amd.potential import evaluatePotentials
+
+    pot = _pots()["MWPotential2014"]
+    pt, pa = _packed(pot, True)
+    yo = numpy.array([1.2, 0.1, 0.9, 0.4])
+    t = numpy.linspace(0.0, 5.0, 51)
+    flip = numpy.array([1, -1, -1, 1])
+    for code in (3, 0):
+        fwd, *_ = oracle_integrate(pt, pa, yo, t, code, dt=0.01, planar=True)
+        back, *_ = oracle_integrate(pt, pa, fwd[0, -1] * flip, t, code, dt=0.01, planar=True)
+        assert numpy.abs(back[0, -1] * flip - yo).max() < 1e-11, code
+    bar = bar_potential()
+    res, *_ = oracle_integrate(*_packed(bar, True), planar_ics(4, seed=3), numpy.linspace(0, 50, 501),
+                               6, planar=True)
+    for o in res:
+        R, vR, vT, phi = o.T
+        tt = numpy.linspace(0, 50, 501)
+        EJ = evaluatePotentials(bar, R, 0.0, phi, tt) + 0.5 * (vR**2 + vT**2) - 1.85 * R * vT
+        assert numpy.abs(EJ / EJ[0] - 1).max() < 1e-8
