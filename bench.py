@@ -10,7 +10,9 @@ over the K timed steps.  |dE/E| is measured on device afterwards.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 NAME: mw2014_dop853_1m (default), loghalo_bar_dop853_256k, mw2014_symplec4_1m,
-mw2014_symplec4_1k, batching, serving, adjoint, copy (counter calibration).
+mw2014_symplec4_1k, mw2014_planar_dop853_1m, mw2014_planar_symplec4_1m (planar
+orbits, integratePlanarOrbit), batching, serving, adjoint, copy (counter
+calibration).
 
 Prints one JSON line (see the keys below).  The CPU baseline leg times the
 CPU oracle (a C++/OpenMP restatement of galpy's C integrators) on a bounded
@@ -68,17 +70,20 @@ def cpu_baseline_leg(w, sample, count_sample):
     from helpers import oracle_count_ops, oracle_integrate  # the oracle is test infrastructure
 
     from galpy_amd.integrate import METHOD_CODES
-    from galpy_amd.potential import parse_pot
+    from galpy_amd.potential import parse_pot, parse_pot_planar
 
-    _, pt, pa, _ = parse_pot(w["pot"])
+    planar = w["planar"]
+    _, pt, pa, _ = (parse_pot_planar if planar else parse_pot)(w["pot"])
     code = METHOD_CODES[w["method"]]
     nthreads = len(os.sched_getaffinity(0))
     yo = w["yo"][:sample]
     t0 = time.perf_counter()
-    _, err, stats, _ = oracle_integrate(pt, pa, yo, w["t"], code, dt=w.get("dt"), nthreads=nthreads)
+    _, err, stats, _ = oracle_integrate(pt, pa, yo, w["t"], code, dt=w.get("dt"), nthreads=nthreads,
+                                        planar=planar)
     el = time.perf_counter() - t0
     steps = stats[:, 0].sum()
-    ops = oracle_count_ops(pt, pa, w["yo"][:count_sample], w["t"], code, dt=w.get("dt"))
+    ops = oracle_count_ops(pt, pa, w["yo"][:count_sample], w["t"], code, dt=w.get("dt"),
+                           planar=planar)
     flops = ops["add"] + ops["mul"] + ops["div"] + ops["sqrt"] + ops["special"]
     return {
         "value": steps / el,
@@ -327,11 +332,12 @@ def main():
     w = workloads.make(args.workload, n=args.norbits)
     n, t_np = w["n"], w["t"]
     nt = len(t_np)
-    table = PotentialTable(w["pot"])
+    ncol = 4 if w["planar"] else 6  # doubles per phase-space row
+    table = PotentialTable(w["pot"], planar=w["planar"])
     yo0 = torch.tensor(w["yo"], dtype=torch.float64, device=dev)
     yo = torch.empty_like(yo0)
     tt = torch.tensor(t_np, dtype=torch.float64, device=dev)
-    result = torch.empty((n, nt, 6), dtype=torch.float64, device=dev)
+    result = torch.empty((n, nt, ncol), dtype=torch.float64, device=dev)
     err = torch.zeros(n, dtype=torch.int32, device=dev)
     stats = torch.zeros((n, 3), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
@@ -382,13 +388,14 @@ def main():
         "config": {
             "workload": w["name"],
             "orbits": n,
+            "orbit_dim": 2 if w["planar"] else 3,
             "method": w["method"],
             "potential": "MWPotential2014" if w["omega"] == 0.0 else "LogarithmicHalo(q=0.9)+DehnenBar",
             "t": [float(t_np[0]), float(t_np[-1]), nt],
             "rtol": 1e-12,
             "atol": 1e-12,
             "dt": w.get("dt"),
-            "output_bytes": n * nt * 48,
+            "output_bytes": n * nt * 8 * ncol,
         },
         "orbit_steps_per_pass": steps_per_pass,
         "rejected_steps_per_pass": rejected,
@@ -415,7 +422,7 @@ def main():
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": MEASURED_TRAFFIC.get(w["name"]) if args.norbits is None else None,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "algorithmic_bytes": float(n * nt * 48 + 2 * n * 48),
+            "algorithmic_bytes": float(n * nt * 8 * ncol + 2 * n * 8 * ncol),
             "kernel": "gh_dop853_kernel" if w["method"] == "dop853_hip" else "gh_symplectic_kernel",
             "note": ("FP64 vector ALU bound (no MFMA: nothing in the path is a contraction; "
                      "78.6 TFLOP/s is MI355X's FP64 peak). achieved = counted algorithmic FP64 ops "
@@ -424,7 +431,7 @@ def main():
             "flop_per_orbit_step": opc["flop_per_orbit_step"],
             "special_per_orbit_step": opc["special_per_orbit_step"],
             "rhs_per_orbit_step": opc["rhs_per_orbit_step"],
-            "output_gbs": n * nt * 48 / kernel_s / 1e9,
+            "output_gbs": n * nt * 8 * ncol / kernel_s / 1e9,
         }
     print(json.dumps(out), flush=True)
 

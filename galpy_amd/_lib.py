@@ -80,6 +80,7 @@ SIGNATURES = {
 # the planar entry points take the same arguments as the full ones
 SIGNATURES["integratePlanarOrbit"] = SIGNATURES["integrateFullOrbit"]
 SIGNATURES["integratePlanarOrbit_hip"] = SIGNATURES["integrateFullOrbit_hip"]
+SIGNATURES["gh_planar_energy_drift_hip"] = SIGNATURES["gh_energy_drift_hip"]
 
 
 def load(path=None):

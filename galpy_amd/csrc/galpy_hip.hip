@@ -719,18 +719,21 @@ __global__ void gh_iota(int n, int* __restrict__ v) {
 __global__ void gh_set_counter(unsigned long long* c, unsigned long long v) { *c = v; }
 
 // Energy / Jacobi drift along stored cylindrical rows.
+template <int N>
 __global__ void __launch_bounds__(kBlock)
 gh_energy_kernel(int nobj, int nt, const double* __restrict__ rows, const double* __restrict__ t,
                  int indiv_t, const GhPotTable* __restrict__ T, double omega,
                  double* __restrict__ e0, double* __restrict__ demax) {
+  // N = 6: rows (R,vR,vT,z,vz,phi); N = 4: planar rows (R,vR,vT,phi), z = vz = 0
   const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (orbit >= nobj) return;
   const double* tt = t + (indiv_t ? orbit * nt : 0);
-  const double* r = rows + orbit * (long)nt * 6;
+  const double* r = rows + orbit * (long)nt * N;
   double E0 = 0.0, dmax = 0.0;
   for (int j = 0; j < nt; j++) {
-    const double R = r[6L * j + 0], vR = r[6L * j + 1], vT = r[6L * j + 2];
-    const double z = r[6L * j + 3], vz = r[6L * j + 4], phi = r[6L * j + 5];
+    const double R = r[(long)N * j + 0], vR = r[(long)N * j + 1], vT = r[(long)N * j + 2];
+    const double z = N == 6 ? r[(long)N * j + 3] : 0.0, vz = N == 6 ? r[(long)N * j + 4] : 0.0;
+    const double phi = r[(long)N * j + N - 1];
     double sp, cp;
     sincos(phi, &sp, &cp);
     const double E = gh_potential(T, R, z, cp, sp, tt[j]) + 0.5 * (vR * vR + vT * vT + vz * vz) -
@@ -1192,8 +1195,20 @@ GH_EXPORT int gh_energy_drift_hip(int nobj, int nt, const double* d_rows, const 
   g_errbuf[0] = 0;
   if (nobj <= 0) return 0;
   const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
-  hipLaunchKernelGGL(gh_energy_kernel, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows, d_t,
-                     indiv_t, ((const GhPotHandle*)pot)->dev, omega, d_e0, d_demax);
+  hipLaunchKernelGGL(gh_energy_kernel<6>, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows,
+                     d_t, indiv_t, ((const GhPotHandle*)pot)->dev, omega, d_e0, d_demax);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+GH_EXPORT int gh_planar_energy_drift_hip(int nobj, int nt, const double* d_rows, const double* d_t,
+                                         int indiv_t, const void* pot, double omega, double* d_e0,
+                                         double* d_demax, void* stream) {
+  g_errbuf[0] = 0;
+  if (nobj <= 0) return 0;
+  const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
+  hipLaunchKernelGGL(gh_energy_kernel<4>, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows,
+                     d_t, indiv_t, ((const GhPotHandle*)pot)->dev, omega, d_e0, d_demax);
   GH_CHECK(hipGetLastError());
   return 0;
 }

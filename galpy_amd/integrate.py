@@ -8,7 +8,7 @@ packing through ``parse_pot``, tolerances as natural logs
 an orbit reports -10.  The call goes through ctypes to the C-ABI symbol
 ``integrateFullOrbit`` of libgalpy_hip.so.  ``integratePlanarOrbit_hip`` is
 the same for planar orbits (galpy's ``integratePlanarOrbit_c``,
-galpy/orbit/integratePlanarOrbit.py:989-1100): rows of (R,vR,vT,phi), the
+galpy/orbit/integratePlanarOrbit.py:989-1122): rows of (R,vR,vT,phi), the
 planar packing ``parse_pot_planar`` and the symbol ``integratePlanarOrbit``.
 
 ``integrate_device`` is the HBM-resident path: inputs and outputs are torch
@@ -59,7 +59,7 @@ def integrateFullOrbit_hip(pot, yo, t, int_method, rtol=None, atol=None, progres
 def integratePlanarOrbit_hip(pot, yo, t, int_method, rtol=None, atol=None, progressbar=False,
                              dt=None):
     """Integrate N planar orbits on the GPU (galpy's ``integratePlanarOrbit_c``,
-    integratePlanarOrbit.py:989-1100): ``yo`` is ``[N,4]`` or ``[4]`` in
+    integratePlanarOrbit.py:989-1122): ``yo`` is ``[N,4]`` or ``[4]`` in
     (R,vR,vT,phi); returns ``(result, err)`` with ``result`` of shape
     ``(N,nt,4)`` (or ``(nt,4)``)."""
     return _integrate_host("integratePlanarOrbit", parse_pot_planar, 4, pot, yo, t, int_method,
@@ -195,19 +195,22 @@ def integrate_device(table, yo, t, int_method, rtol=None, atol=None, dt=None, re
 
 
 def energy_drift_device(table, rows, t, omega=0.0, stream=None):
-    """Per-orbit E(0) and max |E(t)-E(0)|/|E(0)| (Jacobi with ``omega``) on device."""
+    """Per-orbit E(0) and max |E(t)-E(0)|/|E(0)| (Jacobi with ``omega``) on device,
+    along full ``[N,nt,6]`` or planar ``[N,nt,4]`` rows."""
     import torch
 
-    nobj, nt, _ = rows.shape
+    nobj, nt, ncol = rows.shape
+    assert ncol in (4, 6) and rows.is_contiguous()
     e0 = torch.empty(nobj, dtype=torch.float64, device=rows.device)
     demax = torch.empty(nobj, dtype=torch.float64, device=rows.device)
     if stream is None:
         stream = torch.cuda.current_stream(rows.device)
-    rc = _lib.load().gh_energy_drift_hip(
+    fn = "gh_planar_energy_drift_hip" if ncol == 4 else "gh_energy_drift_hip"
+    rc = getattr(_lib.load(), fn)(
         nobj, nt, rows.data_ptr(), t.data_ptr(), int(t.dim() > 1), table.ptr, float(omega),
         e0.data_ptr(), demax.data_ptr(), stream.cuda_stream,
     )
-    _lib.check(rc, "gh_energy_drift_hip")
+    _lib.check(rc, fn)
     return e0, demax
 
 

@@ -6,7 +6,7 @@ rows, or planar ``[R,vR,vT,phi]`` rows (one or many orbits, :293),
 ``integrate(t, pot, method, dt, rtol, atol)`` (:1987-2061) dispatching to the
 C-ABI through ``integrateFullOrbit_hip`` / ``integratePlanarOrbit_hip`` (the
 ``integrateFullOrbit_c`` / ``integratePlanarOrbit_c`` calls at :1806 and
-:1867), and the phase-space and
+:1795), and the phase-space and
 integral accessors used by galpy's tests: ``R, vR, vT, z, vz, phi, x, y,
 vx, vy`` (:2868), ``getOrbit``, ``E`` (:2902), ``Jacobi`` (:3298), ``L``,
 ``Lz``, ``ER``, ``Ez``, ``rperi``, ``rap``, ``zmax``, ``e``.

@@ -2,7 +2,8 @@
 
 Initial conditions follow one law for every configuration: R~U(0.5,2),
 z~N(0,0.1), phi~U(0,2 pi), vR,vz~N(0,0.1), vT~N(1,0.1), rows
-(R,vR,vT,z,vz,phi) in galpy's natural units.
+(R,vR,vT,z,vz,phi) in galpy's natural units; planar workloads keep
+(R,vR,vT,phi) of the same draws.
 """
 
 import numpy
@@ -46,6 +47,13 @@ WORKLOADS = {
     # 262,144 orbits in a rotating bar (Jacobi integral instead of energy)
     "loghalo_bar_dop853_256k": dict(pot=loghalo_bar, n=262144, t=(0.0, 500.0, 1001),
                                    method="dop853_hip", omega=1.85, seed=1),
+    # planar orbits (integratePlanarOrbit): the DOP853 and fixed-step
+    # configurations above in the plane z = 0
+    "mw2014_planar_dop853_1m": dict(pot=mw2014, n=1048576, t=(0.0, 500.0, 1001),
+                                   method="dop853_hip", omega=0.0, seed=0, planar=True),
+    "mw2014_planar_symplec4_1m": dict(pot=mw2014, n=1048576, t=(0.0, 100.0, 101),
+                                     method="symplec4_hip", omega=0.0, seed=0, dt=0.01,
+                                     planar=True),
 }
 
 
@@ -56,6 +64,9 @@ def make(name, n=None):
     t0, t1, nt = w["t"]
     w["t"] = numpy.linspace(t0, t1, nt)
     w["yo"] = ics(w["n"], seed=w["seed"])
+    w["planar"] = bool(w.get("planar", False))
+    if w["planar"]:
+        w["yo"] = numpy.ascontiguousarray(w["yo"][:, [0, 1, 2, 5]])
     w["pot"] = w["pot"]()
     w["name"] = name
     return w

@@ -4,14 +4,17 @@
  * Drop-in boundary for galpy's C orbit integration: the host packs the
  * potential exactly as galpy's `_parse_pot` does (integer pot_type codes,
  * float64 pot_args), and calls one function for all orbits, as galpy's
- * Python layer calls `integrateFullOrbit` through ctypes
- * (galpy/orbit/integrateFullOrbit.py:687-734).  Plain pointers and sizes
- * only; no HIP or torch types appear in these signatures.
+ * Python layer calls `integrateFullOrbit` (galpy/orbit/integrateFullOrbit.py:
+ * 687-734) or, for planar orbits, `integratePlanarOrbit`
+ * (galpy/orbit/integratePlanarOrbit.py:1059-1110) through ctypes.  Plain
+ * pointers and sizes only; no HIP or torch types appear in these signatures.
  *
- * Supported potentials (pot_type codes, integrateFullOrbit.c:80-345):
+ * Supported potentials (pot_type codes, integrateFullOrbit.c:80-345,
+ * integratePlanarOrbit.c:64-223):
  *   0 LogarithmicHaloPotential, 1 DehnenBarPotential, 5 MiyamotoNagaiPotential,
  *   8 HernquistPotential, 9 NFWPotential, 15 PowerSphericalPotentialwCutoff.
- * Supported odeint_type codes (integrateFullOrbit.c:1091-1132):
+ * Supported odeint_type codes (integrateFullOrbit.c:1091-1132,
+ * integratePlanarOrbit.c:788-825):
  *   0 leapfrog, 3 symplec4, 6 dop853.
  * Any other code, or nt < 2, sets err[i] = -1 for every orbit and records a
  * message readable with galpy_hip_last_error().
@@ -101,6 +104,10 @@ const void *gh_potential_device_ptr(const void *pot);
 int gh_energy_drift_hip(int nobj, int nt, const double *d_rows, const double *d_t, int indiv_t,
                         const void *pot, double omega, double *d_e0, double *d_demax,
                         void *stream);
+/* The same along planar rows d_rows[nobj*nt*4] (R,vR,vT,phi), z = vz = 0. */
+int gh_planar_energy_drift_hip(int nobj, int nt, const double *d_rows, const double *d_t,
+                               int indiv_t, const void *pot, double omega, double *d_e0,
+                               double *d_demax, void *stream);
 
 /* Potential value and cylindrical forces at n points (device pointers). */
 int gh_eval_hip(int n, const double *d_R, const double *d_z, const double *d_phi,

@@ -93,17 +93,18 @@ def oracle_potential(pot_type, pot_args, R, z, phi=None, t=None):
     return out
 
 
-def oracle_count_ops(pot_type, pot_args, yo, t, method, dt=None):
+def oracle_count_ops(pot_type, pot_args, yo, t, method, dt=None, planar=False):
     lib = oracle()
     yo = numpy.array(numpy.atleast_2d(yo), dtype=numpy.float64, order="C")
     t = numpy.ascontiguousarray(t, dtype=numpy.float64)
     n, nt = len(yo), t.shape[-1]
-    res = numpy.empty((n, nt, 6))
+    res = numpy.empty((n, nt, 4 if planar else 6))
     err = numpy.zeros(n, dtype=numpy.int32)
     out = numpy.zeros(8)
     pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
-    rc = lib.oracle_count_ops(
+    fn = lib.oracle_count_ops_planar if planar else lib.oracle_count_ops
+    rc = fn(
         ctypes.c_int(n), _p(yo), ctypes.c_int(nt), _p(t), ctypes.c_int(int(t.ndim > 1)), ctypes.c_int(len(pt)),
         _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(LN1EM12),
         ctypes.c_double(LN1EM12), _p(res), _p(err), ctypes.c_int(method), _p(out),
