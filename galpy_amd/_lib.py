@@ -77,6 +77,43 @@ SIGNATURES = {
 }
 
 
+_DXDV_ONE = [
+    ndpointer(dtype=numpy.float64, flags=_F),
+    ctypes.c_int,
+    ndpointer(dtype=numpy.float64, flags=_F),
+    ctypes.c_int,
+    ndpointer(dtype=numpy.int32, flags=_F),
+    ndpointer(dtype=numpy.float64, flags=_F),
+    ctypes.c_void_p,
+    ctypes.c_double,
+    ctypes.c_double,
+    ctypes.c_double,
+    ndpointer(dtype=numpy.float64, flags=_F),
+    ndpointer(dtype=numpy.int32, flags=_F),
+    ctypes.c_int,
+]
+SIGNATURES["integrateFullOrbit_dxdv"] = (None, _DXDV_ONE)
+SIGNATURES["integratePlanarOrbit_dxdv"] = (None, _DXDV_ONE + [ctypes.c_void_p])
+SIGNATURES["gh_integrate_dxdv"] = (
+    None,
+    [
+        ctypes.c_int, ctypes.c_int, ndpointer(dtype=numpy.float64, flags=_F), ctypes.c_int,
+        ndpointer(dtype=numpy.float64, flags=_F), ctypes.c_int, ctypes.c_int,
+        ndpointer(dtype=numpy.int32, flags=_F), ndpointer(dtype=numpy.float64, flags=_F),
+        ctypes.c_double, ctypes.c_double, ctypes.c_double,
+        ndpointer(dtype=numpy.float64, flags=_F), ndpointer(dtype=numpy.int32, flags=_F),
+        ctypes.c_int,
+    ],
+)
+SIGNATURES["gh_integrate_dxdv_hip"] = (
+    ctypes.c_int,
+    [
+        ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+        ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+    ],
+)
+
 # the planar entry points take the same arguments as the full ones
 SIGNATURES["integratePlanarOrbit"] = SIGNATURES["integrateFullOrbit"]
 SIGNATURES["integratePlanarOrbit_hip"] = SIGNATURES["integrateFullOrbit_hip"]

@@ -214,10 +214,10 @@ __device__ __forceinline__ int gh_dop_dense(const F& T, const double* __restrict
   while (done < nt - 1 && pos_neg * t[done + 1] < pos_neg * t_cur) {
     const double sd = (t[done + 1] - J.t_old) / h;
     const double s1 = 1.0 - sd;
-    double yo[N];
+    double yo[N];  // positions, then velocities, in component order
     gh_rkn_dense_eval(h, sd, s1, J.q0, J.p0, J.qn, J.pn, A, yo);
     done++;
-    if (STORE) GhPhase<V>::store(out + (long)N * done, GhPhase<V>::q(yo), GhPhase<V>::p(yo));
+    if (STORE) GhPhase<V>::store(out + (long)N * done, gh_vec_at<V>(yo), gh_vec_at<V>(yo + N / 2));
   }
   return done - J.done0;
 }

@@ -730,3 +730,807 @@ __device__ __forceinline__ void gh_rkn_dense_eval(double h, double sd, double s1
     yo[3] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
   }
 }
+// position argument of stage 2
+__device__ __forceinline__ GhVec6 gh_rkn_q2(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp2 * p0.x);
+  Q.y = q0.y + h * (ghrk::qp2 * p0.y);
+  Q.z = q0.z + h * (ghrk::qp2 * p0.z);
+  Q.dx = q0.dx + h * (ghrk::qp2 * p0.dx);
+  Q.dy = q0.dy + h * (ghrk::qp2 * p0.dy);
+  Q.dz = q0.dz + h * (ghrk::qp2 * p0.dz);
+  return Q;
+}
+
+// position argument of stage 3
+__device__ __forceinline__ GhVec6 gh_rkn_q3(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp3 * p0.x + h * (ghrk::qa3_1 * A[1].x));
+  Q.y = q0.y + h * (ghrk::qp3 * p0.y + h * (ghrk::qa3_1 * A[1].y));
+  Q.z = q0.z + h * (ghrk::qp3 * p0.z + h * (ghrk::qa3_1 * A[1].z));
+  Q.dx = q0.dx + h * (ghrk::qp3 * p0.dx + h * (ghrk::qa3_1 * A[1].dx));
+  Q.dy = q0.dy + h * (ghrk::qp3 * p0.dy + h * (ghrk::qa3_1 * A[1].dy));
+  Q.dz = q0.dz + h * (ghrk::qp3 * p0.dz + h * (ghrk::qa3_1 * A[1].dz));
+  return Q;
+}
+
+// position argument of stage 4
+__device__ __forceinline__ GhVec6 gh_rkn_q4(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp4 * p0.x + h * (ghrk::qa4_1 * A[1].x + ghrk::qa4_2 * A[2].x));
+  Q.y = q0.y + h * (ghrk::qp4 * p0.y + h * (ghrk::qa4_1 * A[1].y + ghrk::qa4_2 * A[2].y));
+  Q.z = q0.z + h * (ghrk::qp4 * p0.z + h * (ghrk::qa4_1 * A[1].z + ghrk::qa4_2 * A[2].z));
+  Q.dx = q0.dx + h * (ghrk::qp4 * p0.dx + h * (ghrk::qa4_1 * A[1].dx + ghrk::qa4_2 * A[2].dx));
+  Q.dy = q0.dy + h * (ghrk::qp4 * p0.dy + h * (ghrk::qa4_1 * A[1].dy + ghrk::qa4_2 * A[2].dy));
+  Q.dz = q0.dz + h * (ghrk::qp4 * p0.dz + h * (ghrk::qa4_1 * A[1].dz + ghrk::qa4_2 * A[2].dz));
+  return Q;
+}
+
+// position argument of stage 5
+__device__ __forceinline__ GhVec6 gh_rkn_q5(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp5 * p0.x + h * (ghrk::qa5_1 * A[1].x + ghrk::qa5_2 * A[2].x + ghrk::qa5_3 * A[3].x));
+  Q.y = q0.y + h * (ghrk::qp5 * p0.y + h * (ghrk::qa5_1 * A[1].y + ghrk::qa5_2 * A[2].y + ghrk::qa5_3 * A[3].y));
+  Q.z = q0.z + h * (ghrk::qp5 * p0.z + h * (ghrk::qa5_1 * A[1].z + ghrk::qa5_2 * A[2].z + ghrk::qa5_3 * A[3].z));
+  Q.dx = q0.dx + h * (ghrk::qp5 * p0.dx + h * (ghrk::qa5_1 * A[1].dx + ghrk::qa5_2 * A[2].dx + ghrk::qa5_3 * A[3].dx));
+  Q.dy = q0.dy + h * (ghrk::qp5 * p0.dy + h * (ghrk::qa5_1 * A[1].dy + ghrk::qa5_2 * A[2].dy + ghrk::qa5_3 * A[3].dy));
+  Q.dz = q0.dz + h * (ghrk::qp5 * p0.dz + h * (ghrk::qa5_1 * A[1].dz + ghrk::qa5_2 * A[2].dz + ghrk::qa5_3 * A[3].dz));
+  return Q;
+}
+
+// position argument of stage 6
+__device__ __forceinline__ GhVec6 gh_rkn_q6(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp6 * p0.x + h * (ghrk::qa6_1 * A[1].x + ghrk::qa6_3 * A[3].x + ghrk::qa6_4 * A[4].x));
+  Q.y = q0.y + h * (ghrk::qp6 * p0.y + h * (ghrk::qa6_1 * A[1].y + ghrk::qa6_3 * A[3].y + ghrk::qa6_4 * A[4].y));
+  Q.z = q0.z + h * (ghrk::qp6 * p0.z + h * (ghrk::qa6_1 * A[1].z + ghrk::qa6_3 * A[3].z + ghrk::qa6_4 * A[4].z));
+  Q.dx = q0.dx + h * (ghrk::qp6 * p0.dx + h * (ghrk::qa6_1 * A[1].dx + ghrk::qa6_3 * A[3].dx + ghrk::qa6_4 * A[4].dx));
+  Q.dy = q0.dy + h * (ghrk::qp6 * p0.dy + h * (ghrk::qa6_1 * A[1].dy + ghrk::qa6_3 * A[3].dy + ghrk::qa6_4 * A[4].dy));
+  Q.dz = q0.dz + h * (ghrk::qp6 * p0.dz + h * (ghrk::qa6_1 * A[1].dz + ghrk::qa6_3 * A[3].dz + ghrk::qa6_4 * A[4].dz));
+  return Q;
+}
+
+// position argument of stage 7
+__device__ __forceinline__ GhVec6 gh_rkn_q7(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp7 * p0.x + h * (ghrk::qa7_1 * A[1].x + ghrk::qa7_3 * A[3].x + ghrk::qa7_4 * A[4].x + ghrk::qa7_5 * A[5].x));
+  Q.y = q0.y + h * (ghrk::qp7 * p0.y + h * (ghrk::qa7_1 * A[1].y + ghrk::qa7_3 * A[3].y + ghrk::qa7_4 * A[4].y + ghrk::qa7_5 * A[5].y));
+  Q.z = q0.z + h * (ghrk::qp7 * p0.z + h * (ghrk::qa7_1 * A[1].z + ghrk::qa7_3 * A[3].z + ghrk::qa7_4 * A[4].z + ghrk::qa7_5 * A[5].z));
+  Q.dx = q0.dx + h * (ghrk::qp7 * p0.dx + h * (ghrk::qa7_1 * A[1].dx + ghrk::qa7_3 * A[3].dx + ghrk::qa7_4 * A[4].dx + ghrk::qa7_5 * A[5].dx));
+  Q.dy = q0.dy + h * (ghrk::qp7 * p0.dy + h * (ghrk::qa7_1 * A[1].dy + ghrk::qa7_3 * A[3].dy + ghrk::qa7_4 * A[4].dy + ghrk::qa7_5 * A[5].dy));
+  Q.dz = q0.dz + h * (ghrk::qp7 * p0.dz + h * (ghrk::qa7_1 * A[1].dz + ghrk::qa7_3 * A[3].dz + ghrk::qa7_4 * A[4].dz + ghrk::qa7_5 * A[5].dz));
+  return Q;
+}
+
+// position argument of stage 8
+__device__ __forceinline__ GhVec6 gh_rkn_q8(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp8 * p0.x + h * (ghrk::qa8_1 * A[1].x + ghrk::qa8_3 * A[3].x + ghrk::qa8_4 * A[4].x + ghrk::qa8_5 * A[5].x + ghrk::qa8_6 * A[6].x));
+  Q.y = q0.y + h * (ghrk::qp8 * p0.y + h * (ghrk::qa8_1 * A[1].y + ghrk::qa8_3 * A[3].y + ghrk::qa8_4 * A[4].y + ghrk::qa8_5 * A[5].y + ghrk::qa8_6 * A[6].y));
+  Q.z = q0.z + h * (ghrk::qp8 * p0.z + h * (ghrk::qa8_1 * A[1].z + ghrk::qa8_3 * A[3].z + ghrk::qa8_4 * A[4].z + ghrk::qa8_5 * A[5].z + ghrk::qa8_6 * A[6].z));
+  Q.dx = q0.dx + h * (ghrk::qp8 * p0.dx + h * (ghrk::qa8_1 * A[1].dx + ghrk::qa8_3 * A[3].dx + ghrk::qa8_4 * A[4].dx + ghrk::qa8_5 * A[5].dx + ghrk::qa8_6 * A[6].dx));
+  Q.dy = q0.dy + h * (ghrk::qp8 * p0.dy + h * (ghrk::qa8_1 * A[1].dy + ghrk::qa8_3 * A[3].dy + ghrk::qa8_4 * A[4].dy + ghrk::qa8_5 * A[5].dy + ghrk::qa8_6 * A[6].dy));
+  Q.dz = q0.dz + h * (ghrk::qp8 * p0.dz + h * (ghrk::qa8_1 * A[1].dz + ghrk::qa8_3 * A[3].dz + ghrk::qa8_4 * A[4].dz + ghrk::qa8_5 * A[5].dz + ghrk::qa8_6 * A[6].dz));
+  return Q;
+}
+
+// position argument of stage 9
+__device__ __forceinline__ GhVec6 gh_rkn_q9(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp9 * p0.x + h * (ghrk::qa9_1 * A[1].x + ghrk::qa9_3 * A[3].x + ghrk::qa9_4 * A[4].x + ghrk::qa9_5 * A[5].x + ghrk::qa9_6 * A[6].x + ghrk::qa9_7 * A[7].x));
+  Q.y = q0.y + h * (ghrk::qp9 * p0.y + h * (ghrk::qa9_1 * A[1].y + ghrk::qa9_3 * A[3].y + ghrk::qa9_4 * A[4].y + ghrk::qa9_5 * A[5].y + ghrk::qa9_6 * A[6].y + ghrk::qa9_7 * A[7].y));
+  Q.z = q0.z + h * (ghrk::qp9 * p0.z + h * (ghrk::qa9_1 * A[1].z + ghrk::qa9_3 * A[3].z + ghrk::qa9_4 * A[4].z + ghrk::qa9_5 * A[5].z + ghrk::qa9_6 * A[6].z + ghrk::qa9_7 * A[7].z));
+  Q.dx = q0.dx + h * (ghrk::qp9 * p0.dx + h * (ghrk::qa9_1 * A[1].dx + ghrk::qa9_3 * A[3].dx + ghrk::qa9_4 * A[4].dx + ghrk::qa9_5 * A[5].dx + ghrk::qa9_6 * A[6].dx + ghrk::qa9_7 * A[7].dx));
+  Q.dy = q0.dy + h * (ghrk::qp9 * p0.dy + h * (ghrk::qa9_1 * A[1].dy + ghrk::qa9_3 * A[3].dy + ghrk::qa9_4 * A[4].dy + ghrk::qa9_5 * A[5].dy + ghrk::qa9_6 * A[6].dy + ghrk::qa9_7 * A[7].dy));
+  Q.dz = q0.dz + h * (ghrk::qp9 * p0.dz + h * (ghrk::qa9_1 * A[1].dz + ghrk::qa9_3 * A[3].dz + ghrk::qa9_4 * A[4].dz + ghrk::qa9_5 * A[5].dz + ghrk::qa9_6 * A[6].dz + ghrk::qa9_7 * A[7].dz));
+  return Q;
+}
+
+// position argument of stage 10
+__device__ __forceinline__ GhVec6 gh_rkn_q10(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp10 * p0.x + h * (ghrk::qa10_1 * A[1].x + ghrk::qa10_3 * A[3].x + ghrk::qa10_4 * A[4].x + ghrk::qa10_5 * A[5].x + ghrk::qa10_6 * A[6].x + ghrk::qa10_7 * A[7].x + ghrk::qa10_8 * A[8].x));
+  Q.y = q0.y + h * (ghrk::qp10 * p0.y + h * (ghrk::qa10_1 * A[1].y + ghrk::qa10_3 * A[3].y + ghrk::qa10_4 * A[4].y + ghrk::qa10_5 * A[5].y + ghrk::qa10_6 * A[6].y + ghrk::qa10_7 * A[7].y + ghrk::qa10_8 * A[8].y));
+  Q.z = q0.z + h * (ghrk::qp10 * p0.z + h * (ghrk::qa10_1 * A[1].z + ghrk::qa10_3 * A[3].z + ghrk::qa10_4 * A[4].z + ghrk::qa10_5 * A[5].z + ghrk::qa10_6 * A[6].z + ghrk::qa10_7 * A[7].z + ghrk::qa10_8 * A[8].z));
+  Q.dx = q0.dx + h * (ghrk::qp10 * p0.dx + h * (ghrk::qa10_1 * A[1].dx + ghrk::qa10_3 * A[3].dx + ghrk::qa10_4 * A[4].dx + ghrk::qa10_5 * A[5].dx + ghrk::qa10_6 * A[6].dx + ghrk::qa10_7 * A[7].dx + ghrk::qa10_8 * A[8].dx));
+  Q.dy = q0.dy + h * (ghrk::qp10 * p0.dy + h * (ghrk::qa10_1 * A[1].dy + ghrk::qa10_3 * A[3].dy + ghrk::qa10_4 * A[4].dy + ghrk::qa10_5 * A[5].dy + ghrk::qa10_6 * A[6].dy + ghrk::qa10_7 * A[7].dy + ghrk::qa10_8 * A[8].dy));
+  Q.dz = q0.dz + h * (ghrk::qp10 * p0.dz + h * (ghrk::qa10_1 * A[1].dz + ghrk::qa10_3 * A[3].dz + ghrk::qa10_4 * A[4].dz + ghrk::qa10_5 * A[5].dz + ghrk::qa10_6 * A[6].dz + ghrk::qa10_7 * A[7].dz + ghrk::qa10_8 * A[8].dz));
+  return Q;
+}
+
+// position argument of stage 11
+__device__ __forceinline__ GhVec6 gh_rkn_q11(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp11 * p0.x + h * (ghrk::qa11_1 * A[1].x + ghrk::qa11_3 * A[3].x + ghrk::qa11_4 * A[4].x + ghrk::qa11_5 * A[5].x + ghrk::qa11_6 * A[6].x + ghrk::qa11_7 * A[7].x + ghrk::qa11_8 * A[8].x + ghrk::qa11_9 * A[9].x));
+  Q.y = q0.y + h * (ghrk::qp11 * p0.y + h * (ghrk::qa11_1 * A[1].y + ghrk::qa11_3 * A[3].y + ghrk::qa11_4 * A[4].y + ghrk::qa11_5 * A[5].y + ghrk::qa11_6 * A[6].y + ghrk::qa11_7 * A[7].y + ghrk::qa11_8 * A[8].y + ghrk::qa11_9 * A[9].y));
+  Q.z = q0.z + h * (ghrk::qp11 * p0.z + h * (ghrk::qa11_1 * A[1].z + ghrk::qa11_3 * A[3].z + ghrk::qa11_4 * A[4].z + ghrk::qa11_5 * A[5].z + ghrk::qa11_6 * A[6].z + ghrk::qa11_7 * A[7].z + ghrk::qa11_8 * A[8].z + ghrk::qa11_9 * A[9].z));
+  Q.dx = q0.dx + h * (ghrk::qp11 * p0.dx + h * (ghrk::qa11_1 * A[1].dx + ghrk::qa11_3 * A[3].dx + ghrk::qa11_4 * A[4].dx + ghrk::qa11_5 * A[5].dx + ghrk::qa11_6 * A[6].dx + ghrk::qa11_7 * A[7].dx + ghrk::qa11_8 * A[8].dx + ghrk::qa11_9 * A[9].dx));
+  Q.dy = q0.dy + h * (ghrk::qp11 * p0.dy + h * (ghrk::qa11_1 * A[1].dy + ghrk::qa11_3 * A[3].dy + ghrk::qa11_4 * A[4].dy + ghrk::qa11_5 * A[5].dy + ghrk::qa11_6 * A[6].dy + ghrk::qa11_7 * A[7].dy + ghrk::qa11_8 * A[8].dy + ghrk::qa11_9 * A[9].dy));
+  Q.dz = q0.dz + h * (ghrk::qp11 * p0.dz + h * (ghrk::qa11_1 * A[1].dz + ghrk::qa11_3 * A[3].dz + ghrk::qa11_4 * A[4].dz + ghrk::qa11_5 * A[5].dz + ghrk::qa11_6 * A[6].dz + ghrk::qa11_7 * A[7].dz + ghrk::qa11_8 * A[8].dz + ghrk::qa11_9 * A[9].dz));
+  return Q;
+}
+
+// position argument of stage 12
+__device__ __forceinline__ GhVec6 gh_rkn_q12(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp12 * p0.x + h * (ghrk::qa12_1 * A[1].x + ghrk::qa12_3 * A[3].x + ghrk::qa12_4 * A[4].x + ghrk::qa12_5 * A[5].x + ghrk::qa12_6 * A[6].x + ghrk::qa12_7 * A[7].x + ghrk::qa12_8 * A[8].x + ghrk::qa12_9 * A[9].x + ghrk::qa12_10 * A[10].x));
+  Q.y = q0.y + h * (ghrk::qp12 * p0.y + h * (ghrk::qa12_1 * A[1].y + ghrk::qa12_3 * A[3].y + ghrk::qa12_4 * A[4].y + ghrk::qa12_5 * A[5].y + ghrk::qa12_6 * A[6].y + ghrk::qa12_7 * A[7].y + ghrk::qa12_8 * A[8].y + ghrk::qa12_9 * A[9].y + ghrk::qa12_10 * A[10].y));
+  Q.z = q0.z + h * (ghrk::qp12 * p0.z + h * (ghrk::qa12_1 * A[1].z + ghrk::qa12_3 * A[3].z + ghrk::qa12_4 * A[4].z + ghrk::qa12_5 * A[5].z + ghrk::qa12_6 * A[6].z + ghrk::qa12_7 * A[7].z + ghrk::qa12_8 * A[8].z + ghrk::qa12_9 * A[9].z + ghrk::qa12_10 * A[10].z));
+  Q.dx = q0.dx + h * (ghrk::qp12 * p0.dx + h * (ghrk::qa12_1 * A[1].dx + ghrk::qa12_3 * A[3].dx + ghrk::qa12_4 * A[4].dx + ghrk::qa12_5 * A[5].dx + ghrk::qa12_6 * A[6].dx + ghrk::qa12_7 * A[7].dx + ghrk::qa12_8 * A[8].dx + ghrk::qa12_9 * A[9].dx + ghrk::qa12_10 * A[10].dx));
+  Q.dy = q0.dy + h * (ghrk::qp12 * p0.dy + h * (ghrk::qa12_1 * A[1].dy + ghrk::qa12_3 * A[3].dy + ghrk::qa12_4 * A[4].dy + ghrk::qa12_5 * A[5].dy + ghrk::qa12_6 * A[6].dy + ghrk::qa12_7 * A[7].dy + ghrk::qa12_8 * A[8].dy + ghrk::qa12_9 * A[9].dy + ghrk::qa12_10 * A[10].dy));
+  Q.dz = q0.dz + h * (ghrk::qp12 * p0.dz + h * (ghrk::qa12_1 * A[1].dz + ghrk::qa12_3 * A[3].dz + ghrk::qa12_4 * A[4].dz + ghrk::qa12_5 * A[5].dz + ghrk::qa12_6 * A[6].dz + ghrk::qa12_7 * A[7].dz + ghrk::qa12_8 * A[8].dz + ghrk::qa12_9 * A[9].dz + ghrk::qa12_10 * A[10].dz));
+  return Q;
+}
+
+// position argument of stage 14
+__device__ __forceinline__ GhVec6 gh_rkn_q14(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp14 * p0.x + ghrk::qn14 * pn.x + h * (ghrk::qa14_1 * A[1].x + ghrk::qa14_4 * A[4].x + ghrk::qa14_5 * A[5].x + ghrk::qa14_6 * A[6].x + ghrk::qa14_7 * A[7].x + ghrk::qa14_8 * A[8].x + ghrk::qa14_9 * A[9].x + ghrk::qa14_10 * A[10].x + ghrk::qa14_11 * A[11].x));
+  Q.y = q0.y + h * (ghrk::qp14 * p0.y + ghrk::qn14 * pn.y + h * (ghrk::qa14_1 * A[1].y + ghrk::qa14_4 * A[4].y + ghrk::qa14_5 * A[5].y + ghrk::qa14_6 * A[6].y + ghrk::qa14_7 * A[7].y + ghrk::qa14_8 * A[8].y + ghrk::qa14_9 * A[9].y + ghrk::qa14_10 * A[10].y + ghrk::qa14_11 * A[11].y));
+  Q.z = q0.z + h * (ghrk::qp14 * p0.z + ghrk::qn14 * pn.z + h * (ghrk::qa14_1 * A[1].z + ghrk::qa14_4 * A[4].z + ghrk::qa14_5 * A[5].z + ghrk::qa14_6 * A[6].z + ghrk::qa14_7 * A[7].z + ghrk::qa14_8 * A[8].z + ghrk::qa14_9 * A[9].z + ghrk::qa14_10 * A[10].z + ghrk::qa14_11 * A[11].z));
+  Q.dx = q0.dx + h * (ghrk::qp14 * p0.dx + ghrk::qn14 * pn.dx + h * (ghrk::qa14_1 * A[1].dx + ghrk::qa14_4 * A[4].dx + ghrk::qa14_5 * A[5].dx + ghrk::qa14_6 * A[6].dx + ghrk::qa14_7 * A[7].dx + ghrk::qa14_8 * A[8].dx + ghrk::qa14_9 * A[9].dx + ghrk::qa14_10 * A[10].dx + ghrk::qa14_11 * A[11].dx));
+  Q.dy = q0.dy + h * (ghrk::qp14 * p0.dy + ghrk::qn14 * pn.dy + h * (ghrk::qa14_1 * A[1].dy + ghrk::qa14_4 * A[4].dy + ghrk::qa14_5 * A[5].dy + ghrk::qa14_6 * A[6].dy + ghrk::qa14_7 * A[7].dy + ghrk::qa14_8 * A[8].dy + ghrk::qa14_9 * A[9].dy + ghrk::qa14_10 * A[10].dy + ghrk::qa14_11 * A[11].dy));
+  Q.dz = q0.dz + h * (ghrk::qp14 * p0.dz + ghrk::qn14 * pn.dz + h * (ghrk::qa14_1 * A[1].dz + ghrk::qa14_4 * A[4].dz + ghrk::qa14_5 * A[5].dz + ghrk::qa14_6 * A[6].dz + ghrk::qa14_7 * A[7].dz + ghrk::qa14_8 * A[8].dz + ghrk::qa14_9 * A[9].dz + ghrk::qa14_10 * A[10].dz + ghrk::qa14_11 * A[11].dz));
+  return Q;
+}
+
+// position argument of stage 15
+__device__ __forceinline__ GhVec6 gh_rkn_q15(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp15 * p0.x + ghrk::qn15 * pn.x + h * (ghrk::qa15_1 * A[1].x + ghrk::qa15_4 * A[4].x + ghrk::qa15_5 * A[5].x + ghrk::qa15_6 * A[6].x + ghrk::qa15_7 * A[7].x + ghrk::qa15_8 * A[8].x + ghrk::qa15_9 * A[9].x + ghrk::qa15_10 * A[10].x + ghrk::qa15_11 * A[11].x + ghrk::qa15_12 * A[12].x + ghrk::qa15_13 * A[13].x));
+  Q.y = q0.y + h * (ghrk::qp15 * p0.y + ghrk::qn15 * pn.y + h * (ghrk::qa15_1 * A[1].y + ghrk::qa15_4 * A[4].y + ghrk::qa15_5 * A[5].y + ghrk::qa15_6 * A[6].y + ghrk::qa15_7 * A[7].y + ghrk::qa15_8 * A[8].y + ghrk::qa15_9 * A[9].y + ghrk::qa15_10 * A[10].y + ghrk::qa15_11 * A[11].y + ghrk::qa15_12 * A[12].y + ghrk::qa15_13 * A[13].y));
+  Q.z = q0.z + h * (ghrk::qp15 * p0.z + ghrk::qn15 * pn.z + h * (ghrk::qa15_1 * A[1].z + ghrk::qa15_4 * A[4].z + ghrk::qa15_5 * A[5].z + ghrk::qa15_6 * A[6].z + ghrk::qa15_7 * A[7].z + ghrk::qa15_8 * A[8].z + ghrk::qa15_9 * A[9].z + ghrk::qa15_10 * A[10].z + ghrk::qa15_11 * A[11].z + ghrk::qa15_12 * A[12].z + ghrk::qa15_13 * A[13].z));
+  Q.dx = q0.dx + h * (ghrk::qp15 * p0.dx + ghrk::qn15 * pn.dx + h * (ghrk::qa15_1 * A[1].dx + ghrk::qa15_4 * A[4].dx + ghrk::qa15_5 * A[5].dx + ghrk::qa15_6 * A[6].dx + ghrk::qa15_7 * A[7].dx + ghrk::qa15_8 * A[8].dx + ghrk::qa15_9 * A[9].dx + ghrk::qa15_10 * A[10].dx + ghrk::qa15_11 * A[11].dx + ghrk::qa15_12 * A[12].dx + ghrk::qa15_13 * A[13].dx));
+  Q.dy = q0.dy + h * (ghrk::qp15 * p0.dy + ghrk::qn15 * pn.dy + h * (ghrk::qa15_1 * A[1].dy + ghrk::qa15_4 * A[4].dy + ghrk::qa15_5 * A[5].dy + ghrk::qa15_6 * A[6].dy + ghrk::qa15_7 * A[7].dy + ghrk::qa15_8 * A[8].dy + ghrk::qa15_9 * A[9].dy + ghrk::qa15_10 * A[10].dy + ghrk::qa15_11 * A[11].dy + ghrk::qa15_12 * A[12].dy + ghrk::qa15_13 * A[13].dy));
+  Q.dz = q0.dz + h * (ghrk::qp15 * p0.dz + ghrk::qn15 * pn.dz + h * (ghrk::qa15_1 * A[1].dz + ghrk::qa15_4 * A[4].dz + ghrk::qa15_5 * A[5].dz + ghrk::qa15_6 * A[6].dz + ghrk::qa15_7 * A[7].dz + ghrk::qa15_8 * A[8].dz + ghrk::qa15_9 * A[9].dz + ghrk::qa15_10 * A[10].dz + ghrk::qa15_11 * A[11].dz + ghrk::qa15_12 * A[12].dz + ghrk::qa15_13 * A[13].dz));
+  return Q;
+}
+
+// position argument of stage 16
+__device__ __forceinline__ GhVec6 gh_rkn_q16(double h, const GhVec6& q0, const GhVec6& p0,
+                                             const GhVec6& pn, const GhVec6* A) {
+  (void)pn;
+  GhVec6 Q;
+  Q.x = q0.x + h * (ghrk::qp16 * p0.x + ghrk::qn16 * pn.x + h * (ghrk::qa16_1 * A[1].x + ghrk::qa16_4 * A[4].x + ghrk::qa16_5 * A[5].x + ghrk::qa16_6 * A[6].x + ghrk::qa16_7 * A[7].x + ghrk::qa16_8 * A[8].x + ghrk::qa16_9 * A[9].x + ghrk::qa16_10 * A[10].x + ghrk::qa16_11 * A[11].x + ghrk::qa16_12 * A[12].x + ghrk::qa16_13 * A[13].x + ghrk::qa16_14 * A[14].x));
+  Q.y = q0.y + h * (ghrk::qp16 * p0.y + ghrk::qn16 * pn.y + h * (ghrk::qa16_1 * A[1].y + ghrk::qa16_4 * A[4].y + ghrk::qa16_5 * A[5].y + ghrk::qa16_6 * A[6].y + ghrk::qa16_7 * A[7].y + ghrk::qa16_8 * A[8].y + ghrk::qa16_9 * A[9].y + ghrk::qa16_10 * A[10].y + ghrk::qa16_11 * A[11].y + ghrk::qa16_12 * A[12].y + ghrk::qa16_13 * A[13].y + ghrk::qa16_14 * A[14].y));
+  Q.z = q0.z + h * (ghrk::qp16 * p0.z + ghrk::qn16 * pn.z + h * (ghrk::qa16_1 * A[1].z + ghrk::qa16_4 * A[4].z + ghrk::qa16_5 * A[5].z + ghrk::qa16_6 * A[6].z + ghrk::qa16_7 * A[7].z + ghrk::qa16_8 * A[8].z + ghrk::qa16_9 * A[9].z + ghrk::qa16_10 * A[10].z + ghrk::qa16_11 * A[11].z + ghrk::qa16_12 * A[12].z + ghrk::qa16_13 * A[13].z + ghrk::qa16_14 * A[14].z));
+  Q.dx = q0.dx + h * (ghrk::qp16 * p0.dx + ghrk::qn16 * pn.dx + h * (ghrk::qa16_1 * A[1].dx + ghrk::qa16_4 * A[4].dx + ghrk::qa16_5 * A[5].dx + ghrk::qa16_6 * A[6].dx + ghrk::qa16_7 * A[7].dx + ghrk::qa16_8 * A[8].dx + ghrk::qa16_9 * A[9].dx + ghrk::qa16_10 * A[10].dx + ghrk::qa16_11 * A[11].dx + ghrk::qa16_12 * A[12].dx + ghrk::qa16_13 * A[13].dx + ghrk::qa16_14 * A[14].dx));
+  Q.dy = q0.dy + h * (ghrk::qp16 * p0.dy + ghrk::qn16 * pn.dy + h * (ghrk::qa16_1 * A[1].dy + ghrk::qa16_4 * A[4].dy + ghrk::qa16_5 * A[5].dy + ghrk::qa16_6 * A[6].dy + ghrk::qa16_7 * A[7].dy + ghrk::qa16_8 * A[8].dy + ghrk::qa16_9 * A[9].dy + ghrk::qa16_10 * A[10].dy + ghrk::qa16_11 * A[11].dy + ghrk::qa16_12 * A[12].dy + ghrk::qa16_13 * A[13].dy + ghrk::qa16_14 * A[14].dy));
+  Q.dz = q0.dz + h * (ghrk::qp16 * p0.dz + ghrk::qn16 * pn.dz + h * (ghrk::qa16_1 * A[1].dz + ghrk::qa16_4 * A[4].dz + ghrk::qa16_5 * A[5].dz + ghrk::qa16_6 * A[6].dz + ghrk::qa16_7 * A[7].dz + ghrk::qa16_8 * A[8].dz + ghrk::qa16_9 * A[9].dz + ghrk::qa16_10 * A[10].dz + ghrk::qa16_11 * A[11].dz + ghrk::qa16_12 * A[12].dz + ghrk::qa16_13 * A[13].dz + ghrk::qa16_14 * A[14].dz));
+  return Q;
+}
+
+// New state (q_new, p_new) and the two error sums of leung_dop853.c:376-404:
+// per component, sk = atol + rtol max(|y0|, |y_new|), err3 += (e3/sk)^2,
+// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k (one
+// reciprocal of sk per component; the norm only steers the step size).
+__device__ __forceinline__ void gh_rkn_solution(double h, const GhVec6& q0, const GhVec6& p0,
+                                               const GhVec6* A, double atol, double rtol,
+                                               GhVec6& qn, GhVec6& pn, double& err5,
+                                               double& err3) {
+  err5 = 0.0;
+  err3 = 0.0;
+  {
+    const double kq = ghrk::bp * p0.x + h * (ghrk::ba_1 * A[1].x + ghrk::ba_4 * A[4].x + ghrk::ba_5 * A[5].x + ghrk::ba_6 * A[6].x + ghrk::ba_7 * A[7].x + ghrk::ba_8 * A[8].x + ghrk::ba_9 * A[9].x + ghrk::ba_10 * A[10].x + ghrk::ba_11 * A[11].x);
+    qn.x = q0.x + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.x), fabs(qn.x)));
+    double sqr = (ghrk::e3p * p0.x + h * (ghrk::e3a_1 * A[1].x + ghrk::e3a_4 * A[4].x + ghrk::e3a_5 * A[5].x + ghrk::e3a_6 * A[6].x + ghrk::e3a_7 * A[7].x + ghrk::e3a_8 * A[8].x + ghrk::e3a_9 * A[9].x + ghrk::e3a_10 * A[10].x + ghrk::e3a_11 * A[11].x)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.x + h * (ghrk::e5a_1 * A[1].x + ghrk::e5a_4 * A[4].x + ghrk::e5a_5 * A[5].x + ghrk::e5a_6 * A[6].x + ghrk::e5a_7 * A[7].x + ghrk::e5a_8 * A[8].x + ghrk::e5a_9 * A[9].x + ghrk::e5a_10 * A[10].x + ghrk::e5a_11 * A[11].x)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kq = ghrk::bp * p0.y + h * (ghrk::ba_1 * A[1].y + ghrk::ba_4 * A[4].y + ghrk::ba_5 * A[5].y + ghrk::ba_6 * A[6].y + ghrk::ba_7 * A[7].y + ghrk::ba_8 * A[8].y + ghrk::ba_9 * A[9].y + ghrk::ba_10 * A[10].y + ghrk::ba_11 * A[11].y);
+    qn.y = q0.y + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.y), fabs(qn.y)));
+    double sqr = (ghrk::e3p * p0.y + h * (ghrk::e3a_1 * A[1].y + ghrk::e3a_4 * A[4].y + ghrk::e3a_5 * A[5].y + ghrk::e3a_6 * A[6].y + ghrk::e3a_7 * A[7].y + ghrk::e3a_8 * A[8].y + ghrk::e3a_9 * A[9].y + ghrk::e3a_10 * A[10].y + ghrk::e3a_11 * A[11].y)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.y + h * (ghrk::e5a_1 * A[1].y + ghrk::e5a_4 * A[4].y + ghrk::e5a_5 * A[5].y + ghrk::e5a_6 * A[6].y + ghrk::e5a_7 * A[7].y + ghrk::e5a_8 * A[8].y + ghrk::e5a_9 * A[9].y + ghrk::e5a_10 * A[10].y + ghrk::e5a_11 * A[11].y)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kq = ghrk::bp * p0.z + h * (ghrk::ba_1 * A[1].z + ghrk::ba_4 * A[4].z + ghrk::ba_5 * A[5].z + ghrk::ba_6 * A[6].z + ghrk::ba_7 * A[7].z + ghrk::ba_8 * A[8].z + ghrk::ba_9 * A[9].z + ghrk::ba_10 * A[10].z + ghrk::ba_11 * A[11].z);
+    qn.z = q0.z + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.z), fabs(qn.z)));
+    double sqr = (ghrk::e3p * p0.z + h * (ghrk::e3a_1 * A[1].z + ghrk::e3a_4 * A[4].z + ghrk::e3a_5 * A[5].z + ghrk::e3a_6 * A[6].z + ghrk::e3a_7 * A[7].z + ghrk::e3a_8 * A[8].z + ghrk::e3a_9 * A[9].z + ghrk::e3a_10 * A[10].z + ghrk::e3a_11 * A[11].z)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.z + h * (ghrk::e5a_1 * A[1].z + ghrk::e5a_4 * A[4].z + ghrk::e5a_5 * A[5].z + ghrk::e5a_6 * A[6].z + ghrk::e5a_7 * A[7].z + ghrk::e5a_8 * A[8].z + ghrk::e5a_9 * A[9].z + ghrk::e5a_10 * A[10].z + ghrk::e5a_11 * A[11].z)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kq = ghrk::bp * p0.dx + h * (ghrk::ba_1 * A[1].dx + ghrk::ba_4 * A[4].dx + ghrk::ba_5 * A[5].dx + ghrk::ba_6 * A[6].dx + ghrk::ba_7 * A[7].dx + ghrk::ba_8 * A[8].dx + ghrk::ba_9 * A[9].dx + ghrk::ba_10 * A[10].dx + ghrk::ba_11 * A[11].dx);
+    qn.dx = q0.dx + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.dx), fabs(qn.dx)));
+    double sqr = (ghrk::e3p * p0.dx + h * (ghrk::e3a_1 * A[1].dx + ghrk::e3a_4 * A[4].dx + ghrk::e3a_5 * A[5].dx + ghrk::e3a_6 * A[6].dx + ghrk::e3a_7 * A[7].dx + ghrk::e3a_8 * A[8].dx + ghrk::e3a_9 * A[9].dx + ghrk::e3a_10 * A[10].dx + ghrk::e3a_11 * A[11].dx)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.dx + h * (ghrk::e5a_1 * A[1].dx + ghrk::e5a_4 * A[4].dx + ghrk::e5a_5 * A[5].dx + ghrk::e5a_6 * A[6].dx + ghrk::e5a_7 * A[7].dx + ghrk::e5a_8 * A[8].dx + ghrk::e5a_9 * A[9].dx + ghrk::e5a_10 * A[10].dx + ghrk::e5a_11 * A[11].dx)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kq = ghrk::bp * p0.dy + h * (ghrk::ba_1 * A[1].dy + ghrk::ba_4 * A[4].dy + ghrk::ba_5 * A[5].dy + ghrk::ba_6 * A[6].dy + ghrk::ba_7 * A[7].dy + ghrk::ba_8 * A[8].dy + ghrk::ba_9 * A[9].dy + ghrk::ba_10 * A[10].dy + ghrk::ba_11 * A[11].dy);
+    qn.dy = q0.dy + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.dy), fabs(qn.dy)));
+    double sqr = (ghrk::e3p * p0.dy + h * (ghrk::e3a_1 * A[1].dy + ghrk::e3a_4 * A[4].dy + ghrk::e3a_5 * A[5].dy + ghrk::e3a_6 * A[6].dy + ghrk::e3a_7 * A[7].dy + ghrk::e3a_8 * A[8].dy + ghrk::e3a_9 * A[9].dy + ghrk::e3a_10 * A[10].dy + ghrk::e3a_11 * A[11].dy)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.dy + h * (ghrk::e5a_1 * A[1].dy + ghrk::e5a_4 * A[4].dy + ghrk::e5a_5 * A[5].dy + ghrk::e5a_6 * A[6].dy + ghrk::e5a_7 * A[7].dy + ghrk::e5a_8 * A[8].dy + ghrk::e5a_9 * A[9].dy + ghrk::e5a_10 * A[10].dy + ghrk::e5a_11 * A[11].dy)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kq = ghrk::bp * p0.dz + h * (ghrk::ba_1 * A[1].dz + ghrk::ba_4 * A[4].dz + ghrk::ba_5 * A[5].dz + ghrk::ba_6 * A[6].dz + ghrk::ba_7 * A[7].dz + ghrk::ba_8 * A[8].dz + ghrk::ba_9 * A[9].dz + ghrk::ba_10 * A[10].dz + ghrk::ba_11 * A[11].dz);
+    qn.dz = q0.dz + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.dz), fabs(qn.dz)));
+    double sqr = (ghrk::e3p * p0.dz + h * (ghrk::e3a_1 * A[1].dz + ghrk::e3a_4 * A[4].dz + ghrk::e3a_5 * A[5].dz + ghrk::e3a_6 * A[6].dz + ghrk::e3a_7 * A[7].dz + ghrk::e3a_8 * A[8].dz + ghrk::e3a_9 * A[9].dz + ghrk::e3a_10 * A[10].dz + ghrk::e3a_11 * A[11].dz)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.dz + h * (ghrk::e5a_1 * A[1].dz + ghrk::e5a_4 * A[4].dz + ghrk::e5a_5 * A[5].dz + ghrk::e5a_6 * A[6].dz + ghrk::e5a_7 * A[7].dz + ghrk::e5a_8 * A[8].dz + ghrk::e5a_9 * A[9].dz + ghrk::e5a_10 * A[10].dz + ghrk::e5a_11 * A[11].dz)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].x + ghdp::b6 * A[6].x + ghdp::b7 * A[7].x + ghdp::b8 * A[8].x + ghdp::b9 * A[9].x + ghdp::b10 * A[10].x + ghdp::b11 * A[11].x + ghdp::b12 * A[12].x;
+    pn.x = p0.x + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.x), fabs(pn.x)));
+    double sqr = (kp - ghdp::bhh1 * A[1].x - ghdp::bhh2 * A[9].x - ghdp::bhh3 * A[12].x) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].x + ghdp::er6 * A[6].x + ghdp::er7 * A[7].x + ghdp::er8 * A[8].x + ghdp::er9 * A[9].x + ghdp::er10 * A[10].x + ghdp::er11 * A[11].x + ghdp::er12 * A[12].x) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].y + ghdp::b6 * A[6].y + ghdp::b7 * A[7].y + ghdp::b8 * A[8].y + ghdp::b9 * A[9].y + ghdp::b10 * A[10].y + ghdp::b11 * A[11].y + ghdp::b12 * A[12].y;
+    pn.y = p0.y + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.y), fabs(pn.y)));
+    double sqr = (kp - ghdp::bhh1 * A[1].y - ghdp::bhh2 * A[9].y - ghdp::bhh3 * A[12].y) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].y + ghdp::er6 * A[6].y + ghdp::er7 * A[7].y + ghdp::er8 * A[8].y + ghdp::er9 * A[9].y + ghdp::er10 * A[10].y + ghdp::er11 * A[11].y + ghdp::er12 * A[12].y) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].z + ghdp::b6 * A[6].z + ghdp::b7 * A[7].z + ghdp::b8 * A[8].z + ghdp::b9 * A[9].z + ghdp::b10 * A[10].z + ghdp::b11 * A[11].z + ghdp::b12 * A[12].z;
+    pn.z = p0.z + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.z), fabs(pn.z)));
+    double sqr = (kp - ghdp::bhh1 * A[1].z - ghdp::bhh2 * A[9].z - ghdp::bhh3 * A[12].z) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].z + ghdp::er6 * A[6].z + ghdp::er7 * A[7].z + ghdp::er8 * A[8].z + ghdp::er9 * A[9].z + ghdp::er10 * A[10].z + ghdp::er11 * A[11].z + ghdp::er12 * A[12].z) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].dx + ghdp::b6 * A[6].dx + ghdp::b7 * A[7].dx + ghdp::b8 * A[8].dx + ghdp::b9 * A[9].dx + ghdp::b10 * A[10].dx + ghdp::b11 * A[11].dx + ghdp::b12 * A[12].dx;
+    pn.dx = p0.dx + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.dx), fabs(pn.dx)));
+    double sqr = (kp - ghdp::bhh1 * A[1].dx - ghdp::bhh2 * A[9].dx - ghdp::bhh3 * A[12].dx) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].dx + ghdp::er6 * A[6].dx + ghdp::er7 * A[7].dx + ghdp::er8 * A[8].dx + ghdp::er9 * A[9].dx + ghdp::er10 * A[10].dx + ghdp::er11 * A[11].dx + ghdp::er12 * A[12].dx) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].dy + ghdp::b6 * A[6].dy + ghdp::b7 * A[7].dy + ghdp::b8 * A[8].dy + ghdp::b9 * A[9].dy + ghdp::b10 * A[10].dy + ghdp::b11 * A[11].dy + ghdp::b12 * A[12].dy;
+    pn.dy = p0.dy + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.dy), fabs(pn.dy)));
+    double sqr = (kp - ghdp::bhh1 * A[1].dy - ghdp::bhh2 * A[9].dy - ghdp::bhh3 * A[12].dy) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].dy + ghdp::er6 * A[6].dy + ghdp::er7 * A[7].dy + ghdp::er8 * A[8].dy + ghdp::er9 * A[9].dy + ghdp::er10 * A[10].dy + ghdp::er11 * A[11].dy + ghdp::er12 * A[12].dy) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].dz + ghdp::b6 * A[6].dz + ghdp::b7 * A[7].dz + ghdp::b8 * A[8].dz + ghdp::b9 * A[9].dz + ghdp::b10 * A[10].dz + ghdp::b11 * A[11].dz + ghdp::b12 * A[12].dz;
+    pn.dz = p0.dz + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.dz), fabs(pn.dz)));
+    double sqr = (kp - ghdp::bhh1 * A[1].dz - ghdp::bhh2 * A[9].dz - ghdp::bhh3 * A[12].dz) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].dz + ghdp::er6 * A[6].dz + ghdp::er7 * A[7].dz + ghdp::er8 * A[8].dz + ghdp::er9 * A[9].dz + ghdp::er10 * A[10].dz + ghdp::er11 * A[11].dz + ghdp::er12 * A[12].dz) * isk;
+    err5 += sqr * sqr;
+  }
+}
+
+// Dense output (leung_dop853.c:430-474) of the accepted step at the time
+// t_old + sd h (s1 = 1 - sd), after the dense stages 14-16: per component
+// the coefficients rc1..rc8 of the continuous extension, then
+// rc1 + sd (rc2 + s1 (rc3 + sd (rc4 + s1 (rc5 + sd (rc6 + s1 (rc7 + sd rc8)))))).
+__device__ __forceinline__ void gh_rkn_dense_eval(double h, double sd, double s1, const GhVec6& q0,
+                                                 const GhVec6& p0, const GhVec6& qn,
+                                                 const GhVec6& pn, const GhVec6* A, double* yo) {
+  {
+    const double ydiff = qn.x - q0.x;
+    const double bspl = h * p0.x - ydiff;
+    const double r0 = q0.x, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.x - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.x + ghrk::dn4 * pn.x + h * (ghrk::da4_1 * A[1].x + ghrk::da4_4 * A[4].x + ghrk::da4_5 * A[5].x + ghrk::da4_6 * A[6].x + ghrk::da4_7 * A[7].x + ghrk::da4_8 * A[8].x + ghrk::da4_9 * A[9].x + ghrk::da4_10 * A[10].x + ghrk::da4_11 * A[11].x + ghrk::da4_12 * A[12].x + ghrk::da4_13 * A[13].x + ghrk::da4_14 * A[14].x + ghrk::da4_15 * A[15].x));
+    const double r5 = h * (ghrk::dp5 * p0.x + ghrk::dn5 * pn.x + h * (ghrk::da5_1 * A[1].x + ghrk::da5_4 * A[4].x + ghrk::da5_5 * A[5].x + ghrk::da5_6 * A[6].x + ghrk::da5_7 * A[7].x + ghrk::da5_8 * A[8].x + ghrk::da5_9 * A[9].x + ghrk::da5_10 * A[10].x + ghrk::da5_11 * A[11].x + ghrk::da5_12 * A[12].x + ghrk::da5_13 * A[13].x + ghrk::da5_14 * A[14].x + ghrk::da5_15 * A[15].x));
+    const double r6 = h * (ghrk::dp6 * p0.x + ghrk::dn6 * pn.x + h * (ghrk::da6_1 * A[1].x + ghrk::da6_4 * A[4].x + ghrk::da6_5 * A[5].x + ghrk::da6_6 * A[6].x + ghrk::da6_7 * A[7].x + ghrk::da6_8 * A[8].x + ghrk::da6_9 * A[9].x + ghrk::da6_10 * A[10].x + ghrk::da6_11 * A[11].x + ghrk::da6_12 * A[12].x + ghrk::da6_13 * A[13].x + ghrk::da6_14 * A[14].x + ghrk::da6_15 * A[15].x));
+    const double r7 = h * (ghrk::dp7 * p0.x + ghrk::dn7 * pn.x + h * (ghrk::da7_1 * A[1].x + ghrk::da7_4 * A[4].x + ghrk::da7_5 * A[5].x + ghrk::da7_6 * A[6].x + ghrk::da7_7 * A[7].x + ghrk::da7_8 * A[8].x + ghrk::da7_9 * A[9].x + ghrk::da7_10 * A[10].x + ghrk::da7_11 * A[11].x + ghrk::da7_12 * A[12].x + ghrk::da7_13 * A[13].x + ghrk::da7_14 * A[14].x + ghrk::da7_15 * A[15].x));
+    yo[0] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = qn.y - q0.y;
+    const double bspl = h * p0.y - ydiff;
+    const double r0 = q0.y, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.y - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.y + ghrk::dn4 * pn.y + h * (ghrk::da4_1 * A[1].y + ghrk::da4_4 * A[4].y + ghrk::da4_5 * A[5].y + ghrk::da4_6 * A[6].y + ghrk::da4_7 * A[7].y + ghrk::da4_8 * A[8].y + ghrk::da4_9 * A[9].y + ghrk::da4_10 * A[10].y + ghrk::da4_11 * A[11].y + ghrk::da4_12 * A[12].y + ghrk::da4_13 * A[13].y + ghrk::da4_14 * A[14].y + ghrk::da4_15 * A[15].y));
+    const double r5 = h * (ghrk::dp5 * p0.y + ghrk::dn5 * pn.y + h * (ghrk::da5_1 * A[1].y + ghrk::da5_4 * A[4].y + ghrk::da5_5 * A[5].y + ghrk::da5_6 * A[6].y + ghrk::da5_7 * A[7].y + ghrk::da5_8 * A[8].y + ghrk::da5_9 * A[9].y + ghrk::da5_10 * A[10].y + ghrk::da5_11 * A[11].y + ghrk::da5_12 * A[12].y + ghrk::da5_13 * A[13].y + ghrk::da5_14 * A[14].y + ghrk::da5_15 * A[15].y));
+    const double r6 = h * (ghrk::dp6 * p0.y + ghrk::dn6 * pn.y + h * (ghrk::da6_1 * A[1].y + ghrk::da6_4 * A[4].y + ghrk::da6_5 * A[5].y + ghrk::da6_6 * A[6].y + ghrk::da6_7 * A[7].y + ghrk::da6_8 * A[8].y + ghrk::da6_9 * A[9].y + ghrk::da6_10 * A[10].y + ghrk::da6_11 * A[11].y + ghrk::da6_12 * A[12].y + ghrk::da6_13 * A[13].y + ghrk::da6_14 * A[14].y + ghrk::da6_15 * A[15].y));
+    const double r7 = h * (ghrk::dp7 * p0.y + ghrk::dn7 * pn.y + h * (ghrk::da7_1 * A[1].y + ghrk::da7_4 * A[4].y + ghrk::da7_5 * A[5].y + ghrk::da7_6 * A[6].y + ghrk::da7_7 * A[7].y + ghrk::da7_8 * A[8].y + ghrk::da7_9 * A[9].y + ghrk::da7_10 * A[10].y + ghrk::da7_11 * A[11].y + ghrk::da7_12 * A[12].y + ghrk::da7_13 * A[13].y + ghrk::da7_14 * A[14].y + ghrk::da7_15 * A[15].y));
+    yo[1] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = qn.z - q0.z;
+    const double bspl = h * p0.z - ydiff;
+    const double r0 = q0.z, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.z - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.z + ghrk::dn4 * pn.z + h * (ghrk::da4_1 * A[1].z + ghrk::da4_4 * A[4].z + ghrk::da4_5 * A[5].z + ghrk::da4_6 * A[6].z + ghrk::da4_7 * A[7].z + ghrk::da4_8 * A[8].z + ghrk::da4_9 * A[9].z + ghrk::da4_10 * A[10].z + ghrk::da4_11 * A[11].z + ghrk::da4_12 * A[12].z + ghrk::da4_13 * A[13].z + ghrk::da4_14 * A[14].z + ghrk::da4_15 * A[15].z));
+    const double r5 = h * (ghrk::dp5 * p0.z + ghrk::dn5 * pn.z + h * (ghrk::da5_1 * A[1].z + ghrk::da5_4 * A[4].z + ghrk::da5_5 * A[5].z + ghrk::da5_6 * A[6].z + ghrk::da5_7 * A[7].z + ghrk::da5_8 * A[8].z + ghrk::da5_9 * A[9].z + ghrk::da5_10 * A[10].z + ghrk::da5_11 * A[11].z + ghrk::da5_12 * A[12].z + ghrk::da5_13 * A[13].z + ghrk::da5_14 * A[14].z + ghrk::da5_15 * A[15].z));
+    const double r6 = h * (ghrk::dp6 * p0.z + ghrk::dn6 * pn.z + h * (ghrk::da6_1 * A[1].z + ghrk::da6_4 * A[4].z + ghrk::da6_5 * A[5].z + ghrk::da6_6 * A[6].z + ghrk::da6_7 * A[7].z + ghrk::da6_8 * A[8].z + ghrk::da6_9 * A[9].z + ghrk::da6_10 * A[10].z + ghrk::da6_11 * A[11].z + ghrk::da6_12 * A[12].z + ghrk::da6_13 * A[13].z + ghrk::da6_14 * A[14].z + ghrk::da6_15 * A[15].z));
+    const double r7 = h * (ghrk::dp7 * p0.z + ghrk::dn7 * pn.z + h * (ghrk::da7_1 * A[1].z + ghrk::da7_4 * A[4].z + ghrk::da7_5 * A[5].z + ghrk::da7_6 * A[6].z + ghrk::da7_7 * A[7].z + ghrk::da7_8 * A[8].z + ghrk::da7_9 * A[9].z + ghrk::da7_10 * A[10].z + ghrk::da7_11 * A[11].z + ghrk::da7_12 * A[12].z + ghrk::da7_13 * A[13].z + ghrk::da7_14 * A[14].z + ghrk::da7_15 * A[15].z));
+    yo[2] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = qn.dx - q0.dx;
+    const double bspl = h * p0.dx - ydiff;
+    const double r0 = q0.dx, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.dx - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.dx + ghrk::dn4 * pn.dx + h * (ghrk::da4_1 * A[1].dx + ghrk::da4_4 * A[4].dx + ghrk::da4_5 * A[5].dx + ghrk::da4_6 * A[6].dx + ghrk::da4_7 * A[7].dx + ghrk::da4_8 * A[8].dx + ghrk::da4_9 * A[9].dx + ghrk::da4_10 * A[10].dx + ghrk::da4_11 * A[11].dx + ghrk::da4_12 * A[12].dx + ghrk::da4_13 * A[13].dx + ghrk::da4_14 * A[14].dx + ghrk::da4_15 * A[15].dx));
+    const double r5 = h * (ghrk::dp5 * p0.dx + ghrk::dn5 * pn.dx + h * (ghrk::da5_1 * A[1].dx + ghrk::da5_4 * A[4].dx + ghrk::da5_5 * A[5].dx + ghrk::da5_6 * A[6].dx + ghrk::da5_7 * A[7].dx + ghrk::da5_8 * A[8].dx + ghrk::da5_9 * A[9].dx + ghrk::da5_10 * A[10].dx + ghrk::da5_11 * A[11].dx + ghrk::da5_12 * A[12].dx + ghrk::da5_13 * A[13].dx + ghrk::da5_14 * A[14].dx + ghrk::da5_15 * A[15].dx));
+    const double r6 = h * (ghrk::dp6 * p0.dx + ghrk::dn6 * pn.dx + h * (ghrk::da6_1 * A[1].dx + ghrk::da6_4 * A[4].dx + ghrk::da6_5 * A[5].dx + ghrk::da6_6 * A[6].dx + ghrk::da6_7 * A[7].dx + ghrk::da6_8 * A[8].dx + ghrk::da6_9 * A[9].dx + ghrk::da6_10 * A[10].dx + ghrk::da6_11 * A[11].dx + ghrk::da6_12 * A[12].dx + ghrk::da6_13 * A[13].dx + ghrk::da6_14 * A[14].dx + ghrk::da6_15 * A[15].dx));
+    const double r7 = h * (ghrk::dp7 * p0.dx + ghrk::dn7 * pn.dx + h * (ghrk::da7_1 * A[1].dx + ghrk::da7_4 * A[4].dx + ghrk::da7_5 * A[5].dx + ghrk::da7_6 * A[6].dx + ghrk::da7_7 * A[7].dx + ghrk::da7_8 * A[8].dx + ghrk::da7_9 * A[9].dx + ghrk::da7_10 * A[10].dx + ghrk::da7_11 * A[11].dx + ghrk::da7_12 * A[12].dx + ghrk::da7_13 * A[13].dx + ghrk::da7_14 * A[14].dx + ghrk::da7_15 * A[15].dx));
+    yo[3] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = qn.dy - q0.dy;
+    const double bspl = h * p0.dy - ydiff;
+    const double r0 = q0.dy, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.dy - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.dy + ghrk::dn4 * pn.dy + h * (ghrk::da4_1 * A[1].dy + ghrk::da4_4 * A[4].dy + ghrk::da4_5 * A[5].dy + ghrk::da4_6 * A[6].dy + ghrk::da4_7 * A[7].dy + ghrk::da4_8 * A[8].dy + ghrk::da4_9 * A[9].dy + ghrk::da4_10 * A[10].dy + ghrk::da4_11 * A[11].dy + ghrk::da4_12 * A[12].dy + ghrk::da4_13 * A[13].dy + ghrk::da4_14 * A[14].dy + ghrk::da4_15 * A[15].dy));
+    const double r5 = h * (ghrk::dp5 * p0.dy + ghrk::dn5 * pn.dy + h * (ghrk::da5_1 * A[1].dy + ghrk::da5_4 * A[4].dy + ghrk::da5_5 * A[5].dy + ghrk::da5_6 * A[6].dy + ghrk::da5_7 * A[7].dy + ghrk::da5_8 * A[8].dy + ghrk::da5_9 * A[9].dy + ghrk::da5_10 * A[10].dy + ghrk::da5_11 * A[11].dy + ghrk::da5_12 * A[12].dy + ghrk::da5_13 * A[13].dy + ghrk::da5_14 * A[14].dy + ghrk::da5_15 * A[15].dy));
+    const double r6 = h * (ghrk::dp6 * p0.dy + ghrk::dn6 * pn.dy + h * (ghrk::da6_1 * A[1].dy + ghrk::da6_4 * A[4].dy + ghrk::da6_5 * A[5].dy + ghrk::da6_6 * A[6].dy + ghrk::da6_7 * A[7].dy + ghrk::da6_8 * A[8].dy + ghrk::da6_9 * A[9].dy + ghrk::da6_10 * A[10].dy + ghrk::da6_11 * A[11].dy + ghrk::da6_12 * A[12].dy + ghrk::da6_13 * A[13].dy + ghrk::da6_14 * A[14].dy + ghrk::da6_15 * A[15].dy));
+    const double r7 = h * (ghrk::dp7 * p0.dy + ghrk::dn7 * pn.dy + h * (ghrk::da7_1 * A[1].dy + ghrk::da7_4 * A[4].dy + ghrk::da7_5 * A[5].dy + ghrk::da7_6 * A[6].dy + ghrk::da7_7 * A[7].dy + ghrk::da7_8 * A[8].dy + ghrk::da7_9 * A[9].dy + ghrk::da7_10 * A[10].dy + ghrk::da7_11 * A[11].dy + ghrk::da7_12 * A[12].dy + ghrk::da7_13 * A[13].dy + ghrk::da7_14 * A[14].dy + ghrk::da7_15 * A[15].dy));
+    yo[4] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = qn.dz - q0.dz;
+    const double bspl = h * p0.dz - ydiff;
+    const double r0 = q0.dz, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.dz - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.dz + ghrk::dn4 * pn.dz + h * (ghrk::da4_1 * A[1].dz + ghrk::da4_4 * A[4].dz + ghrk::da4_5 * A[5].dz + ghrk::da4_6 * A[6].dz + ghrk::da4_7 * A[7].dz + ghrk::da4_8 * A[8].dz + ghrk::da4_9 * A[9].dz + ghrk::da4_10 * A[10].dz + ghrk::da4_11 * A[11].dz + ghrk::da4_12 * A[12].dz + ghrk::da4_13 * A[13].dz + ghrk::da4_14 * A[14].dz + ghrk::da4_15 * A[15].dz));
+    const double r5 = h * (ghrk::dp5 * p0.dz + ghrk::dn5 * pn.dz + h * (ghrk::da5_1 * A[1].dz + ghrk::da5_4 * A[4].dz + ghrk::da5_5 * A[5].dz + ghrk::da5_6 * A[6].dz + ghrk::da5_7 * A[7].dz + ghrk::da5_8 * A[8].dz + ghrk::da5_9 * A[9].dz + ghrk::da5_10 * A[10].dz + ghrk::da5_11 * A[11].dz + ghrk::da5_12 * A[12].dz + ghrk::da5_13 * A[13].dz + ghrk::da5_14 * A[14].dz + ghrk::da5_15 * A[15].dz));
+    const double r6 = h * (ghrk::dp6 * p0.dz + ghrk::dn6 * pn.dz + h * (ghrk::da6_1 * A[1].dz + ghrk::da6_4 * A[4].dz + ghrk::da6_5 * A[5].dz + ghrk::da6_6 * A[6].dz + ghrk::da6_7 * A[7].dz + ghrk::da6_8 * A[8].dz + ghrk::da6_9 * A[9].dz + ghrk::da6_10 * A[10].dz + ghrk::da6_11 * A[11].dz + ghrk::da6_12 * A[12].dz + ghrk::da6_13 * A[13].dz + ghrk::da6_14 * A[14].dz + ghrk::da6_15 * A[15].dz));
+    const double r7 = h * (ghrk::dp7 * p0.dz + ghrk::dn7 * pn.dz + h * (ghrk::da7_1 * A[1].dz + ghrk::da7_4 * A[4].dz + ghrk::da7_5 * A[5].dz + ghrk::da7_6 * A[6].dz + ghrk::da7_7 * A[7].dz + ghrk::da7_8 * A[8].dz + ghrk::da7_9 * A[9].dz + ghrk::da7_10 * A[10].dz + ghrk::da7_11 * A[11].dz + ghrk::da7_12 * A[12].dz + ghrk::da7_13 * A[13].dz + ghrk::da7_14 * A[14].dz + ghrk::da7_15 * A[15].dz));
+    yo[5] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.x - p0.x;
+    const double bspl = h * A[1].x - ydiff;
+    const double r0 = p0.x, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].x - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].x + ghdp::d46 * A[6].x + ghdp::d47 * A[7].x + ghdp::d48 * A[8].x + ghdp::d49 * A[9].x + ghdp::d410 * A[10].x + ghdp::d411 * A[11].x + ghdp::d412 * A[12].x + ghdp::d413 * A[13].x + ghdp::d414 * A[14].x + ghdp::d415 * A[15].x + ghdp::d416 * A[16].x);
+    const double r5 = h * (ghdp::d51 * A[1].x + ghdp::d56 * A[6].x + ghdp::d57 * A[7].x + ghdp::d58 * A[8].x + ghdp::d59 * A[9].x + ghdp::d510 * A[10].x + ghdp::d511 * A[11].x + ghdp::d512 * A[12].x + ghdp::d513 * A[13].x + ghdp::d514 * A[14].x + ghdp::d515 * A[15].x + ghdp::d516 * A[16].x);
+    const double r6 = h * (ghdp::d61 * A[1].x + ghdp::d66 * A[6].x + ghdp::d67 * A[7].x + ghdp::d68 * A[8].x + ghdp::d69 * A[9].x + ghdp::d610 * A[10].x + ghdp::d611 * A[11].x + ghdp::d612 * A[12].x + ghdp::d613 * A[13].x + ghdp::d614 * A[14].x + ghdp::d615 * A[15].x + ghdp::d616 * A[16].x);
+    const double r7 = h * (ghdp::d71 * A[1].x + ghdp::d76 * A[6].x + ghdp::d77 * A[7].x + ghdp::d78 * A[8].x + ghdp::d79 * A[9].x + ghdp::d710 * A[10].x + ghdp::d711 * A[11].x + ghdp::d712 * A[12].x + ghdp::d713 * A[13].x + ghdp::d714 * A[14].x + ghdp::d715 * A[15].x + ghdp::d716 * A[16].x);
+    yo[6] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.y - p0.y;
+    const double bspl = h * A[1].y - ydiff;
+    const double r0 = p0.y, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].y - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].y + ghdp::d46 * A[6].y + ghdp::d47 * A[7].y + ghdp::d48 * A[8].y + ghdp::d49 * A[9].y + ghdp::d410 * A[10].y + ghdp::d411 * A[11].y + ghdp::d412 * A[12].y + ghdp::d413 * A[13].y + ghdp::d414 * A[14].y + ghdp::d415 * A[15].y + ghdp::d416 * A[16].y);
+    const double r5 = h * (ghdp::d51 * A[1].y + ghdp::d56 * A[6].y + ghdp::d57 * A[7].y + ghdp::d58 * A[8].y + ghdp::d59 * A[9].y + ghdp::d510 * A[10].y + ghdp::d511 * A[11].y + ghdp::d512 * A[12].y + ghdp::d513 * A[13].y + ghdp::d514 * A[14].y + ghdp::d515 * A[15].y + ghdp::d516 * A[16].y);
+    const double r6 = h * (ghdp::d61 * A[1].y + ghdp::d66 * A[6].y + ghdp::d67 * A[7].y + ghdp::d68 * A[8].y + ghdp::d69 * A[9].y + ghdp::d610 * A[10].y + ghdp::d611 * A[11].y + ghdp::d612 * A[12].y + ghdp::d613 * A[13].y + ghdp::d614 * A[14].y + ghdp::d615 * A[15].y + ghdp::d616 * A[16].y);
+    const double r7 = h * (ghdp::d71 * A[1].y + ghdp::d76 * A[6].y + ghdp::d77 * A[7].y + ghdp::d78 * A[8].y + ghdp::d79 * A[9].y + ghdp::d710 * A[10].y + ghdp::d711 * A[11].y + ghdp::d712 * A[12].y + ghdp::d713 * A[13].y + ghdp::d714 * A[14].y + ghdp::d715 * A[15].y + ghdp::d716 * A[16].y);
+    yo[7] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.z - p0.z;
+    const double bspl = h * A[1].z - ydiff;
+    const double r0 = p0.z, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].z - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].z + ghdp::d46 * A[6].z + ghdp::d47 * A[7].z + ghdp::d48 * A[8].z + ghdp::d49 * A[9].z + ghdp::d410 * A[10].z + ghdp::d411 * A[11].z + ghdp::d412 * A[12].z + ghdp::d413 * A[13].z + ghdp::d414 * A[14].z + ghdp::d415 * A[15].z + ghdp::d416 * A[16].z);
+    const double r5 = h * (ghdp::d51 * A[1].z + ghdp::d56 * A[6].z + ghdp::d57 * A[7].z + ghdp::d58 * A[8].z + ghdp::d59 * A[9].z + ghdp::d510 * A[10].z + ghdp::d511 * A[11].z + ghdp::d512 * A[12].z + ghdp::d513 * A[13].z + ghdp::d514 * A[14].z + ghdp::d515 * A[15].z + ghdp::d516 * A[16].z);
+    const double r6 = h * (ghdp::d61 * A[1].z + ghdp::d66 * A[6].z + ghdp::d67 * A[7].z + ghdp::d68 * A[8].z + ghdp::d69 * A[9].z + ghdp::d610 * A[10].z + ghdp::d611 * A[11].z + ghdp::d612 * A[12].z + ghdp::d613 * A[13].z + ghdp::d614 * A[14].z + ghdp::d615 * A[15].z + ghdp::d616 * A[16].z);
+    const double r7 = h * (ghdp::d71 * A[1].z + ghdp::d76 * A[6].z + ghdp::d77 * A[7].z + ghdp::d78 * A[8].z + ghdp::d79 * A[9].z + ghdp::d710 * A[10].z + ghdp::d711 * A[11].z + ghdp::d712 * A[12].z + ghdp::d713 * A[13].z + ghdp::d714 * A[14].z + ghdp::d715 * A[15].z + ghdp::d716 * A[16].z);
+    yo[8] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.dx - p0.dx;
+    const double bspl = h * A[1].dx - ydiff;
+    const double r0 = p0.dx, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].dx - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].dx + ghdp::d46 * A[6].dx + ghdp::d47 * A[7].dx + ghdp::d48 * A[8].dx + ghdp::d49 * A[9].dx + ghdp::d410 * A[10].dx + ghdp::d411 * A[11].dx + ghdp::d412 * A[12].dx + ghdp::d413 * A[13].dx + ghdp::d414 * A[14].dx + ghdp::d415 * A[15].dx + ghdp::d416 * A[16].dx);
+    const double r5 = h * (ghdp::d51 * A[1].dx + ghdp::d56 * A[6].dx + ghdp::d57 * A[7].dx + ghdp::d58 * A[8].dx + ghdp::d59 * A[9].dx + ghdp::d510 * A[10].dx + ghdp::d511 * A[11].dx + ghdp::d512 * A[12].dx + ghdp::d513 * A[13].dx + ghdp::d514 * A[14].dx + ghdp::d515 * A[15].dx + ghdp::d516 * A[16].dx);
+    const double r6 = h * (ghdp::d61 * A[1].dx + ghdp::d66 * A[6].dx + ghdp::d67 * A[7].dx + ghdp::d68 * A[8].dx + ghdp::d69 * A[9].dx + ghdp::d610 * A[10].dx + ghdp::d611 * A[11].dx + ghdp::d612 * A[12].dx + ghdp::d613 * A[13].dx + ghdp::d614 * A[14].dx + ghdp::d615 * A[15].dx + ghdp::d616 * A[16].dx);
+    const double r7 = h * (ghdp::d71 * A[1].dx + ghdp::d76 * A[6].dx + ghdp::d77 * A[7].dx + ghdp::d78 * A[8].dx + ghdp::d79 * A[9].dx + ghdp::d710 * A[10].dx + ghdp::d711 * A[11].dx + ghdp::d712 * A[12].dx + ghdp::d713 * A[13].dx + ghdp::d714 * A[14].dx + ghdp::d715 * A[15].dx + ghdp::d716 * A[16].dx);
+    yo[9] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.dy - p0.dy;
+    const double bspl = h * A[1].dy - ydiff;
+    const double r0 = p0.dy, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].dy - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].dy + ghdp::d46 * A[6].dy + ghdp::d47 * A[7].dy + ghdp::d48 * A[8].dy + ghdp::d49 * A[9].dy + ghdp::d410 * A[10].dy + ghdp::d411 * A[11].dy + ghdp::d412 * A[12].dy + ghdp::d413 * A[13].dy + ghdp::d414 * A[14].dy + ghdp::d415 * A[15].dy + ghdp::d416 * A[16].dy);
+    const double r5 = h * (ghdp::d51 * A[1].dy + ghdp::d56 * A[6].dy + ghdp::d57 * A[7].dy + ghdp::d58 * A[8].dy + ghdp::d59 * A[9].dy + ghdp::d510 * A[10].dy + ghdp::d511 * A[11].dy + ghdp::d512 * A[12].dy + ghdp::d513 * A[13].dy + ghdp::d514 * A[14].dy + ghdp::d515 * A[15].dy + ghdp::d516 * A[16].dy);
+    const double r6 = h * (ghdp::d61 * A[1].dy + ghdp::d66 * A[6].dy + ghdp::d67 * A[7].dy + ghdp::d68 * A[8].dy + ghdp::d69 * A[9].dy + ghdp::d610 * A[10].dy + ghdp::d611 * A[11].dy + ghdp::d612 * A[12].dy + ghdp::d613 * A[13].dy + ghdp::d614 * A[14].dy + ghdp::d615 * A[15].dy + ghdp::d616 * A[16].dy);
+    const double r7 = h * (ghdp::d71 * A[1].dy + ghdp::d76 * A[6].dy + ghdp::d77 * A[7].dy + ghdp::d78 * A[8].dy + ghdp::d79 * A[9].dy + ghdp::d710 * A[10].dy + ghdp::d711 * A[11].dy + ghdp::d712 * A[12].dy + ghdp::d713 * A[13].dy + ghdp::d714 * A[14].dy + ghdp::d715 * A[15].dy + ghdp::d716 * A[16].dy);
+    yo[10] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.dz - p0.dz;
+    const double bspl = h * A[1].dz - ydiff;
+    const double r0 = p0.dz, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].dz - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].dz + ghdp::d46 * A[6].dz + ghdp::d47 * A[7].dz + ghdp::d48 * A[8].dz + ghdp::d49 * A[9].dz + ghdp::d410 * A[10].dz + ghdp::d411 * A[11].dz + ghdp::d412 * A[12].dz + ghdp::d413 * A[13].dz + ghdp::d414 * A[14].dz + ghdp::d415 * A[15].dz + ghdp::d416 * A[16].dz);
+    const double r5 = h * (ghdp::d51 * A[1].dz + ghdp::d56 * A[6].dz + ghdp::d57 * A[7].dz + ghdp::d58 * A[8].dz + ghdp::d59 * A[9].dz + ghdp::d510 * A[10].dz + ghdp::d511 * A[11].dz + ghdp::d512 * A[12].dz + ghdp::d513 * A[13].dz + ghdp::d514 * A[14].dz + ghdp::d515 * A[15].dz + ghdp::d516 * A[16].dz);
+    const double r6 = h * (ghdp::d61 * A[1].dz + ghdp::d66 * A[6].dz + ghdp::d67 * A[7].dz + ghdp::d68 * A[8].dz + ghdp::d69 * A[9].dz + ghdp::d610 * A[10].dz + ghdp::d611 * A[11].dz + ghdp::d612 * A[12].dz + ghdp::d613 * A[13].dz + ghdp::d614 * A[14].dz + ghdp::d615 * A[15].dz + ghdp::d616 * A[16].dz);
+    const double r7 = h * (ghdp::d71 * A[1].dz + ghdp::d76 * A[6].dz + ghdp::d77 * A[7].dz + ghdp::d78 * A[8].dz + ghdp::d79 * A[9].dz + ghdp::d710 * A[10].dz + ghdp::d711 * A[11].dz + ghdp::d712 * A[12].dz + ghdp::d713 * A[13].dz + ghdp::d714 * A[14].dz + ghdp::d715 * A[15].dz + ghdp::d716 * A[16].dz);
+    yo[11] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+}
+// position argument of stage 2
+__device__ __forceinline__ GhVec4 gh_rkn_q2(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp2 * p0.x);
+  Q.y = q0.y + h * (ghrk::qp2 * p0.y);
+  Q.dx = q0.dx + h * (ghrk::qp2 * p0.dx);
+  Q.dy = q0.dy + h * (ghrk::qp2 * p0.dy);
+  return Q;
+}
+
+// position argument of stage 3
+__device__ __forceinline__ GhVec4 gh_rkn_q3(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp3 * p0.x + h * (ghrk::qa3_1 * A[1].x));
+  Q.y = q0.y + h * (ghrk::qp3 * p0.y + h * (ghrk::qa3_1 * A[1].y));
+  Q.dx = q0.dx + h * (ghrk::qp3 * p0.dx + h * (ghrk::qa3_1 * A[1].dx));
+  Q.dy = q0.dy + h * (ghrk::qp3 * p0.dy + h * (ghrk::qa3_1 * A[1].dy));
+  return Q;
+}
+
+// position argument of stage 4
+__device__ __forceinline__ GhVec4 gh_rkn_q4(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp4 * p0.x + h * (ghrk::qa4_1 * A[1].x + ghrk::qa4_2 * A[2].x));
+  Q.y = q0.y + h * (ghrk::qp4 * p0.y + h * (ghrk::qa4_1 * A[1].y + ghrk::qa4_2 * A[2].y));
+  Q.dx = q0.dx + h * (ghrk::qp4 * p0.dx + h * (ghrk::qa4_1 * A[1].dx + ghrk::qa4_2 * A[2].dx));
+  Q.dy = q0.dy + h * (ghrk::qp4 * p0.dy + h * (ghrk::qa4_1 * A[1].dy + ghrk::qa4_2 * A[2].dy));
+  return Q;
+}
+
+// position argument of stage 5
+__device__ __forceinline__ GhVec4 gh_rkn_q5(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp5 * p0.x + h * (ghrk::qa5_1 * A[1].x + ghrk::qa5_2 * A[2].x + ghrk::qa5_3 * A[3].x));
+  Q.y = q0.y + h * (ghrk::qp5 * p0.y + h * (ghrk::qa5_1 * A[1].y + ghrk::qa5_2 * A[2].y + ghrk::qa5_3 * A[3].y));
+  Q.dx = q0.dx + h * (ghrk::qp5 * p0.dx + h * (ghrk::qa5_1 * A[1].dx + ghrk::qa5_2 * A[2].dx + ghrk::qa5_3 * A[3].dx));
+  Q.dy = q0.dy + h * (ghrk::qp5 * p0.dy + h * (ghrk::qa5_1 * A[1].dy + ghrk::qa5_2 * A[2].dy + ghrk::qa5_3 * A[3].dy));
+  return Q;
+}
+
+// position argument of stage 6
+__device__ __forceinline__ GhVec4 gh_rkn_q6(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp6 * p0.x + h * (ghrk::qa6_1 * A[1].x + ghrk::qa6_3 * A[3].x + ghrk::qa6_4 * A[4].x));
+  Q.y = q0.y + h * (ghrk::qp6 * p0.y + h * (ghrk::qa6_1 * A[1].y + ghrk::qa6_3 * A[3].y + ghrk::qa6_4 * A[4].y));
+  Q.dx = q0.dx + h * (ghrk::qp6 * p0.dx + h * (ghrk::qa6_1 * A[1].dx + ghrk::qa6_3 * A[3].dx + ghrk::qa6_4 * A[4].dx));
+  Q.dy = q0.dy + h * (ghrk::qp6 * p0.dy + h * (ghrk::qa6_1 * A[1].dy + ghrk::qa6_3 * A[3].dy + ghrk::qa6_4 * A[4].dy));
+  return Q;
+}
+
+// position argument of stage 7
+__device__ __forceinline__ GhVec4 gh_rkn_q7(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp7 * p0.x + h * (ghrk::qa7_1 * A[1].x + ghrk::qa7_3 * A[3].x + ghrk::qa7_4 * A[4].x + ghrk::qa7_5 * A[5].x));
+  Q.y = q0.y + h * (ghrk::qp7 * p0.y + h * (ghrk::qa7_1 * A[1].y + ghrk::qa7_3 * A[3].y + ghrk::qa7_4 * A[4].y + ghrk::qa7_5 * A[5].y));
+  Q.dx = q0.dx + h * (ghrk::qp7 * p0.dx + h * (ghrk::qa7_1 * A[1].dx + ghrk::qa7_3 * A[3].dx + ghrk::qa7_4 * A[4].dx + ghrk::qa7_5 * A[5].dx));
+  Q.dy = q0.dy + h * (ghrk::qp7 * p0.dy + h * (ghrk::qa7_1 * A[1].dy + ghrk::qa7_3 * A[3].dy + ghrk::qa7_4 * A[4].dy + ghrk::qa7_5 * A[5].dy));
+  return Q;
+}
+
+// position argument of stage 8
+__device__ __forceinline__ GhVec4 gh_rkn_q8(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp8 * p0.x + h * (ghrk::qa8_1 * A[1].x + ghrk::qa8_3 * A[3].x + ghrk::qa8_4 * A[4].x + ghrk::qa8_5 * A[5].x + ghrk::qa8_6 * A[6].x));
+  Q.y = q0.y + h * (ghrk::qp8 * p0.y + h * (ghrk::qa8_1 * A[1].y + ghrk::qa8_3 * A[3].y + ghrk::qa8_4 * A[4].y + ghrk::qa8_5 * A[5].y + ghrk::qa8_6 * A[6].y));
+  Q.dx = q0.dx + h * (ghrk::qp8 * p0.dx + h * (ghrk::qa8_1 * A[1].dx + ghrk::qa8_3 * A[3].dx + ghrk::qa8_4 * A[4].dx + ghrk::qa8_5 * A[5].dx + ghrk::qa8_6 * A[6].dx));
+  Q.dy = q0.dy + h * (ghrk::qp8 * p0.dy + h * (ghrk::qa8_1 * A[1].dy + ghrk::qa8_3 * A[3].dy + ghrk::qa8_4 * A[4].dy + ghrk::qa8_5 * A[5].dy + ghrk::qa8_6 * A[6].dy));
+  return Q;
+}
+
+// position argument of stage 9
+__device__ __forceinline__ GhVec4 gh_rkn_q9(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp9 * p0.x + h * (ghrk::qa9_1 * A[1].x + ghrk::qa9_3 * A[3].x + ghrk::qa9_4 * A[4].x + ghrk::qa9_5 * A[5].x + ghrk::qa9_6 * A[6].x + ghrk::qa9_7 * A[7].x));
+  Q.y = q0.y + h * (ghrk::qp9 * p0.y + h * (ghrk::qa9_1 * A[1].y + ghrk::qa9_3 * A[3].y + ghrk::qa9_4 * A[4].y + ghrk::qa9_5 * A[5].y + ghrk::qa9_6 * A[6].y + ghrk::qa9_7 * A[7].y));
+  Q.dx = q0.dx + h * (ghrk::qp9 * p0.dx + h * (ghrk::qa9_1 * A[1].dx + ghrk::qa9_3 * A[3].dx + ghrk::qa9_4 * A[4].dx + ghrk::qa9_5 * A[5].dx + ghrk::qa9_6 * A[6].dx + ghrk::qa9_7 * A[7].dx));
+  Q.dy = q0.dy + h * (ghrk::qp9 * p0.dy + h * (ghrk::qa9_1 * A[1].dy + ghrk::qa9_3 * A[3].dy + ghrk::qa9_4 * A[4].dy + ghrk::qa9_5 * A[5].dy + ghrk::qa9_6 * A[6].dy + ghrk::qa9_7 * A[7].dy));
+  return Q;
+}
+
+// position argument of stage 10
+__device__ __forceinline__ GhVec4 gh_rkn_q10(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp10 * p0.x + h * (ghrk::qa10_1 * A[1].x + ghrk::qa10_3 * A[3].x + ghrk::qa10_4 * A[4].x + ghrk::qa10_5 * A[5].x + ghrk::qa10_6 * A[6].x + ghrk::qa10_7 * A[7].x + ghrk::qa10_8 * A[8].x));
+  Q.y = q0.y + h * (ghrk::qp10 * p0.y + h * (ghrk::qa10_1 * A[1].y + ghrk::qa10_3 * A[3].y + ghrk::qa10_4 * A[4].y + ghrk::qa10_5 * A[5].y + ghrk::qa10_6 * A[6].y + ghrk::qa10_7 * A[7].y + ghrk::qa10_8 * A[8].y));
+  Q.dx = q0.dx + h * (ghrk::qp10 * p0.dx + h * (ghrk::qa10_1 * A[1].dx + ghrk::qa10_3 * A[3].dx + ghrk::qa10_4 * A[4].dx + ghrk::qa10_5 * A[5].dx + ghrk::qa10_6 * A[6].dx + ghrk::qa10_7 * A[7].dx + ghrk::qa10_8 * A[8].dx));
+  Q.dy = q0.dy + h * (ghrk::qp10 * p0.dy + h * (ghrk::qa10_1 * A[1].dy + ghrk::qa10_3 * A[3].dy + ghrk::qa10_4 * A[4].dy + ghrk::qa10_5 * A[5].dy + ghrk::qa10_6 * A[6].dy + ghrk::qa10_7 * A[7].dy + ghrk::qa10_8 * A[8].dy));
+  return Q;
+}
+
+// position argument of stage 11
+__device__ __forceinline__ GhVec4 gh_rkn_q11(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp11 * p0.x + h * (ghrk::qa11_1 * A[1].x + ghrk::qa11_3 * A[3].x + ghrk::qa11_4 * A[4].x + ghrk::qa11_5 * A[5].x + ghrk::qa11_6 * A[6].x + ghrk::qa11_7 * A[7].x + ghrk::qa11_8 * A[8].x + ghrk::qa11_9 * A[9].x));
+  Q.y = q0.y + h * (ghrk::qp11 * p0.y + h * (ghrk::qa11_1 * A[1].y + ghrk::qa11_3 * A[3].y + ghrk::qa11_4 * A[4].y + ghrk::qa11_5 * A[5].y + ghrk::qa11_6 * A[6].y + ghrk::qa11_7 * A[7].y + ghrk::qa11_8 * A[8].y + ghrk::qa11_9 * A[9].y));
+  Q.dx = q0.dx + h * (ghrk::qp11 * p0.dx + h * (ghrk::qa11_1 * A[1].dx + ghrk::qa11_3 * A[3].dx + ghrk::qa11_4 * A[4].dx + ghrk::qa11_5 * A[5].dx + ghrk::qa11_6 * A[6].dx + ghrk::qa11_7 * A[7].dx + ghrk::qa11_8 * A[8].dx + ghrk::qa11_9 * A[9].dx));
+  Q.dy = q0.dy + h * (ghrk::qp11 * p0.dy + h * (ghrk::qa11_1 * A[1].dy + ghrk::qa11_3 * A[3].dy + ghrk::qa11_4 * A[4].dy + ghrk::qa11_5 * A[5].dy + ghrk::qa11_6 * A[6].dy + ghrk::qa11_7 * A[7].dy + ghrk::qa11_8 * A[8].dy + ghrk::qa11_9 * A[9].dy));
+  return Q;
+}
+
+// position argument of stage 12
+__device__ __forceinline__ GhVec4 gh_rkn_q12(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp12 * p0.x + h * (ghrk::qa12_1 * A[1].x + ghrk::qa12_3 * A[3].x + ghrk::qa12_4 * A[4].x + ghrk::qa12_5 * A[5].x + ghrk::qa12_6 * A[6].x + ghrk::qa12_7 * A[7].x + ghrk::qa12_8 * A[8].x + ghrk::qa12_9 * A[9].x + ghrk::qa12_10 * A[10].x));
+  Q.y = q0.y + h * (ghrk::qp12 * p0.y + h * (ghrk::qa12_1 * A[1].y + ghrk::qa12_3 * A[3].y + ghrk::qa12_4 * A[4].y + ghrk::qa12_5 * A[5].y + ghrk::qa12_6 * A[6].y + ghrk::qa12_7 * A[7].y + ghrk::qa12_8 * A[8].y + ghrk::qa12_9 * A[9].y + ghrk::qa12_10 * A[10].y));
+  Q.dx = q0.dx + h * (ghrk::qp12 * p0.dx + h * (ghrk::qa12_1 * A[1].dx + ghrk::qa12_3 * A[3].dx + ghrk::qa12_4 * A[4].dx + ghrk::qa12_5 * A[5].dx + ghrk::qa12_6 * A[6].dx + ghrk::qa12_7 * A[7].dx + ghrk::qa12_8 * A[8].dx + ghrk::qa12_9 * A[9].dx + ghrk::qa12_10 * A[10].dx));
+  Q.dy = q0.dy + h * (ghrk::qp12 * p0.dy + h * (ghrk::qa12_1 * A[1].dy + ghrk::qa12_3 * A[3].dy + ghrk::qa12_4 * A[4].dy + ghrk::qa12_5 * A[5].dy + ghrk::qa12_6 * A[6].dy + ghrk::qa12_7 * A[7].dy + ghrk::qa12_8 * A[8].dy + ghrk::qa12_9 * A[9].dy + ghrk::qa12_10 * A[10].dy));
+  return Q;
+}
+
+// position argument of stage 14
+__device__ __forceinline__ GhVec4 gh_rkn_q14(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp14 * p0.x + ghrk::qn14 * pn.x + h * (ghrk::qa14_1 * A[1].x + ghrk::qa14_4 * A[4].x + ghrk::qa14_5 * A[5].x + ghrk::qa14_6 * A[6].x + ghrk::qa14_7 * A[7].x + ghrk::qa14_8 * A[8].x + ghrk::qa14_9 * A[9].x + ghrk::qa14_10 * A[10].x + ghrk::qa14_11 * A[11].x));
+  Q.y = q0.y + h * (ghrk::qp14 * p0.y + ghrk::qn14 * pn.y + h * (ghrk::qa14_1 * A[1].y + ghrk::qa14_4 * A[4].y + ghrk::qa14_5 * A[5].y + ghrk::qa14_6 * A[6].y + ghrk::qa14_7 * A[7].y + ghrk::qa14_8 * A[8].y + ghrk::qa14_9 * A[9].y + ghrk::qa14_10 * A[10].y + ghrk::qa14_11 * A[11].y));
+  Q.dx = q0.dx + h * (ghrk::qp14 * p0.dx + ghrk::qn14 * pn.dx + h * (ghrk::qa14_1 * A[1].dx + ghrk::qa14_4 * A[4].dx + ghrk::qa14_5 * A[5].dx + ghrk::qa14_6 * A[6].dx + ghrk::qa14_7 * A[7].dx + ghrk::qa14_8 * A[8].dx + ghrk::qa14_9 * A[9].dx + ghrk::qa14_10 * A[10].dx + ghrk::qa14_11 * A[11].dx));
+  Q.dy = q0.dy + h * (ghrk::qp14 * p0.dy + ghrk::qn14 * pn.dy + h * (ghrk::qa14_1 * A[1].dy + ghrk::qa14_4 * A[4].dy + ghrk::qa14_5 * A[5].dy + ghrk::qa14_6 * A[6].dy + ghrk::qa14_7 * A[7].dy + ghrk::qa14_8 * A[8].dy + ghrk::qa14_9 * A[9].dy + ghrk::qa14_10 * A[10].dy + ghrk::qa14_11 * A[11].dy));
+  return Q;
+}
+
+// position argument of stage 15
+__device__ __forceinline__ GhVec4 gh_rkn_q15(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp15 * p0.x + ghrk::qn15 * pn.x + h * (ghrk::qa15_1 * A[1].x + ghrk::qa15_4 * A[4].x + ghrk::qa15_5 * A[5].x + ghrk::qa15_6 * A[6].x + ghrk::qa15_7 * A[7].x + ghrk::qa15_8 * A[8].x + ghrk::qa15_9 * A[9].x + ghrk::qa15_10 * A[10].x + ghrk::qa15_11 * A[11].x + ghrk::qa15_12 * A[12].x + ghrk::qa15_13 * A[13].x));
+  Q.y = q0.y + h * (ghrk::qp15 * p0.y + ghrk::qn15 * pn.y + h * (ghrk::qa15_1 * A[1].y + ghrk::qa15_4 * A[4].y + ghrk::qa15_5 * A[5].y + ghrk::qa15_6 * A[6].y + ghrk::qa15_7 * A[7].y + ghrk::qa15_8 * A[8].y + ghrk::qa15_9 * A[9].y + ghrk::qa15_10 * A[10].y + ghrk::qa15_11 * A[11].y + ghrk::qa15_12 * A[12].y + ghrk::qa15_13 * A[13].y));
+  Q.dx = q0.dx + h * (ghrk::qp15 * p0.dx + ghrk::qn15 * pn.dx + h * (ghrk::qa15_1 * A[1].dx + ghrk::qa15_4 * A[4].dx + ghrk::qa15_5 * A[5].dx + ghrk::qa15_6 * A[6].dx + ghrk::qa15_7 * A[7].dx + ghrk::qa15_8 * A[8].dx + ghrk::qa15_9 * A[9].dx + ghrk::qa15_10 * A[10].dx + ghrk::qa15_11 * A[11].dx + ghrk::qa15_12 * A[12].dx + ghrk::qa15_13 * A[13].dx));
+  Q.dy = q0.dy + h * (ghrk::qp15 * p0.dy + ghrk::qn15 * pn.dy + h * (ghrk::qa15_1 * A[1].dy + ghrk::qa15_4 * A[4].dy + ghrk::qa15_5 * A[5].dy + ghrk::qa15_6 * A[6].dy + ghrk::qa15_7 * A[7].dy + ghrk::qa15_8 * A[8].dy + ghrk::qa15_9 * A[9].dy + ghrk::qa15_10 * A[10].dy + ghrk::qa15_11 * A[11].dy + ghrk::qa15_12 * A[12].dy + ghrk::qa15_13 * A[13].dy));
+  return Q;
+}
+
+// position argument of stage 16
+__device__ __forceinline__ GhVec4 gh_rkn_q16(double h, const GhVec4& q0, const GhVec4& p0,
+                                             const GhVec4& pn, const GhVec4* A) {
+  (void)pn;
+  GhVec4 Q;
+  Q.x = q0.x + h * (ghrk::qp16 * p0.x + ghrk::qn16 * pn.x + h * (ghrk::qa16_1 * A[1].x + ghrk::qa16_4 * A[4].x + ghrk::qa16_5 * A[5].x + ghrk::qa16_6 * A[6].x + ghrk::qa16_7 * A[7].x + ghrk::qa16_8 * A[8].x + ghrk::qa16_9 * A[9].x + ghrk::qa16_10 * A[10].x + ghrk::qa16_11 * A[11].x + ghrk::qa16_12 * A[12].x + ghrk::qa16_13 * A[13].x + ghrk::qa16_14 * A[14].x));
+  Q.y = q0.y + h * (ghrk::qp16 * p0.y + ghrk::qn16 * pn.y + h * (ghrk::qa16_1 * A[1].y + ghrk::qa16_4 * A[4].y + ghrk::qa16_5 * A[5].y + ghrk::qa16_6 * A[6].y + ghrk::qa16_7 * A[7].y + ghrk::qa16_8 * A[8].y + ghrk::qa16_9 * A[9].y + ghrk::qa16_10 * A[10].y + ghrk::qa16_11 * A[11].y + ghrk::qa16_12 * A[12].y + ghrk::qa16_13 * A[13].y + ghrk::qa16_14 * A[14].y));
+  Q.dx = q0.dx + h * (ghrk::qp16 * p0.dx + ghrk::qn16 * pn.dx + h * (ghrk::qa16_1 * A[1].dx + ghrk::qa16_4 * A[4].dx + ghrk::qa16_5 * A[5].dx + ghrk::qa16_6 * A[6].dx + ghrk::qa16_7 * A[7].dx + ghrk::qa16_8 * A[8].dx + ghrk::qa16_9 * A[9].dx + ghrk::qa16_10 * A[10].dx + ghrk::qa16_11 * A[11].dx + ghrk::qa16_12 * A[12].dx + ghrk::qa16_13 * A[13].dx + ghrk::qa16_14 * A[14].dx));
+  Q.dy = q0.dy + h * (ghrk::qp16 * p0.dy + ghrk::qn16 * pn.dy + h * (ghrk::qa16_1 * A[1].dy + ghrk::qa16_4 * A[4].dy + ghrk::qa16_5 * A[5].dy + ghrk::qa16_6 * A[6].dy + ghrk::qa16_7 * A[7].dy + ghrk::qa16_8 * A[8].dy + ghrk::qa16_9 * A[9].dy + ghrk::qa16_10 * A[10].dy + ghrk::qa16_11 * A[11].dy + ghrk::qa16_12 * A[12].dy + ghrk::qa16_13 * A[13].dy + ghrk::qa16_14 * A[14].dy));
+  return Q;
+}
+
+// New state (q_new, p_new) and the two error sums of leung_dop853.c:376-404:
+// per component, sk = atol + rtol max(|y0|, |y_new|), err3 += (e3/sk)^2,
+// err5 += (e5/sk)^2, with e3 = sum (b - bhh) k, e5 = sum er k (one
+// reciprocal of sk per component; the norm only steers the step size).
+__device__ __forceinline__ void gh_rkn_solution(double h, const GhVec4& q0, const GhVec4& p0,
+                                               const GhVec4* A, double atol, double rtol,
+                                               GhVec4& qn, GhVec4& pn, double& err5,
+                                               double& err3) {
+  err5 = 0.0;
+  err3 = 0.0;
+  {
+    const double kq = ghrk::bp * p0.x + h * (ghrk::ba_1 * A[1].x + ghrk::ba_4 * A[4].x + ghrk::ba_5 * A[5].x + ghrk::ba_6 * A[6].x + ghrk::ba_7 * A[7].x + ghrk::ba_8 * A[8].x + ghrk::ba_9 * A[9].x + ghrk::ba_10 * A[10].x + ghrk::ba_11 * A[11].x);
+    qn.x = q0.x + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.x), fabs(qn.x)));
+    double sqr = (ghrk::e3p * p0.x + h * (ghrk::e3a_1 * A[1].x + ghrk::e3a_4 * A[4].x + ghrk::e3a_5 * A[5].x + ghrk::e3a_6 * A[6].x + ghrk::e3a_7 * A[7].x + ghrk::e3a_8 * A[8].x + ghrk::e3a_9 * A[9].x + ghrk::e3a_10 * A[10].x + ghrk::e3a_11 * A[11].x)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.x + h * (ghrk::e5a_1 * A[1].x + ghrk::e5a_4 * A[4].x + ghrk::e5a_5 * A[5].x + ghrk::e5a_6 * A[6].x + ghrk::e5a_7 * A[7].x + ghrk::e5a_8 * A[8].x + ghrk::e5a_9 * A[9].x + ghrk::e5a_10 * A[10].x + ghrk::e5a_11 * A[11].x)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kq = ghrk::bp * p0.y + h * (ghrk::ba_1 * A[1].y + ghrk::ba_4 * A[4].y + ghrk::ba_5 * A[5].y + ghrk::ba_6 * A[6].y + ghrk::ba_7 * A[7].y + ghrk::ba_8 * A[8].y + ghrk::ba_9 * A[9].y + ghrk::ba_10 * A[10].y + ghrk::ba_11 * A[11].y);
+    qn.y = q0.y + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.y), fabs(qn.y)));
+    double sqr = (ghrk::e3p * p0.y + h * (ghrk::e3a_1 * A[1].y + ghrk::e3a_4 * A[4].y + ghrk::e3a_5 * A[5].y + ghrk::e3a_6 * A[6].y + ghrk::e3a_7 * A[7].y + ghrk::e3a_8 * A[8].y + ghrk::e3a_9 * A[9].y + ghrk::e3a_10 * A[10].y + ghrk::e3a_11 * A[11].y)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.y + h * (ghrk::e5a_1 * A[1].y + ghrk::e5a_4 * A[4].y + ghrk::e5a_5 * A[5].y + ghrk::e5a_6 * A[6].y + ghrk::e5a_7 * A[7].y + ghrk::e5a_8 * A[8].y + ghrk::e5a_9 * A[9].y + ghrk::e5a_10 * A[10].y + ghrk::e5a_11 * A[11].y)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kq = ghrk::bp * p0.dx + h * (ghrk::ba_1 * A[1].dx + ghrk::ba_4 * A[4].dx + ghrk::ba_5 * A[5].dx + ghrk::ba_6 * A[6].dx + ghrk::ba_7 * A[7].dx + ghrk::ba_8 * A[8].dx + ghrk::ba_9 * A[9].dx + ghrk::ba_10 * A[10].dx + ghrk::ba_11 * A[11].dx);
+    qn.dx = q0.dx + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.dx), fabs(qn.dx)));
+    double sqr = (ghrk::e3p * p0.dx + h * (ghrk::e3a_1 * A[1].dx + ghrk::e3a_4 * A[4].dx + ghrk::e3a_5 * A[5].dx + ghrk::e3a_6 * A[6].dx + ghrk::e3a_7 * A[7].dx + ghrk::e3a_8 * A[8].dx + ghrk::e3a_9 * A[9].dx + ghrk::e3a_10 * A[10].dx + ghrk::e3a_11 * A[11].dx)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.dx + h * (ghrk::e5a_1 * A[1].dx + ghrk::e5a_4 * A[4].dx + ghrk::e5a_5 * A[5].dx + ghrk::e5a_6 * A[6].dx + ghrk::e5a_7 * A[7].dx + ghrk::e5a_8 * A[8].dx + ghrk::e5a_9 * A[9].dx + ghrk::e5a_10 * A[10].dx + ghrk::e5a_11 * A[11].dx)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kq = ghrk::bp * p0.dy + h * (ghrk::ba_1 * A[1].dy + ghrk::ba_4 * A[4].dy + ghrk::ba_5 * A[5].dy + ghrk::ba_6 * A[6].dy + ghrk::ba_7 * A[7].dy + ghrk::ba_8 * A[8].dy + ghrk::ba_9 * A[9].dy + ghrk::ba_10 * A[10].dy + ghrk::ba_11 * A[11].dy);
+    qn.dy = q0.dy + h * kq;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(q0.dy), fabs(qn.dy)));
+    double sqr = (ghrk::e3p * p0.dy + h * (ghrk::e3a_1 * A[1].dy + ghrk::e3a_4 * A[4].dy + ghrk::e3a_5 * A[5].dy + ghrk::e3a_6 * A[6].dy + ghrk::e3a_7 * A[7].dy + ghrk::e3a_8 * A[8].dy + ghrk::e3a_9 * A[9].dy + ghrk::e3a_10 * A[10].dy + ghrk::e3a_11 * A[11].dy)) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghrk::e5p * p0.dy + h * (ghrk::e5a_1 * A[1].dy + ghrk::e5a_4 * A[4].dy + ghrk::e5a_5 * A[5].dy + ghrk::e5a_6 * A[6].dy + ghrk::e5a_7 * A[7].dy + ghrk::e5a_8 * A[8].dy + ghrk::e5a_9 * A[9].dy + ghrk::e5a_10 * A[10].dy + ghrk::e5a_11 * A[11].dy)) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].x + ghdp::b6 * A[6].x + ghdp::b7 * A[7].x + ghdp::b8 * A[8].x + ghdp::b9 * A[9].x + ghdp::b10 * A[10].x + ghdp::b11 * A[11].x + ghdp::b12 * A[12].x;
+    pn.x = p0.x + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.x), fabs(pn.x)));
+    double sqr = (kp - ghdp::bhh1 * A[1].x - ghdp::bhh2 * A[9].x - ghdp::bhh3 * A[12].x) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].x + ghdp::er6 * A[6].x + ghdp::er7 * A[7].x + ghdp::er8 * A[8].x + ghdp::er9 * A[9].x + ghdp::er10 * A[10].x + ghdp::er11 * A[11].x + ghdp::er12 * A[12].x) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].y + ghdp::b6 * A[6].y + ghdp::b7 * A[7].y + ghdp::b8 * A[8].y + ghdp::b9 * A[9].y + ghdp::b10 * A[10].y + ghdp::b11 * A[11].y + ghdp::b12 * A[12].y;
+    pn.y = p0.y + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.y), fabs(pn.y)));
+    double sqr = (kp - ghdp::bhh1 * A[1].y - ghdp::bhh2 * A[9].y - ghdp::bhh3 * A[12].y) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].y + ghdp::er6 * A[6].y + ghdp::er7 * A[7].y + ghdp::er8 * A[8].y + ghdp::er9 * A[9].y + ghdp::er10 * A[10].y + ghdp::er11 * A[11].y + ghdp::er12 * A[12].y) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].dx + ghdp::b6 * A[6].dx + ghdp::b7 * A[7].dx + ghdp::b8 * A[8].dx + ghdp::b9 * A[9].dx + ghdp::b10 * A[10].dx + ghdp::b11 * A[11].dx + ghdp::b12 * A[12].dx;
+    pn.dx = p0.dx + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.dx), fabs(pn.dx)));
+    double sqr = (kp - ghdp::bhh1 * A[1].dx - ghdp::bhh2 * A[9].dx - ghdp::bhh3 * A[12].dx) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].dx + ghdp::er6 * A[6].dx + ghdp::er7 * A[7].dx + ghdp::er8 * A[8].dx + ghdp::er9 * A[9].dx + ghdp::er10 * A[10].dx + ghdp::er11 * A[11].dx + ghdp::er12 * A[12].dx) * isk;
+    err5 += sqr * sqr;
+  }
+  {
+    const double kp = ghdp::b1 * A[1].dy + ghdp::b6 * A[6].dy + ghdp::b7 * A[7].dy + ghdp::b8 * A[8].dy + ghdp::b9 * A[9].dy + ghdp::b10 * A[10].dy + ghdp::b11 * A[11].dy + ghdp::b12 * A[12].dy;
+    pn.dy = p0.dy + h * kp;
+    const double isk = gh_rcp(atol + rtol * fmax(fabs(p0.dy), fabs(pn.dy)));
+    double sqr = (kp - ghdp::bhh1 * A[1].dy - ghdp::bhh2 * A[9].dy - ghdp::bhh3 * A[12].dy) * isk;
+    err3 += sqr * sqr;
+    sqr = (ghdp::er1 * A[1].dy + ghdp::er6 * A[6].dy + ghdp::er7 * A[7].dy + ghdp::er8 * A[8].dy + ghdp::er9 * A[9].dy + ghdp::er10 * A[10].dy + ghdp::er11 * A[11].dy + ghdp::er12 * A[12].dy) * isk;
+    err5 += sqr * sqr;
+  }
+}
+
+// Dense output (leung_dop853.c:430-474) of the accepted step at the time
+// t_old + sd h (s1 = 1 - sd), after the dense stages 14-16: per component
+// the coefficients rc1..rc8 of the continuous extension, then
+// rc1 + sd (rc2 + s1 (rc3 + sd (rc4 + s1 (rc5 + sd (rc6 + s1 (rc7 + sd rc8)))))).
+__device__ __forceinline__ void gh_rkn_dense_eval(double h, double sd, double s1, const GhVec4& q0,
+                                                 const GhVec4& p0, const GhVec4& qn,
+                                                 const GhVec4& pn, const GhVec4* A, double* yo) {
+  {
+    const double ydiff = qn.x - q0.x;
+    const double bspl = h * p0.x - ydiff;
+    const double r0 = q0.x, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.x - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.x + ghrk::dn4 * pn.x + h * (ghrk::da4_1 * A[1].x + ghrk::da4_4 * A[4].x + ghrk::da4_5 * A[5].x + ghrk::da4_6 * A[6].x + ghrk::da4_7 * A[7].x + ghrk::da4_8 * A[8].x + ghrk::da4_9 * A[9].x + ghrk::da4_10 * A[10].x + ghrk::da4_11 * A[11].x + ghrk::da4_12 * A[12].x + ghrk::da4_13 * A[13].x + ghrk::da4_14 * A[14].x + ghrk::da4_15 * A[15].x));
+    const double r5 = h * (ghrk::dp5 * p0.x + ghrk::dn5 * pn.x + h * (ghrk::da5_1 * A[1].x + ghrk::da5_4 * A[4].x + ghrk::da5_5 * A[5].x + ghrk::da5_6 * A[6].x + ghrk::da5_7 * A[7].x + ghrk::da5_8 * A[8].x + ghrk::da5_9 * A[9].x + ghrk::da5_10 * A[10].x + ghrk::da5_11 * A[11].x + ghrk::da5_12 * A[12].x + ghrk::da5_13 * A[13].x + ghrk::da5_14 * A[14].x + ghrk::da5_15 * A[15].x));
+    const double r6 = h * (ghrk::dp6 * p0.x + ghrk::dn6 * pn.x + h * (ghrk::da6_1 * A[1].x + ghrk::da6_4 * A[4].x + ghrk::da6_5 * A[5].x + ghrk::da6_6 * A[6].x + ghrk::da6_7 * A[7].x + ghrk::da6_8 * A[8].x + ghrk::da6_9 * A[9].x + ghrk::da6_10 * A[10].x + ghrk::da6_11 * A[11].x + ghrk::da6_12 * A[12].x + ghrk::da6_13 * A[13].x + ghrk::da6_14 * A[14].x + ghrk::da6_15 * A[15].x));
+    const double r7 = h * (ghrk::dp7 * p0.x + ghrk::dn7 * pn.x + h * (ghrk::da7_1 * A[1].x + ghrk::da7_4 * A[4].x + ghrk::da7_5 * A[5].x + ghrk::da7_6 * A[6].x + ghrk::da7_7 * A[7].x + ghrk::da7_8 * A[8].x + ghrk::da7_9 * A[9].x + ghrk::da7_10 * A[10].x + ghrk::da7_11 * A[11].x + ghrk::da7_12 * A[12].x + ghrk::da7_13 * A[13].x + ghrk::da7_14 * A[14].x + ghrk::da7_15 * A[15].x));
+    yo[0] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = qn.y - q0.y;
+    const double bspl = h * p0.y - ydiff;
+    const double r0 = q0.y, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.y - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.y + ghrk::dn4 * pn.y + h * (ghrk::da4_1 * A[1].y + ghrk::da4_4 * A[4].y + ghrk::da4_5 * A[5].y + ghrk::da4_6 * A[6].y + ghrk::da4_7 * A[7].y + ghrk::da4_8 * A[8].y + ghrk::da4_9 * A[9].y + ghrk::da4_10 * A[10].y + ghrk::da4_11 * A[11].y + ghrk::da4_12 * A[12].y + ghrk::da4_13 * A[13].y + ghrk::da4_14 * A[14].y + ghrk::da4_15 * A[15].y));
+    const double r5 = h * (ghrk::dp5 * p0.y + ghrk::dn5 * pn.y + h * (ghrk::da5_1 * A[1].y + ghrk::da5_4 * A[4].y + ghrk::da5_5 * A[5].y + ghrk::da5_6 * A[6].y + ghrk::da5_7 * A[7].y + ghrk::da5_8 * A[8].y + ghrk::da5_9 * A[9].y + ghrk::da5_10 * A[10].y + ghrk::da5_11 * A[11].y + ghrk::da5_12 * A[12].y + ghrk::da5_13 * A[13].y + ghrk::da5_14 * A[14].y + ghrk::da5_15 * A[15].y));
+    const double r6 = h * (ghrk::dp6 * p0.y + ghrk::dn6 * pn.y + h * (ghrk::da6_1 * A[1].y + ghrk::da6_4 * A[4].y + ghrk::da6_5 * A[5].y + ghrk::da6_6 * A[6].y + ghrk::da6_7 * A[7].y + ghrk::da6_8 * A[8].y + ghrk::da6_9 * A[9].y + ghrk::da6_10 * A[10].y + ghrk::da6_11 * A[11].y + ghrk::da6_12 * A[12].y + ghrk::da6_13 * A[13].y + ghrk::da6_14 * A[14].y + ghrk::da6_15 * A[15].y));
+    const double r7 = h * (ghrk::dp7 * p0.y + ghrk::dn7 * pn.y + h * (ghrk::da7_1 * A[1].y + ghrk::da7_4 * A[4].y + ghrk::da7_5 * A[5].y + ghrk::da7_6 * A[6].y + ghrk::da7_7 * A[7].y + ghrk::da7_8 * A[8].y + ghrk::da7_9 * A[9].y + ghrk::da7_10 * A[10].y + ghrk::da7_11 * A[11].y + ghrk::da7_12 * A[12].y + ghrk::da7_13 * A[13].y + ghrk::da7_14 * A[14].y + ghrk::da7_15 * A[15].y));
+    yo[1] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = qn.dx - q0.dx;
+    const double bspl = h * p0.dx - ydiff;
+    const double r0 = q0.dx, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.dx - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.dx + ghrk::dn4 * pn.dx + h * (ghrk::da4_1 * A[1].dx + ghrk::da4_4 * A[4].dx + ghrk::da4_5 * A[5].dx + ghrk::da4_6 * A[6].dx + ghrk::da4_7 * A[7].dx + ghrk::da4_8 * A[8].dx + ghrk::da4_9 * A[9].dx + ghrk::da4_10 * A[10].dx + ghrk::da4_11 * A[11].dx + ghrk::da4_12 * A[12].dx + ghrk::da4_13 * A[13].dx + ghrk::da4_14 * A[14].dx + ghrk::da4_15 * A[15].dx));
+    const double r5 = h * (ghrk::dp5 * p0.dx + ghrk::dn5 * pn.dx + h * (ghrk::da5_1 * A[1].dx + ghrk::da5_4 * A[4].dx + ghrk::da5_5 * A[5].dx + ghrk::da5_6 * A[6].dx + ghrk::da5_7 * A[7].dx + ghrk::da5_8 * A[8].dx + ghrk::da5_9 * A[9].dx + ghrk::da5_10 * A[10].dx + ghrk::da5_11 * A[11].dx + ghrk::da5_12 * A[12].dx + ghrk::da5_13 * A[13].dx + ghrk::da5_14 * A[14].dx + ghrk::da5_15 * A[15].dx));
+    const double r6 = h * (ghrk::dp6 * p0.dx + ghrk::dn6 * pn.dx + h * (ghrk::da6_1 * A[1].dx + ghrk::da6_4 * A[4].dx + ghrk::da6_5 * A[5].dx + ghrk::da6_6 * A[6].dx + ghrk::da6_7 * A[7].dx + ghrk::da6_8 * A[8].dx + ghrk::da6_9 * A[9].dx + ghrk::da6_10 * A[10].dx + ghrk::da6_11 * A[11].dx + ghrk::da6_12 * A[12].dx + ghrk::da6_13 * A[13].dx + ghrk::da6_14 * A[14].dx + ghrk::da6_15 * A[15].dx));
+    const double r7 = h * (ghrk::dp7 * p0.dx + ghrk::dn7 * pn.dx + h * (ghrk::da7_1 * A[1].dx + ghrk::da7_4 * A[4].dx + ghrk::da7_5 * A[5].dx + ghrk::da7_6 * A[6].dx + ghrk::da7_7 * A[7].dx + ghrk::da7_8 * A[8].dx + ghrk::da7_9 * A[9].dx + ghrk::da7_10 * A[10].dx + ghrk::da7_11 * A[11].dx + ghrk::da7_12 * A[12].dx + ghrk::da7_13 * A[13].dx + ghrk::da7_14 * A[14].dx + ghrk::da7_15 * A[15].dx));
+    yo[2] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = qn.dy - q0.dy;
+    const double bspl = h * p0.dy - ydiff;
+    const double r0 = q0.dy, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * pn.dy - bspl;
+    const double r4 = h * (ghrk::dp4 * p0.dy + ghrk::dn4 * pn.dy + h * (ghrk::da4_1 * A[1].dy + ghrk::da4_4 * A[4].dy + ghrk::da4_5 * A[5].dy + ghrk::da4_6 * A[6].dy + ghrk::da4_7 * A[7].dy + ghrk::da4_8 * A[8].dy + ghrk::da4_9 * A[9].dy + ghrk::da4_10 * A[10].dy + ghrk::da4_11 * A[11].dy + ghrk::da4_12 * A[12].dy + ghrk::da4_13 * A[13].dy + ghrk::da4_14 * A[14].dy + ghrk::da4_15 * A[15].dy));
+    const double r5 = h * (ghrk::dp5 * p0.dy + ghrk::dn5 * pn.dy + h * (ghrk::da5_1 * A[1].dy + ghrk::da5_4 * A[4].dy + ghrk::da5_5 * A[5].dy + ghrk::da5_6 * A[6].dy + ghrk::da5_7 * A[7].dy + ghrk::da5_8 * A[8].dy + ghrk::da5_9 * A[9].dy + ghrk::da5_10 * A[10].dy + ghrk::da5_11 * A[11].dy + ghrk::da5_12 * A[12].dy + ghrk::da5_13 * A[13].dy + ghrk::da5_14 * A[14].dy + ghrk::da5_15 * A[15].dy));
+    const double r6 = h * (ghrk::dp6 * p0.dy + ghrk::dn6 * pn.dy + h * (ghrk::da6_1 * A[1].dy + ghrk::da6_4 * A[4].dy + ghrk::da6_5 * A[5].dy + ghrk::da6_6 * A[6].dy + ghrk::da6_7 * A[7].dy + ghrk::da6_8 * A[8].dy + ghrk::da6_9 * A[9].dy + ghrk::da6_10 * A[10].dy + ghrk::da6_11 * A[11].dy + ghrk::da6_12 * A[12].dy + ghrk::da6_13 * A[13].dy + ghrk::da6_14 * A[14].dy + ghrk::da6_15 * A[15].dy));
+    const double r7 = h * (ghrk::dp7 * p0.dy + ghrk::dn7 * pn.dy + h * (ghrk::da7_1 * A[1].dy + ghrk::da7_4 * A[4].dy + ghrk::da7_5 * A[5].dy + ghrk::da7_6 * A[6].dy + ghrk::da7_7 * A[7].dy + ghrk::da7_8 * A[8].dy + ghrk::da7_9 * A[9].dy + ghrk::da7_10 * A[10].dy + ghrk::da7_11 * A[11].dy + ghrk::da7_12 * A[12].dy + ghrk::da7_13 * A[13].dy + ghrk::da7_14 * A[14].dy + ghrk::da7_15 * A[15].dy));
+    yo[3] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.x - p0.x;
+    const double bspl = h * A[1].x - ydiff;
+    const double r0 = p0.x, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].x - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].x + ghdp::d46 * A[6].x + ghdp::d47 * A[7].x + ghdp::d48 * A[8].x + ghdp::d49 * A[9].x + ghdp::d410 * A[10].x + ghdp::d411 * A[11].x + ghdp::d412 * A[12].x + ghdp::d413 * A[13].x + ghdp::d414 * A[14].x + ghdp::d415 * A[15].x + ghdp::d416 * A[16].x);
+    const double r5 = h * (ghdp::d51 * A[1].x + ghdp::d56 * A[6].x + ghdp::d57 * A[7].x + ghdp::d58 * A[8].x + ghdp::d59 * A[9].x + ghdp::d510 * A[10].x + ghdp::d511 * A[11].x + ghdp::d512 * A[12].x + ghdp::d513 * A[13].x + ghdp::d514 * A[14].x + ghdp::d515 * A[15].x + ghdp::d516 * A[16].x);
+    const double r6 = h * (ghdp::d61 * A[1].x + ghdp::d66 * A[6].x + ghdp::d67 * A[7].x + ghdp::d68 * A[8].x + ghdp::d69 * A[9].x + ghdp::d610 * A[10].x + ghdp::d611 * A[11].x + ghdp::d612 * A[12].x + ghdp::d613 * A[13].x + ghdp::d614 * A[14].x + ghdp::d615 * A[15].x + ghdp::d616 * A[16].x);
+    const double r7 = h * (ghdp::d71 * A[1].x + ghdp::d76 * A[6].x + ghdp::d77 * A[7].x + ghdp::d78 * A[8].x + ghdp::d79 * A[9].x + ghdp::d710 * A[10].x + ghdp::d711 * A[11].x + ghdp::d712 * A[12].x + ghdp::d713 * A[13].x + ghdp::d714 * A[14].x + ghdp::d715 * A[15].x + ghdp::d716 * A[16].x);
+    yo[4] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.y - p0.y;
+    const double bspl = h * A[1].y - ydiff;
+    const double r0 = p0.y, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].y - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].y + ghdp::d46 * A[6].y + ghdp::d47 * A[7].y + ghdp::d48 * A[8].y + ghdp::d49 * A[9].y + ghdp::d410 * A[10].y + ghdp::d411 * A[11].y + ghdp::d412 * A[12].y + ghdp::d413 * A[13].y + ghdp::d414 * A[14].y + ghdp::d415 * A[15].y + ghdp::d416 * A[16].y);
+    const double r5 = h * (ghdp::d51 * A[1].y + ghdp::d56 * A[6].y + ghdp::d57 * A[7].y + ghdp::d58 * A[8].y + ghdp::d59 * A[9].y + ghdp::d510 * A[10].y + ghdp::d511 * A[11].y + ghdp::d512 * A[12].y + ghdp::d513 * A[13].y + ghdp::d514 * A[14].y + ghdp::d515 * A[15].y + ghdp::d516 * A[16].y);
+    const double r6 = h * (ghdp::d61 * A[1].y + ghdp::d66 * A[6].y + ghdp::d67 * A[7].y + ghdp::d68 * A[8].y + ghdp::d69 * A[9].y + ghdp::d610 * A[10].y + ghdp::d611 * A[11].y + ghdp::d612 * A[12].y + ghdp::d613 * A[13].y + ghdp::d614 * A[14].y + ghdp::d615 * A[15].y + ghdp::d616 * A[16].y);
+    const double r7 = h * (ghdp::d71 * A[1].y + ghdp::d76 * A[6].y + ghdp::d77 * A[7].y + ghdp::d78 * A[8].y + ghdp::d79 * A[9].y + ghdp::d710 * A[10].y + ghdp::d711 * A[11].y + ghdp::d712 * A[12].y + ghdp::d713 * A[13].y + ghdp::d714 * A[14].y + ghdp::d715 * A[15].y + ghdp::d716 * A[16].y);
+    yo[5] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.dx - p0.dx;
+    const double bspl = h * A[1].dx - ydiff;
+    const double r0 = p0.dx, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].dx - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].dx + ghdp::d46 * A[6].dx + ghdp::d47 * A[7].dx + ghdp::d48 * A[8].dx + ghdp::d49 * A[9].dx + ghdp::d410 * A[10].dx + ghdp::d411 * A[11].dx + ghdp::d412 * A[12].dx + ghdp::d413 * A[13].dx + ghdp::d414 * A[14].dx + ghdp::d415 * A[15].dx + ghdp::d416 * A[16].dx);
+    const double r5 = h * (ghdp::d51 * A[1].dx + ghdp::d56 * A[6].dx + ghdp::d57 * A[7].dx + ghdp::d58 * A[8].dx + ghdp::d59 * A[9].dx + ghdp::d510 * A[10].dx + ghdp::d511 * A[11].dx + ghdp::d512 * A[12].dx + ghdp::d513 * A[13].dx + ghdp::d514 * A[14].dx + ghdp::d515 * A[15].dx + ghdp::d516 * A[16].dx);
+    const double r6 = h * (ghdp::d61 * A[1].dx + ghdp::d66 * A[6].dx + ghdp::d67 * A[7].dx + ghdp::d68 * A[8].dx + ghdp::d69 * A[9].dx + ghdp::d610 * A[10].dx + ghdp::d611 * A[11].dx + ghdp::d612 * A[12].dx + ghdp::d613 * A[13].dx + ghdp::d614 * A[14].dx + ghdp::d615 * A[15].dx + ghdp::d616 * A[16].dx);
+    const double r7 = h * (ghdp::d71 * A[1].dx + ghdp::d76 * A[6].dx + ghdp::d77 * A[7].dx + ghdp::d78 * A[8].dx + ghdp::d79 * A[9].dx + ghdp::d710 * A[10].dx + ghdp::d711 * A[11].dx + ghdp::d712 * A[12].dx + ghdp::d713 * A[13].dx + ghdp::d714 * A[14].dx + ghdp::d715 * A[15].dx + ghdp::d716 * A[16].dx);
+    yo[6] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+  {
+    const double ydiff = pn.dy - p0.dy;
+    const double bspl = h * A[1].dy - ydiff;
+    const double r0 = p0.dy, r1 = ydiff, r2 = bspl;
+    const double r3 = ydiff - h * A[13].dy - bspl;
+    const double r4 = h * (ghdp::d41 * A[1].dy + ghdp::d46 * A[6].dy + ghdp::d47 * A[7].dy + ghdp::d48 * A[8].dy + ghdp::d49 * A[9].dy + ghdp::d410 * A[10].dy + ghdp::d411 * A[11].dy + ghdp::d412 * A[12].dy + ghdp::d413 * A[13].dy + ghdp::d414 * A[14].dy + ghdp::d415 * A[15].dy + ghdp::d416 * A[16].dy);
+    const double r5 = h * (ghdp::d51 * A[1].dy + ghdp::d56 * A[6].dy + ghdp::d57 * A[7].dy + ghdp::d58 * A[8].dy + ghdp::d59 * A[9].dy + ghdp::d510 * A[10].dy + ghdp::d511 * A[11].dy + ghdp::d512 * A[12].dy + ghdp::d513 * A[13].dy + ghdp::d514 * A[14].dy + ghdp::d515 * A[15].dy + ghdp::d516 * A[16].dy);
+    const double r6 = h * (ghdp::d61 * A[1].dy + ghdp::d66 * A[6].dy + ghdp::d67 * A[7].dy + ghdp::d68 * A[8].dy + ghdp::d69 * A[9].dy + ghdp::d610 * A[10].dy + ghdp::d611 * A[11].dy + ghdp::d612 * A[12].dy + ghdp::d613 * A[13].dy + ghdp::d614 * A[14].dy + ghdp::d615 * A[15].dy + ghdp::d616 * A[16].dy);
+    const double r7 = h * (ghdp::d71 * A[1].dy + ghdp::d76 * A[6].dy + ghdp::d77 * A[7].dy + ghdp::d78 * A[8].dy + ghdp::d79 * A[9].dy + ghdp::d710 * A[10].dy + ghdp::d711 * A[11].dy + ghdp::d712 * A[12].dy + ghdp::d713 * A[13].dy + ghdp::d714 * A[14].dy + ghdp::d715 * A[15].dy + ghdp::d716 * A[16].dy);
+    yo[7] = r0 + sd * (r1 + s1 * (r2 + sd * (r3 + s1 * (r4 + sd * (r5 + s1 * (r6 + sd * r7))))));
+  }
+}

@@ -21,6 +21,7 @@
 #include "../../include/galpy_hip.h"
 #include "batch_dev.h"
 #include "adjoint_dev.h"
+#include "dxdv_dev.h"
 
 #define GH_EXPORT extern "C" __attribute__((visibility("default")))
 
@@ -261,7 +262,10 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
 // ---------------------------------------------------------------------------
 
 // leapfrog (bovy_symplecticode.c:132-224) and symplec4 (:250-377), for full
-// (V = GhVec3) or planar (V = GhVec2) orbits
+// (V = GhVec3) or planar (V = GhVec2) orbits; with V = GhVec6 / GhVec4 the
+// variational steppers of integrate_dxdv (leapfrog_dxdv, symplec4_dxdv,
+// bovy_symplecticode.c:1053-1268: the same drift/kick sequence on the orbit
+// and one deviation, the step estimated from the orbit alone)
 template <int METHOD, class Spec, class V>
 __global__ void __launch_bounds__(kBlock)
 gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
@@ -291,9 +295,9 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
   double dt = dt_in;
   if (dt_in == GH_DT_SENTINEL) {
     if (METHOD == 0)
-      dt = gh_leapfrog_estimate_step(F, q, p, init_dt, tt[0], rtol, atol);
+      dt = gh_leapfrog_estimate_step(F, gh_base(q), gh_base(p), init_dt, tt[0], rtol, atol);
     else
-      dt = gh_symplec4_estimate_step(F, q, p, init_dt, tt[0], rtol, atol);
+      dt = gh_symplec4_estimate_step(F, gh_base(q), gh_base(p), init_dt, tt[0], rtol, atol);
   }
   const long ndt = (long)(init_dt / dt);
   double to = tt[0];
@@ -399,6 +403,20 @@ __device__ __forceinline__ void gh_put(double* v, GhVec2 a) {
   v[0] = a.x;
   v[1] = a.y;
 }
+__device__ __forceinline__ void gh_put(double* v, GhVec6 a) {
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.dx;
+  v[4] = a.dy;
+  v[5] = a.dz;
+}
+__device__ __forceinline__ void gh_put(double* v, GhVec4 a) {
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.dx;
+  v[3] = a.dy;
+}
 
 // the job as kJobDoubles<V> doubles, in queue field order
 template <class V>
@@ -471,6 +489,17 @@ __device__ __forceinline__ void gh_job_vec(const double* __restrict__ qd, int sl
 __device__ __forceinline__ void gh_job_vec(const double* __restrict__ qd, int slot, int k,
                                            GhVec2& v) {
   v = GhVec2{qd[k * kDenseCap + slot], qd[(k + 1) * kDenseCap + slot]};
+}
+__device__ __forceinline__ void gh_job_vec(const double* __restrict__ qd, int slot, int k,
+                                           GhVec6& v) {
+  v = GhVec6{qd[k * kDenseCap + slot],       qd[(k + 1) * kDenseCap + slot],
+             qd[(k + 2) * kDenseCap + slot], qd[(k + 3) * kDenseCap + slot],
+             qd[(k + 4) * kDenseCap + slot], qd[(k + 5) * kDenseCap + slot]};
+}
+__device__ __forceinline__ void gh_job_vec(const double* __restrict__ qd, int slot, int k,
+                                           GhVec4& v) {
+  v = GhVec4{qd[k * kDenseCap + slot], qd[(k + 1) * kDenseCap + slot],
+             qd[(k + 2) * kDenseCap + slot], qd[(k + 3) * kDenseCap + slot]};
 }
 template <class V>
 __device__ __forceinline__ V gh_job_vec(const double* __restrict__ qd, int slot, int k) {
@@ -551,7 +580,7 @@ __device__ __forceinline__ void gh_job_run(const F& T, const double* __restrict_
                         gh_job_vec<V>(qd, slot, 2 * D), gh_job_vec<V>(qd, slot, 3 * D), A, yo);
     }
     done++;
-    GhPhase<V>::store(out + (long)N * done, GhPhase<V>::q(yo), GhPhase<V>::p(yo));
+    GhPhase<V>::store(out + (long)N * done, gh_vec_at<V>(yo), gh_vec_at<V>(yo + N / 2));
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
   }
 }
@@ -827,7 +856,29 @@ struct GhPotHandle {
   GhPotTable* dev;
   int npot;
   int types[GH_MAX_POT];
+  int nonaxi[GH_MAX_POT];  // component exerts a phi torque (bar, triaxial halo)
 };
+
+void fill_handle(const GhPotTable& table, GhPotHandle* h) {
+  h->npot = table.npot;
+  for (int i = 0; i < GH_MAX_POT; i++) {
+    h->types[i] = table.comp[i].type;
+    h->nonaxi[i] = table.comp[i].nonaxi;
+  }
+}
+
+// Components with a Hessian on the device (the adjoint's): time-independent
+// and axisymmetric -- not the bar, not a triaxial logarithmic halo.
+bool hessian_capable(const GhPotHandle* h) {
+  for (int i = 0; i < h->npot; i++) {
+    const int c = h->types[i];
+    if (c != GH_POWERSPHCUTOFF && c != GH_MIYAMOTONAGAI && c != GH_NFW && c != GH_HERNQUIST &&
+        c != GH_LOGHALO)
+      return false;
+    if (h->nonaxi[i]) return false;
+  }
+  return true;
+}
 
 // DOP853 launch at WPE waves per SIMD: optional pilot + longest-first
 // order, then the persistent kernel.  The pilot's cost keys and the order
@@ -926,15 +977,21 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
     case 6: {
       int dev = 0;
       GH_CHECK(hipGetDevice(&dev));
-      if (std::is_same<V, GhVec2>::value)
+      if constexpr (std::is_same<V, GhVec3>::value) {
+        const bool two = dop_waves_forced() != 1;
+        return two ? launch_dop853<Spec, 2, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
+                                               d_result, d_err, d_stats, cancel, dev, stream)
+                   : launch_dop853<Spec, 1, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
+                                               d_result, d_err, d_stats, cancel, dev, stream);
+      } else if constexpr (std::is_same<V, GhVec2>::value) {
         return launch_dop853<Spec, 2, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
                                          d_result, d_err, d_stats, cancel, dev, stream);
-      const bool two = dop_waves_forced() != 1;
-      return two ? launch_dop853<Spec, 2, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
-                                             d_result, d_err, d_stats, cancel, dev, stream)
-                 : launch_dop853<Spec, 1, GhVec3>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol,
-                                                  atol, d_result, d_err, d_stats, cancel, dev,
-                                                  stream);
+      } else {
+        // variational states: twice the registers and LDS per job, one wave
+        // per SIMD (the LDS queue allows four to six workgroups per CU)
+        return launch_dop853<Spec, 1, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
+                                         d_result, d_err, d_stats, cancel, dev, stream);
+      }
     }
     default:
       set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, 6 dop853)",
@@ -982,18 +1039,51 @@ int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indi
 #undef GH_LAUNCH
 }
 
+// integrate_dxdv: V = GhVec6 (full orbits, rows of 12) or GhVec4 (planar,
+// rows of 8); MWPotential2014 has a specialised kernel, any other
+// composition with device Hessians runs the run-time dispatch.
+template <class V>
+int launch_integrate_dxdv(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                          const GhPotHandle* h, double dt, double rtol, double atol,
+                          double* d_result, int* d_err, int odeint_type, double* d_stats,
+                          hipStream_t stream) {
+  if (nobj == 0) return 0;
+  if (nobj < 0 || nt < 2) {
+    set_error("invalid sizes nobj=%d nt=%d (need nt >= 2)", nobj, nt);
+    return -1;
+  }
+  if (!hessian_capable(h)) {
+    set_error("integrate_dxdv needs time-independent, axisymmetric components with a Hessian "
+              "(PowerSphericalPotentialwCutoff, MiyamotoNagai, NFW, Hernquist, axisymmetric "
+              "LogarithmicHalo)");
+    return -1;
+  }
+  using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
+  if (types_are(h, {GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW}))
+    return launch_spec<MW2014, V>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result,
+                                  d_err, odeint_type, d_stats, stream);
+  return launch_spec<GhSpec<>, V>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result,
+                                  d_err, odeint_type, d_stats, stream);
+}
+
 // SIGINT during the blocking host call sets the polled flag, as galpy's
 // handler sets `interrupted` (bovy_symplecticode.c:38-45).
 void on_sigint(int) {
   if (g_irq_host) *g_irq_host = 1;
 }
 
-// Blocking host-buffer integration of full (planar = false, rows of 6) or
-// planar (rows of 4) orbits, streamed through the GPU in chunks.
+// What a host call integrates: galpy's full or planar orbits, or their
+// variational equations (integrate_dxdv).
+enum GhMode { GH_FULL = 0, GH_PLANAR = 1, GH_DXDV_FULL = 2, GH_DXDV_PLANAR = 3 };
+
+// Blocking host-buffer integration, streamed through the GPU in chunks of
+// orbits.  Rows: (R,vR,vT,z,vz,phi), (R,vR,vT,phi), or the rectangular
+// (q, p | dq, dp) of 12 / 8 doubles.
 int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npot, int* pot_type,
                    double* pot_args, double dt, double rtol, double atol, double* result,
-                   int* err, int odeint_type, void (*cb)(void), bool planar) {
-  const size_t N = planar ? 4 : 6;  // doubles per phase-space row
+                   int* err, int odeint_type, void (*cb)(void), int mode) {
+  const bool planar = mode == GH_PLANAR || mode == GH_DXDV_PLANAR;
+  const size_t N = mode == GH_FULL ? 6 : mode == GH_PLANAR ? 4 : mode == GH_DXDV_FULL ? 12 : 8;
   if (nobj <= 0) return 0;
   if (nt < 2) {
     set_error("nt=%d: need at least two output times", nt);
@@ -1011,8 +1101,7 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
   void* d_table = nullptr;
   GhPotHandle handle;
   handle.dev = nullptr;
-  handle.npot = table.npot;
-  for (int i = 0; i < GH_MAX_POT; i++) handle.types[i] = table.comp[i].type;
+  fill_handle(table, &handle);
   double *d_yo = nullptr, *d_t = nullptr, *d_res = nullptr;
   int* d_err = nullptr;
   int rc = 0;
@@ -1046,11 +1135,25 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
       if (hipMemcpyAsync(d_yo, yo + N * o0, (size_t)n * N * 8, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
       if (indiv_t && hipMemcpyAsync(d_t, t + (size_t)nt * o0, (size_t)n * nt * 8,
                                     hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
-      const int lrc =
-          planar ? launch_integrate<GhVec2>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
-                                            atol, d_res, d_err, odeint_type, nullptr, stream)
-                 : launch_integrate<GhVec3>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
-                                            atol, d_res, d_err, odeint_type, nullptr, stream);
+      int lrc;
+      switch (mode) {
+        case GH_FULL:
+          lrc = launch_integrate<GhVec3>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol, atol,
+                                         d_res, d_err, odeint_type, nullptr, stream);
+          break;
+        case GH_PLANAR:
+          lrc = launch_integrate<GhVec2>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol, atol,
+                                         d_res, d_err, odeint_type, nullptr, stream);
+          break;
+        case GH_DXDV_FULL:
+          lrc = launch_integrate_dxdv<GhVec6>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
+                                              atol, d_res, d_err, odeint_type, nullptr, stream);
+          break;
+        default:
+          lrc = launch_integrate_dxdv<GhVec4>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
+                                              atol, d_res, d_err, odeint_type, nullptr, stream);
+          break;
+      }
       if (lrc != 0) { rc = -1; break; }
       if (hipMemcpyAsync(result + N * nt * o0, d_res, (size_t)n * nt * N * 8,
                          hipMemcpyDeviceToHost, stream) != hipSuccess ||
@@ -1086,7 +1189,7 @@ static void integrate_host_abi(const char* name, int nobj, double* yo, int nt, d
                                int indiv_t, int npot, int* pot_type, double* pot_args,
                                void* pot_tfuncs, double dt, double rtol, double atol,
                                double* result, int* err, int odeint_type, void (*cb)(void),
-                               bool planar) {
+                               int mode) {
   g_errbuf[0] = 0;
   int rc;
   if (pot_tfuncs != nullptr) {
@@ -1095,7 +1198,7 @@ static void integrate_host_abi(const char* name, int nobj, double* yo, int nt, d
   } else {
     for (int i = 0; i < nobj; i++) err[i] = 0;
     rc = host_integrate(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol,
-                        result, err, odeint_type, cb, planar);
+                        result, err, odeint_type, cb, mode);
   }
   if (rc != 0) {
     for (int i = 0; i < nobj; i++) err[i] = -1;
@@ -1108,7 +1211,7 @@ GH_EXPORT void integrateFullOrbit(int nobj, double* yo, int nt, double* t, int i
                                   double rtol, double atol, double* result, int* err,
                                   int odeint_type, void (*cb)(void)) {
   integrate_host_abi("integrateFullOrbit", nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args,
-                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, cb, false);
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, cb, GH_FULL);
 }
 
 GH_EXPORT void integratePlanarOrbit(int nobj, double* yo, int nt, double* t, int indiv_t,
@@ -1116,7 +1219,60 @@ GH_EXPORT void integratePlanarOrbit(int nobj, double* yo, int nt, double* t, int
                                     double dt, double rtol, double atol, double* result, int* err,
                                     int odeint_type, void (*cb)(void)) {
   integrate_host_abi("integratePlanarOrbit", nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args,
-                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, cb, true);
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, cb, GH_PLANAR);
+}
+
+GH_EXPORT void integrateFullOrbit_dxdv(double* yo, int nt, double* t, int npot, int* pot_type,
+                                       double* pot_args, void* pot_tfuncs, double dt, double rtol,
+                                       double atol, double* result, int* err, int odeint_type) {
+  integrate_host_abi("integrateFullOrbit_dxdv", 1, yo, nt, t, 0, npot, pot_type, pot_args,
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, nullptr, GH_DXDV_FULL);
+}
+
+GH_EXPORT void integratePlanarOrbit_dxdv(double* yo, int nt, double* t, int npot, int* pot_type,
+                                         double* pot_args, void* pot_tfuncs, double dt,
+                                         double rtol, double atol, double* result, int* err,
+                                         int odeint_type, void (*cb)(void)) {
+  (void)cb;  // galpy's integratePlanarOrbit_dxdv takes it and does not call it
+  integrate_host_abi("integratePlanarOrbit_dxdv", 1, yo, nt, t, 0, npot, pot_type, pot_args,
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, nullptr,
+                     GH_DXDV_PLANAR);
+}
+
+GH_EXPORT void gh_integrate_dxdv(int nobj, int dim, double* yo, int nt, double* t, int indiv_t,
+                                 int npot, int* pot_type, double* pot_args, double dt,
+                                 double rtol, double atol, double* result, int* err,
+                                 int odeint_type) {
+  if (dim != 2 && dim != 3) {
+    g_errbuf[0] = 0;
+    set_error("gh_integrate_dxdv: dim must be 2 (planar) or 3, got %d", dim);
+    for (int i = 0; i < nobj; i++) err[i] = -1;
+    return;
+  }
+  integrate_host_abi("gh_integrate_dxdv", nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args,
+                     nullptr, dt, rtol, atol, result, err, odeint_type, nullptr,
+                     dim == 3 ? GH_DXDV_FULL : GH_DXDV_PLANAR);
+}
+
+GH_EXPORT int gh_integrate_dxdv_hip(int nobj, int dim, double* d_yo, int nt, const double* d_t,
+                                    int indiv_t, const void* pot, double dt, double rtol,
+                                    double atol, double* d_result, int* d_err, int odeint_type,
+                                    double* d_stats, void* stream) {
+  g_errbuf[0] = 0;
+  if (pot == nullptr) {
+    set_error("potential handle is NULL");
+    return -1;
+  }
+  if (dim == 3)
+    return launch_integrate_dxdv<GhVec6>(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot,
+                                         dt, rtol, atol, d_result, d_err, odeint_type, d_stats,
+                                         (hipStream_t)stream);
+  if (dim == 2)
+    return launch_integrate_dxdv<GhVec4>(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot,
+                                         dt, rtol, atol, d_result, d_err, odeint_type, d_stats,
+                                         (hipStream_t)stream);
+  set_error("gh_integrate_dxdv_hip: dim must be 2 (planar) or 3, got %d", dim);
+  return -1;
 }
 
 GH_EXPORT int integrateFullOrbit_hip(int nobj, double* d_yo, int nt, const double* d_t,
@@ -1155,8 +1311,7 @@ static int upload_potential(int npot, const int* pot_type, const double* pot_arg
   GH_CHECK(hipMemcpy(d, &table, sizeof(GhPotTable), hipMemcpyHostToDevice));
   GhPotHandle* h = new GhPotHandle;
   h->dev = (GhPotTable*)d;
-  h->npot = table.npot;
-  for (int i = 0; i < GH_MAX_POT; i++) h->types[i] = table.comp[i].type;
+  fill_handle(table, h);
   *pot = h;
   return 0;
 }
@@ -1316,15 +1471,6 @@ gh_adjoint_kernel(int n, const double* __restrict__ wT, const double* __restrict
   }
 }
 
-bool adjoint_capable(const GhPotHandle* h) {
-  for (int i = 0; i < h->npot; i++) {
-    const int c = h->types[i];
-    if (c != GH_POWERSPHCUTOFF && c != GH_MIYAMOTONAGAI && c != GH_NFW && c != GH_HERNQUIST &&
-        c != GH_LOGHALO)
-      return false;
-  }
-  return true;
-}
 
 template <class Spec>
 void launch_flow(int method, dim3 grid, hipStream_t st, int n, const double* w0, double* wT,
@@ -1381,9 +1527,10 @@ GH_EXPORT int gh_adjoint_hip(int n, const double* d_wT, const double* d_gT, doub
     set_error("gh_adjoint_hip supports leapfrog (0) and symplec4 (3), got %d", method);
     return -1;
   }
-  if (!adjoint_capable(h)) {
-    set_error("the adjoint needs time-independent components with a Hessian "
-              "(PowerSphericalPotentialwCutoff, MiyamotoNagai, NFW, Hernquist, LogarithmicHalo)");
+  if (!hessian_capable(h)) {
+    set_error("the adjoint needs time-independent, axisymmetric components with a Hessian "
+              "(PowerSphericalPotentialwCutoff, MiyamotoNagai, NFW, Hernquist, axisymmetric "
+              "LogarithmicHalo)");
     return -1;
   }
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));

@@ -207,13 +207,15 @@ def emit(vals):
     L.extend(consts)
     w("}  // namespace ghrk")
     w("")
-    # the same functions for full (x, y, z) and planar (x, y) states: the
-    # position/velocity split is per component, so the planar sums are the
-    # spatial ones with the z component left out
+    # the same functions for full (x, y, z) and planar (x, y) states, and for
+    # the variational states of integrate_dxdv (orbit + deviation): the
+    # position/velocity split is per component, so every variant is the same
+    # sums over a different set of components
     def ghdp(key):
         return "ghdp::" + nm[key]
 
-    for V, C in (("GhVec3", ("x", "y", "z")), ("GhVec2", ("x", "y"))):
+    for V, C in (("GhVec3", ("x", "y", "z")), ("GhVec2", ("x", "y")),
+                 ("GhVec6", ("x", "y", "z", "dx", "dy", "dz")), ("GhVec4", ("x", "y", "dx", "dy"))):
         # stage arguments: positions only -- the stage velocities V_s enter every
         # later sum through the expanded coefficients, so they are never formed
         for s in MAIN_STAGES + DENSE_STAGES:

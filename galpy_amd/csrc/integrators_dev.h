@@ -33,12 +33,56 @@ struct GhVec2 {
   double x, y;
 };
 
+// Position (or velocity, acceleration) of an orbit together with one
+// phase-space deviation, for the variational equations of galpy's
+// integrate_dxdv (integrateFullOrbit.c:1234-1306, integratePlanarOrbit.c:940-1012):
+// (x, y, z | dx, dy, dz) for full orbits, (x, y | dx, dy) for planar ones.
+struct GhVec6 {
+  double x, y, z, dx, dy, dz;
+};
+
+struct GhVec4 {
+  double x, y, dx, dy;
+};
+
+// a vector from consecutive doubles in component order (x, y, z, dx, ...)
+__device__ __forceinline__ void gh_vec_at(const double* w, GhVec3& v) { v = GhVec3{w[0], w[1], w[2]}; }
+__device__ __forceinline__ void gh_vec_at(const double* w, GhVec2& v) { v = GhVec2{w[0], w[1]}; }
+__device__ __forceinline__ void gh_vec_at(const double* w, GhVec6& v) {
+  v = GhVec6{w[0], w[1], w[2], w[3], w[4], w[5]};
+}
+__device__ __forceinline__ void gh_vec_at(const double* w, GhVec4& v) {
+  v = GhVec4{w[0], w[1], w[2], w[3]};
+}
+template <class V>
+__device__ __forceinline__ V gh_vec_at(const double* w) {
+  V v;
+  gh_vec_at(w, v);
+  return v;
+}
+
+// the orbit part of a state (the step estimators look at the orbit only,
+// bovy_symplecticode.c:1081-1084)
+__device__ __forceinline__ GhVec3 gh_base(GhVec3 v) { return v; }
+__device__ __forceinline__ GhVec2 gh_base(GhVec2 v) { return v; }
+__device__ __forceinline__ GhVec3 gh_base(GhVec6 v) { return GhVec3{v.x, v.y, v.z}; }
+__device__ __forceinline__ GhVec2 gh_base(GhVec4 v) { return GhVec2{v.x, v.y}; }
+
 __device__ __forceinline__ GhVec3 gh_drift(GhVec3 q, GhVec3 p, double h) {
   return GhVec3{q.x + h * p.x, q.y + h * p.y, q.z + h * p.z};
 }
 
 __device__ __forceinline__ GhVec2 gh_drift(GhVec2 q, GhVec2 p, double h) {
   return GhVec2{q.x + h * p.x, q.y + h * p.y};
+}
+
+__device__ __forceinline__ GhVec6 gh_drift(GhVec6 q, GhVec6 p, double h) {
+  return GhVec6{q.x + h * p.x,   q.y + h * p.y,   q.z + h * p.z,
+                q.dx + h * p.dx, q.dy + h * p.dy, q.dz + h * p.dz};
+}
+
+__device__ __forceinline__ GhVec4 gh_drift(GhVec4 q, GhVec4 p, double h) {
+  return GhVec4{q.x + h * p.x, q.y + h * p.y, q.dx + h * p.dx, q.dy + h * p.dy};
 }
 
 template <class F>
@@ -260,6 +304,62 @@ struct GhPhase<GhVec3> {
   }
   __device__ static __forceinline__ void store(double* __restrict__ out, GhVec3 q, GhVec3 p) {
     gh_store_cyl(out, q.x, q.y, q.z, p.x, p.y, p.z);
+  }
+};
+
+// Variational states: rows are galpy's rectangular (q, p | dq, dp)
+// (save_qp_dxdv, bovy_symplecticode.c:79-90) -- no coordinate transform.
+template <>
+struct GhPhase<GhVec6> {
+  static constexpr int N = 12;
+  __device__ static __forceinline__ void to_rect(const double* v, double* w) {
+#pragma unroll
+    for (int k = 0; k < N; k++) w[k] = v[k];
+  }
+  __device__ static __forceinline__ GhVec6 q(const double* w) {
+    return GhVec6{w[0], w[1], w[2], w[6], w[7], w[8]};
+  }
+  __device__ static __forceinline__ GhVec6 p(const double* w) {
+    return GhVec6{w[3], w[4], w[5], w[9], w[10], w[11]};
+  }
+  __device__ static __forceinline__ void set(double* w, GhVec6 q, GhVec6 p) {
+    w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = p.x; w[4] = p.y; w[5] = p.z;
+    w[6] = q.dx; w[7] = q.dy; w[8] = q.dz; w[9] = p.dx; w[10] = p.dy; w[11] = p.dz;
+  }
+  __device__ static __forceinline__ void store(double* __restrict__ out, GhVec6 q, GhVec6 p) {
+    double2* o = reinterpret_cast<double2*>(out);
+    o[0] = make_double2(q.x, q.y);
+    o[1] = make_double2(q.z, p.x);
+    o[2] = make_double2(p.y, p.z);
+    o[3] = make_double2(q.dx, q.dy);
+    o[4] = make_double2(q.dz, p.dx);
+    o[5] = make_double2(p.dy, p.dz);
+  }
+};
+
+template <>
+struct GhPhase<GhVec4> {
+  static constexpr int N = 8;
+  __device__ static __forceinline__ void to_rect(const double* v, double* w) {
+#pragma unroll
+    for (int k = 0; k < N; k++) w[k] = v[k];
+  }
+  __device__ static __forceinline__ GhVec4 q(const double* w) {
+    return GhVec4{w[0], w[1], w[4], w[5]};
+  }
+  __device__ static __forceinline__ GhVec4 p(const double* w) {
+    return GhVec4{w[2], w[3], w[6], w[7]};
+  }
+  __device__ static __forceinline__ void set(double* w, GhVec4 q, GhVec4 p) {
+    w[0] = q.x; w[1] = q.y; w[2] = p.x; w[3] = p.y;
+    w[4] = q.dx; w[5] = q.dy; w[6] = p.dx; w[7] = p.dy;
+  }
+  __device__ static __forceinline__ void store(double* __restrict__ out, GhVec4 q, GhVec4 p) {
+    double2* o = reinterpret_cast<double2*>(out);
+    o[0] = make_double2(q.x, q.y);
+    o[1] = make_double2(p.x, p.y);
+    o[2] = make_double2(q.dx, q.dy);
+    o[3] = make_double2(p.dx, p.dy);
   }
 };
 

@@ -228,3 +228,95 @@ def eval_device(table, R, z, phi, t, stream=None):
     )
     _lib.check(rc, "gh_eval_hip")
     return tuple(outs)
+
+
+# ---------------------------------------------------------------------------
+# integrate_dxdv: the orbit plus one phase-space deviation (rectangular)
+# ---------------------------------------------------------------------------
+def _dxdv_host(dim, pot, yo, dyo, t, int_method, dt, rtol, atol):
+    """One call of gh_integrate_dxdv (or the single-orbit drop-in symbol) for
+    rectangular states yo [N, 2 dim] and deviations dyo [N, 2 dim]."""
+    lib = _lib.load()
+    single = numpy.ndim(yo) == 1
+    yo = numpy.atleast_2d(numpy.asarray(yo, dtype=numpy.float64))
+    dyo = numpy.atleast_2d(numpy.asarray(dyo, dtype=numpy.float64))
+    ncol = 2 * dim
+    if yo.shape[-1] != ncol or dyo.shape != yo.shape:
+        raise ValueError(f"integrate_dxdv takes states and deviations of {ncol} rectangular coordinates")
+    nobj = len(yo)
+    w = numpy.require(numpy.hstack((yo, dyo)), dtype=numpy.float64, requirements=["C", "W"])
+    rtol, atol = _parse_tol(rtol, atol)
+    npot, pot_type, pot_args, _ = (parse_pot if dim == 3 else parse_pot_planar)(pot)
+    code = _parse_integrator(int_method)
+    t = numpy.require(t, dtype=numpy.float64, requirements=["C", "W"])
+    nt = t.shape[-1]
+    result = numpy.empty((nobj, nt, 2 * ncol))
+    err = numpy.zeros(nobj, dtype=numpy.int32)
+    dtv = DT_SENTINEL if dt is None else float(dt)
+    if single and t.ndim == 1:
+        # the drop-in symbols: galpy's one-orbit signature
+        if dim == 3:
+            lib.integrateFullOrbit_dxdv(w[0], nt, t, npot, pot_type, pot_args, None, dtv, rtol, atol,
+                                        result[0], err, code)
+        else:
+            lib.integratePlanarOrbit_dxdv(w[0], nt, t, npot, pot_type, pot_args, None, dtv, rtol,
+                                          atol, result[0], err, code, None)
+    else:
+        lib.gh_integrate_dxdv(nobj, dim, w, nt, t, int(t.ndim > 1), npot, pot_type, pot_args, dtv,
+                              rtol, atol, result, err, code)
+    if numpy.any(err == -10):
+        raise KeyboardInterrupt("Orbit integration interrupted by CTRL-C (SIGINT)")
+    if numpy.any(err == -1):
+        raise RuntimeError(f"HIP orbit integration failed: {_lib.last_error()}")
+    if single:
+        return (result[0], err[0])
+    return (result, err)
+
+
+def integrateFullOrbit_dxdv_hip(pot, yo, dyo, t, int_method, dt=None, rtol=None, atol=None):
+    """galpy's ``integrateFullOrbit_dxdv_c`` (integrateFullOrbit.py:752-858):
+    the orbit ``yo`` = (x,y,z,vx,vy,vz) and the deviation ``dyo`` =
+    (dx,dy,dz,dvx,dvy,dvz), rectangular, integrated together; returns
+    ``(result, err)`` with rows of 12.  ``yo``/``dyo`` may also be ``[N,6]``
+    (one call for all orbits; galpy maps the one-orbit call)."""
+    return _dxdv_host(3, pot, yo, dyo, t, int_method, dt, rtol, atol)
+
+
+def integratePlanarOrbit_dxdv_hip(pot, yo, dyo, t, int_method, dt=None, rtol=None, atol=None):
+    """galpy's ``integratePlanarOrbit_dxdv_c`` (integratePlanarOrbit.py:1124-1225):
+    (x,y,vx,vy) and (dx,dy,dvx,dvy), rows of 8."""
+    return _dxdv_host(2, pot, yo, dyo, t, int_method, dt, rtol, atol)
+
+
+def integrate_dxdv_device(table, w, t, int_method, rtol=None, atol=None, dt=None, result=None,
+                          err=None, stats=None, stream=None):
+    """gh_integrate_dxdv_hip on device tensors: ``w`` is ``[N,12]`` (full
+    orbits, ``table`` from the full packing) or ``[N,8]`` (planar, a planar
+    table), rectangular (q, p | dq, dp); ``result`` ``[N,nt,12 or 8]``.
+    Asynchronous on ``stream``; returns ``(result, err, stats)``."""
+    import torch
+
+    assert w.is_cuda and w.dtype == torch.float64 and w.is_contiguous() and w.dim() == 2
+    ncol = w.shape[-1]
+    assert ncol in (8, 12)
+    dim = 3 if ncol == 12 else 2
+    if (dim == 2) != getattr(table, "planar", False):
+        raise ValueError("planar deviations need a PotentialTable(pot, planar=True), full ones a full one")
+    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+    nobj, nt = w.shape[0], t.shape[-1]
+    if result is None:
+        result = torch.empty((nobj, nt, ncol), dtype=torch.float64, device=w.device)
+    if err is None:
+        err = torch.zeros(nobj, dtype=torch.int32, device=w.device)
+    assert result.shape == (nobj, nt, ncol) and result.is_contiguous()
+    rtol, atol = _parse_tol(rtol, atol)
+    if stream is None:
+        stream = torch.cuda.current_stream(w.device)
+    code = METHOD_CODES[int_method] if isinstance(int_method, str) else int(int_method)
+    rc = _lib.load().gh_integrate_dxdv_hip(
+        nobj, dim, w.data_ptr(), nt, t.data_ptr(), int(t.dim() > 1), table.ptr,
+        DT_SENTINEL if dt is None else float(dt), rtol, atol, result.data_ptr(), err.data_ptr(), code,
+        _ptr(stats), stream.cuda_stream,
+    )
+    _lib.check(rc, "gh_integrate_dxdv_hip")
+    return result, err, stats

@@ -19,7 +19,13 @@ the C-ABI overwrites its ``yo`` argument.
 
 import numpy
 
-from .integrate import METHOD_CODES, integrateFullOrbit_hip, integratePlanarOrbit_hip
+from .integrate import (
+    METHOD_CODES,
+    integrateFullOrbit_dxdv_hip,
+    integrateFullOrbit_hip,
+    integratePlanarOrbit_dxdv_hip,
+    integratePlanarOrbit_hip,
+)
 from .potential import evaluatePotentials, flatten
 
 _METHODS = tuple(METHOD_CODES)
@@ -101,6 +107,79 @@ class Orbit:
         self.t = t
         self._pot = pot
         return None
+
+    def integrate_dxdv(self, dxdv, t, pot, method="dop853_hip", rectIn=False, rectOut=False,
+                       dt=None, rtol=None, atol=None):
+        """Integrate the orbits together with one phase-space deviation each
+        (Orbits.py:2276-2470 for the HIP methods): ``dxdv`` is ``[N, phasedim]``
+        (or ``[phasedim]``), cylindrical (dR,dvR,dvT,dz,dvz,dphi) /
+        (dR,dvR,dvT,dphi) unless ``rectIn``; the deviation along the orbit
+        comes back from ``getOrbit_dxdv`` cylindrical unless ``rectOut``
+        (integrateFullOrbit.py:926-1037, integratePlanarOrbit.py:1452-1540)."""
+        if method.lower() not in _METHODS:
+            raise ValueError(
+                f"{method} is not a method of this engine; use one of {', '.join(_METHODS)}"
+            )
+        t = numpy.array(t, dtype=float)
+        if t.ndim != 1 or len(t) < 2:
+            raise ValueError("integrate_dxdv needs a 1D array of at least two output times")
+        steps = numpy.diff(t)
+        if not numpy.allclose(steps, steps[0], rtol=1e-10, atol=1e-14):
+            raise ValueError("Input time array must be equally spaced")
+        if not _check_integrate_dt(t, dt):
+            raise ValueError("dt input (integrator stepsize) for Orbit.integrate must be an integer divisor of the output stepsize")
+        flatten(pot)
+        d = numpy.atleast_2d(numpy.array(dxdv, dtype=float))
+        if d.shape[-1] != self.phasedim():
+            raise ValueError(f"dxdv must have {self.phasedim()} components per orbit")
+        if len(d) == 1 and len(self.vxvv) > 1:
+            d = numpy.repeat(d, len(self.vxvv), axis=0)
+        y = self.vxvv
+        if self._planar:
+            R, vR, vT, phi = y.T
+            yr = numpy.stack(_cyl_to_rect_state(R, vR, vT, phi), axis=1)
+            dr = d if rectIn else numpy.stack(_cyl_to_rect_dev(R, vR, vT, phi, *d.T), axis=1)
+            out, err = integratePlanarOrbit_dxdv_hip(pot, yr, dr, t, method.lower(), dt=dt,
+                                                     rtol=rtol, atol=atol)
+            x, yy, vx, vy = numpy.moveaxis(out[..., :4], -1, 0)
+            R, vR, vT, phi = _rect_to_cyl_state(x, yy, vx, vy, wrap=True)
+            base = numpy.stack([R, vR, vT, phi], axis=-1)
+            dev = out[..., 4:]
+            if not rectOut:
+                dR, dvR, dvT, dphi = _rect_to_cyl_dev(R, vR, vT, phi, *numpy.moveaxis(dev, -1, 0))
+                dev = numpy.stack([dR, dvR, dvT, dphi], axis=-1)
+        else:
+            R, vR, vT, z, vz, phi = y.T
+            x, yy, vx, vy = _cyl_to_rect_state(R, vR, vT, phi)
+            yr = numpy.stack([x, yy, z, vx, vy, vz], axis=1)
+            if rectIn:
+                dr = d
+            else:
+                dR, dvR, dvT, dz, dvz, dphi = d.T
+                dx, dy, dvx, dvy = _cyl_to_rect_dev(R, vR, vT, phi, dR, dvR, dvT, dphi)
+                dr = numpy.stack([dx, dy, dz, dvx, dvy, dvz], axis=1)
+            out, err = integrateFullOrbit_dxdv_hip(pot, yr, dr, t, method.lower(), dt=dt,
+                                                   rtol=rtol, atol=atol)
+            x, yy, z, vx, vy, vz = numpy.moveaxis(out[..., :6], -1, 0)
+            R, vR, vT, phi = _rect_to_cyl_state(x, yy, vx, vy, wrap=False)
+            base = numpy.stack([R, vR, vT, z, vz, phi], axis=-1)
+            dev = out[..., 6:]
+            if not rectOut:
+                dx, dy, dz, dvx, dvy, dvz = numpy.moveaxis(dev, -1, 0)
+                dR, dvR, dvT, dphi = _rect_to_cyl_dev(R, vR, vT, phi, dx, dy, dvx, dvy)
+                dev = numpy.stack([dR, dvR, dvT, dz, dvz, dphi], axis=-1)
+        self.orbit = base
+        self.orbit_dxdv = dev
+        self._err = err
+        self.t = t
+        self._pot = pot
+        return None
+
+    def getOrbit_dxdv(self):
+        """The deviation along the orbit from ``integrate_dxdv`` (Orbits.py getOrbit_dxdv)."""
+        if getattr(self, "orbit_dxdv", None) is None:
+            raise AttributeError("Integrate orbit with integrate_dxdv first")
+        return self._out(self.orbit_dxdv.copy())
 
     def _check_integrated(self):
         if self.orbit is None:
@@ -252,3 +331,36 @@ class Orbit:
 
 
 Orbits = Orbit
+
+
+# Coordinate maps of integrate_dxdv (integratePlanarOrbit.py:1452-1476 and
+# 1517-1540; integrateFullOrbit.py:926-946 and 1003-1037 with
+# coords.cyl_to_rect_jac, util/coords.py:1637-1690, inverted in closed form).
+def _cyl_to_rect_state(R, vR, vT, phi):
+    c, s = numpy.cos(phi), numpy.sin(phi)
+    return R * c, R * s, vR * c - vT * s, vR * s + vT * c
+
+
+def _rect_to_cyl_state(x, y, vx, vy, wrap):
+    R = numpy.sqrt(x**2 + y**2)
+    if wrap:  # planar: phi in [0, 2 pi) from arccos, as integratePlanarOrbit.py:1519-1520
+        phi = numpy.arccos(x / R)
+        phi = numpy.where(y < 0.0, 2.0 * numpy.pi - phi, phi)
+    else:  # full: atan2, as coords.rect_to_cyl
+        phi = numpy.arctan2(y, x)
+    c, s = numpy.cos(phi), numpy.sin(phi)
+    return R, vx * c + vy * s, vy * c - vx * s, phi
+
+
+def _cyl_to_rect_dev(R, vR, vT, phi, dR, dvR, dvT, dphi):
+    c, s = numpy.cos(phi), numpy.sin(phi)
+    return (c * dR - R * s * dphi, s * dR + R * c * dphi,
+            c * dvR - s * dvT - (vR * s + vT * c) * dphi,
+            s * dvR + c * dvT + (vR * c - vT * s) * dphi)
+
+
+def _rect_to_cyl_dev(R, vR, vT, phi, dx, dy, dvx, dvy):
+    c, s = numpy.cos(phi), numpy.sin(phi)
+    dR = c * dx + s * dy
+    dphi = (c * dy - s * dx) / R
+    return dR, c * dvx + s * dvy + vT * dphi, c * dvy - s * dvx - vR * dphi, dphi

@@ -63,6 +63,46 @@ void integratePlanarOrbit(int nobj, double *yo, int nt, double *t, int indiv_t, 
                           double rtol, double atol, double *result, int *err, int odeint_type,
                           void (*cb)(void));
 
+/* Replaces integrateFullOrbit.c:1234-1306 (`integrateFullOrbit_dxdv`), same
+ * signature and meaning: ONE orbit and one phase-space deviation, all
+ * rectangular -- yo[12] = (x,y,z,vx,vy,vz, dx,dy,dz,dvx,dvy,dvz), result[nt*12]
+ * rows in that layout, *err one int.  odeint_type 0 / 3: galpy's variational
+ * steppers leapfrog_dxdv / symplec4_dxdv (bovy_symplecticode.c:1053-1268,
+ * step estimated from the orbit alone, orbit bit-identical to a plain run);
+ * 6: DOP853 on the 12-dim variational system (evalRectDeriv_dxdv,
+ * integrateFullOrbit.c:1496-1600).  The kick of the deviation is K dq with K
+ * the Cartesian Hessian of -Phi: components need a Hessian on the device
+ * (PowerSphericalPotentialwCutoff, MiyamotoNagai, NFW, Hernquist,
+ * axisymmetric LogarithmicHalo); others set *err = -1. */
+void integrateFullOrbit_dxdv(double *yo, int nt, double *t, int npot, int *pot_type,
+                             double *pot_args, void *pot_tfuncs, double dt, double rtol,
+                             double atol, double *result, int *err, int odeint_type);
+
+/* Replaces integratePlanarOrbit.c:940-1012 (`integratePlanarOrbit_dxdv`):
+ * the planar twin, yo[8] = (x,y,vx,vy, dx,dy,dvx,dvy), result[nt*8], planar
+ * packing; DOP853 on the 8-dim system (evalPlanarRectDeriv_dxdv,
+ * integratePlanarOrbit.c:1124-1175).  cb is accepted and, as in galpy, not
+ * called. */
+void integratePlanarOrbit_dxdv(double *yo, int nt, double *t, int npot, int *pot_type,
+                               double *pot_args, void *pot_tfuncs, double dt, double rtol,
+                               double atol, double *result, int *err, int odeint_type,
+                               void (*cb)(void));
+
+/* integrate_dxdv for many orbits in one call (no reference counterpart:
+ * galpy maps the single-orbit call over orbits from Python,
+ * integrateFullOrbit.py:980-999, integratePlanarOrbit.py:1497-1516).
+ * dim 3: yo[nobj*12], result[nobj*nt*12], full packing; dim 2: yo[nobj*8],
+ * result[nobj*nt*8], planar packing.  t[nt] or t[nobj*nt] (indiv_t). */
+void gh_integrate_dxdv(int nobj, int dim, double *yo, int nt, double *t, int indiv_t, int npot,
+                       int *pot_type, double *pot_args, double dt, double rtol, double atol,
+                       double *result, int *err, int odeint_type);
+/* The same with device buffers, a potential handle (gh_potential_upload for
+ * dim 3, gh_planar_potential_upload for dim 2) and a stream; d_stats as for
+ * integrateFullOrbit_hip. */
+int gh_integrate_dxdv_hip(int nobj, int dim, double *d_yo, int nt, const double *d_t, int indiv_t,
+                          const void *pot, double dt, double rtol, double atol, double *d_result,
+                          int *d_err, int odeint_type, double *d_stats, void *stream);
+
 /* Device-resident twin of integrateFullOrbit for batching/serving callers
  * (no reference counterpart; it is integrateFullOrbit.c:1046 with the
  * buffers already in HBM).  d_yo, d_t, d_result, d_err and d_stats are device

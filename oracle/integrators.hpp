@@ -29,7 +29,10 @@ struct IntStats {
 };
 
 // evalRectForce / evalRectDeriv (full orbits, D = 3) or evalPlanarRectForce /
-// evalPlanarRectDeriv (planar orbits, D = 2)
+// evalPlanarRectDeriv (planar orbits, D = 2).  D = 6 and D = 4 are the
+// variational systems of integrate_dxdv (orbit + one deviation, 12 and 8
+// first-order components), whose right-hand sides are evalRectDeriv_dxdv and
+// evalPlanarRectDeriv_dxdv; only DOP853 integrates those through deriv().
 template <class R, int D = 3> struct ForceCtx {
   int npot;
   const Component* P;
@@ -42,6 +45,8 @@ template <class R, int D = 3> struct ForceCtx {
   void deriv(R t, const R* q, R* a) const {
     (*nrhs)++;
     if constexpr (D == 2) planar_rect_deriv(npot, P, t, q, a);
+    else if constexpr (D == 6) rect_deriv_dxdv(npot, P, t, q, a);
+    else if constexpr (D == 4) planar_rect_deriv_dxdv(npot, P, t, q, a);
     else rect_deriv(npot, P, t, q, a);
   }
 };
@@ -262,38 +267,51 @@ void symplec4(const ForceCtx<R, D>& F, const double* yo, int nt, double dt_in, c
 // Variational (tangent-map) steppers, bovy_symplecticode.c:1053-1268:
 // the same DKD sequence on the augmented state [q | dq_1..dq_nde],
 // [p | dp_1..dp_nde]; the kick of each deviation column is h K dq_j
-// (rect_force_dxdv).  The step is estimated from the base orbit only, so the
-// base trajectory is bit-identical to a plain run.  w = rect state
-// [x,y,z,vx,vy,vz] followed by nde deviation blocks of 6 in the same order;
-// out = the final augmented state.  t = output grid (only its spacing and
+// (rect_force_dxdv, or planar_rect_force_dxdv for DB = 2).  The step is
+// estimated from the base orbit only (:1081-1084), so the base trajectory is
+// bit-identical to a plain run.  w = rectangular base state [q(DB), p(DB)]
+// followed by nde deviation blocks [dq(DB), dp(DB)] (save_qp_dxdv's layout,
+// :79-90); final = the final augmented state (nullable); traj (nullable) =
+// all nt rows in the same layout.  t = output grid (only its spacing and
 // count matter, as for the plain steppers).  Returns false if a component
 // has no restated Hessian.
-template <class R>
-bool symplectic_dxdv(int npot, const Component* P, int method, const double* w_in, int nde, int nt,
-                     const double* t, double dt_in, double rtol, double atol, double* out) {
+template <int DB, class R>
+bool symplectic_dxdv_traj(int npot, const Component* P, int method, const double* w_in, int nde,
+                          int nt, const double* t, double dt_in, double rtol, double atol,
+                          double* final_out, double* traj) {
   const int nb = 1 + nde;
+  constexpr int W = 2 * DB;  // doubles per block
   R q[3 * 7], p[3 * 7], a[3 * 7];
   for (int b = 0; b < nb; b++)
-    for (int k = 0; k < 3; k++) {
-      q[3 * b + k] = w_in[6 * b + k];
-      p[3 * b + k] = w_in[6 * b + 3 + k];
+    for (int k = 0; k < DB; k++) {
+      q[DB * b + k] = w_in[W * b + k];
+      p[DB * b + k] = w_in[W * b + DB + k];
     }
+  auto save = [&](double* out) {
+    for (int b = 0; b < nb; b++)
+      for (int k = 0; k < DB; k++) {
+        out[W * b + k] = dval(q[DB * b + k]);
+        out[W * b + DB + k] = dval(p[DB * b + k]);
+      }
+  };
+  if (traj) save(traj);
   long long nrhs = 0;
-  ForceCtx<R> F{npot, P, &nrhs};
+  ForceCtx<R, DB> F{npot, P, &nrhs};
   R init_dt = R(t[1]) - R(t[0]);
   R dt = dt_in;
   if (dt_in == kDtSentinel)
-    dt = (method == 0) ? leapfrog_estimate_step<3>(F, q, p, init_dt, R(t[0]), rtol, atol)
-                       : symplec4_estimate_step<3>(F, q, p, init_dt, R(t[0]), rtol, atol);
+    dt = (method == 0) ? leapfrog_estimate_step<DB>(F, q, p, init_dt, R(t[0]), rtol, atol)
+                       : symplec4_estimate_step<DB>(F, q, p, init_dt, R(t[0]), rtol, atol);
   long ndt = (long)(dval(init_dt / dt));
   R to = t[0];
   auto drift = [&](R h) {
-    for (int i = 0; i < 3 * nb; i++) q[i] = q[i] + h * p[i];
+    for (int i = 0; i < DB * nb; i++) q[i] = q[i] + h * p[i];
   };
   bool ok = true;
   auto kick = [&](R tk, R h) {
-    ok &= rect_force_dxdv(npot, P, tk, q, a, nde);
-    for (int i = 0; i < 3 * nb; i++) p[i] = p[i] + h * a[i];
+    if constexpr (DB == 3) ok &= rect_force_dxdv(npot, P, tk, q, a, nde);
+    else ok &= planar_rect_force_dxdv(npot, P, tk, q, a, nde);
+    for (int i = 0; i < DB * nb; i++) p[i] = p[i] + h * a[i];
   };
   const R c1 = S4C1, c2 = S4C2, c3 = S4C3, c4 = S4C4, d1 = S4D1, d2 = S4D2, d3 = S4D3;
   for (int ii = 0; ii < nt - 1; ii++) {
@@ -331,13 +349,18 @@ bool symplectic_dxdv(int npot, const Component* P, int method, const double* w_i
       drift(c4 * dt);
       to += c4 * dt;
     }
+    if (traj) save(traj + (long)(ii + 1) * W * nb);
   }
-  for (int b = 0; b < nb; b++)
-    for (int k = 0; k < 3; k++) {
-      out[6 * b + k] = dval(q[3 * b + k]);
-      out[6 * b + 3 + k] = dval(p[3 * b + k]);
-    }
+  if (final_out) save(final_out);
   return ok;
+}
+
+// the final augmented state of a full orbit (DB = 3)
+template <class R>
+bool symplectic_dxdv(int npot, const Component* P, int method, const double* w_in, int nde, int nt,
+                     const double* t, double dt_in, double rtol, double atol, double* out) {
+  return symplectic_dxdv_traj<3, R>(npot, P, method, w_in, nde, nt, t, dt_in, rtol, atol, out,
+                                    nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -445,8 +468,9 @@ void dop853(const ForceCtx<R, D>& F, double* y0d, int nt, const double* t, doubl
   const R facc1 = R(1.0) / R(0.333), facc2 = R(1.0) / R(6.0);
   R hmax = R(t[nt - 1]) - R(t[0]);
   R pos_neg = csign(R(1.0), hmax);
-  R y0[6], yy1[6], k1[6], k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6];
-  R rc1[6], rc2[6], rc3[6], rc4[6], rc5[6], rc6[6], rc7[6], rc8[6];
+  R y0[2 * D], yy1[2 * D], k1[2 * D], k2[2 * D], k3[2 * D], k4[2 * D], k5[2 * D], k6[2 * D],
+      k7[2 * D], k8[2 * D], k9[2 * D], k10[2 * D];
+  R rc1[2 * D], rc2[2 * D], rc3[2 * D], rc4[2 * D], rc5[2 * D], rc6[2 * D], rc7[2 * D], rc8[2 * D];
   long long nacc = 0, nrej = 0;
   for (int i = 0; i < dim; i++) { y0[i] = y0d[i]; result[i] = y0d[i]; }
   result += dim;

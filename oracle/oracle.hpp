@@ -573,6 +573,86 @@ void planar_rect_deriv(int npot, const Component* P, R t, const R* q, R* a) {
   planar_rect_force(npot, P, t, q, a + 2);
 }
 
+// Components with a restated Hessian (cyl_hessian): time-independent and
+// axisymmetric.
+inline bool has_hessian(int npot, const Component* P) {
+  for (int i = 0; i < npot; i++) {
+    const int c = P[i].type;
+    if (c == kDehnenBar || (c == kLogHalo && P[i].p[3] < 1)) return false;
+  }
+  return true;
+}
+
+// evalRectDeriv_dxdv, integrateFullOrbit.c:1496-1600, for conservative
+// components (the dissipative Jacobian blocks vanish): the 12-dim state
+// (x,y,z,vx,vy,vz | dx,dy,dz,dvx,dvy,dvz) of integrate_dxdv's DOP853 path;
+// the deviation's acceleration is K (dx,dy,dz) with K assembled as in
+// rect_force_dxdv.  Returns false for a component without a Hessian.
+template <class R>
+bool rect_deriv_dxdv(int npot, const Component* P, R t, const R* q, R* a) {
+  R qq[6] = {q[0], q[1], q[2], q[6], q[7], q[8]};
+  R aa[6] = {0., 0., 0., 0., 0., 0.};
+  const bool ok = rect_force_dxdv(npot, P, t, qq, aa, 1);
+  a[0] = q[3]; a[1] = q[4]; a[2] = q[5];
+  a[3] = aa[0]; a[4] = aa[1]; a[5] = aa[2];
+  a[6] = q[9]; a[7] = q[10]; a[8] = q[11];
+  a[9] = aa[3]; a[10] = aa[4]; a[11] = aa[5];
+  return ok;
+}
+
+// evalPlanarRectForce_dxdv, integratePlanarOrbit.c:1228-1275: planar
+// acceleration and, per deviation column, the kick tangent K (dx, dy).  galpy
+// wires calcPlanarR2deriv / phi2deriv / Rphideriv to each component's planar
+// second derivatives, which for the axisymmetric components here are the
+// full ones in the plane (R2deriv at z = 0) with vanishing phi terms; the
+// bar and the triaxial halo have none here (returns false).
+template <class R>
+bool planar_rect_force_dxdv(int npot, const Component* P, R t, const R* q, R* a, int nde) {
+  R x = q[0], y = q[1];
+  R Rc = sqrt(x * x + y * y);
+  R phi = acos(x / Rc);
+  R sinphi = y / Rc;
+  R cosphi = x / Rc;
+  if (y < R(0.)) phi = R(2. * M_PI) - phi;
+  R Rforce, phitorque;
+  planar_forces(npot, P, Rc, phi, t, Rforce, phitorque);
+  a[0] = cosphi * Rforce - R(1.) / Rc * sinphi * phitorque;
+  a[1] = sinphi * Rforce + R(1.) / Rc * cosphi * phitorque;
+  if (nde == 0) return true;
+  R R2deriv, z2d, Rzd;
+  if (!cyl_hessian(npot, P, Rc, R(0.), R2deriv, z2d, Rzd)) return false;
+  const R phi2deriv = 0., Rphideriv = 0.;
+  R dFxdx = -cosphi * cosphi * R2deriv + R(2.) * cosphi * sinphi / Rc / Rc * phitorque +
+            sinphi * sinphi / Rc * Rforce + R(2.) * sinphi * cosphi / Rc * Rphideriv -
+            sinphi * sinphi / Rc / Rc * phi2deriv;
+  R dFxdy = -sinphi * cosphi * R2deriv + (sinphi * sinphi - cosphi * cosphi) / Rc / Rc * phitorque -
+            cosphi * sinphi / Rc * Rforce - (cosphi * cosphi - sinphi * sinphi) / Rc * Rphideriv +
+            cosphi * sinphi / Rc / Rc * phi2deriv;
+  R dFydy = -sinphi * sinphi * R2deriv - R(2.) * sinphi * cosphi / Rc / Rc * phitorque -
+            R(2.) * sinphi * cosphi / Rc * Rphideriv + cosphi * cosphi / Rc * Rforce -
+            cosphi * cosphi / Rc / Rc * phi2deriv;
+  for (int j = 1; j <= nde; j++) {
+    R dx = q[2 * j], dy = q[2 * j + 1];
+    a[2 * j] = dFxdx * dx + dFxdy * dy;
+    a[2 * j + 1] = dFxdy * dx + dFydy * dy;
+  }
+  return true;
+}
+
+// evalPlanarRectDeriv_dxdv, integratePlanarOrbit.c:1124-1175: the 8-dim state
+// (x,y,vx,vy | dx,dy,dvx,dvy)
+template <class R>
+bool planar_rect_deriv_dxdv(int npot, const Component* P, R t, const R* q, R* a) {
+  R qq[4] = {q[0], q[1], q[4], q[5]};
+  R aa[4] = {0., 0., 0., 0.};
+  const bool ok = planar_rect_force_dxdv(npot, P, t, qq, aa, 1);
+  a[0] = q[2]; a[1] = q[3];
+  a[2] = aa[0]; a[3] = aa[1];
+  a[4] = q[6]; a[5] = q[7];
+  a[6] = aa[2]; a[7] = aa[3];
+  return ok;
+}
+
 // Potential value: galpy_potentials.c evaluatePotentials analog (sum in
 // component order); the bar uses the Python expression.
 template <class R>

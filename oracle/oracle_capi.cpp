@@ -281,3 +281,94 @@ ORC_EXPORT int oracle_eval_hessian(int npot, int* pot_type, double* pot_args, in
     if (!cyl_hessian<double>(npot, P, R[i], z[i], R2[i], z2[i], Rz[i])) return -1;
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// integrate_dxdv: the orbit plus one phase-space deviation, rectangular rows
+// (q, p | dq, dp) of 12 (full) or 8 (planar) doubles; symplectic methods by
+// the tangent-map steppers, DOP853 on the variational system.
+// ---------------------------------------------------------------------------
+namespace {
+// integrateFullOrbit_dxdv, integrateFullOrbit.c:1234-1306 (DB = 3) and
+// integratePlanarOrbit_dxdv, integratePlanarOrbit.c:940-1012 (DB = 2).
+template <int DB>
+void dxdv_one(int npot, const Component* P, int odeint_type, double* yo, int nt, const double* t,
+              double dt, double rtol, double atol, double* result, int* err) {
+  *err = 0;
+  bool ok = true;
+  switch (odeint_type) {
+    case 0:
+    case 3:
+      ok = symplectic_dxdv_traj<DB, double>(npot, P, odeint_type, yo, 1, nt, t, dt, rtol, atol,
+                                            nullptr, result);
+      break;
+    case 6: {
+      long long nrhs = 0;
+      ForceCtx<double, 2 * DB> F{npot, P, &nrhs};
+      dop853<2 * DB>(F, yo, nt, t, rtol, atol, result, err, nullptr);
+      break;
+    }
+    default:
+      *err = -1;
+  }
+  if (!ok) *err = -1;
+}
+
+template <int DB>
+int run_dxdv(int nobj, double* yo, int nt, const double* t, int indiv_t, int npot,
+             const int* pot_type, const double* pot_args, double dt, double rtol, double atol,
+             double* result, int* err, int odeint_type, int nthreads) {
+  constexpr int N = 4 * DB;
+  Component P[kMaxPot];
+  if (npot > kMaxPot || npot < 0 || nt < 2 ||
+      parse_potential(npot, pot_type, pot_args, P, DB == 2) != 0 || !has_hessian(npot, P)) {
+    for (int i = 0; i < nobj; i++) err[i] = -1;
+    return -1;
+  }
+  int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+  if (nobj < nth) nth = nobj > 0 ? nobj : 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nth)
+  for (int ii = 0; ii < nobj; ii++)
+    dxdv_one<DB>(npot, P, odeint_type, yo + (long)N * ii, nt, t + (long)nt * ii * indiv_t, dt,
+                 rtol, atol, result + (long)N * nt * ii, err + ii);
+  return 0;
+}
+}  // namespace
+
+// Same signature and argument meaning as galpy's integrateFullOrbit_dxdv
+// (one orbit; yo[12], result[nt*12]).
+ORC_EXPORT void oracle_integrateFullOrbit_dxdv(double* yo, int nt, double* t, int npot,
+                                               int* pot_type, double* pot_args, void* pot_tfuncs,
+                                               double dt, double rtol, double atol,
+                                               double* result, int* err, int odeint_type) {
+  (void)pot_tfuncs;
+  run_dxdv<3>(1, yo, nt, t, 0, npot, pot_type, pot_args, dt, rtol, atol, result, err,
+              odeint_type, 1);
+}
+
+// Same signature and argument meaning as galpy's integratePlanarOrbit_dxdv
+// (one orbit; yo[8], result[nt*8]; planar packing).
+ORC_EXPORT void oracle_integratePlanarOrbit_dxdv(double* yo, int nt, double* t, int npot,
+                                                 int* pot_type, double* pot_args,
+                                                 void* pot_tfuncs, double dt, double rtol,
+                                                 double atol, double* result, int* err,
+                                                 int odeint_type, void (*cb)(void)) {
+  (void)pot_tfuncs;
+  (void)cb;
+  run_dxdv<2>(1, yo, nt, t, 0, npot, pot_type, pot_args, dt, rtol, atol, result, err,
+              odeint_type, 1);
+}
+
+// Many orbits at once (dim 3: rows of 12, full packing; dim 2: rows of 8,
+// planar packing), OpenMP over orbits.
+ORC_EXPORT int oracle_integrate_dxdv(int nobj, int dim, double* yo, int nt, double* t, int indiv_t,
+                                     int npot, int* pot_type, double* pot_args, double dt,
+                                     double rtol, double atol, double* result, int* err,
+                                     int odeint_type, int nthreads) {
+  if (dim == 3)
+    return run_dxdv<3>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result,
+                       err, odeint_type, nthreads);
+  if (dim == 2)
+    return run_dxdv<2>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result,
+                       err, odeint_type, nthreads);
+  return -1;
+}

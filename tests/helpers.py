@@ -128,6 +128,28 @@ def ics(n, seed=0):
     return numpy.stack([R, vR, vT, z, vz, phi], axis=1)
 
 
+def oracle_integrate_dxdv(pot_type, pot_args, w, t, method, dim=3, dt=None, rtol=LN1EM12,
+                          atol=LN1EM12, nthreads=0):
+    """The oracle's integrate_dxdv for many orbits: w [n, 4 dim] rectangular
+    (q, p | dq, dp) -> (result [n, nt, 4 dim], err)."""
+    lib = oracle()
+    w = numpy.array(numpy.atleast_2d(w), dtype=numpy.float64, order="C")
+    t = numpy.ascontiguousarray(t, dtype=numpy.float64)
+    n, nt = len(w), t.shape[-1]
+    assert w.shape[1] == 4 * dim
+    res = numpy.empty((n, nt, 4 * dim))
+    err = numpy.zeros(n, dtype=numpy.int32)
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    lib.oracle_integrate_dxdv(
+        ctypes.c_int(n), ctypes.c_int(dim), _p(w), ctypes.c_int(nt), _p(t), ctypes.c_int(int(t.ndim > 1)),
+        ctypes.c_int(len(pt)), _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt),
+        ctypes.c_double(rtol), ctypes.c_double(atol), _p(res), _p(err), ctypes.c_int(method),
+        ctypes.c_int(nthreads),
+    )
+    return res, err
+
+
 def planar_ics(n, seed=0):
     """Planar rows (R,vR,vT,phi) of the same law as ics (the z, vz draws are
     dropped)."""

@@ -1,0 +1,176 @@
+"""GPU parity of integrate_dxdv (integrateFullOrbit_dxdv,
+integratePlanarOrbit_dxdv, gh_integrate_dxdv, gh_integrate_dxdv_hip) against
+the oracle's integrate_dxdv on the same seeded inputs.
+
+Bars: symplectic steppers 1e-10 on every row of the orbit and the deviation
+(relative, unit floor) with the oracle's per-orbit step; DOP853 on the
+variational system 1e-8.  At 65,536 orbits: the state transition matrices
+built from the GPU's planar tangent maps are symplectic, and the full tangent
+map in the plane equals the planar one."""
+
+import numpy
+import pytest
+
+from helpers import bar_potential, cyl_to_rect, ics, oracle_integrate_dxdv, planar_ics
+
+pytestmark = pytest.mark.gpu
+
+
+def _mw():
+    from galpy_amd.potential import MWPotential2014
+
+    return MWPotential2014
+
+
+def _packed(pot, planar):
+    from galpy_amd.potential import parse_pot, parse_pot_planar
+
+    _, pt, pa, _ = (parse_pot_planar if planar else parse_pot)(pot)
+    return pt, pa
+
+
+def _polar_to_rect(v):
+    R, vR, vT, phi = numpy.atleast_2d(v).T
+    c, s = numpy.cos(phi), numpy.sin(phi)
+    return numpy.stack([R * c, R * s, vR * c - vT * s, vR * s + vT * c], axis=1)
+
+
+def _rel(a, b):
+    return (numpy.abs(a - b) / numpy.maximum(numpy.abs(b), 1.0)).max()
+
+
+def _inputs(dim, n, seed):
+    y0 = cyl_to_rect(ics(n, seed=seed)) if dim == 3 else _polar_to_rect(planar_ics(n, seed=seed))
+    d0 = numpy.random.default_rng(seed + 100).normal(size=y0.shape)
+    return y0, d0
+
+
+def _hip(dim, pot, y0, d0, t, method, dt=None):
+    from galpy_amd.integrate import integrateFullOrbit_dxdv_hip, integratePlanarOrbit_dxdv_hip
+
+    f = integrateFullOrbit_dxdv_hip if dim == 3 else integratePlanarOrbit_dxdv_hip
+    return f(pot, y0, d0, t, method, dt=dt)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+@pytest.mark.parametrize("method,code", [("symplec4_hip", 3), ("leapfrog_hip", 0)])
+def test_dxdv_symplectic_mw2014(dim, method, code):
+    pot = _mw()
+    y0, d0 = _inputs(dim, 128, seed=1)
+    t = numpy.linspace(0.0, 10.0, 201)
+    res, err = _hip(dim, pot, y0, d0, t, method)  # estimated step
+    ref, rerr = oracle_integrate_dxdv(*_packed(pot, dim == 2), numpy.hstack([y0, d0]), t, code, dim=dim)
+    assert numpy.all(err == 0) and numpy.all(rerr == 0)
+    assert res.shape == (128, 201, 4 * dim)
+    assert _rel(res, ref) < 1e-10
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_dxdv_dop853_mw2014(dim):
+    pot = _mw()
+    y0, d0 = _inputs(dim, 128, seed=2)
+    t = numpy.linspace(0.0, 10.0, 101)
+    res, err = _hip(dim, pot, y0, d0, t, "dop853_hip")
+    ref, rerr = oracle_integrate_dxdv(*_packed(pot, dim == 2), numpy.hstack([y0, d0]), t, 6, dim=dim)
+    assert numpy.all(err == 0) and numpy.all(rerr == 0)
+    assert _rel(res, ref) < 1e-8
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_dxdv_generic_kernel_and_single_orbit(dim):
+    """Compositions other than MWPotential2014 run the run-time dispatch
+    kernels; one orbit goes through galpy's one-orbit drop-in symbol."""
+    from galpy_amd.potential import HernquistPotential, LogarithmicHaloPotential, NFWPotential
+
+    for pot in ([HernquistPotential(a=1.3), NFWPotential(a=3.0, amp=2.0)],
+                [LogarithmicHaloPotential(q=0.8)]):
+        y0, d0 = _inputs(dim, 64, seed=3)
+        t = numpy.linspace(0.0, 10.0, 101)
+        pt, pa = _packed(pot, dim == 2)
+        for method, code, tol in (("symplec4_hip", 3, 1e-10), ("dop853_hip", 6, 1e-8)):
+            res, err = _hip(dim, pot, y0, d0, t, method, dt=0.01 if code == 3 else None)
+            ref, _ = oracle_integrate_dxdv(pt, pa, numpy.hstack([y0, d0]), t, code, dim=dim,
+                                           dt=0.01 if code == 3 else None)
+            assert _rel(res, ref) < tol
+            one, e1 = _hip(dim, pot, y0[5], d0[5], t, method, dt=0.01 if code == 3 else None)
+            assert one.shape == (101, 4 * dim) and e1 == 0
+            assert numpy.array_equal(one, res[5])
+
+
+def test_dxdv_rejects_potentials_without_hessian():
+    from galpy_amd import _lib
+    from galpy_amd.potential import LogarithmicHaloPotential
+
+    for pot in (bar_potential(), [LogarithmicHaloPotential(q=0.9, b=0.8)]):
+        y0, d0 = _inputs(3, 2, seed=4)
+        with pytest.raises(RuntimeError):
+            _hip(3, pot, y0, d0, numpy.linspace(0, 1, 3), "symplec4_hip")
+        assert "Hessian" in _lib.last_error()
+
+
+def test_orbit_integrate_dxdv_frontend():
+    """Orbit.integrate_dxdv with cylindrical deviations in and out
+    (Orbits.py:2276): the deviation equals the finite difference of two
+    Orbit.integrate runs, planar and full."""
+    from galpy_amd.orbit import Orbit
+
+    pot = _mw()
+    t = numpy.linspace(0.0, 5.0, 51)
+    eps = 1e-6
+    for vxvv in (planar_ics(4, seed=5), ics(4, seed=5)):
+        d = numpy.random.default_rng(7).normal(size=vxvv.shape) * 0.1
+        o = Orbit(vxvv)
+        o.integrate_dxdv(d, t, pot, method="symplec4_hip", dt=0.01)
+        dev = o.getOrbit_dxdv()
+        hi, lo = Orbit(vxvv + eps * d), Orbit(vxvv - eps * d)
+        hi.integrate(t, pot, method="symplec4_hip", dt=0.01)
+        lo.integrate(t, pot, method="symplec4_hip", dt=0.01)
+        fd = (hi.getOrbit() - lo.getOrbit()) / (2 * eps)
+        fd[..., -1] = ((hi.getOrbit()[..., -1] - lo.getOrbit()[..., -1] + numpy.pi) % (2 * numpy.pi)
+                       - numpy.pi) / (2 * eps)
+        assert numpy.abs(dev - fd).max() < 1e-5 * max(1.0, numpy.abs(fd).max())
+        # the orbit itself is integrate's
+        plain = Orbit(vxvv)
+        plain.integrate(t, pot, method="symplec4_hip", dt=0.01)
+        a, b = o.getOrbit(), plain.getOrbit()
+        dphi = numpy.abs((a[..., -1] - b[..., -1] + numpy.pi) % (2 * numpy.pi) - numpy.pi)
+        assert numpy.abs(a[..., :-1] - b[..., :-1]).max() < 1e-12 and dphi.max() < 1e-12
+        # rectangular deviations in and out
+        o2 = Orbit(vxvv)
+        o2.integrate_dxdv(numpy.ones_like(vxvv), t, pot, method="dop853_hip", rectIn=True, rectOut=True)
+        assert o2.getOrbit_dxdv().shape == vxvv.shape[:1] + (51, vxvv.shape[1])
+
+
+def test_dxdv_at_scale_symplectic_stm():
+    """65,536 planar orbits: the 4x4 state transition matrices from four
+    tangent-map runs on the GPU are symplectic (M^T Omega M = Omega); the
+    full tangent map with z = vz = dz = dvz = 0 gives the same rows."""
+    import torch
+
+    from galpy_amd.integrate import PotentialTable, integrate_dxdv_device
+
+    n = 65536
+    pot = _mw()
+    tp, tf = PotentialTable(pot, planar=True), PotentialTable(pot)
+    y0 = _polar_to_rect(planar_ics(n, seed=8))
+    t = torch.linspace(0.0, 10.0, 11, dtype=torch.float64, device="cuda")
+    M = numpy.empty((n, 4, 4))
+    for j in range(4):
+        d0 = numpy.zeros((n, 4))
+        d0[:, j] = 1.0
+        w = torch.tensor(numpy.hstack([y0, d0]), device="cuda")
+        rp, ep, _ = integrate_dxdv_device(tp, w, t, "symplec4_hip", dt=0.01)
+        wf = numpy.zeros((n, 12))
+        wf[:, [0, 1, 3, 4]] = y0
+        wf[:, [6, 7, 9, 10]] = d0
+        rf, ef, _ = integrate_dxdv_device(tf, torch.tensor(wf, device="cuda"), t, "symplec4_hip", dt=0.01)
+        torch.cuda.synchronize()
+        assert int((ep != 0).sum()) == 0 and int((ef != 0).sum()) == 0
+        rp, rf = rp.cpu().numpy(), rf.cpu().numpy()
+        assert numpy.all(rf[..., [2, 5, 8, 11]] == 0.0)
+        assert _rel(rp, rf[..., [0, 1, 3, 4, 6, 7, 9, 10]]) < 1e-11
+        M[:, :, j] = rp[:, -1, 4:]
+    Om = numpy.block([[numpy.zeros((2, 2)), numpy.eye(2)], [-numpy.eye(2), numpy.zeros((2, 2))]])
+    defect = numpy.abs(numpy.einsum("nji,jk,nkl->nil", M, Om, M) - Om).max(axis=(1, 2))
+    scale = numpy.abs(M).max(axis=(1, 2)) ** 2
+    assert numpy.all(defect <= 1e-12 * numpy.maximum(scale, 1.0)), defect.max()
