@@ -11,8 +11,9 @@ over the K timed steps.  |dE/E| is measured on device afterwards.
 
 NAME: mw2014_dop853_1m (default), loghalo_bar_dop853_256k, mw2014_symplec4_1m,
 mw2014_symplec4_1k, mw2014_planar_dop853_1m, mw2014_planar_symplec4_1m (planar
-orbits, integratePlanarOrbit), batching, serving, adjoint, copy (counter
-calibration).
+orbits, integratePlanarOrbit), mw2014_dxdv_symplec4_65k, mw2014_dxdv_dop853_65k
+(integrate_dxdv: orbit + one deviation), batching, serving, adjoint, copy
+(counter calibration).
 
 Prints one JSON line (see the keys below).  The CPU baseline leg times the
 CPU oracle (a C++/OpenMP restatement of galpy's C integrators) on a bounded
@@ -67,7 +68,8 @@ def cpu_baseline_leg(w, sample, count_sample):
     """CPU oracle on a bounded sample of the same workload (OpenMP, all host
     cores), plus the op count per orbit-step of the same algorithm."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from helpers import oracle_count_ops, oracle_integrate  # the oracle is test infrastructure
+    # the oracle is test infrastructure
+    from helpers import oracle_count_ops, oracle_count_ops_dxdv, oracle_integrate, oracle_integrate_dxdv
 
     from galpy_amd.integrate import METHOD_CODES
     from galpy_amd.potential import parse_pot, parse_pot_planar
@@ -77,13 +79,23 @@ def cpu_baseline_leg(w, sample, count_sample):
     code = METHOD_CODES[w["method"]]
     nthreads = len(os.sched_getaffinity(0))
     yo = w["yo"][:sample]
-    t0 = time.perf_counter()
-    _, err, stats, _ = oracle_integrate(pt, pa, yo, w["t"], code, dt=w.get("dt"), nthreads=nthreads,
-                                        planar=planar)
-    el = time.perf_counter() - t0
-    steps = stats[:, 0].sum()
-    ops = oracle_count_ops(pt, pa, w["yo"][:count_sample], w["t"], code, dt=w.get("dt"),
-                           planar=planar)
+    if w["dxdv"]:
+        nsteps = numpy.zeros(len(yo), dtype=numpy.int64)
+        t0 = time.perf_counter()
+        oracle_integrate_dxdv(pt, pa, yo, w["t"], code, dim=3, dt=w.get("dt"), nthreads=nthreads,
+                              steps=nsteps)
+        el = time.perf_counter() - t0
+        steps = float(nsteps.sum())
+        ops = oracle_count_ops_dxdv(pt, pa, w["yo"][:count_sample], w["t"], code, dim=3, dt=w.get("dt"))
+        ops["rhs"] = None
+    else:
+        t0 = time.perf_counter()
+        _, err, stats, _ = oracle_integrate(pt, pa, yo, w["t"], code, dt=w.get("dt"), nthreads=nthreads,
+                                            planar=planar)
+        el = time.perf_counter() - t0
+        steps = stats[:, 0].sum()
+        ops = oracle_count_ops(pt, pa, w["yo"][:count_sample], w["t"], code, dt=w.get("dt"),
+                               planar=planar)
     flops = ops["add"] + ops["mul"] + ops["div"] + ops["sqrt"] + ops["special"]
     return {
         "value": steps / el,
@@ -94,7 +106,7 @@ def cpu_baseline_leg(w, sample, count_sample):
     }, {
         "flop_per_orbit_step": flops / ops["steps"],
         "special_per_orbit_step": ops["special"] / ops["steps"],
-        "rhs_per_orbit_step": ops["rhs"] / ops["steps"],
+        "rhs_per_orbit_step": None if ops["rhs"] is None else ops["rhs"] / ops["steps"],
         "count_sample": count_sample,
     }
 
@@ -323,7 +335,12 @@ def main():
     import torch
 
     from galpy_amd import _lib, workloads
-    from galpy_amd.integrate import PotentialTable, energy_drift_device, integrate_device
+    from galpy_amd.integrate import (
+        PotentialTable,
+        energy_drift_device,
+        integrate_device,
+        integrate_dxdv_device,
+    )
 
     _lib.load()  # fail loudly when the HIP library is missing
     if not torch.cuda.is_available():
@@ -332,7 +349,7 @@ def main():
     w = workloads.make(args.workload, n=args.norbits)
     n, t_np = w["n"], w["t"]
     nt = len(t_np)
-    ncol = 4 if w["planar"] else 6  # doubles per phase-space row
+    ncol = 12 if w["dxdv"] else 4 if w["planar"] else 6  # doubles per row
     table = PotentialTable(w["pot"], planar=w["planar"])
     yo0 = torch.tensor(w["yo"], dtype=torch.float64, device=dev)
     yo = torch.empty_like(yo0)
@@ -346,8 +363,12 @@ def main():
         yo.copy_(yo0)
         if ev is not None:
             ev[0].record(stream)
-        integrate_device(table, yo, tt, w["method"], dt=w.get("dt"), result=result, err=err, stats=stats,
-                         stream=stream)
+        if w["dxdv"]:
+            integrate_dxdv_device(table, yo, tt, w["method"], dt=w.get("dt"), result=result, err=err,
+                                  stats=stats, stream=stream)
+        else:
+            integrate_device(table, yo, tt, w["method"], dt=w.get("dt"), result=result, err=err,
+                             stats=stats, stream=stream)
         if ev is not None:
             ev[1].record(stream)
 
@@ -367,7 +388,17 @@ def main():
     steps_per_pass = float(st[:, 0].sum())
     rejected = float(st[:, 1].sum())
     nerr = int((err != 0).sum().item())
-    e0, demax = energy_drift_device(table, result, tt, omega=w["omega"])
+    if w["dxdv"]:  # the orbit's energy along the rectangular rows
+        q = result[..., :6]
+        Rr = torch.sqrt(q[..., 0] ** 2 + q[..., 1] ** 2)
+        ph = torch.atan2(q[..., 1], q[..., 0])
+        cp, sp = torch.cos(ph), torch.sin(ph)
+        cyl = torch.stack([Rr, q[..., 3] * cp + q[..., 4] * sp, q[..., 4] * cp - q[..., 3] * sp,
+                           q[..., 2], q[..., 5], ph], dim=-1).contiguous()
+        e0, demax = energy_drift_device(table, cyl, tt, omega=w["omega"])
+        del q, Rr, ph, cp, sp, cyl
+    else:
+        e0, demax = energy_drift_device(table, result, tt, omega=w["omega"])
     dem = demax.cpu().numpy()
     torch.cuda.synchronize()
 
@@ -389,6 +420,7 @@ def main():
             "workload": w["name"],
             "orbits": n,
             "orbit_dim": 2 if w["planar"] else 3,
+            "deviations_per_orbit": 1 if w["dxdv"] else 0,
             "method": w["method"],
             "potential": "MWPotential2014" if w["omega"] == 0.0 else "LogarithmicHalo(q=0.9)+DehnenBar",
             "t": [float(t_np[0]), float(t_np[-1]), nt],

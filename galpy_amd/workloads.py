@@ -54,6 +54,13 @@ WORKLOADS = {
     "mw2014_planar_symplec4_1m": dict(pot=mw2014, n=1048576, t=(0.0, 100.0, 101),
                                      method="symplec4_hip", omega=0.0, seed=0, dt=0.01,
                                      planar=True),
+    # integrate_dxdv: 65,536 orbits with one phase-space deviation each
+    # (standard-normal components, rectangular), symplec4 at dt = 0.01 for
+    # 2,000 steps, the orbit and deviation stored every 10 steps
+    "mw2014_dxdv_symplec4_65k": dict(pot=mw2014, n=65536, t=(0.0, 20.0, 201),
+                                    method="symplec4_hip", omega=0.0, seed=0, dt=0.01, dxdv=True),
+    "mw2014_dxdv_dop853_65k": dict(pot=mw2014, n=65536, t=(0.0, 20.0, 201),
+                                  method="dop853_hip", omega=0.0, seed=0, dxdv=True),
 }
 
 
@@ -65,8 +72,16 @@ def make(name, n=None):
     w["t"] = numpy.linspace(t0, t1, nt)
     w["yo"] = ics(w["n"], seed=w["seed"])
     w["planar"] = bool(w.get("planar", False))
+    w["dxdv"] = bool(w.get("dxdv", False))
     if w["planar"]:
         w["yo"] = numpy.ascontiguousarray(w["yo"][:, [0, 1, 2, 5]])
+    if w["dxdv"]:
+        # rectangular (x,y,z,vx,vy,vz | dx,dy,dz,dvx,dvy,dvz)
+        R, vR, vT, z, vz, phi = w["yo"].T
+        c, s = numpy.cos(phi), numpy.sin(phi)
+        rect = numpy.stack([R * c, R * s, z, vR * c - vT * s, vR * s + vT * c, vz], axis=1)
+        dev = numpy.random.default_rng(w["seed"] + 1).normal(size=rect.shape)
+        w["yo"] = numpy.ascontiguousarray(numpy.hstack([rect, dev]))
     w["pot"] = w["pot"]()
     w["name"] = name
     return w

@@ -290,21 +290,34 @@ ORC_EXPORT int oracle_eval_hessian(int npot, int* pot_type, double* pot_args, in
 namespace {
 // integrateFullOrbit_dxdv, integrateFullOrbit.c:1234-1306 (DB = 3) and
 // integratePlanarOrbit_dxdv, integratePlanarOrbit.c:940-1012 (DB = 2).
-template <int DB>
+template <int DB, class R = double>
 void dxdv_one(int npot, const Component* P, int odeint_type, double* yo, int nt, const double* t,
-              double dt, double rtol, double atol, double* result, int* err) {
+              double dt, double rtol, double atol, double* result, int* err,
+              IntStats* st = nullptr) {
   *err = 0;
   bool ok = true;
   switch (odeint_type) {
     case 0:
     case 3:
-      ok = symplectic_dxdv_traj<DB, double>(npot, P, odeint_type, yo, 1, nt, t, dt, rtol, atol,
-                                            nullptr, result);
+      ok = symplectic_dxdv_traj<DB, R>(npot, P, odeint_type, yo, 1, nt, t, dt, rtol, atol,
+                                       nullptr, result);
+      if (st) {  // orbit-steps: the step estimate again, in plain (uncounted) doubles
+        long long nrhs = 0;
+        ForceCtx<double, DB> F{npot, P, &nrhs};
+        double qo[3], po[3];
+        for (int k = 0; k < DB; k++) { qo[k] = yo[k]; po[k] = yo[DB + k]; }
+        const double init_dt = t[1] - t[0];
+        double d = dt;
+        if (dt == kDtSentinel)
+          d = (odeint_type == 0) ? leapfrog_estimate_step<DB>(F, qo, po, init_dt, t[0], rtol, atol)
+                                 : symplec4_estimate_step<DB>(F, qo, po, init_dt, t[0], rtol, atol);
+        st->steps = (long long)(init_dt / d) * (nt - 1);
+      }
       break;
     case 6: {
       long long nrhs = 0;
-      ForceCtx<double, 2 * DB> F{npot, P, &nrhs};
-      dop853<2 * DB>(F, yo, nt, t, rtol, atol, result, err, nullptr);
+      ForceCtx<R, 2 * DB> F{npot, P, &nrhs};
+      dop853<2 * DB>(F, yo, nt, t, rtol, atol, result, err, st);
       break;
     }
     default:
@@ -316,7 +329,8 @@ void dxdv_one(int npot, const Component* P, int odeint_type, double* yo, int nt,
 template <int DB>
 int run_dxdv(int nobj, double* yo, int nt, const double* t, int indiv_t, int npot,
              const int* pot_type, const double* pot_args, double dt, double rtol, double atol,
-             double* result, int* err, int odeint_type, int nthreads) {
+             double* result, int* err, int odeint_type, int nthreads,
+             long long* steps = nullptr) {
   constexpr int N = 4 * DB;
   Component P[kMaxPot];
   if (npot > kMaxPot || npot < 0 || nt < 2 ||
@@ -327,9 +341,12 @@ int run_dxdv(int nobj, double* yo, int nt, const double* t, int indiv_t, int npo
   int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
   if (nobj < nth) nth = nobj > 0 ? nobj : 1;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nth)
-  for (int ii = 0; ii < nobj; ii++)
+  for (int ii = 0; ii < nobj; ii++) {
+    IntStats st = {0, 0, 0, 0.};
     dxdv_one<DB>(npot, P, odeint_type, yo + (long)N * ii, nt, t + (long)nt * ii * indiv_t, dt,
-                 rtol, atol, result + (long)N * nt * ii, err + ii);
+                 rtol, atol, result + (long)N * nt * ii, err + ii, steps ? &st : nullptr);
+    if (steps) steps[ii] = st.steps;
+  }
   return 0;
 }
 }  // namespace
@@ -359,16 +376,51 @@ ORC_EXPORT void oracle_integratePlanarOrbit_dxdv(double* yo, int nt, double* t, 
 }
 
 // Many orbits at once (dim 3: rows of 12, full packing; dim 2: rows of 8,
-// planar packing), OpenMP over orbits.
+// planar packing), OpenMP over orbits; steps[i] (nullable) = orbit-steps.
 ORC_EXPORT int oracle_integrate_dxdv(int nobj, int dim, double* yo, int nt, double* t, int indiv_t,
                                      int npot, int* pot_type, double* pot_args, double dt,
                                      double rtol, double atol, double* result, int* err,
-                                     int odeint_type, int nthreads) {
+                                     int odeint_type, int nthreads, long long* steps) {
   if (dim == 3)
     return run_dxdv<3>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result,
-                       err, odeint_type, nthreads);
+                       err, odeint_type, nthreads, steps);
   if (dim == 2)
     return run_dxdv<2>(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol, result,
-                       err, odeint_type, nthreads);
+                       err, odeint_type, nthreads, steps);
   return -1;
+}
+
+// Op-counting integrate_dxdv (single thread), out[0..5] = add, mul, div, sqrt,
+// special, orbit-steps, summed over the orbits.  For the symplectic methods
+// the count includes the plain step estimate that the device runs.
+ORC_EXPORT int oracle_count_ops_dxdv(int nobj, int dim, double* yo, int nt, double* t, int npot,
+                                     int* pot_type, double* pot_args, double dt, double rtol,
+                                     double atol, double* result, int* err, int odeint_type,
+                                     double* out) {
+  if (dim != 2 && dim != 3) return -1;
+  const bool planar = dim == 2;
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P, planar) != 0 ||
+      !has_hessian(npot, P))
+    return -1;
+  const int N = 4 * dim;
+  g_ops = OpCount{0, 0, 0, 0, 0};
+  long long steps = 0;
+  for (int ii = 0; ii < nobj; ii++) {
+    IntStats st = {0, 0, 0, 0.};
+    if (dim == 3)
+      dxdv_one<3, CountedD>(npot, P, odeint_type, yo + (long)N * ii, nt, t, dt, rtol, atol,
+                            result + (long)N * nt * ii, err + ii, &st);
+    else
+      dxdv_one<2, CountedD>(npot, P, odeint_type, yo + (long)N * ii, nt, t, dt, rtol, atol,
+                            result + (long)N * nt * ii, err + ii, &st);
+    steps += st.steps;
+  }
+  out[0] = (double)g_ops.add;
+  out[1] = (double)g_ops.mul;
+  out[2] = (double)g_ops.div;
+  out[3] = (double)g_ops.sqrt;
+  out[4] = (double)g_ops.special;
+  out[5] = (double)steps;
+  return 0;
 }

@@ -129,9 +129,10 @@ def ics(n, seed=0):
 
 
 def oracle_integrate_dxdv(pot_type, pot_args, w, t, method, dim=3, dt=None, rtol=LN1EM12,
-                          atol=LN1EM12, nthreads=0):
+                          atol=LN1EM12, nthreads=0, steps=None):
     """The oracle's integrate_dxdv for many orbits: w [n, 4 dim] rectangular
-    (q, p | dq, dp) -> (result [n, nt, 4 dim], err)."""
+    (q, p | dq, dp) -> (result [n, nt, 4 dim], err); ``steps`` (an int64
+    array of n, optional) receives the orbit-steps."""
     lib = oracle()
     w = numpy.array(numpy.atleast_2d(w), dtype=numpy.float64, order="C")
     t = numpy.ascontiguousarray(t, dtype=numpy.float64)
@@ -145,9 +146,29 @@ def oracle_integrate_dxdv(pot_type, pot_args, w, t, method, dim=3, dt=None, rtol
         ctypes.c_int(n), ctypes.c_int(dim), _p(w), ctypes.c_int(nt), _p(t), ctypes.c_int(int(t.ndim > 1)),
         ctypes.c_int(len(pt)), _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt),
         ctypes.c_double(rtol), ctypes.c_double(atol), _p(res), _p(err), ctypes.c_int(method),
-        ctypes.c_int(nthreads),
+        ctypes.c_int(nthreads), None if steps is None else _p(steps),
     )
     return res, err
+
+
+def oracle_count_ops_dxdv(pot_type, pot_args, w, t, method, dim=3, dt=None):
+    """Counted FP64 ops of the oracle's integrate_dxdv (see oracle_count_ops)."""
+    lib = oracle()
+    w = numpy.array(numpy.atleast_2d(w), dtype=numpy.float64, order="C")
+    t = numpy.ascontiguousarray(t, dtype=numpy.float64)
+    n, nt = len(w), len(t)
+    res = numpy.empty((n, nt, 4 * dim))
+    err = numpy.zeros(n, dtype=numpy.int32)
+    out = numpy.zeros(6)
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    rc = lib.oracle_count_ops_dxdv(
+        ctypes.c_int(n), ctypes.c_int(dim), _p(w), ctypes.c_int(nt), _p(t), ctypes.c_int(len(pt)), _p(pt),
+        _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(LN1EM12),
+        ctypes.c_double(LN1EM12), _p(res), _p(err), ctypes.c_int(method), _p(out),
+    )
+    assert rc == 0
+    return dict(zip(["add", "mul", "div", "sqrt", "special", "steps"], out))
 
 
 def planar_ics(n, seed=0):

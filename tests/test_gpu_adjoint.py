@@ -115,7 +115,7 @@ def test_gradients_match_central_differences(method):
     assert (w0rec - w0t).abs().max().item() < 1e-10
 
 
-def test_adjoint_rejects_time_dependent_potential():
+def test_adjoint_rejects_potentials_without_device_hessian():
     import torch
 
     from galpy_amd import workloads
@@ -127,6 +127,13 @@ def test_adjoint_rejects_time_dependent_potential():
     w = torch.ones((4, 6), dtype=torch.float64, device="cuda")
     with pytest.raises(RuntimeError, match="adjoint"):
         adjoint_raw(tab, w, w, DT, 10)
+    # a triaxial halo exerts a torque; the device Hessian is the axisymmetric
+    # one, so it is refused rather than differentiated wrongly
+    from galpy_amd.potential import LogarithmicHaloPotential
+
+    _, pt, pa, _ = parse_pot([LogarithmicHaloPotential(q=0.9, b=0.8)])
+    with pytest.raises(RuntimeError, match="axisymmetric"):
+        adjoint_raw(_RawTable(pt, pa), w, w, DT, 10)
 
 
 @pytest.mark.parametrize("method,code", [("symplec4_hip", 3), ("leapfrog_hip", 0)])
