@@ -40,6 +40,14 @@ _SIGS = {
 }
 MAX_POT = 8
 NPAR = 3
+# packed arguments per component in galpy's order (integrateFullOrbit.py:65-365)
+_NARGS = {0: 4, 1: 6, 5: 3, 7: 2, 8: 2, 9: 2, 10: 2, 14: 2, 15: 5, 17: 2, 18: 2, 19: 2,
+          20: 2, 33: 2, 34: 3, 35: 3}
+# the packed arguments the adjoint kernel accumulates dL/dtheta for:
+# MiyamotoNagai (amp, a, b), NFW and Hernquist (amp, a), every other
+# component with a device Hessian its amplitude only
+_DIFFERENTIABLE = {5: (0, 1, 2), 8: (0, 1), 9: (0, 1), 0: (0,), 15: (0,), 7: (0,), 10: (0,),
+                   14: (0,), 17: (0,), 18: (0,), 19: (0,), 20: (0,), 33: (0,), 34: (0,), 35: (0,)}
 
 
 def _lib_adj():
@@ -84,12 +92,17 @@ class ParamSpec:
 
     def __init__(self, pot, params):
         self.npot, self.pot_type, self.pot_args, _ = parse_pot(pot)
-        nargs = {0: 4, 1: 6, 5: 3, 8: 2, 9: 2, 15: 5}
-        self.offsets = numpy.cumsum([0] + [nargs[int(c)] for c in self.pot_type])[:-1]
+        self.offsets = numpy.cumsum([0] + [_NARGS[int(c)] for c in self.pot_type])[:-1]
         self.params = [(int(c), int(j)) for c, j in params]
         for c, j in self.params:
-            if j >= NPAR:
-                raise ValueError("only packed arguments 0..2 of a component are differentiable")
+            if not 0 <= c < self.npot:
+                raise ValueError(f"component index {c} out of range for {self.npot} components")
+            code = int(self.pot_type[c])
+            if j not in _DIFFERENTIABLE.get(code, ()):
+                raise ValueError(
+                    f"packed argument {j} of component {c} (pot_type {code}) is not differentiable; "
+                    f"the adjoint differentiates arguments {sorted(_DIFFERENTIABLE.get(code, ()))}"
+                )
 
     def theta0(self):
         return numpy.array([self.pot_args[self.offsets[c] + j] for c, j in self.params])

@@ -149,6 +149,56 @@ __device__ __forceinline__ void gh_comp_adj<GH_LOGHALO>(const GhPotTable* __rest
   dth[0] += -vl * iD;
 }
 
+// the spherical components of potential_dev.h (gh_sph_g): g and g'/r; the
+// amplitude is the only differentiable parameter (g is linear in it)
+#define GH_SPH_ADJ(CODE)                                                                      \
+  template <>                                                                                 \
+  __device__ __forceinline__ void gh_comp_adj<CODE>(const GhPotTable* __restrict__ T, int i,  \
+                                                    GhVec3 x, double r2, double rinv,         \
+                                                    GhVec3 lam, GhVec3& a, GhVec3& k,         \
+                                                    double* dth) {                            \
+    const GhComponent* __restrict__ C = &T->comp[i];                                          \
+    double g, gpr;                                                                            \
+    gh_sph_g<CODE>(C, r2 * rinv, rinv, g, gpr);                                               \
+    gh_sph_adj(x, lam, g, gpr, a, k);                                                         \
+    dth[0] += (g / C->p[0]) * (x.x * lam.x + x.y * lam.y + x.z * lam.z);                      \
+  }
+GH_SPH_ADJ(GH_POWERSPH)
+GH_SPH_ADJ(GH_JAFFE)
+GH_SPH_ADJ(GH_ISOCHRONE)
+GH_SPH_ADJ(GH_PLUMMER)
+GH_SPH_ADJ(GH_PSEUDOISO)
+GH_SPH_ADJ(GH_BURKERT)
+GH_SPH_ADJ(GH_DEHNENCORESPH)
+GH_SPH_ADJ(GH_DEHNENSPH)
+GH_SPH_ADJ(GH_HOMOGSPHERE)
+#undef GH_SPH_ADJ
+
+template <>
+__device__ __forceinline__ void gh_comp_adj<GH_KUZMINDISK>(const GhPotTable* __restrict__ T, int i,
+                                                           GhVec3 x, double r2, double rinv,
+                                                           GhVec3 lam, GhVec3& a, GhVec3& k,
+                                                           double* dth) {
+  // a = -amp (x, y, w) D^-3/2, w = sign(z) (a + |z|), D = R^2 + (a + |z|)^2;
+  // K as galpy's KuzminDiskPotential.c second derivatives: the z-z entry
+  // uses (a + |z|)^2 also at z = 0, where sign(z), and so w, vanish
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0];
+  const double az = C->p[1] + fabs(x.z);
+  const double iD = rsqrt(x.x * x.x + x.y * x.y + az * az);
+  const double D32 = iD * iD * iD;
+  const double D52 = D32 * iD * iD;
+  const double w = (x.z == 0.0) ? 0.0 : copysign(az, x.z);
+  a.x += -amp * x.x * D32;
+  a.y += -amp * x.y * D32;
+  a.z += -amp * w * D32;
+  const double u = x.x * lam.x + x.y * lam.y;
+  k.x += -amp * D32 * lam.x + 3.0 * amp * x.x * D52 * (u + w * lam.z);
+  k.y += -amp * D32 * lam.y + 3.0 * amp * x.y * D52 * (u + w * lam.z);
+  k.z += -amp * D32 * lam.z + 3.0 * amp * D52 * (w * u + az * az * lam.z);
+  dth[0] += -(u + w * lam.z) * D32;
+}
+
 __device__ __forceinline__ void gh_comp_adj_dyn(const GhPotTable* __restrict__ T, int i, GhVec3 x,
                                                 double r2, double rinv, GhVec3 lam, GhVec3& a,
                                                 GhVec3& k, double* dth) {
@@ -158,6 +208,16 @@ __device__ __forceinline__ void gh_comp_adj_dyn(const GhPotTable* __restrict__ T
     case GH_NFW: gh_comp_adj<GH_NFW>(T, i, x, r2, rinv, lam, a, k, dth); break;
     case GH_HERNQUIST: gh_comp_adj<GH_HERNQUIST>(T, i, x, r2, rinv, lam, a, k, dth); break;
     case GH_LOGHALO: gh_comp_adj<GH_LOGHALO>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_POWERSPH: gh_comp_adj<GH_POWERSPH>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_JAFFE: gh_comp_adj<GH_JAFFE>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_ISOCHRONE: gh_comp_adj<GH_ISOCHRONE>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_PLUMMER: gh_comp_adj<GH_PLUMMER>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_PSEUDOISO: gh_comp_adj<GH_PSEUDOISO>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_KUZMINDISK: gh_comp_adj<GH_KUZMINDISK>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_BURKERT: gh_comp_adj<GH_BURKERT>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_DEHNENCORESPH: gh_comp_adj<GH_DEHNENCORESPH>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_DEHNENSPH: gh_comp_adj<GH_DEHNENSPH>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_HOMOGSPHERE: gh_comp_adj<GH_HOMOGSPHERE>(T, i, x, r2, rinv, lam, a, k, dth); break;
     default: break;
   }
 }

@@ -81,6 +81,10 @@ int nargs_of(int code, bool planar) {
     case GH_HERNQUIST: return 2;
     case GH_NFW: return 2;
     case GH_POWERSPHCUTOFF: return planar ? 3 : 5;
+    // integrateFullOrbit.c:135-603 / integratePlanarOrbit.c:136-418
+    case GH_POWERSPH: case GH_JAFFE: case GH_ISOCHRONE: case GH_PLUMMER: case GH_PSEUDOISO:
+    case GH_KUZMINDISK: case GH_BURKERT: case GH_DEHNENCORESPH: return 2;
+    case GH_DEHNENSPH: case GH_HOMOGSPHERE: return 3;
     default: return -1;
   }
 }
@@ -251,6 +255,18 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
         C.c[0] = 1.0 / C.p[3];
         C.nonaxi = 1;
         break;
+      case GH_ISOCHRONE:
+      case GH_PLUMMER: C.c[0] = C.p[1] * C.p[1]; break;  // b^2
+      case GH_PSEUDOISO:
+      case GH_BURKERT: C.c[0] = 1.0 / C.p[1]; break;  // 1/a
+      case GH_DEHNENSPH:
+        if (!(C.p[2] >= 0.0 && C.p[2] < 3.0)) {
+          set_error("DehnenSphericalPotential alpha=%g outside [0, 3)", C.p[2]);
+          return -1;
+        }
+        C.c[0] = 1.0 / (3.0 - C.p[2]);
+        break;
+      default: break;
     }
     T->nonaxi |= C.nonaxi;
   }
@@ -868,14 +884,10 @@ void fill_handle(const GhPotTable& table, GhPotHandle* h) {
 }
 
 // Components with a Hessian on the device (the adjoint's): time-independent
-// and axisymmetric -- not the bar, not a triaxial logarithmic halo.
+// and axisymmetric -- every type but the bar and a triaxial logarithmic halo.
 bool hessian_capable(const GhPotHandle* h) {
   for (int i = 0; i < h->npot; i++) {
-    const int c = h->types[i];
-    if (c != GH_POWERSPHCUTOFF && c != GH_MIYAMOTONAGAI && c != GH_NFW && c != GH_HERNQUIST &&
-        c != GH_LOGHALO)
-      return false;
-    if (h->nonaxi[i]) return false;
+    if (h->types[i] == GH_DEHNENBAR || h->nonaxi[i]) return false;
   }
   return true;
 }
@@ -1054,8 +1066,7 @@ int launch_integrate_dxdv(int nobj, double* d_yo, int nt, const double* d_t, int
   }
   if (!hessian_capable(h)) {
     set_error("integrate_dxdv needs time-independent, axisymmetric components with a Hessian "
-              "(PowerSphericalPotentialwCutoff, MiyamotoNagai, NFW, Hernquist, axisymmetric "
-              "LogarithmicHalo)");
+              "(every supported type but DehnenBar and a triaxial LogarithmicHalo)");
     return -1;
   }
   using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
@@ -1529,8 +1540,7 @@ GH_EXPORT int gh_adjoint_hip(int n, const double* d_wT, const double* d_gT, doub
   }
   if (!hessian_capable(h)) {
     set_error("the adjoint needs time-independent, axisymmetric components with a Hessian "
-              "(PowerSphericalPotentialwCutoff, MiyamotoNagai, NFW, Hernquist, axisymmetric "
-              "LogarithmicHalo)");
+              "(every supported type but DehnenBar and a triaxial LogarithmicHalo)");
     return -1;
   }
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));

@@ -29,9 +29,19 @@ enum GhPotCode {
   GH_LOGHALO = 0,
   GH_DEHNENBAR = 1,
   GH_MIYAMOTONAGAI = 5,
+  GH_POWERSPH = 7,
   GH_HERNQUIST = 8,
   GH_NFW = 9,
+  GH_JAFFE = 10,
+  GH_ISOCHRONE = 14,
   GH_POWERSPHCUTOFF = 15,
+  GH_PLUMMER = 17,
+  GH_PSEUDOISO = 18,
+  GH_KUZMINDISK = 19,
+  GH_BURKERT = 20,
+  GH_DEHNENCORESPH = 33,
+  GH_DEHNENSPH = 34,
+  GH_HOMOGSPHERE = 35,
 };
 
 struct GhComponent {
@@ -330,6 +340,167 @@ __device__ __forceinline__ void gh_comp_force<GH_DEHNENBAR>(const GhPotTable* __
   }
 }
 
+// ---------------------------------------------------------------------------
+// Spherical components (PowerSphericalPotential.c, JaffePotential.c,
+// IsochronePotential.c, PlummerPotential.c, PseudoIsothermalPotential.c,
+// BurkertPotential.c, DehnenCorePotential.c, DehnenPotential.c,
+// HomogeneousSpherePotential.c): F = g(r) x with g = -Phi'(r)/r, and
+// gpr = g'(r)/r for Hessian-vector products (K v = g v + gpr (x.v) x).
+// Each file writes Rforce = -Phi'(r) R/r with its own closed form of Phi'.
+// ---------------------------------------------------------------------------
+template <int TYPE>
+__device__ __forceinline__ void gh_sph_g(const GhComponent* __restrict__ C, double r, double rinv,
+                                         double& g, double& gpr);
+
+template <>
+__device__ __forceinline__ void gh_sph_g<GH_POWERSPH>(const GhComponent* __restrict__ C, double r,
+                                                      double rinv, double& g, double& gpr) {
+  // Phi' = amp r^(1-alpha)
+  const double amp = C->p[0], alpha = C->p[1];
+  const double ra = exp(-alpha * log(r));
+  g = -amp * ra;
+  gpr = alpha * amp * ra * rinv * rinv;
+}
+
+template <>
+__device__ __forceinline__ void gh_sph_g<GH_JAFFE>(const GhComponent* __restrict__ C, double r,
+                                                   double rinv, double& g, double& gpr) {
+  // Phi' = amp / (r (r + a))
+  const double amp = C->p[0], a = C->p[1];
+  const double iar = gh_rcp(a + r);
+  const double ri2 = rinv * rinv;
+  g = -amp * ri2 * iar;
+  gpr = amp * (3.0 * r + 2.0 * a) * ri2 * ri2 * iar * iar;
+}
+
+template <>
+__device__ __forceinline__ void gh_sph_g<GH_ISOCHRONE>(const GhComponent* __restrict__ C, double r,
+                                                       double rinv, double& g, double& gpr) {
+  // Phi' = amp r / (rb (b + rb)^2), rb = sqrt(r^2 + b^2)
+  const double amp = C->p[0], b = C->p[1];
+  const double irb = gh_rsqrt(r * r + C->c[0]);
+  const double rb = (r * r + C->c[0]) * irb;
+  const double ib = gh_rcp(b + rb);
+  g = -amp * irb * ib * ib;
+  gpr = amp * (b + 3.0 * rb) * irb * irb * irb * ib * ib * ib;
+}
+
+template <>
+__device__ __forceinline__ void gh_sph_g<GH_PLUMMER>(const GhComponent* __restrict__ C, double r,
+                                                     double rinv, double& g, double& gpr) {
+  // Phi' = amp r (r^2 + b^2)^(-3/2)
+  const double amp = C->p[0];
+  const double is = gh_rsqrt(r * r + C->c[0]);
+  const double is3 = is * is * is;
+  g = -amp * is3;
+  gpr = 3.0 * amp * is3 * is * is;
+}
+
+template <>
+__device__ __forceinline__ void gh_sph_g<GH_PSEUDOISO>(const GhComponent* __restrict__ C, double r,
+                                                       double rinv, double& g, double& gpr) {
+  // Phi' = amp (1/r - a atan(r/a)/r^2) / a, Phi'' from planarR2deriv;
+  // gpr = (Phi' - r Phi'') / r^3
+  const double amp = C->p[0], a = C->p[1], ia = C->c[0];
+  const double at = atan(r * ia);
+  const double Phip = amp * (rinv - a * rinv * rinv * at) * ia;
+  const double Phipp =
+      amp * rinv * rinv * (2.0 * a * rinv * at - (2.0 * a * a + r * r) / (a * a + r * r)) * ia;
+  g = -Phip * rinv;
+  gpr = (Phip - r * Phipp) * rinv * rinv * rinv;
+}
+
+template <>
+__device__ __forceinline__ void gh_sph_g<GH_BURKERT>(const GhComponent* __restrict__ C, double r,
+                                                     double rinv, double& g, double& gpr) {
+  // Phi' = -amp a pi B / x^2, B = pi - 2 atan(1/x) - 2 log(1+x) - log(1+x^2),
+  // x = r/a; Phi'' from planarR2deriv
+  const double amp = C->p[0], a = C->p[1];
+  const double x = r * C->c[0];
+  const double B = M_PI - 2.0 * atan(1.0 / x) - 2.0 * log1p(x) - log1p(x * x);
+  const double Phip = -amp * a * M_PI * B / (x * x);
+  const double r5 = r * r * r * r * r;
+  const double Phipp =
+      -amp * M_PI * a * a * a / r5 * (-4.0 * r5 / (a * a + r * r) / (a + r) - 2.0 * r * r * B);
+  g = -Phip * rinv;
+  gpr = (Phip - r * Phipp) * rinv * rinv * rinv;
+}
+
+template <>
+__device__ __forceinline__ void gh_sph_g<GH_DEHNENCORESPH>(const GhComponent* __restrict__ C,
+                                                           double r, double rinv, double& g,
+                                                           double& gpr) {
+  // Phi' = amp r / (3 (a + r)^3)
+  const double amp = C->p[0], a = C->p[1];
+  const double iar = gh_rcp(a + r);
+  const double iar3 = iar * iar * iar;
+  g = -amp * iar3 * (1.0 / 3.0);
+  gpr = amp * rinv * iar3 * iar;
+}
+
+template <>
+__device__ __forceinline__ void gh_sph_g<GH_DEHNENSPH>(const GhComponent* __restrict__ C, double r,
+                                                       double rinv, double& g, double& gpr) {
+  // Phi' = amp r (1 + a/r)^alpha / ((a + r)^3 (3 - alpha));
+  // g'/g = -(alpha a + 3 r) / (r (r + a))
+  const double amp = C->p[0], a = C->p[1], alpha = C->p[2];
+  const double iar = gh_rcp(a + r);
+  const double fac = exp(alpha * log(1.0 + a * rinv));
+  g = -amp * fac * iar * iar * iar * C->c[0];
+  gpr = -g * (alpha * a + 3.0 * r) * rinv * rinv * iar;
+}
+
+template <>
+__device__ __forceinline__ void gh_sph_g<GH_HOMOGSPHERE>(const GhComponent* __restrict__ C,
+                                                         double r, double rinv, double& g,
+                                                         double& gpr) {
+  // inside (r^2 < R^2): Phi' = 2 amp r; outside: Phi' = 2 amp R^3 / r^2
+  const double amp = C->p[0], R2 = C->p[1], R3 = C->p[2];
+  if (r * r < R2) {
+    g = -2.0 * amp;
+    gpr = 0.0;
+  } else {
+    const double ri3 = rinv * rinv * rinv;
+    g = -2.0 * amp * R3 * ri3;
+    gpr = 6.0 * amp * R3 * ri3 * rinv * rinv;
+  }
+}
+
+#define GH_SPH_FORCE(CODE)                                                                  \
+  template <>                                                                               \
+  __device__ __forceinline__ void gh_comp_force<CODE>(const GhPotTable* __restrict__ T, int i, \
+                                                      const GhGeom& g, GhAcc& F) {          \
+    double gr, gpr;                                                                         \
+    gh_sph_g<CODE>(&T->comp[i], g.r, g.rinv, gr, gpr);                                      \
+    F.FRoR += gr;                                                                           \
+    F.Fz += gr * g.z;                                                                       \
+  }
+GH_SPH_FORCE(GH_POWERSPH)
+GH_SPH_FORCE(GH_JAFFE)
+GH_SPH_FORCE(GH_ISOCHRONE)
+GH_SPH_FORCE(GH_PLUMMER)
+GH_SPH_FORCE(GH_PSEUDOISO)
+GH_SPH_FORCE(GH_BURKERT)
+GH_SPH_FORCE(GH_DEHNENCORESPH)
+GH_SPH_FORCE(GH_DEHNENSPH)
+GH_SPH_FORCE(GH_HOMOGSPHERE)
+#undef GH_SPH_FORCE
+
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_KUZMINDISK>(const GhPotTable* __restrict__ T,
+                                                             int i, const GhGeom& g, GhAcc& F) {
+  // KuzminDiskPotential.c:16-37: Phi = -amp (R^2 + (a + |z|)^2)^(-1/2);
+  // zforce carries sign(z) (0 in the plane)
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0];
+  const double az = C->p[1] + fabs(g.z);
+  const double iD = gh_rsqrt(g.R2 + az * az);
+  const double D32 = iD * iD * iD;
+  const double w = (g.z == 0.0) ? 0.0 : copysign(az, g.z);
+  F.FRoR += -amp * D32;
+  F.Fz += -amp * w * D32;
+}
+
 // run-time dispatch on the component type (generic kernels)
 __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__ T, int i,
                                                   const GhGeom& g, GhAcc& F) {
@@ -340,6 +511,16 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
     case GH_HERNQUIST: gh_comp_force<GH_HERNQUIST>(T, i, g, F); break;
     case GH_LOGHALO: gh_comp_force<GH_LOGHALO>(T, i, g, F); break;
     case GH_DEHNENBAR: gh_comp_force<GH_DEHNENBAR>(T, i, g, F); break;
+    case GH_POWERSPH: gh_comp_force<GH_POWERSPH>(T, i, g, F); break;
+    case GH_JAFFE: gh_comp_force<GH_JAFFE>(T, i, g, F); break;
+    case GH_ISOCHRONE: gh_comp_force<GH_ISOCHRONE>(T, i, g, F); break;
+    case GH_PLUMMER: gh_comp_force<GH_PLUMMER>(T, i, g, F); break;
+    case GH_PSEUDOISO: gh_comp_force<GH_PSEUDOISO>(T, i, g, F); break;
+    case GH_KUZMINDISK: gh_comp_force<GH_KUZMINDISK>(T, i, g, F); break;
+    case GH_BURKERT: gh_comp_force<GH_BURKERT>(T, i, g, F); break;
+    case GH_DEHNENCORESPH: gh_comp_force<GH_DEHNENCORESPH>(T, i, g, F); break;
+    case GH_DEHNENSPH: gh_comp_force<GH_DEHNENSPH>(T, i, g, F); break;
+    case GH_HOMOGSPHERE: gh_comp_force<GH_HOMOGSPHERE>(T, i, g, F); break;
     default: break;
   }
 }
@@ -479,6 +660,54 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
         const double zq2 = z2 * C->c[0];
         const double Rt2 = (onem1overb2 < 1.0) ? R2 * (1.0 - onem1overb2 * sinphi * sinphi) : R2;
         pot += 0.5 * amp * log(Rt2 + zq2 + core2);
+        break;
+      }
+      case GH_KUZMINDISK: {  // KuzminDiskPotential.c:5-15
+        const double az = C->p[1] + fabs(z);
+        pot += -C->p[0] / sqrt(R2 + az * az);
+        break;
+      }
+      case GH_POWERSPH: {  // PowerSphericalPotential.c Eval
+        const double alpha = C->p[1];
+        pot += (alpha == 2.0) ? 0.5 * C->p[0] * log(r2)
+                              : -C->p[0] * pow(r2, 1.0 - 0.5 * alpha) / (alpha - 2.0);
+        break;
+      }
+      case GH_JAFFE: {
+        const double r = sqrt(r2);
+        pot += -C->p[0] * log(1.0 + C->p[1] / r) / C->p[1];
+        break;
+      }
+      case GH_ISOCHRONE: pot += -C->p[0] / (C->p[1] + sqrt(r2 + C->c[0])); break;
+      case GH_PLUMMER: pot += -C->p[0] / sqrt(r2 + C->c[0]); break;
+      case GH_PSEUDOISO: {
+        const double a = C->p[1], r = sqrt(r2);
+        pot += C->p[0] * (0.5 * log(1.0 + r2 / (a * a)) + a / r * atan(r / a)) / a;
+        break;
+      }
+      case GH_BURKERT: {
+        const double a = C->p[1], x = sqrt(r2) / a;
+        pot += -C->p[0] * a * a * M_PI / x *
+               (-M_PI + 2.0 * (1.0 + x) * atan(1.0 / x) + 2.0 * (1.0 + x) * log1p(x) +
+                (1.0 - x) * log1p(x * x));
+        break;
+      }
+      case GH_DEHNENCORESPH: {
+        const double a = C->p[1], r = sqrt(r2), u = r / (r + a);
+        pot += -C->p[0] * (1.0 - u * u) / (6.0 * a);
+        break;
+      }
+      case GH_DEHNENSPH: {
+        const double a = C->p[1], alpha = C->p[2], r = sqrt(r2);
+        if (alpha == 2.0)  // the Jaffe limit of the closed form (0/0 at alpha = 2)
+          pot += -C->p[0] * log1p(a / r) / a;
+        else
+          pot += -C->p[0] * (1.0 - pow(r / (r + a), 2.0 - alpha)) / (a * (2.0 - alpha) * (3.0 - alpha));
+        break;
+      }
+      case GH_HOMOGSPHERE: {
+        const double R2s = C->p[1], R3s = C->p[2];
+        pot += (r2 < R2s) ? C->p[0] * (r2 - 3.0 * R2s) : -2.0 * C->p[0] * R3s / sqrt(r2);
         break;
       }
       case GH_DEHNENBAR: {

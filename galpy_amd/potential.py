@@ -1,11 +1,15 @@
 """Potentials for the MI355X orbit engine and galpy's C packing of them.
 
-The classes mirror galpy's potential objects for the six force kernels the
+The classes mirror galpy's potential objects for the force kernels the
 engine implements on the GPU: their constructor arguments, amplitude
 conventions and ``normalize`` behave as in galpy
 (galpy/potential/Potential.py:902-920, PowerSphericalPotentialwCutoff.py:30-70,
-MiyamotoNagaiPotential.py:24-66, TwoPowerSphericalPotential.py:526,776-851,
-LogarithmicHaloPotential.py:12,32-75, DehnenBarPotential.py:55-146).
+MiyamotoNagaiPotential.py:24-66, TwoPowerSphericalPotential.py:301-851,
+LogarithmicHaloPotential.py:12,32-75, DehnenBarPotential.py:55-146,
+PowerSphericalPotential.py:25-60,400-440, IsochronePotential.py:24-53,
+PlummerPotential.py:23-51, PseudoIsothermalPotential.py:20-53,
+KuzminDiskPotential.py:24-52, BurkertPotential.py:20-53,
+HomogeneousSpherePotential.py:23-52).
 Host-side evaluation (normalization, energies of a few points) uses numpy and
 scipy like galpy's Python layer; the integration itself runs in HIP.
 
@@ -351,6 +355,261 @@ class DehnenBarPotential(Potential):
 
     def c_args(self):
         return [self._amp * self._af, self._tform, self._tsteady, self._rb, self._omegab, self._barphi]
+
+
+def _set_normalize(pot, normalize):
+    if normalize or (isinstance(normalize, (int, float)) and not isinstance(normalize, bool)):
+        pot.normalize(normalize)
+
+
+class _SphericalPotential(Potential):
+    """Spherical components given by Phi(r) / amp (``_phi``) and
+    dPhi/dr / amp (``_dphi``); forces are -Phi'(r) (R, z) / r."""
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return self._phi(numpy.sqrt(R**2.0 + z**2.0))
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        r = numpy.sqrt(R**2.0 + z**2.0)
+        return -self._dphi(r) * R / r
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        r = numpy.sqrt(R**2.0 + z**2.0)
+        return -self._dphi(r) * z / r
+
+
+class PowerSphericalPotential(_SphericalPotential):
+    """rho ~ r^-alpha; pot_type 7, args [amp, alpha].  amp is the mass
+    within r1 unless alpha = 3 (PowerSphericalPotential.py:25-60)."""
+
+    pot_code = 7
+
+    def __init__(self, amp=1.0, alpha=1.0, normalize=False, r1=1.0):
+        Potential.__init__(self, amp=amp)
+        self.alpha = alpha
+        if self.alpha != 3.0:
+            self._amp *= r1 ** (self.alpha - 3.0) * 4.0 * numpy.pi / (3.0 - self.alpha)
+        _set_normalize(self, normalize)
+
+    def _phi(self, r):
+        if self.alpha == 2.0:
+            return numpy.log(r)
+        return -(r ** (2.0 - self.alpha)) / (self.alpha - 2.0)
+
+    def _dphi(self, r):
+        return r ** (1.0 - self.alpha)
+
+    def c_args(self):
+        return [self._amp, self.alpha]
+
+
+class KeplerPotential(PowerSphericalPotential):
+    """Phi = -amp / r: PowerSphericalPotential with alpha = 3
+    (PowerSphericalPotential.py:400-440), packed as pot_type 7."""
+
+    def __init__(self, amp=1.0, normalize=False):
+        PowerSphericalPotential.__init__(self, amp=amp, alpha=3.0, normalize=normalize)
+
+
+class JaffePotential(_SphericalPotential):
+    """Phi = -amp log(1 + a/r) / a; pot_type 10, args [amp, a]."""
+
+    pot_code = 10
+
+    def __init__(self, amp=1.0, a=1.0, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self.a = a
+        _set_normalize(self, normalize)
+
+    def _phi(self, r):
+        return -numpy.log(1.0 + self.a / r) / self.a
+
+    def _dphi(self, r):
+        return 1.0 / (r * (r + self.a))
+
+    def c_args(self):
+        return [self._amp, self.a]
+
+
+class IsochronePotential(_SphericalPotential):
+    """Phi = -amp / (b + sqrt(r^2 + b^2)); pot_type 14, args [amp, b]."""
+
+    pot_code = 14
+
+    def __init__(self, amp=1.0, b=1.0, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self.b = b
+        _set_normalize(self, normalize)
+
+    def _phi(self, r):
+        return -1.0 / (self.b + numpy.sqrt(r**2.0 + self.b**2.0))
+
+    def _dphi(self, r):
+        rb = numpy.sqrt(r**2.0 + self.b**2.0)
+        return r / rb / (self.b + rb) ** 2.0
+
+    def c_args(self):
+        return [self._amp, self.b]
+
+
+class PlummerPotential(_SphericalPotential):
+    """Phi = -amp / sqrt(r^2 + b^2); pot_type 17, args [amp, b]."""
+
+    pot_code = 17
+
+    def __init__(self, amp=1.0, b=0.8, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self._b = b
+        self._b2 = b**2.0
+        _set_normalize(self, normalize)
+
+    def _phi(self, r):
+        return -1.0 / numpy.sqrt(r**2.0 + self._b2)
+
+    def _dphi(self, r):
+        return r * (r**2.0 + self._b2) ** -1.5
+
+    def c_args(self):
+        return [self._amp, self._b]
+
+
+class PseudoIsothermalPotential(_SphericalPotential):
+    """rho ~ 1 / (1 + r^2/a^2); pot_type 18, args [amp, a]."""
+
+    pot_code = 18
+
+    def __init__(self, amp=1.0, a=1.0, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self._a = a
+        self._a2 = a**2.0
+        _set_normalize(self, normalize)
+
+    def _phi(self, r):
+        r = numpy.asarray(r, dtype=float)
+        with numpy.errstate(divide="ignore", invalid="ignore"):
+            out = (0.5 * numpy.log(1.0 + r**2.0 / self._a2) + self._a / r * numpy.arctan(r / self._a)) / self._a
+        return numpy.where(r == 0.0, 1.0 / self._a, out)
+
+    def _dphi(self, r):
+        return (1.0 / r - self._a / r**2.0 * numpy.arctan(r / self._a)) / self._a
+
+    def c_args(self):
+        return [self._amp, self._a]
+
+
+class KuzminDiskPotential(Potential):
+    """Phi = -amp / sqrt(R^2 + (a + |z|)^2); pot_type 19, args [amp, a]."""
+
+    pot_code = 19
+
+    def __init__(self, amp=1.0, a=1.0, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self._a = a
+        _set_normalize(self, normalize)
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return -1.0 / numpy.sqrt(R**2.0 + (self._a + numpy.fabs(z)) ** 2.0)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return -R * (R**2.0 + (self._a + numpy.fabs(z)) ** 2.0) ** -1.5
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        az = self._a + numpy.fabs(z)
+        return -numpy.sign(z) * az * (R**2.0 + az**2.0) ** -1.5
+
+    def c_args(self):
+        return [self._amp, self._a]
+
+
+class BurkertPotential(_SphericalPotential):
+    """rho = amp / ((1 + r/a)(1 + (r/a)^2)); pot_type 20, args [amp, a]."""
+
+    pot_code = 20
+
+    def __init__(self, amp=1.0, a=2.0, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self.a = a
+        _set_normalize(self, normalize)
+
+    def _phi(self, r):
+        x = r / self.a
+        return -(self.a**2.0) * numpy.pi / x * (
+            -numpy.pi + 2.0 * (1.0 + x) * numpy.arctan(1.0 / x) + 2.0 * (1.0 + x) * numpy.log(1.0 + x)
+            + (1.0 - x) * numpy.log(1.0 + x**2.0)
+        )
+
+    def _dphi(self, r):
+        # enclosed mass / r^2
+        x = r / self.a
+        return (numpy.pi * self.a**3.0 / r**2.0
+                * (2.0 * numpy.log(1.0 + x) + numpy.log(1.0 + x**2.0) - 2.0 * numpy.arctan(x)))
+
+    def c_args(self):
+        return [self._amp, self.a]
+
+
+class DehnenSphericalPotential(_SphericalPotential):
+    """Dehnen (1993) profile with inner slope alpha; pot_type 34, args
+    [amp, a, alpha] (TwoPowerSphericalPotential.py:301-355)."""
+
+    pot_code = 34
+
+    def __init__(self, amp=1.0, a=1.0, alpha=1.5, normalize=False):
+        if (alpha < 0.0) or (alpha >= 3.0):
+            raise OSError("DehnenSphericalPotential requires 0 <= alpha < 3")
+        Potential.__init__(self, amp=amp)
+        self.a = a
+        self.alpha = alpha
+        _set_normalize(self, normalize)
+
+    def _phi(self, r):
+        if self.alpha == 2.0:  # the Jaffe limit
+            return -numpy.log(1.0 + self.a / r) / self.a
+        return -(1.0 - (r / (r + self.a)) ** (2.0 - self.alpha)) / (
+            self.a * (2.0 - self.alpha) * (3.0 - self.alpha))
+
+    def _dphi(self, r):
+        return r ** (1.0 - self.alpha) * (r + self.a) ** (self.alpha - 3.0) / (3.0 - self.alpha)
+
+    def c_args(self):
+        return [self._amp, self.a, self.alpha]
+
+
+class DehnenCoreSphericalPotential(DehnenSphericalPotential):
+    """The cored (alpha = 0) Dehnen profile; pot_type 33, args [amp, a]
+    (TwoPowerSphericalPotential.py:436-470)."""
+
+    pot_code = 33
+
+    def __init__(self, amp=1.0, a=1.0, normalize=False):
+        DehnenSphericalPotential.__init__(self, amp=amp, a=a, alpha=0.0, normalize=normalize)
+
+    def c_args(self):
+        return [self._amp, self.a]
+
+
+class HomogeneousSpherePotential(_SphericalPotential):
+    """Uniform-density sphere of radius R; pot_type 35, args [amp, R^2, R^3]."""
+
+    pot_code = 35
+
+    def __init__(self, amp=1.0, R=1.1, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self.R = R
+        self._R2 = R**2.0
+        self._R3 = R**3.0
+        _set_normalize(self, normalize)
+
+    def _phi(self, r):
+        r = numpy.asarray(r, dtype=float)
+        return numpy.where(r**2.0 < self._R2, r**2.0 - 3.0 * self._R2, -2.0 * self._R3 / numpy.maximum(r, 1e-300))
+
+    def _dphi(self, r):
+        r = numpy.asarray(r, dtype=float)
+        return numpy.where(r**2.0 < self._R2, 2.0 * r, 2.0 * self._R3 / numpy.maximum(r, 1e-300) ** 2.0)
+
+    def c_args(self):
+        return [self._amp, self._R2, self._R3]
 
 
 def MWPotential2014_factory():

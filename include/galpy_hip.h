@@ -12,7 +12,12 @@
  * Supported potentials (pot_type codes, integrateFullOrbit.c:80-345,
  * integratePlanarOrbit.c:64-223):
  *   0 LogarithmicHaloPotential, 1 DehnenBarPotential, 5 MiyamotoNagaiPotential,
- *   8 HernquistPotential, 9 NFWPotential, 15 PowerSphericalPotentialwCutoff.
+ *   7 PowerSphericalPotential (and KeplerPotential), 8 HernquistPotential,
+ *   9 NFWPotential, 10 JaffePotential, 14 IsochronePotential,
+ *   15 PowerSphericalPotentialwCutoff, 17 PlummerPotential,
+ *   18 PseudoIsothermalPotential, 19 KuzminDiskPotential, 20 BurkertPotential,
+ *   33 DehnenCoreSphericalPotential, 34 DehnenSphericalPotential,
+ *   35 HomogeneousSpherePotential.
  * Supported odeint_type codes (integrateFullOrbit.c:1091-1132,
  * integratePlanarOrbit.c:788-825):
  *   0 leapfrog, 3 symplec4, 6 dop853.
@@ -71,9 +76,9 @@ void integratePlanarOrbit(int nobj, double *yo, int nt, double *t, int indiv_t, 
  * step estimated from the orbit alone, orbit bit-identical to a plain run);
  * 6: DOP853 on the 12-dim variational system (evalRectDeriv_dxdv,
  * integrateFullOrbit.c:1496-1600).  The kick of the deviation is K dq with K
- * the Cartesian Hessian of -Phi: components need a Hessian on the device
- * (PowerSphericalPotentialwCutoff, MiyamotoNagai, NFW, Hernquist,
- * axisymmetric LogarithmicHalo); others set *err = -1. */
+ * the Cartesian Hessian of -Phi: every time-independent axisymmetric
+ * component has one on the device; DehnenBar and a triaxial
+ * LogarithmicHalo set *err = -1. */
 void integrateFullOrbit_dxdv(double *yo, int nt, double *t, int npot, int *pot_type,
                              double *pot_args, void *pot_tfuncs, double dt, double rtol,
                              double atol, double *result, int *err, int odeint_type);
@@ -181,7 +186,8 @@ int gh_flow_hip(int n, const double *d_w0, double *d_wT, double t0, double dt, l
  * d_lam0 = dL/dw(0) ([n][6]), d_w0rec = the recomputed initial states
  * (nullable) and d_dtheta[n][8][3] = dL/d(packed argument j of component c)
  * per orbit (nullable; slots: MiyamotoNagai amp,a,b; NFW amp,a; Hernquist
- * amp,a; PowerSphericalPotentialwCutoff and LogarithmicHalo amp). */
+ * amp,a; every other component amp only).  DehnenBar and a triaxial
+ * LogarithmicHalo are rejected. */
 int gh_adjoint_hip(int n, const double *d_wT, const double *d_gT, double dt, long long nsteps,
                    const void *pot, int method, double *d_lam0, double *d_w0rec, double *d_dtheta,
                    void *stream);

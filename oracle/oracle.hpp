@@ -31,7 +31,7 @@ namespace orc {
 
 using std::sqrt; using std::log; using std::exp; using std::pow;
 using std::fabs; using std::sin; using std::cos; using std::acos;
-using std::atan2; using std::tgamma;
+using std::atan2; using std::atan; using std::tgamma;
 
 inline double dval(double x) { return x; }
 
@@ -76,6 +76,7 @@ inline CountedD sin(CountedD a) { g_ops.special++; return CountedD(std::sin(a.v)
 inline CountedD cos(CountedD a) { g_ops.special++; return CountedD(std::cos(a.v)); }
 inline CountedD acos(CountedD a) { g_ops.special++; return CountedD(std::acos(a.v)); }
 inline CountedD atan2(CountedD a, CountedD b) { g_ops.special++; return CountedD(std::atan2(a.v, b.v)); }
+inline CountedD atan(CountedD a) { g_ops.special++; return CountedD(std::atan(a.v)); }
 
 template <class R> inline R rmax(R a, R b) { return (a > b) ? a : b; }
 template <class R> inline R rmin(R a, R b) { return (a < b) ? a : b; }
@@ -147,8 +148,10 @@ template <class R> R upper_incgamma(double a, R x, double gamma_a) {
 // and kept read-only; constants that the reference recomputes on every call
 // (Gamma(a), 2 pi rc^(3-alpha)) are precomputed with the same operations.
 // ---------------------------------------------------------------------------
-enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kHernquist = 8,
-               kNFW = 9, kPowerSphCutoff = 15 };
+enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, kHernquist = 8,
+               kNFW = 9, kJaffe = 10, kIsochrone = 14, kPowerSphCutoff = 15, kPlummer = 17,
+               kPseudoIso = 18, kKuzminDisk = 19, kBurkert = 20, kDehnenCoreSph = 33,
+               kDehnenSph = 34, kHomogSphere = 35 };
 
 struct Component {
   int type;
@@ -167,6 +170,9 @@ inline int component_nargs(int code, bool planar = false) {
     case kHernquist: return 2;
     case kNFW: return 2;
     case kPowerSphCutoff: return planar ? 3 : 5;
+    case kPowerSph: case kJaffe: case kIsochrone: case kPlummer: case kPseudoIso:
+    case kKuzminDisk: case kBurkert: case kDehnenCoreSph: return 2;
+    case kDehnenSph: case kHomogSphere: return 3;
     default: return -1;
   }
 }
@@ -333,6 +339,160 @@ template <class R> R bar_eval(const Component& C, R Rc, R z, R phi, R t) {
   return out * R(amp) * smooth * cos(R(2.) * (phi - R(omegab) * t - R(barphi)));
 }
 
+// ---------------------------------------------------------------------------
+// Spherical components (codes 7, 10, 14, 17, 18, 20, 33, 34, 35): each
+// reference file writes Rforce = f(r) R, zforce = f(r) z, planarRforce =
+// -Phi'(r) and planarR2deriv = Phi''(r), and builds R2deriv / z2deriv /
+// Rzderiv from Phi' and Phi'' (e.g. IsochronePotential.c:60-92).  sph_dphi
+// restates planarRforce (negated) and planarR2deriv at radius r; the 3D
+// force is -Phi'(r) x / r.
+// ---------------------------------------------------------------------------
+template <class R> void sph_dphi(const Component& C, R r, R& Phip, R& Phipp) {
+  double amp = C.p[0];
+  switch (C.type) {
+    case kPowerSph: {  // PowerSphericalPotential.c: planarRforce, planarR2deriv
+      double alpha = C.p[1];
+      Phip = R(amp) * pow(r, R(-alpha + 1.));
+      Phipp = R(amp) * R(1. - alpha) * pow(r, R(-alpha));
+      break;
+    }
+    case kJaffe: {  // JaffePotential.c
+      double a = C.p[1];
+      Phip = R(amp) * pow(r, R(-2.)) / (R(1.) + R(a) / r);
+      Phipp = -R(amp) * (R(a) + R(2.) * r) * pow(r, R(-4.)) * pow(R(1.) + R(a) / r, R(-2.));
+      break;
+    }
+    case kIsochrone: {  // IsochronePotential.c
+      double b = C.p[1];
+      R r2 = r * r;
+      R rb = sqrt(r2 + R(b * b));
+      Phip = R(amp) * r / rb * pow(R(b) + rb, R(-2.));
+      Phipp = -R(amp) * (-R(b * b * b) - R(b * b) * rb + R(2.) * r2 * rb) * pow(rb * (R(b) + rb), R(-3.));
+      break;
+    }
+    case kPlummer: {  // PlummerPotential.c
+      double b2 = C.p[1] * C.p[1];
+      Phip = R(amp) * r * pow(r * r + R(b2), R(-1.5));
+      Phipp = R(amp) * (R(b2) - R(2.) * r * r) * pow(r * r + R(b2), R(-2.5));
+      break;
+    }
+    case kPseudoIso: {  // PseudoIsothermalPotential.c
+      double a = C.p[1], a2 = a * a;
+      R at = atan(r / R(a));
+      Phip = R(amp) * (R(1.) / r - R(a) / r / r * at) / R(a);
+      Phipp = R(amp) / (r * r) * (R(2. * a) / r * at - (R(2. * a2) + r * r) / (R(a2) + r * r)) / R(a);
+      break;
+    }
+    case kBurkert: {  // BurkertPotential.c
+      double a = C.p[1];
+      R x = r / R(a);
+      R B = R(M_PI) - R(2.) * atan(R(1.) / x) - R(2.) * log(R(1.) + x) - log(R(1.) + x * x);
+      Phip = -R(amp) * R(a) * R(M_PI) / x / x * B;
+      R r5 = pow(r, R(5));
+      Phipp = -R(amp) * R(M_PI) * R(a * a * a) / r5 *
+              (R(-4.) * r5 / (R(a * a) + r * r) / (R(a) + r) - R(2.) * r * r * B);
+      break;
+    }
+    case kDehnenCoreSph: {  // DehnenCorePotential.c
+      double a = C.p[1];
+      Phip = R(amp) * (r / pow(R(a) + r, R(3.)) / R(3.));
+      Phipp = -R(amp) * (pow(R(a) + r, R(-4.)) * (R(2.) * r - R(a))) / R(3.);
+      break;
+    }
+    case kDehnenSph: {  // DehnenPotential.c
+      double a = C.p[1], alpha = C.p[2];
+      Phip = R(amp) * (r * pow(R(1.) + R(a) / r, R(alpha)) / pow(R(a) + r, R(3.)) / R(3. - alpha));
+      Phipp = -R(amp) * (pow(R(1.) + R(a) / r, R(alpha)) * pow(R(a) + r, R(-4.)) *
+                         (R(2.) * r + R(a * (alpha - 1.)))) / R(3. - alpha);
+      break;
+    }
+    case kHomogSphere: {  // HomogeneousSpherePotential.c
+      double R2 = C.p[1], R3 = C.p[2];
+      if (r * r < R(R2)) {
+        Phip = R(2.) * R(amp) * r;
+        Phipp = R(2.) * R(amp);
+      } else {
+        Phip = R(2.) * R(amp) * R(R3) / (r * r);
+        Phipp = R(-4.) * R(amp) * R(R3) / pow(r * r, R(1.5));
+      }
+      break;
+    }
+    default:
+      Phip = 0.;
+      Phipp = 0.;
+  }
+}
+
+inline bool is_spherical_new(int code) {
+  return code == kPowerSph || code == kJaffe || code == kIsochrone || code == kPlummer ||
+         code == kPseudoIso || code == kBurkert || code == kDehnenCoreSph || code == kDehnenSph ||
+         code == kHomogSphere;
+}
+
+// 3D forces of the spherical components as their files write them
+// (Rforce = -amp R ..., zforce = -amp z ...): F = -Phi'(r) (R, z) / r.
+template <class R> void sph_forces(const Component& C, R Rc, R z, R& FR, R& Fz) {
+  R r = sqrt(Rc * Rc + z * z);
+  R Phip, Phipp;
+  sph_dphi(C, r, Phip, Phipp);
+  FR = -Phip * Rc / r;
+  Fz = -Phip * z / r;
+}
+
+// Potential values (the files' Eval functions).
+template <class R> R sph_eval(const Component& C, R r) {
+  double amp = C.p[0];
+  switch (C.type) {
+    case kPowerSph: {
+      double alpha = C.p[1];
+      if (alpha == 2.) return R(0.5) * R(amp) * log(r * r);
+      return -R(amp) * pow(r * r, R(1. - 0.5 * alpha)) / R(alpha - 2.);
+    }
+    case kJaffe: return -R(amp) * log(R(1.) + R(C.p[1]) / r) / R(C.p[1]);
+    case kIsochrone: return -R(amp) / (R(C.p[1]) + sqrt(r * r + R(C.p[1] * C.p[1])));
+    case kPlummer: return -R(amp) / sqrt(r * r + R(C.p[1] * C.p[1]));
+    case kPseudoIso: {
+      double a = C.p[1];
+      return R(amp) * (R(0.5) * log(R(1.) + r * r / R(a * a)) + R(a) / r * atan(r / R(a))) / R(a);
+    }
+    case kBurkert: {
+      double a = C.p[1];
+      R x = r / R(a);
+      return -R(amp) * R(a * a * M_PI) / x *
+             (R(-M_PI) + R(2.) * (R(1.) + x) * atan(R(1.) / x) + R(2.) * (R(1.) + x) * log(R(1.) + x) +
+              (R(1.) - x) * log(R(1.) + x * x));
+    }
+    case kDehnenCoreSph: {
+      double a = C.p[1];
+      return -R(amp) * (R(1.) - pow(r / (r + R(a)), R(2.))) / R(6. * a);
+    }
+    case kDehnenSph: {
+      double a = C.p[1], alpha = C.p[2];
+      return -R(amp) * (R(1.) - pow(r / (r + R(a)), R(2. - alpha))) / R(a * (2. - alpha) * (3. - alpha));
+    }
+    case kHomogSphere: {
+      double R2 = C.p[1], R3 = C.p[2];
+      if (r * r < R(R2)) return R(amp) * (r * r - R(3. * R2));
+      return R(-2.) * R(amp) * R(R3) / r;
+    }
+    default: return R(0.);
+  }
+}
+
+// KuzminDiskPotential.c: Eval, Rforce, zforce, planarRforce and the second
+// derivatives (R2deriv, z2deriv, Rzderiv)
+template <class R> void kuzmin_forces(const Component& C, R Rc, R z, R& FR, R& Fz) {
+  double amp = C.p[0], a = C.p[1];
+  R az = R(a) + fabs(z);
+  R dm = pow(Rc * Rc + pow(az, R(2.)), R(-1.5));
+  FR = -R(amp) * Rc * dm;
+  R zsign = (z > R(0.)) ? R(1.) : ((z < R(0.)) ? R(-1.) : R(0.));
+  Fz = -zsign * R(amp) * dm * az;
+}
+template <class R> R kuzmin_eval(const Component& C, R Rc, R z) {
+  return -R(C.p[0]) * pow(Rc * Rc + pow(R(C.p[1]) + fabs(z), R(2.)), R(-0.5));
+}
+
 // Aggregated cylindrical forces: galpy_potentials.c:120-179 sums each of
 // Rforce, phitorque, zforce over components in index order starting from 0.
 // Summing the three per component gives the identical three sums.
@@ -355,6 +515,9 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kHernquist: hq_forces(C, Rc, z, fr, fz); break;
       case kLogHalo: lh_forces(C, Rc, z, phi, fr, fz, fp); break;
       case kDehnenBar: bar_forces(C, Rc, z, phi, t, fr, fz, fp); break;
+      case kKuzminDisk: kuzmin_forces(C, Rc, z, fr, fz); break;
+      default:
+        if (is_spherical_new(C.type)) sph_forces(C, Rc, z, fr, fz);
     }
     FR += fr; Fz += fz; Fphi += fp;
   }
@@ -427,6 +590,17 @@ bool cyl_hessian(int npot, const Component* P, R Rc, R z, R& R2, R& z2, R& Rz) {
         Rz += Rc * z * (d2phi / r2 - dphi / (r2 * r));
         break;
       }
+      case kKuzminDisk: {  // KuzminDiskPotential.c R2deriv, z2deriv, Rzderiv
+        double amp = C.p[0], a = C.p[1];
+        R az = R(a) + fabs(z);
+        R denom = Rc * Rc + az * az;
+        R dm15 = R(1.) / (denom * sqrt(denom));
+        R zsign = (z > R(0.)) ? R(1.) : ((z < R(0.)) ? R(-1.) : R(0.));
+        R2 += R(amp) * (dm15 - R(3.) * Rc * Rc * dm15 / denom);
+        z2 += R(amp) * (dm15 - R(3.) * az * az * dm15 / denom);
+        Rz += R(amp) * (R(-3.) * zsign * Rc * az * R(1.) / (denom * denom * sqrt(denom)));
+        break;
+      }
       case kLogHalo: {
         double amp = C.p[0], q = C.p[1], c = C.p[2], onem1overb2 = C.p[3];
         if (onem1overb2 < 1) return false;
@@ -436,8 +610,17 @@ bool cyl_hessian(int npot, const Component* P, R Rc, R z, R& R2, R& z2, R& Rz) {
         Rz += -R(amp) * R(2.) * Rc * z / R(q * q) / (D * D);
         break;
       }
-      default:
-        return false;
+      default: {
+        if (!is_spherical_new(C.type)) return false;
+        // R2deriv = Phi'' R^2/r^2 + Phi' z^2/r^3, z2deriv = Phi'' z^2/r^2 +
+        // Phi' R^2/r^3, Rzderiv = R z (Phi''/r^2 - Phi'/r^3)
+        R Phip, Phipp;
+        sph_dphi(C, r, Phip, Phipp);
+        R ir2 = R(1.) / r2, ir3 = ir2 / r;
+        R2 += Phipp * Rc * Rc * ir2 + Phip * z * z * ir3;
+        z2 += Phipp * z * z * ir2 + Phip * Rc * Rc * ir3;
+        Rz += Rc * z * (Phipp * ir2 - Phip * ir3);
+      }
     }
   }
   return true;
@@ -545,6 +728,15 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
       case kHernquist: fr = hq_planar_Rforce(C, Rc); break;
       case kLogHalo: lh_planar_forces(C, Rc, phi, fr, fp); break;
       case kDehnenBar: bar_planar_forces(C, Rc, phi, t, fr, fp); break;
+      case kKuzminDisk:  // KuzminDiskPotential.c planarRforce
+        fr = -R(C.p[0]) * Rc * pow(Rc * Rc + R(C.p[1] * C.p[1]), R(-1.5));
+        break;
+      default:
+        if (is_spherical_new(C.type)) {  // planarRforce = -Phi'(R)
+          R Phip, Phipp;
+          sph_dphi(C, Rc, Phip, Phipp);
+          fr = -Phip;
+        }
     }
     FR += fr; Fphi += fp;
   }
@@ -667,6 +859,9 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kHernquist: pot += hq_eval(C, Rc, z); break;
       case kLogHalo: pot += lh_eval(C, Rc, z, phi); break;
       case kDehnenBar: pot += bar_eval(C, Rc, z, phi, t); break;
+      case kKuzminDisk: pot += kuzmin_eval(C, Rc, z); break;
+      default:
+        if (is_spherical_new(C.type)) pot += sph_eval(C, sqrt(Rc * Rc + z * z));
     }
   }
   return pot;

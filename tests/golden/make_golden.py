@@ -6,10 +6,13 @@
 incgamma.json    lower/upper incomplete gamma gamma(a,x), Gamma(a,x) for the
                  exponents of PowerSphericalPotentialwCutoff (a = 1.5 - alpha/2,
                  1 - alpha/2) over x in [1e-8, 1e3].
-potentials.json  forces and potential values of the six supported potentials
-                 at seeded points, from closed forms in mpmath; forces are the
+potentials.json  forces and potential values of the supported potentials at
+                 seeded points, from closed forms in mpmath; forces are the
                  50-digit numerical derivatives of the closed-form potential;
-                 second derivatives R2/z2/Rz for the axisymmetric ones.
+                 second derivatives R2/z2/Rz for the axisymmetric ones.  The
+                 spherical and Kuzmin closed forms are checked against
+                 Poisson's equation with their documented densities when the
+                 file is written.
 """
 
 import json
@@ -79,6 +82,92 @@ def bar_phi(R, z, phi, t, ampaf, tform, tsteady, rb, omegab, barphi):
     return ampaf * smooth * mp.cos(2 * (phi - omegab * t - barphi)) * out
 
 
+def _r(R, z):
+    return mp.sqrt(R * R + z * z)
+
+
+def powersph_phi(R, z, amp, alpha):
+    r = _r(R, z)
+    if alpha == 2:
+        return amp * mp.log(r)
+    return -amp * r ** (2 - alpha) / (alpha - 2)
+
+
+def jaffe_phi(R, z, amp, a):
+    return -amp * mp.log(1 + a / _r(R, z)) / a
+
+
+def isochrone_phi(R, z, amp, b):
+    return -amp / (b + mp.sqrt(R * R + z * z + b * b))
+
+
+def plummer_phi(R, z, amp, b):
+    return -amp / mp.sqrt(R * R + z * z + b * b)
+
+
+def pseudoiso_phi(R, z, amp, a):
+    r = _r(R, z)
+    return amp * (mp.log(1 + r * r / (a * a)) / 2 + a / r * mp.atan(r / a)) / a
+
+
+def kuzmin_phi(R, z, amp, a):
+    return -amp / mp.sqrt(R * R + (a + abs(z)) ** 2)
+
+
+def burkert_phi(R, z, amp, a):
+    x = _r(R, z) / a
+    return -amp * a * a * mp.pi / x * (
+        -mp.pi + 2 * (1 + x) * mp.atan(1 / x) + 2 * (1 + x) * mp.log(1 + x) + (1 - x) * mp.log(1 + x * x))
+
+
+def dehnen_phi(R, z, amp, a, alpha):
+    r = _r(R, z)
+    return -amp * (1 - (r / (r + a)) ** (2 - alpha)) / (a * (2 - alpha) * (3 - alpha))
+
+
+def homog_phi(R, z, amp, R2, R3):
+    r2 = R * R + z * z
+    if r2 < R2:
+        return amp * (r2 - 3 * R2)
+    return -2 * amp * R3 / mp.sqrt(r2)
+
+
+# 4 pi rho of each closed form above, from the documented densities
+# (natural units, G = 1): Poisson's equation pins normalisations and
+# zero-free constants that a retyped potential would not
+def four_pi_rho(name, R, z, args):
+    r = _r(R, z)
+    A = mp.mpf(args[0])
+    if name.startswith("powersph") or name == "kepler":
+        alpha = mp.mpf(args[1])
+        return A * (3 - alpha) * r ** (-alpha)
+    if name == "jaffe":
+        a = mp.mpf(args[1])
+        return A * a / (r * r * (r + a) ** 2)
+    if name == "isochrone":
+        b = mp.mpf(args[1])
+        s = mp.sqrt(r * r + b * b)
+        return A * (3 * (b + s) * s * s - r * r * (b + 3 * s)) / ((b + s) ** 3 * s**3)
+    if name == "plummer":
+        b = mp.mpf(args[1])
+        return 3 * A * b * b * (r * r + b * b) ** mp.mpf(-2.5)
+    if name == "pseudoiso":
+        a = mp.mpf(args[1])
+        return A / (a**3 * (1 + r * r / (a * a)))
+    if name == "kuzmin":  # vacuum off the plane
+        return mp.mpf(0)
+    if name == "burkert":
+        x = r / mp.mpf(args[1])
+        return 4 * mp.pi * A / ((1 + x) * (1 + x * x))
+    if name.startswith("dehnen") and name != "dehnenbar":
+        a = mp.mpf(args[1])
+        alpha = mp.mpf(args[2]) if len(args) > 2 else mp.mpf(0)
+        return A * a * r ** (-alpha) * (r + a) ** (alpha - 4)
+    if name == "homogsphere":
+        return 6 * A if r * r < mp.mpf(args[1]) else mp.mpf(0)
+    raise KeyError(name)
+
+
 def forces(fn, R, z, phi, t):
     """(-dPhi/dR, -dPhi/dz, -dPhi/dphi) by 50-digit numerical differentiation."""
     FR = -mp.diff(lambda v: fn(v, z, phi, t), R)
@@ -108,6 +197,29 @@ def potentials():
         "dehnenbar": (1, [0.01, float(-4 * tb), float(-2 * tb), 0.8, 1.85, 25.0 * numpy.pi / 180.0],
                       lambda R, z, phi, t: bar_phi(R, z, phi, t, mp.mpf("0.01"), -4 * tb, -2 * tb, mp.mpf("0.8"), omegab, mp.mpf(25.0 * numpy.pi / 180.0))),
     }
+    # the spherical and Kuzmin components, packed as galpy packs them
+    # (integrateFullOrbit.py:82-92,176-213,358-365); Phi from the closed form
+    # with the packed arguments
+    new = {
+        "powersph": (7, ["1.3", "1.7"], powersph_phi),
+        "powersph_a2": (7, ["0.7", "2"], powersph_phi),
+        "kepler": (7, ["0.9", "3"], powersph_phi),
+        "jaffe": (10, ["1.1", "0.8"], jaffe_phi),
+        "isochrone": (14, ["1.2", "0.7"], isochrone_phi),
+        "plummer": (17, ["0.9", "0.8"], plummer_phi),
+        "pseudoiso": (18, ["1.4", "1.3"], pseudoiso_phi),
+        "kuzmin": (19, ["1.0", "0.6"], kuzmin_phi),
+        "burkert": (20, ["0.8", "2.0"], burkert_phi),
+        "dehnencore": (33, ["1.5", "1.2"], lambda R, z, amp, a: dehnen_phi(R, z, amp, a, 0)),
+        "dehnen": (34, ["1.3", "0.9", "1.5"], dehnen_phi),
+        "dehnen_a05": (34, ["0.7", "1.4", "0.5"], dehnen_phi),
+        "homogsphere": (35, ["1.0", "1.21", "1.331"], homog_phi),
+    }
+    for name, (code, sargs, phi_fn) in new.items():
+        margs = [mp.mpf(float(v)) for v in sargs]  # the packed doubles exactly
+        cases[name] = (code, [float(v) for v in sargs],
+                       (lambda f, m: (lambda R, z, phi, t: f(R, z, *m)))(phi_fn, margs))
+    with_hessian = {"psc", "mn", "nfw", "hernquist", "loghalo_q09"} | set(new)
     rng = numpy.random.default_rng(2024)
     out = {}
     for name, (code, args, fn) in cases.items():
@@ -120,11 +232,18 @@ def potentials():
             FR, Fz, Fp = forces(fn, mp.mpf(R), mp.mpf(z), mp.mpf(phi), mp.mpf(t))
             row = {"R": R, "z": z, "phi": phi, "t": t, "pot": float(fn(mp.mpf(R), mp.mpf(z), mp.mpf(phi), mp.mpf(t))),
                    "FR": float(FR), "Fz": float(Fz), "Fphi": float(Fp)}
-            if name in ("psc", "mn", "nfw", "hernquist", "loghalo_q09"):
+            if name in with_hessian:
                 f2 = lambda a, b: fn(a, b, mp.mpf(phi), mp.mpf(t))
-                row["R2"] = float(mp.diff(f2, (mp.mpf(R), mp.mpf(z)), (2, 0)))
-                row["z2"] = float(mp.diff(f2, (mp.mpf(R), mp.mpf(z)), (0, 2)))
+                R2 = mp.diff(f2, (mp.mpf(R), mp.mpf(z)), (2, 0))
+                z2 = mp.diff(f2, (mp.mpf(R), mp.mpf(z)), (0, 2))
+                row["R2"] = float(R2)
+                row["z2"] = float(z2)
                 row["Rz"] = float(mp.diff(f2, (mp.mpf(R), mp.mpf(z)), (1, 1)))
+                if name in new:
+                    # Poisson: Phi_RR + Phi_R / R + Phi_zz = 4 pi rho
+                    lap = R2 - FR / R + z2
+                    rho4 = four_pi_rho(name, mp.mpf(R), mp.mpf(z), [mp.mpf(v) for v in cases[name][1]])
+                    assert abs(lap - rho4) <= mp.mpf("1e-25") * max(1, abs(rho4)), (name, R, z, lap, rho4)
             pts.append(row)
         out[name] = {"pot_type": [code], "pot_args": args, "points": pts}
     return out

@@ -206,6 +206,70 @@ def bar_potential():
     return [LogarithmicHaloPotential(q=0.9), DehnenBarPotential(omegab=1.85, rb=0.8, Af=0.01)]
 
 
+def spherical_a():
+    """Plummer + Kuzmin disk + Burkert + Jaffe, normalized so that the
+    circular velocity at R = 1 is 1."""
+    from galpy_amd.potential import BurkertPotential, JaffePotential, KuzminDiskPotential, PlummerPotential
+
+    return [PlummerPotential(b=0.8, normalize=0.3), KuzminDiskPotential(a=0.5, normalize=0.3),
+            BurkertPotential(a=2.0, normalize=0.2), JaffePotential(a=2.0, normalize=0.2)]
+
+
+def spherical_b():
+    """Isochrone + pseudo-isothermal + cored Dehnen + Dehnen (alpha 1.5) +
+    homogeneous sphere + power law + Kepler (seven components)."""
+    from galpy_amd.potential import (
+        DehnenCoreSphericalPotential,
+        DehnenSphericalPotential,
+        HomogeneousSpherePotential,
+        IsochronePotential,
+        KeplerPotential,
+        PowerSphericalPotential,
+        PseudoIsothermalPotential,
+    )
+
+    return [IsochronePotential(b=0.7, normalize=0.2), PseudoIsothermalPotential(a=1.3, normalize=0.15),
+            DehnenCoreSphericalPotential(a=1.2, normalize=0.15),
+            DehnenSphericalPotential(a=0.9, alpha=1.5, normalize=0.15),
+            HomogeneousSpherePotential(R=1.5, normalize=0.1), PowerSphericalPotential(alpha=1.7, normalize=0.15),
+            KeplerPotential(normalize=0.1)]
+
+
+def orbit_diffs(a, b, phi_last=True):
+    """Per-orbit max over times and columns of |a - b| (relative, unit
+    floor); with phi_last the last column is an angle, compared mod 2 pi.
+    a, b: [N, nt, ncol]."""
+    d = numpy.abs(a - b) / numpy.maximum(numpy.abs(b), 1.0)
+    if phi_last:
+        d[..., -1] = numpy.abs((a[..., -1] - b[..., -1] + numpy.pi) % (2 * numpy.pi) - numpy.pi)
+    return d.reshape(d.shape[0], -1).max(axis=1)
+
+
+def ulp_sensitivity(run, y0, phi_last=True, nrep=3):
+    """Conditioning of each orbit problem: the per-orbit change of the
+    oracle's own result when every initial coordinate moves by one ulp
+    (max over nrep random sign patterns).  run(y0) returns the oracle's
+    [N, nt, ncol] rows for initial states y0."""
+    ref = run(y0)
+    worst = numpy.zeros(len(y0))
+    for k in range(nrep):
+        sign = numpy.random.default_rng(1000 + k).choice([-1.0, 1.0], size=numpy.shape(y0))
+        worst = numpy.maximum(worst, orbit_diffs(run(y0 * (1.0 + sign * 2.0**-52)), ref, phi_last))
+    return ref, worst
+
+
+def assert_parity(res, ref, bar, sens=None, factor=0.0, phi_last=True, label=""):
+    """Every orbit within `bar`, or, for an orbit whose problem is so
+    ill-conditioned that the oracle's own result moves by sens under a
+    one-ulp change of its inputs, within factor * sens (rounding
+    differences are then amplified beyond the bar by the problem itself)."""
+    d = orbit_diffs(res, ref, phi_last)
+    allowed = numpy.full(len(d), float(bar)) if sens is None else numpy.maximum(bar, factor * sens)
+    bad = numpy.flatnonzero(d > allowed)
+    assert bad.size == 0, (label, bad[:5], d[bad[:5]], allowed[bad[:5]])
+    return d
+
+
 def phase_diff(a, b):
     """Max over rows of |a-b| per component with unit floor, phi modulo 2 pi.
     a, b: [..., 6] rows (R,vR,vT,z,vz,phi) or [..., 4] planar rows

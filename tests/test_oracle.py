@@ -106,7 +106,12 @@ def test_host_log_ge1_vs_mpmath():
 
 
 # ---------------------------------------------------------------- potentials
-@pytest.mark.parametrize("name", ["psc", "mn", "nfw", "hernquist", "loghalo_q09", "loghalo_triax", "dehnenbar"])
+NEW_SPHERICAL = ["powersph", "powersph_a2", "kepler", "jaffe", "isochrone", "plummer", "pseudoiso", "kuzmin",
+                 "burkert", "dehnencore", "dehnen", "dehnen_a05", "homogsphere"]
+
+
+@pytest.mark.parametrize("name", ["psc", "mn", "nfw", "hernquist", "loghalo_q09", "loghalo_triax", "dehnenbar"]
+                         + NEW_SPHERICAL)
 def test_oracle_potentials_vs_mpmath(name):
     case = _load("potentials.json")[name]
     pts = case["points"]

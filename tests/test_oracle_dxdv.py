@@ -21,7 +21,9 @@ def _mw():
     return pt, pa
 
 
-@pytest.mark.parametrize("name", ["psc", "mn", "nfw", "hernquist", "loghalo_q09"])
+@pytest.mark.parametrize("name", ["psc", "mn", "nfw", "hernquist", "loghalo_q09", "powersph", "powersph_a2",
+                                  "kepler", "jaffe", "isochrone", "plummer", "pseudoiso", "kuzmin", "burkert",
+                                  "dehnencore", "dehnen", "dehnen_a05", "homogsphere"])
 def test_hessians_vs_mpmath(name):
     case = json.load(open(os.path.join(GOLDEN, "potentials.json")))[name]
     pts = case["points"]

@@ -28,6 +28,8 @@ from helpers import (
     oracle_planar_forces,
     phase_diff,
     planar_ics,
+    spherical_a,
+    spherical_b,
 )
 
 
@@ -40,6 +42,8 @@ def _pots():
         "Hernquist": [HernquistPotential(a=1.3)],
         "LogHalo q=0.8": [LogarithmicHaloPotential(q=0.8)],
         "triaxial LogHalo": [LogarithmicHaloPotential(q=0.9, b=0.8)],
+        "spherical A": spherical_a(),
+        "spherical B": spherical_b(),
     }
 
 
@@ -78,7 +82,7 @@ def test_planar_forces_are_full_forces_in_plane(name):
     assert numpy.abs(Pphi - Fphi).max() <= 4e-15 * max(numpy.abs(Fphi).max(), 1e-300)
 
 
-@pytest.mark.parametrize("name", ["MWPotential2014", "LogHalo+bar"])
+@pytest.mark.parametrize("name", ["MWPotential2014", "LogHalo+bar", "spherical A", "spherical B"])
 @pytest.mark.parametrize("code", [3, 0])
 def test_planar_orbit_is_full_orbit_in_plane(name, code):
     pot = _pots()[name]
