@@ -422,7 +422,7 @@ def main():
             "orbit_dim": 2 if w["planar"] else 3,
             "deviations_per_orbit": 1 if w["dxdv"] else 0,
             "method": w["method"],
-            "potential": "MWPotential2014" if w["omega"] == 0.0 else "LogarithmicHalo(q=0.9)+DehnenBar",
+            "potential": w["potential"],
             "t": [float(t_np[0]), float(t_np[-1]), nt],
             "rtol": 1e-12,
             "atol": 1e-12,

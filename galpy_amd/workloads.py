@@ -34,6 +34,23 @@ def loghalo_bar():
     )
 
 
+def mn_plummer_burkert():
+    """MWPotential2014's disk with a Plummer bulge (b = 0.3) and a Burkert
+    halo (a = 2) in place of its bulge and NFW halo, carrying 0.6, 0.05 and
+    0.35 of the radial force at R = 1 (vc(1) = 1)."""
+    return P.CompositePotential(
+        P.MiyamotoNagaiPotential(a=3.0 / 8.0, b=0.28 / 8.0, normalize=0.6),
+        P.PlummerPotential(b=0.3, normalize=0.05),
+        P.BurkertPotential(a=2.0, normalize=0.35),
+    )
+
+
+POTENTIAL_NAMES = {
+    mw2014: "MWPotential2014",
+    loghalo_bar: "LogarithmicHalo(q=0.9)+DehnenBar",
+    mn_plummer_burkert: "MiyamotoNagai(a=3/8,b=0.28/8)+Plummer(b=0.3)+Burkert(a=2)",
+}
+
 WORKLOADS = {
     # 1,000 orbits, symplec4 with per-orbit step estimation, dense outputs
     "mw2014_symplec4_1k": dict(pot=mw2014, n=1000, t=(0.0, 100.0, 10001), method="symplec4_hip",
@@ -61,6 +78,13 @@ WORKLOADS = {
                                     method="symplec4_hip", omega=0.0, seed=0, dt=0.01, dxdv=True),
     "mw2014_dxdv_dop853_65k": dict(pot=mw2014, n=65536, t=(0.0, 20.0, 201),
                                   method="dop853_hip", omega=0.0, seed=0, dxdv=True),
+    # a composition without a specialised kernel (run-time dispatch of the
+    # component codes): fixed-step symplec4 at scale, and DOP853 at galpy's
+    # default tolerances
+    "mpb_symplec4_1m": dict(pot=mn_plummer_burkert, n=1048576, t=(0.0, 100.0, 101),
+                           method="symplec4_hip", omega=0.0, seed=0, dt=0.01),
+    "mpb_dop853_256k": dict(pot=mn_plummer_burkert, n=262144, t=(0.0, 500.0, 1001),
+                           method="dop853_hip", omega=0.0, seed=0),
 }
 
 
@@ -82,6 +106,7 @@ def make(name, n=None):
         rect = numpy.stack([R * c, R * s, z, vR * c - vT * s, vR * s + vT * c, vz], axis=1)
         dev = numpy.random.default_rng(w["seed"] + 1).normal(size=rect.shape)
         w["yo"] = numpy.ascontiguousarray(numpy.hstack([rect, dev]))
+    w["potential"] = POTENTIAL_NAMES[w["pot"]]
     w["pot"] = w["pot"]()
     w["name"] = name
     return w
