@@ -92,6 +92,8 @@ class ParamSpec:
 
     def __init__(self, pot, params):
         self.npot, self.pot_type, self.pot_args, _ = parse_pot(pot)
+        if any(int(c) < 0 for c in self.pot_type):
+            raise ValueError("the adjoint does not differentiate wrapped (time-dependent) potentials")
         self.offsets = numpy.cumsum([0] + [_NARGS[int(c)] for c in self.pot_type])[:-1]
         self.params = [(int(c), int(j)) for c, j in params]
         for c, j in self.params:

@@ -216,6 +216,7 @@ __device__ int gh_batch_run_tile(const gh_batch_group& g, const gh_batch_tile& t
 enum GhSpecCode { GH_SPEC_ANY = 0, GH_SPEC_MW2014, GH_SPEC_LOGBAR, GH_SPEC_HERN, GH_SPEC_LOG };
 
 __device__ __forceinline__ int gh_spec_code(const GhPotTable* __restrict__ T) {
+  if (T->has_wrap) return GH_SPEC_ANY;  // wrappers: run-time dispatch only
   const int n = T->npot;
   const int c0 = T->comp[0].type;
   if (n == 3 && c0 == GH_POWERSPHCUTOFF && T->comp[1].type == GH_MIYAMOTONAGAI &&

@@ -85,6 +85,12 @@ int nargs_of(int code, bool planar) {
     case GH_POWERSPH: case GH_JAFFE: case GH_ISOCHRONE: case GH_PLUMMER: case GH_PSEUDOISO:
     case GH_KUZMINDISK: case GH_BURKERT: case GH_DEHNENCORESPH: return 2;
     case GH_DEHNENSPH: case GH_HOMOGSPHERE: return 3;
+    // wrappers' own arguments, read after the wrapped components
+    // (integrateFullOrbit.c:821-895, integratePlanarOrbit.c:591-663)
+    case GH_DEHNENSMOOTHWRAP: return 4;   // amp, tform, tsteady, grow
+    case GH_SOLIDBODYROTWRAP: return 3;   // amp, omega, pa
+    case GH_COROTATINGROTWRAP: return 5;  // amp, vpo, beta, pa, to
+    case GH_GAUSSIANAMPWRAP: return 3;    // amp, to, sigma^2
     default: return -1;
   }
 }
@@ -181,22 +187,15 @@ int fit_upper_gamma_table(double a, double xbig, double tab[GH_TAB_MAXSEG][GH_TA
   return e0;
 }
 
-int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPotTable* T,
-                     bool planar = false) {
-  std::memset(T, 0, sizeof(*T));
-  if (npot < 1 || npot > GH_MAX_POT) {
-    set_error("npot=%d outside [1, %d]", npot, GH_MAX_POT);
+// One (leaf) component's arguments and precomputed constants.
+int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
+                     bool planar) {
+  const int na = nargs_of(code, planar);
+  if (na < 0 || code < 0) {
+    set_error("pot_type %d (component %d) has no HIP kernel", code, i);
     return -1;
   }
-  T->npot = npot;
-  int nslots = 0;
-  for (int i = 0; i < npot; i++) {
-    const int code = pot_type[i];
-    const int na = nargs_of(code, planar);
-    if (na < 0) {
-      set_error("pot_type %d (component %d) has no HIP kernel", code, i);
-      return -1;
-    }
+  {
     GhComponent& C = T->comp[i];
     C.type = code;
     for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
@@ -270,6 +269,78 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
     }
     T->nonaxi |= C.nonaxi;
   }
+  return 0;
+}
+
+// n components in galpy's order (parse_leapFuncArgs_Full,
+// integrateFullOrbit.c:55-1045): a wrapper's code is followed by its wrapped
+// components' codes; its arguments are nwrapped, the wrapped components'
+// arguments, then its own (:1010-1031).  Leaves go to T->comp in the order
+// they appear (galpy's summation order); every leaf lists the wrappers
+// around it, outermost first.
+int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPotTable* T,
+                      int& nleaf, int& nslots, int depth, bool planar) {
+  for (int k = 0; k < n; k++) {
+    const int code = *pot_type++;
+    if (code >= 0) {
+      if (nleaf >= GH_MAX_POT) {
+        set_error("more than %d components (wrapped ones included)", GH_MAX_POT);
+        return -1;
+      }
+      if (decode_component(code, pot_args, T, nleaf, nslots, planar) != 0) return -1;
+      nleaf++;
+      continue;
+    }
+    const int na = nargs_of(code, planar);
+    if (na < 0) {
+      set_error("wrapper pot_type %d has no HIP kernel", code);
+      return -1;
+    }
+    if (depth >= GH_WRAP_DEPTH || T->nwraps >= GH_MAX_WRAP) {
+      set_error("wrappers nested more than %d deep or more than %d in total", GH_WRAP_DEPTH,
+                GH_MAX_WRAP);
+      return -1;
+    }
+    const double nwd = *pot_args++;
+    const int nw = (int)nwd;
+    if (!(nwd >= 1.0) || nw != nwd) {
+      set_error("wrapper pot_type %d wraps %g components", code, nwd);
+      return -1;
+    }
+    const int first = nleaf;
+    if (decode_components(nw, pot_type, pot_args, T, nleaf, nslots, depth + 1, planar) != 0)
+      return -1;
+    const int w = T->nwraps++;
+    GhWrap& W = T->wrap[w];
+    W.kind = code;
+    for (int j = 0; j < na; j++) W.p[j] = *pot_args++;
+    if (code == GH_GAUSSIANAMPWRAP && !(W.p[2] > 0.0)) {
+      set_error("GaussianAmplitudeWrapperPotential sigma^2=%g must be positive", W.p[2]);
+      return -1;
+    }
+    for (int l = first; l < nleaf; l++) {  // this wrapper encloses the ones already listed
+      GhComponent& C = T->comp[l];
+      for (int j = C.nwrap; j > 0; j--) C.wrap[j] = C.wrap[j - 1];
+      C.wrap[0] = w;
+      C.nwrap++;
+    }
+    T->has_wrap = 1;
+  }
+  return 0;
+}
+
+int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPotTable* T,
+                     bool planar = false) {
+  std::memset(T, 0, sizeof(*T));
+  if (npot < 1 || npot > GH_MAX_POT) {
+    set_error("npot=%d outside [1, %d]", npot, GH_MAX_POT);
+    return -1;
+  }
+  int nleaf = 0, nslots = 0;
+  const int* pt = pot_type;
+  const double* pa = pot_args;
+  if (decode_components(npot, pt, pa, T, nleaf, nslots, 0, planar) != 0) return -1;
+  T->npot = nleaf;
   return 0;
 }
 
@@ -873,10 +944,12 @@ struct GhPotHandle {
   int npot;
   int types[GH_MAX_POT];
   int nonaxi[GH_MAX_POT];  // component exerts a phi torque (bar, triaxial halo)
+  int has_wrap;            // wrapped components: run-time dispatch, no Hessian
 };
 
 void fill_handle(const GhPotTable& table, GhPotHandle* h) {
   h->npot = table.npot;
+  h->has_wrap = table.has_wrap;
   for (int i = 0; i < GH_MAX_POT; i++) {
     h->types[i] = table.comp[i].type;
     h->nonaxi[i] = table.comp[i].nonaxi;
@@ -886,6 +959,7 @@ void fill_handle(const GhPotTable& table, GhPotHandle* h) {
 // Components with a Hessian on the device (the adjoint's): time-independent
 // and axisymmetric -- every type but the bar and a triaxial logarithmic halo.
 bool hessian_capable(const GhPotHandle* h) {
+  if (h->has_wrap) return false;  // time-dependent (and rotating) wrappers
   for (int i = 0; i < h->npot; i++) {
     if (h->types[i] == GH_DEHNENBAR || h->nonaxi[i]) return false;
   }
@@ -1015,7 +1089,7 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
 }
 
 bool types_are(const GhPotHandle* h, std::initializer_list<int> codes) {
-  if ((int)codes.size() != h->npot) return false;
+  if (h->has_wrap || (int)codes.size() != h->npot) return false;
   int i = 0;
   for (int c : codes)
     if (h->types[i++] != c) return false;
@@ -1066,7 +1140,7 @@ int launch_integrate_dxdv(int nobj, double* d_yo, int nt, const double* d_t, int
   }
   if (!hessian_capable(h)) {
     set_error("integrate_dxdv needs time-independent, axisymmetric components with a Hessian "
-              "(every supported type but DehnenBar and a triaxial LogarithmicHalo)");
+              "(every supported type but DehnenBar and a triaxial LogarithmicHalo, unwrapped)");
     return -1;
   }
   using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
@@ -1540,7 +1614,7 @@ GH_EXPORT int gh_adjoint_hip(int n, const double* d_wT, const double* d_gT, doub
   }
   if (!hessian_capable(h)) {
     set_error("the adjoint needs time-independent, axisymmetric components with a Hessian "
-              "(every supported type but DehnenBar and a triaxial LogarithmicHalo)");
+              "(every supported type but DehnenBar and a triaxial LogarithmicHalo, unwrapped)");
     return -1;
   }
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
@@ -1668,7 +1742,7 @@ GH_EXPORT int gh_host_bulge_lower_gamma(int npot, const int* pot_type, const dou
   g_errbuf[0] = 0;
   static GhPotTable T;
   if (decode_potential(npot, pot_type, pot_args, &T) != 0) return -1;
-  if (comp < 0 || comp >= npot || T.comp[comp].type != GH_POWERSPHCUTOFF) {
+  if (comp < 0 || comp >= T.npot || T.comp[comp].type != GH_POWERSPHCUTOFF) {
     set_error("component %d is not a PowerSphericalPotentialwCutoff", comp);
     return -1;
   }

@@ -17,6 +17,8 @@
 #include "gh_math.h"
 
 #define GH_MAX_POT 8
+#define GH_MAX_WRAP 8      // wrappers per potential
+#define GH_WRAP_DEPTH 3    // wrappers nested around one component
 #define GH_IG_NSER 32  // terms of the incomplete-gamma series (enough for a <= 1.5)
 // piecewise-polynomial table of Gamma(a,x) on [a+1, xbig): GH_TAB_PER_OCT
 // segments per binary octave of x, degree GH_TAB_NC-1 in the local coordinate
@@ -42,20 +44,40 @@ enum GhPotCode {
   GH_DEHNENCORESPH = 33,
   GH_DEHNENSPH = 34,
   GH_HOMOGSPHERE = 35,
+  // wrappers (integrateFullOrbit.c:821-895)
+  GH_DEHNENSMOOTHWRAP = -1,
+  GH_SOLIDBODYROTWRAP = -2,
+  GH_COROTATINGROTWRAP = -4,
+  GH_GAUSSIANAMPWRAP = -5,
 };
 
 struct GhComponent {
   int type;
   int nonaxi;    // 1 if the component exerts a phi torque
+  int nwrap;     // wrappers around this component
+  int wrap[GH_WRAP_DEPTH];  // their indices in GhPotTable::wrap, outermost first
   double p[8];   // raw galpy arguments, in galpy's order
   double c[14];  // precomputed constants, per type (decoded on the host)
 };
 
+// A wrapper (DehnenSmoothWrapperPotential.c, SolidBodyRotationWrapperPotential.c,
+// CorotatingRotationWrapperPotential.c, GaussianAmplitudeWrapperPotential.c):
+// its code and its own arguments in galpy's order.  galpy nests wrappers
+// around lists of components; the host flattens the nesting so that every
+// leaf component lists the wrappers around it.
+struct GhWrap {
+  int kind;
+  int pad;
+  double p[6];
+};
+
 struct GhPotTable {
   int npot;
-  int nonaxi;   // some component needs phi
-  int pad[2];
+  int nonaxi;    // some component needs phi
+  int has_wrap;  // some component is wrapped (run-time dispatch only)
+  int nwraps;
   GhComponent comp[GH_MAX_POT];
+  GhWrap wrap[GH_MAX_WRAP];
   // series coefficients 1/(a (a+1) ... (a+n)) of the incomplete gamma function
   // for the force exponent a = 1.5 - alpha/2 ([0]) and the potential-value
   // exponent 1 - alpha/2 ([1]) of PowerSphericalPotentialwCutoff components
@@ -525,6 +547,78 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
   }
 }
 
+// The wrappers around component C at cylindrical radius R and time t: the
+// product of their amplitudes and time factors, the total angle offset (the
+// wrapped component is evaluated at phi - offset) and d offset / dR.  Nested
+// wrappers compose by multiplying factors and adding offsets, as galpy's
+// recursion through calcRforce/calcphitorque does:
+//   DehnenSmooth (DehnenSmoothWrapperPotential.c:4-16): amp * smooth(t),
+//     Dehnen's quintic between tform and tsteady (1 - smooth when decaying);
+//   GaussianAmplitude (GaussianAmplitudeWrapperPotential.c:3-5):
+//     amp * exp(-(t - to)^2 / (2 sigma^2));
+//   SolidBodyRotation (SolidBodyRotationWrapperPotential.c:4-9):
+//     offset omega t + pa;
+//   CorotatingRotation (CorotatingRotationWrapperPotential.c:3-15): offset
+//     vpo R^(beta-1) (t - to) + pa, whose R derivative adds
+//     -phitorque * d offset / dR to the radial force.
+__device__ __forceinline__ void gh_wrap_transform(const GhPotTable* __restrict__ T,
+                                                  const GhComponent* __restrict__ C, double R,
+                                                  double t, double& fac, double& off,
+                                                  double& doff) {
+  fac = 1.0;
+  off = 0.0;
+  doff = 0.0;
+  for (int k = 0; k < C->nwrap; k++) {
+    const GhWrap* __restrict__ W = &T->wrap[C->wrap[k]];
+    fac *= W->p[0];
+    switch (W->kind) {
+      case GH_DEHNENSMOOTHWRAP: {
+        const double sm = gh_bar_smooth(t, W->p[1], W->p[2]);
+        fac *= (W->p[3] != 0.0) ? sm : 1.0 - sm;
+        break;
+      }
+      case GH_GAUSSIANAMPWRAP: {
+        const double d = t - W->p[1];
+        fac *= exp(-0.5 * d * d / W->p[2]);
+        break;
+      }
+      case GH_SOLIDBODYROTWRAP: off += W->p[1] * t + W->p[2]; break;
+      case GH_COROTATINGROTWRAP: {
+        const double dt = t - W->p[4];
+        const double Rb = pow(R, W->p[2] - 1.0);
+        off += W->p[1] * Rb * dt + W->p[3];
+        doff += W->p[1] * (W->p[2] - 1.0) * (Rb / R) * dt;
+        break;
+      }
+      default: break;
+    }
+  }
+}
+
+// Force of a wrapped component: the component's own force at the position
+// rotated by -offset (its F_R, F_z and F_phi are the same cylindrical
+// components in either frame), scaled by the wrappers' factor, with the
+// corotating term -F_phi d offset / dR in F_R (in the accumulated form:
+// F_R / R gains -(F_phi / R^2) R d offset / dR).
+__device__ __noinline__ void gh_comp_force_wrapped(const GhPotTable* __restrict__ T, int i,
+                                                   const GhGeom& g, GhAcc& F) {
+  const double R = sqrt(g.R2);
+  double fac, off, doff;
+  gh_wrap_transform(T, &T->comp[i], R, g.t, fac, off, doff);
+  GhGeom gw = g;
+  if (off != 0.0) {
+    double so, co;
+    sincos(off, &so, &co);
+    gw.x = g.x * co + g.y * so;
+    gw.y = g.y * co - g.x * so;
+  }
+  GhAcc L = {0.0, 0.0, 0.0};
+  gh_comp_force_dyn(T, i, gw, L);
+  F.FRoR += fac * (L.FRoR - L.FphioR2 * R * doff);
+  F.Fz += fac * L.Fz;
+  F.FphioR2 += fac * L.FphioR2;
+}
+
 // A potential composition known at compile time: GhSpec<15, 5, 9> is
 // MWPotential2014; GhSpec<> is any composition, dispatched at run time.
 // Specialised kernels inline only the components they use, which keeps the
@@ -540,7 +634,16 @@ struct GhAccSum<GhSpec<>> {
   __device__ static __forceinline__ void add(const GhPotTable* __restrict__ T, const GhGeom& g,
                                              GhAcc& F) {
     const int npot = T->npot;
-    for (int i = 0; i < npot; i++) gh_comp_force_dyn(T, i, g, F);
+    if (T->has_wrap) {
+      for (int i = 0; i < npot; i++) {
+        if (T->comp[i].nwrap)
+          gh_comp_force_wrapped(T, i, g, F);
+        else
+          gh_comp_force_dyn(T, i, g, F);
+      }
+    } else {
+      for (int i = 0; i < npot; i++) gh_comp_force_dyn(T, i, g, F);
+    }
   }
 };
 
@@ -622,14 +725,26 @@ struct GhForce {
 // Potential value (per-component Eval functions; DehnenBar from
 // DehnenBarPotential.py:173-200, which has no C Eval).
 __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T, double R,
-                                               double z, double cosphi, double sinphi,
+                                               double z, double cosphi0, double sinphi0,
                                                double t) {
-  double pot = 0.0;
+  double total = 0.0;
   const double R2 = R * R;
   const double z2 = z * z;
   const double r2 = R2 + z2;
   for (int i = 0; i < T->npot; i++) {
     const GhComponent* __restrict__ C = &T->comp[i];
+    // wrapped components: the wrappers' Python _evaluate, the component at
+    // phi - offset times the factor
+    double fac = 1.0, cosphi = cosphi0, sinphi = sinphi0;
+    if (C->nwrap) {
+      double off, doff;
+      gh_wrap_transform(T, C, R, t, fac, off, doff);
+      double so, co;
+      sincos(off, &so, &co);
+      cosphi = cosphi0 * co + sinphi0 * so;
+      sinphi = sinphi0 * co - cosphi0 * so;
+    }
+    double pot = 0.0;
     switch (C->type) {
       case GH_POWERSPHCUTOFF: {
         // PowerSphericalPotentialwCutoff.c:12-24
@@ -732,6 +847,7 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
       default:
         break;
     }
+    total += fac * pot;
   }
-  return pot;
+  return total;
 }

@@ -612,6 +612,137 @@ class HomogeneousSpherePotential(_SphericalPotential):
         return [self._amp, self._R2, self._R3]
 
 
+class WrapperPotential(Potential):
+    """A potential that modifies the potential it wraps (galpy's
+    WrapperPotential.py:91-150): the wrapped components are evaluated at
+    phi - offset(R, t) and scaled by amp * factor(t).  galpy packs the
+    wrapper's code followed by the wrapped codes, and as arguments the number
+    of wrapped components, their arguments, then the wrapper's own
+    (integrateFullOrbit.py:396-448)."""
+
+    def __init__(self, amp=1.0, pot=None):
+        Potential.__init__(self, amp=amp)
+        if pot is None:
+            raise ValueError("a wrapper needs the potential it wraps")
+        self._pot = pot
+        self.isNonAxi = any(q.isNonAxi for q in flatten(pot))
+
+    def _factor(self, t):
+        return 1.0
+
+    def _offset(self, R, t):
+        """(offset, d offset / dR)"""
+        return 0.0, 0.0
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        off, _ = self._offset(R, t)
+        return self._factor(t) * evaluatePotentials(self._pot, R, z, phi - off, t)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        off, doff = self._offset(R, t)
+        return self._factor(t) * (evaluateRforces(self._pot, R, z, phi - off, t)
+                                  - evaluatephitorques(self._pot, R, z, phi - off, t) * doff)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        off, _ = self._offset(R, t)
+        return self._factor(t) * evaluatezforces(self._pot, R, z, phi - off, t)
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        off, _ = self._offset(R, t)
+        return self._factor(t) * evaluatephitorques(self._pot, R, z, phi - off, t)
+
+    def c_args(self):
+        raise TypeError("wrappers are packed by parse_pot")
+
+
+class DehnenSmoothWrapperPotential(WrapperPotential):
+    """Grows (or, with decay, removes) the wrapped potential between tform
+    and tsteady with Dehnen's quintic; pot_type -1, own arguments
+    [amp, tform, tsteady, grow] (DehnenSmoothWrapperPotential.py:28-56)."""
+
+    pot_code = -1
+
+    def __init__(self, amp=1.0, pot=None, tform=-4.0, tsteady=None, decay=False):
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        self._tform = tform
+        self._tsteady = self._tform / 2.0 if tsteady is None else self._tform + tsteady
+        self._grow = not decay
+
+    def _factor(self, t):
+        t = numpy.asarray(t, dtype=float)
+        xi = 2.0 * (t - self._tform) / (self._tsteady - self._tform) - 1.0
+        mid = 3.0 / 16.0 * xi**5.0 - 5.0 / 8.0 * xi**3.0 + 15.0 / 16.0 * xi + 0.5
+        smooth = numpy.where(t < self._tform, 0.0, numpy.where(t < self._tsteady, mid, 1.0))
+        return smooth if self._grow else 1.0 - smooth
+
+    def wrapper_args(self):
+        return [self._amp, self._tform, self._tsteady, float(int(self._grow))]
+
+
+class SolidBodyRotationWrapperPotential(WrapperPotential):
+    """Rotates the wrapped potential at pattern speed omega from angle pa;
+    pot_type -2, own arguments [amp, omega, pa]."""
+
+    pot_code = -2
+
+    def __init__(self, amp=1.0, pot=None, omega=1.0, pa=0.0):
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        self._omega = omega
+        self._pa = pa
+
+    def OmegaP(self):
+        return self._omega
+
+    def _offset(self, R, t):
+        return self._omega * numpy.asarray(t, dtype=float) + self._pa, 0.0
+
+    def wrapper_args(self):
+        return [self._amp, self._omega, self._pa]
+
+
+class CorotatingRotationWrapperPotential(WrapperPotential):
+    """Winds the wrapped potential up with the rotation curve
+    vpo R^beta: offset vpo R^(beta-1) (t - to) + pa; pot_type -4, own
+    arguments [amp, vpo, beta, pa, to]."""
+
+    pot_code = -4
+
+    def __init__(self, amp=1.0, pot=None, vpo=1.0, beta=0.0, to=0.0, pa=0.0):
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        self._vpo = vpo
+        self._beta = beta
+        self._pa = pa
+        self._to = to
+
+    def _offset(self, R, t):
+        R = numpy.asarray(R, dtype=float)
+        dt = numpy.asarray(t, dtype=float) - self._to
+        off = self._vpo * R ** (self._beta - 1.0) * dt + self._pa
+        doff = self._vpo * (self._beta - 1.0) * R ** (self._beta - 2.0) * dt
+        return off, doff
+
+    def wrapper_args(self):
+        return [self._amp, self._vpo, self._beta, self._pa, self._to]
+
+
+class GaussianAmplitudeWrapperPotential(WrapperPotential):
+    """Scales the wrapped potential by exp(-(t - to)^2 / (2 sigma^2));
+    pot_type -5, own arguments [amp, to, sigma^2]."""
+
+    pot_code = -5
+
+    def __init__(self, amp=1.0, pot=None, to=0.0, sigma=1.0):
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        self._to = to
+        self._sigma2 = sigma**2.0
+
+    def _factor(self, t):
+        return numpy.exp(-0.5 * (numpy.asarray(t, dtype=float) - self._to) ** 2.0 / self._sigma2)
+
+    def wrapper_args(self):
+        return [self._amp, self._to, self._sigma2]
+
+
 def MWPotential2014_factory():
     """mwpotentials.py:25-29 (Bovy 2014, Table 1)."""
     return CompositePotential(
@@ -635,21 +766,40 @@ def flatten(pot):
     return out
 
 
-def parse_pot(pot):
-    """galpy's ``_parse_pot`` (integrateFullOrbit.py:43-614) for the supported
-    potentials: returns ``(npot, pot_type int32[], pot_args float64[], [])``.
-
-    Raises NotImplementedError for a potential without a GPU kernel, as galpy
-    does for a potential without a C implementation (:604-611)."""
-    comps = flatten(pot)
+def _pack(comps, planar):
+    """Codes and arguments of a list of components in galpy's order; a
+    wrapper's code is followed by its wrapped codes, its arguments are
+    nwrapped, the wrapped arguments, then its own
+    (integrateFullOrbit.py:396-448, integratePlanarOrbit.py:507-565)."""
     pot_type, pot_args = [], []
     for p in comps:
+        if isinstance(p, WrapperPotential):
+            inner = flatten(p._pot)
+            wt, wa = _pack(inner, planar)
+            pot_type.append(p.pot_code)
+            pot_type.extend(wt)
+            pot_args.append(float(len(inner)))
+            pot_args.extend(wa)
+            pot_args.extend(p.wrapper_args())
+            continue
         if not isinstance(p, Potential) or p.pot_code is None:
             raise NotImplementedError(
                 f"Potential {type(p).__name__} is not supported by the HIP orbit-integration backend"
             )
         pot_type.append(p.pot_code)
-        pot_args.extend(p.c_args())
+        pot_args.extend(p.c_args_planar() if planar else p.c_args())
+    return pot_type, pot_args
+
+
+def parse_pot(pot):
+    """galpy's ``_parse_pot`` (integrateFullOrbit.py:43-614) for the supported
+    potentials: returns ``(npot, pot_type int32[], pot_args float64[], [])``;
+    npot counts the top-level components (a wrapper is one).
+
+    Raises NotImplementedError for a potential without a GPU kernel, as galpy
+    does for a potential without a C implementation (:604-611)."""
+    comps = flatten(pot)
+    pot_type, pot_args = _pack(comps, False)
     return (
         len(comps),
         numpy.array(pot_type, dtype=numpy.int32, order="C"),
@@ -663,14 +813,7 @@ def parse_pot_planar(pot):
     supported potentials, taken in the plane z = 0: returns
     ``(npot, pot_type int32[], pot_args float64[], [])``."""
     comps = flatten(pot)
-    pot_type, pot_args = [], []
-    for p in comps:
-        if not isinstance(p, Potential) or p.pot_code is None:
-            raise NotImplementedError(
-                f"Potential {type(p).__name__} is not supported by the HIP orbit-integration backend"
-            )
-        pot_type.append(p.pot_code)
-        pot_args.extend(p.c_args_planar())
+    pot_type, pot_args = _pack(comps, True)
     return (
         len(comps),
         numpy.array(pot_type, dtype=numpy.int32, order="C"),
@@ -689,3 +832,7 @@ def evaluateRforces(pot, R, z=0.0, phi=0.0, t=0.0):
 
 def evaluatezforces(pot, R, z=0.0, phi=0.0, t=0.0):
     return sum(p.zforce(R, z, phi, t) for p in flatten(pot))
+
+
+def evaluatephitorques(pot, R, z=0.0, phi=0.0, t=0.0):
+    return sum(p.phitorque(R, z, phi, t) for p in flatten(pot))

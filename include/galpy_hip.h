@@ -17,7 +17,14 @@
  *   15 PowerSphericalPotentialwCutoff, 17 PlummerPotential,
  *   18 PseudoIsothermalPotential, 19 KuzminDiskPotential, 20 BurkertPotential,
  *   33 DehnenCoreSphericalPotential, 34 DehnenSphericalPotential,
- *   35 HomogeneousSpherePotential.
+ *   35 HomogeneousSpherePotential;
+ * and the wrappers (integrateFullOrbit.c:821-895, packed as galpy packs them:
+ * the wrapper's code, then the wrapped components' codes; as arguments
+ * nwrapped, the wrapped components' arguments, then the wrapper's own)
+ *   -1 DehnenSmoothWrapperPotential, -2 SolidBodyRotationWrapperPotential,
+ *   -4 CorotatingRotationWrapperPotential, -5 GaussianAmplitudeWrapperPotential,
+ * nested up to 3 deep; at most 8 (leaf) components and 8 wrappers in all.
+ * npot counts the top-level components, as in galpy.
  * Supported odeint_type codes (integrateFullOrbit.c:1091-1132,
  * integratePlanarOrbit.c:788-825):
  *   0 leapfrog, 3 symplec4, 6 dop853.

@@ -151,13 +151,29 @@ template <class R> R upper_incgamma(double a, R x, double gamma_a) {
 enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, kHernquist = 8,
                kNFW = 9, kJaffe = 10, kIsochrone = 14, kPowerSphCutoff = 15, kPlummer = 17,
                kPseudoIso = 18, kKuzminDisk = 19, kBurkert = 20, kDehnenCoreSph = 33,
-               kDehnenSph = 34, kHomogSphere = 35 };
+               kDehnenSph = 34, kHomogSphere = 35,
+               // wrappers (integrateFullOrbit.c:821-895)
+               kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
+               kGaussianAmpWrap = -5 };
+
+// Components of one parse, wrapped ones included (the C-ABI entry points
+// accept up to this many).
+constexpr int kOracleMaxComp = 16;
 
 struct Component {
   int type;
   double p[8];  // raw arguments in galpy's order
   double c[6];  // precomputed constants
+  // a wrapper's wrapped components (potentialArg.nwrapped /
+  // wrappedPotentialArg, galpy_potentials.h), stored in the same parse array
+  int nwrapped = 0;
+  const Component* wrapped = nullptr;
 };
+
+inline bool is_wrapper(int code) {
+  return code == kDehnenSmoothWrap || code == kSolidBodyRotWrap || code == kCorotatingRotWrap ||
+         code == kGaussianAmpWrap;
+}
 
 // nargs per code, as consumed from pot_args (integrateFullOrbit.c:80-345;
 // planar packing integratePlanarOrbit.c:64-223, where
@@ -173,21 +189,42 @@ inline int component_nargs(int code, bool planar = false) {
     case kPowerSph: case kJaffe: case kIsochrone: case kPlummer: case kPseudoIso:
     case kKuzminDisk: case kBurkert: case kDehnenCoreSph: return 2;
     case kDehnenSph: case kHomogSphere: return 3;
+    // wrappers' own arguments, read after the wrapped components
+    case kDehnenSmoothWrap: return 4;   // amp, tform, tsteady, grow
+    case kSolidBodyRotWrap: return 3;   // amp, omega, pa
+    case kCorotatingRotWrap: return 5;  // amp, vpo, beta, pa, to
+    case kGaussianAmpWrap: return 3;    // amp, to, sigma^2
     default: return -1;
   }
 }
 
-// Decode; returns 0 on success, -1 on an unsupported code.
-inline int parse_potential(int npot, const int* pot_type, const double* pot_args,
-                           Component* out, bool planar = false) {
-  for (int i = 0; i < npot; i++) {
-    int code = pot_type[i];
+// Decode n components into dst, consuming codes and arguments in galpy's
+// order (parse_leapFuncArgs_Full, integrateFullOrbit.c:55-1045): a wrapper's
+// code is followed by its wrapped components' codes; its arguments are
+// nwrapped, the wrapped components' arguments, then its own
+// (integrateFullOrbit.c:1010-1018, 1026-1031).  Wrapped components are
+// stored in out[next...].
+inline int parse_components(int n, const int*& pot_type, const double*& pot_args, Component* out,
+                            int& next, Component* dst, bool planar) {
+  for (int i = 0; i < n; i++) {
+    int code = *pot_type++;
     int na = component_nargs(code, planar);
     if (na < 0) return -1;
-    Component& C = out[i];
+    Component& C = dst[i];
     C.type = code;
+    C.nwrapped = 0;
+    C.wrapped = nullptr;
     std::memset(C.p, 0, sizeof(C.p));
     std::memset(C.c, 0, sizeof(C.c));
+    if (code < 0) {
+      int nw = (int)*pot_args++;
+      if (nw < 1 || next + nw > kOracleMaxComp) return -1;
+      Component* kids = out + next;
+      next += nw;
+      if (parse_components(nw, pot_type, pot_args, out, next, kids, planar) != 0) return -1;
+      C.nwrapped = nw;
+      C.wrapped = kids;
+    }
     for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
     if (code == kPowerSphCutoff) {
       double alpha = C.p[1], rc = C.p[2];
@@ -199,6 +236,55 @@ inline int parse_potential(int npot, const int* pot_type, const double* pot_args
     }
   }
   return 0;
+}
+
+// Decode galpy's packed potential; out needs room for kOracleMaxComp
+// components (npot top-level ones first).  Returns 0 on success, -1 on an
+// unsupported code or too many components.
+inline int parse_potential(int npot, const int* pot_type, const double* pot_args,
+                           Component* out, bool planar = false) {
+  if (npot < 0 || npot > kOracleMaxComp) return -1;
+  int next = npot;
+  return parse_components(npot, pot_type, pot_args, out, next, out, planar);
+}
+
+// DehnenSmoothWrapperPotential.c:4-16 (dehnenSmooth) and
+// GaussianAmplitudeWrapperPotential.c:3-5 (gaussSmooth)
+template <class R> R dehnen_smooth(R t, double tform, double tsteady, bool grow) {
+  R smooth;
+  if (t < R(tform))
+    smooth = 0.;
+  else if (t < R(tsteady)) {
+    R xi = R(2.) * (t - R(tform)) / R(tsteady - tform) - R(1.);
+    smooth = R(3. / 16.) * pow(xi, R(5.)) - R(5. / 8.) * pow(xi, R(3.)) + R(15. / 16.) * xi + R(.5);
+  } else
+    smooth = 1.;
+  return grow ? smooth : R(1.) - smooth;
+}
+template <class R> R gauss_smooth(R t, double to, double sigma2) {
+  return exp(R(-0.5) * (t - R(to)) * (t - R(to)) / R(sigma2));
+}
+
+// The amplitude factor and the angle offset of a wrapper at (R, t), and
+// dOffset/dR (CorotatingRotationWrapperPotential.c:3-15): the wrapped
+// components are evaluated at phi - offset and their forces scaled by the
+// factor; the corotating wrapper's radial force gains -phitorque dOffset/dR.
+template <class R> void wrapper_transform(const Component& C, R Rc, R t, R& fac, R& off, R& doff) {
+  fac = R(C.p[0]);
+  off = 0.;
+  doff = 0.;
+  switch (C.type) {
+    case kDehnenSmoothWrap: fac = fac * dehnen_smooth(t, C.p[1], C.p[2], C.p[3] != 0.); break;
+    case kGaussianAmpWrap: fac = fac * gauss_smooth(t, C.p[1], C.p[2]); break;
+    case kSolidBodyRotWrap: off = R(C.p[1]) * t + R(C.p[2]); break;  // SolidBodyRotationWrapperPotential.c:4-9
+    case kCorotatingRotWrap: {
+      double vpo = C.p[1], beta = C.p[2], pa = C.p[3], to = C.p[4];
+      off = R(vpo) * pow(Rc, R(beta - 1.)) * (t - R(to)) + R(pa);
+      doff = R(vpo * (beta - 1.)) * pow(Rc, R(beta - 2.)) * (t - R(to));
+      break;
+    }
+    default: break;
+  }
 }
 
 // PowerSphericalPotentialwCutoff.c:9-11 (mass), :25-46 (Rforce), :60-81 (zforce)
@@ -517,7 +603,18 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kDehnenBar: bar_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kKuzminDisk: kuzmin_forces(C, Rc, z, fr, fz); break;
       default:
-        if (is_spherical_new(C.type)) sph_forces(C, Rc, z, fr, fz);
+        if (is_spherical_new(C.type)) {
+          sph_forces(C, Rc, z, fr, fz);
+        } else if (is_wrapper(C.type)) {
+          // calcRforce/calczforce/calcphitorque of the wrapped components at
+          // phi - offset (the wrappers' Rforce/zforce/phitorque functions)
+          R fac, off, doff, wr, wz, wp;
+          wrapper_transform(C, Rc, t, fac, off, doff);
+          cyl_forces(C.nwrapped, C.wrapped, Rc, z, phi - off, t, wr, wz, wp);
+          fr = fac * (wr - wp * doff);
+          fz = fac * wz;
+          fp = fac * wp;
+        }
     }
     FR += fr; Fz += fz; Fphi += fp;
   }
@@ -736,6 +833,12 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
           R Phip, Phipp;
           sph_dphi(C, Rc, Phip, Phipp);
           fr = -Phip;
+        } else if (is_wrapper(C.type)) {  // the wrappers' Planar functions
+          R fac, off, doff, wr, wp;
+          wrapper_transform(C, Rc, t, fac, off, doff);
+          planar_forces(C.nwrapped, C.wrapped, Rc, phi - off, t, wr, wp);
+          fr = fac * (wr - wp * doff);
+          fp = fac * wp;
         }
     }
     FR += fr; Fphi += fp;
@@ -770,7 +873,7 @@ void planar_rect_deriv(int npot, const Component* P, R t, const R* q, R* a) {
 inline bool has_hessian(int npot, const Component* P) {
   for (int i = 0; i < npot; i++) {
     const int c = P[i].type;
-    if (c == kDehnenBar || (c == kLogHalo && P[i].p[3] < 1)) return false;
+    if (c == kDehnenBar || (c == kLogHalo && P[i].p[3] < 1) || is_wrapper(c)) return false;
   }
   return true;
 }
@@ -861,7 +964,15 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kDehnenBar: pot += bar_eval(C, Rc, z, phi, t); break;
       case kKuzminDisk: pot += kuzmin_eval(C, Rc, z); break;
       default:
-        if (is_spherical_new(C.type)) pot += sph_eval(C, sqrt(Rc * Rc + z * z));
+        if (is_spherical_new(C.type)) {
+          pot += sph_eval(C, sqrt(Rc * Rc + z * z));
+        } else if (is_wrapper(C.type)) {
+          // the wrappers' Python _evaluate (the wrapped potential at
+          // phi - offset and the same t, times the factor)
+          R fac, off, doff;
+          wrapper_transform(C, Rc, t, fac, off, doff);
+          pot += fac * potential_value(C.nwrapped, C.wrapped, Rc, z, phi - off, t);
+        }
     }
   }
   return pot;

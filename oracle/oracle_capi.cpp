@@ -23,7 +23,7 @@ using namespace orc;
 
 namespace {
 
-constexpr int kMaxPot = 16;
+constexpr int kMaxPot = kOracleMaxComp;
 
 // One orbit: integrateFullOrbit.c:1133-1141 (D = 3, rows of 6) or
 // integratePlanarOrbit.c:826-835 (D = 2, rows of 4).

@@ -30,6 +30,35 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+# packed arguments per component (integrateFullOrbit.py:65-365; the planar
+# packing gives PowerSphericalPotentialwCutoff 3) and wrappers' own arguments
+_NARGS = {0: 4, 1: 6, 5: 3, 7: 2, 8: 2, 9: 2, 10: 2, 14: 2, 15: 5, 17: 2, 18: 2, 19: 2, 20: 2,
+          33: 2, 34: 3, 35: 3, -1: 4, -2: 3, -4: 5, -5: 3}
+
+
+def npot_of(pot_type, pot_args, planar=False):
+    """galpy's npot for packed arrays: the number of top-level components
+    (a wrapper and the components it wraps count once)."""
+    ti = ai = 0
+
+    def walk():
+        nonlocal ti, ai
+        code = int(pot_type[ti])
+        ti += 1
+        if code < 0:
+            nw = int(pot_args[ai])
+            ai += 1
+            for _ in range(nw):
+                walk()
+        ai += 3 if (planar and code == 15) else _NARGS[code]
+
+    n = 0
+    while ti < len(pot_type):
+        walk()
+        n += 1
+    return n
+
+
 def oracle_integrate(pot_type, pot_args, yo, t, method, dt=None, rtol=LN1EM12, atol=LN1EM12,
                      nthreads=0, planar=False):
     """Run the oracle's integrateFullOrbit (returns result, err, stats[N,4],
@@ -54,7 +83,7 @@ def oracle_integrate(pot_type, pot_args, yo, t, method, dt=None, rtol=LN1EM12, a
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
     fn = lib.oracle_integrate_planar_stats if planar else lib.oracle_integrate_stats
     fn(
-        ctypes.c_int(n), _p(yo), ctypes.c_int(nt), _p(t), ctypes.c_int(indiv), ctypes.c_int(len(pt)),
+        ctypes.c_int(n), _p(yo), ctypes.c_int(nt), _p(t), ctypes.c_int(indiv), ctypes.c_int(npot_of(pt, pa, planar)),
         _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(rtol),
         ctypes.c_double(atol), _p(res), _p(err), ctypes.c_int(method), _p(steps), _p(rej), _p(rhs),
         _p(dtu), ctypes.c_int(nthreads),
@@ -73,7 +102,7 @@ def oracle_forces(pot_type, pot_args, R, z, phi=None, t=None):
     out = [numpy.empty(n) for _ in range(3)]
     pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
-    rc = lib.oracle_eval_forces(len(pt), _p(pt), _p(pa), n, _p(R), _p(z), _p(phi), _p(t), *[_p(o) for o in out])
+    rc = lib.oracle_eval_forces(npot_of(pt, pa), _p(pt), _p(pa), n, _p(R), _p(z), _p(phi), _p(t), *[_p(o) for o in out])
     assert rc == 0
     return tuple(out)
 
@@ -88,7 +117,7 @@ def oracle_potential(pot_type, pot_args, R, z, phi=None, t=None):
     out = numpy.empty(n)
     pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
-    rc = lib.oracle_eval_potential(len(pt), _p(pt), _p(pa), n, _p(R), _p(z), _p(phi), _p(t), _p(out))
+    rc = lib.oracle_eval_potential(npot_of(pt, pa), _p(pt), _p(pa), n, _p(R), _p(z), _p(phi), _p(t), _p(out))
     assert rc == 0
     return out
 
@@ -105,7 +134,7 @@ def oracle_count_ops(pot_type, pot_args, yo, t, method, dt=None, planar=False):
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
     fn = lib.oracle_count_ops_planar if planar else lib.oracle_count_ops
     rc = fn(
-        ctypes.c_int(n), _p(yo), ctypes.c_int(nt), _p(t), ctypes.c_int(int(t.ndim > 1)), ctypes.c_int(len(pt)),
+        ctypes.c_int(n), _p(yo), ctypes.c_int(nt), _p(t), ctypes.c_int(int(t.ndim > 1)), ctypes.c_int(npot_of(pt, pa, planar)),
         _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(LN1EM12),
         ctypes.c_double(LN1EM12), _p(res), _p(err), ctypes.c_int(method), _p(out),
     )
@@ -144,7 +173,7 @@ def oracle_integrate_dxdv(pot_type, pot_args, w, t, method, dim=3, dt=None, rtol
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
     lib.oracle_integrate_dxdv(
         ctypes.c_int(n), ctypes.c_int(dim), _p(w), ctypes.c_int(nt), _p(t), ctypes.c_int(int(t.ndim > 1)),
-        ctypes.c_int(len(pt)), _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt),
+        ctypes.c_int(npot_of(pt, pa, dim == 2)), _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt),
         ctypes.c_double(rtol), ctypes.c_double(atol), _p(res), _p(err), ctypes.c_int(method),
         ctypes.c_int(nthreads), None if steps is None else _p(steps),
     )
@@ -163,7 +192,7 @@ def oracle_count_ops_dxdv(pot_type, pot_args, w, t, method, dim=3, dt=None):
     pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
     rc = lib.oracle_count_ops_dxdv(
-        ctypes.c_int(n), ctypes.c_int(dim), _p(w), ctypes.c_int(nt), _p(t), ctypes.c_int(len(pt)), _p(pt),
+        ctypes.c_int(n), ctypes.c_int(dim), _p(w), ctypes.c_int(nt), _p(t), ctypes.c_int(npot_of(pt, pa, dim == 2)), _p(pt),
         _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(LN1EM12),
         ctypes.c_double(LN1EM12), _p(res), _p(err), ctypes.c_int(method), _p(out),
     )
@@ -194,7 +223,7 @@ def oracle_planar_forces(pot_type, pot_args, R, phi=None, t=None):
     FR, Fphi = numpy.empty(n), numpy.empty(n)
     pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
-    rc = lib.oracle_eval_planar_forces(len(pt), _p(pt), _p(pa), n, _p(R), _p(phi), _p(t), _p(FR), _p(Fphi))
+    rc = lib.oracle_eval_planar_forces(npot_of(pt, pa, True), _p(pt), _p(pa), n, _p(R), _p(phi), _p(t), _p(FR), _p(Fphi))
     assert rc == 0
     return FR, Fphi
 
@@ -292,7 +321,7 @@ def oracle_dxdv(pot_type, pot_args, w, nde, t, method, dt=None):
     pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
     rc = lib.oracle_symplectic_dxdv(
-        ctypes.c_int(len(w)), _p(w), ctypes.c_int(nde), ctypes.c_int(len(t)), _p(t), ctypes.c_int(len(pt)),
+        ctypes.c_int(len(w)), _p(w), ctypes.c_int(nde), ctypes.c_int(len(t)), _p(t), ctypes.c_int(npot_of(pt, pa)),
         _p(pt), _p(pa), ctypes.c_double(DT_SENTINEL if dt is None else dt), ctypes.c_double(LN1EM12),
         ctypes.c_double(LN1EM12), ctypes.c_int(method), _p(out),
     )
@@ -322,7 +351,7 @@ def oracle_hessian(pot_type, pot_args, R, z):
     out = [numpy.empty(R.size) for _ in range(3)]
     pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
-    rc = lib.oracle_eval_hessian(len(pt), _p(pt), _p(pa), R.size, _p(R), _p(z), *[_p(o) for o in out])
+    rc = lib.oracle_eval_hessian(npot_of(pt, pa), _p(pt), _p(pa), R.size, _p(R), _p(z), *[_p(o) for o in out])
     assert rc == 0
     return tuple(out)
 
