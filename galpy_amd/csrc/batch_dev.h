@@ -212,11 +212,13 @@ __device__ int gh_batch_run_tile(const gh_batch_group& g, const gh_batch_tile& t
 
 // Compositions with a specialised force: the same ones the single-group
 // kernels specialise (MWPotential2014, LogarithmicHalo + DehnenBar,
-// Hernquist, LogarithmicHalo); 0 = any other.
-enum GhSpecCode { GH_SPEC_ANY = 0, GH_SPEC_MW2014, GH_SPEC_LOGBAR, GH_SPEC_HERN, GH_SPEC_LOG };
+// Hernquist, LogarithmicHalo); 0 = any other; compositions with wrapped
+// components run their own run-time dispatch instance.
+enum GhSpecCode { GH_SPEC_ANY = 0, GH_SPEC_MW2014, GH_SPEC_LOGBAR, GH_SPEC_HERN, GH_SPEC_LOG,
+                  GH_SPEC_WRAPPED };
 
 __device__ __forceinline__ int gh_spec_code(const GhPotTable* __restrict__ T) {
-  if (T->has_wrap) return GH_SPEC_ANY;  // wrappers: run-time dispatch only
+  if (T->has_wrap) return GH_SPEC_WRAPPED;
   const int n = T->npot;
   const int c0 = T->comp[0].type;
   if (n == 3 && c0 == GH_POWERSPHCUTOFF && T->comp[1].type == GH_MIYAMOTONAGAI &&
@@ -280,6 +282,12 @@ gh_batch_kernel(const gh_batch_group* __restrict__ groups, const gh_batch_tile* 
         break;
       case GH_SPEC_LOG:
         left = gh_batch_run_tile<OPL, DOP, GhSpec<GH_LOGHALO>>(g, tile, budget, cancel);
+        break;
+      case GH_SPEC_WRAPPED:
+        if constexpr (GEN)
+          left = gh_batch_run_tile<OPL, DOP, GhSpecWrapped>(g, tile, budget, cancel);
+        else
+          left = gh_batch_reject_tile(g, tile);
         break;
       default:
         if constexpr (GEN)

@@ -873,7 +873,8 @@ gh_eval_kernel(int n, const double* __restrict__ R, const double* __restrict__ z
   double sp, cp;
   sincos(phi[i], &sp, &cp);
   if (pot) pot[i] = gh_potential(T, R[i], z[i], cp, sp, t[i]);
-  const GhCyl F = gh_cyl_forces(T, R[i], R[i] * R[i], z[i], cp, sp, t[i]);
+  // the wrapper-aware sum serves every table (evaluation is not a hot path)
+  const GhCyl F = gh_cyl_forces<GhSpecWrapped>(T, R[i], R[i] * R[i], z[i], cp, sp, t[i]);
   if (FR) FR[i] = F.FR;
   if (Fz) Fz[i] = F.Fz;
   if (Fphi) Fphi[i] = F.Fphi;
@@ -1121,6 +1122,7 @@ int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indi
   if (types_are(h, {GH_LOGHALO, GH_DEHNENBAR})) GH_LAUNCH(LogBar);
   if (types_are(h, {GH_HERNQUIST})) GH_LAUNCH(Hern);
   if (types_are(h, {GH_LOGHALO})) GH_LAUNCH(Log);
+  if (h->has_wrap) GH_LAUNCH(GhSpecWrapped);
   GH_LAUNCH(Any);
 #undef GH_LAUNCH
 }
@@ -1596,6 +1598,8 @@ GH_EXPORT int gh_flow_hip(int n, const double* d_w0, double* d_wT, double t0, do
   using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
   if (types_are(h, {GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW}))
     launch_flow<MW2014>(method, grid, st, n, d_w0, d_wT, t0, dt, nsteps, h->dev);
+  else if (h->has_wrap)
+    launch_flow<GhSpecWrapped>(method, grid, st, n, d_w0, d_wT, t0, dt, nsteps, h->dev);
   else
     launch_flow<GhSpec<>>(method, grid, st, n, d_w0, d_wT, t0, dt, nsteps, h->dev);
   GH_CHECK(hipGetLastError());

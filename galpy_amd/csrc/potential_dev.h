@@ -595,28 +595,40 @@ __device__ __forceinline__ void gh_wrap_transform(const GhPotTable* __restrict__
   }
 }
 
-// Force of a wrapped component: the component's own force at the position
-// rotated by -offset (its F_R, F_z and F_phi are the same cylindrical
-// components in either frame), scaled by the wrappers' factor, with the
-// corotating term -F_phi d offset / dR in F_R (in the accumulated form:
-// F_R / R gains -(F_phi / R^2) R d offset / dR).
-__device__ __noinline__ void gh_comp_force_wrapped(const GhPotTable* __restrict__ T, int i,
-                                                   const GhGeom& g, GhAcc& F) {
-  const double R = sqrt(g.R2);
-  double fac, off, doff;
-  gh_wrap_transform(T, &T->comp[i], R, g.t, fac, off, doff);
+// Force of component i with the wrappers around it: the component's own
+// force at the position rotated by -offset (its F_R, F_z and F_phi are the
+// same cylindrical components in either frame), scaled by the wrappers'
+// factor, with the corotating term -F_phi d offset / dR in F_R (in the
+// accumulated form: F_R / R gains -(F_phi / R^2) R d offset / dR).  One
+// inlined dispatch serves wrapped and bare components alike, so the
+// run-time dispatch kernels carry no call.
+__device__ __forceinline__ void gh_comp_force_any(const GhPotTable* __restrict__ T, int i,
+                                                  const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
   GhGeom gw = g;
-  if (off != 0.0) {
+  double fac = 1.0, corr = 0.0;
+  const bool wrapped = C->nwrap != 0;
+  if (wrapped) {
+    const double R = sqrt(g.R2);
+    double off, doff;
+    gh_wrap_transform(T, C, R, g.t, fac, off, doff);
     double so, co;
     sincos(off, &so, &co);
     gw.x = g.x * co + g.y * so;
     gw.y = g.y * co - g.x * so;
+    corr = R * doff;
   }
   GhAcc L = {0.0, 0.0, 0.0};
   gh_comp_force_dyn(T, i, gw, L);
-  F.FRoR += fac * (L.FRoR - L.FphioR2 * R * doff);
-  F.Fz += fac * L.Fz;
-  F.FphioR2 += fac * L.FphioR2;
+  if (wrapped) {
+    F.FRoR += fac * (L.FRoR - L.FphioR2 * corr);
+    F.Fz += fac * L.Fz;
+    F.FphioR2 += fac * L.FphioR2;
+  } else {
+    F.FRoR += L.FRoR;
+    F.Fz += L.Fz;
+    F.FphioR2 += L.FphioR2;
+  }
 }
 
 // A potential composition known at compile time: GhSpec<15, 5, 9> is
@@ -634,16 +646,21 @@ struct GhAccSum<GhSpec<>> {
   __device__ static __forceinline__ void add(const GhPotTable* __restrict__ T, const GhGeom& g,
                                              GhAcc& F) {
     const int npot = T->npot;
-    if (T->has_wrap) {
-      for (int i = 0; i < npot; i++) {
-        if (T->comp[i].nwrap)
-          gh_comp_force_wrapped(T, i, g, F);
-        else
-          gh_comp_force_dyn(T, i, g, F);
-      }
-    } else {
-      for (int i = 0; i < npot; i++) gh_comp_force_dyn(T, i, g, F);
-    }
+    for (int i = 0; i < npot; i++) gh_comp_force_dyn(T, i, g, F);
+  }
+};
+
+// Any composition with wrapped components: its own kernel instances, so that
+// the wrapper transform adds nothing to the run-time dispatch kernels of
+// unwrapped compositions.
+struct GhSpecWrapped {};
+
+template <>
+struct GhAccSum<GhSpecWrapped> {
+  __device__ static __forceinline__ void add(const GhPotTable* __restrict__ T, const GhGeom& g,
+                                             GhAcc& F) {
+    const int npot = T->npot;
+    for (int i = 0; i < npot; i++) gh_comp_force_any(T, i, g, F);
   }
 };
 

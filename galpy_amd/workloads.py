@@ -45,10 +45,23 @@ def mn_plummer_burkert():
     )
 
 
+def mw2014_rotating_triaxial():
+    """MWPotential2014 plus a triaxial logarithmic perturbation (amp 0.05,
+    q = 0.9, b = 0.8) turning rigidly at omega = 0.6 through
+    SolidBodyRotationWrapperPotential: time-dependent in the inertial frame,
+    with the Jacobi integral E - 0.6 Lz conserved."""
+    return P.CompositePotential(
+        *P.MWPotential2014_factory(),
+        P.SolidBodyRotationWrapperPotential(pot=P.LogarithmicHaloPotential(amp=0.05, q=0.9, b=0.8),
+                                            omega=0.6, pa=0.0),
+    )
+
+
 POTENTIAL_NAMES = {
     mw2014: "MWPotential2014",
     loghalo_bar: "LogarithmicHalo(q=0.9)+DehnenBar",
     mn_plummer_burkert: "MiyamotoNagai(a=3/8,b=0.28/8)+Plummer(b=0.3)+Burkert(a=2)",
+    mw2014_rotating_triaxial: "MWPotential2014+SolidBodyRotationWrapper(LogarithmicHalo(amp=0.05,q=0.9,b=0.8),omega=0.6)",
 }
 
 WORKLOADS = {
@@ -85,6 +98,9 @@ WORKLOADS = {
                            method="symplec4_hip", omega=0.0, seed=0, dt=0.01),
     "mpb_dop853_256k": dict(pot=mn_plummer_burkert, n=262144, t=(0.0, 500.0, 1001),
                            method="dop853_hip", omega=0.0, seed=0),
+    # a wrapped (rotating) component: DOP853, drift of the Jacobi integral
+    "mw2014_rottriax_dop853_256k": dict(pot=mw2014_rotating_triaxial, n=262144, t=(0.0, 500.0, 1001),
+                                       method="dop853_hip", omega=0.6, seed=0),
 }
 
 

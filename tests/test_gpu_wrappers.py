@@ -230,3 +230,21 @@ def test_wrappers_refused_where_there_is_no_hessian():
     with pytest.raises(RuntimeError):
         integrateFullOrbit_dxdv_hip(pot, y0, numpy.ones_like(y0), numpy.linspace(0, 1, 3), "symplec4_hip")
     assert "Hessian" in _lib.last_error()
+
+
+@pytest.mark.parametrize("method,code", [("symplec4_hip", 3), ("leapfrog_hip", 0)])
+def test_flow_runs_the_wrappers(method, code):
+    """gh_flow_hip (the differentiation layer's forward map) on a wrapped
+    potential equals the oracle's fixed-step integration."""
+    import torch
+
+    from galpy_amd.adjoint import flow_raw
+    from galpy_amd.integrate import PotentialTable
+    from helpers import oracle_dxdv
+
+    pot = _cases()["grown rotating triaxial halo + MN"]
+    w0 = cyl_to_rect(ics(128, seed=6))
+    nsteps, dt = 400, 0.01
+    wT = flow_raw(PotentialTable(pot), torch.tensor(w0, device="cuda"), dt, nsteps, method).cpu().numpy()
+    ref = oracle_dxdv(*_pk(pot), w0, 0, numpy.arange(nsteps + 1) * dt, code, dt=dt)
+    assert numpy.abs(wT - ref).max() < 1e-10 * max(1.0, numpy.abs(ref).max())
