@@ -288,6 +288,73 @@ def bench_adjoint(args):
                       "methods": per}), flush=True)
 
 
+def bench_summary(args):
+    """On-device post-processing: rperi, rap, zmax and e (galpy's
+    Orbit.rperi/rap/zmax/e, analytic=False) over the trajectories of the
+    headline integration (MWPotential2014, DOP853, 1,048,576 orbits x 1001
+    outputs, 50.4 GB resident in HBM; integrated once, untimed).  A step is
+    one gh_orbit_summary_hip pass over all rows; the kernel reads every row
+    once, so it is priced against HBM bandwidth."""
+    import torch
+
+    from galpy_amd import _lib, workloads
+    from galpy_amd.integrate import PotentialTable, integrate_device, orbit_summary_device
+
+    _lib.load()
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a HIP device")
+    w = workloads.make("mw2014_dop853_1m", n=args.norbits)
+    n, nt = w["n"], len(w["t"])
+    table = PotentialTable(w["pot"])
+    tt = torch.tensor(w["t"], dtype=torch.float64, device="cuda")
+    rows, err, _ = integrate_device(table, torch.tensor(w["yo"], device="cuda"), tt, "dop853_hip")
+    torch.cuda.synchronize()
+    assert int((err != 0).sum()) == 0
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        orbit_summary_device(rows, stream=stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        out = orbit_summary_device(rows, stream=stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kms = [a.elapsed_time(b) for a, b in ev]
+    read = n * nt * 6 * 8
+    written = 4 * n * 8
+    achieved = (read + written) / (numpy.mean(kms) / 1e3) / 1e9
+    # the reference computes these with numpy on the host (Orbits.py:3642-4010):
+    # the same expressions on a bounded sample of the rows, one core
+    sample = min(n, 16384)
+    host = rows[:sample].cpu().numpy()
+    c0 = time.perf_counter()
+    R, z = host[..., 0], host[..., 3]
+    r = numpy.sqrt(R**2 + z**2)
+    rp, ra = numpy.amin(r, axis=-1), numpy.amax(r, axis=-1)
+    _ = ((ra - rp) / (ra + rp), numpy.amax(numpy.fabs(z), axis=-1))
+    cel = time.perf_counter() - c0
+    same = numpy.array_equal(out["rperi"][:sample].cpu().numpy(), rp) and numpy.array_equal(
+        out["rap"][:sample].cpu().numpy(), ra)
+    print(json.dumps({
+        "metric": "trajectory rows summarised/s (rperi, rap, zmax, e) on 1 MI355X",
+        "value": n * nt * args.steps / el, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "summary_mw2014_dop853_1m", "orbits": n, "outputs": nt,
+                   "row_bytes": 48, "source": "MWPotential2014 DOP853 t=linspace(0,500,1001)"},
+        "kernel_ms": kms, "matches_numpy_on_sample": bool(same),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None, "kernel": "gh_summary_kernel<6>",
+                     "algorithmic_bytes": read + written,
+                     "note": "algorithmic bytes = every row read once (48 B) + 4 doubles written per orbit"},
+        "cpu_baseline": {"value": sample * nt / cel, "unit": "rows/s", "cores": 1, "kind": "reference",
+                         "sample": f"galpy's numpy expressions on the first {sample} orbits' rows ({cel:.2f} s)"},
+    }), flush=True)
+
+
 def bench_copy(args):
     """A plain device-to-device copy of known size: the calibration run for
     the FETCH_SIZE / WRITE_SIZE hardware counters (bytes read and written by
@@ -327,6 +394,9 @@ def main():
         return
     if args.workload == "copy":
         bench_copy(args)
+        return
+    if args.workload.startswith("summary"):
+        bench_summary(args)
         return
     if args.workload.startswith("adjoint"):
         bench_adjoint(args)

@@ -118,6 +118,11 @@ SIGNATURES["gh_integrate_dxdv_hip"] = (
 SIGNATURES["integratePlanarOrbit"] = SIGNATURES["integrateFullOrbit"]
 SIGNATURES["integratePlanarOrbit_hip"] = SIGNATURES["integrateFullOrbit_hip"]
 SIGNATURES["gh_planar_energy_drift_hip"] = SIGNATURES["gh_energy_drift_hip"]
+SIGNATURES["gh_orbit_summary_hip"] = (
+    ctypes.c_int,
+    [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+     ctypes.c_void_p, ctypes.c_double] + [ctypes.c_void_p] * 7,
+)
 
 
 def load(path=None):

@@ -383,6 +383,8 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
   if (dt_in == GH_DT_SENTINEL) {
     if (METHOD == 0)
       dt = gh_leapfrog_estimate_step(F, gh_base(q), gh_base(p), init_dt, tt[0], rtol, atol);
+    else if (METHOD == 4)
+      dt = gh_symplec6_estimate_step(F, gh_base(q), gh_base(p), init_dt, tt[0], rtol, atol);
     else
       dt = gh_symplec4_estimate_step(F, gh_base(q), gh_base(p), init_dt, tt[0], rtol, atol);
   }
@@ -404,6 +406,22 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
       p = gh_drift(p, gh_accel(F, to + dt / 2.0, q12), dt);
       q = gh_drift(q12, p, dt / 2.0);
       to = to + dt;
+    } else if (METHOD == 4) {
+      // symplec6 (bovy_symplecticode.c:403-586): seven kicks per step, the
+      // last drift of a step merged with the first of the next, (c8 + c1) dt
+      q = gh_drift(q, p, gh_s6c[0] * dt);
+      to += gh_s6c[0] * dt;
+      const double cmerge = gh_s6c[7] + gh_s6c[0];
+      for (long jj = 0; jj < ndt; jj++) {
+        const bool last = jj == ndt - 1;
+#pragma unroll 1
+        for (int k = 0; k < 7; k++) {
+          p = gh_drift(p, gh_accel(F, to, q), gh_s6d[k] * dt);
+          const double cn = (k == 6) ? (last ? gh_s6c[7] : cmerge) : gh_s6c[k + 1];
+          q = gh_drift(q, p, cn * dt);
+          to += cn * dt;
+        }
+      }
     } else {
       const double c1 = GH_S4C1, c2 = GH_S4C2, c3 = GH_S4C3, c4 = GH_S4C4;
       const double d1 = GH_S4D1, d2 = GH_S4D2, d3 = GH_S4D3;
@@ -863,6 +881,63 @@ gh_energy_kernel(int nobj, int nt, const double* __restrict__ rows, const double
   demax[orbit] = dmax;
 }
 
+// Post-processing of stored trajectories (Orbits.py rperi :3733-3778,
+// rap :3686-3731, e :3642-3684, zmax :3966-4010, E :2902-3115, Jacobi
+// :3298-3360): r = sqrt(R^2 + z^2) over all outputs, rperi = min r,
+// rap = max r, e = (rap - rperi) / (rap + rperi), zmax = max |z|; E at every
+// output, and E - omega R vT.  One wavefront per orbit: lane k takes rows
+// k, k + 64, ..., so a wavefront reads 64 consecutive rows (64 N doubles) at
+// a time, coalesced, and E rows are written likewise; min/max/|z| reduce
+// across the wavefront by shuffles.  No FMA contraction in r, so rperi, rap
+// and zmax are exactly numpy's of the same rows.
+constexpr int kSummaryOrbitsPerBlock = 4;
+
+template <int N>
+__global__ void __launch_bounds__(64 * kSummaryOrbitsPerBlock)
+gh_summary_kernel(int nobj, int nt, const double* __restrict__ rows, const double* __restrict__ t,
+                  int indiv_t, const GhPotTable* __restrict__ T, double omega,
+                  double* __restrict__ rperi, double* __restrict__ rap, double* __restrict__ zmax,
+                  double* __restrict__ ecc, double* __restrict__ E, double* __restrict__ EJ) {
+#pragma clang fp contract(off)
+  const long orbit = (long)blockIdx.x * kSummaryOrbitsPerBlock + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (orbit >= nobj) return;  // uniform across the wavefront
+  const double* __restrict__ r = rows + orbit * (long)nt * N;
+  const double* __restrict__ tt = t + (indiv_t ? orbit * nt : 0);
+  double rmin = INFINITY, rmax = -INFINITY, zm = 0.0;
+  const bool energies = (E != nullptr) || (EJ != nullptr);
+  for (int j = lane; j < nt; j += 64) {
+    const double* __restrict__ row = r + (long)N * j;
+    const double R = row[0];
+    const double z = (N == 6) ? row[3] : 0.0;
+    const double rs = sqrt(R * R + z * z);
+    rmin = fmin(rmin, rs);
+    rmax = fmax(rmax, rs);
+    zm = fmax(zm, fabs(z));
+    if (energies) {
+      const double vR = row[1], vT = row[2], vz = (N == 6) ? row[4] : 0.0;
+      double sp, cp;
+      sincos(row[N - 1], &sp, &cp);
+      const double En = gh_potential(T, R, z, cp, sp, tt[j]) + vR * vR / 2.0 + vT * vT / 2.0 +
+                        vz * vz / 2.0;
+      if (E) E[orbit * (long)nt + j] = En;
+      if (EJ) EJ[orbit * (long)nt + j] = En - omega * R * vT;
+    }
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    rmin = fmin(rmin, __shfl_xor(rmin, m));
+    rmax = fmax(rmax, __shfl_xor(rmax, m));
+    zm = fmax(zm, __shfl_xor(zm, m));
+  }
+  if (lane == 0) {
+    if (rperi) rperi[orbit] = rmin;
+    if (rap) rap[orbit] = rmax;
+    if (zmax) zmax[orbit] = zm;
+    if (ecc) ecc[orbit] = (rmax - rmin) / (rmax + rmin);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
 gh_eval_kernel(int n, const double* __restrict__ R, const double* __restrict__ z,
                const double* __restrict__ phi, const double* __restrict__ t,
@@ -1061,6 +1136,16 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                          nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
                          cancel);
       break;
+    case 4:  // symplec6: plain orbits (galpy has no symplec6 tangent map)
+      if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
+        hipLaunchKernelGGL((gh_symplectic_kernel<4, Spec, V>), grid, block, 0, stream, nobj,
+                           d_yo, nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err,
+                           d_stats, cancel);
+        break;
+      } else {
+        set_error("symplec6 has no tangent-map (integrate_dxdv) stepper");
+        return -1;
+      }
     case 6: {
       int dev = 0;
       GH_CHECK(hipGetDevice(&dev));
@@ -1081,8 +1166,8 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
       }
     }
     default:
-      set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, 6 dop853)",
-                odeint_type);
+      set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, "
+                "4 symplec6, 6 dop853)", odeint_type);
       return -1;
   }
   GH_CHECK(hipGetLastError());
@@ -1176,9 +1261,9 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
     set_error("nt=%d: need at least two output times", nt);
     return -1;
   }
-  if (odeint_type != 0 && odeint_type != 3 && odeint_type != 6) {
-    set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, 6 dop853)",
-              odeint_type);
+  if (odeint_type != 0 && odeint_type != 3 && odeint_type != 4 && odeint_type != 6) {
+    set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, "
+              "4 symplec6, 6 dop853)", odeint_type);
     return -1;
   }
   GhPotTable table;
@@ -1451,6 +1536,34 @@ GH_EXPORT int gh_planar_energy_drift_hip(int nobj, int nt, const double* d_rows,
   const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
   hipLaunchKernelGGL(gh_energy_kernel<4>, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows,
                      d_t, indiv_t, ((const GhPotHandle*)pot)->dev, omega, d_e0, d_demax);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+GH_EXPORT int gh_orbit_summary_hip(int nobj, int nt, int ncol, const double* d_rows,
+                                   const double* d_t, int indiv_t, const void* pot, double omega,
+                                   double* d_rperi, double* d_rap, double* d_zmax, double* d_e,
+                                   double* d_E, double* d_EJ, void* stream) {
+  g_errbuf[0] = 0;
+  if (nobj <= 0) return 0;
+  if (nt < 1 || (ncol != 6 && ncol != 4)) {
+    set_error("gh_orbit_summary_hip: nt=%d ncol=%d (need nt >= 1, ncol 6 or 4)", nt, ncol);
+    return -1;
+  }
+  const bool energies = d_E != nullptr || d_EJ != nullptr;
+  if (energies && (pot == nullptr || d_t == nullptr)) {
+    set_error("gh_orbit_summary_hip: E and the Jacobi integral need the potential and the times");
+    return -1;
+  }
+  const GhPotTable* T = pot ? ((const GhPotHandle*)pot)->dev : nullptr;
+  const dim3 grid((unsigned)((nobj + kSummaryOrbitsPerBlock - 1) / kSummaryOrbitsPerBlock));
+  const dim3 block(64 * kSummaryOrbitsPerBlock);
+  if (ncol == 6)
+    hipLaunchKernelGGL(gh_summary_kernel<6>, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows,
+                       d_t, indiv_t, T, omega, d_rperi, d_rap, d_zmax, d_e, d_E, d_EJ);
+  else
+    hipLaunchKernelGGL(gh_summary_kernel<4>, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows,
+                       d_t, indiv_t, T, omega, d_rperi, d_rap, d_zmax, d_e, d_E, d_EJ);
   GH_CHECK(hipGetLastError());
   return 0;
 }

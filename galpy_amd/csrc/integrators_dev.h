@@ -227,6 +227,58 @@ __device__ __forceinline__ double gh_symplec4_estimate_step(const F& T, V qo, V 
   return dt;
 }
 
+// symplec6 coefficients (Yoshida 1990), bovy_symplecticode.c:100-107: drifts
+// c1..c8, kicks d1..d7.  The seven-kick loops below stay rolled (one force
+// evaluation per loop body), so a run-time dispatch kernel does not carry
+// the force switch 28 times.
+__constant__ double gh_s6c[8] = {0.392256805238780, 0.510043411918458, -0.471053385409758,
+                                 0.687531682525198e-1, 0.687531682525198e-1, -0.471053385409758,
+                                 0.510043411918458, 0.392256805238780};
+__constant__ double gh_s6d[7] = {0.784513610477560, 0.235573213359357, -0.117767998417887e1,
+                                 0.131518632068391e1, -0.117767998417887e1, 0.235573213359357,
+                                 0.784513610477560};
+
+// symplec6_estimate_step, bovy_symplecticode.c:810-1000: one step of dt
+// against two of dt/2 (middle drifts merged as (c1 + c8) dt/2), the scales
+// and norm of the symplec4 estimator.
+template <class F, class V>
+__device__ __forceinline__ double gh_symplec6_estimate_step(const F& T, V qo, V po, double dt,
+                                                            double to, double rtol, double atol) {
+  double err = 2.0;
+  const double init_dt = dt;
+  double sq, sp;
+  gh_est_scales(qo, po, rtol, atol, sq, sp);
+  dt *= 2.0;
+  while (err > 1.0 && init_dt / dt < GH_MAX_DT_REDUCE) {
+    dt /= 2.0;
+    V q = gh_drift(qo, po, gh_s6c[0] * dt);
+    V p = po;
+    to += gh_s6c[0] * dt;
+#pragma unroll 1
+    for (int k = 0; k < 7; k++) {
+      p = gh_drift(p, gh_accel(T, to, q), gh_s6d[k] * dt);
+      q = gh_drift(q, p, gh_s6c[k + 1] * dt);
+      to += gh_s6c[k + 1] * dt;
+    }
+    const V q11 = q, p11 = p;
+    to -= dt;
+    q = gh_drift(qo, po, gh_s6c[0] * dt / 2.0);
+    p = po;
+    to += gh_s6c[0] * dt / 2.0;
+#pragma unroll 1
+    for (int k = 0; k < 14; k++) {
+      const int kk = k < 7 ? k : k - 7;
+      p = gh_drift(p, gh_accel(T, to, q), gh_s6d[kk] * dt / 2.0);
+      const double cn = (k == 6) ? gh_s6c[0] + gh_s6c[7] : gh_s6c[kk + 1];
+      q = gh_drift(q, p, cn * dt / 2.0);
+      to += cn * dt / 2.0;
+    }
+    err = gh_est_err(q11, q, p11, p, sq, sp);
+    to -= dt;
+  }
+  return dt;
+}
+
 // bovy_coords.c:101-115 (cylindrical -> rectangular)
 __device__ __forceinline__ void gh_cyl_to_rect(const double* v, double* w) {
   double sp, cp;

@@ -28,6 +28,7 @@ DT_SENTINEL = -9999.99
 METHOD_CODES = {
     "leapfrog_hip": 0,
     "symplec4_hip": 3,
+    "symplec6_hip": 4,
     "dop853_hip": 6,
 }
 
@@ -212,6 +213,42 @@ def energy_drift_device(table, rows, t, omega=0.0, stream=None):
     )
     _lib.check(rc, fn)
     return e0, demax
+
+
+def orbit_summary_device(rows, t=None, table=None, omega=0.0, energies=False, jacobi=False,
+                         stream=None):
+    """Per-orbit rperi, rap, zmax, e of stored rows on the device (galpy's
+    ``Orbit.rperi/rap/zmax/e`` with ``analytic=False``, Orbits.py:3642-4010),
+    and optionally E and the Jacobi integral E - omega Lz at every output
+    (``Orbit.E``, ``Orbit.Jacobi``).  ``rows``: float64 cuda tensor
+    ``[N, nt, 6]`` (R,vR,vT,z,vz,phi) or planar ``[N, nt, 4]``; the energies
+    need ``t`` ([nt] or [N, nt]) and ``table`` (a PotentialTable).  Returns a
+    dict of device tensors; asynchronous on ``stream``."""
+    import torch
+
+    nobj, nt, ncol = rows.shape
+    if ncol not in (4, 6) or not rows.is_contiguous() or rows.dtype != torch.float64:
+        raise ValueError("rows must be a contiguous float64 [N, nt, 6] or [N, nt, 4] tensor")
+    if (energies or jacobi) and (table is None or t is None):
+        raise ValueError("E and the Jacobi integral need the potential table and the times")
+    dev = rows.device
+    out = {k: torch.empty(nobj, dtype=torch.float64, device=dev) for k in ("rperi", "rap", "e")}
+    if ncol == 6:
+        out["zmax"] = torch.empty(nobj, dtype=torch.float64, device=dev)
+    if energies:
+        out["E"] = torch.empty((nobj, nt), dtype=torch.float64, device=dev)
+    if jacobi:
+        out["Jacobi"] = torch.empty((nobj, nt), dtype=torch.float64, device=dev)
+    if stream is None:
+        stream = torch.cuda.current_stream(dev)
+    p = lambda k: out[k].data_ptr() if k in out else None
+    rc = _lib.load().gh_orbit_summary_hip(
+        nobj, nt, ncol, rows.data_ptr(), None if t is None else t.data_ptr(),
+        int(t is not None and t.dim() > 1), None if table is None else table.ptr, float(omega),
+        p("rperi"), p("rap"), p("zmax"), p("e"), p("E"), p("Jacobi"), stream.cuda_stream,
+    )
+    _lib.check(rc, "gh_orbit_summary_hip")
+    return out
 
 
 def eval_device(table, R, z, phi, t, stream=None):

@@ -156,6 +156,23 @@ const void *gh_potential_device_ptr(const void *pot);
 int gh_energy_drift_hip(int nobj, int nt, const double *d_rows, const double *d_t, int indiv_t,
                         const void *pot, double omega, double *d_e0, double *d_demax,
                         void *stream);
+
+/* Post-processing of stored trajectories on the device (galpy computes these
+ * on the host from the integrated rows: Orbits.py rperi :3733-3778, rap
+ * :3686-3731, e :3642-3684, zmax :3966-4010, E :2902-3115, Jacobi
+ * :3298-3360), so only per-orbit numbers need to cross PCIe.  d_rows is
+ * [nobj][nt][ncol] with ncol 6 (R,vR,vT,z,vz,phi) or 4 (planar R,vR,vT,phi);
+ * per orbit: d_rperi = min r, d_rap = max r (r = sqrt(R^2 + z^2) over all
+ * outputs), d_zmax = max |z| (0 for planar rows), d_e = (rap - rperi) /
+ * (rap + rperi); per output ([nobj][nt]): d_E = Phi + v^2/2 and d_EJ =
+ * E - omega R vT.  Every output pointer may be NULL; the energies need pot
+ * (an uploaded potential) and d_t ([nt], or [nobj][nt] with indiv_t).
+ * Device pointers; asynchronous on stream.  No reference counterpart as a C
+ * function. */
+int gh_orbit_summary_hip(int nobj, int nt, int ncol, const double *d_rows, const double *d_t,
+                         int indiv_t, const void *pot, double omega, double *d_rperi,
+                         double *d_rap, double *d_zmax, double *d_e, double *d_E, double *d_EJ,
+                         void *stream);
 /* The same along planar rows d_rows[nobj*nt*4] (R,vR,vT,phi), z = vz = 0. */
 int gh_planar_energy_drift_hip(int nobj, int nt, const double *d_rows, const double *d_t,
                                int indiv_t, const void *pot, double omega, double *d_e0,

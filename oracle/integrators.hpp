@@ -173,6 +173,63 @@ R symplec4_estimate_step(const ForceCtx<R, D>& F, const R* qo, const R* po, R dt
   return dt;
 }
 
+// symplec6 coefficients (Yoshida 1990), bovy_symplecticode.c:100-107
+constexpr double S6C[8] = {0.392256805238780, 0.510043411918458, -0.471053385409758,
+                           0.687531682525198e-1, 0.687531682525198e-1, -0.471053385409758,
+                           0.510043411918458, 0.392256805238780};
+constexpr double S6D[7] = {0.784513610477560, 0.235573213359357, -0.117767998417887e1,
+                           0.131518632068391e1, -0.117767998417887e1, 0.235573213359357,
+                           0.784513610477560};
+
+// symplec6_estimate_step, bovy_symplecticode.c:810-1000: the symplec4
+// estimator with the seven-kick splitting -- one step of dt against two of
+// dt/2 (the middle drifts merged as (c1 + c8) dt/2), the same scales and
+// norm, halving from 2 init_dt.
+template <int D, class R>
+R symplec6_estimate_step(const ForceCtx<R, D>& F, const R* qo, const R* po, R dt, R to,
+                         double rtol, double atol) {
+  R err = 2.;
+  R init_dt = dt;
+  R q[3], p[3], q11[3], p11[3], a[3];
+  R sq, sp;
+  estimate_scales<D>(qo, po, rtol, atol, sq, sp);
+  dt *= 2.;
+  while (err > R(1.) && init_dt / dt < R(kMaxDtReduce)) {
+    dt /= 2.;
+    // one step of dt
+    leapq<D>(qo, po, R(S6C[0]) * dt, q);
+    to += R(S6C[0]) * dt;
+    for (int i = 0; i < D; i++) p[i] = po[i];
+    for (int k = 0; k < 7; k++) {
+      F.force(to, q, a);
+      leapp<D>(p, R(S6D[k]) * dt, a, p);
+      leapq<D>(q, p, R(S6C[k + 1]) * dt, q);
+      to += R(S6C[k + 1]) * dt;
+    }
+    for (int i = 0; i < D; i++) {
+      q11[i] = q[i];
+      p11[i] = p[i];
+    }
+    to -= dt;
+    // two steps of dt/2
+    leapq<D>(qo, po, R(S6C[0]) * dt / R(2.), q);
+    to += R(S6C[0]) * dt / R(2.);
+    for (int i = 0; i < D; i++) p[i] = po[i];
+    for (int half = 0; half < 2; half++) {
+      for (int k = 0; k < 7; k++) {
+        F.force(to, q, a);
+        leapp<D>(p, R(S6D[k]) * dt / R(2.), a, p);
+        const R cn = (k == 6 && half == 0) ? R(S6C[0] + S6C[7]) : R(S6C[k + 1]);
+        leapq<D>(q, p, cn * dt / R(2.), q);
+        to += cn * dt / R(2.);
+      }
+    }
+    err = estimate_err<D>(q11, q, p11, p, sq, sp);
+    to -= dt;
+  }
+  return dt;
+}
+
 // save_qp, bovy_symplecticode.c:74-78
 template <int D, class R> inline void save_qp(const R* q, const R* p, double* out) {
   for (int i = 0; i < D; i++) { out[i] = dval(q[i]); out[D + i] = dval(p[i]); }
@@ -259,6 +316,41 @@ void symplec4(const ForceCtx<R, D>& F, const double* yo, int nt, double dt_in, c
     to += c4 * dt;
     for (int k = 0; k < D; k++) po[k] = p12[k];
     save_qp<D>(qo, po, result);
+    result += 2 * D;
+  }
+  if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
+}
+
+// symplec6, bovy_symplecticode.c:403-586: the symplec4 loop with seven
+// kicks (the last drift of a step and the first of the next merged as
+// (c8 + c1) dt).
+template <int D, class R>
+void symplec6(const ForceCtx<R, D>& F, const double* yo, int nt, double dt_in, const double* t,
+              double rtol, double atol, double* result, int* err, IntStats* st) {
+  R q[3], p[3], a[3];
+  for (int i = 0; i < D; i++) { q[i] = yo[i]; p[i] = yo[D + i]; }
+  save_qp<D>(q, p, result);
+  result += 2 * D;
+  *err = 0;
+  R init_dt = R(t[1]) - R(t[0]);
+  R dt = dt_in;
+  if (dt_in == kDtSentinel) dt = symplec6_estimate_step<D>(F, q, p, init_dt, R(t[0]), rtol, atol);
+  long ndt = (long)(dval(init_dt / dt));
+  R to = t[0];
+  for (int ii = 0; ii < nt - 1; ii++) {
+    leapq<D>(q, p, R(S6C[0]) * dt, q);
+    to += R(S6C[0]) * dt;
+    for (long jj = 0; jj < ndt; jj++) {
+      const bool last = jj == ndt - 1;
+      for (int k = 0; k < 7; k++) {
+        F.force(to, q, a);
+        leapp<D>(p, R(S6D[k]) * dt, a, p);
+        const R cn = (k == 6 && !last) ? R(S6C[7] + S6C[0]) : R(S6C[k + 1]);
+        leapq<D>(q, p, cn * dt, q);
+        to += cn * dt;
+      }
+    }
+    save_qp<D>(q, p, result);
     result += 2 * D;
   }
   if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
