@@ -22,6 +22,7 @@
 #include "batch_dev.h"
 #include "adjoint_dev.h"
 #include "dxdv_dev.h"
+#include "rk_dev.h"
 
 #define GH_EXPORT extern "C" __attribute__((visibility("default")))
 
@@ -449,6 +450,62 @@ gh_symplectic_kernel(int nobj, double* __restrict__ yo, int nt, const double* __
       to += c4 * dt;
     }
     GhPhase<V>::store(out + (long)N * (ii + 1), q, p);
+  }
+  err[orbit] = e;
+  if (stats) {
+    stats[orbit * 3 + 0] = (double)(ndt * (long)(nt - 1));
+    stats[orbit * 3 + 1] = 0.0;
+    stats[orbit * 3 + 2] = dt;
+  }
+}
+
+// bovy_rk4 / bovy_rk6 (bovy_rk.c:66-140, 171-268), one orbit per lane:
+// ndt = (long)(init_dt / dt) steps per output interval on the rectangular
+// state, rows stored as the symplectic kernel stores them.
+template <int ORDER, class Spec, class V>
+__global__ void __launch_bounds__(kBlock)
+gh_rk_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t, int indiv_t,
+             double dt_in, double rtol, double atol, const GhPotTable* __restrict__ T,
+             double* __restrict__ result, int* __restrict__ err, double* __restrict__ stats,
+             const volatile int* cancel) {
+  const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (orbit >= nobj) return;
+  constexpr int N = GhPhase<V>::N;
+  const GhForce<Spec> F{T};
+  const double* tt = t + (indiv_t ? orbit * nt : 0);
+  double yn[N], yn1[N];
+  {
+    double v[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = yo[orbit * N + k];
+    GhPhase<V>::to_rect(v, yn);
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      yo[orbit * N + k] = yn[k];  // galpy converts yo in place
+      yn1[k] = yn[k];
+    }
+  }
+  double* out = result + orbit * (long)nt * N;
+  GhPhase<V>::store(out, GhPhase<V>::q(yn), GhPhase<V>::p(yn));
+  int e = 0;
+  const double init_dt = tt[1] - tt[0];
+  double dt = dt_in;
+  if (dt_in == GH_DT_SENTINEL)
+    dt = gh_rk_estimate_step<ORDER, GhForce<Spec>, V>(F, yn, init_dt, tt[0], rtol, atol);
+  const long ndt = (long)(init_dt / dt);
+  double to = tt[0];
+  for (int ii = 0; ii < nt - 1; ii++) {
+    if (cancel != nullptr && (ii & 63) == 0 && *cancel) {
+      e = -10;
+      break;
+    }
+    for (long jj = 0; jj < ndt; jj++) {
+      gh_rk_onestep<ORDER, GhForce<Spec>, V>(F, yn, yn1, to, dt);
+      to += dt;
+#pragma unroll
+      for (int k = 0; k < N; k++) yn[k] = yn1[k];
+    }
+    GhPhase<V>::store(out + (long)N * (ii + 1), GhPhase<V>::q(yn1), GhPhase<V>::p(yn1));
   }
   err[orbit] = e;
   if (stats) {
@@ -1136,6 +1193,22 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                          nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
                          cancel);
       break;
+    case 1:  // RK4 and RK6: plain orbits
+    case 2:
+      if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
+        if (odeint_type == 1)
+          hipLaunchKernelGGL((gh_rk_kernel<4, Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
+                             d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
+                             cancel);
+        else
+          hipLaunchKernelGGL((gh_rk_kernel<6, Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
+                             d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
+                             cancel);
+        break;
+      } else {
+        set_error("rk4 and rk6 have no tangent-map (integrate_dxdv) stepper here");
+        return -1;
+      }
     case 4:  // symplec6: plain orbits (galpy has no symplec6 tangent map)
       if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
         hipLaunchKernelGGL((gh_symplectic_kernel<4, Spec, V>), grid, block, 0, stream, nobj,
@@ -1166,8 +1239,8 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
       }
     }
     default:
-      set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, "
-                "4 symplec6, 6 dop853)", odeint_type);
+      set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 1 rk4, 2 rk6, "
+                "3 symplec4, 4 symplec6, 6 dop853)", odeint_type);
       return -1;
   }
   GH_CHECK(hipGetLastError());
@@ -1261,9 +1334,9 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
     set_error("nt=%d: need at least two output times", nt);
     return -1;
   }
-  if (odeint_type != 0 && odeint_type != 3 && odeint_type != 4 && odeint_type != 6) {
-    set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 3 symplec4, "
-              "4 symplec6, 6 dop853)", odeint_type);
+  if (odeint_type < 0 || odeint_type > 6 || odeint_type == 5) {
+    set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 1 rk4, 2 rk6, "
+              "3 symplec4, 4 symplec6, 6 dop853)", odeint_type);
     return -1;
   }
   GhPotTable table;

@@ -29,6 +29,8 @@ METHOD_CODES = {
     "leapfrog_hip": 0,
     "symplec4_hip": 3,
     "symplec6_hip": 4,
+    "rk4_hip": 1,
+    "rk6_hip": 2,
     "dop853_hip": 6,
 }
 

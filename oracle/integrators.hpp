@@ -356,6 +356,134 @@ void symplec6(const ForceCtx<R, D>& F, const double* yo, int nt, double dt_in, c
   if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
 }
 
+// ---------------------------------------------------------------------------
+// Runge-Kutta steppers of bovy_rk.c on the rectangular state y = [q, p]
+// (2D values) with the derivative evalRectDeriv (integrateFullOrbit.c:1091-1105).
+// ---------------------------------------------------------------------------
+// bovy_rk4_onestep, bovy_rk.c:142-169: yn1 (equal to yn on entry) gains the
+// weighted slopes; ynk holds the stage states.
+template <int D, class R>
+void rk4_onestep(const ForceCtx<R, D>& F, const R* yn, R* yn1, R tn, R dt) {
+  constexpr int N = 2 * D;
+  R ynk[N], a[N];
+  F.deriv(tn, yn, a);
+  for (int i = 0; i < N; i++) yn1[i] += dt * a[i] / R(6.);
+  for (int i = 0; i < N; i++) ynk[i] = yn[i] + dt * a[i] / R(2.);
+  F.deriv(tn + dt / R(2.), ynk, a);
+  for (int i = 0; i < N; i++) yn1[i] += dt * a[i] / R(3.);
+  for (int i = 0; i < N; i++) ynk[i] = yn[i] + dt * a[i] / R(2.);
+  F.deriv(tn + dt / R(2.), ynk, a);
+  for (int i = 0; i < N; i++) yn1[i] += dt * a[i] / R(3.);
+  for (int i = 0; i < N; i++) ynk[i] = yn[i] + dt * a[i];
+  F.deriv(tn + dt, ynk, a);
+  for (int i = 0; i < N; i++) yn1[i] += dt * a[i] / R(6.);
+}
+
+// bovy_rk6_onestep, bovy_rk.c:270-326 (seven evaluations)
+template <int D, class R>
+void rk6_onestep(const ForceCtx<R, D>& F, const R* yn, R* yn1, R tn, R dt) {
+  constexpr int N = 2 * D;
+  R ynk[N], a[N], k1[N], k2[N], k3[N], k4[N], k5[N];
+  F.deriv(tn, yn, a);
+  for (int i = 0; i < N; i++) yn1[i] += R(11.) * dt * a[i] / R(120.);
+  for (int i = 0; i < N; i++) k1[i] = dt * a[i];
+  for (int i = 0; i < N; i++) ynk[i] = yn[i] + k1[i] / R(3.);
+  F.deriv(tn + dt / R(3.), ynk, a);
+  for (int i = 0; i < N; i++) k2[i] = dt * a[i];
+  for (int i = 0; i < N; i++) ynk[i] = yn[i] + R(2.) * k2[i] / R(3.);
+  F.deriv(tn + R(2.) * dt / R(3.), ynk, a);
+  for (int i = 0; i < N; i++) yn1[i] += R(81.) * dt * a[i] / R(120.);
+  for (int i = 0; i < N; i++) k3[i] = dt * a[i];
+  for (int i = 0; i < N; i++) ynk[i] = yn[i] + (k1[i] + R(4.) * k2[i] - k3[i]) / R(12.);
+  F.deriv(tn + dt / R(3.), ynk, a);
+  for (int i = 0; i < N; i++) yn1[i] += R(81.) * dt * a[i] / R(120.);
+  for (int i = 0; i < N; i++) k4[i] = dt * a[i];
+  for (int i = 0; i < N; i++)
+    ynk[i] = yn[i] + (-k1[i] + R(18.) * k2[i] - R(3.) * k3[i] - R(6.) * k4[i]) / R(16.);
+  F.deriv(tn + dt / R(2.), ynk, a);
+  for (int i = 0; i < N; i++) yn1[i] -= R(32.) * dt * a[i] / R(120.);
+  for (int i = 0; i < N; i++) k5[i] = dt * a[i];
+  for (int i = 0; i < N; i++)
+    ynk[i] = yn[i] + (R(9.) * k2[i] - R(3.) * k3[i] - R(6.) * k4[i] + R(4.) * k5[i]) / R(8.);
+  F.deriv(tn + dt / R(2.), ynk, a);
+  for (int i = 0; i < N; i++) yn1[i] -= R(32.) * dt * a[i] / R(120.);
+  for (int i = 0; i < N; i++) k5[i] = dt * a[i];  // k6 in k5's place, as the reference
+  for (int i = 0; i < N; i++)
+    ynk[i] = yn[i] + (R(9.) * k1[i] - R(36.) * k2[i] + R(63.) * k3[i] + R(72.) * k4[i] -
+                      R(64.) * k5[i]) / R(44.);
+  F.deriv(tn + dt, ynk, a);
+  for (int i = 0; i < N; i++) yn1[i] += R(11.) * dt * a[i] / R(120.);
+}
+
+// rk4_estimate_step / rk6_estimate_step, bovy_rk.c:328-500: one step of dt
+// against two of dt/2 in the scaled norm of the state's largest log|y|;
+// dt is divided by ceil(err^(1/5)) (RK4) or ceil(err^(1/7)) (RK6) until that
+// is 1 or the reduction would pass 10000.
+template <int ORDER, int D, class R>
+R rk_estimate_step(const ForceCtx<R, D>& F, const R* yo, R dt, R to, double rtol, double atol) {
+  constexpr int N = 2 * D;
+  R err = 2.;
+  R init_dt = dt;
+  R max_val = log(fabs(yo[0]));
+  for (int i = 1; i < N; i++)
+    if (log(fabs(yo[i])) > max_val) max_val = log(fabs(yo[i]));
+  R c = rmax(R(atol), R(rtol) + max_val);
+  R s = log(exp(R(atol) - c) + exp(R(rtol) + max_val - c)) + c;
+  const double expo = 1. / (ORDER == 4 ? 5. : 7.);
+  while (err > R(1.)) {
+    R y1[N], y21[N], y2[N];
+    for (int i = 0; i < N; i++) { y1[i] = yo[i]; y21[i] = yo[i]; }
+    if (ORDER == 4) {
+      rk4_onestep<D>(F, yo, y1, to, dt);
+      rk4_onestep<D>(F, yo, y21, to, dt / R(2.));
+      for (int i = 0; i < N; i++) y2[i] = y21[i];
+      rk4_onestep<D>(F, y21, y2, to + dt / R(2.), dt / R(2.));
+    } else {
+      rk6_onestep<D>(F, yo, y1, to, dt);
+      rk6_onestep<D>(F, yo, y21, to, dt / R(2.));
+      for (int i = 0; i < N; i++) y2[i] = y21[i];
+      rk6_onestep<D>(F, y21, y2, to + dt / R(2.), dt / R(2.));
+    }
+    err = 0.;
+    for (int i = 0; i < N; i++) err += exp(R(2.) * log(fabs(y1[i] - y2[i])) - R(2.) * s);
+    err = sqrt(err / R(N));
+    const double f = std::ceil(std::pow(dval(err), expo));
+    if (f > 1. && dval(init_dt / dt) * f < kMaxDtReduce)
+      dt /= R(f);
+    else
+      break;
+  }
+  return dt;
+}
+
+// bovy_rk4 / bovy_rk6, bovy_rk.c:66-140, 171-268: ndt = (long)(init_dt/dt)
+// steps per output interval.
+template <int ORDER, int D, class R>
+void rk_fixed(const ForceCtx<R, D>& F, const double* yo, int nt, double dt_in, const double* t,
+              double rtol, double atol, double* result, int* err, IntStats* st) {
+  constexpr int N = 2 * D;
+  R yn[N], yn1[N];
+  for (int i = 0; i < N; i++) { yn[i] = yo[i]; yn1[i] = yo[i]; result[i] = yo[i]; }
+  result += N;
+  *err = 0;
+  R init_dt = R(t[1]) - R(t[0]);
+  R dt = dt_in;
+  if (dt_in == kDtSentinel) dt = rk_estimate_step<ORDER, D>(F, yn, init_dt, R(t[0]), rtol, atol);
+  long ndt = (long)(dval(init_dt / dt));
+  R to = t[0];
+  for (int ii = 0; ii < nt - 1; ii++) {
+    for (long jj = 0; jj < ndt; jj++) {
+      if (ORDER == 4) rk4_onestep<D>(F, yn, yn1, to, dt);
+      else rk6_onestep<D>(F, yn, yn1, to, dt);
+      to += dt;
+      for (int k = 0; k < N; k++) yn[k] = yn1[k];
+    }
+    for (int k = 0; k < N; k++) result[k] = dval(yn1[k]);
+    result += N;
+  }
+  if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
+}
+
 // Variational (tangent-map) steppers, bovy_symplecticode.c:1053-1268:
 // the same DKD sequence on the augmented state [q | dq_1..dq_nde],
 // [p | dp_1..dp_nde]; the kick of each deviation column is h K dq_j

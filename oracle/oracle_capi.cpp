@@ -41,6 +41,8 @@ void integrate_one(int npot, const Component* P, int odeint_type, double* yo, in
   switch (odeint_type) {
     case 0: leapfrog<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
     case 3: symplec4<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
+    case 1: rk_fixed<4, D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
+    case 2: rk_fixed<6, D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
     case 4: symplec6<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
     case 6: dop853<D>(F, yo, nt, t, rtol, atol, result, err, &local); break;
     default: *err = -1; return;

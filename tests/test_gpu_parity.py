@@ -230,9 +230,12 @@ def test_unsupported_method_and_potential_raise():
     t = numpy.linspace(0, 1, 5)
     res = numpy.empty((4, 5, 6))
     err = numpy.zeros(4, dtype=numpy.int32)
-    lib.integrateFullOrbit(4, yo.copy(), 5, t, 0, len(pt), pt, pa, None, -9999.99, 0.0, 0.0, res, err, 1, None)
-    assert numpy.all(err == -1)
-    assert "odeint_type" in _lib.last_error()
+    for code in (5, 7, 8, -1):  # DOPR54 and IAS15 have no HIP integrator
+        err[:] = 0
+        lib.integrateFullOrbit(4, yo.copy(), 5, t, 0, len(pt), pt, pa, None, -9999.99, 0.0, 0.0, res, err,
+                               code, None)
+        assert numpy.all(err == -1)
+        assert "odeint_type" in _lib.last_error()
     bad = numpy.array([42], dtype=numpy.int32)
     err[:] = 0
     lib.integrateFullOrbit(4, yo.copy(), 5, t, 0, 1, bad, pa, None, -9999.99, 0.0, 0.0, res, err, 3, None)
