@@ -98,6 +98,14 @@ WORKLOADS = {
                            method="symplec4_hip", omega=0.0, seed=0, dt=0.01),
     "mpb_dop853_256k": dict(pot=mn_plummer_burkert, n=262144, t=(0.0, 500.0, 1001),
                            method="dop853_hip", omega=0.0, seed=0),
+    # the other fixed-step integrators of galpy at the force-kernel workload's
+    # size: symplec6 (7 force evaluations per step), RK4 (4) and RK6 (7)
+    "mw2014_symplec6_1m": dict(pot=mw2014, n=1048576, t=(0.0, 100.0, 101), method="symplec6_hip",
+                              omega=0.0, seed=0, dt=0.01),
+    "mw2014_rk4_1m": dict(pot=mw2014, n=1048576, t=(0.0, 100.0, 101), method="rk4_hip",
+                         omega=0.0, seed=0, dt=0.01),
+    "mw2014_rk6_1m": dict(pot=mw2014, n=1048576, t=(0.0, 100.0, 101), method="rk6_hip",
+                         omega=0.0, seed=0, dt=0.01),
     # a wrapped (rotating) component: DOP853, drift of the Jacobi integral
     "mw2014_rottriax_dop853_256k": dict(pot=mw2014_rotating_triaxial, n=262144, t=(0.0, 500.0, 1001),
                                        method="dop853_hip", omega=0.6, seed=0),
