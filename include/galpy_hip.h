@@ -27,7 +27,8 @@
  * npot counts the top-level components, as in galpy.
  * Supported odeint_type codes (integrateFullOrbit.c:1091-1132,
  * integratePlanarOrbit.c:788-825):
- *   0 leapfrog, 3 symplec4, 6 dop853.
+ *   0 leapfrog, 1 rk4, 2 rk6, 3 symplec4, 4 symplec6, 6 dop853
+ * (integrate_dxdv: 0, 3 and 6, as galpy's tangent-map steppers and DOP853).
  * Any other code, or nt < 2, sets err[i] = -1 for every orbit and records a
  * message readable with galpy_hip_last_error().
  */
