@@ -199,6 +199,31 @@ __device__ __forceinline__ void gh_comp_adj<GH_KUZMINDISK>(const GhPotTable* __r
   dth[0] += -(u + w * lam.z) * D32;
 }
 
+template <>
+__device__ __forceinline__ void gh_comp_adj<GH_FLATPOWER>(const GhPotTable* __restrict__ T, int i,
+                                                          GhVec3 x, double r2, double rinv,
+                                                          GhVec3 lam, GhVec3& a, GhVec3& k,
+                                                          double* dth) {
+  // a = g w, g = -amp m2^(-alpha/2 - 1), w = (x, y, z/q^2),
+  // m2 = R^2 + z^2/q^2 + core^2; K lam = g (lam.x, lam.y, lam.z/q^2)
+  // + 2 g' (w . lam) w with g' = dg/dm2 = -g (alpha/2 + 1) / m2 (the second
+  // derivatives of FlattenedPowerPotential.c:83-133)
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0], alpha = C->p[1], iq2 = C->c[0];
+  const double m2 = C->p[3] + x.x * x.x + x.y * x.y + x.z * x.z * iq2;
+  const double g = -amp * exp((-0.5 * alpha - 1.0) * log(m2));
+  const double gp = -g * (0.5 * alpha + 1.0) / m2;
+  const GhVec3 w{x.x, x.y, x.z * iq2};
+  a.x += g * w.x;
+  a.y += g * w.y;
+  a.z += g * w.z;
+  const double wl = w.x * lam.x + w.y * lam.y + w.z * lam.z;
+  k.x += g * lam.x + 2.0 * gp * wl * w.x;
+  k.y += g * lam.y + 2.0 * gp * wl * w.y;
+  k.z += g * lam.z * iq2 + 2.0 * gp * wl * w.z;
+  dth[0] += (g / amp) * wl;
+}
+
 __device__ __forceinline__ void gh_comp_adj_dyn(const GhPotTable* __restrict__ T, int i, GhVec3 x,
                                                 double r2, double rinv, GhVec3 lam, GhVec3& a,
                                                 GhVec3& k, double* dth) {
@@ -214,6 +239,7 @@ __device__ __forceinline__ void gh_comp_adj_dyn(const GhPotTable* __restrict__ T
     case GH_PLUMMER: gh_comp_adj<GH_PLUMMER>(T, i, x, r2, rinv, lam, a, k, dth); break;
     case GH_PSEUDOISO: gh_comp_adj<GH_PSEUDOISO>(T, i, x, r2, rinv, lam, a, k, dth); break;
     case GH_KUZMINDISK: gh_comp_adj<GH_KUZMINDISK>(T, i, x, r2, rinv, lam, a, k, dth); break;
+    case GH_FLATPOWER: gh_comp_adj<GH_FLATPOWER>(T, i, x, r2, rinv, lam, a, k, dth); break;
     case GH_BURKERT: gh_comp_adj<GH_BURKERT>(T, i, x, r2, rinv, lam, a, k, dth); break;
     case GH_DEHNENCORESPH: gh_comp_adj<GH_DEHNENCORESPH>(T, i, x, r2, rinv, lam, a, k, dth); break;
     case GH_DEHNENSPH: gh_comp_adj<GH_DEHNENSPH>(T, i, x, r2, rinv, lam, a, k, dth); break;

@@ -86,6 +86,8 @@ int nargs_of(int code, bool planar) {
     case GH_POWERSPH: case GH_JAFFE: case GH_ISOCHRONE: case GH_PLUMMER: case GH_PSEUDOISO:
     case GH_KUZMINDISK: case GH_BURKERT: case GH_DEHNENCORESPH: return 2;
     case GH_DEHNENSPH: case GH_HOMOGSPHERE: return 3;
+    case GH_FLATPOWER: return planar ? 3 : 4;  // planar: amp, alpha, core^2
+    case GH_NULLPOT: return 0;
     // wrappers' own arguments, read after the wrapped components
     // (integrateFullOrbit.c:821-895, integratePlanarOrbit.c:591-663)
     case GH_DEHNENSMOOTHWRAP: return 4;   // amp, tform, tsteady, grow
@@ -200,6 +202,10 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
     GhComponent& C = T->comp[i];
     C.type = code;
     for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
+    if (code == GH_FLATPOWER && planar) {  // full layout, q^2 = 1 (z = 0 in the plane)
+      C.p[3] = C.p[2];
+      C.p[2] = 1.0;
+    }
     switch (code) {
       case GH_POWERSPHCUTOFF: {
         const double amp = C.p[0], alpha = C.p[1], rc = C.p[2];
@@ -257,6 +263,7 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
         break;
       case GH_ISOCHRONE:
       case GH_PLUMMER: C.c[0] = C.p[1] * C.p[1]; break;  // b^2
+      case GH_FLATPOWER: C.c[0] = 1.0 / C.p[2]; break;   // 1/q^2
       case GH_PSEUDOISO:
       case GH_BURKERT: C.c[0] = 1.0 / C.p[1]; break;  // 1/a
       case GH_DEHNENSPH:
@@ -512,6 +519,85 @@ gh_rk_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict
     stats[orbit * 3 + 0] = (double)(ndt * (long)(nt - 1));
     stats[orbit * 3 + 1] = 0.0;
     stats[orbit * 3 + 2] = dt;
+  }
+}
+
+// bovy_dopr54 / bovy_dopr54_onestep (bovy_rk.c:501-616), one orbit per lane:
+// the first step from the RK4 estimator; each output interval (of the grid's
+// first spacing, as the reference) is stepped to its end with the last step
+// clipped onto it; a step reduced more than 10000-fold from the interval's
+// first one (or NaN) is forced through at that floor and makes err odd.
+// Every pass of the loop either advances t or shrinks the step towards that
+// floor, so the loop ends; it is also capped at kDopr54MaxIter passes per
+// output interval (err = -3 then), so that no input can keep a lane busy.
+constexpr long kDopr54MaxIter = 1L << 22;
+
+template <class Spec, class V>
+__global__ void __launch_bounds__(kBlock)
+gh_dopr54_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
+                 int indiv_t, double dt_in, double rtol, double atol,
+                 const GhPotTable* __restrict__ T, double* __restrict__ result,
+                 int* __restrict__ err, double* __restrict__ stats, const volatile int* cancel) {
+  const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (orbit >= nobj) return;
+  constexpr int N = GhPhase<V>::N;
+  const GhForce<Spec> F{T};
+  const double* tt = t + (indiv_t ? orbit * nt : 0);
+  double yn[N];
+  {
+    double v[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = yo[orbit * N + k];
+    GhPhase<V>::to_rect(v, yn);
+#pragma unroll
+    for (int k = 0; k < N; k++) yo[orbit * N + k] = yn[k];  // galpy converts yo in place
+  }
+  double* out = result + orbit * (long)nt * N;
+  GhPhase<V>::store(out, GhPhase<V>::q(yn), GhPhase<V>::p(yn));
+  const double dt = tt[1] - tt[0];
+  double dt_one = dt_in;
+  if (dt_in == GH_DT_SENTINEL)
+    dt_one = gh_rk_estimate_step<4, GhForce<Spec>, V>(F, yn, dt, tt[0], rtol, atol);
+  double to = tt[0];
+  double a1[N];
+  gh_rk_deriv<GhForce<Spec>, V>(F, to, yn, a1);
+  int e = 0;
+  long nacc = 0, nrej = 0;
+  for (int ii = 0; ii < nt - 1 && e != -3; ii++) {
+    if (cancel != nullptr && (ii & 63) == 0 && *cancel) {
+      e = -10;
+      break;
+    }
+    const double init_dt_one = dt_one, init_to = to;
+    long iter = 0;
+    while ((dt >= 0.0 && to < init_to + dt) || (dt < 0.0 && to > init_to + dt)) {
+      if (++iter > kDopr54MaxIter) {
+        e = -3;
+        break;
+      }
+      bool accept = false;
+      if (init_dt_one / dt_one > 10000.0 || dt_one != dt_one) {
+        dt_one = init_dt_one / 10000.0;
+        accept = true;
+        if (e % 2 == 0) e += 1;
+      }
+      if (dt >= 0.0 && dt_one > init_to + dt - to) dt_one = init_to + dt - to;
+      if (dt < 0.0 && dt_one < init_to + dt - to) dt_one = init_to + dt - to;
+      bool accepted;
+      dt_one = gh_dopr54_actualstep<GhForce<Spec>, V>(F, yn, dt_one, to, rtol, atol, a1, accept,
+                                                       accepted);
+      if (accepted)
+        nacc++;
+      else
+        nrej++;
+    }
+    GhPhase<V>::store(out + (long)N * (ii + 1), GhPhase<V>::q(yn), GhPhase<V>::p(yn));
+  }
+  err[orbit] = e;
+  if (stats) {
+    stats[orbit * 3 + 0] = (double)nacc;
+    stats[orbit * 3 + 1] = (double)nrej;
+    stats[orbit * 3 + 2] = dt_one;
   }
 }
 
@@ -1209,6 +1295,16 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
         set_error("rk4 and rk6 have no tangent-map (integrate_dxdv) stepper here");
         return -1;
       }
+    case 5:  // DOPR54: plain orbits
+      if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
+        hipLaunchKernelGGL((gh_dopr54_kernel<Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
+                           d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
+                           cancel);
+        break;
+      } else {
+        set_error("dopr54 has no tangent-map (integrate_dxdv) stepper here");
+        return -1;
+      }
     case 4:  // symplec6: plain orbits (galpy has no symplec6 tangent map)
       if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
         hipLaunchKernelGGL((gh_symplectic_kernel<4, Spec, V>), grid, block, 0, stream, nobj,
@@ -1240,7 +1336,7 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
     }
     default:
       set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 1 rk4, 2 rk6, "
-                "3 symplec4, 4 symplec6, 6 dop853)", odeint_type);
+                "3 symplec4, 4 symplec6, 5 dopr54, 6 dop853)", odeint_type);
       return -1;
   }
   GH_CHECK(hipGetLastError());
@@ -1334,9 +1430,9 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
     set_error("nt=%d: need at least two output times", nt);
     return -1;
   }
-  if (odeint_type < 0 || odeint_type > 6 || odeint_type == 5) {
+  if (odeint_type < 0 || odeint_type > 6) {
     set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 1 rk4, 2 rk6, "
-              "3 symplec4, 4 symplec6, 6 dop853)", odeint_type);
+              "3 symplec4, 4 symplec6, 5 dopr54, 6 dop853)", odeint_type);
     return -1;
   }
   GhPotTable table;

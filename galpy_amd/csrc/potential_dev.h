@@ -44,6 +44,8 @@ enum GhPotCode {
   GH_DEHNENCORESPH = 33,
   GH_DEHNENSPH = 34,
   GH_HOMOGSPHERE = 35,
+  GH_FLATPOWER = 12,
+  GH_NULLPOT = 40,
   // wrappers (integrateFullOrbit.c:821-895)
   GH_DEHNENSMOOTHWRAP = -1,
   GH_SOLIDBODYROTWRAP = -2,
@@ -523,6 +525,24 @@ __device__ __forceinline__ void gh_comp_force<GH_KUZMINDISK>(const GhPotTable* _
   F.Fz += -amp * w * D32;
 }
 
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_FLATPOWER>(const GhPotTable* __restrict__ T,
+                                                            int i, const GhGeom& g, GhAcc& F) {
+  // FlattenedPowerPotential.c:19-64: F = -amp m^(-alpha/2 - 1) (x, y, z/q^2),
+  // m^2 = R^2 + z^2/q^2 + core^2 (the alpha = 0 branch is the same
+  // expression, -amp / m^2); c[0] = 1/q^2
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0], alpha = C->p[1], iq2 = C->c[0];
+  const double m2 = C->p[3] + g.R2 + g.z2 * iq2;
+  const double f = (alpha == 0.0) ? -amp * gh_rcp(m2) : -amp * exp((-0.5 * alpha - 1.0) * log(m2));
+  F.FRoR += f;
+  F.Fz += f * g.z * iq2;
+}
+
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_NULLPOT>(const GhPotTable* __restrict__, int,
+                                                          const GhGeom&, GhAcc&) {}
+
 // run-time dispatch on the component type (generic kernels)
 __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__ T, int i,
                                                   const GhGeom& g, GhAcc& F) {
@@ -539,6 +559,7 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
     case GH_PLUMMER: gh_comp_force<GH_PLUMMER>(T, i, g, F); break;
     case GH_PSEUDOISO: gh_comp_force<GH_PSEUDOISO>(T, i, g, F); break;
     case GH_KUZMINDISK: gh_comp_force<GH_KUZMINDISK>(T, i, g, F); break;
+    case GH_FLATPOWER: gh_comp_force<GH_FLATPOWER>(T, i, g, F); break;
     case GH_BURKERT: gh_comp_force<GH_BURKERT>(T, i, g, F); break;
     case GH_DEHNENCORESPH: gh_comp_force<GH_DEHNENCORESPH>(T, i, g, F); break;
     case GH_DEHNENSPH: gh_comp_force<GH_DEHNENSPH>(T, i, g, F); break;
@@ -792,6 +813,12 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
         const double zq2 = z2 * C->c[0];
         const double Rt2 = (onem1overb2 < 1.0) ? R2 * (1.0 - onem1overb2 * sinphi * sinphi) : R2;
         pot += 0.5 * amp * log(Rt2 + zq2 + core2);
+        break;
+      }
+      case GH_FLATPOWER: {  // FlattenedPowerPotential.c:3-18
+        const double alpha = C->p[1], m2 = C->p[3] + R2 + z2 * C->c[0];
+        pot += (alpha == 0.0) ? 0.5 * C->p[0] * log(m2)
+                              : -C->p[0] * pow(m2, -0.5 * alpha) / alpha;
         break;
       }
       case GH_KUZMINDISK: {  // KuzminDiskPotential.c:5-15

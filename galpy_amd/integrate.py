@@ -31,6 +31,7 @@ METHOD_CODES = {
     "symplec6_hip": 4,
     "rk4_hip": 1,
     "rk6_hip": 2,
+    "dopr54_hip": 5,
     "dop853_hip": 6,
 }
 

@@ -612,6 +612,62 @@ class HomogeneousSpherePotential(_SphericalPotential):
         return [self._amp, self._R2, self._R3]
 
 
+class FlattenedPowerPotential(Potential):
+    """Phi = -amp / (alpha m^alpha), m^2 = R^2 + z^2/q^2 + core^2 (amp/2 log m^2
+    at alpha = 0); pot_type 12, args [amp, alpha, q^2, core^2], planar
+    [amp, alpha, core^2] (FlattenedPowerPotential.py:30-80,
+    integratePlanarOrbit.py:183-186)."""
+
+    pot_code = 12
+
+    def __init__(self, amp=1.0, alpha=0.5, q=0.9, core=_CORE, normalize=False, r1=1.0):
+        Potential.__init__(self, amp=amp)
+        self.alpha = alpha
+        self.q2 = q**2.0
+        self.core2 = core**2.0
+        self._amp *= r1**self.alpha
+        _set_normalize(self, normalize)
+
+    def _m2(self, R, z):
+        return self.core2 + R**2.0 + z**2.0 / self.q2
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        if self.alpha == 0.0:
+            return 0.5 * numpy.log(self._m2(R, z))
+        return -self._m2(R, z) ** (-0.5 * self.alpha) / self.alpha
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return -self._m2(R, z) ** (-0.5 * self.alpha - 1.0) * R
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return -self._m2(R, z) ** (-0.5 * self.alpha - 1.0) * z / self.q2
+
+    def c_args(self):
+        return [self._amp, self.alpha, self.q2, self.core2]
+
+    def c_args_planar(self):
+        return [self._amp, self.alpha, self.core2]
+
+
+class NullPotential(Potential):
+    """No force; pot_type 40 without arguments.  The packing drops it unless
+    every component is one (integrateFullOrbit.py:51-56)."""
+
+    pot_code = 40
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * numpy.asarray(R, dtype=float)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * numpy.asarray(R, dtype=float)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * numpy.asarray(R, dtype=float)
+
+    def c_args(self):
+        return []
+
+
 class WrapperPotential(Potential):
     """A potential that modifies the potential it wraps (galpy's
     WrapperPotential.py:91-150): the wrapped components are evaluated at
@@ -791,6 +847,13 @@ def _pack(comps, planar):
     return pot_type, pot_args
 
 
+def _purge_null(comps):
+    """integrateFullOrbit.py:51-56: NullPotentials are dropped unless nothing
+    else is left."""
+    kept = [p for p in comps if not isinstance(p, NullPotential)]
+    return kept if kept else comps
+
+
 def parse_pot(pot):
     """galpy's ``_parse_pot`` (integrateFullOrbit.py:43-614) for the supported
     potentials: returns ``(npot, pot_type int32[], pot_args float64[], [])``;
@@ -798,7 +861,7 @@ def parse_pot(pot):
 
     Raises NotImplementedError for a potential without a GPU kernel, as galpy
     does for a potential without a C implementation (:604-611)."""
-    comps = flatten(pot)
+    comps = _purge_null(flatten(pot))
     pot_type, pot_args = _pack(comps, False)
     return (
         len(comps),
@@ -812,7 +875,7 @@ def parse_pot_planar(pot):
     """galpy's planar ``_parse_pot`` (integratePlanarOrbit.py:33-663) for the
     supported potentials, taken in the plane z = 0: returns
     ``(npot, pot_type int32[], pot_args float64[], [])``."""
-    comps = flatten(pot)
+    comps = _purge_null(flatten(pot))
     pot_type, pot_args = _pack(comps, True)
     return (
         len(comps),

@@ -106,6 +106,10 @@ WORKLOADS = {
                          omega=0.0, seed=0, dt=0.01),
     "mw2014_rk6_1m": dict(pot=mw2014, n=1048576, t=(0.0, 100.0, 101), method="rk6_hip",
                          omega=0.0, seed=0, dt=0.01),
+    # DOPR54 (galpy's 5(4) Dormand-Prince) at galpy's default tolerances, the
+    # DOP853 workload's grid and potential at a quarter of the orbits
+    "mw2014_dopr54_256k": dict(pot=mw2014, n=262144, t=(0.0, 500.0, 1001), method="dopr54_hip",
+                              omega=0.0, seed=0),
     # a wrapped (rotating) component: DOP853, drift of the Jacobi integral
     "mw2014_rottriax_dop853_256k": dict(pot=mw2014_rotating_triaxial, n=262144, t=(0.0, 500.0, 1001),
                                        method="dop853_hip", omega=0.6, seed=0),

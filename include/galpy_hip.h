@@ -17,7 +17,7 @@
  *   15 PowerSphericalPotentialwCutoff, 17 PlummerPotential,
  *   18 PseudoIsothermalPotential, 19 KuzminDiskPotential, 20 BurkertPotential,
  *   33 DehnenCoreSphericalPotential, 34 DehnenSphericalPotential,
- *   35 HomogeneousSpherePotential;
+ *   35 HomogeneousSpherePotential, 12 FlattenedPowerPotential, 40 NullPotential;
  * and the wrappers (integrateFullOrbit.c:821-895, packed as galpy packs them:
  * the wrapper's code, then the wrapped components' codes; as arguments
  * nwrapped, the wrapped components' arguments, then the wrapper's own)
@@ -27,7 +27,7 @@
  * npot counts the top-level components, as in galpy.
  * Supported odeint_type codes (integrateFullOrbit.c:1091-1132,
  * integratePlanarOrbit.c:788-825):
- *   0 leapfrog, 1 rk4, 2 rk6, 3 symplec4, 4 symplec6, 6 dop853
+ *   0 leapfrog, 1 rk4, 2 rk6, 3 symplec4, 4 symplec6, 5 dopr54, 6 dop853
  * (integrate_dxdv: 0, 3 and 6, as galpy's tangent-map steppers and DOP853).
  * Any other code, or nt < 2, sets err[i] = -1 for every orbit and records a
  * message readable with galpy_hip_last_error().
@@ -49,7 +49,11 @@ extern "C" {
  *   dt              fixed step, or -9999.99 to estimate one per orbit
  *   rtol, atol      natural logarithms of the tolerances
  *   result[nobj*nt*6] output rows (R,vR,vT,z,vz,phi); row 0 is the IC
- *   err[nobj]       0 ok, -10 interrupted by SIGINT, -1 unsupported input
+ *   err[nobj]       0 ok, -10 interrupted by SIGINT, -1 unsupported input;
+ *                   dopr54 as bovy_dopr54 (bovy_rk.c:501-616): odd when a
+ *                   step was forced through at 1/10000 of the interval's first
+ *                   step, -3 when one output interval took more than 2^22
+ *                   step attempts (a bound the reference does not have)
  *   pot_tfuncs      must be NULL (no supported potential uses time functions)
  *   cb              called once per finished orbit (may be NULL)
  * All pointers are host memory.  Work is streamed through the GPU in chunks

@@ -484,6 +484,135 @@ void rk_fixed(const ForceCtx<R, D>& F, const double* yo, int nt, double dt_in, c
   if (st) { st->steps = (long long)ndt * (nt - 1); st->dt = dval(dt); }
 }
 
+// bovy_dopr54_actualstep, bovy_rk.c:617-760: one Dormand-Prince 5(4) step
+// of dt from (to, yo) with the first-same-as-last slope a1; the correction
+// 0.85 err^-0.2 rounded to a power of two in [2^-3, 2^3]; accepted (yo, a1,
+// to advanced) if that power is >= 0 or `accept` is forced.  Returns the next
+// step, dt 2^power.
+template <int D, class R>
+R dopr54_actualstep(const ForceCtx<R, D>& F, R* yo, R dt, R& to, double rtol, double atol, R* a1,
+                    bool accept, bool& accepted) {
+  constexpr int N = 2 * D;
+  const double c2 = 0.2, c3 = 0.3, c4 = 0.8, c5 = 8. / 9.;
+  const double a21 = 0.2, a31 = 3. / 40., a41 = 44. / 45., a51 = 19372. / 6561, a61 = 9017. / 3168.,
+               a71 = 35. / 384., a32 = 9. / 40., a42 = -56. / 15., a52 = -25360. / 2187.,
+               a62 = -355. / 33., a43 = 32. / 9., a53 = 64448. / 6561., a63 = 46732. / 5247.,
+               a73 = 500. / 1113., a54 = -212. / 729., a64 = 49. / 176., a74 = 125. / 192.,
+               a65 = -5103. / 18656., a75 = -2187. / 6784., a76 = 11. / 84.;
+  const double b1 = 35. / 384., b3 = 500. / 1113., b4 = 125. / 192., b5 = -2187. / 6784.,
+               b6 = 11. / 84.;
+  const double be1 = b1 - 5179. / 57600., be3 = b3 - 7571. / 16695., be4 = b4 - 393. / 640.,
+               be5 = b5 + 92097. / 339200., be6 = b6 - 187. / 2100., be7 = -1. / 40.;
+  R yn1[N], yerr[N], ynk[N], a[N], k1[N], k2[N], k3[N], k4[N], k5[N], k6[N];
+  for (int i = 0; i < N; i++) yn1[i] = yo[i];
+  for (int i = 0; i < N; i++) a[i] = a1[i];
+  for (int i = 0; i < N; i++) {
+    k1[i] = dt * a[i];
+    yn1[i] += R(b1) * k1[i];
+    yerr[i] = R(be1) * k1[i];
+    ynk[i] = yo[i] + R(a21) * k1[i];
+  }
+  F.deriv(to + R(c2) * dt, ynk, a);
+  for (int i = 0; i < N; i++) {
+    k2[i] = dt * a[i];
+    ynk[i] = yo[i] + R(a31) * k1[i] + R(a32) * k2[i];
+  }
+  F.deriv(to + R(c3) * dt, ynk, a);
+  for (int i = 0; i < N; i++) {
+    k3[i] = dt * a[i];
+    yn1[i] += R(b3) * k3[i];
+    yerr[i] += R(be3) * k3[i];
+    ynk[i] = yo[i] + R(a41) * k1[i] + R(a42) * k2[i] + R(a43) * k3[i];
+  }
+  F.deriv(to + R(c4) * dt, ynk, a);
+  for (int i = 0; i < N; i++) {
+    k4[i] = dt * a[i];
+    yn1[i] += R(b4) * k4[i];
+    yerr[i] += R(be4) * k4[i];
+    ynk[i] = yo[i] + R(a51) * k1[i] + R(a52) * k2[i] + R(a53) * k3[i] + R(a54) * k4[i];
+  }
+  F.deriv(to + R(c5) * dt, ynk, a);
+  for (int i = 0; i < N; i++) {
+    k5[i] = dt * a[i];
+    yn1[i] += R(b5) * k5[i];
+    yerr[i] += R(be5) * k5[i];
+    ynk[i] = yo[i] + R(a61) * k1[i] + R(a62) * k2[i] + R(a63) * k3[i] + R(a64) * k4[i] +
+             R(a65) * k5[i];
+  }
+  F.deriv(to + dt, ynk, a);
+  for (int i = 0; i < N; i++) {
+    k6[i] = dt * a[i];
+    yn1[i] += R(b6) * k6[i];
+    yerr[i] += R(be6) * k6[i];
+    ynk[i] = yo[i] + R(a71) * k1[i] + R(a73) * k3[i] + R(a74) * k4[i] + R(a75) * k5[i] +
+             R(a76) * k6[i];
+  }
+  F.deriv(to + dt, ynk, a);
+  for (int i = 0; i < N; i++) yerr[i] += R(be7) * dt * a[i];
+  R max_val = log(fabs(yo[0]));
+  for (int i = 1; i < N; i++)
+    if (log(fabs(yo[i])) > max_val) max_val = log(fabs(yo[i]));
+  R c = rmax(R(atol), R(rtol) + max_val);
+  R s = log(exp(R(atol) - c) + exp(R(rtol) + max_val - c)) + c;
+  R err = 0.;
+  for (int i = 0; i < N; i++) err += exp(R(2.) * log(fabs(yerr[i])) - R(2.) * s);
+  err = sqrt(err / R(N));
+  const double corr = 0.85 * std::pow(dval(err), -.2);
+  double powertwo = std::round(std::log(corr) / std::log(2.));
+  if (powertwo > 3.) powertwo = 3.;
+  else if (powertwo < -3.) powertwo = -3.;
+  accepted = (powertwo >= 0.) || accept;
+  if (accepted) {
+    for (int i = 0; i < N; i++) {
+      a1[i] = a[i];
+      yo[i] = yn1[i];
+    }
+    to += dt;
+  }
+  return dt * R(std::pow(2., powertwo));
+}
+
+// bovy_dopr54 / bovy_dopr54_onestep, bovy_rk.c:501-616: the first step from
+// the RK4 estimator; each output interval of the grid's first spacing is
+// stepped to its end (the last step clipped to land on it); a step reduced
+// more than 10000-fold from the interval's first one (or NaN) is forced
+// through at that floor and flags err as odd.
+template <int D, class R>
+void dopr54(const ForceCtx<R, D>& F, const double* yo_in, int nt, double dt_one_in,
+            const double* t, double rtol, double atol, double* result, int* err, IntStats* st) {
+  constexpr int N = 2 * D;
+  R yn[N], a1[N];
+  for (int i = 0; i < N; i++) { yn[i] = yo_in[i]; result[i] = yo_in[i]; }
+  result += N;
+  *err = 0;
+  const R dt = R(t[1]) - R(t[0]);
+  R dt_one = dt_one_in;
+  if (dt_one_in == kDtSentinel) dt_one = rk_estimate_step<4, D>(F, yn, dt, R(t[0]), rtol, atol);
+  R to = t[0];
+  F.deriv(to, yn, a1);
+  long long nacc = 0, nrej = 0;
+  for (int ii = 0; ii < nt - 1; ii++) {
+    const R init_dt_one = dt_one, init_to = to;
+    while ((dt >= R(0.) && to < init_to + dt) || (dt < R(0.) && to > init_to + dt)) {
+      bool accept = false;
+      if (init_dt_one / dt_one > R(10000.) || dval(dt_one) != dval(dt_one)) {
+        dt_one = init_dt_one / R(10000.);
+        accept = true;
+        if (*err % 2 == 0) *err += 1;
+      }
+      if (dt >= R(0.) && dt_one > init_to + dt - to) dt_one = init_to + dt - to;
+      if (dt < R(0.) && dt_one < init_to + dt - to) dt_one = init_to + dt - to;
+      bool accepted;
+      dt_one = dopr54_actualstep<D>(F, yn, dt_one, to, rtol, atol, a1, accept, accepted);
+      if (accepted) nacc++;
+      else nrej++;
+    }
+    for (int k = 0; k < N; k++) result[k] = dval(yn[k]);
+    result += N;
+  }
+  if (st) { st->steps = nacc; st->rejected = nrej; st->dt = dval(dt_one); }
+}
+
 // Variational (tangent-map) steppers, bovy_symplecticode.c:1053-1268:
 // the same DKD sequence on the augmented state [q | dq_1..dq_nde],
 // [p | dp_1..dp_nde]; the kick of each deviation column is h K dq_j

@@ -151,7 +151,7 @@ template <class R> R upper_incgamma(double a, R x, double gamma_a) {
 enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, kHernquist = 8,
                kNFW = 9, kJaffe = 10, kIsochrone = 14, kPowerSphCutoff = 15, kPlummer = 17,
                kPseudoIso = 18, kKuzminDisk = 19, kBurkert = 20, kDehnenCoreSph = 33,
-               kDehnenSph = 34, kHomogSphere = 35,
+               kDehnenSph = 34, kHomogSphere = 35, kFlattenedPower = 12, kNull = 40,
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5 };
@@ -188,6 +188,8 @@ inline int component_nargs(int code, bool planar = false) {
     case kPowerSphCutoff: return planar ? 3 : 5;
     case kPowerSph: case kJaffe: case kIsochrone: case kPlummer: case kPseudoIso:
     case kKuzminDisk: case kBurkert: case kDehnenCoreSph: return 2;
+    case kFlattenedPower: return planar ? 3 : 4;  // planar: amp, alpha, core^2
+    case kNull: return 0;
     case kDehnenSph: case kHomogSphere: return 3;
     // wrappers' own arguments, read after the wrapped components
     case kDehnenSmoothWrap: return 4;   // amp, tform, tsteady, grow
@@ -226,6 +228,13 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       C.wrapped = kids;
     }
     for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
+    if (code == kFlattenedPower && planar) {
+      // integratePlanarOrbit.py:183-186 packs (amp, alpha, core^2); held in
+      // the full layout (q^2 = 1): at z = 0 the full functions are the planar
+      // ones (FlattenedPowerPotential.c PlanarRforce, PlanarR2deriv)
+      C.p[3] = C.p[2];
+      C.p[2] = 1.;
+    }
     if (code == kPowerSphCutoff) {
       double alpha = C.p[1], rc = C.p[2];
       C.c[0] = 1.5 - 0.5 * alpha;                  // a for the force
@@ -602,6 +611,19 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kLogHalo: lh_forces(C, Rc, z, phi, fr, fz, fp); break;
       case kDehnenBar: bar_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kKuzminDisk: kuzmin_forces(C, Rc, z, fr, fz); break;
+      case kFlattenedPower: {  // FlattenedPowerPotential.c Rforce, zforce
+        double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
+        if (alpha == 0.) {
+          fr = -R(amp) * Rc / (Rc * Rc + z * z / R(q2) + R(core2));
+          fz = -R(amp) * z / R(q2) / (Rc * Rc + z * z / R(q2) + R(core2));
+        } else {
+          R m2 = R(core2) + Rc * Rc + z * z / R(q2);
+          fr = -R(amp) * pow(m2, R(-0.5 * alpha - 1.)) * Rc;
+          fz = -R(amp) * pow(m2, R(-0.5 * alpha - 1.)) * z / R(q2);
+        }
+        break;
+      }
+      case kNull: break;
       default:
         if (is_spherical_new(C.type)) {
           sph_forces(C, Rc, z, fr, fz);
@@ -687,6 +709,23 @@ bool cyl_hessian(int npot, const Component* P, R Rc, R z, R& R2, R& z2, R& Rz) {
         Rz += Rc * z * (d2phi / r2 - dphi / (r2 * r));
         break;
       }
+      case kFlattenedPower: {  // FlattenedPowerPotential.c R2deriv, z2deriv, Rzderiv
+        double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
+        if (alpha == 0.) {
+          R denom = R(1.) / (Rc * Rc + z * z / R(q2) + R(core2));
+          R2 += R(amp) * (denom - R(2.) * Rc * Rc * denom * denom);
+          z2 += R(amp) / R(q2) * (denom - R(2.) * z * z * denom * denom / R(q2));
+          Rz += R(-2.) * R(amp) * Rc * z / R(q2) * denom * denom;
+        } else {
+          R m2 = R(core2) + Rc * Rc + z * z / R(q2);
+          R2 += -R(amp) * pow(m2, R(-0.5 * alpha - 1.)) * (R(alpha + 2.) * Rc * Rc / m2 - R(1.));
+          z2 += -R(amp) / R(q2) * pow(m2, R(-0.5 * alpha - 1.)) *
+                (R(alpha + 2.) * z * z / m2 / R(q2) - R(1.));
+          Rz += -R(amp) * R(alpha + 2.) * Rc * z / R(q2) * pow(m2, R(-0.5 * alpha - 2.));
+        }
+        break;
+      }
+      case kNull: break;
       case kKuzminDisk: {  // KuzminDiskPotential.c R2deriv, z2deriv, Rzderiv
         double amp = C.p[0], a = C.p[1];
         R az = R(a) + fabs(z);
@@ -825,6 +864,15 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
       case kHernquist: fr = hq_planar_Rforce(C, Rc); break;
       case kLogHalo: lh_planar_forces(C, Rc, phi, fr, fp); break;
       case kDehnenBar: bar_planar_forces(C, Rc, phi, t, fr, fp); break;
+      case kFlattenedPower: {  // FlattenedPowerPotential.c PlanarRforce
+        double amp = C.p[0], alpha = C.p[1], core2 = C.p[3];
+        if (alpha == 0.)
+          fr = -R(amp) * Rc / (Rc * Rc + R(core2));
+        else
+          fr = -R(amp) * pow(R(core2) + Rc * Rc, R(-0.5 * alpha - 1.)) * Rc;
+        break;
+      }
+      case kNull: break;
       case kKuzminDisk:  // KuzminDiskPotential.c planarRforce
         fr = -R(C.p[0]) * Rc * pow(Rc * Rc + R(C.p[1] * C.p[1]), R(-1.5));
         break;
@@ -963,6 +1011,15 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kLogHalo: pot += lh_eval(C, Rc, z, phi); break;
       case kDehnenBar: pot += bar_eval(C, Rc, z, phi, t); break;
       case kKuzminDisk: pot += kuzmin_eval(C, Rc, z); break;
+      case kFlattenedPower: {  // FlattenedPowerPotential.c Eval
+        double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
+        if (alpha == 0.)
+          pot += R(0.5) * R(amp) * log(Rc * Rc + z * z / R(q2) + R(core2));
+        else
+          pot += -R(amp) * pow(R(core2) + Rc * Rc + z * z / R(q2), R(-0.5 * alpha)) / R(alpha);
+        break;
+      }
+      case kNull: break;
       default:
         if (is_spherical_new(C.type)) {
           pot += sph_eval(C, sqrt(Rc * Rc + z * z));

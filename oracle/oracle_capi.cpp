@@ -44,6 +44,7 @@ void integrate_one(int npot, const Component* P, int odeint_type, double* yo, in
     case 1: rk_fixed<4, D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
     case 2: rk_fixed<6, D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
     case 4: symplec6<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
+    case 5: dopr54<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
     case 6: dop853<D>(F, yo, nt, t, rtol, atol, result, err, &local); break;
     default: *err = -1; return;
   }

@@ -125,6 +125,13 @@ def dehnen_phi(R, z, amp, a, alpha):
     return -amp * (1 - (r / (r + a)) ** (2 - alpha)) / (a * (2 - alpha) * (3 - alpha))
 
 
+def flatpower_phi(R, z, amp, alpha, q2, core2):
+    m2 = core2 + R * R + z * z / q2
+    if alpha == 0:
+        return amp * mp.log(m2) / 2
+    return -amp * m2 ** (-alpha / 2) / alpha
+
+
 def homog_phi(R, z, amp, R2, R3):
     r2 = R * R + z * z
     if r2 < R2:
@@ -165,6 +172,14 @@ def four_pi_rho(name, R, z, args):
         return A * a * r ** (-alpha) * (r + a) ** (alpha - 4)
     if name == "homogsphere":
         return 6 * A if r * r < mp.mpf(args[1]) else mp.mpf(0)
+    if name.startswith("flatpower"):
+        # the Laplacian of f(m^2), m^2 = R^2 + z^2/q^2 + core^2, derived by hand:
+        # amp m^(-alpha-4) / q^2 [R^2 (1 - alpha q^2) + z^2 (2 - (1+alpha)/q^2)
+        # + core^2 (2 q^2 + 1)]
+        alpha, q2, c2 = (mp.mpf(v) for v in args[1:4])
+        m2 = c2 + R * R + z * z / q2
+        return A / q2 * (R * R * (1 - alpha * q2) + z * z * (2 - (1 + alpha) / q2) + c2 * (2 * q2 + 1)) * \
+            m2 ** (-alpha / 2 - 2)
     raise KeyError(name)
 
 
@@ -214,6 +229,8 @@ def potentials():
         "dehnen": (34, ["1.3", "0.9", "1.5"], dehnen_phi),
         "dehnen_a05": (34, ["0.7", "1.4", "0.5"], dehnen_phi),
         "homogsphere": (35, ["1.0", "1.21", "1.331"], homog_phi),
+        "flatpower": (12, ["1.1", "0.5", "0.81", "0.01"], flatpower_phi),
+        "flatpower_a0": (12, ["0.9", "0", "0.64", "0.04"], flatpower_phi),
     }
     for name, (code, sargs, phi_fn) in new.items():
         margs = [mp.mpf(float(v)) for v in sargs]  # the packed doubles exactly

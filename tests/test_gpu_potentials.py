@@ -66,6 +66,8 @@ def _single_types():
         "DehnenSpherical": [P.DehnenSphericalPotential(a=0.9, alpha=1.5, normalize=1.0)],
         "DehnenSpherical alpha=0.5": [P.DehnenSphericalPotential(a=1.4, alpha=0.5, normalize=1.0)],
         "HomogeneousSphere": [P.HomogeneousSpherePotential(R=1.5, normalize=1.0)],
+        "FlattenedPower": [P.FlattenedPowerPotential(alpha=0.5, q=0.9, core=0.1, normalize=1.0)],
+        "FlattenedPower alpha=0": [P.FlattenedPowerPotential(alpha=0.0, q=0.8, core=0.2, normalize=1.0)],
     }
 
 
@@ -90,7 +92,7 @@ def test_device_forces_and_potential_match_oracle_and_mpmath():
 
     golden = json.load(open(os.path.join(GOLDEN, "potentials.json")))
     names = ["powersph", "powersph_a2", "kepler", "jaffe", "isochrone", "plummer", "pseudoiso", "kuzmin",
-             "burkert", "dehnencore", "dehnen", "dehnen_a05", "homogsphere"]
+             "burkert", "dehnencore", "dehnen", "dehnen_a05", "homogsphere", "flatpower", "flatpower_a0"]
     for name in names:
         case = golden[name]
         pts = case["points"]
@@ -306,3 +308,29 @@ def test_adjoint_amplitude_gradients(method, code):
             e[:, j] = h
             fd = ((per_orbit(w0t + e) - per_orbit(w0t - e)) / (2 * h)).cpu().numpy()
             assert numpy.abs(g[keep, j] - fd[keep]).max() <= 1e-6 * max(1e-3, numpy.abs(fd[keep]).max()), j
+
+
+def test_flattened_power_planar_and_null_potential():
+    """FlattenedPowerPotential's planar packing (amp, alpha, core^2) held in
+    the full layout at q^2 = 1; a NullPotential leaves orbits on straight
+    lines."""
+    from galpy_amd import potential as P
+    from galpy_amd.integrate import integrateFullOrbit_hip, integratePlanarOrbit_hip
+
+    t = numpy.linspace(0.0, 10.0, 101)
+    for pot in ([P.FlattenedPowerPotential(alpha=0.5, q=0.9, core=0.1, normalize=1.0)],
+                [P.FlattenedPowerPotential(alpha=0.0, q=0.8, core=0.2, normalize=1.0)]):
+        yo = planar_ics(128, seed=21)
+        pt, pa = _packed(pot, True)
+        assert len(pa) == 3
+        for method, code, tol in (("symplec4_hip", 3, 1e-10), ("dop853_hip", 6, 1e-8)):
+            res, err = integratePlanarOrbit_hip(pot, yo.copy(), t, method)
+            ref, rerr, _, _ = oracle_integrate(pt, pa, yo, t, code, planar=True)
+            assert numpy.all(err == 0) and numpy.all(rerr == 0)
+            assert phase_diff(res, ref) < tol, method
+    yo = ics(64, seed=22)
+    res, err = integrateFullOrbit_hip([P.NullPotential()], yo.copy(), t[:11], "dop853_hip")
+    assert numpy.all(err == 0)
+    x = cyl_to_rect(res.reshape(-1, 6)).reshape(64, 11, 6)
+    x0 = cyl_to_rect(yo)
+    assert numpy.abs(x[..., :3] - (x0[:, None, :3] + x0[:, None, 3:] * t[None, :11, None])).max() < 1e-12

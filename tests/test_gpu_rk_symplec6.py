@@ -90,3 +90,29 @@ def test_no_tangent_map_for_rk_and_symplec6():
         with pytest.raises(RuntimeError):
             integrateFullOrbit_dxdv_hip(_mw(), y0, numpy.ones_like(y0), numpy.linspace(0, 1, 3), method)
         assert "tangent-map" in _lib.last_error()
+
+
+def test_dopr54_matches_oracle():
+    """DOPR54 (odeint 5): adaptive, so the DOP853 bar, 1e-8 at every output,
+    with the oracle's accepted and rejected step counts."""
+    import torch
+
+    from galpy_amd.integrate import PotentialTable, integrate_device, integratePlanarOrbit_hip
+
+    pot = _mw()
+    yo = ics(512, seed=8)
+    t = numpy.linspace(0.0, 10.0, 101)
+    stats = torch.zeros((len(yo), 3), dtype=torch.float64, device="cuda")
+    res, err, _ = integrate_device(PotentialTable(pot), torch.tensor(yo, device="cuda"),
+                                   torch.tensor(t, device="cuda"), "dopr54_hip", stats=stats)
+    ref, rerr, st, _ = oracle_integrate(*_packed(pot), yo, t, 5)
+    assert int((err != 0).sum()) == 0 and numpy.all(rerr == 0)
+    s = stats.cpu().numpy()
+    same = (s[:, 0] == st[:, 0]) & (s[:, 1] == st[:, 1])
+    assert same.mean() > 0.99
+    assert phase_diff(res.cpu().numpy(), ref) < 1e-8
+    yp = planar_ics(128, seed=9)
+    res, err = integratePlanarOrbit_hip(bar_potential(), yp.copy(), t, "dopr54_hip")
+    ref, rerr, _, _ = oracle_integrate(*_packed(bar_potential(), True), yp, t, 5, planar=True)
+    assert numpy.all(err == 0) and numpy.all(rerr == 0)
+    assert phase_diff(res, ref) < 1e-8

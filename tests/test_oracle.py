@@ -107,7 +107,7 @@ def test_host_log_ge1_vs_mpmath():
 
 # ---------------------------------------------------------------- potentials
 NEW_SPHERICAL = ["powersph", "powersph_a2", "kepler", "jaffe", "isochrone", "plummer", "pseudoiso", "kuzmin",
-                 "burkert", "dehnencore", "dehnen", "dehnen_a05", "homogsphere"]
+                 "burkert", "dehnencore", "dehnen", "dehnen_a05", "homogsphere", "flatpower", "flatpower_a0"]
 
 
 @pytest.mark.parametrize("name", ["psc", "mn", "nfw", "hernquist", "loghalo_q09", "loghalo_triax", "dehnenbar"]

@@ -23,7 +23,8 @@ def _mw():
 
 @pytest.mark.parametrize("name", ["psc", "mn", "nfw", "hernquist", "loghalo_q09", "powersph", "powersph_a2",
                                   "kepler", "jaffe", "isochrone", "plummer", "pseudoiso", "kuzmin", "burkert",
-                                  "dehnencore", "dehnen", "dehnen_a05", "homogsphere"])
+                                  "dehnencore", "dehnen", "dehnen_a05", "homogsphere", "flatpower",
+                                  "flatpower_a0"])
 def test_hessians_vs_mpmath(name):
     case = json.load(open(os.path.join(GOLDEN, "potentials.json")))[name]
     pts = case["points"]

@@ -76,3 +76,25 @@ def test_rk_step_rule(code):
     assert numpy.allclose(m, numpy.round(m), rtol=1e-12)
     ndt = (0.1 / dt).astype(numpy.int64)
     assert numpy.array_equal(st[:, 0].astype(numpy.int64), ndt * 100)
+
+
+def test_dopr54_converges_with_tolerance_and_conserves_energy():
+    """bovy_dopr54 (bovy_rk.c:501-760): the error against DOP853 at 1e-14
+    falls with the tolerance (a factor ~100 per factor 100), no orbit is
+    flagged, and at galpy's default tolerances the energy is conserved."""
+    from galpy_amd.potential import MWPotential2014, evaluatePotentials
+
+    pt, pa = _mw()
+    yo = ics(8, seed=7)
+    t = numpy.linspace(0.0, 10.0, 101)
+    ref, *_ = oracle_integrate(pt, pa, yo, t, 6, rtol=TIGHT, atol=TIGHT)
+    errs = []
+    for tol in (1e-8, 1e-10, 1e-12):
+        res, err, st, _ = oracle_integrate(pt, pa, yo, t, 5, rtol=numpy.log(tol), atol=numpy.log(tol))
+        assert numpy.all(err == 0)
+        errs.append(phase_diff(res, ref))
+    assert errs[0] / errs[1] > 30 and errs[1] / errs[2] > 30, errs
+    res, *_ = oracle_integrate(pt, pa, yo, numpy.linspace(0.0, 50.0, 501), 5)
+    R, vR, vT, z, vz, _ = numpy.moveaxis(res, -1, 0)
+    E = evaluatePotentials(MWPotential2014, R, z) + 0.5 * (vR**2 + vT**2 + vz**2)
+    assert numpy.abs(E / E[:, :1] - 1.0).max() < 1e-9
