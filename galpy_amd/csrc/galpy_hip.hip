@@ -23,6 +23,7 @@
 #include "adjoint_dev.h"
 #include "dxdv_dev.h"
 #include "rk_dev.h"
+#include "ias15_dev.h"
 
 #define GH_EXPORT extern "C" __attribute__((visibility("default")))
 
@@ -598,6 +599,116 @@ gh_dopr54_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
     stats[orbit * 3 + 0] = (double)nacc;
     stats[orbit * 3 + 1] = (double)nrej;
     stats[orbit * 3 + 2] = dt_one;
+  }
+}
+
+// wez_ias15 (wez_ias15.c:137-425), one orbit per lane.  The first step is
+// the grid's first spacing (or the caller's dt); the integration runs until
+// the accepted steps add up to |nt dt|, one step length past the last output
+// time as in the reference; outputs come from the accepted step's
+// interpolating polynomial at each grid time it passes.  Rows the run never
+// reaches (only when the caller's dt is shorter than the grid spacing:
+// undefined in the reference, whose result array is uninitialised) are set
+// to NaN.  Every force of a step is evaluated at the step's end time, as the
+// reference does.  A run is capped at kIas15MaxAttempts step attempts
+// (err = -3 then), a bound the reference does not have, so that no input
+// keeps a lane busy.
+constexpr long kIas15MaxAttempts = 1L << 22;
+
+template <class Spec, class V>
+__global__ void __launch_bounds__(kBlock)
+gh_ias15_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
+                int indiv_t, double dt_in, const GhPotTable* __restrict__ T,
+                double* __restrict__ result, int* __restrict__ err, double* __restrict__ stats,
+                const volatile int* cancel) {
+  const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (orbit >= nobj) return;
+  constexpr int N = GhPhase<V>::N;
+  constexpr int D = N / 2;
+  const GhForce<Spec> F{T};
+  const double* tt = t + (indiv_t ? orbit * nt : 0);
+  double x[D], v[D];
+  {
+    double w[N], y[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) w[k] = yo[orbit * N + k];
+    GhPhase<V>::to_rect(w, y);
+#pragma unroll
+    for (int k = 0; k < N; k++) yo[orbit * N + k] = y[k];  // galpy converts yo in place
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      x[i] = y[i];
+      v[i] = y[D + i];
+    }
+  }
+  double* out = result + orbit * (long)nt * N;
+  auto put = [&](int row, const double* q, const double* p) {
+    double y[N];
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      y[i] = q[i];
+      y[D + i] = p[i];
+    }
+    GhPhase<V>::store(out + (long)N * row, GhPhase<V>::q(y), GhPhase<V>::p(y));
+  };
+  put(0, x, v);
+  double B[7][D], E[7][D], G[7][D], F0[D];
+#pragma unroll
+  for (int k = 0; k < 7; k++)
+#pragma unroll
+    for (int i = 0; i < D; i++) B[k][i] = E[k][i] = 0.0;
+  const double init_dt = tt[1] - tt[0];
+  double dt = dt_in == GH_DT_SENTINEL ? init_dt : dt_in;
+  double to = tt[0];
+  double time_remaining = fabs((double)nt * dt);
+  int row = 1, e = 0;
+  long nacc = 0, nrej = 0;
+  while (time_remaining > 0.0) {
+    if (((nacc + nrej) & 63) == 0) {
+      if (cancel != nullptr && *cancel) {
+        e = -10;
+        break;
+      }
+      if (nacc + nrej >= kIas15MaxAttempts) {
+        e = -3;
+        break;
+      }
+    }
+    const double dt_temp = dt;
+    const double to_temp = to + dt_temp;
+    const double dt_required =
+        gh_ias15_attempt<GhForce<Spec>, V, D>(F, x, v, dt_temp, to_temp, F0, B, G);
+    if (fabs(dt_temp) > fabs(dt_required)) {
+      dt = dt_required;
+      nrej++;
+      continue;
+    }
+    nacc++;
+    time_remaining -= fabs(dt);
+    while (row < nt && (init_dt > 0.0 ? (to < tt[row] && tt[row] <= to_temp)
+                                      : (to > tt[row] && tt[row] >= to_temp))) {
+      const double hs = fabs(tt[row] - to) / fabs(to_temp - to);
+      double xs[D], vs[D];
+      gh_ias15_position<D>(xs, x, v, hs, dt_temp, F0, B);
+      gh_ias15_velocity<D>(vs, v, hs, dt_temp, F0, B);
+      put(row, xs, vs);
+      row++;
+    }
+    gh_ias15_advance<D>(x, v, dt_temp, F0, B, E);
+    to = to_temp;
+    dt = dt_required;
+  }
+  if (e == 0) {
+    double nan[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) nan[i] = __builtin_nan("");
+    for (; row < nt; row++) put(row, nan, nan);
+  }
+  err[orbit] = e;
+  if (stats) {
+    stats[orbit * 3 + 0] = (double)nacc;
+    stats[orbit * 3 + 1] = (double)nrej;
+    stats[orbit * 3 + 2] = dt;
   }
 }
 
@@ -1305,6 +1416,15 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
         set_error("dopr54 has no tangent-map (integrate_dxdv) stepper here");
         return -1;
       }
+    case 7:  // IAS15: plain orbits
+      if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
+        hipLaunchKernelGGL((gh_ias15_kernel<Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
+                           d_t, indiv_t, dt, d_table, d_result, d_err, d_stats, cancel);
+        break;
+      } else {
+        set_error("ias15 has no tangent-map (integrate_dxdv) stepper here");
+        return -1;
+      }
     case 4:  // symplec6: plain orbits (galpy has no symplec6 tangent map)
       if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
         hipLaunchKernelGGL((gh_symplectic_kernel<4, Spec, V>), grid, block, 0, stream, nobj,
@@ -1336,7 +1456,7 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
     }
     default:
       set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 1 rk4, 2 rk6, "
-                "3 symplec4, 4 symplec6, 5 dopr54, 6 dop853)", odeint_type);
+                "3 symplec4, 4 symplec6, 5 dopr54, 6 dop853, 7 ias15)", odeint_type);
       return -1;
   }
   GH_CHECK(hipGetLastError());
@@ -1430,9 +1550,9 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
     set_error("nt=%d: need at least two output times", nt);
     return -1;
   }
-  if (odeint_type < 0 || odeint_type > 6) {
+  if (odeint_type < 0 || odeint_type > 7) {
     set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 1 rk4, 2 rk6, "
-              "3 symplec4, 4 symplec6, 5 dopr54, 6 dop853)", odeint_type);
+              "3 symplec4, 4 symplec6, 5 dopr54, 6 dop853, 7 ias15)", odeint_type);
     return -1;
   }
   GhPotTable table;

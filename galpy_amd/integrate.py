@@ -33,6 +33,7 @@ METHOD_CODES = {
     "rk6_hip": 2,
     "dopr54_hip": 5,
     "dop853_hip": 6,
+    "ias15_hip": 7,
 }
 
 

@@ -11,8 +11,9 @@ integral accessors used by galpy's tests: ``R, vR, vT, z, vz, phi, x, y,
 vx, vy`` (:2868), ``getOrbit``, ``E`` (:2902), ``Jacobi`` (:3298), ``L``,
 ``Lz``, ``ER``, ``Ez``, ``rperi``, ``rap``, ``zmax``, ``e``.
 
-Only the HIP methods ``leapfrog_hip``, ``symplec4_hip`` and ``dop853_hip``
-integrate; there is no silent fallback to another backend.  As in galpy,
+Only the HIP methods (``leapfrog_hip``, ``rk4_hip``, ``rk6_hip``,
+``symplec4_hip``, ``symplec6_hip``, ``dopr54_hip``, ``dop853_hip``,
+``ias15_hip``) integrate; there is no silent fallback to another backend.  As in galpy,
 ``numpy.copy`` of the initial conditions is passed (Orbits.py:1793) because
 the C-ABI overwrites its ``yo`` argument.
 """

@@ -110,6 +110,9 @@ WORKLOADS = {
     # DOP853 workload's grid and potential at a quarter of the orbits
     "mw2014_dopr54_256k": dict(pot=mw2014, n=262144, t=(0.0, 500.0, 1001), method="dopr54_hip",
                               omega=0.0, seed=0),
+    # IAS15 (galpy's 15th-order Gauss-Radau), the same grid and orbit count
+    "mw2014_ias15_256k": dict(pot=mw2014, n=262144, t=(0.0, 500.0, 1001), method="ias15_hip",
+                             omega=0.0, seed=0),
     # a wrapped (rotating) component: DOP853, drift of the Jacobi integral
     "mw2014_rottriax_dop853_256k": dict(pot=mw2014_rotating_triaxial, n=262144, t=(0.0, 500.0, 1001),
                                        method="dop853_hip", omega=0.6, seed=0),

@@ -986,4 +986,237 @@ void dop853(const ForceCtx<R, D>& F, double* y0d, int nt, const double* t, doubl
   if (st) { st->steps = nacc; st->rejected = nrej; st->dt = 0.; }
 }
 
+// ---------------------------------------------------------------------------
+// IAS15 (wez_ias15.c:137-425), galpy's 15th-order Gauss-Radau integrator
+// (Everhart 1985; Rein & Spiegel 2015) on the rectangular positions q and
+// velocities p.  Per step: the force at the step start, then a
+// predictor-corrector loop over the seven Radau substeps that refines the
+// polynomial coefficients B of the force (through their divided-difference
+// form G) until the change of G[6] relative to the largest start-of-step
+// acceleration component falls below 1e-16, or 12 passes; the step is
+// accepted when the next step dt (B[6] based, 1e-9 precision parameter,
+// seventh root by Newton's method) is not smaller than the one taken.
+// Outputs come from the step's interpolating polynomial at every grid time
+// the accepted step passes.  Behaviour kept from the reference:
+//   * every force in a step is evaluated at the step's END time to + dt;
+//   * the integration stops once the accepted steps add up to |nt dt|
+//     (dt the grid's first spacing, or the caller's dt), one dt past the
+//     last output time; rows never passed stay unwritten;
+//   * B is carried to the next step by the binomial predictor at step
+//     ratio 1 and is not reset after a rejected step.
+// ---------------------------------------------------------------------------
+namespace radau {
+// Gauss-Radau spacings on [0, 1] (h[0] = 0, then the 7 nodes of the 8-point
+// Radau rule), wez_ias15.c:43-52
+constexpr double h[8] = {0.0,
+                         0.05626256053692215,
+                         0.18024069173689236,
+                         0.35262471711316964,
+                         0.54715362633055538,
+                         0.73421017721541053,
+                         0.88532094683909577,
+                         0.97752061356128750};
+// rr[k][j] = 1 / (h[k] - h[j]), j < k (wez_ias15.c:113-148)
+constexpr double rr[8][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {17.773808914078, 0, 0, 0, 0, 0, 0},
+    {5.548136718537217, 8.065938648381888, 0, 0, 0, 0, 0},
+    {2.835876078644439, 3.3742499769626355, 5.801001559264062, 0, 0, 0, 0},
+    {1.8276402675175978, 2.0371118353585844, 2.725442211808226, 5.140624105810932, 0, 0, 0},
+    {1.3620078160624696, 1.4750402175604116, 1.8051535801402514, 2.620644926387035,
+     5.3459768998711095, 0, 0},
+    {1.1295338753367898, 1.2061876660584456, 1.418278263734739, 1.8772424961868102,
+     2.957116017290456, 6.617662013702422, 0},
+    {1.0229963298234868, 1.0854721939386425, 1.2542646222818779, 1.6002665494908161,
+     2.3235983002196945, 4.109975778344558, 10.846026190236847}};
+// G -> B conversion, cc[k][j] (c_{k,j} of wez_ias15.c:54-81): B[j] gains
+// cc[k-1][j] times the change of G[k-1], B[k-1] the change itself
+constexpr double cc[7][7] = {
+    {1, 0, 0, 0, 0, 0, 0},
+    {-0.05626256053692215, 1, 0, 0, 0, 0, 0},
+    {0.01014080283006363, -0.23650325227381452, 1, 0, 0, 0, 0},
+    {-0.003575897729251617, 0.09353769525946207, -0.5891279693869842, 1, 0, 0, 0},
+    {0.001956565409947221, -0.05475538688906869, 0.4158812000823069, -1.1362815957175396, 1, 0,
+     0},
+    {-0.0014365302363708915, 0.042158527721268706, -0.3600995965020568, 1.250150711840691,
+     -1.87049177293295, 1, 0},
+    {0.0012717903090268678, -0.03876035791590677, 0.360962243452846, -1.466884208400427,
+     2.906136259308429, -2.7558127197720457, 1}};
+// B -> G conversion, dd[k][j] (d_{k,j} of wez_ias15.c:83-111): G[j] =
+// sum over k >= j of dd[k][j] B[k]
+constexpr double dd[7][7] = {
+    {1, 0, 0, 0, 0, 0, 0},
+    {0.05626256053692215, 1, 0, 0, 0, 0, 0},
+    {0.0031654757181708297, 0.23650325227381452, 1, 0, 0, 0, 0},
+    {0.00017809776922174343, 0.04579298550602792, 0.5891279693869842, 1, 0, 0, 0},
+    {1.002023652232913e-05, 0.008431857153525702, 0.25353406905456927, 1.1362815957175396, 1, 0,
+     0},
+    {5.637641639318209e-07, 0.001529784002500466, 0.09783423653244401, 0.8752546646840911,
+     1.87049177293295, 1, 0},
+    {3.171881540176138e-08, 0.0002762930909826477, 0.03602855398373646, 0.5767330002770787,
+     2.2485887607691595, 2.7558127197720457, 1}};
+constexpr double kDeltaB = 1e-16;    // predictor-corrector convergence (e_deltab)
+constexpr double kPrecision = 1e-9;  // step-size precision parameter (e_b)
+constexpr int kMaxPasses = 12;       // predictor-corrector passes per step
+}  // namespace radau
+
+// a^(1/7) by 20 Newton steps from 1 after scaling a into [1e-7, 1e2] by
+// factors of 1e7 (wez_ias15.c:587-603); NaN/inf/zero are not scaled.
+template <class R> R ias15_seventhroot(R a) {
+  R scale = 1.;
+  while (a < R(1e-7) && std::isnormal(dval(a))) {
+    scale = scale * R(0.1);
+    a = a * R(1e7);
+  }
+  while (a > R(1e2) && std::isnormal(dval(a))) {
+    scale = scale * R(10.);
+    a = a * R(1e-7);
+  }
+  R x = 1.;
+  for (int k = 0; k < 20; k++) {
+    R x6 = x * x * x * x * x * x;
+    x += (a / x6 - x) / R(7.);
+  }
+  return x * scale;
+}
+
+// position at substep fraction hn of a step of length T (wez_ias15.c:459-476)
+template <int D, class R>
+void ias15_position(R* y, const R* y1, const R* v, R hn, R T, const R (*F0), const R (*B)[D]) {
+  for (int i = 0; i < D; i++) {
+    const R poly =
+        F0[i] / R(2.) +
+        hn * (B[0][i] / R(6.) +
+              hn * (B[1][i] / R(12.) +
+                    hn * (B[2][i] / R(20.) +
+                          hn * (B[3][i] / R(30.) +
+                                hn * (B[4][i] / R(42.) +
+                                      hn * (B[5][i] / R(56.) + hn * (B[6][i] / R(72.))))))));
+    y[i] = y1[i] + (v[i] * hn * T) + ((hn * T) * (hn * T)) * poly;
+  }
+}
+// velocity at substep fraction hn (wez_ias15.c:441-456)
+template <int D, class R>
+void ias15_velocity(R* v, const R* v1, R hn, R T, const R* F0, const R (*B)[D]) {
+  for (int i = 0; i < D; i++) {
+    const R poly =
+        F0[i] + hn * (B[0][i] / R(2.) +
+                      hn * (B[1][i] / R(3.) +
+                            hn * (B[2][i] / R(4.) +
+                                  hn * (B[3][i] / R(5.) +
+                                        hn * (B[4][i] / R(6.) +
+                                              hn * (B[5][i] / R(7.) + hn * (B[6][i] / R(8.))))))));
+    v[i] = v1[i] + (hn * T * poly);
+  }
+}
+
+template <int D, class R>
+void ias15(const ForceCtx<R, D>& F, const double* yo, int nt, double dt_in, const double* t,
+           double, double, double* result, int* err, IntStats* st) {
+  using namespace radau;
+  R x[D], v[D], xs[D], vs[D], a[D], F0[D];
+  R B[7][D], E[7][D], G[7][D];
+  for (int i = 0; i < D; i++) {
+    x[i] = yo[i];
+    v[i] = yo[D + i];
+  }
+  for (int k = 0; k < 7; k++)
+    for (int i = 0; i < D; i++) B[k][i] = E[k][i] = G[k][i] = 0.;
+  for (int i = 0; i < 2 * D; i++) result[i] = yo[i];
+  *err = 0;
+  const R init_dt = R(t[1]) - R(t[0]);
+  R dt = dt_in == kDtSentinel ? init_dt : R(dt_in);
+  R to = t[0];
+  R time_remaining = fabs(R(nt) * dt);
+  int row = 1;
+  long long nacc = 0, nrej = 0;
+  while (time_remaining > R(0.)) {
+    const R dt_temp = dt;
+    const R to_temp = to + dt_temp;
+    F.force(to_temp, x, a);
+    for (int i = 0; i < D; i++) F0[i] = a[i];
+    for (int j = 0; j < 7; j++)  // G from B (update_Gs_from_Bs, wez_ias15.c:514-529)
+      for (int i = 0; i < D; i++) {
+        R g = dd[j][j] * B[j][i];
+        for (int k = j + 1; k < 7; k++) g = g + dd[k][j] * B[k][i];
+        G[j][i] = g;
+      }
+    R ierr = R(kDeltaB + 1.);
+    for (int pass = 0; pass < kMaxPasses && !(ierr < R(kDeltaB)); pass++) {
+      R max_dg = 0., max_a = 0.;
+      for (int k = 1; k < 8; k++) {
+        ias15_position<D>(xs, x, v, R(h[k]), dt_temp, F0, B);
+        F.force(to_temp, xs, a);
+        for (int i = 0; i < D; i++) {
+          // G[k-1] from the new force by divided differences (wez_ias15.c:478-496)
+          R g = (a[i] - F0[i]) * R(rr[k][0]);
+          for (int j = 1; j < k; j++) g = (g - G[j - 1][i]) * R(rr[k][j]);
+          const R dg = g - G[k - 1][i];
+          G[k - 1][i] = g;
+          // B from the change of G[k-1] (wez_ias15.c:554-585)
+          for (int j = 0; j < k - 1; j++) B[j][i] += dg * R(cc[k - 1][j]);
+          B[k - 1][i] += dg;
+          if (k == 7) {
+            if (fabs(dg) > max_dg) max_dg = fabs(dg);
+            if (fabs(F0[i]) > max_a) max_a = fabs(F0[i]);
+          }
+        }
+      }
+      ierr = max_dg / max_a;
+    }
+    R max_b6 = 0., max_a = 0.;
+    for (int i = 0; i < D; i++) {
+      if (fabs(B[6][i]) > max_b6) max_b6 = fabs(B[6][i]);
+      if (fabs(F0[i]) > max_a) max_a = fabs(F0[i]);
+    }
+    const R dt_required = dt_temp * ias15_seventhroot(R(kPrecision) / (max_b6 / max_a));
+    if (fabs(dt_temp) > fabs(dt_required)) {
+      dt = dt_required;
+      nrej++;
+      continue;
+    }
+    nacc++;
+    time_remaining -= fabs(dt);
+    // outputs inside the step (wez_ias15.c:381-405)
+    while (row < nt && (init_dt > R(0.) ? (to < R(t[row]) && R(t[row]) <= to_temp)
+                                        : (to > R(t[row]) && R(t[row]) >= to_temp))) {
+      const R hs = fabs(R(t[row]) - to) / fabs(to_temp - to);
+      ias15_position<D>(xs, x, v, hs, dt_temp, F0, B);
+      ias15_velocity<D>(vs, v, hs, dt_temp, F0, B);
+      for (int i = 0; i < D; i++) {
+        result[2 * D * row + i] = dval(xs[i]);
+        result[2 * D * row + D + i] = dval(vs[i]);
+      }
+      row++;
+    }
+    ias15_position<D>(x, x, v, R(1.), dt_temp, F0, B);
+    ias15_velocity<D>(v, v, R(1.), dt_temp, F0, B);
+    // binomial predictor of the next step's B at step ratio 1 (wez_ias15.c:531-552)
+    for (int i = 0; i < D; i++) {
+      R bd[7], en[7];
+      for (int k = 0; k < 7; k++) bd[k] = B[k][i] - E[k][i];
+      en[0] = (B[0][i] + R(2.) * B[1][i] + R(3.) * B[2][i] + R(4.) * B[3][i] + R(5.) * B[4][i] +
+               R(6.) * B[5][i] + R(7.) * B[6][i]);
+      en[1] = (B[1][i] + R(3.) * B[2][i] + R(6.) * B[3][i] + R(10.) * B[4][i] + R(15.) * B[5][i] +
+               R(21.) * B[6][i]);
+      en[2] = (B[2][i] + R(4.) * B[3][i] + R(10.) * B[4][i] + R(20.) * B[5][i] + R(35.) * B[6][i]);
+      en[3] = (B[3][i] + R(5.) * B[4][i] + R(15.) * B[5][i] + R(35.) * B[6][i]);
+      en[4] = (B[4][i] + R(6.) * B[5][i] + R(21.) * B[6][i]);
+      en[5] = (B[5][i] + R(7.) * B[6][i]);
+      en[6] = B[6][i];
+      for (int k = 0; k < 7; k++) {
+        E[k][i] = en[k];
+        B[k][i] = en[k] + bd[k];
+      }
+    }
+    to = to_temp;
+    dt = dt_required;
+  }
+  // rows never reached (a caller's dt shorter than the grid spacing): the
+  // reference leaves them uninitialised; NaN here, as on the device
+  for (; row < nt; row++)
+    for (int i = 0; i < 2 * D; i++) result[2 * D * row + i] = std::nan("");
+  if (st) { st->steps = nacc; st->rejected = nrej; st->dt = dval(dt); }
+}
+
 }  // namespace orc

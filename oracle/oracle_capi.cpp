@@ -46,6 +46,7 @@ void integrate_one(int npot, const Component* P, int odeint_type, double* yo, in
     case 4: symplec6<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
     case 5: dopr54<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
     case 6: dop853<D>(F, yo, nt, t, rtol, atol, result, err, &local); break;
+    case 7: ias15<D>(F, yo, nt, dt, t, rtol, atol, result, err, &local); break;
     default: *err = -1; return;
   }
   for (int j = 0; j < nt; j++) {  // integrateFullOrbit.c:1139-1140, integratePlanarOrbit.c:832-833
