@@ -27,7 +27,7 @@
  * npot counts the top-level components, as in galpy.
  * Supported odeint_type codes (integrateFullOrbit.c:1091-1132,
  * integratePlanarOrbit.c:788-825):
- *   0 leapfrog, 1 rk4, 2 rk6, 3 symplec4, 4 symplec6, 5 dopr54, 6 dop853
+ *   0 leapfrog, 1 rk4, 2 rk6, 3 symplec4, 4 symplec6, 5 dopr54, 6 dop853, 7 ias15
  * (integrate_dxdv: 0, 3 and 6, as galpy's tangent-map steppers and DOP853).
  * Any other code, or nt < 2, sets err[i] = -1 for every orbit and records a
  * message readable with galpy_hip_last_error().
@@ -53,7 +53,10 @@ extern "C" {
  *                   dopr54 as bovy_dopr54 (bovy_rk.c:501-616): odd when a
  *                   step was forced through at 1/10000 of the interval's first
  *                   step, -3 when one output interval took more than 2^22
- *                   step attempts (a bound the reference does not have)
+ *                   step attempts (a bound the reference does not have);
+ *                   ias15: -3 after 2^22 step attempts in all; rows past
+ *                   the end of the run (a dt shorter than the grid spacing,
+ *                   wez_ias15.c:280) are NaN, uninitialised in the reference
  *   pot_tfuncs      must be NULL (no supported potential uses time functions)
  *   cb              called once per finished orbit (may be NULL)
  * All pointers are host memory.  Work is streamed through the GPU in chunks
