@@ -179,6 +179,79 @@ class MiyamotoNagaiPotential(Potential):
         return [self._amp, self._a, self._b]
 
 
+# Smith et al. (2015) fits used by MN3ExponentialDiskPotential.py:207-336:
+# quartic polynomials in b/Rd of the three Miyamoto-Nagai masses and scale
+# lengths (their Table 1: exponential-disk density matched everywhere; Table
+# 2: positive density everywhere), and cubics from hz/Rd to b/Rd.
+_MN3_TAB1 = {  # coefficients of brd^4, brd^3, brd^2, brd, 1
+    "mass": ((-0.0090, 0.0640, -0.1653, 0.1164, 1.9487),
+             (0.0173, -0.0903, 0.0877, 0.2029, -1.3077),
+             (-0.0051, 0.0287, -0.0361, -0.0544, 0.2242)),
+    "a": ((-0.0358, 0.2610, -0.6987, -0.1193, 2.0074),
+          (-0.0830, 0.4992, -0.7967, -1.2966, 4.4441),
+          (-0.0247, 0.1718, -0.4124, -0.5944, 0.7333)),
+}
+_MN3_TAB2 = {
+    "mass": ((0.0036, -0.0330, 0.1117, -0.1335, 0.1749),
+             (-0.0131, 0.1090, -0.3035, 0.2921, -5.7976),
+             (-0.0048, 0.0454, -0.1425, 0.1012, 6.7120)),
+    "a": ((-0.0158, 0.0993, -0.2070, -0.7089, 0.6445),
+          (-0.0319, 0.1514, -0.1279, -0.9325, 2.6836),
+          (-0.0326, 0.1816, -0.2943, -0.6329, 2.3193)),
+}
+_MN3_B_EXP = (-0.269, 1.080, 1.092)  # coefficients of hz^3, hz^2, hz
+_MN3_B_SECH = (-0.033, 0.262, 0.659)
+
+
+def _mn3_quartic(c, x):
+    # the reference's form c4 x^4 + c3 x^3 + c2 x^2 + c1 x + c0, term by term
+    return c[0] * x**4.0 + c[1] * x**3.0 + c[2] * x**2.0 + c[3] * x + c[4]
+
+
+class MN3ExponentialDiskPotential(Potential):
+    """Three Miyamoto-Nagai disks approximating an exponential (or sech^2)
+    disk of scale length hr and height hz (MN3ExponentialDiskPotential.py:
+    33-143); packed as three pot_type 5 components with amplitudes
+    amp * mass_i (integrateFullOrbit.py:182-198, integratePlanarOrbit.py:
+    197-215).  amp is the disk's central density, as in galpy."""
+
+    pot_code = None
+
+    def __init__(self, amp=1.0, hr=1.0 / 3.0, hz=1.0 / 16.0, sech=False, posdens=False,
+                 normalize=False):
+        Potential.__init__(self, amp=amp)
+        self._hr = hr
+        self._hz = hz
+        self._amp *= 4.0 * numpy.pi * self._hr**2.0 * self._hz
+        c = _MN3_B_SECH if sech else _MN3_B_EXP
+        x = self._hz / self._hr
+        self._brd = c[0] * x**3.0 + c[1] * x**2.0 + c[2] * x
+        if self._brd < 0.0:
+            raise OSError("MN3ExponentialDiskPotential's b/Rd is negative for the given hz")
+        self._b = self._brd * self._hr
+        tab = _MN3_TAB2 if posdens else _MN3_TAB1
+        self._mn3 = [
+            MiyamotoNagaiPotential(amp=_mn3_quartic(m, self._brd), a=_mn3_quartic(a, self._brd) * self._hr,
+                                   b=self._b)
+            for m, a in zip(tab["mass"], tab["a"])
+        ]
+        if normalize or (isinstance(normalize, (int, float)) and not isinstance(normalize, bool)):
+            self.normalize(normalize)
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return sum(mn(R, z) for mn in self._mn3)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return sum(mn.Rforce(R, z) for mn in self._mn3)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return sum(mn.zforce(R, z) for mn in self._mn3)
+
+    def expand(self):
+        """The three packed components: [amp * mass_i, a_i, b]."""
+        return [[self._amp * mn._amp, mn._a, mn._b] for mn in self._mn3]
+
+
 class NFWPotential(Potential):
     """pot_type 9, args [amp, a]."""
 
@@ -834,9 +907,14 @@ def _pack(comps, planar):
             wt, wa = _pack(inner, planar)
             pot_type.append(p.pot_code)
             pot_type.extend(wt)
-            pot_args.append(float(len(inner)))
+            pot_args.append(float(_npot(inner)))
             pot_args.extend(wa)
             pot_args.extend(p.wrapper_args())
+            continue
+        if isinstance(p, MN3ExponentialDiskPotential):
+            for args in p.expand():
+                pot_type.append(MiyamotoNagaiPotential.pot_code)
+                pot_args.extend(args)
             continue
         if not isinstance(p, Potential) or p.pot_code is None:
             raise NotImplementedError(
@@ -845,6 +923,12 @@ def _pack(comps, planar):
         pot_type.append(p.pot_code)
         pot_args.extend(p.c_args_planar() if planar else p.c_args())
     return pot_type, pot_args
+
+
+def _npot(comps):
+    """galpy's npot: an MN3ExponentialDiskPotential counts as its three
+    Miyamoto-Nagai disks (integrateFullOrbit.py:184)."""
+    return sum(3 if isinstance(p, MN3ExponentialDiskPotential) else 1 for p in comps)
 
 
 def _purge_null(comps):
@@ -864,7 +948,7 @@ def parse_pot(pot):
     comps = _purge_null(flatten(pot))
     pot_type, pot_args = _pack(comps, False)
     return (
-        len(comps),
+        _npot(comps),
         numpy.array(pot_type, dtype=numpy.int32, order="C"),
         numpy.array(pot_args, dtype=numpy.float64, order="C"),
         [],
@@ -878,7 +962,7 @@ def parse_pot_planar(pot):
     comps = _purge_null(flatten(pot))
     pot_type, pot_args = _pack(comps, True)
     return (
-        len(comps),
+        _npot(comps),
         numpy.array(pot_type, dtype=numpy.int32, order="C"),
         numpy.array(pot_args, dtype=numpy.float64, order="C"),
         [],

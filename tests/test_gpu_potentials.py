@@ -334,3 +334,29 @@ def test_flattened_power_planar_and_null_potential():
     x = cyl_to_rect(res.reshape(-1, 6)).reshape(64, 11, 6)
     x0 = cyl_to_rect(yo)
     assert numpy.abs(x[..., :3] - (x0[:, None, :3] + x0[:, None, 3:] * t[None, :11, None])).max() < 1e-12
+
+
+def test_mn3_exponential_disk_orbits():
+    """MN3ExponentialDiskPotential reaches the device as three Miyamoto-Nagai
+    components (galpy's packing): full and planar orbits with a halo, the
+    fixed-step and DOP853 bars."""
+    from galpy_amd import potential as P
+    from galpy_amd.integrate import integrateFullOrbit_hip, integratePlanarOrbit_hip
+
+    t = numpy.linspace(0.0, 10.0, 101)
+    pot = [P.MN3ExponentialDiskPotential(hr=1.0 / 3.0, hz=1.0 / 16.0, normalize=0.6),
+           P.NFWPotential(a=2.0, normalize=0.4)]
+    yo = ics(128, seed=23)
+    pt, pa = _packed(pot)
+    assert list(pt) == [5, 5, 5, 9]
+    for method, code, tol in (("symplec4_hip", 3, 1e-10), ("dop853_hip", 6, 1e-8)):
+        res, err = integrateFullOrbit_hip(pot, yo.copy(), t, method)
+        ref, rerr, _, _ = oracle_integrate(pt, pa, yo, t, code)
+        assert numpy.all(err == 0) and numpy.all(rerr == 0)
+        assert phase_diff(res, ref) < tol, method
+    yp = planar_ics(128, seed=24)
+    ppt, ppa = _packed(pot, True)
+    res, err = integratePlanarOrbit_hip(pot, yp.copy(), t, "dop853_hip")
+    ref, rerr, _, _ = oracle_integrate(ppt, ppa, yp, t, 6, planar=True)
+    assert numpy.all(err == 0) and numpy.all(rerr == 0)
+    assert phase_diff(res, ref) < 1e-8
