@@ -89,6 +89,7 @@ int nargs_of(int code, bool planar) {
     case GH_DEHNENSPH: case GH_HOMOGSPHERE: return 3;
     case GH_FLATPOWER: return planar ? 3 : 4;  // planar: amp, alpha, core^2
     case GH_NULLPOT: return 0;
+    case GH_KKSTAECKEL: return 3;  // amp, ac, Delta
     // wrappers' own arguments, read after the wrapped components
     // (integrateFullOrbit.c:821-895, integratePlanarOrbit.c:591-663)
     case GH_DEHNENSMOOTHWRAP: return 4;   // amp, tform, tsteady, grow
@@ -265,6 +266,11 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
       case GH_ISOCHRONE:
       case GH_PLUMMER: C.c[0] = C.p[1] * C.p[1]; break;  // b^2
       case GH_FLATPOWER: C.c[0] = 1.0 / C.p[2]; break;   // 1/q^2
+      case GH_KKSTAECKEL:  // gamma, alpha, Delta^2
+        C.c[2] = C.p[2] * C.p[2];
+        C.c[0] = C.c[2] / (1.0 - C.p[1] * C.p[1]);
+        C.c[1] = C.c[0] - C.c[2];
+        break;
       case GH_PSEUDOISO:
       case GH_BURKERT: C.c[0] = 1.0 / C.p[1]; break;  // 1/a
       case GH_DEHNENSPH:
@@ -1291,7 +1297,7 @@ void fill_handle(const GhPotTable& table, GhPotHandle* h) {
 bool hessian_capable(const GhPotHandle* h) {
   if (h->has_wrap) return false;  // time-dependent (and rotating) wrappers
   for (int i = 0; i < h->npot; i++) {
-    if (h->types[i] == GH_DEHNENBAR || h->nonaxi[i]) return false;
+    if (h->types[i] == GH_DEHNENBAR || h->types[i] == GH_KKSTAECKEL || h->nonaxi[i]) return false;
   }
   return true;
 }

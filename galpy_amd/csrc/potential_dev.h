@@ -45,6 +45,7 @@ enum GhPotCode {
   GH_DEHNENSPH = 34,
   GH_HOMOGSPHERE = 35,
   GH_FLATPOWER = 12,
+  GH_KKSTAECKEL = 16,
   GH_NULLPOT = 40,
   // wrappers (integrateFullOrbit.c:821-895)
   GH_DEHNENSMOOTHWRAP = -1,
@@ -526,6 +527,29 @@ __device__ __forceinline__ void gh_comp_force<GH_KUZMINDISK>(const GhPotTable* _
 }
 
 template <>
+__device__ __forceinline__ void gh_comp_force<GH_KKSTAECKEL>(const GhPotTable* __restrict__ T,
+                                                             int i, const GhGeom& g, GhAcc& F) {
+  // KuzminKutuzovStaeckelPotential.c:24-85: Phi = -amp / (sqrt(l) + sqrt(n))
+  // in the spheroidal coordinates l >= -alpha >= n >= -gamma of (R, z)
+  // (focal distance Delta, gamma = Delta^2 / (1 - ac^2), alpha = gamma -
+  // Delta^2); forces by the chain rule through l and n.  c[0] = gamma,
+  // c[1] = alpha, c[2] = Delta^2.
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0], gam = C->c[0], alp = C->c[1], D2 = C->c[2];
+  const double r2 = g.R2 + g.z2;
+  const double dm = r2 - D2;
+  const double s = sqrt(dm * dm + 4.0 * D2 * g.R2);
+  const double is = 1.0 / s;
+  const double term = r2 - alp - gam;
+  const double sl = sqrt(0.5 * (term + s)), sn = sqrt(0.5 * (term - s));
+  const double isum = 1.0 / (sl + sn);
+  const double dVdl = 0.5 / sl * isum * isum, dVdn = 0.5 / sn * isum * isum;
+  const double uR = (r2 + D2) * is, uz = dm * is;
+  F.FRoR += -amp * ((1.0 + uR) * dVdl + (1.0 - uR) * dVdn);
+  F.Fz += -amp * g.z * ((1.0 + uz) * dVdl + (1.0 - uz) * dVdn);
+}
+
+template <>
 __device__ __forceinline__ void gh_comp_force<GH_FLATPOWER>(const GhPotTable* __restrict__ T,
                                                             int i, const GhGeom& g, GhAcc& F) {
   // FlattenedPowerPotential.c:19-64: F = -amp m^(-alpha/2 - 1) (x, y, z/q^2),
@@ -560,6 +584,7 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
     case GH_PSEUDOISO: gh_comp_force<GH_PSEUDOISO>(T, i, g, F); break;
     case GH_KUZMINDISK: gh_comp_force<GH_KUZMINDISK>(T, i, g, F); break;
     case GH_FLATPOWER: gh_comp_force<GH_FLATPOWER>(T, i, g, F); break;
+    case GH_KKSTAECKEL: gh_comp_force<GH_KKSTAECKEL>(T, i, g, F); break;
     case GH_BURKERT: gh_comp_force<GH_BURKERT>(T, i, g, F); break;
     case GH_DEHNENCORESPH: gh_comp_force<GH_DEHNENCORESPH>(T, i, g, F); break;
     case GH_DEHNENSPH: gh_comp_force<GH_DEHNENSPH>(T, i, g, F); break;
@@ -819,6 +844,13 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
         const double alpha = C->p[1], m2 = C->p[3] + R2 + z2 * C->c[0];
         pot += (alpha == 0.0) ? 0.5 * C->p[0] * log(m2)
                               : -C->p[0] * pow(m2, -0.5 * alpha) / alpha;
+        break;
+      }
+      case GH_KKSTAECKEL: {  // KuzminKutuzovStaeckelPotential.c:5-23 (n clamped at 0)
+        const double dm = r2 - C->c[2];
+        const double s = sqrt(dm * dm + 4.0 * C->c[2] * R2), term = r2 - C->c[1] - C->c[0];
+        const double n = 0.5 * (term - s);
+        pot += -C->p[0] / (sqrt(0.5 * (term + s)) + sqrt(n > 0.0 ? n : 0.0));
         break;
       }
       case GH_KUZMINDISK: {  // KuzminDiskPotential.c:5-15

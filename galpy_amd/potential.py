@@ -252,6 +252,54 @@ class MN3ExponentialDiskPotential(Potential):
         return [[self._amp * mn._amp, mn._a, mn._b] for mn in self._mn3]
 
 
+class KuzminKutuzovStaeckelPotential(Potential):
+    """pot_type 16, args [amp, ac, Delta] (KuzminKutuzovStaeckelPotential.py:
+    16-140, integrateFullOrbit.py:199-201): Phi = -amp / (sqrt(l) + sqrt(n))
+    with (l, n) the prolate spheroidal coordinates of focal distance Delta
+    whose coordinate surfaces have axis ratio ac (ac > 1: gamma = Delta^2 /
+    (1 - ac^2) < 0, alpha = gamma - Delta^2)."""
+
+    pot_code = 16
+
+    def __init__(self, amp=1.0, ac=5.0, Delta=1.0, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self._ac = ac
+        self._Delta = Delta
+        self._gamma = self._Delta**2.0 / (1.0 - self._ac**2.0)
+        self._alpha = self._gamma - self._Delta**2.0
+        if normalize or (isinstance(normalize, (int, float)) and not isinstance(normalize, bool)):
+            self.normalize(normalize)
+
+    def _ln(self, R, z):
+        # the roots l >= n of (t + alpha)(t + gamma) - R^2 (t + gamma) - z^2 (t + alpha)
+        r2 = R**2.0 + z**2.0
+        term = r2 - self._alpha - self._gamma
+        sd = numpy.sqrt((r2 - self._Delta**2.0) ** 2.0 + 4.0 * self._Delta**2.0 * R**2.0)
+        return 0.5 * (term + sd), 0.5 * (term - sd), sd
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        l, n, _ = self._ln(R, z)
+        return -1.0 / (numpy.sqrt(l) + numpy.sqrt(numpy.maximum(n, 0.0)))
+
+    def _dV(self, R, z):
+        l, n, sd = self._ln(R, z)
+        s2 = (numpy.sqrt(l) + numpy.sqrt(n)) ** 2.0
+        return 0.5 / numpy.sqrt(l) / s2, 0.5 / numpy.sqrt(n) / s2, sd
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        dVdl, dVdn, sd = self._dV(R, z)
+        u = (R**2.0 + z**2.0 + self._Delta**2.0) / sd
+        return -R * ((1.0 + u) * dVdl + (1.0 - u) * dVdn)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        dVdl, dVdn, sd = self._dV(R, z)
+        u = (R**2.0 + z**2.0 - self._Delta**2.0) / sd
+        return -z * ((1.0 + u) * dVdl + (1.0 - u) * dVdn)
+
+    def c_args(self):
+        return [self._amp, self._ac, self._Delta]
+
+
 class NFWPotential(Potential):
     """pot_type 9, args [amp, a]."""
 

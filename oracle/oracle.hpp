@@ -152,6 +152,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kNFW = 9, kJaffe = 10, kIsochrone = 14, kPowerSphCutoff = 15, kPlummer = 17,
                kPseudoIso = 18, kKuzminDisk = 19, kBurkert = 20, kDehnenCoreSph = 33,
                kDehnenSph = 34, kHomogSphere = 35, kFlattenedPower = 12, kNull = 40,
+               kKKStaeckel = 16,
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5 };
@@ -190,6 +191,7 @@ inline int component_nargs(int code, bool planar = false) {
     case kKuzminDisk: case kBurkert: case kDehnenCoreSph: return 2;
     case kFlattenedPower: return planar ? 3 : 4;  // planar: amp, alpha, core^2
     case kNull: return 0;
+    case kKKStaeckel: return 3;  // amp, ac, Delta
     case kDehnenSph: case kHomogSphere: return 3;
     // wrappers' own arguments, read after the wrapped components
     case kDehnenSmoothWrap: return 4;   // amp, tform, tsteady, grow
@@ -588,6 +590,50 @@ template <class R> R kuzmin_eval(const Component& C, R Rc, R z) {
   return -R(C.p[0]) * pow(Rc * Rc + pow(R(C.p[1]) + fabs(z), R(2.)), R(-0.5));
 }
 
+// KuzminKutuzovStaeckelPotential.c: Eval (:5-23, n clamped at 0), Rforce
+// (:24-45), zforce (:64-85) and planarRforce (:46-63, in the plane n =
+// -gamma and l = R^2 - alpha), with the reference's coordinate set-up:
+// gamma = Delta^2 / (1 - ac^2), alpha = gamma - Delta^2 and (l, n) the roots
+// of the spheroidal-coordinate quadratic of (R, z).
+template <class R> void kks_coords(const Component& C, R Rc, R z, R& l, R& n, R& sd) {
+  const double ac = C.p[1], Delta = C.p[2];
+  const double gamma = Delta * Delta / (1. - ac * ac);
+  const double alpha = gamma - Delta * Delta;
+  R term = Rc * Rc + z * z - R(alpha) - R(gamma);
+  R discr = pow(Rc * Rc + z * z - R(Delta * Delta), R(2.)) + (R(4.) * R(Delta * Delta) * Rc * Rc);
+  sd = sqrt(discr);
+  l = R(0.5) * (term + sd);
+  n = R(0.5) * (term - sd);
+}
+template <class R> void kks_forces(const Component& C, R Rc, R z, R& FR, R& Fz) {
+  const double amp = C.p[0], Delta = C.p[2];
+  R l, n, sd;
+  kks_coords(C, Rc, z, l, n, sd);
+  R dVdl = R(0.5) / sqrt(l) / pow(sqrt(l) + sqrt(n), R(2.));
+  R dVdn = R(0.5) / sqrt(n) / pow(sqrt(l) + sqrt(n), R(2.));
+  R dldR = Rc * (R(1.) + (Rc * Rc + z * z + R(Delta * Delta)) / sd);
+  R dndR = Rc * (R(1.) - (Rc * Rc + z * z + R(Delta * Delta)) / sd);
+  R dldz = z * (R(1.) + (Rc * Rc + z * z - R(Delta * Delta)) / sd);
+  R dndz = z * (R(1.) - (Rc * Rc + z * z - R(Delta * Delta)) / sd);
+  FR = -R(amp) * (dldR * dVdl + dndR * dVdn);
+  Fz = -R(amp) * (dldz * dVdl + dndz * dVdn);
+}
+template <class R> R kks_eval(const Component& C, R Rc, R z) {
+  R l, n, sd;
+  kks_coords(C, Rc, z, l, n, sd);
+  n = (n > R(0.)) ? n : R(0.);
+  return -R(C.p[0]) / (sqrt(l) + sqrt(n));
+}
+template <class R> R kks_planar_rforce(const Component& C, R Rc) {
+  const double amp = C.p[0], ac = C.p[1], Delta = C.p[2];
+  const double gamma = Delta * Delta / (1. - ac * ac);
+  const double alpha = gamma - Delta * Delta;
+  R l = Rc * Rc - R(alpha);
+  R n = R(-gamma);
+  R dVdl = R(0.5) / sqrt(l) / pow(sqrt(l) + sqrt(n), R(2.));
+  return -R(amp) * (R(2.) * Rc) * dVdl;
+}
+
 // Aggregated cylindrical forces: galpy_potentials.c:120-179 sums each of
 // Rforce, phitorque, zforce over components in index order starting from 0.
 // Summing the three per component gives the identical three sums.
@@ -611,6 +657,7 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kLogHalo: lh_forces(C, Rc, z, phi, fr, fz, fp); break;
       case kDehnenBar: bar_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kKuzminDisk: kuzmin_forces(C, Rc, z, fr, fz); break;
+      case kKKStaeckel: kks_forces(C, Rc, z, fr, fz); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Rforce, zforce
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.) {
@@ -873,6 +920,7 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
         break;
       }
       case kNull: break;
+      case kKKStaeckel: fr = kks_planar_rforce(C, Rc); break;
       case kKuzminDisk:  // KuzminDiskPotential.c planarRforce
         fr = -R(C.p[0]) * Rc * pow(Rc * Rc + R(C.p[1] * C.p[1]), R(-1.5));
         break;
@@ -921,7 +969,8 @@ void planar_rect_deriv(int npot, const Component* P, R t, const R* q, R* a) {
 inline bool has_hessian(int npot, const Component* P) {
   for (int i = 0; i < npot; i++) {
     const int c = P[i].type;
-    if (c == kDehnenBar || (c == kLogHalo && P[i].p[3] < 1) || is_wrapper(c)) return false;
+    if (c == kDehnenBar || c == kKKStaeckel || (c == kLogHalo && P[i].p[3] < 1) || is_wrapper(c))
+      return false;
   }
   return true;
 }
@@ -1011,6 +1060,7 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kLogHalo: pot += lh_eval(C, Rc, z, phi); break;
       case kDehnenBar: pot += bar_eval(C, Rc, z, phi, t); break;
       case kKuzminDisk: pot += kuzmin_eval(C, Rc, z); break;
+      case kKKStaeckel: pot += kks_eval(C, Rc, z); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Eval
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.)

@@ -360,3 +360,46 @@ def test_mn3_exponential_disk_orbits():
     ref, rerr, _, _ = oracle_integrate(ppt, ppa, yp, t, 6, planar=True)
     assert numpy.all(err == 0) and numpy.all(rerr == 0)
     assert phase_diff(res, ref) < 1e-8
+
+
+def test_kuzmin_kutuzov_staeckel():
+    """KuzminKutuzovStaeckelPotential (pot_type 16): the device's forces and
+    potential against the oracle's, orbits (full, planar) at the fixed-step
+    and DOP853 bars; integrate_dxdv refuses it (no device Hessian)."""
+    import torch
+
+    from galpy_amd import _lib
+    from galpy_amd import potential as P
+    from galpy_amd.adjoint import _RawTable
+    from galpy_amd.integrate import eval_device, integrateFullOrbit_dxdv_hip, integrateFullOrbit_hip
+    from galpy_amd.integrate import integratePlanarOrbit_hip
+
+    pot = [P.KuzminKutuzovStaeckelPotential(ac=5.0, Delta=1.0, normalize=0.7),
+           P.NFWPotential(a=2.0, normalize=0.3)]
+    pt, pa = _packed(pot)
+    rng = numpy.random.default_rng(9)
+    R, z = rng.uniform(0.05, 6.0, 2000), rng.normal(0.0, 0.8, 2000)
+    tab = _RawTable(pt, pa)
+    dev = [torch.tensor(v, device="cuda") for v in (R, z, numpy.zeros_like(R), numpy.zeros_like(R))]
+    pot_d, FR, Fz, _ = (o.cpu().numpy() for o in eval_device(tab, *dev))
+    oFR, oFz, _ = oracle_forces(pt, pa, R, z)
+    for got, want in ((FR, oFR), (Fz, oFz), (pot_d, oracle_potential(pt, pa, R, z))):
+        assert numpy.abs(got - want).max() <= 1e-13 * numpy.abs(want).max()
+    del tab
+    t = numpy.linspace(0.0, 10.0, 101)
+    yo = ics(128, seed=25)
+    for method, code, tol in (("symplec4_hip", 3, 1e-10), ("dop853_hip", 6, 1e-8)):
+        res, err = integrateFullOrbit_hip(pot, yo.copy(), t, method)
+        ref, rerr, _, _ = oracle_integrate(pt, pa, yo, t, code)
+        assert numpy.all(err == 0) and numpy.all(rerr == 0)
+        assert phase_diff(res, ref) < tol, method
+    yp = planar_ics(128, seed=26)
+    ppt, ppa = _packed(pot, True)
+    res, err = integratePlanarOrbit_hip(pot, yp.copy(), t, "dop853_hip")
+    ref, rerr, _, _ = oracle_integrate(ppt, ppa, yp, t, 6, planar=True)
+    assert numpy.all(err == 0) and numpy.all(rerr == 0)
+    assert phase_diff(res, ref) < 1e-8
+    y0 = cyl_to_rect(ics(2, seed=27))
+    with pytest.raises(RuntimeError):
+        integrateFullOrbit_dxdv_hip(pot, y0, numpy.ones_like(y0), numpy.linspace(0, 1, 3), "symplec4_hip")
+    assert "Hessian" in _lib.last_error()
