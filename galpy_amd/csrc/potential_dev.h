@@ -46,6 +46,7 @@ enum GhPotCode {
   GH_HOMOGSPHERE = 35,
   GH_FLATPOWER = 12,
   GH_KKSTAECKEL = 16,
+  GH_SOFTNEEDLEBAR = 25,
   GH_NULLPOT = 40,
   // wrappers (integrateFullOrbit.c:821-895)
   GH_DEHNENSMOOTHWRAP = -1,
@@ -526,6 +527,41 @@ __device__ __forceinline__ void gh_comp_force<GH_KUZMINDISK>(const GhPotTable* _
   F.Fz += -amp * w * D32;
 }
 
+// SoftenedNeedleBarPotential.c:5-12, 30-63 (Long & Murali 1992): in the frame
+// rotated by pa + omegab t (x_b = R cos(phi - pa - omegab t)), with T+- =
+// sqrt((a +- x_b)^2 + y_b^2 + (b + sqrt(z^2 + c^2))^2), Phi = amp/(2a)
+// log((x_b - a + T-) / (x_b + a + T+)); the bar-frame force is rotated back
+// and added as F_R/R and F_phi/R^2 of the inertial (x, y).  p: amp, a, b,
+// c^2, pa, omegab (the reference's seven cache slots are not read).
+__device__ __forceinline__ void gh_needle_frame(const GhComponent* __restrict__ C, double x,
+                                                double y, double t, double& cp, double& sp,
+                                                double& xb, double& yb) {
+  sincos(C->p[4] + C->p[5] * t, &sp, &cp);
+  xb = cp * x + sp * y;
+  yb = cp * y - sp * x;
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_SOFTNEEDLEBAR>(const GhPotTable* __restrict__ T,
+                                                                int i, const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0], a = C->p[1], b = C->p[2];
+  double cp, sp, xb, yb;
+  gh_needle_frame(C, g.x, g.y, g.t, cp, sp, xb, yb);
+  const double zc = sqrt(g.z2 + C->p[3]);
+  const double bz = b + zc;
+  const double second = yb * yb + bz * bz;
+  const double Tp = sqrt((a + xb) * (a + xb) + second), Tm = sqrt((a - xb) * (a - xb) + second);
+  const double S = Tp + Tm, iTT = 1.0 / (Tp * Tm);
+  const double Fx = -2.0 * xb * iTT / S;
+  const double common = 0.5 * iTT * (S - 4.0 * xb * xb / S) / second;
+  const double Fy = -yb * common;
+  const double Ax = amp * (cp * Fx - sp * Fy), Ay = amp * (sp * Fx + cp * Fy);
+  const double iR2 = gh_rcp(g.R2);
+  F.FRoR += (g.x * Ax + g.y * Ay) * iR2;
+  F.FphioR2 += (g.x * Ay - g.y * Ax) * iR2;
+  F.Fz += -amp * g.z * common * bz / zc;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_KKSTAECKEL>(const GhPotTable* __restrict__ T,
                                                              int i, const GhGeom& g, GhAcc& F) {
@@ -585,6 +621,7 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
     case GH_KUZMINDISK: gh_comp_force<GH_KUZMINDISK>(T, i, g, F); break;
     case GH_FLATPOWER: gh_comp_force<GH_FLATPOWER>(T, i, g, F); break;
     case GH_KKSTAECKEL: gh_comp_force<GH_KKSTAECKEL>(T, i, g, F); break;
+    case GH_SOFTNEEDLEBAR: gh_comp_force<GH_SOFTNEEDLEBAR>(T, i, g, F); break;
     case GH_BURKERT: gh_comp_force<GH_BURKERT>(T, i, g, F); break;
     case GH_DEHNENCORESPH: gh_comp_force<GH_DEHNENCORESPH>(T, i, g, F); break;
     case GH_DEHNENSPH: gh_comp_force<GH_DEHNENSPH>(T, i, g, F); break;
@@ -844,6 +881,15 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
         const double alpha = C->p[1], m2 = C->p[3] + R2 + z2 * C->c[0];
         pot += (alpha == 0.0) ? 0.5 * C->p[0] * log(m2)
                               : -C->p[0] * pow(m2, -0.5 * alpha) / alpha;
+        break;
+      }
+      case GH_SOFTNEEDLEBAR: {  // SoftenedNeedleBarPotential.c:13-29
+        double cp, sp, xb, yb;
+        gh_needle_frame(C, R * cosphi, R * sinphi, t, cp, sp, xb, yb);
+        const double a = C->p[1], bz = C->p[2] + sqrt(z2 + C->p[3]);
+        const double second = yb * yb + bz * bz;
+        const double Tp = sqrt((a + xb) * (a + xb) + second), Tm = sqrt((a - xb) * (a - xb) + second);
+        pot += 0.5 * C->p[0] * log((xb - a + Tm) / (xb + a + Tp)) / a;
         break;
       }
       case GH_KKSTAECKEL: {  // KuzminKutuzovStaeckelPotential.c:5-23 (n clamped at 0)

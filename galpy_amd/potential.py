@@ -300,6 +300,61 @@ class KuzminKutuzovStaeckelPotential(Potential):
         return [self._amp, self._ac, self._Delta]
 
 
+class SoftenedNeedleBarPotential(Potential):
+    """pot_type 25, args [amp, a, b, c^2, pa, omegab] and seven cache slots
+    (SoftenedNeedleBarPotential.py:30-200, integrateFullOrbit.py:303-306):
+    Long & Murali's softened needle of half-length a, rotating at omegab from
+    angle pa, Phi = amp/(2a) log((x - a + T-) / (x + a + T+)) in the bar
+    frame, T+- = sqrt((a +- x)^2 + y^2 + (b + sqrt(z^2 + c^2))^2)."""
+
+    pot_code = 25
+
+    def __init__(self, amp=1.0, a=4.0, b=0.0, c=1.0, normalize=False, pa=0.4, omegab=1.8):
+        Potential.__init__(self, amp=amp)
+        self._a = a
+        self._b = b
+        self._c2 = c**2.0
+        self._pa = pa
+        self._omegab = omegab
+        if normalize or (isinstance(normalize, (int, float)) and not isinstance(normalize, bool)):
+            self.normalize(normalize)
+
+    def _frame(self, R, z, phi, t):
+        ang = phi - self._pa - self._omegab * t
+        x, y = R * numpy.cos(ang), R * numpy.sin(ang)
+        zc = numpy.sqrt(self._c2 + z**2.0)
+        second = y**2.0 + (self._b + zc) ** 2.0
+        Tp = numpy.sqrt((self._a + x) ** 2.0 + second)
+        Tm = numpy.sqrt((self._a - x) ** 2.0 + second)
+        return x, y, zc, second, Tp, Tm
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        x, _, _, _, Tp, Tm = self._frame(R, z, phi, t)
+        return numpy.log((x - self._a + Tm) / (x + self._a + Tp)) / 2.0 / self._a
+
+    def _xyz(self, R, z, phi, t):
+        x, y, zc, second, Tp, Tm = self._frame(R, z, phi, t)
+        S = Tp + Tm
+        Fx = -2.0 * x / Tp / Tm / S
+        common = 0.5 / Tp / Tm * (S - 4.0 * x**2.0 / S) / second
+        cp, sp = numpy.cos(self._pa + self._omegab * t), numpy.sin(self._pa + self._omegab * t)
+        return cp * Fx + sp * y * common, sp * Fx - cp * y * common, -z * common * (self._b + zc) / zc
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        Fx, Fy, _ = self._xyz(R, z, phi, t)
+        return numpy.cos(phi) * Fx + numpy.sin(phi) * Fy
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        Fx, Fy, _ = self._xyz(R, z, phi, t)
+        return R * (-numpy.sin(phi) * Fx + numpy.cos(phi) * Fy)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return self._xyz(R, z, phi, t)[2]
+
+    def c_args(self):
+        return [self._amp, self._a, self._b, self._c2, self._pa, self._omegab] + [0.0] * 7
+
+
 class NFWPotential(Potential):
     """pot_type 9, args [amp, a]."""
 

@@ -18,7 +18,9 @@
  *   18 PseudoIsothermalPotential, 19 KuzminDiskPotential, 20 BurkertPotential,
  *   33 DehnenCoreSphericalPotential, 34 DehnenSphericalPotential,
  *   35 HomogeneousSpherePotential, 12 FlattenedPowerPotential, 40 NullPotential,
- *   16 KuzminKutuzovStaeckelPotential (no integrate_dxdv: no device Hessian);
+ *   16 KuzminKutuzovStaeckelPotential, 25 SoftenedNeedleBarPotential (13
+ *   arguments, the last 7 galpy's cache slots, unread); neither takes
+ *   integrate_dxdv (no device Hessian);
  * and the wrappers (integrateFullOrbit.c:821-895, packed as galpy packs them:
  * the wrapper's code, then the wrapped components' codes; as arguments
  * nwrapped, the wrapped components' arguments, then the wrapper's own)

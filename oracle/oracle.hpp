@@ -152,7 +152,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kNFW = 9, kJaffe = 10, kIsochrone = 14, kPowerSphCutoff = 15, kPlummer = 17,
                kPseudoIso = 18, kKuzminDisk = 19, kBurkert = 20, kDehnenCoreSph = 33,
                kDehnenSph = 34, kHomogSphere = 35, kFlattenedPower = 12, kNull = 40,
-               kKKStaeckel = 16,
+               kKKStaeckel = 16, kSoftNeedleBar = 25,
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5 };
@@ -192,6 +192,7 @@ inline int component_nargs(int code, bool planar = false) {
     case kFlattenedPower: return planar ? 3 : 4;  // planar: amp, alpha, core^2
     case kNull: return 0;
     case kKKStaeckel: return 3;  // amp, ac, Delta
+    case kSoftNeedleBar: return 13;  // amp, a, b, c2, pa, omegab + 7 cache slots
     case kDehnenSph: case kHomogSphere: return 3;
     // wrappers' own arguments, read after the wrapped components
     case kDehnenSmoothWrap: return 4;   // amp, tform, tsteady, grow
@@ -634,6 +635,36 @@ template <class R> R kks_planar_rforce(const Component& C, R Rc) {
   return -R(amp) * (R(2.) * Rc) * dVdl;
 }
 
+// SoftenedNeedleBarPotential.c: compute_TpTm (:5-12), Eval (:13-29), the
+// aligned-frame forces rotated back (:30-63), Rforce (:64-85), phitorque
+// (:90-...) and zforce; the planar versions are these at z = 0.
+template <class R> void needle_forces(const Component& C, R Rc, R z, R phi, R t, R& FR, R& Fz,
+                                      R& Fphi) {
+  const double amp = C.p[0], a = C.p[1], b = C.p[2], c2 = C.p[3], pa = C.p[4], om = C.p[5];
+  R x = Rc * cos(phi - R(pa) - R(om) * t), y = Rc * sin(phi - R(pa) - R(om) * t);
+  R secondpart = y * y + pow(R(b) + sqrt(z * z + R(c2)), R(2.));
+  R Tp = sqrt(pow(R(a) + x, R(2.)) + secondpart);
+  R Tm = sqrt(pow(R(a) - x, R(2.)) + secondpart);
+  R zc = sqrt(z * z + R(c2));
+  R Fx = R(-2.) * x / Tp / Tm / (Tp + Tm);
+  R Fy = -y / R(2.) / Tp / Tm * (Tp + Tm - R(4.) * x * x / (Tp + Tm)) / (y * y + pow(R(b) + zc, R(2.)));
+  R Fzz = -z / R(2.) / Tp / Tm * (Tp + Tm - R(4.) * x * x / (Tp + Tm)) /
+          (y * y + pow(R(b) + zc, R(2.))) * (R(b) + zc) / zc;
+  R cp = cos(R(pa) + R(om) * t), sp = sin(R(pa) + R(om) * t);
+  R Fxi = cp * Fx - sp * Fy, Fyi = sp * Fx + cp * Fy;
+  FR = R(amp) * (cos(phi) * Fxi + sin(phi) * Fyi);
+  Fphi = R(amp) * Rc * (-sin(phi) * Fxi + cos(phi) * Fyi);
+  Fz = R(amp) * Fzz;
+}
+template <class R> R needle_eval(const Component& C, R Rc, R z, R phi, R t) {
+  const double amp = C.p[0], a = C.p[1], b = C.p[2], c2 = C.p[3], pa = C.p[4], om = C.p[5];
+  R x = Rc * cos(phi - R(pa) - R(om) * t), y = Rc * sin(phi - R(pa) - R(om) * t);
+  R secondpart = y * y + pow(R(b) + sqrt(z * z + R(c2)), R(2.));
+  R Tp = sqrt(pow(R(a) + x, R(2.)) + secondpart);
+  R Tm = sqrt(pow(R(a) - x, R(2.)) + secondpart);
+  return R(0.5) * R(amp) * log((x - R(a) + Tm) / (x + R(a) + Tp)) / R(a);
+}
+
 // Aggregated cylindrical forces: galpy_potentials.c:120-179 sums each of
 // Rforce, phitorque, zforce over components in index order starting from 0.
 // Summing the three per component gives the identical three sums.
@@ -658,6 +689,7 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kDehnenBar: bar_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kKuzminDisk: kuzmin_forces(C, Rc, z, fr, fz); break;
       case kKKStaeckel: kks_forces(C, Rc, z, fr, fz); break;
+      case kSoftNeedleBar: needle_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Rforce, zforce
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.) {
@@ -921,6 +953,11 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
       }
       case kNull: break;
       case kKKStaeckel: fr = kks_planar_rforce(C, Rc); break;
+      case kSoftNeedleBar: {
+        R fz0;
+        needle_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
+        break;
+      }
       case kKuzminDisk:  // KuzminDiskPotential.c planarRforce
         fr = -R(C.p[0]) * Rc * pow(Rc * Rc + R(C.p[1] * C.p[1]), R(-1.5));
         break;
@@ -969,7 +1006,7 @@ void planar_rect_deriv(int npot, const Component* P, R t, const R* q, R* a) {
 inline bool has_hessian(int npot, const Component* P) {
   for (int i = 0; i < npot; i++) {
     const int c = P[i].type;
-    if (c == kDehnenBar || c == kKKStaeckel || (c == kLogHalo && P[i].p[3] < 1) || is_wrapper(c))
+    if (c == kDehnenBar || c == kKKStaeckel || c == kSoftNeedleBar || (c == kLogHalo && P[i].p[3] < 1) || is_wrapper(c))
       return false;
   }
   return true;
@@ -1061,6 +1098,7 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kDehnenBar: pot += bar_eval(C, Rc, z, phi, t); break;
       case kKuzminDisk: pot += kuzmin_eval(C, Rc, z); break;
       case kKKStaeckel: pot += kks_eval(C, Rc, z); break;
+      case kSoftNeedleBar: pot += needle_eval(C, Rc, z, phi, t); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Eval
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.)

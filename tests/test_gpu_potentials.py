@@ -403,3 +403,50 @@ def test_kuzmin_kutuzov_staeckel():
     with pytest.raises(RuntimeError):
         integrateFullOrbit_dxdv_hip(pot, y0, numpy.ones_like(y0), numpy.linspace(0, 1, 3), "symplec4_hip")
     assert "Hessian" in _lib.last_error()
+
+
+def test_softened_needle_bar():
+    """SoftenedNeedleBarPotential (pot_type 25, rotating): the device's
+    forces, torque and potential at t != 0 against the oracle's; orbits with
+    a halo (full: symplec4 and DOP853; planar: DOP853), each orbit held to
+    its bar or to 10x the oracle's own one-ulp sensitivity; integrate_dxdv
+    refuses it (non-axisymmetric)."""
+    import torch
+
+    from galpy_amd import _lib
+    from galpy_amd import potential as P
+    from galpy_amd.adjoint import _RawTable
+    from galpy_amd.integrate import eval_device, integrateFullOrbit_dxdv_hip, integrateFullOrbit_hip
+    from galpy_amd.integrate import integratePlanarOrbit_hip
+
+    pot = [P.LogarithmicHaloPotential(normalize=1.0),
+           P.SoftenedNeedleBarPotential(amp=0.05, a=1.5, b=0.1, c=0.5, pa=0.3, omegab=1.6)]
+    pt, pa = _packed(pot)
+    rng = numpy.random.default_rng(10)
+    n = 2000
+    R, z = rng.uniform(0.05, 6.0, n), rng.normal(0.0, 0.8, n)
+    phi, t = rng.uniform(-numpy.pi, numpy.pi, n), rng.uniform(0.0, 5.0, n)
+    tab = _RawTable(pt, pa)
+    dev = [torch.tensor(v, device="cuda") for v in (R, z, phi, t)]
+    pot_d, FR, Fz, Fp = (o.cpu().numpy() for o in eval_device(tab, *dev))
+    oFR, oFz, oFp = oracle_forces(pt, pa, R, z, phi, t)
+    for got, want in ((FR, oFR), (Fz, oFz), (Fp, oFp), (pot_d, oracle_potential(pt, pa, R, z, phi, t))):
+        assert numpy.abs(got - want).max() <= 1e-13 * numpy.abs(want).max()
+    del tab
+    tt = numpy.linspace(0.0, 10.0, 101)
+    yo = ics(128, seed=28)
+    for method, code, bar in (("symplec4_hip", 3, 1e-10), ("dop853_hip", 6, 1e-8)):
+        res, err = integrateFullOrbit_hip(pot, yo.copy(), tt, method)
+        ref, sens = ulp_sensitivity(lambda y: oracle_integrate(pt, pa, y, tt, code)[0], yo)
+        assert numpy.all(err == 0)
+        assert_parity(res, ref, bar, sens, factor=10.0 if code == 6 else 1e3, label=method)
+    yp = planar_ics(128, seed=29)
+    ppt, ppa = _packed(pot, True)
+    res, err = integratePlanarOrbit_hip(pot, yp.copy(), tt, "dop853_hip")
+    ref, sens = ulp_sensitivity(lambda y: oracle_integrate(ppt, ppa, y, tt, 6, planar=True)[0], yp)
+    assert numpy.all(err == 0)
+    assert_parity(res, ref, 1e-8, sens, factor=10.0, label="planar")
+    y0 = cyl_to_rect(ics(2, seed=30))
+    with pytest.raises(RuntimeError):
+        integrateFullOrbit_dxdv_hip(pot, y0, numpy.ones_like(y0), numpy.linspace(0, 1, 3), "dop853_hip")
+    assert "Hessian" in _lib.last_error()
