@@ -46,6 +46,7 @@ enum GhPotCode {
   GH_HOMOGSPHERE = 35,
   GH_FLATPOWER = 12,
   GH_KKSTAECKEL = 16,
+  GH_ELLIPDISK = 4,  // planar only (integratePlanarOrbit's code 4)
   GH_SOFTNEEDLEBAR = 25,
   GH_NULLPOT = 40,
   // wrappers (integrateFullOrbit.c:821-895)
@@ -562,6 +563,36 @@ __device__ __forceinline__ void gh_comp_force<GH_SOFTNEEDLEBAR>(const GhPotTable
   F.Fz += -amp * g.z * common * bz / zc;
 }
 
+// EllipticalDiskPotential.c:4-52 (planar only): Phi = amp smooth(t)
+// twophio/2 R^p cos(2 (phi - phib)), smooth Dehnen's growth polynomial
+// between tform and tsteady (1 when tform is NaN, i.e. None in galpy).
+// p: amp (divided by r1^p in galpy), tform, tsteady, twophio, p, phib;
+// c[0] = cos(2 phib), c[1] = sin(2 phib).
+__device__ __forceinline__ double gh_ellipdisk_smooth(double t, double tform, double tsteady) {
+  if (tform != tform) return 1.0;
+  if (t < tform) return 0.0;
+  if (t < tsteady) {
+    const double xi = 2.0 * (t - tform) / (tsteady - tform) - 1.0;
+    const double xi3 = xi * xi * xi;
+    return 3.0 / 16.0 * xi3 * xi * xi - 5.0 / 8.0 * xi3 + 15.0 / 16.0 * xi + 0.5;
+  }
+  return 1.0;
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_ELLIPDISK>(const GhPotTable* __restrict__ T, int i,
+                                                            const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double as = C->p[0] * gh_ellipdisk_smooth(g.t, C->p[1], C->p[2]);
+  const double p = C->p[4];
+  const double iR2 = gh_rcp(g.R2);
+  const double cos2p = (g.x * g.x - g.y * g.y) * iR2, sin2p = 2.0 * g.x * g.y * iR2;
+  const double cosd = cos2p * C->c[0] + sin2p * C->c[1];  // cos(2 (phi - phib))
+  const double sind = sin2p * C->c[0] - cos2p * C->c[1];  // sin(2 (phi - phib))
+  const double Rpm2 = pow(g.R2, 0.5 * p - 1.0);           // R^(p-2)
+  F.FRoR += -as * p * C->p[3] * 0.5 * Rpm2 * cosd;
+  F.FphioR2 += as * C->p[3] * Rpm2 * sind;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_KKSTAECKEL>(const GhPotTable* __restrict__ T,
                                                              int i, const GhGeom& g, GhAcc& F) {
@@ -621,6 +652,7 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
     case GH_KUZMINDISK: gh_comp_force<GH_KUZMINDISK>(T, i, g, F); break;
     case GH_FLATPOWER: gh_comp_force<GH_FLATPOWER>(T, i, g, F); break;
     case GH_KKSTAECKEL: gh_comp_force<GH_KKSTAECKEL>(T, i, g, F); break;
+    case GH_ELLIPDISK: gh_comp_force<GH_ELLIPDISK>(T, i, g, F); break;
     case GH_SOFTNEEDLEBAR: gh_comp_force<GH_SOFTNEEDLEBAR>(T, i, g, F); break;
     case GH_BURKERT: gh_comp_force<GH_BURKERT>(T, i, g, F); break;
     case GH_DEHNENCORESPH: gh_comp_force<GH_DEHNENCORESPH>(T, i, g, F); break;
@@ -890,6 +922,12 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
         const double second = yb * yb + bz * bz;
         const double Tp = sqrt((a + xb) * (a + xb) + second), Tm = sqrt((a - xb) * (a - xb) + second);
         pot += 0.5 * C->p[0] * log((xb - a + Tm) / (xb + a + Tp)) / a;
+        break;
+      }
+      case GH_ELLIPDISK: {  // EllipticalDiskPotential.py _evaluate (planar only)
+        const double c2 = cosphi * cosphi - sinphi * sinphi, s2 = 2.0 * sinphi * cosphi;
+        pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] * 0.5 *
+               pow(R2, 0.5 * C->p[4]) * (c2 * C->c[0] + s2 * C->c[1]);
         break;
       }
       case GH_KKSTAECKEL: {  // KuzminKutuzovStaeckelPotential.c:5-23 (n clamped at 0)

@@ -355,6 +355,61 @@ class SoftenedNeedleBarPotential(Potential):
         return [self._amp, self._a, self._b, self._c2, self._pa, self._omegab] + [0.0] * 7
 
 
+class EllipticalDiskPotential(Potential):
+    """Planar only: integratePlanarOrbit's pot_type 4, args [amp, tform,
+    tsteady, twophio, p, phib] with tform = tsteady = NaN when tform is None
+    (EllipticalDiskPotential.py:24-150, integratePlanarOrbit.py:127-135):
+    Phi = amp smooth(t) twophio/2 (R/r1)^p cos(2 (phi - phib)), smooth
+    Dehnen's growth polynomial from tform to tsteady (tform + 2 unless
+    given, as tform + tsteady)."""
+
+    pot_code = 4
+
+    def __init__(self, amp=1.0, phib=25.0 * numpy.pi / 180.0, p=1.0, twophio=0.01, r1=1.0,
+                 tform=None, tsteady=None):
+        Potential.__init__(self, amp=amp)
+        self._amp /= r1**p
+        self._phib = phib
+        self._twophio = twophio
+        self._p = p
+        self._tform = tform
+        if tform is None:
+            self._tsteady = None
+        elif tsteady is None:
+            self._tsteady = tform + 2.0
+        else:
+            self._tsteady = tform + tsteady
+
+    def _smooth(self, t):
+        if self._tform is None:
+            return 1.0
+        t = numpy.asarray(t, dtype=float)
+        xi = 2.0 * (t - self._tform) / (self._tsteady - self._tform) - 1.0
+        mid = 3.0 / 16.0 * xi**5.0 - 5.0 / 8.0 * xi**3.0 + 15.0 / 16.0 * xi + 0.5
+        return numpy.where(t < self._tform, 0.0, numpy.where(t < self._tsteady, mid, 1.0))
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return self._smooth(t) * self._twophio / 2.0 * R**self._p * numpy.cos(2.0 * (phi - self._phib))
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return (-self._smooth(t) * self._p * self._twophio / 2.0 * R ** (self._p - 1.0)
+                * numpy.cos(2.0 * (phi - self._phib)))
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * R
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return self._smooth(t) * self._twophio * R**self._p * numpy.sin(2.0 * (phi - self._phib))
+
+    def c_args(self):
+        raise NotImplementedError("EllipticalDiskPotential is a planar potential: integrate planar orbits")
+
+    def c_args_planar(self):
+        tf = float("nan") if self._tform is None else self._tform
+        ts = float("nan") if self._tform is None else self._tsteady
+        return [self._amp, tf, ts, self._twophio, self._p, self._phib]
+
+
 class NFWPotential(Potential):
     """pot_type 9, args [amp, a]."""
 

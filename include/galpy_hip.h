@@ -20,7 +20,8 @@
  *   35 HomogeneousSpherePotential, 12 FlattenedPowerPotential, 40 NullPotential,
  *   16 KuzminKutuzovStaeckelPotential, 25 SoftenedNeedleBarPotential (13
  *   arguments, the last 7 galpy's cache slots, unread); neither takes
- *   integrate_dxdv (no device Hessian);
+ *   integrate_dxdv (no device Hessian); for planar orbits only, 4
+ *   EllipticalDiskPotential (integratePlanarOrbit.c's code 4);
  * and the wrappers (integrateFullOrbit.c:821-895, packed as galpy packs them:
  * the wrapper's code, then the wrapped components' codes; as arguments
  * nwrapped, the wrapped components' arguments, then the wrapper's own)

@@ -152,7 +152,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kNFW = 9, kJaffe = 10, kIsochrone = 14, kPowerSphCutoff = 15, kPlummer = 17,
                kPseudoIso = 18, kKuzminDisk = 19, kBurkert = 20, kDehnenCoreSph = 33,
                kDehnenSph = 34, kHomogSphere = 35, kFlattenedPower = 12, kNull = 40,
-               kKKStaeckel = 16, kSoftNeedleBar = 25,
+               kKKStaeckel = 16, kSoftNeedleBar = 25, kEllipDisk = 4,  // 4: planar only
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5 };
@@ -192,6 +192,7 @@ inline int component_nargs(int code, bool planar = false) {
     case kFlattenedPower: return planar ? 3 : 4;  // planar: amp, alpha, core^2
     case kNull: return 0;
     case kKKStaeckel: return 3;  // amp, ac, Delta
+    case kEllipDisk: return planar ? 6 : -1;  // planar only
     case kSoftNeedleBar: return 13;  // amp, a, b, c2, pa, omegab + 7 cache slots
     case kDehnenSph: case kHomogSphere: return 3;
     // wrappers' own arguments, read after the wrapped components
@@ -665,6 +666,27 @@ template <class R> R needle_eval(const Component& C, R Rc, R z, R phi, R t) {
   return R(0.5) * R(amp) * log((x - R(a) + Tm) / (x + R(a) + Tp)) / R(a);
 }
 
+// EllipticalDiskPotential.c (planar only): EllipticalDiskSmooth (:4-20),
+// Rforce (:21-35), phitorque (:36-50).
+template <class R> R ellipdisk_smooth(R t, double tform, double tsteady) {
+  R smooth;
+  if (!std::isnan(tform)) {
+    if (t < R(tform)) smooth = 0.;
+    else if (t < R(tsteady)) {
+      R deltat = t - R(tform);
+      R xi = R(2.) * deltat / R(tsteady - tform) - R(1.);
+      smooth = (R(3. / 16.) * pow(xi, R(5.)) - R(5. / 8.) * pow(xi, R(3.)) + R(15. / 16.) * xi + R(.5));
+    } else smooth = 1.;
+  } else smooth = 1.;
+  return smooth;
+}
+template <class R> void ellipdisk_planar_forces(const Component& C, R Rc, R phi, R t, R& FR, R& Fphi) {
+  const double amp = C.p[0], twophio = C.p[3], p = C.p[4], phib = C.p[5];
+  R smooth = ellipdisk_smooth(t, C.p[1], C.p[2]);
+  FR = -R(amp) * smooth * R(p) * R(twophio) / R(2.) * pow(Rc, R(p - 1.)) * cos(R(2.) * (phi - R(phib)));
+  Fphi = R(amp) * smooth * R(twophio) * pow(Rc, R(p)) * sin(R(2.) * (phi - R(phib)));
+}
+
 // Aggregated cylindrical forces: galpy_potentials.c:120-179 sums each of
 // Rforce, phitorque, zforce over components in index order starting from 0.
 // Summing the three per component gives the identical three sums.
@@ -953,6 +975,7 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
       }
       case kNull: break;
       case kKKStaeckel: fr = kks_planar_rforce(C, Rc); break;
+      case kEllipDisk: ellipdisk_planar_forces(C, Rc, phi, t, fr, fp); break;
       case kSoftNeedleBar: {
         R fz0;
         needle_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);

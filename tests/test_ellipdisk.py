@@ -1,0 +1,42 @@
+"""EllipticalDiskPotential (planar only, integratePlanarOrbit's pot_type 4):
+the oracle's planar force and torque are mpmath derivatives of
+Phi = amp smooth(t) twophio/2 (R/r1)^p cos(2 (phi - phib)); Dehnen's growth
+polynomial is 0 before tform, 1/2 half-way, 1 after tsteady, with zero slope
+at both ends; the full-orbit packing refuses it, as galpy has no 3D
+version."""
+
+import numpy
+import pytest
+
+from helpers import oracle_planar_forces
+
+
+def test_ellipdisk_against_mpmath_and_smooth():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 30
+    from galpy_amd.potential import EllipticalDiskPotential, parse_pot, parse_pot_planar
+
+    pot = EllipticalDiskPotential(amp=1.3, phib=0.4, p=0.7, twophio=0.05, r1=1.2, tform=1.0, tsteady=3.0)
+    _, pt, pa, _ = parse_pot_planar([pot])
+    assert list(pt) == [4] and len(pa) == 6
+    with pytest.raises(NotImplementedError):
+        parse_pot([pot])
+    rng = numpy.random.default_rng(4)
+    R, phi = rng.uniform(0.1, 4.0, 20), rng.uniform(-3.0, 3.0, 20)
+    for t, sm in ((0.5, 0.0), (2.5, 0.5), (5.0, 1.0)):
+        FR, Fp = oracle_planar_forces(pt, pa, R, phi, numpy.full(20, t))
+        for i in range(20):
+            f = lambda r, ph: (1.3 / mpmath.mpf(1.2) ** 0.7 * sm * mpmath.mpf(0.05) / 2
+                               * mpmath.mpf(r) ** 0.7 * mpmath.cos(2 * (ph - mpmath.mpf(0.4))))
+            wR = -float(mpmath.diff(lambda r: f(r, phi[i]), R[i]))
+            wp = -float(mpmath.diff(lambda ph: f(R[i], ph), phi[i]))
+            assert abs(FR[i] - wR) <= 1e-13 * max(abs(wR), 1e-6) + 1e-18
+            assert abs(Fp[i] - wp) <= 1e-13 * max(abs(wp), 1e-6) + 1e-18
+        assert numpy.allclose(pot.Rforce(R, 0.0, phi, t), FR, rtol=1e-13, atol=1e-18)
+        assert numpy.allclose(pot.phitorque(R, 0.0, phi, t), Fp, rtol=1e-13, atol=1e-18)
+    # zero slope of the growth at tform = 1 and tform + tsteady = 4 (galpy's
+    # tsteady is the duration of the growth)
+    one = lambda t: oracle_planar_forces(pt, pa, numpy.array([1.0]), numpy.array([0.0]), numpy.array([t]))[0][0]
+    full = one(10.0)
+    for t0 in (1.0, 4.0):
+        assert abs(one(t0 + 1e-4) - one(t0 - 1e-4)) < 1e-7 * abs(full)

@@ -450,3 +450,22 @@ def test_softened_needle_bar():
     with pytest.raises(RuntimeError):
         integrateFullOrbit_dxdv_hip(pot, y0, numpy.ones_like(y0), numpy.linspace(0, 1, 3), "dop853_hip")
     assert "Hessian" in _lib.last_error()
+
+
+def test_elliptical_disk_planar_orbits():
+    """EllipticalDiskPotential (planar only, code 4) with a halo, growing
+    between tform and tsteady: planar orbits, symplec4 and DOP853, each
+    orbit held to its bar or to 10x the oracle's own one-ulp sensitivity."""
+    from galpy_amd import potential as P
+    from galpy_amd.integrate import integratePlanarOrbit_hip
+
+    pot = [P.LogarithmicHaloPotential(normalize=1.0),
+           P.EllipticalDiskPotential(twophio=0.05, p=0.5, phib=0.3, tform=1.0, tsteady=2.0)]
+    ppt, ppa = _packed(pot, True)
+    t = numpy.linspace(0.0, 10.0, 101)
+    yp = planar_ics(128, seed=31)
+    for method, code, bar, factor in (("symplec4_hip", 3, 1e-10, 1e3), ("dop853_hip", 6, 1e-8, 10.0)):
+        res, err = integratePlanarOrbit_hip(pot, yp.copy(), t, method)
+        ref, sens = ulp_sensitivity(lambda y: oracle_integrate(ppt, ppa, y, t, code, planar=True)[0], yp)
+        assert numpy.all(err == 0)
+        assert_parity(res, ref, bar, sens, factor=factor, label=method)
