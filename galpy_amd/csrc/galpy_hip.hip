@@ -91,6 +91,7 @@ int nargs_of(int code, bool planar) {
     case GH_NULLPOT: return 0;
     case GH_KKSTAECKEL: return 3;  // amp, ac, Delta
     case GH_ELLIPDISK: return planar ? 6 : -1;  // planar only
+    case GH_LOPSIDED: return planar ? 4 : -1;   // planar only
     case GH_SOFTNEEDLEBAR: return 13;  // amp, a, b, c^2, pa, omegab + 7 cache slots
     // wrappers' own arguments, read after the wrapped components
     // (integrateFullOrbit.c:821-895, integratePlanarOrbit.c:591-663)
@@ -269,6 +270,11 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
       case GH_PLUMMER: C.c[0] = C.p[1] * C.p[1]; break;  // b^2
       case GH_FLATPOWER: C.c[0] = 1.0 / C.p[2]; break;   // 1/q^2
       case GH_SOFTNEEDLEBAR: C.nonaxi = 1; break;
+      case GH_LOPSIDED:
+        C.c[0] = cos(C.p[3]);
+        C.c[1] = sin(C.p[3]);
+        C.nonaxi = 1;
+        break;
       case GH_ELLIPDISK:
         C.c[0] = cos(2.0 * C.p[5]);
         C.c[1] = sin(2.0 * C.p[5]);

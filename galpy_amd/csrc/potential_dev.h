@@ -47,6 +47,7 @@ enum GhPotCode {
   GH_FLATPOWER = 12,
   GH_KKSTAECKEL = 16,
   GH_ELLIPDISK = 4,  // planar only (integratePlanarOrbit's code 4)
+  GH_LOPSIDED = 6,   // planar only (integratePlanarOrbit's code 6)
   GH_SOFTNEEDLEBAR = 25,
   GH_NULLPOT = 40,
   // wrappers (integrateFullOrbit.c:821-895)
@@ -593,6 +594,20 @@ __device__ __forceinline__ void gh_comp_force<GH_ELLIPDISK>(const GhPotTable* __
   F.FphioR2 += as * C->p[3] * Rpm2 * sind;
 }
 
+// LopsidedDiskPotential.c (planar only): Phi = amp phio R^p cos(phi - phib);
+// p: amp, phio, p, phib; c[0] = cos(phib), c[1] = sin(phib).
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_LOPSIDED>(const GhPotTable* __restrict__ T, int i,
+                                                           const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double ap = C->p[0] * C->p[1], p = C->p[2];
+  const double Rcos = g.x * C->c[0] + g.y * C->c[1];  // R cos(phi - phib)
+  const double Rsin = g.y * C->c[0] - g.x * C->c[1];  // R sin(phi - phib)
+  const double Rpm3 = pow(g.R2, 0.5 * p - 1.5);      // R^(p-3)
+  F.FRoR += -ap * p * Rpm3 * Rcos;
+  F.FphioR2 += ap * Rpm3 * Rsin;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_KKSTAECKEL>(const GhPotTable* __restrict__ T,
                                                              int i, const GhGeom& g, GhAcc& F) {
@@ -653,6 +668,7 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
     case GH_FLATPOWER: gh_comp_force<GH_FLATPOWER>(T, i, g, F); break;
     case GH_KKSTAECKEL: gh_comp_force<GH_KKSTAECKEL>(T, i, g, F); break;
     case GH_ELLIPDISK: gh_comp_force<GH_ELLIPDISK>(T, i, g, F); break;
+    case GH_LOPSIDED: gh_comp_force<GH_LOPSIDED>(T, i, g, F); break;
     case GH_SOFTNEEDLEBAR: gh_comp_force<GH_SOFTNEEDLEBAR>(T, i, g, F); break;
     case GH_BURKERT: gh_comp_force<GH_BURKERT>(T, i, g, F); break;
     case GH_DEHNENCORESPH: gh_comp_force<GH_DEHNENCORESPH>(T, i, g, F); break;
@@ -924,6 +940,9 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
         pot += 0.5 * C->p[0] * log((xb - a + Tm) / (xb + a + Tp)) / a;
         break;
       }
+      case GH_LOPSIDED:  // CosmphiDiskPotential.py _evaluate at m = 1 (planar only)
+        pot += C->p[0] * C->p[1] * pow(R2, 0.5 * C->p[2]) * (cosphi * C->c[0] + sinphi * C->c[1]);
+        break;
       case GH_ELLIPDISK: {  // EllipticalDiskPotential.py _evaluate (planar only)
         const double c2 = cosphi * cosphi - sinphi * sinphi, s2 = 2.0 * sinphi * cosphi;
         pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] * 0.5 *

@@ -410,6 +410,40 @@ class EllipticalDiskPotential(Potential):
         return [self._amp, tf, ts, self._twophio, self._p, self._phib]
 
 
+class LopsidedDiskPotential(Potential):
+    """Planar only: integratePlanarOrbit's pot_type 6, args [amp, phio, p,
+    phib] (CosmphiDiskPotential.py:220-240 with m = 1,
+    integratePlanarOrbit.py:142-144): Phi = amp phio R^p cos(phi - phib).
+    As in galpy, r1 is accepted and not used (LopsidedDiskPotential does not
+    pass it on)."""
+
+    pot_code = 6
+
+    def __init__(self, amp=1.0, phib=25.0 * numpy.pi / 180.0, p=1.0, phio=0.01, r1=1.0):
+        Potential.__init__(self, amp=amp)
+        self._phib = phib
+        self._mphio = phio
+        self._p = p
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return self._mphio * R**self._p * numpy.cos(phi - self._phib)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return -self._p * self._mphio * R ** (self._p - 1.0) * numpy.cos(phi - self._phib)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * R
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return self._mphio * R**self._p * numpy.sin(phi - self._phib)
+
+    def c_args(self):
+        raise NotImplementedError("LopsidedDiskPotential is a planar potential: integrate planar orbits")
+
+    def c_args_planar(self):
+        return [self._amp, self._mphio, self._p, self._phib]
+
+
 class NFWPotential(Potential):
     """pot_type 9, args [amp, a]."""
 

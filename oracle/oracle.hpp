@@ -152,7 +152,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kNFW = 9, kJaffe = 10, kIsochrone = 14, kPowerSphCutoff = 15, kPlummer = 17,
                kPseudoIso = 18, kKuzminDisk = 19, kBurkert = 20, kDehnenCoreSph = 33,
                kDehnenSph = 34, kHomogSphere = 35, kFlattenedPower = 12, kNull = 40,
-               kKKStaeckel = 16, kSoftNeedleBar = 25, kEllipDisk = 4,  // 4: planar only
+               kKKStaeckel = 16, kSoftNeedleBar = 25, kEllipDisk = 4, kLopsided = 6,  // 4, 6: planar only
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5 };
@@ -193,6 +193,7 @@ inline int component_nargs(int code, bool planar = false) {
     case kNull: return 0;
     case kKKStaeckel: return 3;  // amp, ac, Delta
     case kEllipDisk: return planar ? 6 : -1;  // planar only
+    case kLopsided: return planar ? 4 : -1;   // planar only
     case kSoftNeedleBar: return 13;  // amp, a, b, c2, pa, omegab + 7 cache slots
     case kDehnenSph: case kHomogSphere: return 3;
     // wrappers' own arguments, read after the wrapped components
@@ -976,6 +977,12 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
       case kNull: break;
       case kKKStaeckel: fr = kks_planar_rforce(C, Rc); break;
       case kEllipDisk: ellipdisk_planar_forces(C, Rc, phi, t, fr, fp); break;
+      case kLopsided: {  // LopsidedDiskPotential.c Rforce, phitorque (amp, phio, p, phib)
+        const double amp = C.p[0], phio = C.p[1], pp = C.p[2], phib = C.p[3];
+        fr = -R(amp) * R(pp) * R(phio) * pow(Rc, R(pp - 1.)) * cos(phi - R(phib));
+        fp = R(amp) * R(phio) * pow(Rc, R(pp)) * sin(phi - R(phib));
+        break;
+      }
       case kSoftNeedleBar: {
         R fz0;
         needle_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);

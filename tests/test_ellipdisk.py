@@ -1,4 +1,5 @@
-"""EllipticalDiskPotential (planar only, integratePlanarOrbit's pot_type 4):
+"""EllipticalDiskPotential and LopsidedDiskPotential (planar only,
+integratePlanarOrbit's pot_type 4 and 6).  EllipticalDisk:
 the oracle's planar force and torque are mpmath derivatives of
 Phi = amp smooth(t) twophio/2 (R/r1)^p cos(2 (phi - phib)); Dehnen's growth
 polynomial is 0 before tform, 1/2 half-way, 1 after tsteady, with zero slope
@@ -40,3 +41,26 @@ def test_ellipdisk_against_mpmath_and_smooth():
     full = one(10.0)
     for t0 in (1.0, 4.0):
         assert abs(one(t0 + 1e-4) - one(t0 - 1e-4)) < 1e-7 * abs(full)
+
+
+def test_lopsided_against_mpmath():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 30
+    from galpy_amd.potential import LopsidedDiskPotential, parse_pot, parse_pot_planar
+
+    pot = LopsidedDiskPotential(amp=1.1, phib=0.6, p=-0.3, phio=0.02, r1=3.0)
+    _, pt, pa, _ = parse_pot_planar([pot])
+    assert list(pt) == [6] and list(pa) == [1.1, 0.02, -0.3, 0.6]  # r1 unused, as in galpy
+    with pytest.raises(NotImplementedError):
+        parse_pot([pot])
+    rng = numpy.random.default_rng(6)
+    R, phi = rng.uniform(0.1, 4.0, 20), rng.uniform(-3.0, 3.0, 20)
+    FR, Fp = oracle_planar_forces(pt, pa, R, phi, numpy.zeros(20))
+    f = lambda r, ph: mpmath.mpf(1.1) * mpmath.mpf(0.02) * mpmath.mpf(r) ** mpmath.mpf(-0.3) * mpmath.cos(ph - mpmath.mpf(0.6))
+    for i in range(20):
+        wR = -float(mpmath.diff(lambda r: f(r, phi[i]), R[i]))
+        wp = -float(mpmath.diff(lambda ph: f(R[i], ph), phi[i]))
+        assert abs(FR[i] - wR) <= 1e-13 * max(abs(wR), 1e-6)
+        assert abs(Fp[i] - wp) <= 1e-13 * max(abs(wp), 1e-6)
+    assert numpy.allclose(pot.Rforce(R, 0.0, phi), FR, rtol=1e-13, atol=0)
+    assert numpy.allclose(pot.phitorque(R, 0.0, phi), Fp, rtol=1e-13, atol=0)

@@ -453,14 +453,16 @@ def test_softened_needle_bar():
 
 
 def test_elliptical_disk_planar_orbits():
-    """EllipticalDiskPotential (planar only, code 4) with a halo, growing
-    between tform and tsteady: planar orbits, symplec4 and DOP853, each
+    """EllipticalDiskPotential (planar only, code 4, growing between tform
+    and tform + tsteady) and LopsidedDiskPotential (code 6) with a halo:
+    planar orbits, symplec4 and DOP853, each
     orbit held to its bar or to 10x the oracle's own one-ulp sensitivity."""
     from galpy_amd import potential as P
     from galpy_amd.integrate import integratePlanarOrbit_hip
 
     pot = [P.LogarithmicHaloPotential(normalize=1.0),
-           P.EllipticalDiskPotential(twophio=0.05, p=0.5, phib=0.3, tform=1.0, tsteady=2.0)]
+           P.EllipticalDiskPotential(twophio=0.05, p=0.5, phib=0.3, tform=1.0, tsteady=2.0),
+           P.LopsidedDiskPotential(phio=0.02, p=-0.2, phib=1.1)]
     ppt, ppa = _packed(pot, True)
     t = numpy.linspace(0.0, 10.0, 101)
     yp = planar_ics(128, seed=31)
