@@ -41,7 +41,7 @@ _SIGS = {
 MAX_POT = 8
 NPAR = 3
 # packed arguments per component in galpy's order (integrateFullOrbit.py:65-365)
-_NARGS = {0: 4, 1: 6, 4: 6, 5: 3, 6: 4, 16: 3, 25: 13, 7: 2, 8: 2, 9: 2, 10: 2, 14: 2, 15: 5, 17: 2, 18: 2, 19: 2,
+_NARGS = {0: 4, 1: 6, 3: 8, 4: 6, 5: 3, 6: 4, 16: 3, 25: 13, 7: 2, 8: 2, 9: 2, 10: 2, 14: 2, 15: 5, 17: 2, 18: 2, 19: 2,
           20: 2, 33: 2, 34: 3, 35: 3, 12: 4, 40: 0}
 # the packed arguments the adjoint kernel accumulates dL/dtheta for:
 # MiyamotoNagai (amp, a, b), NFW and Hernquist (amp, a), every other

@@ -92,6 +92,7 @@ int nargs_of(int code, bool planar) {
     case GH_KKSTAECKEL: return 3;  // amp, ac, Delta
     case GH_ELLIPDISK: return planar ? 6 : -1;  // planar only
     case GH_LOPSIDED: return planar ? 4 : -1;   // planar only
+    case GH_STEADYSPIRAL: return planar ? 8 : -1;  // planar only
     case GH_SOFTNEEDLEBAR: return 13;  // amp, a, b, c^2, pa, omegab + 7 cache slots
     // wrappers' own arguments, read after the wrapped components
     // (integrateFullOrbit.c:821-895, integratePlanarOrbit.c:591-663)
@@ -270,6 +271,7 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
       case GH_PLUMMER: C.c[0] = C.p[1] * C.p[1]; break;  // b^2
       case GH_FLATPOWER: C.c[0] = 1.0 / C.p[2]; break;   // 1/q^2
       case GH_SOFTNEEDLEBAR: C.nonaxi = 1; break;
+      case GH_STEADYSPIRAL: C.nonaxi = 1; break;
       case GH_LOPSIDED:
         C.c[0] = cos(C.p[3]);
         C.c[1] = sin(C.p[3]);

@@ -48,6 +48,7 @@ enum GhPotCode {
   GH_KKSTAECKEL = 16,
   GH_ELLIPDISK = 4,  // planar only (integratePlanarOrbit's code 4)
   GH_LOPSIDED = 6,   // planar only (integratePlanarOrbit's code 6)
+  GH_STEADYSPIRAL = 3,  // planar only (integratePlanarOrbit's code 3)
   GH_SOFTNEEDLEBAR = 25,
   GH_NULLPOT = 40,
   // wrappers (integrateFullOrbit.c:821-895)
@@ -608,6 +609,24 @@ __device__ __forceinline__ void gh_comp_force<GH_LOPSIDED>(const GhPotTable* __r
   F.FphioR2 += ap * Rpm3 * Rsin;
 }
 
+// SteadyLogSpiralPotential.c (planar only): Phi = amp smooth(t) A/alpha
+// cos(alpha ln R - m (phi - omegas t - gamma)), smooth the growth polynomial
+// of gh_ellipdisk_smooth; p: amp, tform, tsteady, A, alpha, m, omegas, gamma.
+__device__ __forceinline__ double gh_spiral_arg(const GhComponent* __restrict__ C, double R2,
+                                                double phi, double t) {
+  return C->p[4] * 0.5 * log(R2) - C->p[5] * (phi - C->p[6] * t - C->p[7]);
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_STEADYSPIRAL>(const GhPotTable* __restrict__ T,
+                                                               int i, const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double aA = C->p[0] * gh_ellipdisk_smooth(g.t, C->p[1], C->p[2]) * C->p[3];
+  const double sn = sin(gh_spiral_arg(C, g.R2, atan2(g.y, g.x), g.t));
+  const double iR2 = gh_rcp(g.R2);
+  F.FRoR += aA * sn * iR2;
+  F.FphioR2 += -aA / C->p[4] * C->p[5] * sn * iR2;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_KKSTAECKEL>(const GhPotTable* __restrict__ T,
                                                              int i, const GhGeom& g, GhAcc& F) {
@@ -669,6 +688,7 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
     case GH_KKSTAECKEL: gh_comp_force<GH_KKSTAECKEL>(T, i, g, F); break;
     case GH_ELLIPDISK: gh_comp_force<GH_ELLIPDISK>(T, i, g, F); break;
     case GH_LOPSIDED: gh_comp_force<GH_LOPSIDED>(T, i, g, F); break;
+    case GH_STEADYSPIRAL: gh_comp_force<GH_STEADYSPIRAL>(T, i, g, F); break;
     case GH_SOFTNEEDLEBAR: gh_comp_force<GH_SOFTNEEDLEBAR>(T, i, g, F); break;
     case GH_BURKERT: gh_comp_force<GH_BURKERT>(T, i, g, F); break;
     case GH_DEHNENCORESPH: gh_comp_force<GH_DEHNENCORESPH>(T, i, g, F); break;
@@ -940,6 +960,10 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
         pot += 0.5 * C->p[0] * log((xb - a + Tm) / (xb + a + Tp)) / a;
         break;
       }
+      case GH_STEADYSPIRAL:  // SteadyLogSpiralPotential.py _evaluate (planar only)
+        pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
+               cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
+        break;
       case GH_LOPSIDED:  // CosmphiDiskPotential.py _evaluate at m = 1 (planar only)
         pot += C->p[0] * C->p[1] * pow(R2, 0.5 * C->p[2]) * (cosphi * C->c[0] + sinphi * C->c[1]);
         break;

@@ -444,6 +444,64 @@ class LopsidedDiskPotential(Potential):
         return [self._amp, self._mphio, self._p, self._phib]
 
 
+class SteadyLogSpiralPotential(Potential):
+    """Planar only: integratePlanarOrbit's pot_type 3, args [amp, tform,
+    tsteady, A, alpha, m, omegas, gamma] (NaN times when tform is None;
+    SteadyLogSpiralPotential.py:20-150, integratePlanarOrbit.py:99-126):
+    Phi = amp smooth(t) A/alpha cos(alpha ln R - m (phi - omegas t - gamma)),
+    alpha = m / tan(p) when the pitch angle p is given; tform and tsteady in
+    units of the pattern period 2 pi / omegas (tsteady: duration, default 2)."""
+
+    pot_code = 3
+
+    def __init__(self, amp=1.0, omegas=0.65, A=-0.035, alpha=-7.0, m=2, gamma=numpy.pi / 4.0, p=None,
+                 tform=None, tsteady=None):
+        Potential.__init__(self, amp=amp)
+        self._omegas = omegas
+        self._A = A
+        self._m = m
+        self._gamma = gamma
+        self._alpha = self._m / numpy.tan(p) if p is not None else alpha
+        ts = 2.0 * numpy.pi / self._omegas if self._omegas != 0.0 else 1.0
+        self._tform = None if tform is None else tform * ts
+        if self._tform is None:
+            self._tsteady = None
+        else:
+            self._tsteady = self._tform + (2.0 if tsteady is None else tsteady) * ts
+
+    def _smooth(self, t):
+        if self._tform is None:
+            return 1.0
+        t = numpy.asarray(t, dtype=float)
+        xi = 2.0 * (t - self._tform) / (self._tsteady - self._tform) - 1.0
+        mid = 3.0 / 16.0 * xi**5.0 - 5.0 / 8.0 * xi**3.0 + 15.0 / 16.0 * xi + 0.5
+        return numpy.where(t < self._tform, 0.0, numpy.where(t < self._tsteady, mid, 1.0))
+
+    def _arg(self, R, phi, t):
+        return self._alpha * numpy.log(R) - self._m * (phi - self._omegas * t - self._gamma)
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return self._smooth(t) * self._A / self._alpha * numpy.cos(self._arg(R, phi, t))
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return self._smooth(t) * self._A / R * numpy.sin(self._arg(R, phi, t))
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * R
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return -self._smooth(t) * self._A / self._alpha * self._m * numpy.sin(self._arg(R, phi, t))
+
+    def c_args(self):
+        raise NotImplementedError("SteadyLogSpiralPotential is a planar potential: integrate planar orbits")
+
+    def c_args_planar(self):
+        nan = float("nan")
+        tf = nan if self._tform is None else self._tform
+        ts = nan if self._tform is None else self._tsteady
+        return [self._amp, tf, ts, self._A, self._alpha, float(self._m), self._omegas, self._gamma]
+
+
 class NFWPotential(Potential):
     """pot_type 9, args [amp, a]."""
 

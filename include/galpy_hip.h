@@ -21,8 +21,8 @@
  *   16 KuzminKutuzovStaeckelPotential, 25 SoftenedNeedleBarPotential (13
  *   arguments, the last 7 galpy's cache slots, unread); neither takes
  *   integrate_dxdv (no device Hessian); for planar orbits only, 4
- *   EllipticalDiskPotential and 6 LopsidedDiskPotential (integratePlanarOrbit.c's
- *   codes);
+ *   EllipticalDiskPotential, 6 LopsidedDiskPotential and 3
+ *   SteadyLogSpiralPotential (integratePlanarOrbit.c's codes);
  * and the wrappers (integrateFullOrbit.c:821-895, packed as galpy packs them:
  * the wrapper's code, then the wrapped components' codes; as arguments
  * nwrapped, the wrapped components' arguments, then the wrapper's own)

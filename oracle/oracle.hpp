@@ -152,7 +152,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kNFW = 9, kJaffe = 10, kIsochrone = 14, kPowerSphCutoff = 15, kPlummer = 17,
                kPseudoIso = 18, kKuzminDisk = 19, kBurkert = 20, kDehnenCoreSph = 33,
                kDehnenSph = 34, kHomogSphere = 35, kFlattenedPower = 12, kNull = 40,
-               kKKStaeckel = 16, kSoftNeedleBar = 25, kEllipDisk = 4, kLopsided = 6,  // 4, 6: planar only
+               kKKStaeckel = 16, kSoftNeedleBar = 25, kEllipDisk = 4, kLopsided = 6, kSteadySpiral = 3,  // 3, 4, 6: planar only
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5 };
@@ -194,6 +194,7 @@ inline int component_nargs(int code, bool planar = false) {
     case kKKStaeckel: return 3;  // amp, ac, Delta
     case kEllipDisk: return planar ? 6 : -1;  // planar only
     case kLopsided: return planar ? 4 : -1;   // planar only
+    case kSteadySpiral: return planar ? 8 : -1;  // planar only
     case kSoftNeedleBar: return 13;  // amp, a, b, c2, pa, omegab + 7 cache slots
     case kDehnenSph: case kHomogSphere: return 3;
     // wrappers' own arguments, read after the wrapped components
@@ -977,6 +978,14 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
       case kNull: break;
       case kKKStaeckel: fr = kks_planar_rforce(C, Rc); break;
       case kEllipDisk: ellipdisk_planar_forces(C, Rc, phi, t, fr, fp); break;
+      case kSteadySpiral: {  // SteadyLogSpiralPotential.c Rforce, phitorque
+        const double amp = C.p[0], A = C.p[3], alpha = C.p[4], m = C.p[5], om = C.p[6], gam = C.p[7];
+        R smooth = ellipdisk_smooth(t, C.p[1], C.p[2]);  // dehnenSpiralSmooth: the same polynomial
+        fr = R(amp) * smooth * R(A) / Rc * sin(R(alpha) * log(Rc) - R(m) * (phi - R(om) * t - R(gam)));
+        fp = -R(amp) * smooth * R(A) / R(alpha) * R(m) *
+             sin(R(alpha) * log(Rc) - R(m) * (phi - R(om) * t - R(gam)));
+        break;
+      }
       case kLopsided: {  // LopsidedDiskPotential.c Rforce, phitorque (amp, phio, p, phib)
         const double amp = C.p[0], phio = C.p[1], pp = C.p[2], phib = C.p[3];
         fr = -R(amp) * R(pp) * R(phio) * pow(Rc, R(pp - 1.)) * cos(phi - R(phib));

@@ -454,7 +454,8 @@ def test_softened_needle_bar():
 
 def test_elliptical_disk_planar_orbits():
     """EllipticalDiskPotential (planar only, code 4, growing between tform
-    and tform + tsteady) and LopsidedDiskPotential (code 6) with a halo:
+    and tform + tsteady), LopsidedDiskPotential (code 6) and
+    SteadyLogSpiralPotential (code 3, growing) with a halo:
     planar orbits, symplec4 and DOP853, each
     orbit held to its bar or to 10x the oracle's own one-ulp sensitivity."""
     from galpy_amd import potential as P
@@ -462,7 +463,8 @@ def test_elliptical_disk_planar_orbits():
 
     pot = [P.LogarithmicHaloPotential(normalize=1.0),
            P.EllipticalDiskPotential(twophio=0.05, p=0.5, phib=0.3, tform=1.0, tsteady=2.0),
-           P.LopsidedDiskPotential(phio=0.02, p=-0.2, phib=1.1)]
+           P.LopsidedDiskPotential(phio=0.02, p=-0.2, phib=1.1),
+           P.SteadyLogSpiralPotential(A=-0.02, tform=0.2, tsteady=0.5)]
     ppt, ppa = _packed(pot, True)
     t = numpy.linspace(0.0, 10.0, 101)
     yp = planar_ics(128, seed=31)
