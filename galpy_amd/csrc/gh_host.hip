@@ -1,0 +1,937 @@
+// MI355X orbit-integration engine: host unit.  Decodes galpy's packed
+// potential into the device table, chooses the kernel instance for a
+// composition, drives host-buffer calls through the GPU, and exports the
+// C-ABI.  The C-ABI mirrors galpy's integrateFullOrbit
+// (galpy/orbit/orbit_c_ext/integrateFullOrbit.c:1046-1150): cylindrical
+// initial conditions in, cylindrical rows out, one call for all orbits.
+// Declarations and argument meaning: include/galpy_hip.h.
+#include <hip/hip_runtime.h>
+#include <signal.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "gh_internal.h"
+#include "dxdv_dev.h"
+
+namespace gh {
+
+thread_local char g_errbuf[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_errbuf, sizeof(g_errbuf), fmt, ap);
+  va_end(ap);
+}
+
+// ---------------------------------------------------------------------------
+// Interrupt flag: fine-grained pinned host memory that running kernels poll.
+// ---------------------------------------------------------------------------
+volatile int* g_irq_host = nullptr;
+static int* g_irq_dev = nullptr;
+
+int* irq_device_ptr() {
+  if (g_irq_dev) return g_irq_dev;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    return nullptr;
+  g_irq_host = static_cast<volatile int*>(p);
+  *g_irq_host = 0;
+  void* d = nullptr;
+  if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) return nullptr;
+  g_irq_dev = static_cast<int*>(d);
+  return g_irq_dev;
+}
+
+// Work counters for the persistent kernels: one small device array per
+// device, allocated on first use and kept for the life of the process; each
+// launch takes the next slot, so concurrent launches on different streams
+// never share a counter.
+constexpr int kCounterSlots = 4096;
+static unsigned long long* g_counters[64] = {nullptr};
+static unsigned g_counter_next = 0;
+
+unsigned long long* next_counter(int dev) {
+  if (dev < 0 || dev >= 64) return nullptr;
+  if (!g_counters[dev]) {
+    void* p = nullptr;
+    if (hipMalloc(&p, kCounterSlots * sizeof(unsigned long long)) != hipSuccess) return nullptr;
+    g_counters[dev] = static_cast<unsigned long long*>(p);
+  }
+  const unsigned slot = __atomic_fetch_add(&g_counter_next, 1u, __ATOMIC_RELAXED) % kCounterSlots;
+  return g_counters[dev] + slot;
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Host decode of galpy's packed potential into the device table.
+// ---------------------------------------------------------------------------
+// Arguments per component: integrateFullOrbit.c:80-345 for full orbits,
+// integratePlanarOrbit.c:64-223 for planar ones (whose
+// PowerSphericalPotentialwCutoff has no cache slots: 3 arguments, not 5).
+int nargs_of(int code, bool planar) {
+  switch (code) {
+    case GH_LOGHALO: return 4;
+    case GH_DEHNENBAR: return 6;
+    case GH_MIYAMOTONAGAI: return 3;
+    case GH_HERNQUIST: return 2;
+    case GH_NFW: return 2;
+    case GH_POWERSPHCUTOFF: return planar ? 3 : 5;
+    // integrateFullOrbit.c:135-603 / integratePlanarOrbit.c:136-418
+    case GH_POWERSPH: case GH_JAFFE: case GH_ISOCHRONE: case GH_PLUMMER: case GH_PSEUDOISO:
+    case GH_KUZMINDISK: case GH_BURKERT: case GH_DEHNENCORESPH: return 2;
+    case GH_DEHNENSPH: case GH_HOMOGSPHERE: return 3;
+    case GH_FLATPOWER: return planar ? 3 : 4;  // planar: amp, alpha, core^2
+    case GH_NULLPOT: return 0;
+    case GH_KKSTAECKEL: return 3;  // amp, ac, Delta
+    case GH_ELLIPDISK: return planar ? 6 : -1;  // planar only
+    case GH_LOPSIDED: return planar ? 4 : -1;   // planar only
+    case GH_STEADYSPIRAL: return planar ? 8 : -1;  // planar only
+    case GH_SOFTNEEDLEBAR: return 13;  // amp, a, b, c^2, pa, omegab + 7 cache slots
+    // wrappers' own arguments, read after the wrapped components
+    // (integrateFullOrbit.c:821-895, integratePlanarOrbit.c:591-663)
+    case GH_DEHNENSMOOTHWRAP: return 4;   // amp, tform, tsteady, grow
+    case GH_SOLIDBODYROTWRAP: return 3;   // amp, omega, pa
+    case GH_COROTATINGROTWRAP: return 5;  // amp, vpo, beta, pa, to
+    case GH_GAUSSIANAMPWRAP: return 3;    // amp, to, sigma^2
+    default: return -1;
+  }
+}
+
+// smallest x >= a+1 with 2 e^-x x^(a-1) < 2^-60 Gamma(a): beyond it the upper
+// incomplete gamma is below 1/128 ulp of Gamma(a)
+double incgamma_xbig(double a, double gamma_a) {
+  double x = std::max(a + 1.0, 2.0 * (a - 1.0) + 1.0);
+  const double lim = std::ldexp(gamma_a, -60);
+  while (2.0 * std::exp(-x + (a - 1.0) * std::log(x)) >= lim) x += 0.25;
+  return x;
+}
+
+int fill_series(double a, double* ser) {
+  // ser[n] = 1/(a (a+1) ... (a+n)); the last term at x = a+1 must be negligible
+  double v = 1.0 / a;
+  for (int n = 0; n < GH_IG_NSER; n++) {
+    ser[n] = v;
+    v /= (a + n + 1.0);
+  }
+  const double last = ser[GH_IG_NSER - 1] * std::pow(a + 1.0, GH_IG_NSER - 1);
+  return (last < 1e-18 * ser[0]) ? 0 : -1;
+}
+
+// Upper incomplete gamma Gamma(a,x), x >= a+1, in long double: Legendre's
+// continued fraction by the modified Lentz method (A&S 6.5.31).  Used only on
+// the host to fit the device tables.
+long double upper_gamma_ld(long double a, long double x) {
+  const long double tiny = 1e-300L;
+  long double b = x + 1.0L - a, c = 1.0L / tiny, d = 1.0L / b, h = d;
+  for (int i = 1; i < 100000; i++) {
+    const long double an = -i * (i - a);
+    b += 2.0L;
+    d = an * d + b;
+    if (fabsl(d) < tiny) d = tiny;
+    c = b + an / c;
+    if (fabsl(c) < tiny) c = tiny;
+    d = 1.0L / d;
+    const long double del = d * c;
+    h *= del;
+    if (fabsl(del - 1.0L) < 1e-19L) break;
+  }
+  return expl(a * logl(x) - x) * h;
+}
+
+// Fit the piecewise polynomial table of Gamma(a,x) on the octaves covering
+// [a+1, xbig): GH_TAB_PER_OCT segments per octave, interpolation at the
+// GH_TAB_NC Chebyshev nodes of each segment, stored as monomial coefficients
+// in the local coordinate t in [-1, 1).  Returns the first octave's frexp
+// exponent, or a large value if the range needs more than GH_TAB_MAXSEG.
+int fit_upper_gamma_table(double a, double xbig, double tab[GH_TAB_MAXSEG][GH_TAB_NC]) {
+  int e0, e1;
+  std::frexp(a + 1.0, &e0);
+  std::frexp(xbig, &e1);
+  if ((e1 - e0 + 1) * GH_TAB_PER_OCT > GH_TAB_MAXSEG) return 1 << 20;
+  const int N = GH_TAB_NC;
+  for (int e = e0; e <= e1; e++) {
+    for (int k = 0; k < GH_TAB_PER_OCT; k++) {
+      long double f[GH_TAB_NC], cheb[GH_TAB_NC];
+      for (int j = 0; j < N; j++) {
+        const long double tj = cosl(M_PI * (j + 0.5L) / N);
+        const long double m = 0.5L + (k + (tj + 1.0L) / 2.0L) / (2.0L * GH_TAB_PER_OCT);
+        f[j] = upper_gamma_ld(a, ldexpl(m, e));
+      }
+      for (int n = 0; n < N; n++) {
+        long double acc = 0.0L;
+        for (int j = 0; j < N; j++) acc += f[j] * cosl(M_PI * n * (j + 0.5L) / N);
+        cheb[n] = acc * 2.0L / N;
+      }
+      cheb[0] *= 0.5L;
+      // Chebyshev -> monomial: T_{n+1} = 2 t T_n - T_{n-1}
+      long double Tm1[GH_TAB_NC] = {0}, T0[GH_TAB_NC] = {0}, mono[GH_TAB_NC] = {0};
+      T0[0] = 1.0L;
+      for (int n = 0; n < N; n++) {
+        for (int q = 0; q < N; q++) mono[q] += cheb[n] * T0[q];
+        long double Tn1[GH_TAB_NC] = {0};
+        for (int q = 0; q < N; q++) {
+          if (q + 1 < N) Tn1[q + 1] += 2.0L * T0[q];
+          Tn1[q] -= (n == 0) ? 0.0L : Tm1[q];
+        }
+        if (n == 0) {  // T_1 = t
+          for (int q = 0; q < N; q++) Tn1[q] = 0.0L;
+          Tn1[1] = 1.0L;
+        }
+        for (int q = 0; q < N; q++) {
+          Tm1[q] = T0[q];
+          T0[q] = Tn1[q];
+        }
+      }
+      const int seg = (e - e0) * GH_TAB_PER_OCT + k;
+      for (int q = 0; q < N; q++) tab[seg][q] = (double)mono[q];
+    }
+  }
+  return e0;
+}
+
+// One (leaf) component's arguments and precomputed constants.
+int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
+                     bool planar) {
+  const int na = nargs_of(code, planar);
+  if (na < 0 || code < 0) {
+    set_error("pot_type %d (component %d) has no HIP kernel", code, i);
+    return -1;
+  }
+  {
+    GhComponent& C = T->comp[i];
+    C.type = code;
+    for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
+    if (code == GH_FLATPOWER && planar) {  // full layout, q^2 = 1 (z = 0 in the plane)
+      C.p[3] = C.p[2];
+      C.p[2] = 1.0;
+    }
+    switch (code) {
+      case GH_POWERSPHCUTOFF: {
+        const double amp = C.p[0], alpha = C.p[1], rc = C.p[2];
+        const double a = 1.5 - 0.5 * alpha, a2 = 1.0 - 0.5 * alpha;
+        if (!(a > 0.0)) {
+          set_error("PowerSphericalPotentialwCutoff alpha=%g needs alpha < 3", alpha);
+          return -1;
+        }
+        C.c[0] = a;
+        C.c[1] = std::tgamma(a);
+        C.c[2] = amp * 2.0 * M_PI * std::pow(rc, 3.0 - alpha);
+        C.c[3] = 1.0 / (rc * rc);
+        C.c[4] = incgamma_xbig(a, C.c[1]);
+        C.c[9] = std::ldexp(C.c[1], -58);  // continued-fraction stopping tolerance
+        if (fill_series(a, T->ser[i][0]) != 0) {
+          set_error("incomplete-gamma series too short for a=%g", a);
+          return -1;
+        }
+        if (a2 > 0.0) {
+          C.c[5] = a2;
+          C.c[6] = std::tgamma(a2);
+          C.c[7] = incgamma_xbig(a2, C.c[6]);
+          C.c[10] = std::ldexp(C.c[6], -58);
+          fill_series(a2, T->ser[i][1]);
+        } else {  // galpy's Eval calls gsl_sf_gamma(1 - alpha/2): undefined here
+          C.c[5] = a2;
+          C.c[6] = NAN;
+          C.c[7] = INFINITY;
+          for (int n = 0; n < GH_IG_NSER; n++) T->ser[i][1][n] = NAN;
+        }
+        C.c[8] = 1.0 / rc;
+        if (nslots >= GH_TAB_SLOTS) {
+          set_error("at most %d PowerSphericalPotentialwCutoff components per potential", GH_TAB_SLOTS);
+          return -1;
+        }
+        C.c[11] = nslots;
+        C.c[12] = fit_upper_gamma_table(a, C.c[4], T->tab[nslots]);
+        if (C.c[12] > 1000) {
+          set_error("incomplete-gamma table range too wide for a=%g", a);
+          return -1;
+        }
+        nslots++;
+        break;
+      }
+      case GH_MIYAMOTONAGAI: C.c[0] = C.p[2] * C.p[2]; break;
+      case GH_NFW: C.c[0] = 1.0 / C.p[1]; break;
+      case GH_HERNQUIST: C.c[0] = 0.5 * C.p[0]; break;
+      case GH_LOGHALO:
+        C.c[0] = 1.0 / (C.p[1] * C.p[1]);
+        C.nonaxi = C.p[3] < 1.0;
+        break;
+      case GH_DEHNENBAR:
+        C.c[0] = 1.0 / C.p[3];
+        C.nonaxi = 1;
+        break;
+      case GH_ISOCHRONE:
+      case GH_PLUMMER: C.c[0] = C.p[1] * C.p[1]; break;  // b^2
+      case GH_FLATPOWER: C.c[0] = 1.0 / C.p[2]; break;   // 1/q^2
+      case GH_SOFTNEEDLEBAR: C.nonaxi = 1; break;
+      case GH_STEADYSPIRAL: C.nonaxi = 1; break;
+      case GH_LOPSIDED:
+        C.c[0] = cos(C.p[3]);
+        C.c[1] = sin(C.p[3]);
+        C.nonaxi = 1;
+        break;
+      case GH_ELLIPDISK:
+        C.c[0] = cos(2.0 * C.p[5]);
+        C.c[1] = sin(2.0 * C.p[5]);
+        C.nonaxi = 1;
+        break;
+      case GH_KKSTAECKEL:  // gamma, alpha, Delta^2
+        C.c[2] = C.p[2] * C.p[2];
+        C.c[0] = C.c[2] / (1.0 - C.p[1] * C.p[1]);
+        C.c[1] = C.c[0] - C.c[2];
+        break;
+      case GH_PSEUDOISO:
+      case GH_BURKERT: C.c[0] = 1.0 / C.p[1]; break;  // 1/a
+      case GH_DEHNENSPH:
+        if (!(C.p[2] >= 0.0 && C.p[2] < 3.0)) {
+          set_error("DehnenSphericalPotential alpha=%g outside [0, 3)", C.p[2]);
+          return -1;
+        }
+        C.c[0] = 1.0 / (3.0 - C.p[2]);
+        break;
+      default: break;
+    }
+    T->nonaxi |= C.nonaxi;
+  }
+  return 0;
+}
+
+// n components in galpy's order (parse_leapFuncArgs_Full,
+// integrateFullOrbit.c:55-1045): a wrapper's code is followed by its wrapped
+// components' codes; its arguments are nwrapped, the wrapped components'
+// arguments, then its own (:1010-1031).  Leaves go to T->comp in the order
+// they appear (galpy's summation order); every leaf lists the wrappers
+// around it, outermost first.
+int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPotTable* T,
+                      int& nleaf, int& nslots, int depth, bool planar) {
+  for (int k = 0; k < n; k++) {
+    const int code = *pot_type++;
+    if (code >= 0) {
+      if (nleaf >= GH_MAX_POT) {
+        set_error("more than %d components (wrapped ones included)", GH_MAX_POT);
+        return -1;
+      }
+      if (decode_component(code, pot_args, T, nleaf, nslots, planar) != 0) return -1;
+      nleaf++;
+      continue;
+    }
+    const int na = nargs_of(code, planar);
+    if (na < 0) {
+      set_error("wrapper pot_type %d has no HIP kernel", code);
+      return -1;
+    }
+    if (depth >= GH_WRAP_DEPTH || T->nwraps >= GH_MAX_WRAP) {
+      set_error("wrappers nested more than %d deep or more than %d in total", GH_WRAP_DEPTH,
+                GH_MAX_WRAP);
+      return -1;
+    }
+    const double nwd = *pot_args++;
+    const int nw = (int)nwd;
+    if (!(nwd >= 1.0) || nw != nwd) {
+      set_error("wrapper pot_type %d wraps %g components", code, nwd);
+      return -1;
+    }
+    const int first = nleaf;
+    if (decode_components(nw, pot_type, pot_args, T, nleaf, nslots, depth + 1, planar) != 0)
+      return -1;
+    const int w = T->nwraps++;
+    GhWrap& W = T->wrap[w];
+    W.kind = code;
+    for (int j = 0; j < na; j++) W.p[j] = *pot_args++;
+    if (code == GH_GAUSSIANAMPWRAP && !(W.p[2] > 0.0)) {
+      set_error("GaussianAmplitudeWrapperPotential sigma^2=%g must be positive", W.p[2]);
+      return -1;
+    }
+    for (int l = first; l < nleaf; l++) {  // this wrapper encloses the ones already listed
+      GhComponent& C = T->comp[l];
+      for (int j = C.nwrap; j > 0; j--) C.wrap[j] = C.wrap[j - 1];
+      C.wrap[0] = w;
+      C.nwrap++;
+    }
+    T->has_wrap = 1;
+  }
+  return 0;
+}
+
+int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPotTable* T,
+                     bool planar = false) {
+  std::memset(T, 0, sizeof(*T));
+  if (npot < 1 || npot > GH_MAX_POT) {
+    set_error("npot=%d outside [1, %d]", npot, GH_MAX_POT);
+    return -1;
+  }
+  int nleaf = 0, nslots = 0;
+  const int* pt = pot_type;
+  const double* pa = pot_args;
+  if (decode_components(npot, pt, pa, T, nleaf, nslots, 0, planar) != 0) return -1;
+  T->npot = nleaf;
+  return 0;
+}
+// ---------------------------------------------------------------------------
+// Evaluation kernels along stored rows and at points (not integrators)
+// ---------------------------------------------------------------------------
+// Energy / Jacobi drift along stored cylindrical rows.
+template <int N>
+__global__ void __launch_bounds__(kBlock)
+gh_energy_kernel(int nobj, int nt, const double* __restrict__ rows, const double* __restrict__ t,
+                 int indiv_t, const GhPotTable* __restrict__ T, double omega,
+                 double* __restrict__ e0, double* __restrict__ demax) {
+  // N = 6: rows (R,vR,vT,z,vz,phi); N = 4: planar rows (R,vR,vT,phi), z = vz = 0
+  const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (orbit >= nobj) return;
+  const double* tt = t + (indiv_t ? orbit * nt : 0);
+  const double* r = rows + orbit * (long)nt * N;
+  double E0 = 0.0, dmax = 0.0;
+  for (int j = 0; j < nt; j++) {
+    const double R = r[(long)N * j + 0], vR = r[(long)N * j + 1], vT = r[(long)N * j + 2];
+    const double z = N == 6 ? r[(long)N * j + 3] : 0.0, vz = N == 6 ? r[(long)N * j + 4] : 0.0;
+    const double phi = r[(long)N * j + N - 1];
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    const double E = gh_potential(T, R, z, cp, sp, tt[j]) + 0.5 * (vR * vR + vT * vT + vz * vz) -
+                     omega * R * vT;
+    if (j == 0)
+      E0 = E;
+    else
+      dmax = fmax(dmax, fabs((E - E0) / E0));
+  }
+  e0[orbit] = E0;
+  demax[orbit] = dmax;
+}
+
+// Post-processing of stored trajectories (Orbits.py rperi :3733-3778,
+// rap :3686-3731, e :3642-3684, zmax :3966-4010, E :2902-3115, Jacobi
+// :3298-3360): r = sqrt(R^2 + z^2) over all outputs, rperi = min r,
+// rap = max r, e = (rap - rperi) / (rap + rperi), zmax = max |z|; E at every
+// output, and E - omega R vT.  One wavefront per orbit: lane k takes rows
+// k, k + 64, ..., so a wavefront reads 64 consecutive rows (64 N doubles) at
+// a time, coalesced, and E rows are written likewise; min/max/|z| reduce
+// across the wavefront by shuffles.  No FMA contraction in r, so rperi, rap
+// and zmax are exactly numpy's of the same rows.
+constexpr int kSummaryOrbitsPerBlock = 4;
+
+template <int N>
+__global__ void __launch_bounds__(64 * kSummaryOrbitsPerBlock)
+gh_summary_kernel(int nobj, int nt, const double* __restrict__ rows, const double* __restrict__ t,
+                  int indiv_t, const GhPotTable* __restrict__ T, double omega,
+                  double* __restrict__ rperi, double* __restrict__ rap, double* __restrict__ zmax,
+                  double* __restrict__ ecc, double* __restrict__ E, double* __restrict__ EJ) {
+#pragma clang fp contract(off)
+  const long orbit = (long)blockIdx.x * kSummaryOrbitsPerBlock + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (orbit >= nobj) return;  // uniform across the wavefront
+  const double* __restrict__ r = rows + orbit * (long)nt * N;
+  const double* __restrict__ tt = t + (indiv_t ? orbit * nt : 0);
+  double rmin = INFINITY, rmax = -INFINITY, zm = 0.0;
+  const bool energies = (E != nullptr) || (EJ != nullptr);
+  for (int j = lane; j < nt; j += 64) {
+    const double* __restrict__ row = r + (long)N * j;
+    const double R = row[0];
+    const double z = (N == 6) ? row[3] : 0.0;
+    const double rs = sqrt(R * R + z * z);
+    rmin = fmin(rmin, rs);
+    rmax = fmax(rmax, rs);
+    zm = fmax(zm, fabs(z));
+    if (energies) {
+      const double vR = row[1], vT = row[2], vz = (N == 6) ? row[4] : 0.0;
+      double sp, cp;
+      sincos(row[N - 1], &sp, &cp);
+      const double En = gh_potential(T, R, z, cp, sp, tt[j]) + vR * vR / 2.0 + vT * vT / 2.0 +
+                        vz * vz / 2.0;
+      if (E) E[orbit * (long)nt + j] = En;
+      if (EJ) EJ[orbit * (long)nt + j] = En - omega * R * vT;
+    }
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    rmin = fmin(rmin, __shfl_xor(rmin, m));
+    rmax = fmax(rmax, __shfl_xor(rmax, m));
+    zm = fmax(zm, __shfl_xor(zm, m));
+  }
+  if (lane == 0) {
+    if (rperi) rperi[orbit] = rmin;
+    if (rap) rap[orbit] = rmax;
+    if (zmax) zmax[orbit] = zm;
+    if (ecc) ecc[orbit] = (rmax - rmin) / (rmax + rmin);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+gh_eval_kernel(int n, const double* __restrict__ R, const double* __restrict__ z,
+               const double* __restrict__ phi, const double* __restrict__ t,
+               const GhPotTable* __restrict__ T, double* __restrict__ pot,
+               double* __restrict__ FR, double* __restrict__ Fz, double* __restrict__ Fphi) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sp, cp;
+  sincos(phi[i], &sp, &cp);
+  if (pot) pot[i] = gh_potential(T, R[i], z[i], cp, sp, t[i]);
+  // the wrapper-aware sum serves every table (evaluation is not a hot path)
+  const GhCyl F = gh_cyl_forces<GhSpecWrapped>(T, R[i], R[i] * R[i], z[i], cp, sp, t[i]);
+  if (FR) FR[i] = F.FR;
+  if (Fz) Fz[i] = F.Fz;
+  if (Fphi) Fphi[i] = F.Fphi;
+}
+// Compositions with a specialised kernel: MWPotential2014, the rotating-bar
+// model, and single Hernquist / logarithmic halos; anything else runs the
+// generic kernel that dispatches on the component type at run time.
+// V = GhVec3 integrates full orbits, GhVec2 planar ones.
+template <class V>
+int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                     const GhPotHandle* h, double dt, double rtol, double atol, double* d_result,
+                     int* d_err, int odeint_type, double* d_stats, hipStream_t stream) {
+  if (nobj == 0) return 0;
+  if (nobj < 0 || nt < 2) {
+    set_error("invalid sizes nobj=%d nt=%d (need nt >= 2)", nobj, nt);
+    return -1;
+  }
+#define GH_LAUNCH(SPEC)                                                                        \
+  return launch_spec<SPEC, V>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result, \
+                              d_err, odeint_type, d_stats, stream)
+  using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
+  using LogBar = GhSpec<GH_LOGHALO, GH_DEHNENBAR>;
+  using Hern = GhSpec<GH_HERNQUIST>;
+  using Log = GhSpec<GH_LOGHALO>;
+  using Any = GhSpec<>;
+  if (types_are(h, {GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW})) GH_LAUNCH(MW2014);
+  if (types_are(h, {GH_LOGHALO, GH_DEHNENBAR})) GH_LAUNCH(LogBar);
+  if (types_are(h, {GH_HERNQUIST})) GH_LAUNCH(Hern);
+  if (types_are(h, {GH_LOGHALO})) GH_LAUNCH(Log);
+  if (h->has_wrap) GH_LAUNCH(GhSpecWrapped);
+  GH_LAUNCH(Any);
+#undef GH_LAUNCH
+}
+
+// integrate_dxdv: V = GhVec6 (full orbits, rows of 12) or GhVec4 (planar,
+// rows of 8); MWPotential2014 has a specialised kernel, any other
+// composition with device Hessians runs the run-time dispatch.
+template <class V>
+int launch_integrate_dxdv(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                          const GhPotHandle* h, double dt, double rtol, double atol,
+                          double* d_result, int* d_err, int odeint_type, double* d_stats,
+                          hipStream_t stream) {
+  if (nobj == 0) return 0;
+  if (nobj < 0 || nt < 2) {
+    set_error("invalid sizes nobj=%d nt=%d (need nt >= 2)", nobj, nt);
+    return -1;
+  }
+  if (!hessian_capable(h)) {
+    set_error("integrate_dxdv needs time-independent, axisymmetric components with a Hessian "
+              "(every supported type but DehnenBar and a triaxial LogarithmicHalo, unwrapped)");
+    return -1;
+  }
+  using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
+  if (types_are(h, {GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW}))
+    return launch_spec<MW2014, V>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result,
+                                  d_err, odeint_type, d_stats, stream);
+  return launch_spec<GhSpec<>, V>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result,
+                                  d_err, odeint_type, d_stats, stream);
+}
+
+// SIGINT during the blocking host call sets the polled flag, as galpy's
+// handler sets `interrupted` (bovy_symplecticode.c:38-45).
+void on_sigint(int) {
+  if (g_irq_host) *g_irq_host = 1;
+}
+
+// What a host call integrates: galpy's full or planar orbits, or their
+// variational equations (integrate_dxdv).
+enum GhMode { GH_FULL = 0, GH_PLANAR = 1, GH_DXDV_FULL = 2, GH_DXDV_PLANAR = 3 };
+
+// Blocking host-buffer integration, streamed through the GPU in chunks of
+// orbits.  Rows: (R,vR,vT,z,vz,phi), (R,vR,vT,phi), or the rectangular
+// (q, p | dq, dp) of 12 / 8 doubles.
+int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npot, int* pot_type,
+                   double* pot_args, double dt, double rtol, double atol, double* result,
+                   int* err, int odeint_type, void (*cb)(void), int mode) {
+  const bool planar = mode == GH_PLANAR || mode == GH_DXDV_PLANAR;
+  const size_t N = mode == GH_FULL ? 6 : mode == GH_PLANAR ? 4 : mode == GH_DXDV_FULL ? 12 : 8;
+  if (nobj <= 0) return 0;
+  if (nt < 2) {
+    set_error("nt=%d: need at least two output times", nt);
+    return -1;
+  }
+  if (odeint_type < 0 || odeint_type > 7) {
+    set_error("odeint_type %d has no HIP integrator (supported: 0 leapfrog, 1 rk4, 2 rk6, "
+              "3 symplec4, 4 symplec6, 5 dopr54, 6 dop853, 7 ias15)", odeint_type);
+    return -1;
+  }
+  GhPotTable table;
+  if (decode_potential(npot, pot_type, pot_args, &table, planar) != 0) return -1;
+  hipStream_t stream;
+  GH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  void* d_table = nullptr;
+  GhPotHandle handle;
+  handle.dev = nullptr;
+  fill_handle(table, &handle);
+  double *d_yo = nullptr, *d_t = nullptr, *d_res = nullptr;
+  int* d_err = nullptr;
+  int rc = 0;
+  struct sigaction old_action, action;
+  std::memset(&action, 0, sizeof(action));
+  action.sa_handler = on_sigint;
+  irq_device_ptr();
+  if (g_irq_host) *g_irq_host = 0;
+  sigaction(SIGINT, &action, &old_action);
+  do {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { rc = -1; set_error("hipMemGetInfo failed"); break; }
+    const size_t per_orbit = (size_t)nt * N * 8 + N * 8 + 4 + (indiv_t ? (size_t)nt * 8 : 0);
+    size_t budget = free_b / 2;
+    long chunk = (long)std::min<size_t>((size_t)nobj, std::max<size_t>(1, budget / per_orbit));
+    if (hipMalloc(&d_table, sizeof(GhPotTable)) != hipSuccess ||
+        hipMalloc((void**)&d_yo, (size_t)chunk * N * 8) != hipSuccess ||
+        hipMalloc((void**)&d_t, (size_t)nt * 8 * (indiv_t ? chunk : 1)) != hipSuccess ||
+        hipMalloc((void**)&d_res, (size_t)chunk * nt * N * 8) != hipSuccess ||
+        hipMalloc((void**)&d_err, (size_t)chunk * 4) != hipSuccess) {
+      rc = -1;
+      set_error("device allocation failed (chunk of %ld orbits, nt=%d)", chunk, nt);
+      break;
+    }
+    if (hipMemcpyAsync(d_table, &table, sizeof(GhPotTable), hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+    handle.dev = (GhPotTable*)d_table;
+    if (!indiv_t &&
+        hipMemcpyAsync(d_t, t, (size_t)nt * 8, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+    for (long o0 = 0; o0 < nobj && rc == 0; o0 += chunk) {
+      const long n = std::min<long>(chunk, nobj - o0);
+      if (hipMemcpyAsync(d_yo, yo + N * o0, (size_t)n * N * 8, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+      if (indiv_t && hipMemcpyAsync(d_t, t + (size_t)nt * o0, (size_t)n * nt * 8,
+                                    hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+      int lrc;
+      switch (mode) {
+        case GH_FULL:
+          lrc = launch_integrate<GhVec3>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol, atol,
+                                         d_res, d_err, odeint_type, nullptr, stream);
+          break;
+        case GH_PLANAR:
+          lrc = launch_integrate<GhVec2>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol, atol,
+                                         d_res, d_err, odeint_type, nullptr, stream);
+          break;
+        case GH_DXDV_FULL:
+          lrc = launch_integrate_dxdv<GhVec6>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
+                                              atol, d_res, d_err, odeint_type, nullptr, stream);
+          break;
+        default:
+          lrc = launch_integrate_dxdv<GhVec4>((int)n, d_yo, nt, d_t, indiv_t, &handle, dt, rtol,
+                                              atol, d_res, d_err, odeint_type, nullptr, stream);
+          break;
+      }
+      if (lrc != 0) { rc = -1; break; }
+      if (hipMemcpyAsync(result + N * nt * o0, d_res, (size_t)n * nt * N * 8,
+                         hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipMemcpyAsync(err + o0, d_err, (size_t)n * 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipMemcpyAsync(yo + N * o0, d_yo, (size_t)n * N * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipStreamSynchronize(stream) != hipSuccess) {
+        rc = -1;
+        set_error("device copy/sync failed: %s", hipGetErrorString(hipGetLastError()));
+        break;
+      }
+      if (cb)
+        for (long i = 0; i < n; i++) cb();
+    }
+  } while (0);
+  sigaction(SIGINT, &old_action, nullptr);
+  if (g_irq_host) *g_irq_host = 0;
+  (void)hipStreamSynchronize(stream);
+  (void)hipFree(d_table);
+  (void)hipFree(d_yo);
+  (void)hipFree(d_t);
+  (void)hipFree(d_res);
+  (void)hipFree(d_err);
+  (void)hipStreamDestroy(stream);
+  return rc;
+}
+}  // namespace
+}  // namespace gh
+
+using namespace gh;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+static void integrate_host_abi(const char* name, int nobj, double* yo, int nt, double* t,
+                               int indiv_t, int npot, int* pot_type, double* pot_args,
+                               void* pot_tfuncs, double dt, double rtol, double atol,
+                               double* result, int* err, int odeint_type, void (*cb)(void),
+                               int mode) {
+  g_errbuf[0] = 0;
+  int rc;
+  if (pot_tfuncs != nullptr) {
+    set_error("pot_tfuncs must be NULL: no supported potential uses time functions");
+    rc = -1;
+  } else {
+    for (int i = 0; i < nobj; i++) err[i] = 0;
+    rc = host_integrate(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol,
+                        result, err, odeint_type, cb, mode);
+  }
+  if (rc != 0) {
+    for (int i = 0; i < nobj; i++) err[i] = -1;
+    fprintf(stderr, "galpy_hip: %s failed: %s\n", name, g_errbuf);
+  }
+}
+
+GH_EXPORT void integrateFullOrbit(int nobj, double* yo, int nt, double* t, int indiv_t, int npot,
+                                  int* pot_type, double* pot_args, void* pot_tfuncs, double dt,
+                                  double rtol, double atol, double* result, int* err,
+                                  int odeint_type, void (*cb)(void)) {
+  integrate_host_abi("integrateFullOrbit", nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args,
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, cb, GH_FULL);
+}
+
+GH_EXPORT void integratePlanarOrbit(int nobj, double* yo, int nt, double* t, int indiv_t,
+                                    int npot, int* pot_type, double* pot_args, void* pot_tfuncs,
+                                    double dt, double rtol, double atol, double* result, int* err,
+                                    int odeint_type, void (*cb)(void)) {
+  integrate_host_abi("integratePlanarOrbit", nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args,
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, cb, GH_PLANAR);
+}
+
+GH_EXPORT void integrateFullOrbit_dxdv(double* yo, int nt, double* t, int npot, int* pot_type,
+                                       double* pot_args, void* pot_tfuncs, double dt, double rtol,
+                                       double atol, double* result, int* err, int odeint_type) {
+  integrate_host_abi("integrateFullOrbit_dxdv", 1, yo, nt, t, 0, npot, pot_type, pot_args,
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, nullptr, GH_DXDV_FULL);
+}
+
+GH_EXPORT void integratePlanarOrbit_dxdv(double* yo, int nt, double* t, int npot, int* pot_type,
+                                         double* pot_args, void* pot_tfuncs, double dt,
+                                         double rtol, double atol, double* result, int* err,
+                                         int odeint_type, void (*cb)(void)) {
+  (void)cb;  // galpy's integratePlanarOrbit_dxdv takes it and does not call it
+  integrate_host_abi("integratePlanarOrbit_dxdv", 1, yo, nt, t, 0, npot, pot_type, pot_args,
+                     pot_tfuncs, dt, rtol, atol, result, err, odeint_type, nullptr,
+                     GH_DXDV_PLANAR);
+}
+
+GH_EXPORT void gh_integrate_dxdv(int nobj, int dim, double* yo, int nt, double* t, int indiv_t,
+                                 int npot, int* pot_type, double* pot_args, double dt,
+                                 double rtol, double atol, double* result, int* err,
+                                 int odeint_type) {
+  if (dim != 2 && dim != 3) {
+    g_errbuf[0] = 0;
+    set_error("gh_integrate_dxdv: dim must be 2 (planar) or 3, got %d", dim);
+    for (int i = 0; i < nobj; i++) err[i] = -1;
+    return;
+  }
+  integrate_host_abi("gh_integrate_dxdv", nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args,
+                     nullptr, dt, rtol, atol, result, err, odeint_type, nullptr,
+                     dim == 3 ? GH_DXDV_FULL : GH_DXDV_PLANAR);
+}
+
+GH_EXPORT int gh_integrate_dxdv_hip(int nobj, int dim, double* d_yo, int nt, const double* d_t,
+                                    int indiv_t, const void* pot, double dt, double rtol,
+                                    double atol, double* d_result, int* d_err, int odeint_type,
+                                    double* d_stats, void* stream) {
+  g_errbuf[0] = 0;
+  if (pot == nullptr) {
+    set_error("potential handle is NULL");
+    return -1;
+  }
+  if (dim == 3)
+    return launch_integrate_dxdv<GhVec6>(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot,
+                                         dt, rtol, atol, d_result, d_err, odeint_type, d_stats,
+                                         (hipStream_t)stream);
+  if (dim == 2)
+    return launch_integrate_dxdv<GhVec4>(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot,
+                                         dt, rtol, atol, d_result, d_err, odeint_type, d_stats,
+                                         (hipStream_t)stream);
+  set_error("gh_integrate_dxdv_hip: dim must be 2 (planar) or 3, got %d", dim);
+  return -1;
+}
+
+GH_EXPORT int integrateFullOrbit_hip(int nobj, double* d_yo, int nt, const double* d_t,
+                                     int indiv_t, const void* pot, double dt, double rtol,
+                                     double atol, double* d_result, int* d_err, int odeint_type,
+                                     double* d_stats, void* stream) {
+  g_errbuf[0] = 0;
+  if (pot == nullptr) {
+    set_error("potential handle is NULL");
+    return -1;
+  }
+  return launch_integrate<GhVec3>(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot, dt, rtol,
+                                  atol, d_result, d_err, odeint_type, d_stats, (hipStream_t)stream);
+}
+
+GH_EXPORT int integratePlanarOrbit_hip(int nobj, double* d_yo, int nt, const double* d_t,
+                                       int indiv_t, const void* pot, double dt, double rtol,
+                                       double atol, double* d_result, int* d_err, int odeint_type,
+                                       double* d_stats, void* stream) {
+  g_errbuf[0] = 0;
+  if (pot == nullptr) {
+    set_error("potential handle is NULL");
+    return -1;
+  }
+  return launch_integrate<GhVec2>(nobj, d_yo, nt, d_t, indiv_t, (const GhPotHandle*)pot, dt, rtol,
+                                  atol, d_result, d_err, odeint_type, d_stats, (hipStream_t)stream);
+}
+
+static int upload_potential(int npot, const int* pot_type, const double* pot_args, void** pot,
+                            bool planar) {
+  g_errbuf[0] = 0;
+  GhPotTable table;
+  if (decode_potential(npot, pot_type, pot_args, &table, planar) != 0) return -1;
+  void* d = nullptr;
+  GH_CHECK(hipMalloc(&d, sizeof(GhPotTable)));
+  GH_CHECK(hipMemcpy(d, &table, sizeof(GhPotTable), hipMemcpyHostToDevice));
+  GhPotHandle* h = new GhPotHandle;
+  h->dev = (GhPotTable*)d;
+  fill_handle(table, h);
+  *pot = h;
+  return 0;
+}
+
+GH_EXPORT int gh_potential_upload(int npot, const int* pot_type, const double* pot_args,
+                                  void** pot) {
+  return upload_potential(npot, pot_type, pot_args, pot, false);
+}
+
+GH_EXPORT int gh_planar_potential_upload(int npot, const int* pot_type, const double* pot_args,
+                                         void** pot) {
+  return upload_potential(npot, pot_type, pot_args, pot, true);
+}
+
+GH_EXPORT int gh_potential_free(void* pot) {
+  if (pot == nullptr) return 0;
+  GhPotHandle* h = (GhPotHandle*)pot;
+  const hipError_t e = hipFree(h->dev);
+  delete h;
+  if (e != hipSuccess) {
+    set_error("hipFree failed: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+// device address of the table behind a handle (for callers that build their
+// own launch descriptors, e.g. batched work lists)
+GH_EXPORT const void* gh_potential_device_ptr(const void* pot) {
+  return pot ? ((const GhPotHandle*)pot)->dev : nullptr;
+}
+
+GH_EXPORT int gh_energy_drift_hip(int nobj, int nt, const double* d_rows, const double* d_t,
+                                  int indiv_t, const void* pot, double omega, double* d_e0,
+                                  double* d_demax, void* stream) {
+  g_errbuf[0] = 0;
+  if (nobj <= 0) return 0;
+  const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
+  hipLaunchKernelGGL(gh_energy_kernel<6>, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows,
+                     d_t, indiv_t, ((const GhPotHandle*)pot)->dev, omega, d_e0, d_demax);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+GH_EXPORT int gh_planar_energy_drift_hip(int nobj, int nt, const double* d_rows, const double* d_t,
+                                         int indiv_t, const void* pot, double omega, double* d_e0,
+                                         double* d_demax, void* stream) {
+  g_errbuf[0] = 0;
+  if (nobj <= 0) return 0;
+  const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
+  hipLaunchKernelGGL(gh_energy_kernel<4>, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows,
+                     d_t, indiv_t, ((const GhPotHandle*)pot)->dev, omega, d_e0, d_demax);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+GH_EXPORT int gh_orbit_summary_hip(int nobj, int nt, int ncol, const double* d_rows,
+                                   const double* d_t, int indiv_t, const void* pot, double omega,
+                                   double* d_rperi, double* d_rap, double* d_zmax, double* d_e,
+                                   double* d_E, double* d_EJ, void* stream) {
+  g_errbuf[0] = 0;
+  if (nobj <= 0) return 0;
+  if (nt < 1 || (ncol != 6 && ncol != 4)) {
+    set_error("gh_orbit_summary_hip: nt=%d ncol=%d (need nt >= 1, ncol 6 or 4)", nt, ncol);
+    return -1;
+  }
+  const bool energies = d_E != nullptr || d_EJ != nullptr;
+  if (energies && (pot == nullptr || d_t == nullptr)) {
+    set_error("gh_orbit_summary_hip: E and the Jacobi integral need the potential and the times");
+    return -1;
+  }
+  const GhPotTable* T = pot ? ((const GhPotHandle*)pot)->dev : nullptr;
+  const dim3 grid((unsigned)((nobj + kSummaryOrbitsPerBlock - 1) / kSummaryOrbitsPerBlock));
+  const dim3 block(64 * kSummaryOrbitsPerBlock);
+  if (ncol == 6)
+    hipLaunchKernelGGL(gh_summary_kernel<6>, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows,
+                       d_t, indiv_t, T, omega, d_rperi, d_rap, d_zmax, d_e, d_E, d_EJ);
+  else
+    hipLaunchKernelGGL(gh_summary_kernel<4>, grid, block, 0, (hipStream_t)stream, nobj, nt, d_rows,
+                       d_t, indiv_t, T, omega, d_rperi, d_rap, d_zmax, d_e, d_E, d_EJ);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+
+GH_EXPORT int gh_eval_hip(int n, const double* d_R, const double* d_z, const double* d_phi,
+                          const double* d_t, const void* pot, double* d_pot, double* d_FR,
+                          double* d_Fz, double* d_Fphi, void* stream) {
+  g_errbuf[0] = 0;
+  if (n <= 0) return 0;
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
+  hipLaunchKernelGGL(gh_eval_kernel, grid, block, 0, (hipStream_t)stream, n, d_R, d_z, d_phi, d_t,
+                     ((const GhPotHandle*)pot)->dev, d_pot, d_FR, d_Fz, d_Fphi);
+  GH_CHECK(hipGetLastError());
+  return 0;
+}
+// Host evaluation of the force-kernel incomplete gamma gamma(a, x) of
+// component `comp` (a PowerSphericalPotentialwCutoff) exactly as the device
+// computes it from the decoded table: table on [a+1, xbig), series below,
+// Gamma(a) above.  For checks of the table fit without a GPU.
+GH_EXPORT int gh_host_bulge_lower_gamma(int npot, const int* pot_type, const double* pot_args,
+                                        int comp, int n, const double* x, double* out) {
+  g_errbuf[0] = 0;
+  static GhPotTable T;
+  if (decode_potential(npot, pot_type, pot_args, &T) != 0) return -1;
+  if (comp < 0 || comp >= T.npot || T.comp[comp].type != GH_POWERSPHCUTOFF) {
+    set_error("component %d is not a PowerSphericalPotentialwCutoff", comp);
+    return -1;
+  }
+  const GhComponent& C = T.comp[comp];
+  const double a = C.c[0];
+  const double* tab = &T.tab[(int)C.c[11]][0][0];
+  for (int i = 0; i < n; i++) {
+    const double xi = x[i];
+    double g;
+    if (xi >= C.c[4]) {
+      g = C.c[1];
+    } else if (xi <= 0.0) {
+      g = 0.0;
+    } else if (xi < a + 1.0) {
+      const double pref = std::exp(a * std::log(xi) - xi);
+      double sacc = T.ser[comp][0][GH_IG_NSER - 1];
+      for (int q = GH_IG_NSER - 2; q >= 0; q--) sacc = std::fma(sacc, xi, T.ser[comp][0][q]);
+      g = pref * sacc;
+    } else {
+      int e;
+      const double m = std::frexp(xi, &e);
+      const double u = (m - 0.5) * (2.0 * GH_TAB_PER_OCT);
+      int k = (int)u;
+      k = k > GH_TAB_PER_OCT - 1 ? GH_TAB_PER_OCT - 1 : k;
+      const int seg = (e - (int)C.c[12]) * GH_TAB_PER_OCT + k;
+      const double t = 2.0 * (u - k) - 1.0;
+      const double* c = tab + seg * GH_TAB_NC;
+      double v = c[GH_TAB_NC - 1];
+      for (int q = GH_TAB_NC - 2; q >= 0; q--) v = std::fma(v, t, c[q]);
+      g = C.c[1] - v;
+    }
+    out[i] = g;
+  }
+  return 0;
+}
+
+GH_EXPORT int gh_host_log_ge1(int n, const double* y, double* out) {
+  for (int i = 0; i < n; i++) out[i] = gh_log_ge1(y[i]);
+  return 0;
+}
+
+GH_EXPORT const char* galpy_hip_last_error(void) { return g_errbuf; }
+
+GH_EXPORT void galpy_hip_set_interrupt(int value) {
+  irq_device_ptr();
+  if (g_irq_host) *g_irq_host = value;
+}

@@ -231,10 +231,10 @@ __device__ __forceinline__ double gh_symplec4_estimate_step(const F& T, V qo, V 
 // c1..c8, kicks d1..d7.  The seven-kick loops below stay rolled (one force
 // evaluation per loop body), so a run-time dispatch kernel does not carry
 // the force switch 28 times.
-__constant__ double gh_s6c[8] = {0.392256805238780, 0.510043411918458, -0.471053385409758,
+static __constant__ double gh_s6c[8] = {0.392256805238780, 0.510043411918458, -0.471053385409758,
                                  0.687531682525198e-1, 0.687531682525198e-1, -0.471053385409758,
                                  0.510043411918458, 0.392256805238780};
-__constant__ double gh_s6d[7] = {0.784513610477560, 0.235573213359357, -0.117767998417887e1,
+static __constant__ double gh_s6d[7] = {0.784513610477560, 0.235573213359357, -0.117767998417887e1,
                                  0.131518632068391e1, -0.117767998417887e1, 0.235573213359357,
                                  0.784513610477560};
 
