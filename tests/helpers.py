@@ -14,7 +14,10 @@ ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 LN1EM12 = -12.0 * numpy.log(10.0)
 DT_SENTINEL = -9999.99
 
+ORACLE_FMA_PATH = os.path.join(ROOT, "oracle", "liboracle_fma.so")
+
 _oracle = None
+_oracle_fma = None
 
 
 def oracle():
@@ -24,6 +27,18 @@ def oracle():
             raise RuntimeError("oracle not built: run `make -C oracle`")
         _oracle = ctypes.CDLL(ORACLE_PATH)
     return _oracle
+
+
+def oracle_fma():
+    """The oracle compiled with fused multiply-adds: the same algorithm with
+    a different, equally valid rounding.  Its distance from oracle() measures
+    how strongly a problem amplifies rounding differences."""
+    global _oracle_fma
+    if _oracle_fma is None:
+        if not os.path.exists(ORACLE_FMA_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _oracle_fma = ctypes.CDLL(ORACLE_FMA_PATH)
+    return _oracle_fma
 
 
 def _p(a):
@@ -60,12 +75,12 @@ def npot_of(pot_type, pot_args, planar=False):
 
 
 def oracle_integrate(pot_type, pot_args, yo, t, method, dt=None, rtol=LN1EM12, atol=LN1EM12,
-                     nthreads=0, planar=False):
+                     nthreads=0, planar=False, fma=False):
     """Run the oracle's integrateFullOrbit (returns result, err, stats[N,4],
     yo_after) where stats = (steps, rejected, rhs, dt).  planar=True runs
     its integratePlanarOrbit instead: rows of (R,vR,vT,phi) and the planar
-    packing."""
-    lib = oracle()
+    packing.  fma=True runs the FMA-rounded build (oracle_fma)."""
+    lib = oracle_fma() if fma else oracle()
     yo = numpy.array(numpy.atleast_2d(yo), dtype=numpy.float64, order="C")
     t = numpy.ascontiguousarray(t, dtype=numpy.float64)
     n = len(yo)
