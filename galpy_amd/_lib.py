@@ -117,6 +117,7 @@ SIGNATURES["gh_integrate_dxdv_hip"] = (
 # the planar entry points take the same arguments as the full ones
 SIGNATURES["integratePlanarOrbit"] = SIGNATURES["integrateFullOrbit"]
 SIGNATURES["integratePlanarOrbit_hip"] = SIGNATURES["integrateFullOrbit_hip"]
+SIGNATURES["integrateFullOrbit_hip_noalloc"] = SIGNATURES["integrateFullOrbit_hip"]
 SIGNATURES["gh_planar_energy_drift_hip"] = SIGNATURES["gh_energy_drift_hip"]
 SIGNATURES["gh_orbit_summary_hip"] = (
     ctypes.c_int,

@@ -60,7 +60,7 @@ def units():
     inst = [(f"gh_instances_{k}.o", "gh_instances.hip", [f"-DGH_INSTANCE={k}"])
             for k in range(NUM_INSTANCES)]
     other = [("gh_batch.o", "gh_batch.hip", []), ("gh_adjoint.o", "gh_adjoint.hip", []),
-             ("gh_host.o", "gh_host.hip", [])]
+             ("gh_host.o", "gh_host.hip", []), ("gh_noalloc.o", "gh_noalloc.hip", [])]
     # longest compiles first (the run-time dispatch and wrapper-aware
     # instances, the batch kernel), so the parallel build ends sooner
     heavy = [inst[k] for k in HEAVY_INSTANCES] + other[:1]

@@ -486,7 +486,8 @@ gh_eval_kernel(int n, const double* __restrict__ R, const double* __restrict__ z
 template <class V>
 int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                      const GhPotHandle* h, double dt, double rtol, double atol, double* d_result,
-                     int* d_err, int odeint_type, double* d_stats, hipStream_t stream) {
+                     int* d_err, int odeint_type, double* d_stats, hipStream_t stream,
+                     const GhWorkspace* ws = nullptr) {
   if (nobj == 0) return 0;
   if (nobj < 0 || nt < 2) {
     set_error("invalid sizes nobj=%d nt=%d (need nt >= 2)", nobj, nt);
@@ -494,7 +495,7 @@ int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indi
   }
 #define GH_LAUNCH(SPEC)                                                                        \
   return launch_spec<SPEC, V>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result, \
-                              d_err, odeint_type, d_stats, stream)
+                              d_err, odeint_type, d_stats, stream, ws)
   using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
   using LogBar = GhSpec<GH_LOGHALO, GH_DEHNENBAR>;
   using Hern = GhSpec<GH_HERNQUIST>;
@@ -530,9 +531,9 @@ int launch_integrate_dxdv(int nobj, double* d_yo, int nt, const double* d_t, int
   using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
   if (types_are(h, {GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW}))
     return launch_spec<MW2014, V>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result,
-                                  d_err, odeint_type, d_stats, stream);
+                                  d_err, odeint_type, d_stats, stream, nullptr);
   return launch_spec<GhSpec<>, V>(nobj, d_yo, nt, d_t, indiv_t, h->dev, dt, rtol, atol, d_result,
-                                  d_err, odeint_type, d_stats, stream);
+                                  d_err, odeint_type, d_stats, stream, nullptr);
 }
 
 // SIGINT during the blocking host call sets the polled flag, as galpy's
@@ -649,6 +650,17 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
   return rc;
 }
 }  // namespace
+
+int launch_full_orbits(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                       const GhPotHandle* h, double dt, double rtol, double atol, double* d_result,
+                       int* d_err, int odeint_type, double* d_stats, hipStream_t stream,
+                       const GhWorkspace* ws) {
+  return launch_integrate<GhVec3>(nobj, d_yo, nt, d_t, indiv_t, h, dt, rtol, atol, d_result, d_err,
+                                  odeint_type, d_stats, stream, ws);
+}
+
+const int* irq_device_ptr_if_ready() { return g_irq_dev; }
+
 }  // namespace gh
 
 using namespace gh;

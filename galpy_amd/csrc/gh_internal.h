@@ -80,6 +80,18 @@ inline bool hessian_capable(const GhPotHandle* h) {
   return true;
 }
 
+// Caller-provided device memory for a launch's temporaries (the DOP853
+// dense-output overflow area, the pilot's cost keys and claim order with
+// the radix sort's temporary, the work counter).  With a workspace a launch
+// allocates nothing: it carves these from `ptr` (256-byte aligned, `bytes`
+// long) and fails with an error when they do not fit.  `cancel` is the
+// interrupt flag the kernels poll (nullable).
+struct GhWorkspace {
+  char* ptr;
+  size_t bytes;
+  const int* cancel;
+};
+
 // Compositions with kernels of their own.
 using SpecMW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;
 using SpecLogBar = GhSpec<GH_LOGHALO, GH_DEHNENBAR>;
@@ -89,16 +101,19 @@ using SpecAny = GhSpec<>;
 
 // Launch of galpy integrator `odeint_type` for orbits of kind V (GhVec3 full,
 // GhVec2 planar, GhVec6 / GhVec4 their variational systems) in composition
-// Spec.  Buffers are device buffers; returns 0 or -1 with the error set.
+// Spec.  Buffers are device buffers; `ws` (nullable) provides the
+// temporaries, which are otherwise allocated stream-ordered.  Returns 0 or
+// -1 with the error set.
 template <class Spec, class V>
 int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                 const GhPotTable* d_table, double dt, double rtol, double atol, double* d_result,
-                int* d_err, int odeint_type, double* d_stats, hipStream_t stream);
+                int* d_err, int odeint_type, double* d_stats, hipStream_t stream,
+                const GhWorkspace* ws);
 
 #define GH_LAUNCH_SPEC_INSTANCE(EXT, SPEC, V)                                                   \
   EXT template int launch_spec<SPEC, V>(int, double*, int, const double*, int,                  \
                                         const GhPotTable*, double, double, double, double*,    \
-                                        int*, int, double*, hipStream_t)
+                                        int*, int, double*, hipStream_t, const GhWorkspace*)
 
 #define GH_LAUNCH_SPEC_ALL(EXT)                            \
   GH_LAUNCH_SPEC_INSTANCE(EXT, SpecMW2014, GhVec3);        \
@@ -119,5 +134,17 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
   GH_LAUNCH_SPEC_INSTANCE(EXT, SpecAny, GhVec4)
 
 GH_LAUNCH_SPEC_ALL(extern);
+
+// Full-orbit launch for an uploaded potential (the composition picks the
+// kernel instance), with an optional workspace: integrateFullOrbit_hip's
+// body.
+int launch_full_orbits(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                       const GhPotHandle* h, double dt, double rtol, double atol, double* d_result,
+                       int* d_err, int odeint_type, double* d_stats, hipStream_t stream,
+                       const GhWorkspace* ws);
+
+// Interrupt flag's device address if it is already set up, else nullptr
+// (never allocates).
+const int* irq_device_ptr_if_ready();
 
 }  // namespace gh

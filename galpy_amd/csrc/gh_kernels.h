@@ -812,12 +812,15 @@ int dop_resident_blocks(int dev) {
   return n;
 }
 // DOP853 launch at WPE waves per SIMD: optional pilot + longest-first
-// order, then the persistent kernel.  The pilot's cost keys and the order
-// live in one stream-ordered temporary allocation, freed after the launch.
+// order, then the persistent kernel.  The temporaries (dense-output overflow
+// area, pilot cost keys and order, the radix sort's temporary) live in one
+// stream-ordered allocation freed after the launch, or, with a workspace,
+// in the caller's memory (then nothing is allocated).
 template <class Spec, int WPE, class V>
 int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                   const GhPotTable* d_table, double rtol, double atol, double* d_result,
-                  int* d_err, double* d_stats, const int* cancel, int dev, hipStream_t stream) {
+                  int* d_err, double* d_stats, const int* cancel, int dev, hipStream_t stream,
+                  const GhWorkspace* ws) {
   const int resident = dop_resident_blocks<Spec, WPE, V>(dev);
   if (resident <= 0) {
     set_error("could not query DOP853 occupancy on device %d", dev);
@@ -826,15 +829,10 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
   const unsigned nblocks =
       (unsigned)std::min<long>((long)((nobj + kBlock - 1) / kBlock), (long)resident);
   const unsigned long long lanes = (unsigned long long)nblocks * kBlock;
-  unsigned long long* counter = next_counter(dev);
-  if (counter == nullptr) {
-    set_error("could not set up the DOP853 work counter on device %d", dev);
-    return -1;
-  }
-  // One stream-ordered temporary allocation: the dense-output overflow area
-  // (kOverflowBytes per workgroup) and, when some lane gets more than one
-  // orbit, the pilot's cost keys and the longest-first order (otherwise all
-  // orbits start at once and the order is moot).
+  // Layout of the temporaries: [counter 256 B, workspace only][overflow:
+  // kOverflowBytes per workgroup][ordered: cost, sorted cost, iota, order
+  // (4 n4), pilot grid (256 B), sort temporary].  The order only matters
+  // when some lane gets more than one orbit.
   const bool ordered = !indiv_t && (unsigned long long)nobj > lanes && dop_order_enabled();
   size_t sort_bytes = 0;
   hipError_t e = hipSuccess;
@@ -842,11 +840,29 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
     e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, (const unsigned*)nullptr,
                                                      (unsigned*)nullptr, (const int*)nullptr,
                                                      (int*)nullptr, nobj, 0, 32, stream);
+  const size_t head = ws ? 256 : 0;
   const size_t ovf = ((size_t)nblocks * kOverflowBytes<V> + 255) / 256 * 256;
   const size_t n4 = ordered ? ((size_t)nobj * 4 + 255) / 256 * 256 : 0;
+  const size_t need = head + ovf + (ordered ? 4 * n4 + 256 + sort_bytes : 0);
+  unsigned long long* counter = nullptr;
   void* work = nullptr;
-  if (e == hipSuccess)
-    e = hipMallocAsync(&work, ovf + (ordered ? 4 * n4 + 256 + sort_bytes : 0), stream);
+  if (ws) {
+    if (need > ws->bytes) {
+      set_error("DOP853 temporaries for %d orbits (%zu bytes: %zu overflow area for %u workgroups, "
+                "%zu pilot order and sort) exceed the workspace's %zu bytes",
+                nobj, need, ovf, nblocks, need - head - ovf, ws->bytes);
+      return -1;
+    }
+    counter = reinterpret_cast<unsigned long long*>(ws->ptr);
+    work = ws->ptr + head;
+  } else {
+    counter = next_counter(dev);
+    if (counter == nullptr) {
+      set_error("could not set up the DOP853 work counter on device %d", dev);
+      return -1;
+    }
+    if (e == hipSuccess) e = hipMallocAsync(&work, need, stream);
+  }
   char* d_overflow = static_cast<char*>(work);
   const int* d_order = nullptr;
   if (e == hipSuccess && ordered) {
@@ -875,7 +891,7 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
                        d_stats, cancel, counter, d_order, nullptr, d_overflow);
     e = hipGetLastError();
   }
-  if (work) {
+  if (work && !ws) {
     const hipError_t ef = hipFreeAsync(work, stream);
     if (e == hipSuccess) e = ef;
   }
@@ -893,8 +909,9 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
 template <class Spec, class V>
 int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                 const GhPotTable* d_table, double dt, double rtol, double atol, double* d_result,
-                int* d_err, int odeint_type, double* d_stats, hipStream_t stream) {
-  const int* cancel = irq_device_ptr();
+                int* d_err, int odeint_type, double* d_stats, hipStream_t stream,
+                const GhWorkspace* ws) {
+  const int* cancel = ws ? ws->cancel : irq_device_ptr();
   const dim3 grid((unsigned)((nobj + kBlock - 1) / kBlock)), block(kBlock);
   switch (odeint_type) {
     case 0:
@@ -958,17 +975,17 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
       if constexpr (std::is_same<V, GhVec3>::value) {
         const bool two = dop_waves_forced() != 1;
         return two ? launch_dop853<Spec, 2, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
-                                               d_result, d_err, d_stats, cancel, dev, stream)
+                                               d_result, d_err, d_stats, cancel, dev, stream, ws)
                    : launch_dop853<Spec, 1, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
-                                               d_result, d_err, d_stats, cancel, dev, stream);
+                                               d_result, d_err, d_stats, cancel, dev, stream, ws);
       } else if constexpr (std::is_same<V, GhVec2>::value) {
         return launch_dop853<Spec, 2, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
-                                         d_result, d_err, d_stats, cancel, dev, stream);
+                                         d_result, d_err, d_stats, cancel, dev, stream, ws);
       } else {
         // variational states: twice the registers and LDS per job, one wave
         // per SIMD (the LDS queue allows four to six workgroups per CU)
         return launch_dop853<Spec, 1, V>(nobj, d_yo, nt, d_t, indiv_t, d_table, rtol, atol,
-                                         d_result, d_err, d_stats, cancel, dev, stream);
+                                         d_result, d_err, d_stats, cancel, dev, stream, ws);
       }
     }
     default:

@@ -141,6 +141,27 @@ int integrateFullOrbit_hip(int nobj, double *d_yo, int nt, const double *d_t, in
                            double *d_result, int *d_err, int odeint_type, double *d_stats,
                            void *stream);
 
+/* Bytes of the per-GPU scratch array behind integrateFullOrbit_hip_noalloc. */
+#define GH_NOALLOC_SCRATCH_BYTES (80u << 20)
+
+/* integrateFullOrbit_hip without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments, results and errors as integrateFullOrbit_hip; the launch
+ * temporaries (DOP853 dense-output overflow area, pilot order and sort
+ * temporary, work counter) live in one static device array of
+ * GH_NOALLOC_SCRATCH_BYTES per GPU, kept until the process ends.  A DOP853
+ * call whose temporaries do not fit (roughly 48 MiB of overflow area at the
+ * MI355X's 2,048 resident workgroups plus 16 bytes per orbit when there are
+ * more orbits than resident lanes: about 2 million orbits) returns -1 with
+ * a message and launches nothing.  Calls that overlap on one GPU race on the
+ * array.  The interrupt flag is polled only if it was set up before (by a
+ * host-buffer call or galpy_hip_set_interrupt). */
+int integrateFullOrbit_hip_noalloc(int nobj, double *d_yo, int nt, const double *d_t,
+                                   int indiv_t, const void *pot, double dt, double rtol,
+                                   double atol, double *d_result, int *d_err, int odeint_type,
+                                   double *d_stats, void *stream);
+
 /* Device-resident twin of integratePlanarOrbit (integratePlanarOrbit.c:743
  * with the buffers in HBM): as integrateFullOrbit_hip with d_yo[nobj*4]
  * and d_result[nobj*nt*4]; pot from gh_planar_potential_upload. */

@@ -15,6 +15,7 @@ def _declared():
         if fn.endswith(".h"):
             src = open(os.path.join(inc, fn)).read()
             src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            src = re.sub(r"^\s*#.*$", "", src, flags=re.M)  # preprocessor lines
             names += re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", src)
     return sorted(set(names))
 
