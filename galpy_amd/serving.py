@@ -6,10 +6,17 @@ The server keeps every admitted request's inputs, outputs and per-orbit
 integrator state resident in HBM and runs iterations: each iteration is one
 bounded persistent launch (``gh_batch_launch``) over the unfinished tiles of
 all active requests, every orbit advancing at most ``budget`` steps.
-Between iterations finished requests are retired (their memory released)
-and queued requests are admitted in arrival order while their buffers fit in
-the HBM budget -- so work that arrives while other requests are on the GPU
-joins the next launch and shares it.  Latency = completion - arrival.
+Between iterations finished requests are retired and queued requests are
+admitted in arrival order while their buffers fit in the HBM budget -- so
+work that arrives while other requests are on the GPU joins the next launch
+and shares it.
+
+Retiring a request frees its inputs and integrator state at once; its result
+rows, error codes and step counts drain to pinned host memory on a copy
+stream that overlaps the following launches, and their HBM stays counted
+against the budget until that copy has completed (``drain=False`` keeps
+results on the device, counted until ``release``).  Latency = the result on
+the host (or, without draining, the last launch done) - arrival.
 """
 
 import collections
@@ -46,7 +53,15 @@ class Request:
 
     def nbytes(self, state_bytes):
         """HBM the request holds while active."""
-        return self.n * self.nt * 48 + self.n * (48 + 4 + 24 + state_bytes) + self.nt * 8
+        return self.input_bytes(state_bytes) + self.output_bytes()
+
+    def input_bytes(self, state_bytes):
+        """HBM freed when the request retires: initial states and integrator state."""
+        return self.n * (48 + state_bytes)
+
+    def output_bytes(self):
+        """HBM held until the results leave the device: rows, err, stats, t."""
+        return self.n * self.nt * 48 + self.n * (4 + 24) + self.nt * 8
 
     @property
     def latency(self):
@@ -74,7 +89,7 @@ class Server:
     ``hbm_fraction`` of the free device memory at start)."""
 
     def __init__(self, budget=4000, opl=1, max_tiles=None, hbm_bytes=None, hbm_fraction=0.8,
-                 device="cuda"):
+                 device="cuda", drain=True):
         import torch
 
         self.torch = torch
@@ -97,6 +112,10 @@ class Server:
         self.iterations = 0
         self.d_counter = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.per_tile = 64 * self.opl
+        self.drain = bool(drain)
+        self.draining = []  # (request, copy-done event)
+        self.copy_stream = torch.cuda.Stream(device=self.device) if self.drain else None
+        self.peak_in_use = 0
 
     # -- request lifecycle ----------------------------------------------------
     def submit(self, req):
@@ -133,15 +152,55 @@ class Server:
             r.admitted_at = now
             r.status = "active"
             self.in_use += r.nbytes(self.state_bytes)
+            self.peak_in_use = max(self.peak_in_use, self.in_use)
             self.active.append(r)
 
     def _retire(self, r, now):
-        r.done_at = now
-        r.status = "done"
-        self.in_use -= r.nbytes(self.state_bytes)
-        for name in ("d_yo", "d_state"):
-            setattr(r, name, None)
-        self.finished.append(r)
+        r.d_yo = r.d_state = None
+        self.in_use -= r.input_bytes(self.state_bytes)
+        if not self.drain:
+            r.done_at = now
+            r.status = "done"
+            self.finished.append(r)
+            return
+        torch = self.torch
+        host = [torch.empty(x.shape, dtype=x.dtype, pin_memory=True) for x in (r.result, r.err, r.stats)]
+        self.copy_stream.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(self.copy_stream):
+            for h, x in zip(host, (r.result, r.err, r.stats)):
+                h.copy_(x, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self.copy_stream)
+        r.host = host
+        r.status = "draining"
+        self.draining.append((r, ev))
+
+    def _poll_drains(self, now_fn, wait=False):
+        """Finish the requests whose results reached the host (with wait,
+        block for the oldest one first)."""
+        if wait and self.draining:
+            self.draining[0][1].synchronize()
+        keep = []
+        for r, ev in self.draining:
+            if ev.query():
+                # the device buffers were kept alive by these references until now
+                r.result, r.err, r.stats = r.host
+                r.host = None
+                r.d_t = None
+                self.in_use -= r.output_bytes()
+                r.done_at = now_fn()
+                r.status = "done"
+                self.finished.append(r)
+            else:
+                keep.append((r, ev))
+        self.draining = keep
+
+    def release(self, r):
+        """Drop a finished request's device results (drain=False only)."""
+        if self.drain or r.status != "done" or r.result is None or not r.result.is_cuda:
+            return
+        r.result = r.err = r.stats = r.d_t = None
+        self.in_use -= r.output_bytes()
 
     # -- one iteration ------------------------------------------------------------
     def step(self, now_fn):
@@ -189,17 +248,23 @@ class Server:
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter()
         now_fn = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+        clock = lambda: time.perf_counter() - t0
         i = 0
         while True:
-            now = now_fn() if realtime else time.perf_counter() - t0
+            now = now_fn() if realtime else clock()
             while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
                 self.submit(reqs[i])
                 i += 1
+            self._poll_drains(clock)
             self._admit(now)
             if self.active:
-                self.step(lambda: time.perf_counter() - t0)
+                self.step(clock)
+            elif self.queue and self.draining:
+                self._poll_drains(clock, wait=True)  # admission waits for memory
             elif i < len(reqs):
                 time.sleep(max(0.0, min(reqs[i].arrival - now, 0.005)))
+            elif self.draining:
+                self._poll_drains(clock, wait=True)
             elif not self.queue:
                 break
         self.makespan = time.perf_counter() - t0
