@@ -58,15 +58,29 @@ def parse():
     ap.add_argument("--count-sample", type=int, default=64, help="orbits in the op-count sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ngroups", type=int, default=4096, help="groups of the batching workload")
-    ap.add_argument("--nreq", type=int, default=200, help="requests per offered rate (serving)")
-    ap.add_argument("--rates", default="50,500,5000", help="offered request rates, req/s (serving)")
+    ap.add_argument("--nreq", type=int, default=300, help="requests per offered rate (serving)")
+    ap.add_argument("--rates", default="50,5000", help="offered request rates, req/s (serving)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving (latency) leg")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="CPU-baseline runs (median reported)")
     ap.add_argument("--budget", type=int, default=4000, help="steps per orbit per serving iteration")
     return ap.parse_args()
 
 
-def cpu_baseline_leg(w, sample, count_sample):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_leg(w, sample, count_sample, reps=5):
     """CPU oracle on a bounded sample of the same workload (OpenMP, all host
-    cores), plus the op count per orbit-step of the same algorithm."""
+    cores; the median of `reps` timed runs), plus the op count per
+    orbit-step of the same algorithm."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     # the oracle is test infrastructure
     from helpers import oracle_count_ops, oracle_count_ops_dxdv, oracle_integrate, oracle_integrate_dxdv
@@ -79,20 +93,25 @@ def cpu_baseline_leg(w, sample, count_sample):
     code = METHOD_CODES[w["method"]]
     nthreads = len(os.sched_getaffinity(0))
     yo = w["yo"][:sample]
+    times = []
     if w["dxdv"]:
-        nsteps = numpy.zeros(len(yo), dtype=numpy.int64)
-        t0 = time.perf_counter()
-        oracle_integrate_dxdv(pt, pa, yo, w["t"], code, dim=3, dt=w.get("dt"), nthreads=nthreads,
-                              steps=nsteps)
-        el = time.perf_counter() - t0
+        for _ in range(reps):
+            nsteps = numpy.zeros(len(yo), dtype=numpy.int64)
+            t0 = time.perf_counter()
+            oracle_integrate_dxdv(pt, pa, yo, w["t"], code, dim=3, dt=w.get("dt"), nthreads=nthreads,
+                                  steps=nsteps)
+            times.append(time.perf_counter() - t0)
+        el = float(numpy.median(times))
         steps = float(nsteps.sum())
         ops = oracle_count_ops_dxdv(pt, pa, w["yo"][:count_sample], w["t"], code, dim=3, dt=w.get("dt"))
         ops["rhs"] = None
     else:
-        t0 = time.perf_counter()
-        _, err, stats, _ = oracle_integrate(pt, pa, yo, w["t"], code, dt=w.get("dt"), nthreads=nthreads,
-                                            planar=planar)
-        el = time.perf_counter() - t0
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            _, err, stats, _ = oracle_integrate(pt, pa, yo, w["t"], code, dt=w.get("dt"),
+                                                nthreads=nthreads, planar=planar)
+            times.append(time.perf_counter() - t0)
+        el = float(numpy.median(times))
         steps = stats[:, 0].sum()
         ops = oracle_count_ops(pt, pa, w["yo"][:count_sample], w["t"], code, dt=w.get("dt"),
                                planar=planar)
@@ -102,7 +121,10 @@ def cpu_baseline_leg(w, sample, count_sample):
         "unit": "orbit-steps/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": f"first {sample} of the {w['n']} orbits, full t grid ({int(steps)} orbit-steps, {el:.2f} s)",
+        "sample": (f"first {sample} of the {w['n']} orbits, full t grid ({int(steps)} orbit-steps); "
+                   f"median of {reps} runs: {el:.2f} s (runs {', '.join(f'{x:.2f}' for x in times)})"),
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
     }, {
         "flop_per_orbit_step": flops / ops["steps"],
         "special_per_orbit_step": ops["special"] / ops["steps"],
@@ -184,25 +206,37 @@ def bench_batching(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_serving(args):
-    """Poisson request streams at several offered rates through the
-    continuous-batching server; reports p50/p99 latency and orbit-steps/s."""
+def serving_leg(nreq, rates, budget):
+    """Poisson request streams drawn by the serving law (N log-uniform in
+    [1, 1e5], potentials from MWPotential2014 / Hernquist / LogarithmicHalo /
+    LogarithmicHalo + rotating DehnenBar, lengths U(1, 1000) with 2..501
+    outputs, symplec4 at dt = 0.01 or DOP853 50/50) through the
+    continuous-batching server, one stream per offered rate; results drain
+    to host memory, latency = result on the host - arrival."""
     import torch
 
     from galpy_amd.serving import Server, make_requests
 
-    rates = [float(r) for r in args.rates.split(",")]
-    sweep = []
     # warm-up: code load and first-launch set-up outside the measured streams
-    Server(budget=args.budget).run(make_requests(8, 1000.0, seed=99, max_orbits=64, max_len=2.0, max_out=5),
-                                   realtime=False)
+    Server(budget=budget).run(make_requests(8, 1000.0, seed=99, max_orbits=64, max_len=2.0, max_out=5),
+                              realtime=False)
+    sweep = []
     for k, rate in enumerate(rates):
-        reqs = make_requests(args.nreq, rate, seed=k)
-        srv = Server(budget=args.budget)
+        reqs = make_requests(nreq, rate, seed=k)
+        srv = Server(budget=budget)
         m = srv.run(reqs, realtime=True)
         m["rate"] = rate
         sweep.append(m)
+        del srv, reqs
         torch.cuda.empty_cache()
+    return sweep
+
+
+def bench_serving(args):
+    """Poisson request streams at several offered rates through the
+    continuous-batching server; reports p50/p99 latency and orbit-steps/s."""
+    rates = [float(r) for r in args.rates.split(",")]
+    sweep = serving_leg(args.nreq, rates, args.budget)
     best = max(sweep, key=lambda m: m["orbit_steps_per_s"])
     out = {
         "metric": "p50/p99 request latency and FP64 orbit-steps/s, continuous batching",
@@ -473,8 +507,9 @@ def main():
     torch.cuda.synchronize()
 
     value = steps_per_pass * args.steps / elapsed
+    frac_ok = float((dem <= 1e-8).mean())
     out = {
-        "metric": "FP64 orbit-steps/s on 1 MI355X at |dE/E|<=1e-8",
+        "metric": "FP64 orbit-steps/s on 1 MI355X at |dE/E|<=1e-8; p50/p99 request latency",
         "value": value,
         "unit": "orbit-steps/s",
         "n_gpus": 1,
@@ -500,6 +535,7 @@ def main():
             "output_bytes": n * nt * 8 * ncol,
         },
         "orbit_steps_per_pass": steps_per_pass,
+        "frac_orbits_dE_le_1e-8": frac_ok,
         "rejected_steps_per_pass": rejected,
         "orbits_with_error": nerr,
         "dE_over_E": {
@@ -512,8 +548,28 @@ def main():
         "kernel_ms": kernel_ms,
     }
     kernel_s = numpy.mean(kernel_ms) / 1e3
+    del result, err, stats, e0, demax
+    torch.cuda.empty_cache()
+    if not args.no_serving and args.norbits is None and w["name"] == "mw2014_dop853_1m":
+        rates = [float(r) for r in args.rates.split(",")]
+        sweep = serving_leg(args.nreq, rates, args.budget)
+        out["latency"] = {
+            "p50_s": {str(m["rate"]): m.get("p50_latency_s") for m in sweep},
+            "p99_s": {str(m["rate"]): m.get("p99_latency_s") for m in sweep},
+            "orbit_steps_per_s": {str(m["rate"]): m["orbit_steps_per_s"] for m in sweep},
+            "offered_req_per_s": {str(m["rate"]): m.get("offered_req_per_s") for m in sweep},
+            "achieved_req_per_s": {str(m["rate"]): m["achieved_req_per_s"] for m in sweep},
+            "completed": {str(m["rate"]): m["completed"] for m in sweep},
+            "config": {"workload": "serving_poisson", "requests_per_rate": args.nreq, "rates_req_per_s": rates,
+                       "orbits_per_request": "log-uniform [1, 1e5]", "length": "U(1, 1000), 2..501 outputs",
+                       "potentials": "MWPotential2014 / Hernquist(a~U(0.5,3)) / LogarithmicHalo(q~U(0.7,1)) / "
+                                     "LogarithmicHalo(q=0.9)+DehnenBar(omegab=1.85)",
+                       "methods": "symplec4 (dt=0.01) / dop853 (rtol=atol=1e-12) 50/50",
+                       "budget_steps_per_iteration": args.budget,
+                       "latency": "result in host memory - arrival"},
+        }
     if not args.no_cpu_baseline:
-        cpu, opc = cpu_baseline_leg(w, min(args.cpu_sample, n), min(args.count_sample, n))
+        cpu, opc = cpu_baseline_leg(w, min(args.cpu_sample, n), min(args.count_sample, n), args.cpu_reps)
         out["cpu_baseline"] = cpu
         achieved = opc["flop_per_orbit_step"] * steps_per_pass / kernel_s / 1e12
         out["roofline"] = {
