@@ -72,6 +72,7 @@ SIGNATURES = {
         [ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_void_p] * 5 + [ctypes.c_void_p],
     ),
     "gh_host_log_ge1": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "gh_host_sincos": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 3),
     "galpy_hip_last_error": (ctypes.c_char_p, []),
     "galpy_hip_set_interrupt": (None, [ctypes.c_int]),
 }

@@ -936,6 +936,15 @@ GH_EXPORT int gh_host_bulge_lower_gamma(int npot, const int* pot_type, const dou
   return 0;
 }
 
+GH_EXPORT int gh_host_sincos(int n, const double* x, double* s, double* c) {
+  for (int i = 0; i < n; i++) {
+    const GhSinCos v = gh_sincos(x[i]);
+    s[i] = v.s;
+    c[i] = v.c;
+  }
+  return 0;
+}
+
 GH_EXPORT int gh_host_log_ge1(int n, const double* y, double* out) {
   for (int i = 0; i < n; i++) out[i] = gh_log_ge1(y[i]);
   return 0;

@@ -52,3 +52,62 @@ __host__ __device__ inline double gh_log_ge1(double y) {
   const double lf = fma(-s, f - R, f);   // log(1 + f)
   return fma(ed, LN2_HI, fma(ed, LN2_LO, lf));
 }
+
+// sin and cos of one argument with a single argument reduction, entirely in
+// registers (the library's sincos returns cos through a pointer and carries
+// a large-argument reduction with a stack array, which costs scratch-memory
+// traffic in every force evaluation of the rotating components).
+//
+// Reduction (Cody-Waite, with fused multiply-adds): k = nearest integer to
+// x 2/pi, r = x - k pi/2 with pi/2 split into three doubles P1 + P2 + P3
+// (each the rounding of what remains of pi/2), so that |r| <= pi/4 with an
+// absolute error ~1e-16 for |k| < 2^30.  Then the Taylor polynomials
+//   sin r = r + r^3 sum_{j=1..9} (-1)^j r^(2j-2) / (2j+1)!,
+//   cos r = 1 - r^2/2 + r^4 sum_{j=2..9} (-1)^j r^(2j-4) / (2j)!,
+// whose truncation error at |r| = pi/4 is below 1e-22, and the quadrant
+// k mod 4 picks the signs and the swap.  Arguments with |k| >= 2^30, and
+// non-finite ones, go to the library routines.
+struct GhSinCos {
+  double s, c;
+};
+
+#ifdef __HIP_DEVICE_COMPILE__
+__device__ __noinline__ inline GhSinCos gh_sincos_library(double x) {
+  return GhSinCos{sin(x), cos(x)};
+}
+#else
+inline GhSinCos gh_sincos_library(double x) { return GhSinCos{sin(x), cos(x)}; }
+#endif
+
+__host__ __device__ inline GhSinCos gh_sincos(double x) {
+  const double kd = rint(x * 0x1.45f306dc9c883p-1);  // x 2/pi
+  if (!(fabs(kd) < 0x1p30)) return gh_sincos_library(x);
+  double r = fma(-kd, 0x1.921fb54442d18p+0, x);
+  r = fma(-kd, 0x1.1a62633145c07p-54, r);
+  r = fma(-kd, -0x1.f1976b7ed8fbcp-110, r);
+  const double z = r * r;
+  double ps = -0x1.2f49b46814157p-57;  // -1/19!
+  ps = fma(ps, z, 0x1.952c77030ad4ap-49);
+  ps = fma(ps, z, -0x1.ae7f3e733b81fp-41);
+  ps = fma(ps, z, 0x1.6124613a86d09p-33);
+  ps = fma(ps, z, -0x1.ae64567f544e4p-26);
+  ps = fma(ps, z, 0x1.71de3a556c734p-19);
+  ps = fma(ps, z, -0x1.a01a01a01a01ap-13);
+  ps = fma(ps, z, 0x1.1111111111111p-7);
+  ps = fma(ps, z, -0x1.5555555555555p-3);  // -1/3!
+  const double sr = fma(r * z, ps, r);
+  double pc = -0x1.6827863b97d97p-53;  // -1/18!
+  pc = fma(pc, z, 0x1.ae7f3e733b81fp-45);
+  pc = fma(pc, z, -0x1.93974a8c07c9dp-37);
+  pc = fma(pc, z, 0x1.1eed8eff8d898p-29);
+  pc = fma(pc, z, -0x1.27e4fb7789f5cp-22);
+  pc = fma(pc, z, 0x1.a01a01a01a01ap-16);
+  pc = fma(pc, z, -0x1.6c16c16c16c17p-10);
+  pc = fma(pc, z, 0x1.5555555555555p-5);  // 1/4!
+  const double cr = fma(z * z, pc, fma(-0.5, z, 1.0));
+  const int q = (int)(long long)kd & 3;
+  GhSinCos out;
+  out.s = (q == 0) ? sr : (q == 1) ? cr : (q == 2) ? -sr : -cr;
+  out.c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;
+  return out;
+}

@@ -216,8 +216,8 @@ __device__ __forceinline__ double gh_bar_smooth(double t, double tform, double t
 __device__ __forceinline__ void gh_bar_angle(double cos2p, double sin2p, double omegab,
                                              double barphi, double t, double& cang,
                                              double& sang) {
-  double sb, cb;
-  sincos(2.0 * (omegab * t + barphi), &sb, &cb);
+  const GhSinCos b = gh_sincos(2.0 * (omegab * t + barphi));
+  const double sb = b.s, cb = b.c;
   cang = cos2p * cb + sin2p * sb;
   sang = sin2p * cb - cos2p * sb;
 }
@@ -323,20 +323,28 @@ __device__ __forceinline__ void gh_comp_force<GH_LOGHALO>(const GhPotTable* __re
   const GhComponent* __restrict__ C = &T->comp[i];
   const double amp = C->p[0], core2 = C->p[2], onem1overb2 = C->p[3];
   const double zq2 = g.z2 * C->c[0];  // (z/q)^2
+  // Each branch fills scalars and F is updated once below: conditional
+  // updates of F's fields make the compiler select a field address, which
+  // keeps the accumulator in scratch memory.
+  double fr, fz, fp;
   if (onem1overb2 < 1.0) {
     // R_t^2 = R^2 (1 - (1 - 1/b^2) sin^2 phi) = R^2 - (1 - 1/b^2) y^2
     const double Rt2 = g.R2 - onem1overb2 * g.y * g.y;
     const double inv = gh_rcp(Rt2 + zq2 + core2);
     const double iR2 = gh_rcp(g.R2);
-    F.FRoR += -amp * Rt2 * inv * iR2;
-    F.Fz += -amp * g.z * C->c[0] * inv;
+    fr = -amp * Rt2 * inv * iR2;
+    fz = -amp * g.z * C->c[0] * inv;
     // F_phi = amp R^2 inv sin(phi) cos(phi) (1 - 1/b^2) = amp inv x y (1 - 1/b^2)
-    F.FphioR2 += amp * inv * g.x * g.y * onem1overb2 * iR2;
+    fp = amp * inv * g.x * g.y * onem1overb2 * iR2;
   } else {
     const double inv = gh_rcp(g.R2 + zq2 + core2);
-    F.FRoR += -amp * inv;
-    F.Fz += -amp * g.z * C->c[0] * inv;
+    fr = -amp * inv;
+    fz = -amp * g.z * C->c[0] * inv;
+    fp = 0.0;
   }
+  F.FRoR += fr;
+  F.Fz += fz;
+  F.FphioR2 += fp;
 }
 
 template <>
@@ -354,19 +362,23 @@ __device__ __forceinline__ void gh_comp_force<GH_DEHNENBAR>(const GhPotTable* __
   const double r2inv = g.rinv * g.rinv;
   const double r4inv = r2inv * r2inv;
   const double as = amp * smooth;
+  double fr, fp, fz;  // filled per branch, added once (see the logarithmic halo)
   if (g.r <= rb) {
     const double u = g.r * C->c[0];  // r/rb
     const double u3 = u * u * u;
-    F.FRoR += -as * cang * (u3 * (3.0 * g.R2 + 2.0 * g.z2) - 4.0 * g.z2) * r4inv;
-    F.FphioR2 += 2.0 * as * sang * (u3 - 2.0) * r2inv;
-    F.Fz += -as * cang * (u3 + 4.0) * g.R2 * g.z * r4inv;
+    fr = -as * cang * (u3 * (3.0 * g.R2 + 2.0 * g.z2) - 4.0 * g.z2) * r4inv;
+    fp = 2.0 * as * sang * (u3 - 2.0) * r2inv;
+    fz = -as * cang * (u3 + 4.0) * g.R2 * g.z * r4inv;
   } else {
     const double u = rb * g.rinv;
     const double u3 = u * u * u;
-    F.FRoR += -as * cang * u3 * r4inv * (3.0 * g.R2 - 2.0 * g.z2);
-    F.FphioR2 += -2.0 * as * sang * u3 * r2inv;
-    F.Fz += -5.0 * as * cang * u3 * g.R2 * g.z * r4inv;
+    fr = -as * cang * u3 * r4inv * (3.0 * g.R2 - 2.0 * g.z2);
+    fp = -2.0 * as * sang * u3 * r2inv;
+    fz = -5.0 * as * cang * u3 * g.R2 * g.z * r4inv;
   }
+  F.FRoR += fr;
+  F.FphioR2 += fp;
+  F.Fz += fz;
 }
 
 // ---------------------------------------------------------------------------
@@ -539,7 +551,9 @@ __device__ __forceinline__ void gh_comp_force<GH_KUZMINDISK>(const GhPotTable* _
 __device__ __forceinline__ void gh_needle_frame(const GhComponent* __restrict__ C, double x,
                                                 double y, double t, double& cp, double& sp,
                                                 double& xb, double& yb) {
-  sincos(C->p[4] + C->p[5] * t, &sp, &cp);
+  const GhSinCos a = gh_sincos(C->p[4] + C->p[5] * t);
+  sp = a.s;
+  cp = a.c;
   xb = cp * x + sp * y;
   yb = cp * y - sp * x;
 }
@@ -616,12 +630,19 @@ __device__ __forceinline__ double gh_spiral_arg(const GhComponent* __restrict__ 
                                                 double phi, double t) {
   return C->p[4] * 0.5 * log(R2) - C->p[5] * (phi - C->p[6] * t - C->p[7]);
 }
+// phi as galpy's planar force evaluation forms it, acos(x/R) and 2 pi - that
+// for y < 0 (integratePlanarOrbit.c:1022-1025): in [0, 2 pi), which decides
+// the spiral's phase for a non-integer multiplicity m.
+__device__ __forceinline__ double gh_phi_acos(double x, double y, double R2) {
+  const double phi = acos(x / sqrt(R2));
+  return (y < 0.0) ? 2.0 * M_PI - phi : phi;
+}
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_STEADYSPIRAL>(const GhPotTable* __restrict__ T,
                                                                int i, const GhGeom& g, GhAcc& F) {
   const GhComponent* __restrict__ C = &T->comp[i];
   const double aA = C->p[0] * gh_ellipdisk_smooth(g.t, C->p[1], C->p[2]) * C->p[3];
-  const double sn = sin(gh_spiral_arg(C, g.R2, atan2(g.y, g.x), g.t));
+  const double sn = gh_sincos(gh_spiral_arg(C, g.R2, gh_phi_acos(g.x, g.y, g.R2), g.t)).s;
   const double iR2 = gh_rcp(g.R2);
   F.FRoR += aA * sn * iR2;
   F.FphioR2 += -aA / C->p[4] * C->p[5] * sn * iR2;
@@ -763,23 +784,21 @@ __device__ __forceinline__ void gh_comp_force_any(const GhPotTable* __restrict__
     const double R = sqrt(g.R2);
     double off, doff;
     gh_wrap_transform(T, C, R, g.t, fac, off, doff);
-    double so, co;
-    sincos(off, &so, &co);
+    const GhSinCos sc = gh_sincos(off);
+    const double so = sc.s, co = sc.c;
     gw.x = g.x * co + g.y * so;
     gw.y = g.y * co - g.x * so;
     corr = R * doff;
   }
   GhAcc L = {0.0, 0.0, 0.0};
   gh_comp_force_dyn(T, i, gw, L);
-  if (wrapped) {
-    F.FRoR += fac * (L.FRoR - L.FphioR2 * corr);
-    F.Fz += fac * L.Fz;
-    F.FphioR2 += fac * L.FphioR2;
-  } else {
-    F.FRoR += L.FRoR;
-    F.Fz += L.Fz;
-    F.FphioR2 += L.FphioR2;
-  }
+  // scalars per branch, F updated once (no field-address select)
+  const double fr = wrapped ? fac * (L.FRoR - L.FphioR2 * corr) : L.FRoR;
+  const double fz = wrapped ? fac * L.Fz : L.Fz;
+  const double fp = wrapped ? fac * L.FphioR2 : L.FphioR2;
+  F.FRoR += fr;
+  F.Fz += fz;
+  F.FphioR2 += fp;
 }
 
 // A potential composition known at compile time: GhSpec<15, 5, 9> is
@@ -907,8 +926,8 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
     if (C->nwrap) {
       double off, doff;
       gh_wrap_transform(T, C, R, t, fac, off, doff);
-      double so, co;
-      sincos(off, &so, &co);
+      const GhSinCos sc = gh_sincos(off);
+      const double so = sc.s, co = sc.c;
       cosphi = cosphi0 * co + sinphi0 * so;
       sinphi = sinphi0 * co - cosphi0 * so;
     }

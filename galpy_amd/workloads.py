@@ -26,6 +26,15 @@ def mw2014():
     return P.MWPotential2014_factory()
 
 
+def mw2014_reordered():
+    """MWPotential2014's three components in another order (NFW halo,
+    MiyamotoNagai disk, PowerSphericalPotentialwCutoff bulge): the same
+    physics through the run-time dispatch kernel instead of the specialised
+    one (the sum's order differs, so results agree to rounding)."""
+    bulge, disk, halo = P.MWPotential2014_factory()
+    return P.CompositePotential(halo, disk, bulge)
+
+
 def loghalo_bar():
     """LogarithmicHaloPotential(q=0.9) + DehnenBarPotential(omegab=1.85, rb=0.8, Af=0.01)."""
     return P.CompositePotential(
@@ -62,6 +71,7 @@ POTENTIAL_NAMES = {
     loghalo_bar: "LogarithmicHalo(q=0.9)+DehnenBar",
     mn_plummer_burkert: "MiyamotoNagai(a=3/8,b=0.28/8)+Plummer(b=0.3)+Burkert(a=2)",
     mw2014_rotating_triaxial: "MWPotential2014+SolidBodyRotationWrapper(LogarithmicHalo(amp=0.05,q=0.9,b=0.8),omega=0.6)",
+    mw2014_reordered: "MWPotential2014 (components as NFW, MiyamotoNagai, PowerSphericalwCutoff)",
 }
 
 WORKLOADS = {
@@ -98,6 +108,13 @@ WORKLOADS = {
                            method="symplec4_hip", omega=0.0, seed=0, dt=0.01),
     "mpb_dop853_256k": dict(pot=mn_plummer_burkert, n=262144, t=(0.0, 500.0, 1001),
                            method="dop853_hip", omega=0.0, seed=0),
+    # MWPotential2014 through the run-time dispatch kernels: the cost of
+    # dispatching on component codes, against the specialised kernels on the
+    # same orbits (mw2014_symplec4_1m, mw2014_dop853_1m at 262,144 orbits)
+    "mw2014gen_symplec4_1m": dict(pot=mw2014_reordered, n=1048576, t=(0.0, 100.0, 101),
+                                 method="symplec4_hip", omega=0.0, seed=0, dt=0.01),
+    "mw2014gen_dop853_256k": dict(pot=mw2014_reordered, n=262144, t=(0.0, 500.0, 1001),
+                                 method="dop853_hip", omega=0.0, seed=0),
     # the other fixed-step integrators of galpy at the force-kernel workload's
     # size: symplec6 (7 force evaluations per step), RK4 (4) and RK6 (7)
     "mw2014_symplec6_1m": dict(pot=mw2014, n=1048576, t=(0.0, 100.0, 101), method="symplec6_hip",

@@ -228,6 +228,12 @@ int gh_host_bulge_lower_gamma(int npot, const int *pot_type, const double *pot_a
  * checks without a GPU. */
 int gh_host_log_ge1(int n, const double *y, double *out);
 
+/* Host evaluation of the force kernels' sine and cosine of one argument (the
+ * pattern angles of DehnenBarPotential.c:18-124, SoftenedNeedleBarPotential.c
+ * and the rotating wrappers), the same code the device runs; for checks
+ * without a GPU. */
+int gh_host_sincos(int n, const double *x, double *s, double *c);
+
 /* ------------------------------------------------------------------------
  * Differentiation (no reference counterpart; galpy offers the forward
  * tangent map integrate_dxdv, bovy_symplecticode.c:1053-1268).

@@ -473,3 +473,55 @@ def test_elliptical_disk_planar_orbits():
         ref, sens = ulp_sensitivity(lambda y: oracle_integrate(ppt, ppa, y, t, code, planar=True)[0], yp)
         assert numpy.all(err == 0)
         assert_parity(res, ref, bar, sens, factor=factor, label=method)
+
+
+def test_planar_only_pointwise_forces_potential_and_energy():
+    """EllipticalDisk (code 4), LopsidedDisk (6) and SteadyLogSpiral (3, at a
+    non-integer multiplicity m = 1.5) on the device, point by point: planar
+    F_R and F_phi against the oracle's at angles all around the circle (y < 0
+    included: the spiral's phase takes phi in [0, 2 pi) as galpy's
+    evalPlanarRectForce forms it) and at times just before, at and after the
+    ends of both growth windows; potential values against the host classes'
+    _evaluate (phi in (-pi, pi], as stored rows carry it); and the planar
+    energy drift kernel against numpy on device-integrated rows."""
+    import torch
+
+    from galpy_amd import potential as P
+    from galpy_amd.integrate import PotentialTable, energy_drift_device, eval_device, integratePlanarOrbit_hip
+
+    from helpers import oracle_planar_forces
+
+    spiral = P.SteadyLogSpiralPotential(A=-0.02, alpha=-7.0, m=1.5, tform=0.2, tsteady=0.5)
+    pot = [P.LogarithmicHaloPotential(normalize=1.0),
+           P.EllipticalDiskPotential(twophio=0.05, p=0.5, phib=0.3, tform=1.0, tsteady=2.0),
+           P.LopsidedDiskPotential(phio=0.02, p=-0.2, phib=1.1), spiral]
+    ppt, ppa = _packed(pot, True)
+    tab = PotentialTable(pot, planar=True)
+    ts = 2.0 * numpy.pi / 0.65
+    edges = [1.0, 3.0, 0.2 * ts, 0.7 * ts]  # EllipticalDisk tform, tform + tsteady; spiral's
+    times = numpy.concatenate([[e - 1e-6, e, e + 1e-6, e - 1e-12, e + 1e-12] for e in edges] + [[0.0, 2.0, 10.0]])
+    rng = numpy.random.default_rng(41)
+    npt = 64
+    R = numpy.tile(rng.uniform(0.2, 3.0, npt), len(times))
+    phi = numpy.tile(rng.uniform(0.0, 2.0 * numpy.pi, npt), len(times))
+    t = numpy.repeat(times, npt)
+    assert (numpy.sin(phi) < 0).sum() > npt // 4
+    dev = [torch.tensor(v, device="cuda") for v in (R, numpy.zeros_like(R), phi, t)]
+    pot_d, FR, _, Fp = (o.cpu().numpy() for o in eval_device(tab, *dev))
+    oFR, oFp = oracle_planar_forces(ppt, ppa, R, phi, t)
+    assert numpy.abs(FR - oFR).max() <= 1e-13 * numpy.abs(oFR).max()
+    assert numpy.abs(Fp - oFp).max() <= 1e-13 * numpy.abs(oFp).max()
+    phw = numpy.arctan2(numpy.sin(phi), numpy.cos(phi))
+    want = P.evaluatePotentials(pot, R, 0.0, phw, t)
+    assert numpy.abs(pot_d - want).max() <= 1e-13 * numpy.abs(want).max()
+    # energy drift along device rows, against numpy on the same rows
+    yp = planar_ics(64, seed=43)
+    tt = numpy.linspace(0.0, 5.0, 51)
+    rows, err = integratePlanarOrbit_hip(pot, yp.copy(), tt, "dop853_hip")
+    assert numpy.all(err == 0)
+    e0, demax = energy_drift_device(tab, torch.tensor(rows, device="cuda"), torch.tensor(tt, device="cuda"))
+    Rr, vR, vT, ph = (rows[..., k] for k in range(4))
+    E = P.evaluatePotentials(pot, Rr, 0.0, ph, numpy.broadcast_to(tt, Rr.shape)) + 0.5 * (vR**2 + vT**2)
+    d_np = numpy.abs((E - E[:, :1]) / E[:, :1]).max(axis=1)
+    assert numpy.abs(e0.cpu().numpy() - E[:, 0]).max() <= 1e-13 * numpy.abs(E[:, 0]).max()
+    assert numpy.abs(demax.cpu().numpy() - d_np).max() <= 1e-12 * max(d_np.max(), 1e-6)
