@@ -132,19 +132,23 @@ __device__ __forceinline__ void gh_comp_adj<GH_LOGHALO>(const GhPotTable* __rest
                                                         GhVec3 x, double r2, double rinv,
                                                         GhVec3 lam, GhVec3& a, GhVec3& k,
                                                         double* dth) {
-  // axisymmetric halo: a = -amp v / D, v = (x, y, z/q^2), D = R^2 + z^2/q^2 + c
+  // Phi = amp/2 ln(x^2 + y^2/b^2 + z^2/q^2 + c) (R^2 (1 - (1 - 1/b^2) sin^2 phi)
+  // = x^2 + y^2/b^2; b^2 = 1 for the axisymmetric halo, packed as 1 - 1/b^2
+  // = 2): a = -amp w / D, w = (x, y/b^2, z/q^2), D the log's argument,
+  // K lam = -amp (lam.x, lam.y/b^2, lam.z/q^2) / D + 2 amp w (w.lam) / D^2
   const GhComponent* __restrict__ C = &T->comp[i];
-  const double amp = C->p[0], core2 = C->p[2];
+  const double amp = C->p[0], core2 = C->p[2], e = C->p[3];
   const double iq2 = C->c[0];
-  const double D = x.x * x.x + x.y * x.y + x.z * x.z * iq2 + core2;
+  const double ib2 = (e < 1.0) ? 1.0 - e : 1.0;
+  const double D = x.x * x.x + ib2 * x.y * x.y + x.z * x.z * iq2 + core2;
   const double iD = 1.0 / D;
-  const GhVec3 v{x.x, x.y, x.z * iq2};
+  const GhVec3 v{x.x, ib2 * x.y, x.z * iq2};
   a.x += -amp * v.x * iD;
   a.y += -amp * v.y * iD;
   a.z += -amp * v.z * iD;
   const double vl = v.x * lam.x + v.y * lam.y + v.z * lam.z;
   k.x += -amp * lam.x * iD + 2.0 * amp * v.x * vl * iD * iD;
-  k.y += -amp * lam.y * iD + 2.0 * amp * v.y * vl * iD * iD;
+  k.y += -amp * ib2 * lam.y * iD + 2.0 * amp * v.y * vl * iD * iD;
   k.z += -amp * lam.z * iq2 * iD + 2.0 * amp * v.z * vl * iD * iD;
   dth[0] += -vl * iD;
 }
@@ -271,8 +275,8 @@ struct GhAdjSum<GhSpec<Codes...>> {
 };
 
 // Kick adjoint at q: returns a(q); adds h K lam_p to lam_q and
-// h (da/dtheta)^T lam_p to dth.  Only time-independent, axisymmetric
-// components have a Hessian here (checked on the host).
+// h (da/dtheta)^T lam_p to dth.  Only time-independent components have a
+// Hessian here (checked on the host).
 template <class Spec>
 __device__ __forceinline__ GhVec3 gh_kick_adjoint(const GhPotTable* __restrict__ T, GhVec3 q,
                                                   double h, GhVec3 lam_p, GhVec3& lam_q,

@@ -155,9 +155,10 @@ GH_EXPORT int gh_adjoint_hip(int n, const double* d_wT, const double* d_gT, doub
     set_error("gh_adjoint_hip supports leapfrog (0) and symplec4 (3), got %d", method);
     return -1;
   }
-  if (!hessian_capable(h)) {
-    set_error("the adjoint needs time-independent, axisymmetric components with a Hessian "
-              "(every supported type but DehnenBar and a triaxial LogarithmicHalo, unwrapped)");
+  if (!adjoint_capable(h)) {
+    set_error("the adjoint needs time-independent components with a Hessian (every supported "
+              "type but DehnenBar, KuzminKutuzovStaeckel, SoftenedNeedleBar and the planar-only "
+              "disks, unwrapped)");
     return -1;
   }
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));

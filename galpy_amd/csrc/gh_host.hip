@@ -524,8 +524,8 @@ int launch_integrate_dxdv(int nobj, double* d_yo, int nt, const double* d_t, int
     return -1;
   }
   if (!hessian_capable(h)) {
-    set_error("integrate_dxdv needs time-independent, axisymmetric components with a Hessian "
-              "(every supported type but DehnenBar and a triaxial LogarithmicHalo, unwrapped)");
+    set_error("integrate_dxdv needs components with a Hessian (every supported type but "
+              "KuzminKutuzovStaeckel, SoftenedNeedleBar and the planar-only disks, unwrapped)");
     return -1;
   }
   using MW2014 = GhSpec<GH_POWERSPHCUTOFF, GH_MIYAMOTONAGAI, GH_NFW>;

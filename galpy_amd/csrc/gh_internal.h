@@ -70,13 +70,26 @@ inline bool types_are(const GhPotHandle* h, std::initializer_list<int> codes) {
   return true;
 }
 
-// Components with a Hessian on the device (the adjoint's): time-independent
-// and axisymmetric -- every type but the bar and a triaxial logarithmic halo.
+// Components with a Hessian-vector product on the device (integrate_dxdv's
+// kick of the deviation): every unwrapped type but KuzminKutuzovStaeckel,
+// SoftenedNeedleBar and the planar-only disks and spiral.
 inline bool hessian_capable(const GhPotHandle* h) {
   if (h->has_wrap) return false;  // time-dependent (and rotating) wrappers
   for (int i = 0; i < h->npot; i++) {
-    if (h->types[i] == GH_DEHNENBAR || h->types[i] == GH_KKSTAECKEL || h->nonaxi[i]) return false;
+    const int c = h->types[i];
+    if (c == GH_KKSTAECKEL || c == GH_SOFTNEEDLEBAR || c == GH_ELLIPDISK || c == GH_LOPSIDED ||
+        c == GH_STEADYSPIRAL)
+      return false;
   }
+  return true;
+}
+
+// The adjoint's: a Hessian and time-independent -- no bar either (its kick
+// adjoint carries no time).
+inline bool adjoint_capable(const GhPotHandle* h) {
+  if (!hessian_capable(h)) return false;
+  for (int i = 0; i < h->npot; i++)
+    if (h->types[i] == GH_DEHNENBAR) return false;
   return true;
 }
 

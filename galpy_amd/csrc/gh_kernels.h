@@ -924,51 +924,36 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                          nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
                          cancel);
       break;
-    case 1:  // RK4 and RK6: plain orbits
-    case 2:
-      if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
-        if (odeint_type == 1)
-          hipLaunchKernelGGL((gh_rk_kernel<4, Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
-                             d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
-                             cancel);
-        else
-          hipLaunchKernelGGL((gh_rk_kernel<6, Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
-                             d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
-                             cancel);
-        break;
-      } else {
-        set_error("rk4 and rk6 have no tangent-map (integrate_dxdv) stepper here");
-        return -1;
-      }
-    case 5:  // DOPR54: plain orbits
-      if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
-        hipLaunchKernelGGL((gh_dopr54_kernel<Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
+    case 1:  // RK4 and RK6: plain orbits, and the variational systems of
+    case 2:  // integrate_dxdv (integrateFullOrbit.c:1272-1296)
+      if (odeint_type == 1)
+        hipLaunchKernelGGL((gh_rk_kernel<4, Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
                            d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
                            cancel);
-        break;
-      } else {
-        set_error("dopr54 has no tangent-map (integrate_dxdv) stepper here");
-        return -1;
-      }
-    case 7:  // IAS15: plain orbits
+      else
+        hipLaunchKernelGGL((gh_rk_kernel<6, Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
+                           d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
+                           cancel);
+      break;
+    case 5:  // DOPR54: likewise
+      hipLaunchKernelGGL((gh_dopr54_kernel<Spec, V>), grid, block, 0, stream, nobj, d_yo, nt, d_t,
+                         indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+      break;
+    case 7:  // IAS15: plain orbits (galpy has no ias15 integrate_dxdv)
       if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
         hipLaunchKernelGGL((gh_ias15_kernel<Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
                            d_t, indiv_t, dt, d_table, d_result, d_err, d_stats, cancel);
         break;
       } else {
-        set_error("ias15 has no tangent-map (integrate_dxdv) stepper here");
+        set_error("ias15 has no integrate_dxdv (as in galpy, which refuses it upstream)");
         return -1;
       }
-    case 4:  // symplec6: plain orbits (galpy has no symplec6 tangent map)
-      if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
-        hipLaunchKernelGGL((gh_symplectic_kernel<4, Spec, V>), grid, block, 0, stream, nobj,
-                           d_yo, nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err,
-                           d_stats, cancel);
-        break;
-      } else {
-        set_error("symplec6 has no tangent-map (integrate_dxdv) stepper");
-        return -1;
-      }
+    case 4:  // symplec6: plain orbits, and symplec6_dxdv's tangent map
+             // (bovy_symplecticode.c:1269-, step estimated from the orbit)
+      hipLaunchKernelGGL((gh_symplectic_kernel<4, Spec, V>), grid, block, 0, stream, nobj, d_yo,
+                         nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
+                         cancel);
+      break;
     case 6: {
       int dev = 0;
       GH_CHECK(hipGetDevice(&dev));

@@ -1,6 +1,7 @@
 // Runge-Kutta integrators of galpy (bovy_rk.c) on the device: RK4 and RK6
 // with galpy's fixed steps and step estimators, one orbit per lane, on the
-// rectangular state y = [q, p] (N = 2 D doubles) with the derivative
+// rectangular state y = [q, p] (N = 2 D doubles; the variational systems of
+// integrate_dxdv in galpy's row layout) with the derivative
 // evalRectDeriv / evalPlanarRectDeriv (integrateFullOrbit.c:1097-1105,
 // integratePlanarOrbit.c:795-803): dq/dt = p, dp/dt = the force.  The stage
 // arithmetic keeps the reference's order of operations (e.g. yn1 += dt a / 6
@@ -18,13 +19,13 @@ __device__ __forceinline__ void gh_rk_put(double* a, GhVec2 v) {
   a[1] = v.y;
 }
 
-// a = dy/dt at (t, y)
+// a = dy/dt at (t, y), in the row layout of GhPhase<V>: (q, p) for plain
+// orbits, galpy's (q, p | dq, dp) for the variational systems
+// (evalRectDeriv_dxdv, integrateFullOrbit.c:1496-1600: the deviation's
+// position derivative is its velocity, its velocity's is K dq)
 template <class F, class V>
 __device__ __forceinline__ void gh_rk_deriv(const F& T, double t, const double* y, double* a) {
-  constexpr int D = GhPhase<V>::N / 2;
-#pragma unroll
-  for (int i = 0; i < D; i++) a[i] = y[D + i];
-  gh_rk_put(a + D, gh_accel(T, t, gh_vec_at<V>(y)));
+  GhPhase<V>::set(a, GhPhase<V>::p(y), gh_accel(T, t, GhPhase<V>::q(y)));
 }
 
 // bovy_rk4_onestep, bovy_rk.c:142-169 (yn1 equals yn on entry)

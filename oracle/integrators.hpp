@@ -649,8 +649,9 @@ bool symplectic_dxdv_traj(int npot, const Component* P, int method, const double
   R init_dt = R(t[1]) - R(t[0]);
   R dt = dt_in;
   if (dt_in == kDtSentinel)
-    dt = (method == 0) ? leapfrog_estimate_step<DB>(F, q, p, init_dt, R(t[0]), rtol, atol)
-                       : symplec4_estimate_step<DB>(F, q, p, init_dt, R(t[0]), rtol, atol);
+    dt = (method == 0)   ? leapfrog_estimate_step<DB>(F, q, p, init_dt, R(t[0]), rtol, atol)
+         : (method == 4) ? symplec6_estimate_step<DB>(F, q, p, init_dt, R(t[0]), rtol, atol)
+                         : symplec4_estimate_step<DB>(F, q, p, init_dt, R(t[0]), rtol, atol);
   long ndt = (long)(dval(init_dt / dt));
   R to = t[0];
   auto drift = [&](R h) {
@@ -674,6 +675,20 @@ bool symplectic_dxdv_traj(int npot, const Component* P, int method, const double
       kick(to + dt / R(2.), dt);
       drift(dt / R(2.));
       to = to + dt;
+    } else if (method == 4) {
+      // symplec6_dxdv, bovy_symplecticode.c:1269-: symplec6's drift/kick
+      // sequence on the augmented state (as symplec6 above)
+      drift(R(S6C[0]) * dt);
+      to += R(S6C[0]) * dt;
+      for (long jj = 0; jj < ndt; jj++) {
+        const bool last = jj == ndt - 1;
+        for (int k = 0; k < 7; k++) {
+          kick(to, R(S6D[k]) * dt);
+          const R cn = (k == 6 && !last) ? R(S6C[7] + S6C[0]) : R(S6C[k + 1]);
+          drift(cn * dt);
+          to += cn * dt;
+        }
+      }
     } else {
       drift(c1 * dt);
       to += c1 * dt;

@@ -865,10 +865,129 @@ bool cyl_hessian(int npot, const Component* P, R Rc, R z, R& R2, R& z2, R& Rz) {
   return true;
 }
 
+// All six cylindrical second derivatives of DehnenBarPotential,
+// DehnenBarPotential.c:180-316 (R2deriv, phi2deriv, Rphideriv, z2deriv,
+// Rzderiv, zphideriv), in the reference's operation order.
+template <class R>
+void bar_hessian(const Component& C, R Rc, R z, R phi, R t, R& R2d, R& phi2d, R& Rphid, R& z2d,
+                 R& Rzd, R& zphid) {
+  double amp = C.p[0], tform = C.p[1], tsteady = C.p[2], rb = C.p[3], omegab = C.p[4], barphi = C.p[5];
+  R smooth = dehnen_smooth(t, tform, tsteady);
+  R R2 = Rc * Rc, z2 = z * z;
+  R r2 = R2 + z2, r4 = r2 * r2, r6 = r4 * r2;
+  R ang = R(2.) * (phi - R(omegab) * t - R(barphi));
+  R cprefac = R(amp) * smooth * cos(ang);
+  R sprefac = R(-2.) * R(amp) * smooth * sin(ang);
+  R rb3 = R(rb * rb * rb);
+  if (r2 <= R(rb * rb)) {
+    R r3rb3 = pow(r2, R(1.5)) / rb3;
+    R2d = cprefac * (r3rb3 * ((R(9.) * R2 + R(2.) * z2) / r4 - R2 / r6 * (R(3.) * R2 + R(2.) * z2)) +
+                     R(4.) * z2 / r6 * (R(4.) * R2 - r2));
+    phi2d = -R(4.) * cprefac * (r3rb3 - R(2.)) * R2 / r2;
+    Rphid = sprefac * (r3rb3 * Rc * (R(3.) * R2 + R(2.) * z2) - R(4.) * Rc * z2) / r4;
+    z2d = cprefac * R2 / r6 * (r3rb3 * (r2 - z2) + R(4.) * (r2 - R(4.) * z2));
+    Rzd = cprefac * Rc * z / r6 * (r3rb3 * (R(2.) * r2 - R2) + R(8.) * (r2 - R(2.) * R2));
+    zphid = sprefac * (r3rb3 + R(4.)) * R2 * z / r4;
+  } else {
+    R rbr3 = pow(R(rb), R(3.)) * pow(r2, R(-1.5));
+    R2d = cprefac * rbr3 / r6 * ((r2 - R(7.) * R2) * (R(3.) * R2 - R(2.) * z2) + R(6.) * R2 * r2);
+    phi2d = R(4.) * cprefac * rbr3 * R2 / r2;
+    Rphid = sprefac * rbr3 * Rc / r4 * (R(3.) * R2 - R(2.) * z2);
+    z2d = cprefac * R(5.) * rbr3 * R2 / r6 * (r2 - R(7.) * z2);
+    Rzd = cprefac * R(5.) * rbr3 * Rc * z / r6 * (R(2.) * r2 - R(7.) * R2);
+    zphid = sprefac * R(5.) * rbr3 * R2 * z / r4;
+  }
+}
+
+// The same six for a triaxial LogarithmicHaloPotential (1 - 1/b^2 < 1),
+// LogarithmicHaloPotential.c:157-283.
+template <class R>
+void loghalo_triax_hessian(const Component& C, R Rc, R z, R phi, R& R2d, R& phi2d, R& Rphid,
+                           R& z2d, R& Rzd, R& zphid) {
+  double amp = C.p[0], q = C.p[1], c = C.p[2], onem1overb2 = C.p[3];
+  R zq = z / R(q);
+  R q2 = R(q * q);
+  R R2 = Rc * Rc;
+  R Rt2 = R2 * (R(1.) - R(onem1overb2) * pow(sin(phi), R(2)));
+  R denom = R(1.) / (Rt2 + zq * zq + R(c));
+  R2d = R(amp) * (denom - R(2.) * Rt2 * denom * denom) * Rt2 / (Rc * Rc);
+  z2d = R(amp) * (denom / q2 - R(2.) * z * z * denom * denom / (q2 * q2));
+  Rzd = -R(amp) * R(2.) * Rt2 / Rc * z / q2 * denom * denom;
+  R s2phi = sin(R(2.) * phi);
+  phi2d = -R(amp) * R(onem1overb2) *
+          (R(0.5) * R2 * R2 * s2phi * s2phi * R(onem1overb2) * denom * denom + R2 * denom * cos(R(2.) * phi));
+  Rphid = -R(amp) * (denom - Rt2 * denom * denom) * Rc * sin(R(2.) * phi) * R(onem1overb2);
+  zphid = R(amp) * R(2.) * Rc * Rc * z * sin(phi) * cos(phi) * R(onem1overb2) * denom * denom / q2;
+}
+
+// calcR2deriv .. calczphideriv (galpy_potentials.c), summed over the
+// components in index order: the axisymmetric components through
+// cyl_hessian (their phi derivatives vanish), the bar and the triaxial halo
+// through the functions above.  Returns false for a component without a
+// restated Hessian.
+template <class R>
+bool cyl_hessian_full(int npot, const Component* P, R Rc, R z, R phi, R t, R& R2, R& z2, R& Rz,
+                      R& phi2, R& Rphi, R& zphi) {
+  R2 = 0.; z2 = 0.; Rz = 0.; phi2 = 0.; Rphi = 0.; zphi = 0.;
+  for (int i = 0; i < npot; i++) {
+    const Component& C = P[i];
+    R a = 0., b = 0., cc = 0., d = 0., e = 0., f = 0.;
+    if (C.type == kDehnenBar) {
+      bar_hessian(C, Rc, z, phi, t, a, d, e, b, cc, f);
+    } else if (C.type == kLogHalo && C.p[3] < 1) {
+      loghalo_triax_hessian(C, Rc, z, phi, a, d, e, b, cc, f);
+    } else if (!cyl_hessian(1, &C, Rc, z, a, b, cc)) {
+      return false;
+    }
+    R2 += a; z2 += b; Rz += cc; phi2 += d; Rphi += e; zphi += f;
+  }
+  return true;
+}
+
+// calcPlanarR2deriv / phi2deriv / Rphideriv: the planar functions of the bar
+// (DehnenBarPotential.c:125-179) and the triaxial halo
+// (LogarithmicHaloPotential.c:108-156); the other components' planar
+// second derivatives are their full ones in the plane.
+template <class R>
+bool planar_hessian(int npot, const Component* P, R Rc, R phi, R t, R& R2, R& phi2, R& Rphi) {
+  R2 = 0.; phi2 = 0.; Rphi = 0.;
+  for (int i = 0; i < npot; i++) {
+    const Component& C = P[i];
+    R a = 0., d = 0., e = 0.;
+    if (C.type == kDehnenBar) {
+      double amp = C.p[0], tform = C.p[1], tsteady = C.p[2], rb = C.p[3], omegab = C.p[4], barphi = C.p[5];
+      R smooth = dehnen_smooth(t, tform, tsteady);
+      R ang = R(2.) * (phi - R(omegab) * t - R(barphi));
+      if (Rc <= R(rb)) {
+        a = R(6.) * R(amp) * smooth * cos(ang) * pow(Rc / R(rb), R(3.)) / Rc / Rc;
+        d = -R(4.) * R(amp) * smooth * cos(ang) * (pow(Rc / R(rb), R(3.)) - R(2.));
+        e = -R(6.) * R(amp) * smooth * sin(ang) * pow(Rc / R(rb), R(3.)) / Rc;
+      } else {
+        a = -R(12.) * R(amp) * smooth * cos(ang) * pow(R(rb) / Rc, R(3.)) / Rc / Rc;
+        d = R(4.) * R(amp) * smooth * cos(ang) * pow(R(rb) / Rc, R(3.));
+        e = -R(6.) * R(amp) * smooth * sin(ang) * pow(R(rb) / Rc, R(3.)) / Rc;
+      }
+    } else if (C.type == kLogHalo && C.p[3] < 1) {
+      double amp = C.p[0], c = C.p[2], onem1overb2 = C.p[3];
+      R Rt2 = Rc * Rc * (R(1.) - R(onem1overb2) * pow(sin(phi), R(2)));
+      a = R(amp) * (R(1.) - R(2.) * Rt2 / (Rt2 + R(c))) / (Rt2 + R(c)) * Rt2 / Rc / Rc;
+      d = -R(amp) * R(onem1overb2) *
+          (R(0.5) * pow(Rc * Rc * sin(R(2.) * phi), R(2.)) * R(onem1overb2) / (Rt2 + R(c)) / (Rt2 + R(c)) +
+           Rc * Rc / (Rt2 + R(c)) * cos(R(2.) * phi));
+      e = -R(amp) * R(c) / (Rt2 + R(c)) / (Rt2 + R(c)) * Rc * sin(R(2.) * phi) * R(onem1overb2);
+    } else {
+      R z2d, Rzd;
+      if (!cyl_hessian(1, &C, Rc, R(0.), a, z2d, Rzd)) return false;
+    }
+    R2 += a; phi2 += d; Rphi += e;
+  }
+  return true;
+}
+
 // evalRectForce_dxdv, integrateFullOrbit.c:1619-1681: base acceleration in
 // a[0..2] (as rect_force) and, per deviation column j, K dq_j with K the
-// Cartesian Hessian of -Phi assembled from the cylindrical second derivatives
-// (axisymmetric: the phitorque and phi-derivative terms vanish).
+// Cartesian Hessian of -Phi assembled from the six cylindrical second
+// derivatives and the (conservative) forces, in the reference's form.
 template <class R>
 bool rect_force_dxdv(int npot, const Component* P, R t, const R* q, R* a, int nde) {
   rect_force(npot, P, t, q, a);
@@ -876,18 +995,26 @@ bool rect_force_dxdv(int npot, const Component* P, R t, const R* q, R* a, int nd
   R x = q[0], y = q[1], z = q[2];
   R Rc = sqrt(x * x + y * y);
   R sinphi = y / Rc, cosphi = x / Rc;
-  R R2d, z2d, Rzd;
-  if (!cyl_hessian(npot, P, Rc, z, R2d, z2d, Rzd)) return false;
   R phi = acos(x / Rc);
   if (y < R(0.)) phi = R(2. * M_PI) - phi;
-  R FR, Fz, Fphi;
-  cyl_forces(npot, P, Rc, z, phi, t, FR, Fz, Fphi);
-  R dFxdx = -cosphi * cosphi * R2d + sinphi * sinphi / Rc * FR;
-  R dFxdy = -sinphi * cosphi * R2d - cosphi * sinphi / Rc * FR;
-  R dFydy = -sinphi * sinphi * R2d + cosphi * cosphi / Rc * FR;
-  R dFxdz = -cosphi * Rzd;
-  R dFydz = -sinphi * Rzd;
-  R dFzdz = -z2d;
+  R R2deriv, z2deriv, Rzderiv, phi2deriv, Rphideriv, zphideriv;
+  if (!cyl_hessian_full(npot, P, Rc, z, phi, t, R2deriv, z2deriv, Rzderiv, phi2deriv, Rphideriv,
+                        zphideriv))
+    return false;
+  R RforceK, zforce, phitorqueK;
+  cyl_forces(npot, P, Rc, z, phi, t, RforceK, zforce, phitorqueK);
+  R dFxdx = -cosphi * cosphi * R2deriv + R(2.) * cosphi * sinphi / Rc / Rc * phitorqueK +
+            sinphi * sinphi / Rc * RforceK + R(2.) * sinphi * cosphi / Rc * Rphideriv -
+            sinphi * sinphi / Rc / Rc * phi2deriv;
+  R dFxdy = -sinphi * cosphi * R2deriv + (sinphi * sinphi - cosphi * cosphi) / Rc / Rc * phitorqueK -
+            cosphi * sinphi / Rc * RforceK - (cosphi * cosphi - sinphi * sinphi) / Rc * Rphideriv +
+            cosphi * sinphi / Rc / Rc * phi2deriv;
+  R dFydy = -sinphi * sinphi * R2deriv - R(2.) * sinphi * cosphi / Rc / Rc * phitorqueK -
+            R(2.) * sinphi * cosphi / Rc * Rphideriv + cosphi * cosphi / Rc * RforceK -
+            cosphi * cosphi / Rc / Rc * phi2deriv;
+  R dFxdz = -cosphi * Rzderiv + sinphi / Rc * zphideriv;
+  R dFydz = -sinphi * Rzderiv - cosphi / Rc * zphideriv;
+  R dFzdz = -z2deriv;
   for (int j = 1; j <= nde; j++) {
     R dx = q[3 * j], dy = q[3 * j + 1], dz = q[3 * j + 2];
     a[3 * j] = dFxdx * dx + dFxdy * dy + dFxdz * dz;
@@ -1040,12 +1167,13 @@ void planar_rect_deriv(int npot, const Component* P, R t, const R* q, R* a) {
   planar_rect_force(npot, P, t, q, a + 2);
 }
 
-// Components with a restated Hessian (cyl_hessian): time-independent and
-// axisymmetric.
+// Components with a restated Hessian (cyl_hessian_full, planar_hessian):
+// the axisymmetric ones, the bar and the triaxial halo.
 inline bool has_hessian(int npot, const Component* P) {
   for (int i = 0; i < npot; i++) {
     const int c = P[i].type;
-    if (c == kDehnenBar || c == kKKStaeckel || c == kSoftNeedleBar || (c == kLogHalo && P[i].p[3] < 1) || is_wrapper(c))
+    if (c == kKKStaeckel || c == kSoftNeedleBar || is_wrapper(c) || c == kEllipDisk || c == kLopsided ||
+        c == kSteadySpiral)
       return false;
   }
   return true;
@@ -1071,9 +1199,7 @@ bool rect_deriv_dxdv(int npot, const Component* P, R t, const R* q, R* a) {
 // evalPlanarRectForce_dxdv, integratePlanarOrbit.c:1228-1275: planar
 // acceleration and, per deviation column, the kick tangent K (dx, dy).  galpy
 // wires calcPlanarR2deriv / phi2deriv / Rphideriv to each component's planar
-// second derivatives, which for the axisymmetric components here are the
-// full ones in the plane (R2deriv at z = 0) with vanishing phi terms; the
-// bar and the triaxial halo have none here (returns false).
+// second derivatives (planar_hessian).
 template <class R>
 bool planar_rect_force_dxdv(int npot, const Component* P, R t, const R* q, R* a, int nde) {
   R x = q[0], y = q[1];
@@ -1087,9 +1213,8 @@ bool planar_rect_force_dxdv(int npot, const Component* P, R t, const R* q, R* a,
   a[0] = cosphi * Rforce - R(1.) / Rc * sinphi * phitorque;
   a[1] = sinphi * Rforce + R(1.) / Rc * cosphi * phitorque;
   if (nde == 0) return true;
-  R R2deriv, z2d, Rzd;
-  if (!cyl_hessian(npot, P, Rc, R(0.), R2deriv, z2d, Rzd)) return false;
-  const R phi2deriv = 0., Rphideriv = 0.;
+  R R2deriv, phi2deriv, Rphideriv;
+  if (!planar_hessian(npot, P, Rc, phi, t, R2deriv, phi2deriv, Rphideriv)) return false;
   R dFxdx = -cosphi * cosphi * R2deriv + R(2.) * cosphi * sinphi / Rc / Rc * phitorque +
             sinphi * sinphi / Rc * Rforce + R(2.) * sinphi * cosphi / Rc * Rphideriv -
             sinphi * sinphi / Rc / Rc * phi2deriv;

@@ -287,6 +287,33 @@ ORC_EXPORT int oracle_eval_hessian(int npot, int* pot_type, double* pot_args, in
   return 0;
 }
 
+// All six cylindrical second derivatives (calcR2deriv, calcz2deriv,
+// calcRzderiv, calcphi2deriv, calcRphideriv, calczphideriv) at (R, z, phi, t).
+ORC_EXPORT int oracle_eval_hessian_full(int npot, int* pot_type, double* pot_args, int n,
+                                        const double* R, const double* z, const double* phi,
+                                        const double* t, double* R2, double* z2, double* Rz,
+                                        double* phi2, double* Rphi, double* zphi) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  for (int i = 0; i < n; i++)
+    if (!cyl_hessian_full<double>(npot, P, R[i], z[i], phi[i], t[i], R2[i], z2[i], Rz[i], phi2[i],
+                                  Rphi[i], zphi[i]))
+      return -1;
+  return 0;
+}
+
+// calcPlanarR2deriv, calcPlanarphi2deriv, calcPlanarRphideriv at (R, phi, t)
+// (planar packing).
+ORC_EXPORT int oracle_eval_planar_hessian(int npot, int* pot_type, double* pot_args, int n,
+                                          const double* R, const double* phi, const double* t,
+                                          double* R2, double* phi2, double* Rphi) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P, true) != 0) return -1;
+  for (int i = 0; i < n; i++)
+    if (!planar_hessian<double>(npot, P, R[i], phi[i], t[i], R2[i], phi2[i], Rphi[i])) return -1;
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // integrate_dxdv: the orbit plus one phase-space deviation, rectangular rows
 // (q, p | dq, dp) of 12 (full) or 8 (planar) doubles; symplectic methods by
@@ -304,6 +331,7 @@ void dxdv_one(int npot, const Component* P, int odeint_type, double* yo, int nt,
   switch (odeint_type) {
     case 0:
     case 3:
+    case 4:
       ok = symplectic_dxdv_traj<DB, R>(npot, P, odeint_type, yo, 1, nt, t, dt, rtol, atol,
                                        nullptr, result);
       if (st) {  // orbit-steps: the step estimate again, in plain (uncounted) doubles
@@ -314,15 +342,22 @@ void dxdv_one(int npot, const Component* P, int odeint_type, double* yo, int nt,
         const double init_dt = t[1] - t[0];
         double d = dt;
         if (dt == kDtSentinel)
-          d = (odeint_type == 0) ? leapfrog_estimate_step<DB>(F, qo, po, init_dt, t[0], rtol, atol)
-                                 : symplec4_estimate_step<DB>(F, qo, po, init_dt, t[0], rtol, atol);
+          d = (odeint_type == 0)   ? leapfrog_estimate_step<DB>(F, qo, po, init_dt, t[0], rtol, atol)
+              : (odeint_type == 4) ? symplec6_estimate_step<DB>(F, qo, po, init_dt, t[0], rtol, atol)
+                                   : symplec4_estimate_step<DB>(F, qo, po, init_dt, t[0], rtol, atol);
         st->steps = (long long)(init_dt / d) * (nt - 1);
       }
       break;
+    case 1:  // RK4, RK6, DOPR54 and DOP853 on the variational system
+    case 2:  // (evalRectDeriv_dxdv, integrateFullOrbit.c:1272-1296)
+    case 5:
     case 6: {
       long long nrhs = 0;
       ForceCtx<R, 2 * DB> F{npot, P, &nrhs};
-      dop853<2 * DB>(F, yo, nt, t, rtol, atol, result, err, st);
+      if (odeint_type == 1) rk_fixed<4, 2 * DB>(F, yo, nt, dt, t, rtol, atol, result, err, st);
+      else if (odeint_type == 2) rk_fixed<6, 2 * DB>(F, yo, nt, dt, t, rtol, atol, result, err, st);
+      else if (odeint_type == 5) dopr54<2 * DB>(F, yo, nt, dt, t, rtol, atol, result, err, st);
+      else dop853<2 * DB>(F, yo, nt, t, rtol, atol, result, err, st);
       break;
     }
     default:

@@ -376,3 +376,32 @@ def cyl_to_rect(v):
     R, vR, vT, z, vz, phi = v.T
     c, s = numpy.cos(phi), numpy.sin(phi)
     return numpy.stack([R * c, R * s, z, vR * c - vT * s, vR * s + vT * c, vz], axis=1)
+
+
+def oracle_hessian_full(pot_type, pot_args, R, z, phi, t):
+    """The oracle's six cylindrical second derivatives (R2, z2, Rz, phi2,
+    Rphi, zphi) at (R, z, phi, t); full packing."""
+    lib = oracle()
+    arrs = [numpy.ascontiguousarray(numpy.broadcast_to(v, numpy.shape(R)), dtype=float) for v in (R, z, phi, t)]
+    n = arrs[0].size
+    out = [numpy.empty(n) for _ in range(6)]
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    rc = lib.oracle_eval_hessian_full(npot_of(pt, pa), _p(pt), _p(pa), n, *[_p(a) for a in arrs],
+                                      *[_p(o) for o in out])
+    assert rc == 0
+    return tuple(out)
+
+
+def oracle_planar_hessian(pot_type, pot_args, R, phi, t):
+    """The oracle's planar second derivatives (R2, phi2, Rphi); planar packing."""
+    lib = oracle()
+    arrs = [numpy.ascontiguousarray(numpy.broadcast_to(v, numpy.shape(R)), dtype=float) for v in (R, phi, t)]
+    n = arrs[0].size
+    out = [numpy.empty(n) for _ in range(3)]
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    rc = lib.oracle_eval_planar_hessian(npot_of(pt, pa, True), _p(pt), _p(pa), n, *[_p(a) for a in arrs],
+                                        *[_p(o) for o in out])
+    assert rc == 0
+    return tuple(out)

@@ -126,14 +126,42 @@ def test_adjoint_rejects_potentials_without_device_hessian():
     tab = _RawTable(pt, pa)
     w = torch.ones((4, 6), dtype=torch.float64, device="cuda")
     with pytest.raises(RuntimeError, match="adjoint"):
-        adjoint_raw(tab, w, w, DT, 10)
-    # a triaxial halo exerts a torque; the device Hessian is the axisymmetric
-    # one, so it is refused rather than differentiated wrongly
+        adjoint_raw(tab, w, w, DT, 10)  # the bar: time-dependent
+
+
+@pytest.mark.parametrize("method,code", [("symplec4_hip", 3), ("leapfrog_hip", 0)])
+def test_ic_and_amp_gradient_triaxial_halo(method, code):
+    """A triaxial logarithmic halo (time-independent, exerting a torque):
+    dL/dw0 = M^T dL/dwT with M from the oracle's tangent map (its Hessian has
+    the phi derivatives of LogarithmicHaloPotential.c:157-283), and dL/damp
+    equals central differences (h = 1e-6)."""
+    import torch
+
+    from galpy_amd.adjoint import ParamSpec, _RawTable, flow_raw, orbit_flow
     from galpy_amd.potential import LogarithmicHaloPotential
 
-    _, pt, pa, _ = parse_pot([LogarithmicHaloPotential(q=0.9, b=0.8)])
-    with pytest.raises(RuntimeError, match="axisymmetric"):
-        adjoint_raw(_RawTable(pt, pa), w, w, DT, 10)
+    pot = [LogarithmicHaloPotential(q=0.9, b=0.8)]
+    spec = ParamSpec(pot, [(0, 0)])
+    w0 = cyl_to_rect(ics(64, seed=44))
+    nsteps = 300
+    theta = torch.tensor(spec.theta0(), device="cuda", requires_grad=True)
+    w = torch.tensor(w0, device="cuda").requires_grad_(True)
+    target = torch.tensor(w0 + numpy.random.default_rng(6).normal(0, 0.05, w0.shape), device="cuda")
+    _loss(orbit_flow(w, theta, spec, DT, nsteps, method), target).backward()
+    t = numpy.arange(nsteps + 1) * DT
+    M, wT_ref = oracle_stm(spec.pot_type, spec.pot_args, w0, t, code, dt=DT)
+    lamT = 2.0 * (wT_ref - target.cpu().numpy())
+    want = numpy.einsum("nij,ni->nj", M, lamT)
+    got = w.grad.cpu().numpy()
+    assert numpy.abs(got - want).max() <= 1e-8 * numpy.abs(want).max()
+    h = 1e-6
+
+    def L(th):
+        return _loss(flow_raw(_RawTable(*spec.packed(numpy.array([th]))), torch.tensor(w0, device="cuda"), DT,
+                              nsteps, method), target).item()
+
+    fd = (L(spec.theta0()[0] + h) - L(spec.theta0()[0] - h)) / (2 * h)
+    assert abs(theta.grad.item() - fd) <= 1e-6 * abs(fd)
 
 
 @pytest.mark.parametrize("method,code", [("symplec4_hip", 3), ("leapfrog_hip", 0)])

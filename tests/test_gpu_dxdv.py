@@ -2,9 +2,10 @@
 integratePlanarOrbit_dxdv, gh_integrate_dxdv, gh_integrate_dxdv_hip) against
 the oracle's integrate_dxdv on the same seeded inputs.
 
-Bars: symplectic steppers 1e-10 on every row of the orbit and the deviation
-(relative, unit floor) with the oracle's per-orbit step; DOP853 on the
-variational system 1e-8.  At 65,536 orbits: the state transition matrices
+Bars: the fixed-step methods (leapfrog, symplec4, symplec6 by their tangent
+maps; RK4, RK6 on the variational system) 1e-10 on every row of the orbit and
+the deviation (relative, unit floor) with the oracle's per-orbit step; the
+adaptive ones (DOPR54, DOP853 on the variational system) 1e-8.  At 65,536 orbits: the state transition matrices
 built from the GPU's planar tangent maps are symplectic, and the full tangent
 map in the plane equals the planar one."""
 
@@ -97,11 +98,35 @@ def test_dxdv_generic_kernel_and_single_orbit(dim):
             assert numpy.array_equal(one, res[5])
 
 
-def test_dxdv_rejects_potentials_without_hessian():
-    from galpy_amd import _lib
+@pytest.mark.parametrize("dim", [3, 2])
+@pytest.mark.parametrize("code", [0, 1, 2, 3, 4, 5, 6])
+def test_dxdv_every_method_rotating_bar_and_triaxial_halo(code, dim):
+    """Every integrator galpy routes integrate_dxdv through
+    (integrateFullOrbit.c:1272-1296), with the time-dependent bar of the
+    LogarithmicHalo + DehnenBar workload and with a triaxial halo (whose
+    Hessians have phi derivatives): fixed steps (dt = 0.01) to 1e-10, the
+    adaptive methods at galpy's default tolerances to 1e-8, orbit and
+    deviation, every row."""
     from galpy_amd.potential import LogarithmicHaloPotential
 
+    names = {0: "leapfrog_hip", 1: "rk4_hip", 2: "rk6_hip", 3: "symplec4_hip", 4: "symplec6_hip",
+             5: "dopr54_hip", 6: "dop853_hip"}
+    fixed = code in (0, 1, 2, 3, 4)
     for pot in (bar_potential(), [LogarithmicHaloPotential(q=0.9, b=0.8)]):
+        y0, d0 = _inputs(dim, 96, seed=7 + code)
+        t = numpy.linspace(0.0, 5.0, 51)
+        res, err = _hip(dim, pot, y0, d0, t, names[code], dt=0.01 if fixed else None)
+        ref, rerr = oracle_integrate_dxdv(*_packed(pot, dim == 2), numpy.hstack([y0, d0]), t, code,
+                                          dim=dim, dt=0.01 if fixed else None)
+        assert numpy.all(err == rerr)
+        assert _rel(res, ref) < (1e-10 if fixed else 1e-8), (code, dim, _rel(res, ref))
+
+
+def test_dxdv_rejects_potentials_without_hessian():
+    from galpy_amd import _lib
+    from galpy_amd.potential import KuzminKutuzovStaeckelPotential, SoftenedNeedleBarPotential
+
+    for pot in ([KuzminKutuzovStaeckelPotential(ac=5.0, Delta=0.5)], [SoftenedNeedleBarPotential(amp=0.1)]):
         y0, d0 = _inputs(3, 2, seed=4)
         with pytest.raises(RuntimeError):
             _hip(3, pot, y0, d0, numpy.linspace(0, 1, 3), "symplec4_hip")
