@@ -196,9 +196,66 @@ int fit_upper_gamma_table(double a, double xbig, double tab[GH_TAB_MAXSEG][GH_TA
   return e0;
 }
 
+bool is_ellipsoidal(int code) {
+  return code == GH_TRIAXHERNQUIST || code == GH_TRIAXNFW || code == GH_TRIAXJAFFE ||
+         code == GH_TWOPOWERTRIAX || code == GH_PERFECTELLIPSOID || code == GH_TRIAXGAUSSIAN ||
+         code == GH_POWERTRIAX;
+}
+
+// An EllipsoidalPotential (integrateFullOrbit.py:221-300): amp, 15 cache
+// slots, npsi, the psi arguments, b^2, c^2, aligned, rot[9], glorder, the
+// nodes s_k and galpy's weights.  Keeps amp and the psi arguments in p, the
+// rotation and per node (1/(1+tau), 1/(b^2+tau), 1/(c^2+tau), w) in ext,
+// tau = 1/s^2 - 1 as EllipsoidalPotential.c:64 forms it.
+int decode_ellipsoidal(int code, const double*& pot_args, GhPotTable* T, int i) {
+  GhComponent& C = T->comp[i];
+  const double* a = pot_args;
+  const int npsi = (int)a[16];
+  if (npsi < 1 || npsi > 7) {
+    set_error("pot_type %d: %d psi arguments", code, npsi);
+    return -1;
+  }
+  const double* e = a + 17 + npsi;
+  const double b2 = e[0], c2 = e[1];
+  const bool aligned = e[2] != 0.0;
+  const int glorder = (int)e[12];
+  if (glorder < 1 || T->next + 9 + 4 * glorder > GH_EXT_LEN) {
+    set_error("pot_type %d: glorder %d does not fit the table (%d doubles left)", code, glorder,
+              GH_EXT_LEN - T->next - 9);
+    return -1;
+  }
+  C.type = code;
+  C.p[0] = a[0];
+  for (int j = 0; j < npsi; j++) C.p[1 + j] = a[17 + j];
+  C.c[0] = b2;
+  C.c[1] = c2;
+  C.c[2] = aligned ? 1.0 : 0.0;
+  C.c[3] = glorder;
+  if (code == GH_TWOPOWERTRIAX) C.c[4] = 1.0 / C.p[1];
+  C.ext = T->next;
+  double* x = T->ext + T->next;
+  for (int j = 0; j < 9; j++) x[j] = aligned ? (j % 4 == 0 ? 1.0 : 0.0) : e[3 + j];
+  const double* glx = e + 13;
+  const double* glw = glx + glorder;
+  for (int k = 0; k < glorder; k++) {
+    const double tau = 1.0 / glx[k] / glx[k] - 1.0;
+    x[9 + 4 * k] = 1.0 / (1.0 + tau);
+    x[10 + 4 * k] = 1.0 / (b2 + tau);
+    x[11 + 4 * k] = 1.0 / (c2 + tau);
+    x[12 + 4 * k] = glw[k];
+  }
+  T->next += 9 + 4 * glorder;
+  T->has_wrap = 1;  // runs in the extended dispatch instances
+  C.nonaxi = !aligned || std::fabs(std::sqrt(b2) - 1.0) > 1e-10;
+  T->nonaxi |= C.nonaxi;
+  pot_args += 30 + npsi + 2 * glorder;
+  return 0;
+}
+
 // One (leaf) component's arguments and precomputed constants.
 int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
                      bool planar) {
+  if (is_ellipsoidal(code)) return decode_ellipsoidal(code, pot_args, T, i);
   const int na = nargs_of(code, planar);
   if (na < 0 || code < 0) {
     set_error("pot_type %d (component %d) has no HIP kernel", code, i);
@@ -362,7 +419,7 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
 
 int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPotTable* T,
                      bool planar = false) {
-  std::memset(T, 0, sizeof(*T));
+  std::memset(T, 0, offsetof(GhPotTable, ext));  // ext is read only up to next
   if (npot < 1 || npot > GH_MAX_POT) {
     set_error("npot=%d outside [1, %d]", npot, GH_MAX_POT);
     return -1;
@@ -596,7 +653,7 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
       set_error("device allocation failed (chunk of %ld orbits, nt=%d)", chunk, nt);
       break;
     }
-    if (hipMemcpyAsync(d_table, &table, sizeof(GhPotTable), hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
+    if (hipMemcpyAsync(d_table, &table, gh_table_bytes(table), hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
     handle.dev = (GhPotTable*)d_table;
     if (!indiv_t &&
         hipMemcpyAsync(d_t, t, (size_t)nt * 8, hipMemcpyHostToDevice, stream) != hipSuccess) { rc = -1; break; }
@@ -791,7 +848,7 @@ static int upload_potential(int npot, const int* pot_type, const double* pot_arg
   if (decode_potential(npot, pot_type, pot_args, &table, planar) != 0) return -1;
   void* d = nullptr;
   GH_CHECK(hipMalloc(&d, sizeof(GhPotTable)));
-  GH_CHECK(hipMemcpy(d, &table, sizeof(GhPotTable), hipMemcpyHostToDevice));
+  GH_CHECK(hipMemcpy(d, &table, gh_table_bytes(table), hipMemcpyHostToDevice));
   GhPotHandle* h = new GhPotHandle;
   h->dev = (GhPotTable*)d;
   fill_handle(table, h);

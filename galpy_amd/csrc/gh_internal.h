@@ -50,7 +50,7 @@ struct GhPotHandle {
   int npot;
   int types[GH_MAX_POT];
   int nonaxi[GH_MAX_POT];  // component exerts a phi torque (bar, triaxial halo)
-  int has_wrap;            // wrapped components: run-time dispatch, no Hessian
+  int has_wrap;            // wrapped or heavy components: extended dispatch, no Hessian
 };
 
 inline void fill_handle(const GhPotTable& table, GhPotHandle* h) {
@@ -76,10 +76,13 @@ inline bool types_are(const GhPotHandle* h, std::initializer_list<int> codes) {
 inline bool hessian_capable(const GhPotHandle* h) {
   if (h->has_wrap) return false;  // time-dependent (and rotating) wrappers
   for (int i = 0; i < h->npot; i++) {
-    const int c = h->types[i];
-    if (c == GH_KKSTAECKEL || c == GH_SOFTNEEDLEBAR || c == GH_ELLIPDISK || c == GH_LOPSIDED ||
-        c == GH_STEADYSPIRAL)
-      return false;
+    switch (h->types[i]) {  // the components with a Hessian-vector product
+      case GH_POWERSPHCUTOFF: case GH_MIYAMOTONAGAI: case GH_NFW: case GH_HERNQUIST:
+      case GH_LOGHALO: case GH_DEHNENBAR: case GH_POWERSPH: case GH_JAFFE: case GH_ISOCHRONE:
+      case GH_PLUMMER: case GH_PSEUDOISO: case GH_KUZMINDISK: case GH_FLATPOWER: case GH_BURKERT:
+      case GH_DEHNENCORESPH: case GH_DEHNENSPH: case GH_HOMOGSPHERE: case GH_NULLPOT: break;
+      default: return false;
+    }
   }
   return true;
 }

@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "gh_math.h"
 
 #define GH_MAX_POT 8
@@ -26,6 +28,9 @@
 #define GH_TAB_MAXSEG 64
 #define GH_TAB_NC 11
 #define GH_TAB_SLOTS 2  // PowerSphericalPotentialwCutoff components with a table
+// variable-length per-component data (quadrature nodes, rotations): doubles
+// shared by all components of a table
+#define GH_EXT_LEN 8192
 
 enum GhPotCode {
   GH_LOGHALO = 0,
@@ -51,6 +56,14 @@ enum GhPotCode {
   GH_STEADYSPIRAL = 3,  // planar only (integratePlanarOrbit's code 3)
   GH_SOFTNEEDLEBAR = 25,
   GH_NULLPOT = 40,
+  // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
+  GH_TRIAXHERNQUIST = 21,
+  GH_TRIAXNFW = 22,
+  GH_TRIAXJAFFE = 23,
+  GH_PERFECTELLIPSOID = 30,
+  GH_TRIAXGAUSSIAN = 37,
+  GH_POWERTRIAX = 38,
+  GH_TWOPOWERTRIAX = 43,
   // wrappers (integrateFullOrbit.c:821-895)
   GH_DEHNENSMOOTHWRAP = -1,
   GH_SOLIDBODYROTWRAP = -2,
@@ -63,6 +76,7 @@ struct GhComponent {
   int nonaxi;    // 1 if the component exerts a phi torque
   int nwrap;     // wrappers around this component
   int wrap[GH_WRAP_DEPTH];  // their indices in GhPotTable::wrap, outermost first
+  int ext;       // offset of the component's variable-length data in GhPotTable::ext
   double p[8];   // raw galpy arguments, in galpy's order
   double c[14];  // precomputed constants, per type (decoded on the host)
 };
@@ -81,7 +95,7 @@ struct GhWrap {
 struct GhPotTable {
   int npot;
   int nonaxi;    // some component needs phi
-  int has_wrap;  // some component is wrapped (run-time dispatch only)
+  int has_wrap;  // some component is wrapped or heavy (extended dispatch only)
   int nwraps;
   GhComponent comp[GH_MAX_POT];
   GhWrap wrap[GH_MAX_WRAP];
@@ -92,7 +106,16 @@ struct GhPotTable {
   // Gamma(a,x) tables for the force exponent of up to GH_TAB_SLOTS
   // PowerSphericalPotentialwCutoff components (slot in comp.c[11])
   double tab[GH_TAB_SLOTS][GH_TAB_MAXSEG][GH_TAB_NC];
+  // variable-length data, last so that a table is uploaded up to next
+  int next;  // doubles of ext in use
+  int pad;
+  double ext[GH_EXT_LEN];
 };
+
+// bytes of a table that carry data (the prefix a host-to-device copy needs)
+__host__ __device__ inline size_t gh_table_bytes(const GhPotTable& T) {
+  return offsetof(GhPotTable, ext) + sizeof(double) * (size_t)T.next;
+}
 
 // ---------------------------------------------------------------------------
 // Lower incomplete gamma gamma(a, x) (replaces GSL's gsl_sf_gamma_inc_P and
@@ -685,6 +708,140 @@ __device__ __forceinline__ void gh_comp_force<GH_FLATPOWER>(const GhPotTable* __
   F.Fz += f * g.z * iq2;
 }
 
+// ---------------------------------------------------------------------------
+// EllipsoidalPotential family: the Gauss-Legendre force sum of
+// EllipsoidalPotentialxyzforces_xyz (EllipsoidalPotential.c:40-77) in the
+// frame of the ellipsoid,
+//   F = amp sum_k w_k rho(m_k) (x u1_k, y u2_k, z u3_k),
+//   m_k^2 = x^2 u1_k + y^2 u2_k + z^2 u3_k,
+// u1 = 1/(1+tau), u2 = 1/(b^2+tau), u3 = 1/(c^2+tau), tau = 1/s_k^2 - 1,
+// w_k galpy's packed weight (-4 pi b c glw / D folded in), rotated back by
+// the transpose of the frame's rotation (galpy_potentials.c:391-410).  The
+// host stores per node (u1, u2, u3, w), so the loop multiplies where galpy
+// divides, and the node data of a wavefront are one scalar load stream.
+// Densities rho(m) (TwoPowerTriaxialPotential.c:13-117,
+// PerfectEllipsoidPotential.c:9-20, TriaxialGaussianPotential.c:8-16,
+// PowerTriaxialPotential.c:9-20), written on m^2:
+//   21 TriaxialHernquist  a^4 / (m (m+a)^3)     p = amp, a, a^4
+//   22 TriaxialNFW        a^3 / (m (m+a)^2)     p = amp, a, a^3
+//   23 TriaxialJaffe      a^4 / (m (m+a))^2     p = amp, a, a^2
+//   43 TwoPowerTriaxial   (a/m)^alpha (1+m/a)^(alpha-beta)
+//                          p = amp, a, alpha, beta, beta-alpha, ...
+//   30 PerfectEllipsoid   1 / (a^2 + m^2)^2     p = amp, a^2
+//   37 TriaxialGaussian   exp(m^2 / (-2 sigma^2))  p = amp, -2 sigma^2
+//   38 PowerTriaxial      m^-alpha              p = amp, alpha
+// c[0] = b^2, c[1] = c^2, c[2] = aligned, c[3] = glorder, c[4] = 1/a for 43;
+// ext = rotation (9), then (u1, u2, u3, w) per node.
+// ---------------------------------------------------------------------------
+template <int TYPE>
+__device__ __forceinline__ double gh_ellip_mdens(const GhComponent* __restrict__ C, double m2) {
+  if constexpr (TYPE == GH_TRIAXHERNQUIST || TYPE == GH_TRIAXNFW || TYPE == GH_TRIAXJAFFE) {
+    const double im = gh_rsqrt(m2);
+    const double mpa = m2 * im + C->p[1];
+    if constexpr (TYPE == GH_TRIAXHERNQUIST) return C->p[2] * im * gh_rcp(mpa * mpa * mpa);
+    if constexpr (TYPE == GH_TRIAXNFW) return C->p[2] * im * gh_rcp(mpa * mpa);
+    const double q = im * gh_rcp(mpa);
+    return C->p[2] * C->p[2] * (q * q);
+  } else if constexpr (TYPE == GH_TWOPOWERTRIAX) {
+    const double moa = sqrt(m2) * C->c[4];
+    return exp(-C->p[2] * log(moa) - C->p[4] * log1p(moa));
+  } else if constexpr (TYPE == GH_PERFECTELLIPSOID) {
+    const double q = gh_rcp(C->p[1] + m2);
+    return q * q;
+  } else if constexpr (TYPE == GH_TRIAXGAUSSIAN) {
+    return exp(m2 / C->p[1]);
+  } else if constexpr (TYPE == GH_POWERTRIAX) {
+    return exp(-0.5 * C->p[1] * log(m2));
+  } else {
+    return 0.0;
+  }
+}
+
+// run-time member (generic kernels): one loop, a wavefront-uniform branch
+// per node
+__device__ __forceinline__ double gh_ellip_mdens_dyn(const GhComponent* __restrict__ C,
+                                                     double m2) {
+  switch (C->type) {
+    case GH_TRIAXHERNQUIST: return gh_ellip_mdens<GH_TRIAXHERNQUIST>(C, m2);
+    case GH_TRIAXNFW: return gh_ellip_mdens<GH_TRIAXNFW>(C, m2);
+    case GH_TRIAXJAFFE: return gh_ellip_mdens<GH_TRIAXJAFFE>(C, m2);
+    case GH_TWOPOWERTRIAX: return gh_ellip_mdens<GH_TWOPOWERTRIAX>(C, m2);
+    case GH_PERFECTELLIPSOID: return gh_ellip_mdens<GH_PERFECTELLIPSOID>(C, m2);
+    case GH_TRIAXGAUSSIAN: return gh_ellip_mdens<GH_TRIAXGAUSSIAN>(C, m2);
+    default: return gh_ellip_mdens<GH_POWERTRIAX>(C, m2);
+  }
+}
+
+// Cartesian force (times amp) of component C at (x, y, z); TYPE 0 picks the
+// member at run time.
+template <int TYPE>
+__device__ __forceinline__ void gh_ellip_xyz(const GhPotTable* __restrict__ T,
+                                             const GhComponent* __restrict__ C, double x,
+                                             double y, double z, double& Ax, double& Ay,
+                                             double& Az) {
+  const double* __restrict__ e = &T->ext[C->ext];
+  const bool aligned = C->c[2] != 0.0;
+  double xe = x, ye = y, ze = z;
+  if (!aligned) {
+    xe = e[0] * x + e[1] * y + e[2] * z;
+    ye = e[3] * x + e[4] * y + e[5] * z;
+    ze = e[6] * x + e[7] * y + e[8] * z;
+  }
+  const double x2 = xe * xe, y2 = ye * ye, z2 = ze * ze;
+  const double* __restrict__ nd = e + 9;
+  const int n = (int)C->c[3];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int k = 0; k < n; k++, nd += 4) {
+    const double u1 = nd[0], u2 = nd[1], u3 = nd[2];
+    const double m2 = fma(x2, u1, fma(y2, u2, z2 * u3));
+    const double rho = (TYPE == 0) ? gh_ellip_mdens_dyn(C, m2) : gh_ellip_mdens<TYPE>(C, m2);
+    const double td = nd[3] * rho;
+    fx = fma(td * u1, xe, fx);
+    fy = fma(td * u2, ye, fy);
+    fz = fma(td * u3, ze, fz);
+  }
+  if (!aligned) {
+    const double gx = e[0] * fx + e[3] * fy + e[6] * fz;
+    const double gy = e[1] * fx + e[4] * fy + e[7] * fz;
+    const double gz = e[2] * fx + e[5] * fy + e[8] * fz;
+    fx = gx;
+    fy = gy;
+    fz = gz;
+  }
+  const double amp = C->p[0];
+  Ax = amp * fx;
+  Ay = amp * fy;
+  Az = amp * fz;
+}
+
+// In the accumulated form: F_R / R = (x Ax + y Ay) / R^2, F_phi / R^2 =
+// (x Ay - y Ax) / R^2 (EllipsoidalPotentialRforce / phitorque,
+// EllipsoidalPotential.c:85-105, with cos phi = x / R, sin phi = y / R).
+template <int TYPE>
+__device__ __forceinline__ void gh_ellip_force(const GhPotTable* __restrict__ T, int i,
+                                               const GhGeom& g, GhAcc& F) {
+  double Ax, Ay, Az;
+  gh_ellip_xyz<TYPE>(T, &T->comp[i], g.x, g.y, g.z, Ax, Ay, Az);
+  const double iR2 = gh_rcp(g.R2);
+  F.FRoR += (g.x * Ax + g.y * Ay) * iR2;
+  F.FphioR2 += (g.x * Ay - g.y * Ax) * iR2;
+  F.Fz += Az;
+}
+#define GH_ELLIP_FORCE(CODE)                                                                  \
+  template <>                                                                                 \
+  __device__ __forceinline__ void gh_comp_force<CODE>(const GhPotTable* __restrict__ T, int i, \
+                                                      const GhGeom& g, GhAcc& F) {            \
+    gh_ellip_force<CODE>(T, i, g, F);                                                         \
+  }
+GH_ELLIP_FORCE(GH_TRIAXHERNQUIST)
+GH_ELLIP_FORCE(GH_TRIAXNFW)
+GH_ELLIP_FORCE(GH_TRIAXJAFFE)
+GH_ELLIP_FORCE(GH_TWOPOWERTRIAX)
+GH_ELLIP_FORCE(GH_PERFECTELLIPSOID)
+GH_ELLIP_FORCE(GH_TRIAXGAUSSIAN)
+GH_ELLIP_FORCE(GH_POWERTRIAX)
+#undef GH_ELLIP_FORCE
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_NULLPOT>(const GhPotTable* __restrict__, int,
                                                           const GhGeom&, GhAcc&) {}
@@ -716,6 +873,31 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
     case GH_DEHNENSPH: gh_comp_force<GH_DEHNENSPH>(T, i, g, F); break;
     case GH_HOMOGSPHERE: gh_comp_force<GH_HOMOGSPHERE>(T, i, g, F); break;
     default: break;
+  }
+}
+
+// Components whose forces are quadratures or expansions over many terms
+// (the EllipsoidalPotential family): inlined into the run-time dispatch
+// they would raise its register demand for every composition, so they run
+// only in the extended dispatch instances (with the wrappers).
+__host__ __device__ inline bool gh_is_heavy(int type) {
+  return type == GH_TRIAXHERNQUIST || type == GH_TRIAXNFW || type == GH_TRIAXJAFFE ||
+         type == GH_TWOPOWERTRIAX || type == GH_PERFECTELLIPSOID || type == GH_TRIAXGAUSSIAN ||
+         type == GH_POWERTRIAX;
+}
+
+// run-time dispatch over every component type (extended instances)
+__device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__ T, int i,
+                                                  const GhGeom& g, GhAcc& F) {
+  switch (T->comp[i].type) {
+    case GH_TRIAXHERNQUIST:
+    case GH_TRIAXNFW:
+    case GH_TRIAXJAFFE:
+    case GH_TWOPOWERTRIAX:
+    case GH_PERFECTELLIPSOID:
+    case GH_TRIAXGAUSSIAN:
+    case GH_POWERTRIAX: gh_ellip_force<0>(T, i, g, F); break;
+    default: gh_comp_force_dyn(T, i, g, F); break;
   }
 }
 
@@ -791,7 +973,7 @@ __device__ __forceinline__ void gh_comp_force_any(const GhPotTable* __restrict__
     corr = R * doff;
   }
   GhAcc L = {0.0, 0.0, 0.0};
-  gh_comp_force_dyn(T, i, gw, L);
+  gh_comp_force_ext(T, i, gw, L);
   // scalars per branch, F updated once (no field-address select)
   const double fr = wrapped ? fac * (L.FRoR - L.FphioR2 * corr) : L.FRoR;
   const double fz = wrapped ? fac * L.Fz : L.Fz;
@@ -820,9 +1002,9 @@ struct GhAccSum<GhSpec<>> {
   }
 };
 
-// Any composition with wrapped components: its own kernel instances, so that
-// the wrapper transform adds nothing to the run-time dispatch kernels of
-// unwrapped compositions.
+// Any composition with wrapped or heavy components: its own kernel
+// instances, so that neither the wrapper transform nor the quadrature
+// forces add to the run-time dispatch kernels of the other compositions.
 struct GhSpecWrapped {};
 
 template <>
@@ -908,6 +1090,49 @@ struct GhForce {
     ay = y * F.FRoR + x * F.FphioR2;
   }
 };
+
+// EllipsoidalPotentialEval (EllipsoidalPotential.c:6-39): -amp/2 sum_k w_k
+// psi(m_k) with galpy's packed weights; psi of each member
+// (TwoPowerTriaxialPotential.c:7-89, PerfectEllipsoid.c:4-8,
+// TriaxialGaussian.c:4-7, PowerTriaxial.c:4-8).  TwoPowerTriaxial's psi needs
+// the Gauss hypergeometric function and has no device form here (NaN).
+__device__ inline double gh_ellip_potential(const GhPotTable* __restrict__ T,
+                                            const GhComponent* __restrict__ C, double x, double y,
+                                            double z) {
+  const double* __restrict__ e = &T->ext[C->ext];
+  double xe = x, ye = y, ze = z;
+  if (C->c[2] == 0.0) {
+    xe = e[0] * x + e[1] * y + e[2] * z;
+    ye = e[3] * x + e[4] * y + e[5] * z;
+    ze = e[6] * x + e[7] * y + e[8] * z;
+  }
+  const double* __restrict__ nd = e + 9;
+  const int n = (int)C->c[3];
+  double out = 0.0;
+  for (int k = 0; k < n; k++, nd += 4) {
+    const double m2 = xe * xe * nd[0] + ye * ye * nd[1] + ze * ze * nd[2];
+    double psi;
+    switch (C->type) {
+      case GH_TRIAXHERNQUIST: {
+        const double mpa = sqrt(m2) + C->p[1];
+        psi = -C->p[2] / (mpa * mpa);
+        break;
+      }
+      case GH_TRIAXNFW: psi = -2.0 * C->p[2] / (sqrt(m2) + C->p[1]); break;
+      case GH_TRIAXJAFFE: {
+        const double m = sqrt(m2), mpa = m + C->p[1];
+        psi = 2.0 * C->p[2] * (C->p[1] / mpa + log(m / mpa));
+        break;
+      }
+      case GH_PERFECTELLIPSOID: psi = -1.0 / (C->p[1] + m2); break;
+      case GH_TRIAXGAUSSIAN: psi = C->p[1] * exp(m2 / C->p[1]); break;
+      case GH_POWERTRIAX: psi = 2.0 / (2.0 - C->p[1]) * pow(m2, 1.0 - 0.5 * C->p[1]); break;
+      default: psi = NAN; break;
+    }
+    out += nd[3] * psi;
+  }
+  return -0.5 * C->p[0] * out;
+}
 
 // Potential value (per-component Eval functions; DehnenBar from
 // DehnenBarPotential.py:173-200, which has no C Eval).
@@ -1066,6 +1291,13 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
         pot += out * amp * smooth * cang;
         break;
       }
+      case GH_TRIAXHERNQUIST:
+      case GH_TRIAXNFW:
+      case GH_TRIAXJAFFE:
+      case GH_TWOPOWERTRIAX:
+      case GH_PERFECTELLIPSOID:
+      case GH_TRIAXGAUSSIAN:
+      case GH_POWERTRIAX: pot += gh_ellip_potential(T, C, R * cosphi, R * sinphi, z); break;
       default:
         break;
     }

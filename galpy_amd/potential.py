@@ -991,6 +991,312 @@ class NullPotential(Potential):
         return []
 
 
+def _zvec_rotation(zvec):
+    """The matrix that turns the unit vector zvec into the z axis
+    (galpy/util/__init__.py:169-195 with inv=True): the transpose of
+    Rodrigues' rotation taking e_z to zvec about e_z x zvec; the identity when
+    zvec is e_z and minus the identity when it is -e_z, as galpy sets them."""
+    zvec = numpy.asarray(zvec, dtype=float)
+    zvec = zvec / numpy.sqrt(numpy.sum(zvec**2.0))
+    cth = zvec[2]
+    if numpy.fabs(cth - 1.0) < 1e-10:
+        return numpy.eye(3)
+    if numpy.fabs(cth + 1.0) < 1e-10:
+        return -numpy.eye(3)
+    k = numpy.cross([0.0, 0.0, 1.0], zvec)
+    k /= numpy.sqrt(numpy.sum(k**2.0))
+    K = numpy.array([[0.0, -k[2], k[1]], [k[2], 0.0, -k[0]], [-k[1], k[0], 0.0]])
+    sth = numpy.sqrt(1.0 - cth**2.0)
+    M = cth * numpy.eye(3) + sth * K + (1.0 - cth) * numpy.outer(k, k)
+    return M.T
+
+
+class EllipsoidalPotential(Potential):
+    """Densities constant on ellipsoids m^2 = x^2 + y^2/b^2 + z^2/c^2 of a
+    frame turned by pa about z and tilted so that zvec is its z axis
+    (EllipsoidalPotential.py:39-135).  With s in (0, 1), tau = 1/s^2 - 1 and
+    D(s) = sqrt((1 + (b^2-1) s^2)(1 + (c^2-1) s^2)), the potential and the
+    forces are Gauss-Legendre sums over s (EllipsoidalPotential.py:376-444):
+      Phi = 2 pi b c sum w psi(m_s) / D,  m_s^2 = x^2/(1+tau) + y^2/(b^2+tau)
+            + z^2/(c^2+tau),
+      F_x = -4 pi b c sum w rho(m_s) x / (1+tau) / D  (y, z alike),
+    with psi(m) = -int_m^inf dm'^2 rho(m').  galpy's packing: amp, 15 cache
+    slots, the number of psi arguments and those arguments, b^2, c^2,
+    aligned, the 3x3 rotation, glorder, the nodes and the weights with
+    -4 pi b c / D folded in (integrateFullOrbit.py:221-300)."""
+
+    isNonAxi = False
+
+    def __init__(self, amp=1.0, b=1.0, c=1.0, zvec=None, pa=None, glorder=50):
+        Potential.__init__(self, amp=amp)
+        self._b = float(b)
+        self._c = float(c)
+        self._b2 = self._b**2.0
+        self._c2 = self._c**2.0
+        if zvec is None and (pa is None or numpy.fabs(pa) < 1e-10):
+            self._aligned = True
+            self._rot = numpy.eye(3)
+        else:
+            self._aligned = False
+            pa_rot = numpy.eye(3)
+            if pa is not None:
+                pa_rot = numpy.array([[numpy.cos(pa), numpy.sin(pa), 0.0],
+                                      [-numpy.sin(pa), numpy.cos(pa), 0.0],
+                                      [0.0, 0.0, 1.0]])
+            zvec_rot = numpy.eye(3) if zvec is None else _zvec_rotation(zvec)
+            self._rot = numpy.dot(pa_rot, zvec_rot)
+        if glorder is None:
+            raise NotImplementedError("glorder=None (adaptive quadrature) has no HIP kernel")
+        self._glorder = int(glorder)
+        glx, glw = numpy.polynomial.legendre.leggauss(self._glorder)
+        self._glx = 0.5 * glx + 0.5
+        self._glw = 0.5 * glw
+        self.isNonAxi = (not self._aligned) or numpy.fabs(self._b - 1.0) > 1e-10
+
+    def _frame(self, R, z, phi):
+        x, y = R * numpy.cos(phi), R * numpy.sin(phi)
+        if self._aligned:
+            return x, y, z
+        v = numpy.dot(self._rot, numpy.array([x, y, z], dtype=float))
+        return v[0], v[1], v[2]
+
+    def _nodes(self, x, y, z):
+        s = self._glx
+        tau = 1.0 / s**2.0 - 1.0
+        D = numpy.sqrt((1.0 + (self._b2 - 1.0) * s**2.0) * (1.0 + (self._c2 - 1.0) * s**2.0))
+        m = numpy.sqrt(x**2.0 / (1.0 + tau) + y**2.0 / (self._b2 + tau) + z**2.0 / (self._c2 + tau))
+        return tau, D, m
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        x, y, zz = self._frame(float(R), float(z), float(phi))
+        _, D, m = self._nodes(x, y, zz)
+        return 2.0 * numpy.pi * self._b * self._c * numpy.sum(self._glw * self._psi(m) / D)
+
+    def _forces_xyz(self, R, z, phi):
+        x, y, zz = self._frame(float(R), float(z), float(phi))
+        tau, D, m = self._nodes(x, y, zz)
+        w = -4.0 * numpy.pi * self._b * self._c * self._glw * self._mdens(m) / D
+        F = numpy.array([numpy.sum(w * x / (1.0 + tau)), numpy.sum(w * y / (self._b2 + tau)),
+                         numpy.sum(w * zz / (self._c2 + tau))])
+        return F if self._aligned else numpy.dot(self._rot.T, F)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        F = self._forces_xyz(R, z, phi)
+        return numpy.cos(phi) * F[0] + numpy.sin(phi) * F[1]
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return self._forces_xyz(R, z, phi)[2]
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        F = self._forces_xyz(R, z, phi)
+        return R * (-numpy.sin(phi) * F[0] + numpy.cos(phi) * F[1])
+
+    def _psi_args(self):
+        raise NotImplementedError
+
+    def c_args(self):
+        s = self._glx
+        D = numpy.sqrt((1.0 + (self._b2 - 1.0) * s**2.0) * (1.0 + (self._c2 - 1.0) * s**2.0))
+        pa = list(self._psi_args())
+        return ([self._amp] + [0.0] * 15 + [float(len(pa))] + pa
+                + [self._b2, self._c2, float(int(self._aligned))] + list(self._rot.flatten())
+                + [float(self._glorder)] + list(self._glx)
+                + list(-4.0 * numpy.pi * self._glw * self._b * self._c / D))
+
+
+class TriaxialHernquistPotential(EllipsoidalPotential):
+    """rho = amp / (4 pi a^3) / ((m/a) (1 + m/a)^3); pot_type 21, psi
+    arguments [a, a^4] (TwoPowerTriaxialPotential.py:193-310)."""
+
+    pot_code = 21
+
+    def __init__(self, amp=1.0, a=2.0, normalize=False, b=1.0, c=1.0, zvec=None, pa=None,
+                 glorder=50):
+        EllipsoidalPotential.__init__(self, amp=amp, b=b, c=c, zvec=zvec, pa=pa, glorder=glorder)
+        self.a = a
+        self.a4 = a**4
+        self._amp /= 4.0 * numpy.pi * a**3
+        _set_normalize(self, normalize)
+
+    def _psi(self, m):
+        return -self.a4 / (m + self.a) ** 2.0
+
+    def _mdens(self, m):
+        return self.a4 / m / (m + self.a) ** 3.0
+
+    def _psi_args(self):
+        return [self.a, self.a4]
+
+
+class TriaxialNFWPotential(EllipsoidalPotential):
+    """rho = amp / (4 pi a^3) / ((m/a) (1 + m/a)^2); pot_type 22, psi
+    arguments [a, a^3] (TwoPowerTriaxialPotential.py:429-575; the conc/mvir
+    set-up is not offered)."""
+
+    pot_code = 22
+
+    def __init__(self, amp=1.0, a=2.0, b=1.0, c=1.0, zvec=None, pa=None, normalize=False,
+                 glorder=50):
+        EllipsoidalPotential.__init__(self, amp=amp, b=b, c=c, zvec=zvec, pa=pa, glorder=glorder)
+        self.a = a
+        self.a3 = a**3
+        self._amp /= 4.0 * numpy.pi * self.a3
+        _set_normalize(self, normalize)
+
+    def _psi(self, m):
+        return -2.0 * self.a3 / (m + self.a)
+
+    def _mdens(self, m):
+        return self.a3 / m / (m + self.a) ** 2.0
+
+    def _psi_args(self):
+        return [self.a, self.a3]
+
+
+class TriaxialJaffePotential(EllipsoidalPotential):
+    """rho = amp / (4 pi a^3) / ((m/a)^2 (1 + m/a)^2); pot_type 23, psi
+    arguments [a, a^2] (TwoPowerTriaxialPotential.py:313-426)."""
+
+    pot_code = 23
+
+    def __init__(self, amp=1.0, a=2.0, b=1.0, c=1.0, zvec=None, pa=None, normalize=False,
+                 glorder=50):
+        EllipsoidalPotential.__init__(self, amp=amp, b=b, c=c, zvec=zvec, pa=pa, glorder=glorder)
+        self.a = a
+        self.a2 = a**2
+        self._amp /= 4.0 * numpy.pi * self.a2 * self.a
+        _set_normalize(self, normalize)
+
+    def _psi(self, m):
+        return 2.0 * self.a2 * (self.a / (m + self.a) + numpy.log(m / (m + self.a)))
+
+    def _mdens(self, m):
+        return self.a2**2.0 / m**2.0 / (m + self.a) ** 2.0
+
+    def _psi_args(self):
+        return [self.a, self.a2]
+
+
+class TwoPowerTriaxialPotential(EllipsoidalPotential):
+    """rho = amp / (4 pi a^3) / ((m/a)^alpha (1 + m/a)^(beta - alpha)),
+    0 <= alpha < 3 < ... beta > 2; pot_type 43, psi arguments [a, alpha,
+    beta, beta-alpha, 2-alpha, 3-alpha, psi_inf] with psi_inf =
+    Gamma(beta-2) Gamma(3-alpha) / Gamma(beta-alpha) (0 at alpha = 2)
+    (TwoPowerTriaxialPotential.py:20-190)."""
+
+    pot_code = 43
+
+    def __init__(self, amp=1.0, a=5.0, alpha=1.5, beta=3.5, b=1.0, c=1.0, zvec=None, pa=None,
+                 glorder=50, normalize=False):
+        EllipsoidalPotential.__init__(self, amp=amp, b=b, c=c, zvec=zvec, pa=pa, glorder=glorder)
+        if beta <= 2.0 or alpha >= 3.0:
+            raise OSError("TwoPowerTriaxialPotential requires 0 <= alpha < 3 and beta > 2")
+        self.a = a
+        self.alpha = alpha
+        self.beta = beta
+        self.betaminusalpha = beta - alpha
+        self.twominusalpha = 2.0 - alpha
+        self.threeminusalpha = 3.0 - alpha
+        self.psi_inf = 0.0
+        if self.twominusalpha != 0.0:
+            self.psi_inf = (special.gamma(beta - 2.0) * special.gamma(3.0 - alpha)
+                            / special.gamma(self.betaminusalpha))
+        self._amp /= 4.0 * numpy.pi * a**3
+        _set_normalize(self, normalize)
+
+    def _psi(self, m):
+        a2 = self.a**2.0
+        if self.twominusalpha == 0.0:
+            aom = self.a / m
+            bma = self.betaminusalpha
+            return -2.0 * a2 * aom**bma / bma * special.hyp2f1(bma, bma, bma + 1.0, -aom)
+        moa = m / self.a
+        return -2.0 * a2 * (self.psi_inf - moa**self.twominusalpha / self.twominusalpha
+                            * special.hyp2f1(self.twominusalpha, self.betaminusalpha,
+                                             self.threeminusalpha, -moa))
+
+    def _mdens(self, m):
+        return (self.a / m) ** self.alpha / (1.0 + m / self.a) ** self.betaminusalpha
+
+    def _psi_args(self):
+        return [self.a, self.alpha, self.beta, self.betaminusalpha, self.twominusalpha,
+                self.threeminusalpha, self.psi_inf if self.twominusalpha != 0.0 else 0.0]
+
+
+class PerfectEllipsoidPotential(EllipsoidalPotential):
+    """rho = amp a / (pi^2 b c) / (a^2 + m^2)^2; pot_type 30, psi argument
+    [a^2] (PerfectEllipsoidPotential.py:16-110)."""
+
+    pot_code = 30
+
+    def __init__(self, amp=1.0, a=5.0, b=1.0, c=1.0, zvec=None, pa=None, glorder=50,
+                 normalize=False):
+        EllipsoidalPotential.__init__(self, amp=amp, b=b, c=c, zvec=zvec, pa=pa, glorder=glorder)
+        self.a = a
+        self.a2 = a**2
+        self._amp *= a / (numpy.pi**2 * self._b * self._c)
+        _set_normalize(self, normalize)
+
+    def _psi(self, m):
+        return -1.0 / (self.a2 + m**2.0)
+
+    def _mdens(self, m):
+        return 1.0 / (self.a2 + m**2.0) ** 2.0
+
+    def _psi_args(self):
+        return [self.a2]
+
+
+class TriaxialGaussianPotential(EllipsoidalPotential):
+    """rho = amp / ((2 pi)^1.5 sigma^3 b c) exp(-m^2 / (2 sigma^2));
+    pot_type 37, psi argument [-2 sigma^2] (TriaxialGaussianPotential.py:17-115)."""
+
+    pot_code = 37
+
+    def __init__(self, amp=1.0, sigma=5.0, b=1.0, c=1.0, zvec=None, pa=None, glorder=50,
+                 normalize=False):
+        EllipsoidalPotential.__init__(self, amp=amp, b=b, c=c, zvec=zvec, pa=pa, glorder=glorder)
+        self._sigma = sigma
+        self._twosigma2 = 2.0 * sigma**2
+        self._amp /= (2.0 * numpy.pi) ** 1.5 * sigma**3.0 * self._b * self._c
+        _set_normalize(self, normalize)
+
+    def _psi(self, m):
+        return -self._twosigma2 * numpy.exp(-(m**2.0) / self._twosigma2)
+
+    def _mdens(self, m):
+        return numpy.exp(-(m**2) / self._twosigma2)
+
+    def _psi_args(self):
+        return [-self._twosigma2]
+
+
+class PowerTriaxialPotential(EllipsoidalPotential):
+    """rho = amp (3 - alpha) / (4 pi) m^-alpha (amp the mass within r1 in the
+    spherical case); pot_type 38, psi argument [alpha]
+    (PowerTriaxialPotential.py:17-115)."""
+
+    pot_code = 38
+
+    def __init__(self, amp=1.0, alpha=1.0, r1=1.0, b=1.0, c=1.0, zvec=None, pa=None, glorder=50,
+                 normalize=False):
+        EllipsoidalPotential.__init__(self, amp=amp, b=b, c=c, zvec=zvec, pa=pa, glorder=glorder)
+        self.alpha = alpha
+        if alpha != 3.0:
+            self._amp *= r1 ** (alpha - 3.0) * 4.0 * numpy.pi / (3.0 - alpha)
+        self._amp *= (3.0 - alpha) / 4.0 / numpy.pi
+        _set_normalize(self, normalize)
+
+    def _psi(self, m):
+        return 2.0 / (2.0 - self.alpha) * m ** (2.0 - self.alpha)
+
+    def _mdens(self, m):
+        return m**-self.alpha
+
+    def _psi_args(self):
+        return [self.alpha]
+
+
 class WrapperPotential(Potential):
     """A potential that modifies the potential it wraps (galpy's
     WrapperPotential.py:91-150): the wrapped components are evaluated at

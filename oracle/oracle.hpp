@@ -155,7 +155,16 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kKKStaeckel = 16, kSoftNeedleBar = 25, kEllipDisk = 4, kLopsided = 6, kSteadySpiral = 3,  // 3, 4, 6: planar only
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
-               kGaussianAmpWrap = -5 };
+               kGaussianAmpWrap = -5,
+               // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
+               kTriaxHernquist = 21, kTriaxNFW = 22, kTriaxJaffe = 23, kPerfectEllipsoid = 30,
+               kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43 };
+
+inline bool is_ellipsoidal(int code) {
+  return code == kTriaxHernquist || code == kTriaxNFW || code == kTriaxJaffe ||
+         code == kPerfectEllipsoid || code == kTriaxGaussian || code == kPowerTriax ||
+         code == kTwoPowerTriax;
+}
 
 // Components of one parse, wrapped ones included (the C-ABI entry points
 // accept up to this many).
@@ -169,6 +178,9 @@ struct Component {
   // wrappedPotentialArg, galpy_potentials.h), stored in the same parse array
   int nwrapped = 0;
   const Component* wrapped = nullptr;
+  // a component with a variable number of arguments reads them where they
+  // lie in pot_args, as galpy's potentialArg.args does (valid for the call)
+  const double* args = nullptr;
 };
 
 inline bool is_wrapper(int code) {
@@ -206,6 +218,15 @@ inline int component_nargs(int code, bool planar = false) {
   }
 }
 
+// Arguments of an EllipsoidalPotential: amp, 15 cache slots, npsi, the psi
+// arguments, b^2, c^2, aligned, rot[9], glorder, glx[glorder], glw[glorder]
+// (integrateFullOrbit.py:221-300; nargs as integrateFullOrbit.c:437).
+inline int ellipsoidal_nargs(const double* a) {
+  const int npsi = (int)a[16];
+  const int glorder = (int)a[17 + npsi + 12];
+  return 30 + npsi + 2 * glorder;
+}
+
 // Decode n components into dst, consuming codes and arguments in galpy's
 // order (parse_leapFuncArgs_Full, integrateFullOrbit.c:55-1045): a wrapper's
 // code is followed by its wrapped components' codes; its arguments are
@@ -216,7 +237,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
                             int& next, Component* dst, bool planar) {
   for (int i = 0; i < n; i++) {
     int code = *pot_type++;
-    int na = component_nargs(code, planar);
+    int na = is_ellipsoidal(code) ? ellipsoidal_nargs(pot_args) : component_nargs(code, planar);
     if (na < 0) return -1;
     Component& C = dst[i];
     C.type = code;
@@ -232,6 +253,12 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       if (parse_components(nw, pot_type, pot_args, out, next, kids, planar) != 0) return -1;
       C.nwrapped = nw;
       C.wrapped = kids;
+    }
+    C.args = pot_args;
+    if (is_ellipsoidal(code)) {
+      C.p[0] = pot_args[0];
+      pot_args += na;
+      continue;
     }
     for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
     if (code == kFlattenedPower && planar) {
@@ -689,6 +716,115 @@ template <class R> void ellipdisk_planar_forces(const Component& C, R Rc, R phi,
   Fphi = R(amp) * smooth * R(twophio) * pow(Rc, R(p)) * sin(R(2.) * (phi - R(phib)));
 }
 
+// ---------------------------------------------------------------------------
+// EllipsoidalPotential family.  The density rho(m) and psi(m) =
+// -int_m^inf dm'^2 rho of each member (TwoPowerTriaxialPotential.c:7-118,
+// PerfectEllipsoidPotential.c:4-20, TriaxialGaussianPotential.c:4-16,
+// PowerTriaxialPotential.c:4-20; psi arguments start at args + 17), the
+// Gauss-Legendre force sum EllipsoidalPotentialxyzforces_xyz
+// (EllipsoidalPotential.c:40-77), its cylindrical projections (:97-114) and
+// the potential EllipsoidalPotentialEval (:6-39).  TwoPowerTriaxial's psi
+// needs the Gauss hypergeometric function and is not restated (NaN).
+// ---------------------------------------------------------------------------
+template <class R> R ellip_mdens(int type, const double* pa, R m) {
+  switch (type) {
+    case kTriaxHernquist: { R mpa = m + R(pa[0]); return R(pa[1]) * pow(mpa, R(-3.)) / m; }
+    case kTriaxJaffe: { R mpa = m + R(pa[0]); return R(pa[1]) * R(pa[1]) * pow(m * mpa, R(-2.)); }
+    case kTriaxNFW: { R mpa = m + R(pa[0]); return R(pa[1]) / m / mpa / mpa; }
+    case kTwoPowerTriax: {
+      R mpa = m / R(pa[0]) + R(1.);
+      return pow(R(pa[0]) / m, R(pa[1])) / pow(mpa, R(pa[3]));
+    }
+    case kPerfectEllipsoid: { R m2 = m * m; return R(1.) / (R(pa[0]) + m2) / (R(pa[0]) + m2); }
+    case kTriaxGaussian: return exp(m * m / R(pa[0]));
+    case kPowerTriax: return pow(m * m, R(-pa[0] / 2.));
+    default: return R(NAN);
+  }
+}
+template <class R> R ellip_psi(int type, const double* pa, R m) {
+  switch (type) {
+    case kTriaxHernquist: { R mpa = m + R(pa[0]); return -R(pa[1]) / mpa / mpa; }
+    case kTriaxJaffe: { R mpa = m + R(pa[0]); return R(2.) * R(pa[1]) * (R(pa[0]) / mpa + log(m / mpa)); }
+    case kTriaxNFW: return R(-2.) * R(pa[1]) / (m + R(pa[0]));
+    case kPerfectEllipsoid: return R(-1.) / (R(pa[0]) + m * m);
+    case kTriaxGaussian: return R(pa[0]) * exp(m * m / R(pa[0]));
+    case kPowerTriax: return R(2.) / R(2. - pa[0]) * pow(m * m, R(1. - pa[0] / 2.));
+    default: return R(NAN);
+  }
+}
+// the layout behind the psi arguments
+struct EllipLayout {
+  const double* psi;
+  double b2, c2;
+  bool aligned;
+  const double* rot;
+  int glorder;
+  const double* glx;
+  const double* glw;
+};
+inline EllipLayout ellip_layout(const Component& C) {
+  EllipLayout L;
+  const double* a = C.args;
+  L.psi = a + 17;
+  const double* e = a + 17 + (int)a[16];
+  L.b2 = e[0];
+  L.c2 = e[1];
+  L.aligned = e[2] != 0.;
+  L.rot = e + 3;
+  L.glorder = (int)e[12];
+  L.glx = e + 13;
+  L.glw = e + 13 + L.glorder;
+  return L;
+}
+// galpy_potentials.c:391-410
+template <class R> void rotate_xyz(R& x, R& y, R& z, const double* rot) {
+  R xp = R(rot[0]) * x + R(rot[1]) * y + R(rot[2]) * z;
+  R yp = R(rot[3]) * x + R(rot[4]) * y + R(rot[5]) * z;
+  R zp = R(rot[6]) * x + R(rot[7]) * y + R(rot[8]) * z;
+  x = xp; y = yp; z = zp;
+}
+template <class R> void rotate_force_xyz(R& Fx, R& Fy, R& Fz, const double* rot) {
+  R Fxp = R(rot[0]) * Fx + R(rot[3]) * Fy + R(rot[6]) * Fz;
+  R Fyp = R(rot[1]) * Fx + R(rot[4]) * Fy + R(rot[7]) * Fz;
+  R Fzp = R(rot[2]) * Fx + R(rot[5]) * Fy + R(rot[8]) * Fz;
+  Fx = Fxp; Fy = Fyp; Fz = Fzp;
+}
+// Cartesian forces / amp at (x, y, z) in the inertial frame
+template <class R> void ellip_forces_xyz(const Component& C, R x, R y, R z, R& Fx, R& Fy, R& Fz) {
+  const EllipLayout L = ellip_layout(C);
+  if (!L.aligned) rotate_xyz(x, y, z, L.rot);
+  Fx = 0.; Fy = 0.; Fz = 0.;
+  for (int ii = 0; ii < L.glorder; ii++) {
+    R t = R(1.) / R(L.glx[ii]) / R(L.glx[ii]) - R(1.);
+    R td = R(L.glw[ii]) * ellip_mdens(C.type, L.psi,
+                                      sqrt(x * x / (R(1.) + t) + y * y / (R(L.b2) + t) + z * z / (R(L.c2) + t)));
+    Fx += td * x / (R(1.) + t);
+    Fy += td * y / (R(L.b2) + t);
+    Fz += td * z / (R(L.c2) + t);
+  }
+  if (!L.aligned) rotate_force_xyz(Fx, Fy, Fz, L.rot);
+}
+template <class R> void ellip_forces(const Component& C, R Rc, R z, R phi, R& FR, R& Fz, R& Fphi) {
+  R Fx, Fy, Fzz;
+  ellip_forces_xyz(C, Rc * cos(phi), Rc * sin(phi), z, Fx, Fy, Fzz);
+  const R amp = R(C.p[0]);
+  FR = amp * (cos(phi) * Fx + sin(phi) * Fy);
+  Fphi = amp * Rc * (-sin(phi) * Fx + cos(phi) * Fy);
+  Fz = amp * Fzz;
+}
+template <class R> R ellip_eval(const Component& C, R Rc, R z, R phi) {
+  const EllipLayout L = ellip_layout(C);
+  R x = Rc * cos(phi), y = Rc * sin(phi);
+  if (!L.aligned) rotate_xyz(x, y, z, L.rot);
+  R out = 0.;
+  for (int ii = 0; ii < L.glorder; ii++) {
+    R s = R(1.) / R(L.glx[ii]) / R(L.glx[ii]) - R(1.);
+    out += R(L.glw[ii]) *
+           ellip_psi(C.type, L.psi, sqrt(x * x / (R(1.) + s) + y * y / (R(L.b2) + s) + z * z / (R(L.c2) + s)));
+  }
+  return R(-0.5) * R(C.p[0]) * out;
+}
+
 // Aggregated cylindrical forces: galpy_potentials.c:120-179 sums each of
 // Rforce, phitorque, zforce over components in index order starting from 0.
 // Summing the three per component gives the identical three sums.
@@ -728,7 +864,9 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       }
       case kNull: break;
       default:
-        if (is_spherical_new(C.type)) {
+        if (is_ellipsoidal(C.type)) {
+          ellip_forces(C, Rc, z, phi, fr, fz, fp);
+        } else if (is_spherical_new(C.type)) {
           sph_forces(C, Rc, z, fr, fz);
         } else if (is_wrapper(C.type)) {
           // calcRforce/calczforce/calcphitorque of the wrapped components at
@@ -1128,7 +1266,10 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
         fr = -R(C.p[0]) * Rc * pow(Rc * Rc + R(C.p[1] * C.p[1]), R(-1.5));
         break;
       default:
-        if (is_spherical_new(C.type)) {  // planarRforce = -Phi'(R)
+        if (is_ellipsoidal(C.type)) {  // EllipsoidalPotential.c:116-123: the full forces at z = 0
+          R fz0;
+          ellip_forces(C, Rc, R(0.), phi, fr, fz0, fp);
+        } else if (is_spherical_new(C.type)) {  // planarRforce = -Phi'(R)
           R Phip, Phipp;
           sph_dphi(C, Rc, Phip, Phipp);
           fr = -Phip;
@@ -1173,7 +1314,7 @@ inline bool has_hessian(int npot, const Component* P) {
   for (int i = 0; i < npot; i++) {
     const int c = P[i].type;
     if (c == kKKStaeckel || c == kSoftNeedleBar || is_wrapper(c) || c == kEllipDisk || c == kLopsided ||
-        c == kSteadySpiral)
+        c == kSteadySpiral || is_ellipsoidal(c))
       return false;
   }
   return true;
@@ -1273,7 +1414,9 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       }
       case kNull: break;
       default:
-        if (is_spherical_new(C.type)) {
+        if (is_ellipsoidal(C.type)) {
+          pot += ellip_eval(C, Rc, z, phi);
+        } else if (is_spherical_new(C.type)) {
           pot += sph_eval(C, sqrt(Rc * Rc + z * z));
         } else if (is_wrapper(C.type)) {
           // the wrappers' Python _evaluate (the wrapped potential at

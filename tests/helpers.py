@@ -51,6 +51,18 @@ _NARGS = {0: 4, 1: 6, 3: 8, 4: 6, 5: 3, 6: 4, 16: 3, 25: 13, 7: 2, 8: 2, 9: 2, 1
           33: 2, 34: 3, 35: 3, 12: 4, 40: 0, -1: 4, -2: 3, -4: 5, -5: 3}
 
 
+_ELLIPSOIDAL = (21, 22, 23, 30, 37, 38, 43)
+
+
+def _nargs(code, pot_args, ai, planar):
+    """Arguments of one component or wrapper starting at pot_args[ai]; an
+    EllipsoidalPotential has 30 + npsi + 2 glorder (integrateFullOrbit.c:437)."""
+    if code in _ELLIPSOIDAL:
+        npsi = int(pot_args[ai + 16])
+        return 30 + npsi + 2 * int(pot_args[ai + 29 + npsi])
+    return 3 if (planar and code in (15, 12)) else _NARGS[code]
+
+
 def npot_of(pot_type, pot_args, planar=False):
     """galpy's npot for packed arrays: the number of top-level components
     (a wrapper and the components it wraps count once)."""
@@ -65,7 +77,7 @@ def npot_of(pot_type, pot_args, planar=False):
             ai += 1
             for _ in range(nw):
                 walk()
-        ai += 3 if (planar and code in (15, 12)) else _NARGS[code]
+        ai += _nargs(code, pot_args, ai, planar)
 
     n = 0
     while ti < len(pot_type):
