@@ -87,6 +87,8 @@ int nargs_of(int code, bool planar) {
     case GH_POWERSPH: case GH_JAFFE: case GH_ISOCHRONE: case GH_PLUMMER: case GH_PSEUDOISO:
     case GH_KUZMINDISK: case GH_BURKERT: case GH_DEHNENCORESPH: return 2;
     case GH_DEHNENSPH: case GH_HOMOGSPHERE: return 3;
+    case GH_EINASTO: case GH_EXPTRUNCNFW: return 3;  // amp, h, n / amp, a, rc
+    case GH_TWOPOWERSPH: return 4;                    // amp, a, alpha, beta
     case GH_FLATPOWER: return planar ? 3 : 4;  // planar: amp, alpha, core^2
     case GH_NULLPOT: return 0;
     case GH_KKSTAECKEL: return 3;  // amp, ac, Delta
@@ -252,6 +254,103 @@ int decode_ellipsoidal(int code, const double*& pot_args, GhPotTable* T, int i) 
   return 0;
 }
 
+// e^x E1(x) on the host in long double (A&S 5.1.11 / 5.1.22, as on the device)
+double host_e1_scaled(double xd) {
+  const long double x = xd;
+  if (x <= 1.0L) {
+    long double sum = 0.0L, term = 1.0L;
+    for (int k = 1; k < 80; k++) {
+      term *= -x / k;
+      sum += term / k;
+    }
+    return (double)(expl(x) * (-0.5772156649015328606065120900824024L - logl(x) - sum));
+  }
+  long double b = x + 1.0L, c = 1e300L, d = 1.0L / b, h = d;
+  for (int i = 1; i < 5000; i++) {
+    const long double an = -(long double)i * i;
+    b += 2.0L;
+    d = 1.0L / (an * d + b);
+    c = b + an / c;
+    const long double del = c * d;
+    h *= del;
+    if (fabsl(del - 1.0L) < 1e-19L) break;
+  }
+  return (double)h;
+}
+
+// TwoPowerSphericalPotential: the enclosed-mass tables of gh_twopower_mass
+// (series coefficients s_n = (B)_n/(A+1)_n for x <= 3; the tail expansion's
+// d_k = c_k / (3 - beta - k), c_k = (-1)^k (B)_k / k!, the constant K and
+// the logarithm's coefficient for x > 3), in long double.
+int decode_twopower(GhPotTable* T, GhComponent& C) {
+  const double a = C.p[1], alpha = C.p[2], beta = C.p[3];
+  if (!(a > 0.0) || !(alpha < 3.0) || !(beta - alpha > 0.0)) {
+    set_error("TwoPowerSphericalPotential a=%g alpha=%g beta=%g: need a > 0, alpha < 3, beta > alpha",
+              a, alpha, beta);
+    return -1;
+  }
+  const long double A = 3.0L - alpha, B = (long double)beta - alpha;
+  const double amb = 3.0 - beta;
+  double* e = T->ext + T->next;
+  const int room = GH_EXT_LEN - T->next;
+  // series at w <= 3/4: until the tail is below 1e-19 of the sum
+  const long double w0 = 0.75L;
+  long double sn = 1.0L, S = 0.0L, wn = 1.0L;
+  int ns = 0;
+  bool done = false;
+  for (int n = 0; n < room && n < 3000 && !done; n++) {
+    e[n] = (double)sn;
+    S += sn * wn;
+    ns = n + 1;
+    // the terms shrink from here on (ratio < 1) and are negligible
+    done = n > 8 && sn * wn < 1e-19L * S && (B + n) / (A + 1.0L + n) * w0 < 1.0L;
+    sn *= (B + n) / (A + 1.0L + n);
+    wn *= w0;
+  }
+  if (!done) {
+    set_error("TwoPowerSphericalPotential alpha=%g beta=%g: series did not converge", alpha, beta);
+    return -1;
+  }
+  const long double I3 = powl(3.0L, A) / A * powl(4.0L, -B) * S;
+  // tail: k* = 3 - beta when a nonnegative integer
+  const int kstar = (amb >= 0.0 && amb == std::floor(amb)) ? (int)amb : -1;
+  double* d = e + ns;
+  long double ck = 1.0L, K = I3, cstar = 0.0L;
+  int nd = 0;
+  done = false;
+  for (int k = 0; k < room - ns && k < 2000 && !done; k++) {
+    if (k > 0) ck *= -(B + k - 1.0L) / k;
+    nd = k + 1;
+    if (k == kstar) {
+      cstar = ck;
+      d[k] = 0.0;
+      K -= ck * logl(3.0L);
+      continue;
+    }
+    const long double dk = ck / ((long double)amb - k);
+    d[k] = (double)dk;
+    const long double at3 = dk * powl(3.0L, (long double)amb - k);
+    K -= at3;
+    done = k > kstar + 2 && k > 4 && fabsl(at3) < 1e-19L * fabsl(I3);
+  }
+  if (!done) {
+    set_error("TwoPowerSphericalPotential alpha=%g beta=%g: series tables do not fit", alpha, beta);
+    return -1;
+  }
+  C.ext = T->next;
+  T->next += ns + nd;
+  C.c[0] = (double)A;
+  C.c[1] = (double)B;
+  C.c[2] = amb;
+  C.c[3] = ns;
+  C.c[4] = nd;
+  C.c[5] = (double)K;
+  C.c[6] = (double)cstar;
+  C.c[7] = 1.0 / a;
+  C.c[8] = (double)(1.0L / A);
+  return 0;
+}
+
 // One (leaf) component's arguments and precomputed constants.
 int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
                      bool planar) {
@@ -346,6 +445,47 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
         break;
       case GH_PSEUDOISO:
       case GH_BURKERT: C.c[0] = 1.0 / C.p[1]; break;  // 1/a
+      case GH_EINASTO: {
+        const double h = C.p[1], n = C.p[2];
+        if (!(h > 0.0) || !(n > 0.0)) {
+          set_error("EinastoPotential h=%g n=%g must be positive", h, n);
+          return -1;
+        }
+        C.c[0] = 3.0 * n;
+        C.c[1] = std::tgamma(3.0 * n);
+        C.c[2] = std::ldexp(C.c[1], -58);
+        C.c[3] = 2.0 * n;
+        C.c[4] = std::tgamma(2.0 * n);
+        C.c[5] = std::ldexp(C.c[4], -58);
+        C.c[6] = 4.0 * M_PI * h * h * h * n;
+        C.c[7] = 1.0 / h;
+        C.c[8] = 1.0 / n;
+        T->has_wrap = 1;  // extended dispatch
+        break;
+      }
+      case GH_EXPTRUNCNFW: {  // ExpTruncNFWPotential.c:11-28
+        const double a = C.p[1], rc = C.p[2];
+        if (!(a > 0.0) || !(rc > 0.0)) {
+          set_error("ExpTruncNFWPotential a=%g rc=%g must be positive", a, rc);
+          return -1;
+        }
+        C.c[0] = a / rc;
+        C.c[1] = host_e1_scaled(a / rc);
+        C.c[2] = 1.0 / rc;
+        C.c[3] = 1e-3 * (a < rc ? a : rc);
+        C.c[4] = 0.5;
+        C.c[5] = -(1.0 / rc + 2.0 / a) / 3.0;
+        C.c[6] = (0.5 / (rc * rc) + 2.0 / (rc * a) + 3.0 / (a * a)) / 4.0;
+        C.c[7] = -(1.0 / (6.0 * rc * rc * rc) + 1.0 / (rc * rc * a) + 3.0 / (rc * a * a) +
+                   4.0 / (a * a * a)) / 5.0;
+        C.c[8] = 1.0 / (a * a);
+        T->has_wrap = 1;
+        break;
+      }
+      case GH_TWOPOWERSPH:
+        if (decode_twopower(T, C) != 0) return -1;
+        T->has_wrap = 1;
+        break;
       case GH_DEHNENSPH:
         if (!(C.p[2] >= 0.0 && C.p[2] < 3.0)) {
           set_error("DehnenSphericalPotential alpha=%g outside [0, 3)", C.p[2]);

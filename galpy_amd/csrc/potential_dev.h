@@ -64,6 +64,10 @@ enum GhPotCode {
   GH_TRIAXGAUSSIAN = 37,
   GH_POWERTRIAX = 38,
   GH_TWOPOWERTRIAX = 43,
+  // spherical components with special functions (integrateFullOrbit.c:731-780)
+  GH_EINASTO = 41,
+  GH_TWOPOWERSPH = 42,
+  GH_EXPTRUNCNFW = 46,
   // wrappers (integrateFullOrbit.c:821-895)
   GH_DEHNENSMOOTHWRAP = -1,
   GH_SOLIDBODYROTWRAP = -2,
@@ -137,6 +141,8 @@ __host__ __device__ inline size_t gh_table_bytes(const GhPotTable& T) {
 //             the rounded result and gamma = Gamma(a).
 // cf_tol = 2^-58 Gamma(a) is precomputed on the host.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double gh_incgamma_cf(double a, double x, double thr);
+
 __device__ __forceinline__ double gh_lower_incgamma(double a, double x, double gamma_a,
                                                    double xbig, double cf_tol,
                                                    const double* __restrict__ ser) {
@@ -149,8 +155,12 @@ __device__ __forceinline__ double gh_lower_incgamma(double a, double x, double g
     for (int n = GH_IG_NSER - 2; n >= 0; n--) s = fma(s, x, ser[n]);
     return pref * s;
   }
-  // allowed change of the continued-fraction value (Gamma(a,x) = pref * f)
-  const double thr = cf_tol / pref;
+  return gamma_a - pref * gh_incgamma_cf(a, x, cf_tol / pref);
+}
+
+// Legendre's continued fraction f of Gamma(a, x) = e^-x x^a f (x >= a+1),
+// by the forward recurrence described above; thr = allowed change of f.
+__device__ __forceinline__ double gh_incgamma_cf(double a, double x, double thr) {
   double b = x + 1.0 - a;
   double Am2 = 0.0, Bm2 = 1.0;  // A_0, B_0
   double Am1 = 1.0, Bm1 = b;    // A_1, B_1: f_1 = 1/b_1
@@ -174,7 +184,54 @@ __device__ __forceinline__ double gh_lower_incgamma(double a, double x, double g
       Bm1 *= 0x1p-400;
     }
   }
-  return gamma_a - pref * (Am1 / Bm1);
+  return Am1 / Bm1;
+}
+
+// gamma(a, x) for any a > 0 with the series summed term by term (x < a+1,
+// stopping below 2^-56 of the sum) instead of host coefficients; the same
+// continued fraction above.
+__device__ inline double gh_lower_incgamma_gen(double a, double x, double gamma_a, double cf_tol) {
+  if (x <= 0.0) return 0.0;
+  const double pref = exp(a * log(x) - x);
+  if (x < a + 1.0) {
+    double term = 1.0 / a, sum = term, ap = a;
+    for (int n = 1; n < 2000; n++) {
+      ap += 1.0;
+      term *= x / ap;
+      sum += term;
+      if (term < sum * 0x1p-56) break;
+    }
+    return pref * sum;
+  }
+  return gamma_a - pref * gh_incgamma_cf(a, x, cf_tol / pref);
+}
+
+// e^x E1(x), x > 0 (replaces GSL's gsl_sf_expint_E1 at
+// ExpTruncNFWPotential.c:25,32): A&S 5.1.11 below x = 1,
+// -gamma_E - ln x - sum_k (-x)^k / (k k!), and the continued fraction
+// A&S 5.1.22, 1/(x + 1 - 1/(x + 3 - 4/(x + 5 - ...))), by Lentz's method above.
+__device__ inline double gh_e1_scaled(double x) {
+  if (x <= 1.0) {
+    double sum = 0.0, term = 1.0;
+    for (int k = 1; k < 60; k++) {
+      term *= -x / k;
+      const double add = term / k;
+      sum += add;
+      if (fabs(add) < fabs(sum) * 0x1p-56) break;
+    }
+    return exp(x) * (-0.57721566490153286061 - log(x) - sum);
+  }
+  double b = x + 1.0, c = 1e300, d = 1.0 / b, h = d;
+  for (int i = 1; i < 1000; i++) {
+    const double an = -(double)i * i;
+    b += 2.0;
+    d = 1.0 / fma(an, d, b);
+    c = b + an / c;
+    const double del = c * d;
+    h *= del;
+    if (fabs(del - 1.0) < 0x1p-53) break;
+  }
+  return h;
 }
 
 // Series branch of gh_lower_incgamma (x < a+1, i.e. r < rc sqrt(a+1)), kept
@@ -842,6 +899,93 @@ GH_ELLIP_FORCE(GH_TRIAXGAUSSIAN)
 GH_ELLIP_FORCE(GH_POWERTRIAX)
 #undef GH_ELLIP_FORCE
 
+// ---------------------------------------------------------------------------
+// Spherical components whose enclosed mass needs a special function; each
+// adds g(r) x, g = -Phi'(r)/r, like the closed-form spherical components.
+//
+// EinastoPotential.c:27-47: Phi' = amp 4 pi h n s^-2 gamma(3n, s^(1/n)),
+// s = r/h (galpy writes -Gamma(3n) (Q(3n, x) - 1), which cancels for
+// x << 3n; gamma(3n, x) is the same function without the cancellation).
+// c[0] = 3n, c[1] = Gamma(3n), c[2] = 2^-58 Gamma(3n), c[3] = 2n,
+// c[4] = Gamma(2n), c[5] = 2^-58 Gamma(2n), c[6] = 4 pi h^3 n, c[7] = 1/h,
+// c[8] = 1/n.
+__device__ __forceinline__ double gh_einasto_x(const GhComponent* __restrict__ C, double r) {
+  return exp(C->c[8] * log(r * C->c[7]));  // (r/h)^(1/n)
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_EINASTO>(const GhPotTable* __restrict__ T, int i,
+                                                          const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double gam = gh_lower_incgamma_gen(C->c[0], gh_einasto_x(C, g.r), C->c[1], C->c[2]);
+  const double gr = -C->p[0] * C->c[6] * gam * (g.rinv * g.rinv * g.rinv);
+  F.FRoR += gr;
+  F.Fz += gr * g.z;
+}
+
+// ExpTruncNFWPotential.c:11-28,53-58: Phi' = amp F(r) / r^2 with
+// F = (1 + alpha) (e^alpha E1(alpha) - e^alpha E1(beta)) - 1 + a e^(-r/rc)/(a + r),
+// alpha = a/rc, beta = (a + r)/rc, and e^alpha E1(beta) = e^(-r/rc) e^beta
+// E1(beta); below 1e-3 min(a, rc) galpy's quartic series.
+// c[0] = alpha, c[1] = e^alpha E1(alpha), c[2] = 1/rc, c[3] = series
+// threshold, c[4..7] = series coefficients, c[8] = 1/a^2.
+__device__ inline double gh_exptruncnfw_F(const GhComponent* __restrict__ C, double r,
+                                          double& ex, double& e1b) {
+  const double a = C->p[1];
+  ex = exp(-r * C->c[2]);
+  if (r < C->c[3]) {
+    e1b = 0.0;
+    return (r * r * C->c[8]) * (C->c[4] + r * (C->c[5] + r * (C->c[6] + r * C->c[7])));
+  }
+  e1b = gh_e1_scaled((a + r) * C->c[2]);
+  return (1.0 + C->c[0]) * (C->c[1] - ex * e1b) - 1.0 + a * ex / (a + r);
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_EXPTRUNCNFW>(const GhPotTable* __restrict__ T,
+                                                              int i, const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  double ex, e1b;
+  const double gr = -C->p[0] * gh_exptruncnfw_F(C, g.r, ex, e1b) * (g.rinv * g.rinv * g.rinv);
+  F.FRoR += gr;
+  F.Fz += gr * g.z;
+}
+
+// TwoPowerSphericalPotential.c:37-49: Phi' = amp r^(1-alpha) a^(alpha-3) /
+// (3-alpha) 2F1(3-alpha, beta-alpha; 4-alpha; -r/a) = amp I(r/a) / r^2 with
+// I(x) = int_0^x u^(A-1) (1+u)^-B du, A = 3 - alpha, B = beta - alpha (the
+// enclosed mass over amp).  galpy evaluates 2F1 with the xsf library; here
+//   x <= 3: I = x^A/A (1+x)^-B sum_n s_n w^n, w = x/(1+x), s_n =
+//           (B)_n/(A+1)_n (Pfaff's transformation; positive terms);
+//   x > 3 : I = K + x^(3-beta) sum_k d_k x^-k [+ c* ln x], the binomial
+//           series of (1+1/u)^-B integrated from 3 (ln for the u^-1 term when
+//           3 - beta is a nonnegative integer), K collecting the constants.
+// The host computes s_n, d_k, K and c* in long double (ext: s_n then d_k).
+// c[0] = A, c[1] = B, c[2] = 3 - beta, c[3] = #s, c[4] = #d, c[5] = K,
+// c[6] = c*, c[7] = 1/a, c[8] = 1/A.
+__device__ inline double gh_twopower_mass(const GhPotTable* __restrict__ T,
+                                          const GhComponent* __restrict__ C, double x) {
+  const double* __restrict__ e = &T->ext[C->ext];
+  const int ns = (int)C->c[3], nd = (int)C->c[4];
+  if (x <= 3.0) {
+    const double w = x / (1.0 + x);
+    double sum = e[ns - 1];
+    for (int n = ns - 2; n >= 0; n--) sum = fma(sum, w, e[n]);
+    return exp(C->c[0] * log(x) - C->c[1] * log1p(x)) * C->c[8] * sum;
+  }
+  const double* __restrict__ d = e + ns;
+  const double u = 1.0 / x, lx = log(x);
+  double p = d[nd - 1];
+  for (int k = nd - 2; k >= 0; k--) p = fma(p, u, d[k]);
+  return C->c[5] + exp(C->c[2] * lx) * p + C->c[6] * lx;
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_TWOPOWERSPH>(const GhPotTable* __restrict__ T,
+                                                              int i, const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double gr = -C->p[0] * gh_twopower_mass(T, C, g.r * C->c[7]) * (g.rinv * g.rinv * g.rinv);
+  F.FRoR += gr;
+  F.Fz += gr * g.z;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_NULLPOT>(const GhPotTable* __restrict__, int,
                                                           const GhGeom&, GhAcc&) {}
@@ -883,7 +1027,8 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
 __host__ __device__ inline bool gh_is_heavy(int type) {
   return type == GH_TRIAXHERNQUIST || type == GH_TRIAXNFW || type == GH_TRIAXJAFFE ||
          type == GH_TWOPOWERTRIAX || type == GH_PERFECTELLIPSOID || type == GH_TRIAXGAUSSIAN ||
-         type == GH_POWERTRIAX;
+         type == GH_POWERTRIAX || type == GH_EINASTO || type == GH_TWOPOWERSPH ||
+         type == GH_EXPTRUNCNFW;
 }
 
 // run-time dispatch over every component type (extended instances)
@@ -897,6 +1042,9 @@ __device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__
     case GH_PERFECTELLIPSOID:
     case GH_TRIAXGAUSSIAN:
     case GH_POWERTRIAX: gh_ellip_force<0>(T, i, g, F); break;
+    case GH_EINASTO: gh_comp_force<GH_EINASTO>(T, i, g, F); break;
+    case GH_EXPTRUNCNFW: gh_comp_force<GH_EXPTRUNCNFW>(T, i, g, F); break;
+    case GH_TWOPOWERSPH: gh_comp_force<GH_TWOPOWERSPH>(T, i, g, F); break;
     default: gh_comp_force_dyn(T, i, g, F); break;
   }
 }
@@ -1298,6 +1446,23 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
       case GH_PERFECTELLIPSOID:
       case GH_TRIAXGAUSSIAN:
       case GH_POWERTRIAX: pot += gh_ellip_potential(T, C, R * cosphi, R * sinphi, z); break;
+      case GH_EINASTO: {  // EinastoPotential.c revaluate: -amp 4 pi h^2 n (gamma(3n,x)/s + Gamma(2n,x))
+        const double r = sqrt(r2), x = gh_einasto_x(C, r);
+        const double g3 = gh_lower_incgamma_gen(C->c[0], x, C->c[1], C->c[2]);
+        const double G2 = C->c[4] - gh_lower_incgamma_gen(C->c[3], x, C->c[4], C->c[5]);
+        const double h = C->p[1];
+        pot += -C->p[0] * C->c[6] / h * (g3 / (r * C->c[7]) + G2);
+        break;
+      }
+      case GH_EXPTRUNCNFW: {  // ExpTruncNFWPotential.c revaluate: -amp (F/r + G)
+        const double r = sqrt(r2);
+        double ex, e1b;
+        const double Fm = gh_exptruncnfw_F(C, r, ex, e1b);
+        if (r < C->c[3]) e1b = gh_e1_scaled((C->p[1] + r) * C->c[2]);
+        pot += -C->p[0] * (Fm / r + ex / (C->p[1] + r) - ex * e1b * C->c[2]);
+        break;
+      }
+      case GH_TWOPOWERSPH: pot += NAN; break;  // its Eval needs 2F1 outside the family above
       default:
         break;
     }

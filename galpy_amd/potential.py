@@ -935,6 +935,121 @@ class HomogeneousSpherePotential(_SphericalPotential):
         return [self._amp, self._R2, self._R3]
 
 
+class EinastoPotential(_SphericalPotential):
+    """rho = amp exp(-(r/h)^(1/n)); pot_type 41, args [amp, h, n]
+    (EinastoPotential.py:20-140).  rs (the scale radius, with amp the
+    density there) or rm2 (the radius where the logarithmic slope is -2)
+    may be given instead of h."""
+
+    pot_code = 41
+
+    def __init__(self, amp=1.0, h=2.0, n=1, rs=None, rm2=None, normalize=False):
+        Potential.__init__(self, amp=amp)
+        if rs is not None:
+            from scipy.optimize import fsolve
+
+            est = (3 * n - 1 / 3 + 8 / (1215 * n) + 184 / (229635 * n**2) + 1048 / (31000725 * n**3)
+                   - 17557576 / (1242974068875 * n**4))
+            dn = fsolve(lambda x: 2 * special.gammaincc(3 * n, x) * special.gamma(3 * n)
+                        - special.gamma(3 * n), est)[0]
+            self._amp = amp * numpy.e**dn
+            h = rs / (dn**n)
+        elif rm2 is not None:
+            self._amp = amp * numpy.e ** (2 * n)
+            h = rm2 / ((2 * n) ** n)
+        self.h = h
+        self.n = n
+        _set_normalize(self, normalize)
+
+    def _phi(self, r):
+        s = numpy.asarray(r, dtype=float) / self.h
+        x = s ** (1.0 / self.n)
+        g3 = special.gamma(3 * self.n)
+        return -4.0 * numpy.pi * self.h**2 * self.n * (
+            g3 * special.gammainc(3 * self.n, x) / s
+            + special.gamma(2 * self.n) * special.gammaincc(2 * self.n, x))
+
+    def _dphi(self, r):
+        s = numpy.asarray(r, dtype=float) / self.h
+        return (4.0 * numpy.pi * self.h * self.n * special.gamma(3 * self.n) * s**-2.0
+                * special.gammainc(3 * self.n, s ** (1.0 / self.n)))
+
+    def c_args(self):
+        return [self._amp, self.h, self.n]
+
+
+class ExpTruncNFWPotential(_SphericalPotential):
+    """NFW density times exp(-r/rc): rho = amp / (4 pi a^3) exp(-r/rc) /
+    ((r/a) (1 + r/a)^2); pot_type 46, args [amp, a, rc]
+    (ExpTruncNFWPotential.py:20-340).  mass sets amp from the total mass."""
+
+    pot_code = 46
+
+    def __init__(self, amp=1.0, a=1.0, rc=2.0, mass=None, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self.a = a
+        self.rc = rc
+        self._alpha = a / rc
+        self._Ftot = numpy.exp(self._alpha) * (1.0 + self._alpha) * special.exp1(self._alpha) - 1.0
+        if mass is not None:
+            self._amp = mass / self._Ftot
+        _set_normalize(self, normalize)
+
+    def _F(self, r):
+        r = numpy.asarray(r, dtype=float)
+        beta = (self.a + r) / self.rc
+        return (numpy.exp(self._alpha) * (1.0 + self._alpha) * (special.exp1(self._alpha) - special.exp1(beta))
+                - 1.0 + self.a * numpy.exp(-r / self.rc) / (self.a + r))
+
+    def _phi(self, r):
+        r = numpy.asarray(r, dtype=float)
+        G = (numpy.exp(-r / self.rc) / (self.a + r)
+             - numpy.exp(self._alpha) * special.exp1((self.a + r) / self.rc) / self.rc)
+        return -(self._F(r) / r + G)
+
+    def _dphi(self, r):
+        r = numpy.asarray(r, dtype=float)
+        return self._F(r) / r**2.0
+
+    def c_args(self):
+        return [self._amp, self.a, self.rc]
+
+
+class TwoPowerSphericalPotential(_SphericalPotential):
+    """rho = amp / (4 pi a^3) / ((r/a)^alpha (1 + r/a)^(beta - alpha));
+    pot_type 42, args [amp, a, alpha, beta] (TwoPowerSphericalPotential.py:
+    31-180; galpy packs this code for any alpha, beta, the Hernquist, Jaffe
+    and NFW classes having codes of their own)."""
+
+    pot_code = 42
+
+    def __init__(self, amp=1.0, a=5.0, alpha=1.5, beta=3.5, normalize=False):
+        Potential.__init__(self, amp=amp)
+        self.a = a
+        self.alpha = alpha
+        self.beta = beta
+        _set_normalize(self, normalize)
+
+    def _dphi(self, r):
+        r = numpy.asarray(r, dtype=float)
+        return (r ** (1.0 - self.alpha) * self.a ** (self.alpha - 3.0) / (3.0 - self.alpha)
+                * special.hyp2f1(3.0 - self.alpha, self.beta - self.alpha, 4.0 - self.alpha, -r / self.a))
+
+    def _phi(self, r):
+        r = numpy.asarray(r, dtype=float)
+        a, al, be = self.a, self.alpha, self.beta
+        if be == 3.0:
+            return (1.0 / a * (1.0 - (r / a) ** (2.0 - al) / (3.0 - al)
+                               * special.hyp2f1(3.0 - al, 2.0 - al, 4.0 - al, -r / a)) / (al - 2.0))
+        r = r + 1e-11
+        return (special.gamma(be - 3.0) * ((r / a) ** (3.0 - be) / special.gamma(be - 1.0)
+                                          * special.hyp2f1(be - 3.0, be - al, be - 1.0, -a / r)
+                                          - special.gamma(3.0 - al) / special.gamma(be - al)) / r)
+
+    def c_args(self):
+        return [self._amp, self.a, self.alpha, self.beta]
+
+
 class FlattenedPowerPotential(Potential):
     """Phi = -amp / (alpha m^alpha), m^2 = R^2 + z^2/q^2 + core^2 (amp/2 log m^2
     at alpha = 0); pot_type 12, args [amp, alpha, q^2, core^2], planar

@@ -143,6 +143,84 @@ template <class R> R upper_incgamma(double a, R x, double gamma_a) {
 }
 
 // ---------------------------------------------------------------------------
+// Exponential integral E1(x), x > 0 (replaces GSL's gsl_sf_expint_E1 at
+// ExpTruncNFWPotential.c:25,32), from A&S 5.1.11 and 5.1.22:
+//   x <= 1: E1 = -gamma_E - ln x - sum_{k>=1} (-x)^k / (k k!)
+//   x > 1 : E1 = e^-x / (x + 1 - 1^2/(x + 3 - 2^2/(x + 5 - ...))), modified Lentz.
+// e1_scaled returns e^x E1(x).  Pinned against mpmath.
+// ---------------------------------------------------------------------------
+constexpr double kEulerGamma = 0.57721566490153286060651209008240243;
+template <class R> R e1_scaled(R x) {
+  if (x <= R(1.)) {
+    R sum = 0., term = 1.;
+    for (int k = 1; k < 100; k++) {
+      term = term * (-x) / R(k);
+      R add = term / R(k);
+      sum += add;
+      if (fabs(add) < fabs(sum) * R(kIGEps)) break;
+    }
+    return exp(x) * (R(-kEulerGamma) - log(x) - sum);
+  }
+  R b = x + R(1.);
+  R c = R(1. / kIGTiny);
+  R d = R(1.) / b;
+  R h = d;
+  for (int i = 1; i < 1000; i++) {
+    R an = R(-(double)i * i);
+    b += R(2.);
+    d = R(1.) / (an * d + b);
+    c = b + an / c;
+    R del = c * d;
+    h *= del;
+    if (fabs(del - R(1.)) < R(kIGEps)) break;
+  }
+  return h;
+}
+template <class R> R expint_e1(R x) { return exp(-x) * e1_scaled(x); }
+
+// ---------------------------------------------------------------------------
+// Gauss hypergeometric 2F1(A, B; A+1; -x) for x >= 0, the family galpy's
+// TwoPowerSphericalPotential.c calls (the reference evaluates it with the
+// vendored xsf library's hyp2f1, wrap_xsf.cpp).  Written from the
+// mathematics: 2F1(A, B; A+1; -x) = A x^-A I(x), I(x) = int_0^x u^(A-1)
+// (1+u)^-B du, and
+//   x <= 3: Pfaff's transformation, 2F1 = (1+x)^-B sum_n (B)_n/(A+1)_n w^n,
+//           w = x/(1+x) <= 3/4 (positive terms for B > 0);
+//   x > 3 : I(x) = I(3) + int_3^x u^(A-B-1) (1+1/u)^-B du with the binomial
+//           series of (1+1/u)^-B integrated term by term (a logarithm for
+//           the term whose power is u^-1, when 3 - beta = A - B is a
+//           nonnegative integer).
+// Pinned against mpmath.hyp2f1.
+// ---------------------------------------------------------------------------
+template <class R> R hyp2f1_ap1_pfaff(double A, double B, R x) {
+  R w = x / (R(1.) + x);
+  R sum = 1., term = 1.;
+  for (int n = 0; n < 5000; n++) {
+    term = term * R((B + n) / (A + 1. + n)) * w;
+    sum += term;
+    if (fabs(term) < fabs(sum) * R(kIGEps)) break;
+  }
+  return pow(R(1.) + x, R(-B)) * sum;
+}
+// amb = A - B as the caller knows it exactly (3 - beta)
+template <class R> R hyp2f1_ap1(double A, double B, double amb, R x) {
+  const double x0 = 3.;
+  if (x <= R(x0)) return hyp2f1_ap1_pfaff(A, B, x);
+  R I = R(pow(x0, A) / A) * hyp2f1_ap1_pfaff(A, B, R(x0));
+  R ck = 1.;  // (-1)^k (B)_k / k!
+  R lx = log(x / R(x0));
+  for (int k = 0; k < 2000; k++) {
+    if (k > 0) ck = ck * R(-(B + k - 1.) / k);
+    double e = amb - k;
+    R add = (e == 0.) ? ck * lx : ck * (pow(x, R(e)) - R(pow(x0, e))) / R(e);
+    I += add;
+    if (k > 2 && fabs(add) < fabs(I) * R(kIGEps) && fabs(ck) * pow(R(x0), R(e)) < fabs(I) * R(kIGEps))
+      break;
+  }
+  return R(A) * pow(x, R(-A)) * I;
+}
+
+// ---------------------------------------------------------------------------
 // Potential components.  Parameters are decoded once from galpy's packed
 // (pot_type, pot_args) arrays (integrateFullOrbit.c:69-1045, cases 0,1,5,8,9,15)
 // and kept read-only; constants that the reference recomputes on every call
@@ -158,7 +236,9 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kGaussianAmpWrap = -5,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
                kTriaxHernquist = 21, kTriaxNFW = 22, kTriaxJaffe = 23, kPerfectEllipsoid = 30,
-               kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43 };
+               kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43,
+               // spherical components with special functions (integrateFullOrbit.c:731-780)
+               kEinasto = 41, kTwoPowerSph = 42, kExpTruncNFW = 46 };
 
 inline bool is_ellipsoidal(int code) {
   return code == kTriaxHernquist || code == kTriaxNFW || code == kTriaxJaffe ||
@@ -209,6 +289,8 @@ inline int component_nargs(int code, bool planar = false) {
     case kSteadySpiral: return planar ? 8 : -1;  // planar only
     case kSoftNeedleBar: return 13;  // amp, a, b, c2, pa, omegab + 7 cache slots
     case kDehnenSph: case kHomogSphere: return 3;
+    case kEinasto: case kExpTruncNFW: return 3;  // amp, h, n / amp, a, rc
+    case kTwoPowerSph: return 4;                  // amp, a, alpha, beta
     // wrappers' own arguments, read after the wrapped components
     case kDehnenSmoothWrap: return 4;   // amp, tform, tsteady, grow
     case kSolidBodyRotWrap: return 3;   // amp, omega, pa
@@ -475,6 +557,29 @@ template <class R> R bar_eval(const Component& C, R Rc, R z, R phi, R t) {
 // restates planarRforce (negated) and planarR2deriv at radius r; the 3D
 // force is -Phi'(r) x / r.
 // ---------------------------------------------------------------------------
+// ExpTruncNFWPotential.c:11-33: F(r) = M(<r)/amp (a series below 1e-3
+// min(a, rc)) and G(r) of the potential -(F/r + G)
+template <class R> R exptruncnfw_F(R r, double a, double rc) {
+  double small_r_thresh = 1e-3 * (a < rc ? a : rc);
+  if (r < R(small_r_thresh)) {
+    double c2 = 0.5;
+    double c3 = -(1.0 / rc + 2.0 / a) / 3.0;
+    double c4 = (0.5 / (rc * rc) + 2.0 / (rc * a) + 3.0 / (a * a)) / 4.0;
+    double c5 = -(1.0 / (6.0 * rc * rc * rc) + 1.0 / (rc * rc * a) + 3.0 / (rc * a * a) + 4.0 / (a * a * a)) / 5.0;
+    return (r * r / R(a * a)) * (R(c2) + r * (R(c3) + r * (R(c4) + r * R(c5))));
+  }
+  double alpha = a / rc;
+  R beta = (R(a) + r) / R(rc);
+  // exp(alpha) (1 + alpha) (E1(alpha) - E1(beta)), with exp(alpha) E1(beta)
+  // = exp(-r/rc) (e^beta E1(beta)) so that nothing overflows at large r
+  R ex = exp(-r / R(rc));
+  return R(1. + alpha) * (e1_scaled(R(alpha)) - ex * e1_scaled(beta)) - R(1.) + R(a) * ex / (R(a) + r);
+}
+template <class R> R exptruncnfw_G(R r, double a, double rc) {
+  R beta = (R(a) + r) / R(rc);
+  return exp(-r / R(rc)) / (R(a) + r) - exp(-r / R(rc)) * e1_scaled(beta) / R(rc);
+}
+
 template <class R> void sph_dphi(const Component& C, R r, R& Phip, R& Phipp) {
   double amp = C.p[0];
   switch (C.type) {
@@ -545,6 +650,43 @@ template <class R> void sph_dphi(const Component& C, R r, R& Phip, R& Phipp) {
       }
       break;
     }
+    case kEinasto: {
+      // EinastoPotential.c rforce (:27-47), r2deriv (:49-70).  galpy forms
+      // Q(3n, x) - 1, which cancels to nothing for x = s^(1/n) << 3n (a
+      // relative error of 1e-3 at r = 1e-3 h for n = 2.5); restated with
+      // -P(3n, x) = -gamma(3n, x) / Gamma(3n), the same function without the
+      // cancellation.
+      double h = C.p[1], n = C.p[2];
+      R s = r / R(h);
+      double g3n = tgamma(3. * n);
+      R xs = pow(s, R(1. / n));
+      R Qm1 = -lower_incgamma(3. * n, xs, g3n) / R(g3n);
+      Phip = -R(amp) * (R(4. * M_PI * h * n * g3n) * pow(s, R(-2.)) * Qm1);
+      Phipp = R(amp) * -(R(4. * M_PI * n * g3n) *
+                         ((R(-2.) * pow(s, R(-3.))) * Qm1 - (R(1. / n) * exp(-xs) / R(g3n))));
+      break;
+    }
+    case kExpTruncNFW: {  // ExpTruncNFWPotential.c rforce, r2deriv and F (:11-28)
+      double a = C.p[1], rc = C.p[2];
+      R F = exptruncnfw_F(r, a, rc);
+      Phip = R(amp) * F / (r * r);
+      R dens = exp(-r / R(rc)) / (R(4. * M_PI * a * a) * r * pow(R(1.) + r / R(a), R(2.)));
+      Phipp = R(amp) * (R(4. * M_PI) * dens - R(2.) * F / (r * r * r));
+      break;
+    }
+    case kTwoPowerSph: {  // TwoPowerSphericalPotential.c PlanarRforce, PlanarR2deriv
+      double a = C.p[1], alpha = C.p[2], beta = C.p[3];
+      R Aa = R(pow(a, alpha - 3.) / (3. - alpha));
+      R hyper = hyp2f1_ap1(3. - alpha, beta - alpha, 3. - beta, r / R(a));
+      R hyper_deriv = R((3. - alpha) * (beta - alpha) / (4. - alpha)) *
+                      hyp2f1_ap1(4. - alpha, 1. + beta - alpha, 3. - beta, r / R(a));
+      Phip = R(amp) * pow(r, R(1. - alpha)) * Aa * hyper;
+      R term1 = Aa * pow(r, R(-alpha)) * hyper;
+      R term2 = R(-alpha) * Aa * pow(r, R(-alpha)) * hyper;
+      R term3 = -Aa * pow(r, R(1. - alpha)) * R(1. / a) * hyper_deriv;
+      Phipp = R(amp) * (term1 + term2 + term3);
+      break;
+    }
     default:
       Phip = 0.;
       Phipp = 0.;
@@ -554,7 +696,7 @@ template <class R> void sph_dphi(const Component& C, R r, R& Phip, R& Phipp) {
 inline bool is_spherical_new(int code) {
   return code == kPowerSph || code == kJaffe || code == kIsochrone || code == kPlummer ||
          code == kPseudoIso || code == kBurkert || code == kDehnenCoreSph || code == kDehnenSph ||
-         code == kHomogSphere;
+         code == kHomogSphere || code == kEinasto || code == kExpTruncNFW || code == kTwoPowerSph;
 }
 
 // 3D forces of the spherical components as their files write them
@@ -603,6 +745,19 @@ template <class R> R sph_eval(const Component& C, R r) {
       if (r * r < R(R2)) return R(amp) * (r * r - R(3. * R2));
       return R(-2.) * R(amp) * R(R3) / r;
     }
+    case kEinasto: {  // EinastoPotential.c revaluate (:9-26; GSL's gamma_inc is the upper Gamma(a,x));
+      double h = C.p[1], n = C.p[2];
+      R s = r / R(h);
+      R xs = pow(s, R(1. / n));
+      double g3n = tgamma(3. * n);
+      // 1 - Q(3n, x) as P(3n, x) (see sph_dphi)
+      return R(amp) * (-(R(4. * M_PI * h * h * n * g3n)) * pow(s, R(-1.)) *
+                       (lower_incgamma(3. * n, xs, g3n) / R(g3n) +
+                        s * upper_incgamma(2. * n, xs, tgamma(2. * n)) / R(g3n)));
+    }
+    case kExpTruncNFW:  // ExpTruncNFWPotential.c revaluate
+      return R(amp) * -(exptruncnfw_F(r, C.p[1], C.p[2]) / r + exptruncnfw_G(r, C.p[1], C.p[2]));
+    case kTwoPowerSph: return R(NAN);  // its Eval needs 2F1 outside the restated family
     default: return R(0.);
   }
 }

@@ -464,3 +464,15 @@ ORC_EXPORT int oracle_count_ops_dxdv(int nobj, int dim, double* yo, int nt, doub
   out[5] = (double)steps;
   return 0;
 }
+
+// e^x E1(x) and 2F1(A, B; A+1; -x) at n points (their own pins against
+// mpmath and, for 2F1, against the reference's xsf build)
+ORC_EXPORT int oracle_e1_scaled(int n, const double* x, double* out) {
+  for (int i = 0; i < n; i++) out[i] = e1_scaled(x[i]);
+  return 0;
+}
+ORC_EXPORT int oracle_hyp2f1_ap1(int n, double A, double B, double amb, const double* x,
+                                 double* out) {
+  for (int i = 0; i < n; i++) out[i] = hyp2f1_ap1(A, B, amb, x[i]);
+  return 0;
+}
