@@ -48,9 +48,9 @@ def test_error_reporting_without_gpu_work():
                    ctypes.c_void_p, ctypes.c_void_p]
     import numpy
 
-    pt = numpy.array([42], dtype=numpy.int32)
+    pt = numpy.array([999], dtype=numpy.int32)  # no galpy potential has this code
     pa = numpy.zeros(4)
     x = numpy.ones(1)
     out = numpy.zeros(1)
     assert fn(1, pt.ctypes.data, pa.ctypes.data, 0, 1, x.ctypes.data, out.ctypes.data) != 0
-    assert "pot_type 42" in _lib.last_error()
+    assert "pot_type 999" in _lib.last_error()

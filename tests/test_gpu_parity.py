@@ -236,11 +236,11 @@ def test_unsupported_method_and_potential_raise():
                                code, None)
         assert numpy.all(err == -1)
         assert "odeint_type" in _lib.last_error()
-    bad = numpy.array([42], dtype=numpy.int32)
+    bad = numpy.array([999], dtype=numpy.int32)  # no galpy potential has this code
     err[:] = 0
     lib.integrateFullOrbit(4, yo.copy(), 5, t, 0, 1, bad, pa, None, -9999.99, 0.0, 0.0, res, err, 3, None)
     assert numpy.all(err == -1)
-    assert "pot_type 42" in _lib.last_error()
+    assert "pot_type 999" in _lib.last_error()
 
 
 def test_galpypaper_golden_numbers():
