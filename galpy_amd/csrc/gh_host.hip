@@ -95,6 +95,9 @@ int nargs_of(int code, bool planar) {
     case GH_ELLIPDISK: return planar ? 6 : -1;  // planar only
     case GH_LOPSIDED: return planar ? 4 : -1;   // planar only
     case GH_STEADYSPIRAL: return planar ? 8 : -1;  // planar only
+    case GH_TRANSIENTSPIRAL: return planar ? 8 : -1;  // planar only
+    case GH_COSMPHIDISK: return planar ? 9 : -1;      // planar only
+    case GH_HENONHEILES: return planar ? 1 : -1;      // planar only
     case GH_SOFTNEEDLEBAR: return 13;  // amp, a, b, c^2, pa, omegab + 7 cache slots
     // wrappers' own arguments, read after the wrapped components
     // (integrateFullOrbit.c:821-895, integratePlanarOrbit.c:591-663)
@@ -363,7 +366,13 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
   {
     GhComponent& C = T->comp[i];
     C.type = code;
-    for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
+    // the first 8 arguments in p; the few components with more keep the
+    // rest in c (decoded below)
+    double extra[8];
+    for (int j = 0; j < na; j++) {
+      if (j < 8) C.p[j] = *pot_args++;
+      else extra[j - 8] = *pot_args++;
+    }
     if (code == GH_FLATPOWER && planar) {  // full layout, q^2 = 1 (z = 0 in the plane)
       C.p[3] = C.p[2];
       C.p[2] = 1.0;
@@ -428,6 +437,14 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
       case GH_FLATPOWER: C.c[0] = 1.0 / C.p[2]; break;   // 1/q^2
       case GH_SOFTNEEDLEBAR: C.nonaxi = 1; break;
       case GH_STEADYSPIRAL: C.nonaxi = 1; break;
+      case GH_TRANSIENTSPIRAL: C.nonaxi = 1; break;
+      case GH_HENONHEILES: C.nonaxi = 1; break;
+      case GH_COSMPHIDISK:
+        C.p[4] = (double)(int)C.p[4];  // m, read as an int (CosmphiDiskPotential.c:13)
+        C.c[0] = extra[0];             // r1p
+        C.c[2] = C.p[1] / C.p[4];      // mphio / m
+        C.nonaxi = 1;
+        break;
       case GH_LOPSIDED:
         C.c[0] = cos(C.p[3]);
         C.c[1] = sin(C.p[3]);

@@ -54,6 +54,9 @@ enum GhPotCode {
   GH_ELLIPDISK = 4,  // planar only (integratePlanarOrbit's code 4)
   GH_LOPSIDED = 6,   // planar only (integratePlanarOrbit's code 6)
   GH_STEADYSPIRAL = 3,  // planar only (integratePlanarOrbit's code 3)
+  GH_TRANSIENTSPIRAL = 2,  // planar only (integratePlanarOrbit's code 2)
+  GH_COSMPHIDISK = 28,     // planar only (integratePlanarOrbit's code 28)
+  GH_HENONHEILES = 29,     // planar only (integratePlanarOrbit's code 29)
   GH_SOFTNEEDLEBAR = 25,
   GH_NULLPOT = 40,
   // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
@@ -728,6 +731,61 @@ __device__ __forceinline__ void gh_comp_force<GH_STEADYSPIRAL>(const GhPotTable*
   F.FphioR2 += -aA / C->p[4] * C->p[5] * sn * iR2;
 }
 
+// TransientLogSpiralPotential.c:5-42 (planar only): Phi = amp A
+// exp(-(t - to)^2 / (2 sigma^2)) / alpha cos(alpha ln R - m (phi - omegas t
+// - gamma)), the spiral's argument as gh_spiral_arg forms it (p[4..7] are
+// alpha, m, omegas, gamma in both spirals); p: amp, A, to, sigma^2, alpha,
+// m, omegas, gamma.
+__device__ __forceinline__ double gh_transient_amp(const GhComponent* __restrict__ C, double t) {
+  const double d = t - C->p[2];
+  return C->p[0] * C->p[1] * exp(-d * d / 2.0 / C->p[3]);
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_TRANSIENTSPIRAL>(const GhPotTable* __restrict__ T,
+                                                                  int i, const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double aA = gh_transient_amp(C, g.t);
+  const double sn = gh_sincos(gh_spiral_arg(C, g.R2, gh_phi_acos(g.x, g.y, g.R2), g.t)).s;
+  const double iR2 = gh_rcp(g.R2);
+  F.FRoR += aA * sn * iR2;
+  F.FphioR2 += -aA / C->p[4] * C->p[5] * sn * iR2;
+}
+
+// CosmphiDiskPotential.c:4-37 (planar only): Phi = amp mphio/m R^p
+// cos(m phi - mphib) outside the break radius rb, amp mphio/m rbp (2 r1p -
+// rbp R^-p) cos(m phi - mphib) inside (F_R continuous at
+// rb; the potential too when r1 = 1); p: amp, mphio, p, mphib, m, rb, rbp,
+// rb2p; c[0] = r1p (galpy's ninth argument), c[2] = mphio / m.
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_COSMPHIDISK>(const GhPotTable* __restrict__ T,
+                                                              int i, const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double amp = C->p[0], pp = C->p[2];
+  const GhSinCos sc = gh_sincos(C->p[4] * gh_phi_acos(g.x, g.y, g.R2) - C->p[3]);
+  const double iR2 = gh_rcp(g.R2);
+  const double Rp = exp(0.5 * pp * log(g.R2));  // R^p
+  if (sqrt(g.R2) <= C->p[5]) {  // R <= rb as galpy compares it
+    F.FRoR += -amp * pp * C->c[2] * C->p[7] / Rp * iR2 * sc.c;
+    F.FphioR2 += amp * C->p[1] * C->p[6] * (2.0 * C->c[0] - C->p[6] / Rp) * iR2 * sc.s;
+  } else {
+    F.FRoR += -amp * pp * C->c[2] * Rp * iR2 * sc.c;
+    F.FphioR2 += amp * C->p[1] * Rp * iR2 * sc.s;
+  }
+}
+
+// HenonHeilesPotential.c:4-14 (planar only): Phi = amp/2 (R^2 + 2/3 R^3
+// sin 3 phi) = amp/2 (x^2 + y^2) + amp/3 (3 x^2 y - y^3); p: amp.
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_HENONHEILES>(const GhPotTable* __restrict__ T,
+                                                              int i, const GhGeom& g, GhAcc& F) {
+  const double amp = T->comp[i].p[0];
+  const double iR2 = gh_rcp(g.R2);
+  const double s3 = (3.0 * g.x * g.x - g.y * g.y) * g.y;  // R^3 sin 3 phi
+  const double c3 = (g.x * g.x - 3.0 * g.y * g.y) * g.x;  // R^3 cos 3 phi
+  F.FRoR += -amp * (1.0 + s3 * iR2);
+  F.FphioR2 += -amp * c3 * iR2;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_KKSTAECKEL>(const GhPotTable* __restrict__ T,
                                                              int i, const GhGeom& g, GhAcc& F) {
@@ -1011,6 +1069,9 @@ __device__ __forceinline__ void gh_comp_force_dyn(const GhPotTable* __restrict__
     case GH_ELLIPDISK: gh_comp_force<GH_ELLIPDISK>(T, i, g, F); break;
     case GH_LOPSIDED: gh_comp_force<GH_LOPSIDED>(T, i, g, F); break;
     case GH_STEADYSPIRAL: gh_comp_force<GH_STEADYSPIRAL>(T, i, g, F); break;
+    case GH_TRANSIENTSPIRAL: gh_comp_force<GH_TRANSIENTSPIRAL>(T, i, g, F); break;
+    case GH_COSMPHIDISK: gh_comp_force<GH_COSMPHIDISK>(T, i, g, F); break;
+    case GH_HENONHEILES: gh_comp_force<GH_HENONHEILES>(T, i, g, F); break;
     case GH_SOFTNEEDLEBAR: gh_comp_force<GH_SOFTNEEDLEBAR>(T, i, g, F); break;
     case GH_BURKERT: gh_comp_force<GH_BURKERT>(T, i, g, F); break;
     case GH_DEHNENCORESPH: gh_comp_force<GH_DEHNENCORESPH>(T, i, g, F); break;
@@ -1355,6 +1416,18 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
       case GH_STEADYSPIRAL:  // SteadyLogSpiralPotential.py _evaluate (planar only)
         pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
                cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
+        break;
+      case GH_TRANSIENTSPIRAL:  // TransientLogSpiralPotential.py _evaluate (planar only)
+        pot += gh_transient_amp(C, t) / C->p[4] * cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
+        break;
+      case GH_COSMPHIDISK: {  // CosmphiDiskPotential.py _evaluate (planar only)
+        const double cm = cos(C->p[4] * atan2(sinphi, cosphi) - C->p[3]);
+        const double Rp = pow(R, C->p[2]);
+        pot += C->p[0] * C->c[2] * cm * ((R < C->p[5]) ? C->p[6] * (2.0 * C->c[0] - C->p[6] / Rp) : Rp);
+        break;
+      }
+      case GH_HENONHEILES:  // HenonHeilesPotential.py _evaluate (planar only)
+        pot += C->p[0] * 0.5 * R2 * (1.0 + 2.0 / 3.0 * R * (3.0 * cosphi * cosphi - sinphi * sinphi) * sinphi);
         break;
       case GH_LOPSIDED:  // CosmphiDiskPotential.py _evaluate at m = 1 (planar only)
         pot += C->p[0] * C->p[1] * pow(R2, 0.5 * C->p[2]) * (cosphi * C->c[0] + sinphi * C->c[1]);

@@ -502,6 +502,145 @@ class SteadyLogSpiralPotential(Potential):
         return [self._amp, tf, ts, self._A, self._alpha, float(self._m), self._omegas, self._gamma]
 
 
+class TransientLogSpiralPotential(Potential):
+    """Planar only: integratePlanarOrbit's pot_type 2, args [amp, A, to,
+    sigma^2, alpha, m, omegas, gamma] (TransientLogSpiralPotential.py:20-160,
+    integratePlanarOrbit.py:94-98): Phi = amp A exp(-(t - to)^2 / (2
+    sigma^2)) / alpha cos(alpha ln R - m (phi - omegas t - gamma)),
+    alpha = m / tan(p) when the pitch angle p is given."""
+
+    pot_code = 2
+    isNonAxi = True
+
+    def __init__(self, amp=1.0, omegas=0.65, A=-0.035, alpha=-7.0, m=2, gamma=numpy.pi / 4.0, p=None,
+                 sigma=1.0, to=0.0):
+        Potential.__init__(self, amp=amp)
+        self._omegas = omegas
+        self._A = A
+        self._m = m
+        self._gamma = gamma
+        self._to = to
+        self._sigma2 = sigma**2.0
+        self._alpha = self._m / numpy.tan(p) if p is not None else alpha
+
+    def _g(self, t):
+        return numpy.exp(-((numpy.asarray(t, dtype=float) - self._to) ** 2.0) / 2.0 / self._sigma2)
+
+    def _arg(self, R, phi, t):
+        return self._alpha * numpy.log(R) - self._m * (phi - self._omegas * t - self._gamma)
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return self._A * self._g(t) / self._alpha * numpy.cos(self._arg(R, phi, t))
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return self._A * self._g(t) / R * numpy.sin(self._arg(R, phi, t))
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * R
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return -self._A * self._g(t) / self._alpha * self._m * numpy.sin(self._arg(R, phi, t))
+
+    def c_args(self):
+        raise NotImplementedError("TransientLogSpiralPotential is a planar potential: integrate planar orbits")
+
+    def c_args_planar(self):
+        return [self._amp, self._A, self._to, self._sigma2, self._alpha, float(self._m), self._omegas,
+                self._gamma]
+
+
+class CosmphiDiskPotential(Potential):
+    """Planar only: integratePlanarOrbit's pot_type 28, args [amp, mphio, p,
+    mphib, m, rb, rbp, rb2p, r1p] (CosmphiDiskPotential.py:26-200,
+    integratePlanarOrbit.py:410-414): Phi = amp mphio/m (R/r1)^p cos(m phi -
+    mphib) outside the break radius rb and amp mphio/m rb^p (2 r1^p - rb^p
+    R^-p) cos(m phi - mphib) / r1^p inside (F_R is continuous at rb, the
+    potential only for r1 = 1); (cp, sp) = m phio (cos m phib, sin m phib)
+    may be given instead of (phib, phio)."""
+
+    pot_code = 28
+    isNonAxi = True
+
+    def __init__(self, amp=1.0, phib=25.0 * numpy.pi / 180.0, p=1.0, phio=0.01, m=4, r1=1.0, rb=None,
+                 cp=None, sp=None):
+        Potential.__init__(self, amp=amp)
+        self._r1p = r1**p
+        self._amp /= self._r1p
+        self._m = int(m)
+        if cp is None or sp is None:
+            self._phib = phib
+            self._mphio = phio * self._m
+        else:
+            self._mphio = numpy.sqrt(cp * cp + sp * sp)
+            self._phib = numpy.arctan(sp / cp) / self._m
+            if m < 2.0 and cp < 0.0:
+                self._phib = numpy.pi + self._phib
+        self._p = p
+        if rb is None:
+            self._rb, self._rbp, self._rb2p = 0.0, 1.0, 1.0
+        else:
+            self._rb = rb
+            self._rbp = self._rb**self._p
+            self._rb2p = self._rbp**2.0
+        self._mphib = self._m * self._phib
+
+    def _inner(self, R):
+        return numpy.asarray(R, dtype=float) < self._rb
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        R = numpy.asarray(R, dtype=float)
+        c = numpy.cos(self._m * phi - self._mphib) * self._mphio / self._m
+        return c * numpy.where(self._inner(R), self._rbp * (2.0 * self._r1p - self._rbp / R**self._p),
+                               R**self._p)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        R = numpy.asarray(R, dtype=float)
+        c = -self._p * self._mphio / self._m * numpy.cos(self._m * phi - self._mphib)
+        return c * numpy.where(self._inner(R), self._rb2p / R ** (self._p + 1.0), R ** (self._p - 1.0))
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * numpy.asarray(R, dtype=float)
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        R = numpy.asarray(R, dtype=float)
+        s = self._mphio * numpy.sin(self._m * phi - self._mphib)
+        return s * numpy.where(self._inner(R), self._rbp * (2.0 * self._r1p - self._rbp / R**self._p),
+                               R**self._p)
+
+    def c_args(self):
+        raise NotImplementedError("CosmphiDiskPotential is a planar potential: integrate planar orbits")
+
+    def c_args_planar(self):
+        return [self._amp, self._mphio, self._p, self._mphib, float(self._m), self._rb, self._rbp,
+                self._rb2p, self._r1p]
+
+
+class HenonHeilesPotential(Potential):
+    """Planar only: integratePlanarOrbit's pot_type 29, args [amp]
+    (HenonHeilesPotential.py:9-60): Phi = amp/2 (R^2 + 2/3 R^3 sin 3 phi)."""
+
+    pot_code = 29
+    isNonAxi = True
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return 0.5 * R * R * (1.0 + 2.0 / 3.0 * R * numpy.sin(3.0 * phi))
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return -R * (1.0 + R * numpy.sin(3.0 * phi))
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * numpy.asarray(R, dtype=float)
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return -(R**3.0) * numpy.cos(3.0 * phi)
+
+    def c_args(self):
+        raise NotImplementedError("HenonHeilesPotential is a planar potential: integrate planar orbits")
+
+    def c_args_planar(self):
+        return [self._amp]
+
+
 class NFWPotential(Potential):
     """pot_type 9, args [amp, a]."""
 

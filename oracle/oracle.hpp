@@ -31,13 +31,13 @@ namespace orc {
 
 using std::sqrt; using std::log; using std::exp; using std::pow;
 using std::fabs; using std::sin; using std::cos; using std::acos;
-using std::atan2; using std::atan; using std::tgamma;
+using std::atan2; using std::atan; using std::tgamma; using std::cosh; using std::tanh;
 
 inline double dval(double x) { return x; }
 
 // ---------------------------------------------------------------------------
 // Op-counting scalar.  Each IEEE add/sub/mul/div/sqrt is one counted op; each
-// libm transcendental call (exp, log, pow, sin, cos, acos, atan2) is counted
+// libm transcendental call (exp, log, pow, sin, cos, cosh, tanh, acos, atan2) is counted
 // separately as a "special" call.  Comparisons, fabs, negation and selects are
 // free.  Counters are thread-local so OpenMP loops can count in parallel.
 // ---------------------------------------------------------------------------
@@ -77,6 +77,8 @@ inline CountedD cos(CountedD a) { g_ops.special++; return CountedD(std::cos(a.v)
 inline CountedD acos(CountedD a) { g_ops.special++; return CountedD(std::acos(a.v)); }
 inline CountedD atan2(CountedD a, CountedD b) { g_ops.special++; return CountedD(std::atan2(a.v, b.v)); }
 inline CountedD atan(CountedD a) { g_ops.special++; return CountedD(std::atan(a.v)); }
+inline CountedD cosh(CountedD a) { g_ops.special++; return CountedD(std::cosh(a.v)); }
+inline CountedD tanh(CountedD a) { g_ops.special++; return CountedD(std::tanh(a.v)); }
 
 template <class R> inline R rmax(R a, R b) { return (a > b) ? a : b; }
 template <class R> inline R rmin(R a, R b) { return (a < b) ? a : b; }
@@ -231,6 +233,8 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kPseudoIso = 18, kKuzminDisk = 19, kBurkert = 20, kDehnenCoreSph = 33,
                kDehnenSph = 34, kHomogSphere = 35, kFlattenedPower = 12, kNull = 40,
                kKKStaeckel = 16, kSoftNeedleBar = 25, kEllipDisk = 4, kLopsided = 6, kSteadySpiral = 3,  // 3, 4, 6: planar only
+               kTransientSpiral = 2, kCosmphiDisk = 28, kHenonHeiles = 29,  // planar only
+               kSpiralArms = 27,
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5,
@@ -252,7 +256,7 @@ constexpr int kOracleMaxComp = 16;
 
 struct Component {
   int type;
-  double p[8];  // raw arguments in galpy's order
+  double p[16];  // raw arguments in galpy's order
   double c[6];  // precomputed constants
   // a wrapper's wrapped components (potentialArg.nwrapped /
   // wrappedPotentialArg, galpy_potentials.h), stored in the same parse array
@@ -287,6 +291,9 @@ inline int component_nargs(int code, bool planar = false) {
     case kEllipDisk: return planar ? 6 : -1;  // planar only
     case kLopsided: return planar ? 4 : -1;   // planar only
     case kSteadySpiral: return planar ? 8 : -1;  // planar only
+    case kTransientSpiral: return planar ? 8 : -1;  // amp, A, to, sigma^2, alpha, m, omegas, gamma
+    case kCosmphiDisk: return planar ? 9 : -1;  // amp, mphio, p, mphib, m, rb, rbp, rb2p, r1p
+    case kHenonHeiles: return planar ? 1 : -1;  // amp
     case kSoftNeedleBar: return 13;  // amp, a, b, c2, pa, omegab + 7 cache slots
     case kDehnenSph: case kHomogSphere: return 3;
     case kEinasto: case kExpTruncNFW: return 3;  // amp, h, n / amp, a, rc
@@ -303,6 +310,11 @@ inline int component_nargs(int code, bool planar = false) {
 // Arguments of an EllipsoidalPotential: amp, 15 cache slots, npsi, the psi
 // arguments, b^2, c^2, aligned, rot[9], glorder, glx[glorder], glw[glorder]
 // (integrateFullOrbit.py:221-300; nargs as integrateFullOrbit.c:437).
+// SpiralArmsPotential: nCs, amp, N, sin_alpha, tan_alpha, r_ref, phi_ref,
+// Rs, H, omega, Cs[nCs] (integrateFullOrbit.py:337-352; nargs as
+// integrateFullOrbit.c:545).
+inline int spiralarms_nargs(const double* a) { return 10 + (int)a[0]; }
+
 inline int ellipsoidal_nargs(const double* a) {
   const int npsi = (int)a[16];
   const int glorder = (int)a[17 + npsi + 12];
@@ -319,7 +331,9 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
                             int& next, Component* dst, bool planar) {
   for (int i = 0; i < n; i++) {
     int code = *pot_type++;
-    int na = is_ellipsoidal(code) ? ellipsoidal_nargs(pot_args) : component_nargs(code, planar);
+    int na = is_ellipsoidal(code)    ? ellipsoidal_nargs(pot_args)
+             : code == kSpiralArms ? spiralarms_nargs(pot_args)
+                                   : component_nargs(code, planar);
     if (na < 0) return -1;
     Component& C = dst[i];
     C.type = code;
@@ -337,7 +351,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       C.wrapped = kids;
     }
     C.args = pot_args;
-    if (is_ellipsoidal(code)) {
+    if (is_ellipsoidal(code) || code == kSpiralArms) {
       C.p[0] = pot_args[0];
       pot_args += na;
       continue;
@@ -1005,6 +1019,7 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kKuzminDisk: kuzmin_forces(C, Rc, z, fr, fz); break;
       case kKKStaeckel: kks_forces(C, Rc, z, fr, fz); break;
       case kSoftNeedleBar: needle_forces(C, Rc, z, phi, t, fr, fz, fp); break;
+      case kSpiralArms: spiral_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Rforce, zforce
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.) {
@@ -1036,6 +1051,71 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
     }
     FR += fr; Fz += fz; Fphi += fp;
   }
+}
+
+// SpiralArmsPotential.c (Cox & Gomez 2002 arms): the helpers gam, dgam_dR,
+// K, B, D, dK_dR, dB_dR, dD_dR (:904-940) and the forces Rforce (:61-126),
+// zforce (:128-170), phitorque (:172-210), Eval (:21-58), in the reference's
+// order.  The planar functions (:625-716) are these at z = 0.
+template <class R> R spa_K(R Rc, double n, double N, double sa) { return R(n * N) / Rc / R(sa); }
+template <class R> R spa_B(R Rc, double H, double n, double N, double sa) {
+  R HNn_R = R(H * N * n) / Rc;
+  return HNn_R / R(sa) * (R(0.4) * HNn_R / R(sa) + R(1.));
+}
+template <class R> R spa_D(R Rc, double H, double n, double N, double sa) {
+  double HNn = H * N * n;
+  return (R(0.3 * HNn * HNn / sa) / Rc + R(HNn) + Rc * R(sa)) / (R(0.3 * HNn) + Rc * R(sa));
+}
+template <class R> R spa_dK(R Rc, double n, double N, double sa) { return R(-n * N) / (Rc * Rc) / R(sa); }
+template <class R> R spa_dB(R Rc, double H, double n, double N, double sa) {
+  double HNn = H * N * n;
+  return R(-HNn) / Rc / Rc / Rc / R(sa) / R(sa) * (R(0.8 * HNn) + Rc * R(sa));
+}
+template <class R> R spa_dD(R Rc, double H, double n, double N, double sa) {
+  R k = R(H * N * n) / Rc / R(sa);
+  return k * (R(0.3) * (k + R(0.3) * k * k + R(1.)) / Rc / pow(R(0.3) * k + R(1.), R(2.)) -
+              (R(1.) / Rc * (R(1.) + R(0.6) * k) / (R(0.3) * k + R(1.))));
+}
+template <class R> void spiral_forces(const Component& C, R Rc, R z, R phi, R t, R& FR, R& Fz, R& Fphi) {
+  const double* a = C.args;
+  const int nCs = (int)a[0];
+  const double amp = a[1], N = a[2], sa = a[3], ta = a[4], r_ref = a[5], phi_ref = a[6], Rs = a[7], H = a[8],
+               omega = a[9];
+  R g = R(N) * ((phi - R(omega) * t) - R(phi_ref) - log(Rc / R(r_ref)) / R(ta));
+  R dg = R(-N) / Rc / R(ta);
+  R sr = 0., sz = 0., sp = 0.;
+  for (int k = 1; k <= nCs; k++) {
+    const double n = k, Cn = a[9 + k];
+    R Kn = spa_K(Rc, n, N, sa), Bn = spa_B(Rc, H, n, N, sa), Dn = spa_D(Rc, H, n, N, sa);
+    R dKn = spa_dK(Rc, n, N, sa), dBn = spa_dB(Rc, H, n, N, sa), dDn = spa_dD(Rc, H, n, N, sa);
+    R cng = cos(R(n) * g), sng = sin(R(n) * g);
+    R zKB = z * Kn / Bn;
+    R sech = R(1.) / cosh(zKB);
+    sr += R(Cn) * pow(sech, Bn) / Dn *
+          ((R(n) * dg / Kn * sng + cng * (z * tanh(zKB) * (dKn / Kn - dBn / Bn) - dBn / Kn * log(sech) +
+                                          dKn / Kn / Kn + dDn / Dn / Kn)) +
+           cng / Kn / R(Rs));
+    sz += R(Cn) / Dn * cng * tanh(zKB) / pow(cosh(zKB), Bn);
+    sp += R(N * n * Cn) / Dn / Kn / pow(cosh(z * Kn / Bn), Bn) * sng;
+  }
+  R pre = R(-amp * H) * exp(-(Rc - R(r_ref)) / R(Rs));
+  FR = pre * sr;
+  Fz = pre * sz;
+  Fphi = pre * sp;
+}
+template <class R> R spiral_eval(const Component& C, R Rc, R z, R phi, R t) {
+  const double* a = C.args;
+  const int nCs = (int)a[0];
+  const double amp = a[1], N = a[2], sa = a[3], ta = a[4], r_ref = a[5], phi_ref = a[6], Rs = a[7], H = a[8],
+               omega = a[9];
+  R g = R(N) * ((phi - R(omega) * t) - R(phi_ref) - log(Rc / R(r_ref)) / R(ta));
+  R sum = 0.;
+  for (int k = 1; k <= nCs; k++) {
+    const double n = k, Cn = a[9 + k];
+    R Kn = spa_K(Rc, n, N, sa), Bn = spa_B(Rc, H, n, N, sa), Dn = spa_D(Rc, H, n, N, sa);
+    sum += R(Cn) / Kn / Dn * cos(R(n) * g) / pow(cosh(Kn * z / Bn), Bn);
+  }
+  return R(-amp * H) * exp(-(Rc - R(r_ref)) / R(Rs)) * sum;
 }
 
 // Second derivatives d2Phi/dR2, d2Phi/dz2, d2Phi/dRdz of the axisymmetric
@@ -1406,6 +1486,32 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
              sin(R(alpha) * log(Rc) - R(m) * (phi - R(om) * t - R(gam)));
         break;
       }
+      case kTransientSpiral: {  // TransientLogSpiralPotential.c:5-42
+        const double amp = C.p[0], A = C.p[1], to = C.p[2], sigma2 = C.p[3], alpha = C.p[4], m = C.p[5],
+                     om = C.p[6], gam = C.p[7];
+        R g = exp(-pow(t - R(to), R(2.)) / R(2.) / R(sigma2));
+        R sn = sin(R(alpha) * log(Rc) - R(m) * (phi - R(om) * t - R(gam)));
+        fr = R(amp) * R(A) * g / Rc * sn;
+        fp = -R(amp) * R(A) * g / R(alpha) * R(m) * sn;
+        break;
+      }
+      case kCosmphiDisk: {  // CosmphiDiskPotential.c:4-37 (m read as an int)
+        const double amp = C.p[0], mphio = C.p[1], pp = C.p[2], mphib = C.p[3], rb = C.p[5], rbp = C.p[6],
+                     rb2p = C.p[7], r1p = C.p[8];
+        const int m = (int)C.p[4];
+        if (Rc <= R(rb)) {
+          fr = -R(amp) * R(pp) * R(mphio) / R(m) * R(rb2p) / pow(Rc, R(pp + 1.)) * cos(R(m) * phi - R(mphib));
+          fp = R(amp) * R(mphio) * R(rbp) * (R(2.) * R(r1p) - R(rbp) / pow(Rc, R(pp))) * sin(R(m) * phi - R(mphib));
+        } else {
+          fr = -R(amp) * R(pp) * R(mphio) / R(m) * pow(Rc, R(pp - 1.)) * cos(R(m) * phi - R(mphib));
+          fp = R(amp) * R(mphio) * pow(Rc, R(pp)) * sin(R(m) * phi - R(mphib));
+        }
+        break;
+      }
+      case kHenonHeiles:  // HenonHeilesPotential.c:4-14
+        fr = -R(C.p[0]) * Rc * (R(1.) + Rc * sin(R(3.) * phi));
+        fp = -R(C.p[0]) * pow(Rc, R(3.)) * cos(R(3.) * phi);
+        break;
       case kLopsided: {  // LopsidedDiskPotential.c Rforce, phitorque (amp, phio, p, phib)
         const double amp = C.p[0], phio = C.p[1], pp = C.p[2], phib = C.p[3];
         fr = -R(amp) * R(pp) * R(phio) * pow(Rc, R(pp - 1.)) * cos(phi - R(phib));
@@ -1415,6 +1521,11 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
       case kSoftNeedleBar: {
         R fz0;
         needle_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
+        break;
+      }
+      case kSpiralArms: {  // SpiralArmsPotential.c:625-716: the full forces at z = 0
+        R fz0;
+        spiral_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
         break;
       }
       case kKuzminDisk:  // KuzminDiskPotential.c planarRforce
@@ -1469,7 +1580,8 @@ inline bool has_hessian(int npot, const Component* P) {
   for (int i = 0; i < npot; i++) {
     const int c = P[i].type;
     if (c == kKKStaeckel || c == kSoftNeedleBar || is_wrapper(c) || c == kEllipDisk || c == kLopsided ||
-        c == kSteadySpiral || is_ellipsoidal(c))
+        c == kSteadySpiral || c == kTransientSpiral || c == kCosmphiDisk || c == kHenonHeiles ||
+        c == kSpiralArms || is_ellipsoidal(c))
       return false;
   }
   return true;
@@ -1559,6 +1671,7 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kKuzminDisk: pot += kuzmin_eval(C, Rc, z); break;
       case kKKStaeckel: pot += kks_eval(C, Rc, z); break;
       case kSoftNeedleBar: pot += needle_eval(C, Rc, z, phi, t); break;
+      case kSpiralArms: pot += spiral_eval(C, Rc, z, phi, t); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Eval
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.)
