@@ -354,10 +354,44 @@ int decode_twopower(GhPotTable* T, GhComponent& C) {
   return 0;
 }
 
+// SpiralArmsPotential (integrateFullOrbit.py:337-352, integratePlanarOrbit.py:
+// 391-409, the same in both packings): nCs, amp, N, sin alpha, tan alpha,
+// r_ref, phi_ref, Rs, H, omega, then C_1..C_nCs (kept in ext).
+int decode_spiralarms(const double*& pot_args, GhPotTable* T, int i) {
+  GhComponent& C = T->comp[i];
+  const double* a = pot_args;
+  const int nCs = (int)a[0];
+  if (nCs < 1 || T->next + nCs > GH_EXT_LEN) {
+    set_error("SpiralArmsPotential with %d Cs does not fit the table", nCs);
+    return -1;
+  }
+  if (!(a[7] > 0.0) || !(a[8] > 0.0) || a[3] == 0.0 || a[4] == 0.0 || a[2] == 0.0) {
+    set_error("SpiralArmsPotential needs Rs > 0, H > 0, N != 0 and a nonzero pitch angle");
+    return -1;
+  }
+  C.type = GH_SPIRALARMS;
+  for (int j = 0; j < 8; j++) C.p[j] = a[1 + j];  // amp, N, sin a, tan a, r_ref, phi_ref, Rs, H
+  C.c[0] = a[9];
+  C.c[1] = nCs;
+  C.c[2] = 1.0 / a[7];
+  C.c[3] = 1.0 / a[4];
+  C.c[4] = std::log(a[5]);
+  C.c[5] = a[8] * a[2] / a[3];
+  C.ext = T->next;
+  for (int k = 0; k < nCs; k++) T->ext[T->next + k] = a[10 + k];
+  T->next += nCs;
+  T->has_wrap = 1;  // extended dispatch instances
+  C.nonaxi = 1;
+  T->nonaxi = 1;
+  pot_args += 10 + nCs;
+  return 0;
+}
+
 // One (leaf) component's arguments and precomputed constants.
 int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
                      bool planar) {
   if (is_ellipsoidal(code)) return decode_ellipsoidal(code, pot_args, T, i);
+  if (code == GH_SPIRALARMS) return decode_spiralarms(pot_args, T, i);
   const int na = nargs_of(code, planar);
   if (na < 0 || code < 0) {
     set_error("pot_type %d (component %d) has no HIP kernel", code, i);

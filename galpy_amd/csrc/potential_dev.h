@@ -57,6 +57,7 @@ enum GhPotCode {
   GH_TRANSIENTSPIRAL = 2,  // planar only (integratePlanarOrbit's code 2)
   GH_COSMPHIDISK = 28,     // planar only (integratePlanarOrbit's code 28)
   GH_HENONHEILES = 29,     // planar only (integratePlanarOrbit's code 29)
+  GH_SPIRALARMS = 27,
   GH_SOFTNEEDLEBAR = 25,
   GH_NULLPOT = 40,
   // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
@@ -1044,6 +1045,83 @@ __device__ __forceinline__ void gh_comp_force<GH_TWOPOWERSPH>(const GhPotTable* 
   F.Fz += gr * g.z;
 }
 
+// SpiralArmsPotential.c (Cox & Gomez 2002 arms), all three forces in one
+// pass over the terms n = 1..nCs (the reference recomputes K, B, D in each
+// of Rforce :61-126, zforce :128-170 and phitorque :172-210).  Per term,
+// with kappa = H K_n = H N n / (R sin alpha):
+//   B_n = kappa (1 + 0.4 kappa), D_n = (0.3 kappa^2 + kappa + 1) / (0.3 kappa + 1),
+//   dK/dR = -K/R, dB/dR = -kappa (1 + 0.8 kappa) / R,
+//   dD/dR = kappa [0.3 (kappa + 0.3 kappa^2 + 1) / (0.3 kappa + 1)^2
+//                  - (1 + 0.6 kappa) / (0.3 kappa + 1)] / R,
+// u = z K/B; tanh u and log cosh u from e = exp(-2|u|) (one exp and one
+// log1p), sech^B = exp(-B log cosh u); cos n g, sin n g by the angle-addition
+// recurrence from sincos g, g = N (phi - omega t - phi_ref - ln(R/r_ref) /
+// tan alpha) with phi from acos as galpy's evalRectForce forms it.
+// p: amp, N, sin alpha, tan alpha, r_ref, phi_ref, Rs, H; c[0] = omega,
+// c[1] = nCs, c[2] = 1/Rs, c[3] = 1/tan alpha, c[4] = ln r_ref,
+// c[5] = H N / sin alpha; ext = C_1..C_nCs.
+struct GhSpiralSums {
+  double sr, sz, sp, pot;
+};
+template <bool POT>
+__device__ __forceinline__ GhSpiralSums gh_spiral_sums(const GhPotTable* __restrict__ T,
+                                                       const GhComponent* __restrict__ C, double R,
+                                                       double z, double phi, double t) {
+  const double* __restrict__ Cs = &T->ext[C->ext];
+  const int nCs = (int)C->c[1];
+  const double N = C->p[1], iR = 1.0 / R;
+  const double g = N * (phi - C->c[0] * t - C->p[5] - (log(R) - C->c[4]) * C->c[3]);
+  const double dg = -N * iR * C->c[3];
+  const GhSinCos sc1 = gh_sincos(g);
+  double cng = sc1.c, sng = sc1.s;
+  const double K1 = N * iR / C->p[2], kap1 = C->c[5] * iR, az = fabs(z);
+  GhSpiralSums S = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 1; k <= nCs; k++) {
+    const double n = k;
+    const double Kn = n * K1, kap = n * kap1;
+    const double q = 0.3 * kap + 1.0, iq = 1.0 / q;
+    const double Bn = kap * (1.0 + 0.4 * kap);
+    const double Dn = (0.3 * kap * kap + kap + 1.0) * iq;
+    const double u = az / (C->p[7] * (1.0 + 0.4 * kap));  // |z| K / B
+    const double e = exp(-2.0 * u);
+    const double lch = u + log1p(e) - M_LN2;               // log cosh u
+    const double th = copysign((1.0 - e) / (1.0 + e), z);  // tanh(z K / B)
+    const double sechB = exp(-Bn * lch);
+    const double Cn = Cs[k - 1], iK = 1.0 / Kn, iD = 1.0 / Dn;
+    if (POT) {
+      S.pot += Cn * iK * iD * cng * sechB;
+    } else {
+      const double dKoK = -iR, dBoB = -(1.0 + 0.8 * kap) * iR / (1.0 + 0.4 * kap);
+      const double dB = Bn * dBoB;
+      const double dD = kap * iR * (0.3 * (kap + 0.3 * kap * kap + 1.0) * iq * iq - (1.0 + 0.6 * kap) * iq);
+      S.sr += Cn * sechB * iD *
+              (n * dg * iK * sng +
+               cng * (z * th * (dKoK - dBoB) + dB * iK * lch + dKoK * iK + dD * iD * iK) +
+               cng * iK * C->c[2]);
+      S.sz += Cn * iD * cng * th * sechB;
+      S.sp += N * n * Cn * iD * iK * sechB * sng;
+    }
+    const double c2 = cng * sc1.c - sng * sc1.s;
+    sng = sng * sc1.c + cng * sc1.s;
+    cng = c2;
+  }
+  return S;
+}
+__device__ __forceinline__ double gh_spiral_pre(const GhComponent* __restrict__ C, double R) {
+  return -C->p[0] * C->p[7] * exp(-(R - C->p[4]) * C->c[2]);
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_SPIRALARMS>(const GhPotTable* __restrict__ T,
+                                                             int i, const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double R = sqrt(g.R2);
+  const GhSpiralSums S = gh_spiral_sums<false>(T, C, R, g.z, gh_phi_acos(g.x, g.y, g.R2), g.t);
+  const double pre = gh_spiral_pre(C, R), iR2 = gh_rcp(g.R2);
+  F.FRoR += pre * S.sr * (R * iR2);
+  F.Fz += pre * S.sz;
+  F.FphioR2 += pre * S.sp * iR2;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_NULLPOT>(const GhPotTable* __restrict__, int,
                                                           const GhGeom&, GhAcc&) {}
@@ -1089,7 +1167,7 @@ __host__ __device__ inline bool gh_is_heavy(int type) {
   return type == GH_TRIAXHERNQUIST || type == GH_TRIAXNFW || type == GH_TRIAXJAFFE ||
          type == GH_TWOPOWERTRIAX || type == GH_PERFECTELLIPSOID || type == GH_TRIAXGAUSSIAN ||
          type == GH_POWERTRIAX || type == GH_EINASTO || type == GH_TWOPOWERSPH ||
-         type == GH_EXPTRUNCNFW;
+         type == GH_EXPTRUNCNFW || type == GH_SPIRALARMS;
 }
 
 // run-time dispatch over every component type (extended instances)
@@ -1106,6 +1184,7 @@ __device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__
     case GH_EINASTO: gh_comp_force<GH_EINASTO>(T, i, g, F); break;
     case GH_EXPTRUNCNFW: gh_comp_force<GH_EXPTRUNCNFW>(T, i, g, F); break;
     case GH_TWOPOWERSPH: gh_comp_force<GH_TWOPOWERSPH>(T, i, g, F); break;
+    case GH_SPIRALARMS: gh_comp_force<GH_SPIRALARMS>(T, i, g, F); break;
     default: gh_comp_force_dyn(T, i, g, F); break;
   }
 }
@@ -1416,6 +1495,10 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
       case GH_STEADYSPIRAL:  // SteadyLogSpiralPotential.py _evaluate (planar only)
         pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
                cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
+        break;
+      case GH_SPIRALARMS:  // SpiralArmsPotential.c:21-58 (Eval)
+        pot += gh_spiral_pre(C, R) *
+               gh_spiral_sums<true>(T, C, R, z, atan2(sinphi, cosphi), t).pot;
         break;
       case GH_TRANSIENTSPIRAL:  // TransientLogSpiralPotential.py _evaluate (planar only)
         pot += gh_transient_amp(C, t) / C->p[4] * cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));

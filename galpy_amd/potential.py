@@ -641,6 +641,84 @@ class HenonHeilesPotential(Potential):
         return [self._amp]
 
 
+class SpiralArmsPotential(Potential):
+    """Cox & Gomez (2002) spiral arms, pot_type 27, args [nCs, amp, N,
+    sin alpha, tan alpha, r_ref, phi_ref, Rs, H, omega, C_1..C_nCs] in both
+    packings (SpiralArmsPotential.py:20-120, integrateFullOrbit.py:337-352,
+    integratePlanarOrbit.py:391-409):
+    Phi = -amp H exp(-(R - r_ref)/Rs) sum_n C_n / (K_n D_n) cos(n gamma)
+          / cosh(K_n z / B_n)^B_n,
+    gamma = N (phi - omega t - phi_ref - ln(R/r_ref) / tan alpha),
+    K_n = n N / (R sin alpha), B_n = K_n H (1 + 0.4 K_n H),
+    D_n = (1 + K_n H + 0.3 (K_n H)^2) / (1 + 0.3 K_n H).
+    As in galpy, N and alpha are stored with flipped signs (a left-handed
+    pattern), and amp is a density amplitude (rho_0 = 1/(4 pi))."""
+
+    pot_code = 27
+    isNonAxi = True
+
+    def __init__(self, amp=1.0, N=2, alpha=0.2, r_ref=1.0, phi_ref=0.0, Rs=0.3, H=0.125, omega=0.0, Cs=(1.0,)):
+        Potential.__init__(self, amp=amp)
+        self._N = -N
+        self._alpha = -alpha
+        self._sin_alpha = numpy.sin(-alpha)
+        self._tan_alpha = numpy.tan(-alpha)
+        self._r_ref = r_ref
+        self._phi_ref = phi_ref
+        self._Rs = Rs
+        self._H = H
+        self._Cs = numpy.array(Cs, dtype=float)
+        self._omega = omega
+
+    def _terms(self, R, z, phi, t):
+        R = numpy.asarray(R, dtype=float)[..., None]
+        z = numpy.asarray(z, dtype=float)[..., None]
+        phi = numpy.asarray(phi, dtype=float)[..., None]
+        t = numpy.asarray(t, dtype=float)[..., None]
+        n = numpy.arange(1, len(self._Cs) + 1, dtype=float)
+        HNn = self._H * self._N * n
+        K = n * self._N / R / self._sin_alpha
+        B = HNn / R / self._sin_alpha * (0.4 * HNn / R / self._sin_alpha + 1.0)
+        D = (0.3 * HNn**2 / self._sin_alpha / R + HNn + R * self._sin_alpha) / (0.3 * HNn + R * self._sin_alpha)
+        g = self._N * (phi - self._omega * t - self._phi_ref - numpy.log(R / self._r_ref) / self._tan_alpha)
+        pre = -self._H * numpy.exp(-(R[..., 0] - self._r_ref) / self._Rs)
+        return R, z, n, K, B, D, g, pre
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        R, z, n, K, B, D, g, pre = self._terms(R, z, phi, t)
+        return pre * numpy.sum(self._Cs / K / D * numpy.cos(n * g) / numpy.cosh(K * z / B) ** B, axis=-1)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        R, z, n, K, B, D, g, pre = self._terms(R, z, phi, t)
+        u = K * z / B
+        return pre * numpy.sum(self._Cs / D * numpy.cos(n * g) * numpy.tanh(u) / numpy.cosh(u) ** B, axis=-1)
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        R, z, n, K, B, D, g, pre = self._terms(R, z, phi, t)
+        return pre * numpy.sum(self._N * n * self._Cs / D / K / numpy.cosh(K * z / B) ** B * numpy.sin(n * g),
+                               axis=-1)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        R, z, n, K, B, D, g, pre = self._terms(R, z, phi, t)
+        HNn = self._H * self._N * n
+        k = HNn / R / self._sin_alpha
+        dg = -self._N / R / self._tan_alpha
+        dK = -n * self._N / R**2 / self._sin_alpha
+        dB = -HNn / R**3 / self._sin_alpha**2 * (0.8 * HNn + R * self._sin_alpha)
+        dD = k * (0.3 * (k + 0.3 * k**2 + 1.0) / R / (0.3 * k + 1.0) ** 2 - (1.0 + 0.6 * k) / R / (0.3 * k + 1.0))
+        u = z * K / B
+        sech = 1.0 / numpy.cosh(u)
+        cg, sg = numpy.cos(n * g), numpy.sin(n * g)
+        terms = self._Cs * sech**B / D * ((n * dg / K * sg + cg * (z * numpy.tanh(u) * (dK / K - dB / B)
+                                                                 - dB / K * numpy.log(sech) + dK / K**2
+                                                                 + dD / D / K)) + cg / K / self._Rs)
+        return pre * numpy.sum(terms, axis=-1)
+
+    def c_args(self):
+        return [float(len(self._Cs)), self._amp, self._N, self._sin_alpha, self._tan_alpha, self._r_ref,
+                self._phi_ref, self._Rs, self._H, self._omega] + list(self._Cs)
+
+
 class NFWPotential(Potential):
     """pot_type 9, args [amp, a]."""
 

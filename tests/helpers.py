@@ -60,6 +60,8 @@ def _nargs(code, pot_args, ai, planar):
     if code in _ELLIPSOIDAL:
         npsi = int(pot_args[ai + 16])
         return 30 + npsi + 2 * int(pot_args[ai + 29 + npsi])
+    if code == 27:  # SpiralArmsPotential: 10 + nCs
+        return 10 + int(pot_args[ai])
     return 3 if (planar and code in (15, 12)) else _NARGS[code]
 
 
