@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 
 #include "gh_internal.h"
 #include "dxdv_dev.h"
@@ -387,11 +388,46 @@ int decode_spiralarms(const double*& pot_args, GhPotTable* T, int i) {
   return 0;
 }
 
+// DoubleExponentialDiskPotential (integrateFullOrbit.py:93-107, the same in
+// the planar packing): amp, -4 pi alpha amp, alpha, beta, de_n, then the J0
+// nodes, J1 nodes, J0 weights and J1 weights, stored in ext as J0 (node,
+// weight) pairs followed by J1 pairs.
+int decode_dedisk(const double*& pot_args, GhPotTable* T, int i, bool planar) {
+  GhComponent& C = T->comp[i];
+  const double* a = pot_args;
+  const int n = (int)a[4];
+  if (n < 1 || T->next + 4 * n > GH_EXT_LEN) {
+    set_error("DoubleExponentialDiskPotential with de_n=%d does not fit the table (%d doubles left)",
+              n, GH_EXT_LEN - T->next);
+    return -1;
+  }
+  C.type = GH_DEDISK;
+  for (int j = 0; j < 4; j++) C.p[j] = a[j];
+  C.c[0] = n;
+  C.c[1] = a[2] * a[2];
+  C.c[2] = a[3] * a[3];
+  C.c[3] = planar ? 1.0 : 0.0;
+  C.ext = T->next;
+  double* e = T->ext + T->next;
+  const double *j0x = a + 5, *j1x = a + 5 + n, *j0w = a + 5 + 2 * n, *j1w = a + 5 + 3 * n;
+  for (int k = 0; k < n; k++) {
+    e[2 * k] = j0x[k];
+    e[2 * k + 1] = j0w[k];
+    e[2 * n + 2 * k] = j1x[k];
+    e[2 * n + 2 * k + 1] = j1w[k];
+  }
+  T->next += 4 * n;
+  T->has_wrap = 1;  // extended dispatch instances
+  pot_args += 5 + 4 * n;
+  return 0;
+}
+
 // One (leaf) component's arguments and precomputed constants.
 int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
                      bool planar) {
   if (is_ellipsoidal(code)) return decode_ellipsoidal(code, pot_args, T, i);
   if (code == GH_SPIRALARMS) return decode_spiralarms(pot_args, T, i);
+  if (code == GH_DEDISK) return decode_dedisk(pot_args, T, i, planar);
   const int na = nargs_of(code, planar);
   if (na < 0 || code < 0) {
     set_error("pot_type %d (component %d) has no HIP kernel", code, i);
@@ -812,7 +848,8 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
               "3 symplec4, 4 symplec6, 5 dopr54, 6 dop853, 7 ias15)", odeint_type);
     return -1;
   }
-  GhPotTable table;
+  std::unique_ptr<GhPotTable> table_ptr(new GhPotTable);
+  GhPotTable& table = *table_ptr;
   if (decode_potential(npot, pot_type, pot_args, &table, planar) != 0) return -1;
   hipStream_t stream;
   GH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -835,7 +872,7 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
     const size_t per_orbit = (size_t)nt * N * 8 + N * 8 + 4 + (indiv_t ? (size_t)nt * 8 : 0);
     size_t budget = free_b / 2;
     long chunk = (long)std::min<size_t>((size_t)nobj, std::max<size_t>(1, budget / per_orbit));
-    if (hipMalloc(&d_table, sizeof(GhPotTable)) != hipSuccess ||
+    if (hipMalloc(&d_table, gh_table_alloc_bytes(table)) != hipSuccess ||
         hipMalloc((void**)&d_yo, (size_t)chunk * N * 8) != hipSuccess ||
         hipMalloc((void**)&d_t, (size_t)nt * 8 * (indiv_t ? chunk : 1)) != hipSuccess ||
         hipMalloc((void**)&d_res, (size_t)chunk * nt * N * 8) != hipSuccess ||
@@ -1035,10 +1072,11 @@ GH_EXPORT int integratePlanarOrbit_hip(int nobj, double* d_yo, int nt, const dou
 static int upload_potential(int npot, const int* pot_type, const double* pot_args, void** pot,
                             bool planar) {
   g_errbuf[0] = 0;
-  GhPotTable table;
+  std::unique_ptr<GhPotTable> table_ptr(new GhPotTable);
+  GhPotTable& table = *table_ptr;
   if (decode_potential(npot, pot_type, pot_args, &table, planar) != 0) return -1;
   void* d = nullptr;
-  GH_CHECK(hipMalloc(&d, sizeof(GhPotTable)));
+  GH_CHECK(hipMalloc(&d, gh_table_alloc_bytes(table)));
   GH_CHECK(hipMemcpy(d, &table, gh_table_bytes(table), hipMemcpyHostToDevice));
   GhPotHandle* h = new GhPotHandle;
   h->dev = (GhPotTable*)d;

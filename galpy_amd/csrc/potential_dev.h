@@ -28,9 +28,13 @@
 #define GH_TAB_MAXSEG 64
 #define GH_TAB_NC 11
 #define GH_TAB_SLOTS 2  // PowerSphericalPotentialwCutoff components with a table
-// variable-length per-component data (quadrature nodes, rotations): doubles
-// shared by all components of a table
-#define GH_EXT_LEN 8192
+// variable-length per-component data (quadrature nodes, rotations, series
+// and spline tables): doubles shared by all components of a table.  Host
+// tables live on the heap and device tables are allocated to the used
+// prefix (gh_table_bytes), so the bound costs nothing when unused; it fits
+// a DoubleExponentialDiskPotential at galpy's default de_n = 10000 (4 x
+// 10000 nodes and weights) with room to spare.
+#define GH_EXT_LEN 65536
 
 enum GhPotCode {
   GH_LOGHALO = 0,
@@ -58,6 +62,7 @@ enum GhPotCode {
   GH_COSMPHIDISK = 28,     // planar only (integratePlanarOrbit's code 28)
   GH_HENONHEILES = 29,     // planar only (integratePlanarOrbit's code 29)
   GH_SPIRALARMS = 27,
+  GH_DEDISK = 11,  // DoubleExponentialDiskPotential
   GH_SOFTNEEDLEBAR = 25,
   GH_NULLPOT = 40,
   // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
@@ -123,6 +128,10 @@ struct GhPotTable {
 // bytes of a table that carry data (the prefix a host-to-device copy needs)
 __host__ __device__ inline size_t gh_table_bytes(const GhPotTable& T) {
   return offsetof(GhPotTable, ext) + sizeof(double) * (size_t)T.next;
+}
+// device allocation for a table: its data, rounded up to 256 bytes
+__host__ __device__ inline size_t gh_table_alloc_bytes(const GhPotTable& T) {
+  return (gh_table_bytes(T) + 255) & ~(size_t)255;
 }
 
 // ---------------------------------------------------------------------------
@@ -1045,6 +1054,60 @@ __device__ __forceinline__ void gh_comp_force<GH_TWOPOWERSPH>(const GhPotTable* 
   F.Fz += gr * g.z;
 }
 
+// DoubleExponentialDiskPotential.c: the disk's Hankel integrals by Ogata's
+// double-exponential quadrature over the packed nodes x_k and weights w_k
+// (ext: J0 pairs (x_k, w_k), then J1 pairs), each sum stopped once a term is
+// at most 1e-15 of the running sum (galpy divides the term by the sum and
+// compares; here the comparison is multiplied out) or the nodes run out:
+//   F_R = A/R sum_J1 w u (alpha^2 + u^2)^-3/2 (beta e^-u|z| - u e^-beta|z|) / (beta^2 - u^2)
+//   F_z = +-A beta/R sum_J0 w u (alpha^2 + u^2)^-3/2 (e^-u|z| - e^-beta|z|) / (beta^2 - u^2)
+//   Phi = A/R sum_J0 w (alpha^2 + u^2)^-3/2 (beta e^-u|z| - u e^-beta|z|) / (beta^2 - u^2)
+// u = x_k / R, A = -4 pi alpha amp (galpy's second argument); planar
+// tables use galpy's in-plane form of F_R, w u (alpha^2 + u^2)^-3/2 /
+// (beta + u) (PlanarRforce, :63-86).
+// p: amp, A, alpha, beta; c[0] = de_n, c[1] = alpha^2, c[2] = beta^2,
+// c[3] = 1 for the planar packing.
+template <int KIND>  // 0: F_R (J1), 1: F_z (J0), 2: Phi (J0), 3: planar F_R (J1)
+__device__ __forceinline__ double gh_dedisk_sum(const GhPotTable* __restrict__ T,
+                                                const GhComponent* __restrict__ C, double iR,
+                                                double fz) {
+  const int n = (int)C->c[0];
+  const double* __restrict__ xw = &T->ext[C->ext + ((KIND == 0 || KIND == 3) ? 2 * n : 0)];
+  const double a2 = C->c[1], b2 = C->c[2], beta = C->p[3];
+  const double ebz = exp(-beta * fz);
+  double out = 0.0;
+  for (int k = 0; k < n; k++) {
+    const double u = xw[2 * k] * iR;
+    const double rs = gh_rsqrt(a2 + u * u);
+    const double w = xw[2 * k + 1] * rs * rs * rs;
+    double term;
+    if (KIND == 3) {
+      term = w * u / (beta + u);
+    } else {
+      const double eu = exp(-u * fz);
+      const double num = (KIND == 1) ? u * (eu - ebz) : beta * eu - u * ebz;
+      term = ((KIND == 0) ? w * u : w) * num / (b2 - u * u);
+    }
+    out += term;
+    if (!(fabs(term) > 1e-15 * fabs(out))) break;
+  }
+  return out;
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_DEDISK>(const GhPotTable* __restrict__ T, int i,
+                                                         const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double iR = gh_rsqrt(g.R2), A = C->p[1];
+  const double fz = fabs(g.z);
+  if (C->c[3] != 0.0) {  // planar packing
+    F.FRoR += A * gh_dedisk_sum<3>(T, C, iR, 0.0) * iR * iR;
+    return;
+  }
+  F.FRoR += A * gh_dedisk_sum<0>(T, C, iR, fz) * iR * iR;
+  const double sz = A * C->p[3] * gh_dedisk_sum<1>(T, C, iR, fz) * iR;
+  F.Fz += (g.z > 0.0) ? sz : -sz;
+}
+
 // SpiralArmsPotential.c (Cox & Gomez 2002 arms), all three forces in one
 // pass over the terms n = 1..nCs (the reference recomputes K, B, D in each
 // of Rforce :61-126, zforce :128-170 and phitorque :172-210).  Per term,
@@ -1167,7 +1230,7 @@ __host__ __device__ inline bool gh_is_heavy(int type) {
   return type == GH_TRIAXHERNQUIST || type == GH_TRIAXNFW || type == GH_TRIAXJAFFE ||
          type == GH_TWOPOWERTRIAX || type == GH_PERFECTELLIPSOID || type == GH_TRIAXGAUSSIAN ||
          type == GH_POWERTRIAX || type == GH_EINASTO || type == GH_TWOPOWERSPH ||
-         type == GH_EXPTRUNCNFW || type == GH_SPIRALARMS;
+         type == GH_EXPTRUNCNFW || type == GH_SPIRALARMS || type == GH_DEDISK;
 }
 
 // run-time dispatch over every component type (extended instances)
@@ -1185,6 +1248,7 @@ __device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__
     case GH_EXPTRUNCNFW: gh_comp_force<GH_EXPTRUNCNFW>(T, i, g, F); break;
     case GH_TWOPOWERSPH: gh_comp_force<GH_TWOPOWERSPH>(T, i, g, F); break;
     case GH_SPIRALARMS: gh_comp_force<GH_SPIRALARMS>(T, i, g, F); break;
+    case GH_DEDISK: gh_comp_force<GH_DEDISK>(T, i, g, F); break;
     default: gh_comp_force_dyn(T, i, g, F); break;
   }
 }
@@ -1495,6 +1559,9 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
       case GH_STEADYSPIRAL:  // SteadyLogSpiralPotential.py _evaluate (planar only)
         pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
                cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
+        break;
+      case GH_DEDISK:  // DoubleExponentialDiskPotential.c:6-34 (Eval)
+        pot += C->p[1] * gh_dedisk_sum<2>(T, C, 1.0 / R, fabs(z)) / R;
         break;
       case GH_SPIRALARMS:  // SpiralArmsPotential.c:21-58 (Eval)
         pot += gh_spiral_pre(C, R) *

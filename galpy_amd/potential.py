@@ -719,6 +719,74 @@ class SpiralArmsPotential(Potential):
                 self._phi_ref, self._Rs, self._H, self._omega] + list(self._Cs)
 
 
+def _de_psi(t):
+    return t * numpy.tanh(numpy.pi / 2.0 * numpy.sinh(t))
+
+
+def _de_psiprime(t):
+    return (numpy.sinh(numpy.pi * numpy.sinh(t)) + numpy.pi * t * numpy.cosh(t)) / (
+        numpy.cosh(numpy.pi * numpy.sinh(t)) + 1.0)
+
+
+class DoubleExponentialDiskPotential(Potential):
+    """rho = amp exp(-R/hr - |z|/hz); pot_type 11, args [amp, -4 pi alpha
+    amp, alpha, beta, de_n, J0 nodes, J1 nodes, J0 weights, J1 weights]
+    with alpha = 1/hr, beta = 1/hz (DoubleExponentialDiskPotential.py:20-140,
+    integrateFullOrbit.py:93-107).  The disk's Hankel integrals are summed by
+    Ogata's (2005) double-exponential formula for Bessel integrals: nodes
+    pi/h psi(h xi_k) at the Bessel zeros xi_k pi, psi(t) = t tanh(pi/2
+    sinh t), as galpy builds them (weights past the overflow of psi' are NaN
+    there too, and the sums stop well before them)."""
+
+    pot_code = 11
+
+    def __init__(self, amp=1.0, hr=1.0 / 3.0, hz=1.0 / 16.0, normalize=False, de_h=1e-3, de_n=10000):
+        Potential.__init__(self, amp=amp)
+        self._hr, self._hz = hr, hz
+        self._alpha = 1.0 / hr
+        self._beta = 1.0 / hz
+        self._de_h, self._de_n = de_h, de_n
+        with numpy.errstate(over="ignore", invalid="ignore"):
+            j0z = special.jn_zeros(0, de_n) / numpy.pi
+            j1z = special.jn_zeros(1, de_n) / numpy.pi
+            self._de_j0_xs = numpy.pi / de_h * _de_psi(de_h * j0z)
+            self._de_j0_weights = (2.0 / (numpy.pi * j0z * special.j1(numpy.pi * j0z) ** 2.0)
+                                   * special.j0(self._de_j0_xs) * _de_psiprime(de_h * j0z))
+            self._de_j1_xs = numpy.pi / de_h * _de_psi(de_h * j1z)
+            self._de_j1_weights = (2.0 / (numpy.pi * j1z * special.jv(2, numpy.pi * j1z) ** 2.0)
+                                   * special.j1(self._de_j1_xs) * _de_psiprime(de_h * j1z))
+        _set_normalize(self, normalize)
+
+    def _sum(self, R, z, xs, ws, fun):
+        shape = numpy.broadcast(numpy.asarray(R), numpy.asarray(z)).shape
+        R = numpy.broadcast_to(numpy.asarray(R, dtype=float), shape).reshape(-1, 1)
+        z = numpy.broadcast_to(numpy.asarray(z, dtype=float), shape).reshape(-1, 1)
+        with numpy.errstate(over="ignore", invalid="ignore"):
+            out = numpy.nansum(fun(xs[None, :] / R, numpy.fabs(z)) * ws[None, :], axis=1)
+        return out.reshape(shape) if shape else float(out[0])
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        a2, b = self._alpha**2.0, self._beta
+        f = lambda x, fz: (a2 + x * x) ** -1.5 * (b * numpy.exp(-x * fz) - x * numpy.exp(-b * fz)) / (b * b - x * x)
+        return -4.0 * numpy.pi * self._alpha * self._sum(R, z, self._de_j0_xs, self._de_j0_weights, f) / R
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        a2, b = self._alpha**2.0, self._beta
+        f = lambda x, fz: x * (a2 + x * x) ** -1.5 * (b * numpy.exp(-x * fz) - x * numpy.exp(-b * fz)) / (b * b - x * x)
+        return -4.0 * numpy.pi * self._alpha * self._sum(R, z, self._de_j1_xs, self._de_j1_weights, f) / R
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        a2, b = self._alpha**2.0, self._beta
+        f = lambda x, fz: x * (a2 + x * x) ** -1.5 * (numpy.exp(-x * fz) - numpy.exp(-b * fz)) / (b * b - x * x)
+        out = -4.0 * numpy.pi * self._alpha * b * self._sum(R, z, self._de_j0_xs, self._de_j0_weights, f) / R
+        return numpy.where(numpy.asarray(z) > 0.0, out, -out) if numpy.ndim(out) else (out if z > 0.0 else -out)
+
+    def c_args(self):
+        return ([self._amp, -4.0 * numpy.pi * self._alpha * self._amp, self._alpha, self._beta,
+                 float(len(self._de_j1_xs))] + list(self._de_j0_xs) + list(self._de_j1_xs)
+                + list(self._de_j0_weights) + list(self._de_j1_weights))
+
+
 class NFWPotential(Potential):
     """pot_type 9, args [amp, a]."""
 

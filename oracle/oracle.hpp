@@ -234,7 +234,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kDehnenSph = 34, kHomogSphere = 35, kFlattenedPower = 12, kNull = 40,
                kKKStaeckel = 16, kSoftNeedleBar = 25, kEllipDisk = 4, kLopsided = 6, kSteadySpiral = 3,  // 3, 4, 6: planar only
                kTransientSpiral = 2, kCosmphiDisk = 28, kHenonHeiles = 29,  // planar only
-               kSpiralArms = 27,
+               kSpiralArms = 27, kDoubleExpDisk = 11,
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5,
@@ -314,6 +314,10 @@ inline int component_nargs(int code, bool planar = false) {
 // Rs, H, omega, Cs[nCs] (integrateFullOrbit.py:337-352; nargs as
 // integrateFullOrbit.c:545).
 inline int spiralarms_nargs(const double* a) { return 10 + (int)a[0]; }
+// DoubleExponentialDiskPotential: amp, -4 pi alpha amp, alpha, beta, de_n,
+// then the J0 nodes, J1 nodes, J0 weights, J1 weights, de_n each
+// (integrateFullOrbit.py:93-107).
+inline int dedisk_nargs(const double* a) { return 5 + 4 * (int)a[4]; }
 
 inline int ellipsoidal_nargs(const double* a) {
   const int npsi = (int)a[16];
@@ -333,6 +337,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
     int code = *pot_type++;
     int na = is_ellipsoidal(code)    ? ellipsoidal_nargs(pot_args)
              : code == kSpiralArms ? spiralarms_nargs(pot_args)
+             : code == kDoubleExpDisk ? dedisk_nargs(pot_args)
                                    : component_nargs(code, planar);
     if (na < 0) return -1;
     Component& C = dst[i];
@@ -351,7 +356,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       C.wrapped = kids;
     }
     C.args = pot_args;
-    if (is_ellipsoidal(code) || code == kSpiralArms) {
+    if (is_ellipsoidal(code) || code == kSpiralArms || code == kDoubleExpDisk) {
       C.p[0] = pot_args[0];
       pot_args += na;
       continue;
@@ -1020,6 +1025,7 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kKKStaeckel: kks_forces(C, Rc, z, fr, fz); break;
       case kSoftNeedleBar: needle_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kSpiralArms: spiral_forces(C, Rc, z, phi, t, fr, fz, fp); break;
+      case kDoubleExpDisk: dedisk_forces(C, Rc, z, fr, fz); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Rforce, zforce
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.) {
@@ -1051,6 +1057,71 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
     }
     FR += fr; Fz += fz; Fphi += fp;
   }
+}
+
+// DoubleExponentialDiskPotential.c: Ogata's double-exponential quadrature of
+// the disk's Hankel integrals over packed J0/J1 nodes and weights, each sum
+// stopped once a term is below 1e-15 of the running sum or the nodes run
+// out: Eval (:6-34, J0), Rforce (:35-62, J1), PlanarRforce (:63-86, J1,
+// the z = 0 form 1/(beta + x)), zforce (:87-120, J0).  args[1] is
+// -4 pi alpha amp.
+template <class R> void dedisk_forces(const Component& C, R Rc, R z, R& FR, R& Fz) {
+  const double* a = C.args;
+  const double amp = a[1], alpha = a[2], beta = a[3];
+  const int n = (int)a[4];
+  const double *j0x = a + 5, *j1x = a + 5 + n, *j0w = a + 5 + 2 * n, *j1w = a + 5 + 3 * n;
+  R alpha2 = R(alpha) * R(alpha), beta2 = R(beta) * R(beta);
+  R fz = fabs(z);
+  R ebetafz = exp(-R(beta) * fz);
+  R out = 0., prev = 1.;
+  for (int ii = 0; fabs(prev) > R(1e-15) && ii < n; ii++) {
+    R x = R(j1x[ii]) / Rc;
+    prev = R(j1w[ii]) * x * pow(alpha2 + x * x, R(-1.5)) * (R(beta) * exp(-x * fz) - x * ebetafz) / (beta2 - x * x);
+    out += prev;
+    prev /= out;
+  }
+  FR = R(amp) * out / Rc;
+  out = 0.;
+  prev = 1.;
+  for (int ii = 0; fabs(prev) > R(1e-15) && ii < n; ii++) {
+    R x = R(j0x[ii]) / Rc;
+    prev = R(j0w[ii]) * pow(alpha2 + x * x, R(-1.5)) * x * (exp(-x * fz) - ebetafz) / (beta2 - x * x);
+    out += prev;
+    prev /= out;
+  }
+  Fz = (z > R(0.)) ? R(amp) * out * R(beta) / Rc : -R(amp) * out * R(beta) / Rc;
+}
+template <class R> R dedisk_planar_Rforce(const Component& C, R Rc) {
+  const double* a = C.args;
+  const double amp = a[1], alpha = a[2], beta = a[3];
+  const int n = (int)a[4];
+  const double *j1x = a + 5 + n, *j1w = a + 5 + 3 * n;
+  R alpha2 = R(alpha) * R(alpha);
+  R out = 0., prev = 1.;
+  for (int ii = 0; fabs(prev) > R(1e-15) && ii < n; ii++) {
+    R x = R(j1x[ii]) / Rc;
+    prev = R(j1w[ii]) * x * pow(alpha2 + x * x, R(-1.5)) / (R(beta) + x);
+    out += prev;
+    prev /= out;
+  }
+  return R(amp) * out / Rc;
+}
+template <class R> R dedisk_eval(const Component& C, R Rc, R z) {
+  const double* a = C.args;
+  const double amp = a[1], alpha = a[2], beta = a[3];
+  const int n = (int)a[4];
+  const double *j0x = a + 5, *j0w = a + 5 + 2 * n;
+  R alpha2 = R(alpha) * R(alpha), beta2 = R(beta) * R(beta);
+  R fz = fabs(z);
+  R ebetafz = exp(-R(beta) * fz);
+  R out = 0., prev = 1.;
+  for (int ii = 0; fabs(prev) > R(1e-15) && ii < n; ii++) {
+    R x = R(j0x[ii]) / Rc;
+    prev = R(j0w[ii]) * pow(alpha2 + x * x, R(-1.5)) * (R(beta) * exp(-x * fz) - x * ebetafz) / (beta2 - x * x);
+    out += prev;
+    prev /= out;
+  }
+  return R(amp) * out / Rc;
 }
 
 // SpiralArmsPotential.c (Cox & Gomez 2002 arms): the helpers gam, dgam_dR,
@@ -1523,6 +1594,7 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
         needle_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
         break;
       }
+      case kDoubleExpDisk: fr = dedisk_planar_Rforce(C, Rc); break;
       case kSpiralArms: {  // SpiralArmsPotential.c:625-716: the full forces at z = 0
         R fz0;
         spiral_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
@@ -1581,7 +1653,7 @@ inline bool has_hessian(int npot, const Component* P) {
     const int c = P[i].type;
     if (c == kKKStaeckel || c == kSoftNeedleBar || is_wrapper(c) || c == kEllipDisk || c == kLopsided ||
         c == kSteadySpiral || c == kTransientSpiral || c == kCosmphiDisk || c == kHenonHeiles ||
-        c == kSpiralArms || is_ellipsoidal(c))
+        c == kSpiralArms || c == kDoubleExpDisk || is_ellipsoidal(c))
       return false;
   }
   return true;
@@ -1672,6 +1744,7 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kKKStaeckel: pot += kks_eval(C, Rc, z); break;
       case kSoftNeedleBar: pot += needle_eval(C, Rc, z, phi, t); break;
       case kSpiralArms: pot += spiral_eval(C, Rc, z, phi, t); break;
+      case kDoubleExpDisk: pot += dedisk_eval(C, Rc, z); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Eval
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.)

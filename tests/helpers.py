@@ -62,6 +62,8 @@ def _nargs(code, pot_args, ai, planar):
         return 30 + npsi + 2 * int(pot_args[ai + 29 + npsi])
     if code == 27:  # SpiralArmsPotential: 10 + nCs
         return 10 + int(pot_args[ai])
+    if code == 11:  # DoubleExponentialDiskPotential: 5 + 4 de_n
+        return 5 + 4 * int(pot_args[ai + 4])
     return 3 if (planar and code in (15, 12)) else _NARGS[code]
 
 
