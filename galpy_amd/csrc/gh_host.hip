@@ -106,6 +106,11 @@ int nargs_of(int code, bool planar) {
     case GH_SOLIDBODYROTWRAP: return 3;   // amp, omega, pa
     case GH_COROTATINGROTWRAP: return 5;  // amp, vpo, beta, pa, to
     case GH_GAUSSIANAMPWRAP: return 3;    // amp, to, sigma^2
+    // amp, 7 cache slots, rot[9], rotSet, offsetSet, offset[3], 10 cache
+    // slots (integrateFullOrbit.py:546-568; no planar version in galpy)
+    case GH_ROTATETILTWRAP: return planar ? -1 : 32;
+    case GH_KUZMINLIKEWRAP: return 3;  // amp, a, b^2
+    case GH_CYLSEPWRAP: return 3;      // amp, Rp, Phi(Rp, 0)
     default: return -1;
   }
 }
@@ -628,7 +633,14 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
     const int w = T->nwraps++;
     GhWrap& W = T->wrap[w];
     W.kind = code;
-    for (int j = 0; j < na; j++) W.p[j] = *pot_args++;
+    if (code == GH_ROTATETILTWRAP) {  // amp, rot, rotSet, offsetSet, offset (caches skipped)
+      const double* a = pot_args;
+      W.p[0] = a[0];
+      for (int j = 0; j < 14; j++) W.p[1 + j] = a[8 + j];
+      pot_args += na;
+    } else {
+      for (int j = 0; j < na; j++) W.p[j] = *pot_args++;
+    }
     if (code == GH_GAUSSIANAMPWRAP && !(W.p[2] > 0.0)) {
       set_error("GaussianAmplitudeWrapperPotential sigma^2=%g must be positive", W.p[2]);
       return -1;
@@ -656,6 +668,23 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
   const double* pa = pot_args;
   if (decode_components(npot, pt, pa, T, nleaf, nslots, 0, planar) != 0) return -1;
   T->npot = nleaf;
+  // leaves whose wrappers move the evaluation point take the general walk
+  // (two passes under a CylindricallySeparable wrapper)
+  for (int l = 0; l < nleaf; l++) {
+    GhComponent& C = T->comp[l];
+    int moved = 0, cylsep = 0;
+    for (int k = 0; k < C.nwrap; k++) {
+      const int kind = T->wrap[C.wrap[k]].kind;
+      moved |= kind == GH_ROTATETILTWRAP || kind == GH_KUZMINLIKEWRAP || kind == GH_CYLSEPWRAP;
+      cylsep += kind == GH_CYLSEPWRAP;
+    }
+    if (cylsep > 1) {
+      set_error("a component under more than one CylindricallySeparablePotentialWrapper");
+      return -1;
+    }
+    C.mapwrap = moved ? 1 + cylsep : 0;
+    if (moved) T->nonaxi = 1;
+  }
   return 0;
 }
 // ---------------------------------------------------------------------------

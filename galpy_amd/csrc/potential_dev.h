@@ -82,6 +82,9 @@ enum GhPotCode {
   GH_SOLIDBODYROTWRAP = -2,
   GH_COROTATINGROTWRAP = -4,
   GH_GAUSSIANAMPWRAP = -5,
+  GH_ROTATETILTWRAP = -8,
+  GH_KUZMINLIKEWRAP = -10,
+  GH_CYLSEPWRAP = -12,
 };
 
 struct GhComponent {
@@ -90,6 +93,7 @@ struct GhComponent {
   int nwrap;     // wrappers around this component
   int wrap[GH_WRAP_DEPTH];  // their indices in GhPotTable::wrap, outermost first
   int ext;       // offset of the component's variable-length data in GhPotTable::ext
+  int mapwrap;   // wrappers that move the evaluation point: 0 none, else passes (1 or 2)
   double p[8];   // raw galpy arguments, in galpy's order
   double c[14];  // precomputed constants, per type (decoded on the host)
 };
@@ -102,7 +106,7 @@ struct GhComponent {
 struct GhWrap {
   int kind;
   int pad;
-  double p[6];
+  double p[16];  // RotateAndTilt: amp, rot[9], rotSet, offsetSet, offset[3]
 };
 
 struct GhPotTable {
@@ -1301,6 +1305,203 @@ __device__ __forceinline__ void gh_wrap_transform(const GhPotTable* __restrict__
   }
 }
 
+// Wrappers that move the point where the wrapped components are evaluated,
+// walked from the outermost in (gh_wrap_forward: the point, time and
+// amplitude factor handed inwards) and back out (gh_wrap_backward: the
+// Cartesian force handed outwards); galpy recurses through the same maps:
+//   SolidBodyRotation / CorotatingRotation: the point turned by -offset
+//     about z; the force turned back, the corotating wrapper adding
+//     -F_phi d offset / dR along R (CorotatingRotationWrapperPotential.c:16-40);
+//   RotateAndTilt (RotateAndTiltWrapperPotential.c:21-62): x' = rot x +
+//     offset; F = rot^T F';
+//   KuzminLike (KuzminLikeWrapperPotential.c:5-92): the point (xi, 0, 0),
+//     xi = sqrt(R^2 + (a + sqrt(z^2 + b^2))^2); its radial force F' gives
+//     F_R = F' R / xi and F_z = F' (a + s) z / (xi s), s = sqrt(z^2 + b^2);
+//   CylindricallySeparable (CylindricallySeparablePotentialWrapper.c:14-27):
+//     two passes at t = 0, the radial force at (R, 0, 0) and the vertical
+//     force at (Rp, 0, z); every map is linear in the force, so the passes'
+//     forces add;
+//   DehnenSmooth, GaussianAmplitude: factors only.
+// rec (when not null) keeps what the way back needs: {cos, sin, d offset/dR,
+// x, y} for the rotations, {x, y, (a + s) z / s, xi} for KuzminLike, {x, y,
+// R} for CylindricallySeparable.
+__device__ __forceinline__ void gh_wrap_forward(const GhWrap* __restrict__ W, int pass,
+                                                double& x, double& y, double& z, double& t,
+                                                double& fac, double* rec) {
+  fac *= W->p[0];
+  switch (W->kind) {
+    case GH_DEHNENSMOOTHWRAP: {
+      const double sm = gh_bar_smooth(t, W->p[1], W->p[2]);
+      fac *= (W->p[3] != 0.0) ? sm : 1.0 - sm;
+      break;
+    }
+    case GH_GAUSSIANAMPWRAP: {
+      const double d = t - W->p[1];
+      fac *= exp(-0.5 * d * d / W->p[2]);
+      break;
+    }
+    case GH_SOLIDBODYROTWRAP:
+    case GH_COROTATINGROTWRAP: {
+      double off, doff = 0.0;
+      if (W->kind == GH_SOLIDBODYROTWRAP) {
+        off = W->p[1] * t + W->p[2];
+      } else {
+        const double R = sqrt(x * x + y * y), dt = t - W->p[4];
+        const double Rb = pow(R, W->p[2] - 1.0);
+        off = W->p[1] * Rb * dt + W->p[3];
+        doff = W->p[1] * (W->p[2] - 1.0) * (Rb / R) * dt;
+      }
+      const GhSinCos sc = gh_sincos(off);
+      if (rec) {
+        rec[0] = sc.c;
+        rec[1] = sc.s;
+        rec[2] = doff;
+        rec[3] = x;
+        rec[4] = y;
+      }
+      const double xn = x * sc.c + y * sc.s;
+      y = y * sc.c - x * sc.s;
+      x = xn;
+      break;
+    }
+    case GH_ROTATETILTWRAP: {
+      if (W->p[10] != 0.0) {
+        const double* __restrict__ m = &W->p[1];
+        const double xn = m[0] * x + m[1] * y + m[2] * z;
+        const double yn = m[3] * x + m[4] * y + m[5] * z;
+        z = m[6] * x + m[7] * y + m[8] * z;
+        x = xn;
+        y = yn;
+      }
+      if (W->p[11] != 0.0) {
+        x += W->p[12];
+        y += W->p[13];
+        z += W->p[14];
+      }
+      break;
+    }
+    case GH_KUZMINLIKEWRAP: {
+      const double sz = sqrt(z * z + W->p[2]), as = W->p[1] + sz;
+      const double xi = sqrt(x * x + y * y + as * as);
+      if (rec) {
+        rec[0] = x;
+        rec[1] = y;
+        rec[2] = as * z / sz;
+        rec[3] = xi;
+      }
+      x = xi;
+      y = 0.0;
+      z = 0.0;
+      break;
+    }
+    case GH_CYLSEPWRAP: {
+      const double R = sqrt(x * x + y * y);
+      if (rec) {
+        rec[0] = x;
+        rec[1] = y;
+        rec[2] = R;
+      }
+      x = (pass == 0) ? R : W->p[1];
+      y = 0.0;
+      if (pass == 0) z = 0.0;
+      t = 0.0;
+      break;
+    }
+    default: break;
+  }
+}
+__device__ __forceinline__ void gh_wrap_backward(const GhWrap* __restrict__ W, int pass,
+                                                 const double* rec, double& fx, double& fy,
+                                                 double& fz) {
+  switch (W->kind) {
+    case GH_SOLIDBODYROTWRAP:
+    case GH_COROTATINGROTWRAP: {
+      const double c = rec[0], s = rec[1];
+      const double gx = c * fx - s * fy;
+      fy = s * fx + c * fy;
+      fx = gx;
+      if (W->kind == GH_COROTATINGROTWRAP) {
+        const double xk = rec[3], yk = rec[4];
+        const double corr = -(xk * fy - yk * fx) * rec[2] * gh_rsqrt(xk * xk + yk * yk);
+        fx += corr * xk;
+        fy += corr * yk;
+      }
+      break;
+    }
+    case GH_ROTATETILTWRAP: {
+      if (W->p[10] != 0.0) {
+        const double* __restrict__ m = &W->p[1];
+        const double gx = m[0] * fx + m[3] * fy + m[6] * fz;
+        const double gy = m[1] * fx + m[4] * fy + m[7] * fz;
+        fz = m[2] * fx + m[5] * fy + m[8] * fz;
+        fx = gx;
+        fy = gy;
+      }
+      break;
+    }
+    case GH_KUZMINLIKEWRAP: {  // the inner point is (xi, 0, 0): its radial force is fx
+      const double f = fx / rec[3];
+      fx = f * rec[0];
+      fy = f * rec[1];
+      fz = f * rec[2];
+      break;
+    }
+    case GH_CYLSEPWRAP: {
+      if (pass == 0) {  // radial force at (R, 0, 0)
+        const double f = fx / rec[2];
+        fx = f * rec[0];
+        fy = f * rec[1];
+        fz = 0.0;
+      } else {  // vertical force at (Rp, 0, z)
+        fx = 0.0;
+        fy = 0.0;
+      }
+      break;
+    }
+    default: break;
+  }
+}
+
+// Force of a component whose wrappers move the evaluation point: the leaf's
+// force at the moved point, mapped back out through the wrappers (one pass,
+// two with a CylindricallySeparable wrapper).
+__device__ inline void gh_comp_force_mapped(const GhPotTable* __restrict__ T, int i,
+                                            const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const int nw = C->nwrap;
+  double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+  for (int pass = 0; pass < C->mapwrap; pass++) {
+    double x = g.x, y = g.y, z = g.z, t = g.t, fac = 1.0;
+    double rec[GH_WRAP_DEPTH][5];
+#pragma unroll
+    for (int k = 0; k < GH_WRAP_DEPTH; k++)
+      if (k < nw) gh_wrap_forward(&T->wrap[C->wrap[k]], pass, x, y, z, t, fac, rec[k]);
+    GhGeom gl;
+    gl.x = x;
+    gl.y = y;
+    gl.z = z;
+    gl.R2 = x * x + y * y;
+    gl.z2 = z * z;
+    gl.r2 = gl.R2 + gl.z2;
+    gl.rinv = gh_rsqrt(gl.r2);
+    gl.r = gl.r2 * gl.rinv;
+    gl.t = t;
+    GhAcc L = {0.0, 0.0, 0.0};
+    gh_comp_force_ext(T, i, gl, L);
+    double fx = x * L.FRoR - y * L.FphioR2, fy = y * L.FRoR + x * L.FphioR2, fz = L.Fz;
+#pragma unroll
+    for (int k = GH_WRAP_DEPTH - 1; k >= 0; k--)
+      if (k < nw) gh_wrap_backward(&T->wrap[C->wrap[k]], pass, rec[k], fx, fy, fz);
+    Fx += fac * fx;
+    Fy += fac * fy;
+    Fz += fac * fz;
+  }
+  const double iR2 = gh_rcp(g.R2);
+  F.FRoR += (g.x * Fx + g.y * Fy) * iR2;
+  F.FphioR2 += (g.x * Fy - g.y * Fx) * iR2;
+  F.Fz += Fz;
+}
+
 // Force of component i with the wrappers around it: the component's own
 // force at the position rotated by -offset (its F_R, F_z and F_phi are the
 // same cylindrical components in either frame), scaled by the wrappers'
@@ -1311,6 +1512,10 @@ __device__ __forceinline__ void gh_wrap_transform(const GhPotTable* __restrict__
 __device__ __forceinline__ void gh_comp_force_any(const GhPotTable* __restrict__ T, int i,
                                                   const GhGeom& g, GhAcc& F) {
   const GhComponent* __restrict__ C = &T->comp[i];
+  if (C->mapwrap) {
+    gh_comp_force_mapped(T, i, g, F);
+    return;
+  }
   GhGeom gw = g;
   double fac = 1.0, corr = 0.0;
   const bool wrapped = C->nwrap != 0;
@@ -1486,19 +1691,234 @@ __device__ inline double gh_ellip_potential(const GhPotTable* __restrict__ T,
   return -0.5 * C->p[0] * out;
 }
 
-// Potential value (per-component Eval functions; DehnenBar from
+// Potential value of component i alone at (R, z, phi, t), cosphi/sinphi =
+// cos/sin phi (per-component Eval functions; DehnenBar from
 // DehnenBarPotential.py:173-200, which has no C Eval).
+__device__ __forceinline__ double gh_comp_potential(const GhPotTable* __restrict__ T, int i,
+                                                    double R, double z, double cosphi,
+                                                    double sinphi, double t) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double R2 = R * R;
+  const double z2 = z * z;
+  const double r2 = R2 + z2;
+  double pot = 0.0;
+  switch (C->type) {
+    case GH_POWERSPHCUTOFF: {
+      // PowerSphericalPotentialwCutoff.c:12-24
+      const double r = sqrt(r2);
+      const double x = r2 * C->c[3];
+      const double g2 = gh_lower_incgamma(C->c[5], x, C->c[6], C->c[7], C->c[10], T->ser[i][1]);
+      const double g1 = gh_lower_incgamma(C->c[0], x, C->c[1], C->c[4], C->c[9], T->ser[i][0]);
+      pot += C->c[2] / r * (r * C->c[8] * g2 - g1);
+      break;
+    }
+    case GH_MIYAMOTONAGAI: {
+      const double as = C->p[1] + sqrt(z2 + C->c[0]);
+      pot += -C->p[0] / sqrt(R2 + as * as);
+      break;
+    }
+    case GH_NFW: {
+      const double r = sqrt(r2);
+      pot += -C->p[0] * gh_log_ge1(1.0 + r * C->c[0]) / r;
+      break;
+    }
+    case GH_HERNQUIST: {
+      const double r = sqrt(r2);
+      pot += -C->p[0] / (1.0 + r / C->p[1]) / 2.0 / C->p[1];
+      break;
+    }
+    case GH_LOGHALO: {
+      const double amp = C->p[0], core2 = C->p[2], onem1overb2 = C->p[3];
+      const double zq2 = z2 * C->c[0];
+      const double Rt2 = (onem1overb2 < 1.0) ? R2 * (1.0 - onem1overb2 * sinphi * sinphi) : R2;
+      pot += 0.5 * amp * log(Rt2 + zq2 + core2);
+      break;
+    }
+    case GH_FLATPOWER: {  // FlattenedPowerPotential.c:3-18
+      const double alpha = C->p[1], m2 = C->p[3] + R2 + z2 * C->c[0];
+      pot += (alpha == 0.0) ? 0.5 * C->p[0] * log(m2)
+                            : -C->p[0] * pow(m2, -0.5 * alpha) / alpha;
+      break;
+    }
+    case GH_SOFTNEEDLEBAR: {  // SoftenedNeedleBarPotential.c:13-29
+      double cp, sp, xb, yb;
+      gh_needle_frame(C, R * cosphi, R * sinphi, t, cp, sp, xb, yb);
+      const double a = C->p[1], bz = C->p[2] + sqrt(z2 + C->p[3]);
+      const double second = yb * yb + bz * bz;
+      const double Tp = sqrt((a + xb) * (a + xb) + second), Tm = sqrt((a - xb) * (a - xb) + second);
+      pot += 0.5 * C->p[0] * log((xb - a + Tm) / (xb + a + Tp)) / a;
+      break;
+    }
+    case GH_STEADYSPIRAL:  // SteadyLogSpiralPotential.py _evaluate (planar only)
+      pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
+             cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
+      break;
+    case GH_DEDISK:  // DoubleExponentialDiskPotential.c:6-34 (Eval)
+      pot += C->p[1] * gh_dedisk_sum<2>(T, C, 1.0 / R, fabs(z)) / R;
+      break;
+    case GH_SPIRALARMS:  // SpiralArmsPotential.c:21-58 (Eval)
+      pot += gh_spiral_pre(C, R) *
+             gh_spiral_sums<true>(T, C, R, z, atan2(sinphi, cosphi), t).pot;
+      break;
+    case GH_TRANSIENTSPIRAL:  // TransientLogSpiralPotential.py _evaluate (planar only)
+      pot += gh_transient_amp(C, t) / C->p[4] * cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
+      break;
+    case GH_COSMPHIDISK: {  // CosmphiDiskPotential.py _evaluate (planar only)
+      const double cm = cos(C->p[4] * atan2(sinphi, cosphi) - C->p[3]);
+      const double Rp = pow(R, C->p[2]);
+      pot += C->p[0] * C->c[2] * cm * ((R < C->p[5]) ? C->p[6] * (2.0 * C->c[0] - C->p[6] / Rp) : Rp);
+      break;
+    }
+    case GH_HENONHEILES:  // HenonHeilesPotential.py _evaluate (planar only)
+      pot += C->p[0] * 0.5 * R2 * (1.0 + 2.0 / 3.0 * R * (3.0 * cosphi * cosphi - sinphi * sinphi) * sinphi);
+      break;
+    case GH_LOPSIDED:  // CosmphiDiskPotential.py _evaluate at m = 1 (planar only)
+      pot += C->p[0] * C->p[1] * pow(R2, 0.5 * C->p[2]) * (cosphi * C->c[0] + sinphi * C->c[1]);
+      break;
+    case GH_ELLIPDISK: {  // EllipticalDiskPotential.py _evaluate (planar only)
+      const double c2 = cosphi * cosphi - sinphi * sinphi, s2 = 2.0 * sinphi * cosphi;
+      pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] * 0.5 *
+             pow(R2, 0.5 * C->p[4]) * (c2 * C->c[0] + s2 * C->c[1]);
+      break;
+    }
+    case GH_KKSTAECKEL: {  // KuzminKutuzovStaeckelPotential.c:5-23 (n clamped at 0)
+      const double dm = r2 - C->c[2];
+      const double s = sqrt(dm * dm + 4.0 * C->c[2] * R2), term = r2 - C->c[1] - C->c[0];
+      const double n = 0.5 * (term - s);
+      pot += -C->p[0] / (sqrt(0.5 * (term + s)) + sqrt(n > 0.0 ? n : 0.0));
+      break;
+    }
+    case GH_KUZMINDISK: {  // KuzminDiskPotential.c:5-15
+      const double az = C->p[1] + fabs(z);
+      pot += -C->p[0] / sqrt(R2 + az * az);
+      break;
+    }
+    case GH_POWERSPH: {  // PowerSphericalPotential.c Eval
+      const double alpha = C->p[1];
+      pot += (alpha == 2.0) ? 0.5 * C->p[0] * log(r2)
+                            : -C->p[0] * pow(r2, 1.0 - 0.5 * alpha) / (alpha - 2.0);
+      break;
+    }
+    case GH_JAFFE: {
+      const double r = sqrt(r2);
+      pot += -C->p[0] * log(1.0 + C->p[1] / r) / C->p[1];
+      break;
+    }
+    case GH_ISOCHRONE: pot += -C->p[0] / (C->p[1] + sqrt(r2 + C->c[0])); break;
+    case GH_PLUMMER: pot += -C->p[0] / sqrt(r2 + C->c[0]); break;
+    case GH_PSEUDOISO: {
+      const double a = C->p[1], r = sqrt(r2);
+      pot += C->p[0] * (0.5 * log(1.0 + r2 / (a * a)) + a / r * atan(r / a)) / a;
+      break;
+    }
+    case GH_BURKERT: {
+      const double a = C->p[1], x = sqrt(r2) / a;
+      pot += -C->p[0] * a * a * M_PI / x *
+             (-M_PI + 2.0 * (1.0 + x) * atan(1.0 / x) + 2.0 * (1.0 + x) * log1p(x) +
+              (1.0 - x) * log1p(x * x));
+      break;
+    }
+    case GH_DEHNENCORESPH: {
+      const double a = C->p[1], r = sqrt(r2), u = r / (r + a);
+      pot += -C->p[0] * (1.0 - u * u) / (6.0 * a);
+      break;
+    }
+    case GH_DEHNENSPH: {
+      const double a = C->p[1], alpha = C->p[2], r = sqrt(r2);
+      if (alpha == 2.0)  // the Jaffe limit of the closed form (0/0 at alpha = 2)
+        pot += -C->p[0] * log1p(a / r) / a;
+      else
+        pot += -C->p[0] * (1.0 - pow(r / (r + a), 2.0 - alpha)) / (a * (2.0 - alpha) * (3.0 - alpha));
+      break;
+    }
+    case GH_HOMOGSPHERE: {
+      const double R2s = C->p[1], R3s = C->p[2];
+      pot += (r2 < R2s) ? C->p[0] * (r2 - 3.0 * R2s) : -2.0 * C->p[0] * R3s / sqrt(r2);
+      break;
+    }
+    case GH_DEHNENBAR: {
+      const double amp = C->p[0], rb = C->p[3];
+      const double smooth = gh_bar_smooth(t, C->p[1], C->p[2]);
+      const double cos2p = cosphi * cosphi - sinphi * sinphi;
+      const double sin2p = 2.0 * sinphi * cosphi;
+      double cang, sang;
+      gh_bar_angle(cos2p, sin2p, C->p[4], C->p[5], t, cang, sang);
+      const double r = sqrt(r2);
+      double out;
+      if (r <= rb) {
+        const double u = r / rb;
+        out = (u * u * u - 2.0) * R2 / r2;
+      } else {
+        const double u = rb / r;
+        out = -u * u * u * R2 / r2;
+      }
+      pot += out * amp * smooth * cang;
+      break;
+    }
+    case GH_TRIAXHERNQUIST:
+    case GH_TRIAXNFW:
+    case GH_TRIAXJAFFE:
+    case GH_TWOPOWERTRIAX:
+    case GH_PERFECTELLIPSOID:
+    case GH_TRIAXGAUSSIAN:
+    case GH_POWERTRIAX: pot += gh_ellip_potential(T, C, R * cosphi, R * sinphi, z); break;
+    case GH_EINASTO: {  // EinastoPotential.c revaluate: -amp 4 pi h^2 n (gamma(3n,x)/s + Gamma(2n,x))
+      const double r = sqrt(r2), x = gh_einasto_x(C, r);
+      const double g3 = gh_lower_incgamma_gen(C->c[0], x, C->c[1], C->c[2]);
+      const double G2 = C->c[4] - gh_lower_incgamma_gen(C->c[3], x, C->c[4], C->c[5]);
+      const double h = C->p[1];
+      pot += -C->p[0] * C->c[6] / h * (g3 / (r * C->c[7]) + G2);
+      break;
+    }
+    case GH_EXPTRUNCNFW: {  // ExpTruncNFWPotential.c revaluate: -amp (F/r + G)
+      const double r = sqrt(r2);
+      double ex, e1b;
+      const double Fm = gh_exptruncnfw_F(C, r, ex, e1b);
+      if (r < C->c[3]) e1b = gh_e1_scaled((C->p[1] + r) * C->c[2]);
+      pot += -C->p[0] * (Fm / r + ex / (C->p[1] + r) - ex * e1b * C->c[2]);
+      break;
+    }
+    case GH_TWOPOWERSPH: pot += NAN; break;  // its Eval needs 2F1 outside the family above
+    default:
+      break;
+  }
+  return pot;
+}
+
+// Potential of a component whose wrappers move the evaluation point (the
+// same walk as gh_comp_force_mapped): the leaf's value at the moved point(s)
+// times the wrappers' factors; CylindricallySeparable adds its -amp Phi(Rp, 0).
+__device__ inline double gh_mapped_potential(const GhPotTable* __restrict__ T, int i, double x0,
+                                             double y0, double z0, double t0) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  double total = 0.0;
+  for (int pass = 0; pass < C->mapwrap; pass++) {
+    double x = x0, y = y0, z = z0, t = t0, fac = 1.0;
+    for (int k = 0; k < C->nwrap; k++) {
+      const GhWrap* __restrict__ W = &T->wrap[C->wrap[k]];
+      gh_wrap_forward(W, pass, x, y, z, t, fac, nullptr);
+      if (W->kind == GH_CYLSEPWRAP && pass == 1) total -= fac * W->p[2];
+    }
+    const double R = sqrt(x * x + y * y);
+    const double cp = (R > 0.0) ? x / R : 1.0, sp = (R > 0.0) ? y / R : 0.0;
+    total += fac * gh_comp_potential(T, i, R, z, cp, sp, t);
+  }
+  return total;
+}
+
+// Potential value: the sum over components; wrapped components follow the
+// wrappers' Python _evaluate (the component at phi - offset times the factor,
+// or at the point the wrappers move it to).
 __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T, double R,
                                                double z, double cosphi0, double sinphi0,
                                                double t) {
   double total = 0.0;
-  const double R2 = R * R;
-  const double z2 = z * z;
-  const double r2 = R2 + z2;
   for (int i = 0; i < T->npot; i++) {
     const GhComponent* __restrict__ C = &T->comp[i];
-    // wrapped components: the wrappers' Python _evaluate, the component at
-    // phi - offset times the factor
+    if (C->mapwrap) {
+      total += gh_mapped_potential(T, i, R * cosphi0, R * sinphi0, z, t);
+      continue;
+    }
     double fac = 1.0, cosphi = cosphi0, sinphi = sinphi0;
     if (C->nwrap) {
       double off, doff;
@@ -1508,188 +1928,7 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
       cosphi = cosphi0 * co + sinphi0 * so;
       sinphi = sinphi0 * co - cosphi0 * so;
     }
-    double pot = 0.0;
-    switch (C->type) {
-      case GH_POWERSPHCUTOFF: {
-        // PowerSphericalPotentialwCutoff.c:12-24
-        const double r = sqrt(r2);
-        const double x = r2 * C->c[3];
-        const double g2 = gh_lower_incgamma(C->c[5], x, C->c[6], C->c[7], C->c[10], T->ser[i][1]);
-        const double g1 = gh_lower_incgamma(C->c[0], x, C->c[1], C->c[4], C->c[9], T->ser[i][0]);
-        pot += C->c[2] / r * (r * C->c[8] * g2 - g1);
-        break;
-      }
-      case GH_MIYAMOTONAGAI: {
-        const double as = C->p[1] + sqrt(z2 + C->c[0]);
-        pot += -C->p[0] / sqrt(R2 + as * as);
-        break;
-      }
-      case GH_NFW: {
-        const double r = sqrt(r2);
-        pot += -C->p[0] * gh_log_ge1(1.0 + r * C->c[0]) / r;
-        break;
-      }
-      case GH_HERNQUIST: {
-        const double r = sqrt(r2);
-        pot += -C->p[0] / (1.0 + r / C->p[1]) / 2.0 / C->p[1];
-        break;
-      }
-      case GH_LOGHALO: {
-        const double amp = C->p[0], core2 = C->p[2], onem1overb2 = C->p[3];
-        const double zq2 = z2 * C->c[0];
-        const double Rt2 = (onem1overb2 < 1.0) ? R2 * (1.0 - onem1overb2 * sinphi * sinphi) : R2;
-        pot += 0.5 * amp * log(Rt2 + zq2 + core2);
-        break;
-      }
-      case GH_FLATPOWER: {  // FlattenedPowerPotential.c:3-18
-        const double alpha = C->p[1], m2 = C->p[3] + R2 + z2 * C->c[0];
-        pot += (alpha == 0.0) ? 0.5 * C->p[0] * log(m2)
-                              : -C->p[0] * pow(m2, -0.5 * alpha) / alpha;
-        break;
-      }
-      case GH_SOFTNEEDLEBAR: {  // SoftenedNeedleBarPotential.c:13-29
-        double cp, sp, xb, yb;
-        gh_needle_frame(C, R * cosphi, R * sinphi, t, cp, sp, xb, yb);
-        const double a = C->p[1], bz = C->p[2] + sqrt(z2 + C->p[3]);
-        const double second = yb * yb + bz * bz;
-        const double Tp = sqrt((a + xb) * (a + xb) + second), Tm = sqrt((a - xb) * (a - xb) + second);
-        pot += 0.5 * C->p[0] * log((xb - a + Tm) / (xb + a + Tp)) / a;
-        break;
-      }
-      case GH_STEADYSPIRAL:  // SteadyLogSpiralPotential.py _evaluate (planar only)
-        pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
-               cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
-        break;
-      case GH_DEDISK:  // DoubleExponentialDiskPotential.c:6-34 (Eval)
-        pot += C->p[1] * gh_dedisk_sum<2>(T, C, 1.0 / R, fabs(z)) / R;
-        break;
-      case GH_SPIRALARMS:  // SpiralArmsPotential.c:21-58 (Eval)
-        pot += gh_spiral_pre(C, R) *
-               gh_spiral_sums<true>(T, C, R, z, atan2(sinphi, cosphi), t).pot;
-        break;
-      case GH_TRANSIENTSPIRAL:  // TransientLogSpiralPotential.py _evaluate (planar only)
-        pot += gh_transient_amp(C, t) / C->p[4] * cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
-        break;
-      case GH_COSMPHIDISK: {  // CosmphiDiskPotential.py _evaluate (planar only)
-        const double cm = cos(C->p[4] * atan2(sinphi, cosphi) - C->p[3]);
-        const double Rp = pow(R, C->p[2]);
-        pot += C->p[0] * C->c[2] * cm * ((R < C->p[5]) ? C->p[6] * (2.0 * C->c[0] - C->p[6] / Rp) : Rp);
-        break;
-      }
-      case GH_HENONHEILES:  // HenonHeilesPotential.py _evaluate (planar only)
-        pot += C->p[0] * 0.5 * R2 * (1.0 + 2.0 / 3.0 * R * (3.0 * cosphi * cosphi - sinphi * sinphi) * sinphi);
-        break;
-      case GH_LOPSIDED:  // CosmphiDiskPotential.py _evaluate at m = 1 (planar only)
-        pot += C->p[0] * C->p[1] * pow(R2, 0.5 * C->p[2]) * (cosphi * C->c[0] + sinphi * C->c[1]);
-        break;
-      case GH_ELLIPDISK: {  // EllipticalDiskPotential.py _evaluate (planar only)
-        const double c2 = cosphi * cosphi - sinphi * sinphi, s2 = 2.0 * sinphi * cosphi;
-        pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] * 0.5 *
-               pow(R2, 0.5 * C->p[4]) * (c2 * C->c[0] + s2 * C->c[1]);
-        break;
-      }
-      case GH_KKSTAECKEL: {  // KuzminKutuzovStaeckelPotential.c:5-23 (n clamped at 0)
-        const double dm = r2 - C->c[2];
-        const double s = sqrt(dm * dm + 4.0 * C->c[2] * R2), term = r2 - C->c[1] - C->c[0];
-        const double n = 0.5 * (term - s);
-        pot += -C->p[0] / (sqrt(0.5 * (term + s)) + sqrt(n > 0.0 ? n : 0.0));
-        break;
-      }
-      case GH_KUZMINDISK: {  // KuzminDiskPotential.c:5-15
-        const double az = C->p[1] + fabs(z);
-        pot += -C->p[0] / sqrt(R2 + az * az);
-        break;
-      }
-      case GH_POWERSPH: {  // PowerSphericalPotential.c Eval
-        const double alpha = C->p[1];
-        pot += (alpha == 2.0) ? 0.5 * C->p[0] * log(r2)
-                              : -C->p[0] * pow(r2, 1.0 - 0.5 * alpha) / (alpha - 2.0);
-        break;
-      }
-      case GH_JAFFE: {
-        const double r = sqrt(r2);
-        pot += -C->p[0] * log(1.0 + C->p[1] / r) / C->p[1];
-        break;
-      }
-      case GH_ISOCHRONE: pot += -C->p[0] / (C->p[1] + sqrt(r2 + C->c[0])); break;
-      case GH_PLUMMER: pot += -C->p[0] / sqrt(r2 + C->c[0]); break;
-      case GH_PSEUDOISO: {
-        const double a = C->p[1], r = sqrt(r2);
-        pot += C->p[0] * (0.5 * log(1.0 + r2 / (a * a)) + a / r * atan(r / a)) / a;
-        break;
-      }
-      case GH_BURKERT: {
-        const double a = C->p[1], x = sqrt(r2) / a;
-        pot += -C->p[0] * a * a * M_PI / x *
-               (-M_PI + 2.0 * (1.0 + x) * atan(1.0 / x) + 2.0 * (1.0 + x) * log1p(x) +
-                (1.0 - x) * log1p(x * x));
-        break;
-      }
-      case GH_DEHNENCORESPH: {
-        const double a = C->p[1], r = sqrt(r2), u = r / (r + a);
-        pot += -C->p[0] * (1.0 - u * u) / (6.0 * a);
-        break;
-      }
-      case GH_DEHNENSPH: {
-        const double a = C->p[1], alpha = C->p[2], r = sqrt(r2);
-        if (alpha == 2.0)  // the Jaffe limit of the closed form (0/0 at alpha = 2)
-          pot += -C->p[0] * log1p(a / r) / a;
-        else
-          pot += -C->p[0] * (1.0 - pow(r / (r + a), 2.0 - alpha)) / (a * (2.0 - alpha) * (3.0 - alpha));
-        break;
-      }
-      case GH_HOMOGSPHERE: {
-        const double R2s = C->p[1], R3s = C->p[2];
-        pot += (r2 < R2s) ? C->p[0] * (r2 - 3.0 * R2s) : -2.0 * C->p[0] * R3s / sqrt(r2);
-        break;
-      }
-      case GH_DEHNENBAR: {
-        const double amp = C->p[0], rb = C->p[3];
-        const double smooth = gh_bar_smooth(t, C->p[1], C->p[2]);
-        const double cos2p = cosphi * cosphi - sinphi * sinphi;
-        const double sin2p = 2.0 * sinphi * cosphi;
-        double cang, sang;
-        gh_bar_angle(cos2p, sin2p, C->p[4], C->p[5], t, cang, sang);
-        const double r = sqrt(r2);
-        double out;
-        if (r <= rb) {
-          const double u = r / rb;
-          out = (u * u * u - 2.0) * R2 / r2;
-        } else {
-          const double u = rb / r;
-          out = -u * u * u * R2 / r2;
-        }
-        pot += out * amp * smooth * cang;
-        break;
-      }
-      case GH_TRIAXHERNQUIST:
-      case GH_TRIAXNFW:
-      case GH_TRIAXJAFFE:
-      case GH_TWOPOWERTRIAX:
-      case GH_PERFECTELLIPSOID:
-      case GH_TRIAXGAUSSIAN:
-      case GH_POWERTRIAX: pot += gh_ellip_potential(T, C, R * cosphi, R * sinphi, z); break;
-      case GH_EINASTO: {  // EinastoPotential.c revaluate: -amp 4 pi h^2 n (gamma(3n,x)/s + Gamma(2n,x))
-        const double r = sqrt(r2), x = gh_einasto_x(C, r);
-        const double g3 = gh_lower_incgamma_gen(C->c[0], x, C->c[1], C->c[2]);
-        const double G2 = C->c[4] - gh_lower_incgamma_gen(C->c[3], x, C->c[4], C->c[5]);
-        const double h = C->p[1];
-        pot += -C->p[0] * C->c[6] / h * (g3 / (r * C->c[7]) + G2);
-        break;
-      }
-      case GH_EXPTRUNCNFW: {  // ExpTruncNFWPotential.c revaluate: -amp (F/r + G)
-        const double r = sqrt(r2);
-        double ex, e1b;
-        const double Fm = gh_exptruncnfw_F(C, r, ex, e1b);
-        if (r < C->c[3]) e1b = gh_e1_scaled((C->p[1] + r) * C->c[2]);
-        pot += -C->p[0] * (Fm / r + ex / (C->p[1] + r) - ex * e1b * C->c[2]);
-        break;
-      }
-      case GH_TWOPOWERSPH: pot += NAN; break;  // its Eval needs 2F1 outside the family above
-      default:
-        break;
-    }
-    total += fac * pot;
+    total += fac * gh_comp_potential(T, i, R, z, cosphi, sinphi, t);
   }
   return total;
 }

@@ -1828,6 +1828,149 @@ class GaussianAmplitudeWrapperPotential(WrapperPotential):
         return [self._amp, self._to, self._sigma2]
 
 
+class RotateAndTiltWrapperPotential(WrapperPotential):
+    """Rotates and tilts the wrapped potential (RotateAndTiltWrapperPotential.py
+    :20-210): it is evaluated at x' = rot x + offset, its Cartesian force
+    turned back by rot^T; rot turns by galaxy_pa about z and takes zvec to
+    the z axis, or is set from (inclination, sky_pa, galaxy_pa); pot_type -8,
+    own arguments [amp, 7 cache slots, rot (row-major), rotSet, offsetSet,
+    offset, 4 NaN and 6 zero cache slots] (integrateFullOrbit.py:546-568).
+    Full orbits only, as in galpy."""
+
+    pot_code = -8
+
+    def __init__(self, amp=1.0, inclination=None, galaxy_pa=None, sky_pa=None, zvec=None, offset=None, pot=None):
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        zvec, galaxy_pa = self._parse_inclination(inclination, sky_pa, zvec, galaxy_pa)
+        self._offset_xyz = None if offset is None else numpy.array(offset, dtype=float)
+        pa_rot = numpy.eye(3) if galaxy_pa is None else numpy.array(
+            [[numpy.cos(galaxy_pa), numpy.sin(galaxy_pa), 0.0], [-numpy.sin(galaxy_pa), numpy.cos(galaxy_pa), 0.0],
+             [0.0, 0.0, 1.0]])
+        zvec_rot = numpy.eye(3) if zvec is None else _zvec_rotation(zvec)
+        self._rot = numpy.dot(pa_rot, zvec_rot)
+        self._norot = bool((self._rot == numpy.eye(3)).all())
+        self.isNonAxi = True
+
+    @staticmethod
+    def _parse_inclination(inclination, sky_pa, zvec, galaxy_pa):
+        """RotateAndTiltWrapperPotential.py:112-137."""
+        if inclination is None:
+            return zvec, galaxy_pa
+        sky_pa = 0.0 if sky_pa is None else sky_pa
+        zvec_rot = numpy.dot(
+            numpy.array([[numpy.sin(sky_pa), numpy.cos(sky_pa), 0.0], [-numpy.cos(sky_pa), numpy.sin(sky_pa), 0.0],
+                         [0.0, 0.0, 1.0]]),
+            numpy.array([[1.0, 0.0, 0.0], [0.0, -numpy.cos(inclination), -numpy.sin(inclination)],
+                         [0.0, -numpy.sin(inclination), numpy.cos(inclination)]]))
+        zvec = numpy.dot(zvec_rot, numpy.array([0.0, 0.0, 1.0]))
+        int_rot = _zvec_rotation(zvec).T  # the rotation taking e_z to zvec
+        pa = numpy.dot(int_rot, numpy.dot(zvec_rot, [1.0, 0.0, 0.0]))
+        return zvec, (0.0 if galaxy_pa is None else galaxy_pa) + numpy.arctan2(pa[1], pa[0])
+
+    def _aligned(self, R, z, phi):
+        R, z, phi = (numpy.asarray(v, dtype=float) for v in numpy.broadcast_arrays(R, z, phi))
+        xyz = numpy.stack([R * numpy.cos(phi), R * numpy.sin(phi), z])
+        if not self._norot:
+            xyz = numpy.tensordot(self._rot, xyz, axes=1)
+        if self._offset_xyz is not None:
+            xyz = xyz + self._offset_xyz.reshape((3,) + (1,) * (xyz.ndim - 1))
+        return xyz
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        x, y, zp = self._aligned(R, z, phi)
+        return evaluatePotentials(self._pot, numpy.sqrt(x * x + y * y), zp, numpy.arctan2(y, x), t)
+
+    def _cart_forces(self, R, z, phi, t):
+        x, y, zp = self._aligned(R, z, phi)
+        Rp, php = numpy.sqrt(x * x + y * y), numpy.arctan2(y, x)
+        fr = evaluateRforces(self._pot, Rp, zp, php, t)
+        fp = evaluatephitorques(self._pot, Rp, zp, php, t)
+        F = numpy.stack([numpy.cos(php) * fr - numpy.sin(php) * fp / Rp,
+                         numpy.sin(php) * fr + numpy.cos(php) * fp / Rp,
+                         evaluatezforces(self._pot, Rp, zp, php, t) + 0.0 * Rp])
+        return F if self._norot else numpy.tensordot(self._rot.T, F, axes=1)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        Fx, Fy, _ = self._cart_forces(R, z, phi, t)
+        return numpy.cos(phi) * Fx + numpy.sin(phi) * Fy
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        Fx, Fy, _ = self._cart_forces(R, z, phi, t)
+        return R * (-numpy.sin(phi) * Fx + numpy.cos(phi) * Fy)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return self._cart_forces(R, z, phi, t)[2]
+
+    def wrapper_args(self):
+        off = [0.0, 0.0, 0.0] if self._offset_xyz is None else list(self._offset_xyz)
+        return ([self._amp] + [0.0] * 7 + list(self._rot.flatten()) + [float(not self._norot),
+                float(self._offset_xyz is not None)] + off + [numpy.nan] * 4 + [0.0] * 6)
+
+
+class KuzminLikeWrapperPotential(WrapperPotential):
+    """Phi(R, z) = amp Phi_pot(xi, 0), xi = sqrt(R^2 + (a + sqrt(z^2 + b^2))^2)
+    for an axisymmetric wrapped potential (KuzminLikeWrapperPotential.py
+    :20-120); pot_type -10, own arguments [amp, a, b^2]."""
+
+    pot_code = -10
+
+    def __init__(self, amp=1.0, a=1.1, b=0.0, pot=None):
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        if any(q.isNonAxi for q in flatten(pot)):
+            raise RuntimeError("KuzminLikeWrapperPotential only works for spherical or axisymmetric potentials")
+        self._a = a
+        self._b2 = b**2.0
+        self.isNonAxi = False
+
+    def _xi(self, R, z):
+        return numpy.sqrt(R**2.0 + (self._a + numpy.sqrt(z**2.0 + self._b2)) ** 2.0)
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return evaluatePotentials(self._pot, self._xi(R, z), 0.0, phi, t)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return evaluateRforces(self._pot, self._xi(R, z), 0.0, phi, t) * R / self._xi(R, z)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        s = numpy.sqrt(z**2.0 + self._b2)
+        return evaluateRforces(self._pot, self._xi(R, z), 0.0, phi, t) * (self._a + s) * z / self._xi(R, z) / s
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * numpy.asarray(R, dtype=float)
+
+    def wrapper_args(self):
+        return [self._amp, self._a, self._b2]
+
+
+class CylindricallySeparablePotentialWrapper(WrapperPotential):
+    """Phi(R, z) = amp [Phi_pot(R, 0) + Phi_pot(Rp, z) - Phi_pot(Rp, 0)]
+    (CylindricallySeparablePotentialWrapper.py:15-160); pot_type -12, own
+    arguments [amp, Rp, Phi_pot(Rp, 0)]."""
+
+    pot_code = -12
+
+    def __init__(self, amp=1.0, pot=None, Rp=1.0):
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        self._Rp = Rp
+        self._refpot = evaluatePotentials(self._pot, self._Rp, 0.0)
+        self.isNonAxi = False
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return evaluatePotentials(self._pot, R, 0.0) + evaluatePotentials(self._pot, self._Rp, z) - self._refpot
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return evaluateRforces(self._pot, R, 0.0) + 0.0 * numpy.asarray(z, dtype=float)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return evaluatezforces(self._pot, self._Rp, z) + 0.0 * numpy.asarray(R, dtype=float)
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * numpy.asarray(R, dtype=float)
+
+    def wrapper_args(self):
+        return [self._amp, self._Rp, self._refpot]
+
+
 def MWPotential2014_factory():
     """mwpotentials.py:25-29 (Bovy 2014, Table 1)."""
     return CompositePotential(
@@ -1859,6 +2002,10 @@ def _pack(comps, planar):
     pot_type, pot_args = [], []
     for p in comps:
         if isinstance(p, WrapperPotential):
+            if planar and isinstance(p, RotateAndTiltWrapperPotential):  # integratePlanarOrbit.py:592-599
+                raise NotImplementedError(
+                    "Planar orbit integration for RotateAndTiltWrapperPotential is not implemented; "
+                    "integrate an orbit with (z, vz) = (0, 0) instead")
             inner = flatten(p._pot)
             wt, wa = _pack(inner, planar)
             pot_type.append(p.pot_code)

@@ -237,7 +237,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kSpiralArms = 27, kDoubleExpDisk = 11,
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
-               kGaussianAmpWrap = -5,
+               kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
                kTriaxHernquist = 21, kTriaxNFW = 22, kTriaxJaffe = 23, kPerfectEllipsoid = 30,
                kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43,
@@ -269,7 +269,8 @@ struct Component {
 
 inline bool is_wrapper(int code) {
   return code == kDehnenSmoothWrap || code == kSolidBodyRotWrap || code == kCorotatingRotWrap ||
-         code == kGaussianAmpWrap;
+         code == kGaussianAmpWrap || code == kRotateTiltWrap || code == kKuzminLikeWrap ||
+         code == kCylSepWrap;
 }
 
 // nargs per code, as consumed from pot_args (integrateFullOrbit.c:80-345;
@@ -303,6 +304,10 @@ inline int component_nargs(int code, bool planar = false) {
     case kSolidBodyRotWrap: return 3;   // amp, omega, pa
     case kCorotatingRotWrap: return 5;  // amp, vpo, beta, pa, to
     case kGaussianAmpWrap: return 3;    // amp, to, sigma^2
+    // amp, 7 cache slots, rot[9], rotSet, offsetSet, offset[3], 10 cache slots
+    case kRotateTiltWrap: return planar ? -1 : 32;
+    case kKuzminLikeWrap: return 3;     // amp, a, b^2
+    case kCylSepWrap: return 3;         // amp, Rp, refpot
     default: return -1;
   }
 }
@@ -361,7 +366,10 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       pot_args += na;
       continue;
     }
-    for (int j = 0; j < na; j++) C.p[j] = *pot_args++;
+    for (int j = 0; j < na; j++) {  // the first 16 in p (all of them but RotateAndTilt's)
+      const double v = *pot_args++;
+      if (j < 16) C.p[j] = v;
+    }
     if (code == kFlattenedPower && planar) {
       // integratePlanarOrbit.py:183-186 packs (amp, alpha, core^2); held in
       // the full layout (q^2 = 1): at z = 0 the full functions are the planar
@@ -1002,6 +1010,57 @@ template <class R> R ellip_eval(const Component& C, R Rc, R z, R phi) {
 // Aggregated cylindrical forces: galpy_potentials.c:120-179 sums each of
 // Rforce, phitorque, zforce over components in index order starting from 0.
 // Summing the three per component gives the identical three sums.
+// KuzminLikeWrapperPotential.c:5-8: xi = sqrt(R^2 + (a + sqrt(z^2 + b^2))^2)
+template <class R> R kuzminlike_xi(R Rc, R z, double a, double b2) {
+  R asqrtbz = R(a) + sqrt(z * z + R(b2));
+  return sqrt(Rc * Rc + asqrtbz * asqrtbz);
+}
+// RotateAndTiltWrapperPotential.c:21-43: the point in the aligned frame,
+// rot . x + offset (own arguments: amp, 7 cache slots, rot[9] at 8,
+// rotSet 17, offsetSet 18, offset 19-21)
+template <class R> void rottilt_point(const Component& C, R Rc, R z, R phi, R& x, R& y, R& zz) {
+  const double* a = C.args;
+  const double* rot = a + 8;
+  x = Rc * cos(phi);
+  y = Rc * sin(phi);
+  zz = z;
+  if (a[17] != 0.) {  // rotate, galpy_potentials.c:391-399
+    R xp = R(rot[0]) * x + R(rot[1]) * y + R(rot[2]) * zz;
+    R yp = R(rot[3]) * x + R(rot[4]) * y + R(rot[5]) * zz;
+    R zp = R(rot[6]) * x + R(rot[7]) * y + R(rot[8]) * zz;
+    x = xp; y = yp; zz = zp;
+  }
+  if (a[18] != 0.) {
+    x += R(a[19]); y += R(a[20]); zz += R(a[21]);
+  }
+}
+template <class R>
+void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t, R& FR, R& Fz, R& Fphi);
+// RotateAndTiltWrapperPotentialxyzforces (:21-62) and its Rforce, phitorque,
+// zforce (:80-106): the wrapped forces at the aligned-frame point, to
+// Cartesian there, rotated back by rot^T (rotate_force, galpy_potentials.c:400-409)
+template <class R> void rottilt_forces(const Component& C, R Rc, R z, R phi, R t, R& FR, R& Fz, R& Fp) {
+  const double* a = C.args;
+  const double* rot = a + 8;
+  R x, y, zz;
+  rottilt_point(C, Rc, z, phi, x, y, zz);
+  R Rp = sqrt(x * x + y * y), php = atan2(y, x);
+  R wr, wz, wp;
+  cyl_forces(C.nwrapped, C.wrapped, Rp, zz, php, t, wr, wz, wp);
+  R Fx = cos(php) * wr - sin(php) * wp / Rp;
+  R Fy = sin(php) * wr + cos(php) * wp / Rp;
+  R Fzz = wz;
+  if (a[17] != 0.) {
+    R Fxp = R(rot[0]) * Fx + R(rot[3]) * Fy + R(rot[6]) * Fzz;
+    R Fyp = R(rot[1]) * Fx + R(rot[4]) * Fy + R(rot[7]) * Fzz;
+    R Fzp = R(rot[2]) * Fx + R(rot[5]) * Fy + R(rot[8]) * Fzz;
+    Fx = Fxp; Fy = Fyp; Fzz = Fzp;
+  }
+  FR = R(a[0]) * (cos(phi) * Fx + sin(phi) * Fy);
+  Fp = R(a[0]) * Rc * (-sin(phi) * Fx + cos(phi) * Fy);
+  Fz = R(a[0]) * Fzz;
+}
+
 template <class R>
 void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
                 R& FR, R& Fz, R& Fphi) {
@@ -1044,6 +1103,27 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
           ellip_forces(C, Rc, z, phi, fr, fz, fp);
         } else if (is_spherical_new(C.type)) {
           sph_forces(C, Rc, z, fr, fz);
+        } else if (C.type == kRotateTiltWrap) {
+          rottilt_forces(C, Rc, z, phi, t, fr, fz, fp);
+        } else if (C.type == kKuzminLikeWrap) {
+          // KuzminLikeWrapperPotential.c:58-92: the wrapped radial force at
+          // (xi, 0), phi = 0, times dxi/dR and dxi/dz
+          const double amp = C.p[0], a = C.p[1], b2 = C.p[2];
+          R xi = kuzminlike_xi(Rc, z, a, b2);
+          R wr, wz, wp;
+          cyl_forces(C.nwrapped, C.wrapped, xi, R(0.), R(0.), t, wr, wz, wp);
+          R sqrtbz = sqrt(z * z + R(b2));
+          fr = R(amp) * wr * (Rc / xi);
+          fz = R(amp) * wr * ((R(a) + sqrtbz) * z / xi / sqrtbz);
+        } else if (C.type == kCylSepWrap) {
+          // CylindricallySeparablePotentialWrapper.c:14-27: the wrapped
+          // radial force at (R, 0) and vertical force at (Rp, z), phi = t = 0
+          const double amp = C.p[0], Rp = C.p[1];
+          R wr, wz, wp;
+          cyl_forces(C.nwrapped, C.wrapped, Rc, R(0.), R(0.), R(0.), wr, wz, wp);
+          fr = R(amp) * wr;
+          cyl_forces(C.nwrapped, C.wrapped, R(Rp), z, R(0.), R(0.), wr, wz, wp);
+          fz = R(amp) * wz;
         } else if (is_wrapper(C.type)) {
           // calcRforce/calczforce/calcphitorque of the wrapped components at
           // phi - offset (the wrappers' Rforce/zforce/phitorque functions)
@@ -1611,6 +1691,16 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
           R Phip, Phipp;
           sph_dphi(C, Rc, Phip, Phipp);
           fr = -Phip;
+        } else if (C.type == kKuzminLikeWrap) {  // KuzminLikeWrapperPotential.c:93-107
+          const double amp = C.p[0], a = C.p[1], b2 = C.p[2];
+          R xi = kuzminlike_xi(Rc, R(0.), a, b2);
+          R wr, wp;
+          planar_forces(C.nwrapped, C.wrapped, xi, R(0.), t, wr, wp);
+          fr = R(amp) * wr * (Rc / xi);
+        } else if (C.type == kCylSepWrap) {  // CylindricallySeparablePotentialWrapper.c:28-34
+          R wr, wp;
+          planar_forces(C.nwrapped, C.wrapped, Rc, R(0.), R(0.), wr, wp);
+          fr = R(C.p[0]) * wr;
         } else if (is_wrapper(C.type)) {  // the wrappers' Planar functions
           R fac, off, doff, wr, wp;
           wrapper_transform(C, Rc, t, fac, off, doff);
@@ -1759,6 +1849,18 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
           pot += ellip_eval(C, Rc, z, phi);
         } else if (is_spherical_new(C.type)) {
           pot += sph_eval(C, sqrt(Rc * Rc + z * z));
+        } else if (C.type == kRotateTiltWrap) {  // RotateAndTiltWrapperPotential.py _evaluate
+          R xp, yp, zp;
+          rottilt_point(C, Rc, z, phi, xp, yp, zp);
+          pot += R(C.p[0]) * potential_value(C.nwrapped, C.wrapped, sqrt(xp * xp + yp * yp), zp,
+                                             atan2(yp, xp), t);
+        } else if (C.type == kKuzminLikeWrap) {  // KuzminLikeWrapperPotential.py _evaluate
+          pot += R(C.p[0]) * potential_value(C.nwrapped, C.wrapped, kuzminlike_xi(Rc, z, C.p[1], C.p[2]),
+                                             R(0.), phi, t);
+        } else if (C.type == kCylSepWrap) {  // CylindricallySeparablePotentialWrapper.py _evaluate
+          pot += R(C.p[0]) * (potential_value(C.nwrapped, C.wrapped, Rc, R(0.), R(0.), R(0.)) +
+                              potential_value(C.nwrapped, C.wrapped, R(C.p[1]), z, R(0.), R(0.)) -
+                              R(C.p[2]));
         } else if (is_wrapper(C.type)) {
           // the wrappers' Python _evaluate (the wrapped potential at
           // phi - offset and the same t, times the factor)
