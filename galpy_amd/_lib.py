@@ -58,6 +58,15 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
     ),
+    "gh_potential_upload_tfuncs": (
+        ctypes.c_int,
+        [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+         ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
+    ),
+    "gh_host_tfunc_eval": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
+    ),
     "gh_potential_free": (ctypes.c_int, [ctypes.c_void_p]),
     "gh_potential_device_ptr": (ctypes.c_void_p, [ctypes.c_void_p]),
     "gh_energy_drift_hip": (

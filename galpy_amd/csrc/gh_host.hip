@@ -111,6 +111,7 @@ int nargs_of(int code, bool planar) {
     case GH_ROTATETILTWRAP: return planar ? -1 : 32;
     case GH_KUZMINLIKEWRAP: return 3;  // amp, a, b^2
     case GH_CYLSEPWRAP: return 3;      // amp, Rp, Phi(Rp, 0)
+    case GH_TIMEDEPAMPWRAP: return 1;  // amp (and one time function)
     default: return -1;
   }
 }
@@ -591,6 +592,73 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
   }
   return 0;
 }
+// The caller's time functions (galpy's pot_tfuncs: C function pointers,
+// ctypes callbacks into Python for galpy) and the time range the tables
+// must cover; consumed in galpy's order (wrapped components first).
+struct GhTfuncs {
+  double (**fns)(double);
+  int used;
+  double ta, tb;
+};
+
+// Piecewise Chebyshev table of f over [ta, tb] in T->ext (layout in
+// gh_tfunc_eval): 18 coefficients per segment, computed in long double from
+// f at the Chebyshev nodes; segments doubled from 16 until f at three
+// off-node points per segment is reproduced to 4e-16 of max |f| (or 2048
+// segments, for functions that are not smooth).
+int build_tfunc_table(double (*f)(double), double ta, double tb, GhPotTable* T, int& off) {
+  const int nc = 18;
+  if (!(tb > ta)) {
+    ta -= 1.0;
+    tb += 1.0;
+  }
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int nseg = 16;; nseg *= 2) {
+    if (T->next + 5 + nseg * nc > GH_EXT_LEN) {
+      set_error("time-function table of %d segments does not fit the potential table", nseg);
+      return -1;
+    }
+    double* e = T->ext + T->next;
+    const long double h = ((long double)tb - ta) / nseg;
+    double err = 0.0, scale = 0.0;
+    for (int k = 0; k < nseg; k++) {
+      long double fv[nc];
+      for (int j = 0; j < nc; j++) {
+        const long double x = cosl(pi * (j + 0.5L) / nc);
+        fv[j] = f((double)(ta + h * (k + 0.5L * (x + 1.0L))));
+      }
+      double* c = e + 5 + k * nc;
+      for (int m = 0; m < nc; m++) {
+        long double sum = 0.0L;
+        for (int j = 0; j < nc; j++) sum += fv[j] * cosl(pi * m * (j + 0.5L) / nc);
+        c[m] = (double)(sum * 2.0L / nc * (m == 0 ? 0.5L : 1.0L));
+      }
+      for (double u : {-0.8, 0.1, 0.77}) {
+        long double b1 = 0.0L, b2 = 0.0L;
+        for (int m = nc - 1; m >= 1; m--) {
+          const long double b0 = 2.0L * u * b1 - b2 + c[m];
+          b2 = b1;
+          b1 = b0;
+        }
+        const double approx = (double)(u * b1 - b2 + c[0]);
+        const double exact = f((double)(ta + h * (k + 0.5L * (u + 1.0L))));
+        err = std::max(err, std::fabs(approx - exact));
+        scale = std::max(scale, std::fabs(exact));
+      }
+    }
+    if (err <= 4e-16 * scale || nseg >= 2048) {
+      e[0] = ta;
+      e[1] = tb;
+      e[2] = nseg;
+      e[3] = nc;
+      e[4] = (double)(nseg / ((long double)tb - ta));
+      off = T->next;
+      T->next += 5 + nseg * nc;
+      return 0;
+    }
+  }
+}
+
 
 // n components in galpy's order (parse_leapFuncArgs_Full,
 // integrateFullOrbit.c:55-1045): a wrapper's code is followed by its wrapped
@@ -599,7 +667,7 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
 // they appear (galpy's summation order); every leaf lists the wrappers
 // around it, outermost first.
 int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPotTable* T,
-                      int& nleaf, int& nslots, int depth, bool planar) {
+                      int& nleaf, int& nslots, int depth, bool planar, GhTfuncs* tf) {
   for (int k = 0; k < n; k++) {
     const int code = *pot_type++;
     if (code >= 0) {
@@ -628,7 +696,7 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
       return -1;
     }
     const int first = nleaf;
-    if (decode_components(nw, pot_type, pot_args, T, nleaf, nslots, depth + 1, planar) != 0)
+    if (decode_components(nw, pot_type, pot_args, T, nleaf, nslots, depth + 1, planar, tf) != 0)
       return -1;
     const int w = T->nwraps++;
     GhWrap& W = T->wrap[w];
@@ -640,6 +708,15 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
       pot_args += na;
     } else {
       for (int j = 0; j < na; j++) W.p[j] = *pot_args++;
+    }
+    if (code == GH_TIMEDEPAMPWRAP) {  // its time function A(t), tabulated
+      if (!tf || !tf->fns) {
+        set_error("TimeDependentAmplitudeWrapperPotential needs its time function (pot_tfuncs)");
+        return -1;
+      }
+      int off = 0;
+      if (build_tfunc_table(tf->fns[tf->used++], tf->ta, tf->tb, T, off) != 0) return -1;
+      W.p[1] = off;
     }
     if (code == GH_GAUSSIANAMPWRAP && !(W.p[2] > 0.0)) {
       set_error("GaussianAmplitudeWrapperPotential sigma^2=%g must be positive", W.p[2]);
@@ -657,7 +734,7 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
 }
 
 int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPotTable* T,
-                     bool planar = false) {
+                     bool planar = false, GhTfuncs* tf = nullptr) {
   std::memset(T, 0, offsetof(GhPotTable, ext));  // ext is read only up to next
   if (npot < 1 || npot > GH_MAX_POT) {
     set_error("npot=%d outside [1, %d]", npot, GH_MAX_POT);
@@ -666,7 +743,7 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
   int nleaf = 0, nslots = 0;
   const int* pt = pot_type;
   const double* pa = pot_args;
-  if (decode_components(npot, pt, pa, T, nleaf, nslots, 0, planar) != 0) return -1;
+  if (decode_components(npot, pt, pa, T, nleaf, nslots, 0, planar, tf) != 0) return -1;
   T->npot = nleaf;
   // leaves whose wrappers move the evaluation point take the general walk
   // (two passes under a CylindricallySeparable wrapper)
@@ -863,8 +940,9 @@ enum GhMode { GH_FULL = 0, GH_PLANAR = 1, GH_DXDV_FULL = 2, GH_DXDV_PLANAR = 3 }
 // orbits.  Rows: (R,vR,vT,z,vz,phi), (R,vR,vT,phi), or the rectangular
 // (q, p | dq, dp) of 12 / 8 doubles.
 int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npot, int* pot_type,
-                   double* pot_args, double dt, double rtol, double atol, double* result,
-                   int* err, int odeint_type, void (*cb)(void), int mode) {
+                   double* pot_args, double (**tfuncs)(double), double dt, double rtol,
+                   double atol, double* result, int* err, int odeint_type, void (*cb)(void),
+                   int mode) {
   const bool planar = mode == GH_PLANAR || mode == GH_DXDV_PLANAR;
   const size_t N = mode == GH_FULL ? 6 : mode == GH_PLANAR ? 4 : mode == GH_DXDV_FULL ? 12 : 8;
   if (nobj <= 0) return 0;
@@ -879,7 +957,20 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
   }
   std::unique_ptr<GhPotTable> table_ptr(new GhPotTable);
   GhPotTable& table = *table_ptr;
-  if (decode_potential(npot, pot_type, pot_args, &table, planar) != 0) return -1;
+  // time functions are tabulated over the output times, 5% wider each side
+  GhTfuncs tf = {tfuncs, 0, 0.0, 0.0};
+  if (tfuncs) {
+    const size_t nall = (size_t)nt * (indiv_t ? nobj : 1);
+    double lo = t[0], hi = t[0];
+    for (size_t k = 1; k < nall; k++) {
+      lo = std::min(lo, t[k]);
+      hi = std::max(hi, t[k]);
+    }
+    const double pad = 0.05 * (hi - lo);
+    tf.ta = lo - pad;
+    tf.tb = hi + pad;
+  }
+  if (decode_potential(npot, pot_type, pot_args, &table, planar, &tf) != 0) return -1;
   hipStream_t stream;
   GH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   void* d_table = nullptr;
@@ -988,15 +1079,10 @@ static void integrate_host_abi(const char* name, int nobj, double* yo, int nt, d
                                double* result, int* err, int odeint_type, void (*cb)(void),
                                int mode) {
   g_errbuf[0] = 0;
-  int rc;
-  if (pot_tfuncs != nullptr) {
-    set_error("pot_tfuncs must be NULL: no supported potential uses time functions");
-    rc = -1;
-  } else {
-    for (int i = 0; i < nobj; i++) err[i] = 0;
-    rc = host_integrate(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args, dt, rtol, atol,
-                        result, err, odeint_type, cb, mode);
-  }
+  for (int i = 0; i < nobj; i++) err[i] = 0;
+  const int rc = host_integrate(nobj, yo, nt, t, indiv_t, npot, pot_type, pot_args,
+                                (double (**)(double))pot_tfuncs, dt, rtol, atol, result, err,
+                                odeint_type, cb, mode);
   if (rc != 0) {
     for (int i = 0; i < nobj; i++) err[i] = -1;
     fprintf(stderr, "galpy_hip: %s failed: %s\n", name, g_errbuf);
@@ -1099,11 +1185,11 @@ GH_EXPORT int integratePlanarOrbit_hip(int nobj, double* d_yo, int nt, const dou
 }
 
 static int upload_potential(int npot, const int* pot_type, const double* pot_args, void** pot,
-                            bool planar) {
+                            bool planar, GhTfuncs* tf = nullptr) {
   g_errbuf[0] = 0;
   std::unique_ptr<GhPotTable> table_ptr(new GhPotTable);
   GhPotTable& table = *table_ptr;
-  if (decode_potential(npot, pot_type, pot_args, &table, planar) != 0) return -1;
+  if (decode_potential(npot, pot_type, pot_args, &table, planar, tf) != 0) return -1;
   void* d = nullptr;
   GH_CHECK(hipMalloc(&d, gh_table_alloc_bytes(table)));
   GH_CHECK(hipMemcpy(d, &table, gh_table_bytes(table), hipMemcpyHostToDevice));
@@ -1122,6 +1208,26 @@ GH_EXPORT int gh_potential_upload(int npot, const int* pot_type, const double* p
 GH_EXPORT int gh_planar_potential_upload(int npot, const int* pot_type, const double* pot_args,
                                          void** pot) {
   return upload_potential(npot, pot_type, pot_args, pot, true);
+}
+
+// Host evaluation of the device's time-function table of fns[0] over
+// [t0, t1] at ts[n] (for CPU tests of the table).
+GH_EXPORT int gh_host_tfunc_eval(double (**fns)(double), double t0, double t1, int n,
+                                 const double* ts, double* out) {
+  g_errbuf[0] = 0;
+  static GhPotTable T;
+  T.next = 0;
+  int off = 0;
+  if (build_tfunc_table(fns[0], std::min(t0, t1), std::max(t0, t1), &T, off) != 0) return -1;
+  for (int k = 0; k < n; k++) out[k] = gh_tfunc_eval(&T, off, ts[k]);
+  return 0;
+}
+
+GH_EXPORT int gh_potential_upload_tfuncs(int npot, const int* pot_type, const double* pot_args,
+                                         double (**pot_tfuncs)(double), double t0, double t1,
+                                         int planar, void** pot) {
+  GhTfuncs tf = {pot_tfuncs, 0, std::min(t0, t1), std::max(t0, t1)};
+  return upload_potential(npot, pot_type, pot_args, pot, planar != 0, &tf);
 }
 
 GH_EXPORT int gh_potential_free(void* pot) {

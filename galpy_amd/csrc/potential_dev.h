@@ -85,6 +85,7 @@ enum GhPotCode {
   GH_ROTATETILTWRAP = -8,
   GH_KUZMINLIKEWRAP = -10,
   GH_CYLSEPWRAP = -12,
+  GH_TIMEDEPAMPWRAP = -9,
 };
 
 struct GhComponent {
@@ -1257,6 +1258,29 @@ __device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__
   }
 }
 
+// A caller's time function A(t) (galpy's pot_tfuncs, Python callbacks that
+// the reference calls from C) as a piecewise Chebyshev table built on the
+// host from those callbacks over the integration's time range: ext[off] =
+// t_a, t_b, nseg, ncoef, nseg / (t_b - t_a), then ncoef coefficients per
+// segment; evaluated by Clenshaw's recurrence (segments clamped at the ends).
+__host__ __device__ inline double gh_tfunc_eval(const GhPotTable* __restrict__ T, int off,
+                                                double t) {
+  const double* __restrict__ e = &T->ext[off];
+  const int nseg = (int)e[2], nc = (int)e[3];
+  const double s = (t - e[0]) * e[4];
+  int k = (int)floor(s);
+  k = k < 0 ? 0 : (k >= nseg ? nseg - 1 : k);
+  const double u = 2.0 * (s - k) - 1.0, u2 = 2.0 * u;
+  const double* __restrict__ c = e + 5 + k * nc;
+  double b1 = 0.0, b2 = 0.0;
+  for (int j = nc - 1; j >= 1; j--) {
+    const double b0 = fma(u2, b1, c[j] - b2);
+    b2 = b1;
+    b1 = b0;
+  }
+  return fma(u, b1, c[0] - b2);
+}
+
 // The wrappers around component C at cylindrical radius R and time t: the
 // product of their amplitudes and time factors, the total angle offset (the
 // wrapped component is evaluated at phi - offset) and d offset / dR.  Nested
@@ -1292,6 +1316,7 @@ __device__ __forceinline__ void gh_wrap_transform(const GhPotTable* __restrict__
         fac *= exp(-0.5 * d * d / W->p[2]);
         break;
       }
+      case GH_TIMEDEPAMPWRAP: fac *= gh_tfunc_eval(T, (int)W->p[1], t); break;
       case GH_SOLIDBODYROTWRAP: off += W->p[1] * t + W->p[2]; break;
       case GH_COROTATINGROTWRAP: {
         const double dt = t - W->p[4];
@@ -1321,11 +1346,12 @@ __device__ __forceinline__ void gh_wrap_transform(const GhPotTable* __restrict__
 //     two passes at t = 0, the radial force at (R, 0, 0) and the vertical
 //     force at (Rp, 0, z); every map is linear in the force, so the passes'
 //     forces add;
-//   DehnenSmooth, GaussianAmplitude: factors only.
+//   DehnenSmooth, GaussianAmplitude, TimeDependentAmplitude: factors only.
 // rec (when not null) keeps what the way back needs: {cos, sin, d offset/dR,
 // x, y} for the rotations, {x, y, (a + s) z / s, xi} for KuzminLike, {x, y,
 // R} for CylindricallySeparable.
-__device__ __forceinline__ void gh_wrap_forward(const GhWrap* __restrict__ W, int pass,
+__device__ __forceinline__ void gh_wrap_forward(const GhPotTable* __restrict__ T,
+                                                const GhWrap* __restrict__ W, int pass,
                                                 double& x, double& y, double& z, double& t,
                                                 double& fac, double* rec) {
   fac *= W->p[0];
@@ -1340,6 +1366,7 @@ __device__ __forceinline__ void gh_wrap_forward(const GhWrap* __restrict__ W, in
       fac *= exp(-0.5 * d * d / W->p[2]);
       break;
     }
+    case GH_TIMEDEPAMPWRAP: fac *= gh_tfunc_eval(T, (int)W->p[1], t); break;
     case GH_SOLIDBODYROTWRAP:
     case GH_COROTATINGROTWRAP: {
       double off, doff = 0.0;
@@ -1475,7 +1502,7 @@ __device__ inline void gh_comp_force_mapped(const GhPotTable* __restrict__ T, in
     double rec[GH_WRAP_DEPTH][5];
 #pragma unroll
     for (int k = 0; k < GH_WRAP_DEPTH; k++)
-      if (k < nw) gh_wrap_forward(&T->wrap[C->wrap[k]], pass, x, y, z, t, fac, rec[k]);
+      if (k < nw) gh_wrap_forward(T, &T->wrap[C->wrap[k]], pass, x, y, z, t, fac, rec[k]);
     GhGeom gl;
     gl.x = x;
     gl.y = y;
@@ -1896,7 +1923,7 @@ __device__ inline double gh_mapped_potential(const GhPotTable* __restrict__ T, i
     double x = x0, y = y0, z = z0, t = t0, fac = 1.0;
     for (int k = 0; k < C->nwrap; k++) {
       const GhWrap* __restrict__ W = &T->wrap[C->wrap[k]];
-      gh_wrap_forward(W, pass, x, y, z, t, fac, nullptr);
+      gh_wrap_forward(T, W, pass, x, y, z, t, fac, nullptr);
       if (W->kind == GH_CYLSEPWRAP && pass == 1) total -= fac * W->p[2];
     }
     const double R = sqrt(x * x + y * y);

@@ -71,6 +71,21 @@ def integratePlanarOrbit_hip(pot, yo, t, int_method, rtol=None, atol=None, progr
                            rtol, atol, progressbar, dt)
 
 
+TFUNC_TYPE = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double)
+
+
+def tfunc_array(tfuncs):
+    """galpy's ``_prep_tfuncs`` (integrateFullOrbit.py:970-999 without numba):
+    the time functions as a C array of ``double f(double)`` pointers, or None
+    when there are none.  The returned object keeps the callbacks alive."""
+    if not tfuncs:
+        return None
+    cbs = [TFUNC_TYPE(lambda t, f=f: float(f(t))) for f in tfuncs]
+    arr = (TFUNC_TYPE * len(cbs))(*cbs)
+    arr._keep = cbs
+    return arr
+
+
 def _integrate_host(symbol, parse, ncol, pot, yo, t, int_method, rtol, atol, progressbar, dt):
     lib = _lib.load()
     single_obj = len(numpy.shape(yo)) == 1
@@ -79,7 +94,8 @@ def _integrate_host(symbol, parse, ncol, pot, yo, t, int_method, rtol, atol, pro
         raise ValueError(f"{symbol} takes initial conditions of {ncol} phase-space coordinates")
     nobj = len(yo)
     rtol, atol = _parse_tol(rtol, atol)
-    npot, pot_type, pot_args, _ = parse(pot)
+    npot, pot_type, pot_args, tfuncs = parse(pot)
+    tf = tfunc_array(tfuncs)
     int_method_c = _parse_integrator(int_method)
     if dt is None:
         dt = DT_SENTINEL
@@ -99,7 +115,8 @@ def _integrate_host(symbol, parse, ncol, pot, yo, t, int_method, rtol, atol, pro
             cb = None
     getattr(lib, symbol)(
         ctypes.c_int(nobj), yo, ctypes.c_int(nt), t, ctypes.c_int(int(indiv_t)),
-        ctypes.c_int(npot), pot_type, pot_args, None, ctypes.c_double(dt),
+        ctypes.c_int(npot), pot_type, pot_args, ctypes.cast(tf, ctypes.c_void_p) if tf is not None else None,
+        ctypes.c_double(dt),
         ctypes.c_double(rtol), ctypes.c_double(atol), result, err,
         ctypes.c_int(int_method_c), ctypes.cast(cb, ctypes.c_void_p) if cb is not None else None,
     )
@@ -116,22 +133,31 @@ def _integrate_host(symbol, parse, ncol, pot, yo, t, int_method, rtol, atol, pro
 
 class PotentialTable:
     """A potential decoded into a device-resident table (gh_potential_upload,
-    or gh_planar_potential_upload from the planar packing when ``planar``)."""
+    or gh_planar_potential_upload from the planar packing when ``planar``;
+    a potential with time functions needs ``trange``, the time range they
+    are tabulated over)."""
 
-    def __init__(self, pot, planar=False):
+    def __init__(self, pot, planar=False, trange=None):
         self.pot = pot
         self.planar = bool(planar)
-        npot, pot_type, pot_args, _ = (parse_pot_planar if planar else parse_pot)(pot)
+        npot, pot_type, pot_args, tfuncs = (parse_pot_planar if planar else parse_pot)(pot)
         self.npot, self.pot_type, self.pot_args = npot, pot_type, pot_args
         lib = _lib.load()
         handle = ctypes.c_void_p()
-        upload = "gh_planar_potential_upload" if planar else "gh_potential_upload"
-        _lib.check(
-            getattr(lib, upload)(
-                npot, pot_type.ctypes.data, pot_args.ctypes.data, ctypes.byref(handle)
-            ),
-            upload,
-        )
+        if tfuncs:
+            # time functions are tabulated over the times the table serves
+            if trange is None:
+                raise ValueError("this potential has time functions: give trange=(t0, t1), the times it "
+                                 "will be integrated over")
+            self._tf = tfunc_array(tfuncs)
+            upload = "gh_potential_upload_tfuncs"
+            rc = lib.gh_potential_upload_tfuncs(npot, pot_type.ctypes.data, pot_args.ctypes.data,
+                                                ctypes.cast(self._tf, ctypes.c_void_p), float(trange[0]),
+                                                float(trange[1]), int(self.planar), ctypes.byref(handle))
+        else:
+            upload = "gh_planar_potential_upload" if planar else "gh_potential_upload"
+            rc = getattr(lib, upload)(npot, pot_type.ctypes.data, pot_args.ctypes.data, ctypes.byref(handle))
+        _lib.check(rc, upload)
         self.handle = handle
 
     @property

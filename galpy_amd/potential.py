@@ -1971,6 +1971,31 @@ class CylindricallySeparablePotentialWrapper(WrapperPotential):
         return [self._amp, self._Rp, self._refpot]
 
 
+class TimeDependentAmplitudeWrapperPotential(WrapperPotential):
+    """Multiplies the wrapped potential by a function of time A(t)
+    (TimeDependentAmplitudeWrapperPotential.py:15-90); pot_type -9, own
+    arguments [amp] and one time function, A (integrateFullOrbit.py:569-580).
+    The GPU evaluates A from a table built by calling it on the host."""
+
+    pot_code = -9
+
+    def __init__(self, amp=1.0, A=None, pot=None):
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        if A is None or not callable(A):
+            raise ValueError("TimeDependentAmplitudeWrapperPotential needs a function A(t)")
+        self._A = A
+
+    def _factor(self, t):
+        t = numpy.asarray(t, dtype=float)
+        return numpy.vectorize(lambda s: float(self._A(float(s))))(t) if t.ndim else float(self._A(float(t)))
+
+    def wrapper_args(self):
+        return [self._amp]
+
+    def wrapper_tfuncs(self):
+        return [self._A]
+
+
 def MWPotential2014_factory():
     """mwpotentials.py:25-29 (Bovy 2014, Table 1)."""
     return CompositePotential(
@@ -1994,12 +2019,16 @@ def flatten(pot):
     return out
 
 
-def _pack(comps, planar):
+def _pack(comps, planar, tfuncs=None):
     """Codes and arguments of a list of components in galpy's order; a
     wrapper's code is followed by its wrapped codes, its arguments are
     nwrapped, the wrapped arguments, then its own
-    (integrateFullOrbit.py:396-448, integratePlanarOrbit.py:507-565)."""
+    (integrateFullOrbit.py:396-448, integratePlanarOrbit.py:507-565); time
+    functions are appended to ``tfuncs`` in the same order (wrapped
+    components first, integrateFullOrbit.py:569-580)."""
     pot_type, pot_args = [], []
+    if tfuncs is None:
+        tfuncs = []
     for p in comps:
         if isinstance(p, WrapperPotential):
             if planar and isinstance(p, RotateAndTiltWrapperPotential):  # integratePlanarOrbit.py:592-599
@@ -2007,12 +2036,13 @@ def _pack(comps, planar):
                     "Planar orbit integration for RotateAndTiltWrapperPotential is not implemented; "
                     "integrate an orbit with (z, vz) = (0, 0) instead")
             inner = flatten(p._pot)
-            wt, wa = _pack(inner, planar)
+            wt, wa = _pack(inner, planar, tfuncs)
             pot_type.append(p.pot_code)
             pot_type.extend(wt)
             pot_args.append(float(_npot(inner)))
             pot_args.extend(wa)
             pot_args.extend(p.wrapper_args())
+            tfuncs.extend(getattr(p, "wrapper_tfuncs", lambda: [])())
             continue
         if isinstance(p, MN3ExponentialDiskPotential):
             for args in p.expand():
@@ -2043,18 +2073,20 @@ def _purge_null(comps):
 
 def parse_pot(pot):
     """galpy's ``_parse_pot`` (integrateFullOrbit.py:43-614) for the supported
-    potentials: returns ``(npot, pot_type int32[], pot_args float64[], [])``;
+    potentials: returns ``(npot, pot_type int32[], pot_args float64[],
+    pot_tfuncs)``, pot_tfuncs the components' time functions (callables);
     npot counts the top-level components (a wrapper is one).
 
     Raises NotImplementedError for a potential without a GPU kernel, as galpy
     does for a potential without a C implementation (:604-611)."""
     comps = _purge_null(flatten(pot))
-    pot_type, pot_args = _pack(comps, False)
+    tfuncs = []
+    pot_type, pot_args = _pack(comps, False, tfuncs)
     return (
         _npot(comps),
         numpy.array(pot_type, dtype=numpy.int32, order="C"),
         numpy.array(pot_args, dtype=numpy.float64, order="C"),
-        [],
+        tfuncs,
     )
 
 
@@ -2063,12 +2095,13 @@ def parse_pot_planar(pot):
     supported potentials, taken in the plane z = 0: returns
     ``(npot, pot_type int32[], pot_args float64[], [])``."""
     comps = _purge_null(flatten(pot))
-    pot_type, pot_args = _pack(comps, True)
+    tfuncs = []
+    pot_type, pot_args = _pack(comps, True, tfuncs)
     return (
         _npot(comps),
         numpy.array(pot_type, dtype=numpy.int32, order="C"),
         numpy.array(pot_args, dtype=numpy.float64, order="C"),
-        [],
+        tfuncs,
     )
 
 

@@ -62,7 +62,11 @@ extern "C" {
  *                   ias15: -3 after 2^22 step attempts in all; rows past
  *                   the end of the run (a dt shorter than the grid spacing,
  *                   wez_ias15.c:280) are NaN, uninitialised in the reference
- *   pot_tfuncs      must be NULL (no supported potential uses time functions)
+ *   pot_tfuncs      the components' time functions double f(double t), in
+ *                   galpy's order (integrateFullOrbit.c:1034-1038), or NULL
+ *                   when none takes one; each is tabulated on the host as a
+ *                   piecewise Chebyshev series over the output times (5%
+ *                   wider each side) and the device evaluates the table
  *   cb              called once per finished orbit (may be NULL)
  * All pointers are host memory.  Work is streamed through the GPU in chunks
  * of orbits sized to free device memory. */
@@ -178,6 +182,18 @@ int gh_potential_upload(int npot, const int *pot_type, const double *pot_args, v
 /* The same from galpy's planar packing (integratePlanarOrbit.c:55-739). */
 int gh_planar_potential_upload(int npot, const int *pot_type, const double *pot_args,
                                void **pot);
+/* gh_potential_upload / gh_planar_potential_upload (planar != 0) for a
+ * potential whose components take time functions (pot_tfuncs as for
+ * integrateFullOrbit): each is tabulated on the host over [t0, t1], the
+ * times the handle will be integrated over. */
+int gh_potential_upload_tfuncs(int npot, const int *pot_type, const double *pot_args,
+                               double (**pot_tfuncs)(double), double t0, double t1, int planar,
+                               void **pot);
+/* Host evaluation of the table gh_potential_upload_tfuncs builds for the
+ * time function fns[0] over [t0, t1], at ts[n] (for CPU tests; no
+ * reference counterpart). */
+int gh_host_tfunc_eval(double (**fns)(double), double t0, double t1, int n, const double *ts,
+                       double *out);
 int gh_potential_free(void *pot);
 /* Device address of the table behind a handle. */
 const void *gh_potential_device_ptr(const void *pot);

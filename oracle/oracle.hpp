@@ -238,6 +238,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
+               kTimeDepAmpWrap = -9,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
                kTriaxHernquist = 21, kTriaxNFW = 22, kTriaxJaffe = 23, kPerfectEllipsoid = 30,
                kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43,
@@ -265,12 +266,20 @@ struct Component {
   // a component with a variable number of arguments reads them where they
   // lie in pot_args, as galpy's potentialArg.args does (valid for the call)
   const double* args = nullptr;
+  // a wrapper's time function (potentialArg.tfuncs, galpy_potentials.h)
+  double (*tfunc)(double) = nullptr;
 };
+
+// The caller's time functions (galpy's pot_tfuncs), consumed in parse order
+// by the components that take them (integrateFullOrbit.c:1034-1038):
+// wrapped components first, then the wrapper.  Set before a parse.
+inline double (**g_tfuncs)(double) = nullptr;
+inline int g_ntfuncs = 0;
 
 inline bool is_wrapper(int code) {
   return code == kDehnenSmoothWrap || code == kSolidBodyRotWrap || code == kCorotatingRotWrap ||
          code == kGaussianAmpWrap || code == kRotateTiltWrap || code == kKuzminLikeWrap ||
-         code == kCylSepWrap;
+         code == kCylSepWrap || code == kTimeDepAmpWrap;
 }
 
 // nargs per code, as consumed from pot_args (integrateFullOrbit.c:80-345;
@@ -308,6 +317,7 @@ inline int component_nargs(int code, bool planar = false) {
     case kRotateTiltWrap: return planar ? -1 : 32;
     case kKuzminLikeWrap: return 3;     // amp, a, b^2
     case kCylSepWrap: return 3;         // amp, Rp, refpot
+    case kTimeDepAmpWrap: return 1;     // amp (and one time function)
     default: return -1;
   }
 }
@@ -337,7 +347,7 @@ inline int ellipsoidal_nargs(const double* a) {
 // (integrateFullOrbit.c:1010-1018, 1026-1031).  Wrapped components are
 // stored in out[next...].
 inline int parse_components(int n, const int*& pot_type, const double*& pot_args, Component* out,
-                            int& next, Component* dst, bool planar) {
+                            int& next, Component* dst, bool planar, int& tf_next) {
   for (int i = 0; i < n; i++) {
     int code = *pot_type++;
     int na = is_ellipsoidal(code)    ? ellipsoidal_nargs(pot_args)
@@ -356,7 +366,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       if (nw < 1 || next + nw > kOracleMaxComp) return -1;
       Component* kids = out + next;
       next += nw;
-      if (parse_components(nw, pot_type, pot_args, out, next, kids, planar) != 0) return -1;
+      if (parse_components(nw, pot_type, pot_args, out, next, kids, planar, tf_next) != 0) return -1;
       C.nwrapped = nw;
       C.wrapped = kids;
     }
@@ -369,6 +379,11 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
     for (int j = 0; j < na; j++) {  // the first 16 in p (all of them but RotateAndTilt's)
       const double v = *pot_args++;
       if (j < 16) C.p[j] = v;
+    }
+    C.tfunc = nullptr;
+    if (code == kTimeDepAmpWrap) {
+      if (tf_next >= g_ntfuncs) return -1;  // needs a time function
+      C.tfunc = g_tfuncs[tf_next++];
     }
     if (code == kFlattenedPower && planar) {
       // integratePlanarOrbit.py:183-186 packs (amp, alpha, core^2); held in
@@ -395,8 +410,8 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
 inline int parse_potential(int npot, const int* pot_type, const double* pot_args,
                            Component* out, bool planar = false) {
   if (npot < 0 || npot > kOracleMaxComp) return -1;
-  int next = npot;
-  return parse_components(npot, pot_type, pot_args, out, next, out, planar);
+  int next = npot, tf_next = 0;
+  return parse_components(npot, pot_type, pot_args, out, next, out, planar, tf_next);
 }
 
 // DehnenSmoothWrapperPotential.c:4-16 (dehnenSmooth) and
@@ -427,6 +442,8 @@ template <class R> void wrapper_transform(const Component& C, R Rc, R t, R& fac,
   switch (C.type) {
     case kDehnenSmoothWrap: fac = fac * dehnen_smooth(t, C.p[1], C.p[2], C.p[3] != 0.); break;
     case kGaussianAmpWrap: fac = fac * gauss_smooth(t, C.p[1], C.p[2]); break;
+    // TimeDependentAmplitudeWrapperPotential.c:3-40: amp A(t)
+    case kTimeDepAmpWrap: fac = fac * R(C.tfunc(dval(t))); break;
     case kSolidBodyRotWrap: off = R(C.p[1]) * t + R(C.p[2]); break;  // SolidBodyRotationWrapperPotential.c:4-9
     case kCorotatingRotWrap: {
       double vpo = C.p[1], beta = C.p[2], pa = C.p[3], to = C.p[4];

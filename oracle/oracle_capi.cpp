@@ -21,6 +21,13 @@ using namespace orc;
 
 #define ORC_EXPORT extern "C" __attribute__((visibility("default")))
 
+// The caller's time functions for the following calls (galpy's pot_tfuncs;
+// NULL / 0 for none).
+ORC_EXPORT void oracle_set_tfuncs(double (**fns)(double), int n) {
+  orc::g_tfuncs = fns;
+  orc::g_ntfuncs = n;
+}
+
 namespace {
 
 constexpr int kMaxPot = kOracleMaxComp;
