@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <vector>
 
 #include "gh_internal.h"
 #include "dxdv_dev.h"
@@ -206,6 +207,58 @@ int fit_upper_gamma_table(double a, double xbig, double tab[GH_TAB_MAXSEG][GH_TA
     }
   }
   return e0;
+}
+
+// GSL's natural cubic spline through (xs, ys), n >= 2 (gsl_interp_cspline,
+// c_i = y''/2, c_0 = c_{n-1} = 0; Thomas' algorithm in long double) into
+// T->ext in gh_spline_eval's layout; off receives its offset.
+int build_spline(const double* xs, const double* ys, int n, GhPotTable* T, int& off) {
+  if (n < 2 || T->next + 1 + n + 5 * (n - 1) > GH_EXT_LEN) {
+    set_error("spline of %d knots does not fit the potential table", n);
+    return -1;
+  }
+  for (int i = 0; i + 1 < n; i++)
+    if (!(xs[i + 1] > xs[i])) {
+      set_error("spline knots must increase");
+      return -1;
+    }
+  std::vector<long double> c(n, 0.0L);
+  if (n >= 3) {
+    const int m = n - 2;
+    std::vector<long double> diag(m), offd(m), rhs(m);
+    for (int i = 1; i <= m; i++) {
+      const long double h0 = (long double)xs[i] - xs[i - 1], h1 = (long double)xs[i + 1] - xs[i];
+      diag[i - 1] = 2.0L * (h0 + h1);
+      offd[i - 1] = h1;
+      rhs[i - 1] = 3.0L * (((long double)ys[i + 1] - ys[i]) / h1 - ((long double)ys[i] - ys[i - 1]) / h0);
+    }
+    for (int i = 1; i < m; i++) {
+      const long double w = offd[i - 1] / diag[i - 1];
+      diag[i] -= w * offd[i - 1];
+      rhs[i] -= w * rhs[i - 1];
+    }
+    c[m] = rhs[m - 1] / diag[m - 1];
+    for (int i = m - 2; i >= 0; i--) c[i + 1] = (rhs[i] - offd[i] * c[i + 2]) / diag[i];
+  }
+  double* e = T->ext + T->next;
+  e[0] = n;
+  for (int i = 0; i < n; i++) e[1 + i] = xs[i];
+  long double integ = 0.0L;
+  for (int i = 0; i + 1 < n; i++) {
+    const long double h = (long double)xs[i + 1] - xs[i];
+    const long double b = ((long double)ys[i + 1] - ys[i]) / h - h * (c[i + 1] + 2.0L * c[i]) / 3.0L;
+    const long double d = (c[i + 1] - c[i]) / (3.0L * h);
+    double* q = e + 1 + n + 5 * i;
+    q[0] = ys[i];
+    q[1] = (double)b;
+    q[2] = (double)c[i];
+    q[3] = (double)d;
+    q[4] = (double)integ;
+    integ += h * (ys[i] + h * (b / 2.0L + h * (c[i] / 3.0L + h * d / 4.0L)));
+  }
+  off = T->next;
+  T->next += 1 + n + 5 * (n - 1);
+  return 0;
 }
 
 bool is_ellipsoidal(int code) {
@@ -428,12 +481,30 @@ int decode_dedisk(const double*& pot_args, GhPotTable* T, int i, bool planar) {
   return 0;
 }
 
+// interpSphericalPotential (integrateFullOrbit.py:366-373, the same in the
+// planar packing): nr, rgrid, the force on rgrid, amp, rmin, rmax,
+// M(<rmax), Phi0, Phimax; the force's natural spline in ext.
+int decode_interpsph(const double*& pot_args, GhPotTable* T, int i) {
+  GhComponent& C = T->comp[i];
+  const double* a = pot_args;
+  const int nr = (int)a[0];
+  C.type = GH_INTERPSPH;
+  int off = 0;
+  if (build_spline(a + 1, a + 1 + nr, nr, T, off) != 0) return -1;
+  C.ext = off;
+  for (int j = 0; j < 6; j++) C.p[j] = a[1 + 2 * nr + j];
+  T->has_wrap = 1;  // extended dispatch instances
+  pot_args += 1 + 2 * nr + 6;
+  return 0;
+}
+
 // One (leaf) component's arguments and precomputed constants.
 int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
                      bool planar) {
   if (is_ellipsoidal(code)) return decode_ellipsoidal(code, pot_args, T, i);
   if (code == GH_SPIRALARMS) return decode_spiralarms(pot_args, T, i);
   if (code == GH_DEDISK) return decode_dedisk(pot_args, T, i, planar);
+  if (code == GH_INTERPSPH) return decode_interpsph(pot_args, T, i);
   const int na = nargs_of(code, planar);
   if (na < 0 || code < 0) {
     set_error("pot_type %d (component %d) has no HIP kernel", code, i);
@@ -679,7 +750,7 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
       nleaf++;
       continue;
     }
-    const int na = nargs_of(code, planar);
+    const int na = (code == GH_MOVINGOBJWRAP) ? 0 : nargs_of(code, planar);  // counted below
     if (na < 0) {
       set_error("wrapper pot_type %d has no HIP kernel", code);
       return -1;
@@ -701,7 +772,27 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
     const int w = T->nwraps++;
     GhWrap& W = T->wrap[w];
     W.kind = code;
-    if (code == GH_ROTATETILTWRAP) {  // amp, rot, rotSet, offsetSet, offset (caches skipped)
+    if (code == GH_MOVINGOBJWRAP) {
+      // initMovingObjectSplines (integrateFullOrbit.c:1392-1431; planar
+      // integratePlanarOrbit.c:1177-1215): nPts, t, x, y[, z], amp, t0, tf;
+      // natural splines of the positions over (t - t0) / (tf - t0)
+      const double* a = pot_args;
+      const int n = (int)a[0], nc = planar ? 2 : 3;
+      const double* tarr = a + 1;
+      const double to = tarr[(nc + 1) * n + 1], tf = tarr[(nc + 1) * n + 2];
+      std::vector<double> tt(n);
+      for (int j = 0; j < n; j++) tt[j] = (tarr[j] - to) / (tf - to);
+      W.p[0] = tarr[(nc + 1) * n];
+      W.p[1] = to;
+      W.p[2] = tf;
+      W.p[5] = -1.0;
+      for (int k = 0; k < nc; k++) {
+        int off = 0;
+        if (build_spline(tt.data(), tarr + (k + 1) * n, n, T, off) != 0) return -1;
+        W.p[3 + k] = off;
+      }
+      pot_args += 1 + (nc + 1) * n + 3;
+    } else if (code == GH_ROTATETILTWRAP) {  // amp, rot, rotSet, offsetSet, offset (caches skipped)
       const double* a = pot_args;
       W.p[0] = a[0];
       for (int j = 0; j < 14; j++) W.p[1 + j] = a[8 + j];
@@ -752,7 +843,8 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
     int moved = 0, cylsep = 0;
     for (int k = 0; k < C.nwrap; k++) {
       const int kind = T->wrap[C.wrap[k]].kind;
-      moved |= kind == GH_ROTATETILTWRAP || kind == GH_KUZMINLIKEWRAP || kind == GH_CYLSEPWRAP;
+      moved |= kind == GH_ROTATETILTWRAP || kind == GH_KUZMINLIKEWRAP || kind == GH_CYLSEPWRAP ||
+               kind == GH_MOVINGOBJWRAP;
       cylsep += kind == GH_CYLSEPWRAP;
     }
     if (cylsep > 1) {

@@ -86,6 +86,8 @@ enum GhPotCode {
   GH_KUZMINLIKEWRAP = -10,
   GH_CYLSEPWRAP = -12,
   GH_TIMEDEPAMPWRAP = -9,
+  GH_MOVINGOBJWRAP = -6,
+  GH_INTERPSPH = 36,
 };
 
 struct GhComponent {
@@ -138,6 +140,41 @@ __host__ __device__ inline size_t gh_table_bytes(const GhPotTable& T) {
 __host__ __device__ inline size_t gh_table_alloc_bytes(const GhPotTable& T) {
   return (gh_table_bytes(T) + 255) & ~(size_t)255;
 }
+
+// GSL's natural cubic spline (gsl_interp_cspline, which galpy's C builds for
+// interpSphericalPotential and MovingObjectPotential), coefficients computed
+// on the host: ext[off] = n, x[0..n-1], then per interval i < n-1 the value
+// y_i, b_i, c_i, d_i (y = y_i + d (b_i + d (c_i + d d_i)), d = x - x_i) and
+// the integral I_i from x[0] to x[i].  The interval by binary search.
+__device__ __forceinline__ int gh_spline_find(const double* __restrict__ x, int n, double t) {
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (x[mid] > t) hi = mid;
+    else lo = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ double gh_spline_eval(const GhPotTable* __restrict__ T, int off,
+                                                 double t) {
+  const double* __restrict__ e = &T->ext[off];
+  const int n = (int)e[0];
+  const int i = gh_spline_find(e + 1, n, t);
+  const double* __restrict__ q = e + 1 + n + 5 * i;
+  const double d = t - e[1 + i];
+  return fma(d, fma(d, fma(d, q[3], q[2]), q[1]), q[0]);
+}
+// integral of the spline from x[0] to t
+__device__ __forceinline__ double gh_spline_integ(const GhPotTable* __restrict__ T, int off,
+                                                  double t) {
+  const double* __restrict__ e = &T->ext[off];
+  const int n = (int)e[0];
+  const int i = gh_spline_find(e + 1, n, t);
+  const double* __restrict__ q = e + 1 + n + 5 * i;
+  const double d = t - e[1 + i];
+  return q[4] + d * fma(d, fma(d, fma(d, 0.25 * q[3], q[2] / 3.0), 0.5 * q[1]), q[0]);
+}
+
 
 // ---------------------------------------------------------------------------
 // Lower incomplete gamma gamma(a, x) (replaces GSL's gsl_sf_gamma_inc_P and
@@ -1190,6 +1227,25 @@ __device__ __forceinline__ void gh_comp_force<GH_SPIRALARMS>(const GhPotTable* _
   F.FphioR2 += pre * S.sp * iR2;
 }
 
+// interpSphericalPotential.c:5-37 through SphericalPotential.c:13-38: the
+// radial force is the natural spline of the tabulated force on [rmin, rmax),
+// 0 inside rmin, -M/r^2 outside rmax.  p: amp, rmin, rmax, M(<rmax), Phi0,
+// Phimax; ext = the spline.
+__device__ __forceinline__ double gh_interpsph_rforce(const GhPotTable* __restrict__ T,
+                                                      const GhComponent* __restrict__ C,
+                                                      double r) {
+  if (r >= C->p[2]) return -C->p[3] / (r * r);
+  return (r < C->p[1]) ? 0.0 : gh_spline_eval(T, C->ext, r);
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_INTERPSPH>(const GhPotTable* __restrict__ T,
+                                                            int i, const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double f = C->p[0] * gh_interpsph_rforce(T, C, g.r) * g.rinv;
+  F.FRoR += f;
+  F.Fz += f * g.z;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_NULLPOT>(const GhPotTable* __restrict__, int,
                                                           const GhGeom&, GhAcc&) {}
@@ -1235,7 +1291,8 @@ __host__ __device__ inline bool gh_is_heavy(int type) {
   return type == GH_TRIAXHERNQUIST || type == GH_TRIAXNFW || type == GH_TRIAXJAFFE ||
          type == GH_TWOPOWERTRIAX || type == GH_PERFECTELLIPSOID || type == GH_TRIAXGAUSSIAN ||
          type == GH_POWERTRIAX || type == GH_EINASTO || type == GH_TWOPOWERSPH ||
-         type == GH_EXPTRUNCNFW || type == GH_SPIRALARMS || type == GH_DEDISK;
+         type == GH_EXPTRUNCNFW || type == GH_SPIRALARMS || type == GH_DEDISK ||
+         type == GH_INTERPSPH;
 }
 
 // run-time dispatch over every component type (extended instances)
@@ -1254,6 +1311,7 @@ __device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__
     case GH_TWOPOWERSPH: gh_comp_force<GH_TWOPOWERSPH>(T, i, g, F); break;
     case GH_SPIRALARMS: gh_comp_force<GH_SPIRALARMS>(T, i, g, F); break;
     case GH_DEDISK: gh_comp_force<GH_DEDISK>(T, i, g, F); break;
+    case GH_INTERPSPH: gh_comp_force<GH_INTERPSPH>(T, i, g, F); break;
     default: gh_comp_force_dyn(T, i, g, F); break;
   }
 }
@@ -1346,6 +1404,10 @@ __device__ __forceinline__ void gh_wrap_transform(const GhPotTable* __restrict__
 //     two passes at t = 0, the radial force at (R, 0, 0) and the vertical
 //     force at (Rp, 0, z); every map is linear in the force, so the passes'
 //     forces add;
+//   MovingObject (MovingObjectPotential.c:12-88): the wrapped potential at
+//     (Rdist, phi, z_obj - z), Rdist the cylindrical distance to the object
+//     (positions from natural splines over the normalised time); its radial
+//     force F' gives F = -F' (x_obj - x, y_obj - y) / Rdist, F_z = -F_z';
 //   DehnenSmooth, GaussianAmplitude, TimeDependentAmplitude: factors only.
 // rec (when not null) keeps what the way back needs: {cos, sin, d offset/dR,
 // x, y} for the rotations, {x, y, (a + s) z / s, xi} for KuzminLike, {x, y,
@@ -1407,6 +1469,26 @@ __device__ __forceinline__ void gh_wrap_forward(const GhPotTable* __restrict__ T
       }
       break;
     }
+    case GH_MOVINGOBJWRAP: {  // p: amp, t0, tf, spline offsets of x, y, z (z < 0: planar)
+      double d = (t - W->p[1]) / (W->p[2] - W->p[1]);
+      d = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+      const double dx = gh_spline_eval(T, (int)W->p[3], d) - x;
+      const double dy = gh_spline_eval(T, (int)W->p[4], d) - y;
+      const double oz = (W->p[5] >= 0.0) ? gh_spline_eval(T, (int)W->p[5], d) : 0.0;
+      const double R = sqrt(x * x + y * y), Rd = sqrt(dx * dx + dy * dy);
+      const double cp = (R > 0.0) ? x / R : 1.0, sp = (R > 0.0) ? y / R : 0.0;
+      if (rec) {
+        rec[0] = dx;
+        rec[1] = dy;
+        rec[2] = Rd;
+        rec[3] = cp;
+        rec[4] = sp;
+      }
+      x = Rd * cp;  // the wrapped potential at (Rdist, phi, z_obj - z)
+      y = Rd * sp;
+      z = oz - z;
+      break;
+    }
     case GH_KUZMINLIKEWRAP: {
       const double sz = sqrt(z * z + W->p[2]), as = W->p[1] + sz;
       const double xi = sqrt(x * x + y * y + as * as);
@@ -1464,6 +1546,14 @@ __device__ __forceinline__ void gh_wrap_backward(const GhWrap* __restrict__ W, i
         fx = gx;
         fy = gy;
       }
+      break;
+    }
+    case GH_MOVINGOBJWRAP: {  // MovingObjectPotential.c:12-88
+      const double FRw = rec[3] * fx + rec[4] * fy;  // wrapped radial force at (Rdist, phi)
+      const double f = -FRw / rec[2];
+      fx = f * rec[0];
+      fy = f * rec[1];
+      fz = -fz;
       break;
     }
     case GH_KUZMINLIKEWRAP: {  // the inner point is (xi, 0, 0): its radial force is fx
@@ -1780,6 +1870,13 @@ __device__ __forceinline__ double gh_comp_potential(const GhPotTable* __restrict
       pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
              cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
       break;
+    case GH_INTERPSPH: {  // interpSphericalPotential.c:5-20 (revaluate)
+      const double r = sqrt(r2);
+      const double v = (r >= C->p[2]) ? -C->p[3] / r + C->p[5]
+                       : (r < C->p[1]) ? 0.0 : -gh_spline_integ(T, C->ext, r) + C->p[4];
+      pot += C->p[0] * v;
+      break;
+    }
     case GH_DEDISK:  // DoubleExponentialDiskPotential.c:6-34 (Eval)
       pot += C->p[1] * gh_dedisk_sum<2>(T, C, 1.0 / R, fabs(z)) / R;
       break;

@@ -22,7 +22,7 @@ planarPotentialFromRZPotential / planarPotentialFromFullPotential).
 """
 
 import numpy
-from scipy import special
+from scipy import interpolate, special
 
 _CORE = 10**-8  # LogarithmicHaloPotential.py:12
 
@@ -1335,6 +1335,51 @@ class TwoPowerSphericalPotential(_SphericalPotential):
         return [self._amp, self.a, self.alpha, self.beta]
 
 
+class interpSphericalPotential(_SphericalPotential):
+    """A spherical potential from its tabulated radial force (interpSphericalPotential.py
+    :15-130); pot_type 36, args [nr, rgrid, rforce on rgrid, amp, rmin, rmax,
+    M(<rmax), Phi0, Phimax] (integrateFullOrbit.py:366-373).  ``rforce`` is a
+    function of r or a potential (then Phi0 is its value at rgrid[0]).  As in
+    galpy, the host evaluation interpolates with scipy's spline through the
+    grid, while the C (and GPU) evaluation uses GSL's natural cubic spline of
+    the same table; beyond rgrid[-1] the mass inside it acts as a point
+    mass, inside rgrid[0] the force is 0."""
+
+    pot_code = 36
+
+    def __init__(self, rforce=None, rgrid=numpy.geomspace(0.01, 20, 101), Phi0=None):
+        Potential.__init__(self, amp=1.0)
+        self._rgrid = numpy.array(rgrid, dtype=float)
+        if isinstance(rforce, (Potential, list, tuple)):
+            _rforce = lambda r: evaluateRforces(rforce, r, 0.0)
+            Phi0 = evaluatePotentials(rforce, self._rgrid[0], 0.0)
+        else:
+            _rforce = rforce
+            Phi0 = 0.0 if Phi0 is None else Phi0
+        self._rforce_grid = numpy.array([float(_rforce(r)) for r in self._rgrid])
+        self._force_spline = interpolate.InterpolatedUnivariateSpline(self._rgrid, self._rforce_grid, k=3, ext=0)
+        self._pot_spline = self._force_spline.antiderivative()
+        self._Phi0 = Phi0 + self._pot_spline(self._rgrid[0])
+        self._rmin = self._rgrid[0]
+        self._rmax = self._rgrid[-1]
+        self._total_mass = -(self._rmax**2.0) * self._force_spline(self._rmax)
+        self._Phimax = -self._pot_spline(self._rmax) + self._Phi0 + self._total_mass / self._rmax
+
+    def _dphi(self, r):
+        r = numpy.asarray(r, dtype=float)
+        return numpy.where(r >= self._rmax, self._total_mass / r**2.0, -self._force_spline(numpy.minimum(r, self._rmax)))
+
+    def _phi(self, r):
+        r = numpy.asarray(r, dtype=float)
+        return numpy.where(r >= self._rmax, -self._total_mass / r + self._Phimax,
+                           -self._pot_spline(numpy.minimum(r, self._rmax)) + self._Phi0)
+
+    def c_args(self):
+        return ([float(len(self._rgrid))] + list(self._rgrid) + list(self._rforce_grid)
+                + [self._amp, self._rmin, self._rmax, float(self._total_mass), float(self._Phi0),
+                   float(self._Phimax)])
+
+
 class FlattenedPowerPotential(Potential):
     """Phi = -amp / (alpha m^alpha), m^2 = R^2 + z^2/q^2 + core^2 (amp/2 log m^2
     at alpha = 0); pot_type 12, args [amp, alpha, q^2, core^2], planar
@@ -1996,6 +2041,70 @@ class TimeDependentAmplitudeWrapperPotential(WrapperPotential):
         return [self._A]
 
 
+class MovingObjectPotential(WrapperPotential):
+    """The potential of an object moving on a given orbit
+    (MovingObjectPotential.py:20-150): the axisymmetric potential ``pot``
+    (default a Plummer sphere, amp 0.06, b 0.01) centred on the orbit's
+    position at each time; pot_type -6, own arguments [n, t[n], x[n], y[n],
+    z[n], amp, t[0], t[-1]] (planar: without z; integrateFullOrbit.py:449-463,
+    integratePlanarOrbit.py:570-588).  ``orbit``: an integrated Orbit (its
+    stored times and positions are the spline's knots).  galpy's C, and so
+    the GPU, interpolates the position with natural cubic splines over the
+    normalised time, clamped at the ends; the host evaluation here uses the
+    same splines."""
+
+    pot_code = -6
+
+    def __init__(self, orbit, pot=None, amp=1.0):
+        if pot is None:
+            pot = PlummerPotential(amp=0.06, b=0.01)
+        if any(q.isNonAxi for q in flatten(pot)):
+            raise NotImplementedError("MovingObjectPotential for non-axisymmetric potentials is not supported")
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        self._t = numpy.array(orbit.t, dtype=float)
+        self._x = numpy.array(orbit.x(self._t), dtype=float).reshape(-1)
+        self._y = numpy.array(orbit.y(self._t), dtype=float).reshape(-1)
+        self._z = numpy.array(orbit.z(self._t), dtype=float).reshape(-1)
+        tt = (self._t - self._t[0]) / (self._t[-1] - self._t[0])
+        self._spl = [interpolate.CubicSpline(tt, v, bc_type="natural") for v in (self._x, self._y, self._z)]
+        self.isNonAxi = True
+
+    def _pos(self, t):
+        d = numpy.clip((numpy.asarray(t, dtype=float) - self._t[0]) / (self._t[-1] - self._t[0]), 0.0, 1.0)
+        return [s(d) for s in self._spl]
+
+    def _rel(self, R, z, phi, t):
+        ox, oy, oz = self._pos(t)
+        x, y = R * numpy.cos(phi), R * numpy.sin(phi)
+        return x, y, ox, oy, oz, numpy.sqrt((x - ox) ** 2 + (y - oy) ** 2)
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        x, y, ox, oy, oz, Rd = self._rel(R, z, phi, t)
+        return evaluatePotentials(self._pot, Rd, oz - z, phi, t)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        x, y, ox, oy, oz, Rd = self._rel(R, z, phi, t)
+        RF = evaluateRforces(self._pot, Rd, oz - z, phi, t)
+        return -RF * (numpy.cos(phi) * (ox - x) + numpy.sin(phi) * (oy - y)) / Rd
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        x, y, ox, oy, oz, Rd = self._rel(R, z, phi, t)
+        return -evaluatezforces(self._pot, Rd, oz - z, phi, t)
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        x, y, ox, oy, oz, Rd = self._rel(R, z, phi, t)
+        RF = evaluateRforces(self._pot, Rd, oz - z, phi, t)
+        return -RF * R * (numpy.cos(phi) * (oy - y) - numpy.sin(phi) * (ox - x)) / Rd
+
+    def wrapper_args(self):
+        return ([float(len(self._t))] + list(self._t) + list(self._x) + list(self._y) + list(self._z)
+                + [self._amp, self._t[0], self._t[-1]])
+
+    def wrapper_args_planar(self):
+        return ([float(len(self._t))] + list(self._t) + list(self._x) + list(self._y)
+                + [self._amp, self._t[0], self._t[-1]])
+
+
 def MWPotential2014_factory():
     """mwpotentials.py:25-29 (Bovy 2014, Table 1)."""
     return CompositePotential(
@@ -2041,7 +2150,8 @@ def _pack(comps, planar, tfuncs=None):
             pot_type.extend(wt)
             pot_args.append(float(_npot(inner)))
             pot_args.extend(wa)
-            pot_args.extend(p.wrapper_args())
+            pot_args.extend(p.wrapper_args_planar() if planar and hasattr(p, "wrapper_args_planar")
+                            else p.wrapper_args())
             tfuncs.extend(getattr(p, "wrapper_tfuncs", lambda: [])())
             continue
         if isinstance(p, MN3ExponentialDiskPotential):

@@ -24,6 +24,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
+#include <vector>
 #include <cfloat>
 #include <algorithm>
 
@@ -238,7 +240,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
-               kTimeDepAmpWrap = -9,
+               kTimeDepAmpWrap = -9, kMovingObjectWrap = -6, kInterpSph = 36,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
                kTriaxHernquist = 21, kTriaxNFW = 22, kTriaxJaffe = 23, kPerfectEllipsoid = 30,
                kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43,
@@ -255,6 +257,73 @@ inline bool is_ellipsoidal(int code) {
 // accept up to this many).
 constexpr int kOracleMaxComp = 16;
 
+// GSL's natural cubic spline (gsl_interp_cspline, the interpolation galpy's C
+// builds with gsl_spline_init; GSL is absent here), restated from its
+// published definition: with c_i = y''(x_i)/2, c_0 = c_{n-1} = 0 and, for
+// 0 < i < n-1, h_{i-1} c_{i-1} + 2 (h_{i-1} + h_i) c_i + h_i c_{i+1} =
+// 3 ((y_{i+1} - y_i)/h_i - (y_i - y_{i-1})/h_{i-1}) (solved by Thomas'
+// algorithm); on [x_i, x_{i+1}], with d = x - x_i,
+//   y = y_i + d (b_i + d (c_i + d d_i)), b_i = (y_{i+1} - y_i)/h_i -
+//   h_i (c_{i+1} + 2 c_i)/3, d_i = (c_{i+1} - c_i)/(3 h_i);
+// integrals term by term.  Pinned against scipy's CubicSpline(bc_type=
+// 'natural').
+struct NatSpline {
+  std::vector<double> x, y, b, c, d;
+  void build(const double* xs, const double* ys, int n) {
+    x.assign(xs, xs + n);
+    y.assign(ys, ys + n);
+    c.assign(n, 0.);
+    b.assign(n, 0.);
+    d.assign(n, 0.);
+    if (n >= 3) {
+      const int m = n - 2;
+      std::vector<double> diag(m), off(m), rhs(m);
+      for (int i = 1; i <= m; i++) {
+        const double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+        diag[i - 1] = 2. * (h0 + h1);
+        off[i - 1] = h1;
+        rhs[i - 1] = 3. * ((y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0);
+      }
+      for (int i = 1; i < m; i++) {  // forward elimination
+        const double w = off[i - 1] / diag[i - 1];
+        diag[i] -= w * off[i - 1];
+        rhs[i] -= w * rhs[i - 1];
+      }
+      c[m] = rhs[m - 1] / diag[m - 1];
+      for (int i = m - 2; i >= 0; i--) c[i + 1] = (rhs[i] - off[i] * c[i + 2]) / diag[i];
+    }
+    for (int i = 0; i + 1 < n; i++) {
+      const double h = x[i + 1] - x[i];
+      b[i] = (y[i + 1] - y[i]) / h - h * (c[i + 1] + 2. * c[i]) / 3.;
+      d[i] = (c[i + 1] - c[i]) / (3. * h);
+    }
+  }
+  // interval of t: the largest i with x[i] <= t, within [0, n-2]
+  int find(double t) const {
+    int lo = 0, hi = (int)x.size() - 1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (x[mid] > t) hi = mid; else lo = mid;
+    }
+    return lo;
+  }
+  template <class R> R eval(R t) const {
+    const int i = find(dval(t));
+    R dx = t - R(x[i]);
+    return R(y[i]) + dx * (R(b[i]) + dx * (R(c[i]) + dx * R(d[i])));
+  }
+  // integral over [x[0], t]
+  template <class R> R integ(R t) const {
+    const int k = find(dval(t));
+    R sum = 0.;
+    for (int i = 0; i <= k; i++) {
+      R u = (i < k) ? R(x[i + 1] - x[i]) : t - R(x[i]);
+      sum += u * (R(y[i]) + u * (R(b[i]) / R(2.) + u * (R(c[i]) / R(3.) + u * R(d[i]) / R(4.))));
+    }
+    return sum;
+  }
+};
+
 struct Component {
   int type;
   double p[16];  // raw arguments in galpy's order
@@ -268,6 +337,9 @@ struct Component {
   const double* args = nullptr;
   // a wrapper's time function (potentialArg.tfuncs, galpy_potentials.h)
   double (*tfunc)(double) = nullptr;
+  // cubic splines a component interpolates (potentialArg.spline1d): built
+  // at parse time, shared by copies of the component
+  std::shared_ptr<std::vector<NatSpline>> spl;
 };
 
 // The caller's time functions (galpy's pot_tfuncs), consumed in parse order
@@ -279,7 +351,7 @@ inline int g_ntfuncs = 0;
 inline bool is_wrapper(int code) {
   return code == kDehnenSmoothWrap || code == kSolidBodyRotWrap || code == kCorotatingRotWrap ||
          code == kGaussianAmpWrap || code == kRotateTiltWrap || code == kKuzminLikeWrap ||
-         code == kCylSepWrap || code == kTimeDepAmpWrap;
+         code == kCylSepWrap || code == kTimeDepAmpWrap || code == kMovingObjectWrap;
 }
 
 // nargs per code, as consumed from pot_args (integrateFullOrbit.c:80-345;
@@ -333,6 +405,12 @@ inline int spiralarms_nargs(const double* a) { return 10 + (int)a[0]; }
 // then the J0 nodes, J1 nodes, J0 weights, J1 weights, de_n each
 // (integrateFullOrbit.py:93-107).
 inline int dedisk_nargs(const double* a) { return 5 + 4 * (int)a[4]; }
+// interpSphericalPotential: nr, rgrid[nr], rforce[nr], amp, rmin, rmax,
+// M(<rmax), Phi0, Phimax (integrateFullOrbit.py:366-373)
+inline int interpsph_nargs(const double* a) { return 1 + 2 * (int)a[0] + 6; }
+// MovingObjectPotential's own arguments: nPts, t[nPts], x, y[, z], amp, t0,
+// tf (integrateFullOrbit.py:449-463; the planar packing has no z)
+inline int movingobject_nargs(const double* a, bool planar) { return 1 + (planar ? 3 : 4) * (int)a[0] + 3; }
 
 inline int ellipsoidal_nargs(const double* a) {
   const int npsi = (int)a[16];
@@ -353,6 +431,8 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
     int na = is_ellipsoidal(code)    ? ellipsoidal_nargs(pot_args)
              : code == kSpiralArms ? spiralarms_nargs(pot_args)
              : code == kDoubleExpDisk ? dedisk_nargs(pot_args)
+             : code == kInterpSph ? interpsph_nargs(pot_args)
+             : code == kMovingObjectWrap ? 0  // counted after its wrapped components
                                    : component_nargs(code, planar);
     if (na < 0) return -1;
     Component& C = dst[i];
@@ -371,6 +451,29 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       C.wrapped = kids;
     }
     C.args = pot_args;
+    C.spl.reset();
+    if (code == kInterpSph) {  // interpSphericalPotential's force spline
+      const int nr = (int)pot_args[0];
+      C.spl = std::make_shared<std::vector<NatSpline>>(1);
+      (*C.spl)[0].build(pot_args + 1, pot_args + 1 + nr, nr);
+      for (int j = 0; j < 6; j++) C.p[j] = pot_args[1 + 2 * nr + j];  // amp, rmin, rmax, Mmax, Phi0, Phimax
+      pot_args += na;
+      continue;
+    }
+    if (code == kMovingObjectWrap) {  // initMovingObjectSplines, integrateFullOrbit.c:1392-1431
+      const int n = (int)pot_args[0], nc = planar ? 2 : 3;
+      const double* tarr = pot_args + 1;
+      const double to = tarr[(nc + 1) * n + 1], tf = tarr[(nc + 1) * n + 2];
+      std::vector<double> tt(n);
+      for (int j = 0; j < n; j++) tt[j] = (tarr[j] - to) / (tf - to);
+      C.spl = std::make_shared<std::vector<NatSpline>>(nc);
+      for (int k = 0; k < nc; k++) (*C.spl)[k].build(tt.data(), tarr + (k + 1) * n, n);
+      C.p[0] = tarr[(nc + 1) * n];  // amp
+      C.p[1] = to;
+      C.p[2] = tf;
+      pot_args += movingobject_nargs(pot_args, planar);
+      continue;
+    }
     if (is_ellipsoidal(code) || code == kSpiralArms || code == kDoubleExpDisk) {
       C.p[0] = pot_args[0];
       pot_args += na;
@@ -1027,6 +1130,32 @@ template <class R> R ellip_eval(const Component& C, R Rc, R z, R phi) {
 // Aggregated cylindrical forces: galpy_potentials.c:120-179 sums each of
 // Rforce, phitorque, zforce over components in index order starting from 0.
 // Summing the three per component gives the identical three sums.
+// interpSphericalPotential.c:5-37 with SphericalPotential.c:4-38: the radial
+// force is the natural spline of the tabulated force on [rmin, rmax), 0
+// inside rmin and -M/r^2 outside rmax; the potential is minus the spline's
+// integral from rmin plus Phi0, and -M/r + Phimax outside rmax.
+// p: amp, rmin, rmax, M(<rmax), Phi0, Phimax.
+template <class R> R interpsph_rforce(const Component& C, R r) {
+  if (r >= R(C.p[2])) return -R(C.p[3]) / r / r;
+  return r < R(C.p[1]) ? R(0.) : (*C.spl)[0].eval(r);
+}
+template <class R> R interpsph_reval(const Component& C, R r) {
+  if (r >= R(C.p[2])) return -R(C.p[3]) / r + R(C.p[5]);
+  return r < R(C.p[1]) ? R(0.) : -(*C.spl)[0].integ(r) + R(C.p[4]);
+}
+// MovingObjectPotential.c:12-120: the object's position from natural splines
+// of its orbit over the normalised time (t - t0)/(tf - t0), clamped to
+// [0, 1]; the wrapped (axisymmetric) potential at the cylindrical distance
+// and the height difference.  p: amp, t0, tf.
+template <class R> void movingobject_pos(const Component& C, R t, R& ox, R& oy, R& oz) {
+  R d = (t - R(C.p[1])) / R(C.p[2] - C.p[1]);
+  if (d < R(0.)) d = 0.;
+  if (d > R(1.)) d = 1.;
+  ox = (*C.spl)[0].eval(d);
+  oy = (*C.spl)[1].eval(d);
+  oz = C.spl->size() > 2 ? (*C.spl)[2].eval(d) : R(0.);
+}
+
 // KuzminLikeWrapperPotential.c:5-8: xi = sqrt(R^2 + (a + sqrt(z^2 + b^2))^2)
 template <class R> R kuzminlike_xi(R Rc, R z, double a, double b2) {
   R asqrtbz = R(a) + sqrt(z * z + R(b2));
@@ -1102,6 +1231,13 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kSoftNeedleBar: needle_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kSpiralArms: spiral_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kDoubleExpDisk: dedisk_forces(C, Rc, z, fr, fz); break;
+      case kInterpSph: {
+        R r = sqrt(Rc * Rc + z * z);
+        R f = R(C.p[0]) * interpsph_rforce(C, r);
+        fr = f * Rc / r;
+        fz = f * z / r;
+        break;
+      }
       case kFlattenedPower: {  // FlattenedPowerPotential.c Rforce, zforce
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.) {
@@ -1120,6 +1256,16 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
           ellip_forces(C, Rc, z, phi, fr, fz, fp);
         } else if (is_spherical_new(C.type)) {
           sph_forces(C, Rc, z, fr, fz);
+        } else if (C.type == kMovingObjectWrap) {
+          R ox, oy, oz;
+          movingobject_pos(C, t, ox, oy, oz);
+          R x = Rc * cos(phi), y = Rc * sin(phi);
+          R Rdist = pow(pow(x - ox, R(2)) + pow(y - oy, R(2)), R(0.5));
+          R wr, wz, wp;
+          cyl_forces(C.nwrapped, C.wrapped, Rdist, oz - z, phi, t, wr, wz, wp);
+          fr = -R(C.p[0]) * wr * (cos(phi) * (ox - x) + sin(phi) * (oy - y)) / Rdist;
+          fz = -R(C.p[0]) * wz;
+          fp = -R(C.p[0]) * wr * Rc * (cos(phi) * (oy - y) - sin(phi) * (ox - x)) / Rdist;
         } else if (C.type == kRotateTiltWrap) {
           rottilt_forces(C, Rc, z, phi, t, fr, fz, fp);
         } else if (C.type == kKuzminLikeWrap) {
@@ -1692,6 +1838,7 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
         break;
       }
       case kDoubleExpDisk: fr = dedisk_planar_Rforce(C, Rc); break;
+      case kInterpSph: fr = R(C.p[0]) * interpsph_rforce(C, Rc); break;  // SphericalPotential.c:22-29
       case kSpiralArms: {  // SpiralArmsPotential.c:625-716: the full forces at z = 0
         R fz0;
         spiral_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
@@ -1708,6 +1855,15 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
           R Phip, Phipp;
           sph_dphi(C, Rc, Phip, Phipp);
           fr = -Phip;
+        } else if (C.type == kMovingObjectWrap) {  // MovingObjectPotential.c:89-130
+          R ox, oy, oz;
+          movingobject_pos(C, t, ox, oy, oz);
+          R x = Rc * cos(phi), y = Rc * sin(phi);
+          R Rdist = pow(pow(x - ox, R(2)) + pow(y - oy, R(2)), R(0.5));
+          R wr, wp;
+          planar_forces(C.nwrapped, C.wrapped, Rdist, phi, t, wr, wp);
+          fr = -R(C.p[0]) * wr * (cos(phi) * (ox - x) + sin(phi) * (oy - y)) / Rdist;
+          fp = -R(C.p[0]) * wr * Rc * (cos(phi) * (oy - y) - sin(phi) * (ox - x)) / Rdist;
         } else if (C.type == kKuzminLikeWrap) {  // KuzminLikeWrapperPotential.c:93-107
           const double amp = C.p[0], a = C.p[1], b2 = C.p[2];
           R xi = kuzminlike_xi(Rc, R(0.), a, b2);
@@ -1852,6 +2008,7 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kSoftNeedleBar: pot += needle_eval(C, Rc, z, phi, t); break;
       case kSpiralArms: pot += spiral_eval(C, Rc, z, phi, t); break;
       case kDoubleExpDisk: pot += dedisk_eval(C, Rc, z); break;
+      case kInterpSph: pot += R(C.p[0]) * interpsph_reval(C, sqrt(Rc * Rc + z * z)); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Eval
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];
         if (alpha == 0.)
@@ -1866,6 +2023,12 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
           pot += ellip_eval(C, Rc, z, phi);
         } else if (is_spherical_new(C.type)) {
           pot += sph_eval(C, sqrt(Rc * Rc + z * z));
+        } else if (C.type == kMovingObjectWrap) {  // MovingObjectPotential.py _evaluate
+          R ox, oy, oz;
+          movingobject_pos(C, t, ox, oy, oz);
+          R x = Rc * cos(phi), y = Rc * sin(phi);
+          pot += R(C.p[0]) * potential_value(C.nwrapped, C.wrapped, sqrt((x - ox) * (x - ox) + (y - oy) * (y - oy)),
+                                             oz - z, phi, t);
         } else if (C.type == kRotateTiltWrap) {  // RotateAndTiltWrapperPotential.py _evaluate
           R xp, yp, zp;
           rottilt_point(C, Rc, z, phi, xp, yp, zp);

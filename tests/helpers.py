@@ -64,6 +64,10 @@ def _nargs(code, pot_args, ai, planar):
         return 10 + int(pot_args[ai])
     if code == 11:  # DoubleExponentialDiskPotential: 5 + 4 de_n
         return 5 + 4 * int(pot_args[ai + 4])
+    if code == 36:  # interpSphericalPotential: 1 + 2 nr + 6
+        return 1 + 2 * int(pot_args[ai]) + 6
+    if code == -6:  # MovingObjectPotential's own: 1 + (3|4) n + 3
+        return 1 + (3 if planar else 4) * int(pot_args[ai]) + 3
     return 3 if (planar and code in (15, 12)) else _NARGS[code]
 
 
