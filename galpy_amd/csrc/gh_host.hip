@@ -498,6 +498,36 @@ int decode_interpsph(const double*& pot_args, GhPotTable* T, int i) {
   return 0;
 }
 
+// SCFPotential (integratePlanarOrbit.py:726-762, the same in both
+// packings): a, isNonAxi, N, L, M, Nt, the coefficient blocks (static
+// Acos[, Asin] or tgrid and the time-cubic blocks), 11 cache slots; the
+// blocks are copied to ext as they are.
+int decode_scf(const double*& pot_args, GhPotTable* T, int i) {
+  GhComponent& C = T->comp[i];
+  const double* a = pot_args;
+  const int nonaxi = (int)a[1], N = (int)a[2], L = (int)a[3], M = (int)a[4], Nt = (int)a[5];
+  const long NLM = (long)N * L * M;
+  const long nblk = Nt == 0 ? (1 + nonaxi) * NLM : Nt + (1 + nonaxi) * NLM * 4 * (Nt - 1);
+  if (N < 1 || L < 1 || M < 1 || Nt == 1 || Nt < 0 || (nonaxi && M != L)) {
+    set_error("SCFPotential N=%d L=%d M=%d Nt=%d isNonAxi=%d: unsupported shape", N, L, M, Nt, nonaxi);
+    return -1;
+  }
+  if (T->next + nblk > GH_EXT_LEN) {
+    set_error("SCFPotential coefficients (%ld doubles) do not fit the potential table", nblk);
+    return -1;
+  }
+  C.type = GH_SCF;
+  for (int j = 0; j < 6; j++) C.p[j] = a[j];
+  C.ext = T->next;
+  for (long k = 0; k < nblk; k++) T->ext[T->next + k] = a[6 + k];
+  T->next += (int)nblk;
+  C.nonaxi = nonaxi;
+  T->nonaxi |= nonaxi;
+  T->has_wrap = 1;  // extended dispatch instances
+  pot_args += 6 + nblk + 11;
+  return 0;
+}
+
 // One (leaf) component's arguments and precomputed constants.
 int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
                      bool planar) {
@@ -505,6 +535,7 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
   if (code == GH_SPIRALARMS) return decode_spiralarms(pot_args, T, i);
   if (code == GH_DEDISK) return decode_dedisk(pot_args, T, i, planar);
   if (code == GH_INTERPSPH) return decode_interpsph(pot_args, T, i);
+  if (code == GH_SCF) return decode_scf(pot_args, T, i);
   const int na = nargs_of(code, planar);
   if (na < 0 || code < 0) {
     set_error("pot_type %d (component %d) has no HIP kernel", code, i);

@@ -88,6 +88,7 @@ enum GhPotCode {
   GH_TIMEDEPAMPWRAP = -9,
   GH_MOVINGOBJWRAP = -6,
   GH_INTERPSPH = 36,
+  GH_SCF = 24,
 };
 
 struct GhComponent {
@@ -1246,6 +1247,150 @@ __device__ __forceinline__ void gh_comp_force<GH_INTERPSPH>(const GhPotTable* __
   F.Fz += f * g.z;
 }
 
+// SCFPotential.c (Hernquist & Ostriker 1992 basis): the sums of
+// sum_spher_forces / sum_expansion (:366-438) without the per-call arrays:
+// the outer loop runs over m (cos m phi, sin m phi and P_m^m by recurrence
+// from the previous m), the next over l >= m (P_l^m and dP_l^m/dx by the
+// three-term recurrence, (a r / (a + r)^2)^l by products), the inner over n,
+// where C_n^(2l+3/2)(xi) and C_(n-1)^(2l+5/2)(xi) (dC/dxi / (2 alpha)) are
+// run by their recurrences.  Coefficients (ext): Acos[N L M] and Asin, or,
+// for a time-dependent expansion, tgrid[Nt] and per coefficient the cubic in
+// t - tgrid[i] (Horner).  p: a, isNonAxi, N, L, M, Nt.
+// Returns (dPhi/dr, dPhi/dtheta, dPhi/dphi) with the reference's signs in
+// F[0..2] (minus, minus, plus) and the potential.
+template <bool POT>
+__device__ inline void gh_scf_sums(const GhPotTable* __restrict__ T,
+                                   const GhComponent* __restrict__ C, double R, double z,
+                                   double cphi, double sphi, double t, double* F, double& V) {
+  const double a = C->p[0];
+  const bool nonaxi = C->p[1] != 0.0;
+  const int N = (int)C->p[2], L = (int)C->p[3], M = (int)C->p[4], Nt = (int)C->p[5];
+  const int NLM = N * L * M;
+  const double* __restrict__ e = &T->ext[C->ext];
+  int it = 0;
+  double dt = 0.0;
+  const double* __restrict__ ca = e;
+  const double* __restrict__ cs_ = e + NLM;
+  if (Nt > 0) {  // scf_find_time_interval, clamped to the boundary intervals
+    const double* __restrict__ tg = e;
+    if (t <= tg[0]) it = 0;
+    else if (t >= tg[Nt - 1]) it = Nt - 2;
+    else {
+      int lo = 0, hi = Nt - 2;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tg[mid] <= t) lo = mid;
+        else hi = mid - 1;
+      }
+      it = lo;
+    }
+    dt = t - tg[it];
+    ca = e + Nt;
+    cs_ = ca + (size_t)NLM * 4 * (Nt - 1);
+  }
+  const double r2 = R * R + z * z, r = sqrt(r2);
+  const double x = z / r, sth = R / r;
+  const double ar = a + r, xi = (r - a) / ar, q = r * a / (ar * ar);
+  const double iar = 1.0 / ar, ird = 1.0 / (r * ar * ar * ar), ix2m1 = 1.0 / (x * x - 1.0);
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0, pot = 0.0;
+  const int Meff = nonaxi ? (M < L ? M : L) : 1;
+  double cm = 1.0, sm = 0.0, pmm = 1.0, qm = 1.0;
+  for (int m = 0; m < Meff; m++) {
+    if (m > 0) {
+      const double c2 = cm * cphi - sm * sphi;
+      sm = sm * cphi + cm * sphi;
+      cm = c2;
+      pmm *= -(2.0 * m - 1.0) * sth;
+      qm *= q;
+    }
+    double plm1 = 0.0, plm2 = 0.0, ql = qm;
+    for (int l = m; l < L; l++) {
+      double pl;
+      if (l == m) pl = pmm;
+      else if (l == m + 1) pl = (2.0 * m + 1.0) * x * pmm;
+      else pl = ((2.0 * l - 1.0) * x * plm1 - (l + m - 1.0) * plm2) / (l - m);
+      const double dpl = (l == 0) ? 0.0 : (l * x * pl - (l + m) * ((l == m) ? 0.0 : plm1)) * ix2m1;
+      if (l > m) plm2 = plm1;
+      plm1 = pl;
+      const double rt = -ql * iar, rd = ql * ird;
+      const double lin = (2 * l + 1) * r * ar - l * ar * ar;
+      const double alpha = 2.0 * l + 1.5;
+      double c0 = 1.0, c1 = 0.0, h0 = 1.0, h1 = 0.0;  // C_n^alpha, C_(n-1)^(alpha+1)
+      for (int n = 0; n < N; n++) {
+        double Cn, Hn;  // C_n^alpha(xi), C_(n-1)^(alpha+1)(xi)
+        if (n == 0) {
+          Cn = 1.0;
+          Hn = 0.0;
+        } else if (n == 1) {
+          Cn = 2.0 * alpha * xi;
+          Hn = 1.0;
+        } else {
+          Cn = (2.0 * (n - 1.0 + alpha) * xi * c0 - (n - 2.0 + 2.0 * alpha) * c1) / n;
+          const double k = n - 1;  // C_k^(alpha+1) from C_(k-1), C_(k-2)
+          Hn = (k == 1) ? 2.0 * (alpha + 1.0) * xi
+                        : (2.0 * (k - 1.0 + alpha + 1.0) * xi * h0 - (k - 2.0 + 2.0 * (alpha + 1.0)) * h1) / k;
+        }
+        if (n >= 1) {
+          c1 = c0;
+          c0 = Cn;
+        } else {
+          c0 = Cn;
+        }
+        if (n >= 2) {
+          h1 = h0;
+          h0 = Hn;
+        } else if (n == 1) {
+          h0 = Hn;
+        }
+        const int ci = n * L * M + l * M + m;
+        double Ac, As = 0.0;
+        if (Nt == 0) {
+          Ac = ca[ci];
+          if (nonaxi) As = cs_[ci];
+        } else {
+          const double* cp = ca + ((size_t)it * NLM + ci) * 4;
+          Ac = fma(fma(fma(cp[0], dt, cp[1]), dt, cp[2]), dt, cp[3]);
+          if (nonaxi) {
+            const double* sp = cs_ + ((size_t)it * NLM + ci) * 4;
+            As = fma(fma(fma(sp[0], dt, sp[1]), dt, sp[2]), dt, sp[3]);
+          }
+        }
+        const double csum = Ac * cm + As * sm;
+        const double phT = rt * Cn;
+        if (POT) {
+          pot += csum * pl * phT;
+        } else {
+          const double sdiff = Ac * sm - As * cm;
+          const double dC = 2.0 * alpha * Hn;
+          const double dphT = rd * (lin * Cn - 2.0 * a * r * dC);
+          F0 -= csum * pl * dphT;
+          F1 -= csum * dpl * phT;
+          F2 += m * sdiff * pl * phT;
+        }
+      }
+      ql *= q;
+    }
+  }
+  const double s4 = 3.5449077018110320546;  // sqrt(4 pi)
+  F[0] = F0 * s4;
+  F[1] = F1 * s4 * (-sth);
+  F[2] = F2 * s4;
+  V = pot * s4;
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_SCF>(const GhPotTable* __restrict__ T, int i,
+                                                      const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double R = sqrt(g.R2);
+  const double cp = (R > 0.0) ? g.x / R : 1.0, sp = (R > 0.0) ? g.y / R : 0.0;
+  double S[3], V;
+  gh_scf_sums<false>(T, C, R, g.z, cp, sp, g.t, S, V);
+  const double ir2 = g.rinv * g.rinv, iR = 1.0 / R;
+  F.FRoR += (S[0] * g.rinv + S[1] * g.z * ir2 * iR);  // F_R / R
+  F.Fz += S[0] * g.z * g.rinv - S[1] * R * ir2;
+  F.FphioR2 += S[2] * iR * iR;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_NULLPOT>(const GhPotTable* __restrict__, int,
                                                           const GhGeom&, GhAcc&) {}
@@ -1292,7 +1437,7 @@ __host__ __device__ inline bool gh_is_heavy(int type) {
          type == GH_TWOPOWERTRIAX || type == GH_PERFECTELLIPSOID || type == GH_TRIAXGAUSSIAN ||
          type == GH_POWERTRIAX || type == GH_EINASTO || type == GH_TWOPOWERSPH ||
          type == GH_EXPTRUNCNFW || type == GH_SPIRALARMS || type == GH_DEDISK ||
-         type == GH_INTERPSPH;
+         type == GH_INTERPSPH || type == GH_SCF;
 }
 
 // run-time dispatch over every component type (extended instances)
@@ -1312,6 +1457,7 @@ __device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__
     case GH_SPIRALARMS: gh_comp_force<GH_SPIRALARMS>(T, i, g, F); break;
     case GH_DEDISK: gh_comp_force<GH_DEDISK>(T, i, g, F); break;
     case GH_INTERPSPH: gh_comp_force<GH_INTERPSPH>(T, i, g, F); break;
+    case GH_SCF: gh_comp_force<GH_SCF>(T, i, g, F); break;
     default: gh_comp_force_dyn(T, i, g, F); break;
   }
 }
@@ -1870,6 +2016,12 @@ __device__ __forceinline__ double gh_comp_potential(const GhPotTable* __restrict
       pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
              cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
       break;
+    case GH_SCF: {  // SCFPotential.c:828-863 (Eval)
+      double S[3], V;
+      gh_scf_sums<true>(T, C, R, z, cosphi, sinphi, t, S, V);
+      pot += V;
+      break;
+    }
     case GH_INTERPSPH: {  // interpSphericalPotential.c:5-20 (revaluate)
       const double r = sqrt(r2);
       const double v = (r >= C->p[2]) ? -C->p[3] / r + C->p[5]

@@ -1380,6 +1380,140 @@ class interpSphericalPotential(_SphericalPotential):
                    float(self._Phimax)])
 
 
+def sph_harm_normalization(L, M):
+    """beta_lm = sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) (2 - delta_m0) for
+    0 <= l < L, 0 <= m < M, zero where m > l (galpy/util/special.py:122-160)."""
+    l = numpy.arange(L)[:, None]
+    m = numpy.arange(M)[None, :]
+    with numpy.errstate(invalid="ignore"):
+        ln = 0.5 * (numpy.log(2 * l + 1.0) - numpy.log(4.0 * numpy.pi) + special.gammaln(l - m + 1.0)
+                    - special.gammaln(l + m + 1.0))
+    out = numpy.exp(ln) * numpy.where(m == 0, 1.0, 2.0)
+    out[m > l] = 0.0
+    return out
+
+
+class SCFPotential(Potential):
+    """Hernquist & Ostriker (1992) basis-function expansion (SCFPotential.py
+    :20-330): Phi = sum_nlm N_lm P_l^m(cos theta) Phi_nl(r) (Acos cos m phi +
+    Asin sin m phi), Phi_nl = -sqrt(4 pi) (a r)^l / (a + r)^(2l+1)
+    C_n^(2l+3/2)((r - a)/(r + a)); amp is halved so that Acos = [[[1]]] is
+    galpy's Hernquist potential of the same amp.  pot_type 24, args [a,
+    isNonAxi, N, L, M, Nt, amp Acos N_lm (and Asin), 11 cache slots] or, with
+    a time grid, tgrid and the coefficients' cubic splines in time as
+    [Nt-1][NLM][4] blocks (integratePlanarOrbit.py:726-762).  ``Acos``/
+    ``Asin``: (N, L, M) arrays, or with ``tgrid`` (Nt, N, L, M) arrays or
+    callables of t."""
+
+    pot_code = 24
+
+    def __init__(self, amp=1.0, Acos=numpy.array([[[1.0]]]), Asin=None, a=1.0, tgrid=None, normalize=False):
+        Potential.__init__(self, amp=amp / 2.0)
+        self._a = a
+        self._tdep = tgrid is not None
+        if self._tdep:
+            from scipy.interpolate import CubicSpline
+
+            self._tgrid = numpy.array(tgrid, dtype=float)
+            Ac = numpy.array([Acos(t) for t in self._tgrid]) if callable(Acos) else numpy.array(Acos, dtype=float)
+            As = None if Asin is None else (numpy.array([Asin(t) for t in self._tgrid]) if callable(Asin)
+                                            else numpy.array(Asin, dtype=float))
+            Nt, N, L, M = Ac.shape
+            self.isNonAxi = not (As is None or L == 1 or (numpy.all(Ac[..., 1:] == 0) and numpy.all(As == 0)))
+            NN = sph_harm_normalization(L, M)
+            self._Acos_all = Ac * NN[None, None]
+            self._Asin_all = numpy.zeros_like(self._Acos_all) if As is None else As * NN[None, None]
+            self._coeff_shape = (N, L, M)
+            self._Acos_interp = CubicSpline(self._tgrid, self._Acos_all.reshape(Nt, -1))
+            self._Asin_interp = CubicSpline(self._tgrid, self._Asin_all.reshape(Nt, -1))
+        else:
+            Ac = numpy.array(Acos, dtype=float)
+            N, L, M = Ac.shape
+            self.isNonAxi = not (Asin is None or L == 1 or (numpy.all(Ac[:, :, 1:] == 0)
+                                                            and numpy.all(numpy.asarray(Asin) == 0)))
+            NN = sph_harm_normalization(L, M)
+            self._Acos = Ac * NN[None]
+            self._Asin = numpy.zeros_like(self._Acos) if Asin is None else numpy.asarray(Asin, dtype=float) * NN[None]
+            self._coeff_shape = (N, L, M)
+        _set_normalize(self, normalize)
+
+    def _coeffs(self, t):
+        if not self._tdep:
+            return self._Acos, self._Asin
+        N, L, M = self._coeff_shape
+        return (self._Acos_interp(float(t)).reshape(N, L, M), self._Asin_interp(float(t)).reshape(N, L, M))
+
+    def _spher(self, R, z, phi, t):
+        """Phi and its derivatives in (r, theta, phi) at scalar points."""
+        a = self._a
+        r = numpy.sqrt(R * R + z * z)
+        th = numpy.arctan2(R, z)
+        x = numpy.cos(th)
+        xi = (r - a) / (r + a)
+        Ac, As = self._coeffs(t)
+        N, L, M = self._coeff_shape
+        pot = dr = dth = dph = 0.0
+        for l in range(L):
+            base = -((a * r) ** l) / (a + r) ** (2 * l + 1)
+            dbase = base * (l / r - (2 * l + 1) / (a + r))
+            for m in range(min(l + 1, M) if self.isNonAxi else 1):
+                P = special.lpmv(m, l, x)
+                Pm1 = special.lpmv(m, l - 1, x) if l - 1 >= m else 0.0
+                dPdx = (l * x * P - (l + m) * Pm1) / (x * x - 1.0) if l > 0 else 0.0
+                c, sn = numpy.cos(m * phi), numpy.sin(m * phi)
+                for n in range(N):
+                    A = Ac[n, l, m] * c + As[n, l, m] * sn
+                    dA = (-Ac[n, l, m] * sn + As[n, l, m] * c) * m
+                    al = 2 * l + 1.5
+                    Cn = special.eval_gegenbauer(n, al, xi)
+                    dCn = 2 * al * special.eval_gegenbauer(n - 1, al + 1, xi) if n > 0 else 0.0
+                    ph = base * Cn
+                    dph_r = dbase * Cn + base * dCn * 2 * a / (a + r) ** 2
+                    pot += A * P * ph
+                    dr += A * P * dph_r
+                    dth += A * dPdx * (-numpy.sin(th)) * ph
+                    dph += dA * P * ph
+        s4 = numpy.sqrt(4 * numpy.pi)
+        return s4 * pot, s4 * dr, s4 * dth, s4 * dph, r, th
+
+    def _vec(self, f, R, z, phi, t):
+        R, z, phi, t = numpy.broadcast_arrays(*(numpy.asarray(v, dtype=float) for v in (R, z, phi, t)))
+        out = numpy.array([f(*v) for v in zip(R.ravel(), z.ravel(), phi.ravel(), t.ravel())])
+        return out.reshape(R.shape) if R.shape else float(out[0])
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return self._vec(lambda *v: self._spher(*v)[0], R, z, phi, t)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        def f(*v):
+            _, dr, dth, _, r, th = self._spher(*v)
+            return -(dr * numpy.sin(th) + dth * numpy.cos(th) / r)
+        return self._vec(f, R, z, phi, t)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        def f(*v):
+            _, dr, dth, _, r, th = self._spher(*v)
+            return -(dr * numpy.cos(th) - dth * numpy.sin(th) / r)
+        return self._vec(f, R, z, phi, t)
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return self._vec(lambda *v: -self._spher(*v)[3], R, z, phi, t)
+
+    def c_args(self):
+        N, L, M = self._coeff_shape
+        cache = [-1.0] + [0.0] * 10
+        if self._tdep:
+            out = [self._a, float(self.isNonAxi), N, L, M, float(len(self._tgrid))] + list(self._tgrid)
+            out += list(self._amp * numpy.ascontiguousarray(self._Acos_interp.c.transpose(1, 2, 0)).ravel())
+            if self.isNonAxi:
+                out += list(self._amp * numpy.ascontiguousarray(self._Asin_interp.c.transpose(1, 2, 0)).ravel())
+            return out + cache
+        out = [self._a, float(self.isNonAxi), N, L, M, 0.0] + list(self._amp * self._Acos.flatten(order="C"))
+        if self.isNonAxi:
+            out += list(self._amp * self._Asin.flatten(order="C"))
+        return out + cache
+
+
 class FlattenedPowerPotential(Potential):
     """Phi = -amp / (alpha m^alpha), m^2 = R^2 + z^2/q^2 + core^2 (amp/2 log m^2
     at alpha = 0); pot_type 12, args [amp, alpha, q^2, core^2], planar

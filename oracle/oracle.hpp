@@ -240,7 +240,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
-               kTimeDepAmpWrap = -9, kMovingObjectWrap = -6, kInterpSph = 36,
+               kTimeDepAmpWrap = -9, kMovingObjectWrap = -6, kInterpSph = 36, kSCF = 24,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
                kTriaxHernquist = 21, kTriaxNFW = 22, kTriaxJaffe = 23, kPerfectEllipsoid = 30,
                kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43,
@@ -408,6 +408,14 @@ inline int dedisk_nargs(const double* a) { return 5 + 4 * (int)a[4]; }
 // interpSphericalPotential: nr, rgrid[nr], rforce[nr], amp, rmin, rmax,
 // M(<rmax), Phi0, Phimax (integrateFullOrbit.py:366-373)
 inline int interpsph_nargs(const double* a) { return 1 + 2 * (int)a[0] + 6; }
+// SCFPotential: a, isNonAxi, N, L, M, Nt, then Acos[N L M] (and Asin when
+// non-axisymmetric), or tgrid[Nt] and the time-PPoly blocks [Nt-1][NLM][4]
+// of Acos (and Asin), then 11 cache slots (integrateFullOrbit.c nargs of
+// case 24; packing integratePlanarOrbit.py:726-762)
+inline int scf_nargs(const double* a) {
+  const int nonaxi = (int)a[1], NLM = (int)a[2] * (int)a[3] * (int)a[4], Nt = (int)a[5];
+  return Nt == 0 ? 6 + (1 + nonaxi) * NLM + 11 : 6 + Nt + (1 + nonaxi) * NLM * 4 * (Nt - 1) + 11;
+}
 // MovingObjectPotential's own arguments: nPts, t[nPts], x, y[, z], amp, t0,
 // tf (integrateFullOrbit.py:449-463; the planar packing has no z)
 inline int movingobject_nargs(const double* a, bool planar) { return 1 + (planar ? 3 : 4) * (int)a[0] + 3; }
@@ -432,6 +440,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
              : code == kSpiralArms ? spiralarms_nargs(pot_args)
              : code == kDoubleExpDisk ? dedisk_nargs(pot_args)
              : code == kInterpSph ? interpsph_nargs(pot_args)
+             : code == kSCF ? scf_nargs(pot_args)
              : code == kMovingObjectWrap ? 0  // counted after its wrapped components
                                    : component_nargs(code, planar);
     if (na < 0) return -1;
@@ -474,7 +483,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       pot_args += movingobject_nargs(pot_args, planar);
       continue;
     }
-    if (is_ellipsoidal(code) || code == kSpiralArms || code == kDoubleExpDisk) {
+    if (is_ellipsoidal(code) || code == kSpiralArms || code == kDoubleExpDisk || code == kSCF) {
       C.p[0] = pot_args[0];
       pot_args += na;
       continue;
@@ -1156,6 +1165,147 @@ template <class R> void movingobject_pos(const Component& C, R t, R& ox, R& oy, 
   oz = C.spl->size() > 2 ? (*C.spl)[2].eval(d) : R(0.);
 }
 
+// SCFPotential.c (Hernquist & Ostriker 1992 basis), with GSL's Gegenbauer
+// and Legendre arrays (absent here) restated from their recurrences:
+//   C_0^a = 1, C_1^a = 2 a x, C_n^a = (2 (n - 1 + a) x C_{n-1}^a -
+//     (n - 2 + 2a) C_{n-2}^a) / n  (gsl_sf_gegenpoly_array);
+//   P_m^m = (-1)^m (2m-1)!! (1 - x^2)^(m/2), P_{m+1}^m = (2m+1) x P_m^m,
+//   P_l^m = ((2l-1) x P_{l-1}^m - (l+m-1) P_{l-2}^m) / (l - m)
+//     (no normalisation, Condon-Shortley phase: GSL_SF_LEGENDRE_NONE,
+//     csphase -1), dP_l^m/dx = (l x P_l^m - (l+m) P_{l-1}^m) / (x^2 - 1);
+// the sums as SCFPotential.c:366-438 (sum_expansion, sum_spher_forces) and
+// the forces :865-895, coefficients interpolated in time (:121-172) when
+// the potential is time-dependent.  Pinned by mpmath/scipy (Gegenbauer,
+// Legendre), the Hernquist case, forces = -grad Phi and Poisson's equation.
+inline void scf_gegenpoly(int nmax, double alpha, double x, double* c) {
+  c[0] = 1.;
+  if (nmax >= 1) c[1] = 2. * alpha * x;
+  for (int k = 2; k <= nmax; k++)
+    c[k] = (2. * (k - 1. + alpha) * x * c[k - 1] - (k - 2. + 2. * alpha) * c[k - 2]) / k;
+}
+// P_l^m(x) and dP_l^m/dx for 0 <= m < Meff, m <= l < L, stored [m][l]
+inline void scf_legendre(double x, int L, int Meff, std::vector<double>& P, std::vector<double>& dP) {
+  const double s = std::sqrt((1. - x) * (1. + x));
+  double pmm = 1.;
+  for (int m = 0; m < Meff && m < L; m++) {
+    if (m > 0) pmm *= -(2. * m - 1.) * s;
+    double pm2 = 0., pm1 = pmm;
+    for (int l = m; l < L; l++) {
+      double pl;
+      if (l == m) pl = pmm;
+      else if (l == m + 1) pl = (2. * m + 1.) * x * pmm;
+      else pl = ((2. * l - 1.) * x * pm1 - (l + m - 1.) * pm2) / (l - m);
+      if (l > m) { pm2 = pm1; pm1 = pl; }
+      const double plm1 = (l == m) ? 0. : pm2;
+      P[m * L + l] = pl;
+      dP[m * L + l] = (l == 0) ? 0. : (l * x * pl - (l + m) * plm1) / (x * x - 1.);
+    }
+  }
+}
+// the (possibly time-interpolated) coefficient arrays of an SCF component
+inline void scf_coeffs(const Component& C, double t, std::vector<double>& Ac, std::vector<double>& As) {
+  const double* a = C.args;
+  const int nonaxi = (int)a[1], NLM = (int)a[2] * (int)a[3] * (int)a[4], Nt = (int)a[5];
+  Ac.assign(NLM, 0.);
+  As.assign(NLM, 0.);
+  if (Nt == 0) {
+    for (int i = 0; i < NLM; i++) Ac[i] = a[6 + i];
+    if (nonaxi) for (int i = 0; i < NLM; i++) As[i] = a[6 + NLM + i];
+    return;
+  }
+  const double* tg = a + 6;
+  int it;  // scf_find_time_interval: clamped to the boundary intervals
+  if (t <= tg[0]) it = 0;
+  else if (t >= tg[Nt - 1]) it = Nt - 2;
+  else {
+    int lo = 0, hi = Nt - 2;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tg[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    it = lo;
+  }
+  const double dt = t - tg[it];
+  const double* ppc = tg + Nt;
+  const double* pps = ppc + (size_t)NLM * 4 * (Nt - 1);
+  for (int ci = 0; ci < NLM; ci++) {
+    const double* cp = ppc + ((size_t)it * NLM + ci) * 4;
+    Ac[ci] = ((cp[0] * dt + cp[1]) * dt + cp[2]) * dt + cp[3];
+    if (nonaxi) {
+      const double* sp = pps + ((size_t)it * NLM + ci) * 4;
+      As[ci] = ((sp[0] * dt + sp[1]) * dt + sp[2]) * dt + sp[3];
+    }
+  }
+}
+// spherical derivatives F = (dPhi/dr, dPhi/dtheta, dPhi/dphi) with the
+// reference's signs (F[0], F[1] minus the derivatives, F[2] the torque),
+// and the potential; evaluated in double (not op-counted)
+inline void scf_spher(const Component& C, double Rc, double z, double phi, double t, double* F, double& pot) {
+  const double* a = C.args;
+  const double sa = a[0];
+  const int nonaxi = (int)a[1], N = (int)a[2], L = (int)a[3], M = (int)a[4];
+  const int Meff = nonaxi ? M : 1;
+  const double r = std::sqrt(Rc * Rc + z * z), theta = std::atan2(Rc, z);
+  const double xi = (r - sa) / (r + sa);
+  std::vector<double> Ac, As, P(Meff * L, 0.), dP(Meff * L, 0.), Cg(N * L), dC(N * L, 0.), phiT(N * L), dphiT(N * L);
+  scf_coeffs(C, t, Ac, As);
+  for (int l = 0; l < L; l++) {
+    scf_gegenpoly(N - 1, 1.5 + 2 * l, xi, &Cg[l * N]);
+    if (N > 1) scf_gegenpoly(N - 2, 2.5 + 2 * l, xi, &dC[l * N + 1]);
+    for (int n = 0; n < N; n++) dC[l * N + n] *= 2.0 * (2 * l + 1.5);
+  }
+  double rterms = -1.0 / (r + sa);  // compute_phiTilde
+  for (int l = 0; l < L; l++) {
+    if (l > 0) rterms *= r * sa / ((sa + r) * (sa + r));
+    for (int n = 0; n < N; n++) phiT[l * N + n] = rterms * Cg[l * N + n];
+  }
+  const double ar = sa + r;  // compute_dphiTilde
+  double rterm = 1.0 / (r * ar * ar * ar);
+  for (int l = 0; l < L; l++) {
+    if (l > 0) rterm *= sa * r / (ar * ar);
+    for (int n = 0; n < N; n++) {
+      const int i = l * N + n;
+      dphiT[i] = rterm * (((2 * l + 1) * r * ar - l * ar * ar) * Cg[i] - 2.0 * sa * r * dC[i]);
+    }
+  }
+  scf_legendre(std::cos(theta), L, Meff, P, dP);
+  F[0] = F[1] = F[2] = 0.;
+  pot = 0.;
+  for (int l = 0; l < L; l++) {
+    for (int m = 0; m <= (nonaxi ? l : 0); m++) {
+      const double mc = std::cos(m * phi), ms = std::sin(m * phi);
+      const double Pl = P[m * L + l], dPl = dP[m * L + l];
+      for (int n = 0; n < N; n++) {
+        const int ci = n * L * M + l * M + m, ri = l * N + n;
+        const double cs = nonaxi ? Ac[ci] * mc + As[ci] * ms : Ac[ci];
+        const double sd = nonaxi ? Ac[ci] * ms - As[ci] * mc : 0.;
+        F[0] -= cs * Pl * dphiT[ri];
+        F[1] -= cs * dPl * phiT[ri];
+        F[2] += m * sd * Pl * phiT[ri];
+        pot += cs * Pl * phiT[ri];
+      }
+    }
+  }
+  const double s4 = std::sqrt(4.0 * M_PI);
+  F[0] *= s4;
+  F[1] *= s4 * (-std::sin(theta));
+  F[2] *= s4;
+  pot *= s4;
+}
+template <class R> void scf_forces(const Component& C, R Rc, R z, R phi, R t, R& FR, R& Fz, R& Fp) {
+  double F[3], pot;
+  scf_spher(C, dval(Rc), dval(z), dval(phi), dval(t), F, pot);
+  const double Rd = dval(Rc), zd = dval(z), r = std::sqrt(Rd * Rd + zd * zd);
+  FR = R(F[0] * (Rd / r) + F[1] * (zd / (r * r)));
+  Fz = R(F[0] * (zd / r) + F[1] * (-Rd / (r * r)));
+  Fp = R(F[2]);
+}
+template <class R> R scf_eval(const Component& C, R Rc, R z, R phi, R t) {
+  double F[3], pot;
+  scf_spher(C, dval(Rc), dval(z), dval(phi), dval(t), F, pot);
+  return R(pot);
+}
+
 // KuzminLikeWrapperPotential.c:5-8: xi = sqrt(R^2 + (a + sqrt(z^2 + b^2))^2)
 template <class R> R kuzminlike_xi(R Rc, R z, double a, double b2) {
   R asqrtbz = R(a) + sqrt(z * z + R(b2));
@@ -1231,6 +1381,7 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kSoftNeedleBar: needle_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kSpiralArms: spiral_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kDoubleExpDisk: dedisk_forces(C, Rc, z, fr, fz); break;
+      case kSCF: scf_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kInterpSph: {
         R r = sqrt(Rc * Rc + z * z);
         R f = R(C.p[0]) * interpsph_rforce(C, r);
@@ -1838,6 +1989,11 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
         break;
       }
       case kDoubleExpDisk: fr = dedisk_planar_Rforce(C, Rc); break;
+      case kSCF: {  // SCFPotential.c:896-906: the full forces at z = 0
+        R fz0;
+        scf_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
+        break;
+      }
       case kInterpSph: fr = R(C.p[0]) * interpsph_rforce(C, Rc); break;  // SphericalPotential.c:22-29
       case kSpiralArms: {  // SpiralArmsPotential.c:625-716: the full forces at z = 0
         R fz0;
@@ -1916,7 +2072,7 @@ inline bool has_hessian(int npot, const Component* P) {
     const int c = P[i].type;
     if (c == kKKStaeckel || c == kSoftNeedleBar || is_wrapper(c) || c == kEllipDisk || c == kLopsided ||
         c == kSteadySpiral || c == kTransientSpiral || c == kCosmphiDisk || c == kHenonHeiles ||
-        c == kSpiralArms || c == kDoubleExpDisk || is_ellipsoidal(c))
+        c == kSpiralArms || c == kDoubleExpDisk || c == kInterpSph || c == kSCF || is_ellipsoidal(c))
       return false;
   }
   return true;
@@ -2008,6 +2164,7 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kSoftNeedleBar: pot += needle_eval(C, Rc, z, phi, t); break;
       case kSpiralArms: pot += spiral_eval(C, Rc, z, phi, t); break;
       case kDoubleExpDisk: pot += dedisk_eval(C, Rc, z); break;
+      case kSCF: pot += scf_eval(C, Rc, z, phi, t); break;
       case kInterpSph: pot += R(C.p[0]) * interpsph_reval(C, sqrt(Rc * Rc + z * z)); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Eval
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];

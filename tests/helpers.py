@@ -64,6 +64,9 @@ def _nargs(code, pot_args, ai, planar):
         return 10 + int(pot_args[ai])
     if code == 11:  # DoubleExponentialDiskPotential: 5 + 4 de_n
         return 5 + 4 * int(pot_args[ai + 4])
+    if code == 24:  # SCFPotential
+        nonaxi, NLM, Nt = int(pot_args[ai + 1]), int(pot_args[ai + 2] * pot_args[ai + 3] * pot_args[ai + 4]), int(pot_args[ai + 5])
+        return 6 + (1 + nonaxi) * NLM + 11 if Nt == 0 else 6 + Nt + (1 + nonaxi) * NLM * 4 * (Nt - 1) + 11
     if code == 36:  # interpSphericalPotential: 1 + 2 nr + 6
         return 1 + 2 * int(pot_args[ai]) + 6
     if code == -6:  # MovingObjectPotential's own: 1 + (3|4) n + 3

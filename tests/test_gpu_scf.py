@@ -1,0 +1,92 @@
+"""SCFPotential (pot_type 24) on the GPU: static axisymmetric and
+non-axisymmetric expansions and a time-dependent one; pointwise forces and
+potential against the oracle, full and planar orbits against the oracle."""
+
+import numpy
+import pytest
+
+from helpers import (assert_parity, ics, oracle_forces, oracle_integrate, oracle_potential, planar_ics,
+                     ulp_sensitivity)
+
+pytestmark = pytest.mark.gpu
+
+
+def _coeffs(N, L, seed, scale=0.2):
+    rng = numpy.random.default_rng(seed)
+    Ac, As = scale * rng.normal(size=(N, L, L)), scale * rng.normal(size=(N, L, L))
+    for l in range(L):
+        Ac[:, l, l + 1:] = 0.0
+        As[:, l, l + 1:] = 0.0
+    As[:, :, 0] = 0.0
+    Ac[0, 0, 0] = 1.0
+    return Ac, As
+
+
+def _pot(which):
+    from galpy_amd import potential as P
+
+    if which == "axi":
+        Ac = numpy.zeros((6, 5, 1))
+        Ac[:, :, 0] = _coeffs(6, 5, 1)[0][:, :, 0]
+        Ac[:, 1::2, 0] = 0.0  # reflection-symmetric
+        return [P.SCFPotential(amp=2.0, Acos=Ac, a=0.9), P.MiyamotoNagaiPotential(a=0.5, b=0.1, normalize=0.3)]
+    Ac, As = _coeffs(5, 5, 2)
+    if which == "nonaxi":
+        return [P.SCFPotential(amp=2.0, Acos=Ac, Asin=As, a=0.9)]
+    tgrid = numpy.linspace(-1.0, 12.0, 14)
+    return [P.SCFPotential(amp=2.0, Acos=lambda t: Ac * (1.0 + 0.1 * numpy.sin(t)),
+                           Asin=lambda t: As * numpy.cos(0.3 * t), a=0.9, tgrid=tgrid)]
+
+
+@pytest.mark.parametrize("which", ["axi", "nonaxi", "tdep"])
+def test_pointwise_forces_and_potential(which):
+    import torch
+
+    from galpy_amd import potential as P
+    from galpy_amd.integrate import PotentialTable, eval_device
+
+    pot = _pot(which)
+    _, pt, pa, _ = P.parse_pot(pot)
+    rng = numpy.random.default_rng(121)
+    n = 2048
+    R, z = rng.uniform(0.05, 3.0, n), rng.uniform(-1.5, 1.5, n)
+    phi, t = rng.uniform(0.0, 2.0 * numpy.pi, n), rng.uniform(-2.0, 13.0, n)
+    dev = [torch.tensor(v, device="cuda") for v in (R, z, phi, t)]
+    pot_d, FR, Fz, Fp = (o.cpu().numpy() for o in eval_device(PotentialTable(pot), *dev))
+    oFR, oFz, oFp = oracle_forces(pt, pa, R, z, phi, t)
+    assert numpy.abs(FR - oFR).max() <= 1e-13 * numpy.abs(oFR).max()
+    assert numpy.abs(Fz - oFz).max() <= 1e-13 * numpy.abs(oFz).max()
+    assert numpy.abs(Fp - oFp).max() <= 1e-13 * max(numpy.abs(oFp).max(), 1e-300)
+    want = oracle_potential(pt, pa, R, z, phi, t)
+    assert numpy.abs(pot_d - want).max() <= 1e-13 * numpy.abs(want).max()
+
+
+@pytest.mark.parametrize("which", ["axi", "nonaxi", "tdep"])
+@pytest.mark.parametrize("method,code,bar,factor", [("symplec4_hip", 3, 1e-10, 1e3),
+                                                     ("dop853_hip", 6, 1e-8, 10.0)])
+def test_full_orbits(which, method, code, bar, factor):
+    from galpy_amd import potential as P
+    from galpy_amd.integrate import integrateFullOrbit_hip
+
+    pot = _pot(which)
+    _, pt, pa, _ = P.parse_pot(pot)
+    t = numpy.linspace(0.0, 10.0, 101)
+    yo = ics(128, seed=123)
+    res, err = integrateFullOrbit_hip(pot, yo.copy(), t, method)
+    assert numpy.all(err == 0)
+    ref, sens = ulp_sensitivity(lambda y: oracle_integrate(pt, pa, y, t, code)[0], yo)
+    assert_parity(res, ref, bar, sens, factor=factor, label=f"{which} {method}")
+
+
+def test_planar_orbits():
+    from galpy_amd import potential as P
+    from galpy_amd.integrate import integratePlanarOrbit_hip
+
+    pot = _pot("nonaxi")
+    _, pt, pa, _ = P.parse_pot_planar(pot)
+    t = numpy.linspace(0.0, 10.0, 101)
+    yp = planar_ics(128, seed=125)
+    res, err = integratePlanarOrbit_hip(pot, yp.copy(), t, "dop853_hip")
+    assert numpy.all(err == 0)
+    ref, sens = ulp_sensitivity(lambda y: oracle_integrate(pt, pa, y, t, 6, planar=True)[0], yp)
+    assert_parity(res, ref, 1e-8, sens, factor=10.0, label="planar")
