@@ -528,6 +528,32 @@ int decode_scf(const double*& pot_args, GhPotTable* T, int i) {
   return 0;
 }
 
+// One [Sigma_i, h_i] pair of a DiskSCFPotential (integratePlanarOrbit.py:
+// 778-812, the same in both packings): nsigma_args (3 or 4), Sigma type,
+// amp, h[, Rhole], hz type, hz h.
+int decode_diskscf(const double*& pot_args, GhPotTable* T, int i, bool planar) {
+  GhComponent& C = T->comp[i];
+  const double* a = pot_args;
+  const int ns = (int)a[0];
+  if ((ns != 3 && ns != 4) || ((int)a[1] == 1) != (ns == 4)) {
+    set_error("DiskSCFPotential pair with %d Sigma arguments of type %g", ns, a[1]);
+    return -1;
+  }
+  C.type = GH_DISKSCF;
+  C.p[0] = a[1];
+  C.p[1] = a[2];
+  C.p[2] = a[3];
+  C.p[3] = ns == 4 ? a[4] : 0.0;
+  C.p[4] = a[1 + ns];
+  C.p[5] = a[2 + ns];
+  C.c[0] = 1.0 / C.p[2];
+  C.c[1] = 1.0 / C.p[5];
+  C.c[2] = planar ? 1.0 : 0.0;
+  T->has_wrap = 1;  // extended dispatch instances
+  pot_args += ns + 3;
+  return 0;
+}
+
 // One (leaf) component's arguments and precomputed constants.
 int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
                      bool planar) {
@@ -536,6 +562,7 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
   if (code == GH_DEDISK) return decode_dedisk(pot_args, T, i, planar);
   if (code == GH_INTERPSPH) return decode_interpsph(pot_args, T, i);
   if (code == GH_SCF) return decode_scf(pot_args, T, i);
+  if (code == GH_DISKSCF) return decode_diskscf(pot_args, T, i, planar);
   const int na = nargs_of(code, planar);
   if (na < 0 || code < 0) {
     set_error("pot_type %d (component %d) has no HIP kernel", code, i);

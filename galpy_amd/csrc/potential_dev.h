@@ -89,6 +89,7 @@ enum GhPotCode {
   GH_MOVINGOBJWRAP = -6,
   GH_INTERPSPH = 36,
   GH_SCF = 24,
+  GH_DISKSCF = 26,
 };
 
 struct GhComponent {
@@ -1391,6 +1392,54 @@ __device__ __forceinline__ void gh_comp_force<GH_SCF>(const GhPotTable* __restri
   F.FphioR2 += S[2] * iR * iR;
 }
 
+// DiskSCFPotential.c:33-171: one [Sigma_i, h_i] approximation pair,
+// Phi = Sigma(r) H(z); F_R = -Sigma'(r) H R / r, F_z = -Sigma'(r) H z / r -
+// Sigma(r) H'(z); the planar packing's force -Sigma'(R) H(0).  Sigma: amp
+// exp(-r/h) (type 0) or amp exp(-Rhole/r - r/h) (type 1); H: exponential
+// (0) or sech^2 (1).  p: sigma type, amp, h, Rhole, hz type, hz h; c[0] = 1/h,
+// c[1] = 1/hz, c[2] = 1 for the planar packing.
+__device__ __forceinline__ void gh_dscf_sigma(const GhComponent* __restrict__ C, double r,
+                                              double& S, double& dS) {
+  if (C->p[0] == 0.0) {
+    S = C->p[1] * exp(-r * C->c[0]);
+    dS = -S * C->c[0];
+  } else {
+    const double ir = 1.0 / r;
+    S = C->p[1] * exp(-C->p[3] * ir - r * C->c[0]);
+    dS = S * (C->p[3] * ir * ir - C->c[0]);
+  }
+}
+__device__ __forceinline__ void gh_dscf_hz(const GhComponent* __restrict__ C, double z, double& H,
+                                           double& dH) {
+  const double fz = fabs(z), h = C->p[5], ih = C->c[1];
+  const double e = exp(-fz * ih);
+  if (C->p[4] == 0.0) {
+    H = 0.5 * (e - 1.0 + fz * ih) * h;
+    dH = copysign(0.5 * (1.0 - e), z);
+  } else {
+    H = h * (log1p(e) + 0.5 * fz * ih - M_LN2);
+    dH = copysign(0.5 * (1.0 - e) / (1.0 + e), z);  // tanh(z / (2 h)) / 2
+  }
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_DISKSCF>(const GhPotTable* __restrict__ T, int i,
+                                                          const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  double S, dS, H, dH;
+  if (C->c[2] != 0.0) {  // planar packing
+    const double R = sqrt(g.R2);
+    gh_dscf_sigma(C, R, S, dS);
+    gh_dscf_hz(C, 0.0, H, dH);
+    F.FRoR += -dS * H / R;
+    return;
+  }
+  gh_dscf_sigma(C, g.r, S, dS);
+  gh_dscf_hz(C, g.z, H, dH);
+  const double f = -dS * H * g.rinv;
+  F.FRoR += f;
+  F.Fz += f * g.z - S * dH;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_NULLPOT>(const GhPotTable* __restrict__, int,
                                                           const GhGeom&, GhAcc&) {}
@@ -1437,7 +1486,7 @@ __host__ __device__ inline bool gh_is_heavy(int type) {
          type == GH_TWOPOWERTRIAX || type == GH_PERFECTELLIPSOID || type == GH_TRIAXGAUSSIAN ||
          type == GH_POWERTRIAX || type == GH_EINASTO || type == GH_TWOPOWERSPH ||
          type == GH_EXPTRUNCNFW || type == GH_SPIRALARMS || type == GH_DEDISK ||
-         type == GH_INTERPSPH || type == GH_SCF;
+         type == GH_INTERPSPH || type == GH_SCF || type == GH_DISKSCF;
 }
 
 // run-time dispatch over every component type (extended instances)
@@ -1458,6 +1507,7 @@ __device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__
     case GH_DEDISK: gh_comp_force<GH_DEDISK>(T, i, g, F); break;
     case GH_INTERPSPH: gh_comp_force<GH_INTERPSPH>(T, i, g, F); break;
     case GH_SCF: gh_comp_force<GH_SCF>(T, i, g, F); break;
+    case GH_DISKSCF: gh_comp_force<GH_DISKSCF>(T, i, g, F); break;
     default: gh_comp_force_dyn(T, i, g, F); break;
   }
 }
@@ -2016,6 +2066,13 @@ __device__ __forceinline__ double gh_comp_potential(const GhPotTable* __restrict
       pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
              cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
       break;
+    case GH_DISKSCF: {  // DiskSCFPotential.c:108-120 (Eval)
+      double S, dS, H, dH;
+      gh_dscf_sigma(C, sqrt(r2), S, dS);
+      gh_dscf_hz(C, z, H, dH);
+      pot += S * H;
+      break;
+    }
     case GH_SCF: {  // SCFPotential.c:828-863 (Eval)
       double S[3], V;
       gh_scf_sums<true>(T, C, R, z, cosphi, sinphi, t, S, V);

@@ -1514,6 +1514,128 @@ class SCFPotential(Potential):
         return out + cache
 
 
+def scf_compute_coeffs_axi(dens, N, L, a=1.0, radial_order=None, costheta_order=None):
+    """Expansion coefficients (N, L, 1) of an axisymmetric density dens(R, z)
+    in SCFPotential's basis (the m = 0 case of galpy's scf_compute_coeffs,
+    SCFPotential.py): product Gauss-Legendre quadrature over xi = (r - a)/(r
+    + a) and cos(theta), the basis normalisation I_nl of Hernquist &
+    Ostriker (1992)."""
+    nxi = radial_order if radial_order is not None else max(N + 3 * L // 2 + 1, 20)
+    nct = costheta_order if costheta_order is not None else max(L + 1, 20)
+    xg, wg = numpy.polynomial.legendre.leggauss(nxi)
+    cg, cw = numpy.polynomial.legendre.leggauss(nct)
+    xi = xg[:, None]
+    ct = cg[None, :]
+    r = a * (1.0 + xi) / (1.0 - xi)
+    R, z = r * numpy.sqrt(1.0 - ct**2), r * ct
+    rho = dens(R, z)
+    dV = (1.0 + xi) ** 2 * (1.0 - xi) ** -4.0
+    n = numpy.arange(N)[:, None]
+    l = numpy.arange(L)[None, :]
+    out = numpy.zeros((N, L))
+    for ll in range(L):
+        Pl = special.eval_legendre(ll, ct)
+        for nn in range(N):
+            phi_nl = -(a**3) * (1.0 + xi) ** ll * (1.0 - xi) ** (ll + 1.0) * special.eval_gegenbauer(nn, 2 * ll + 1.5, xi)
+            out[nn, ll] = 2.0 * numpy.pi * numpy.sum(wg[:, None] * cw[None, :] * rho * phi_nl * Pl * dV)
+    K = 0.5 * n * (n + 4 * l + 3) + (l + 1) * (2 * l + 1)
+    NN = numpy.exp(-(2 * l) * numpy.log(2.0))
+    lnI = (-(8 * l + 6) * numpy.log(2.0) + special.gammaln(n + 4 * l + 3) - special.gammaln(n + 1)
+           - numpy.log(n + 2 * l + 1.5) - 2 * special.gammaln(2 * l + 1.5))
+    I = -K * 4.0 * numpy.pi * numpy.exp(lnI)
+    return (2.0 / I * out * NN * numpy.sqrt(2 * l + 1.0))[:, :, None]
+
+
+class DiskSCFPotential(Potential):
+    """A disk-like density as an SCF expansion plus the Kuijken & Dubinski
+    approximation pairs Sigma_i(r) H_i(z) (DiskSCFPotential.py and
+    KuijkenDubinskiDiskExpansionPotential.py): the pairs carry the disk's
+    thinness, the SCF expansion (N, L) the axisymmetric remainder of the
+    density.  Sigma: dict (or list of dicts) {"type": "exp" or "expwhole",
+    "amp", "h"[, "Rhole"]}; hz: {"type": "exp" or "sech2", "h"}.  galpy packs
+    it as the SCF (pot_type 24, amp folded in) followed by one pot_type 26
+    per pair, [nsigma_args, type, 4 pi amp, h[, Rhole], hz type, hz h]
+    (integrateFullOrbit.py:323-336, integratePlanarOrbit.py:778-812), each
+    counted in npot."""
+
+    pot_code = None
+
+    def __init__(self, amp=1.0, dens=None, Sigma=None, hz=None, N=10, L=10, a=1.0, radial_order=None,
+                 costheta_order=None):
+        Potential.__init__(self, amp=amp)
+        self._Sigma_dict = list(Sigma) if isinstance(Sigma, (list, tuple)) else [Sigma]
+        self._hz_dict = list(hz) if isinstance(hz, (list, tuple)) else [hz]
+        if len(self._hz_dict) == 1 and len(self._Sigma_dict) > 1:
+            self._hz_dict = self._hz_dict * len(self._Sigma_dict)
+        self._dens = dens
+        me = scf_compute_coeffs_axi(self._me_dens, N, L, a=a, radial_order=radial_order,
+                                    costheta_order=costheta_order)
+        self._me = SCFPotential(Acos=me, a=a)
+
+    def _pair_terms(self, Sg, hz, r, z):
+        amp, h = 4.0 * numpy.pi * Sg.get("amp", 1.0), Sg.get("h", 1.0 / 3.0)
+        if Sg.get("type", "exp") == "exp" and "Rhole" not in Sg:
+            S = amp * numpy.exp(-r / h)
+            dS, d2S = -S / h, S / h**2
+        else:
+            Rh = Sg.get("Rhole", 0.5)
+            S = amp * numpy.exp(-Rh / r - r / h)
+            dS = S * (Rh / r**2 - 1.0 / h)
+            d2S = S * ((Rh / r**2 - 1.0 / h) ** 2 - 2.0 * Rh / r**3)
+        hh, fz = hz.get("h", 0.0375), numpy.fabs(z)
+        if hz.get("type", "exp") == "exp":
+            hzv = 0.5 * numpy.exp(-fz / hh) / hh
+            H = 0.5 * (numpy.exp(-fz / hh) - 1.0 + fz / hh) * hh
+            dH = 0.5 * numpy.sign(z) * (1.0 - numpy.exp(-fz / hh))
+        else:
+            hzv = 0.25 / numpy.cosh(0.5 * z / hh) ** 2 / hh
+            H = hh * (numpy.log1p(numpy.exp(-fz / hh)) + 0.5 * fz / hh - numpy.log(2.0))
+            dH = 0.5 * numpy.tanh(0.5 * z / hh)
+        return S, dS, d2S, hzv, H, dH
+
+    def _me_dens(self, R, z):
+        """The density left to the SCF expansion: dens minus the pairs'
+        densities (DiskSCFPotential.c:221-231, per pair)."""
+        r = numpy.sqrt(R * R + z * z)
+        out = self._dens(R, z)
+        for Sg, hz in zip(self._Sigma_dict, self._hz_dict):
+            S, dS, d2S, hzv, H, dH = self._pair_terms(Sg, hz, r, z)
+            out = out - (S * hzv + d2S * H + 2.0 / r * dS * (H + z * dH)) / (4.0 * numpy.pi)
+        return out
+
+    def _pairs(self, fun, R, z):
+        R, z = numpy.asarray(R, dtype=float), numpy.asarray(z, dtype=float)
+        r = numpy.sqrt(R * R + z * z)
+        return sum(fun(*self._pair_terms(Sg, hz, r, z), R, z, r) for Sg, hz in zip(self._Sigma_dict, self._hz_dict))
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return self._me._amp * self._me._evaluate(R, z, phi, t) + self._pairs(lambda S, dS, d2S, hzv, H, dH, R, z, r: S * H, R, z)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return self._me._amp * self._me._Rforce(R, z, phi, t) + self._pairs(
+            lambda S, dS, d2S, hzv, H, dH, R, z, r: -dS * H * R / r, R, z)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return self._me._amp * self._me._zforce(R, z, phi, t) + self._pairs(
+            lambda S, dS, d2S, hzv, H, dH, R, z, r: -dS * H * z / r - S * dH, R, z)
+
+    def pack(self):
+        """(pot_type, pot_args, npot) in galpy's packing."""
+        me_args = self._me.c_args()
+        N, L, M = self._me._coeff_shape
+        me_args = me_args[:6] + [v * self._amp for v in me_args[6:6 + N * L * M]] + me_args[6 + N * L * M:]
+        pt, pa = [24], list(me_args)
+        for Sg, hz in zip(self._Sigma_dict, self._hz_dict):
+            pt.append(26)
+            famp = 4.0 * numpy.pi * Sg.get("amp", 1.0) * self._amp
+            if Sg.get("type", "exp") == "exp" and "Rhole" not in Sg:
+                pa.extend([3, 0, famp, Sg.get("h", 1.0 / 3.0)])
+            else:
+                pa.extend([4, 1, famp, Sg.get("h", 1.0 / 3.0), Sg.get("Rhole", 0.5)])
+            pa.extend([0 if hz.get("type", "exp") == "exp" else 1, hz.get("h", 0.0375)])
+        return pt, pa, len(pt)
+
+
 class FlattenedPowerPotential(Potential):
     """Phi = -amp / (alpha m^alpha), m^2 = R^2 + z^2/q^2 + core^2 (amp/2 log m^2
     at alpha = 0); pot_type 12, args [amp, alpha, q^2, core^2], planar
@@ -2293,6 +2415,11 @@ def _pack(comps, planar, tfuncs=None):
                 pot_type.append(MiyamotoNagaiPotential.pot_code)
                 pot_args.extend(args)
             continue
+        if isinstance(p, DiskSCFPotential):
+            pt, pa, _ = p.pack()
+            pot_type.extend(pt)
+            pot_args.extend(pa)
+            continue
         if not isinstance(p, Potential) or p.pot_code is None:
             raise NotImplementedError(
                 f"Potential {type(p).__name__} is not supported by the HIP orbit-integration backend"
@@ -2305,7 +2432,8 @@ def _pack(comps, planar, tfuncs=None):
 def _npot(comps):
     """galpy's npot: an MN3ExponentialDiskPotential counts as its three
     Miyamoto-Nagai disks (integrateFullOrbit.py:184)."""
-    return sum(3 if isinstance(p, MN3ExponentialDiskPotential) else 1 for p in comps)
+    return sum(3 if isinstance(p, MN3ExponentialDiskPotential)
+               else (1 + len(p._Sigma_dict) if isinstance(p, DiskSCFPotential) else 1) for p in comps)
 
 
 def _purge_null(comps):

@@ -241,6 +241,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
                kTimeDepAmpWrap = -9, kMovingObjectWrap = -6, kInterpSph = 36, kSCF = 24,
+               kDiskSCF = 26,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
                kTriaxHernquist = 21, kTriaxNFW = 22, kTriaxJaffe = 23, kPerfectEllipsoid = 30,
                kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43,
@@ -441,6 +442,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
              : code == kDoubleExpDisk ? dedisk_nargs(pot_args)
              : code == kInterpSph ? interpsph_nargs(pot_args)
              : code == kSCF ? scf_nargs(pot_args)
+             : code == kDiskSCF ? (int)pot_args[0] + 3
              : code == kMovingObjectWrap ? 0  // counted after its wrapped components
                                    : component_nargs(code, planar);
     if (na < 0) return -1;
@@ -483,7 +485,8 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       pot_args += movingobject_nargs(pot_args, planar);
       continue;
     }
-    if (is_ellipsoidal(code) || code == kSpiralArms || code == kDoubleExpDisk || code == kSCF) {
+    if (is_ellipsoidal(code) || code == kSpiralArms || code == kDoubleExpDisk || code == kSCF ||
+        code == kDiskSCF) {
       C.p[0] = pot_args[0];
       pot_args += na;
       continue;
@@ -1306,6 +1309,41 @@ template <class R> R scf_eval(const Component& C, R Rc, R z, R phi, R t) {
   return R(pot);
 }
 
+// DiskSCFPotential.c:33-120 (one [Sigma_i, h_i] approximation pair of a
+// DiskSCFPotential): Phi = Sigma(r) H(z), Sigma exponential (type 0: amp
+// exp(-r/h)) or with a central hole (type 1: amp exp(-Rhole/r - r/h)), H
+// the integral twice over of the vertical profile (type 0 exponential,
+// 1 sech^2).  args: nsigma_args, the Sigma arguments, hz type, hz h.
+template <class R> void dscf_sigma(const double* sa, R r, R& S, R& dS) {
+  if ((int)sa[0] == 0) {
+    S = R(sa[1]) * exp(-r / R(sa[2]));
+    dS = -R(sa[1]) * exp(-r / R(sa[2])) / R(sa[2]);
+  } else {
+    S = R(sa[1]) * exp(-R(sa[3]) / r - r / R(sa[2]));
+    dS = R(sa[1]) * (R(sa[3]) / r / r - R(1.) / R(sa[2])) * exp(-R(sa[3]) / r - r / R(sa[2]));
+  }
+}
+template <class R> void dscf_hz(const double* ha, R z, R& H, R& dH) {
+  R fz = fabs(z);
+  const double h = ha[1];
+  if ((int)ha[0] == 0) {
+    H = R(0.5) * (exp(-fz / R(h)) - R(1.) + fz / R(h)) * R(h);
+    dH = (z >= R(0.) ? R(1.) : R(-1.)) * R(0.5) * (R(1.) - exp(-fz / R(h)));  // copysign(., z)
+  } else {
+    H = R(h) * (log(R(1.) + exp(-fz / R(h))) + R(0.5) * fz / R(h) - R(M_LN2));
+    dH = R(0.5) * tanh(R(0.5) * z / R(h));
+  }
+}
+template <class R> void dscf_forces(const Component& C, R Rc, R z, R& FR, R& Fz) {
+  const double* a = C.args;
+  const int ns = (int)a[0];
+  R r = sqrt(Rc * Rc + z * z), S, dS, H, dH;
+  dscf_sigma(a + 1, r, S, dS);
+  dscf_hz(a + 1 + ns, z, H, dH);
+  FR = -dS * H * Rc / r;
+  Fz = -dS * H * z / r - S * dH;
+}
+
 // KuzminLikeWrapperPotential.c:5-8: xi = sqrt(R^2 + (a + sqrt(z^2 + b^2))^2)
 template <class R> R kuzminlike_xi(R Rc, R z, double a, double b2) {
   R asqrtbz = R(a) + sqrt(z * z + R(b2));
@@ -1382,6 +1420,7 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kSpiralArms: spiral_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kDoubleExpDisk: dedisk_forces(C, Rc, z, fr, fz); break;
       case kSCF: scf_forces(C, Rc, z, phi, t, fr, fz, fp); break;
+      case kDiskSCF: dscf_forces(C, Rc, z, fr, fz); break;
       case kInterpSph: {
         R r = sqrt(Rc * Rc + z * z);
         R f = R(C.p[0]) * interpsph_rforce(C, r);
@@ -1989,6 +2028,14 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
         break;
       }
       case kDoubleExpDisk: fr = dedisk_planar_Rforce(C, Rc); break;
+      case kDiskSCF: {  // DiskSCFPotential.c:133-144: -Sigma'(R) H(0)
+        const double* a = C.args;
+        R S, dS, H, dH;
+        dscf_sigma(a + 1, Rc, S, dS);
+        dscf_hz(a + 1 + (int)a[0], R(0.), H, dH);
+        fr = -dS * H;
+        break;
+      }
       case kSCF: {  // SCFPotential.c:896-906: the full forces at z = 0
         R fz0;
         scf_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
@@ -2072,7 +2119,7 @@ inline bool has_hessian(int npot, const Component* P) {
     const int c = P[i].type;
     if (c == kKKStaeckel || c == kSoftNeedleBar || is_wrapper(c) || c == kEllipDisk || c == kLopsided ||
         c == kSteadySpiral || c == kTransientSpiral || c == kCosmphiDisk || c == kHenonHeiles ||
-        c == kSpiralArms || c == kDoubleExpDisk || c == kInterpSph || c == kSCF || is_ellipsoidal(c))
+        c == kSpiralArms || c == kDoubleExpDisk || c == kInterpSph || c == kSCF || c == kDiskSCF || is_ellipsoidal(c))
       return false;
   }
   return true;
@@ -2165,6 +2212,14 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kSpiralArms: pot += spiral_eval(C, Rc, z, phi, t); break;
       case kDoubleExpDisk: pot += dedisk_eval(C, Rc, z); break;
       case kSCF: pot += scf_eval(C, Rc, z, phi, t); break;
+      case kDiskSCF: {  // DiskSCFPotential.c:108-120 (Eval)
+        const double* a = C.args;
+        R S, dS, H, dH;
+        dscf_sigma(a + 1, sqrt(Rc * Rc + z * z), S, dS);
+        dscf_hz(a + 1 + (int)a[0], z, H, dH);
+        pot += S * H;
+        break;
+      }
       case kInterpSph: pot += R(C.p[0]) * interpsph_reval(C, sqrt(Rc * Rc + z * z)); break;
       case kFlattenedPower: {  // FlattenedPowerPotential.c Eval
         double amp = C.p[0], alpha = C.p[1], q2 = C.p[2], core2 = C.p[3];

@@ -67,6 +67,8 @@ def _nargs(code, pot_args, ai, planar):
     if code == 24:  # SCFPotential
         nonaxi, NLM, Nt = int(pot_args[ai + 1]), int(pot_args[ai + 2] * pot_args[ai + 3] * pot_args[ai + 4]), int(pot_args[ai + 5])
         return 6 + (1 + nonaxi) * NLM + 11 if Nt == 0 else 6 + Nt + (1 + nonaxi) * NLM * 4 * (Nt - 1) + 11
+    if code == 26:  # DiskSCFPotential pair: nsigma + 3
+        return int(pot_args[ai]) + 3
     if code == 36:  # interpSphericalPotential: 1 + 2 nr + 6
         return 1 + 2 * int(pot_args[ai]) + 6
     if code == -6:  # MovingObjectPotential's own: 1 + (3|4) n + 3

@@ -1,5 +1,6 @@
 """SCFPotential (pot_type 24) on the GPU: static axisymmetric and
-non-axisymmetric expansions and a time-dependent one; pointwise forces and
+non-axisymmetric expansions and a time-dependent one, and a DiskSCFPotential
+(the SCF part plus its pot_type 26 approximation pairs); pointwise forces and
 potential against the oracle, full and planar orbits against the oracle."""
 
 import numpy
@@ -30,6 +31,12 @@ def _pot(which):
         Ac[:, :, 0] = _coeffs(6, 5, 1)[0][:, :, 0]
         Ac[:, 1::2, 0] = 0.0  # reflection-symmetric
         return [P.SCFPotential(amp=2.0, Acos=Ac, a=0.9), P.MiyamotoNagaiPotential(a=0.5, b=0.1, normalize=0.3)]
+    if which == "disk":
+        dens = lambda R, z: 13.5 * numpy.exp(-3.0 * R) * numpy.exp(-27.0 * numpy.fabs(z))
+        disk = P.DiskSCFPotential(dens=dens, Sigma=[{"type": "exp", "h": 1.0 / 3.0, "amp": 1.0},
+                                                   {"type": "expwhole", "h": 0.5, "amp": 0.2, "Rhole": 0.4}],
+                                  hz=[{"type": "exp", "h": 1.0 / 27.0}, {"type": "sech2", "h": 0.05}], N=8, L=8, a=1.0)
+        return [disk, P.NFWPotential(a=2.0, normalize=0.4)]
     Ac, As = _coeffs(5, 5, 2)
     if which == "nonaxi":
         return [P.SCFPotential(amp=2.0, Acos=Ac, Asin=As, a=0.9)]
@@ -38,7 +45,7 @@ def _pot(which):
                            Asin=lambda t: As * numpy.cos(0.3 * t), a=0.9, tgrid=tgrid)]
 
 
-@pytest.mark.parametrize("which", ["axi", "nonaxi", "tdep"])
+@pytest.mark.parametrize("which", ["axi", "nonaxi", "tdep", "disk"])
 def test_pointwise_forces_and_potential(which):
     import torch
 
@@ -61,7 +68,7 @@ def test_pointwise_forces_and_potential(which):
     assert numpy.abs(pot_d - want).max() <= 1e-13 * numpy.abs(want).max()
 
 
-@pytest.mark.parametrize("which", ["axi", "nonaxi", "tdep"])
+@pytest.mark.parametrize("which", ["axi", "nonaxi", "tdep", "disk"])
 @pytest.mark.parametrize("method,code,bar,factor", [("symplec4_hip", 3, 1e-10, 1e3),
                                                      ("dop853_hip", 6, 1e-8, 10.0)])
 def test_full_orbits(which, method, code, bar, factor):
@@ -78,11 +85,12 @@ def test_full_orbits(which, method, code, bar, factor):
     assert_parity(res, ref, bar, sens, factor=factor, label=f"{which} {method}")
 
 
-def test_planar_orbits():
+@pytest.mark.parametrize("which", ["nonaxi", "disk"])
+def test_planar_orbits(which):
     from galpy_amd import potential as P
     from galpy_amd.integrate import integratePlanarOrbit_hip
 
-    pot = _pot("nonaxi")
+    pot = _pot(which)
     _, pt, pa, _ = P.parse_pot_planar(pot)
     t = numpy.linspace(0.0, 10.0, 101)
     yp = planar_ics(128, seed=125)
