@@ -554,6 +554,38 @@ int decode_diskscf(const double*& pot_args, GhPotTable* T, int i, bool planar) {
   return 0;
 }
 
+// interpRZPotential (integrateFullOrbit.py:111-173; full orbits only, as in
+// galpy): nR, nz, three second-derivative flags, the grids, the B-spline
+// coefficients of the potential, R force, z force (and of the flagged
+// second derivatives, not needed here), amp, logR.
+int decode_interprz(const double*& pot_args, GhPotTable* T, int i, bool planar) {
+  GhComponent& C = T->comp[i];
+  const double* a = pot_args;
+  const int nR = (int)a[0], nz = (int)a[1];
+  const int nextra = (int)a[2] + (int)a[3] + (int)a[4];
+  if (planar) {
+    set_error("interpRZPotential has no planar version");
+    return -1;
+  }
+  const long n = (long)nR + nz + 3L * nR * nz;
+  if (nR < 2 || nz < 2 || T->next + n > GH_EXT_LEN) {
+    set_error("interpRZPotential grid %d x %d does not fit the potential table", nR, nz);
+    return -1;
+  }
+  C.type = GH_INTERPRZ;
+  const double* tail = a + 5 + nR + nz + (long)(3 + nextra) * nR * nz;
+  C.p[0] = tail[0];
+  C.p[1] = tail[1];
+  C.c[0] = nR;
+  C.c[1] = nz;
+  C.ext = T->next;
+  for (long k = 0; k < n; k++) T->ext[T->next + k] = a[5 + k];
+  T->next += (int)n;
+  T->has_wrap = 1;  // extended dispatch instances
+  pot_args = tail + 2;
+  return 0;
+}
+
 // One (leaf) component's arguments and precomputed constants.
 int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
                      bool planar) {
@@ -563,6 +595,7 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
   if (code == GH_INTERPSPH) return decode_interpsph(pot_args, T, i);
   if (code == GH_SCF) return decode_scf(pot_args, T, i);
   if (code == GH_DISKSCF) return decode_diskscf(pot_args, T, i, planar);
+  if (code == GH_INTERPRZ) return decode_interprz(pot_args, T, i, planar);
   const int na = nargs_of(code, planar);
   if (na < 0 || code < 0) {
     set_error("pot_type %d (component %d) has no HIP kernel", code, i);

@@ -90,6 +90,7 @@ enum GhPotCode {
   GH_INTERPSPH = 36,
   GH_SCF = 24,
   GH_DISKSCF = 26,
+  GH_INTERPRZ = 13,
 };
 
 struct GhComponent {
@@ -1440,6 +1441,71 @@ __device__ __forceinline__ void gh_comp_force<GH_DISKSCF>(const GhPotTable* __re
   F.Fz += f * g.z - S * dH;
 }
 
+// interpRZPotential.c:6-70: cubic B-spline interpolation on an (R or
+// log R, |z|) grid (util/interp_2d/interp_2d.c:92-114 and Thevenaz's
+// cubic_bspline_2d_interpol with mirror boundaries); R force, z force (odd
+// in z) and potential from their own coefficient grids at the same indices
+// and weights.  p: amp, logR; c[0] = nR, c[1] = nz; ext: R grid, z grid,
+// then the potential's, R force's and z force's coefficients [nR][nz].
+__device__ __forceinline__ int gh_mirror(long k, long n) {
+  const long n2 = 2L * n - 2L;
+  if (n == 1L) return 0;
+  k = (k < 0L) ? (-k - n2 * ((-k) / n2)) : (k - n2 * (k / n2));
+  return (int)((n <= k) ? n2 - k : k);
+}
+struct GhBspl {
+  int xi[4], yi[4];
+  double xw[4], yw[4];
+};
+__device__ __forceinline__ void gh_bspl_weights(double u, int n, int* idx, double* w) {
+  const long i = (long)floor(u) - 1L;
+  const double d = u - (double)(i + 1L);
+  w[3] = (1.0 / 6.0) * d * d * d;
+  w[0] = (1.0 / 6.0) + 0.5 * d * (d - 1.0) - w[3];
+  w[2] = d + w[0] - 2.0 * w[3];
+  w[1] = 1.0 - w[0] - w[2] - w[3];
+  for (int k = 0; k < 4; k++) idx[k] = gh_mirror(i + k, n);
+}
+__device__ __forceinline__ GhBspl gh_interprz_setup(const GhPotTable* __restrict__ T,
+                                                    const GhComponent* __restrict__ C, double R,
+                                                    double z) {
+  const int nR = (int)C->c[0], nz = (int)C->c[1];
+  const double* __restrict__ Rg = &T->ext[C->ext];
+  const double* __restrict__ zg = Rg + nR;
+  double x = (C->p[1] == 1.0) ? ((R > 0.0) ? log(R) : -20.72326583694641) : R;
+  double y = fabs(z);
+  x = x > Rg[nR - 1] ? Rg[nR - 1] : (x < Rg[0] ? Rg[0] : x);
+  y = y > zg[nz - 1] ? zg[nz - 1] : (y < zg[0] ? zg[0] : y);
+  const int ix = gh_spline_find(Rg, nR, x), iy = gh_spline_find(zg, nz, y);
+  GhBspl B;
+  gh_bspl_weights(ix + (x - Rg[ix]) / (Rg[ix + 1] - Rg[ix]), nR, B.xi, B.xw);
+  gh_bspl_weights(iy + (y - zg[iy]) / (zg[iy + 1] - zg[iy]), nz, B.yi, B.yw);
+  return B;
+}
+__device__ __forceinline__ double gh_interprz_sum(const GhPotTable* __restrict__ T,
+                                                  const GhComponent* __restrict__ C,
+                                                  const GhBspl& B, int grid) {
+  const int nR = (int)C->c[0], nz = (int)C->c[1];
+  const double* __restrict__ co = &T->ext[C->ext + nR + nz + grid * nR * nz];
+  double out = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) out += co[B.xi[a] * nz + B.yi[b]] * B.xw[a] * B.yw[b];
+  return out;
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_INTERPRZ>(const GhPotTable* __restrict__ T, int i,
+                                                           const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double R = sqrt(g.R2);
+  const GhBspl B = gh_interprz_setup(T, C, R, g.z);
+  const double amp = C->p[0];
+  F.FRoR += amp * gh_interprz_sum(T, C, B, 1) / R;
+  const double fz = amp * gh_interprz_sum(T, C, B, 2);
+  F.Fz += (g.z < 0.0) ? -fz : fz;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_NULLPOT>(const GhPotTable* __restrict__, int,
                                                           const GhGeom&, GhAcc&) {}
@@ -1486,7 +1552,7 @@ __host__ __device__ inline bool gh_is_heavy(int type) {
          type == GH_TWOPOWERTRIAX || type == GH_PERFECTELLIPSOID || type == GH_TRIAXGAUSSIAN ||
          type == GH_POWERTRIAX || type == GH_EINASTO || type == GH_TWOPOWERSPH ||
          type == GH_EXPTRUNCNFW || type == GH_SPIRALARMS || type == GH_DEDISK ||
-         type == GH_INTERPSPH || type == GH_SCF || type == GH_DISKSCF;
+         type == GH_INTERPSPH || type == GH_SCF || type == GH_DISKSCF || type == GH_INTERPRZ;
 }
 
 // run-time dispatch over every component type (extended instances)
@@ -1508,6 +1574,7 @@ __device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__
     case GH_INTERPSPH: gh_comp_force<GH_INTERPSPH>(T, i, g, F); break;
     case GH_SCF: gh_comp_force<GH_SCF>(T, i, g, F); break;
     case GH_DISKSCF: gh_comp_force<GH_DISKSCF>(T, i, g, F); break;
+    case GH_INTERPRZ: gh_comp_force<GH_INTERPRZ>(T, i, g, F); break;
     default: gh_comp_force_dyn(T, i, g, F); break;
   }
 }
@@ -2065,6 +2132,9 @@ __device__ __forceinline__ double gh_comp_potential(const GhPotTable* __restrict
     case GH_STEADYSPIRAL:  // SteadyLogSpiralPotential.py _evaluate (planar only)
       pot += C->p[0] * gh_ellipdisk_smooth(t, C->p[1], C->p[2]) * C->p[3] / C->p[4] *
              cos(gh_spiral_arg(C, R2, atan2(sinphi, cosphi), t));
+      break;
+    case GH_INTERPRZ:  // interpRZPotential.c:6-20 (Eval)
+      pot += C->p[0] * gh_interprz_sum(T, C, gh_interprz_setup(T, C, R, z), 0);
       break;
     case GH_DISKSCF: {  // DiskSCFPotential.c:108-120 (Eval)
       double S, dS, H, dH;

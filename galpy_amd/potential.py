@@ -1588,7 +1588,8 @@ class DiskSCFPotential(Potential):
             H = 0.5 * (numpy.exp(-fz / hh) - 1.0 + fz / hh) * hh
             dH = 0.5 * numpy.sign(z) * (1.0 - numpy.exp(-fz / hh))
         else:
-            hzv = 0.25 / numpy.cosh(0.5 * z / hh) ** 2 / hh
+            ex = numpy.exp(-fz / hh)  # 1/cosh^2(z/2h) = 4 e^(-|z|/h) / (1 + e^(-|z|/h))^2
+            hzv = ex / (1.0 + ex) ** 2 / hh
             H = hh * (numpy.log1p(numpy.exp(-fz / hh)) + 0.5 * fz / hh - numpy.log(2.0))
             dH = 0.5 * numpy.tanh(0.5 * z / hh)
         return S, dS, d2S, hzv, H, dH
@@ -1634,6 +1635,64 @@ class DiskSCFPotential(Potential):
                 pa.extend([4, 1, famp, Sg.get("h", 1.0 / 3.0), Sg.get("Rhole", 0.5)])
             pa.extend([0 if hz.get("type", "exp") == "exp" else 1, hz.get("h", 0.0375)])
         return pt, pa, len(pt)
+
+
+class interpRZPotential(Potential):
+    """An axisymmetric potential interpolated on an (R, z) grid
+    (interpRZPotential.py:30-460) with cubic B-splines (mirror boundaries),
+    as galpy's C evaluates it; pot_type 13, args [nR, nz, 0, 0, 0, the R
+    grid (log R when logR), the z grid, the B-spline coefficients of the
+    potential, R force and z force, amp, logR] (integrateFullOrbit.py:111-173;
+    the second-derivative grids are not built here).  rgrid and zgrid are
+    (start, stop, n) for numpy.linspace; z >= 0 (the potential is symmetric
+    in z).  Full orbits only, as in galpy."""
+
+    pot_code = 13
+
+    def __init__(self, RZPot=None, rgrid=(numpy.log(0.01), numpy.log(20.0), 101), zgrid=(0.0, 1.0, 101),
+                 logR=False):
+        from scipy import ndimage
+
+        Potential.__init__(self, amp=1.0)
+        self._logR = bool(logR)
+        self._rg = numpy.linspace(*rgrid)
+        self._zgrid = numpy.linspace(*zgrid)
+        self._rgrid = numpy.exp(self._rg) if self._logR else self._rg
+        RR, ZZ = numpy.meshgrid(self._rgrid, self._zgrid, indexing="ij")
+        grids = [evaluatePotentials(RZPot, RR, ZZ), evaluateRforces(RZPot, RR, ZZ), evaluatezforces(RZPot, RR, ZZ)]
+        self._coeffs = [ndimage.spline_filter(numpy.asarray(g, dtype=float), order=3, mode="mirror") for g in grids]
+
+    def _interp(self, k, R, z):
+        from scipy import ndimage
+
+        R, z = numpy.broadcast_arrays(numpy.asarray(R, dtype=float), numpy.asarray(z, dtype=float))
+        x = numpy.where(R > 0.0, numpy.log(numpy.where(R > 0.0, R, 1.0)), -20.72326583694641) if self._logR else R
+        x = numpy.clip(x, self._rg[0], self._rg[-1])
+        y = numpy.clip(numpy.fabs(z), self._zgrid[0], self._zgrid[-1])
+        ix = numpy.clip(numpy.searchsorted(self._rg, x, side="right") - 1, 0, len(self._rg) - 2)
+        iy = numpy.clip(numpy.searchsorted(self._zgrid, y, side="right") - 1, 0, len(self._zgrid) - 2)
+        xn = ix + (x - self._rg[ix]) / (self._rg[ix + 1] - self._rg[ix])
+        yn = iy + (y - self._zgrid[iy]) / (self._zgrid[iy + 1] - self._zgrid[iy])
+        out = ndimage.map_coordinates(self._coeffs[k], [xn.ravel(), yn.ravel()], order=3, mode="mirror",
+                                      prefilter=False).reshape(R.shape)
+        return out if out.shape else float(out)
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return self._interp(0, R, z)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return self._interp(1, R, z)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        f = self._interp(2, R, z)
+        return numpy.where(numpy.asarray(z) < 0.0, -f, f) if numpy.ndim(f) else (-f if z < 0.0 else f)
+
+    def c_args(self):
+        return ([float(len(self._rg)), float(len(self._zgrid)), 0.0, 0.0, 0.0] + list(self._rg) + list(self._zgrid)
+                + [v for c in self._coeffs for v in c.flatten(order="C")] + [self._amp, float(self._logR)])
+
+    def c_args_planar(self):
+        raise NotImplementedError("interpRZPotential has no planar C implementation")
 
 
 class FlattenedPowerPotential(Potential):

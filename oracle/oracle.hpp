@@ -241,7 +241,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
                kTimeDepAmpWrap = -9, kMovingObjectWrap = -6, kInterpSph = 36, kSCF = 24,
-               kDiskSCF = 26,
+               kDiskSCF = 26, kInterpRZ = 13,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
                kTriaxHernquist = 21, kTriaxNFW = 22, kTriaxJaffe = 23, kPerfectEllipsoid = 30,
                kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43,
@@ -409,6 +409,15 @@ inline int dedisk_nargs(const double* a) { return 5 + 4 * (int)a[4]; }
 // interpSphericalPotential: nr, rgrid[nr], rforce[nr], amp, rmin, rmax,
 // M(<rmax), Phi0, Phimax (integrateFullOrbit.py:366-373)
 inline int interpsph_nargs(const double* a) { return 1 + 2 * (int)a[0] + 6; }
+// interpRZPotential: nR, nz, three flags for second-derivative grids, the R
+// (or log R) grid, the z grid, the B-spline coefficients of the potential,
+// R force and z force (and of each flagged second derivative), amp, logR
+// (integrateFullOrbit.py:111-173)
+inline int interprz_nargs(const double* a) {
+  const int nR = (int)a[0], nz = (int)a[1];
+  const int nextra = (int)a[2] + (int)a[3] + (int)a[4];
+  return 5 + nR + nz + (3 + nextra) * nR * nz + 2;
+}
 // SCFPotential: a, isNonAxi, N, L, M, Nt, then Acos[N L M] (and Asin when
 // non-axisymmetric), or tgrid[Nt] and the time-PPoly blocks [Nt-1][NLM][4]
 // of Acos (and Asin), then 11 cache slots (integrateFullOrbit.c nargs of
@@ -443,6 +452,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
              : code == kInterpSph ? interpsph_nargs(pot_args)
              : code == kSCF ? scf_nargs(pot_args)
              : code == kDiskSCF ? (int)pot_args[0] + 3
+             : code == kInterpRZ ? interprz_nargs(pot_args)
              : code == kMovingObjectWrap ? 0  // counted after its wrapped components
                                    : component_nargs(code, planar);
     if (na < 0) return -1;
@@ -486,7 +496,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       continue;
     }
     if (is_ellipsoidal(code) || code == kSpiralArms || code == kDoubleExpDisk || code == kSCF ||
-        code == kDiskSCF) {
+        code == kDiskSCF || code == kInterpRZ) {
       C.p[0] = pot_args[0];
       pot_args += na;
       continue;
@@ -1344,6 +1354,67 @@ template <class R> void dscf_forces(const Component& C, R Rc, R z, R& FR, R& Fz)
   Fz = -dS * H * z / r - S * dH;
 }
 
+// interpRZPotential.c:6-70 with interp_2d_eval_cubic_bspline
+// (util/interp_2d/interp_2d.c:92-114) and Thevenaz's cubic B-spline
+// interpolation with mirror boundaries (cubic_bspline_2d_interpol.c): the
+// point clamped to the grid, its fractional grid index from the bracketing
+// interval (binary search), the 4 x 4 weighted coefficients.
+inline int interprz_find(const double* x, int n, double v) {
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (x[mid] > v) hi = mid; else lo = mid;
+  }
+  return lo;
+}
+inline double bspline2d(const double* coeffs, long width, long height, double x, double y) {
+  long xi[4], yi[4];
+  double xw[4], yw[4];
+  const long width2 = 2L * width - 2L, height2 = 2L * height - 2L;
+  long i = (long)std::floor(x) - 1L, j = (long)std::floor(y) - 1L;
+  for (int k = 0; k < 4; k++) { xi[k] = i++; yi[k] = j++; }
+  double w = x - (double)xi[1];
+  xw[3] = (1.0 / 6.0) * w * w * w;
+  xw[0] = (1.0 / 6.0) + (1.0 / 2.0) * w * (w - 1.0) - xw[3];
+  xw[2] = w + xw[0] - 2.0 * xw[3];
+  xw[1] = 1.0 - xw[0] - xw[2] - xw[3];
+  w = y - (double)yi[1];
+  yw[3] = (1.0 / 6.0) * w * w * w;
+  yw[0] = (1.0 / 6.0) + (1.0 / 2.0) * w * (w - 1.0) - yw[3];
+  yw[2] = w + yw[0] - 2.0 * yw[3];
+  yw[1] = 1.0 - yw[0] - yw[2] - yw[3];
+  for (int k = 0; k < 4; k++) {
+    xi[k] = (width == 1L) ? 0L : ((xi[k] < 0L) ? (-xi[k] - width2 * ((-xi[k]) / width2)) : (xi[k] - width2 * (xi[k] / width2)));
+    if (width <= xi[k]) xi[k] = width2 - xi[k];
+    yi[k] = (height == 1L) ? 0L : ((yi[k] < 0L) ? (-yi[k] - height2 * ((-yi[k]) / height2)) : (yi[k] - height2 * (yi[k] / height2)));
+    if (height <= yi[k]) yi[k] = height2 - yi[k];
+  }
+  double out = 0.0;
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) out += coeffs[xi[a] * height + yi[b]] * xw[a] * yw[b];
+  return out;
+}
+// grid: 0 potential, 1 R force, 2 z force
+inline double interprz_eval(const Component& C, int grid, double R, double z) {
+  const double* a = C.args;
+  const int nR = (int)a[0], nz = (int)a[1];
+  const double* Rg = a + 5;
+  const double* zg = Rg + nR;
+  const double* co = zg + nz + (size_t)grid * nR * nz;
+  const double* tail = zg + nz + (size_t)(3 + (int)a[2] + (int)a[3] + (int)a[4]) * nR * nz;
+  const int logR = (int)tail[1];
+  double x = logR == 1 ? (R > 0. ? std::log(R) : -20.72326583694641) : R;
+  double y = std::fabs(z);
+  x = x > Rg[nR - 1] ? Rg[nR - 1] : x;
+  x = x < Rg[0] ? Rg[0] : x;
+  y = y > zg[nz - 1] ? zg[nz - 1] : y;
+  y = y < zg[0] ? zg[0] : y;
+  const int ix = interprz_find(Rg, nR, x), iy = interprz_find(zg, nz, y);
+  const double xn = ix + (x - Rg[ix]) / (Rg[ix + 1] - Rg[ix]);
+  const double yn = iy + (y - zg[iy]) / (zg[iy + 1] - zg[iy]);
+  return tail[0] * bspline2d(co, nR, nz, xn, yn);
+}
+
 // KuzminLikeWrapperPotential.c:5-8: xi = sqrt(R^2 + (a + sqrt(z^2 + b^2))^2)
 template <class R> R kuzminlike_xi(R Rc, R z, double a, double b2) {
   R asqrtbz = R(a) + sqrt(z * z + R(b2));
@@ -1421,6 +1492,12 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kDoubleExpDisk: dedisk_forces(C, Rc, z, fr, fz); break;
       case kSCF: scf_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kDiskSCF: dscf_forces(C, Rc, z, fr, fz); break;
+      case kInterpRZ: {  // interpRZPotential.c:22-70
+        fr = R(interprz_eval(C, 1, dval(Rc), dval(z)));
+        const double fzv = interprz_eval(C, 2, dval(Rc), dval(z));
+        fz = R(dval(z) < 0. ? -fzv : fzv);
+        break;
+      }
       case kInterpSph: {
         R r = sqrt(Rc * Rc + z * z);
         R f = R(C.p[0]) * interpsph_rforce(C, r);
@@ -2119,7 +2196,7 @@ inline bool has_hessian(int npot, const Component* P) {
     const int c = P[i].type;
     if (c == kKKStaeckel || c == kSoftNeedleBar || is_wrapper(c) || c == kEllipDisk || c == kLopsided ||
         c == kSteadySpiral || c == kTransientSpiral || c == kCosmphiDisk || c == kHenonHeiles ||
-        c == kSpiralArms || c == kDoubleExpDisk || c == kInterpSph || c == kSCF || c == kDiskSCF || is_ellipsoidal(c))
+        c == kSpiralArms || c == kDoubleExpDisk || c == kInterpSph || c == kSCF || c == kDiskSCF || c == kInterpRZ || is_ellipsoidal(c))
       return false;
   }
   return true;
@@ -2212,6 +2289,7 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kSpiralArms: pot += spiral_eval(C, Rc, z, phi, t); break;
       case kDoubleExpDisk: pot += dedisk_eval(C, Rc, z); break;
       case kSCF: pot += scf_eval(C, Rc, z, phi, t); break;
+      case kInterpRZ: pot += R(interprz_eval(C, 0, dval(Rc), dval(z))); break;  // interpRZPotential.c:6-20
       case kDiskSCF: {  // DiskSCFPotential.c:108-120 (Eval)
         const double* a = C.args;
         R S, dS, H, dH;

@@ -67,6 +67,9 @@ def _nargs(code, pot_args, ai, planar):
     if code == 24:  # SCFPotential
         nonaxi, NLM, Nt = int(pot_args[ai + 1]), int(pot_args[ai + 2] * pot_args[ai + 3] * pot_args[ai + 4]), int(pot_args[ai + 5])
         return 6 + (1 + nonaxi) * NLM + 11 if Nt == 0 else 6 + Nt + (1 + nonaxi) * NLM * 4 * (Nt - 1) + 11
+    if code == 13:  # interpRZPotential
+        nR, nz = int(pot_args[ai]), int(pot_args[ai + 1])
+        return 5 + nR + nz + (3 + int(pot_args[ai + 2] + pot_args[ai + 3] + pot_args[ai + 4])) * nR * nz + 2
     if code == 26:  # DiskSCFPotential pair: nsigma + 3
         return int(pot_args[ai]) + 3
     if code == 36:  # interpSphericalPotential: 1 + 2 nr + 6
