@@ -112,6 +112,7 @@ int nargs_of(int code, bool planar) {
     case GH_ROTATETILTWRAP: return planar ? -1 : 32;
     case GH_KUZMINLIKEWRAP: return 3;  // amp, a, b^2
     case GH_CYLSEPWRAP: return 3;      // amp, Rp, Phi(Rp, 0)
+    case GH_OBLATESTAECKELWRAP: return 5;  // amp, delta, u0, v0, refpot
     case GH_TIMEDEPAMPWRAP: return 1;  // amp (and one time function)
     default: return -1;
   }
@@ -586,6 +587,88 @@ int decode_interprz(const double*& pot_args, GhPotTable* T, int i, bool planar) 
   return 0;
 }
 
+// MultipoleExpansionPotential (MultipoleExpansionPotential.c:183-399; packing
+// MultipoleExpansionPotential.py:2101-2216, the same in both packings): Nr,
+// L, M, isNonAxi, rgrid[Nr], amp, Nt, then per (l, m < min(l + 1, M)) and
+// trig (cos; sin when m > 0) either the quintic coefficients of I_inner and
+// I_outer [Nr-1][6] and the density at the nodes [Nr] (static), or, after
+// tgrid[Nt], their cubic-in-t blocks [Nt-1][6 (Nr-1)][4] and the density's
+// [Nt-1][Nr][4].  Re-laid out interval-major for the device (one radial
+// interval's coefficients of every block contiguous); the density is not
+// needed for forces.
+int decode_multipole(const double*& pot_args, GhPotTable* T, int i) {
+  GhComponent& C = T->comp[i];
+  const double* a = pot_args;
+  const int Nr = (int)a[0], L = (int)a[1], M = (int)a[2], nonaxi = (int)a[3];
+  if (Nr < 2 || L < 1 || M < 1 || (!nonaxi && M != 1)) {
+    set_error("MultipoleExpansionPotential Nr=%d L=%d M=%d isNonAxi=%d: unsupported shape", Nr, L, M,
+              nonaxi);
+    return -1;
+  }
+  const double* rg = a + 4;
+  const double amp = rg[Nr];
+  const int Nt = (int)rg[Nr + 1];
+  if (Nt == 1 || Nt < 0) {
+    set_error("MultipoleExpansionPotential with Nt=%d time nodes", Nt);
+    return -1;
+  }
+  int nblk = 0;
+  for (int l = 0; l < L; l++) nblk += 2 * std::min(l + 1, M) - 1;
+  const long nr = 6L * (Nr - 1);
+  const long stride = Nt > 0 ? 48 : 12;
+  const long nint = Nt > 0 ? (long)(Nt - 1) * (Nr - 1) : (long)(Nr - 1);
+  const long need = Nr + (Nt > 0 ? Nt : 0) + nint * nblk * stride;
+  if (T->next + need > GH_EXT_LEN) {
+    set_error("MultipoleExpansionPotential coefficients (%ld doubles) do not fit the potential table", need);
+    return -1;
+  }
+  C.type = GH_MULTIPOLE;
+  C.p[0] = amp;
+  C.c[0] = Nr;
+  C.c[1] = L;
+  C.c[2] = M;
+  C.c[3] = Nt;
+  C.c[4] = rg[0];
+  C.c[5] = rg[Nr - 1];
+  C.c[6] = nblk;
+  C.ext = T->next;
+  double* e = T->ext + T->next;
+  for (int k = 0; k < Nr; k++) e[k] = rg[k];
+  const double* src = rg + Nr + 2;
+  double* dst = e + Nr;
+  if (Nt > 0) {
+    for (int k = 0; k < Nt; k++) dst[k] = src[k];
+    src += Nt;
+    dst += Nt;
+    const long per = (long)(Nt - 1) * (2 * nr + Nr) * 4;  // one block's doubles in the packing
+    for (int k = 0; k < nblk; k++, src += per)
+      for (int it = 0; it < Nt - 1; it++)
+        for (int ir = 0; ir < Nr - 1; ir++) {
+          double* d = dst + (((long)it * (Nr - 1) + ir) * nblk + k) * 48;
+          for (int half = 0; half < 2; half++) {  // I_inner, I_outer
+            const double* s0 = src + half * (long)(Nt - 1) * nr * 4 + ((long)it * nr + 6L * ir) * 4;
+            for (int j = 0; j < 24; j++) d[half * 24 + j] = s0[j];
+          }
+        }
+  } else {
+    const long per = 2 * nr + Nr;
+    for (int k = 0; k < nblk; k++, src += per)
+      for (int ir = 0; ir < Nr - 1; ir++) {
+        double* d = dst + ((long)ir * nblk + k) * 12;
+        for (int j = 0; j < 6; j++) {
+          d[j] = src[6L * ir + j];
+          d[6 + j] = src[nr + 6L * ir + j];
+        }
+      }
+  }
+  T->next += (int)need;
+  C.nonaxi = nonaxi;
+  T->nonaxi |= nonaxi;
+  T->has_wrap = 1;  // extended dispatch instances
+  pot_args = src;
+  return 0;
+}
+
 // One (leaf) component's arguments and precomputed constants.
 int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, int& nslots,
                      bool planar) {
@@ -596,6 +679,7 @@ int decode_component(int code, const double*& pot_args, GhPotTable* T, int i, in
   if (code == GH_SCF) return decode_scf(pot_args, T, i);
   if (code == GH_DISKSCF) return decode_diskscf(pot_args, T, i, planar);
   if (code == GH_INTERPRZ) return decode_interprz(pot_args, T, i, planar);
+  if (code == GH_MULTIPOLE) return decode_multipole(pot_args, T, i);
   const int na = nargs_of(code, planar);
   if (na < 0 || code < 0) {
     set_error("pot_type %d (component %d) has no HIP kernel", code, i);
@@ -863,6 +947,7 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
     const int w = T->nwraps++;
     GhWrap& W = T->wrap[w];
     W.kind = code;
+    W.first = first;
     if (code == GH_MOVINGOBJWRAP) {
       // initMovingObjectSplines (integrateFullOrbit.c:1392-1431; planar
       // integratePlanarOrbit.c:1177-1215): nPts, t, x, y[, z], amp, t0, tf;
@@ -900,6 +985,13 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
       if (build_tfunc_table(tf->fns[tf->used++], tf->ta, tf->tb, T, off) != 0) return -1;
       W.p[1] = off;
     }
+    if (code == GH_OBLATESTAECKELWRAP) {
+      W.p[5] = planar ? 1.0 : 0.0;  // the planar force (no V term)
+      if (!(W.p[1] > 0.0)) {
+        set_error("OblateStaeckelWrapperPotential delta=%g must be positive", W.p[1]);
+        return -1;
+      }
+    }
     if (code == GH_GAUSSIANAMPWRAP && !(W.p[2] > 0.0)) {
       set_error("GaussianAmplitudeWrapperPotential sigma^2=%g must be positive", W.p[2]);
       return -1;
@@ -931,18 +1023,24 @@ int decode_potential(int npot, const int* pot_type, const double* pot_args, GhPo
   // (two passes under a CylindricallySeparable wrapper)
   for (int l = 0; l < nleaf; l++) {
     GhComponent& C = T->comp[l];
-    int moved = 0, cylsep = 0;
+    int moved = 0, cylsep = 0, oblate = 0;
     for (int k = 0; k < C.nwrap; k++) {
       const int kind = T->wrap[C.wrap[k]].kind;
       moved |= kind == GH_ROTATETILTWRAP || kind == GH_KUZMINLIKEWRAP || kind == GH_CYLSEPWRAP ||
-               kind == GH_MOVINGOBJWRAP;
+               kind == GH_MOVINGOBJWRAP || kind == GH_OBLATESTAECKELWRAP;
       cylsep += kind == GH_CYLSEPWRAP;
+      oblate += kind == GH_OBLATESTAECKELWRAP;
+      if (kind == GH_OBLATESTAECKELWRAP && k != C.nwrap - 1) {
+        set_error("an OblateStaeckelWrapperPotential around another wrapper is not supported");
+        return -1;
+      }
     }
-    if (cylsep > 1) {
-      set_error("a component under more than one CylindricallySeparablePotentialWrapper");
+    if (cylsep + oblate > 1) {
+      set_error("a component under more than one CylindricallySeparable or OblateStaeckel wrapper");
       return -1;
     }
-    C.mapwrap = moved ? 1 + cylsep : 0;
+    C.mapwrap = moved ? 1 + cylsep + oblate : 0;
+    C.mappot = oblate;
     if (moved) T->nonaxi = 1;
   }
   return 0;
@@ -1166,6 +1264,7 @@ int host_integrate(int nobj, double* yo, int nt, double* t, int indiv_t, int npo
   struct sigaction old_action, action;
   std::memset(&action, 0, sizeof(action));
   action.sa_handler = on_sigint;
+  action.sa_flags = SA_RESTART;  // the runtime's own waits resume after the handler
   irq_device_ptr();
   if (g_irq_host) *g_irq_host = 0;
   sigaction(SIGINT, &action, &old_action);

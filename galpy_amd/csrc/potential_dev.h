@@ -34,7 +34,7 @@
 // prefix (gh_table_bytes), so the bound costs nothing when unused; it fits
 // a DoubleExponentialDiskPotential at galpy's default de_n = 10000 (4 x
 // 10000 nodes and weights) with room to spare.
-#define GH_EXT_LEN 65536
+#define GH_EXT_LEN (1 << 24)
 
 enum GhPotCode {
   GH_LOGHALO = 0,
@@ -91,6 +91,8 @@ enum GhPotCode {
   GH_SCF = 24,
   GH_DISKSCF = 26,
   GH_INTERPRZ = 13,
+  GH_MULTIPOLE = 44,
+  GH_OBLATESTAECKELWRAP = -3,
 };
 
 struct GhComponent {
@@ -100,6 +102,7 @@ struct GhComponent {
   int wrap[GH_WRAP_DEPTH];  // their indices in GhPotTable::wrap, outermost first
   int ext;       // offset of the component's variable-length data in GhPotTable::ext
   int mapwrap;   // wrappers that move the evaluation point: 0 none, else passes (1 or 2)
+  int mappot;    // the mapped walk needs the component's potential too (OblateStaeckel)
   double p[8];   // raw galpy arguments, in galpy's order
   double c[14];  // precomputed constants, per type (decoded on the host)
 };
@@ -111,7 +114,7 @@ struct GhComponent {
 // leaf component lists the wrappers around it.
 struct GhWrap {
   int kind;
-  int pad;
+  int first;     // index of the first component it wraps (its constant terms are added there)
   double p[16];  // RotateAndTilt: amp, rot[9], rotSet, offsetSet, offset[3]
 };
 
@@ -1506,6 +1509,143 @@ __device__ __forceinline__ void gh_comp_force<GH_INTERPRZ>(const GhPotTable* __r
   F.Fz += (g.z < 0.0) ? -fz : fz;
 }
 
+// MultipoleExpansionPotential.c:411-507, 719-1029: Phi = amp sum_lm
+// P_l^m(cos theta) [R^c_lm(r) cos m phi + R^s_lm(r) sin m phi] with R_lm =
+// r^-(l+1) I_inner(r) + r^l I_outer(r), the cumulative integrals quintic
+// piecewise polynomials on the radial grid (value, first and second
+// derivative matched at the nodes); below the grid the density is held at
+// its innermost value, above it only I_inner(rmax) r^-(l+1) remains.
+// p[0] = amp; c: Nr, L, M (1 when axisymmetric), Nt, rmin, rmax, nblk (the
+// (l, m, cos|sin) blocks, l-major, cos before sin).  ext: rgrid[Nr]
+// (tgrid[Nt]), then per radial interval (and time interval, outermost) and
+// block the 12 coefficients of I_inner and I_outer, highest power first, or
+// with a time grid their 48 cubic-in-t coefficients [12][4].  Summed
+// m-outer with register recurrences for P_l^m (Condon-Shortley phase),
+// dP_l^m/dx, cos/sin m phi and r^l, r^-(l+1).
+// Not inlined: one copy serves every call site of the extended instances.
+struct GhMpSums {
+  double F0, F1, F2, V;
+};
+__device__ __noinline__ GhMpSums gh_multipole_sums(const GhPotTable* __restrict__ T,
+                                                   const GhComponent* __restrict__ C, double R,
+                                                   double z, double cphi, double sphi, double t,
+                                                   bool POT) {
+  GhMpSums out = {0.0, 0.0, 0.0, 0.0};
+  const int Nr = (int)C->c[0], L = (int)C->c[1], Meff = (int)C->c[2], Nt = (int)C->c[3];
+  const int nblk = (int)C->c[6];
+  const double rmin = C->c[4], rmax = C->c[5];
+  const double* __restrict__ rg = &T->ext[C->ext];
+  const double* __restrict__ blk = rg + Nr;
+  const double r = sqrt(R * R + z * z);
+  if (!(r > 0.0) || !isfinite(r)) return out;
+  const double ir = 1.0 / r, x = z * ir, sth = R * ir;
+  const int region = (r < rmin) ? 0 : ((r <= rmax) ? 1 : 2);
+  int ii = 0;
+  double dr = 0.0;
+  if (region == 1) {
+    ii = gh_spline_find(rg, Nr, r);
+    dr = r - rg[ii];
+  } else if (region == 2) {
+    ii = Nr - 2;
+    dr = rmax - rg[Nr - 2];
+  }
+  int it = 0;
+  double dt = 0.0;
+  const int stride = (Nt > 0) ? 48 : 12;
+  if (Nt > 0) {
+    const double* __restrict__ tg = blk;
+    it = gh_spline_find(tg, Nt, t);
+    dt = t - tg[it];
+    blk = tg + Nt + (size_t)it * (Nr - 1) * nblk * 48;
+  }
+  blk += (size_t)ii * nblk * stride;
+  const double ix2m1 = 1.0 / (x * x - 1.0);
+  double S0 = 0.0, S1 = 0.0, S2 = 0.0, pot = 0.0;
+  double cm = 1.0, sm = 0.0, pmm = 1.0, rm = 1.0, rnm = ir;
+  for (int m = 0; m < Meff; m++) {
+    if (m > 0) {
+      const double c2 = cm * cphi - sm * sphi;
+      sm = sm * cphi + cm * sphi;
+      cm = c2;
+      pmm *= -(2.0 * m - 1.0) * sth;
+      rm *= r;
+      rnm *= ir;
+    }
+    double plm1 = 0.0, plm2 = 0.0, rl = rm, rnl = rnm;
+    for (int l = m; l < L; l++) {
+      double pl;
+      if (l == m) pl = pmm;
+      else if (l == m + 1) pl = (2.0 * m + 1.0) * x * pmm;
+      else pl = ((2.0 * l - 1.0) * x * plm1 - (l + m - 1.0) * plm2) / (l - m);
+      const double dpl = (l == 0) ? 0.0 : (l * x * pl - (l + m) * ((l == m) ? 0.0 : plm1)) * ix2m1;
+      if (l > m) plm2 = plm1;
+      plm1 = pl;
+      // block of (l, m, cos): sum over l' < l of 2 min(l' + 1, M) - 1, then 2 m - 1
+      const int lb = (l <= Meff) ? l * l : Meff * Meff + (l - Meff) * (2 * Meff - 1);
+      const int k0 = lb + ((m == 0) ? 0 : 2 * m - 1);
+      for (int trig = 0; trig < ((m == 0) ? 1 : 2); trig++) {
+        const double* __restrict__ q = blk + (size_t)(k0 + trig) * stride;
+        double c[12];
+#pragma unroll
+        for (int j = 0; j < 12; j++)
+          c[j] = (Nt > 0) ? ((q[4 * j] * dt + q[4 * j + 1]) * dt + q[4 * j + 2]) * dt + q[4 * j + 3] : q[j];
+        double Rv, dRv;
+        if (region == 1) {
+          const double Ii = ((((c[0] * dr + c[1]) * dr + c[2]) * dr + c[3]) * dr + c[4]) * dr + c[5];
+          const double Io = ((((c[6] * dr + c[7]) * dr + c[8]) * dr + c[9]) * dr + c[10]) * dr + c[11];
+          Rv = rnl * Ii + rl * Io;
+          if (!POT) {
+            const double dIi = (((5.0 * c[0] * dr + 4.0 * c[1]) * dr + 3.0 * c[2]) * dr + 2.0 * c[3]) * dr + c[4];
+            const double dIo = (((5.0 * c[6] * dr + 4.0 * c[7]) * dr + 3.0 * c[8]) * dr + 2.0 * c[9]) * dr + c[10];
+            dRv = -(l + 1) * rnl * ir * Ii + rnl * dIi + l * rl * ir * Io + rl * dIo;
+          }
+        } else if (region == 2) {
+          const double Ii = ((((c[0] * dr + c[1]) * dr + c[2]) * dr + c[3]) * dr + c[4]) * dr + c[5];
+          Rv = Ii * rnl;
+          dRv = -(l + 1) * Ii * rnl * ir;
+        } else {
+          const double P0 = c[4] / pow(rmin, l + 2.0);
+          const double Iie = P0 / (l + 3.0) * pow(r, l + 3.0);
+          const double extra = (l == 2) ? P0 * log(rmin / r)
+                                        : P0 / (2.0 - l) * (pow(rmin, 2.0 - l) - pow(r, 2.0 - l));
+          const double Ioe = c[11] + extra;
+          const double a = pow(r, -(l + 1.0)), b = pow(r, (double)l);
+          Rv = a * Iie + b * Ioe;
+          dRv = -(l + 1) * a * ir * Iie + l * b * ir * Ioe;
+        }
+        const double cs = trig ? sm : cm, sn = trig ? -cm : sm;  // d(cs)/dphi = -m sn
+        if (POT) {
+          pot += pl * Rv * cs;
+        } else {
+          S0 += pl * cs * dRv;
+          S1 += dpl * cs * Rv;
+          S2 -= pl * m * sn * Rv;
+        }
+      }
+      rl *= r;
+      rnl *= ir;
+    }
+  }
+  const double amp = C->p[0];
+  out.F0 = -amp * S0;       // -dPhi/dr
+  out.F1 = amp * S1 * sth;  // -dPhi/dtheta (dP/dtheta = -sin theta dP/dx)
+  out.F2 = -amp * S2;       // -dPhi/dphi
+  out.V = amp * pot;
+  return out;
+}
+template <>
+__device__ __forceinline__ void gh_comp_force<GH_MULTIPOLE>(const GhPotTable* __restrict__ T, int i,
+                                                            const GhGeom& g, GhAcc& F) {
+  const GhComponent* __restrict__ C = &T->comp[i];
+  const double R = sqrt(g.R2);
+  const double cp = (R > 0.0) ? g.x / R : 1.0, sp = (R > 0.0) ? g.y / R : 0.0;
+  const GhMpSums S = gh_multipole_sums(T, C, R, g.z, cp, sp, g.t, false);
+  const double ir2 = g.rinv * g.rinv, iR = 1.0 / R;
+  F.FRoR += (S.F0 * g.rinv + S.F1 * g.z * ir2 * iR);  // F_R / R
+  F.Fz += S.F0 * g.z * g.rinv - S.F1 * R * ir2;
+  F.FphioR2 += S.F2 * iR * iR;
+}
+
 template <>
 __device__ __forceinline__ void gh_comp_force<GH_NULLPOT>(const GhPotTable* __restrict__, int,
                                                           const GhGeom&, GhAcc&) {}
@@ -1552,12 +1692,16 @@ __host__ __device__ inline bool gh_is_heavy(int type) {
          type == GH_TWOPOWERTRIAX || type == GH_PERFECTELLIPSOID || type == GH_TRIAXGAUSSIAN ||
          type == GH_POWERTRIAX || type == GH_EINASTO || type == GH_TWOPOWERSPH ||
          type == GH_EXPTRUNCNFW || type == GH_SPIRALARMS || type == GH_DEDISK ||
-         type == GH_INTERPSPH || type == GH_SCF || type == GH_DISKSCF || type == GH_INTERPRZ;
+         type == GH_INTERPSPH || type == GH_SCF || type == GH_DISKSCF || type == GH_INTERPRZ ||
+         type == GH_MULTIPOLE;
 }
 
-// run-time dispatch over every component type (extended instances)
-__device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__ T, int i,
-                                                  const GhGeom& g, GhAcc& F) {
+// The heavy components' forces, out of line: one copy serves every force
+// call site of the extended instances (inlined, each call site would carry
+// every quadrature and expansion, which multiplies code size and compile
+// time); the call's cost is small beside the work of any of them.
+__device__ __noinline__ GhAcc gh_heavy_force(const GhPotTable* __restrict__ T, int i, GhGeom g) {
+  GhAcc F = {0.0, 0.0, 0.0};
   switch (T->comp[i].type) {
     case GH_TRIAXHERNQUIST:
     case GH_TRIAXNFW:
@@ -1575,7 +1719,22 @@ __device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__
     case GH_SCF: gh_comp_force<GH_SCF>(T, i, g, F); break;
     case GH_DISKSCF: gh_comp_force<GH_DISKSCF>(T, i, g, F); break;
     case GH_INTERPRZ: gh_comp_force<GH_INTERPRZ>(T, i, g, F); break;
-    default: gh_comp_force_dyn(T, i, g, F); break;
+    case GH_MULTIPOLE: gh_comp_force<GH_MULTIPOLE>(T, i, g, F); break;
+    default: break;
+  }
+  return F;
+}
+
+// run-time dispatch over every component type (extended instances)
+__device__ __forceinline__ void gh_comp_force_ext(const GhPotTable* __restrict__ T, int i,
+                                                  const GhGeom& g, GhAcc& F) {
+  if (gh_is_heavy(T->comp[i].type)) {
+    const GhAcc H = gh_heavy_force(T, i, g);
+    F.FRoR += H.FRoR;
+    F.Fz += H.Fz;
+    F.FphioR2 += H.FphioR2;
+  } else {
+    gh_comp_force_dyn(T, i, g, F);
   }
 }
 
@@ -1672,9 +1831,16 @@ __device__ __forceinline__ void gh_wrap_transform(const GhPotTable* __restrict__
 //     (positions from natural splines over the normalised time); its radial
 //     force F' gives F = -F' (x_obj - x, y_obj - y) / Rdist, F_z = -F_z';
 //   DehnenSmooth, GaussianAmplitude, TimeDependentAmplitude: factors only.
-// rec (when not null) keeps what the way back needs: {cos, sin, d offset/dR,
-// x, y} for the rotations, {x, y, (a + s) z / s, xi} for KuzminLike, {x, y,
-// R} for CylindricallySeparable.
+//   OblateStaeckel: the point on the v = v0 curve at the orbit's u (pass 0)
+//     or on the u = u0 curve at its v (pass 1), phi = t = 0; the wrapper's
+//     forces are linear in the component's potential and forces there
+//     (OblateStaeckelWrapperPotential.c:31-229).
+// rec (when not null, GH_WREC doubles) keeps what the way back needs: {cos,
+// sin, d offset/dR, x, y} for the rotations, {x, y, (a + s) z / s, xi} for
+// KuzminLike, {x, y, R} for CylindricallySeparable, {cos, sin, the
+// coefficients of (Phi, F_R, F_z) in F_R and in F_z, the constant terms,
+// the potential's coefficient and constant} for OblateStaeckel.
+#define GH_WREC 12
 __device__ __forceinline__ void gh_wrap_forward(const GhPotTable* __restrict__ T,
                                                 const GhWrap* __restrict__ W, int pass,
                                                 double& x, double& y, double& z, double& t,
@@ -1766,6 +1932,66 @@ __device__ __forceinline__ void gh_wrap_forward(const GhPotTable* __restrict__ T
       z = 0.0;
       break;
     }
+    case GH_OBLATESTAECKELWRAP: {  // p: amp, delta, u0, v0, refpot, planar
+      const double d = W->p[1], u0 = W->p[2], v0 = W->p[3];
+      const bool planar = W->p[5] != 0.0;
+      const double R = sqrt(x * x + y * y), zz = planar ? 0.0 : z;
+      const double d1 = sqrt((zz + d) * (zz + d) + R * R), d2 = sqrt((zz - d) * (zz - d) + R * R);
+      const double u = acosh(0.5 / d * (d1 + d2)), v = acos(0.5 / d * (d1 - d2));
+      const double sh = sinh(u), ch = cosh(u), sv = sin(v), cv = cos(v);
+      double xm, zm;
+      if (pass == 0) {
+        xm = d * sh * sin(v0);
+        zm = d * ch * cos(v0);
+      } else {
+        xm = d * sinh(u0) * sv;
+        zm = d * cosh(u0) * cv;
+      }
+      if (rec) {
+        const double th = tanh(u), tv = tan(v);
+        const double pre = sh * sh + sv * sv, dpu = 2.0 * sh * ch, dpv = 2.0 * sv * cv;
+        const double K = 1.0 / ((d * pre) * (d * pre));
+        const double A1 = d * sv * ch, A2 = th * zz, B1 = R / tv, B2 = A1;
+        const double G = (dpu * A1 + dpv * A2) / pre, H = (dpu / tv * R - dpv * A1) / pre;
+        rec[0] = (R > 0.0) ? x / R : 1.0;
+        rec[1] = (R > 0.0) ? y / R : 0.0;
+        if (pass == 0) {  // U = ch^2 Phi, dU/du = 2 ch sh Phi - ch^2 (F_R R / (th + 1e-12) + F_z z th)
+          const double c2 = ch * ch, fr = c2 * xm / (th + 1e-12), fz = planar ? 0.0 : c2 * zm * th;
+          rec[2] = K * (-2.0 * A1 * ch * sh + G * c2);
+          rec[3] = K * A1 * fr;
+          rec[4] = K * A1 * fz;
+          rec[5] = K * (-2.0 * B1 * ch * sh + H * c2);
+          rec[6] = K * B1 * fr;
+          rec[7] = K * B1 * fz;
+          rec[8] = rec[9] = 0.0;
+          rec[10] = c2 / pre;
+          rec[11] = 0.0;
+        } else if (planar) {  // the planar force has no V
+          for (int j = 2; j < 10; j++) rec[j] = 0.0;
+          const double P0 = sinh(u0) * sinh(u0) + sv * sv;
+          rec[10] = P0 / pre;
+          rec[11] = -W->p[4] / pre;
+        } else {  // V = refpot - P0 Phi, dV/dv = -2 sv cv Phi + P0 (F_R R / tv - F_z z tv)
+          const double P0 = sinh(u0) * sinh(u0) + sv * sv, refpot = W->p[4];
+          const double fr = P0 * xm / tv, fz = -P0 * zm * tv;
+          rec[2] = K * (-2.0 * A2 * sv * cv + G * P0);
+          rec[3] = K * A2 * fr;
+          rec[4] = K * A2 * fz;
+          rec[5] = K * (2.0 * B2 * sv * cv + H * P0);
+          rec[6] = -K * B2 * fr;
+          rec[7] = -K * B2 * fz;
+          rec[8] = -K * G * refpot;
+          rec[9] = -K * H * refpot;
+          rec[10] = P0 / pre;
+          rec[11] = -refpot / pre;
+        }
+      }
+      x = xm;
+      y = 0.0;
+      z = zm;
+      t = 0.0;
+      break;
+    }
     case GH_CYLSEPWRAP: {
       const double R = sqrt(x * x + y * y);
       if (rec) {
@@ -1783,9 +2009,21 @@ __device__ __forceinline__ void gh_wrap_forward(const GhPotTable* __restrict__ T
   }
 }
 __device__ __forceinline__ void gh_wrap_backward(const GhWrap* __restrict__ W, int pass,
-                                                 const double* rec, double& fx, double& fy,
-                                                 double& fz) {
+                                                 const double* rec, double pot, bool first,
+                                                 double& fx, double& fy, double& fz) {
   switch (W->kind) {
+    case GH_OBLATESTAECKELWRAP: {  // fx, fz: the component's F_R, F_z at (x, 0, z)
+      double FR = rec[2] * pot + rec[3] * fx + rec[4] * fz;
+      double Fz = rec[5] * pot + rec[6] * fx + rec[7] * fz;
+      if (first) {
+        FR += rec[8];
+        Fz += rec[9];
+      }
+      fx = FR * rec[0];
+      fy = FR * rec[1];
+      fz = Fz;
+      break;
+    }
     case GH_SOLIDBODYROTWRAP:
     case GH_COROTATINGROTWRAP: {
       const double c = rec[0], s = rec[1];
@@ -1845,14 +2083,20 @@ __device__ __forceinline__ void gh_wrap_backward(const GhWrap* __restrict__ W, i
 // Force of a component whose wrappers move the evaluation point: the leaf's
 // force at the moved point, mapped back out through the wrappers (one pass,
 // two with a CylindricallySeparable wrapper).
-__device__ inline void gh_comp_force_mapped(const GhPotTable* __restrict__ T, int i,
-                                            const GhGeom& g, GhAcc& F) {
+__device__ __forceinline__ double gh_comp_potential(const GhPotTable* __restrict__ T, int i,
+                                                    double R, double z, double cosphi,
+                                                    double sinphi, double t);
+// Out of line: one copy serves every call site (the walk carries the full
+// dispatch and, for OblateStaeckel, the potential's).
+__device__ __noinline__ GhAcc gh_comp_force_mapped(const GhPotTable* __restrict__ T, int i,
+                                                   GhGeom g) {
+  GhAcc F = {0.0, 0.0, 0.0};
   const GhComponent* __restrict__ C = &T->comp[i];
   const int nw = C->nwrap;
   double Fx = 0.0, Fy = 0.0, Fz = 0.0;
   for (int pass = 0; pass < C->mapwrap; pass++) {
     double x = g.x, y = g.y, z = g.z, t = g.t, fac = 1.0;
-    double rec[GH_WRAP_DEPTH][5];
+    double rec[GH_WRAP_DEPTH][GH_WREC];
 #pragma unroll
     for (int k = 0; k < GH_WRAP_DEPTH; k++)
       if (k < nw) gh_wrap_forward(T, &T->wrap[C->wrap[k]], pass, x, y, z, t, fac, rec[k]);
@@ -1869,17 +2113,26 @@ __device__ inline void gh_comp_force_mapped(const GhPotTable* __restrict__ T, in
     GhAcc L = {0.0, 0.0, 0.0};
     gh_comp_force_ext(T, i, gl, L);
     double fx = x * L.FRoR - y * L.FphioR2, fy = y * L.FRoR + x * L.FphioR2, fz = L.Fz;
+    double pot = 0.0;
+    if (C->mappot) {
+      const double R = sqrt(gl.R2);
+      pot = gh_comp_potential(T, i, R, z, (R > 0.0) ? x / R : 1.0, (R > 0.0) ? y / R : 0.0, t);
+    }
 #pragma unroll
     for (int k = GH_WRAP_DEPTH - 1; k >= 0; k--)
-      if (k < nw) gh_wrap_backward(&T->wrap[C->wrap[k]], pass, rec[k], fx, fy, fz);
+      if (k < nw) {
+        const GhWrap* __restrict__ W = &T->wrap[C->wrap[k]];
+        gh_wrap_backward(W, pass, rec[k], pot, W->first == i, fx, fy, fz);
+      }
     Fx += fac * fx;
     Fy += fac * fy;
     Fz += fac * fz;
   }
   const double iR2 = gh_rcp(g.R2);
-  F.FRoR += (g.x * Fx + g.y * Fy) * iR2;
-  F.FphioR2 += (g.x * Fy - g.y * Fx) * iR2;
-  F.Fz += Fz;
+  F.FRoR = (g.x * Fx + g.y * Fy) * iR2;
+  F.FphioR2 = (g.x * Fy - g.y * Fx) * iR2;
+  F.Fz = Fz;
+  return F;
 }
 
 // Force of component i with the wrappers around it: the component's own
@@ -1893,7 +2146,10 @@ __device__ __forceinline__ void gh_comp_force_any(const GhPotTable* __restrict__
                                                   const GhGeom& g, GhAcc& F) {
   const GhComponent* __restrict__ C = &T->comp[i];
   if (C->mapwrap) {
-    gh_comp_force_mapped(T, i, g, F);
+    const GhAcc M = gh_comp_force_mapped(T, i, g);
+    F.FRoR += M.FRoR;
+    F.Fz += M.Fz;
+    F.FphioR2 += M.FphioR2;
     return;
   }
   GhGeom gw = g;
@@ -2143,6 +2399,10 @@ __device__ __forceinline__ double gh_comp_potential(const GhPotTable* __restrict
       pot += S * H;
       break;
     }
+    case GH_MULTIPOLE: {  // MultipoleExpansionPotential.c:719-829 (Eval)
+      pot += gh_multipole_sums(T, C, R, z, cosphi, sinphi, t, true).V;
+      break;
+    }
     case GH_SCF: {  // SCFPotential.c:828-863 (Eval)
       double S[3], V;
       gh_scf_sums<true>(T, C, R, z, cosphi, sinphi, t, S, V);
@@ -2296,15 +2556,22 @@ __device__ inline double gh_mapped_potential(const GhPotTable* __restrict__ T, i
   const GhComponent* __restrict__ C = &T->comp[i];
   double total = 0.0;
   for (int pass = 0; pass < C->mapwrap; pass++) {
-    double x = x0, y = y0, z = z0, t = t0, fac = 1.0;
+    double x = x0, y = y0, z = z0, t = t0, fac = 1.0, coef = 1.0, add = 0.0;
+    double rec[GH_WREC];
     for (int k = 0; k < C->nwrap; k++) {
       const GhWrap* __restrict__ W = &T->wrap[C->wrap[k]];
-      gh_wrap_forward(T, W, pass, x, y, z, t, fac, nullptr);
-      if (W->kind == GH_CYLSEPWRAP && pass == 1) total -= fac * W->p[2];
+      const bool os = W->kind == GH_OBLATESTAECKELWRAP;
+      gh_wrap_forward(T, W, pass, x, y, z, t, fac, os ? rec : nullptr);
+      // the wrappers' constants, once per wrapper (at its first component)
+      if (W->kind == GH_CYLSEPWRAP && pass == 1 && W->first == i) total -= fac * W->p[2];
+      if (os) {  // innermost: Phi = amp (ch^2 Phi_A + P0 Phi_B - refpot) / pre
+        coef = rec[10];
+        add = (W->first == i) ? rec[11] : 0.0;
+      }
     }
     const double R = sqrt(x * x + y * y);
     const double cp = (R > 0.0) ? x / R : 1.0, sp = (R > 0.0) ? y / R : 0.0;
-    total += fac * gh_comp_potential(T, i, R, z, cp, sp, t);
+    total += fac * (coef * gh_comp_potential(T, i, R, z, cp, sp, t) + add);
   }
   return total;
 }

@@ -1594,11 +1594,12 @@ class DiskSCFPotential(Potential):
             dH = 0.5 * numpy.tanh(0.5 * z / hh)
         return S, dS, d2S, hzv, H, dH
 
-    def _me_dens(self, R, z):
-        """The density left to the SCF expansion: dens minus the pairs'
-        densities (DiskSCFPotential.c:221-231, per pair)."""
+    def _me_dens(self, R, z, phi=None):
+        """The density left to the expansion: dens (of (R, z), or of (R, z,
+        phi) when phi is given) minus the pairs' densities
+        (DiskSCFPotential.c:221-231, per pair)."""
         r = numpy.sqrt(R * R + z * z)
-        out = self._dens(R, z)
+        out = self._dens(R, z) if phi is None else self._dens(R, z, phi)
         for Sg, hz in zip(self._Sigma_dict, self._hz_dict):
             S, dS, d2S, hzv, H, dH = self._pair_terms(Sg, hz, r, z)
             out = out - (S * hzv + d2S * H + 2.0 / r * dS * (H + z * dH)) / (4.0 * numpy.pi)
@@ -1625,7 +1626,13 @@ class DiskSCFPotential(Potential):
         me_args = self._me.c_args()
         N, L, M = self._me._coeff_shape
         me_args = me_args[:6] + [v * self._amp for v in me_args[6:6 + N * L * M]] + me_args[6 + N * L * M:]
-        pt, pa = [24], list(me_args)
+        pt, pa = self._pair_args()
+        return [24] + pt, list(me_args) + pa, 1 + len(pt)
+
+    def _pair_args(self):
+        """pot_type 26 codes and arguments of the approximation pairs
+        (integratePlanarOrbit.py:778-812)."""
+        pt, pa = [], []
         for Sg, hz in zip(self._Sigma_dict, self._hz_dict):
             pt.append(26)
             famp = 4.0 * numpy.pi * Sg.get("amp", 1.0) * self._amp
@@ -1634,8 +1641,356 @@ class DiskSCFPotential(Potential):
             else:
                 pa.extend([4, 1, famp, Sg.get("h", 1.0 / 3.0), Sg.get("Rhole", 0.5)])
             pa.extend([0 if hz.get("type", "exp") == "exp" else 1, hz.get("h", 0.0375)])
-        return pt, pa, len(pt)
+        return pt, pa
 
+
+def _quintic_hermite(v, d1, d2, x):
+    """Power-basis coefficients (highest power first, [..., Nr-1, 6]) of the
+    C2 piecewise quintic that takes the value, first and second derivative
+    given at every node of ``x`` -- the polynomial scipy's
+    BPoly.from_derivatives builds with three constraints per node
+    (MultipoleExpansionPotential.py:1015-1079).  With D, D', D'' the end
+    conditions left after the left node's Taylor terms, h the interval,
+    a3 h^3 = 10 D - 4 D' h + D'' h^2 / 2, a4 h^4 = -15 D + 7 D' h - D'' h^2,
+    a5 h^5 = 6 D - 3 D' h + D'' h^2 / 2."""
+    h = numpy.diff(x)
+    a0, a1, a2 = v[..., :-1], d1[..., :-1], 0.5 * d2[..., :-1]
+    D = v[..., 1:] - (a0 + h * (a1 + h * a2))
+    Dp = (d1[..., 1:] - (a1 + 2.0 * a2 * h)) * h
+    Dpp = (d2[..., 1:] - 2.0 * a2) * h * h
+    A = 10.0 * D - 4.0 * Dp + 0.5 * Dpp
+    B = -15.0 * D + 7.0 * Dp - Dpp
+    C = 6.0 * D - 3.0 * Dp + 0.5 * Dpp
+    return numpy.stack([C / h**5, B / h**4, A / h**3, a2, a1, a0], axis=-1)
+
+
+def _legendre_PdP(x, L, M):
+    """P_l^m(x) (Condon-Shortley phase, as GSL's unnormalised arrays) and
+    dP_l^m/dx for l < L, m < min(l + 1, M): arrays [..., L, M]."""
+    x = numpy.asarray(x, dtype=float)
+    P = numpy.zeros(x.shape + (L, M))
+    dP = numpy.zeros(x.shape + (L, M))
+    for m in range(min(L, M)):
+        for l in range(m, L):
+            P[..., l, m] = special.lpmv(m, l, x)
+            if l > 0:
+                prev = P[..., l - 1, m] if l > m else 0.0
+                dP[..., l, m] = (l * x * P[..., l, m] - (l + m) * prev) / (x * x - 1.0)
+    return P, dP
+
+
+class MultipoleExpansionPotential(Potential):
+    """Multipole expansion of a density on a radial grid
+    (MultipoleExpansionPotential.py:44-2216): Phi = amp sum_lm P_l^m(cos theta)
+    [R^c_lm(r) cos m phi + R^s_lm(r) sin m phi], R_lm = r^-(l+1) I_inner(r) +
+    r^l I_outer(r), I_inner = -4 pi/(2l+1) int_0^r a^(l+2) rho_lm,
+    I_outer = -4 pi/(2l+1) int_r^inf a^(1-l) rho_lm, stored as C2 quintic
+    piecewise polynomials on the grid; the density's multipoles rho_lm carry
+    the real spherical-harmonic normalisation beta_lm.  pot_type 44 with
+    galpy's packing (MultipoleExpansionPotential.py:2101-2216).
+
+    ``rho_cos_splines`` / ``rho_sin_splines``: [L][M] splines of r (callable
+    as s(r) and s(r, 1), e.g. scipy's InterpolatedUnivariateSpline), or with
+    ``tgrid`` callables f(r, t) returning the values on ``rgrid``; default a
+    Hernquist monopole.  ``from_density`` builds them from a density."""
+
+    pot_code = 44
+
+    def __init__(self, amp=1.0, rho_cos_splines=None, rho_sin_splines=None,
+                 rgrid=numpy.geomspace(1e-3, 30, 1_001), tgrid=None, normalize=False):
+        from scipy.interpolate import CubicSpline, InterpolatedUnivariateSpline
+
+        Potential.__init__(self, amp=amp)
+        rg = numpy.asarray(rgrid, dtype=float)
+        self._rgrid = rg
+        self._tdep = tgrid is not None
+        if rho_cos_splines is None:  # a Hernquist monopole (MultipoleExpansionPotential.py:311-324)
+            b00 = sph_harm_normalization(1, 1)[0, 0]
+            vals = b00 * b00 * 4.0 * numpy.pi / (2.0 * numpy.pi) / rg / (1.0 + rg) ** 3
+            rho_cos_splines, rho_sin_splines = [[InterpolatedUnivariateSpline(rg, vals, k=3)]], None
+        L, M = len(rho_cos_splines), len(rho_cos_splines[0])
+
+        def values(splines, t=None):
+            if splines is None:
+                return numpy.zeros((L, M, rg.size))
+            if t is None:
+                return numpy.array([[numpy.asarray(s(rg), dtype=float) for s in row] for row in splines])
+            return numpy.array([[numpy.asarray(f(rg, t), dtype=float) for f in row] for row in splines])
+
+        if self._tdep:
+            self._tgrid = numpy.asarray(tgrid, dtype=float)
+            rc = numpy.array([values(rho_cos_splines, t) for t in self._tgrid])  # (Nt, L, M, Nr)
+            rs = numpy.array([values(rho_sin_splines, t) for t in self._tgrid])
+        else:
+            rc, rs = values(rho_cos_splines)[None], values(rho_sin_splines)[None]
+        # isNonAxi: some m > 0 multipole above 1e-12 of the largest m = 0 one
+        # (MultipoleExpansionPotential.py:156-186, 228-252); else M = 1
+        tol = 1e-12 * max(numpy.abs(rc[:, :, 0]).max(), 1e-16)
+        self.isNonAxi = bool(M > 1 and (numpy.abs(rc[:, :, 1:]).max() > tol or numpy.abs(rs[:, :, 1:]).max() > tol))
+        if not self.isNonAxi:
+            M = 1
+            rc, rs = rc[:, :, :1], rs[:, :, :1]
+        self._L, self._M = L, M
+
+        def deriv(arr, splines, l, m):
+            # the density's radial derivative: the given spline's own when
+            # static (s(r, 1)), else that of a cubic spline through the values
+            if not self._tdep and splines is not None:
+                try:
+                    return numpy.asarray(splines[l][m](rg, 1), dtype=float)[None]
+                except TypeError:
+                    pass
+            return numpy.array([InterpolatedUnivariateSpline(rg, a, k=3).derivative()(rg) for a in arr])
+
+        blocks = []  # (l, m, trig) l-major, cos before sin: (Nt, Nr-1, 6) x 2, rho (Nt, Nr)
+        for l in range(L):
+            pref = -4.0 * numpy.pi / (2 * l + 1)
+            for m in range(min(l + 1, M)):
+                for arr, spl in ((rc[:, l, m], rho_cos_splines), (rs[:, l, m], rho_sin_splines))[: 1 + (m > 0)]:
+                    drho = deriv(arr, spl, l, m)
+                    f_in, f_out = rg ** (l + 2) * arr, rg ** (1.0 - l) * arr
+                    I_in = numpy.array([InterpolatedUnivariateSpline(rg, f, k=3).antiderivative()(rg) for f in f_in])
+                    I_in -= I_in[:, :1]
+                    cum = numpy.array([InterpolatedUnivariateSpline(rg, f, k=3).antiderivative()(rg) for f in f_out])
+                    I_out = cum[:, -1:] - cum
+                    d2_in = (l + 2) * rg ** (l + 1) * arr + rg ** (l + 2) * drho
+                    d2_out = -(1.0 - l) * rg ** (-1.0 * l) * arr - rg ** (1.0 - l) * drho
+                    blocks.append((_quintic_hermite(pref * I_in, pref * f_in, pref * d2_in, rg),
+                                   _quintic_hermite(pref * I_out, -pref * f_out, pref * d2_out, rg), arr))
+        self._blocks = blocks
+        if self._tdep:
+            self._tblocks = [tuple(CubicSpline(self._tgrid, b.reshape(b.shape[0], -1)) for b in blk) for blk in blocks]
+        _set_normalize(self, normalize)
+
+    @staticmethod
+    def _density_arity(dens):
+        import inspect
+
+        pars = [p for p in inspect.signature(dens).parameters.values()
+                if p.name != "t" and p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD)]
+        return len(pars), "t" in inspect.signature(dens).parameters
+
+    @staticmethod
+    def _rho_lm(dens, rgrid, L, M, nct, nphi):
+        """The density's multipoles alpha_lm int rho P_l^m cos|sin(m phi)
+        dOmega on the grid, (Nr, L, M) each: Gauss-Legendre in cos theta,
+        the trapezoidal rule in phi (MultipoleExpansionPotential.py:726-842);
+        ``dens(R, z, phi)`` vectorised."""
+        beta = sph_harm_normalization(L, M)
+        alpha = beta.copy()
+        alpha[:, 1:] /= 2.0
+        Nr = rgrid.size
+        if L == 1 and M == 1:
+            rc = numpy.zeros((Nr, 1, 1))
+            rc[:, 0, 0] = alpha[0, 0] * 4.0 * numpy.pi * dens(rgrid, 0.0 * rgrid, 0.0 * rgrid)
+            return rc, numpy.zeros_like(rc)
+        ct, w = numpy.polynomial.legendre.leggauss(nct)
+        st = numpy.sqrt(1.0 - ct**2)
+        P, _ = _legendre_PdP(ct, L, M)  # (nct, L, M)
+        if M == 1:
+            Rr, zz = rgrid[:, None] * st[None], rgrid[:, None] * ct[None]
+            d = dens(Rr, zz, 0.0 * Rr) * 2.0 * numpy.pi  # (Nr, nct)
+            rc = numpy.einsum("rc,c,cl->rl", d, w, P[:, :, 0])[:, :, None] * alpha[None]
+            return rc, numpy.zeros_like(rc)
+        ph = numpy.linspace(0.0, 2.0 * numpy.pi, nphi, endpoint=False)
+        Rr = rgrid[:, None, None] * st[None, :, None] * numpy.ones(nphi)[None, None]
+        zz = rgrid[:, None, None] * ct[None, :, None] * numpy.ones(nphi)[None, None]
+        d = numpy.asarray(dens(Rr, zz, numpy.broadcast_to(ph, Rr.shape)), dtype=float)  # (Nr, nct, nphi)
+        cm, sm = numpy.cos(numpy.outer(ph, numpy.arange(M))), numpy.sin(numpy.outer(ph, numpy.arange(M)))
+        dphi = 2.0 * numpy.pi / nphi
+        rc = numpy.einsum("rcp,pm,c,clm->rlm", d, cm, w, P) * dphi * alpha[None]
+        rs = numpy.einsum("rcp,pm,c,clm->rlm", d, sm, w, P) * dphi * alpha[None]
+        return rc, rs
+
+    @classmethod
+    def from_density(cls, dens, L=6, rgrid=numpy.geomspace(1e-3, 30, 1_001), tgrid=None, symmetry=None,
+                     costheta_order=None, phi_order=None, amp=1.0, normalize=False):
+        """MultipoleExpansionPotential.py:326-492: the expansion of
+        ``dens`` (a function of r, of (R, z) or of (R, z, phi), with a ``t``
+        keyword when time-dependent, then ``tgrid`` is required) to degree
+        L - 1; ``symmetry`` 'spherical' (L = M = 1), 'axisymmetric' (M = 1)
+        or None (M = L)."""
+        from scipy.interpolate import InterpolatedUnivariateSpline, make_interp_spline
+
+        rgrid = numpy.asarray(rgrid, dtype=float)
+        nargs, has_t = cls._density_arity(dens)
+        if has_t and tgrid is None:
+            raise ValueError("tgrid is required when the density function depends on time")
+
+        def at(t):
+            if nargs == 1:
+                return lambda R, z, phi: dens(numpy.sqrt(R * R + z * z), **({"t": t} if has_t else {}))
+            if nargs == 2:
+                return lambda R, z, phi: dens(R, z, **({"t": t} if has_t else {}))
+            return lambda R, z, phi: dens(R, z, phi, **({"t": t} if has_t else {}))
+
+        if symmetry is not None and symmetry.startswith("spher"):
+            L = M = 1
+        elif symmetry is not None and symmetry.startswith("axi"):
+            M = 1
+        else:
+            M = L
+        nct = max(20, L + 1) if costheta_order is None else costheta_order
+        nphi = max(20, 2 * L + 1) if phi_order is None else phi_order
+        beta = sph_harm_normalization(L, M)
+        if has_t:
+            tgrid = numpy.asarray(tgrid, dtype=float)
+            both = [cls._rho_lm(at(t), rgrid, L, M, nct, nphi) for t in tgrid]
+            rc = numpy.array([b[0] for b in both]) * beta[None, None]  # (Nt, Nr, L, M)
+            rs = numpy.array([b[1] for b in both]) * beta[None, None]
+
+            def fn(arr, l, m):
+                sp = make_interp_spline(tgrid, arr[:, :, l, m], k=3)
+                return lambda r, t: sp(t)
+
+            cos_f = [[fn(rc, l, m) for m in range(M)] for l in range(L)]
+            sin_f = [[fn(rs, l, m) for m in range(M)] for l in range(L)]
+            return cls(amp=amp, rho_cos_splines=cos_f, rho_sin_splines=sin_f, rgrid=rgrid, tgrid=tgrid,
+                       normalize=normalize)
+        rc, rs = cls._rho_lm(at(None), rgrid, L, M, nct, nphi)
+        cos_s = [[InterpolatedUnivariateSpline(rgrid, beta[l, m] * rc[:, l, m], k=3) for m in range(M)]
+                 for l in range(L)]
+        sin_s = [[InterpolatedUnivariateSpline(rgrid, beta[l, m] * rs[:, l, m], k=3) for m in range(M)]
+                 for l in range(L)]
+        return cls(amp=amp, rho_cos_splines=cos_s, rho_sin_splines=sin_s, rgrid=rgrid, normalize=normalize)
+
+    def _coeff_blocks(self, t):
+        if not self._tdep:
+            return [(ci[0], co[0]) for ci, co, _ in self._blocks]
+        Nr = self._rgrid.size
+        return [(si(t).reshape(Nr - 1, 6), so(t).reshape(Nr - 1, 6)) for si, so, _ in self._tblocks]
+
+    def _radial(self, ci, co, r, l):
+        """R_lm and dR_lm/dr at radii r (eval_radial_lm,
+        MultipoleExpansionPotential.c:438-507)."""
+        rg = self._rgrid
+        rmin, rmax = rg[0], rg[-1]
+        ir = numpy.clip(numpy.searchsorted(rg, r, side="right") - 1, 0, rg.size - 2)
+        below, above = r < rmin, r > rmax
+        ir = numpy.where(above, rg.size - 2, ir)
+        d = numpy.where(above, rmax - rg[-2], r - rg[ir])
+        hor = lambda c: ((((c[:, 0] * d + c[:, 1]) * d + c[:, 2]) * d + c[:, 3]) * d + c[:, 4]) * d + c[:, 5]
+        dhor = lambda c: (((5 * c[:, 0] * d + 4 * c[:, 1]) * d + 3 * c[:, 2]) * d + 2 * c[:, 3]) * d + c[:, 4]
+        Ii, Io = hor(ci[ir]), hor(co[ir])
+        rl, rnl = r**l, r ** (-(l + 1.0))
+        Rv = rnl * Ii + numpy.where(above, 0.0, rl * Io)
+        dR = -(l + 1) * rnl / r * Ii + numpy.where(above, 0.0, rnl * dhor(ci[ir]) + l * rl / r * Io + rl * dhor(co[ir]))
+        if numpy.any(below):
+            rb = r[below]
+            P0 = ci[0, 4] / rmin ** (l + 2)
+            Iie = P0 / (l + 3) * rb ** (l + 3)
+            extra = P0 * numpy.log(rmin / rb) if l == 2 else P0 / (2 - l) * (rmin ** (2.0 - l) - rb ** (2.0 - l))
+            Ioe = co[0, 5] + extra
+            a, b = rb ** (-(l + 1.0)), rb**l
+            Rv[below] = a * Iie + b * Ioe
+            dR[below] = -(l + 1) * a / rb * Iie + l * b / rb * Ioe
+        return Rv, dR
+
+    def _spher(self, R, z, phi, t):
+        """(Phi, dPhi/dr, dPhi/dtheta, dPhi/dphi) / amp at points (one time)."""
+        R, z, phi = (numpy.atleast_1d(numpy.asarray(v, dtype=float)) for v in numpy.broadcast_arrays(R, z, phi))
+        r = numpy.sqrt(R * R + z * z)
+        x, st = z / r, R / r
+        L, M = self._L, self._M
+        P, dP = _legendre_PdP(x, L, M)
+        out = [numpy.zeros_like(r) for _ in range(4)]
+        k = 0
+        blocks = self._coeff_blocks(t)
+        for l in range(L):
+            for m in range(min(l + 1, M)):
+                for trig in range(1 + (m > 0)):
+                    Rv, dR = self._radial(*blocks[k], r, l)
+                    k += 1
+                    c = numpy.cos(m * phi) if trig == 0 else numpy.sin(m * phi)
+                    dc = -m * numpy.sin(m * phi) if trig == 0 else m * numpy.cos(m * phi)
+                    out[0] += P[:, l, m] * Rv * c
+                    out[1] += P[:, l, m] * dR * c
+                    out[2] += -st * dP[:, l, m] * Rv * c
+                    out[3] += P[:, l, m] * Rv * dc
+        return out
+
+    def _at_times(self, fn, R, z, phi, t):
+        R, z, phi, t = numpy.broadcast_arrays(*(numpy.asarray(v, dtype=float) for v in (R, z, phi, t)))
+        if not self._tdep or t.size == 1 or numpy.all(t == t.flat[0]):
+            out = fn(self._spher(R.ravel(), z.ravel(), phi.ravel(), float(t.flat[0])), R.ravel(), z.ravel())
+            return out.reshape(R.shape) if R.ndim else out[0]
+        res = numpy.empty(R.size)
+        for tv in numpy.unique(t):
+            sel = (t == tv).ravel()
+            res[sel] = fn(self._spher(R.ravel()[sel], z.ravel()[sel], phi.ravel()[sel], tv), R.ravel()[sel],
+                          z.ravel()[sel])
+        return res.reshape(R.shape)
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        return self._at_times(lambda o, R, z: o[0], R, z, phi, t)
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return self._at_times(lambda o, R, z: -(o[1] * R + o[2] * z / numpy.sqrt(R * R + z * z))
+                              / numpy.sqrt(R * R + z * z), R, z, phi, t)
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return self._at_times(lambda o, R, z: -(o[1] * z - o[2] * R / numpy.sqrt(R * R + z * z))
+                              / numpy.sqrt(R * R + z * z), R, z, phi, t)
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return self._at_times(lambda o, R, z: -o[3], R, z, phi, t)
+
+    def c_args(self):
+        rg = self._rgrid
+        Nr = rg.size
+        head = [float(Nr), float(self._L), float(self._M), float(int(self.isNonAxi))]
+        if not self._tdep:
+            parts = [numpy.array(head), rg, numpy.array([self._amp, 0.0])]
+            for ci, co, rho in self._blocks:
+                parts.extend([ci[0].ravel(), co[0].ravel(), rho[0]])
+        else:
+            parts = [numpy.array(head), rg, numpy.array([self._amp, float(self._tgrid.size)]), self._tgrid]
+            for si, so, sr in self._tblocks:
+                parts.extend(numpy.ascontiguousarray(sp.c.transpose(1, 2, 0)).ravel() for sp in (si, so, sr))
+        return numpy.concatenate(parts)
+
+
+class DiskMultipoleExpansionPotential(DiskSCFPotential):
+    """A disk-like density as a multipole expansion plus the Kuijken &
+    Dubinski approximation pairs Sigma_i(r) H_i(z)
+    (DiskMultipoleExpansionPotential.py:12-127): the pairs carry the disk's
+    thinness, the expansion (degree L - 1 on ``rgrid``) the remainder of the
+    density (of (R, z), or of (R, z, phi) when not axisymmetric).  Sigma and
+    hz as for DiskSCFPotential.  galpy packs the expansion (pot_type 44, amp
+    folded in) followed by one pot_type 26 per pair
+    (integrateFullOrbit.py:307-322)."""
+
+    def __init__(self, amp=1.0, dens=None, Sigma=None, hz=None, L=10, rgrid=numpy.geomspace(1e-3, 30, 1001),
+                 symmetry=None, costheta_order=None, phi_order=None):
+        Potential.__init__(self, amp=amp)
+        if dens is None:
+            dens = lambda R, z: 13.5 * numpy.exp(-3.0 * R) * numpy.exp(-27.0 * numpy.fabs(z))
+        Sigma = {"type": "exp", "h": 1.0 / 3.0, "amp": 1.0} if Sigma is None else Sigma
+        hz = {"type": "exp", "h": 1.0 / 27.0} if hz is None else hz
+        self._Sigma_dict = list(Sigma) if isinstance(Sigma, (list, tuple)) else [Sigma]
+        self._hz_dict = list(hz) if isinstance(hz, (list, tuple)) else [hz]
+        if len(self._hz_dict) == 1 and len(self._Sigma_dict) > 1:
+            self._hz_dict = self._hz_dict * len(self._Sigma_dict)
+        self._dens = dens
+        self._dens_nargs = MultipoleExpansionPotential._density_arity(dens)[0]
+        if symmetry is None and self._dens_nargs < 3:
+            symmetry = "axisymmetric"
+        self._me = MultipoleExpansionPotential.from_density(
+            self._me_dens3, L=L, rgrid=rgrid, symmetry=symmetry, costheta_order=costheta_order,
+            phi_order=phi_order)
+
+    def _me_dens3(self, R, z, phi):
+        return self._me_dens(R, z, phi if self._dens_nargs >= 3 else None)
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return self._me._phitorque(R, z, phi, t)
+
+    def pack(self):
+        """(pot_type, pot_args, npot) in galpy's packing."""
+        me_args = numpy.array(self._me.c_args())
+        me_args[4 + int(me_args[0])] *= self._amp  # integratePlanarOrbit.py:765-775
+        pt, pa = self._pair_args()
+        return [44] + pt, list(me_args) + pa, 1 + len(pt)
 
 class interpRZPotential(Potential):
     """An axisymmetric potential interpolated on an (R, z) grid
@@ -2329,6 +2684,84 @@ class CylindricallySeparablePotentialWrapper(WrapperPotential):
 
     def wrapper_args(self):
         return [self._amp, self._Rp, self._refpot]
+
+
+def _os_Rz_to_uv(R, z, delta):
+    """Prolate spheroidal (u, v) of focal length delta (galpy's
+    coords.Rz_to_uv, as OblateStaeckelWrapperPotential.c:4-13 forms it)."""
+    d1 = numpy.sqrt((z + delta) ** 2 + R**2)
+    d2 = numpy.sqrt((z - delta) ** 2 + R**2)
+    return numpy.arccosh(0.5 / delta * (d1 + d2)), numpy.arccos(0.5 / delta * (d1 - d2))
+
+
+class OblateStaeckelWrapperPotential(WrapperPotential):
+    """The wrapped potential in Staeckel form (OblateStaeckelWrapperPotential.py
+    :20-380): Phi = amp (U(u) - V(v)) / (sinh^2 u + sin^2 v) in prolate
+    spheroidal coordinates of focal length delta, U(u) = cosh^2 u Phi(u, pi/2),
+    V(v) = cosh^2 u0 Phi(u0, pi/2) - (sinh^2 u0 + sin^2 v) Phi(u0, v), the
+    wrapped potential taken at phi = t = 0; exact for a Staeckel potential of
+    the same delta.  u0: a number, or (R, z) of the reference point.
+    pot_type -3, own arguments [amp, delta, u0, v0 = pi/2, refpot]."""
+
+    pot_code = -3
+
+    def __init__(self, amp=1.0, pot=None, delta=0.5, u0=0.0):
+        WrapperPotential.__init__(self, amp=amp, pot=pot)
+        self._delta = float(delta)
+        if isinstance(u0, (tuple, list, numpy.ndarray)):
+            u0 = _os_Rz_to_uv(u0[0], u0[1], self._delta)[0]
+        self._u0 = float(u0)
+        self._v0 = numpy.pi / 2.0
+        R0, z0 = self._uv_to_Rz(self._u0, self._v0)
+        self._refpot = float(evaluatePotentials(self._pot, R0, z0)) * numpy.cosh(self._u0) ** 2
+        self.isNonAxi = False
+
+    def _uv_to_Rz(self, u, v):
+        return self._delta * numpy.sinh(u) * numpy.sin(v), self._delta * numpy.cosh(u) * numpy.cos(v)
+
+    def _terms(self, R, z):
+        """u, v, U, dU/du, V, dV/dv (OblateStaeckelWrapperPotential.c:31-137)."""
+        u, v = _os_Rz_to_uv(R, z, self._delta)
+        Ra, za = self._uv_to_Rz(u, self._v0)
+        pa = evaluatePotentials(self._pot, Ra, za)
+        U = numpy.cosh(u) ** 2 * pa
+        dU = 2 * numpy.cosh(u) * numpy.sinh(u) * pa - numpy.cosh(u) ** 2 * (
+            evaluateRforces(self._pot, Ra, za) * Ra / (numpy.tanh(u) + 1e-12)
+            + evaluatezforces(self._pot, Ra, za) * za * numpy.tanh(u))
+        Rb, zb = self._uv_to_Rz(self._u0, v)
+        pb = evaluatePotentials(self._pot, Rb, zb)
+        pre0 = numpy.sinh(self._u0) ** 2 + numpy.sin(v) ** 2
+        V = self._refpot - pre0 * pb
+        dV = -2 * numpy.sin(v) * numpy.cos(v) * pb + pre0 * (
+            evaluateRforces(self._pot, Rb, zb) * Rb / numpy.tan(v) - evaluatezforces(self._pot, Rb, zb) * zb * numpy.tan(v))
+        return u, v, U, dU, V, dV
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        u, v, U, _, V, _ = self._terms(R, z)
+        return (U - V) / (numpy.sinh(u) ** 2 + numpy.sin(v) ** 2)
+
+    def _forces(self, R, z):
+        u, v, U, dU, V, dV = self._terms(R, z)
+        d = self._delta
+        pre = numpy.sinh(u) ** 2 + numpy.sin(v) ** 2
+        dpu, dpv = 2 * numpy.sinh(u) * numpy.cosh(u), 2 * numpy.sin(v) * numpy.cos(v)
+        FR = (-dU * d * numpy.sin(v) * numpy.cosh(u) + dV * numpy.tanh(u) * z
+              + (U - V) * (dpu * d * numpy.sin(v) * numpy.cosh(u) + dpv * numpy.tanh(u) * z) / pre) / (d * pre) ** 2
+        Fz = (-dU * R / numpy.tan(v) - dV * d * numpy.sin(v) * numpy.cosh(u)
+              + (U - V) * (dpu / numpy.tan(v) * R - dpv * d * numpy.sin(v) * numpy.cosh(u)) / pre) / (d * pre) ** 2
+        return FR, Fz
+
+    def _Rforce(self, R, z, phi=0.0, t=0.0):
+        return self._forces(R, z)[0]
+
+    def _zforce(self, R, z, phi=0.0, t=0.0):
+        return self._forces(R, z)[1]
+
+    def _phitorque(self, R, z, phi=0.0, t=0.0):
+        return 0.0 * numpy.asarray(R, dtype=float)
+
+    def wrapper_args(self):
+        return [self._amp, self._delta, self._u0, self._v0, self._refpot]
 
 
 class TimeDependentAmplitudeWrapperPotential(WrapperPotential):

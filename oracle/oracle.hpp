@@ -240,8 +240,9 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
+               kOblateStaeckelWrap = -3,
                kTimeDepAmpWrap = -9, kMovingObjectWrap = -6, kInterpSph = 36, kSCF = 24,
-               kDiskSCF = 26, kInterpRZ = 13,
+               kDiskSCF = 26, kInterpRZ = 13, kMultipole = 44,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
                kTriaxHernquist = 21, kTriaxNFW = 22, kTriaxJaffe = 23, kPerfectEllipsoid = 30,
                kTriaxGaussian = 37, kPowerTriax = 38, kTwoPowerTriax = 43,
@@ -352,7 +353,8 @@ inline int g_ntfuncs = 0;
 inline bool is_wrapper(int code) {
   return code == kDehnenSmoothWrap || code == kSolidBodyRotWrap || code == kCorotatingRotWrap ||
          code == kGaussianAmpWrap || code == kRotateTiltWrap || code == kKuzminLikeWrap ||
-         code == kCylSepWrap || code == kTimeDepAmpWrap || code == kMovingObjectWrap;
+         code == kCylSepWrap || code == kTimeDepAmpWrap || code == kMovingObjectWrap ||
+         code == kOblateStaeckelWrap;
 }
 
 // nargs per code, as consumed from pot_args (integrateFullOrbit.c:80-345;
@@ -390,6 +392,7 @@ inline int component_nargs(int code, bool planar = false) {
     case kRotateTiltWrap: return planar ? -1 : 32;
     case kKuzminLikeWrap: return 3;     // amp, a, b^2
     case kCylSepWrap: return 3;         // amp, Rp, refpot
+    case kOblateStaeckelWrap: return 5; // amp, delta, u0, v0, refpot
     case kTimeDepAmpWrap: return 1;     // amp (and one time function)
     default: return -1;
   }
@@ -426,6 +429,23 @@ inline int scf_nargs(const double* a) {
   const int nonaxi = (int)a[1], NLM = (int)a[2] * (int)a[3] * (int)a[4], Nt = (int)a[5];
   return Nt == 0 ? 6 + (1 + nonaxi) * NLM + 11 : 6 + Nt + (1 + nonaxi) * NLM * 4 * (Nt - 1) + 11;
 }
+// MultipoleExpansionPotential: Nr, L, M, isNonAxi, rgrid[Nr], amp, Nt, then
+// per (l, m < min(l + 1, M)) and trig (cos; sin when m > 0) the I_inner and
+// I_outer quintic coefficients [Nr-1][6] and the density at the nodes [Nr]
+// (static), or, after tgrid[Nt], their cubic-in-t blocks [Nt-1][6 (Nr-1)][4]
+// and [Nt-1][Nr][4] (MultipoleExpansionPotential.c:183-399; packing
+// MultipoleExpansionPotential.py:2101-2216)
+inline int multipole_nblk(int L, int M) {
+  int n = 0;
+  for (int l = 0; l < L; l++) n += 2 * std::min(l + 1, M) - 1;
+  return n;
+}
+inline int multipole_nargs(const double* a) {
+  const long Nr = (long)a[0], L = (long)a[1], M = (long)a[2];
+  const long Nt = (long)a[4 + Nr + 1], nr = 6 * (Nr - 1);
+  const long per = Nt == 0 ? 2 * nr + Nr : 4 * (Nt - 1) * (2 * nr + Nr);
+  return (int)(4 + Nr + 2 + Nt + multipole_nblk((int)L, (int)M) * per);
+}
 // MovingObjectPotential's own arguments: nPts, t[nPts], x, y[, z], amp, t0,
 // tf (integrateFullOrbit.py:449-463; the planar packing has no z)
 inline int movingobject_nargs(const double* a, bool planar) { return 1 + (planar ? 3 : 4) * (int)a[0] + 3; }
@@ -453,6 +473,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
              : code == kSCF ? scf_nargs(pot_args)
              : code == kDiskSCF ? (int)pot_args[0] + 3
              : code == kInterpRZ ? interprz_nargs(pot_args)
+             : code == kMultipole ? multipole_nargs(pot_args)
              : code == kMovingObjectWrap ? 0  // counted after its wrapped components
                                    : component_nargs(code, planar);
     if (na < 0) return -1;
@@ -496,7 +517,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       continue;
     }
     if (is_ellipsoidal(code) || code == kSpiralArms || code == kDoubleExpDisk || code == kSCF ||
-        code == kDiskSCF || code == kInterpRZ) {
+        code == kDiskSCF || code == kInterpRZ || code == kMultipole) {
       C.p[0] = pot_args[0];
       pot_args += na;
       continue;
@@ -1319,6 +1340,183 @@ template <class R> R scf_eval(const Component& C, R Rc, R z, R phi, R t) {
   return R(pot);
 }
 
+// MultipoleExpansionPotential.c:411-507 (eval_radial_lm, below_grid_integrals),
+// :526-600 (the fused time and radius evaluation), :719-1029 (potential and
+// forces): R_lm = r^-(l+1) I_inner + r^l I_outer from the quintic
+// coefficients of the interval holding r (Horner), the density held at its
+// innermost value below rmin, only I_inner(rmax) r^-(l+1) above rmax; summed
+// over l, m with GSL's unnormalised P_l^m(cos theta) (Condon-Shortley phase)
+// and cos/sin m phi from the Chebyshev recurrence.  In double (not
+// op-counted).
+struct MpCoef {
+  const double* a;
+  int Nr, L, M, nonaxi, Nt;
+  const double *rg, *tg, *blocks;
+  double amp;
+  long per;
+};
+inline MpCoef mp_view(const Component& C) {
+  MpCoef v;
+  v.a = C.args;
+  v.Nr = (int)v.a[0];
+  v.L = (int)v.a[1];
+  v.M = (int)v.a[2];
+  v.nonaxi = (int)v.a[3];
+  v.rg = v.a + 4;
+  v.amp = v.rg[v.Nr];
+  v.Nt = (int)v.rg[v.Nr + 1];
+  v.tg = v.rg + v.Nr + 2;
+  v.blocks = v.tg + v.Nt;
+  const long nr = 6L * (v.Nr - 1);
+  v.per = v.Nt == 0 ? 2 * nr + v.Nr : 4L * (v.Nt - 1) * (2 * nr + v.Nr);
+  return v;
+}
+// ppoly_find_interval (:97-124) without its cache: the i with x[i] <= v <
+// x[i+1], clamped to [0, n-2]
+inline int mp_find(const double* x, int n, double v) {
+  int lo = 0, hi = n - 2;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (x[mid] <= v) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+// the six coefficients of interval ir of one PPoly (inner: half 0, outer: 1)
+// of block k, interpolated in time when time-dependent
+inline void mp_coeffs(const MpCoef& v, int k, int half, int ir, int it, double dt, double* c) {
+  const long nr = 6L * (v.Nr - 1);
+  const double* b = v.blocks + k * v.per;
+  if (v.Nt == 0) {
+    const double* q = b + half * nr + 6L * ir;
+    for (int p = 0; p < 6; p++) c[p] = q[p];
+    return;
+  }
+  const double* q = b + half * (long)(v.Nt - 1) * nr * 4 + (long)it * nr * 4 + 6L * ir * 4;
+  for (int p = 0; p < 6; p++) {
+    const double* cp = q + p * 4;
+    c[p] = ((cp[0] * dt + cp[1]) * dt + cp[2]) * dt + cp[3];
+  }
+}
+inline double mp_horner(const double* c, double d) {
+  return ((((c[0] * d + c[1]) * d + c[2]) * d + c[3]) * d + c[4]) * d + c[5];
+}
+inline double mp_horner_d(const double* c, double d) {
+  return (((5 * c[0] * d + 4 * c[1]) * d + 3 * c[2]) * d + 2 * c[3]) * d + c[4];
+}
+// R_lm and dR_lm/dr of block k (eval_radial_lm / eval_radial_lm_timedep)
+inline double mp_radial(const MpCoef& v, int k, double r, int l, double r_l, double r_nl, int it,
+                        double dt, double& dR) {
+  const double rmin = v.rg[0], rmax = v.rg[v.Nr - 1];
+  double ci[6], co[6];
+  if (r < rmin) {
+    mp_coeffs(v, k, 0, 0, it, dt, ci);
+    mp_coeffs(v, k, 1, 0, it, dt, co);
+    const double P0 = mp_horner_d(ci, 0.0) / std::pow(rmin, l + 2);
+    const double Iie = P0 / (l + 3) * std::pow(r, l + 3);
+    const double extra = (l == 2) ? P0 * std::log(rmin / r)
+                                  : P0 / (2 - l) * (std::pow(rmin, 2 - l) - std::pow(r, 2 - l));
+    const double Ioe = mp_horner(co, 0.0) + extra;
+    const double a = std::pow(r, -(l + 1)), b = std::pow(r, l);
+    dR = -(l + 1) * a / r * Iie + l * b / r * Ioe;
+    return a * Iie + b * Ioe;
+  }
+  if (r <= rmax) {
+    const int ir = mp_find(v.rg, v.Nr, r);
+    const double d = r - v.rg[ir];
+    mp_coeffs(v, k, 0, ir, it, dt, ci);
+    mp_coeffs(v, k, 1, ir, it, dt, co);
+    const double Ii = mp_horner(ci, d), Io = mp_horner(co, d);
+    dR = -(l + 1) * r_nl / r * Ii + r_nl * mp_horner_d(ci, d) + l * r_l / r * Io + r_l * mp_horner_d(co, d);
+    return r_nl * Ii + r_l * Io;
+  }
+  const int ir = v.Nr - 2;
+  mp_coeffs(v, k, 0, ir, it, dt, ci);
+  const double Ii = mp_horner(ci, rmax - v.rg[ir]);
+  dR = (-(l + 1)) * Ii * r_nl / r;
+  return Ii * r_nl;
+}
+// F = -amp (dPhi/dr, dPhi/dtheta, dPhi/dphi) (compute_multipole_spher_forces)
+// and the potential (MultipoleExpansionPotentialEval)
+inline void mp_spher(const Component& C, double Rc, double z, double phi, double t, double* F, double& pot) {
+  const MpCoef v = mp_view(C);
+  F[0] = F[1] = F[2] = 0.;
+  pot = 0.;
+  const double r = std::sqrt(Rc * Rc + z * z), theta = std::atan2(Rc, z);
+  if (r == 0.0 || !std::isfinite(r)) return;
+  const double ct = std::cos(theta), st = std::sin(theta);
+  const int L = v.L, M = v.M, Meff = v.nonaxi ? M : 1;
+  std::vector<double> P(std::max(Meff, 1) * L, 0.), dP(std::max(Meff, 1) * L, 0.);
+  scf_legendre(ct, L, Meff, P, dP);
+  std::vector<double> rl(L), rnl(L), cm(M), sm(M);
+  double rp = 1.0, rnp = 1.0 / r;
+  for (int l = 0; l < L; l++) {
+    rl[l] = rp;
+    rnl[l] = rnp;
+    rp *= r;
+    rnp /= r;
+  }
+  cm[0] = 1.0;
+  sm[0] = 0.0;
+  if (M > 1) {
+    cm[1] = std::cos(phi);
+    sm[1] = std::sin(phi);
+    for (int m = 2; m < M; m++) {
+      cm[m] = 2.0 * cm[1] * cm[m - 1] - cm[m - 2];
+      sm[m] = 2.0 * cm[1] * sm[m - 1] - sm[m - 2];
+    }
+  }
+  int it = 0;
+  double dt = 0.;
+  if (v.Nt > 0) {
+    it = mp_find(v.tg, v.Nt, t);
+    dt = t - v.tg[it];
+  }
+  double dr = 0., dth = 0., dph = 0.;
+  int k = 0;
+  for (int l = 0; l < L; l++) {
+    for (int m = 0; m < std::min(l + 1, M); m++) {
+      const double Pl = P[m * L + l], dPl = dP[m * L + l];
+      double dRc, dRs = 0.;
+      const double Rcv = mp_radial(v, k++, r, l, rl[l], rnl[l], it, dt, dRc);
+      dr += Pl * cm[m] * dRc;
+      dth += dPl * (-st) * cm[m] * Rcv;
+      dph += Pl * (-m * sm[m]) * Rcv;
+      double rad = Rcv * cm[m];
+      if (m > 0) {
+        const double Rsv = mp_radial(v, k++, r, l, rl[l], rnl[l], it, dt, dRs);
+        dr += Pl * sm[m] * dRs;
+        dth += dPl * (-st) * sm[m] * Rsv;
+        dph += Pl * (m * cm[m]) * Rsv;
+        rad += Rsv * sm[m];
+      }
+      pot += Pl * rad;
+    }
+  }
+  F[0] = -v.amp * dr;
+  F[1] = -v.amp * dth;
+  F[2] = -v.amp * dph;
+  pot *= v.amp;
+}
+template <class R> void mp_forces(const Component& C, R Rc, R z, R phi, R t, R& FR, R& Fz, R& Fp) {
+  double F[3], pot;
+  mp_spher(C, dval(Rc), dval(z), dval(phi), dval(t), F, pot);
+  const double Rd = dval(Rc), zd = dval(z), r = std::sqrt(Rd * Rd + zd * zd);
+  if (r == 0.0 || !std::isfinite(r)) {
+    FR = R(0.);
+    Fz = R(0.);
+    Fp = R(F[2]);
+    return;
+  }
+  FR = R(F[0] * (Rd / r) + F[1] * (zd / (r * r)));
+  Fz = R(F[0] * (zd / r) + F[1] * (-Rd / (r * r)));
+  Fp = R(F[2]);
+}
+template <class R> R mp_eval(const Component& C, R Rc, R z, R phi, R t) {
+  double F[3], pot;
+  mp_spher(C, dval(Rc), dval(z), dval(phi), dval(t), F, pot);
+  return R(pot);
+}
+
 // DiskSCFPotential.c:33-120 (one [Sigma_i, h_i] approximation pair of a
 // DiskSCFPotential): Phi = Sigma(r) H(z), Sigma exponential (type 0: amp
 // exp(-r/h)) or with a central hole (type 1: amp exp(-Rhole/r - r/h)), H
@@ -1491,6 +1689,7 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
       case kSpiralArms: spiral_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kDoubleExpDisk: dedisk_forces(C, Rc, z, fr, fz); break;
       case kSCF: scf_forces(C, Rc, z, phi, t, fr, fz, fp); break;
+      case kMultipole: mp_forces(C, Rc, z, phi, t, fr, fz, fp); break;
       case kDiskSCF: dscf_forces(C, Rc, z, fr, fz); break;
       case kInterpRZ: {  // interpRZPotential.c:22-70
         fr = R(interprz_eval(C, 1, dval(Rc), dval(z)));
@@ -1545,6 +1744,11 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
           R sqrtbz = sqrt(z * z + R(b2));
           fr = R(amp) * wr * (Rc / xi);
           fz = R(amp) * wr * ((R(a) + sqrtbz) * z / xi / sqrtbz);
+        } else if (C.type == kOblateStaeckelWrap) {
+          double fr_, fz_;
+          os_forces(C, dval(Rc), dval(z), fr_, fz_);
+          fr = R(fr_);
+          fz = R(fz_);
         } else if (C.type == kCylSepWrap) {
           // CylindricallySeparablePotentialWrapper.c:14-27: the wrapped
           // radial force at (R, 0) and vertical force at (Rp, z), phi = t = 0
@@ -1567,6 +1771,82 @@ void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t,
     }
     FR += fr; Fz += fz; Fphi += fp;
   }
+}
+
+// OblateStaeckelWrapperPotential.c:4-229: the wrapped potential put in
+// Staeckel form, Phi = amp (U(u) - V(v)) / (sinh^2 u + sin^2 v), in prolate
+// spheroidal coordinates of focal length delta, with U(u) = cosh^2 u
+// Phi_w(u, v0) and V(v) = refpot - (sinh^2 u0 + sin^2 v) Phi_w(u0, v) from the
+// wrapped potential and forces at phi = t = 0 (U, dUdu, V, dVdv at :31-53,
+// :114-137; the planar force at :215-229 uses the wrapped planar force and
+// no V).  args: amp, delta, u0, v0, refpot.  In double (not op-counted).
+template <class R> void cyl_forces(int npot, const Component* P, R Rc, R z, R phi, R t, R& FR, R& Fz, R& Fphi);
+template <class R> void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fphi);
+template <class R> R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t);
+inline void os_Rz_to_uv(double R, double z, double delta, double& u, double& v) {
+  const double d12 = std::pow(z + delta, 2) + std::pow(R, 2);
+  const double d22 = std::pow(z - delta, 2) + std::pow(R, 2);
+  const double coshu = 0.5 / delta * (std::sqrt(d12) + std::sqrt(d22));
+  const double cosv = 0.5 / delta * (std::sqrt(d12) - std::sqrt(d22));
+  u = std::acosh(coshu);
+  v = std::acos(cosv);
+}
+inline void os_UV(const Component& C, double u, double v, bool planar, double& U, double& dU, double& V,
+                  double& dV) {
+  const double delta = C.p[1], u0 = C.p[2], v0 = C.p[3], refpot = C.p[4];
+  const int n = C.nwrapped;
+  const Component* W = C.wrapped;
+  double fr, fz, fp;
+  // U and dUdu on the curve v = v0
+  const double Ra = delta * std::sinh(u) * std::sin(v0), za = delta * std::cosh(u) * std::cos(v0);
+  const double pa = potential_value(n, W, Ra, za, 0., 0.);
+  U = std::pow(std::cosh(u), 2) * pa;
+  if (planar) {
+    planar_forces(n, W, Ra, 0., 0., fr, fp);
+    dU = 2 * std::cosh(u) * std::sinh(u) * pa - std::pow(std::cosh(u), 2) * fr * Ra / (std::tanh(u) + 1e-12);
+    V = dV = 0.;
+    return;
+  }
+  cyl_forces(n, W, Ra, za, 0., 0., fr, fz, fp);
+  dU = 2 * std::cosh(u) * std::sinh(u) * pa -
+       std::pow(std::cosh(u), 2) * (fr * Ra / (std::tanh(u) + 1e-12) + fz * za * std::tanh(u));
+  // V and dVdv on the curve u = u0
+  const double Rb = delta * std::sinh(u0) * std::sin(v), zb = delta * std::cosh(u0) * std::cos(v);
+  const double pb = potential_value(n, W, Rb, zb, 0., 0.);
+  const double pre = std::pow(std::sinh(u0), 2) + std::pow(std::sin(v), 2);
+  V = refpot - pre * pb;
+  cyl_forces(n, W, Rb, zb, 0., 0., fr, fz, fp);
+  dV = -2 * std::sin(v) * std::cos(v) * pb + pre * (fr * Rb / std::tan(v) - fz * zb * std::tan(v));
+}
+inline void os_forces(const Component& C, double Rc, double z, double& FR, double& Fz) {
+  const double amp = C.p[0], delta = C.p[1];
+  double u, v, U, dU, V, dV;
+  os_Rz_to_uv(Rc, z, delta, u, v);
+  os_UV(C, u, v, false, U, dU, V, dV);
+  const double pre = std::pow(std::sinh(u), 2) + std::pow(std::sin(v), 2);
+  const double dpu = 2 * std::sinh(u) * std::cosh(u), dpv = 2 * std::sin(v) * std::cos(v);
+  FR = amp * ((-dU * delta * std::sin(v) * std::cosh(u) + dV * std::tanh(u) * z +
+               (U - V) * (dpu * delta * std::sin(v) * std::cosh(u) + dpv * std::tanh(u) * z) / pre) /
+              std::pow(delta * pre, 2));
+  Fz = amp * ((-dU * Rc / std::tan(v) - dV * delta * std::sin(v) * std::cosh(u) +
+               (U - V) * (dpu / std::tan(v) * Rc - dpv * delta * std::sin(v) * std::cosh(u)) / pre) /
+              std::pow(delta * pre, 2));
+}
+inline double os_planar_Rforce(const Component& C, double Rc) {
+  const double amp = C.p[0], delta = C.p[1];
+  double u, v, U, dU, V, dV;
+  os_Rz_to_uv(Rc, 0., delta, u, v);
+  os_UV(C, u, v, true, U, dU, V, dV);
+  const double pre = std::pow(std::sinh(u), 2) + std::pow(std::sin(v), 2);
+  const double dpu = 2 * std::sinh(u) * std::cosh(u);
+  return amp * ((-dU * delta * std::sin(v) * std::cosh(u) + U * dpu * delta * std::sin(v) * std::cosh(u) / pre) /
+                std::pow(delta * pre, 2));
+}
+inline double os_eval(const Component& C, double Rc, double z) {
+  double u, v, U, dU, V, dV;
+  os_Rz_to_uv(Rc, z, C.p[1], u, v);
+  os_UV(C, u, v, false, U, dU, V, dV);
+  return C.p[0] * (U - V) / (std::pow(std::sinh(u), 2) + std::pow(std::sin(v), 2));
 }
 
 // DoubleExponentialDiskPotential.c: Ogata's double-exponential quadrature of
@@ -2118,6 +2398,11 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
         scf_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
         break;
       }
+      case kMultipole: {  // MultipoleExpansionPotential.c:1030-1045: the full forces at z = 0
+        R fz0;
+        mp_forces(C, Rc, R(0.), phi, t, fr, fz0, fp);
+        break;
+      }
       case kInterpSph: fr = R(C.p[0]) * interpsph_rforce(C, Rc); break;  // SphericalPotential.c:22-29
       case kSpiralArms: {  // SpiralArmsPotential.c:625-716: the full forces at z = 0
         R fz0;
@@ -2150,6 +2435,8 @@ void planar_forces(int npot, const Component* P, R Rc, R phi, R t, R& FR, R& Fph
           R wr, wp;
           planar_forces(C.nwrapped, C.wrapped, xi, R(0.), t, wr, wp);
           fr = R(amp) * wr * (Rc / xi);
+        } else if (C.type == kOblateStaeckelWrap) {  // OblateStaeckelWrapperPotential.c:215-229
+          fr = R(os_planar_Rforce(C, dval(Rc)));
         } else if (C.type == kCylSepWrap) {  // CylindricallySeparablePotentialWrapper.c:28-34
           R wr, wp;
           planar_forces(C.nwrapped, C.wrapped, Rc, R(0.), R(0.), wr, wp);
@@ -2196,7 +2483,7 @@ inline bool has_hessian(int npot, const Component* P) {
     const int c = P[i].type;
     if (c == kKKStaeckel || c == kSoftNeedleBar || is_wrapper(c) || c == kEllipDisk || c == kLopsided ||
         c == kSteadySpiral || c == kTransientSpiral || c == kCosmphiDisk || c == kHenonHeiles ||
-        c == kSpiralArms || c == kDoubleExpDisk || c == kInterpSph || c == kSCF || c == kDiskSCF || c == kInterpRZ || is_ellipsoidal(c))
+        c == kSpiralArms || c == kDoubleExpDisk || c == kInterpSph || c == kSCF || c == kDiskSCF || c == kInterpRZ || c == kMultipole || is_ellipsoidal(c))
       return false;
   }
   return true;
@@ -2289,6 +2576,7 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
       case kSpiralArms: pot += spiral_eval(C, Rc, z, phi, t); break;
       case kDoubleExpDisk: pot += dedisk_eval(C, Rc, z); break;
       case kSCF: pot += scf_eval(C, Rc, z, phi, t); break;
+      case kMultipole: pot += mp_eval(C, Rc, z, phi, t); break;
       case kInterpRZ: pot += R(interprz_eval(C, 0, dval(Rc), dval(z))); break;  // interpRZPotential.c:6-20
       case kDiskSCF: {  // DiskSCFPotential.c:108-120 (Eval)
         const double* a = C.args;
@@ -2327,6 +2615,8 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
         } else if (C.type == kKuzminLikeWrap) {  // KuzminLikeWrapperPotential.py _evaluate
           pot += R(C.p[0]) * potential_value(C.nwrapped, C.wrapped, kuzminlike_xi(Rc, z, C.p[1], C.p[2]),
                                              R(0.), phi, t);
+        } else if (C.type == kOblateStaeckelWrap) {  // OblateStaeckelWrapperPotential.c:174-182
+          pot += R(os_eval(C, dval(Rc), dval(z)));
         } else if (C.type == kCylSepWrap) {  // CylindricallySeparablePotentialWrapper.py _evaluate
           pot += R(C.p[0]) * (potential_value(C.nwrapped, C.wrapped, Rc, R(0.), R(0.), R(0.)) +
                               potential_value(C.nwrapped, C.wrapped, R(C.p[1]), z, R(0.), R(0.)) -
