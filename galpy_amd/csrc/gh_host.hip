@@ -912,6 +912,69 @@ int build_tfunc_table(double (*f)(double), double ta, double tb, GhPotTable* T, 
 // arguments, then its own (:1010-1031).  Leaves go to T->comp in the order
 // they appear (galpy's summation order); every leaf lists the wrappers
 // around it, outermost first.
+// NonInertialFrameForce (integrateFullOrbit.c:689-721; packing
+// integratePlanarOrbit.py:835-965, the same in both packings): pot_type 39
+// takes 23 arguments (amp, 10 cache slots, rot_acc, lin_acc, omegaz_only,
+// const_freq, Omega_as_func, Omega[3], Omega^2, Omegadot[3]) and its time
+// functions from pot_tfuncs (a0[3], x0[3] and v0[3] when it also rotates,
+// then Omega and Omegadot); 45 prefixes n_spline, nPts, tgrid[nPts] and the
+// n_spline sampled functions (natural splines, initNonInertialFrameForceSplines
+// :1435-1462) and appends tmin, tmax.  Each time input becomes a table or a
+// spline in ext; their offsets are listed at C.ext.
+int decode_noninertial(int code, const double*& pot_args, GhPotTable* T, int i, GhTfuncs* tf) {
+  GhComponent& C = T->comp[i];
+  const double* a = pot_args;
+  int nin = 0;
+  std::vector<int> offs;
+  if (code == GH_NONINERTIALINTERP) {
+    const int ns = (int)a[0], npts = (int)a[1];
+    if (ns < 0 || npts < 2) {
+      set_error("NonInertialFrameForce with %d splines of %d points", ns, npts);
+      return -1;
+    }
+    const double* tg = a + 2;
+    for (int k = 0; k < ns; k++) {
+      int off = 0;
+      if (build_spline(tg, tg + (long)(k + 1) * npts, npts, T, off) != 0) return -1;
+      offs.push_back(off);
+    }
+    nin = ns;
+    a += 2 + (long)(1 + ns) * npts;
+  } else {
+    const int ntf = (int)(3 * a[12] * (1 + 2 * a[11]) + (6 - 4 * a[13]) * a[15]);
+    if (ntf > 0 && (!tf || !tf->fns)) {
+      set_error("NonInertialFrameForce needs its %d time functions (pot_tfuncs)", ntf);
+      return -1;
+    }
+    for (int k = 0; k < ntf; k++) {
+      int off = 0;
+      if (build_tfunc_table(tf->fns[tf->used++], tf->ta, tf->tb, T, off) != 0) return -1;
+      offs.push_back(off);
+    }
+    nin = ntf;
+  }
+  if (T->next + nin > GH_EXT_LEN) {
+    set_error("NonInertialFrameForce tables do not fit the potential table");
+    return -1;
+  }
+  C.type = code;
+  C.p[0] = a[0];
+  for (int j = 0; j < 5; j++) C.p[1 + j] = a[11 + j];
+  C.p[6] = code == GH_NONINERTIALINTERP ? 1.0 : 0.0;
+  for (int j = 0; j < 7; j++) C.c[j] = a[16 + j];
+  C.p[7] = code == GH_NONINERTIALINTERP ? a[23] : 0.0;
+  C.c[7] = code == GH_NONINERTIALINTERP ? a[24] : 0.0;
+  C.ext = T->next;
+  for (int k = 0; k < nin; k++) T->ext[T->next + k] = offs[k];
+  T->next += nin;
+  C.nonaxi = 1;
+  T->nonaxi = 1;
+  T->ndiss++;
+  T->has_wrap = 1;  // extended dispatch instances
+  pot_args = a + (code == GH_NONINERTIALINTERP ? 25 : 23);
+  return 0;
+}
+
 int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPotTable* T,
                       int& nleaf, int& nslots, int depth, bool planar, GhTfuncs* tf) {
   for (int k = 0; k < n; k++) {
@@ -921,7 +984,11 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
         set_error("more than %d components (wrapped ones included)", GH_MAX_POT);
         return -1;
       }
-      if (decode_component(code, pot_args, T, nleaf, nslots, planar) != 0) return -1;
+      if (code == GH_NONINERTIAL || code == GH_NONINERTIALINTERP) {
+        if (decode_noninertial(code, pot_args, T, nleaf, tf) != 0) return -1;
+      } else if (decode_component(code, pot_args, T, nleaf, nslots, planar) != 0) {
+        return -1;
+      }
       nleaf++;
       continue;
     }
@@ -1162,6 +1229,13 @@ int launch_integrate(int nobj, double* d_yo, int nt, const double* d_t, int indi
   if (nobj == 0) return 0;
   if (nobj < 0 || nt < 2) {
     set_error("invalid sizes nobj=%d nt=%d (need nt >= 2)", nobj, nt);
+    return -1;
+  }
+  if (h->ndiss && (odeint_type == 0 || odeint_type == 3 || odeint_type == 4 || odeint_type == 7)) {
+    // the symplectic and IAS15 integrators take position-only forces
+    // (evalRectForce, integrateFullOrbit.c:1091-1132); galpy reroutes them
+    set_error("velocity-dependent forces need odeint_type 1, 2, 5 or 6 (RK4, RK6, DOPR54, DOP853), "
+              "not %d", odeint_type);
     return -1;
   }
 #define GH_LAUNCH(SPEC)                                                                        \

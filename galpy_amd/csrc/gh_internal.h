@@ -51,11 +51,13 @@ struct GhPotHandle {
   int types[GH_MAX_POT];
   int nonaxi[GH_MAX_POT];  // component exerts a phi torque (bar, triaxial halo)
   int has_wrap;            // wrapped or heavy components: extended dispatch, no Hessian
+  int ndiss;               // velocity-dependent components
 };
 
 inline void fill_handle(const GhPotTable& table, GhPotHandle* h) {
   h->npot = table.npot;
   h->has_wrap = table.has_wrap;
+  h->ndiss = table.ndiss;
   for (int i = 0; i < GH_MAX_POT; i++) {
     h->types[i] = table.comp[i].type;
     h->nonaxi[i] = table.comp[i].nonaxi;

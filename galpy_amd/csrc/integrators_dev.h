@@ -100,6 +100,27 @@ __device__ __forceinline__ GhVec2 gh_accel(const F& T, double t, GhVec2 q) {
   return a;
 }
 
+// acceleration with the velocity (evalRectDeriv, integrateFullOrbit.c:
+// 1326-1351; evalPlanarRectDeriv, integratePlanarOrbit.c:1032-1055): the
+// first-order integrators hand it to velocity-dependent forces
+template <class F>
+__device__ __forceinline__ GhVec3 gh_accel_v(const F& T, double t, GhVec3 q, GhVec3 p) {
+  GhVec3 a;
+  T.rect_v(t, q.x, q.y, q.z, p.x, p.y, p.z, a.x, a.y, a.z);
+  return a;
+}
+template <class F>
+__device__ __forceinline__ GhVec2 gh_accel_v(const F& T, double t, GhVec2 q, GhVec2 p) {
+  GhVec2 a;
+  T.rect_v(t, q.x, q.y, p.x, p.y, a.x, a.y);
+  return a;
+}
+// the variational systems of integrate_dxdv: conservative forces only
+template <class F, class V>
+__device__ __forceinline__ V gh_accel_v(const F& T, double t, V q, V) {
+  return gh_accel(T, t, q);
+}
+
 // log-space scale of the step estimators, bovy_symplecticode.c:606-623
 __device__ __forceinline__ void gh_est_scales(GhVec3 q, GhVec3 p, double rtol, double atol,
                                               double& sq, double& sp) {

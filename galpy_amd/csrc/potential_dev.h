@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <type_traits>
 
 #include "gh_math.h"
 
@@ -93,6 +94,9 @@ enum GhPotCode {
   GH_INTERPRZ = 13,
   GH_MULTIPOLE = 44,
   GH_OBLATESTAECKELWRAP = -3,
+  // velocity-dependent forces (integrateFullOrbit.c:689-721)
+  GH_NONINERTIAL = 39,
+  GH_NONINERTIALINTERP = 45,
 };
 
 struct GhComponent {
@@ -134,7 +138,7 @@ struct GhPotTable {
   double tab[GH_TAB_SLOTS][GH_TAB_MAXSEG][GH_TAB_NC];
   // variable-length data, last so that a table is uploaded up to next
   int next;  // doubles of ext in use
-  int pad;
+  int ndiss;  // velocity-dependent components (their forces need the velocity)
   double ext[GH_EXT_LEN];
 };
 
@@ -169,6 +173,16 @@ __device__ __forceinline__ double gh_spline_eval(const GhPotTable* __restrict__ 
   const double* __restrict__ q = e + 1 + n + 5 * i;
   const double d = t - e[1 + i];
   return fma(d, fma(d, fma(d, q[3], q[2]), q[1]), q[0]);
+}
+// derivative of the spline (gsl_spline_eval_deriv: b_i + d (2 c_i + 3 d_i d))
+__device__ __forceinline__ double gh_spline_deriv(const GhPotTable* __restrict__ T, int off,
+                                                  double t) {
+  const double* __restrict__ e = &T->ext[off];
+  const int n = (int)e[0];
+  const int i = gh_spline_find(e + 1, n, t);
+  const double* __restrict__ q = e + 1 + n + 5 * i;
+  const double d = t - e[1 + i];
+  return q[1] + d * (2.0 * q[2] + 3.0 * q[3] * d);
 }
 // integral of the spline from x[0] to t
 __device__ __forceinline__ double gh_spline_integ(const GhPotTable* __restrict__ T, int off,
@@ -2262,11 +2276,171 @@ __device__ __forceinline__ void gh_rect_force(const GhPotTable* __restrict__ T, 
   az = F.Fz;
 }
 
+// NonInertialFrameForce.c:27-170: the fictitious forces of a frame whose
+// origin accelerates by a0(t) and that rotates at Omega(t) (centrifugal,
+// Coriolis and Euler terms; with the origin's x0, v0 when it both rotates
+// and accelerates), in Cartesian form times amp.  p: amp, rot_acc, lin_acc,
+// omegaz_only, const_freq, Omega_as_func, interp (45), tmin; c: Omega[3],
+// Omega^2, Omegadot[3], tmax; ext[C->ext + k]: the table (pot_type 39: the
+// caller's function k tabulated, Omegadot its own) or spline (45: natural
+// spline k, Omegadot its derivative, times clamped to [tmin, tmax]) of time
+// input k in galpy's order (a0, x0, v0, then Omega).
+__device__ __noinline__ void gh_noninertial_xyz(const GhPotTable* __restrict__ T,
+                                                const GhComponent* __restrict__ C, double t,
+                                                double x, double y, double z, double vx,
+                                                double vy, double vz, double& Fx, double& Fy,
+                                                double& Fz) {
+  const double* __restrict__ e = &T->ext[C->ext];
+  const bool interp = C->p[6] != 0.0;
+  const double tq = interp ? fmin(fmax(t, C->p[7]), C->c[7]) : t;
+  auto tdep = [&](int k) {
+    return interp ? gh_spline_eval(T, (int)e[k], tq) : gh_tfunc_eval(T, (int)e[k], t);
+  };
+  auto tdepd = [&](int sk, int tk) {
+    return interp ? gh_spline_deriv(T, (int)e[sk], tq) : gh_tfunc_eval(T, (int)e[tk], t);
+  };
+  const bool rot_acc = C->p[1] != 0.0, lin_acc = C->p[2] != 0.0, as_func = C->p[5] != 0.0;
+  const int la = lin_acc ? 9 : 0;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (rot_acc) {
+    const bool zonly = C->p[3] != 0.0, const_freq = C->p[4] != 0.0;
+    if (zonly) {
+      double Oz, O2, Odz = 0.0;
+      if (as_func) {
+        Oz = tdep(la);
+        O2 = Oz * Oz;
+      } else {
+        Oz = C->c[2];
+        if (!const_freq) {
+          Oz += C->c[6] * t;
+          O2 = Oz * Oz;
+        } else {
+          O2 = C->c[3];
+        }
+      }
+      fx += 2.0 * Oz * vy + O2 * x;
+      fy += -2.0 * Oz * vx + O2 * y;
+      if (!const_freq) {
+        Odz = as_func ? tdepd(la, la + 1) : C->c[6];
+        fx += Odz * y;
+        fy -= Odz * x;
+      }
+      if (lin_acc) {
+        const double x0x = tdep(3), x0y = tdep(4), v0x = tdep(6), v0y = tdep(7);
+        fx += 2.0 * Oz * v0y + O2 * x0x;
+        fy += -2.0 * Oz * v0x + O2 * x0y;
+        if (!const_freq) {
+          fx += Odz * x0y;
+          fy -= Odz * x0x;
+        }
+      }
+    } else {
+      double Ox, Oy, Oz, O2, Odx = 0.0, Ody = 0.0, Odz = 0.0;
+      if (as_func) {
+        Ox = tdep(la);
+        Oy = tdep(la + 1);
+        Oz = tdep(la + 2);
+        O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+      } else {
+        Ox = C->c[0];
+        Oy = C->c[1];
+        Oz = C->c[2];
+        if (!const_freq) {
+          Ox += C->c[4] * t;
+          Oy += C->c[5] * t;
+          Oz += C->c[6] * t;
+          O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        } else {
+          O2 = C->c[3];
+        }
+      }
+      double Ov = Ox * x + Oy * y + Oz * z;
+      fx += 2.0 * (Oz * vy - Oy * vz) + O2 * x - Ox * Ov;
+      fy += -2.0 * (Oz * vx - Ox * vz) + O2 * y - Oy * Ov;
+      fz += 2.0 * (Oy * vx - Ox * vy) + O2 * z - Oz * Ov;
+      if (!const_freq) {
+        if (as_func) {
+          Odx = tdepd(la, la + 3);
+          Ody = tdepd(la + 1, la + 4);
+          Odz = tdepd(la + 2, la + 5);
+        } else {
+          Odx = C->c[4];
+          Ody = C->c[5];
+          Odz = C->c[6];
+        }
+        fx -= -Odz * y + Ody * z;
+        fy -= Odz * x - Odx * z;
+        fz -= -Ody * x + Odx * y;
+      }
+      if (lin_acc) {
+        const double x0x = tdep(3), x0y = tdep(4), x0z = tdep(5);
+        const double v0x = tdep(6), v0y = tdep(7), v0z = tdep(8);
+        Ov = Ox * x0x + Oy * x0y + Oz * x0z;
+        fx += 2.0 * (Oz * v0y - Oy * v0z) + O2 * x0x - Ox * Ov;
+        fy += -2.0 * (Oz * v0x - Ox * v0z) + O2 * x0y - Oy * Ov;
+        fz += 2.0 * (Oy * v0x - Ox * v0y) + O2 * x0z - Oz * Ov;
+        if (!const_freq) {
+          fx -= -Odz * x0y + Ody * x0z;
+          fy -= Odz * x0x - Odx * x0z;
+          fz -= -Ody * x0x + Odx * x0y;
+        }
+      }
+    }
+  }
+  if (lin_acc) {
+    fx -= tdep(0);
+    fy -= tdep(1);
+    fz -= tdep(2);
+  }
+  Fx = C->p[0] * fx;
+  Fy = C->p[0] * fy;
+  Fz = C->p[0] * fz;
+}
+
+// The velocity-dependent components' Cartesian forces at (x, v, t)
+// (calcRforce / calczforce / calcphitorque with velocities,
+// galpy_potentials.c), added to a.
+__device__ __noinline__ void gh_diss_rect(const GhPotTable* __restrict__ T, double t, double x,
+                                          double y, double z, double vx, double vy, double vz,
+                                          double& ax, double& ay, double& az) {
+  for (int i = 0; i < T->npot; i++) {
+    const GhComponent* __restrict__ C = &T->comp[i];
+    if (C->type == GH_NONINERTIAL || C->type == GH_NONINERTIALINTERP) {
+      double fx, fy, fz;
+      gh_noninertial_xyz(T, C, t, x, y, z, vx, vy, vz, fx, fy, fz);
+      ax += fx;
+      ay += fy;
+      az += fz;
+    }
+  }
+}
+
 // Force functor handed to the integrators: the table plus the compile-time
 // composition.
 template <class Spec>
 struct GhForce {
   const GhPotTable* __restrict__ T;
+  // velocity-dependent components present (only the extended instances
+  // carry them)
+  __device__ __forceinline__ bool diss() const {
+    return std::is_same<Spec, GhSpecWrapped>::value && T->ndiss != 0;
+  }
+  // evalRectDeriv's acceleration (integrateFullOrbit.c:1326-1351): the
+  // conservative force plus the velocity-dependent ones
+  __device__ __forceinline__ void rect_v(double t, double x, double y, double z, double vx,
+                                         double vy, double vz, double& ax, double& ay,
+                                         double& az) const {
+    gh_rect_force<Spec>(T, t, x, y, z, ax, ay, az);
+    if (diss()) gh_diss_rect(T, t, x, y, z, vx, vy, vz, ax, ay, az);
+  }
+  __device__ __forceinline__ void rect_v(double t, double x, double y, double vx, double vy,
+                                         double& ax, double& ay) const {
+    rect(t, x, y, ax, ay);
+    if (diss()) {
+      double az = 0.0;
+      gh_diss_rect(T, t, x, y, 0.0, vx, vy, 0.0, ax, ay, az);
+    }
+  }
   __device__ __forceinline__ void rect(double t, double x, double y, double z, double& ax,
                                        double& ay, double& az) const {
     gh_rect_force<Spec>(T, t, x, y, z, ax, ay, az);

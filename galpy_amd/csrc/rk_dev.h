@@ -25,7 +25,7 @@ __device__ __forceinline__ void gh_rk_put(double* a, GhVec2 v) {
 // position derivative is its velocity, its velocity's is K dq)
 template <class F, class V>
 __device__ __forceinline__ void gh_rk_deriv(const F& T, double t, const double* y, double* a) {
-  GhPhase<V>::set(a, GhPhase<V>::p(y), gh_accel(T, t, GhPhase<V>::q(y)));
+  GhPhase<V>::set(a, GhPhase<V>::p(y), gh_accel_v(T, t, GhPhase<V>::q(y), GhPhase<V>::p(y)));
 }
 
 // bovy_rk4_onestep, bovy_rk.c:142-169 (yn1 equals yn on entry)

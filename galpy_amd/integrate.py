@@ -94,7 +94,7 @@ def _integrate_host(symbol, parse, ncol, pot, yo, t, int_method, rtol, atol, pro
         raise ValueError(f"{symbol} takes initial conditions of {ncol} phase-space coordinates")
     nobj = len(yo)
     rtol, atol = _parse_tol(rtol, atol)
-    npot, pot_type, pot_args, tfuncs = parse(pot)
+    npot, pot_type, pot_args, tfuncs = parse(pot, t)
     tf = tfunc_array(tfuncs)
     int_method_c = _parse_integrator(int_method)
     if dt is None:
@@ -140,7 +140,8 @@ class PotentialTable:
     def __init__(self, pot, planar=False, trange=None):
         self.pot = pot
         self.planar = bool(planar)
-        npot, pot_type, pot_args, tfuncs = (parse_pot_planar if planar else parse_pot)(pot)
+        npot, pot_type, pot_args, tfuncs = (parse_pot_planar if planar else parse_pot)(
+            pot, None if trange is None else numpy.asarray(trange, dtype=float))
         self.npot, self.pot_type, self.pot_args = npot, pot_type, pot_args
         lib = _lib.load()
         handle = ctypes.c_void_p()

@@ -2686,6 +2686,109 @@ class CylindricallySeparablePotentialWrapper(WrapperPotential):
         return [self._amp, self._Rp, self._refpot]
 
 
+def _on_grid(f, tgrid):
+    """A time function on a grid: one vectorised call when it takes arrays,
+    else a loop (scalars broadcast)."""
+    try:
+        v = numpy.asarray(f(tgrid), dtype=float)
+        if v.shape == tgrid.shape:
+            return v
+        if v.shape == ():
+            return numpy.full(tgrid.shape, float(v))
+    except Exception:
+        pass
+    return numpy.array([float(f(t)) for t in tgrid])
+
+
+class NonInertialFrameForce(Potential):
+    """The fictitious forces of integrating in a non-inertial frame
+    (NonInertialFrameForce.py:35-360): the frame's origin accelerates by
+    a0(t) (three functions or constants) and the frame rotates at Omega
+    (a number for rotation about z, or three; constants or functions of t,
+    with Omegadot); x0(t), v0(t) the origin's position and velocity when it
+    both rotates and accelerates.  A velocity-dependent force: integrated
+    with RK4, RK6, DOPR54 and DOP853 (galpy reroutes the symplectic methods).
+    Packed as pot_type 39 (the functions passed as pot_tfuncs) or, with
+    cinterp, 45 (natural splines of the functions over the integration's
+    times, cinterp_n points; integratePlanarOrbit.py:835-965)."""
+
+    pot_code = 39
+    dissipative = True
+
+    def __init__(self, amp=1.0, Omega=None, Omegadot=None, x0=None, v0=None, a0=None, cinterp=True,
+                 cinterp_n=3000):
+        Potential.__init__(self, amp=amp)
+        self.isNonAxi = True
+        self._cinterp, self._cinterp_n = bool(cinterp), int(cinterp_n)
+        self._rot_acc = Omega is not None
+        self._omegaz_only = numpy.size(Omega) == 1 if not callable(Omega) else True
+        self._const_freq = Omegadot is None
+        self._Omega_as_func = self._rot_acc and (callable(Omega) if self._omegaz_only else callable(Omega[0]))
+        self._Omega, self._Omegadot = Omega, Omegadot
+        if self._Omega_as_func and Omegadot is None:  # central differences (NonInertialFrameForce.py:28-32)
+            self._const_freq = False
+            fd = lambda f: (lambda t: (f(t + 1e-6) - f(t - 1e-6)) / 2e-6)
+            self._Omegadot = fd(Omega) if self._omegaz_only else [fd(o) for o in Omega]
+        self._lin_acc = a0 is not None
+        self._a0_as_func = self._lin_acc and callable(a0[0])
+        if self._lin_acc:
+            self._a0 = list(a0) if self._a0_as_func else [(lambda t, c=c: c) for c in a0]
+        if self._lin_acc and self._rot_acc:
+            if x0 is None or v0 is None:
+                raise ValueError("x0 and v0 are needed when the frame both rotates and accelerates")
+            self._x0, self._v0 = list(x0), list(v0)
+        self._Omega2 = (float(numpy.sum(numpy.asarray(Omega, dtype=float) ** 2))
+                        if self._rot_acc and not self._Omega_as_func else 0.0)
+
+    def _base_args(self):
+        a = [self._amp] + [0.0] * 10 + [float(self._rot_acc), float(self._lin_acc), float(self._omegaz_only),
+                                        float(self._const_freq), float(self._Omega_as_func)]
+        if self._Omega_as_func or not self._rot_acc:
+            return a + [0.0] * 7
+        a += [0.0, 0.0, float(self._Omega)] if self._omegaz_only else [float(o) for o in self._Omega]
+        a.append(self._Omega2)
+        if self._const_freq:
+            return a + [0.0, 0.0, 0.0]
+        return a + ([0.0, 0.0, float(self._Omegadot)] if self._omegaz_only else [float(o) for o in self._Omegadot])
+
+    def _functions(self):
+        """The primitives in galpy's index order: a0, x0, v0, Omega."""
+        f = []
+        if self._lin_acc:
+            f += self._a0
+            if self._rot_acc:
+                f += self._x0 + self._v0
+        if self._Omega_as_func:
+            f += [self._Omega] if self._omegaz_only else list(self._Omega)
+        return f
+
+    def pack(self, t=None):
+        """(pot_type, pot_args, pot_tfuncs) as galpy's
+        _parse_noninertial_frame_force packs them."""
+        genuine = (self._lin_acc and self._a0_as_func) or (self._lin_acc and self._rot_acc) or self._Omega_as_func
+        if self._cinterp and genuine:
+            if t is None:
+                raise NotImplementedError("NonInertialFrameForce with cinterp=True needs the integration times")
+            tmin, tmax = float(numpy.amin(t)), float(numpy.amax(t))
+            tgrid = numpy.linspace(tmin, tmax, self._cinterp_n)
+            arrs = [_on_grid(f, tgrid) for f in self._functions()]
+            args = [float(len(arrs)), float(self._cinterp_n)] + list(tgrid)
+            for a in arrs:
+                args += list(a)
+            return 45, args + self._base_args() + [tmin, tmax], []
+        tf = []
+        if self._lin_acc:
+            tf += self._a0
+            if self._rot_acc:
+                tf += self._x0 + self._v0
+        if self._Omega_as_func:
+            tf += [self._Omega, self._Omegadot] if self._omegaz_only else list(self._Omega) + list(self._Omegadot)
+        return 39, self._base_args(), tf
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        raise NotImplementedError("a NonInertialFrameForce has no potential")
+
+
 def _os_Rz_to_uv(R, z, delta):
     """Prolate spheroidal (u, v) of focal length delta (galpy's
     coords.Rz_to_uv, as OblateStaeckelWrapperPotential.c:4-13 forms it)."""
@@ -2876,7 +2979,7 @@ def flatten(pot):
     return out
 
 
-def _pack(comps, planar, tfuncs=None):
+def _pack(comps, planar, tfuncs=None, t=None):
     """Codes and arguments of a list of components in galpy's order; a
     wrapper's code is followed by its wrapped codes, its arguments are
     nwrapped, the wrapped arguments, then its own
@@ -2893,7 +2996,7 @@ def _pack(comps, planar, tfuncs=None):
                     "Planar orbit integration for RotateAndTiltWrapperPotential is not implemented; "
                     "integrate an orbit with (z, vz) = (0, 0) instead")
             inner = flatten(p._pot)
-            wt, wa = _pack(inner, planar, tfuncs)
+            wt, wa = _pack(inner, planar, tfuncs, t)
             pot_type.append(p.pot_code)
             pot_type.extend(wt)
             pot_args.append(float(_npot(inner)))
@@ -2911,6 +3014,12 @@ def _pack(comps, planar, tfuncs=None):
             pt, pa, _ = p.pack()
             pot_type.extend(pt)
             pot_args.extend(pa)
+            continue
+        if isinstance(p, NonInertialFrameForce):
+            code, pa, tf = p.pack(t)
+            pot_type.append(code)
+            pot_args.extend(pa)
+            tfuncs.extend(tf)
             continue
         if not isinstance(p, Potential) or p.pot_code is None:
             raise NotImplementedError(
@@ -2935,17 +3044,19 @@ def _purge_null(comps):
     return kept if kept else comps
 
 
-def parse_pot(pot):
+def parse_pot(pot, t=None):
     """galpy's ``_parse_pot`` (integrateFullOrbit.py:43-614) for the supported
     potentials: returns ``(npot, pot_type int32[], pot_args float64[],
     pot_tfuncs)``, pot_tfuncs the components' time functions (callables);
     npot counts the top-level components (a wrapper is one).
 
     Raises NotImplementedError for a potential without a GPU kernel, as galpy
-    does for a potential without a C implementation (:604-611)."""
+    does for a potential without a C implementation (:604-611).  ``t``: the
+    integration's times (a NonInertialFrameForce with cinterp tabulates its
+    functions over their range)."""
     comps = _purge_null(flatten(pot))
     tfuncs = []
-    pot_type, pot_args = _pack(comps, False, tfuncs)
+    pot_type, pot_args = _pack(comps, False, tfuncs, t)
     return (
         _npot(comps),
         numpy.array(pot_type, dtype=numpy.int32, order="C"),
@@ -2954,13 +3065,13 @@ def parse_pot(pot):
     )
 
 
-def parse_pot_planar(pot):
+def parse_pot_planar(pot, t=None):
     """galpy's planar ``_parse_pot`` (integratePlanarOrbit.py:33-663) for the
     supported potentials, taken in the plane z = 0: returns
     ``(npot, pot_type int32[], pot_args float64[], [])``."""
     comps = _purge_null(flatten(pot))
     tfuncs = []
-    pot_type, pot_args = _pack(comps, True, tfuncs)
+    pot_type, pot_args = _pack(comps, True, tfuncs, t)
     return (
         _npot(comps),
         numpy.array(pot_type, dtype=numpy.int32, order="C"),

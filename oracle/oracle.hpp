@@ -240,7 +240,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                // wrappers (integrateFullOrbit.c:821-895)
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
-               kOblateStaeckelWrap = -3,
+               kOblateStaeckelWrap = -3, kNonInertial = 39, kNonInertialInterp = 45,
                kTimeDepAmpWrap = -9, kMovingObjectWrap = -6, kInterpSph = 36, kSCF = 24,
                kDiskSCF = 26, kInterpRZ = 13, kMultipole = 44,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
@@ -314,6 +314,12 @@ struct NatSpline {
     R dx = t - R(x[i]);
     return R(y[i]) + dx * (R(b[i]) + dx * (R(c[i]) + dx * R(d[i])));
   }
+  // derivative (gsl_spline_eval_deriv: b_i + d (2 c_i + 3 d_i d))
+  double deriv(double t) const {
+    const int i = find(t);
+    const double dx = t - x[i];
+    return b[i] + dx * (2.0 * c[i] + 3.0 * d[i] * dx);
+  }
   // integral over [x[0], t]
   template <class R> R integ(R t) const {
     const int k = find(dval(t));
@@ -339,6 +345,8 @@ struct Component {
   const double* args = nullptr;
   // a wrapper's time function (potentialArg.tfuncs, galpy_potentials.h)
   double (*tfunc)(double) = nullptr;
+  // a component with several time functions: the first's index in g_tfuncs
+  int tf0 = 0;
   // cubic splines a component interpolates (potentialArg.spline1d): built
   // at parse time, shared by copies of the component
   std::shared_ptr<std::vector<NatSpline>> spl;
@@ -383,6 +391,7 @@ inline int component_nargs(int code, bool planar = false) {
     case kDehnenSph: case kHomogSphere: return 3;
     case kEinasto: case kExpTruncNFW: return 3;  // amp, h, n / amp, a, rc
     case kTwoPowerSph: return 4;                  // amp, a, alpha, beta
+    case kNonInertial: return 23;  // amp, 10 cache slots, 5 flags, Omega[3], Omega^2, Omegadot[3]
     // wrappers' own arguments, read after the wrapped components
     case kDehnenSmoothWrap: return 4;   // amp, tform, tsteady, grow
     case kSolidBodyRotWrap: return 3;   // amp, omega, pa
@@ -474,6 +483,8 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
              : code == kDiskSCF ? (int)pot_args[0] + 3
              : code == kInterpRZ ? interprz_nargs(pot_args)
              : code == kMultipole ? multipole_nargs(pot_args)
+             : code == kNonInertialInterp ? 0
+             : code == kNonInertial ? 23
              : code == kMovingObjectWrap ? 0  // counted after its wrapped components
                                    : component_nargs(code, planar);
     if (na < 0) return -1;
@@ -500,6 +511,27 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       (*C.spl)[0].build(pot_args + 1, pot_args + 1 + nr, nr);
       for (int j = 0; j < 6; j++) C.p[j] = pot_args[1 + 2 * nr + j];  // amp, rmin, rmax, Mmax, Phi0, Phimax
       pot_args += na;
+      continue;
+    }
+    if (code == kNonInertialInterp) {  // initNonInertialFrameForceSplines, integrateFullOrbit.c:1435-1462
+      const int ns = (int)pot_args[0], npts = (int)pot_args[1];
+      const double* tg = pot_args + 2;
+      C.spl = std::make_shared<std::vector<NatSpline>>(ns);
+      for (int k = 0; k < ns; k++) (*C.spl)[k].build(tg, tg + (k + 1) * npts, npts);
+      C.args = pot_args + 2 + (1 + ns) * npts;  // the 23 arguments of case 39, then tmin, tmax
+      C.p[0] = C.args[0];
+      pot_args = C.args + 25;
+      continue;
+    }
+    if (code == kNonInertial) {  // its time functions, in parse order (integrateFullOrbit.c:689-704)
+      const double* a = pot_args;
+      const int ntf = (int)(3 * a[12] * (1 + 2 * a[11]) + (6 - 4 * a[13]) * a[15]);
+      if (tf_next + ntf > g_ntfuncs) return -1;
+      C.tf0 = tf_next;
+      tf_next += ntf;
+      C.args = a;
+      C.p[0] = a[0];
+      pot_args += 23;
       continue;
     }
     if (code == kMovingObjectWrap) {  // initMovingObjectSplines, integrateFullOrbit.c:1392-1431
@@ -2470,10 +2502,33 @@ void planar_rect_force(int npot, const Component* P, R t, const R* q, R* a) {
 
 // evalPlanarRectDeriv: integratePlanarOrbit.c:1032-1055 (vR, vT feed only
 // dissipative forces, none in scope)
+inline void diss_forces(int npot, const Component* P, double Rc, double z, double phi, double t, double vR,
+                        double vT, double vz, double& FR, double& Fz, double& Fp);
+inline bool has_dissipative(int npot, const Component* P);
+// the velocity-dependent forces of planar orbits (evalPlanarRectDeriv with
+// PlanarRforce(R, phi, t, vR, vT) = Rforce(R, 0, phi, t, vR, vT, 0))
+inline void planar_diss_add(int npot, const Component* P, double t, const double* q, double* a) {
+  const double x = q[0], y = q[1], Rc = std::sqrt(x * x + y * y);
+  double phi = std::acos(x / Rc);
+  const double sinphi = y / Rc, cosphi = x / Rc;
+  if (y < 0.) phi = 2. * M_PI - phi;
+  const double vR = q[2] * cosphi + q[3] * sinphi, vT = -q[2] * sinphi + q[3] * cosphi;
+  double FR, Fz, Fp;
+  diss_forces(npot, P, Rc, 0., phi, t, vR, vT, 0., FR, Fz, Fp);
+  a[2] += cosphi * FR - 1. / Rc * sinphi * Fp;
+  a[3] += sinphi * FR + 1. / Rc * cosphi * Fp;
+}
 template <class R>
 void planar_rect_deriv(int npot, const Component* P, R t, const R* q, R* a) {
   a[0] = q[2]; a[1] = q[3];
   planar_rect_force(npot, P, t, q, a + 2);
+  if (has_dissipative(npot, P)) {
+    const double qd[4] = {dval(q[0]), dval(q[1]), dval(q[2]), dval(q[3])};
+    double ad[4] = {0., 0., 0., 0.};
+    planar_diss_add(npot, P, dval(t), qd, ad);
+    a[2] += R(ad[2]);
+    a[3] += R(ad[3]);
+  }
 }
 
 // Components with a restated Hessian (cyl_hessian_full, planar_hessian):
@@ -2633,6 +2688,142 @@ R potential_value(int npot, const Component* P, R Rc, R z, R phi, R t) {
   return pot;
 }
 
+// NonInertialFrameForce.c:27-170: the fictitious forces of a frame whose
+// origin accelerates by a0(t) and that rotates at Omega(t): -a0 +
+// 2 v x Omega... written out as galpy does (centrifugal, Coriolis, Euler
+// terms, with the frame origin's x0, v0 when both accelerations are on).
+// Time inputs from the caller's functions (pot_type 39; Omegadot its own
+// function) or from natural splines of them (45; Omegadot the Omega
+// spline's derivative, times clamped to [tmin, tmax]).  args: amp, 10 cache
+// slots, rot_acc, lin_acc, omegaz_only, const_freq, Omega_as_func,
+// Omega[3], Omega^2, Omegadot[3] (then tmin, tmax for 45).
+inline void noninertial_xyz(const Component& C, double x, double y, double z, double vx, double vy, double vz,
+                            double t, double& Fx, double& Fy, double& Fz) {
+  const double* a = C.args;
+  const bool interp = C.type == kNonInertialInterp;
+  const double tq = interp ? (t < a[23] ? a[23] : (t > a[24] ? a[24] : t)) : t;
+  auto tdep = [&](int idx) { return interp ? (*C.spl)[idx].eval(tq) : g_tfuncs[C.tf0 + idx](t); };
+  auto tdepd = [&](int sidx, int tidx) { return interp ? (*C.spl)[sidx].deriv(tq) : g_tfuncs[C.tf0 + tidx](t); };
+  const bool rot_acc = a[11] != 0., lin_acc = a[12] != 0., Omega_as_func = a[15] != 0.;
+  const int la = lin_acc ? 1 : 0;
+  Fx = Fy = Fz = 0.;
+  if (rot_acc) {
+    const bool omegaz_only = a[13] != 0., const_freq = a[14] != 0.;
+    if (omegaz_only) {
+      double Oz, O2, Odz = 0.;
+      if (Omega_as_func) {
+        Oz = tdep(9 * la);
+        O2 = Oz * Oz;
+      } else {
+        Oz = a[18];
+        if (!const_freq) {
+          Oz += a[22] * t;
+          O2 = Oz * Oz;
+        } else {
+          O2 = a[19];
+        }
+      }
+      Fx += 2. * Oz * vy + O2 * x;
+      Fy += -2. * Oz * vx + O2 * y;
+      if (!const_freq) {
+        Odz = Omega_as_func ? tdepd(9 * la, 9 * la + 1) : a[22];
+        Fx += Odz * y;
+        Fy -= Odz * x;
+      }
+      if (lin_acc) {
+        const double x0x = tdep(3), x0y = tdep(4), v0x = tdep(6), v0y = tdep(7);
+        Fx += 2. * Oz * v0y + O2 * x0x;
+        Fy += -2. * Oz * v0x + O2 * x0y;
+        if (!const_freq) {
+          Fx += Odz * x0y;
+          Fy -= Odz * x0x;
+        }
+      }
+    } else {
+      double Ox, Oy, Oz, O2, Odx = 0., Ody = 0., Odz = 0.;
+      if (Omega_as_func) {
+        Ox = tdep(9 * la);
+        Oy = tdep(9 * la + 1);
+        Oz = tdep(9 * la + 2);
+        O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+      } else {
+        Ox = a[16];
+        Oy = a[17];
+        Oz = a[18];
+        if (!const_freq) {
+          Ox += a[20] * t;
+          Oy += a[21] * t;
+          Oz += a[22] * t;
+          O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        } else {
+          O2 = a[19];
+        }
+      }
+      double Ov = Ox * x + Oy * y + Oz * z;
+      Fx += 2. * (Oz * vy - Oy * vz) + O2 * x - Ox * Ov;
+      Fy += -2. * (Oz * vx - Ox * vz) + O2 * y - Oy * Ov;
+      Fz += 2. * (Oy * vx - Ox * vy) + O2 * z - Oz * Ov;
+      if (!const_freq) {
+        if (Omega_as_func) {
+          Odx = tdepd(9 * la, 9 * la + 3);
+          Ody = tdepd(9 * la + 1, 9 * la + 4);
+          Odz = tdepd(9 * la + 2, 9 * la + 5);
+        } else {
+          Odx = a[20];
+          Ody = a[21];
+          Odz = a[22];
+        }
+        Fx -= -Odz * y + Ody * z;
+        Fy -= Odz * x - Odx * z;
+        Fz -= -Ody * x + Odx * y;
+      }
+      if (lin_acc) {
+        const double x0x = tdep(3), x0y = tdep(4), x0z = tdep(5), v0x = tdep(6), v0y = tdep(7), v0z = tdep(8);
+        Ov = Ox * x0x + Oy * x0y + Oz * x0z;
+        Fx += 2. * (Oz * v0y - Oy * v0z) + O2 * x0x - Ox * Ov;
+        Fy += -2. * (Oz * v0x - Ox * v0z) + O2 * x0y - Oy * Ov;
+        Fz += 2. * (Oy * v0x - Ox * v0y) + O2 * x0z - Oz * Ov;
+        if (!const_freq) {
+          Fx -= -Odz * x0y + Ody * x0z;
+          Fy -= Odz * x0x - Odx * x0z;
+          Fz -= -Ody * x0x + Odx * x0y;
+        }
+      }
+    }
+  }
+  if (lin_acc) {
+    Fx -= tdep(0);
+    Fy -= tdep(1);
+    Fz -= tdep(2);
+  }
+}
+inline bool is_dissipative(int code) { return code == kNonInertial || code == kNonInertialInterp; }
+// calcRforce / calczforce / calcphitorque with velocities
+// (galpy_potentials.c, the RforceVelocity branch): the dissipative
+// components' cylindrical forces at (R, z, phi, t; vR, vT, vz)
+inline void diss_forces(int npot, const Component* P, double Rc, double z, double phi, double t, double vR,
+                        double vT, double vz, double& FR, double& Fz, double& Fp) {
+  FR = Fz = Fp = 0.;
+  for (int i = 0; i < npot; i++) {
+    const Component& C = P[i];
+    if (!is_dissipative(C.type)) continue;
+    // cyl_to_rect, cyl_to_rect_vec (bovy_coords.c)
+    const double cp = std::cos(phi), sp = std::sin(phi);
+    const double x = Rc * cp, y = Rc * sp, vx = vR * cp - vT * sp, vy = vR * sp + vT * cp;
+    double Fx, Fy, Fzz;
+    noninertial_xyz(C, x, y, z, vx, vy, vz, t, Fx, Fy, Fzz);
+    const double amp = C.p[0];
+    FR += amp * (cp * Fx + sp * Fy);
+    Fp += amp * Rc * (-sp * Fx + cp * Fy);
+    Fz += amp * Fzz;
+  }
+}
+inline bool has_dissipative(int npot, const Component* P) {
+  for (int i = 0; i < npot; i++)
+    if (is_dissipative(P[i].type)) return true;
+  return false;
+}
+
 // evalRectForce: integrateFullOrbit.c:1307-1325
 template <class R>
 void rect_force(int npot, const Component* P, R t, const R* q, R* a) {
@@ -2655,6 +2846,19 @@ template <class R>
 void rect_deriv(int npot, const Component* P, R t, const R* q, R* a) {
   a[0] = q[3]; a[1] = q[4]; a[2] = q[5];
   rect_force(npot, P, t, q, a + 3);
+  if (has_dissipative(npot, P)) {  // the velocity-dependent forces (vR, vT as evalRectDeriv forms them)
+    const double x = dval(q[0]), y = dval(q[1]), Rc = std::sqrt(x * x + y * y);
+    double phi = std::acos(x / Rc);
+    const double sinphi = y / Rc, cosphi = x / Rc;
+    if (y < 0.) phi = 2. * M_PI - phi;
+    const double vR = dval(q[3]) * cosphi + dval(q[4]) * sinphi;
+    const double vT = -dval(q[3]) * sinphi + dval(q[4]) * cosphi;
+    double FR, Fz, Fp;
+    diss_forces(npot, P, Rc, dval(q[2]), phi, dval(t), vR, vT, dval(q[5]), FR, Fz, Fp);
+    a[3] += R(cosphi * FR - 1. / Rc * sinphi * Fp);
+    a[4] += R(sinphi * FR + 1. / Rc * cosphi * Fp);
+    a[5] += R(Fz);
+  }
 }
 
 // bovy_coords.c:101-115

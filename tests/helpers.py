@@ -48,7 +48,7 @@ def _p(a):
 # packed arguments per component (integrateFullOrbit.py:65-365; the planar
 # packing gives PowerSphericalPotentialwCutoff 3) and wrappers' own arguments
 _NARGS = {0: 4, 1: 6, 2: 8, 3: 8, 28: 9, 29: 1, 4: 6, 5: 3, 6: 4, 16: 3, 25: 13, 7: 2, 8: 2, 9: 2, 10: 2, 14: 2, 15: 5, 17: 2, 18: 2, 19: 2, 20: 2,
-          33: 2, 34: 3, 35: 3, 12: 4, 40: 0, 41: 3, 42: 4, 46: 3, -1: 4, -2: 3, -4: 5, -5: 3, -8: 32, -9: 1, -10: 3, -12: 3, -3: 5}
+          33: 2, 34: 3, 35: 3, 12: 4, 40: 0, 41: 3, 42: 4, 46: 3, -1: 4, -2: 3, -4: 5, -5: 3, -8: 32, -9: 1, -10: 3, -12: 3, -3: 5, 39: 23}
 
 
 _ELLIPSOIDAL = (21, 22, 23, 30, 37, 38, 43)
@@ -76,6 +76,8 @@ def _nargs(code, pot_args, ai, planar):
         nblk = sum(2 * min(l + 1, M) - 1 for l in range(L))
         per = 12 * (Nr - 1) + Nr
         return 4 + Nr + 2 + Nt + nblk * (per if Nt == 0 else 4 * (Nt - 1) * per)
+    if code == 45:  # NonInertialFrameForce with splines: n_spline, nPts, tgrid, arrays, 23 + 2
+        return 2 + (1 + int(pot_args[ai])) * int(pot_args[ai + 1]) + 25
     if code == 26:  # DiskSCFPotential pair: nsigma + 3
         return int(pot_args[ai]) + 3
     if code == 36:  # interpSphericalPotential: 1 + 2 nr + 6
