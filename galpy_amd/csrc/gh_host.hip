@@ -992,7 +992,11 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
       nleaf++;
       continue;
     }
-    const int na = (code == GH_MOVINGOBJWRAP) ? 0 : nargs_of(code, planar);  // counted below
+    if (code == GH_CHANDRASEKHAR && planar) {  // galpy has no planar dynamical friction
+      set_error("ChandrasekharDynamicalFrictionForce has no planar version");
+      return -1;
+    }
+    const int na = (code == GH_MOVINGOBJWRAP || code == GH_CHANDRASEKHAR) ? 0 : nargs_of(code, planar);
     if (na < 0) {
       set_error("wrapper pot_type %d has no HIP kernel", code);
       return -1;
@@ -1015,6 +1019,37 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
     GhWrap& W = T->wrap[w];
     W.kind = code;
     W.first = first;
+    if (code == GH_CHANDRASEKHAR) {
+      // ChandrasekharDynamicalFrictionForce (integrateFullOrbit.c:917-927,
+      // initChandrasekharDynamicalFrictionSplines :1464-1491): nPts, r, sigma_r
+      // (a natural spline over (r - ro) / (rf - ro)), then amp, 8 cache slots,
+      // ms, rhm, gamma^2, lnLambda, minr^2, ro, rf.  Its wrapped components
+      // give only the density, not forces.
+      const double* a = pot_args;
+      const int n = (int)a[0];
+      const double* ra = a + 1;
+      const double* own = ra + 2L * n;
+      std::vector<double> rn(n);
+      for (int j = 0; j < n; j++) rn[j] = (ra[j] - own[14]) / (own[15] - own[14]);
+      int off = 0;
+      if (build_spline(rn.data(), ra + n, n, T, off) != 0) return -1;
+      W.p[0] = own[0];
+      for (int j = 0; j < 7; j++) W.p[1 + j] = own[9 + j];  // ms, rhm, gamma^2, lnLambda, minr^2, ro, rf
+      W.p[8] = off;
+      W.p[9] = nleaf - first;
+      for (int l = first; l < nleaf; l++) {
+        if (!gh_has_density(T->comp[l].type) || T->comp[l].nwrap) {
+          set_error("ChandrasekharDynamicalFrictionForce: no device density for pot_type %d",
+                    T->comp[l].type);
+          return -1;
+        }
+        T->comp[l].densonly = 1;
+      }
+      T->ndiss++;
+      T->has_wrap = 1;
+      pot_args = own + 16;
+      continue;
+    }
     if (code == GH_MOVINGOBJWRAP) {
       // initMovingObjectSplines (integrateFullOrbit.c:1392-1431; planar
       // integratePlanarOrbit.c:1177-1215): nPts, t, x, y[, z], amp, t0, tf;

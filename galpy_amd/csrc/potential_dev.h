@@ -97,6 +97,7 @@ enum GhPotCode {
   // velocity-dependent forces (integrateFullOrbit.c:689-721)
   GH_NONINERTIAL = 39,
   GH_NONINERTIALINTERP = 45,
+  GH_CHANDRASEKHAR = -7,
 };
 
 struct GhComponent {
@@ -107,6 +108,7 @@ struct GhComponent {
   int ext;       // offset of the component's variable-length data in GhPotTable::ext
   int mapwrap;   // wrappers that move the evaluation point: 0 none, else passes (1 or 2)
   int mappot;    // the mapped walk needs the component's potential too (OblateStaeckel)
+  int densonly;  // only a density (for dynamical friction): no force, no potential
   double p[8];   // raw galpy arguments, in galpy's order
   double c[14];  // precomputed constants, per type (decoded on the host)
 };
@@ -2159,6 +2161,7 @@ __device__ __noinline__ GhAcc gh_comp_force_mapped(const GhPotTable* __restrict_
 __device__ __forceinline__ void gh_comp_force_any(const GhPotTable* __restrict__ T, int i,
                                                   const GhGeom& g, GhAcc& F) {
   const GhComponent* __restrict__ C = &T->comp[i];
+  if (C->densonly) return;
   if (C->mapwrap) {
     const GhAcc M = gh_comp_force_mapped(T, i, g);
     F.FRoR += M.FRoR;
@@ -2397,6 +2400,95 @@ __device__ __noinline__ void gh_noninertial_xyz(const GhPotTable* __restrict__ T
   Fz = C->p[0] * fz;
 }
 
+// The densities dynamical friction needs (the Dens functions of
+// HernquistPotential.c, NFWPotential.c, PlummerPotential.c,
+// MiyamotoNagaiPotential.c, PowerSphericalPotentialwCutoff.c,
+// LogarithmicHaloPotential.c, JaffePotential.c, IsochronePotential.c,
+// PowerSphericalPotential.c, BurkertPotential.c), from galpy's arguments p.
+__host__ __device__ inline bool gh_has_density(int type) {
+  return type == GH_HERNQUIST || type == GH_NFW || type == GH_PLUMMER ||
+         type == GH_MIYAMOTONAGAI || type == GH_POWERSPHCUTOFF || type == GH_LOGHALO ||
+         type == GH_JAFFE || type == GH_ISOCHRONE || type == GH_POWERSPH || type == GH_BURKERT;
+}
+__device__ inline double gh_comp_density(const GhComponent* __restrict__ C, double R, double z,
+                                         double sphi, double cphi) {
+  const double* __restrict__ p = C->p;
+  const double r2 = R * R + z * z, r = sqrt(r2);
+  switch (C->type) {
+    case GH_HERNQUIST: return p[0] * M_1_PI / 4.0 / p[1] / p[1] / r * pow(1.0 + r / p[1], -3.0);
+    case GH_NFW: return p[0] * M_1_PI / 4.0 / p[1] / p[1] / (1.0 + r / p[1]) / (1.0 + r / p[1]) / r;
+    case GH_PLUMMER: {
+      const double b2 = p[1] * p[1];
+      return 3.0 * p[0] * M_1_PI / 4.0 * b2 * pow(r2 + b2, -2.5);
+    }
+    case GH_MIYAMOTONAGAI: {
+      const double a = p[1], b2 = p[2] * p[2], sbz = sqrt(b2 + z * z);
+      const double asbz = (a + sbz) * (a + sbz);
+      if (a == 0.0) return 3.0 * p[0] * M_1_PI / 4.0 * b2 * pow(R * R + sbz * sbz, -2.5);
+      return p[0] * M_1_PI / 4.0 * b2 * (a * R * R + (a + 3.0 * sbz) * asbz) *
+             pow(R * R + asbz, -2.5) * pow(sbz, -3.0);
+    }
+    case GH_POWERSPHCUTOFF: return p[0] * pow(r, -p[1]) * exp(-r2 / p[2] / p[2]);
+    case GH_LOGHALO: {
+      const double amp = p[0], q = p[1], c = p[2], om = p[3], zq = z / q, q2 = q * q;
+      if (om < 1.0) {
+        const double R2 = R * R, Rt2 = R2 * (1.0 - om * sphi * sphi);
+        const double dn = 1.0 / (Rt2 + zq * zq + c), dn2 = dn * dn;
+        const double s2p = 2.0 * sphi * cphi, c2p = cphi * cphi - sphi * sphi;
+        return amp * M_1_PI / 4.0 *
+               (2.0 * Rt2 / R2 * (dn - Rt2 * dn2) + dn / q2 - 2.0 * zq * zq * dn2 / q2 -
+                om * (2.0 * R2 * s2p * s2p / 4.0 * om * dn2 + dn * c2p));
+      }
+      return amp * M_1_PI / 4.0 / q2 * ((2.0 * q2 + 1.0) * c + R * R + (2.0 - 1.0 / q2) * z * z) /
+             ((R * R + zq * zq + c) * (R * R + zq * zq + c));
+    }
+    case GH_JAFFE: return p[0] * M_1_PI / 4.0 / p[1] * pow(r * (1.0 + r / p[1]), -2.0);
+    case GH_ISOCHRONE: {
+      const double b = p[1], rb = sqrt(r2 + b * b), brbrb = (b + rb) * rb;
+      return p[0] * M_1_PI / 4.0 * (3.0 * brbrb * rb - r2 * (b + 3.0 * rb)) * pow(brbrb, -3.0);
+    }
+    case GH_POWERSPH: return p[0] * M_1_PI / 4.0 * (3.0 - p[1]) * pow(r2, -0.5 * p[1]);
+    case GH_BURKERT: {
+      const double x = r / p[1];
+      return p[0] / (1.0 + x) / (1.0 + x * x);
+    }
+    default: return 0.0;
+  }
+}
+// ChandrasekharDynamicalFrictionForce.c:13-140: F = A v, A = -amp Xfactor(X)
+// lnLambda rho / v^3, X = v / (sqrt 2 sigma_r(r)) with sigma_r a natural
+// spline over (r - ro) / (rf - ro) clamped to [0, 1], Xfactor = erf X -
+// (2/sqrt pi) X e^-X^2, lnLambda constant or 0.5 ln(1 + r^2 / (gamma^2
+// max(rhm, GM/v^2)^2)); zero inside minr.  W->p: amp, ms, rhm, gamma^2,
+// lnLambda (< 0: variable), minr^2, ro, rf, spline offset, the number of
+// density components (from W->first).
+__device__ __noinline__ double gh_chandrasekhar_amp(const GhPotTable* __restrict__ T,
+                                                    const GhWrap* __restrict__ W, double x,
+                                                    double y, double z, double v2) {
+  const double R2 = x * x + y * y, r2 = R2 + z * z;
+  if (r2 < W->p[5]) return 0.0;
+  const double amp = W->p[0], ms = W->p[1], rhm = W->p[2], gamma2 = W->p[3];
+  const double ro = W->p[6], rf = W->p[7];
+  double lnL = W->p[4];
+  const double r = sqrt(r2), v = sqrt(v2);
+  if (lnL < 0.0) {
+    const double GMvs = ms / v / v;
+    lnL = (GMvs < rhm) ? 0.5 * log(1.0 + r2 / gamma2 / rhm / rhm)
+                       : 0.5 * log(1.0 + r2 / gamma2 / GMvs / GMvs);
+  }
+  double d = (r - ro) / (rf - ro);
+  d = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+  const double sr = gh_spline_eval(T, (int)W->p[8], d);
+  const double X = M_SQRT1_2 * v / sr;
+  const double Xf = erf(X) - M_2_SQRTPI * X * exp(-X * X);
+  const double R = sqrt(R2);
+  const double sphi = (R > 0.0) ? y / R : 0.0, cphi = (R > 0.0) ? x / R : 1.0;
+  double dens = 0.0;
+  const int n = (int)W->p[9];
+  for (int k = 0; k < n; k++) dens += gh_comp_density(&T->comp[W->first + k], R, z, sphi, cphi);
+  return -amp * Xf * lnL / v2 / v * dens;
+}
+
 // The velocity-dependent components' Cartesian forces at (x, v, t)
 // (calcRforce / calczforce / calcphitorque with velocities,
 // galpy_potentials.c), added to a.
@@ -2412,6 +2504,14 @@ __device__ __noinline__ void gh_diss_rect(const GhPotTable* __restrict__ T, doub
       ay += fy;
       az += fz;
     }
+  }
+  for (int w = 0; w < T->nwraps; w++) {
+    const GhWrap* __restrict__ W = &T->wrap[w];
+    if (W->kind != GH_CHANDRASEKHAR) continue;
+    const double A = gh_chandrasekhar_amp(T, W, x, y, z, vx * vx + vy * vy + vz * vz);
+    ax += A * vx;
+    ay += A * vy;
+    az += A * vz;
   }
 }
 
@@ -2759,6 +2859,7 @@ __device__ __forceinline__ double gh_potential(const GhPotTable* __restrict__ T,
   double total = 0.0;
   for (int i = 0; i < T->npot; i++) {
     const GhComponent* __restrict__ C = &T->comp[i];
+    if (C->densonly) continue;
     if (C->mapwrap) {
       total += gh_mapped_potential(T, i, R * cosphi0, R * sinphi0, z, t);
       continue;

@@ -2789,6 +2789,50 @@ class NonInertialFrameForce(Potential):
         raise NotImplementedError("a NonInertialFrameForce has no potential")
 
 
+class ChandrasekharDynamicalFrictionForce(Potential):
+    """Chandrasekhar dynamical friction on a satellite of mass GMs moving
+    through the density of ``dens`` (ChandrasekharDynamicalFrictionForce.py
+    :20-330): F = -4 pi GMs amp rho lnLambda [erf X - 2X e^-X^2/sqrt(pi)] v /
+    v^3, X = v / (sqrt 2 sigma_r(r)), lnLambda constant (const_lnLambda) or
+    0.5 ln(1 + r^2 / (gamma^2 max(rhm, GMs/v^2)^2)), nothing inside minr.
+    sigma_r: a function of r, sampled on nr points over [minr, maxr]; the
+    default density is a LogarithmicHalo (normalize=1, q=1) with sigma_r =
+    1/sqrt(2).  Velocity-dependent: integrated with RK4, RK6, DOPR54 and
+    DOP853; full orbits only.  Packed as pot_type -7 followed by the density's
+    codes; arguments: npot of the density, its arguments, nr, the radii,
+    sigma_r there, then amp, 8 cache slots, GMs, rhm, gamma^2, lnLambda (-1:
+    variable), minr^2, minr, maxr (integrateFullOrbit.py:517-545)."""
+
+    pot_code = -7
+    dissipative = True
+
+    def __init__(self, amp=1.0, GMs=0.1, gamma=1.0, rhm=0.0, dens=None, sigmar=None, const_lnLambda=False,
+                 minr=0.0001, maxr=25.0, nr=501):
+        Potential.__init__(self, amp=amp * GMs)
+        self.isNonAxi = True
+        self._ms, self._gamma, self._rhm = GMs, gamma, rhm
+        self._minr, self._maxr = minr, maxr
+        if dens is None:
+            dens = LogarithmicHaloPotential(normalize=1.0, q=1.0)
+            if sigmar is None:
+                sigmar = lambda r: 1.0 / numpy.sqrt(2.0)
+        if sigmar is None:
+            raise NotImplementedError("give sigmar, the radial velocity dispersion as a function of r")
+        self._dens_pot = dens
+        self._rs = numpy.linspace(minr, maxr, nr)
+        self._srs = numpy.array([sigmar(r) for r in self._rs], dtype=float)
+        self._lnLambda = const_lnLambda if const_lnLambda else False
+        self._amp *= 4.0 * numpy.pi
+
+    def own_args(self):
+        return ([float(len(self._rs))] + list(self._rs) + list(self._srs) + [self._amp, -1.0] + [0.0] * 7
+                + [self._ms, self._rhm, self._gamma**2.0, -1.0 if not self._lnLambda else self._lnLambda,
+                   self._minr**2.0, self._rs[0], self._rs[-1]])
+
+    def _evaluate(self, R, z, phi=0.0, t=0.0):
+        raise NotImplementedError("dynamical friction has no potential")
+
+
 def _os_Rz_to_uv(R, z, delta):
     """Prolate spheroidal (u, v) of focal length delta (galpy's
     coords.Rz_to_uv, as OblateStaeckelWrapperPotential.c:4-13 forms it)."""
@@ -3014,6 +3058,17 @@ def _pack(comps, planar, tfuncs=None, t=None):
             pt, pa, _ = p.pack()
             pot_type.extend(pt)
             pot_args.extend(pa)
+            continue
+        if isinstance(p, ChandrasekharDynamicalFrictionForce):
+            if planar:  # integratePlanarOrbit.py: no planar dynamical friction in C
+                raise NotImplementedError("ChandrasekharDynamicalFrictionForce has no planar version")
+            inner = flatten(p._dens_pot)
+            wt, wa = _pack(inner, planar, tfuncs, t)
+            pot_type.append(-7)
+            pot_type.extend(wt)
+            pot_args.append(float(_npot(inner)))
+            pot_args.extend(wa)
+            pot_args.extend(p.own_args())
             continue
         if isinstance(p, NonInertialFrameForce):
             code, pa, tf = p.pack(t)

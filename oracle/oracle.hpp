@@ -241,6 +241,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
                kOblateStaeckelWrap = -3, kNonInertial = 39, kNonInertialInterp = 45,
+               kChandrasekhar = -7,
                kTimeDepAmpWrap = -9, kMovingObjectWrap = -6, kInterpSph = 36, kSCF = 24,
                kDiskSCF = 26, kInterpRZ = 13, kMultipole = 44,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
@@ -485,7 +486,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
              : code == kMultipole ? multipole_nargs(pot_args)
              : code == kNonInertialInterp ? 0
              : code == kNonInertial ? 23
-             : code == kMovingObjectWrap ? 0  // counted after its wrapped components
+             : code == kMovingObjectWrap || code == kChandrasekhar ? 0  // counted after its wrapped components
                                    : component_nargs(code, planar);
     if (na < 0) return -1;
     Component& C = dst[i];
@@ -532,6 +533,19 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       C.args = a;
       C.p[0] = a[0];
       pot_args += 23;
+      continue;
+    }
+    if (code == kChandrasekhar) {  // initChandrasekharDynamicalFrictionSplines, integrateFullOrbit.c:1464-1491
+      const int n = (int)pot_args[0];
+      const double* ra = pot_args + 1;
+      const double* own = ra + 2 * n;  // amp, 8 cache slots, ms, rhm, gamma^2, lnLambda, minr^2, ro, rf
+      std::vector<double> rn(n);
+      for (int j = 0; j < n; j++) rn[j] = (ra[j] - own[14]) / (own[15] - own[14]);
+      C.spl = std::make_shared<std::vector<NatSpline>>(1);
+      (*C.spl)[0].build(rn.data(), ra + n, n);
+      C.args = own;
+      C.p[0] = own[0];
+      pot_args = own + 16;
       continue;
     }
     if (code == kMovingObjectWrap) {  // initMovingObjectSplines, integrateFullOrbit.c:1392-1431
@@ -2797,7 +2811,90 @@ inline void noninertial_xyz(const Component& C, double x, double y, double z, do
     Fz -= tdep(2);
   }
 }
-inline bool is_dissipative(int code) { return code == kNonInertial || code == kNonInertialInterp; }
+// The densities the friction force needs (the Dens functions of
+// HernquistPotential.c, NFWPotential.c, PlummerPotential.c,
+// MiyamotoNagaiPotential.c, PowerSphericalPotentialwCutoff.c,
+// LogarithmicHaloPotential.c, JaffePotential.c, IsochronePotential.c,
+// PowerSphericalPotential.c, BurkertPotential.c); calcDensity
+// (galpy_potentials.c:380-390) sums them.  Pinned by Poisson's equation
+// against each potential's forces.
+inline double comp_density(const Component& C, double R, double z, double phi) {
+  const double* p = C.p;
+  const double r2 = R * R + z * z, r = std::sqrt(r2);
+  switch (C.type) {
+    case kHernquist: return p[0] * M_1_PI / 4. / p[1] / p[1] / r * std::pow(1. + r / p[1], -3.);
+    case kNFW: return p[0] * M_1_PI / 4. / p[1] / p[1] / (1. + r / p[1]) / (1. + r / p[1]) / r;
+    case kPlummer: {
+      const double b2 = p[1] * p[1];
+      return 3. * p[0] * M_1_PI / 4. * b2 * std::pow(r2 + b2, -2.5);
+    }
+    case kMiyamotoNagai: {
+      const double a = p[1], b2 = p[2] * p[2], sqrtbz = std::sqrt(b2 + z * z);
+      double asqrtbz = a + sqrtbz;
+      asqrtbz *= asqrtbz;
+      if (a == 0.) return 3. * p[0] * M_1_PI / 4. * b2 * std::pow(R * R + sqrtbz * sqrtbz, -2.5);
+      return p[0] * M_1_PI / 4. * b2 * (a * R * R + (a + 3. * sqrtbz) * asqrtbz) * std::pow(R * R + asqrtbz, -2.5) *
+             std::pow(sqrtbz, -3.);
+    }
+    case kPowerSphCutoff: return p[0] * std::pow(r, -p[1]) * std::exp(-r2 / p[2] / p[2]);
+    case kLogHalo: {
+      const double amp = p[0], q = p[1], c = p[2], om = p[3], zq = z / q, q2 = q * q;
+      if (om < 1) {
+        const double R2 = R * R, Rt2 = R2 * (1. - om * std::pow(std::sin(phi), 2));
+        const double denom = 1. / (Rt2 + zq * zq + c), denom2 = denom * denom;
+        return amp * M_1_PI / 4. *
+               (2. * Rt2 / R2 * (denom - Rt2 * denom2) + denom / q2 - 2. * zq * zq * denom2 / q2 -
+                om * (2. * R2 * std::pow(std::sin(2. * phi), 2) / 4. * om * denom2 + denom * std::cos(2. * phi)));
+      }
+      return amp * M_1_PI / 4. / q2 * ((2. * q2 + 1.) * c + R * R + (2. - 1. / q2) * z * z) /
+             std::pow(R * R + zq * zq + c, 2.);
+    }
+    case kJaffe: return p[0] * M_1_PI / 4. / p[1] * std::pow(r * (1. + r / p[1]), -2.);
+    case kIsochrone: {
+      const double b = p[1], rb = std::sqrt(r2 + b * b), brbrb = (b + rb) * rb;
+      return p[0] * M_1_PI / 4. * (3. * brbrb * rb - r2 * (b + 3. * rb)) * std::pow(brbrb, -3.);
+    }
+    case kPowerSph: return p[0] * M_1_PI / 4. * (3. - p[1]) * std::pow(r2, -0.5 * p[1]);
+    case kBurkert: {
+      const double x = r / p[1];
+      return p[0] / (1. + x) / (1. + x * x);
+    }
+    default: return std::nan("");
+  }
+}
+inline bool is_dissipative(int code) {
+  return code == kNonInertial || code == kNonInertialInterp || code == kChandrasekhar;
+}
+// ChandrasekharDynamicalFrictionForce.c:13-140: F = A v with A = -amp
+// Xfactor(X) lnLambda rho / v^3, X = v / (sqrt 2 sigma_r(r)) (sigma_r a
+// natural spline over the normalised radius, clamped), Xfactor = erf X -
+// (2/sqrt pi) X e^-X^2, lnLambda constant or 0.5 ln(1 + r^2 / (gamma^2
+// max(rhm, GM/v^2)^2)); zero inside minr.  args: amp, 8 cache slots, ms, rhm,
+// gamma^2, lnLambda (< 0: variable), minr^2, ro, rf.
+inline double chandrasekhar_amplitude(const Component& C, double R, double z, double phi, double vR, double vT,
+                                      double vz) {
+  const double* a = C.args;
+  const double r2 = R * R + z * z;
+  if (r2 < a[13]) return 0.;
+  const double amp = a[0], ms = a[9], rhm = a[10], gamma2 = a[11], ro = a[14], rf = a[15];
+  double lnLambda = a[12];
+  const double r = std::sqrt(r2), v2 = vR * vR + vT * vT + vz * vz, v = std::sqrt(v2);
+  if (lnLambda < 0) {
+    const double GMvs = ms / v / v;
+    if (GMvs < rhm)
+      lnLambda = 0.5 * std::log(1. + r2 / gamma2 / rhm / rhm);
+    else
+      lnLambda = 0.5 * std::log(1. + r2 / gamma2 / GMvs / GMvs);
+  }
+  double d_ind = (r - ro) / (rf - ro);
+  d_ind = d_ind < 0 ? 0. : (d_ind > 1 ? 1. : d_ind);
+  const double sr = (*C.spl)[0].eval(d_ind);
+  const double X = M_SQRT1_2 * v / sr;
+  const double Xfactor = std::erf(X) - M_2_SQRTPI * X * std::exp(-X * X);
+  double dens = 0.;
+  for (int k = 0; k < C.nwrapped; k++) dens += comp_density(C.wrapped[k], R, z, phi);
+  return -amp * Xfactor * lnLambda / v2 / v * dens;
+}
 // calcRforce / calczforce / calcphitorque with velocities
 // (galpy_potentials.c, the RforceVelocity branch): the dissipative
 // components' cylindrical forces at (R, z, phi, t; vR, vT, vz)
@@ -2807,6 +2904,13 @@ inline void diss_forces(int npot, const Component* P, double Rc, double z, doubl
   for (int i = 0; i < npot; i++) {
     const Component& C = P[i];
     if (!is_dissipative(C.type)) continue;
+    if (C.type == kChandrasekhar) {  // Rforce, zforce, phitorque: A vR, A vz, A vT R
+      const double A = chandrasekhar_amplitude(C, Rc, z, phi, vR, vT, vz);
+      FR += A * vR;
+      Fz += A * vz;
+      Fp += A * vT * Rc;
+      continue;
+    }
     // cyl_to_rect, cyl_to_rect_vec (bovy_coords.c)
     const double cp = std::cos(phi), sp = std::sin(phi);
     const double x = Rc * cp, y = Rc * sp, vx = vR * cp - vT * sp, vy = vR * sp + vT * cp;

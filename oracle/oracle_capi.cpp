@@ -230,6 +230,29 @@ ORC_EXPORT int oracle_eval_forces(int npot, int* pot_type, double* pot_args, int
   return 0;
 }
 
+// Density (calcDensity, galpy_potentials.c:380-390) at n points.
+ORC_EXPORT int oracle_eval_density(int npot, int* pot_type, double* pot_args, int n, const double* R,
+                                   const double* z, const double* phi, double* out) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  for (int i = 0; i < n; i++) {
+    out[i] = 0.;
+    for (int k = 0; k < npot; k++) out[i] += comp_density(P[k], R[i], z[i], phi[i]);
+  }
+  return 0;
+}
+
+// The velocity-dependent forces (F_R, F_z, F_phi) at n phase-space points.
+ORC_EXPORT int oracle_eval_diss_forces(int npot, int* pot_type, double* pot_args, int n, const double* R,
+                                       const double* z, const double* phi, const double* t,
+                                       const double* vR, const double* vT, const double* vz, double* FR,
+                                       double* Fz, double* Fphi) {
+  Component P[kMaxPot];
+  if (npot > kMaxPot || parse_potential(npot, pot_type, pot_args, P) != 0) return -1;
+  for (int i = 0; i < n; i++) diss_forces(npot, P, R[i], z[i], phi[i], t[i], vR[i], vT[i], vz[i], FR[i], Fz[i], Fphi[i]);
+  return 0;
+}
+
 // Potential value at n points.
 ORC_EXPORT int oracle_eval_potential(int npot, int* pot_type, double* pot_args, int n,
                                      const double* R, const double* z, const double* phi,

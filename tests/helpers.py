@@ -76,6 +76,8 @@ def _nargs(code, pot_args, ai, planar):
         nblk = sum(2 * min(l + 1, M) - 1 for l in range(L))
         per = 12 * (Nr - 1) + Nr
         return 4 + Nr + 2 + Nt + nblk * (per if Nt == 0 else 4 * (Nt - 1) * per)
+    if code == -7:  # ChandrasekharDynamicalFrictionForce's own: nPts, r, sigma_r, 16
+        return 1 + 2 * int(pot_args[ai]) + 16
     if code == 45:  # NonInertialFrameForce with splines: n_spline, nPts, tgrid, arrays, 23 + 2
         return 2 + (1 + int(pot_args[ai])) * int(pot_args[ai + 1]) + 25
     if code == 26:  # DiskSCFPotential pair: nsigma + 3
@@ -175,6 +177,29 @@ def oracle_forces(pot_type, pot_args, R, z, phi=None, t=None):
     pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
     rc = lib.oracle_eval_forces(npot_of(pt, pa), _p(pt), _p(pa), n, _p(R), _p(z), _p(phi), _p(t), *[_p(o) for o in out])
     assert rc == 0
+    return tuple(out)
+
+
+def oracle_density(pot_type, pot_args, R, z, phi=None):
+    lib = oracle()
+    R = numpy.ascontiguousarray(R, dtype=float)
+    z = numpy.ascontiguousarray(numpy.broadcast_to(z, R.shape), dtype=float)
+    phi = numpy.ascontiguousarray(numpy.zeros(R.size) if phi is None else numpy.broadcast_to(phi, R.shape), dtype=float)
+    out = numpy.empty(R.size)
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    assert lib.oracle_eval_density(npot_of(pt, pa), _p(pt), _p(pa), R.size, _p(R), _p(z), _p(phi), _p(out)) == 0
+    return out
+
+
+def oracle_diss_forces(pot_type, pot_args, R, z, phi, t, vR, vT, vz):
+    lib = oracle()
+    arrs = [numpy.ascontiguousarray(numpy.broadcast_to(v, numpy.shape(R)), dtype=float) for v in (R, z, phi, t, vR, vT, vz)]
+    out = [numpy.empty(arrs[0].size) for _ in range(3)]
+    pt = numpy.ascontiguousarray(pot_type, dtype=numpy.int32)
+    pa = numpy.ascontiguousarray(pot_args, dtype=numpy.float64)
+    assert lib.oracle_eval_diss_forces(npot_of(pt, pa), _p(pt), _p(pa), arrs[0].size, *[_p(a) for a in arrs],
+                                       *[_p(o) for o in out]) == 0
     return tuple(out)
 
 

@@ -1,0 +1,58 @@
+"""ChandrasekharDynamicalFrictionForce (-7) on the GPU: a satellite-like
+set of orbits in MWPotential2014 with friction from MWPotential2014's own
+density (variable Coulomb logarithm, a half-mass radius, a radially varying
+sigma_r), and in the isothermal default, through RK4, RK6, DOPR54 and DOP853
+against the oracle; symplectic methods refused."""
+
+import numpy
+import pytest
+
+from helpers import assert_parity, ics, oracle_integrate, ulp_sensitivity
+
+pytestmark = pytest.mark.gpu
+
+
+def _pot(which):
+    from galpy_amd import potential as P
+
+    if which == "mw":
+        mw = P.MWPotential2014_factory()
+        return mw + [P.ChandrasekharDynamicalFrictionForce(GMs=0.01, rhm=0.02, dens=mw,
+                                                           sigmar=lambda r: 0.45 + 0.15 * numpy.exp(-r / 2.0))]
+    return [P.LogarithmicHaloPotential(normalize=1.0, q=1.0),
+            P.ChandrasekharDynamicalFrictionForce(GMs=0.005, const_lnLambda=3.0)]
+
+
+@pytest.mark.parametrize("which", ["mw", "isothermal"])
+@pytest.mark.parametrize("method,code", [("rk4_hip", 1), ("rk6_hip", 2), ("dopr54_hip", 5), ("dop853_hip", 6)])
+def test_full_orbits(which, method, code):
+    from galpy_amd import potential as P
+    from galpy_amd.integrate import integrateFullOrbit_hip
+
+    pot = _pot(which)
+    t = numpy.linspace(0.0, 10.0, 101)
+    _, pt, pa, _ = P.parse_pot(pot, t=t)
+    if which == "mw":
+        yo = ics(128, seed=71)
+    else:  # orbits that stay out of the isothermal cusp, where the friction (rho ~ 1/r^2) grows without bound
+        rng = numpy.random.default_rng(72)
+        yo = numpy.stack([rng.uniform(1.0, 2.0, 128), rng.normal(0.0, 0.1, 128), rng.uniform(0.8, 1.1, 128),
+                          rng.normal(0.0, 0.1, 128), rng.normal(0.0, 0.1, 128), rng.uniform(0.0, 6.28, 128)],
+                         axis=1)
+    res, err = integrateFullOrbit_hip(pot, yo.copy(), t, method)
+    assert numpy.all(err == 0)
+    ref, sens = ulp_sensitivity(lambda y: oracle_integrate(pt, pa, y, t, code)[0], yo,
+                                nrep=8 if code in (5, 6) else 3)
+    bar, factor = (1e-8, 10.0) if code in (5, 6) else (1e-10, 1e3)
+    assert_parity(res, ref, bar, sens, factor=factor, label=f"{which} {method}")
+
+
+def test_symplectic_refused_and_planar_unsupported():
+    from galpy_amd import potential as P
+    from galpy_amd.integrate import integrateFullOrbit_hip
+
+    pot = _pot("isothermal")
+    with pytest.raises(RuntimeError, match="velocity-dependent"):
+        integrateFullOrbit_hip(pot, ics(4, seed=1), numpy.linspace(0.0, 1.0, 11), "leapfrog_hip")
+    with pytest.raises(NotImplementedError):
+        P.parse_pot_planar(pot)
