@@ -996,7 +996,13 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
       set_error("ChandrasekharDynamicalFrictionForce has no planar version");
       return -1;
     }
-    const int na = (code == GH_MOVINGOBJWRAP || code == GH_CHANDRASEKHAR) ? 0 : nargs_of(code, planar);
+    if (code == GH_FDM && planar) {
+      set_error("FDMDynamicalFrictionForce has no planar version");
+      return -1;
+    }
+    const int na = (code == GH_MOVINGOBJWRAP || code == GH_CHANDRASEKHAR || code == GH_FDM)
+                       ? 0
+                       : nargs_of(code, planar);
     if (na < 0) {
       set_error("wrapper pot_type %d has no HIP kernel", code);
       return -1;
@@ -1019,6 +1025,28 @@ int decode_components(int n, const int*& pot_type, const double*& pot_args, GhPo
     GhWrap& W = T->wrap[w];
     W.kind = code;
     W.first = first;
+    if (code == GH_FDM) {
+      // FDMDynamicalFrictionForce (integrateFullOrbit.c:973-985, packing
+      // integrateFullOrbit.py:468-515): it wraps the Chandrasekhar force of
+      // the same satellite and scales its amplitude; its own arguments repeat
+      // the sigma_r samples and the 16, then m/hbar and the constant factor
+      // (< 0: none).  The wrapped Chandrasekhar entry takes the two numbers.
+      const double* a = pot_args;
+      const int n = (int)a[0];
+      const double* own = a + 1 + 2L * n;
+      GhWrap* C7 = nullptr;
+      for (int k = T->nwraps - 2; k >= 0 && !C7; k--)
+        if (T->wrap[k].kind == GH_CHANDRASEKHAR && T->wrap[k].first >= first) C7 = &T->wrap[k];
+      if (!C7 || nw != 1) {
+        set_error("FDMDynamicalFrictionForce must wrap one ChandrasekharDynamicalFrictionForce");
+        return -1;
+      }
+      C7->p[10] = own[16];
+      C7->p[11] = own[17];
+      C7->p[12] = 1.0;
+      pot_args = own + 18;
+      continue;
+    }
     if (code == GH_CHANDRASEKHAR) {
       // ChandrasekharDynamicalFrictionForce (integrateFullOrbit.c:917-927,
       // initChandrasekharDynamicalFrictionSplines :1464-1491): nPts, r, sigma_r

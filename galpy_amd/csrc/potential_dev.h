@@ -98,6 +98,7 @@ enum GhPotCode {
   GH_NONINERTIAL = 39,
   GH_NONINERTIALINTERP = 45,
   GH_CHANDRASEKHAR = -7,
+  GH_FDM = -11,
 };
 
 struct GhComponent {
@@ -2489,6 +2490,88 @@ __device__ __noinline__ double gh_chandrasekhar_amp(const GhPotTable* __restrict
   return -amp * Xf * lnL / v2 / v * dens;
 }
 
+// The cosine integral Ci(x), x > 0 (gsl_sf_Ci, which
+// FDMDynamicalFrictionForce.c calls): the power series gamma + ln x +
+// sum_k (-x^2)^k / (2k (2k)!) for x <= 4, else -Re E1(i x) from E1's
+// continued fraction (A&S 5.1.22) by the modified Lentz method.
+__device__ inline double gh_cos_integral(double x) {
+  const double euler = 0.57721566490153286061;
+  if (x <= 4.0) {
+    double term = 1.0, sum = 0.0;
+    for (int k = 1; k < 60; k++) {
+      term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+      const double add = term / (2.0 * k);
+      sum += add;
+      if (fabs(add) < 1e-17 * fabs(sum)) break;
+    }
+    return euler + log(x) + sum;
+  }
+  // complex (re, im) arithmetic; z = i x, b = z + 1, c = 1/tiny, d = h = 1/b
+  double br = 1.0, bi = x;
+  double cr = 1e300, ci = 0.0;
+  double inv = 1.0 / (br * br + bi * bi);
+  double dr = br * inv, di = -bi * inv, hr = dr, hi = di;
+  for (int n = 1; n < 1000; n++) {
+    const double an = -double(n) * n;
+    br += 2.0;
+    // d = 1 / (an d + b)
+    double tr = an * dr + br, ti = an * di + bi;
+    inv = 1.0 / (tr * tr + ti * ti);
+    dr = tr * inv;
+    di = -ti * inv;
+    // c = b + an / c
+    inv = 1.0 / (cr * cr + ci * ci);
+    tr = br + an * cr * inv;
+    ti = bi - an * ci * inv;
+    cr = tr;
+    ci = ti;
+    const double er = cr * dr - ci * di, ei = cr * di + ci * dr;  // del = c d
+    tr = hr * er - hi * ei;
+    hi = hr * ei + hi * er;
+    hr = tr;
+    if (fabs(er - 1.0) + fabs(ei) < 1e-16) break;
+  }
+  // -Re(h e^{-i x})
+  return -(hr * cos(x) + hi * sin(x));
+}
+// FDMDynamicalFrictionForce.c:14-80: FDMfactor / CDMfactor for the
+// Chandrasekhar entry W (p[10] = m/hbar, p[11] = constant factor or < 0).
+__device__ __noinline__ double gh_fdm_ratio(const GhPotTable* __restrict__ T,
+                                            const GhWrap* __restrict__ W, double r2, double v2) {
+  const double ms = W->p[1], rhm = W->p[2], gamma2 = W->p[3], ro = W->p[6], rf = W->p[7];
+  const double mhbar = W->p[10], cf = W->p[11];
+  double lnL = W->p[4];
+  const double r = sqrt(r2), v = sqrt(v2);
+  const double GMvs = ms / v / v;
+  if (lnL < 0.0)
+    lnL = (GMvs < rhm) ? 0.5 * log(1.0 + r2 / gamma2 / rhm / rhm)
+                       : 0.5 * log(1.0 + r2 / gamma2 / GMvs / GMvs);
+  double d = (r - ro) / (rf - ro);
+  d = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+  const double sr = gh_spline_eval(T, (int)W->p[8], d);
+  const double X = M_SQRT1_2 * v / sr;
+  const double Xf = erf(X) - M_2_SQRTPI * X * exp(-X * X);
+  const double CDM = lnL * Xf;
+  const double euler = 0.57721566490153286061;
+  double FDM;
+  if (cf < 0.0) {
+    const double kr = 2.0 * mhbar * v * r, Ms = v / sr;
+    if (kr < Ms) {
+      FDM = euler + log(kr) - gh_cos_integral(kr) + sin(kr) / kr - 1.0;
+    } else if (kr > 4 * Ms) {
+      FDM = log(kr / Ms) * Xf;
+    } else {
+      const double Cf = euler + log(Ms) - gh_cos_integral(Ms) + sin(Ms) / Ms - 1.0;
+      const double Cd = log(4.0) * Xf;
+      const double mu = (2 * Ms - kr / 2) / (2 * Ms - Ms / 2);
+      FDM = mu * Cf + (1 - mu) * Cd;
+    }
+  } else {
+    FDM = cf;
+  }
+  return FDM / CDM;
+}
+
 // The velocity-dependent components' Cartesian forces at (x, v, t)
 // (calcRforce / calczforce / calcphitorque with velocities,
 // galpy_potentials.c), added to a.
@@ -2508,7 +2591,12 @@ __device__ __noinline__ void gh_diss_rect(const GhPotTable* __restrict__ T, doub
   for (int w = 0; w < T->nwraps; w++) {
     const GhWrap* __restrict__ W = &T->wrap[w];
     if (W->kind != GH_CHANDRASEKHAR) continue;
-    const double A = gh_chandrasekhar_amp(T, W, x, y, z, vx * vx + vy * vy + vz * vz);
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    double A = gh_chandrasekhar_amp(T, W, x, y, z, v2);
+    if (W->p[12] != 0.0 && A != 0.0) {  // FDMDynamicalFrictionForceAmplitude (:82-99)
+      const double ratio = gh_fdm_ratio(T, W, x * x + y * y + z * z, v2);
+      if (ratio < 1.0 || W->p[11] >= 0.0) A *= ratio;
+    }
     ax += A * vx;
     ay += A * vy;
     az += A * vz;

@@ -2833,6 +2833,27 @@ class ChandrasekharDynamicalFrictionForce(Potential):
         raise NotImplementedError("dynamical friction has no potential")
 
 
+class FDMDynamicalFrictionForce(ChandrasekharDynamicalFrictionForce):
+    """Dynamical friction in fuzzy dark matter (FDMDynamicalFrictionForce.py
+    :8-260): the Chandrasekhar force scaled by FDMfactor / CDMfactor where
+    that is below one (always, for a constant const_FDMfactor), FDMfactor
+    from kr = 2 (m/hbar) v r and the Mach number v / sigma_r.  ``mhbar``: the
+    particle mass over hbar in the internal units (galpy's m converted).
+    Packed as -11 wrapping the Chandrasekhar force (-7) of the same
+    satellite, own arguments its sampled sigma_r, the 16, m/hbar and the
+    constant factor or -1 (integrateFullOrbit.py:468-515)."""
+
+    pot_code = -11
+
+    def __init__(self, amp=1.0, GMs=0.1, gamma=1.0, rhm=0.0, mhbar=1.0, dens=None, sigmar=None,
+                 const_lnLambda=False, const_FDMfactor=False, minr=0.0001, maxr=25.0, nr=501):
+        ChandrasekharDynamicalFrictionForce.__init__(self, amp=amp, GMs=GMs, gamma=gamma, rhm=rhm, dens=dens,
+                                                     sigmar=sigmar, const_lnLambda=const_lnLambda, minr=minr,
+                                                     maxr=maxr, nr=nr)
+        self._mhbar = float(mhbar)
+        self._const_FDMfactor = const_FDMfactor
+
+
 def _os_Rz_to_uv(R, z, delta):
     """Prolate spheroidal (u, v) of focal length delta (galpy's
     coords.Rz_to_uv, as OblateStaeckelWrapperPotential.c:4-13 forms it)."""
@@ -3064,11 +3085,17 @@ def _pack(comps, planar, tfuncs=None, t=None):
                 raise NotImplementedError("ChandrasekharDynamicalFrictionForce has no planar version")
             inner = flatten(p._dens_pot)
             wt, wa = _pack(inner, planar, tfuncs, t)
+            if isinstance(p, FDMDynamicalFrictionForce):  # -11 wrapping the -7 of the same satellite
+                pot_type.append(-11)
+                pot_args.append(1.0)
             pot_type.append(-7)
             pot_type.extend(wt)
             pot_args.append(float(_npot(inner)))
             pot_args.extend(wa)
             pot_args.extend(p.own_args())
+            if isinstance(p, FDMDynamicalFrictionForce):
+                pot_args.extend(p.own_args())
+                pot_args.extend([p._mhbar, -1.0 if not p._const_FDMfactor else p._const_FDMfactor])
             continue
         if isinstance(p, NonInertialFrameForce):
             code, pa, tf = p.pack(t)

@@ -22,6 +22,7 @@
 // written from the mathematics) and is pinned against mpmath.
 #pragma once
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -241,7 +242,7 @@ enum PotCode { kLogHalo = 0, kDehnenBar = 1, kMiyamotoNagai = 5, kPowerSph = 7, 
                kDehnenSmoothWrap = -1, kSolidBodyRotWrap = -2, kCorotatingRotWrap = -4,
                kGaussianAmpWrap = -5, kRotateTiltWrap = -8, kKuzminLikeWrap = -10, kCylSepWrap = -12,
                kOblateStaeckelWrap = -3, kNonInertial = 39, kNonInertialInterp = 45,
-               kChandrasekhar = -7,
+               kChandrasekhar = -7, kFDM = -11,
                kTimeDepAmpWrap = -9, kMovingObjectWrap = -6, kInterpSph = 36, kSCF = 24,
                kDiskSCF = 26, kInterpRZ = 13, kMultipole = 44,
                // EllipsoidalPotential family (integrateFullOrbit.c:420-483, 549, 647, 668, 764)
@@ -486,7 +487,7 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
              : code == kMultipole ? multipole_nargs(pot_args)
              : code == kNonInertialInterp ? 0
              : code == kNonInertial ? 23
-             : code == kMovingObjectWrap || code == kChandrasekhar ? 0  // counted after its wrapped components
+             : code == kMovingObjectWrap || code == kChandrasekhar || code == kFDM ? 0  // counted after its wrapped components
                                    : component_nargs(code, planar);
     if (na < 0) return -1;
     Component& C = dst[i];
@@ -533,6 +534,13 @@ inline int parse_components(int n, const int*& pot_type, const double*& pot_args
       C.args = a;
       C.p[0] = a[0];
       pot_args += 23;
+      continue;
+    }
+    if (code == kFDM) {  // its own nPts, r, sigma_r, then 18: Chandrasekhar's 16, m/hbar, const factor
+      const int n = (int)pot_args[0];
+      C.args = pot_args + 1 + 2 * n;
+      C.p[0] = C.args[0];
+      pot_args = C.args + 18;
       continue;
     }
     if (code == kChandrasekhar) {  // initChandrasekharDynamicalFrictionSplines, integrateFullOrbit.c:1464-1491
@@ -2863,7 +2871,81 @@ inline double comp_density(const Component& C, double R, double z, double phi) {
   }
 }
 inline bool is_dissipative(int code) {
-  return code == kNonInertial || code == kNonInertialInterp || code == kChandrasekhar;
+  return code == kNonInertial || code == kNonInertialInterp || code == kChandrasekhar || code == kFDM;
+}
+// The cosine integral Ci(x), x > 0 (gsl_sf_Ci, which FDMDynamicalFrictionForce.c
+// calls): the power series gamma + ln x + sum_k (-x^2)^k / (2k (2k)!) for
+// x <= 4, else Ci(x) = -Re E1(i x) with E1's continued fraction (A&S 5.1.22)
+// by the modified Lentz method.  Pinned against scipy's sici.
+inline double cos_integral(double x) {
+  const double euler = 0.57721566490153286061;
+  if (x <= 4.0) {
+    double term = 1.0, sum = 0.0;
+    for (int k = 1; k < 60; k++) {
+      term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+      const double add = term / (2.0 * k);
+      sum += add;
+      if (std::fabs(add) < 1e-17 * std::fabs(sum)) break;
+    }
+    return euler + std::log(x) + sum;
+  }
+  // E1(z) = e^-z / (z + 1 - 1/(z + 3 - 4/(z + 5 - ...))), z = i x
+  const std::complex<double> z(0.0, x);
+  const double tiny = 1e-300;
+  std::complex<double> b = z + 1.0, c = 1.0 / tiny, d = 1.0 / b, h = d;
+  for (int n = 1; n < 1000; n++) {
+    const double an = -double(n) * n;
+    b += 2.0;
+    d = 1.0 / (an * d + b);
+    c = b + an / c;
+    const std::complex<double> del = c * d;
+    h *= del;
+    if (std::abs(del - 1.0) < 1e-16) break;
+  }
+  return -(h * std::exp(-z)).real();
+}
+// FDMDynamicalFrictionForce.c:14-99: the Chandrasekhar amplitude times
+// FDMfactor / CDMfactor (when below 1, or always for a constant factor):
+// CDMfactor = lnLambda Xfactor; FDMfactor the zero-dispersion form gamma +
+// ln kr - Ci(kr) + sin(kr)/kr - 1 for kr = 2 (m/hbar) v r < M = v/sigma_r,
+// ln(kr/M) Xfactor past 4M, a linear blend between.  args as
+// Chandrasekhar's, then m/hbar, the constant factor (< 0: none).
+inline double fdm_ratio(const Component& C, double R, double z, double vR, double vT, double vz) {
+  const double* a = C.args;
+  const double ms = a[9], rhm = a[10], gamma2 = a[11], ro = a[14], rf = a[15], mhbar = a[16], cf = a[17];
+  double lnLambda = a[12];
+  const double r2 = R * R + z * z, r = std::sqrt(r2), v2 = vR * vR + vT * vT + vz * vz, v = std::sqrt(v2);
+  const double GMvs = ms / v / v;
+  if (lnLambda < 0) {
+    if (GMvs < rhm)
+      lnLambda = 0.5 * std::log(1. + r2 / gamma2 / rhm / rhm);
+    else
+      lnLambda = 0.5 * std::log(1. + r2 / gamma2 / GMvs / GMvs);
+  }
+  double d_ind = (r - ro) / (rf - ro);
+  d_ind = d_ind < 0 ? 0. : (d_ind > 1 ? 1. : d_ind);
+  const double sr = (*C.wrapped[0].spl)[0].eval(d_ind);
+  const double X = M_SQRT1_2 * v / sr;
+  const double Xfactor = std::erf(X) - M_2_SQRTPI * X * std::exp(-X * X);
+  const double CDMfactor = lnLambda * Xfactor;
+  const double euler = 0.57721566490153286061;
+  double FDMfactor;
+  if (cf < 0) {
+    const double kr = 2.0 * mhbar * v * r, Ms = v / sr;
+    if (kr < Ms) {
+      FDMfactor = euler + std::log(kr) - cos_integral(kr) + std::sin(kr) / kr - 1.0;
+    } else if (kr > 4 * Ms) {
+      FDMfactor = std::log(kr / Ms) * Xfactor;
+    } else {
+      const double C_fdm = euler + std::log(Ms) - cos_integral(Ms) + std::sin(Ms) / Ms - 1.0;
+      const double C_disp = std::log(4.0) * Xfactor;
+      const double mu = (2 * Ms - kr / 2) / (2 * Ms - Ms / 2);
+      FDMfactor = mu * C_fdm + (1 - mu) * C_disp;
+    }
+  } else {
+    FDMfactor = cf;
+  }
+  return FDMfactor / CDMfactor;
 }
 // ChandrasekharDynamicalFrictionForce.c:13-140: F = A v with A = -amp
 // Xfactor(X) lnLambda rho / v^3, X = v / (sqrt 2 sigma_r(r)) (sigma_r a
@@ -2904,8 +2986,16 @@ inline void diss_forces(int npot, const Component* P, double Rc, double z, doubl
   for (int i = 0; i < npot; i++) {
     const Component& C = P[i];
     if (!is_dissipative(C.type)) continue;
-    if (C.type == kChandrasekhar) {  // Rforce, zforce, phitorque: A vR, A vz, A vT R
-      const double A = chandrasekhar_amplitude(C, Rc, z, phi, vR, vT, vz);
+    if (C.type == kChandrasekhar || C.type == kFDM) {  // Rforce, zforce, phitorque: A vR, A vz, A vT R
+      double A;
+      if (C.type == kFDM) {  // FDMDynamicalFrictionForceAmplitude (:82-99)
+        if (Rc * Rc + z * z < C.args[13]) continue;
+        A = chandrasekhar_amplitude(C.wrapped[0], Rc, z, phi, vR, vT, vz);
+        const double ratio = fdm_ratio(C, Rc, z, vR, vT, vz);
+        if (ratio < 1. || C.args[17] >= 0) A *= ratio;
+      } else {
+        A = chandrasekhar_amplitude(C, Rc, z, phi, vR, vT, vz);
+      }
       FR += A * vR;
       Fz += A * vz;
       Fp += A * vT * Rc;

@@ -253,6 +253,11 @@ ORC_EXPORT int oracle_eval_diss_forces(int npot, int* pot_type, double* pot_args
   return 0;
 }
 
+// The cosine integral Ci at n points (x > 0).
+ORC_EXPORT void oracle_cos_integral(int n, const double* x, double* out) {
+  for (int i = 0; i < n; i++) out[i] = cos_integral(x[i]);
+}
+
 // Potential value at n points.
 ORC_EXPORT int oracle_eval_potential(int npot, int* pot_type, double* pot_args, int n,
                                      const double* R, const double* z, const double* phi,

@@ -76,6 +76,8 @@ def _nargs(code, pot_args, ai, planar):
         nblk = sum(2 * min(l + 1, M) - 1 for l in range(L))
         per = 12 * (Nr - 1) + Nr
         return 4 + Nr + 2 + Nt + nblk * (per if Nt == 0 else 4 * (Nt - 1) * per)
+    if code == -11:  # FDMDynamicalFrictionForce's own: nPts, r, sigma_r, 16, m/hbar, factor
+        return 1 + 2 * int(pot_args[ai]) + 18
     if code == -7:  # ChandrasekharDynamicalFrictionForce's own: nPts, r, sigma_r, 16
         return 1 + 2 * int(pot_args[ai]) + 16
     if code == 45:  # NonInertialFrameForce with splines: n_spline, nPts, tgrid, arrays, 23 + 2

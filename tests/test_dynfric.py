@@ -86,3 +86,46 @@ def test_circular_orbit_decay_in_the_isothermal_sphere():
     # small epicyclic wobble as friction slows it)
     slope = numpy.polyfit(t, r2, 1)[0]
     assert abs(slope / (-2 * GMs * lnL * Xf1) - 1.0) < 0.05
+
+
+def test_cos_integral_and_fdm_factor():
+    """The oracle's Ci against scipy's sici; the FDM force is the
+    Chandrasekhar force times FDMfactor/CDMfactor (below one), written out
+    with scipy in its three regimes."""
+    import ctypes
+
+    from helpers import _p, oracle
+
+    from galpy_amd import potential as P
+
+    x = numpy.concatenate([numpy.geomspace(1e-6, 4.0, 100), numpy.geomspace(4.0, 1e4, 100)])
+    ci = numpy.empty_like(x)
+    oracle().oracle_cos_integral(x.size, _p(x), _p(ci))
+    assert numpy.abs(ci - special.sici(x)[1]).max() < 2e-15
+    sig = lambda r: 0.5 + 0.0 * r
+    kw = dict(GMs=0.02, rhm=0.05, sigmar=sig)
+    ch = P.ChandrasekharDynamicalFrictionForce(**kw)
+    rng = numpy.random.default_rng(73)
+    n = 60
+    R, z, phi = rng.uniform(0.3, 3.0, n), rng.uniform(-1.0, 1.0, n), rng.uniform(0, 6.28, n)
+    vR, vT, vz = rng.normal(0, 0.3, n), rng.uniform(0.5, 1.2, n), rng.normal(0, 0.3, n)
+    _, cpt, cpa, _ = P.parse_pot([ch])
+    cFR = oracle_diss_forces(cpt, cpa, R, z, phi, 0.0, vR, vT, vz)[0]
+    for mh in (0.2, 2.0, 40.0):  # zero-dispersion, blended and dispersion regimes
+        f = P.FDMDynamicalFrictionForce(mhbar=mh, **kw)
+        npot, pt, pa, _ = P.parse_pot([f])
+        assert list(pt[:2]) == [-11, -7] and npot == 1
+        FR = oracle_diss_forces(pt, pa, R, z, phi, 0.0, vR, vT, vz)[0]
+        r, v = numpy.hypot(R, z), numpy.sqrt(vR**2 + vT**2 + vz**2)
+        D = numpy.where(0.02 / v**2 < 0.05, 0.05, 0.02 / v**2)
+        lnL = 0.5 * numpy.log(1 + r**2 / D**2)
+        X = v / numpy.sqrt(2) / 0.5
+        Xf = special.erf(X) - 2 / numpy.sqrt(numpy.pi) * X * numpy.exp(-X * X)
+        kr, M = 2 * mh * v * r, v / 0.5
+        zero = lambda k: numpy.euler_gamma + numpy.log(k) - special.sici(k)[1] + numpy.sin(k) / k - 1
+        mu = (2 * M - kr / 2) / (2 * M - M / 2)
+        fdm = numpy.where(kr < M, zero(kr), numpy.where(kr > 4 * M, numpy.log(kr / M) * Xf,
+                                                         mu * zero(M) + (1 - mu) * numpy.log(4.0) * Xf))
+        ratio = fdm / (lnL * Xf)
+        want = numpy.where(ratio < 1, cFR * ratio, cFR)
+        assert numpy.allclose(FR, want, rtol=1e-10, atol=1e-15)

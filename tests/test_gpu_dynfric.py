@@ -1,4 +1,4 @@
-"""ChandrasekharDynamicalFrictionForce (-7) on the GPU: a satellite-like
+"""ChandrasekharDynamicalFrictionForce (-7) and FDMDynamicalFrictionForce (-11) on the GPU: a satellite-like
 set of orbits in MWPotential2014 with friction from MWPotential2014's own
 density (variable Coulomb logarithm, a half-mass radius, a radially varying
 sigma_r), and in the isothermal default, through RK4, RK6, DOPR54 and DOP853
@@ -15,6 +15,10 @@ pytestmark = pytest.mark.gpu
 def _pot(which):
     from galpy_amd import potential as P
 
+    if which == "fdm":  # fuzzy dark matter: kr spans the zero-dispersion, blended and dispersion regimes
+        mw = P.MWPotential2014_factory()
+        return mw + [P.FDMDynamicalFrictionForce(GMs=0.01, rhm=0.02, dens=mw, mhbar=3.0,
+                                                 sigmar=lambda r: 0.45 + 0.15 * numpy.exp(-r / 2.0))]
     if which == "mw":
         mw = P.MWPotential2014_factory()
         return mw + [P.ChandrasekharDynamicalFrictionForce(GMs=0.01, rhm=0.02, dens=mw,
@@ -23,7 +27,7 @@ def _pot(which):
             P.ChandrasekharDynamicalFrictionForce(GMs=0.005, const_lnLambda=3.0)]
 
 
-@pytest.mark.parametrize("which", ["mw", "isothermal"])
+@pytest.mark.parametrize("which", ["mw", "isothermal", "fdm"])
 @pytest.mark.parametrize("method,code", [("rk4_hip", 1), ("rk6_hip", 2), ("dopr54_hip", 5), ("dop853_hip", 6)])
 def test_full_orbits(which, method, code):
     from galpy_amd import potential as P
@@ -32,7 +36,7 @@ def test_full_orbits(which, method, code):
     pot = _pot(which)
     t = numpy.linspace(0.0, 10.0, 101)
     _, pt, pa, _ = P.parse_pot(pot, t=t)
-    if which == "mw":
+    if which in ("mw", "fdm"):
         yo = ics(128, seed=71)
     else:  # orbits that stay out of the isothermal cusp, where the friction (rho ~ 1/r^2) grows without bound
         rng = numpy.random.default_rng(72)
