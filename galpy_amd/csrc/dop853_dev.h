@@ -190,10 +190,18 @@ template <class V, class... Rest>
 __device__ __forceinline__ V gh_pstage(double h, V p0, double a, V x, Rest... rest) {
   return gh_drift(p0, gh_lin(gh_drift(V{}, x, a), rest...), h);
 }
-// a stage's acceleration, with its velocity when the forces need it
-template <class F, class V, class PV>
-__device__ __forceinline__ V gh_stage_acc(const F& T, double t, V q, const PV& pv) {
-  return T.diss() ? gh_accel_v(T, t, q, pv()) : gh_accel(T, t, q);
+// a stage's acceleration, with its velocity p0 + h sum a_j A_j when the
+// forces need it (the pairs by value: nothing of the stages is addressed,
+// and without velocity-dependent forces the sum is dead code)
+template <class F, class V, class... Rest>
+__device__ __forceinline__ V gh_stage_acc(const F& T, double t, V q, double h, V p0, Rest... rest) {
+  if (T.diss()) return gh_accel_v(T, t, q, gh_pstage(h, p0, rest...));
+  return gh_accel(T, t, q);
+}
+template <class F, class V>
+__device__ __forceinline__ V gh_stage_acc_pn(const F& T, double t, V q, V pn) {
+  if (T.diss()) return gh_accel_v(T, t, q, pn);
+  return gh_accel(T, t, q);
 }
 
 // What the dense output of one accepted step needs (leung_dop853.c:430-474):
@@ -224,18 +232,9 @@ __device__ __forceinline__ int gh_dop_dense(const F& T, const double* __restrict
   for (int j = 4; j <= 13; j++) A[j] = J.A[j - 3];
   A[2] = A[3] = V{};  // unused by stages 14-16 and the extension
   const V p0 = J.p0;
-  A[14] = gh_stage_acc(T, J.t_old + c14 * h, gh_rkn_q14(h, J.q0, J.p0, J.pn, A), [&] {
-    return gh_pstage(h, p0, a141, A[1], a147, A[7], a148, A[8], a149, A[9], a1410, A[10], a1411,
-                     A[11], a1412, A[12], a1413, A[13]);
-  });
-  A[15] = gh_stage_acc(T, J.t_old + c15 * h, gh_rkn_q15(h, J.q0, J.p0, J.pn, A), [&] {
-    return gh_pstage(h, p0, a151, A[1], a156, A[6], a157, A[7], a158, A[8], a1511, A[11], a1512,
-                     A[12], a1513, A[13], a1514, A[14]);
-  });
-  A[16] = gh_stage_acc(T, J.t_old + c16 * h, gh_rkn_q16(h, J.q0, J.p0, J.pn, A), [&] {
-    return gh_pstage(h, p0, a161, A[1], a166, A[6], a167, A[7], a168, A[8], a169, A[9], a1613,
-                     A[13], a1614, A[14], a1615, A[15]);
-  });
+  A[14] = gh_stage_acc(T, J.t_old + c14 * h, gh_rkn_q14(h, J.q0, J.p0, J.pn, A), h, p0, a141, A[1], a147, A[7], a148, A[8], a149, A[9], a1410, A[10], a1411, A[11], a1412, A[12], a1413, A[13]);
+  A[15] = gh_stage_acc(T, J.t_old + c15 * h, gh_rkn_q15(h, J.q0, J.p0, J.pn, A), h, p0, a151, A[1], a156, A[6], a157, A[7], a158, A[8], a1511, A[11], a1512, A[12], a1513, A[13], a1514, A[14]);
+  A[16] = gh_stage_acc(T, J.t_old + c16 * h, gh_rkn_q16(h, J.q0, J.p0, J.pn, A), h, p0, a161, A[1], a166, A[6], a167, A[7], a168, A[8], a169, A[9], a1613, A[13], a1614, A[14], a1615, A[15]);
   // the continuous extension is re-formed per requested time (a step holds
   // one output time in the common case), so its 48 coefficients are never
   // live at once
@@ -290,40 +289,21 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
   // with beta = 0 pow(facold, beta) is exactly 1 and never enters a result.)
   V A[17];
   A[1] = GhPhase<V>::p(s.k1);
-  A[2] = gh_stage_acc(T, tc + c2 * h, gh_rkn_q2(h, q0, p0, none, A),
-                      [&] { return gh_pstage(h, p0, a21, A[1]); });
-  A[3] = gh_stage_acc(T, tc + c3 * h, gh_rkn_q3(h, q0, p0, none, A),
-                      [&] { return gh_pstage(h, p0, a31, A[1], a32, A[2]); });
-  A[4] = gh_stage_acc(T, tc + c4 * h, gh_rkn_q4(h, q0, p0, none, A),
-                      [&] { return gh_pstage(h, p0, a41, A[1], a43, A[3]); });
-  A[5] = gh_stage_acc(T, tc + c5 * h, gh_rkn_q5(h, q0, p0, none, A),
-                      [&] { return gh_pstage(h, p0, a51, A[1], a53, A[3], a54, A[4]); });
-  A[6] = gh_stage_acc(T, tc + c6 * h, gh_rkn_q6(h, q0, p0, none, A),
-                      [&] { return gh_pstage(h, p0, a61, A[1], a64, A[4], a65, A[5]); });
-  A[7] = gh_stage_acc(T, tc + c7 * h, gh_rkn_q7(h, q0, p0, none, A),
-                      [&] { return gh_pstage(h, p0, a71, A[1], a74, A[4], a75, A[5], a76, A[6]); });
-  A[8] = gh_stage_acc(T, tc + c8 * h, gh_rkn_q8(h, q0, p0, none, A), [&] {
-    return gh_pstage(h, p0, a81, A[1], a84, A[4], a85, A[5], a86, A[6], a87, A[7]);
-  });
-  A[9] = gh_stage_acc(T, tc + c9 * h, gh_rkn_q9(h, q0, p0, none, A), [&] {
-    return gh_pstage(h, p0, a91, A[1], a94, A[4], a95, A[5], a96, A[6], a97, A[7], a98, A[8]);
-  });
-  A[10] = gh_stage_acc(T, tc + c10 * h, gh_rkn_q10(h, q0, p0, none, A), [&] {
-    return gh_pstage(h, p0, a101, A[1], a104, A[4], a105, A[5], a106, A[6], a107, A[7], a108, A[8],
-                     a109, A[9]);
-  });
-  A[11] = gh_stage_acc(T, tc + c11 * h, gh_rkn_q11(h, q0, p0, none, A), [&] {
-    return gh_pstage(h, p0, a111, A[1], a114, A[4], a115, A[5], a116, A[6], a117, A[7], a118, A[8],
-                     a119, A[9], a1110, A[10]);
-  });
+  A[2] = gh_stage_acc(T, tc + c2 * h, gh_rkn_q2(h, q0, p0, none, A), h, p0, a21, A[1]);
+  A[3] = gh_stage_acc(T, tc + c3 * h, gh_rkn_q3(h, q0, p0, none, A), h, p0, a31, A[1], a32, A[2]);
+  A[4] = gh_stage_acc(T, tc + c4 * h, gh_rkn_q4(h, q0, p0, none, A), h, p0, a41, A[1], a43, A[3]);
+  A[5] = gh_stage_acc(T, tc + c5 * h, gh_rkn_q5(h, q0, p0, none, A), h, p0, a51, A[1], a53, A[3], a54, A[4]);
+  A[6] = gh_stage_acc(T, tc + c6 * h, gh_rkn_q6(h, q0, p0, none, A), h, p0, a61, A[1], a64, A[4], a65, A[5]);
+  A[7] = gh_stage_acc(T, tc + c7 * h, gh_rkn_q7(h, q0, p0, none, A), h, p0, a71, A[1], a74, A[4], a75, A[5], a76, A[6]);
+  A[8] = gh_stage_acc(T, tc + c8 * h, gh_rkn_q8(h, q0, p0, none, A), h, p0, a81, A[1], a84, A[4], a85, A[5], a86, A[6], a87, A[7]);
+  A[9] = gh_stage_acc(T, tc + c9 * h, gh_rkn_q9(h, q0, p0, none, A), h, p0, a91, A[1], a94, A[4], a95, A[5], a96, A[6], a97, A[7], a98, A[8]);
+  A[10] = gh_stage_acc(T, tc + c10 * h, gh_rkn_q10(h, q0, p0, none, A), h, p0, a101, A[1], a104, A[4], a105, A[5], a106, A[6], a107, A[7], a108, A[8], a109, A[9]);
+  A[11] = gh_stage_acc(T, tc + c11 * h, gh_rkn_q11(h, q0, p0, none, A), h, p0, a111, A[1], a114, A[4], a115, A[5], a116, A[6], a117, A[7], a118, A[8], a119, A[9], a1110, A[10]);
   const V q12 = gh_rkn_q12(h, q0, p0, none, A);
   s.t_old_older = s.t_old;
   s.t_old = tc;
   s.t_cur = tc + h;
-  A[12] = gh_stage_acc(T, s.t_cur, q12, [&] {
-    return gh_pstage(h, p0, a121, A[1], a124, A[4], a125, A[5], a126, A[6], a127, A[7], a128, A[8],
-                     a129, A[9], a1210, A[10], a1211, A[11]);
-  });
+  A[12] = gh_stage_acc(T, s.t_cur, q12, h, p0, a121, A[1], a124, A[4], a125, A[5], a126, A[6], a127, A[7], a128, A[8], a129, A[9], a1210, A[10], a1211, A[11]);
   V qn, pn;
   double err5, err3;
   gh_rkn_solution(h, q0, p0, A, atol, rtol, qn, pn, err5, err3);
@@ -336,7 +316,7 @@ __device__ __forceinline__ bool gh_dop_step(const F& T,
   double hnew = h / fac;
   if (errv <= 1.0) {
     s.nacc++;
-    A[13] = gh_stage_acc(T, s.t_cur, qn, [&] { return pn; });  // FSAL: k13 = f(y_new)
+    A[13] = gh_stage_acc_pn(T, s.t_cur, qn, pn);  // FSAL: k13 = f(y_new)
     // dense output only when a requested time falls inside this step; the
     // three extra stages feed nothing else, so they are skipped otherwise --
     // outputs are unchanged

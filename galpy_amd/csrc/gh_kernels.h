@@ -566,11 +566,7 @@ __device__ __forceinline__ void gh_job_run(const F& T, const double* __restrict_
     A14 = gh_stage_acc(T, t_old + c14 * h,
                        gh_rkn_q14(h, gh_job_vec<V>(qd, slot, 0), gh_job_vec<V>(qd, slot, D),
                                   gh_job_vec<V>(qd, slot, 3 * D), A),
-                       [&] {
-                         return gh_pstage(h, gh_job_vec<V>(qd, slot, D), a141, A[1], a147, A[7],
-                                          a148, A[8], a149, A[9], a1410, A[10], a1411, A[11],
-                                          a1412, A[12], a1413, A[13]);
-                       });
+                       h, gh_job_vec<V>(qd, slot, D), a141, A[1], a147, A[7], a148, A[8], a149, A[9], a1410, A[10], a1411, A[11], a1412, A[12], a1413, A[13]);
   }
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   {
@@ -580,11 +576,7 @@ __device__ __forceinline__ void gh_job_run(const F& T, const double* __restrict_
     A15 = gh_stage_acc(T, t_old + c15 * h,
                        gh_rkn_q15(h, gh_job_vec<V>(qd, slot, 0), gh_job_vec<V>(qd, slot, D),
                                   gh_job_vec<V>(qd, slot, 3 * D), A),
-                       [&] {
-                         return gh_pstage(h, gh_job_vec<V>(qd, slot, D), a151, A[1], a156, A[6],
-                                          a157, A[7], a158, A[8], a1511, A[11], a1512, A[12],
-                                          a1513, A[13], a1514, A[14]);
-                       });
+                       h, gh_job_vec<V>(qd, slot, D), a151, A[1], a156, A[6], a157, A[7], a158, A[8], a1511, A[11], a1512, A[12], a1513, A[13], a1514, A[14]);
   }
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   {
@@ -595,11 +587,7 @@ __device__ __forceinline__ void gh_job_run(const F& T, const double* __restrict_
     A16 = gh_stage_acc(T, t_old + c16 * h,
                        gh_rkn_q16(h, gh_job_vec<V>(qd, slot, 0), gh_job_vec<V>(qd, slot, D),
                                   gh_job_vec<V>(qd, slot, 3 * D), A),
-                       [&] {
-                         return gh_pstage(h, gh_job_vec<V>(qd, slot, D), a161, A[1], a166, A[6],
-                                          a167, A[7], a168, A[8], a169, A[9], a1613, A[13],
-                                          a1614, A[14], a1615, A[15]);
-                       });
+                       h, gh_job_vec<V>(qd, slot, D), a161, A[1], a166, A[6], a167, A[7], a168, A[8], a169, A[9], a1613, A[13], a1614, A[14], a1615, A[15]);
   }
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   const int o = qi[kDenseCap + slot];
