@@ -582,8 +582,8 @@ def main():
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "algorithmic_bytes": float(n * nt * 8 * ncol + 2 * n * 8 * ncol),
             "kernel": {"dop853_hip": "gh_dop853_kernel", "rk4_hip": "gh_rk_kernel<4>",
-                       "rk6_hip": "gh_rk_kernel<6>", "dopr54_hip": "gh_dopr54_kernel",
-                       "ias15_hip": "gh_ias15_kernel"}.get(
+                       "rk6_hip": "gh_rk_kernel<6>", "dopr54_hip": "gh_dopr54_persistent",
+                       "ias15_hip": "gh_ias15_persistent"}.get(
                            w["method"], "gh_symplectic_kernel"),
             "note": ("FP64 vector ALU bound (no MFMA: nothing in the path is a contraction; "
                      "78.6 TFLOP/s is MI355X's FP64 peak). achieved = counted algorithmic FP64 ops "

@@ -270,7 +270,117 @@ gh_dopr54_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
   }
 }
 
-// wez_ias15 (wez_ias15.c:137-425), one orbit per lane.  The first step is
+// The same integration with persistent lanes: a lane that finishes its
+// orbit claims the next one from a counter (zeroed before the launch; every
+// lane claims its first orbit there too), so the lanes of a wavefront keep
+// stepping until the queue is empty instead of idling behind the slowest
+// orbit of the launch, and a workgroup that starts late finds the queue
+// empty and leaves at once.
+// Each pass of the loop is one pass of the reference's inner loop (at most
+// one step), one output row or one interval's set-up; per orbit the steps
+// and their order are those of gh_dopr54_kernel.  A lane leaves once no lane of its wavefront holds an orbit or
+// a claim below nobj, which every lane reaches: each orbit ends after at
+// most nt - 1 intervals of at most kDopr54MaxIter passes.
+template <class Spec, class V>
+__global__ void __launch_bounds__(kBlock)
+gh_dopr54_persistent(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
+                     int indiv_t, double dt_in, double rtol, double atol,
+                     const GhPotTable* __restrict__ T, double* __restrict__ result,
+                     int* __restrict__ err, double* __restrict__ stats, const volatile int* cancel,
+                     unsigned long long* __restrict__ counter) {
+  constexpr int N = GhPhase<V>::N;
+  const GhForce<Spec> F{T};
+  long next = (long)atomicAdd(counter, 1ULL);  // the orbit to start next
+  long orbit = 0;
+  double yn[N], a1[N];
+  const double* tt = t;
+  double* out = result;
+  double dt = 0.0, dt_one = 0.0, to = 0.0, init_dt_one = 0.0, init_to = 0.0;
+  long iter = 0, nacc = 0, nrej = 0;
+  int ii = 0, e = 0;
+  bool active = false, fresh = false;  // fresh: interval ii not yet set up
+  while (true) {
+    if (!active && next < nobj) {
+      orbit = next;
+      tt = t + (indiv_t ? orbit * nt : 0);
+      double v[N];
+#pragma unroll
+      for (int k = 0; k < N; k++) v[k] = yo[orbit * N + k];
+      GhPhase<V>::to_rect(v, yn);
+#pragma unroll
+      for (int k = 0; k < N; k++) yo[orbit * N + k] = yn[k];  // galpy converts yo in place
+      out = result + orbit * (long)nt * N;
+      GhPhase<V>::store(out, GhPhase<V>::q(yn), GhPhase<V>::p(yn));
+      dt = tt[1] - tt[0];
+      dt_one = dt_in;
+      if (dt_in == GH_DT_SENTINEL)
+        dt_one = gh_rk_estimate_step<4, GhForce<Spec>, V>(F, yn, dt, tt[0], rtol, atol);
+      to = tt[0];
+      gh_rk_deriv<GhForce<Spec>, V>(F, to, yn, a1);
+      e = 0;
+      nacc = nrej = 0;
+      ii = 0;
+      fresh = true;
+      active = true;
+    }
+    if (active) {
+      bool done = false;
+      if (fresh) {
+        if (ii >= nt - 1 || e == -3) {
+          done = true;
+        } else if (cancel != nullptr && (ii & 63) == 0 && *cancel) {
+          e = -10;
+          done = true;
+        } else {
+          init_dt_one = dt_one;
+          init_to = to;
+          iter = 0;
+          fresh = false;
+        }
+      } else {
+        bool end = !((dt >= 0.0 && to < init_to + dt) || (dt < 0.0 && to > init_to + dt));
+        if (!end && ++iter > kDopr54MaxIter) {
+          e = -3;
+          end = true;
+        }
+        if (end) {
+          GhPhase<V>::store(out + (long)N * (ii + 1), GhPhase<V>::q(yn), GhPhase<V>::p(yn));
+          ii++;
+          fresh = true;
+        } else {
+          bool accept = false;
+          if (init_dt_one / dt_one > 10000.0 || dt_one != dt_one) {
+            dt_one = init_dt_one / 10000.0;
+            accept = true;
+            if (e % 2 == 0) e += 1;
+          }
+          if (dt >= 0.0 && dt_one > init_to + dt - to) dt_one = init_to + dt - to;
+          if (dt < 0.0 && dt_one < init_to + dt - to) dt_one = init_to + dt - to;
+          bool accepted;
+          dt_one = gh_dopr54_actualstep<GhForce<Spec>, V>(F, yn, dt_one, to, rtol, atol, a1,
+                                                           accept, accepted);
+          if (accepted)
+            nacc++;
+          else
+            nrej++;
+        }
+      }
+      if (done) {
+        err[orbit] = e;
+        if (stats) {
+          stats[orbit * 3 + 0] = (double)nacc;
+          stats[orbit * 3 + 1] = (double)nrej;
+          stats[orbit * 3 + 2] = dt_one;
+        }
+        active = false;
+        next = (long)atomicAdd(counter, 1ULL);
+      }
+    }
+    if (!__any(active || next < nobj)) break;
+  }
+}
+
+// wez_ias15 (wez_ias15.c:137-425), one orbit at a time per lane.  The first step is
 // the grid's first spacing (or the caller's dt); the integration runs until
 // the accepted steps add up to |nt dt|, one step length past the last output
 // time as in the reference; outputs come from the accepted step's
@@ -283,100 +393,128 @@ gh_dopr54_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
 // keeps a lane busy.
 constexpr long kIas15MaxAttempts = 1L << 22;
 
+// The output row `row` of an IAS15 orbit from its position q and velocity p.
+template <class V, int D>
+__device__ __forceinline__ void gh_ias15_put(double* out, int row, const double* q,
+                                             const double* p) {
+  constexpr int N = GhPhase<V>::N;
+  double y[N];
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    y[i] = q[i];
+    y[D + i] = p[i];
+  }
+  GhPhase<V>::store(out + (long)N * row, GhPhase<V>::q(y), GhPhase<V>::p(y));
+}
+
+// The IAS15 integration on persistent lanes, as gh_dopr54_persistent: every
+// lane claims its orbits from a counter zeroed before the launch and makes
+// one step attempt (with the rows it passes) per pass of the loop.  Each
+// orbit ends after at most kIas15MaxAttempts attempts, so every lane leaves.
 template <class Spec, class V>
 __global__ void __launch_bounds__(kBlock)
-gh_ias15_kernel(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
-                int indiv_t, double dt_in, const GhPotTable* __restrict__ T,
-                double* __restrict__ result, int* __restrict__ err, double* __restrict__ stats,
-                const volatile int* cancel) {
-  const long orbit = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (orbit >= nobj) return;
+gh_ias15_persistent(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
+                    int indiv_t, double dt_in, const GhPotTable* __restrict__ T,
+                    double* __restrict__ result, int* __restrict__ err,
+                    double* __restrict__ stats, const volatile int* cancel,
+                    unsigned long long* __restrict__ counter) {
   constexpr int N = GhPhase<V>::N;
   constexpr int D = N / 2;
   const GhForce<Spec> F{T};
-  const double* tt = t + (indiv_t ? orbit * nt : 0);
-  double x[D], v[D];
-  {
-    double w[N], y[N];
-#pragma unroll
-    for (int k = 0; k < N; k++) w[k] = yo[orbit * N + k];
-    GhPhase<V>::to_rect(w, y);
-#pragma unroll
-    for (int k = 0; k < N; k++) yo[orbit * N + k] = y[k];  // galpy converts yo in place
-#pragma unroll
-    for (int i = 0; i < D; i++) {
-      x[i] = y[i];
-      v[i] = y[D + i];
-    }
-  }
-  double* out = result + orbit * (long)nt * N;
-  auto put = [&](int row, const double* q, const double* p) {
-    double y[N];
-#pragma unroll
-    for (int i = 0; i < D; i++) {
-      y[i] = q[i];
-      y[D + i] = p[i];
-    }
-    GhPhase<V>::store(out + (long)N * row, GhPhase<V>::q(y), GhPhase<V>::p(y));
-  };
-  put(0, x, v);
-  double B[7][D], E[7][D], G[7][D], F0[D];
-#pragma unroll
-  for (int k = 0; k < 7; k++)
-#pragma unroll
-    for (int i = 0; i < D; i++) B[k][i] = E[k][i] = 0.0;
-  const double init_dt = tt[1] - tt[0];
-  double dt = dt_in == GH_DT_SENTINEL ? init_dt : dt_in;
-  double to = tt[0];
-  double time_remaining = fabs((double)nt * dt);
+  long next = (long)atomicAdd(counter, 1ULL);  // the orbit to start next
+  long orbit = 0;
+  const double* tt = t;
+  double* out = result;
+  double x[D], v[D], B[7][D], E[7][D], G[7][D], F0[D];
+  double init_dt = 0.0, dt = 0.0, to = 0.0, time_remaining = 0.0;
   int row = 1, e = 0;
   long nacc = 0, nrej = 0;
-  while (time_remaining > 0.0) {
-    if (((nacc + nrej) & 63) == 0) {
-      if (cancel != nullptr && *cancel) {
-        e = -10;
-        break;
-      }
-      if (nacc + nrej >= kIas15MaxAttempts) {
-        e = -3;
-        break;
-      }
-    }
-    const double dt_temp = dt;
-    const double to_temp = to + dt_temp;
-    const double dt_required =
-        gh_ias15_attempt<GhForce<Spec>, V, D>(F, x, v, dt_temp, to_temp, F0, B, G);
-    if (fabs(dt_temp) > fabs(dt_required)) {
-      dt = dt_required;
-      nrej++;
-      continue;
-    }
-    nacc++;
-    time_remaining -= fabs(dt);
-    while (row < nt && (init_dt > 0.0 ? (to < tt[row] && tt[row] <= to_temp)
-                                      : (to > tt[row] && tt[row] >= to_temp))) {
-      const double hs = fabs(tt[row] - to) / fabs(to_temp - to);
-      double xs[D], vs[D];
-      gh_ias15_position<D>(xs, x, v, hs, dt_temp, F0, B);
-      gh_ias15_velocity<D>(vs, v, hs, dt_temp, F0, B);
-      put(row, xs, vs);
-      row++;
-    }
-    gh_ias15_advance<D>(x, v, dt_temp, F0, B, E);
-    to = to_temp;
-    dt = dt_required;
-  }
-  if (e == 0) {
-    double nan[D];
+  bool active = false;
+  while (true) {
+    if (!active && next < nobj) {
+      orbit = next;
+      tt = t + (indiv_t ? orbit * nt : 0);
+      {
+        double w[N], y[N];
 #pragma unroll
-    for (int i = 0; i < D; i++) nan[i] = __builtin_nan("");
-    for (; row < nt; row++) put(row, nan, nan);
-  }
-  err[orbit] = e;
-  if (stats) {
-    stats[orbit * 3 + 0] = (double)nacc;
-    stats[orbit * 3 + 1] = (double)nrej;
-    stats[orbit * 3 + 2] = dt;
+        for (int k = 0; k < N; k++) w[k] = yo[orbit * N + k];
+        GhPhase<V>::to_rect(w, y);
+#pragma unroll
+        for (int k = 0; k < N; k++) yo[orbit * N + k] = y[k];  // galpy converts yo in place
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+          x[i] = y[i];
+          v[i] = y[D + i];
+        }
+      }
+      out = result + orbit * (long)nt * N;
+      gh_ias15_put<V, D>(out, 0, x, v);
+#pragma unroll
+      for (int k = 0; k < 7; k++)
+#pragma unroll
+        for (int i = 0; i < D; i++) B[k][i] = E[k][i] = 0.0;
+      init_dt = tt[1] - tt[0];
+      dt = dt_in == GH_DT_SENTINEL ? init_dt : dt_in;
+      to = tt[0];
+      time_remaining = fabs((double)nt * dt);
+      row = 1;
+      e = 0;
+      nacc = nrej = 0;
+      active = true;
+    }
+    if (active) {
+      bool done = !(time_remaining > 0.0);
+      if (!done && ((nacc + nrej) & 63) == 0) {
+        if (cancel != nullptr && *cancel) {
+          e = -10;
+          done = true;
+        } else if (nacc + nrej >= kIas15MaxAttempts) {
+          e = -3;
+          done = true;
+        }
+      }
+      if (!done) {
+        const double dt_temp = dt;
+        const double to_temp = to + dt_temp;
+        const double dt_required =
+            gh_ias15_attempt<GhForce<Spec>, V, D>(F, x, v, dt_temp, to_temp, F0, B, G);
+        if (fabs(dt_temp) > fabs(dt_required)) {
+          dt = dt_required;
+          nrej++;
+        } else {
+          nacc++;
+          time_remaining -= fabs(dt);
+          while (row < nt && (init_dt > 0.0 ? (to < tt[row] && tt[row] <= to_temp)
+                                            : (to > tt[row] && tt[row] >= to_temp))) {
+            const double hs = fabs(tt[row] - to) / fabs(to_temp - to);
+            double xs[D], vs[D];
+            gh_ias15_position<D>(xs, x, v, hs, dt_temp, F0, B);
+            gh_ias15_velocity<D>(vs, v, hs, dt_temp, F0, B);
+            gh_ias15_put<V, D>(out, row, xs, vs);
+            row++;
+          }
+          gh_ias15_advance<D>(x, v, dt_temp, F0, B, E);
+          to = to_temp;
+          dt = dt_required;
+        }
+      } else {
+        if (e == 0) {
+          double nan[D];
+#pragma unroll
+          for (int i = 0; i < D; i++) nan[i] = __builtin_nan("");
+          for (; row < nt; row++) gh_ias15_put<V, D>(out, row, nan, nan);
+        }
+        err[orbit] = e;
+        if (stats) {
+          stats[orbit * 3 + 0] = (double)nacc;
+          stats[orbit * 3 + 1] = (double)nrej;
+          stats[orbit * 3 + 2] = dt;
+        }
+        active = false;
+        next = (long)atomicAdd(counter, 1ULL);
+      }
+    }
+    if (!__any(active || next < nobj)) break;
   }
 }
 
@@ -905,6 +1043,72 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
   return 0;
 }
 
+// DOPR54 (DOPR54 = true) or IAS15 on persistent lanes: twice as many
+// workgroups as the occupancy query says stay resident (or as the orbits
+// fill), the work counter zeroed first; the counter is the workspace's first
+// 256 B when one is given.  The query reports 8 workgroups of 64 per CU for
+// the 120-register MWPotential2014 DOPR54 instance, half of what gfx950's 512
+// registers per SIMD lane hold; the surplus costs nothing when the query is
+// right, because a workgroup that starts after the queue has drained leaves
+// at once.
+template <class Spec, class V, bool DOPR54>
+int launch_persistent(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
+                      const GhPotTable* d_table, double dt, double rtol, double atol,
+                      double* d_result, int* d_err, double* d_stats, const int* cancel, int dev,
+                      hipStream_t stream, const GhWorkspace* ws) {
+  const char* what = DOPR54 ? "DOPR54" : "IAS15";
+  static int cached[64] = {0};
+  int resident = dev >= 0 && dev < 64 ? cached[dev] : 0;
+  if (resident <= 0) {
+    int per_cu = 0, ncu = 0;
+    hipError_t q;
+    if constexpr (DOPR54)
+      q = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dopr54_persistent<Spec, V>,
+                                                       kBlock, 0);
+    else
+      q = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_ias15_persistent<Spec, V>,
+                                                       kBlock, 0);
+    if (q != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      set_error("could not query %s occupancy on device %d", what, dev);
+      return -1;
+    }
+    resident = std::max(1, per_cu) * std::max(1, ncu);
+    if (dev >= 0 && dev < 64) cached[dev] = resident;
+  }
+  unsigned long long* counter = nullptr;
+  if (ws) {
+    if (ws->bytes < 256) {
+      set_error("the %s work counter needs 256 bytes of workspace, %zu given", what, ws->bytes);
+      return -1;
+    }
+    counter = reinterpret_cast<unsigned long long*>(ws->ptr);
+  } else {
+    counter = next_counter(dev);
+    if (counter == nullptr) {
+      set_error("could not set up the %s work counter on device %d", what, dev);
+      return -1;
+    }
+  }
+  const unsigned nblocks =
+      (unsigned)std::min<long>((long)((nobj + kBlock - 1) / kBlock), 2L * resident);
+  hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, 0ULL);
+  if constexpr (DOPR54)
+    hipLaunchKernelGGL((gh_dopr54_persistent<Spec, V>), dim3(nblocks), dim3(kBlock), 0, stream,
+                       nobj, d_yo, nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err,
+                       d_stats, cancel, counter);
+  else
+    hipLaunchKernelGGL((gh_ias15_persistent<Spec, V>), dim3(nblocks), dim3(kBlock), 0, stream,
+                       nobj, d_yo, nt, d_t, indiv_t, dt, d_table, d_result, d_err, d_stats,
+                       cancel, counter);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s launch failed: %s", what, hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
 }  // namespace
 
 // Planar DOP853 runs at 2 waves per SIMD only (its state is two thirds of a
@@ -938,15 +1142,26 @@ int launch_spec(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                            d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats,
                            cancel);
       break;
-    case 5:  // DOPR54: likewise
-      hipLaunchKernelGGL((gh_dopr54_kernel<Spec, V>), grid, block, 0, stream, nobj, d_yo, nt, d_t,
-                         indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+    case 5: {  // DOPR54: plain orbits on persistent lanes, the variational systems as before
+      if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
+        int dev = 0;
+        GH_CHECK(hipGetDevice(&dev));
+        return launch_persistent<Spec, V, true>(nobj, d_yo, nt, d_t, indiv_t, d_table, dt, rtol,
+                                                atol, d_result, d_err, d_stats, cancel, dev,
+                                                stream, ws);
+      } else {
+        hipLaunchKernelGGL((gh_dopr54_kernel<Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
+                           d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err, d_stats, cancel);
+      }
       break;
+    }
     case 7:  // IAS15: plain orbits (galpy has no ias15 integrate_dxdv)
       if constexpr (std::is_same<V, GhVec3>::value || std::is_same<V, GhVec2>::value) {
-        hipLaunchKernelGGL((gh_ias15_kernel<Spec, V>), grid, block, 0, stream, nobj, d_yo, nt,
-                           d_t, indiv_t, dt, d_table, d_result, d_err, d_stats, cancel);
-        break;
+        int dev = 0;
+        GH_CHECK(hipGetDevice(&dev));
+        return launch_persistent<Spec, V, false>(nobj, d_yo, nt, d_t, indiv_t, d_table, dt, rtol,
+                                                 atol, d_result, d_err, d_stats, cancel, dev,
+                                                 stream, ws);
       } else {
         set_error("ias15 has no integrate_dxdv (as in galpy, which refuses it upstream)");
         return -1;
