@@ -280,14 +280,17 @@ gh_dopr54_kernel(int nobj, double* __restrict__ yo, int nt, const double* __rest
 // one step), one output row or one interval's set-up; per orbit the steps
 // and their order are those of gh_dopr54_kernel.  A lane leaves once no lane of its wavefront holds an orbit or
 // a claim below nobj, which every lane reaches: each orbit ends after at
-// most nt - 1 intervals of at most kDopr54MaxIter passes.
-template <class Spec, class V>
+// most nt - 1 intervals of at most kDopr54MaxIter passes.  `order`
+// (nullable) permutes the claims: claim k integrates orbit order[k].  A
+// PILOT launch writes nothing but cost[o] = step attempts (yo untouched).
+template <class Spec, class V, bool PILOT>
 __global__ void __launch_bounds__(kBlock)
 gh_dopr54_persistent(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
                      int indiv_t, double dt_in, double rtol, double atol,
                      const GhPotTable* __restrict__ T, double* __restrict__ result,
                      int* __restrict__ err, double* __restrict__ stats, const volatile int* cancel,
-                     unsigned long long* __restrict__ counter) {
+                     unsigned long long* __restrict__ counter, const int* __restrict__ order,
+                     unsigned* __restrict__ cost) {
   constexpr int N = GhPhase<V>::N;
   const GhForce<Spec> F{T};
   long next = (long)atomicAdd(counter, 1ULL);  // the orbit to start next
@@ -301,16 +304,18 @@ gh_dopr54_persistent(int nobj, double* __restrict__ yo, int nt, const double* __
   bool active = false, fresh = false;  // fresh: interval ii not yet set up
   while (true) {
     if (!active && next < nobj) {
-      orbit = next;
+      orbit = order != nullptr ? (long)order[next] : next;
       tt = t + (indiv_t ? orbit * nt : 0);
       double v[N];
 #pragma unroll
       for (int k = 0; k < N; k++) v[k] = yo[orbit * N + k];
       GhPhase<V>::to_rect(v, yn);
+      if (!PILOT) {
 #pragma unroll
-      for (int k = 0; k < N; k++) yo[orbit * N + k] = yn[k];  // galpy converts yo in place
-      out = result + orbit * (long)nt * N;
-      GhPhase<V>::store(out, GhPhase<V>::q(yn), GhPhase<V>::p(yn));
+        for (int k = 0; k < N; k++) yo[orbit * N + k] = yn[k];  // galpy converts yo in place
+        out = result + orbit * (long)nt * N;
+        GhPhase<V>::store(out, GhPhase<V>::q(yn), GhPhase<V>::p(yn));
+      }
       dt = tt[1] - tt[0];
       dt_one = dt_in;
       if (dt_in == GH_DT_SENTINEL)
@@ -344,7 +349,8 @@ gh_dopr54_persistent(int nobj, double* __restrict__ yo, int nt, const double* __
           end = true;
         }
         if (end) {
-          GhPhase<V>::store(out + (long)N * (ii + 1), GhPhase<V>::q(yn), GhPhase<V>::p(yn));
+          if (!PILOT)
+            GhPhase<V>::store(out + (long)N * (ii + 1), GhPhase<V>::q(yn), GhPhase<V>::p(yn));
           ii++;
           fresh = true;
         } else {
@@ -366,11 +372,15 @@ gh_dopr54_persistent(int nobj, double* __restrict__ yo, int nt, const double* __
         }
       }
       if (done) {
-        err[orbit] = e;
-        if (stats) {
-          stats[orbit * 3 + 0] = (double)nacc;
-          stats[orbit * 3 + 1] = (double)nrej;
-          stats[orbit * 3 + 2] = dt_one;
+        if (PILOT) {
+          cost[orbit] = nacc + nrej > 0xffffffffL ? 0xffffffffu : (unsigned)(nacc + nrej);
+        } else {
+          err[orbit] = e;
+          if (stats) {
+            stats[orbit * 3 + 0] = (double)nacc;
+            stats[orbit * 3 + 1] = (double)nrej;
+            stats[orbit * 3 + 2] = dt_one;
+          }
         }
         active = false;
         next = (long)atomicAdd(counter, 1ULL);
@@ -411,13 +421,15 @@ __device__ __forceinline__ void gh_ias15_put(double* out, int row, const double*
 // lane claims its orbits from a counter zeroed before the launch and makes
 // one step attempt (with the rows it passes) per pass of the loop.  Each
 // orbit ends after at most kIas15MaxAttempts attempts, so every lane leaves.
-template <class Spec, class V>
+// `order` and PILOT as for gh_dopr54_persistent.
+template <class Spec, class V, bool PILOT>
 __global__ void __launch_bounds__(kBlock)
 gh_ias15_persistent(int nobj, double* __restrict__ yo, int nt, const double* __restrict__ t,
                     int indiv_t, double dt_in, const GhPotTable* __restrict__ T,
                     double* __restrict__ result, int* __restrict__ err,
                     double* __restrict__ stats, const volatile int* cancel,
-                    unsigned long long* __restrict__ counter) {
+                    unsigned long long* __restrict__ counter, const int* __restrict__ order,
+                    unsigned* __restrict__ cost) {
   constexpr int N = GhPhase<V>::N;
   constexpr int D = N / 2;
   const GhForce<Spec> F{T};
@@ -432,23 +444,27 @@ gh_ias15_persistent(int nobj, double* __restrict__ yo, int nt, const double* __r
   bool active = false;
   while (true) {
     if (!active && next < nobj) {
-      orbit = next;
+      orbit = order != nullptr ? (long)order[next] : next;
       tt = t + (indiv_t ? orbit * nt : 0);
       {
         double w[N], y[N];
 #pragma unroll
         for (int k = 0; k < N; k++) w[k] = yo[orbit * N + k];
         GhPhase<V>::to_rect(w, y);
+        if (!PILOT) {
 #pragma unroll
-        for (int k = 0; k < N; k++) yo[orbit * N + k] = y[k];  // galpy converts yo in place
+          for (int k = 0; k < N; k++) yo[orbit * N + k] = y[k];  // galpy converts yo in place
+        }
 #pragma unroll
         for (int i = 0; i < D; i++) {
           x[i] = y[i];
           v[i] = y[D + i];
         }
       }
-      out = result + orbit * (long)nt * N;
-      gh_ias15_put<V, D>(out, 0, x, v);
+      if (!PILOT) {
+        out = result + orbit * (long)nt * N;
+        gh_ias15_put<V, D>(out, 0, x, v);
+      }
 #pragma unroll
       for (int k = 0; k < 7; k++)
 #pragma unroll
@@ -490,7 +506,7 @@ gh_ias15_persistent(int nobj, double* __restrict__ yo, int nt, const double* __r
             double xs[D], vs[D];
             gh_ias15_position<D>(xs, x, v, hs, dt_temp, F0, B);
             gh_ias15_velocity<D>(vs, v, hs, dt_temp, F0, B);
-            gh_ias15_put<V, D>(out, row, xs, vs);
+            if (!PILOT) gh_ias15_put<V, D>(out, row, xs, vs);
             row++;
           }
           gh_ias15_advance<D>(x, v, dt_temp, F0, B, E);
@@ -498,17 +514,21 @@ gh_ias15_persistent(int nobj, double* __restrict__ yo, int nt, const double* __r
           dt = dt_required;
         }
       } else {
-        if (e == 0) {
-          double nan[D];
+        if (PILOT) {
+          cost[orbit] = nacc + nrej > 0xffffffffL ? 0xffffffffu : (unsigned)(nacc + nrej);
+        } else {
+          if (e == 0) {
+            double nan[D];
 #pragma unroll
-          for (int i = 0; i < D; i++) nan[i] = __builtin_nan("");
-          for (; row < nt; row++) gh_ias15_put<V, D>(out, row, nan, nan);
-        }
-        err[orbit] = e;
-        if (stats) {
-          stats[orbit * 3 + 0] = (double)nacc;
-          stats[orbit * 3 + 1] = (double)nrej;
-          stats[orbit * 3 + 2] = dt;
+            for (int i = 0; i < D; i++) nan[i] = __builtin_nan("");
+            for (; row < nt; row++) gh_ias15_put<V, D>(out, row, nan, nan);
+          }
+          err[orbit] = e;
+          if (stats) {
+            stats[orbit * 3 + 0] = (double)nacc;
+            stats[orbit * 3 + 1] = (double)nrej;
+            stats[orbit * 3 + 2] = dt;
+          }
         }
         active = false;
         next = (long)atomicAdd(counter, 1ULL);
@@ -1043,6 +1063,24 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
   return 0;
 }
 
+// One launch of the persistent DOPR54 (DOPR54 = true) or IAS15 kernel.
+template <class Spec, class V, bool DOPR54, bool PILOT>
+void persistent_launch(unsigned nblocks, hipStream_t stream, int nobj, double* d_yo, int nt,
+                       const double* d_t, int indiv_t, const GhPotTable* d_table, double dt,
+                       double rtol, double atol, double* d_result, int* d_err, double* d_stats,
+                       const int* cancel, unsigned long long* counter, const int* order,
+                       unsigned* cost) {
+  hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, 0ULL);
+  if constexpr (DOPR54)
+    hipLaunchKernelGGL((gh_dopr54_persistent<Spec, V, PILOT>), dim3(nblocks), dim3(kBlock), 0,
+                       stream, nobj, d_yo, nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result,
+                       d_err, d_stats, cancel, counter, order, cost);
+  else
+    hipLaunchKernelGGL((gh_ias15_persistent<Spec, V, PILOT>), dim3(nblocks), dim3(kBlock), 0,
+                       stream, nobj, d_yo, nt, d_t, indiv_t, dt, d_table, d_result, d_err,
+                       d_stats, cancel, counter, order, cost);
+}
+
 // DOPR54 (DOPR54 = true) or IAS15 on persistent lanes: twice as many
 // workgroups as the occupancy query says stay resident (or as the orbits
 // fill), the work counter zeroed first; the counter is the workspace's first
@@ -1050,7 +1088,11 @@ int launch_dop853(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t
 // the 120-register MWPotential2014 DOPR54 instance, half of what gfx950's 512
 // registers per SIMD lane hold; the surplus costs nothing when the query is
 // right, because a workgroup that starts after the queue has drained leaves
-// at once.
+// at once.  With more orbits than resident lanes (one time grid, no
+// workspace) the orbits are claimed longest first, as for DOP853: a PILOT
+// launch over the first kDopPilotFrac of the time range counts each orbit's
+// step attempts, a radix sort orders them, and the lanes that finish early
+// take the short orbits at the end.
 template <class Spec, class V, bool DOPR54>
 int launch_persistent(int nobj, double* d_yo, int nt, const double* d_t, int indiv_t,
                       const GhPotTable* d_table, double dt, double rtol, double atol,
@@ -1063,11 +1105,11 @@ int launch_persistent(int nobj, double* d_yo, int nt, const double* d_t, int ind
     int per_cu = 0, ncu = 0;
     hipError_t q;
     if constexpr (DOPR54)
-      q = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_dopr54_persistent<Spec, V>,
-                                                       kBlock, 0);
+      q = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, gh_dopr54_persistent<Spec, V, false>, kBlock, 0);
     else
-      q = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gh_ias15_persistent<Spec, V>,
-                                                       kBlock, 0);
+      q = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, gh_ias15_persistent<Spec, V, false>, kBlock, 0);
     if (q != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
       set_error("could not query %s occupancy on device %d", what, dev);
@@ -1092,16 +1134,48 @@ int launch_persistent(int nobj, double* d_yo, int nt, const double* d_t, int ind
   }
   const unsigned nblocks =
       (unsigned)std::min<long>((long)((nobj + kBlock - 1) / kBlock), 2L * resident);
-  hipLaunchKernelGGL(gh_set_counter, dim3(1), dim3(1), 0, stream, counter, 0ULL);
-  if constexpr (DOPR54)
-    hipLaunchKernelGGL((gh_dopr54_persistent<Spec, V>), dim3(nblocks), dim3(kBlock), 0, stream,
-                       nobj, d_yo, nt, d_t, indiv_t, dt, rtol, atol, d_table, d_result, d_err,
-                       d_stats, cancel, counter);
-  else
-    hipLaunchKernelGGL((gh_ias15_persistent<Spec, V>), dim3(nblocks), dim3(kBlock), 0, stream,
-                       nobj, d_yo, nt, d_t, indiv_t, dt, d_table, d_result, d_err, d_stats,
-                       cancel, counter);
-  const hipError_t e = hipGetLastError();
+  const bool ordered =
+      !ws && !indiv_t && nt > 1 && (long)nobj > (long)resident * kBlock && dop_order_enabled();
+  hipError_t e = hipSuccess;
+  void* work = nullptr;
+  const int* d_order = nullptr;
+  if (ordered) {
+    size_t sort_bytes = 0;
+    e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, (const unsigned*)nullptr,
+                                                     (unsigned*)nullptr, (const int*)nullptr,
+                                                     (int*)nullptr, nobj, 0, 32, stream);
+    const size_t n4 = ((size_t)nobj * 4 + 255) / 256 * 256;
+    if (e == hipSuccess) e = hipMallocAsync(&work, 4 * n4 + 256 + sort_bytes, stream);
+    if (e == hipSuccess) {
+      char* w = static_cast<char*>(work);
+      unsigned* d_cost = reinterpret_cast<unsigned*>(w);
+      unsigned* d_cost_sorted = reinterpret_cast<unsigned*>(w + n4);
+      int* d_iota = reinterpret_cast<int*>(w + 2 * n4);
+      int* d_perm = reinterpret_cast<int*>(w + 3 * n4);
+      double* d_pt = reinterpret_cast<double*>(w + 4 * n4);
+      void* d_sort_tmp = w + 4 * n4 + 256;
+      hipLaunchKernelGGL(gh_pilot_grid, dim3(1), dim3(1), 0, stream, d_t, nt, kDopPilotFrac, d_pt);
+      persistent_launch<Spec, V, DOPR54, true>(nblocks, stream, nobj, d_yo, 2, d_pt, 0, d_table,
+                                               dt, rtol, atol, nullptr, nullptr, nullptr, cancel,
+                                               counter, nullptr, d_cost);
+      hipLaunchKernelGGL(gh_iota, dim3((unsigned)((nobj + 255) / 256)), dim3(256), 0, stream,
+                         nobj, d_iota);
+      e = hipcub::DeviceRadixSort::SortPairsDescending(d_sort_tmp, sort_bytes, d_cost,
+                                                       d_cost_sorted, d_iota, d_perm, nobj, 0, 32,
+                                                       stream);
+      d_order = d_perm;
+    }
+  }
+  if (e == hipSuccess) {
+    persistent_launch<Spec, V, DOPR54, false>(nblocks, stream, nobj, d_yo, nt, d_t, indiv_t,
+                                              d_table, dt, rtol, atol, d_result, d_err, d_stats,
+                                              cancel, counter, d_order, nullptr);
+    e = hipGetLastError();
+  }
+  if (work) {
+    const hipError_t ef = hipFreeAsync(work, stream);
+    if (e == hipSuccess) e = ef;
+  }
   if (e != hipSuccess) {
     set_error("%s launch failed: %s", what, hipGetErrorString(e));
     return -1;
