@@ -158,8 +158,9 @@ int integrateFullOrbit_hip(int nobj, double *d_yo, int nt, const double *d_t, in
  * call whose temporaries do not fit (roughly 48 MiB of overflow area at the
  * MI355X's 2,048 resident workgroups plus 16 bytes per orbit when there are
  * more orbits than resident lanes: about 2 million orbits) returns -1 with
- * a message and launches nothing.  Calls that overlap on one GPU race on the
- * array.  The interrupt flag is polled only if it was set up before (by a
+ * a message and launches nothing.  DOPR54 and IAS15 use only the work
+ * counter there and claim orbits in index order (no pilot, no sort), which
+ * changes no result.  Calls that overlap on one GPU race on the array.  The interrupt flag is polled only if it was set up before (by a
  * host-buffer call or galpy_hip_set_interrupt). */
 int integrateFullOrbit_hip_noalloc(int nobj, double *d_yo, int nt, const double *d_t,
                                    int indiv_t, const void *pot, double dt, double rtol,
